@@ -1,0 +1,183 @@
+/*
+ * td1c_hprod.h -- C-ABI of the MI355X (gfx950) orbital Hamiltonian product
+ * ("hprod") and its explicit Runge-Kutta step.
+ *
+ * The library replaces, for one process-wide problem, the Fortran entry
+ * points that td1c's C++ drivers call through include/wrapper.hpp:
+ *
+ *   hprod_htot_        include/wrapper.hpp:159, called by clhprod::htot
+ *                      (src/hprod/clhprod.cpp:62-82)
+ *   hprod_set_wfn0_    include/wrapper.hpp:400, called by clhprod::set_wfn0
+ *                      (src/hprod/clhprod.cpp:50-55)
+ *   bas_rad_bind_, bas_sph_bind_, bas_bas_bind_, control_bind_, ormas_bind_,
+ *   hprod_bind_, hprod_init_ (src/bas/clbas.cpp:60-80,135-141;
+ *                      src/control/clcontrol.cpp:185-189;
+ *                      src/hprod/clhprod.cpp:44-47)
+ *                      -> folded into td1c_ctx_create, which takes the same
+ *                      grid / angular / orbital / control parameters by value
+ *                      and builds every operator table itself.
+ *   clcrk::prop        src/prop/clcrk.cpp:55-87 (rk_order 1..4)
+ *                      -> td1c_rk_prop
+ *   clwfn::mask        src/wfn/clwfn.cpp:337-357, src/wfn/wfn_mask.F90:2-35
+ *                      -> td1c_mask
+ *   clfield::get_value src/field/clfield.cpp:121-140 -> td1c_field_value
+ *
+ * Data layout is td1c's: an orbital vector is complex128
+ * wfn[(ifun*(lmax1+1) + l)*(nrad-1) + irad], radial index fastest
+ * (src/hprod/hprod_htot_dtorb.F90:15-20), and a full wave function is the
+ * nfun*nbas orbital coefficients followed by the lcic CI coefficients
+ * (src/wfn/clwfn.cpp:115-121).  Complex numbers are two doubles (re, im),
+ * the layout of Fortran complex(c_double_complex) and std::complex<double>.
+ *
+ * Errors: every int-returning function returns 0 on success and a negative
+ * code on failure; td1c_last_error() gives the message.  (The reference has
+ * no return codes: it calls abort() or Fortran STOP.)  hprod_htot_ keeps the
+ * reference's void signature and aborts with a message on failure.
+ */
+#ifndef TD1C_HPROD_H
+#define TD1C_HPROD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Problem description: the values the reference reads from its input deck
+ * (clfedvr::read_x, clsph::gen, clbas::read_info, clormas, clcontrol::gen)
+ * and binds into its Fortran modules. */
+typedef struct td1c_problem {
+  /* radial FEDVR grid: nfe elements of ndvr+1 Gauss-Lobatto points each;
+   * edges[0..nfe] are the element boundaries (src/fedvr/clfedvr.cpp:253-311) */
+  int nfe;
+  int ndvr;
+  const double* edges;
+  /* angular expansion (src/sph/clsph.cpp:97-134); nlat <= 0 means lmax2+1,
+   * mmax1 < 0 means max |mval|, mmax2 < 0 means 2*mmax1 */
+  int lmax1, lmax2, mmax1, mmax2, nlat;
+  /* nuclear charge (src/bas/clbas.cpp:35) */
+  double znuc;
+  /* orbitals, ordered frozen core type -2, frozen core type -1, dynamical
+   * core, active (src/ormas/ormas_info.F90:18-44); nfun is the sum */
+  int nfcore2, nfcore1, ndcore, nact;
+  const int* mval;          /* [nfun] magnetic quantum number per orbital */
+  const int* lval;          /* [nfun] l of each orbital (frozen-core density) */
+  int nel_alpha, nel_beta;  /* neltot(1:2) */
+  int mtot;                 /* total M of the CI space */
+  int nradfc;               /* radial extent of frozen-core work, <=0: nrad-1 */
+  /* control flags (src/control/clcontrol.cpp:67-191) */
+  int icomp;                /* 1: real time (tfac = -i dt), 0: imaginary time */
+  int igauge;               /* 0: length1, 1: velocity1 */
+  int isplit;               /* split_type: only 0 (full hprod) is provided */
+  int ioorot;               /* oorot_type: only 0 is provided */
+  int iprojfc;              /* 1 or 2 (default 2) */
+  int jfc_implicit;         /* default 1 */
+  int xfc_implicit;         /* default 1 */
+  double throcc1, throcc2;  /* regularisation of D^-1, (2-D)^-1 (1e-15) */
+  /* absorbing mask (src/fedvr/clfedvr.cpp:361-398,795-810): rmask <= 0
+   * means 0.8 * rmax, mask_cos_order <= 0 means 4 */
+  double rmask;
+  double mask_cos_order;
+  /* arithmetic of the bulk one-body / mean-field work: 0 = complex128
+   * throughout, 1 = complex64 bulk with FP64 state, RK accumulation,
+   * projector, 1-RDM inverse and CI (no counterpart in the reference) */
+  int precision;
+} td1c_problem;
+
+/* Sizes of a created context. */
+typedef struct td1c_sizes {
+  int nrad;      /* FEDVR points including r=0 and r=R; interior = nrad-1 */
+  int nbas;      /* (nrad-1)*(lmax1+1) */
+  int nfun;
+  int lcic;      /* CI length (1 when nact == 0) */
+  int nlat;
+  int lmax1, lmax2;
+  int npair;     /* number of orbital pair densities evaluated per hprod */
+  long long wfn_size;  /* nfun*nbas + lcic complex elements */
+} td1c_sizes;
+
+typedef struct td1c_ctx td1c_ctx;
+
+/* Laser field, one pulse (src/field/clfield1.cpp:14-116):
+ * env_type 0 = zero, 1 = sin2, 2 = flat; gauge 0 = length1, 1 = velocity1.
+ * fint in W/cm^2, wlen in nm, cep in degrees, numcyc optical cycles. */
+typedef struct td1c_field {
+  int env_type;
+  int gauge;
+  double fint;
+  double wlen;
+  double cep;
+  double numcyc;
+  double delay;   /* attoseconds-free: a.u. of time after unit conversion */
+} td1c_field;
+
+/* Builds the FEDVR grid, angular quadrature, every operator table and the
+ * device work space for one problem, on the current HIP device.
+ * Replaces clbas::gen + the *_bind_ calls + hprod_init_. */
+int td1c_ctx_create(const td1c_problem* prob, td1c_ctx** out);
+
+/* Frees the context (hprod_final_). */
+int td1c_ctx_destroy(td1c_ctx* ctx);
+
+/* Sizes of a context. */
+int td1c_get_sizes(const td1c_ctx* ctx, td1c_sizes* out);
+
+/* Frozen-core set-up from the initial wave function (host pointers):
+ * orb0, rhofc, v2jfc (+ v2xfc when given), tmat += v2jfc.
+ * hprod_set_wfn0_ (src/hprod/hprod_set_wfn0.F90:2-130,132-219); the
+ * KLI/LDA local exchange of type -2 cores is not computed here: pass it in
+ * v2xfc (nrad-1 complex values, or NULL for zero), as v2xfc_type = "read"
+ * does in the reference. */
+int td1c_set_wfn0(td1c_ctx* ctx, const double* wfn, const double* cic,
+                  const double* v2xfc);
+
+/* One Hamiltonian product on device buffers, enqueued on `stream`
+ * (a hipStream_t, NULL for the default stream):
+ *   hwfn = dt * d(psi)/dt (orbitals), hcic = dt * dC/dt (CI).
+ * hprod_htot (src/hprod/hprod_htot.F90:2-42) plus the madapt/ladapt
+ * post-ops of clhprod::htot (both no-ops for lconst = false). */
+int td1c_htot(td1c_ctx* ctx, double dtime, const double* lfield,
+              const double* d_wfn, const double* d_cic,
+              double* d_hwfn, double* d_hcic, void* stream);
+
+/* Energies of the last td1c_htot (ene_fcore, ene_dcore, ene_core, ene_act,
+ * ene_tot as bound by hprod_bind_); synchronises the stream. */
+int td1c_get_energies(td1c_ctx* ctx, double ene[5]);
+
+/* One explicit Runge-Kutta step of the whole wave function (orbitals then
+ * CI) held on device: clcrk::prop with rk_order 1..4 ("standard").
+ * lfield_stages holds the 9 field values at t + c_s*dt for each stage
+ * (host memory, nstage*9 doubles).  apply_mask != 0 applies clwfn::mask
+ * afterwards, as tdse_split2 does after every step (src/main/tdse.cpp:225-266). */
+int td1c_rk_prop(td1c_ctx* ctx, int rk_order, double dtime,
+                 const double* lfield_stages, double* d_wfn, int apply_mask,
+                 void* stream);
+
+/* Applies the radial mask to the dynamical orbitals of a device wave
+ * function (wfn_mask, src/wfn/wfn_mask.F90:2-35). */
+int td1c_mask(td1c_ctx* ctx, double* d_wfn, void* stream);
+
+/* Field value at time t: lfield[0..2] chosen gauge, [3..5] E, [6..8] A
+ * (src/field/clfield1.cpp:131-154). t is measured from the pulse start. */
+int td1c_field_value(const td1c_field* field, double t, double lfield[9]);
+
+/* Optical period (a.u.) of the field, used to set dtime = period/nstep. */
+double td1c_field_period(const td1c_field* field);
+
+/* Binds a context as the process-wide one used by hprod_htot_ (the
+ * reference allows a single clhprod per process, clhprod.cpp:32-34). */
+int td1c_bind(td1c_ctx* ctx);
+
+/* Drop-in for the Fortran entry point with the reference's exact argument
+ * list (include/wrapper.hpp:159): host pointers, synchronous, operates on
+ * the bound context.  wfn/hwfn hold nfun*nbas complex values, cic/hcic lcic. */
+void hprod_htot_(const double* dtime, const double* lfield,
+                 const double* wfn, const double* cic,
+                 double* hwfn, double* hcic);
+
+/* Message of the last error on this thread. */
+const char* td1c_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TD1C_HPROD_H */

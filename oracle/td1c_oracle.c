@@ -1,0 +1,1677 @@
+/*
+ * td1c_oracle.c -- TEST INFRASTRUCTURE ONLY.
+ *
+ * An FP64 CPU restatement of td1c's orbital Hamiltonian product
+ * (hprod_htot, src/hprod/hprod_htot.F90:2-208), its set-up (FEDVR grid,
+ * Gauss-Legendre / associated-Legendre tables, operator matrices) and the
+ * explicit Runge-Kutta step clcrk::prop (src/prop/clcrk.cpp:55-87).
+ * It is the checker the GPU library is compared with.  Only tests/, the
+ * smoke entry point and the benchmark's CPU-baseline leg may load it; the
+ * product library never links or calls it.
+ *
+ * Each routine follows the reference routine named in its comment, loop for
+ * loop, with 0-based indices: reference irad = 1..nrad-1 is i = 0..nr-1,
+ * xrad(irad) is xrad[irad] (xrad has nrad+1 entries, index 0 is r = 0).
+ *
+ * Third-party arithmetic the reference calls and that is restated here:
+ *  - LAPACK dpbtrf/dpbtf2 + dpbtrs (band Cholesky, 'L'), netlib algorithm;
+ *  - BLAS zdotu/zaxpy loops;
+ *  - SHTOOLS SHGLQ (Gauss-Legendre nodes/weights, 4pi-orthonormalised
+ *    associated Legendre functions with Condon-Shortley phase, complex
+ *    normalisation): standard three-term recursion;
+ *  - LAPACK dsyev for the Gauss-Lobatto nodes: replaced by Newton iteration
+ *    on P'_n (same roots);
+ *  - LAPACK zheev for the n x n Hermitian 1-RDM: cyclic complex Jacobi.
+ * Parity status of each function is listed in the oracle README section of
+ * the design notes and in tests/test_oracle_pins.py.
+ */
+#include <complex.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../include/td1c_hprod.h"
+
+typedef double _Complex zc;
+
+#define ORC_PI 3.14159265358979323846264338327950288
+
+typedef struct orc {
+  /* problem */
+  int nfe, ndvr, nrad, nr, nradfc;
+  int lmax1, lmax2, mmax1, mmax2, nlat, L1, L2;
+  double znuc;
+  int nfcore2, nfcore1, nfcore, ndcore, ncore, nact, nfun;
+  int *mval, *lval;
+  int nel[3], nelact[3], mtot;
+  int icomp, igauge, isplit, ioorot, iprojfc, jfc_implicit, xfc_implicit;
+  double throcc1, throcc2, rmask, mask_order;
+  int nbas, nbas2, ngrid;
+  /* FEDVR (clfedvr) */
+  double *x0, *x1;
+  double *dvr_x, *dvr_w, *dshape; /* per element: (ndvr+1), (ndvr+1)^2 */
+  int *mapf, *mapb;
+  double *xrad, *wrad, *mask;
+  double *radk, *radk0, *radp; /* (2n+1)*(nrad+1) */
+  /* angular (clsph / sph_gen) */
+  double *cost, *wlat;
+  double *legf1, *legb1, *legf2, *legb2;
+  /* operators (clbas::gen) */
+  double *alph;                 /* (lmax1+1)*(2mmax1+1) */
+  zc *tmat;                     /* (2n+1)*nr*L1 */
+  zc *pmat;                     /* (2n+1)*nr */
+  double *d2ll;                 /* (n+1)*nr*L2 */
+  zc *zfac, *pzfac1, *pzfac2;   /* zfac,pzfac2: nr*L1*(2mmax1+1); pzfac1: L1*(2mmax1+1) */
+  double *d2fac1, *d2fac2, *d2invr, *d2rpl0, *d2rpl1; /* d2fac1: L2; others nr*L2 */
+  /* hprod scratch (mod_hprod) */
+  zc *orb, *h0orb, *h1orb, *gorb, *v2orb;
+  zc *orbg, *gorbg, *v2orbg, *torbg;
+  zc *rho2, *v2sph, *v2ang;
+  zc *orb0, *rhofc, *v2jfc, *v2xfc;
+  zc *den1, *rden, *rrden, *int1e, *den2, *int2e;
+  zc *xmat, *fmat, *bmat, *dcic;
+  double ene_fcore, ene_dcore, ene_core, ene_act, ene_tot;
+  /* CAS CI */
+  int nstr_a, nstr_b, lcic;
+  uint32_t *str_a, *str_b;
+  int *det_a, *det_b;  /* [lcic] string index per determinant */
+  int *det_of;         /* [nstr_b*nstr_a] -> determinant index or -1 */
+  zc *ci_work;         /* nact^2 * lcic */
+  /* RK work */
+  zc *rk_w0, *rk_t, *rk_k;
+} orc;
+
+static void* xcalloc(size_t n, size_t sz) {
+  void* p = calloc(n ? n : 1, sz);
+  if (!p) { fprintf(stderr, "oracle: out of memory\n"); abort(); }
+  return p;
+}
+
+/* ------------------------------------------------------------------ */
+/* Gauss-Lobatto DVR (cldvr::gen, src/dvr/cldvr.cpp:57-221)            */
+/* ------------------------------------------------------------------ */
+
+/* P_n(x) with the same long double recursion as cldvr::get_legendre
+ * (src/dvr/cldvr.cpp:164-190). */
+static double legendre_ld(int n, double x) {
+  if (n == 0) return 1.0;
+  if (n == 1) return x;
+  long double p0 = 1.0L, p1 = x, p2 = 0.0L;
+  for (int k = 2; k <= n; ++k) {
+    long double rk = (long double)k;
+    long double c0 = -(rk - 1.0L) / rk;
+    long double c1 = (2.0L * rk - 1.0L) / rk;
+    p2 = c0 * p0 + c1 * x * p1;
+    p0 = p1;
+    p1 = p2;
+  }
+  return (double)p2;
+}
+
+/* Interior GLL nodes = roots of P'_n.  The reference takes them as
+ * eigenvalues of a Jacobi matrix (cldvr::gen_xpt, :82-138, via dsyev);
+ * here Newton on P'_n from Chebyshev-Lobatto starts, in long double. */
+static void gll_nodes(int n, double* x) {
+  x[0] = -1.0;
+  x[n] = 1.0;
+  for (int k = 1; k < n; ++k) {
+    long double z = -cosl((long double)ORC_PI * k / n);
+    for (int it = 0; it < 100; ++it) {
+      /* P_n, P_{n-1} -> P'_n and P''_n */
+      long double p0 = 1.0L, p1 = z;
+      for (int j = 2; j <= n; ++j) {
+        long double p2 = ((2.0L * j - 1.0L) * z * p1 - (j - 1.0L) * p0) / j;
+        p0 = p1;
+        p1 = p2;
+      }
+      /* p1 = P_n, p0 = P_{n-1} */
+      long double dp = n * (z * p1 - p0) / (z * z - 1.0L);
+      long double d2p = (2.0L * z * dp - n * (n + 1.0L) * p1) / (1.0L - z * z);
+      long double dz = dp / d2p;
+      z -= dz;
+      if (fabsl(dz) < 1e-19L) break;
+    }
+    x[k] = (double)z;
+  }
+}
+
+static void dvr_gen(int n, double xmin, double xmax, double* xpt, double* wpt,
+                    double* dshape) {
+  gll_nodes(n, xpt);
+  /* cldvr::gen_wpt (:140-162) */
+  double xn = (double)n;
+  double scale = 2.0 / (xn * (xn + 1.0));
+  wpt[0] = scale;
+  wpt[n] = scale;
+  for (int k = 1; k < n; ++k) {
+    double p = legendre_ld(n, fabs(xpt[k]));
+    wpt[k] = scale / (p * p);
+  }
+  double lenx = 0.5 * (xmax - xmin), midx = 0.5 * (xmax + xmin);
+  for (int k = 0; k <= n; ++k) {
+    xpt[k] = xpt[k] * lenx + midx;
+    wpt[k] = wpt[k] * lenx;
+  }
+  /* cldvr::gen_dshape (:192-221): dshape[(n+1)*m + k] = f_m'(x_k) */
+  for (int m = 0; m <= n; ++m) {
+    for (int k = 0; k <= n; ++k) {
+      int km = (n + 1) * m + k;
+      if (m != k) {
+        double dshx = 1.0 / (xpt[m] - xpt[k]);
+        for (int l = 0; l <= n; ++l) {
+          if (l != m && l != k) {
+            double diffm = xpt[m] - xpt[l];
+            double diffk = xpt[k] - xpt[l];
+            dshx *= diffk / diffm;
+          }
+        }
+        dshape[km] = dshx;
+      } else {
+        dshape[km] = 0.0;
+      }
+    }
+  }
+  dshape[0] = -0.5 / wpt[0];
+  dshape[(n + 1) * (n + 1) - 1] = 0.5 / wpt[n];
+}
+
+/* ------------------------------------------------------------------ */
+/* FEDVR (clfedvr, src/fedvr/clfedvr.cpp)                              */
+/* ------------------------------------------------------------------ */
+static void fedvr_gen(orc* o) {
+  int n = o->ndvr, nfe = o->nfe;
+  o->nrad = n * nfe;
+  int nrad = o->nrad;
+  o->dvr_x = xcalloc((size_t)nfe * (n + 1), sizeof(double));
+  o->dvr_w = xcalloc((size_t)nfe * (n + 1), sizeof(double));
+  o->dshape = xcalloc((size_t)nfe * (n + 1) * (n + 1), sizeof(double));
+  for (int i = 0; i < nfe; ++i)
+    dvr_gen(n, o->x0[i], o->x1[i], o->dvr_x + i * (n + 1), o->dvr_w + i * (n + 1),
+            o->dshape + (size_t)i * (n + 1) * (n + 1));
+  /* gen_map (:313-338) */
+  o->mapf = xcalloc(nfe, sizeof(int));
+  o->mapb = xcalloc(nrad + 1, sizeof(int));
+  int nb = 0;
+  for (int i = 0; i < nfe; ++i) {
+    o->mapf[i] = nb;
+    for (int m = 0; m < n; ++m) o->mapb[nb++] = i;
+  }
+  o->mapb[nrad] = nfe - 1;
+  /* gen_grid (:400-511), mask-type boundary (ecs off) */
+  o->xrad = xcalloc(nrad + 1, sizeof(double));
+  o->wrad = xcalloc(nrad + 1, sizeof(double));
+  for (int ib = 0; ib <= nrad; ++ib) {
+    int i = o->mapb[ib], m = ib - o->mapf[i];
+    o->xrad[ib] = o->dvr_x[i * (n + 1) + m];
+    o->wrad[ib] = o->dvr_w[i * (n + 1) + m];
+  }
+  for (int i = 1; i < nfe; ++i) o->wrad[o->mapf[i]] += o->dvr_w[(i - 1) * (n + 1) + n];
+  /* gen_mask (:361-398) */
+  o->mask = xcalloc(nrad + 1, sizeof(double));
+  double order = o->mask_order > 0 ? 1.0 / o->mask_order : fabs(o->mask_order);
+  double rul = o->x1[nfe - 1];
+  for (int ib = 1; ib < nrad; ++ib) {
+    int i = o->mapb[ib], m = ib - o->mapf[i];
+    double xv = o->dvr_x[i * (n + 1) + m];
+    double absx = sqrt(xv * xv);
+    if (absx <= o->rmask) {
+      o->mask[ib] = 1.0;
+    } else {
+      double tmp = 0.5 * ORC_PI * (absx - o->rmask) / (rul - o->rmask);
+      o->mask[ib] = pow(cos(tmp), order);
+    }
+  }
+  /* gen_radk (:676-757), fedvr_normalized */
+  size_t bw = 2 * (size_t)n + 1;
+  o->radk = xcalloc(bw * (nrad + 1), sizeof(double));
+  o->radk0 = xcalloc(bw * (nrad + 1), sizeof(double));
+  o->radp = xcalloc(bw * (nrad + 1), sizeof(double));
+  for (int i = 0; i < nfe; ++i) {
+    int ib0 = o->mapf[i];
+    const double* w = o->dvr_w + i * (n + 1);
+    const double* ds = o->dshape + (size_t)i * (n + 1) * (n + 1);
+    for (int m = 0; m <= n; ++m) {
+      int mb = ib0 + m;
+      for (int k = 0; k <= n; ++k) {
+        int kb = ib0 + k;
+        size_t kmb = bw * mb + (n + kb - mb);
+        if (i == 0 && (m == 0 || k == 0)) continue;
+        if (i == nfe - 1 && (m == n || k == n)) continue;
+        double tmp = 0.0, tmp0 = 0.0;
+        for (int l = (i == 0 ? 1 : 0); l <= n; ++l)
+          tmp0 += w[l] * ds[(n + 1) * m + l] * ds[(n + 1) * k + l];
+        for (int l = 0; l <= n; ++l) tmp += w[l] * ds[(n + 1) * m + l] * ds[(n + 1) * k + l];
+        double nrm = sqrt(o->wrad[mb] * o->wrad[kb]);
+        o->radk[kmb] += 0.5 * tmp / nrm;
+        o->radk0[kmb] += 0.5 * tmp0 / nrm;
+      }
+    }
+  }
+  /* gen_radp (:759-792) */
+  for (int i = 0; i < nfe; ++i) {
+    int ib0 = o->mapf[i];
+    const double* w = o->dvr_w + i * (n + 1);
+    const double* ds = o->dshape + (size_t)i * (n + 1) * (n + 1);
+    for (int m = 0; m <= n; ++m) {
+      int mb = ib0 + m;
+      for (int k = 0; k <= n; ++k) {
+        int kb = ib0 + k;
+        size_t kmb = bw * mb + (n + kb - mb);
+        if (i == 0 && (m == 0 || k == 0)) continue;
+        if (i == nfe - 1 && (m == n || k == n)) continue;
+        double tmp = w[m] * ds[(n + 1) * k + m];
+        o->radp[kmb] += tmp / sqrt(o->wrad[mb] * o->wrad[kb]);
+      }
+    }
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* Angular tables (sph_gen, src/sph/sph_gen.F90:2-110)                 */
+/* ------------------------------------------------------------------ */
+
+/* Gauss-Legendre nodes (cos theta) and weights on [-1,1], ascending. */
+static void gauss_legendre(int n, double* x, double* w) {
+  for (int i = 0; i < (n + 1) / 2; ++i) {
+    long double z = cosl((long double)ORC_PI * (i + 0.75L) / (n + 0.5L));
+    long double pp = 0.0L;
+    for (int it = 0; it < 100; ++it) {
+      long double p0 = 1.0L, p1 = z;
+      for (int j = 2; j <= n; ++j) {
+        long double p2 = ((2.0L * j - 1.0L) * z * p1 - (j - 1.0L) * p0) / j;
+        p0 = p1;
+        p1 = p2;
+      }
+      if (n == 1) { p1 = z; p0 = 1.0L; }
+      pp = n * (z * p1 - p0) / (z * z - 1.0L);
+      long double dz = p1 / pp;
+      z -= dz;
+      if (fabsl(dz) < 1e-19L) break;
+    }
+    /* recompute derivative at the converged root */
+    {
+      long double p0 = 1.0L, p1 = z;
+      for (int j = 2; j <= n; ++j) {
+        long double p2 = ((2.0L * j - 1.0L) * z * p1 - (j - 1.0L) * p0) / j;
+        p0 = p1;
+        p1 = p2;
+      }
+      pp = n * (z * p1 - p0) / (z * z - 1.0L);
+    }
+    x[i] = (double)(-z);
+    x[n - 1 - i] = (double)z;
+    double wi = (double)(2.0L / ((1.0L - z * z) * pp * pp));
+    w[i] = wi;
+    w[n - 1 - i] = wi;
+  }
+}
+
+/* sqrt(2 pi) * SHTOOLS norm=4, csphase=-1, cnorm=1 associated Legendre
+ * function = sqrt((2l+1)/2 (l-m)!/(l+m)!) P_l^m(x), P_l^m with the
+ * Condon-Shortley phase, for 0 <= m <= l <= lmax.  p[l] for fixed m. */
+static void assoc_legendre_col(int lmax, int m, double x, double* p) {
+  double s = sqrt(fmax(0.0, 1.0 - x * x));
+  double pmm = sqrt(0.5);
+  for (int k = 1; k <= m; ++k) pmm *= -s * sqrt((2.0 * k + 1.0) / (2.0 * k));
+  for (int l = 0; l < m; ++l) p[l] = 0.0;
+  if (m > lmax) return;
+  p[m] = pmm;
+  if (m + 1 > lmax) return;
+  double pm1 = x * sqrt(2.0 * m + 3.0) * pmm;
+  p[m + 1] = pm1;
+  double pl2 = pmm, pl1 = pm1;
+  for (int l = m + 2; l <= lmax; ++l) {
+    double a = sqrt((4.0 * l * l - 1.0) / ((double)l * l - (double)m * m));
+    double b = sqrt(((l - 1.0) * (l - 1.0) - (double)m * m) / (4.0 * (l - 1.0) * (l - 1.0) - 1.0));
+    double pl = a * (x * pl1 - b * pl2);
+    p[l] = pl;
+    pl2 = pl1;
+    pl1 = pl;
+  }
+}
+
+static void sph_gen(orc* o) {
+  int nlat = o->nlat;
+  o->cost = xcalloc(nlat, sizeof(double));
+  o->wlat = xcalloc(nlat, sizeof(double));
+  gauss_legendre(nlat, o->cost, o->wlat);
+  int L1 = o->L1, L2 = o->L2, M1 = 2 * o->mmax1 + 1, M2 = 2 * o->mmax2 + 1;
+  o->legf1 = xcalloc((size_t)L1 * nlat * M1, sizeof(double));
+  o->legb1 = xcalloc((size_t)L1 * nlat * M1, sizeof(double));
+  o->legf2 = xcalloc((size_t)L2 * nlat * M2, sizeof(double));
+  o->legb2 = xcalloc((size_t)L2 * nlat * M2, sizeof(double));
+  int lmx = o->lmax2 > o->lmax1 ? o->lmax2 : o->lmax1;
+  double* p = xcalloc(lmx + 1, sizeof(double));
+  for (int ilat = 0; ilat < nlat; ++ilat) {
+    for (int m = 0; m <= o->mmax2 || m <= o->mmax1; ++m) {
+      assoc_legendre_col(lmx, m, o->cost[ilat], p);
+      /* negative m copies the positive-m table without a sign
+       * (sph_gen.F90:85-100) */
+      for (int sg = -1; sg <= 1; sg += 2) {
+        int mm = sg * m;
+        if (m == 0 && sg < 0) continue;
+        if (m <= o->mmax1)
+          for (int l = m; l <= o->lmax1; ++l) {
+            o->legf1[((size_t)(mm + o->mmax1) * nlat + ilat) * L1 + l] = p[l];
+            o->legb1[((size_t)(mm + o->mmax1) * L1 + l) * nlat + ilat] = p[l] * o->wlat[ilat];
+          }
+        if (m <= o->mmax2)
+          for (int l = m; l <= o->lmax2; ++l) {
+            o->legf2[((size_t)(mm + o->mmax2) * nlat + ilat) * L2 + l] = p[l];
+            o->legb2[((size_t)(mm + o->mmax2) * L2 + l) * nlat + ilat] = p[l] * o->wlat[ilat];
+          }
+      }
+    }
+  }
+  free(p);
+}
+
+#define LEGF1(l, ilat, m) o->legf1[((size_t)((m) + o->mmax1) * o->nlat + (ilat)) * o->L1 + (l)]
+#define LEGB1(ilat, l, m) o->legb1[((size_t)((m) + o->mmax1) * o->L1 + (l)) * o->nlat + (ilat)]
+#define LEGF2(l, ilat, m) o->legf2[((size_t)((m) + o->mmax2) * o->nlat + (ilat)) * o->L2 + (l)]
+#define LEGB2(ilat, l, m) o->legb2[((size_t)((m) + o->mmax2) * o->L2 + (l)) * o->nlat + (ilat)]
+#define ALPH(l, m) o->alph[(size_t)((m) + o->mmax1) * o->L1 + (l)]
+
+/* ------------------------------------------------------------------ */
+/* Band Cholesky (LAPACK dpbtf2/dpbtrs 'L', netlib reference algorithm) */
+/* ab[(kd+1)*j + (i-j)] = A(i,j) for j <= i <= j+kd, 0-based            */
+/* ------------------------------------------------------------------ */
+static int band_chol(int n, int kd, double* ab) {
+  int ld = kd + 1;
+  for (int j = 0; j < n; ++j) {
+    double ajj = ab[(size_t)ld * j];
+    if (ajj <= 0.0) return j + 1;
+    ajj = sqrt(ajj);
+    ab[(size_t)ld * j] = ajj;
+    int kn = kd < n - 1 - j ? kd : n - 1 - j;
+    if (kn > 0) {
+      double r = 1.0 / ajj;
+      for (int i = 1; i <= kn; ++i) ab[(size_t)ld * j + i] *= r;
+      /* dsyr('L', kn, -1, x, A(j+1..)) */
+      for (int c = 0; c < kn; ++c) {
+        double xc = ab[(size_t)ld * j + 1 + c];
+        if (xc != 0.0) {
+          double t = -xc;
+          for (int r2 = c; r2 < kn; ++r2)
+            ab[(size_t)ld * (j + 1 + c) + (r2 - c)] += ab[(size_t)ld * j + 1 + r2] * t;
+        }
+      }
+    }
+  }
+  return 0;
+}
+
+static void band_chol_solve(int n, int kd, const double* ab, double* x) {
+  int ld = kd + 1;
+  /* dtbsv 'L','N','N' */
+  for (int j = 0; j < n; ++j) {
+    if (x[j] != 0.0) {
+      x[j] /= ab[(size_t)ld * j];
+      double t = x[j];
+      int iu = j + kd < n - 1 ? j + kd : n - 1;
+      for (int i = j + 1; i <= iu; ++i) x[i] -= t * ab[(size_t)ld * j + (i - j)];
+    }
+  }
+  /* dtbsv 'L','T','N' */
+  for (int j = n - 1; j >= 0; --j) {
+    double t = x[j];
+    int iu = j + kd < n - 1 ? j + kd : n - 1;
+    for (int i = iu; i >= j + 1; --i) t -= ab[(size_t)ld * j + (i - j)] * x[i];
+    t /= ab[(size_t)ld * j];
+    x[j] = t;
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* Operator tables (clbas::gen, src/bas/clbas.cpp:100-160)             */
+/* ------------------------------------------------------------------ */
+static int bas_gen(orc* o) {
+  int n = o->ndvr, nr = o->nr, L1 = o->L1, L2 = o->L2;
+  size_t bw = 2 * (size_t)n + 1;
+  int M1 = 2 * o->mmax1 + 1;
+  /* bas_gen_alph (src/bas/bas_gen_alph.F90:2-27) */
+  o->alph = xcalloc((size_t)L1 * M1, sizeof(double));
+  for (int m = -o->mmax1; m <= o->mmax1; ++m)
+    for (int l = abs(m); l <= o->lmax1; ++l) {
+      double dl = l, dm = m;
+      ALPH(l, m) = sqrt(((dl + 1.0) * (dl + 1.0) - dm * dm) / ((2.0 * dl + 1.0) * (2.0 * dl + 3.0)));
+    }
+  /* bas_gen_pmat (src/bas/bas_gen_pmat.F90), ecs off */
+  o->pmat = xcalloc(bw * nr, sizeof(zc));
+  for (int i = 0; i < nr; ++i) {
+    int irad = i + 1;
+    int jll = i - n > 0 ? i - n : 0, jul = i + n < nr - 1 ? i + n : nr - 1;
+    for (int j = jll; j <= jul; ++j) {
+      size_t jb = n + j - i;
+      o->pmat[bw * i + jb] = o->radp[bw * irad + jb];
+    }
+  }
+  /* bas_gen_tmat (src/bas/bas_gen_tmat.F90:2-118): pure Coulomb, ecs off */
+  o->tmat = xcalloc(bw * nr * L1, sizeof(zc));
+  for (int l = 0; l <= o->lmax1; ++l) {
+    double l2 = (double)(l * (l + 1));
+    for (int i = 0; i < nr; ++i) {
+      double oor = 1.0 / o->xrad[i + 1];
+      o->tmat[(bw * ((size_t)l * nr + i)) + n] = oor * (-o->znuc + l2 * oor * 0.5);
+    }
+  }
+  for (int l = 0; l <= o->lmax1; ++l) {
+    const double* rk = l == 0 ? o->radk : o->radk0;
+    for (int i = 0; i < nr; ++i) {
+      int jll = i - n > 0 ? i - n : 0, jul = i + n < nr - 1 ? i + n : nr - 1;
+      for (int j = jll; j <= jul; ++j) {
+        size_t jb = n + j - i;
+        o->tmat[bw * ((size_t)l * nr + i) + jb] += rk[bw * (i + 1) + jb];
+      }
+    }
+  }
+  /* bas_gen_d2ll (src/bas/bas_gen_d2ll.F90:2-140) */
+  size_t ld = n + 1;
+  o->d2ll = xcalloc(ld * nr * L2, sizeof(double));
+  for (int l = 0; l <= o->lmax2; ++l) {
+    double l2 = (double)(l * (l + 1));
+    double* ab = o->d2ll + ld * nr * l;
+    for (int i = 0; i < nr; ++i) {
+      double angk = 1.0 / o->xrad[i + 1];
+      ab[ld * i] = angk * angk * l2;
+    }
+    for (int i = 0; i < nr; ++i) {
+      int jul = i + n < nr - 1 ? i + n : nr - 1;
+      for (int j = i; j <= jul; ++j) ab[ld * i + (j - i)] += 2.0 * o->radk[bw * (i + 1) + n + (j - i)];
+    }
+    int info = band_chol(nr, n, ab);
+    if (info) {
+      fprintf(stderr, "oracle: dpbtrf failed l=%d info=%d\n", l, info);
+      return -1;
+    }
+  }
+  /* bas_gen_zfac, bas_gen_pzfac (fedvr_normalized, ecs off) */
+  o->zfac = xcalloc((size_t)nr * L1 * M1, sizeof(zc));
+  o->pzfac1 = xcalloc((size_t)L1 * M1, sizeof(zc));
+  o->pzfac2 = xcalloc((size_t)nr * L1 * M1, sizeof(zc));
+  for (int m = -o->mmax1; m <= o->mmax1; ++m)
+    for (int l = abs(m); l <= o->lmax1; ++l) {
+      double a = ALPH(l, m);
+      o->pzfac1[(size_t)(m + o->mmax1) * L1 + l] = -I * a;
+      zc tmp = -I * a * ((double)l + 1.0);
+      for (int i = 0; i < nr; ++i) {
+        size_t idx = ((size_t)(m + o->mmax1) * L1 + l) * nr + i;
+        o->zfac[idx] = o->xrad[i + 1] * a;
+        o->pzfac2[idx] = tmp / o->xrad[i + 1];
+      }
+    }
+  /* bas_gen_d2fac (src/bas/bas_gen_d2fac.F90:2-80) */
+  o->d2fac1 = xcalloc(L2, sizeof(double));
+  o->d2fac2 = xcalloc((size_t)nr * L2, sizeof(double));
+  o->d2invr = xcalloc((size_t)nr * L2, sizeof(double));
+  o->d2rpl0 = xcalloc((size_t)nr * L2, sizeof(double));
+  o->d2rpl1 = xcalloc((size_t)nr * L2, sizeof(double));
+  double r2pi = 0.5 / ORC_PI;
+  for (int l = 0; l <= o->lmax2; ++l)
+    for (int i = 0; i < nr; ++i) {
+      double x = o->xrad[i + 1], sw = sqrt(o->wrad[i + 1]);
+      size_t k = (size_t)l * nr + i;
+      o->d2rpl0[k] = pow(x, l) * r2pi;
+      o->d2rpl1[k] = pow(x, l + 1) * sw;
+      o->d2invr[k] = (2 * l + 1) / (x * sw) * r2pi;
+      o->d2fac2[k] = 4.0 * ORC_PI / (2 * l + 1) / (x * sw);
+    }
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* CAS determinant space (ormas_init with nsub = 1)                    */
+/* ------------------------------------------------------------------ */
+static int popcount32(uint32_t v) { return __builtin_popcount(v); }
+
+static int make_strings(int nact, int nel, uint32_t** out) {
+  /* ascending lexicographic order of occupied-index tuples */
+  int cnt = 0, cap = 16;
+  uint32_t* s = xcalloc(cap, sizeof(uint32_t));
+  if (nel == 0) {
+    s[0] = 0;
+    *out = s;
+    return 1;
+  }
+  int idx[32];
+  for (int k = 0; k < nel; ++k) idx[k] = k;
+  for (;;) {
+    uint32_t v = 0;
+    for (int k = 0; k < nel; ++k) v |= 1u << idx[k];
+    if (cnt == cap) {
+      cap *= 2;
+      s = realloc(s, cap * sizeof(uint32_t));
+    }
+    s[cnt++] = v;
+    int k = nel - 1;
+    while (k >= 0 && idx[k] == nact - nel + k) --k;
+    if (k < 0) break;
+    idx[k]++;
+    for (int j = k + 1; j < nel; ++j) idx[j] = idx[j - 1] + 1;
+  }
+  *out = s;
+  return cnt;
+}
+
+static int str_m(const orc* o, uint32_t s) {
+  int m = 0;
+  for (int k = 0; k < o->nact; ++k)
+    if (s >> k & 1u) m += o->mval[o->ncore + k];
+  return m;
+}
+
+/* a+_p a_h |s>: returns sign (0 if zero), new string in *t */
+static int excite(uint32_t s, int p, int h, uint32_t* t) {
+  if (!(s >> h & 1u)) return 0;
+  if (p == h) { *t = s; return 1; }
+  if (s >> p & 1u) return 0;
+  uint32_t lo = p < h ? p : h, hi = p < h ? h : p;
+  uint32_t between = s & (((1u << hi) - 1u) & ~((1u << (lo + 1)) - 1u));
+  *t = (s & ~(1u << h)) | (1u << p);
+  return (popcount32(between) & 1) ? -1 : 1;
+}
+
+static void ci_setup(orc* o) {
+  if (o->nact == 0) {
+    o->lcic = 1;
+    return;
+  }
+  int na = o->nelact[0], nb = o->nelact[1];
+  o->nstr_a = make_strings(o->nact, na, &o->str_a);
+  o->nstr_b = make_strings(o->nact, nb, &o->str_b);
+  size_t nfull = (size_t)o->nstr_a * o->nstr_b;
+  /* full alpha x beta product space (index ib*nstr_a + ia); the CI vector
+   * holds the determinants with M = mtot, beta-major, alpha fastest */
+  o->det_of = xcalloc(nfull, sizeof(int));
+  int cnt = 0;
+  for (int ib = 0; ib < o->nstr_b; ++ib)
+    for (int ia = 0; ia < o->nstr_a; ++ia) {
+      int ok = str_m(o, o->str_a[ia]) + str_m(o, o->str_b[ib]) == o->mtot;
+      o->det_of[(size_t)ib * o->nstr_a + ia] = ok ? cnt++ : -1;
+    }
+  o->lcic = cnt;
+  o->det_a = xcalloc(cnt, sizeof(int));
+  o->det_b = xcalloc(cnt, sizeof(int));
+  for (int ib = 0; ib < o->nstr_b; ++ib)
+    for (int ia = 0; ia < o->nstr_a; ++ia) {
+      int d = o->det_of[(size_t)ib * o->nstr_a + ia];
+      if (d >= 0) {
+        o->det_a[d] = ia;
+        o->det_b[d] = ib;
+      }
+    }
+  o->ci_work = xcalloc((size_t)(o->nact * o->nact + 3) * nfull, sizeof(zc));
+}
+
+static int find_str(const uint32_t* s, int n, uint32_t v) {
+  for (int k = 0; k < n; ++k)
+    if (s[k] == v) return k;
+  return -1;
+}
+
+/* y += fac * E_pq x on full-space vectors (E_pq = sum_sigma a+_p a_q) */
+static void ci_apply_E(const orc* o, int p, int q, zc fac, const zc* x, zc* y) {
+  int na = o->nstr_a, nb = o->nstr_b;
+  for (int ib = 0; ib < nb; ++ib)
+    for (int ia = 0; ia < na; ++ia) {
+      zc xv = x[(size_t)ib * na + ia];
+      if (xv == 0.0) continue;
+      uint32_t t;
+      int sg = excite(o->str_a[ia], p, q, &t);
+      if (sg) y[(size_t)ib * na + find_str(o->str_a, na, t)] += fac * sg * xv;
+      sg = excite(o->str_b[ib], p, q, &t);
+      if (sg) y[(size_t)find_str(o->str_b, nb, t) * na + ia] += fac * sg * xv;
+    }
+}
+
+static void ci_expand(const orc* o, const zc* cic, zc* full) {
+  size_t nfull = (size_t)o->nstr_a * o->nstr_b;
+  for (size_t f = 0; f < nfull; ++f) full[f] = o->det_of[f] >= 0 ? cic[o->det_of[f]] : 0;
+}
+
+/* ormas_mkden1 / ormas_mkden2 (src/ormas/ormas_mkden1.F90:2-30,
+ * ormas_mkden2.F90:2-40): den1(a,b) = <E_ba>,
+ * den2(a,b,c,d) = <a+_b a+_d a_c a_a> = <E_ba E_dc> - delta_ad <E_bc>.
+ * Fortran column-major: den1[a + n*b], den2[a + n*(b + n*(c + n*d))]. */
+static void ci_mkden(orc* o, const zc* cic) {
+  int n = o->nact;
+  if (n == 0) return;
+  size_t nc = (size_t)o->nstr_a * o->nstr_b;
+  zc* D = o->ci_work; /* D[(p*n+q)*nc + f] = (E_pq C) */
+  zc* C = D + (size_t)n * n * nc;
+  memset(D, 0, sizeof(zc) * n * n * nc);
+  ci_expand(o, cic, C);
+  for (int p = 0; p < n; ++p)
+    for (int q = 0; q < n; ++q) ci_apply_E(o, p, q, 1.0, C, D + (size_t)(p * n + q) * nc);
+  for (int a = 0; a < n; ++a)
+    for (int b = 0; b < n; ++b) {
+      zc s = 0;
+      const zc* e = D + (size_t)(b * n + a) * nc;
+      for (size_t d = 0; d < nc; ++d) s += conj(C[d]) * e[d];
+      o->den1[a + n * b] = s;
+    }
+  for (int a = 0; a < n; ++a)
+    for (int b = 0; b < n; ++b)
+      for (int c = 0; c < n; ++c)
+        for (int dd = 0; dd < n; ++dd) {
+          /* <C| E_ba E_dc |C> = (E_ab C)^+ (E_dc C) */
+          const zc* u = D + (size_t)(a * n + b) * nc;
+          const zc* v = D + (size_t)(dd * n + c) * nc;
+          zc s = 0;
+          for (size_t k = 0; k < nc; ++k) s += conj(u[k]) * v[k];
+          if (a == dd) s -= o->den1[c + n * b];
+          o->den2[a + n * (b + n * (c + n * dd))] = s;
+        }
+}
+
+/* sig += H C with H = sum_pq int1e(p,q) E_pq
+ *                   + 1/2 sum_pqrs int2e(p,q,r,s) (E_pq E_rs - delta_qr E_ps)
+ * (the Hamiltonian ormas_hcic_ras applies, src/ormas/ormas_hcic_ras.F90) */
+static void ci_sigma(orc* o, const zc* cic, zc* sig) {
+  int n = o->nact;
+  size_t nc = (size_t)o->nstr_a * o->nstr_b;
+  zc* D = o->ci_work;
+  zc* C = D + (size_t)n * n * nc;
+  zc* S = C + nc;
+  zc* tmp = S + nc;
+  memset(D, 0, sizeof(zc) * n * n * nc);
+  memset(S, 0, sizeof(zc) * nc);
+  ci_expand(o, cic, C);
+  for (int r = 0; r < n; ++r)
+    for (int s = 0; s < n; ++s) ci_apply_E(o, r, s, 1.0, C, D + (size_t)(r * n + s) * nc);
+  for (int p = 0; p < n; ++p)
+    for (int q = 0; q < n; ++q) {
+      zc hpq = o->int1e[p + n * q];
+      for (int r = 0; r < n; ++r) hpq -= 0.5 * o->int2e[p + n * (r + n * (r + n * q))];
+      const zc* e = D + (size_t)(p * n + q) * nc;
+      for (size_t d = 0; d < nc; ++d) S[d] += hpq * e[d];
+    }
+  for (int p = 0; p < n; ++p)
+    for (int q = 0; q < n; ++q) {
+      memset(tmp, 0, sizeof(zc) * nc);
+      for (int r = 0; r < n; ++r)
+        for (int s = 0; s < n; ++s) {
+          zc g = 0.5 * o->int2e[p + n * (q + n * (r + n * s))];
+          if (g == 0.0) continue;
+          const zc* e = D + (size_t)(r * n + s) * nc;
+          for (size_t d = 0; d < nc; ++d) tmp[d] += g * e[d];
+        }
+      ci_apply_E(o, p, q, 1.0, tmp, S);
+    }
+  for (size_t f = 0; f < nc; ++f)
+    if (o->det_of[f] >= 0) sig[o->det_of[f]] += S[f];
+}
+
+/* ------------------------------------------------------------------ */
+/* Hermitian eigen-decomposition (zheev stand-in, cyclic Jacobi)       */
+/* a[i + n*j] Hermitian in, eigenvalues w[], eigenvectors u[:,k]        */
+/* ------------------------------------------------------------------ */
+static void herm_eig(int n, const zc* a_in, double* w, zc* u) {
+  zc* a = xcalloc((size_t)n * n, sizeof(zc));
+  memcpy(a, a_in, sizeof(zc) * n * n);
+  for (int i = 0; i < n * n; ++i) u[i] = 0;
+  for (int i = 0; i < n; ++i) u[i + n * i] = 1.0;
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0;
+    for (int p = 0; p < n; ++p)
+      for (int q = p + 1; q < n; ++q) off += creal(a[p + n * q] * conj(a[p + n * q]));
+    if (off < 1e-300) break;
+    for (int p = 0; p < n; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        zc apq = a[p + n * q];
+        double mag = cabs(apq);
+        if (mag < 1e-300) continue;
+        double app = creal(a[p + n * p]), aqq = creal(a[q + n * q]);
+        double theta = 0.5 * atan2(2.0 * mag, aqq - app);
+        double c = cos(theta), s = sin(theta);
+        zc ph = apq / mag; /* e^{i phi} */
+        /* rotation J: columns p,q: [c, s*ph; -s*conj(ph), c] applied as A <- J^H A J */
+        for (int k = 0; k < n; ++k) {
+          zc akp = a[k + n * p], akq = a[k + n * q];
+          a[k + n * p] = c * akp - s * conj(ph) * akq;
+          a[k + n * q] = s * ph * akp + c * akq;
+        }
+        for (int k = 0; k < n; ++k) {
+          zc apk = a[p + n * k], aqk = a[q + n * k];
+          a[p + n * k] = c * apk - s * ph * aqk;
+          a[q + n * k] = s * conj(ph) * apk + c * aqk;
+        }
+        for (int k = 0; k < n; ++k) {
+          zc ukp = u[k + n * p], ukq = u[k + n * q];
+          u[k + n * p] = c * ukp - s * conj(ph) * ukq;
+          u[k + n * q] = s * ph * ukp + c * ukq;
+        }
+      }
+  }
+  for (int i = 0; i < n; ++i) w[i] = creal(a[i + n * i]);
+  free(a);
+}
+
+/* hprod_invden_inv (src/hprod/hprod_invden.F90:62-107) */
+static void invden_inv(int n, double thresh, const zc* a, zc* inva) {
+  double* w = xcalloc(n, sizeof(double));
+  zc* u = xcalloc((size_t)n * n, sizeof(zc));
+  herm_eig(n, a, w, u);
+  for (int i = 0; i < n * n; ++i) inva[i] = 0;
+  for (int k = 0; k < n; ++k) {
+    double diag = w[k];
+    double invd = thresh > 0.0 ? diag / (diag * diag + thresh) : 1.0 / diag;
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i) inva[i + n * j] += u[i + n * k] * invd * conj(u[j + n * k]);
+  }
+  free(w);
+  free(u);
+}
+
+/* ------------------------------------------------------------------ */
+/* Context creation                                                    */
+/* ------------------------------------------------------------------ */
+orc* orc_create(const td1c_problem* p) {
+  orc* o = xcalloc(1, sizeof(orc));
+  o->nfe = p->nfe;
+  o->ndvr = p->ndvr;
+  o->x0 = xcalloc(p->nfe, sizeof(double));
+  o->x1 = xcalloc(p->nfe, sizeof(double));
+  for (int i = 0; i < p->nfe; ++i) {
+    o->x0[i] = p->edges[i];
+    o->x1[i] = p->edges[i + 1];
+  }
+  o->lmax1 = p->lmax1;
+  o->lmax2 = p->lmax2;
+  o->L1 = p->lmax1 + 1;
+  o->L2 = p->lmax2 + 1;
+  o->nlat = p->nlat > 0 ? p->nlat : p->lmax2 + 1;
+  o->znuc = p->znuc;
+  o->nfcore2 = p->nfcore2;
+  o->nfcore1 = p->nfcore1;
+  o->nfcore = p->nfcore2 + p->nfcore1;
+  o->ndcore = p->ndcore;
+  o->ncore = o->nfcore + p->ndcore;
+  o->nact = p->nact;
+  o->nfun = o->ncore + p->nact;
+  o->mval = xcalloc(o->nfun, sizeof(int));
+  o->lval = xcalloc(o->nfun, sizeof(int));
+  int mmax = 0;
+  for (int i = 0; i < o->nfun; ++i) {
+    o->mval[i] = p->mval ? p->mval[i] : 0;
+    o->lval[i] = p->lval ? p->lval[i] : 0;
+    if (abs(o->mval[i]) > mmax) mmax = abs(o->mval[i]);
+  }
+  o->mmax1 = p->mmax1 >= 0 ? p->mmax1 : mmax;
+  o->mmax2 = p->mmax2 >= 0 ? p->mmax2 : 2 * o->mmax1;
+  o->nel[0] = p->nel_alpha;
+  o->nel[1] = p->nel_beta;
+  o->nel[2] = p->nel_alpha + p->nel_beta;
+  o->nelact[0] = p->nel_alpha - o->ncore;
+  o->nelact[1] = p->nel_beta - o->ncore;
+  o->nelact[2] = o->nelact[0] + o->nelact[1];
+  o->mtot = p->mtot;
+  o->icomp = p->icomp;
+  o->igauge = p->igauge;
+  o->isplit = p->isplit;
+  o->ioorot = p->ioorot;
+  o->iprojfc = p->iprojfc;
+  o->jfc_implicit = p->jfc_implicit;
+  o->xfc_implicit = p->xfc_implicit;
+  o->throcc1 = p->throcc1;
+  o->throcc2 = p->throcc2;
+  double R = p->edges[p->nfe];
+  o->rmask = p->rmask > 0 ? p->rmask : 0.8 * R;
+  o->mask_order = p->mask_cos_order > 0 ? p->mask_cos_order : 4.0;
+
+  fedvr_gen(o);
+  o->nr = o->nrad - 1;
+  o->nradfc = p->nradfc > 0 ? p->nradfc : o->nr;
+  sph_gen(o);
+  if (bas_gen(o)) return NULL;
+  ci_setup(o);
+
+  int nr = o->nr, nf = o->nfun;
+  o->nbas = nr * o->L1;
+  o->nbas2 = nr * o->L2;
+  o->ngrid = nr * o->nlat;
+  size_t sob = (size_t)o->nbas * nf, sog = (size_t)o->ngrid * nf;
+  o->orb = xcalloc(sob, sizeof(zc));
+  o->h0orb = xcalloc(sob, sizeof(zc));
+  o->h1orb = xcalloc(sob, sizeof(zc));
+  o->gorb = xcalloc(sob, sizeof(zc));
+  o->v2orb = xcalloc(sob, sizeof(zc));
+  o->orb0 = xcalloc(sob, sizeof(zc));
+  o->orbg = xcalloc(sog, sizeof(zc));
+  o->gorbg = xcalloc(sog, sizeof(zc));
+  o->v2orbg = xcalloc(sog, sizeof(zc));
+  o->torbg = xcalloc(sog, sizeof(zc));
+  o->rho2 = xcalloc((size_t)o->nbas2 * nf * nf, sizeof(zc));
+  o->v2sph = xcalloc((size_t)o->nbas2 * nf * nf, sizeof(zc));
+  o->v2ang = xcalloc((size_t)o->ngrid * nf * nf, sizeof(zc));
+  o->rhofc = xcalloc(nr, sizeof(zc));
+  o->v2jfc = xcalloc(nr, sizeof(zc));
+  o->v2xfc = xcalloc(nr, sizeof(zc));
+  int na = o->nact > 0 ? o->nact : 1;
+  o->den1 = xcalloc((size_t)na * na, sizeof(zc));
+  o->rden = xcalloc((size_t)na * na, sizeof(zc));
+  o->rrden = xcalloc((size_t)na * na, sizeof(zc));
+  o->int1e = xcalloc((size_t)na * na, sizeof(zc));
+  o->den2 = xcalloc((size_t)na * na * na * na, sizeof(zc));
+  o->int2e = xcalloc((size_t)na * na * na * na, sizeof(zc));
+  o->xmat = xcalloc((size_t)nf * nf, sizeof(zc));
+  o->fmat = xcalloc((size_t)nf * nf, sizeof(zc));
+  o->bmat = xcalloc((size_t)nf * nf, sizeof(zc));
+  o->dcic = xcalloc(o->lcic, sizeof(zc));
+  size_t wsz = sob + o->lcic;
+  o->rk_w0 = xcalloc(wsz, sizeof(zc));
+  o->rk_t = xcalloc(wsz, sizeof(zc));
+  o->rk_k = xcalloc(wsz * 6, sizeof(zc));
+  return o;
+}
+
+void orc_destroy(orc* o) {
+  if (!o) return;
+  void* ptrs[] = {o->x0, o->x1, o->dvr_x, o->dvr_w, o->dshape, o->mapf, o->mapb, o->xrad, o->wrad,
+                  o->mask, o->radk, o->radk0, o->radp, o->cost, o->wlat, o->legf1, o->legb1,
+                  o->legf2, o->legb2, o->alph, o->tmat, o->pmat, o->d2ll, o->zfac, o->pzfac1,
+                  o->pzfac2, o->d2fac1, o->d2fac2, o->d2invr, o->d2rpl0, o->d2rpl1, o->orb,
+                  o->h0orb, o->h1orb, o->gorb, o->v2orb, o->orbg, o->gorbg, o->v2orbg, o->torbg,
+                  o->rho2, o->v2sph, o->v2ang, o->orb0, o->rhofc, o->v2jfc, o->v2xfc, o->den1,
+                  o->rden, o->rrden, o->int1e, o->den2, o->int2e, o->xmat, o->fmat, o->bmat,
+                  o->dcic, o->str_a, o->str_b, o->det_a, o->det_b, o->det_of, o->ci_work,
+                  o->rk_w0, o->rk_t, o->rk_k, o->mval, o->lval};
+  for (size_t k = 0; k < sizeof(ptrs) / sizeof(ptrs[0]); ++k) free(ptrs[k]);
+  free(o);
+}
+
+/* ------------------------------------------------------------------ */
+/* hprod building blocks                                               */
+/* ------------------------------------------------------------------ */
+#define OIDX(irad, l, ifun) (((size_t)(ifun) * o->L1 + (l)) * o->nr + (irad))
+#define GIDX(irad, ilat, ifun) (((size_t)(ifun) * o->nlat + (ilat)) * o->nr + (irad))
+#define SIDX2(irad, l, jfun, ifun) ((((size_t)(ifun) * o->nfun + (jfun)) * o->L2 + (l)) * o->nr + (irad))
+#define GIDX2(irad, ilat, jfun, ifun) ((((size_t)(ifun) * o->nfun + (jfun)) * o->nlat + (ilat)) * o->nr + (irad))
+
+/* bas_sph2ang1 restricted to orbitals [f0,f1) and radial rows [0,dim)
+ * (src/bas/bas_sph2ang1.F90:88-156 and the _fc / _radfc variants) */
+static void sph2ang1(orc* o, const zc* orb1, zc* orb2, int f0, int f1, int dim) {
+  for (int ifun = f0; ifun < f1; ++ifun) {
+    int mi = o->mval[ifun];
+    for (int ilat = 0; ilat < o->nlat; ++ilat) {
+      zc* out = orb2 + GIDX(0, ilat, ifun);
+      for (int i = 0; i < dim; ++i) out[i] = 0;
+      for (int l = abs(mi); l <= o->lmax1; ++l) {
+        double f = LEGF1(l, ilat, mi);
+        const zc* in = orb1 + OIDX(0, l, ifun);
+        for (int i = 0; i < dim; ++i) out[i] += in[i] * f;
+      }
+    }
+  }
+}
+
+/* bas_ang2sph1 (src/bas/bas_ang2sph1.F90:88-157) */
+static void ang2sph1(orc* o, const zc* orb1, zc* orb2, int f0, int f1, int dim) {
+  for (int ifun = f0; ifun < f1; ++ifun) {
+    int mi = o->mval[ifun];
+    for (int l = abs(mi); l <= o->lmax1; ++l) {
+      zc* out = orb2 + OIDX(0, l, ifun);
+      for (int i = 0; i < dim; ++i) out[i] = 0;
+      for (int ilat = 0; ilat < o->nlat; ++ilat) {
+        double b = LEGB1(ilat, l, mi);
+        const zc* in = orb1 + GIDX(0, ilat, ifun);
+        for (int i = 0; i < dim; ++i) out[i] += in[i] * b;
+      }
+    }
+  }
+}
+
+/* hprod_tprodp (src/hprod/hprod_tprod.F90:152-235): overwrites h for
+ * l >= |m|, rows [0, dim) */
+static void tprod(orc* o, const zc* wfn, zc* hwfn, int f0, int f1, int dim) {
+  int n = o->ndvr, nr = o->nr;
+  size_t bw = 2 * (size_t)n + 1;
+  for (int ifun = f0; ifun < f1; ++ifun)
+    for (int l = abs(o->mval[ifun]); l <= o->lmax1; ++l)
+      for (int i = 0; i < dim; ++i) {
+        int jll = i - n > 0 ? i - n : 0, jul = i + n < nr - 1 ? i + n : nr - 1;
+        const zc* t = o->tmat + bw * ((size_t)l * nr + i);
+        zc s = 0;
+        for (int j = jll; j <= jul; ++j) s += wfn[OIDX(j, l, ifun)] * t[n + j - i];
+        hwfn[OIDX(i, l, ifun)] = s;
+      }
+}
+
+/* hprod_zprodp (src/hprod/hprod_zprod.F90:108-135) */
+static void zprod(orc* o, zc zf, const zc* wfn, zc* hwfn, int f0, int f1, int dim) {
+  for (int ifun = f0; ifun < f1; ++ifun) {
+    int m = o->mval[ifun];
+    for (int l = abs(m); l <= o->lmax1 - 1; ++l)
+      for (int i = 0; i < dim; ++i) {
+        zc tmp = zf * o->zfac[((size_t)(m + o->mmax1) * o->L1 + l) * o->nr + i];
+        hwfn[OIDX(i, l, ifun)] += wfn[OIDX(i, l + 1, ifun)] * tmp;
+        hwfn[OIDX(i, l + 1, ifun)] += wfn[OIDX(i, l, ifun)] * tmp;
+      }
+  }
+}
+
+/* hprod_pzprodp (src/hprod/hprod_pzprod.F90:114-213), ecs off */
+static void pzprod(orc* o, zc zf, const zc* wfn, zc* hwfn, int f0, int f1, int dim) {
+  int n = o->ndvr, nr = o->nr, L1 = o->L1;
+  size_t bw = 2 * (size_t)n + 1;
+  zc* dw = xcalloc((size_t)nr * L1, sizeof(zc));
+  for (int ifun = f0; ifun < f1; ++ifun) {
+    int m = o->mval[ifun];
+    memset(dw, 0, sizeof(zc) * nr * L1);
+    for (int l = abs(m); l <= o->lmax1; ++l)
+      for (int i = 0; i < dim; ++i) {
+        int jll = i - n > 0 ? i - n : 0, jul = i + n < nr - 1 ? i + n : nr - 1;
+        zc s = 0;
+        for (int j = jll; j <= jul; ++j) s += wfn[OIDX(j, l, ifun)] * o->pmat[bw * i + n + j - i];
+        dw[(size_t)l * nr + i] += s;
+      }
+    for (int l = abs(m); l <= o->lmax1 - 1; ++l) {
+      zc tmp1 = zf * o->pzfac1[(size_t)(m + o->mmax1) * L1 + l];
+      for (int i = 0; i < dim; ++i) {
+        zc tmp2 = zf * o->pzfac2[((size_t)(m + o->mmax1) * L1 + l) * nr + i];
+        hwfn[OIDX(i, l, ifun)] += dw[(size_t)(l + 1) * nr + i] * tmp1 + wfn[OIDX(i, l + 1, ifun)] * tmp2;
+        hwfn[OIDX(i, l + 1, ifun)] += dw[(size_t)l * nr + i] * tmp1 - wfn[OIDX(i, l, ifun)] * tmp2;
+      }
+    }
+  }
+  free(dw);
+}
+
+/* pair ranges of hprod_mkrho2p / hprod_mkv2mf_poisson / _herm:
+ * type 0 = 'dyn', 1 = 'fc1dyn', 2 = 'fcfc' */
+static void pair_range(orc* o, int type, int* dim, int* i0, int* i1, int* j0, int* j1) {
+  if (type == 0) {
+    *dim = o->nr; *i0 = o->nfcore; *i1 = o->nfun; *j0 = o->nfcore; *j1 = o->nfun;
+  } else if (type == 1) {
+    *dim = o->nradfc; *i0 = o->nfcore2; *i1 = o->nfcore; *j0 = o->nfcore; *j1 = o->nfun;
+  } else {
+    *dim = o->nr; *i0 = 0; *i1 = o->nfcore; *j0 = 0; *j1 = o->nfcore;
+  }
+}
+
+/* hprod_mkrho2p (src/hprod/hprod_mkrho2.F90:115-197) followed by
+ * bas_ang2sph2 of the same range (src/bas/bas_ang2sph2.F90:330-482) */
+static void mkrho2(orc* o, int type) {
+  int dim, i0, i1, j0, j1;
+  pair_range(o, type, &dim, &i0, &i1, &j0, &j1);
+  for (int ifun = i0; ifun < i1; ++ifun)
+    for (int jfun = (j0 > ifun ? j0 : ifun); jfun < (j1 > ifun + 1 ? j1 : ifun + 1); ++jfun)
+      for (int ilat = 0; ilat < o->nlat; ++ilat)
+        for (int i = 0; i < dim; ++i)
+          o->v2ang[GIDX2(i, ilat, jfun, ifun)] =
+              conj(o->orbg[GIDX(i, ilat, jfun)]) * o->orbg[GIDX(i, ilat, ifun)];
+  for (int ifun = i0; ifun < i1; ++ifun)
+    for (int jfun = (j0 > ifun ? j0 : ifun); jfun < (j1 > ifun + 1 ? j1 : ifun + 1); ++jfun) {
+      int mji = -o->mval[jfun] + o->mval[ifun];
+      for (int l = 0; l <= o->lmax2; ++l) {
+        zc* out = o->rho2 + SIDX2(0, l, jfun, ifun);
+        for (int i = 0; i < dim; ++i) out[i] = 0;
+        if (l < abs(mji)) continue;
+        for (int ilat = 0; ilat < o->nlat; ++ilat) {
+          double b = LEGB2(ilat, l, mji);
+          const zc* in = o->v2ang + GIDX2(0, ilat, jfun, ifun);
+          for (int i = 0; i < dim; ++i) out[i] += in[i] * b;
+        }
+      }
+    }
+}
+
+/* hprod_mkv2mf_poissonp (src/hprod/hprod_mkv2mf.F90:118-182) */
+static void poisson(orc* o, int type) {
+  int dim, i0, i1, j0, j1;
+  pair_range(o, type, &dim, &i0, &i1, &j0, &j1);
+  int nr = o->nr, n = o->ndvr;
+  double* rr = xcalloc((size_t)dim * 2, sizeof(double));
+  for (int l = 0; l <= o->lmax2; ++l) o->d2fac1[l] = 1.0 / pow(o->xrad[dim + 1], 2 * l + 1);
+  for (int ifun = i0; ifun < i1; ++ifun)
+    for (int jfun = (j0 > ifun ? j0 : ifun); jfun < (j1 > ifun + 1 ? j1 : ifun + 1); ++jfun)
+      for (int l = 0; l <= o->lmax2; ++l) {
+        int m = -o->mval[ifun] + o->mval[jfun];
+        zc* v = o->v2sph + SIDX2(0, l, jfun, ifun);
+        if (l < abs(m)) {
+          for (int i = 0; i < dim; ++i) v[i] = 0;
+          continue;
+        }
+        const zc* rho = o->rho2 + SIDX2(0, l, jfun, ifun);
+        const double* rpl0 = o->d2rpl0 + (size_t)l * nr;
+        const double* invr = o->d2invr + (size_t)l * nr;
+        double tr = 0, ti = 0;
+        for (int i = 0; i < dim; ++i) {
+          rr[i] = creal(rho[i]);
+          rr[dim + i] = cimag(rho[i]);
+          tr += rr[i] * rpl0[i];
+          ti += rr[dim + i] * rpl0[i];
+          rr[i] *= invr[i];
+          rr[dim + i] *= invr[i];
+        }
+        tr *= o->d2fac1[l];
+        ti *= o->d2fac1[l];
+        const double* ab = o->d2ll + (size_t)(n + 1) * nr * l;
+        band_chol_solve(dim, n, ab, rr);
+        band_chol_solve(dim, n, ab, rr + dim);
+        const double* rpl1 = o->d2rpl1 + (size_t)l * nr;
+        const double* f2 = o->d2fac2 + (size_t)l * nr;
+        for (int i = 0; i < dim; ++i)
+          v[i] = (rr[i] + tr * rpl1[i]) * f2[i] + (rr[dim + i] + ti * rpl1[i]) * f2[i] * I;
+      }
+  free(rr);
+}
+
+/* bas_sph2ang2 of a pair range (src/bas/bas_sph2ang2.F90:334-463) */
+static void sph2ang2(orc* o, int type) {
+  int dim, i0, i1, j0, j1;
+  pair_range(o, type, &dim, &i0, &i1, &j0, &j1);
+  for (int ifun = i0; ifun < i1; ++ifun)
+    for (int jfun = (j0 > ifun ? j0 : ifun); jfun < (j1 > ifun + 1 ? j1 : ifun + 1); ++jfun) {
+      int mji = -o->mval[jfun] + o->mval[ifun];
+      for (int ilat = 0; ilat < o->nlat; ++ilat) {
+        zc* out = o->v2ang + GIDX2(0, ilat, jfun, ifun);
+        for (int i = 0; i < dim; ++i) out[i] = 0;
+        for (int l = abs(mji); l <= o->lmax2; ++l) {
+          double f = LEGF2(l, ilat, mji);
+          const zc* in = o->v2sph + SIDX2(0, l, jfun, ifun);
+          for (int i = 0; i < dim; ++i) out[i] += in[i] * f;
+        }
+      }
+    }
+}
+
+/* hprod_mkv2mf_herm (src/hprod/hprod_mkv2mf.F90:257-345) */
+static void v2herm(orc* o, int type) {
+  int dim, i0, i1, j0, j1;
+  pair_range(o, type, &dim, &i0, &i1, &j0, &j1);
+  for (int ifun = i0; ifun < i1; ++ifun)
+    for (int jfun = (j0 > ifun + 1 ? j0 : ifun + 1); jfun < j1; ++jfun) {
+      for (int l = 0; l <= o->lmax2; ++l)
+        for (int i = 0; i < dim; ++i)
+          o->v2sph[SIDX2(i, l, ifun, jfun)] = conj(o->v2sph[SIDX2(i, l, jfun, ifun)]);
+      for (int ilat = 0; ilat < o->nlat; ++ilat)
+        for (int i = 0; i < dim; ++i)
+          o->v2ang[GIDX2(i, ilat, ifun, jfun)] = conj(o->v2ang[GIDX2(i, ilat, jfun, ifun)]);
+    }
+}
+
+/* hprod_mfprod2 (src/hprod/hprod_mfprod2.F90:2-34 and subroutines
+ * fcx1 :158-200, dcj :238-272, dcx :273-321, actj :322-388,
+ * actx :389-461, act2 :540-604), jfc_implicit = xfc_implicit = true */
+static void mfprod2(orc* o) {
+  int nr = o->nr, nlat = o->nlat, ng = o->ngrid;
+  int nfc = o->nfcore, nfc2 = o->nfcore2, ncore = o->ncore, nf = o->nfun, na = o->nact;
+  zc* v2 = o->v2ang;
+#define V2G(ig, jfun, ifun) v2[((size_t)(ifun) * nf + (jfun)) * ng + (ig)]
+#define WG(ig, ifun) o->orbg[(size_t)(ifun) * ng + (ig)]
+  /* fcx1 */
+  if (o->nfcore1 > 0) {
+    int dim = o->nradfc;
+    for (int ifun = nfc; ifun < nf; ++ifun)
+      for (int jfun = nfc2; jfun < nfc; ++jfun)
+        for (int ilat = 0; ilat < nlat; ++ilat)
+          for (int i = 0; i < dim; ++i) {
+            size_t ig = (size_t)ilat * nr + i;
+            o->gorbg[(size_t)ifun * ng + ig] -= V2G(ig, jfun, ifun) * WG(ig, jfun);
+          }
+  }
+  zc* d2v = xcalloc((size_t)ng * (na > 0 ? na : 1), sizeof(zc));
+  /* dcj */
+  if (o->ndcore > 0) {
+    for (int ig = 0; ig < ng; ++ig) d2v[ig] = 0;
+    for (int jfun = nfc; jfun < ncore; ++jfun)
+      for (int ig = 0; ig < ng; ++ig) d2v[ig] += V2G(ig, jfun, jfun) * 2.0;
+    for (int ifun = nfc; ifun < nf; ++ifun)
+      for (int ig = 0; ig < ng; ++ig) o->gorbg[(size_t)ifun * ng + ig] += d2v[ig] * WG(ig, ifun);
+    /* dcx */
+    for (int ifun = nfc; ifun < nf; ++ifun)
+      for (int jfun = nfc; jfun < ncore; ++jfun)
+        for (int ig = 0; ig < ng; ++ig)
+          o->gorbg[(size_t)ifun * ng + ig] -= V2G(ig, jfun, ifun) * WG(ig, jfun);
+  }
+  if (ncore > 0 && na > 0) {
+    /* actj */
+    for (int ig = 0; ig < ng; ++ig) d2v[ig] = 0;
+    for (int ja = 0; ja < na; ++ja)
+      for (int ka = 0; ka < na; ++ka) {
+        zc dens = o->den1[ka + na * ja];
+        for (int ig = 0; ig < ng; ++ig) d2v[ig] += V2G(ig, ncore + ja, ncore + ka) * dens;
+      }
+    for (int ifun = nfc; ifun < ncore; ++ifun)
+      for (int ig = 0; ig < ng; ++ig) o->v2orbg[(size_t)ifun * ng + ig] += d2v[ig] * WG(ig, ifun);
+    /* actx */
+    for (int ja = 0; ja < na; ++ja) {
+      int jfun = ncore + ja;
+      zc* wd = d2v + (size_t)ja * ng;
+      for (int ig = 0; ig < ng; ++ig) wd[ig] = 0;
+      for (int ka = 0; ka < na; ++ka) {
+        int kfun = ncore + ka;
+        if (o->mval[jfun] == o->mval[kfun]) {
+          zc dens = -o->den1[ka + na * ja] * 0.5;
+          for (int ig = 0; ig < ng; ++ig) wd[ig] += WG(ig, kfun) * dens;
+        }
+      }
+      for (int ifun = nfc; ifun < ncore; ++ifun)
+        for (int ig = 0; ig < ng; ++ig) o->v2orbg[(size_t)ifun * ng + ig] += wd[ig] * V2G(ig, jfun, ifun);
+    }
+  }
+  /* act2 */
+  if (na > 0 && o->nelact[2] >= 2) {
+    for (size_t k = 0; k < (size_t)ng * nf; ++k) o->torbg[k] = 0;
+    for (int ia = 0; ia < na; ++ia) {
+      int ifun = ncore + ia;
+      for (int ja = 0; ja <= ia; ++ja) {
+        int jfun = ncore + ja;
+        for (int ig = 0; ig < ng; ++ig) d2v[ig] = 0;
+        for (int ka = 0; ka < na; ++ka) {
+          int kfun = ncore + ka;
+          for (int la = 0; la < na; ++la) {
+            int lfun = ncore + la;
+            if (o->mval[ifun] + o->mval[kfun] == o->mval[jfun] + o->mval[lfun]) {
+              zc dens = o->den2[ja + na * (ia + na * (la + na * ka))];
+              if (cabs(dens) > 1e-15)
+                for (int ig = 0; ig < ng; ++ig) d2v[ig] += dens * V2G(ig, kfun, lfun);
+            }
+          }
+        }
+        for (int ig = 0; ig < ng; ++ig) o->torbg[(size_t)ifun * ng + ig] += WG(ig, jfun) * d2v[ig];
+        if (ia > ja)
+          for (int ig = 0; ig < ng; ++ig) o->torbg[(size_t)jfun * ng + ig] += WG(ig, ifun) * conj(d2v[ig]);
+      }
+    }
+    for (int ia = 0; ia < na; ++ia) {
+      int ifun = ncore + ia;
+      for (int ka = 0; ka < na; ++ka) {
+        int kfun = ncore + ka;
+        if (o->mval[ifun] == o->mval[kfun]) {
+          zc tmp = o->rden[ia + na * ka];
+          for (int ig = 0; ig < ng; ++ig) o->v2orbg[(size_t)kfun * ng + ig] += tmp * o->torbg[(size_t)ifun * ng + ig];
+        }
+      }
+    }
+  }
+  free(d2v);
+#undef V2G
+#undef WG
+}
+
+/* ------------------------------------------------------------------ */
+/* hprod_htot (src/hprod/hprod_htot.F90:2-208)                          */
+/* ------------------------------------------------------------------ */
+void orc_htotx(orc* o, double dtime, const double* lfield, const zc* wfn, const zc* cic) {
+  int nf = o->nfun, nb = o->nbas, nfc = o->nfcore, ncore = o->ncore, na = o->nact;
+  zc zfield = lfield[2], efield = lfield[5];
+  /* hprod_htot_init (src/hprod/hprod_htot_init.F90) */
+  size_t sob = (size_t)nb * nf, sog = (size_t)o->ngrid * nf;
+  memset(o->orb, 0, sob * sizeof(zc));
+  memset(o->h0orb, 0, sob * sizeof(zc));
+  memset(o->h1orb, 0, sob * sizeof(zc));
+  memset(o->gorb, 0, sob * sizeof(zc));
+  memset(o->v2orb, 0, sob * sizeof(zc));
+  memset(o->orbg, 0, sog * sizeof(zc));
+  memset(o->gorbg, 0, sog * sizeof(zc));
+  memset(o->v2orbg, 0, sog * sizeof(zc));
+  memset(o->torbg, 0, sog * sizeof(zc));
+  memset(o->rho2, 0, (size_t)o->nbas2 * nf * nf * sizeof(zc));
+  memset(o->v2sph, 0, (size_t)o->nbas2 * nf * nf * sizeof(zc));
+  memset(o->v2ang, 0, (size_t)o->ngrid * nf * nf * sizeof(zc));
+  int nn = na > 0 ? na : 1;
+  memset(o->den1, 0, sizeof(zc) * nn * nn);
+  memset(o->rden, 0, sizeof(zc) * nn * nn);
+  memset(o->rrden, 0, sizeof(zc) * nn * nn);
+  memset(o->int1e, 0, sizeof(zc) * nn * nn);
+  memset(o->den2, 0, sizeof(zc) * nn * nn * nn * nn);
+  memset(o->int2e, 0, sizeof(zc) * nn * nn * nn * nn);
+  memset(o->xmat, 0, sizeof(zc) * nf * nf);
+  memset(o->fmat, 0, sizeof(zc) * nf * nf);
+  memset(o->bmat, 0, sizeof(zc) * nf * nf);
+  memset(o->dcic, 0, sizeof(zc) * o->lcic);
+
+  /* hprod_orbin + hprod_fcorb (src/hprod/hprod_orbin.F90:2-105) */
+  memcpy(o->orb + (size_t)nb * nfc, wfn + (size_t)nb * nfc, sizeof(zc) * (size_t)nb * (nf - nfc));
+  sph2ang1(o, o->orb, o->orbg, nfc, nf, o->nr);
+  if (nfc > 0) {
+    if (o->igauge == 0) {
+      memcpy(o->orb, o->orb0, sizeof(zc) * (size_t)nb * nfc);
+      sph2ang1(o, o->orb, o->orbg, 0, nfc, o->nradfc);
+    } else {
+      sph2ang1(o, o->orb0, o->orbg, 0, nfc, o->nradfc);
+      for (int ilat = 0; ilat < o->nlat; ++ilat)
+        for (int i = 0; i < o->nradfc; ++i) {
+          zc faclv = cexp(-I * lfield[8] * o->xrad[i + 1] * o->cost[ilat]);
+          for (int ifun = 0; ifun < nfc; ++ifun) o->orbg[GIDX(i, ilat, ifun)] *= faclv;
+        }
+      ang2sph1(o, o->orbg, o->orb, 0, nfc, o->nradfc);
+    }
+  }
+
+  /* RDMs */
+  if (na > 0) {
+    ci_mkden(o, cic);
+    /* hprod_invden */
+    invden_inv(na, o->throcc1, o->den1, o->rden);
+    if (ncore > 0) {
+      if (na * 2 == o->nelact[2]) {
+        memset(o->rrden, 0, sizeof(zc) * na * na);
+      } else {
+        zc* tmp = xcalloc((size_t)na * na, sizeof(zc));
+        for (int k = 0; k < na * na; ++k) tmp[k] = -o->den1[k];
+        for (int k = 0; k < na; ++k) tmp[k + na * k] += 2.0;
+        invden_inv(na, o->throcc2, tmp, o->rrden);
+        free(tmp);
+      }
+    }
+  }
+
+  /* atomic hamiltonian */
+  tprod(o, o->orb, o->h0orb, nfc, nf, o->nr);
+  if (o->iprojfc == 2) tprod(o, o->orb, o->h0orb, 0, nfc, o->nradfc);
+  /* laser hamiltonian */
+  if (o->igauge == 0)
+    zprod(o, zfield, o->orb, o->h1orb, nfc, nf, o->nr);
+  else
+    pzprod(o, zfield, o->orb, o->h1orb, nfc, nf, o->nr);
+  if (o->iprojfc == 2) {
+    if (o->igauge == 0)
+      zprod(o, zfield, o->orb, o->h1orb, 0, nfc, o->nradfc);
+    else
+      pzprod(o, zfield, o->orb, o->h1orb, 0, nfc, o->nradfc);
+  }
+  if (nfc > 0 && o->igauge == 1) zprod(o, efield, o->orb, o->h1orb, 0, nfc, o->nradfc);
+
+  /* mean field */
+  if (o->nel[2] >= 2) {
+    mkrho2(o, 0);
+    mkrho2(o, 1);
+    poisson(o, 0);
+    sph2ang2(o, 0);
+    v2herm(o, 0);
+    poisson(o, 1);
+    sph2ang2(o, 1);
+    v2herm(o, 1);
+    mfprod2(o);
+    ang2sph1(o, o->gorbg, o->gorb, nfc, nf, o->nr);
+    ang2sph1(o, o->v2orbg, o->v2orb, nfc, nf, o->nr);
+  }
+
+  /* hprod_mkint1_sph (src/hprod/hprod_mkint1_sph.F90:2-103) */
+  for (int ia = 0; ia < na; ++ia) {
+    int ifun = ncore + ia;
+    for (int ja = 0; ja <= ia; ++ja) {
+      int jfun = ncore + ja;
+      zc t = 0;
+      if (o->mval[jfun] == o->mval[ifun]) {
+        const zc *oj = o->orb + (size_t)jfun * nb, *g = o->gorb + (size_t)ifun * nb,
+                 *h1 = o->h1orb + (size_t)ifun * nb, *h0 = o->h0orb + (size_t)ifun * nb;
+        for (int k = 0; k < nb; ++k) t += conj(oj[k]) * (g[k] + h1[k] + h0[k]);
+      }
+      o->int1e[ja + na * ia] = t;
+    }
+  }
+  for (int ia = 0; ia < na; ++ia)
+    for (int ja = ia + 1; ja < na; ++ja) o->int1e[ja + na * ia] = conj(o->int1e[ia + na * ja]);
+  /* hprod_mkint2_sph (src/hprod/hprod_mkint2_sph.F90:2-70) */
+  size_t nb2 = o->nbas2;
+  for (int ia = 0; ia < na; ++ia)
+    for (int ja = 0; ja <= ia; ++ja)
+      for (int ka = 0; ka < na; ++ka)
+        for (int la = 0; la < na; ++la) {
+          int ifun = ncore + ia, jfun = ncore + ja, kfun = ncore + ka, lfun = ncore + la;
+          zc t = 0;
+          if (o->mval[ifun] + o->mval[kfun] == o->mval[jfun] + o->mval[lfun]) {
+            const zc* r = o->rho2 + ((size_t)jfun * nf + ifun) * nb2;
+            const zc* v = o->v2sph + ((size_t)lfun * nf + kfun) * nb2;
+            for (size_t k = 0; k < nb2; ++k) t += r[k] * v[k];
+          }
+          o->int2e[ia + na * (ja + na * (ka + na * la))] = t;
+        }
+  for (int ia = 0; ia < na; ++ia)
+    for (int ja = ia + 1; ja < na; ++ja)
+      for (int ka = 0; ka < na; ++ka)
+        for (int la = 0; la < na; ++la)
+          o->int2e[ia + na * (ja + na * (ka + na * la))] = conj(o->int2e[ja + na * (ia + na * (la + na * ka))]);
+
+  /* energies (hprod_ene_dcore, hprod_ene_act) */
+  o->ene_dcore = 0;
+  o->ene_act = 0;
+  if (o->ndcore > 0) {
+    zc t = 0;
+    for (int ifun = nfc; ifun < ncore; ++ifun) {
+      const zc *oi = o->orb + (size_t)ifun * nb, *g = o->gorb + (size_t)ifun * nb,
+               *h1 = o->h1orb + (size_t)ifun * nb, *h0 = o->h0orb + (size_t)ifun * nb;
+      for (int k = 0; k < nb; ++k) {
+        zc h01 = h1[k] + h0[k];
+        t += conj(oi[k]) * (g[k] + h01 + h01);
+      }
+    }
+    o->ene_dcore = creal(t);
+  }
+  if (na > 0) {
+    zc t1 = 0, t2 = 0;
+    for (int ia = 0; ia < na; ++ia)
+      for (int ja = 0; ja < na; ++ja) {
+        t1 += o->int1e[ia + na * ja] * o->den1[ja + na * ia];
+        for (int ka = 0; ka < na; ++ka)
+          for (int la = 0; la < na; ++la)
+            t2 += o->int2e[ia + na * (ja + na * (ka + na * la))] * o->den2[ja + na * (ia + na * (la + na * ka))];
+      }
+    o->ene_act = creal(t1) + creal(t2) * 0.5;
+  }
+  o->ene_core = o->ene_fcore + o->ene_dcore;
+  o->ene_tot = o->ene_core + o->ene_act;
+
+  zc tfac = o->icomp == 0 ? (zc)dtime : I * dtime;
+#define XM(j, i) o->xmat[(size_t)(i) * nf + (j)]
+#define FM(j, i) o->fmat[(size_t)(i) * nf + (j)]
+  /* mkxmat_fc (src/hprod/hprod_htot_mkxmat.F90:2-77) */
+  if (nfc > 0) {
+    for (int ifun = nfc; ifun < nf; ++ifun)
+      for (int jfun = 0; jfun < nfc; ++jfun) {
+        zc t = 0;
+        if (o->mval[jfun] == o->mval[ifun]) {
+          const zc *h0 = o->h0orb + (size_t)jfun * nb, *h1 = o->h1orb + (size_t)jfun * nb,
+                   *oi = o->orb + (size_t)ifun * nb, *oj = o->orb + (size_t)jfun * nb,
+                   *g = o->gorb + (size_t)ifun * nb, *v = o->v2orb + (size_t)ifun * nb;
+          for (int k = 0; k < nb; ++k) t += conj(h0[k] + h1[k]) * oi[k];
+          for (int k = 0; k < nb; ++k) t += conj(oj[k]) * (g[k] + v[k]);
+        }
+        XM(jfun, ifun) = t * tfac;
+      }
+  }
+  /* full h012 products <j|v2+g+h1+h0|i> */
+#define H012DOT(jfun, ifun, out)                                                   \
+  do {                                                                             \
+    const zc *oj = o->orb + (size_t)(jfun) * nb, *v = o->v2orb + (size_t)(ifun) * nb, \
+             *g = o->gorb + (size_t)(ifun) * nb, *h1 = o->h1orb + (size_t)(ifun) * nb, \
+             *h0 = o->h0orb + (size_t)(ifun) * nb;                                 \
+    zc acc = 0;                                                                    \
+    for (int k = 0; k < nb; ++k) acc += conj(oj[k]) * (v[k] + g[k] + h1[k] + h0[k]); \
+    out = acc;                                                                     \
+  } while (0)
+  /* mkxmat_cc (:79-166) */
+  if (ncore > 0 && ncore != nfc) {
+    for (int ifun = nfc; ifun < ncore; ++ifun)
+      for (int jfun = nfc; jfun < ncore; ++jfun) {
+        zc t = 0;
+        if (o->mval[jfun] == o->mval[ifun]) H012DOT(jfun, ifun, t);
+        XM(jfun, ifun) = t * tfac;
+      }
+  }
+  /* mkxmat_ca (:549-732) / ca0 (:734-834) */
+  if (na > 0 && ncore > 0 && ncore != nfc) {
+    if (na * 2 == o->nelact[2]) {
+      for (int ifun = nfc; ifun < ncore; ++ifun)
+        for (int jfun = ncore; jfun < nf; ++jfun) {
+          zc ac = 0, ca = 0;
+          if (o->mval[jfun] == o->mval[ifun]) {
+            H012DOT(jfun, ifun, ac);
+            H012DOT(ifun, jfun, ca);
+          }
+          XM(jfun, ifun) = ac * tfac;
+          XM(ifun, jfun) = ca * tfac;
+        }
+    } else {
+      for (int ifun = nfc; ifun < ncore; ++ifun)
+        for (int jfun = ncore; jfun < nf; ++jfun) {
+          zc ac = 0, ca = 0;
+          if (o->mval[jfun] == o->mval[ifun]) {
+            H012DOT(jfun, ifun, ac);
+            H012DOT(ifun, jfun, ca);
+          }
+          FM(jfun, ifun) = ac;
+          FM(ifun, jfun) = ca;
+        }
+      for (int ifun = nfc; ifun < ncore; ++ifun)
+        for (int jfun = ncore; jfun < nf; ++jfun) {
+          XM(jfun, ifun) = FM(jfun, ifun) * tfac;
+          XM(ifun, jfun) = FM(ifun, jfun) * tfac;
+        }
+      for (int ifun = nfc; ifun < ncore; ++ifun)
+        for (int ja = 0; ja < na; ++ja) {
+          int jfun = ncore + ja;
+          FM(jfun, ifun) = FM(jfun, ifun) + FM(jfun, ifun);
+          zc t = 0;
+          for (int ka = 0; ka < na; ++ka) t += o->den1[ja + na * ka] * conj(FM(ifun, ncore + ka));
+          FM(jfun, ifun) -= t;
+        }
+      for (int ifun = nfc; ifun < ncore; ++ifun)
+        for (int ja = 0; ja < na; ++ja) {
+          int jfun = ncore + ja;
+          zc t = 0;
+          for (int ka = 0; ka < na; ++ka) t += o->rrden[ja + na * ka] * FM(ncore + ka, ifun);
+          o->bmat[(size_t)ifun * nf + jfun] = -tfac * t;
+          o->bmat[(size_t)jfun * nf + ifun] = -conj(o->bmat[(size_t)ifun * nf + jfun]);
+        }
+      for (int ifun = nfc; ifun < ncore; ++ifun)
+        for (int jfun = ncore; jfun < nf; ++jfun) {
+          XM(jfun, ifun) += o->bmat[(size_t)ifun * nf + jfun];
+          XM(ifun, jfun) += o->bmat[(size_t)jfun * nf + ifun];
+        }
+    }
+  }
+  /* mkxmat_aa1 (:228-378) with a single ORMAS subspace */
+  if (na > 0) {
+    for (int ia = 0; ia < na; ++ia)
+      for (int ja = 0; ja < na; ++ja) {
+        int ifun = ncore + ia, jfun = ncore + ja;
+        if (o->mval[ifun] == o->mval[jfun]) {
+          zc t;
+          H012DOT(jfun, ifun, t);
+          FM(jfun, ifun) = t;
+          XM(jfun, ifun) = t * tfac;
+        }
+      }
+    /* mkxmat_aa2 -> ormas_xact2 (src/ormas/ormas_xact2.F90:2-102) with
+     * nsub = 1: CI sigma, phase, scaling only */
+    ci_sigma(o, cic, o->dcic);
+    for (int d = 0; d < o->lcic; ++d) o->dcic[d] += -o->ene_act * cic[d];
+    zc fac = o->icomp == 1 ? -I * dtime : (zc)(-dtime);
+    for (int d = 0; d < o->lcic; ++d) o->dcic[d] *= fac;
+  }
+#undef H012DOT
+#undef FM
+#undef XM
+}
+
+/* hprod_htot_dtorb (src/hprod/hprod_htot_dtorb.F90:2-111), isplit = 0 */
+static void dtorb(orc* o, double dtime, zc* hwfn) {
+  int nf = o->nfun, nb = o->nbas;
+  zc tfac = o->icomp == 0 ? (zc)(-dtime) : -I * dtime;
+  for (int ifun = o->nfcore; ifun < nf; ++ifun) {
+    zc *h = hwfn + (size_t)ifun * nb;
+    const zc *v = o->v2orb + (size_t)ifun * nb, *g = o->gorb + (size_t)ifun * nb,
+             *h1 = o->h1orb + (size_t)ifun * nb, *h0 = o->h0orb + (size_t)ifun * nb;
+    for (int k = 0; k < nb; ++k) h[k] = (v[k] + g[k] + h1[k] + h0[k]) * tfac;
+  }
+  for (int ifun = o->nfcore; ifun < nf; ++ifun)
+    for (int jfun = 0; jfun < nf; ++jfun)
+      if (o->mval[ifun] == o->mval[jfun]) {
+        zc x = o->xmat[(size_t)ifun * nf + jfun];
+        zc* h = hwfn + (size_t)ifun * nb;
+        const zc* oj = o->orb + (size_t)jfun * nb;
+        for (int k = 0; k < nb; ++k) h[k] += oj[k] * x;
+      }
+}
+
+void orc_htot(orc* o, double dtime, const double* lfield, const zc* wfn, const zc* cic, zc* hwfn,
+              zc* hcic) {
+  orc_htotx(o, dtime, lfield, wfn, cic);
+  memcpy(hcic, o->dcic, sizeof(zc) * o->lcic);
+  memset(hwfn, 0, sizeof(zc) * (size_t)o->nbas * o->nfun);
+  dtorb(o, dtime, hwfn);
+}
+
+/* ------------------------------------------------------------------ */
+/* Frozen core (hprod_set_wfn0 / hprod_v2fc, src/hprod/hprod_set_wfn0.F90) */
+/* ------------------------------------------------------------------ */
+void orc_set_wfn0(orc* o, const zc* wfn, const zc* cic, const zc* v2xfc_in) {
+  (void)cic;
+  int nb = o->nbas, nr = o->nr, nfc = o->nfcore;
+  memcpy(o->orb0, wfn, sizeof(zc) * (size_t)nb * o->nfun);
+  memset(o->rhofc, 0, sizeof(zc) * nr);
+  memset(o->v2jfc, 0, sizeof(zc) * nr);
+  memset(o->v2xfc, 0, sizeof(zc) * nr);
+  o->ene_fcore = 0;
+  if (nfc == 0 || o->nel[2] < 2) return;
+  /* hprod_orbin with lfield = 0 (:158-165): frozen-core part of orb */
+  memset(o->orb, 0, sizeof(zc) * (size_t)nb * o->nfun);
+  memset(o->orbg, 0, sizeof(zc) * (size_t)o->ngrid * o->nfun);
+  if (o->igauge == 0) {
+    memcpy(o->orb, o->orb0, sizeof(zc) * (size_t)nb * nfc);
+  } else {
+    sph2ang1(o, o->orb0, o->orbg, 0, nfc, o->nradfc);
+    ang2sph1(o, o->orbg, o->orb, 0, nfc, o->nradfc);
+  }
+  /* hprod_mkrhofc (:221-253) */
+  for (int ifun = 0; ifun < nfc; ++ifun)
+    if (o->mval[ifun] == 0) {
+      double lfac = 2.0 * o->lval[ifun] + 1.0;
+      for (int i = 0; i < o->nradfc; ++i) {
+        zc c = o->orb[OIDX(i, o->lval[ifun], ifun)];
+        o->rhofc[i] += conj(c) * c * lfac;
+      }
+    }
+  /* hprod_mkv2jfc (:329-385) */
+  {
+    int n = o->ndvr;
+    double* rr = xcalloc((size_t)nr * 2, sizeof(double));
+    double fac1 = 1.0 / o->xrad[nr + 1];
+    double tr = 0, ti = 0;
+    for (int i = 0; i < nr; ++i) {
+      rr[i] = creal(o->rhofc[i]);
+      rr[nr + i] = cimag(o->rhofc[i]);
+      tr += rr[i] * o->d2rpl0[i];
+      ti += rr[nr + i] * o->d2rpl0[i];
+      rr[i] *= o->d2invr[i];
+      rr[nr + i] *= o->d2invr[i];
+    }
+    tr *= fac1;
+    ti *= fac1;
+    band_chol_solve(nr, n, o->d2ll, rr);
+    band_chol_solve(nr, n, o->d2ll, rr + nr);
+    for (int i = 0; i < nr; ++i)
+      o->v2jfc[i] = (rr[i] + tr * o->d2rpl1[i]) * o->d2fac2[i] + (rr[nr + i] + ti * o->d2rpl1[i]) * o->d2fac2[i] * I;
+    free(rr);
+  }
+  if (o->nfcore2 > 0 && v2xfc_in) {
+    memcpy(o->v2xfc, v2xfc_in, sizeof(zc) * nr);
+    if (o->xfc_implicit)
+      for (int i = 0; i < nr; ++i) o->v2jfc[i] += o->v2xfc[i];
+  }
+  /* hprod_addfc_tmat (src/hprod/hprod_addfc_tmat.F90) */
+  if (o->jfc_implicit) {
+    size_t bw = 2 * (size_t)o->ndvr + 1;
+    for (int l = 0; l <= o->lmax1; ++l)
+      for (int i = 0; i < nr; ++i) o->tmat[bw * ((size_t)l * nr + i) + o->ndvr] += o->v2jfc[i];
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* wfn_mask (src/wfn/wfn_mask.F90:2-35)                                 */
+/* ------------------------------------------------------------------ */
+void orc_mask(orc* o, zc* wfn) {
+  int min_irad = 1;
+  for (int irad = 1; irad <= o->nr; ++irad)
+    if (o->xrad[irad] > o->rmask) {
+      min_irad = irad;
+      break;
+    }
+  for (int ifun = o->nfcore; ifun < o->nfun; ++ifun)
+    for (int l = abs(o->mval[ifun]); l <= o->lmax1; ++l)
+      for (int irad = min_irad; irad <= o->nr; ++irad) wfn[OIDX(irad - 1, l, ifun)] *= o->mask[irad];
+}
+
+/* ------------------------------------------------------------------ */
+/* clcrk::prop (src/prop/clcrk.cpp:55-87) with the tableaux of          */
+/* set_coeff_01..04 (:183-235); lfield_stages[9*is] = field(t + c_is dt) */
+/* ------------------------------------------------------------------ */
+int orc_rk_coeff(int order, int* nstage, double* nodes, double* wghts, double* coeff) {
+  for (int k = 0; k < 100; ++k) coeff[k] = 0;
+  if (order == 1) {
+    *nstage = 1; nodes[0] = 0; wghts[0] = 1;
+  } else if (order == 2) {
+    *nstage = 2; nodes[0] = 0; nodes[1] = 0.5; wghts[0] = 0; wghts[1] = 1; coeff[1 * 10 + 0] = 0.5;
+  } else if (order == 3) {
+    *nstage = 3; nodes[0] = 0; nodes[1] = 0.5; nodes[2] = 1;
+    wghts[0] = 1.0 / 6.0; wghts[1] = 2.0 / 3.0; wghts[2] = 1.0 / 6.0;
+    coeff[10] = 0.5; coeff[20] = -1.0; coeff[21] = 2.0;
+  } else if (order == 4) {
+    *nstage = 4; nodes[0] = 0; nodes[1] = 0.5; nodes[2] = 0.5; nodes[3] = 1;
+    wghts[0] = 1.0 / 6.0; wghts[1] = 1.0 / 3.0; wghts[2] = 1.0 / 3.0; wghts[3] = 1.0 / 6.0;
+    coeff[10] = 0.5; coeff[20] = 0; coeff[21] = 0.5; coeff[30] = 0; coeff[31] = 0; coeff[32] = 1.0;
+  } else {
+    return -1;
+  }
+  return 0;
+}
+
+int orc_rk_prop(orc* o, int order, double dtime, const double* lfield_stages, zc* wfn, int apply_mask) {
+  int ns;
+  double nodes[10], wghts[10], coeff[100];
+  if (orc_rk_coeff(order, &ns, nodes, wghts, coeff)) return -1;
+  size_t so = (size_t)o->nbas * o->nfun, sz = so + o->lcic;
+  zc* w0 = o->rk_w0;
+  zc* t = o->rk_t;
+  memcpy(w0, wfn, sz * sizeof(zc));
+  for (int is = 0; is < ns; ++is) {
+    zc dtw = dtime * wghts[is];
+    memcpy(t, w0, sz * sizeof(zc));
+    for (int js = 0; js < is; ++js) {
+      zc fac = dtime * coeff[is * 10 + js];
+      const zc* k = o->rk_k + sz * js;
+      for (size_t q = 0; q < sz; ++q) t[q] += fac * k[q];
+    }
+    zc* k = o->rk_k + sz * is;
+    orc_htot(o, 1.0, lfield_stages + 9 * is, t, t + so, k, k + so);
+    for (size_t q = 0; q < sz; ++q) wfn[q] += dtw * k[q];
+  }
+  if (apply_mask) orc_mask(o, wfn);
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* accessors for tests                                                 */
+/* ------------------------------------------------------------------ */
+void orc_sizes(const orc* o, int* out) {
+  out[0] = o->nrad;
+  out[1] = o->nbas;
+  out[2] = o->nfun;
+  out[3] = o->lcic;
+  out[4] = o->nlat;
+  out[5] = o->L1;
+  out[6] = o->L2;
+  out[7] = o->nr;
+  out[8] = o->mmax1;
+  out[9] = o->mmax2;
+}
+
+const void* orc_table(const orc* o, const char* name) {
+#define T(x) if (!strcmp(name, #x)) return o->x
+  T(xrad); T(wrad); T(mask); T(radk); T(radk0); T(radp); T(cost); T(wlat);
+  T(legf1); T(legb1); T(legf2); T(legb2); T(alph); T(tmat); T(pmat); T(d2ll);
+  T(zfac); T(pzfac1); T(pzfac2); T(d2fac1); T(d2fac2); T(d2invr); T(d2rpl0); T(d2rpl1);
+  T(orb); T(h0orb); T(h1orb); T(gorb); T(v2orb); T(orbg); T(gorbg); T(v2orbg);
+  T(rho2); T(v2sph); T(v2ang); T(orb0); T(rhofc); T(v2jfc); T(den1); T(den2);
+  T(rden); T(rrden); T(int1e); T(int2e); T(xmat); T(dcic); T(dvr_x); T(dvr_w); T(dshape);
+#undef T
+  return NULL;
+}
+
+void orc_energies(const orc* o, double* e) {
+  e[0] = o->ene_fcore;
+  e[1] = o->ene_dcore;
+  e[2] = o->ene_core;
+  e[3] = o->ene_act;
+  e[4] = o->ene_tot;
+}
+
+/* band Cholesky helpers exposed for pin tests */
+int orc_band_chol(int n, int kd, double* ab) { return band_chol(n, kd, ab); }
+void orc_band_chol_solve(int n, int kd, const double* ab, double* x) { band_chol_solve(n, kd, ab, x); }
+void orc_herm_eig(int n, const zc* a, double* w, zc* u) { herm_eig(n, a, w, u); }
+void orc_ci_sigma(orc* o, const zc* cic, zc* sig, const zc* int1e, const zc* int2e) {
+  int n = o->nact;
+  memcpy(o->int1e, int1e, sizeof(zc) * n * n);
+  memcpy(o->int2e, int2e, sizeof(zc) * n * n * n * n);
+  memset(sig, 0, sizeof(zc) * o->lcic);
+  ci_sigma(o, cic, sig);
+}
+void orc_ci_mkden(orc* o, const zc* cic) { ci_mkden(o, cic); }

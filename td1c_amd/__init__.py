@@ -1,0 +1,214 @@
+"""td1c_amd -- MI355X (gfx950) orbital Hamiltonian product and RK step of td1c.
+
+Python mirror of the reference's C++ interface for the hot path:
+
+  HProd        ~ clhprod (src/hprod/clhprod.cpp): set_wfn0, htot, energies
+  CRK          ~ clcrk   (src/prop/clcrk.cpp):   prop (rk_order 1..4)
+  Field        ~ clfield (src/field/clfield.cpp): get_value(t) -> lfield[9]
+
+Wave functions are complex128 torch tensors on the GPU holding the orbital
+coefficients (td1c layout [orbital][l][r]) followed by the CI vector.  All
+compute runs in the HIP library libtd1c_hprod.so through its C-ABI
+(include/td1c_hprod.h); torch only provides device memory and streams.  The
+library must be built (td1c_amd/build.py); there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+from .abi import HEADER_FUNCTIONS, Field, Problem, TD1CField, TD1CProblem, TD1CSizes
+
+__all__ = ["Problem", "Field", "HProd", "CRK", "load_library", "library_path", "field_value", "field_period"]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libtd1c_hprod.so")
+_lib = None
+
+
+def library_path() -> str:
+    return _LIB_PATH
+
+
+def load_library():
+    """Loads libtd1c_hprod.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        raise RuntimeError(
+            f"{_LIB_PATH} is missing: build it with `python td1c_amd/build.py` "
+            "(the hprod has no CPU fallback)")
+    L = ctypes.CDLL(_LIB_PATH)
+    vp = ctypes.c_void_p
+    L.td1c_ctx_create.argtypes = [ctypes.POINTER(TD1CProblem), ctypes.POINTER(vp)]
+    L.td1c_ctx_create.restype = ctypes.c_int
+    L.td1c_ctx_destroy.argtypes = [vp]
+    L.td1c_ctx_destroy.restype = ctypes.c_int
+    L.td1c_get_sizes.argtypes = [vp, ctypes.POINTER(TD1CSizes)]
+    L.td1c_get_sizes.restype = ctypes.c_int
+    L.td1c_set_wfn0.argtypes = [vp, vp, vp, vp]
+    L.td1c_set_wfn0.restype = ctypes.c_int
+    L.td1c_htot.argtypes = [vp, ctypes.c_double, vp, vp, vp, vp, vp, vp]
+    L.td1c_htot.restype = ctypes.c_int
+    L.td1c_get_energies.argtypes = [vp, vp]
+    L.td1c_get_energies.restype = ctypes.c_int
+    L.td1c_rk_prop.argtypes = [vp, ctypes.c_int, ctypes.c_double, vp, vp, ctypes.c_int, vp]
+    L.td1c_rk_prop.restype = ctypes.c_int
+    L.td1c_mask.argtypes = [vp, vp, vp]
+    L.td1c_mask.restype = ctypes.c_int
+    L.td1c_field_value.argtypes = [ctypes.POINTER(TD1CField), ctypes.c_double, vp]
+    L.td1c_field_value.restype = ctypes.c_int
+    L.td1c_field_period.argtypes = [ctypes.POINTER(TD1CField)]
+    L.td1c_field_period.restype = ctypes.c_double
+    L.td1c_bind.argtypes = [vp]
+    L.td1c_bind.restype = ctypes.c_int
+    L.hprod_htot_.argtypes = [vp, vp, vp, vp, vp, vp]
+    L.hprod_htot_.restype = None
+    L.td1c_last_error.argtypes = []
+    L.td1c_last_error.restype = ctypes.c_char_p
+    _lib = L
+    return L
+
+
+def _check(rc: int, what: str):
+    if rc != 0:
+        msg = load_library().td1c_last_error().decode()
+        raise RuntimeError(f"{what} failed ({rc}): {msg}")
+
+
+def _ptr(a) -> ctypes.c_void_p:
+    if hasattr(a, "data_ptr"):
+        return ctypes.c_void_p(a.data_ptr())
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def field_value(field: Field, t: float) -> np.ndarray:
+    """clfield::get_value: lfield[0..2] gauge value, [3..5] E, [6..8] A."""
+    out = np.zeros(9)
+    c = field.to_c()
+    _check(load_library().td1c_field_value(ctypes.byref(c), float(t), _ptr(out)), "td1c_field_value")
+    return out
+
+
+def field_period(field: Field) -> float:
+    c = field.to_c()
+    return float(load_library().td1c_field_period(ctypes.byref(c)))
+
+
+class HProd:
+    """GPU hprod for one problem (clhprod)."""
+
+    def __init__(self, prob: Problem):
+        import torch  # device memory only
+
+        self.torch = torch
+        self.prob = prob
+        L = load_library()
+        c, self._keep = prob.to_c()
+        h = ctypes.c_void_p()
+        _check(L.td1c_ctx_create(ctypes.byref(c), ctypes.byref(h)), "td1c_ctx_create")
+        self.h = h
+        s = TD1CSizes()
+        _check(L.td1c_get_sizes(h, ctypes.byref(s)), "td1c_get_sizes")
+        self.sizes = s
+        self.nbas, self.nfun, self.lcic = s.nbas, s.nfun, s.lcic
+        self.norb = s.nbas * s.nfun
+        self.wfn_size = int(s.wfn_size)
+        self.npair = s.npair
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            load_library().td1c_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _stream(self, stream):
+        if stream is None:
+            return ctypes.c_void_p(self.torch.cuda.current_stream().cuda_stream)
+        return ctypes.c_void_p(stream.cuda_stream if hasattr(stream, "cuda_stream") else stream)
+
+    def set_wfn0(self, wfn: np.ndarray, v2xfc: Optional[np.ndarray] = None):
+        """clhprod::set_wfn0 (host arrays): frozen-core set-up."""
+        w = np.ascontiguousarray(wfn, dtype=np.complex128)
+        if w.size != self.wfn_size:
+            raise ValueError(f"wave function must have {self.wfn_size} complex entries")
+        cic = np.ascontiguousarray(w[self.norb:])
+        vx = None if v2xfc is None else np.ascontiguousarray(v2xfc, dtype=np.complex128)
+        _check(load_library().td1c_set_wfn0(self.h, _ptr(w), _ptr(cic), None if vx is None else _ptr(vx)),
+               "td1c_set_wfn0")
+
+    def to_device(self, wfn: np.ndarray):
+        t = self.torch.from_numpy(np.ascontiguousarray(wfn, dtype=np.complex128))
+        return t.to("cuda")
+
+    def htot(self, dtime: float, lfield: Sequence[float], wfn, hwfn=None, stream=None):
+        """clhprod::htot: hwfn = dt * d(wfn)/dt for a device wave function."""
+        torch = self.torch
+        if wfn.dtype != torch.complex128 or not wfn.is_cuda or wfn.numel() != self.wfn_size:
+            raise ValueError("wfn must be a complex128 CUDA tensor of wfn_size elements")
+        wfn = wfn.contiguous()
+        if hwfn is None:
+            hwfn = torch.empty_like(wfn)
+        lf = np.ascontiguousarray(lfield, dtype=np.float64)
+        if lf.size != 9:
+            raise ValueError("lfield must have 9 entries")
+        base = wfn.data_ptr()
+        hbase = hwfn.data_ptr()
+        off = self.norb * 16
+        _check(load_library().td1c_htot(self.h, float(dtime), _ptr(lf), ctypes.c_void_p(base),
+                                        ctypes.c_void_p(base + off), ctypes.c_void_p(hbase),
+                                        ctypes.c_void_p(hbase + off), self._stream(stream)), "td1c_htot")
+        return hwfn
+
+    def energies(self) -> np.ndarray:
+        """ene_fcore, ene_dcore, ene_core, ene_act, ene_tot of the last htot."""
+        e = np.zeros(5)
+        _check(load_library().td1c_get_energies(self.h, _ptr(e)), "td1c_get_energies")
+        return e
+
+    def mask(self, wfn, stream=None):
+        _check(load_library().td1c_mask(self.h, ctypes.c_void_p(wfn.data_ptr()), self._stream(stream)), "td1c_mask")
+        return wfn
+
+    def bind(self):
+        """Makes this context the one the legacy hprod_htot_ entry point uses."""
+        _check(load_library().td1c_bind(self.h), "td1c_bind")
+
+
+class CRK:
+    """Explicit Runge-Kutta propagator (clcrk, rk_formula = "standard")."""
+
+    NODES = {1: [0.0], 2: [0.0, 0.5], 3: [0.0, 0.5, 1.0], 4: [0.0, 0.5, 0.5, 1.0]}
+
+    def __init__(self, rk_order: int = 4):
+        if rk_order not in self.NODES:
+            raise ValueError("rk_order must be 1..4")
+        self.rk_order = rk_order
+
+    def stage_fields(self, field: Field, time: float, dtime: float) -> np.ndarray:
+        return np.stack([field_value(field, time + dtime * c) for c in self.NODES[self.rk_order]])
+
+    def prop(self, hprod: HProd, wfn, time: float, dtime: float, field: Optional[Field] = None,
+             lfield_stages: Optional[np.ndarray] = None, apply_mask: bool = True, stream=None):
+        """One step: wfn <- wfn + dt sum_s b_s k_s (in place), then the mask
+        (tdse_split2 applies clwfn::mask after every step)."""
+        if lfield_stages is None:
+            if field is None:
+                raise ValueError("need a field or explicit lfield_stages")
+            lfield_stages = self.stage_fields(field, time, dtime)
+        lf = np.ascontiguousarray(lfield_stages, dtype=np.float64)
+        if lf.shape != (len(self.NODES[self.rk_order]), 9):
+            raise ValueError("lfield_stages must be (nstage, 9)")
+        _check(load_library().td1c_rk_prop(hprod.h, self.rk_order, float(dtime), _ptr(lf),
+                                           ctypes.c_void_p(wfn.data_ptr()), int(apply_mask),
+                                           hprod._stream(stream)), "td1c_rk_prop")
+        return wfn

@@ -1,0 +1,826 @@
+// C-ABI of the gfx950 hprod (include/td1c_hprod.h): context creation,
+// frozen-core set-up, the hprod launch sequence, the Runge-Kutta step, the
+// mask and the laser field.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/td1c_hprod.h"
+#include "kernels.hpp"
+#include "setup.hpp"
+
+namespace td1c {
+
+thread_local std::string g_error;
+
+int fail(int code, const std::string& msg) {
+  g_error = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                    \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return fail(-3, std::string(#expr) + ": " + hipGetErrorString(e_));                \
+  } while (0)
+
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  hipError_t alloc(size_t count) {
+    n = count;
+    if (count == 0) return hipSuccess;
+    return hipMalloc(&p, count * sizeof(T));
+  }
+  hipError_t upload(const std::vector<T>& v) {
+    hipError_t e = alloc(v.size());
+    if (e != hipSuccess || v.empty()) return e;
+    return hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+  }
+};
+
+}  // namespace td1c
+
+using namespace td1c;
+
+struct td1c_ctx {
+  Setup S;
+  int npair = 0, nd = 0, nbas = 0, ngrid = 0;
+  int device = 0;
+  // tables
+  DevBuf<double> xr, band_k, band_k0, band_p, alph, cost, mask;
+  DevBuf<double2> tdiag;
+  DevBuf<double> m_l2g, m_g2l, m_g2L, m_L2g;
+  DevBuf<double> pG, pCl, pCr, pWl, pWr, pSd, pSl, pSu, invr, rpl0, rpl1, fac2, fac1;
+  DevBuf<int> mval, pair_m, pj, pi, ptype;
+  // batches
+  DevBuf<TransformBatch> b_l2g_dyn, b_g2l_fc, b_rho, b_L2g, b_g2l_mf;
+  int n_l2g_dyn = 0, n_g2l_fc = 0, n_g2l_mf = 0;
+  // work space
+  DevBuf<double2> lsp;       // orb | h01 | gorb | v2orb
+  double2 *orb = nullptr, *h01 = nullptr, *gorb = nullptr, *v2orb = nullptr;
+  DevBuf<double2> gsp;       // orbg | gorbg | v2orbg
+  double2 *orbg = nullptr, *gorbg = nullptr, *v2orbg = nullptr;
+  DevBuf<double2> orbg0;     // frozen-core theta-grid orbitals at A = 0
+  DevBuf<double2> rhoL, vL, pwork, vg;
+  DevBuf<double2> p_cl, p_cr, p_mp, p_Q;
+  // Gram
+  DevBuf<const double2*> g_left_l, g_right_l, g_left_p, g_right_p;
+  int gl_nL = 0, gl_nR = 0, gp_nL = 0, gp_nR = 0, gram_blk = 0;
+  DevBuf<double2> gram_partial, lprod, pprod;
+  DevBuf<int> mapH, mapG, mapV, mapF, map2e;
+  // island
+  DevBuf<double2> den1, den2, rden, rrden, int1e, int2e, xmat, dcic, ci_work;
+  DevBuf<double> ene;
+  DevBuf<int> det_of, exA, sgA, exB, sgB;
+  double ene_fcore = 0.0;
+  // RK work: w0, t, k[stages] over the full wave function
+  DevBuf<double2> rk_w0, rk_t, rk_k;
+  bool wfn0_set = false;
+
+  ~td1c_ctx() {
+    DevBuf<double>* d[] = {&xr, &band_k, &band_k0, &band_p, &alph, &cost, &mask, &m_l2g, &m_g2l, &m_g2L, &m_L2g,
+                           &pG, &pCl, &pCr, &pWl, &pWr, &pSd, &pSl, &pSu, &invr, &rpl0, &rpl1, &fac2, &fac1, &ene};
+    for (auto* b : d) b->release();
+    DevBuf<double2>* z[] = {&tdiag, &lsp, &gsp, &orbg0, &rhoL, &vL, &pwork, &vg, &p_cl, &p_cr, &p_mp, &p_Q,
+                            &gram_partial, &lprod, &pprod, &den1, &den2, &rden, &rrden, &int1e, &int2e, &xmat,
+                            &dcic, &ci_work, &rk_w0, &rk_t, &rk_k};
+    for (auto* b : z) b->release();
+    DevBuf<int>* i[] = {&mval, &pair_m, &pj, &pi, &ptype, &mapH, &mapG, &mapV, &mapF, &map2e,
+                        &det_of, &exA, &sgA, &exB, &sgB};
+    for (auto* b : i) b->release();
+    DevBuf<TransformBatch>* t[] = {&b_l2g_dyn, &b_g2l_fc, &b_rho, &b_L2g, &b_g2l_mf};
+    for (auto* b : t) b->release();
+    DevBuf<const double2*>* g[] = {&g_left_l, &g_right_l, &g_left_p, &g_right_p};
+    for (auto* b : g) b->release();
+  }
+};
+
+namespace {
+
+td1c_ctx* g_bound = nullptr;
+
+template <class T>
+std::vector<T> flat_or_one(const std::vector<T>& v) {
+  return v.empty() ? std::vector<T>(1, T()) : v;
+}
+
+int build_device(td1c_ctx* c) {
+  Setup& S = c->S;
+  const int nr = S.nr, nf = S.nfun, nfc = S.nfcore, L1 = S.L1, L2 = S.L2, nl = S.nlat;
+  c->nd = S.ndyn;
+  c->nbas = nr * L1;
+  c->ngrid = nr * nl;
+  c->npair = (int)S.pairs.size();
+  HIP_TRY(hipGetDevice(&c->device));
+  HIP_TRY(c->xr.upload(S.xr));
+  HIP_TRY(c->band_k.upload(S.bandT_k));
+  HIP_TRY(c->band_k0.upload(S.bandT_k0));
+  HIP_TRY(c->band_p.upload(S.bandT_p));
+  HIP_TRY(c->alph.upload(S.alph));
+  HIP_TRY(c->cost.upload(S.cost));
+  {
+    std::vector<double> m(nr);
+    for (int i = 0; i < nr; ++i) m[i] = S.mask[i + 1];
+    HIP_TRY(c->mask.upload(m));
+  }
+  {
+    std::vector<double2> td((size_t)L1 * nr);
+    for (size_t q = 0; q < td.size(); ++q) td[q] = make_double2(S.tdiag[2 * q], S.tdiag[2 * q + 1]);
+    HIP_TRY(c->tdiag.upload(td));
+  }
+  HIP_TRY(c->m_l2g.upload(S.mat_l2g));
+  HIP_TRY(c->m_g2l.upload(S.mat_g2l));
+  HIP_TRY(c->m_g2L.upload(S.mat_g2L));
+  HIP_TRY(c->m_L2g.upload(S.mat_L2g));
+  HIP_TRY(c->pG.upload(S.pG));
+  HIP_TRY(c->pCl.upload(S.pCl));
+  HIP_TRY(c->pCr.upload(S.pCr));
+  HIP_TRY(c->pWl.upload(S.pWl));
+  HIP_TRY(c->pWr.upload(S.pWr));
+  HIP_TRY(c->pSd.upload(S.pSd));
+  HIP_TRY(c->pSl.upload(S.pSl));
+  HIP_TRY(c->pSu.upload(S.pSu));
+  HIP_TRY(c->invr.upload(S.d2invr));
+  HIP_TRY(c->rpl0.upload(S.d2rpl0));
+  HIP_TRY(c->rpl1.upload(S.d2rpl1));
+  HIP_TRY(c->fac2.upload(S.d2fac2));
+  HIP_TRY(c->fac1.upload(S.d2fac1));
+  HIP_TRY(c->mval.upload(S.mval));
+  {
+    std::vector<int> pm, pj, pi, pt;
+    for (auto& p : S.pairs) {
+      pm.push_back(S.mval[p.i] - S.mval[p.j]);
+      pj.push_back(p.j);
+      pi.push_back(p.i);
+      pt.push_back(p.type);
+    }
+    HIP_TRY(c->pair_m.upload(flat_or_one(pm)));
+    HIP_TRY(c->pj.upload(flat_or_one(pj)));
+    HIP_TRY(c->pi.upload(flat_or_one(pi)));
+    HIP_TRY(c->ptype.upload(flat_or_one(pt)));
+  }
+  // work space
+  const size_t nb = (size_t)c->nbas, ng = (size_t)c->ngrid;
+  HIP_TRY(c->lsp.alloc(nb * (2 * nf + 2 * c->nd)));
+  c->orb = c->lsp.p;
+  c->h01 = c->orb + nb * nf;
+  c->gorb = c->h01 + nb * nf;
+  c->v2orb = c->gorb + nb * c->nd;
+  HIP_TRY(hipMemset(c->lsp.p, 0, c->lsp.n * sizeof(double2)));
+  HIP_TRY(c->gsp.alloc(ng * (nf + 2 * c->nd)));
+  c->orbg = c->gsp.p;
+  c->gorbg = c->orbg + ng * nf;
+  c->v2orbg = c->gorbg + ng * c->nd;
+  HIP_TRY(hipMemset(c->gsp.p, 0, c->gsp.n * sizeof(double2)));
+  HIP_TRY(c->orbg0.alloc(std::max<size_t>(1, ng * nfc)));
+  HIP_TRY(hipMemset(c->orbg0.p, 0, c->orbg0.n * sizeof(double2)));
+  const size_t np = std::max(1, c->npair);
+  HIP_TRY(c->rhoL.alloc(np * L2 * nr));
+  HIP_TRY(c->vL.alloc(np * L2 * nr));
+  HIP_TRY(c->pwork.alloc(np * L2 * nr));
+  HIP_TRY(c->vg.alloc(np * nl * nr));
+  HIP_TRY(c->p_cl.alloc(np * L2 * S.nfe));
+  HIP_TRY(c->p_cr.alloc(np * L2 * S.nfe));
+  HIP_TRY(c->p_mp.alloc(np * L2 * S.nfe));
+  HIP_TRY(c->p_Q.alloc(np * L2));
+  HIP_TRY(hipMemset(c->p_cl.p, 0, c->p_cl.n * sizeof(double2)));
+  HIP_TRY(hipMemset(c->p_cr.p, 0, c->p_cr.n * sizeof(double2)));
+
+  // transform batches
+  {
+    std::vector<TransformBatch> b;
+    for (int f = nfc; f < nf; ++f)
+      b.push_back({(long long)(f * nb), 0, (long long)(f * ng), S.mval[f] + S.mmax1, nr});
+    c->n_l2g_dyn = (int)b.size();
+    HIP_TRY(c->b_l2g_dyn.upload(b));
+    b.clear();
+    for (int f = 0; f < nfc; ++f)
+      b.push_back({(long long)(f * ng), 0, (long long)(f * nb), S.mval[f] + S.mmax1, S.nradfc});
+    c->n_g2l_fc = (int)b.size();
+    HIP_TRY(c->b_g2l_fc.upload(flat_or_one(b)));
+    b.clear();
+    // pair densities: in0 = orbital j, in1 = orbital i, matrix for m_i - m_j
+    for (int q = 0; q < c->npair; ++q) {
+      auto& p = S.pairs[q];
+      int mji = S.mval[p.i] - S.mval[p.j];
+      b.push_back({(long long)(p.j * ng), (long long)(p.i * ng), (long long)((size_t)q * L2 * nr), mji + S.mmax2,
+                   p.type == 1 ? S.nradfc : nr});
+    }
+    HIP_TRY(c->b_rho.upload(flat_or_one(b)));
+    b.clear();
+    for (int q = 0; q < c->npair; ++q) {
+      auto& p = S.pairs[q];
+      int mji = S.mval[p.i] - S.mval[p.j];
+      b.push_back({(long long)((size_t)q * L2 * nr), 0, (long long)((size_t)q * nl * nr), mji + S.mmax2,
+                   p.type == 1 ? S.nradfc : nr});
+    }
+    HIP_TRY(c->b_L2g.upload(flat_or_one(b)));
+    b.clear();
+    // gorbg and v2orbg (contiguous) -> gorb and v2orb (contiguous)
+    for (int w = 0; w < 2; ++w)
+      for (int f = nfc; f < nf; ++f) {
+        int d = w * c->nd + (f - nfc);
+        b.push_back({(long long)(d * ng), 0, (long long)(d * nb), S.mval[f] + S.mmax1, nr});
+      }
+    c->n_g2l_mf = (int)b.size();
+    HIP_TRY(c->b_g2l_mf.upload(b));
+  }
+
+  // Gram products over the l-space basis: left = orb_0..orb_{nf-1}, h01 of
+  // frozen core; right = h01, gorb, v2orb, orb of the dynamical orbitals.
+  {
+    std::vector<const double2*> L, R;
+    for (int f = 0; f < nf; ++f) L.push_back(c->orb + (size_t)f * nb);
+    for (int f = 0; f < nfc; ++f) L.push_back(c->h01 + (size_t)f * nb);
+    for (int f = nfc; f < nf; ++f) R.push_back(c->h01 + (size_t)f * nb);
+    for (int d = 0; d < c->nd; ++d) R.push_back(c->gorb + (size_t)d * nb);
+    for (int d = 0; d < c->nd; ++d) R.push_back(c->v2orb + (size_t)d * nb);
+    for (int f = nfc; f < nf; ++f) R.push_back(c->orb + (size_t)f * nb);
+    c->gl_nL = (int)L.size();
+    c->gl_nR = (int)R.size();
+    if (c->gl_nL * c->gl_nR > kMaxGram) return fail(-2, "too many orbitals for the Gram kernel");
+    HIP_TRY(c->g_left_l.upload(L));
+    HIP_TRY(c->g_right_l.upload(R));
+    std::vector<int> mH(nf * nf, -1), mG(nf * nf, -1), mV(nf * nf, -1), mF(nf * nf, -1);
+    const int nR = c->gl_nR, nd = c->nd;
+    for (int j = 0; j < nf; ++j)
+      for (int i = nfc; i < nf; ++i) {
+        if (S.mval[j] != S.mval[i]) continue;
+        int di = i - nfc;
+        mH[j * nf + i] = 2 * (j * nR + di);
+        mG[j * nf + i] = 2 * (j * nR + nd + di);
+        mV[j * nf + i] = 2 * (j * nR + 2 * nd + di);
+        if (j < nfc) mF[j * nf + i] = 2 * ((nf + j) * nR + 3 * nd + di);
+      }
+    HIP_TRY(c->mapH.upload(mH));
+    HIP_TRY(c->mapG.upload(mG));
+    HIP_TRY(c->mapV.upload(mV));
+    HIP_TRY(c->mapF.upload(mF));
+    HIP_TRY(c->lprod.alloc((size_t)2 * c->gl_nL * c->gl_nR));
+  }
+  // Gram products over the pair arena for int2e: left = rho of active pairs
+  // (i >= j stored as pair (i, j)), right = v of active pairs.
+  {
+    const int na = S.nact, ncore = S.ncore;
+    std::vector<const double2*> L, R;
+    std::vector<int> act_pairs;
+    for (int q = 0; q < c->npair; ++q) {
+      auto& p = S.pairs[q];
+      if (p.type == 0 && p.i >= ncore) act_pairs.push_back(q);
+    }
+    for (int q : act_pairs) L.push_back(c->rhoL.p + (size_t)q * L2 * nr);
+    for (int q : act_pairs) R.push_back(c->vL.p + (size_t)q * L2 * nr);
+    c->gp_nL = (int)L.size();
+    c->gp_nR = (int)R.size();
+    if (c->gp_nL * c->gp_nR > kMaxGram) return fail(-2, "too many active pairs for the Gram kernel");
+    std::vector<int> m2(std::max(1, na * na * na * na), -1);
+    auto slot = [&](int q) { return (int)(std::find(act_pairs.begin(), act_pairs.end(), q) - act_pairs.begin()); };
+    for (int ia = 0; ia < na; ++ia)
+      for (int ja = 0; ja <= ia; ++ja)
+        for (int ka = 0; ka < na; ++ka)
+          for (int la = 0; la < na; ++la) {
+            int i = ncore + ia, j = ncore + ja, k = ncore + ka, l = ncore + la;
+            if (S.mval[i] + S.mval[k] != S.mval[j] + S.mval[l]) continue;
+            int qa = S.pair_of[(size_t)i * nf + j];  // stored (i, j), i >= j
+            int qb = S.pair_of[(size_t)k * nf + l];
+            int conjb = S.pair_conj[(size_t)k * nf + l];
+            int e = slot(qa) * c->gp_nR + slot(qb);
+            m2[ia + na * (ja + na * (ka + na * la))] = 2 * e + (conjb ? 1 : 0);
+          }
+    HIP_TRY(c->map2e.upload(m2));
+    if (!L.empty()) {
+      HIP_TRY(c->g_left_p.upload(L));
+      HIP_TRY(c->g_right_p.upload(R));
+    }
+    HIP_TRY(c->pprod.alloc(std::max<size_t>(1, (size_t)2 * c->gp_nL * c->gp_nR)));
+  }
+  c->gram_blk = 256;
+  {
+    size_t ent = std::max((size_t)c->gl_nL * c->gl_nR, (size_t)c->gp_nL * c->gp_nR);
+    HIP_TRY(c->gram_partial.alloc(std::max<size_t>(1, ent * 2 * c->gram_blk)));
+  }
+  // island
+  const int na = std::max(1, S.nact);
+  HIP_TRY(c->den1.alloc(na * na));
+  HIP_TRY(c->den2.alloc((size_t)na * na * na * na));
+  HIP_TRY(c->rden.alloc(na * na));
+  HIP_TRY(c->rrden.alloc(na * na));
+  HIP_TRY(c->int1e.alloc(na * na));
+  HIP_TRY(c->int2e.alloc((size_t)na * na * na * na));
+  HIP_TRY(c->xmat.alloc((size_t)nf * nf));
+  HIP_TRY(c->dcic.alloc(S.lcic));
+  HIP_TRY(hipMemset(c->den1.p, 0, c->den1.n * sizeof(double2)));
+  HIP_TRY(hipMemset(c->den2.p, 0, c->den2.n * sizeof(double2)));
+  HIP_TRY(hipMemset(c->rden.p, 0, c->rden.n * sizeof(double2)));
+  HIP_TRY(hipMemset(c->rrden.p, 0, c->rrden.n * sizeof(double2)));
+  HIP_TRY(c->ene.alloc(5));
+  HIP_TRY(hipMemset(c->ene.p, 0, 5 * sizeof(double)));
+  const size_t nfull = (size_t)S.nstr_a * S.nstr_b;
+  HIP_TRY(c->ci_work.alloc(std::max<size_t>(1, ((size_t)S.nact * S.nact + 3) * nfull)));
+  HIP_TRY(c->det_of.upload(flat_or_one(S.det_of)));
+  HIP_TRY(c->exA.upload(flat_or_one(S.exA)));
+  HIP_TRY(c->sgA.upload(flat_or_one(S.sgA)));
+  HIP_TRY(c->exB.upload(flat_or_one(S.exB)));
+  HIP_TRY(c->sgB.upload(flat_or_one(S.sgB)));
+  // RK work
+  const size_t wsz = nb * nf + S.lcic;
+  HIP_TRY(c->rk_w0.alloc(wsz));
+  HIP_TRY(c->rk_t.alloc(wsz));
+  HIP_TRY(c->rk_k.alloc(wsz * kMaxStages));
+  return 0;
+}
+
+IslandArgs island_args(td1c_ctx* c, const double2* cic, double dtime) {
+  Setup& S = c->S;
+  IslandArgs a{};
+  a.nfun = S.nfun; a.nfcore = S.nfcore; a.ncore = S.ncore; a.nact = S.nact; a.lcic = S.lcic;
+  a.nelact2 = S.nelact[2]; a.icomp = S.icomp;
+  a.mval = c->mval.p;
+  a.throcc1 = S.throcc1; a.throcc2 = S.throcc2; a.dtime = dtime;
+  a.nstr_a = S.nstr_a; a.nstr_b = S.nstr_b;
+  a.det_of = c->det_of.p; a.exA = c->exA.p; a.sgA = c->sgA.p; a.exB = c->exB.p; a.sgB = c->sgB.p;
+  a.cic = cic; a.ci_work = c->ci_work.p;
+  a.den1 = c->den1.p; a.den2 = c->den2.p; a.rden = c->rden.p; a.rrden = c->rrden.p;
+  a.lprod = c->lprod.p;
+  a.mapH = c->mapH.p; a.mapG = c->mapG.p; a.mapV = c->mapV.p; a.mapF = c->mapF.p;
+  a.pprod = c->pprod.p; a.map2e = c->map2e.p;
+  a.int1e = c->int1e.p; a.int2e = c->int2e.p; a.xmat = c->xmat.p; a.dcic = c->dcic.p;
+  a.ene = c->ene.p;
+  return a;
+}
+
+// Everything of hprod_htotx up to the X matrix and the CI derivative.
+int enqueue_htotx(td1c_ctx* c, double dtime, const double* lfield, const double2* wfn, const double2* cic,
+                  hipStream_t s) {
+  Setup& S = c->S;
+  const int nr = S.nr, nf = S.nfun, nfc = S.nfcore, L1 = S.L1, L2 = S.L2, nl = S.nlat;
+  const size_t nb = (size_t)c->nbas;
+  const double2 zfield = make_double2(lfield[2], 0.0), efield = make_double2(lfield[5], 0.0);
+  // current dynamical orbitals (hprod_orbin)
+  HIP_TRY(hipMemcpyAsync(c->orb + nb * nfc, wfn + nb * nfc, nb * (nf - nfc) * sizeof(double2),
+                         hipMemcpyDeviceToDevice, s));
+  launch_transform(0, c->orb, c->orbg, c->m_l2g.p, c->b_l2g_dyn.p, c->n_l2g_dyn, nr, L1, nl, s);
+  // frozen core in the velocity gauge (hprod_fcorbg): phase on the grid, back to l-space
+  if (nfc > 0 && S.igauge == 1) {
+    launch_fc_phase(c->orbg0.p, c->orbg, c->xr.p, c->cost.p, lfield[8], nr, S.nradfc, nl, nfc, s);
+    launch_transform(0, c->orbg, c->orb, c->m_g2l.p, c->b_g2l_fc.p, c->n_g2l_fc, nr, nl, L1, s);
+  }
+  IslandArgs ia = island_args(c, cic, dtime);
+  if (S.nact > 0) launch_island_a(ia, s);
+  // one-body part
+  OneBodyArgs ob{};
+  ob.nr = nr; ob.L1 = L1; ob.ndvr = S.n; ob.nfcore = nfc; ob.nradfc = S.nradfc;
+  ob.mmax1 = S.mmax1; ob.igauge = S.igauge; ob.fc_full = S.iprojfc == 2;
+  ob.mval = c->mval.p; ob.band_k = c->band_k.p; ob.band_k0 = c->band_k0.p; ob.band_p = c->band_p.p;
+  ob.xr = c->xr.p; ob.alph = c->alph.p; ob.tdiag = c->tdiag.p; ob.zfield = zfield; ob.efield = efield;
+  ob.f0 = 0; ob.f1 = nf;
+  launch_onebody(c->orb, c->h01, ob, s);
+  // mean field
+  const size_t ng = (size_t)c->ngrid;
+  if (S.nel[2] >= 2 && c->npair > 0) {
+    launch_transform(1, c->orbg, c->rhoL.p, c->m_g2L.p, c->b_rho.p, c->npair, nr, nl, L2, s);
+    PoissonArgs pa{};
+    pa.nr = nr; pa.n = S.n; pa.nfe = S.nfe; pa.ni = S.ni; pa.L2 = L2; pa.npair = c->npair;
+    pa.pair_m = c->pair_m.p;
+    pa.G = c->pG.p; pa.Cl = c->pCl.p; pa.Cr = c->pCr.p; pa.Wl = c->pWl.p; pa.Wr = c->pWr.p;
+    pa.Sd = c->pSd.p; pa.Sl = c->pSl.p; pa.Su = c->pSu.p;
+    pa.invr = c->invr.p; pa.rpl0 = c->rpl0.p; pa.rpl1 = c->rpl1.p; pa.fac2 = c->fac2.p; pa.fac1 = c->fac1.p;
+    pa.cl = c->p_cl.p; pa.cr = c->p_cr.p; pa.mpart = c->p_mp.p; pa.Q = c->p_Q.p;
+    launch_poisson(c->rhoL.p, c->pwork.p, c->vL.p, pa, s);
+    launch_transform(0, c->vL.p, c->vg.p, c->m_L2g.p, c->b_L2g.p, c->npair, nr, L2, nl, s);
+    MfArgs ma{};
+    ma.nr = nr; ma.nlat = nl; ma.nfun = nf; ma.nfcore = nfc; ma.ncore = S.ncore; ma.nact = S.nact;
+    ma.nradfc = S.nradfc; ma.npair = c->npair; ma.nelact2 = S.nelact[2];
+    ma.pj = c->pj.p; ma.pi = c->pi.p; ma.ptype = c->ptype.p; ma.mval = c->mval.p;
+    ma.den1 = c->den1.p; ma.den2 = c->den2.p; ma.rden = c->rden.p;
+    launch_mfprod(c->orbg, c->vg.p, c->gorbg, c->v2orbg, ma, s);
+    launch_transform(0, c->gorbg, c->gorb, c->m_g2l.p, c->b_g2l_mf.p, c->n_g2l_mf, nr, nl, L1, s);
+  } else {
+    HIP_TRY(hipMemsetAsync(c->gorb, 0, 2 * nb * c->nd * sizeof(double2), s));
+  }
+  (void)ng;
+  // inner products
+  GramArgs g{};
+  g.len = nb; g.nL = c->gl_nL; g.nR = c->gl_nR; g.conj_left = 1; g.with_conj_right = 0;
+  g.left = c->g_left_l.p; g.right = c->g_right_l.p;
+  launch_gram(g, c->gram_partial.p, c->gram_blk, c->lprod.p, s);
+  if (c->gp_nL > 0) {
+    GramArgs h{};
+    h.len = (size_t)L2 * nr; h.nL = c->gp_nL; h.nR = c->gp_nR; h.conj_left = 0; h.with_conj_right = 1;
+    h.left = c->g_left_p.p; h.right = c->g_right_p.p;
+    launch_gram(h, c->gram_partial.p, c->gram_blk, c->pprod.p, s);
+  }
+  launch_island_b(ia, s);
+  return 0;
+}
+
+DtorbArgs dtorb_args(td1c_ctx* c, double dtime) {
+  Setup& S = c->S;
+  DtorbArgs d{};
+  d.nbas = c->nbas; d.nr = S.nr; d.nfun = S.nfun; d.nfcore = S.nfcore;
+  d.mval = c->mval.p; d.h01 = c->h01; d.gorb = c->gorb; d.v2orb = c->v2orb; d.orb = c->orb; d.xmat = c->xmat.p;
+  d.tfac = S.icomp == 0 ? make_double2(-dtime, 0.0) : make_double2(0.0, -dtime);
+  return d;
+}
+
+// Host banded solve of the L = 0 Poisson operator for the frozen-core
+// Hartree potential (set-up only, once per run): dense Cholesky of the band.
+std::vector<double> host_poisson_l0(const Setup& S, const std::vector<double>& rhs) {
+  const int nr = S.nr, n = S.n, bw = 2 * n + 1;
+  std::vector<double> ab((size_t)(n + 1) * nr, 0.0);  // lower band, ab[(n+1)*j + (i-j)]
+  for (int j = 0; j < nr; ++j)
+    for (int i = j; i <= std::min(nr - 1, j + n); ++i) ab[(size_t)(n + 1) * j + (i - j)] = 2.0 * S.radk[(size_t)bw * (j + 1) + n + (i - j)];
+  for (int j = 0; j < nr; ++j) {
+    double d = std::sqrt(ab[(size_t)(n + 1) * j]);
+    ab[(size_t)(n + 1) * j] = d;
+    int kn = std::min(n, nr - 1 - j);
+    for (int i = 1; i <= kn; ++i) ab[(size_t)(n + 1) * j + i] /= d;
+    for (int c2 = 1; c2 <= kn; ++c2)
+      for (int r2 = c2; r2 <= kn; ++r2)
+        ab[(size_t)(n + 1) * (j + c2) + (r2 - c2)] -= ab[(size_t)(n + 1) * j + r2] * ab[(size_t)(n + 1) * j + c2];
+  }
+  std::vector<double> x = rhs;
+  for (int j = 0; j < nr; ++j) {
+    x[j] /= ab[(size_t)(n + 1) * j];
+    for (int i = j + 1; i <= std::min(nr - 1, j + n); ++i) x[i] -= x[j] * ab[(size_t)(n + 1) * j + (i - j)];
+  }
+  for (int j = nr - 1; j >= 0; --j) {
+    double t = x[j];
+    for (int i = j + 1; i <= std::min(nr - 1, j + n); ++i) t -= ab[(size_t)(n + 1) * j + (i - j)] * x[i];
+    x[j] = t / ab[(size_t)(n + 1) * j];
+  }
+  return x;
+}
+
+constexpr double kPi = 3.14159265358979323846264338327950288;
+
+}  // namespace
+
+extern "C" {
+
+const char* td1c_last_error(void) { return g_error.c_str(); }
+
+int td1c_ctx_create(const td1c_problem* prob, td1c_ctx** out) {
+  if (!prob || !out) return fail(-1, "null argument");
+  std::unique_ptr<td1c_ctx> c(new td1c_ctx());
+  if (!c->S.build(*prob)) return fail(-2, c->S.error);
+  if (c->S.nfun > kMaxFun) return fail(-2, "more than 16 orbitals are not provided by the mean-field kernel");
+  if (c->S.nact > kMaxAct) return fail(-2, "more than 8 active orbitals are not provided by the mean-field kernel");
+  if (c->S.L2 > 256 || c->S.nlat > 256 || c->S.L1 > 256) return fail(-2, "lmax1/lmax2/nlat above 255 are not provided");
+  if (c->S.ni > 32) return fail(-2, "ndvr above 33 is not provided by the Poisson kernel");
+  int rc = build_device(c.get());
+  if (rc) return rc;
+  *out = c.release();
+  return 0;
+}
+
+int td1c_ctx_destroy(td1c_ctx* ctx) {
+  if (g_bound == ctx) g_bound = nullptr;
+  delete ctx;
+  return 0;
+}
+
+int td1c_get_sizes(const td1c_ctx* c, td1c_sizes* out) {
+  if (!c || !out) return fail(-1, "null argument");
+  out->nrad = c->S.nrad;
+  out->nbas = c->nbas;
+  out->nfun = c->S.nfun;
+  out->lcic = c->S.lcic;
+  out->nlat = c->S.nlat;
+  out->lmax1 = c->S.lmax1;
+  out->lmax2 = c->S.lmax2;
+  out->npair = c->npair;
+  out->wfn_size = (long long)c->nbas * c->S.nfun + c->S.lcic;
+  return 0;
+}
+
+int td1c_set_wfn0(td1c_ctx* c, const double* wfn, const double* cic, const double* v2xfc) {
+  if (!c || !wfn) return fail(-1, "null argument");
+  (void)cic;
+  Setup& S = c->S;
+  const int nr = S.nr, nfc = S.nfcore, L1 = S.L1;
+  const size_t nb = (size_t)c->nbas;
+  c->wfn0_set = true;
+  if (nfc == 0) return 0;
+  const std::complex<double>* w = reinterpret_cast<const std::complex<double>*>(wfn);
+  // frozen-core orbitals (orb0) and their theta-grid image at A = 0
+  HIP_TRY(hipMemcpy(c->orb, wfn, nb * nfc * sizeof(double2), hipMemcpyHostToDevice));
+  {
+    std::vector<TransformBatch> b;
+    for (int f = 0; f < nfc; ++f)
+      b.push_back({(long long)(f * nb), 0, (long long)(f * (size_t)c->ngrid), S.mval[f] + S.mmax1, S.nradfc});
+    DevBuf<TransformBatch> tb;
+    HIP_TRY(tb.upload(b));
+    launch_transform(0, c->orb, c->orbg0.p, c->m_l2g.p, tb.p, nfc, nr, L1, S.nlat, nullptr);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipDeviceSynchronize());
+    tb.release();
+    HIP_TRY(hipMemcpy(c->orbg, c->orbg0.p, (size_t)c->ngrid * nfc * sizeof(double2), hipMemcpyDeviceToDevice));
+  }
+  // frozen-core orbitals as hprod_orbin sees them at lfield = 0
+  std::vector<std::complex<double>> orbfc(nb * nfc);
+  if (S.igauge == 0) {
+    std::copy(w, w + nb * nfc, orbfc.begin());
+  } else {
+    // l -> theta -> l round trip, as hprod_fcorbg + bas_ang2sph1_fc do at A = 0
+    DevBuf<TransformBatch> tb;
+    std::vector<TransformBatch> b;
+    for (int f = 0; f < nfc; ++f)
+      b.push_back({(long long)(f * (size_t)c->ngrid), 0, (long long)(f * nb), S.mval[f] + S.mmax1, S.nradfc});
+    HIP_TRY(tb.upload(b));
+    DevBuf<double2> tmp;
+    HIP_TRY(tmp.alloc(nb * nfc));
+    launch_transform(0, c->orbg0.p, tmp.p, c->m_g2l.p, tb.p, nfc, nr, S.nlat, L1, nullptr);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(orbfc.data(), tmp.p, nb * nfc * sizeof(double2), hipMemcpyDeviceToHost));
+    tmp.release();
+    tb.release();
+  }
+  // frozen-core density and Hartree potential (hprod_mkrhofc, hprod_mkv2jfc,
+  // src/hprod/hprod_set_wfn0.F90:221-385)
+  std::vector<double> rho(nr, 0.0);
+  for (int f = 0; f < nfc; ++f) {
+    if (S.mval[f] != 0) continue;
+    double lfac = 2.0 * S.lval[f] + 1.0;
+    for (int i = 0; i < S.nradfc; ++i) {
+      std::complex<double> z = orbfc[(size_t)f * nb + (size_t)S.lval[f] * nr + i];
+      rho[i] += std::norm(z) * lfac;
+    }
+  }
+  double tr = 0.0;
+  std::vector<double> rhs(nr);
+  for (int i = 0; i < nr; ++i) {
+    tr += rho[i] * S.d2rpl0[i];
+    rhs[i] = rho[i] * S.d2invr[i];
+  }
+  tr *= 1.0 / S.xrad[nr + 1];
+  std::vector<double> x = host_poisson_l0(S, rhs);
+  std::vector<double2> v2jfc(nr);
+  for (int i = 0; i < nr; ++i) v2jfc[i] = make_double2((x[i] + tr * S.d2rpl1[i]) * S.d2fac2[i], 0.0);
+  if (S.nfcore2 > 0 && v2xfc)
+    for (int i = 0; i < nr; ++i) {
+      v2jfc[i].x += v2xfc[2 * i];
+      v2jfc[i].y += v2xfc[2 * i + 1];
+    }
+  // tmat += v2jfc (hprod_addfc_tmat)
+  std::vector<double2> td((size_t)L1 * nr);
+  for (int l = 0; l < L1; ++l)
+    for (int i = 0; i < nr; ++i) {
+      size_t q = (size_t)l * nr + i;
+      td[q] = make_double2(S.tdiag[2 * q] + v2jfc[i].x, S.tdiag[2 * q + 1] + v2jfc[i].y);
+    }
+  HIP_TRY(hipMemcpy(c->tdiag.p, td.data(), td.size() * sizeof(double2), hipMemcpyHostToDevice));
+  return 0;
+}
+
+int td1c_htot(td1c_ctx* c, double dtime, const double* lfield, const double* d_wfn, const double* d_cic,
+              double* d_hwfn, double* d_hcic, void* stream) {
+  if (!c || !lfield || !d_wfn || !d_hwfn) return fail(-1, "null argument");
+  if (c->S.nfcore > 0 && !c->wfn0_set) return fail(-4, "frozen core needs td1c_set_wfn0 first");
+  hipStream_t s = (hipStream_t)stream;
+  const double2* wfn = reinterpret_cast<const double2*>(d_wfn);
+  const double2* cic = reinterpret_cast<const double2*>(d_cic);
+  int rc = enqueue_htotx(c, dtime, lfield, wfn, cic, s);
+  if (rc) return rc;
+  DtorbArgs d = dtorb_args(c, dtime);
+  d.kout = reinterpret_cast<double2*>(d_hwfn);
+  launch_dtorb(d, s);
+  if (d_hcic)
+    HIP_TRY(hipMemcpyAsync(d_hcic, c->dcic.p, c->S.lcic * sizeof(double2), hipMemcpyDeviceToDevice, s));
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+int td1c_get_energies(td1c_ctx* c, double ene[5]) {
+  if (!c || !ene) return fail(-1, "null argument");
+  HIP_TRY(hipDeviceSynchronize());
+  HIP_TRY(hipMemcpy(ene, c->ene.p, 5 * sizeof(double), hipMemcpyDeviceToHost));
+  ene[0] = c->ene_fcore;
+  ene[2] += c->ene_fcore;
+  ene[4] += c->ene_fcore;
+  return 0;
+}
+
+int td1c_mask(td1c_ctx* c, double* d_wfn, void* stream) {
+  if (!c || !d_wfn) return fail(-1, "null argument");
+  launch_mask(reinterpret_cast<double2*>(d_wfn), c->mask.p, c->S.nr, c->S.L1, c->S.nfcore, c->S.nfun,
+              (hipStream_t)stream);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+// clcrk tableaux for rk_order 1..4 (src/prop/clcrk.cpp:183-235)
+static int rk_tableau(int order, int& ns, double* nodes, double* w, double (*a)[kMaxStages]) {
+  for (int i = 0; i < kMaxStages; ++i) {
+    nodes[i] = w[i] = 0;
+    for (int j = 0; j < kMaxStages; ++j) a[i][j] = 0;
+  }
+  switch (order) {
+    case 1: ns = 1; w[0] = 1; return 0;
+    case 2: ns = 2; nodes[1] = 0.5; w[1] = 1; a[1][0] = 0.5; return 0;
+    case 3: ns = 3; nodes[1] = 0.5; nodes[2] = 1; w[0] = 1.0 / 6; w[1] = 2.0 / 3; w[2] = 1.0 / 6;
+            a[1][0] = 0.5; a[2][0] = -1; a[2][1] = 2; return 0;
+    case 4: ns = 4; nodes[1] = 0.5; nodes[2] = 0.5; nodes[3] = 1;
+            w[0] = 1.0 / 6; w[1] = 1.0 / 3; w[2] = 1.0 / 3; w[3] = 1.0 / 6;
+            a[1][0] = 0.5; a[2][1] = 0.5; a[3][2] = 1; return 0;
+    default: return -1;
+  }
+}
+
+int td1c_rk_prop(td1c_ctx* c, int rk_order, double dtime, const double* lfield_stages, double* d_wfn, int apply_mask,
+                 void* stream) {
+  if (!c || !lfield_stages || !d_wfn) return fail(-1, "null argument");
+  if (c->S.nfcore > 0 && !c->wfn0_set) return fail(-4, "frozen core needs td1c_set_wfn0 first");
+  int ns;
+  double nodes[kMaxStages], w[kMaxStages], a[kMaxStages][kMaxStages];
+  if (rk_tableau(rk_order, ns, nodes, w, a)) return fail(-1, "rk_order must be 1..4");
+  hipStream_t s = (hipStream_t)stream;
+  Setup& S = c->S;
+  const size_t so = (size_t)c->nbas * S.nfun, sz = so + S.lcic;
+  double2* wfn = reinterpret_cast<double2*>(d_wfn);
+  double2* w0 = c->rk_w0.p;
+  double2* t = c->rk_t.p;
+  HIP_TRY(hipMemcpyAsync(w0, wfn, sz * sizeof(double2), hipMemcpyDeviceToDevice, s));
+  // stage input of stage 0 is w0 itself
+  const double2* tin = w0;
+  for (int is = 0; is < ns; ++is) {
+    double2* k = c->rk_k.p + sz * is;
+    int rc = enqueue_htotx(c, 1.0, lfield_stages + 9 * is, tin, tin + so, s);
+    if (rc) return rc;
+    DtorbArgs d = dtorb_args(c, 1.0);
+    d.kout = k;
+    d.acc = wfn;
+    d.dtw = dtime * w[is];
+    d.mask = (apply_mask && is == ns - 1) ? c->mask.p : nullptr;
+    if (is + 1 < ns) {
+      d.next = t;
+      d.w0 = w0;
+      d.nprev = is;
+      for (int js = 0; js < is; ++js) {
+        d.kprev[js] = c->rk_k.p + sz * js;
+        d.cprev[js] = dtime * a[is + 1][js];
+      }
+      d.cnew = dtime * a[is + 1][is];
+    }
+    launch_dtorb(d, s);
+    // CI part: k_cic = dcic; acc += dtw k; next = w0 + sum a k
+    HIP_TRY(hipMemcpyAsync(k + so, c->dcic.p, S.lcic * sizeof(double2), hipMemcpyDeviceToDevice, s));
+    AxpyArgs ax{};
+    ax.nterm = 1; ax.k[0] = k + so; ax.c[0] = dtime * w[is];
+    launch_axpy_list(wfn + so, nullptr, ax, S.lcic, s);
+    if (is + 1 < ns) {
+      AxpyArgs an{};
+      an.nterm = is + 1;
+      for (int js = 0; js <= is; ++js) {
+        an.k[js] = c->rk_k.p + sz * js + so;
+        an.c[js] = dtime * a[is + 1][js];
+      }
+      launch_axpy_list(t + so, w0 + so, an, S.lcic, s);
+      tin = t;
+    }
+  }
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Laser field (clfield1, src/field/clfield1.cpp:14-500): one pulse starting at
+// t = 0, envelopes zero / sin2 / flat, gauges length1 / velocity1.
+// ---------------------------------------------------------------------------
+struct PulseConst {
+  double famp, freq, period, cep, tau, tleft, tright, numcyc;
+};
+
+static PulseConst pulse_const(const td1c_field* f) {
+  PulseConst p{};
+  p.cep = f->cep / 180.0 * kPi;
+  p.famp = std::sqrt(f->fint / 3.50944506E+16);
+  p.freq = 1239.84190 / (f->wlen * 27.2113845);
+  p.period = 2.0 * kPi / p.freq;
+  p.numcyc = f->numcyc;
+  p.tau = f->env_type == 0 ? 0.0 : p.numcyc * p.period * 0.5;
+  double delay = f->delay / 24.188843265;
+  // clfield::gen: tshift = -(delay - tau) for a single pulse
+  double tshift = -std::min(0.0, delay - p.tau);
+  double tcenter = tshift + delay;
+  p.tleft = tcenter - p.tau;
+  p.tright = tcenter + p.tau;
+  return p;
+}
+
+static double e_explicit(const td1c_field* f, const PulseConst& p, double t) {
+  double trel = t - p.tleft;
+  if (f->env_type == 2) return p.famp * std::sin(p.freq * trel + p.cep);
+  if (f->env_type == 1) {
+    if (t < p.tleft || t > p.tright) return 0.0;
+    double env = std::sin(kPi * trel / (p.numcyc * p.period));
+    return env * env * p.famp * std::sin(p.freq * trel + p.cep);
+  }
+  return 0.0;
+}
+
+static double a_explicit(const td1c_field* f, const PulseConst& p, double t) {
+  double trel = t - p.tleft;
+  if (f->env_type == 2) return p.famp / p.freq * std::sin(p.freq * trel + p.cep);
+  if (f->env_type == 1) {
+    if (t < p.tleft || t > p.tright) return 0.0;
+    double env = std::sin(kPi * trel / (p.numcyc * p.period));
+    return env * env * p.famp / p.freq * std::sin(p.freq * trel + p.cep);
+  }
+  return 0.0;
+}
+
+// E = -dA/dt by the 6-point central difference of clfield1::get_evalue_numder
+static double e_from_a(const td1c_field* f, const PulseConst& p, double t) {
+  const double h = 2.0 * kPi / (1239.84190 / (800.0 * 27.2113845)) / 100000.0;
+  double s = 0.0;
+  s += a_explicit(f, p, t + 3 * h) / 60.0;
+  s -= a_explicit(f, p, t + 2 * h) / 20.0 * 3.0;
+  s += a_explicit(f, p, t + h) / 4.0 * 3.0;
+  s -= a_explicit(f, p, t - h) / 4.0 * 3.0;
+  s += a_explicit(f, p, t - 2 * h) / 20.0 * 3.0;
+  s -= a_explicit(f, p, t - 3 * h) / 60.0;
+  return -s / h;
+}
+
+int td1c_field_value(const td1c_field* f, double t, double lfield[9]) {
+  if (!f || !lfield) return fail(-1, "null argument");
+  for (int i = 0; i < 9; ++i) lfield[i] = 0.0;
+  if (f->env_type < 0 || f->env_type > 2 || f->gauge < 0 || f->gauge > 1) return fail(-1, "bad field");
+  if (f->env_type == 0) return 0;
+  PulseConst p = pulse_const(f);
+  if (!(t > p.tleft && t < p.tright)) return 0;
+  if (f->gauge == 0) {
+    lfield[2] = e_explicit(f, p, t);
+    lfield[5] = e_explicit(f, p, t);
+    // A in the length gauge is a running integral in the reference
+    // (get_avalue_numint); it is not used by the hprod and is left 0 here.
+  } else {
+    lfield[2] = a_explicit(f, p, t);
+    lfield[5] = e_from_a(f, p, t);
+    lfield[8] = a_explicit(f, p, t);
+  }
+  return 0;
+}
+
+double td1c_field_period(const td1c_field* f) {
+  if (!f) return 0.0;
+  return 2.0 * kPi / (1239.84190 / (f->wlen * 27.2113845));
+}
+
+int td1c_bind(td1c_ctx* c) {
+  g_bound = c;
+  return 0;
+}
+
+void hprod_htot_(const double* dtime, const double* lfield, const double* wfn, const double* cic, double* hwfn,
+                 double* hcic) {
+  td1c_ctx* c = g_bound;
+  if (!c) {
+    std::fprintf(stderr, "hprod_htot_: no bound td1c context\n");
+    std::abort();
+  }
+  const size_t so = (size_t)c->nbas * c->S.nfun * sizeof(double2), sc = (size_t)c->S.lcic * sizeof(double2);
+  double *dw = nullptr, *dc = nullptr, *dh = nullptr, *dhc = nullptr;
+  auto chk = [](hipError_t e, const char* what) {
+    if (e != hipSuccess) {
+      std::fprintf(stderr, "hprod_htot_: %s: %s\n", what, hipGetErrorString(e));
+      std::abort();
+    }
+  };
+  chk(hipMalloc(&dw, so), "hipMalloc");
+  chk(hipMalloc(&dh, so), "hipMalloc");
+  chk(hipMalloc(&dc, sc), "hipMalloc");
+  chk(hipMalloc(&dhc, sc), "hipMalloc");
+  chk(hipMemcpy(dw, wfn, so, hipMemcpyHostToDevice), "hipMemcpy");
+  chk(hipMemcpy(dc, cic, sc, hipMemcpyHostToDevice), "hipMemcpy");
+  if (td1c_htot(c, *dtime, lfield, dw, dc, dh, dhc, nullptr)) {
+    std::fprintf(stderr, "hprod_htot_: %s\n", g_error.c_str());
+    std::abort();
+  }
+  chk(hipDeviceSynchronize(), "hipDeviceSynchronize");
+  chk(hipMemcpy(hwfn, dh, so, hipMemcpyDeviceToHost), "hipMemcpy");
+  chk(hipMemcpy(hcic, dhc, sc, hipMemcpyDeviceToHost), "hipMemcpy");
+  (void)hipFree(dw);
+  (void)hipFree(dh);
+  (void)hipFree(dc);
+  (void)hipFree(dhc);
+}
+
+}  // extern "C"

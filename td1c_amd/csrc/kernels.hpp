@@ -1,0 +1,120 @@
+// Launch arguments shared between the host orchestration (api.hip) and the
+// gfx950 kernels (kernels.hip, island.hip).  Every pointer is device memory.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+
+namespace td1c {
+
+constexpr int kMaxFun = 16;       // orbitals handled by the theta-grid mean-field kernel
+constexpr int kMaxAct = 8;        // active orbitals handled on the theta grid
+constexpr int kMaxActPairs = kMaxAct * kMaxAct;
+constexpr int kMaxStages = 6;
+
+struct TransformBatch {
+  long long in0;   // element offset of the input block (MODE 1: orbital j)
+  long long in1;   // MODE 1: orbital i
+  long long out;   // element offset of the output block
+  int mat;         // matrix index (m + mmax)
+  int rows;        // radial rows to compute (nr, or nradfc for frozen core)
+};
+
+struct OneBodyArgs {
+  int nr, L1, ndvr, nfcore, nradfc, f0, f1, mmax1, igauge, fc_full;
+  const int* mval;
+  const double* band_k;   // [2n+1][nr] kinetic band for l = 0
+  const double* band_k0;  // [2n+1][nr] kinetic band for l >= 1
+  const double* band_p;   // [2n+1][nr] d/dr band
+  const double* xr;       // [nr]
+  const double* alph;     // [2mmax1+1][L1]
+  const double2* tdiag;   // [L1][nr]
+  double2 zfield, efield;
+};
+
+struct PoissonArgs {
+  int nr, n, nfe, ni, L2, npair;
+  const int* pair_m;  // m_j - m_i of each pair
+  const double *G, *Cl, *Cr, *Wl, *Wr, *Sd, *Sl, *Su;
+  const double *invr, *rpl0, *rpl1, *fac2, *fac1;
+  double2 *cl, *cr, *mpart, *Q;  // [npair][L2][nfe], Q: [npair][L2]
+};
+
+struct MfArgs {
+  int nr, nlat, nfun, nfcore, ncore, nact, nradfc, npair, nelact2;
+  const int *pj, *pi, *ptype, *mval;
+  const double2 *den1, *den2, *rden;
+};
+
+constexpr int kMaxGram = 1024;   // nL * nR entries of one Gram launch (4 per thread)
+
+struct GramArgs {
+  size_t len;
+  int nL, nR;
+  int conj_left;        // 1: S = sum conj(L) R (inner products); 0: sum L R (zdotu)
+  int with_conj_right;  // also accumulate sum opA(L) conj(R)
+  const double2* const* left;   // device array of nL device pointers
+  const double2* const* right;  // device array of nR device pointers
+};
+
+struct DtorbArgs {
+  int nbas, nr, nfun, nfcore;
+  const int* mval;
+  const double2 *h01, *gorb, *v2orb, *orb, *xmat;
+  double2 tfac;
+  double2* kout;          // stage derivative (may be null)
+  double2* acc;           // psi += dtw * k (may be null)
+  double dtw;
+  const double* mask;     // applied to acc after the update (may be null)
+  double2* next;          // next stage input (may be null)
+  const double2* w0;
+  int nprev;
+  const double2* kprev[kMaxStages];
+  double cprev[kMaxStages];
+  double cnew;
+};
+
+struct AxpyArgs {
+  int nterm;
+  const double2* k[kMaxStages];
+  double c[kMaxStages];
+};
+
+struct IslandArgs {
+  int nfun, nfcore, ncore, nact, lcic, nelact2, icomp;
+  const int* mval;
+  double throcc1, throcc2, dtime;
+  // CAS tables
+  int nstr_a, nstr_b;
+  const int *det_of, *exA, *sgA, *exB, *sgB;
+  const double2* cic;
+  double2* ci_work;       // (nact^2 + 3) * nstr_a * nstr_b
+  // stage A outputs
+  double2 *den1, *den2, *rden, *rrden;
+  // stage B inputs: Gram results and index maps ([nfun*nfun], -1 = absent)
+  const double2* lprod;
+  const int *mapH, *mapG, *mapV, *mapF;
+  const double2* pprod;
+  const int* map2e;        // [nact^4] (i,j,k,l) with i >= j, -1 otherwise
+  // stage B outputs
+  double2 *int1e, *int2e, *xmat, *dcic;
+  double* ene;             // ene_fcore (input), dcore, core, act, tot
+};
+
+void launch_transform(int mode, const double2* in, double2* out, const double* mats, const TransformBatch* batch,
+                      int nbatch, int nr, int K, int C, hipStream_t s);
+void launch_onebody(const double2* orb, double2* h01, const OneBodyArgs& a, hipStream_t s);
+void launch_poisson(const double2* rho, double2* work, double2* v, const PoissonArgs& a, hipStream_t s);
+void launch_mfprod(const double2* orbg, const double2* vg, double2* gorbg, double2* v2orbg, const MfArgs& a,
+                   hipStream_t s);
+void launch_fc_phase(const double2* orbg0, double2* orbg, const double* xr, const double* cost, double afield, int nr,
+                     int nradfc, int nlat, int nfc, hipStream_t s);
+void launch_gram(const GramArgs& a, double2* partial, int nblk, double2* out, hipStream_t s);
+void launch_dtorb(const DtorbArgs& a, hipStream_t s);
+void launch_mask(double2* wfn, const double* mask, int nr, int L1, int nfcore, int nfun, hipStream_t s);
+void launch_axpy_list(double2* y, const double2* x0, const AxpyArgs& a, int n, hipStream_t s);
+void launch_island_a(const IslandArgs& a, hipStream_t s);
+void launch_island_b(const IslandArgs& a, hipStream_t s);
+
+}  // namespace td1c

@@ -1,0 +1,53 @@
+"""Small problems shared by the parity tests (sizes the oracle finishes in
+well under a second)."""
+from __future__ import annotations
+
+import numpy as np
+
+from td1c_amd.abi import Problem
+from td1c_amd import workloads as W
+
+
+def he_small(**kw):
+    return Problem(nfe=10, ndvr=20, rmax=30.0, lmax1=5, lmax2=10, znuc=2.0, ndcore=1,
+                   nel_alpha=1, nel_beta=1, **kw)
+
+
+def dc3_small(**kw):
+    # three doubly occupied orbitals, one of them m = 1 paired with m = -1
+    kw.setdefault("mval", [0, 1, -1])
+    return Problem(nfe=8, ndvr=20, rmax=24.0, lmax1=6, lmax2=12, znuc=6.0, ndcore=3,
+                   nel_alpha=3, nel_beta=3, **kw)
+
+
+def ne_small(**kw):
+    kw.setdefault("igauge", 1)
+    return Problem(nfe=10, ndvr=20, rmax=30.0, lmax1=7, lmax2=14, znuc=10.0, nfcore1=1, nact=4,
+                   nel_alpha=5, nel_beta=5, lval=[0, 0, 0, 0, 0], **kw)
+
+
+def cas_small(**kw):
+    # 1 dynamical core + CAS(2e, 2o): 4 determinants, D not idempotent
+    return Problem(nfe=8, ndvr=20, rmax=24.0, lmax1=5, lmax2=10, znuc=4.0, ndcore=1, nact=2,
+                   nel_alpha=2, nel_beta=2, **kw)
+
+
+def cas_fc_small(**kw):
+    # frozen core + 1 dynamical core + CAS(2e, 3o)
+    kw.setdefault("igauge", 1)
+    return Problem(nfe=8, ndvr=20, rmax=24.0, lmax1=5, lmax2=10, znuc=6.0, nfcore1=1, ndcore=1, nact=3,
+                   nel_alpha=3, nel_beta=3, lval=[0, 0, 0, 0, 0], **kw)
+
+
+def wfn_for(prob: Problem, lcic: int, seed: int = 11) -> np.ndarray:
+    orbs = W.synth_orbitals(prob, seed)
+    rng = np.random.Generator(np.random.PCG64(seed + 1))
+    cic = rng.standard_normal(lcic) + 1j * rng.standard_normal(lcic)
+    if lcic == 1:
+        cic[:] = 1.0
+    cic /= np.linalg.norm(cic)
+    return np.concatenate([orbs.ravel(), cic])
+
+
+LF_LEN = np.array([0.0, 0.0, 0.05, 0.0, 0.0, 0.05, 0.0, 0.0, 0.0])
+LF_VEL = np.array([0.0, 0.0, 0.3, 0.0, 0.0, 0.04, 0.0, 0.0, 0.3])

@@ -1,0 +1,357 @@
+"""Pins of the FP64 oracle (oracle/td1c_oracle.c) against closed forms,
+library routines, invariants and the reference's own closed-shell RDM
+formula.  CPU only.  Each test names the oracle function it pins."""
+import math
+
+import numpy as np
+import pytest
+import scipy.linalg
+import scipy.special
+
+from td1c_amd.abi import Problem
+from td1c_amd import workloads as W
+from oracle import oracle as O
+
+
+def _prob(**kw):
+    base = dict(nfe=30, ndvr=20, rmax=60.0, lmax1=3, lmax2=6, znuc=1.0, ndcore=1, nel_alpha=1, nel_beta=1)
+    base.update(kw)
+    return Problem(**base)
+
+
+def test_gll_nodes_weights_vs_numpy():
+    """dvr_gen (cldvr::gen): GLL nodes = roots of P'_n plus endpoints,
+    w = 2/(n(n+1) P_n^2); quadrature exact to degree 2n-1."""
+    prob = _prob(nfe=4, rmax=8.0, ndvr=12)
+    o = O.Oracle(prob)
+    n = 12
+    x = o.table("dvr_x").reshape(4, n + 1)
+    w = o.table("dvr_w").reshape(4, n + 1)
+    p = np.polynomial.legendre.Legendre.basis(n)
+    ref = np.concatenate([[-1.0], np.sort(p.deriv().roots().real), [1.0]])
+    np.testing.assert_allclose((x[1] - 3.0), ref, atol=2e-15)  # element [2,4]: x = t + 3
+    wref = 2.0 / (n * (n + 1) * p(ref) ** 2)
+    np.testing.assert_allclose(w[1], wref, rtol=1e-13)
+    for k in range(2 * n):
+        exact = (4.0 ** (k + 1) - 2.0 ** (k + 1)) / (k + 1)
+        assert abs(np.dot(w[1], x[1] ** k) - exact) < 1e-11 * max(1.0, exact)
+
+
+def test_dshape_exact_derivative_on_polynomials():
+    """dshape[(n+1)m + k] = f_m'(x_k) including the endpoint rule -1/(2w_0)."""
+    prob = _prob(nfe=2, rmax=4.0, ndvr=10)
+    o = O.Oracle(prob)
+    n = 10
+    x = o.table("dvr_x").reshape(2, n + 1)[0]
+    ds = o.table("dshape").reshape(2, n + 1, n + 1)[0]  # [m][k]
+    for pw in range(n + 1):
+        f = x ** pw
+        df = pw * x ** (pw - 1) if pw > 0 else np.zeros_like(x)
+        np.testing.assert_allclose(f @ ds, df, atol=1e-10 * max(1, pw ** 2))
+
+
+def test_hydrogenic_spectrum_of_tmat():
+    """bas_gen_tmat + radk/radk0: eigenvalues -Z^2/(2 n^2) per l."""
+    for Z in (1.0, 2.0):
+        prob = _prob(znuc=Z, lmax1=2)
+        o = O.Oracle(prob)
+        n, nr = 20, o.nr
+        t = o.table("tmat").reshape(3, nr, 2 * n + 1)
+        for l in range(3):
+            A = np.zeros((nr, nr))
+            for i in range(nr):
+                for j in range(max(0, i - n), min(nr, i + n + 1)):
+                    A[i, j] = t[l, i, n + j - i].real
+            assert np.allclose(A, A.T, atol=1e-12)
+            ev = np.linalg.eigvalsh(A)
+            for k in range(2):
+                nq = l + 1 + k
+                if nq * nq / Z > 10.0:  # keep the state well inside the 60 a.u. box
+                    continue
+                assert abs(ev[k] + Z * Z / (2.0 * nq * nq)) < 1e-9, (l, k, ev[k])
+
+
+def test_gauss_legendre_vs_numpy():
+    prob = _prob(lmax2=17)
+    o = O.Oracle(prob)
+    x, w = np.polynomial.legendre.leggauss(18)
+    np.testing.assert_allclose(o.table("cost"), x, atol=1e-15)
+    np.testing.assert_allclose(o.table("wlat"), w, rtol=1e-13)
+
+
+def test_legendre_tables_vs_scipy_and_orthonormal():
+    """sph_gen: sqrt(2pi) x SHTOOLS orthonormal P_lm with Condon-Shortley
+    phase; negative m copies the positive-m table without sign."""
+    prob = _prob(lmax1=6, lmax2=12, ndcore=3, nel_alpha=3, nel_beta=3, mval=[0, 2, -2], mmax1=2, mmax2=4)
+    o = O.Oracle(prob)
+    nl, L1, L2 = o.nlat, o.L1, o.L2
+    cost = o.table("cost")
+    f1 = o.table("legf1").reshape(5, nl, L1)   # [m+mmax1][ilat][l]
+    b1 = o.table("legb1").reshape(5, L1, nl)
+    f2 = o.table("legf2").reshape(9, nl, L2)
+    for m in range(0, 3):
+        for l in range(m, L1):
+            norm = math.sqrt((2 * l + 1) / 2.0 * math.factorial(l - m) / math.factorial(l + m))
+            ref = norm * scipy.special.lpmv(m, l, cost)
+            np.testing.assert_allclose(f1[m + 2, :, l], ref, atol=1e-13)
+            np.testing.assert_allclose(f1[-m + 2, :, l], ref, atol=1e-13)
+        G = b1[m + 2][m:, :] @ f1[m + 2][:, m:]
+        np.testing.assert_allclose(G, np.eye(L1 - m), atol=1e-13)
+    for l in range(L2):
+        np.testing.assert_allclose(f2[4, :, l], math.sqrt((2 * l + 1) / 2.0) * scipy.special.eval_legendre(l, cost),
+                                   atol=1e-12)
+
+
+def test_alpha_is_cos_theta_matrix_element():
+    """bas_gen_alph: alph(l,m) = <l+1,m|cos(theta)|l,m>."""
+    prob = _prob(lmax1=8, lmax2=16, ndcore=2, nel_alpha=2, nel_beta=2, mval=[0, 1], mmax1=1, mmax2=2)
+    o = O.Oracle(prob)
+    nl, L1 = o.nlat, o.L1
+    f1 = o.table("legf1").reshape(3, nl, L1)
+    b1 = o.table("legb1").reshape(3, L1, nl)
+    al = o.table("alph").reshape(3, L1)
+    cost = o.table("cost")
+    for m in (0, 1):
+        for l in range(m, L1 - 1):
+            me = np.dot(b1[m + 1, l + 1], cost * f1[m + 1, :, l])
+            assert abs(me - al[m + 1, l]) < 1e-14
+
+
+def test_band_cholesky_vs_scipy():
+    rng = np.random.default_rng(3)
+    n, kd = 50, 6
+    A = np.zeros((n, n))
+    for i in range(n):
+        for j in range(max(0, i - kd), i + 1):
+            A[i, j] = A[j, i] = rng.standard_normal()
+    A += np.eye(n) * 20
+    ab = np.zeros(((kd + 1) * n))
+    for j in range(n):
+        for i in range(j, min(n, j + kd + 1)):
+            ab[(kd + 1) * j + i - j] = A[i, j]
+    assert O.band_chol(ab, kd) == 0
+    ref = scipy.linalg.cholesky_banded(np.array([[A[j + d, j] if j + d < n else 0 for j in range(n)]
+                                                 for d in range(kd + 1)]), lower=True)
+    np.testing.assert_allclose(ab.reshape(n, kd + 1).T, ref, atol=1e-13)
+    b = rng.standard_normal(n)
+    x = b.copy()
+    O.band_chol_solve(ab, kd, x)
+    np.testing.assert_allclose(A @ x, b, atol=1e-12)
+
+
+def test_d2ll_factor_reconstructs_poisson_operator():
+    """bas_gen_d2ll: L L^T = 2 radk + diag(L(L+1)/r^2) on the interior points."""
+    prob = _prob(nfe=6, rmax=12.0, lmax2=4)
+    o = O.Oracle(prob)
+    n, nr = 20, o.nr
+    radk = o.table("radk").reshape(o.nrad + 1, 2 * n + 1)
+    xr = o.table("xrad")
+    d2 = o.table("d2ll").reshape(o.L2, nr, n + 1)
+    for L in (0, 3):
+        A = np.zeros((nr, nr))
+        for i in range(nr):
+            for j in range(max(0, i - n), min(nr, i + n + 1)):
+                A[i, j] = 2 * radk[i + 1, n + j - i]
+            A[i, i] += L * (L + 1) / xr[i + 1] ** 2
+        Lm = np.zeros((nr, nr))
+        for j in range(nr):
+            for d in range(n + 1):
+                if j + d < nr:
+                    Lm[j + d, j] = d2[L, j, d]
+        np.testing.assert_allclose(Lm @ Lm.T, A, atol=1e-9 * np.abs(A).max())
+
+
+def _he_1s(Z=2.0, **kw):
+    prob = _prob(nfe=20, rmax=40.0, znuc=Z, **kw)
+    o = O.Oracle(prob)
+    u = W.hydrogenic_1s(prob, Z)
+    w = np.zeros(o.wfn_size, complex)
+    w[: o.nr] = u
+    w[-1] = 1.0
+    return prob, o, u, w
+
+
+def test_hartree_potential_of_1s_closed_form():
+    """mkrho2 + poisson + sph2ang2 + mfprod2 (dcj/dcx): for one doubly
+    occupied 1s, g = V_H psi with V_H = 1/r - (Z + 1/r) exp(-2 Z r)."""
+    prob, o, u, w = _he_1s()
+    o.htot(1.0, np.zeros(9), w)
+    r, _ = W.gll_grid(prob)
+    vh = 1 / r - (2.0 + 1 / r) * np.exp(-4.0 * r)
+    g = o.table("gorb")[: o.nr]
+    assert np.abs(g - vh * u).max() < 1e-11
+
+
+def test_he_first_order_energy():
+    """ene_dcore for hydrogenic 1s^2 at Z = 2: 2(-Z^2/2) + 5Z/8 = -2.75."""
+    _, o, _, w = _he_1s()
+    o.htot(1.0, np.zeros(9), w)
+    assert abs(o.energies()[1] + 2.75) < 1e-10
+
+
+def _u21(prob):
+    r, w = W.gll_grid(prob)
+    return (1.0 / (2.0 * math.sqrt(6.0))) * r * r * np.exp(-r / 2.0) * np.sqrt(w)
+
+
+def test_length_gauge_dipole_1s_2p():
+    """hprod_zprodp: <2p0|E z|1s> = E 128 sqrt(2)/243 for hydrogen."""
+    prob, o, u, w = _he_1s(Z=1.0)
+    E = 0.05
+    o.htot(1.0, [0, 0, E, 0, 0, E, 0, 0, 0], w)
+    h1 = o.table("h1orb")[: o.nbas]
+    me = np.vdot(_u21(prob), h1[o.nr: 2 * o.nr])
+    assert abs(me - E * 128 * math.sqrt(2) / 243) < 1e-9
+
+
+def test_velocity_gauge_dipole_1s_2p():
+    """hprod_pzprodp: <2p0|A p_z|1s> = i A (E_2p - E_1s) <2p0|z|1s>."""
+    prob, o, u, w = _he_1s(Z=1.0, igauge=1)
+    A = 0.3
+    o.htot(1.0, [0, 0, A, 0, 0, 0, 0, 0, A], w)
+    h1 = o.table("h1orb")[: o.nbas]
+    me = np.vdot(_u21(prob), h1[o.nr: 2 * o.nr])
+    ref = 1j * A * (-0.125 + 0.5) * 128 * math.sqrt(2) / 243
+    assert abs(me - ref) < 1e-8
+
+
+def test_orbital_derivative_is_projected():
+    """hprod_htot_dtorb + mkxmat_cc: for orthonormal core orbitals
+    <phi_j | d phi_i/dt> = 0 (the Q projector)."""
+    prob = Problem(nfe=6, ndvr=20, rmax=18.0, lmax1=4, lmax2=8, znuc=4.0, ndcore=3, nel_alpha=3, nel_beta=3)
+    o = O.Oracle(prob)
+    w = W.make_wfn(prob, 5, o.lcic)
+    h = o.htot(1.0, [0, 0, 0.05, 0, 0, 0.05, 0, 0, 0], w)
+    orbs = w[: o.norb].reshape(3, -1)
+    hs = h[: o.norb].reshape(3, -1)
+    S = orbs.conj() @ hs.T
+    assert np.abs(S).max() < 1e-11 * np.abs(hs).max()
+
+
+def _small_stiff_free():
+    # few GLL points per element keep the spectral radius small enough for
+    # an explicit step of 5e-3
+    return Problem(nfe=10, ndvr=6, rmax=20.0, lmax1=2, lmax2=4, znuc=2.0, ndcore=1, nel_alpha=1, nel_beta=1)
+
+
+def test_rk4_fourth_order_convergence():
+    """orc_rk_prop (clcrk::prop, rk_order 4): error ratio ~16 when dt halves."""
+    prob = _small_stiff_free()
+    o = O.Oracle(prob)
+    w = W.make_wfn(prob, 9, o.lcic)
+    T = 0.08
+
+    def run(nstep):
+        x = w.copy()
+        lf = np.zeros((4, 9))
+        lf[:, 2] = lf[:, 5] = 0.02
+        for _ in range(nstep):
+            x = o.rk_prop(4, T / nstep, lf, x, apply_mask=False)
+        return x
+
+    ref = run(64)
+    e1 = np.linalg.norm(run(8) - ref)
+    e2 = np.linalg.norm(run(16) - ref)
+    assert 12.0 < e1 / e2 < 20.0, e1 / e2
+
+
+def test_rk4_norm_drift_is_high_order():
+    """The RK4 norm drift over a fixed time falls like dt^5 (|R(iy)|^2 =
+    1 - y^6/72 + ...), i.e. by ~32x when dt halves."""
+    prob = _small_stiff_free()
+    o = O.Oracle(prob)
+    w = W.make_wfn(prob, 9, o.lcic)
+    lf = np.zeros((4, 9))
+
+    def drift(nstep, T=0.04):
+        x = w.copy()
+        for _ in range(nstep):
+            x = o.rk_prop(4, T / nstep, lf, x, apply_mask=False)
+        return abs(np.vdot(x[: o.norb], x[: o.norb]).real - 1.0)
+
+    d1, d2 = drift(20), drift(40)
+    assert d1 < 1e-6 and 20.0 < d1 / d2 < 45.0, (d1, d2)
+
+
+def test_closed_shell_rdms_match_reference_formula():
+    """ormas_mkden1/2 for a fully occupied active space: den1 = 2 delta and
+    den2(a,b,c,d) = 4 d_ab d_cd - 2 d_ad d_bc, the closed-shell blocks
+    ormas_mkden2_full writes (src/ormas/ormas_mkden2.F90:60-75)."""
+    prob = Problem(nfe=4, ndvr=8, rmax=8.0, lmax1=2, lmax2=4, znuc=8.0, nact=3, nel_alpha=3, nel_beta=3)
+    o = O.Oracle(prob)
+    d1, d2 = o.ci_mkden(np.ones(1, complex))
+    n = 3
+    np.testing.assert_allclose(d1, 2 * np.eye(n), atol=1e-15)
+    ref = np.zeros((n,) * 4)
+    for a in range(n):
+        for b in range(n):
+            for c in range(n):
+                for d in range(n):
+                    ref[a, b, c, d] = 4 * (a == b) * (c == d) - 2 * (a == d) * (b == c)
+    np.testing.assert_allclose(d2, ref, atol=1e-15)
+
+
+def _cas(nact=3, na=1, nb=1):
+    prob = Problem(nfe=4, ndvr=8, rmax=8.0, lmax1=2, lmax2=4, znuc=8.0, nact=nact, nel_alpha=na, nel_beta=nb)
+    return O.Oracle(prob)
+
+
+def _herm_ints(n, rng):
+    h = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    h = h + h.conj().T
+    # (ij|kl) with the symmetries of orbital products: (ij|kl) = conj(ji|lk) = (kl|ij)
+    g = rng.standard_normal((n,) * 4) + 1j * rng.standard_normal((n,) * 4)
+    g = g + g.transpose(1, 0, 3, 2).conj()
+    g = g + g.transpose(2, 3, 0, 1)
+    return h, g
+
+
+def test_cas_energy_from_rdms_equals_sigma():
+    """hprod_ene_act (E from den1/den2) == Re <C|H C> with ci_sigma's H."""
+    rng = np.random.default_rng(7)
+    o = _cas()
+    c = rng.standard_normal(o.lcic) + 1j * rng.standard_normal(o.lcic)
+    c /= np.linalg.norm(c)
+    h, g = _herm_ints(3, rng)
+    d1, d2 = o.ci_mkden(c)
+    e_rdm = np.einsum("ij,ji->", h, d1) + 0.5 * np.einsum("ijkl,jilk->", g, d2)
+    sig = o.ci_sigma(c, h, g)
+    assert abs(e_rdm - np.vdot(c, sig)) < 1e-12
+
+
+def test_cas_sigma_hermitian_and_rdm_contractions():
+    rng = np.random.default_rng(8)
+    o = _cas(nact=4, na=2, nb=1)
+    n = 4
+    h, g = _herm_ints(n, rng)
+    c1 = rng.standard_normal(o.lcic) + 1j * rng.standard_normal(o.lcic)
+    c2 = rng.standard_normal(o.lcic) + 1j * rng.standard_normal(o.lcic)
+    s1, s2 = o.ci_sigma(c1, h, g), o.ci_sigma(c2, h, g)
+    assert abs(np.vdot(c2, s1) - np.conj(np.vdot(c1, s2))) < 1e-11
+    c1 /= np.linalg.norm(c1)
+    d1, d2 = o.ci_mkden(c1)
+    N = 3
+    assert abs(np.trace(d1) - N) < 1e-12
+    np.testing.assert_allclose(d1, d1.conj().T, atol=1e-13)
+    # sum_c den2(a,b,c,c) = (N-1) den1(a,b)
+    np.testing.assert_allclose(np.einsum("abcc->ab", d2), (N - 1) * d1, atol=1e-12)
+
+
+def test_herm_eig_reconstructs():
+    rng = np.random.default_rng(2)
+    a = rng.standard_normal((6, 6)) + 1j * rng.standard_normal((6, 6))
+    a = a + a.conj().T
+    w, u = O.herm_eig(a)
+    np.testing.assert_allclose(u @ np.diag(w) @ u.conj().T, a, atol=1e-12)
+    np.testing.assert_allclose(np.sort(w), np.linalg.eigvalsh(a), atol=1e-12)
+
+
+def test_mask_closed_form():
+    """clfedvr::gen_mask: cos(pi/2 (r - rmask)/(R - rmask))^(1/4) beyond rmask."""
+    prob = _prob(nfe=10, rmax=20.0, rmask=12.0)
+    o = O.Oracle(prob)
+    x = o.table("xrad")
+    m = o.table("mask")
+    for i in range(1, o.nrad):
+        ref = 1.0 if x[i] <= 12.0 else math.cos(0.5 * math.pi * (x[i] - 12.0) / 8.0) ** 0.25
+        assert abs(m[i] - ref) < 1e-15
