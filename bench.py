@@ -1,0 +1,183 @@
+"""Benchmark: RK4 propagation of the Ne-like workload with the gfx950 hprod.
+
+Workload ne_l31: 80 FEDVR elements x 21 Gauss-Lobatto points on [0,160] a.u.
+(1599 radial points), lmax1 31 / lmax2 62, Z = 10, one frozen 1s core
+(type -1) + 4 active orbitals (one determinant), 800 nm sin^2 pulse of 4
+cycles at 1e14 W/cm^2, velocity gauge, complex128 throughout.
+
+A step is one classic RK4 step of the full wave function (4 hprod
+evaluations + stage combinations) followed by the radial mask, i.e.
+clcrk::prop + clwfn::mask as tdse_split2 runs them.  An explicit RK4 step
+of the full hprod is stable only while dt * rho < 2.8, and the spectral
+radius rho of the FEDVR one-body operator is 1.63e6 Ha here (the
+l(l+1)/2r^2 term at l = 31 on the first Gauss-Lobatto node), so the step
+uses dt = 1e-6 a.u.; the work per step does not depend on dt.
+One propagation = 2000 steps.
+
+Prints one JSON line (rank 0).  --gpus above 1 is not provided.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+NSTEP_PROP = 2000
+
+
+def kernel_bytes(name: str, s: dict) -> float:
+    """Algorithmic HBM bytes of one launch of a named kernel (c = 16 B per
+    complex128 element): every input read once, every output written once;
+    operator tables excluded."""
+    nr, L1, L2, nl = s["nr"], s["L1"], s["L2"], s["nlat"]
+    nf, nd, npair, nfc = s["nfun"], s["ndyn"], s["npair"], s["nfcore"]
+    nb, ng, c = nr * L1, nr * nl, 16
+    table = {
+        "l2theta": nd * (nb + ng) * c,
+        "fc_phase": 2 * nfc * ng * c,
+        "theta2l_fc": nfc * (ng + nb) * c,
+        "onebody": 2 * nf * nb * c,
+        "pair_theta2L": (nf * ng + npair * L2 * nr) * c,
+        "poisson": 2 * npair * L2 * nr * c,
+        "pair_L2theta": npair * (L2 * nr + ng) * c,
+        "mfprod": (nf * ng + npair * ng + 2 * nd * ng) * c,
+        "theta2l_mf": 2 * nd * (ng + nb) * c,
+        "gram_l": (nf + nfc + 4 * nd) * nb * c,
+        "gram_pair": 2 * s["nactpair"] * L2 * nr * c,
+        # k: read h01, g, v2, orb of nf and all orbitals for X; acc r+w; w0 r; next w; k w; earlier k r
+        "dtorb_rk": (nd * 3 + nf + nf * 2 + nf + nf + nf + 1.5 * nf) * nb * c,
+        "dtorb": (nd * 3 + nf + nf) * nb * c,
+        "island_a": 0.0,
+        "island_b": 0.0,
+    }
+    return float(table.get(name, 0.0))
+
+
+def hprod_bytes(s: dict, c: int = 16) -> float:
+    """Compulsory traffic of one hprod in the stage-separated dataflow:
+    2 No 16 + 2 No c + 4 Ng c + 4 Np c + 4 lcic 16."""
+    No = s["nr"] * s["L1"] * s["nfun"]
+    Ng = s["nr"] * s["nlat"] * s["ndyn"]
+    Np = s["nr"] * s["L2"] * s["npair"]
+    return 2 * No * 16 + 2 * No * c + 4 * Ng * c + 4 * Np * c + 4 * s["lcic"] * 16
+
+
+def cpu_baseline(prob, wfn, lfield, seconds: float = 12.0) -> dict:
+    """The FP64 oracle (single-threaded C restatement of hprod_htot) timed on
+    this host on a bounded sample of the same workload."""
+    from oracle.oracle import Oracle
+
+    o = Oracle(prob)
+    o.set_wfn0(wfn)
+    o.htot(1.0, lfield, wfn)  # warm
+    n, t0 = 0, time.perf_counter()
+    while True:
+        o.htot(1.0, lfield, wfn)
+        n += 1
+        el = time.perf_counter() - t0
+        if el > seconds or n >= 50:
+            break
+    return {"value": n / el, "unit": "hprod evals/s", "cores": 1, "kind": "port",
+            "sample": f"{n} hprod evaluations of ne_l31 (FP64 oracle, same inputs) in {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--dt", type=float, default=1.0e-6)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: --gpus {args.gpus} is not provided; this build runs one propagation set on one GPU",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import td1c_amd
+    from td1c_amd import workloads as W
+
+    prob = W.ne_l31()
+    field = W.ne_field()
+    hp = td1c_amd.HProd(prob)
+    wfn = W.make_wfn(prob, 2007, hp.lcic)
+    hp.set_wfn0(wfn)
+    rk = td1c_amd.CRK(4)
+    d = hp.to_device(wfn)
+    total = args.warmup + args.steps
+    lfs = [rk.stage_fields(field, i * args.dt, args.dt) for i in range(total)]
+    stream = torch.cuda.current_stream()
+    for i in range(args.warmup):
+        rk.prop(hp, d, i * args.dt, args.dt, lfield_stages=lfs[i])
+    torch.cuda.synchronize()
+    hp.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.warmup, total):
+        rk.prop(hp, d, i * args.dt, args.dt, lfield_stages=lfs[i])
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    times = hp.get_timing()
+    hp.set_timing(False)
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    norm = float(torch.linalg.vector_norm(d[: hp.norb]).item() ** 2)
+    if not np.isfinite(norm):
+        raise RuntimeError("propagation produced non-finite values")
+
+    s = dict(nr=prob.nr, L1=prob.lmax1 + 1, L2=prob.lmax2 + 1, nlat=prob.lmax2 + 1, nfun=prob.nfun,
+             ndyn=prob.nfun - prob.nfcore, npair=hp.npair, nfcore=prob.nfcore, lcic=hp.lcic,
+             nactpair=prob.nact * (prob.nact + 1) // 2)
+    dom = max(times.items(), key=lambda kv: kv[1][0])
+    dname, (dms, dcount) = dom
+    avg_s = dms / dcount * 1e-3
+    abytes = kernel_bytes(dname, s)
+    achieved = abytes / avg_s / 1e9 if abytes > 0 else 0.0
+    hprods_per_s = 4.0 / (ms_step * 1e-3)
+    step_bytes = 4 * hprod_bytes(s) + 15 * (hp.wfn_size * 16)
+    result = {
+        "metric": "hprod evals/s + propagations/s, mixed vs FP64 at dipole err <=1e-6; %HBM peak",
+        "value": hprods_per_s,
+        "unit": "hprod evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c128",
+        "data": "synthetic (seeded orbitals, one CI determinant)",
+        "config": {"workload": "ne_l31: Ne-like RK4, 80x21 FEDVR (1599 pts) on [0,160], lmax1 31/lmax2 62, "
+                               "Z=10, 1 frozen core + 4 active, 800 nm sin2 1e14 W/cm2 velocity gauge",
+                   "precision": "fp64", "rk_order": 4, "dt": args.dt, "steps_per_propagation": NSTEP_PROP,
+                   "parallelism": "single GPU"},
+        "propagations_per_s": 1.0 / (NSTEP_PROP * ms_step * 1e-3),
+        "hbm_frac_step": step_bytes / (ms_step * 1e-3) / (HBM_PEAK_GBS * 1e9),
+        "roofline": {"bound": "hbm", "kernel": dname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes_per_launch": abytes, "avg_launch_us": avg_s * 1e6},
+        "kernel_ms_per_step": {k: v[0] / args.steps for k, v in sorted(times.items(), key=lambda kv: -kv[1][0])},
+        "norm_after": norm,
+    }
+    if not args.no_cpu:
+        lf0 = lfs[0][0]
+        result["cpu_baseline"] = cpu_baseline(prob, wfn, lf0, args.cpu_seconds)
+    else:
+        result["cpu_baseline"] = None
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

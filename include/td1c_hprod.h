@@ -106,7 +106,7 @@ typedef struct td1c_field {
   double wlen;
   double cep;
   double numcyc;
-  double delay;   /* attoseconds-free: a.u. of time after unit conversion */
+  double delay;   /* pulse delay in attoseconds (converted as 1 a.u. = 24.188843265 as) */
 } td1c_field;
 
 /* Builds the FEDVR grid, angular quadrature, every operator table and the
@@ -161,6 +161,15 @@ int td1c_field_value(const td1c_field* field, double t, double lfield[9]);
 
 /* Optical period (a.u.) of the field, used to set dtime = period/nstep. */
 double td1c_field_period(const td1c_field* field);
+
+/* Per-kernel timing with HIP events recorded on the launch stream around
+ * every kernel the context enqueues (no counterpart in the reference, which
+ * only times whole runs with clock(), src/main/tdse.cpp:128-129).
+ * td1c_set_timing(ctx, 1) starts collecting (and clears), 0 stops;
+ * td1c_get_timing writes {"kernel": [total_ms, launches], ...} as JSON and
+ * clears the collected records. */
+int td1c_set_timing(td1c_ctx* ctx, int on);
+int td1c_get_timing(td1c_ctx* ctx, char* buf, int buflen);
 
 /* Binds a context as the process-wide one used by hprod_htot_ (the
  * reference allows a single clhprod per process, clhprod.cpp:32-34). */

@@ -68,6 +68,10 @@ def load_library():
     L.td1c_bind.restype = ctypes.c_int
     L.hprod_htot_.argtypes = [vp, vp, vp, vp, vp, vp]
     L.hprod_htot_.restype = None
+    L.td1c_set_timing.argtypes = [vp, ctypes.c_int]
+    L.td1c_set_timing.restype = ctypes.c_int
+    L.td1c_get_timing.argtypes = [vp, ctypes.c_char_p, ctypes.c_int]
+    L.td1c_get_timing.restype = ctypes.c_int
     L.td1c_last_error.argtypes = []
     L.td1c_last_error.restype = ctypes.c_char_p
     _lib = L
@@ -178,6 +182,18 @@ class HProd:
     def mask(self, wfn, stream=None):
         _check(load_library().td1c_mask(self.h, ctypes.c_void_p(wfn.data_ptr()), self._stream(stream)), "td1c_mask")
         return wfn
+
+    def set_timing(self, on: bool):
+        """Per-kernel HIP-event timing on the launch stream."""
+        _check(load_library().td1c_set_timing(self.h, int(on)), "td1c_set_timing")
+
+    def get_timing(self) -> dict:
+        """{kernel: (total_ms, launches)} collected since set_timing(True)."""
+        import json
+
+        buf = ctypes.create_string_buffer(1 << 16)
+        _check(load_library().td1c_get_timing(self.h, buf, len(buf)), "td1c_get_timing")
+        return {k: tuple(v) for k, v in json.loads(buf.value.decode()).items()}
 
     def bind(self):
         """Makes this context the one the legacy hprod_htot_ entry point uses."""

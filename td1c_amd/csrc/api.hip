@@ -91,6 +91,12 @@ struct td1c_ctx {
   // RK work: w0, t, k[stages] over the full wave function
   DevBuf<double2> rk_w0, rk_t, rk_k;
   bool wfn0_set = false;
+  // optional per-kernel timing with HIP events on the launch stream
+  bool timing = false;
+  std::vector<hipEvent_t> ev_pool;
+  struct Rec { const char* name; int e0, e1; };
+  std::vector<Rec> recs;
+  int ev_next = 0;
 
   ~td1c_ctx() {
     DevBuf<double>* d[] = {&xr, &band_k, &band_k0, &band_p, &alph, &cost, &mask, &m_l2g, &m_g2l, &m_g2L, &m_L2g,
@@ -107,12 +113,33 @@ struct td1c_ctx {
     for (auto* b : t) b->release();
     DevBuf<const double2*>* g[] = {&g_left_l, &g_right_l, &g_left_p, &g_right_p};
     for (auto* b : g) b->release();
+    for (auto e : ev_pool) (void)hipEventDestroy(e);
   }
 };
 
 namespace {
 
 td1c_ctx* g_bound = nullptr;
+
+// Records an event pair around one launch when timing is on.
+struct Timed {
+  td1c_ctx* c;
+  hipStream_t s;
+  const char* name;
+  int e0 = -1;
+  Timed(td1c_ctx* c_, hipStream_t s_, const char* n) : c(c_), s(s_), name(n) {
+    if (!c->timing || c->ev_next + 2 > (int)c->ev_pool.size()) return;
+    e0 = c->ev_next;
+    c->ev_next += 2;
+    (void)hipEventRecord(c->ev_pool[e0], s);
+  }
+  ~Timed() {
+    if (e0 < 0) return;
+    (void)hipEventRecord(c->ev_pool[e0 + 1], s);
+    c->recs.push_back({name, e0, e0 + 1});
+  }
+};
+#define TIMED(name) Timed timed_##__LINE__(c, s, name)
 
 template <class T>
 std::vector<T> flat_or_one(const std::vector<T>& v) {
@@ -374,14 +401,14 @@ int enqueue_htotx(td1c_ctx* c, double dtime, const double* lfield, const double2
   // current dynamical orbitals (hprod_orbin)
   HIP_TRY(hipMemcpyAsync(c->orb + nb * nfc, wfn + nb * nfc, nb * (nf - nfc) * sizeof(double2),
                          hipMemcpyDeviceToDevice, s));
-  launch_transform(0, c->orb, c->orbg, c->m_l2g.p, c->b_l2g_dyn.p, c->n_l2g_dyn, nr, L1, nl, s);
+  { TIMED("l2theta"); launch_transform(0, c->orb, c->orbg, c->m_l2g.p, c->b_l2g_dyn.p, c->n_l2g_dyn, nr, L1, nl, s); }
   // frozen core in the velocity gauge (hprod_fcorbg): phase on the grid, back to l-space
   if (nfc > 0 && S.igauge == 1) {
-    launch_fc_phase(c->orbg0.p, c->orbg, c->xr.p, c->cost.p, lfield[8], nr, S.nradfc, nl, nfc, s);
-    launch_transform(0, c->orbg, c->orb, c->m_g2l.p, c->b_g2l_fc.p, c->n_g2l_fc, nr, nl, L1, s);
+    { TIMED("fc_phase"); launch_fc_phase(c->orbg0.p, c->orbg, c->xr.p, c->cost.p, lfield[8], nr, S.nradfc, nl, nfc, s); }
+    { TIMED("theta2l_fc"); launch_transform(0, c->orbg, c->orb, c->m_g2l.p, c->b_g2l_fc.p, c->n_g2l_fc, nr, nl, L1, s); }
   }
   IslandArgs ia = island_args(c, cic, dtime);
-  if (S.nact > 0) launch_island_a(ia, s);
+  if (S.nact > 0) { TIMED("island_a"); launch_island_a(ia, s); }
   // one-body part
   OneBodyArgs ob{};
   ob.nr = nr; ob.L1 = L1; ob.ndvr = S.n; ob.nfcore = nfc; ob.nradfc = S.nradfc;
@@ -389,11 +416,11 @@ int enqueue_htotx(td1c_ctx* c, double dtime, const double* lfield, const double2
   ob.mval = c->mval.p; ob.band_k = c->band_k.p; ob.band_k0 = c->band_k0.p; ob.band_p = c->band_p.p;
   ob.xr = c->xr.p; ob.alph = c->alph.p; ob.tdiag = c->tdiag.p; ob.zfield = zfield; ob.efield = efield;
   ob.f0 = 0; ob.f1 = nf;
-  launch_onebody(c->orb, c->h01, ob, s);
+  { TIMED("onebody"); launch_onebody(c->orb, c->h01, ob, s); }
   // mean field
   const size_t ng = (size_t)c->ngrid;
   if (S.nel[2] >= 2 && c->npair > 0) {
-    launch_transform(1, c->orbg, c->rhoL.p, c->m_g2L.p, c->b_rho.p, c->npair, nr, nl, L2, s);
+    { TIMED("pair_theta2L"); launch_transform(1, c->orbg, c->rhoL.p, c->m_g2L.p, c->b_rho.p, c->npair, nr, nl, L2, s); }
     PoissonArgs pa{};
     pa.nr = nr; pa.n = S.n; pa.nfe = S.nfe; pa.ni = S.ni; pa.L2 = L2; pa.npair = c->npair;
     pa.pair_m = c->pair_m.p;
@@ -401,15 +428,15 @@ int enqueue_htotx(td1c_ctx* c, double dtime, const double* lfield, const double2
     pa.Sd = c->pSd.p; pa.Sl = c->pSl.p; pa.Su = c->pSu.p;
     pa.invr = c->invr.p; pa.rpl0 = c->rpl0.p; pa.rpl1 = c->rpl1.p; pa.fac2 = c->fac2.p; pa.fac1 = c->fac1.p;
     pa.cl = c->p_cl.p; pa.cr = c->p_cr.p; pa.mpart = c->p_mp.p; pa.Q = c->p_Q.p;
-    launch_poisson(c->rhoL.p, c->pwork.p, c->vL.p, pa, s);
-    launch_transform(0, c->vL.p, c->vg.p, c->m_L2g.p, c->b_L2g.p, c->npair, nr, L2, nl, s);
+    { TIMED("poisson"); launch_poisson(c->rhoL.p, c->pwork.p, c->vL.p, pa, s); }
+    { TIMED("pair_L2theta"); launch_transform(0, c->vL.p, c->vg.p, c->m_L2g.p, c->b_L2g.p, c->npair, nr, L2, nl, s); }
     MfArgs ma{};
     ma.nr = nr; ma.nlat = nl; ma.nfun = nf; ma.nfcore = nfc; ma.ncore = S.ncore; ma.nact = S.nact;
     ma.nradfc = S.nradfc; ma.npair = c->npair; ma.nelact2 = S.nelact[2];
     ma.pj = c->pj.p; ma.pi = c->pi.p; ma.ptype = c->ptype.p; ma.mval = c->mval.p;
     ma.den1 = c->den1.p; ma.den2 = c->den2.p; ma.rden = c->rden.p;
-    launch_mfprod(c->orbg, c->vg.p, c->gorbg, c->v2orbg, ma, s);
-    launch_transform(0, c->gorbg, c->gorb, c->m_g2l.p, c->b_g2l_mf.p, c->n_g2l_mf, nr, nl, L1, s);
+    { TIMED("mfprod"); launch_mfprod(c->orbg, c->vg.p, c->gorbg, c->v2orbg, ma, s); }
+    { TIMED("theta2l_mf"); launch_transform(0, c->gorbg, c->gorb, c->m_g2l.p, c->b_g2l_mf.p, c->n_g2l_mf, nr, nl, L1, s); }
   } else {
     HIP_TRY(hipMemsetAsync(c->gorb, 0, 2 * nb * c->nd * sizeof(double2), s));
   }
@@ -418,14 +445,14 @@ int enqueue_htotx(td1c_ctx* c, double dtime, const double* lfield, const double2
   GramArgs g{};
   g.len = nb; g.nL = c->gl_nL; g.nR = c->gl_nR; g.conj_left = 1; g.with_conj_right = 0;
   g.left = c->g_left_l.p; g.right = c->g_right_l.p;
-  launch_gram(g, c->gram_partial.p, c->gram_blk, c->lprod.p, s);
+  { TIMED("gram_l"); launch_gram(g, c->gram_partial.p, c->gram_blk, c->lprod.p, s); }
   if (c->gp_nL > 0) {
     GramArgs h{};
     h.len = (size_t)L2 * nr; h.nL = c->gp_nL; h.nR = c->gp_nR; h.conj_left = 0; h.with_conj_right = 1;
     h.left = c->g_left_p.p; h.right = c->g_right_p.p;
-    launch_gram(h, c->gram_partial.p, c->gram_blk, c->pprod.p, s);
+    { TIMED("gram_pair"); launch_gram(h, c->gram_partial.p, c->gram_blk, c->pprod.p, s); }
   }
-  launch_island_b(ia, s);
+  { TIMED("island_b"); launch_island_b(ia, s); }
   return 0;
 }
 
@@ -600,7 +627,7 @@ int td1c_htot(td1c_ctx* c, double dtime, const double* lfield, const double* d_w
   if (rc) return rc;
   DtorbArgs d = dtorb_args(c, dtime);
   d.kout = reinterpret_cast<double2*>(d_hwfn);
-  launch_dtorb(d, s);
+  { TIMED("dtorb"); launch_dtorb(d, s); }
   if (d_hcic)
     HIP_TRY(hipMemcpyAsync(d_hcic, c->dcic.p, c->S.lcic * sizeof(double2), hipMemcpyDeviceToDevice, s));
   HIP_TRY(hipGetLastError());
@@ -678,7 +705,7 @@ int td1c_rk_prop(td1c_ctx* c, int rk_order, double dtime, const double* lfield_s
       }
       d.cnew = dtime * a[is + 1][is];
     }
-    launch_dtorb(d, s);
+    { TIMED("dtorb_rk"); launch_dtorb(d, s); }
     // CI part: k_cic = dcic; acc += dtw k; next = w0 + sum a k
     HIP_TRY(hipMemcpyAsync(k + so, c->dcic.p, S.lcic * sizeof(double2), hipMemcpyDeviceToDevice, s));
     AxpyArgs ax{};
@@ -782,6 +809,45 @@ int td1c_field_value(const td1c_field* f, double t, double lfield[9]) {
 double td1c_field_period(const td1c_field* f) {
   if (!f) return 0.0;
   return 2.0 * kPi / (1239.84190 / (f->wlen * 27.2113845));
+}
+
+int td1c_set_timing(td1c_ctx* c, int on) {
+  if (!c) return fail(-1, "null argument");
+  if (on && c->ev_pool.empty()) {
+    c->ev_pool.resize(8192);
+    for (auto& e : c->ev_pool) HIP_TRY(hipEventCreate(&e));
+  }
+  HIP_TRY(hipDeviceSynchronize());
+  c->timing = on != 0;
+  c->recs.clear();
+  c->ev_next = 0;
+  return 0;
+}
+
+int td1c_get_timing(td1c_ctx* c, char* buf, int buflen) {
+  if (!c || !buf || buflen <= 0) return fail(-1, "null argument");
+  HIP_TRY(hipDeviceSynchronize());
+  std::vector<std::pair<std::string, std::pair<double, int>>> agg;
+  for (auto& r : c->recs) {
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, c->ev_pool[r.e0], c->ev_pool[r.e1]));
+    auto it = std::find_if(agg.begin(), agg.end(), [&](auto& x) { return x.first == r.name; });
+    if (it == agg.end()) agg.push_back({r.name, {ms, 1}});
+    else { it->second.first += ms; it->second.second += 1; }
+  }
+  std::string out = "{";
+  for (size_t k = 0; k < agg.size(); ++k) {
+    char tmp[160];
+    std::snprintf(tmp, sizeof(tmp), "%s\"%s\": [%.6f, %d]", k ? ", " : "", agg[k].first.c_str(), agg[k].second.first,
+                  agg[k].second.second);
+    out += tmp;
+  }
+  out += "}";
+  c->recs.clear();
+  c->ev_next = 0;
+  if ((int)out.size() + 1 > buflen) return fail(-1, "timing buffer too small");
+  std::memcpy(buf, out.c_str(), out.size() + 1);
+  return 0;
 }
 
 int td1c_bind(td1c_ctx* c) {
