@@ -170,6 +170,9 @@ double td1c_field_period(const td1c_field* field);
  * clears the collected records. */
 int td1c_set_timing(td1c_ctx* ctx, int on);
 int td1c_get_timing(td1c_ctx* ctx, char* buf, int buflen);
+/* Restricts the timing to the kernel of that name (NULL: every kernel), so
+ * that a timed run carries events around one kernel only. */
+int td1c_set_timing_kernel(td1c_ctx* ctx, const char* name);
 
 /* Binds a context as the process-wide one used by hprod_htot_ (the
  * reference allows a single clhprod per process, clhprod.cpp:32-34). */

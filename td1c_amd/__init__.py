@@ -72,6 +72,8 @@ def load_library():
     L.td1c_set_timing.restype = ctypes.c_int
     L.td1c_get_timing.argtypes = [vp, ctypes.c_char_p, ctypes.c_int]
     L.td1c_get_timing.restype = ctypes.c_int
+    L.td1c_set_timing_kernel.argtypes = [vp, ctypes.c_char_p]
+    L.td1c_set_timing_kernel.restype = ctypes.c_int
     L.td1c_last_error.argtypes = []
     L.td1c_last_error.restype = ctypes.c_char_p
     _lib = L
@@ -183,9 +185,12 @@ class HProd:
         _check(load_library().td1c_mask(self.h, ctypes.c_void_p(wfn.data_ptr()), self._stream(stream)), "td1c_mask")
         return wfn
 
-    def set_timing(self, on: bool):
-        """Per-kernel HIP-event timing on the launch stream."""
-        _check(load_library().td1c_set_timing(self.h, int(on)), "td1c_set_timing")
+    def set_timing(self, on: bool, only: Optional[str] = None):
+        """Per-kernel HIP-event timing on the launch stream; `only` restricts
+        the events to the kernel of that name."""
+        L = load_library()
+        _check(L.td1c_set_timing_kernel(self.h, only.encode() if only else None), "td1c_set_timing_kernel")
+        _check(L.td1c_set_timing(self.h, int(on)), "td1c_set_timing")
 
     def get_timing(self) -> dict:
         """{kernel: (total_ms, launches)} collected since set_timing(True)."""

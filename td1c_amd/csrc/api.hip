@@ -63,7 +63,8 @@ struct td1c_ctx {
   int device = 0;
   // tables
   DevBuf<double> xr, band_k, band_k0, band_p, alph, cost, mask;
-  DevBuf<double2> tdiag;
+  DevBuf<double2> tdiag, tzor;
+  DevBuf<double> eK0, eK, eP;
   DevBuf<double> m_l2g, m_g2l, m_g2L, m_L2g;
   DevBuf<double> pG, pCl, pCr, pWl, pWr, pSd, pSl, pSu, invr, rpl0, rpl1, fac2, fac1;
   DevBuf<int> mval, pair_m, pj, pi, ptype;
@@ -97,12 +98,14 @@ struct td1c_ctx {
   struct Rec { const char* name; int e0, e1; };
   std::vector<Rec> recs;
   int ev_next = 0;
+  std::string timing_only;  // empty: every kernel
+  std::vector<std::pair<std::string, std::pair<double, int>>> timing_agg;
 
   ~td1c_ctx() {
-    DevBuf<double>* d[] = {&xr, &band_k, &band_k0, &band_p, &alph, &cost, &mask, &m_l2g, &m_g2l, &m_g2L, &m_L2g,
+    DevBuf<double>* d[] = {&eK0, &eK, &eP, &xr, &band_k, &band_k0, &band_p, &alph, &cost, &mask, &m_l2g, &m_g2l, &m_g2L, &m_L2g,
                            &pG, &pCl, &pCr, &pWl, &pWr, &pSd, &pSl, &pSu, &invr, &rpl0, &rpl1, &fac2, &fac1, &ene};
     for (auto* b : d) b->release();
-    DevBuf<double2>* z[] = {&tdiag, &lsp, &gsp, &orbg0, &rhoL, &vL, &pwork, &vg, &p_cl, &p_cr, &p_mp, &p_Q,
+    DevBuf<double2>* z[] = {&tdiag, &tzor, &lsp, &gsp, &orbg0, &rhoL, &vL, &pwork, &vg, &p_cl, &p_cr, &p_mp, &p_Q,
                             &gram_partial, &lprod, &pprod, &den1, &den2, &rden, &rrden, &int1e, &int2e, &xmat,
                             &dcic, &ci_work, &rk_w0, &rk_t, &rk_k};
     for (auto* b : z) b->release();
@@ -121,14 +124,36 @@ namespace {
 
 td1c_ctx* g_bound = nullptr;
 
-// Records an event pair around one launch when timing is on.
+// Folds the completed event pairs into the per-kernel totals and frees the pool.
+hipError_t harvest_timing(td1c_ctx* c) {
+  if (c->recs.empty()) return hipSuccess;
+  hipError_t e = hipEventSynchronize(c->ev_pool[c->recs.back().e1]);
+  if (e != hipSuccess) return e;
+  for (auto& r : c->recs) {
+    float ms = 0.0f;
+    e = hipEventElapsedTime(&ms, c->ev_pool[r.e0], c->ev_pool[r.e1]);
+    if (e != hipSuccess) return e;
+    auto it = std::find_if(c->timing_agg.begin(), c->timing_agg.end(), [&](auto& x) { return x.first == r.name; });
+    if (it == c->timing_agg.end()) c->timing_agg.push_back({r.name, {ms, 1}});
+    else { it->second.first += ms; it->second.second += 1; }
+  }
+  c->recs.clear();
+  c->ev_next = 0;
+  return hipSuccess;
+}
+
+// Records an event pair around one launch when timing is on (for every
+// kernel, or only for the one named by td1c_set_timing_kernel).  A full event
+// pool is folded into the totals first, which waits for the recorded work.
 struct Timed {
   td1c_ctx* c;
   hipStream_t s;
   const char* name;
   int e0 = -1;
   Timed(td1c_ctx* c_, hipStream_t s_, const char* n) : c(c_), s(s_), name(n) {
-    if (!c->timing || c->ev_next + 2 > (int)c->ev_pool.size()) return;
+    if (!c->timing) return;
+    if (!c->timing_only.empty() && c->timing_only != name) return;
+    if (c->ev_next + 2 > (int)c->ev_pool.size() && harvest_timing(c) != hipSuccess) return;
     e0 = c->ev_next;
     c->ev_next += 2;
     (void)hipEventRecord(c->ev_pool[e0], s);
@@ -144,6 +169,17 @@ struct Timed {
 template <class T>
 std::vector<T> flat_or_one(const std::vector<T>& v) {
   return v.empty() ? std::vector<T>(1, T()) : v;
+}
+
+// [nm][K][C] -> [nm][K][Cp] with the columns padded by zeros to a multiple of 32
+std::vector<double> padded_cols(const std::vector<double>& m, int K, int C) {
+  const int Cp = pad_cols(C);
+  const size_t nm = m.size() / ((size_t)K * C);
+  std::vector<double> out(nm * K * Cp, 0.0);
+  for (size_t i = 0; i < nm; ++i)
+    for (int k = 0; k < K; ++k)
+      for (int j = 0; j < C; ++j) out[(i * K + k) * Cp + j] = m[(i * K + k) * C + j];
+  return out;
 }
 
 int build_device(td1c_ctx* c) {
@@ -166,14 +202,21 @@ int build_device(td1c_ctx* c) {
     HIP_TRY(c->mask.upload(m));
   }
   {
-    std::vector<double2> td((size_t)L1 * nr);
-    for (size_t q = 0; q < td.size(); ++q) td[q] = make_double2(S.tdiag[2 * q], S.tdiag[2 * q + 1]);
+    std::vector<double2> td((size_t)L1 * nr), tz((size_t)L1 * nr);
+    for (size_t q = 0; q < td.size(); ++q) {
+      td[q] = make_double2(S.tdiag[2 * q], S.tdiag[2 * q + 1]);
+      tz[q] = make_double2(S.tzor[2 * q], S.tzor[2 * q + 1]);
+    }
     HIP_TRY(c->tdiag.upload(td));
+    HIP_TRY(c->tzor.upload(tz));
   }
-  HIP_TRY(c->m_l2g.upload(S.mat_l2g));
-  HIP_TRY(c->m_g2l.upload(S.mat_g2l));
-  HIP_TRY(c->m_g2L.upload(S.mat_g2L));
-  HIP_TRY(c->m_L2g.upload(S.mat_L2g));
+  HIP_TRY(c->eK0.upload(S.eK0));
+  HIP_TRY(c->eK.upload(S.eK));
+  HIP_TRY(c->eP.upload(S.eP));
+  HIP_TRY(c->m_l2g.upload(padded_cols(S.mat_l2g, S.L1, S.nlat)));
+  HIP_TRY(c->m_g2l.upload(padded_cols(S.mat_g2l, S.nlat, S.L1)));
+  HIP_TRY(c->m_g2L.upload(padded_cols(S.mat_g2L, S.nlat, S.L2)));
+  HIP_TRY(c->m_L2g.upload(padded_cols(S.mat_L2g, S.L2, S.nlat)));
   HIP_TRY(c->pG.upload(S.pG));
   HIP_TRY(c->pCl.upload(S.pCl));
   HIP_TRY(c->pCr.upload(S.pCr));
@@ -398,14 +441,16 @@ int enqueue_htotx(td1c_ctx* c, double dtime, const double* lfield, const double2
   const int nr = S.nr, nf = S.nfun, nfc = S.nfcore, L1 = S.L1, L2 = S.L2, nl = S.nlat;
   const size_t nb = (size_t)c->nbas;
   const double2 zfield = make_double2(lfield[2], 0.0), efield = make_double2(lfield[5], 0.0);
-  // current dynamical orbitals (hprod_orbin)
-  HIP_TRY(hipMemcpyAsync(c->orb + nb * nfc, wfn + nb * nfc, nb * (nf - nfc) * sizeof(double2),
-                         hipMemcpyDeviceToDevice, s));
-  { TIMED("l2theta"); launch_transform(0, c->orb, c->orbg, c->m_l2g.p, c->b_l2g_dyn.p, c->n_l2g_dyn, nr, L1, nl, s); }
+  // current dynamical orbitals (hprod_orbin): copied into the work array by
+  // the l -> theta transform that reads them
+  {
+    TIMED("l2theta");
+    launch_transform(0, wfn, c->orbg, c->m_l2g.p, c->b_l2g_dyn.p, c->n_l2g_dyn, nr, L1, nl, pad_cols(nl), c->orb, s);
+  }
   // frozen core in the velocity gauge (hprod_fcorbg): phase on the grid, back to l-space
   if (nfc > 0 && S.igauge == 1) {
     { TIMED("fc_phase"); launch_fc_phase(c->orbg0.p, c->orbg, c->xr.p, c->cost.p, lfield[8], nr, S.nradfc, nl, nfc, s); }
-    { TIMED("theta2l_fc"); launch_transform(0, c->orbg, c->orb, c->m_g2l.p, c->b_g2l_fc.p, c->n_g2l_fc, nr, nl, L1, s); }
+    { TIMED("theta2l_fc"); launch_transform(0, c->orbg, c->orb, c->m_g2l.p, c->b_g2l_fc.p, c->n_g2l_fc, nr, nl, L1, pad_cols(L1), nullptr, s); }
   }
   IslandArgs ia = island_args(c, cic, dtime);
   if (S.nact > 0) { TIMED("island_a"); launch_island_a(ia, s); }
@@ -416,11 +461,19 @@ int enqueue_htotx(td1c_ctx* c, double dtime, const double* lfield, const double2
   ob.mval = c->mval.p; ob.band_k = c->band_k.p; ob.band_k0 = c->band_k0.p; ob.band_p = c->band_p.p;
   ob.xr = c->xr.p; ob.alph = c->alph.p; ob.tdiag = c->tdiag.p; ob.zfield = zfield; ob.efield = efield;
   ob.f0 = 0; ob.f1 = nf;
-  { TIMED("onebody"); launch_onebody(c->orb, c->h01, ob, s); }
+  OneBodyElArgs oe{};
+  oe.nr = nr; oe.nfe = S.nfe; oe.L1 = L1; oe.ndvr = S.n; oe.nfcore = nfc; oe.nradfc = S.nradfc; oe.nfun = nf;
+  oe.mmax1 = S.mmax1; oe.igauge = S.igauge; oe.fc_full = S.iprojfc == 2; oe.mval = c->mval.p;
+  oe.eK0 = c->eK0.p; oe.eK = c->eK.p; oe.eP = c->eP.p; oe.xr = c->xr.p; oe.alph = c->alph.p; oe.tzor = c->tzor.p;
+  oe.zfield = zfield; oe.efield = efield;
+  {
+    TIMED("onebody");
+    if (!launch_onebody_el(c->orb, c->h01, oe, s)) launch_onebody(c->orb, c->h01, ob, s);
+  }
   // mean field
   const size_t ng = (size_t)c->ngrid;
   if (S.nel[2] >= 2 && c->npair > 0) {
-    { TIMED("pair_theta2L"); launch_transform(1, c->orbg, c->rhoL.p, c->m_g2L.p, c->b_rho.p, c->npair, nr, nl, L2, s); }
+    { TIMED("pair_theta2L"); launch_transform(1, c->orbg, c->rhoL.p, c->m_g2L.p, c->b_rho.p, c->npair, nr, nl, L2, pad_cols(L2), nullptr, s); }
     PoissonArgs pa{};
     pa.nr = nr; pa.n = S.n; pa.nfe = S.nfe; pa.ni = S.ni; pa.L2 = L2; pa.npair = c->npair;
     pa.pair_m = c->pair_m.p;
@@ -429,14 +482,14 @@ int enqueue_htotx(td1c_ctx* c, double dtime, const double* lfield, const double2
     pa.invr = c->invr.p; pa.rpl0 = c->rpl0.p; pa.rpl1 = c->rpl1.p; pa.fac2 = c->fac2.p; pa.fac1 = c->fac1.p;
     pa.cl = c->p_cl.p; pa.cr = c->p_cr.p; pa.mpart = c->p_mp.p; pa.Q = c->p_Q.p;
     { TIMED("poisson"); launch_poisson(c->rhoL.p, c->pwork.p, c->vL.p, pa, s); }
-    { TIMED("pair_L2theta"); launch_transform(0, c->vL.p, c->vg.p, c->m_L2g.p, c->b_L2g.p, c->npair, nr, L2, nl, s); }
+    { TIMED("pair_L2theta"); launch_transform(0, c->vL.p, c->vg.p, c->m_L2g.p, c->b_L2g.p, c->npair, nr, L2, nl, pad_cols(nl), nullptr, s); }
     MfArgs ma{};
-    ma.nr = nr; ma.nlat = nl; ma.nfun = nf; ma.nfcore = nfc; ma.ncore = S.ncore; ma.nact = S.nact;
+    ma.nr = nr; ma.nlat = nl; ma.nfun = nf; ma.nfcore = nfc; ma.nfcore2 = S.nfcore2; ma.ncore = S.ncore; ma.nact = S.nact;
     ma.nradfc = S.nradfc; ma.npair = c->npair; ma.nelact2 = S.nelact[2];
     ma.pj = c->pj.p; ma.pi = c->pi.p; ma.ptype = c->ptype.p; ma.mval = c->mval.p;
     ma.den1 = c->den1.p; ma.den2 = c->den2.p; ma.rden = c->rden.p;
     { TIMED("mfprod"); launch_mfprod(c->orbg, c->vg.p, c->gorbg, c->v2orbg, ma, s); }
-    { TIMED("theta2l_mf"); launch_transform(0, c->gorbg, c->gorb, c->m_g2l.p, c->b_g2l_mf.p, c->n_g2l_mf, nr, nl, L1, s); }
+    { TIMED("theta2l_mf"); launch_transform(0, c->gorbg, c->gorb, c->m_g2l.p, c->b_g2l_mf.p, c->n_g2l_mf, nr, nl, L1, pad_cols(L1), nullptr, s); }
   } else {
     HIP_TRY(hipMemsetAsync(c->gorb, 0, 2 * nb * c->nd * sizeof(double2), s));
   }
@@ -509,7 +562,8 @@ int td1c_ctx_create(const td1c_problem* prob, td1c_ctx** out) {
   if (c->S.nfun > kMaxFun) return fail(-2, "more than 16 orbitals are not provided by the mean-field kernel");
   if (c->S.nact > kMaxAct) return fail(-2, "more than 8 active orbitals are not provided by the mean-field kernel");
   if (c->S.L2 > 256 || c->S.nlat > 256 || c->S.L1 > 256) return fail(-2, "lmax1/lmax2/nlat above 255 are not provided");
-  if (c->S.ni > 32) return fail(-2, "ndvr above 33 is not provided by the Poisson kernel");
+  if (c->S.ni > 31) return fail(-2, "ndvr above 32 is not provided by the Poisson kernel");
+  if (c->S.nfe - 1 > kMaxBridge) return fail(-2, "more than 161 finite elements are not provided by the Poisson kernel");
   int rc = build_device(c.get());
   if (rc) return rc;
   *out = c.release();
@@ -553,7 +607,7 @@ int td1c_set_wfn0(td1c_ctx* c, const double* wfn, const double* cic, const doubl
       b.push_back({(long long)(f * nb), 0, (long long)(f * (size_t)c->ngrid), S.mval[f] + S.mmax1, S.nradfc});
     DevBuf<TransformBatch> tb;
     HIP_TRY(tb.upload(b));
-    launch_transform(0, c->orb, c->orbg0.p, c->m_l2g.p, tb.p, nfc, nr, L1, S.nlat, nullptr);
+    launch_transform(0, c->orb, c->orbg0.p, c->m_l2g.p, tb.p, nfc, nr, L1, S.nlat, pad_cols(S.nlat), nullptr, nullptr);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipDeviceSynchronize());
     tb.release();
@@ -572,7 +626,7 @@ int td1c_set_wfn0(td1c_ctx* c, const double* wfn, const double* cic, const doubl
     HIP_TRY(tb.upload(b));
     DevBuf<double2> tmp;
     HIP_TRY(tmp.alloc(nb * nfc));
-    launch_transform(0, c->orbg0.p, tmp.p, c->m_g2l.p, tb.p, nfc, nr, S.nlat, L1, nullptr);
+    launch_transform(0, c->orbg0.p, tmp.p, c->m_g2l.p, tb.p, nfc, nr, S.nlat, L1, pad_cols(L1), nullptr, nullptr);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipDeviceSynchronize());
     HIP_TRY(hipMemcpy(orbfc.data(), tmp.p, nb * nfc * sizeof(double2), hipMemcpyDeviceToHost));
@@ -606,13 +660,15 @@ int td1c_set_wfn0(td1c_ctx* c, const double* wfn, const double* cic, const doubl
       v2jfc[i].y += v2xfc[2 * i + 1];
     }
   // tmat += v2jfc (hprod_addfc_tmat)
-  std::vector<double2> td((size_t)L1 * nr);
+  std::vector<double2> td((size_t)L1 * nr), tz((size_t)L1 * nr);
   for (int l = 0; l < L1; ++l)
     for (int i = 0; i < nr; ++i) {
       size_t q = (size_t)l * nr + i;
       td[q] = make_double2(S.tdiag[2 * q] + v2jfc[i].x, S.tdiag[2 * q + 1] + v2jfc[i].y);
+      tz[q] = make_double2(S.tzor[2 * q] + v2jfc[i].x, S.tzor[2 * q + 1] + v2jfc[i].y);
     }
   HIP_TRY(hipMemcpy(c->tdiag.p, td.data(), td.size() * sizeof(double2), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(c->tzor.p, tz.data(), tz.size() * sizeof(double2), hipMemcpyHostToDevice));
   return 0;
 }
 
@@ -821,30 +877,30 @@ int td1c_set_timing(td1c_ctx* c, int on) {
   c->timing = on != 0;
   c->recs.clear();
   c->ev_next = 0;
+  c->timing_agg.clear();
+  return 0;
+}
+
+int td1c_set_timing_kernel(td1c_ctx* c, const char* name) {
+  if (!c) return fail(-1, "null argument");
+  c->timing_only = name ? name : "";
   return 0;
 }
 
 int td1c_get_timing(td1c_ctx* c, char* buf, int buflen) {
   if (!c || !buf || buflen <= 0) return fail(-1, "null argument");
   HIP_TRY(hipDeviceSynchronize());
-  std::vector<std::pair<std::string, std::pair<double, int>>> agg;
-  for (auto& r : c->recs) {
-    float ms = 0.0f;
-    HIP_TRY(hipEventElapsedTime(&ms, c->ev_pool[r.e0], c->ev_pool[r.e1]));
-    auto it = std::find_if(agg.begin(), agg.end(), [&](auto& x) { return x.first == r.name; });
-    if (it == agg.end()) agg.push_back({r.name, {ms, 1}});
-    else { it->second.first += ms; it->second.second += 1; }
-  }
+  HIP_TRY(harvest_timing(c));
   std::string out = "{";
-  for (size_t k = 0; k < agg.size(); ++k) {
+  for (size_t k = 0; k < c->timing_agg.size(); ++k) {
+    auto& a = c->timing_agg[k];
     char tmp[160];
-    std::snprintf(tmp, sizeof(tmp), "%s\"%s\": [%.6f, %d]", k ? ", " : "", agg[k].first.c_str(), agg[k].second.first,
-                  agg[k].second.second);
+    std::snprintf(tmp, sizeof(tmp), "%s\"%s\": [%.6f, %d]", k ? ", " : "", a.first.c_str(), a.second.first,
+                  a.second.second);
     out += tmp;
   }
   out += "}";
-  c->recs.clear();
-  c->ev_next = 0;
+  c->timing_agg.clear();
   if ((int)out.size() + 1 > buflen) return fail(-1, "timing buffer too small");
   std::memcpy(buf, out.c_str(), out.size() + 1);
   return 0;

@@ -1,7 +1,9 @@
 // FP64 island of the hprod: CI reduced density matrices, regularised
 // inverse of the 1-RDM, MO integrals, energies, the X matrix and the CI
-// derivative.  All of it is O(nact^4 * lcic) scalar work on nact <= 16 and
-// small CI spaces, so one workgroup does it without a host round trip.
+// derivative.  One 256-thread workgroup does it without a host round trip:
+// the small matrices live in LDS, loops over matrix entries / determinants
+// are spread over the threads, and the n <= 8 Jacobi eigensolver runs on
+// one lane out of LDS.
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
@@ -9,6 +11,10 @@
 namespace td1c {
 
 namespace {
+
+constexpr int kT = 256;
+constexpr int kN2 = kMaxAct * kMaxAct;          // 64
+constexpr int kF2 = kMaxFun * kMaxFun;          // 256
 
 __device__ __forceinline__ double2 zmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -20,25 +26,35 @@ __device__ __forceinline__ double2 zscal(double2 a, double s) { return make_doub
 __device__ __forceinline__ double zabs(double2 a) { return sqrt(a.x * a.x + a.y * a.y); }
 __device__ __forceinline__ double2 zero2() { return make_double2(0.0, 0.0); }
 
-// y[full] += fac * E_pq x[full] on the alpha x beta product space
-__device__ void apply_E(const IslandArgs& a, int p, int q, double2 fac, const double2* x, double2* y) {
+// y[full] = E_pq x[full] (overwrites targets; E_pq is injective on strings)
+// for the determinants handled by this thread, stride kT.  y must be zeroed.
+__device__ void apply_E_par(const IslandArgs& a, int p, int q, const double2* x, double2* y, bool accumulate) {
   const int na = a.nstr_a, nb = a.nstr_b, n = a.nact;
-  for (int ib = 0; ib < nb; ++ib)
-    for (int ia = 0; ia < na; ++ia) {
-      const double2 xv = x[ib * na + ia];
-      if (xv.x == 0.0 && xv.y == 0.0) continue;
-      const double2 fx = zmul(fac, xv);
-      int t = a.exA[(ia * n + p) * n + q];
-      if (t >= 0) {
-        double s = (double)a.sgA[(ia * n + p) * n + q];
-        y[ib * na + t] = zadd(y[ib * na + t], zscal(fx, s));
-      }
-      t = a.exB[(ib * n + p) * n + q];
-      if (t >= 0) {
-        double s = (double)a.sgB[(ib * n + p) * n + q];
-        y[t * na + ia] = zadd(y[t * na + ia], zscal(fx, s));
-      }
+  const int nc = na * nb;
+  // alpha part: target (ib, ta) from (ib, ia); distinct sources -> distinct targets
+  for (int f = threadIdx.x; f < nc; f += kT) {
+    const int ib = f / na, ia = f % na;
+    const double2 xv = x[f];
+    if (xv.x == 0.0 && xv.y == 0.0) continue;
+    const int t = a.exA[(ia * n + p) * n + q];
+    if (t >= 0) {
+      const double s = (double)a.sgA[(ia * n + p) * n + q];
+      y[ib * na + t] = zadd(y[ib * na + t], zscal(xv, s));
     }
+  }
+  __syncthreads();
+  for (int f = threadIdx.x; f < nc; f += kT) {
+    const int ib = f / na, ia = f % na;
+    const double2 xv = x[f];
+    if (xv.x == 0.0 && xv.y == 0.0) continue;
+    const int t = a.exB[(ib * n + p) * n + q];
+    if (t >= 0) {
+      const double s = (double)a.sgB[(ib * n + p) * n + q];
+      y[t * na + ia] = zadd(y[t * na + ia], zscal(xv, s));
+    }
+  }
+  __syncthreads();
+  (void)accumulate;
 }
 
 // Hermitian eigen-decomposition by cyclic complex Jacobi rotations
@@ -84,10 +100,8 @@ __device__ void herm_eig(int n, double2* A, double* w, double2* U) {
   for (int i = 0; i < n; ++i) w[i] = A[i + n * i].x;
 }
 
-// regularised inverse U diag(d/(d^2+thresh)) U^+
-__device__ void invden_inv(int n, double thresh, const double2* a, double2* inva) {
-  double2 A[kMaxAct * kMaxAct * 4], U[kMaxAct * kMaxAct * 4];
-  double w[kMaxAct * 2];
+// regularised inverse U diag(d/(d^2+thresh)) U^+ (one lane; LDS scratch)
+__device__ void invden_inv(int n, double thresh, const double2* a, double2* inva, double2* A, double2* U, double* w) {
   for (int i = 0; i < n * n; ++i) A[i] = a[i];
   herm_eig(n, A, w, U);
   for (int i = 0; i < n * n; ++i) inva[i] = zero2();
@@ -102,49 +116,53 @@ __device__ void invden_inv(int n, double thresh, const double2* a, double2* inva
 
 }  // namespace
 
-// Stage A: den1, den2 from the CI vector (ormas_mkden1/2), rden, rrden
+// Stage A: den1, den2 from the CI vector (ormas_mkden1/2, den1(a,b) = <E_ba>,
+// den2(a,b,c,d) = <E_ba E_dc> - delta_ad den1(c,b)), rden, rrden
 // (hprod_invden, src/hprod/hprod_invden.F90:2-60).
-__global__ void k_island_a(const IslandArgs a) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__global__ void __launch_bounds__(kT) k_island_a(const IslandArgs a) {
+  __shared__ double2 s_d1[kN2], s_A[kN2], s_U[kN2], s_T[kN2], s_out[kN2];
+  __shared__ double s_w[kMaxAct];
   const int n = a.nact;
   if (n == 0) return;
   const int nc = a.nstr_a * a.nstr_b;
   double2* D = a.ci_work;
   double2* C = D + (size_t)n * n * nc;
-  for (int f = 0; f < nc; ++f) C[f] = a.det_of[f] >= 0 ? a.cic[a.det_of[f]] : zero2();
-  for (size_t q = 0; q < (size_t)n * n * nc; ++q) D[q] = zero2();
-  const double2 one = make_double2(1.0, 0.0);
+  for (int f = threadIdx.x; f < nc; f += kT) C[f] = a.det_of[f] >= 0 ? a.cic[a.det_of[f]] : zero2();
+  for (size_t q = threadIdx.x; q < (size_t)n * n * nc; q += kT) D[q] = zero2();
+  __syncthreads();
   for (int p = 0; p < n; ++p)
-    for (int q = 0; q < n; ++q) apply_E(a, p, q, one, C, D + (size_t)(p * n + q) * nc);
-  // den1(a,b) = <E_ba>
-  for (int x = 0; x < n; ++x)
-    for (int y = 0; y < n; ++y) {
-      double2 s = zero2();
-      const double2* e = D + (size_t)(y * n + x) * nc;
-      for (int d = 0; d < nc; ++d) s = zadd(s, zmul(zconj(C[d]), e[d]));
-      a.den1[x + n * y] = s;
-    }
-  // den2(a,b,c,d) = (E_ab C)^+ (E_dc C) - delta_ad den1(c,b)
-  for (int p = 0; p < n; ++p)
-    for (int q = 0; q < n; ++q)
-      for (int r = 0; r < n; ++r)
-        for (int t = 0; t < n; ++t) {
-          const double2* u = D + (size_t)(p * n + q) * nc;
-          const double2* v = D + (size_t)(t * n + r) * nc;
-          double2 s = zero2();
-          for (int k = 0; k < nc; ++k) s = zadd(s, zmul(zconj(u[k]), v[k]));
-          if (p == t) s = zsub(s, a.den1[r + n * q]);
-          a.den2[p + n * (q + n * (r + n * t))] = s;
-        }
-  invden_inv(n, a.throcc1, a.den1, a.rden);
-  if (a.ncore > 0) {
-    if (n * 2 == a.nelact2) {
-      for (int i = 0; i < n * n; ++i) a.rrden[i] = zero2();
-    } else {
-      double2 tmp[kMaxAct * kMaxAct * 4];
-      for (int i = 0; i < n * n; ++i) tmp[i] = zscal(a.den1[i], -1.0);
-      for (int i = 0; i < n; ++i) tmp[i + n * i].x += 2.0;
-      invden_inv(n, a.throcc2, tmp, a.rrden);
+    for (int q = 0; q < n; ++q) apply_E_par(a, p, q, C, D + (size_t)(p * n + q) * nc, true);
+  // den1(x,y) = sum conj(C) (E_yx C)
+  for (int e = threadIdx.x; e < n * n; e += kT) {
+    const int x = e % n, y = e / n;
+    const double2* v = D + (size_t)(y * n + x) * nc;
+    double2 s = zero2();
+    for (int d = 0; d < nc; ++d) s = zadd(s, zmul(zconj(C[d]), v[d]));
+    s_d1[e] = s;
+    a.den1[e] = s;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < n * n * n * n; e += kT) {
+    const int p = e % n, q = (e / n) % n, r = (e / (n * n)) % n, t = e / (n * n * n);
+    const double2* u = D + (size_t)(p * n + q) * nc;
+    const double2* v = D + (size_t)(t * n + r) * nc;
+    double2 s = zero2();
+    for (int k = 0; k < nc; ++k) s = zadd(s, zmul(zconj(u[k]), v[k]));
+    if (p == t) s = zsub(s, s_d1[r + n * q]);
+    a.den2[e] = s;
+  }
+  if (threadIdx.x == 0) {
+    invden_inv(n, a.throcc1, s_d1, s_out, s_A, s_U, s_w);
+    for (int i = 0; i < n * n; ++i) a.rden[i] = s_out[i];
+    if (a.ncore > 0) {
+      if (n * 2 == a.nelact2) {
+        for (int i = 0; i < n * n; ++i) a.rrden[i] = zero2();
+      } else {
+        for (int i = 0; i < n * n; ++i) s_T[i] = zscal(s_d1[i], -1.0);
+        for (int i = 0; i < n; ++i) s_T[i + n * i].x += 2.0;
+        invden_inv(n, a.throcc2, s_T, s_out, s_A, s_U, s_w);
+        for (int i = 0; i < n * n; ++i) a.rrden[i] = s_out[i];
+      }
     }
   }
 }
@@ -153,72 +171,99 @@ __global__ void k_island_a(const IslandArgs a) {
 // (hprod_ene_dcore, hprod_ene_act), X matrix (hprod_htot_mkxmat_fc/cc/ca/aa1,
 // src/hprod/hprod_htot_mkxmat.F90) and the CI derivative (ormas_xact2 with one
 // ORMAS subspace, src/ormas/ormas_xact2.F90:2-102).
-__global__ void k_island_b(const IslandArgs a) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__global__ void __launch_bounds__(kT) k_island_b(const IslandArgs a) {
+  __shared__ double2 s_H[kF2], s_G[kF2], s_V[kF2], s_F[kF2], s_X[kF2], s_FM[kF2];
+  __shared__ double2 s_i1[kN2], s_d1[kN2], s_rr[kN2];
+  __shared__ double s_red[kT];
+  __shared__ double s_eact;
   const int nf = a.nfun, nfc = a.nfcore, ncore = a.ncore, na = a.nact;
-  auto H = [&](int j, int i) { int q = a.mapH[j * nf + i]; return q >= 0 ? a.lprod[q] : zero2(); };
-  auto G = [&](int j, int i) { int q = a.mapG[j * nf + i]; return q >= 0 ? a.lprod[q] : zero2(); };
-  auto V = [&](int j, int i) { int q = a.mapV[j * nf + i]; return q >= 0 ? a.lprod[q] : zero2(); };
-  auto F = [&](int j, int i) { int q = a.mapF[j * nf + i]; return q >= 0 ? a.lprod[q] : zero2(); };
-  auto HGV = [&](int j, int i) { return zadd(V(j, i), zadd(G(j, i), H(j, i))); };
+  const int tid = threadIdx.x;
+  // gather the Gram results
+  for (int e = tid; e < nf * nf; e += kT) {
+    int q;
+    q = a.mapH[e]; s_H[e] = q >= 0 ? a.lprod[q] : zero2();
+    q = a.mapG[e]; s_G[e] = q >= 0 ? a.lprod[q] : zero2();
+    q = a.mapV[e]; s_V[e] = q >= 0 ? a.lprod[q] : zero2();
+    q = a.mapF[e]; s_F[e] = q >= 0 ? a.lprod[q] : zero2();
+    s_X[e] = zero2();
+  }
+  for (int e = tid; e < na * na; e += kT) {
+    s_d1[e] = a.den1[e];
+    s_rr[e] = a.rrden[e];
+  }
+  __syncthreads();
+#define H(j, i) s_H[(j) * nf + (i)]
+#define G(j, i) s_G[(j) * nf + (i)]
+#define V(j, i) s_V[(j) * nf + (i)]
+#define F(j, i) s_F[(j) * nf + (i)]
+#define HGV(j, i) zadd(V(j, i), zadd(G(j, i), H(j, i)))
   // int1e(j,i) = <j|g + h1 + h0|i> for j <= i, Hermitian fill
-  for (int ia = 0; ia < na; ++ia)
-    for (int ja = 0; ja <= ia; ++ja) {
-      int i = ncore + ia, j = ncore + ja;
-      a.int1e[ja + na * ia] = a.mval[i] == a.mval[j] ? zadd(G(j, i), H(j, i)) : zero2();
+  for (int e = tid; e < na * na; e += kT) {
+    const int ja = e % na, ia = e / na;
+    const int i = ncore + ia, j = ncore + ja;
+    double2 v = zero2();
+    if (ja <= ia) {
+      if (a.mval[i] == a.mval[j]) v = zadd(G(j, i), H(j, i));
+    } else if (a.mval[i] == a.mval[j]) {
+      v = zconj(zadd(G(i, j), H(i, j)));
     }
-  for (int ia = 0; ia < na; ++ia)
-    for (int ja = ia + 1; ja < na; ++ja) a.int1e[ja + na * ia] = zconj(a.int1e[ia + na * ja]);
-  // int2e(i,j,k,l), i >= j from the products, then the conjugate fill
-  for (int ia = 0; ia < na; ++ia)
-    for (int ja = 0; ja <= ia; ++ja)
-      for (int ka = 0; ka < na; ++ka)
-        for (int la = 0; la < na; ++la) {
-          int idx = ia + na * (ja + na * (ka + na * la));
-          int q = a.map2e[idx];
-          a.int2e[idx] = q >= 0 ? a.pprod[q] : zero2();
-        }
-  for (int ia = 0; ia < na; ++ia)
-    for (int ja = ia + 1; ja < na; ++ja)
-      for (int ka = 0; ka < na; ++ka)
-        for (int la = 0; la < na; ++la)
-          a.int2e[ia + na * (ja + na * (ka + na * la))] = zconj(a.int2e[ja + na * (ia + na * (la + na * ka))]);
-  // energies
-  double ene_dcore = 0.0, ene_act = 0.0;
-  for (int i = nfc; i < ncore; ++i) {
-    double2 t = zadd(G(i, i), zscal(H(i, i), 2.0));
-    ene_dcore += t.x;
+    s_i1[e] = v;
+    a.int1e[e] = v;
   }
-  if (na > 0) {
-    double2 t1 = zero2(), t2 = zero2();
-    for (int ia = 0; ia < na; ++ia)
-      for (int ja = 0; ja < na; ++ja) {
-        t1 = zadd(t1, zmul(a.int1e[ia + na * ja], a.den1[ja + na * ia]));
-        for (int ka = 0; ka < na; ++ka)
-          for (int la = 0; la < na; ++la)
-            t2 = zadd(t2, zmul(a.int2e[ia + na * (ja + na * (ka + na * la))],
-                               a.den2[ja + na * (ia + na * (la + na * ka))]));
-      }
-    ene_act = t1.x + t2.x * 0.5;
+  // int2e(i,j,k,l): i >= j from the products, the rest by conjugate symmetry
+  for (int e = tid; e < na * na * na * na; e += kT) {
+    const int ia = e % na, ja = (e / na) % na, ka = (e / (na * na)) % na, la = e / (na * na * na);
+    int src = e;
+    bool cj = false;
+    if (ja > ia) {
+      src = ja + na * (ia + na * (la + na * ka));
+      cj = true;
+    }
+    const int q = a.map2e[src];
+    double2 v = q >= 0 ? a.pprod[q] : zero2();
+    if (cj) v = zconj(v);
+    a.int2e[e] = v;
   }
-  a.ene[1] = ene_dcore;
-  a.ene[2] = a.ene[0] + ene_dcore;
-  a.ene[3] = ene_act;
-  a.ene[4] = a.ene[2] + ene_act;
-
+  __syncthreads();
+  // energies: ene_act = Re sum int1e(i,j) den1(j,i) + 1/2 Re sum int2e(i,j,k,l) den2(j,i,l,k)
+  double part = 0.0;
+  for (int e = tid; e < na * na * na * na; e += kT) {
+    const int ia = e % na, ja = (e / na) % na, ka = (e / (na * na)) % na, la = e / (na * na * na);
+    part += 0.5 * zmul(a.int2e[e], a.den2[ja + na * (ia + na * (la + na * ka))]).x;
+  }
+  for (int e = tid; e < na * na; e += kT) {
+    const int ia = e % na, ja = e / na;
+    part += zmul(s_i1[ia + na * ja], s_d1[ja + na * ia]).x;
+  }
+  s_red[tid] = part;
+  __syncthreads();
+  for (int w = kT / 2; w > 0; w >>= 1) {
+    if (tid < w) s_red[tid] += s_red[tid + w];
+    __syncthreads();
+  }
+  if (tid == 0) {
+    double ene_dcore = 0.0;
+    for (int i = nfc; i < ncore; ++i) ene_dcore += zadd(G(i, i), zscal(H(i, i), 2.0)).x;
+    const double ene_act = na > 0 ? s_red[0] : 0.0;
+    s_eact = ene_act;
+    a.ene[1] = ene_dcore;
+    a.ene[2] = a.ene[0] + ene_dcore;
+    a.ene[3] = ene_act;
+    a.ene[4] = a.ene[2] + ene_act;
+  }
   const double2 tfac = a.icomp == 0 ? make_double2(a.dtime, 0.0) : make_double2(0.0, a.dtime);
-  for (int q = 0; q < nf * nf; ++q) a.xmat[q] = zero2();
-#define XM(j, i) a.xmat[(i) * nf + (j)]
-  // fc-dyn block
-  for (int i = nfc; i < nf; ++i)
-    for (int j = 0; j < nfc; ++j)
-      if (a.mval[j] == a.mval[i]) XM(j, i) = zmul(zadd(F(j, i), zadd(G(j, i), V(j, i))), tfac);
-  // core-core block
-  for (int i = nfc; i < ncore; ++i)
-    for (int j = nfc; j < ncore; ++j)
-      if (a.mval[j] == a.mval[i]) XM(j, i) = zmul(HGV(j, i), tfac);
-  // core-active blocks
-  if (na > 0 && ncore > nfc) {
+#define XM(j, i) s_X[(i) * nf + (j)]
+  // fc-dyn, core-core and active-active blocks (one entry per thread)
+  for (int e = tid; e < nf * nf; e += kT) {
+    const int j = e % nf, i = e / nf;
+    if (a.mval[j] != a.mval[i]) continue;
+    if (j < nfc && i >= nfc) XM(j, i) = zmul(zadd(F(j, i), zadd(G(j, i), V(j, i))), tfac);
+    else if (j >= nfc && j < ncore && i >= nfc && i < ncore) XM(j, i) = zmul(HGV(j, i), tfac);
+    else if (j >= ncore && i >= ncore) XM(j, i) = zmul(HGV(j, i), tfac);
+  }
+  __syncthreads();
+  // core-active blocks (mkxmat_ca / ca0), small: one lane
+  if (tid == 0 && na > 0 && ncore > nfc) {
     if (na * 2 == a.nelact2) {
       for (int i = nfc; i < ncore; ++i)
         for (int j = ncore; j < nf; ++j)
@@ -227,8 +272,7 @@ __global__ void k_island_b(const IslandArgs a) {
             XM(i, j) = zmul(HGV(i, j), tfac);
           }
     } else {
-      // F = <A|h|C>, <C|h|A>; B = -tfac rrden (2 F_AC - D F_CA^*) (mkxmat_ca :549-732)
-      double2 FM[kMaxFun * kMaxFun];
+      double2* FM = s_FM;
       for (int q = 0; q < nf * nf; ++q) FM[q] = zero2();
       for (int i = nfc; i < ncore; ++i)
         for (int j = ncore; j < nf; ++j)
@@ -246,27 +290,28 @@ __global__ void k_island_b(const IslandArgs a) {
           int j = ncore + ja;
           double2 fji = zscal(FM[i * nf + j], 2.0);
           double2 t = zero2();
-          for (int ka = 0; ka < na; ++ka) t = zadd(t, zmul(a.den1[ja + na * ka], zconj(FM[(ncore + ka) * nf + i])));
+          for (int ka = 0; ka < na; ++ka) t = zadd(t, zmul(s_d1[ja + na * ka], zconj(FM[(ncore + ka) * nf + i])));
           FM[i * nf + j] = zsub(fji, t);
         }
       for (int i = nfc; i < ncore; ++i)
         for (int ja = 0; ja < na; ++ja) {
           int j = ncore + ja;
           double2 t = zero2();
-          for (int ka = 0; ka < na; ++ka) t = zadd(t, zmul(a.rrden[ja + na * ka], FM[i * nf + ncore + ka]));
+          for (int ka = 0; ka < na; ++ka) t = zadd(t, zmul(s_rr[ja + na * ka], FM[i * nf + ncore + ka]));
           double2 bji = zscal(zmul(tfac, t), -1.0);
           XM(j, i) = zadd(XM(j, i), bji);
           XM(i, j) = zadd(XM(i, j), zscal(zconj(bji), -1.0));
         }
     }
   }
-  // active-active block within the single ORMAS subspace (mkxmat_aa1)
-  for (int ia = 0; ia < na; ++ia)
-    for (int ja = 0; ja < na; ++ja) {
-      int i = ncore + ia, j = ncore + ja;
-      if (a.mval[i] == a.mval[j]) XM(j, i) = zmul(HGV(j, i), tfac);
-    }
+  __syncthreads();
+  for (int e = tid; e < nf * nf; e += kT) a.xmat[e] = s_X[e];
 #undef XM
+#undef H
+#undef G
+#undef V
+#undef F
+#undef HGV
   // CI derivative: dC = fac (H_CI - E_act) C
   if (na > 0) {
     const int nc = a.nstr_a * a.nstr_b;
@@ -274,46 +319,51 @@ __global__ void k_island_b(const IslandArgs a) {
     double2* C = D + (size_t)na * na * nc;
     double2* S = C + nc;
     double2* T = S + nc;
-    for (int f = 0; f < nc; ++f) {
+    for (int f = tid; f < nc; f += kT) {
       C[f] = a.det_of[f] >= 0 ? a.cic[a.det_of[f]] : zero2();
       S[f] = zero2();
     }
-    for (size_t q = 0; q < (size_t)na * na * nc; ++q) D[q] = zero2();
-    const double2 one = make_double2(1.0, 0.0);
+    for (size_t q = tid; q < (size_t)na * na * nc; q += kT) D[q] = zero2();
+    __syncthreads();
     for (int r = 0; r < na; ++r)
-      for (int s = 0; s < na; ++s) apply_E(a, r, s, one, C, D + (size_t)(r * na + s) * nc);
+      for (int s = 0; s < na; ++s) apply_E_par(a, r, s, C, D + (size_t)(r * na + s) * nc, true);
+    // one-body part with h'_pq = int1e(p,q) - 1/2 sum_r int2e(p,r,r,q)
+    for (int f = tid; f < nc; f += kT) {
+      double2 acc = zero2();
+      for (int p = 0; p < na; ++p)
+        for (int q = 0; q < na; ++q) {
+          double2 hpq = s_i1[p + na * q];
+          for (int r = 0; r < na; ++r) hpq = zsub(hpq, zscal(a.int2e[p + na * (r + na * (r + na * q))], 0.5));
+          acc = zadd(acc, zmul(hpq, D[(size_t)(p * na + q) * nc + f]));
+        }
+      S[f] = acc;
+    }
+    __syncthreads();
     for (int p = 0; p < na; ++p)
       for (int q = 0; q < na; ++q) {
-        double2 hpq = a.int1e[p + na * q];
-        for (int r = 0; r < na; ++r) hpq = zsub(hpq, zscal(a.int2e[p + na * (r + na * (r + na * q))], 0.5));
-        const double2* e = D + (size_t)(p * na + q) * nc;
-        for (int d = 0; d < nc; ++d) S[d] = zadd(S[d], zmul(hpq, e[d]));
-      }
-    for (int p = 0; p < na; ++p)
-      for (int q = 0; q < na; ++q) {
-        for (int d = 0; d < nc; ++d) T[d] = zero2();
-        for (int r = 0; r < na; ++r)
-          for (int s = 0; s < na; ++s) {
-            double2 g = zscal(a.int2e[p + na * (q + na * (r + na * s))], 0.5);
-            if (g.x == 0.0 && g.y == 0.0) continue;
-            const double2* e = D + (size_t)(r * na + s) * nc;
-            for (int d = 0; d < nc; ++d) T[d] = zadd(T[d], zmul(g, e[d]));
-          }
-        apply_E(a, p, q, one, T, S);
+        for (int f = tid; f < nc; f += kT) {
+          double2 t = zero2();
+          for (int r = 0; r < na; ++r)
+            for (int s = 0; s < na; ++s)
+              t = zadd(t, zmul(zscal(a.int2e[p + na * (q + na * (r + na * s))], 0.5), D[(size_t)(r * na + s) * nc + f]));
+          T[f] = t;
+        }
+        __syncthreads();
+        apply_E_par(a, p, q, T, S, true);
       }
     const double2 fac = a.icomp == 1 ? make_double2(0.0, -a.dtime) : make_double2(-a.dtime, 0.0);
-    for (int f = 0; f < nc; ++f) {
+    const double eact = s_eact;
+    for (int f = tid; f < nc; f += kT) {
       int d = a.det_of[f];
       if (d < 0) continue;
-      double2 v = zsub(S[f], zscal(a.cic[d], ene_act));
-      a.dcic[d] = zmul(v, fac);
+      a.dcic[d] = zmul(zsub(S[f], zscal(a.cic[d], eact)), fac);
     }
   } else {
-    for (int d = 0; d < a.lcic; ++d) a.dcic[d] = zero2();
+    for (int d = tid; d < a.lcic; d += kT) a.dcic[d] = zero2();
   }
 }
 
-void launch_island_a(const IslandArgs& a, hipStream_t s) { hipLaunchKernelGGL(k_island_a, dim3(1), dim3(64), 0, s, a); }
-void launch_island_b(const IslandArgs& a, hipStream_t s) { hipLaunchKernelGGL(k_island_b, dim3(1), dim3(64), 0, s, a); }
+void launch_island_a(const IslandArgs& a, hipStream_t s) { hipLaunchKernelGGL(k_island_a, dim3(1), dim3(kT), 0, s, a); }
+void launch_island_b(const IslandArgs& a, hipStream_t s) { hipLaunchKernelGGL(k_island_b, dim3(1), dim3(kT), 0, s, a); }
 
 }  // namespace td1c

@@ -8,6 +8,8 @@
 // the same constants the kernels use for their bounds checks.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "kernels.hpp"
 
 namespace td1c {
@@ -32,57 +34,51 @@ __device__ __forceinline__ void cfma(double2& acc, double2 a, double2 b) {
 // ---------------------------------------------------------------------------
 // Legendre transforms along the middle axis (bas_sph2ang1/ang2sph1/
 // ang2sph2/sph2ang2, src/bas/*.F90): out[b][c][r] = sum_k in_b[k][r] M_b[k][c]
-// with a real K x C matrix per batch entry (selected by the m value).
-// MODE 0: input read directly; MODE 1: input formed on the fly as the pair
-// density conj(psi_j) psi_i on the theta grid (hprod_mkrho2p fused in).
-// Block: 64 radial lanes x TCOLS column groups; LDS holds a KT x C slab of M.
+// with a real K x C matrix per batch entry (selected by the m value), stored
+// with its columns padded to Cp (a multiple of 32, zeros beyond C).
+// MODE 0: input read directly (optionally copied to `copy`); MODE 1: input
+// formed on the fly as the pair density conj(psi_j) psi_i on the theta grid
+// (hprod_mkrho2p fused in).
+// Block: 4 wavefronts x 64 radial lanes; wave w owns CPW output columns.
+// The matrix row segment is wave-uniform and is read with scalar loads.
 // ---------------------------------------------------------------------------
-template <int MODE, int CPT>
-__global__ void __launch_bounds__(256) k_transform(const double2* __restrict__ in,
-                                                   double2* __restrict__ out,
+template <int MODE, int CPW>
+__global__ void __launch_bounds__(256) k_transform(const double2* __restrict__ in, double2* __restrict__ out,
                                                    const double* __restrict__ mats,
-                                                   const TransformBatch* __restrict__ batch,
-                                                   int nr, int K, int C) {
-  constexpr int KT = 16;
-  __shared__ double sM[KT * 256];
-  const TransformBatch bt = batch[blockIdx.y];
-  const int r = blockIdx.x * 64 + threadIdx.x;
-  const int cg = threadIdx.y;            // column group 0..3
-  const int c0 = cg * CPT;
-  const double* M = mats + (size_t)bt.mat * K * C;
-  const bool active = r < bt.rows;
-  double2 acc[CPT];
+                                                   const TransformBatch* __restrict__ batch, int nr, int K, int C,
+                                                   int Cp, double2* __restrict__ copy) {
+  const TransformBatch bt = batch[blockIdx.z];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int c0 = __builtin_amdgcn_readfirstlane((blockIdx.y * 4 + wave) * CPW);
+  const int r = blockIdx.x * 64 + lane;
+  if (c0 >= C) return;
+  const double* __restrict__ M = mats + (size_t)bt.mat * K * Cp + c0;
+  double2 acc[CPW];
 #pragma unroll
-  for (int c = 0; c < CPT; ++c) acc[c] = make_double2(0.0, 0.0);
-  const int tid = threadIdx.y * 64 + threadIdx.x;
-  for (int k0 = 0; k0 < K; k0 += KT) {
-    const int kn = min(KT, K - k0);
-    __syncthreads();
-    for (int q = tid; q < kn * C; q += 256) sM[(q / C) * 256 + (q % C)] = M[(size_t)(k0 + q / C) * C + (q % C)];
-    __syncthreads();
-    if (active) {
-      for (int kk = 0; kk < kn; ++kk) {
-        double2 x;
-        if (MODE == 0) {
-          x = in[(size_t)bt.in0 + (size_t)(k0 + kk) * nr + r];
-        } else {
-          double2 a = in[(size_t)bt.in0 + (size_t)(k0 + kk) * nr + r];
-          double2 b = in[(size_t)bt.in1 + (size_t)(k0 + kk) * nr + r];
-          x = cmulc(a, b);
-        }
-        const double* mrow = sM + kk * 256 + c0;
+  for (int c = 0; c < CPW; ++c) acc[c] = make_double2(0.0, 0.0);
+  if (r < bt.rows) {
+    const double2* __restrict__ x0 = in + (size_t)bt.in0 + r;
+    const double2* __restrict__ x1 = in + (size_t)bt.in1 + r;
+#pragma unroll 8
+    for (int k = 0; k < K; ++k) {
+      double2 x;
+      if (MODE == 0) {
+        x = x0[(size_t)k * nr];
+        if (copy != nullptr && c0 == 0) copy[(size_t)bt.in0 + (size_t)k * nr + r] = x;
+      } else {
+        x = cmulc(x0[(size_t)k * nr], x1[(size_t)k * nr]);
+      }
+      const double* __restrict__ mrow = M + (size_t)k * Cp;
 #pragma unroll
-        for (int c = 0; c < CPT; ++c) {
-          double m = mrow[c];
-          acc[c].x = fma(x.x, m, acc[c].x);
-          acc[c].y = fma(x.y, m, acc[c].y);
-        }
+      for (int c = 0; c < CPW; ++c) {
+        const double m = mrow[c];
+        acc[c].x = fma(x.x, m, acc[c].x);
+        acc[c].y = fma(x.y, m, acc[c].y);
       }
     }
-  }
-  if (active) {
 #pragma unroll
-    for (int c = 0; c < CPT; ++c)
+    for (int c = 0; c < CPW; ++c)
       if (c0 + c < C) out[(size_t)bt.out + (size_t)(c0 + c) * nr + r] = acc[c];
   }
 }
@@ -188,110 +184,292 @@ __global__ void __launch_bounds__(256) k_onebody(const double2* __restrict__ orb
 }
 
 // ---------------------------------------------------------------------------
-// Radial Poisson solve per (pair, L) (hprod_mkv2mf_poissonp,
-// src/hprod/hprod_mkv2mf.F90:118-182) by element Schur complement of the
-// same FEDVR operator A_L = 2K + L(L+1)/r^2.
-// P1: per (L, element, pair): y = G_e b_I, bridge contributions, multipole part.
+// One-body part in element-local form (same operator as k_onebody): a block
+// owns GE consecutive elements of one orbital and walks l = 0..lmax1.  Lane
+// (element, m) keeps row m of the element's kinetic and d/dr matrices in
+// registers; psi(l-1), psi(l), psi(l+1) of the block's points slide through
+// LDS.  An interior point needs one row product; a bridge adds the last row
+// of the element on its left (lane 252 recomputes it for the block's first
+// element).  N = ndvr.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(64) k_poisson_elem(const double2* __restrict__ rho, double2* __restrict__ x,
-                                                     const PoissonArgs a) {
-  const int e = blockIdx.x;
-  const int L = blockIdx.y;
-  const int ni = a.ni;
-  __shared__ double sG[32 * 32];
-  __shared__ double sCl[32], sCr[32];
-  const double* G = a.G + (((size_t)L * a.nfe) + e) * ni * ni;
-  for (int q = threadIdx.x; q < ni * ni; q += blockDim.x) sG[q] = G[q];
-  for (int q = threadIdx.x; q < ni; q += blockDim.x) {
-    sCl[q] = a.Cl[(((size_t)L * a.nfe) + e) * ni + q];
-    sCr[q] = a.Cr[(((size_t)L * a.nfe) + e) * ni + q];
+template <int N>
+__global__ void __launch_bounds__(256) k_onebody_el(const double2* __restrict__ orb, double2* __restrict__ h01,
+                                                    const OneBodyElArgs a) {
+  constexpr int NP = N + 1;
+  constexpr int GE = 256 / NP;               // elements per block
+  constexpr int TILE = (GE + 1) * N + 1;     // points of elements e0-1 .. e0+GE-1
+  __shared__ double2 sPsi[3][TILE];
+  __shared__ double2 sT[256], sP1[256], sP0[256];
+  const int f = blockIdx.y;
+  const int e0 = blockIdx.x * GE;
+  const int tid = threadIdx.x;
+  // lane -> (element, local row)
+  int e, m;
+  if (tid < GE * NP) {
+    e = e0 + tid / NP;
+    m = tid % NP;
+  } else if (tid == GE * NP) {
+    e = e0 - 1;
+    m = N;
+  } else {
+    e = -1;
+    m = 0;
   }
-  __syncthreads();
-  const int base = e * a.n;  // interior global index of local point 1
-  const double* invr = a.invr + (size_t)L * a.nr;
-  const double* rpl0 = a.rpl0 + (size_t)L * a.nr;
-  for (int p = threadIdx.x; p < a.npair; p += blockDim.x) {
-    const int pm = a.pair_m[p];
-    if (L < (pm < 0 ? -pm : pm)) continue;
-    const double2* rp = rho + ((size_t)p * a.L2 + L) * a.nr;
-    double2* xp = x + ((size_t)p * a.L2 + L) * a.nr;
-    double2 b[32];
-    double2 mp = make_double2(0.0, 0.0);
-    for (int k = 0; k < ni; ++k) {
-      double2 v = rp[base + k];
-      mp.x = fma(v.x, rpl0[base + k], mp.x);
-      mp.y = fma(v.y, rpl0[base + k], mp.y);
-      b[k] = cscale(v, invr[base + k]);
+  const bool lane_ok = e >= 0 && e < a.nfe;
+  const int mf = a.mval[f];
+  const int am = mf < 0 ? -mf : mf;
+  const bool fc = f < a.nfcore;
+  const int rows = fc ? a.nradfc : a.nr;
+  const int ibase = (e0 - 1) * N - 1;        // global interior index of tile slot 0
+  // tile slot of this lane's element point 0
+  const int eslot = (e - (e0 - 1)) * N;
+  double Kr[NP], Pr[NP];
+#pragma unroll
+  for (int k = 0; k < NP; ++k) {
+    Kr[k] = lane_ok ? a.eK[((size_t)e * NP + k) * NP + m] : 0.0;
+    Pr[k] = lane_ok ? a.eP[((size_t)e * NP + k) * NP + m] : 0.0;
+  }
+  const int i = e * N + m - 1;               // global interior index of (e, m)
+  // owner: interior rows, and row 0 of elements >= 1 (the bridge)
+  const bool owner = lane_ok && tid < GE * NP && m < N && i >= 0 && i < a.nr;
+  const double xr = owner ? a.xr[i] : 1.0;
+  const double* al = a.alph + (size_t)(mf + a.mmax1) * a.L1;
+  const double2* base_f = orb + (size_t)f * a.L1 * a.nr;
+  auto load_tile = [&](int l, int buf) {
+    for (int q = tid; q < TILE; q += 256) {
+      const int gi = ibase + q;
+      double2 v = make_double2(0.0, 0.0);
+      if (l >= 0 && l < a.L1 && gi >= 0 && gi < a.nr) v = base_f[(size_t)l * a.nr + gi];
+      sPsi[buf][q] = v;
     }
-    if (e >= 1) {  // element e owns its left bridge in the multipole sum
-      double2 v = rp[base - 1];
-      mp.x = fma(v.x, rpl0[base - 1], mp.x);
-      mp.y = fma(v.y, rpl0[base - 1], mp.y);
-    }
-    double2 cl = make_double2(0.0, 0.0), cr = make_double2(0.0, 0.0);
-    for (int i = 0; i < ni; ++i) {
-      double2 y = make_double2(0.0, 0.0);
-      const double* g = sG + i * ni;
-      for (int k = 0; k < ni; ++k) {
-        y.x = fma(g[k], b[k].x, y.x);
-        y.y = fma(g[k], b[k].y, y.y);
+  };
+  load_tile(0, 0);
+  load_tile(-1, 2);
+  for (int l = 0; l < a.L1; ++l) {
+    const int b0 = l % 3, b1 = (l + 1) % 3, bm = (l + 2) % 3;
+    __syncthreads();
+    load_tile(l + 1, b1);
+    __syncthreads();
+    // element-row partial products
+    double2 pT = make_double2(0.0, 0.0), pP1 = make_double2(0.0, 0.0), pP0 = make_double2(0.0, 0.0);
+    if (lane_ok && l >= am) {
+      const double2* s0 = sPsi[b0] + eslot;
+      if (l == 0) {
+#pragma unroll
+        for (int k = 0; k < NP; ++k) {
+          const double kk = a.eK0[((size_t)e * NP + k) * NP + m];
+          pT.x = fma(kk, s0[k].x, pT.x);
+          pT.y = fma(kk, s0[k].y, pT.y);
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < NP; ++k) {
+          pT.x = fma(Kr[k], s0[k].x, pT.x);
+          pT.y = fma(Kr[k], s0[k].y, pT.y);
+        }
       }
-      xp[base + i] = y;
-      cl.x = fma(sCl[i], y.x, cl.x);
-      cl.y = fma(sCl[i], y.y, cl.y);
-      cr.x = fma(sCr[i], y.x, cr.x);
-      cr.y = fma(sCr[i], y.y, cr.y);
+      if (a.igauge == 1) {
+        const double2* s1 = sPsi[b1] + eslot;
+        const double2* sm = sPsi[bm] + eslot;
+#pragma unroll
+        for (int k = 0; k < NP; ++k) {
+          pP1.x = fma(Pr[k], s1[k].x, pP1.x);
+          pP1.y = fma(Pr[k], s1[k].y, pP1.y);
+          pP0.x = fma(Pr[k], sm[k].x, pP0.x);
+          pP0.y = fma(Pr[k], sm[k].y, pP0.y);
+        }
+      }
     }
-    size_t q = (((size_t)p * a.L2 + L) * a.nfe) + e;
-    a.cl[q] = cl;
-    a.cr[q] = cr;
-    a.mpart[q] = mp;
+    sT[tid] = pT;
+    sP1[tid] = pP1;
+    sP0[tid] = pP0;
+    __syncthreads();
+    if (owner) {
+      double2 out = make_double2(0.0, 0.0);
+      if (l >= am && i < rows) {
+        double2 T = pT, P1 = pP1, P0 = pP0;
+        if (m == 0) {  // bridge: add the left element's last row
+          const int pl = tid >= NP ? tid - 1 : GE * NP;
+          T = cadd(T, sT[pl]);
+          P1 = cadd(P1, sP1[pl]);
+          P0 = cadd(P0, sP0[pl]);
+        }
+        const int slot = eslot + m;
+        const double2 p0 = sPsi[b0][slot], pu = sPsi[b1][slot], pd = sPsi[bm][slot];
+        double2 h = make_double2(0.0, 0.0);
+        const bool full = !fc || a.fc_full;
+        if (full) {
+          h = T;
+          cfma(h, p0, a.tzor[(size_t)l * a.nr + i]);
+          if (a.igauge == 0) {
+            if (l + 1 < a.L1) cfma(h, pu, cscale(a.zfield, xr * al[l]));
+            if (l - 1 >= am) cfma(h, pd, cscale(a.zfield, xr * al[l - 1]));
+          } else {
+            if (l + 1 < a.L1) {
+              cfma(h, P1, cmul(a.zfield, make_double2(0.0, -al[l])));
+              cfma(h, pu, cmul(a.zfield, make_double2(0.0, -(al[l] * (l + 1.0)) / xr)));
+            }
+            if (l - 1 >= am) {
+              cfma(h, P0, cmul(a.zfield, make_double2(0.0, -al[l - 1])));
+              cfma(h, pd, cscale(cmul(a.zfield, make_double2(0.0, -(al[l - 1] * (double)l) / xr)), -1.0));
+            }
+          }
+        }
+        if (fc && a.igauge == 1) {  // E.z on the frozen core in the velocity gauge
+          if (l + 1 < a.L1) cfma(h, pu, cscale(a.efield, xr * al[l]));
+          if (l - 1 >= am) cfma(h, pd, cscale(a.efield, xr * al[l - 1]));
+        }
+        out = h;
+      }
+      h01[((size_t)f * a.L1 + l) * a.nr + i] = out;
+    }
   }
 }
 
-// P2: per (pair, L): tridiagonal Schur solve on the bridges + multipole sum.
-__global__ void __launch_bounds__(64) k_poisson_bridge(const double2* __restrict__ rho, double2* __restrict__ x,
-                                                       const PoissonArgs a) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= a.npair * a.L2) return;
-  const int p = t / a.L2, L = t % a.L2;
-  const int pm = a.pair_m[p];
-  size_t q0 = ((size_t)p * a.L2 + L) * a.nfe;
-  double2* xp = x + ((size_t)p * a.L2 + L) * a.nr;
-  if (L < (pm < 0 ? -pm : pm)) {
-    a.Q[(size_t)p * a.L2 + L] = make_double2(0.0, 0.0);
-    return;
+// ---------------------------------------------------------------------------
+// Radial Poisson solve per (pair, L) (hprod_mkv2mf_poissonp,
+// src/hprod/hprod_mkv2mf.F90:118-182) by element Schur complement of the
+// same FEDVR operator A_L = 2K + L(L+1)/r^2.
+// P1: block per (element, L); the block's threads form groups of GS lanes,
+// one group per pair, lane i < ni owning interior row i; G_e (the inverse of
+// the element's interior block) sits in LDS and serves every pair.
+//   y = G_e b_I, contributions Cl.y / Cr.y to the bridge equations and the
+//   element's part of the multipole moment sum rho r^L.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_poisson_elem(const double2* __restrict__ rho, double2* __restrict__ x,
+                                                      const PoissonArgs a) {
+  const int e = blockIdx.x;
+  const int L = blockIdx.y;
+  const int ni = a.ni;
+  const int GS = ni + 1;            // lanes per pair group: interior rows + the left bridge
+  const int ngroup = 256 / GS;
+  __shared__ double sG[32 * 32];
+  __shared__ double sCl[32], sCr[32], sInv[32], sRp0[33];
+  __shared__ double2 sB[256], sY[256], sM[256];
+  const double* G = a.G + (((size_t)L * a.nfe) + e) * ni * ni;
+  const int base = e * a.n;  // interior global index of local point 1
+  for (int q = threadIdx.x; q < ni * ni; q += 256) sG[q] = G[q];
+  for (int q = threadIdx.x; q < ni; q += 256) {
+    sCl[q] = a.Cl[(((size_t)L * a.nfe) + e) * ni + q];
+    sCr[q] = a.Cr[(((size_t)L * a.nfe) + e) * ni + q];
+    sInv[q] = a.invr[(size_t)L * a.nr + base + q];
+    sRp0[q] = a.rpl0[(size_t)L * a.nr + base + q];
   }
-  const double2* rp = rho + ((size_t)p * a.L2 + L) * a.nr;
-  const double* invr = a.invr + (size_t)L * a.nr;
+  if (threadIdx.x == 0) sRp0[ni] = e >= 1 ? a.rpl0[(size_t)L * a.nr + base - 1] : 0.0;
+  __syncthreads();
+  const int g = threadIdx.x / GS, lane = threadIdx.x % GS;
+  const bool ingroup = g < ngroup;
+  for (int p0 = 0; p0 < a.npair; p0 += ngroup) {
+    const int p = p0 + g;
+    bool valid = ingroup && p < a.npair;
+    if (valid) {
+      const int pm = a.pair_m[p];
+      valid = L >= (pm < 0 ? -pm : pm);
+    }
+    const double2* rp = rho + ((size_t)p * a.L2 + L) * a.nr;
+    if (valid) {
+      if (lane < ni) {
+        const double2 v = rp[base + lane];
+        sB[threadIdx.x] = cscale(v, sInv[lane]);
+        sM[threadIdx.x] = cscale(v, sRp0[lane]);
+      } else {
+        sM[threadIdx.x] = e >= 1 ? cscale(rp[base - 1], sRp0[ni]) : make_double2(0.0, 0.0);
+      }
+    }
+    __syncthreads();
+    if (valid && lane < ni) {
+      const double* gr = sG + lane * ni;
+      const double2* b = sB + g * GS;
+      double2 y = make_double2(0.0, 0.0);
+      for (int k = 0; k < ni; ++k) {
+        y.x = fma(gr[k], b[k].x, y.x);
+        y.y = fma(gr[k], b[k].y, y.y);
+      }
+      x[((size_t)p * a.L2 + L) * a.nr + base + lane] = y;
+      sY[threadIdx.x] = y;
+    }
+    __syncthreads();
+    if (valid && lane == 0) {
+      const double2* y = sY + g * GS;
+      const double2* m = sM + g * GS;
+      double2 cl = make_double2(0.0, 0.0), cr = make_double2(0.0, 0.0), mp = make_double2(0.0, 0.0);
+      for (int i = 0; i < ni; ++i) {
+        cl.x = fma(sCl[i], y[i].x, cl.x);
+        cl.y = fma(sCl[i], y[i].y, cl.y);
+        cr.x = fma(sCr[i], y[i].x, cr.x);
+        cr.y = fma(sCr[i], y[i].y, cr.y);
+      }
+      for (int i = 0; i <= ni; ++i) mp = cadd(mp, m[i]);
+      const size_t q = (((size_t)p * a.L2 + L) * a.nfe) + e;
+      a.cl[q] = cl;
+      a.cr[q] = cr;
+      a.mpart[q] = mp;
+    }
+    __syncthreads();
+  }
+}
+
+// P2: block per (L, chunk of 16 pairs): bridge right-hand sides staged in
+// LDS by all threads, then one lane per pair runs the tridiagonal (Thomas)
+// solve of the Schur system and sums the multipole moment.
+__global__ void __launch_bounds__(256) k_poisson_bridge(const double2* __restrict__ rho, double2* __restrict__ x,
+                                                        const PoissonArgs a) {
+  constexpr int PC = 16;
+  const int L = blockIdx.x;
+  const int p0 = blockIdx.y * PC;
   const int nb = a.nfe - 1;
-  const double* Sd = a.Sd + (size_t)L * nb;
-  const double* Sl = a.Sl + (size_t)L * nb;
-  const double* Su = a.Su + (size_t)L * nb;
-  // forward: s_k = b(bridge k) - c_R(element k-1) - c_L(element k)
-  double2 prev = make_double2(0.0, 0.0);
-  for (int k = 0; k < nb; ++k) {
-    int gi = (k + 1) * a.n - 1;
-    double2 s = cscale(rp[gi], invr[gi]);
+  __shared__ double2 sS[PC * kMaxBridge];
+  __shared__ double sSd[kMaxBridge], sSl[kMaxBridge], sSu[kMaxBridge];
+  __shared__ double2 sQ[PC * 16];
+  for (int k = threadIdx.x; k < nb; k += 256) {
+    sSd[k] = a.Sd[(size_t)L * nb + k];
+    sSl[k] = a.Sl[(size_t)L * nb + k];
+    sSu[k] = a.Su[(size_t)L * nb + k];
+  }
+  const double* invr = a.invr + (size_t)L * a.nr;
+  for (int t = threadIdx.x; t < PC * nb; t += 256) {
+    const int pl = t / nb, k = t % nb, p = p0 + pl;
+    if (p >= a.npair) continue;
+    const size_t q0 = ((size_t)p * a.L2 + L) * a.nfe;
+    const int gi = (k + 1) * a.n - 1;
+    double2 s = cscale(rho[((size_t)p * a.L2 + L) * a.nr + gi], invr[gi]);
     s = csub(s, a.cr[q0 + k]);
     s = csub(s, a.cl[q0 + k + 1]);
-    if (k > 0) s = csub(s, cscale(prev, Sl[k]));
-    xp[gi] = s;  // holds the forward-eliminated value
-    prev = s;
+    sS[pl * kMaxBridge + k] = s;
   }
-  // backward
-  double2 nxt = make_double2(0.0, 0.0);
-  for (int k = nb - 1; k >= 0; --k) {
-    int gi = (k + 1) * a.n - 1;
-    double2 s = xp[gi];
-    if (k < nb - 1) s = csub(s, cscale(nxt, Su[k]));
-    s = cscale(s, 1.0 / Sd[k]);
-    xp[gi] = s;
-    nxt = s;
+  // multipole moment: 16 lanes per pair sum the element parts
+  {
+    const int pl = threadIdx.x / 16, sub = threadIdx.x % 16, p = p0 + pl;
+    double2 m = make_double2(0.0, 0.0);
+    if (p < a.npair) {
+      const size_t q0 = ((size_t)p * a.L2 + L) * a.nfe;
+      for (int e = sub; e < a.nfe; e += 16) m = cadd(m, a.mpart[q0 + e]);
+    }
+    sQ[threadIdx.x] = m;
   }
-  double2 m = make_double2(0.0, 0.0);
-  for (int e = 0; e < a.nfe; ++e) m = cadd(m, a.mpart[q0 + e]);
-  a.Q[(size_t)p * a.L2 + L] = cscale(m, a.fac1[L]);
+  __syncthreads();
+  const int pl = threadIdx.x, p = p0 + pl;
+  if (pl < PC && p < a.npair) {
+    const int pm = a.pair_m[p];
+    double2* xp = x + ((size_t)p * a.L2 + L) * a.nr;
+    if (L < (pm < 0 ? -pm : pm)) {
+      a.Q[(size_t)p * a.L2 + L] = make_double2(0.0, 0.0);
+    } else {
+      double2* s = sS + pl * kMaxBridge;
+      for (int k = 1; k < nb; ++k) s[k] = csub(s[k], cscale(s[k - 1], sSl[k]));
+      double2 nxt = make_double2(0.0, 0.0);
+      for (int k = nb - 1; k >= 0; --k) {
+        double2 v = s[k];
+        if (k < nb - 1) v = csub(v, cscale(nxt, sSu[k]));
+        v = cscale(v, 1.0 / sSd[k]);
+        xp[(k + 1) * a.n - 1] = v;
+        nxt = v;
+      }
+      double2 m = make_double2(0.0, 0.0);
+      for (int k = 0; k < 16; ++k) m = cadd(m, sQ[pl * 16 + k]);
+      a.Q[(size_t)p * a.L2 + L] = cscale(m, a.fac1[L]);
+    }
+  }
 }
 
 // P3: per (pair, L, r): interior correction and potential assembly
@@ -410,6 +588,111 @@ __global__ void __launch_bounds__(256) k_mfprod(const double2* __restrict__ orbg
   }
 }
 
+// Register-resident variant of k_mfprod for a fixed orbital partition
+// (NFC frozen cores, NF orbitals, NA active): every orbital and pair index is
+// a compile-time constant, the pairs are visited in the order they are
+// stored (dyn-dyn i <= j, then dyn-fc1).
+template <int NFC, int NF, int NA>
+__global__ void __launch_bounds__(256) k_mfprod_t(const double2* __restrict__ orbg, const double2* __restrict__ vg,
+                                                  double2* __restrict__ gorbg, double2* __restrict__ v2orbg,
+                                                  const MfArgs a) {
+  constexpr int NCORE = NF - NA;
+  constexpr int NA2 = NA > 0 ? NA * NA : 1;
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int th = blockIdx.y;
+  if (r >= a.nr) return;
+  const size_t ng = (size_t)a.nlat * a.nr;
+  const size_t g = (size_t)th * a.nr + r;
+  double2 psi[NF], ag[NF], av[NF];
+#pragma unroll
+  for (int f = 0; f < NF; ++f) {
+    psi[f] = orbg[(size_t)f * ng + g];
+    ag[f] = make_double2(0.0, 0.0);
+    av[f] = make_double2(0.0, 0.0);
+  }
+  // actx weights wd1_j = -1/2 sum_k den1(k,j) psi_k (m_k = m_j)
+  double2 wd[NA > 0 ? NA : 1];
+#pragma unroll
+  for (int ja = 0; ja < NA; ++ja) {
+    wd[ja] = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int ka = 0; ka < NA; ++ka)
+      if (a.mval[NCORE + ka] == a.mval[NCORE + ja]) cfma(wd[ja], psi[NCORE + ka], cscale(a.den1[ka + NA * ja], -0.5));
+  }
+  double2 d2v_dc = make_double2(0.0, 0.0);
+  double2 vact[NA2];
+  int p = 0;
+#pragma unroll
+  for (int i = NFC; i < NF; ++i) {
+#pragma unroll
+    for (int j = i; j < NF; ++j) {
+      const double2 v = vg[(size_t)(p++) * ng + g];
+      const bool jdc = j < NCORE, idc = i < NCORE;
+      if (j == i && jdc) d2v_dc = cadd(d2v_dc, cscale(v, 2.0));
+      if (jdc) ag[i] = csub(ag[i], cmul(v, psi[j]));
+      if (idc && j != i) ag[j] = csub(ag[j], cmul(cconj(v), psi[i]));
+      if (j >= NCORE && i >= NCORE) {
+        vact[(j - NCORE) * NA + (i - NCORE)] = v;
+        if (j != i) vact[(i - NCORE) * NA + (j - NCORE)] = cconj(v);
+      } else if (j >= NCORE && idc) {
+        cfma(av[i], wd[j - NCORE], v);
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < NFC; ++i) {
+    if (i < a.nfcore2) continue;  // type -2 cores act through v2xfc only
+#pragma unroll
+    for (int j = NFC; j < NF; ++j) {
+      const double2 v = vg[(size_t)(p++) * ng + g];
+      if (r < a.nradfc) ag[j] = csub(ag[j], cmul(cconj(v), psi[i]));
+    }
+  }
+#pragma unroll
+  for (int f = NFC; f < NF; ++f) cfma(ag[f], d2v_dc, psi[f]);
+  if (NCORE > NFC && NA > 0) {
+    double2 d = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int ja = 0; ja < NA; ++ja)
+#pragma unroll
+      for (int ka = 0; ka < NA; ++ka) cfma(d, vact[ja * NA + ka], a.den1[ka + NA * ja]);
+#pragma unroll
+    for (int f = NFC; f < NCORE; ++f) cfma(av[f], d, psi[f]);
+  }
+  if (NA > 0 && a.nelact2 >= 2) {
+    double2 t[NA > 0 ? NA : 1];
+#pragma unroll
+    for (int q = 0; q < NA; ++q) t[q] = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+      for (int ja = 0; ja <= ia; ++ja) {
+        double2 d = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int ka = 0; ka < NA; ++ka)
+#pragma unroll
+          for (int la = 0; la < NA; ++la) {
+            if (a.mval[NCORE + ia] + a.mval[NCORE + ka] != a.mval[NCORE + ja] + a.mval[NCORE + la]) continue;
+            const double2 dens = a.den2[ja + NA * (ia + NA * (la + NA * ka))];
+            if (sqrt(dens.x * dens.x + dens.y * dens.y) <= 1.0e-15) continue;
+            cfma(d, dens, vact[ka * NA + la]);
+          }
+        cfma(t[ia], psi[NCORE + ja], d);
+        if (ia > ja) cfma(t[ja], psi[NCORE + ia], cconj(d));
+      }
+#pragma unroll
+    for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+      for (int ka = 0; ka < NA; ++ka)
+        if (a.mval[NCORE + ia] == a.mval[NCORE + ka]) cfma(av[NCORE + ka], a.rden[ia + NA * ka], t[ia]);
+  }
+#pragma unroll
+  for (int f = NFC; f < NF; ++f) {
+    gorbg[(size_t)(f - NFC) * ng + g] = ag[f];
+    v2orbg[(size_t)(f - NFC) * ng + g] = av[f];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Velocity-gauge frozen core on the theta grid (hprod_fcorbg,
 // src/hprod/hprod_orbin.F90:68-105): orbg_fc = orbg0_fc * exp(-i A r cos(theta)).
@@ -432,64 +715,119 @@ __global__ void k_fc_phase(const double2* __restrict__ orbg0, double2* __restric
 // Gram products over a common index range, FP64, deterministic two-stage sum
 // (the inner products of hprod_mkint1_sph, hprod_mkint2_sph, hprod_ene_dcore
 // and hprod_htot_mkxmat_*):
-//   S0[a][b] = sum_q opA(Lv_a[q]) * Rv_b[q],  S1[a][b] = sum_q opA(Lv_a[q]) * conj(Rv_b[q])
-// opA = conj when conj_left.  One block stages a CH-long slice of every
-// vector in LDS, so each vector element is read from memory once.
+//   S0[a][b] = sum_q opA(L_a[q]) R_b[q],  S1[a][b] = sum_q opA(L_a[q]) conj(R_b[q])
+// opA = conj when conj_left.  A block owns a TA x TB tile of (a, b) and a
+// slice of q; each thread keeps the tile's accumulators in registers, so each
+// loaded element feeds TA or TB products.  The block result goes to
+// partial[entry][which][block-in-q].
 // ---------------------------------------------------------------------------
-template <int CH>
-__global__ void __launch_bounds__(256) k_gram(const GramArgs a, double2* __restrict__ partial) {
-  extern __shared__ double2 sv[];  // (nL + nR) * CH
-  const int nL = a.nL, nR = a.nR, nE = nL * nR;
-  double2 acc0[4], acc1[4];
+template <int TA, int TB, bool CR>
+__global__ void __launch_bounds__(256) k_gram(const GramArgs a, double2* __restrict__ partial, int nqblk) {
+  const int ntb = (a.nR + TB - 1) / TB;
+  const int ta = blockIdx.y / ntb, tb = blockIdx.y % ntb;
+  const int a0 = ta * TA, b0 = tb * TB;
+  const double sa = a.conj_left ? -1.0 : 1.0;
+  double2 s0[TA][TB], s1[CR ? TA : 1][CR ? TB : 1];
 #pragma unroll
-  for (int k = 0; k < 4; ++k) { acc0[k] = make_double2(0.0, 0.0); acc1[k] = make_double2(0.0, 0.0); }
-  for (size_t q0 = (size_t)blockIdx.x * CH; q0 < a.len; q0 += (size_t)gridDim.x * CH) {
-    __syncthreads();
-    for (int t = threadIdx.x; t < (nL + nR) * CH; t += 256) {
-      const int v = t / CH, c = t % CH;
-      const size_t q = q0 + c;
-      double2 x = make_double2(0.0, 0.0);
-      if (q < a.len) x = v < nL ? a.left[v][q] : a.right[v - nL][q];
-      sv[t] = x;
+  for (int u = 0; u < TA; ++u)
+#pragma unroll
+    for (int v = 0; v < TB; ++v) {
+      s0[u][v] = make_double2(0.0, 0.0);
+      if (CR) s1[u][v] = make_double2(0.0, 0.0);
     }
-    __syncthreads();
+  const double2* Lp[TA];
+  const double2* Rp[TB];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const int e = threadIdx.x + 256 * k;
-      if (e >= nE) break;
-      const int ia = e / nR, ib = e % nR;
-      const double2* x = sv + ia * CH;
-      const double2* y = sv + (nL + ib) * CH;
-      double2 s0 = acc0[k], s1 = acc1[k];
-      const double sa = a.conj_left ? -1.0 : 1.0;
-      for (int c = 0; c < CH; ++c) {
-        const double2 u = make_double2(x[c].x, sa * x[c].y), w = y[c];
-        s0.x = fma(u.x, w.x, s0.x); s0.x = fma(-u.y, w.y, s0.x);
-        s0.y = fma(u.x, w.y, s0.y); s0.y = fma(u.y, w.x, s0.y);
-        if (a.with_conj_right) {
-          s1.x = fma(u.x, w.x, s1.x); s1.x = fma(u.y, w.y, s1.x);
-          s1.y = fma(-u.x, w.y, s1.y); s1.y = fma(u.y, w.x, s1.y);
-        }
+  for (int u = 0; u < TA; ++u) Lp[u] = a0 + u < a.nL ? a.left[a0 + u] : nullptr;
+#pragma unroll
+  for (int v = 0; v < TB; ++v) Rp[v] = b0 + v < a.nR ? a.right[b0 + v] : nullptr;
+  for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < a.len; q += (size_t)nqblk * 256) {
+    double2 x[TA], y[TB];
+#pragma unroll
+    for (int u = 0; u < TA; ++u) {
+      x[u] = Lp[u] ? Lp[u][q] : make_double2(0.0, 0.0);
+      x[u].y *= sa;
+    }
+#pragma unroll
+    for (int v = 0; v < TB; ++v) y[v] = Rp[v] ? Rp[v][q] : make_double2(0.0, 0.0);
+#pragma unroll
+    for (int u = 0; u < TA; ++u)
+#pragma unroll
+      for (int v = 0; v < TB; ++v) {
+        cfma(s0[u][v], x[u], y[v]);
+        if (CR) cfma(s1[u][v], x[u], cconj(y[v]));
       }
-      acc0[k] = s0;
-      acc1[k] = s1;
-    }
   }
+  // block reduction.  The tile holds NV = TA*TB*(CR ? 2 : 1) complex sums =
+  // 2*NV doubles per lane; with 2*NV = 64 a transposing butterfly leaves lane
+  // t with the wave total of double t after 63 exchanges, then the 4 waves
+  // are summed through LDS.
+  constexpr int NV = TA * TB * (CR ? 2 : 1);
+  static_assert(2 * NV == 64, "tile must hold 64 doubles per lane");
+  double v[64];
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const int e = threadIdx.x + 256 * k;
-    if (e >= nE) break;
-    partial[((size_t)e * 2 + 0) * gridDim.x + blockIdx.x] = acc0[k];
-    partial[((size_t)e * 2 + 1) * gridDim.x + blockIdx.x] = acc1[k];
+  for (int u = 0; u < TA; ++u)
+#pragma unroll
+    for (int w = 0; w < TB; ++w)
+#pragma unroll
+      for (int c = 0; c < (CR ? 2 : 1); ++c) {
+        const int e = (u * TB + w) * (CR ? 2 : 1) + c;
+        const double2 t = c == 0 ? s0[u][w] : s1[CR ? u : 0][CR ? w : 0];
+        v[2 * e] = t.x;
+        v[2 * e + 1] = t.y;
+      }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#define TD1C_GRAM_STEP(H)                                      \
+  _Pragma("unroll") for (int j = 0; j < H; ++j) {              \
+    const bool up = (lane & H) != 0;                           \
+    const double send = up ? v[j] : v[j + H];                  \
+    const double keep = up ? v[j + H] : v[j];                  \
+    v[j] = keep + __shfl_xor(send, H, 64);                     \
+  }
+  TD1C_GRAM_STEP(32)
+  TD1C_GRAM_STEP(16)
+  TD1C_GRAM_STEP(8)
+  TD1C_GRAM_STEP(4)
+  TD1C_GRAM_STEP(2)
+  TD1C_GRAM_STEP(1)
+#undef TD1C_GRAM_STEP
+  __shared__ double red[4][64];
+  red[wave][lane] = v[0];
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    const int e = threadIdx.x;
+    const int c = CR ? e % 2 : 0, uv = CR ? e / 2 : e, u = uv / TB, w = uv % TB;
+    if (a0 + u < a.nL && b0 + w < a.nR) {
+      double2 t;
+      t.x = (red[0][2 * e] + red[1][2 * e]) + (red[2][2 * e] + red[3][2 * e]);
+      t.y = (red[0][2 * e + 1] + red[1][2 * e + 1]) + (red[2][2 * e + 1] + red[3][2 * e + 1]);
+      const int entry = (a0 + u) * a.nR + (b0 + w);
+      partial[((size_t)entry * 2 + c) * nqblk + blockIdx.x] = t;
+    }
   }
 }
 
-__global__ void k_gram_final(const double2* __restrict__ partial, int nblk, int nent, double2* __restrict__ out) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+// one wavefront per Gram entry: strided partial sums, then a butterfly
+__global__ void __launch_bounds__(256) k_gram_final(const double2* __restrict__ partial, int nblk, int nent,
+                                                    double2* __restrict__ out, int with_conj) {
+  const int p = blockIdx.x * 4 + threadIdx.x / 64;
+  const int lane = threadIdx.x % 64;
   if (p >= nent) return;
-  double2 s = make_double2(0.0, 0.0);
-  for (int b = 0; b < nblk; ++b) s = cadd(s, partial[(size_t)p * nblk + b]);
-  out[p] = s;
+  if (!with_conj && (p & 1)) {
+    if (lane == 0) out[p] = make_double2(0.0, 0.0);
+    return;
+  }
+  double sx = 0.0, sy = 0.0;
+  for (int b = lane; b < nblk; b += 64) {
+    const double2 v = partial[(size_t)p * nblk + b];
+    sx += v.x;
+    sy += v.y;
+  }
+  for (int w = 32; w > 0; w >>= 1) {
+    sx += __shfl_xor(sx, w, 64);
+    sy += __shfl_xor(sy, w, 64);
+  }
+  if (lane == 0) out[p] = make_double2(sx, sy);
 }
 
 // ---------------------------------------------------------------------------
@@ -566,24 +904,27 @@ __global__ void k_axpy_list(double2* __restrict__ y, const double2* __restrict__
 // explicit instantiations / launch wrappers
 // ---------------------------------------------------------------------------
 void launch_transform(int mode, const double2* in, double2* out, const double* mats, const TransformBatch* batch,
-                      int nbatch, int nr, int K, int C, hipStream_t s) {
-  dim3 block(64, 4);
-  dim3 grid((nr + 63) / 64, nbatch);
-  // columns per thread: ceil(C / 4), instantiated for the sizes used
-  int cpt = (C + 3) / 4;
+                      int nbatch, int nr, int K, int C, int Cp, double2* copy, hipStream_t s) {
+  const int rt = (nr + 63) / 64;
+  // 8 columns per wave when that gives enough blocks, else 4
+  const int blocks8 = rt * ((C + 31) / 32) * nbatch;
+  const int cpw = blocks8 >= 512 ? 8 : 4;
+  dim3 grid(rt, (C + 4 * cpw - 1) / (4 * cpw), nbatch), block(256);
   if (mode == 0) {
-    if (cpt <= 4) hipLaunchKernelGGL((k_transform<0, 4>), grid, block, 0, s, in, out, mats, batch, nr, K, C);
-    else if (cpt <= 8) hipLaunchKernelGGL((k_transform<0, 8>), grid, block, 0, s, in, out, mats, batch, nr, K, C);
-    else if (cpt <= 16) hipLaunchKernelGGL((k_transform<0, 16>), grid, block, 0, s, in, out, mats, batch, nr, K, C);
-    else if (cpt <= 32) hipLaunchKernelGGL((k_transform<0, 32>), grid, block, 0, s, in, out, mats, batch, nr, K, C);
-    else hipLaunchKernelGGL((k_transform<0, 64>), grid, block, 0, s, in, out, mats, batch, nr, K, C);
+    if (cpw == 8) hipLaunchKernelGGL((k_transform<0, 8>), grid, block, 0, s, in, out, mats, batch, nr, K, C, Cp, copy);
+    else hipLaunchKernelGGL((k_transform<0, 4>), grid, block, 0, s, in, out, mats, batch, nr, K, C, Cp, copy);
   } else {
-    if (cpt <= 4) hipLaunchKernelGGL((k_transform<1, 4>), grid, block, 0, s, in, out, mats, batch, nr, K, C);
-    else if (cpt <= 8) hipLaunchKernelGGL((k_transform<1, 8>), grid, block, 0, s, in, out, mats, batch, nr, K, C);
-    else if (cpt <= 16) hipLaunchKernelGGL((k_transform<1, 16>), grid, block, 0, s, in, out, mats, batch, nr, K, C);
-    else if (cpt <= 32) hipLaunchKernelGGL((k_transform<1, 32>), grid, block, 0, s, in, out, mats, batch, nr, K, C);
-    else hipLaunchKernelGGL((k_transform<1, 64>), grid, block, 0, s, in, out, mats, batch, nr, K, C);
+    if (cpw == 8) hipLaunchKernelGGL((k_transform<1, 8>), grid, block, 0, s, in, out, mats, batch, nr, K, C, Cp, nullptr);
+    else hipLaunchKernelGGL((k_transform<1, 4>), grid, block, 0, s, in, out, mats, batch, nr, K, C, Cp, nullptr);
   }
+}
+
+bool launch_onebody_el(const double2* orb, double2* h01, const OneBodyElArgs& a, hipStream_t s) {
+  if (a.ndvr != 20) return false;
+  constexpr int GE = 256 / 21;
+  dim3 grid((a.nfe + GE - 1) / GE, a.nfun);
+  hipLaunchKernelGGL((k_onebody_el<20>), grid, dim3(256), 0, s, orb, h01, a);
+  return true;
 }
 
 void launch_onebody(const double2* orb, double2* h01, const OneBodyArgs& a, hipStream_t s) {
@@ -593,15 +934,33 @@ void launch_onebody(const double2* orb, double2* h01, const OneBodyArgs& a, hipS
 }
 
 void launch_poisson(const double2* rho, double2* work, double2* v, const PoissonArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_poisson_elem, dim3(a.nfe, a.L2), dim3(64), 0, s, rho, work, a);
-  int nt = a.npair * a.L2;
-  hipLaunchKernelGGL(k_poisson_bridge, dim3((nt + 63) / 64), dim3(64), 0, s, rho, work, a);
+  hipLaunchKernelGGL(k_poisson_elem, dim3(a.nfe, a.L2), dim3(256), 0, s, rho, work, a);
+  hipLaunchKernelGGL(k_poisson_bridge, dim3(a.L2, (a.npair + 15) / 16), dim3(256), 0, s, rho, work, a);
   hipLaunchKernelGGL(k_poisson_final, dim3((a.nr + 255) / 256, a.L2, a.npair), dim3(256), 0, s, work, v, a);
 }
 
 void launch_mfprod(const double2* orbg, const double2* vg, double2* gorbg, double2* v2orbg, const MfArgs& a,
                    hipStream_t s) {
-  hipLaunchKernelGGL(k_mfprod, dim3((a.nr + 255) / 256, a.nlat), dim3(256), 0, s, orbg, vg, gorbg, v2orbg, a);
+  dim3 grid((a.nr + 255) / 256, a.nlat), block(256);
+  const int key = a.nfcore * 10000 + a.nfun * 100 + a.nact;
+#define MF_CASE(NFC, NF, NA)                                                                            \
+  case NFC * 10000 + NF * 100 + NA:                                                                     \
+    hipLaunchKernelGGL((k_mfprod_t<NFC, NF, NA>), grid, block, 0, s, orbg, vg, gorbg, v2orbg, a);     \
+    return;
+  switch (key) {
+    MF_CASE(0, 1, 0)
+    MF_CASE(0, 2, 0)
+    MF_CASE(0, 3, 0)
+    MF_CASE(0, 9, 0)
+    MF_CASE(0, 16, 0)
+    MF_CASE(1, 5, 4)
+    MF_CASE(0, 3, 2)
+    MF_CASE(1, 5, 3)
+    default:
+      break;
+  }
+#undef MF_CASE
+  hipLaunchKernelGGL(k_mfprod, grid, block, 0, s, orbg, vg, gorbg, v2orbg, a);
 }
 
 void launch_fc_phase(const double2* orbg0, double2* orbg, const double* xr, const double* cost, double afield, int nr,
@@ -610,12 +969,21 @@ void launch_fc_phase(const double2* orbg0, double2* orbg, const double* xr, cons
                      nradfc, nlat, nfc);
 }
 
-void launch_gram(const GramArgs& a, double2* partial, int nblk, double2* out, hipStream_t s) {
-  constexpr int CH = 32;
-  size_t shm = (size_t)(a.nL + a.nR) * CH * sizeof(double2);
-  hipLaunchKernelGGL((k_gram<CH>), dim3(nblk), dim3(256), shm, s, a, partial);
+void launch_gram(const GramArgs& a, double2* partial, int maxblk, double2* out, hipStream_t s) {
+  // about 8 index values per thread, at most maxblk blocks along the index
+  const int nblk = (int)std::max<size_t>(1, std::min<size_t>(maxblk, (a.len + 2047) / 2048));
+  if (a.with_conj_right) {
+    constexpr int TA = 4, TB = 4;
+    dim3 grid(nblk, ((a.nL + TA - 1) / TA) * ((a.nR + TB - 1) / TB));
+    hipLaunchKernelGGL((k_gram<TA, TB, true>), grid, dim3(256), 0, s, a, partial, nblk);
+  } else {
+    constexpr int TA = 4, TB = 8;
+    dim3 grid(nblk, ((a.nL + TA - 1) / TA) * ((a.nR + TB - 1) / TB));
+    hipLaunchKernelGGL((k_gram<TA, TB, false>), grid, dim3(256), 0, s, a, partial, nblk);
+  }
   int nent = a.nL * a.nR * 2;
-  hipLaunchKernelGGL(k_gram_final, dim3((nent + 63) / 64), dim3(64), 0, s, partial, nblk, nent, out);
+  hipLaunchKernelGGL(k_gram_final, dim3((nent + 3) / 4), dim3(256), 0, s, partial, nblk, nent, out,
+                     a.with_conj_right);
 }
 
 void launch_dtorb(const DtorbArgs& a, hipStream_t s) {

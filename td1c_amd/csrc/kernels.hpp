@@ -12,6 +12,7 @@ constexpr int kMaxFun = 16;       // orbitals handled by the theta-grid mean-fie
 constexpr int kMaxAct = 8;        // active orbitals handled on the theta grid
 constexpr int kMaxActPairs = kMaxAct * kMaxAct;
 constexpr int kMaxStages = 6;
+constexpr int kMaxBridge = 160;   // nfe - 1 bridges handled by the Poisson bridge kernel
 
 struct TransformBatch {
   long long in0;   // element offset of the input block (MODE 1: orbital j)
@@ -33,6 +34,16 @@ struct OneBodyArgs {
   double2 zfield, efield;
 };
 
+struct OneBodyElArgs {
+  int nr, nfe, L1, ndvr, nfcore, nradfc, nfun, mmax1, igauge, fc_full;
+  const int* mval;
+  const double *eK0, *eK, *eP;  // [e][k][m] element matrices
+  const double* xr;
+  const double* alph;
+  const double2* tzor;          // [L1][nr] tmat diagonal without the kinetic part (+ v2jfc)
+  double2 zfield, efield;
+};
+
 struct PoissonArgs {
   int nr, n, nfe, ni, L2, npair;
   const int* pair_m;  // m_j - m_i of each pair
@@ -42,7 +53,7 @@ struct PoissonArgs {
 };
 
 struct MfArgs {
-  int nr, nlat, nfun, nfcore, ncore, nact, nradfc, npair, nelact2;
+  int nr, nlat, nfun, nfcore, nfcore2, ncore, nact, nradfc, npair, nelact2;
   const int *pj, *pi, *ptype, *mval;
   const double2 *den1, *den2, *rden;
 };
@@ -103,8 +114,10 @@ struct IslandArgs {
 };
 
 void launch_transform(int mode, const double2* in, double2* out, const double* mats, const TransformBatch* batch,
-                      int nbatch, int nr, int K, int C, hipStream_t s);
+                      int nbatch, int nr, int K, int C, int Cp, double2* copy, hipStream_t s);
+inline int pad_cols(int C) { return (C + 31) / 32 * 32; }
 void launch_onebody(const double2* orb, double2* h01, const OneBodyArgs& a, hipStream_t s);
+bool launch_onebody_el(const double2* orb, double2* h01, const OneBodyElArgs& a, hipStream_t s);
 void launch_poisson(const double2* rho, double2* work, double2* v, const PoissonArgs& a, hipStream_t s);
 void launch_mfprod(const double2* orbg, const double2* vg, double2* gorbg, double2* v2orbg, const MfArgs& a,
                    hipStream_t s);

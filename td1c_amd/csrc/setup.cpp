@@ -262,6 +262,31 @@ bool Setup::build(const td1c_problem& p) {
       }
   }
 
+  // element-local kinetic / derivative matrices (the per-element terms that
+  // gen_radk / gen_radp sum into the global bands)
+  eK0.assign((size_t)nfe * np * np, 0); eK.assign((size_t)nfe * np * np, 0); eP.assign((size_t)nfe * np * np, 0);
+  for (int e = 0; e < nfe; ++e) {
+    const double* we = &ew[(size_t)e * np];
+    const double* ds = &eds[(size_t)e * np * np];
+    for (int m = 0; m < np; ++m)
+      for (int k = 0; k < np; ++k) {
+        if (e == 0 && (m == 0 || k == 0)) continue;
+        if (e == nfe - 1 && (m == n || k == n)) continue;
+        int mb = e * n + m, kb = e * n + k;
+        double t = 0, t0 = 0;
+        for (int l = 0; l < np; ++l) {
+          double c = we[l] * ds[np * m + l] * ds[np * k + l];
+          t += c;
+          if (!(e == 0 && l == 0)) t0 += c;
+        }
+        double nrm = std::sqrt(wrad[mb] * wrad[kb]);
+        size_t at = ((size_t)e * np + k) * np + m;
+        eK0[at] = 0.5 * t / nrm;
+        eK[at] = 0.5 * t0 / nrm;
+        eP[at] = we[m] * ds[np * k + m] / nrm;
+      }
+  }
+
   // ---- kernel-side one-body tables ----
   xr.assign(nr, 0);
   for (int i = 0; i < nr; ++i) xr[i] = xrad[i + 1];
@@ -277,12 +302,14 @@ bool Setup::build(const td1c_problem& p) {
   // diagonal of tmat (bas_gen_tmat, src/bas/bas_gen_tmat.F90:63-118): the
   // Coulomb+centrifugal value first, then the kinetic diagonal added
   tdiag.assign((size_t)2 * L1 * nr, 0);
+  tzor.assign((size_t)2 * L1 * nr, 0);
   for (int l = 0; l < L1; ++l)
     for (int i = 0; i < nr; ++i) {
       double oor = 1.0 / xr[i];
       double zor = oor * (-znuc + (double)(l * (l + 1)) * oor * 0.5);
       double kd = (l == 0 ? radk : radk0)[bw * (i + 1) + n];
       tdiag[2 * ((size_t)l * nr + i)] = zor + kd;
+      tzor[2 * ((size_t)l * nr + i)] = zor;
     }
   // alph_lm (bas_gen_alph, src/bas/bas_gen_alph.F90:14-25)
   alph.assign((size_t)(2 * mmax1 + 1) * L1, 0);

@@ -39,6 +39,25 @@ def cas_fc_small(**kw):
                    nel_alpha=3, nel_beta=3, lval=[0, 0, 0, 0, 0], **kw)
 
 
+def he_n16(**kw):
+    # ndvr = 16 runs the band one-body kernel instead of the element kernel
+    return Problem(nfe=10, ndvr=16, rmax=30.0, lmax1=5, lmax2=10, znuc=2.0, ndcore=1,
+                   nel_alpha=1, nel_beta=1, **kw)
+
+
+def dc4_small(**kw):
+    # four doubly occupied orbitals: the generic theta-grid mean-field kernel
+    return Problem(nfe=6, ndvr=20, rmax=18.0, lmax1=5, lmax2=10, znuc=8.0, ndcore=4,
+                   nel_alpha=4, nel_beta=4, **kw)
+
+
+def fc2_small(**kw):
+    # one type -2 frozen core (exchange only through v2xfc) + two dynamical cores
+    kw.setdefault("igauge", 1)
+    return Problem(nfe=6, ndvr=20, rmax=18.0, lmax1=5, lmax2=10, znuc=6.0, nfcore2=1, ndcore=2,
+                   nel_alpha=3, nel_beta=3, lval=[0, 0, 0], **kw)
+
+
 def wfn_for(prob: Problem, lcic: int, seed: int = 11) -> np.ndarray:
     orbs = W.synth_orbitals(prob, seed)
     rng = np.random.Generator(np.random.PCG64(seed + 1))
