@@ -355,3 +355,124 @@ def test_mask_closed_form():
     for i in range(1, o.nrad):
         ref = 1.0 if x[i] <= 12.0 else math.cos(0.5 * math.pi * (x[i] - 12.0) / 8.0) ** 0.25
         assert abs(m[i] - ref) < 1e-15
+
+
+def test_two_electron_integrals_hydrogen_slater():
+    """mkrho2 + theta->L transform + poisson + mkint2 with L = 0, 1, 2 and
+    m != 0: int2e over hydrogen 1s, 2p0, 2p+1, 2p-1 (Z = 1) equals the
+    textbook Slater-Condon values F0(1s,1s) = 5/8, F0(1s,2p) = 59/243,
+    G1(1s,2p) = 112/2187, F0(2p,2p) = 93/512, F2(2p,2p) = 45/512 with the
+    Condon-Shortley coefficients c^2(1m,1m') (J = F0 + c^2(m,m) c^2(m',m') F2,
+    K = c^2(m,m')^2 F2; s-p exchange G1/3)."""
+    prob = Problem(nfe=40, ndvr=20, rmax=80.0, lmax1=2, lmax2=4, znuc=1.0, nact=4, nel_alpha=1, nel_beta=1,
+                   mval=[0, 0, 1, -1])
+    o = O.Oracle(prob)
+    r, w = W.gll_grid(prob)
+    nr, L1 = prob.nr, prob.lmax1 + 1
+    wf = np.zeros(o.wfn_size, complex)
+    orb = wf[: 4 * L1 * nr].reshape(4, L1, nr)
+    orb[0, 0] = 2 * r * np.exp(-r) * np.sqrt(w)
+    for f in (1, 2, 3):
+        orb[f, 1] = r * r * np.exp(-r / 2) / (2 * math.sqrt(6)) * np.sqrt(w)
+    rng = np.random.default_rng(3)
+    c = rng.standard_normal(o.lcic) + 1j * rng.standard_normal(o.lcic)
+    wf[4 * L1 * nr:] = c / np.linalg.norm(c)
+    o.set_wfn0(wf)
+    o.htot(1.0, np.zeros(9), wf)
+    g = o.table("int2e").reshape((4,) * 4, order="F")
+    F0ss, F0sp, G1, F0pp, F2pp = 5 / 8, 59 / 243, 112 / 2187, 93 / 512, 45 / 512
+    J = lambda a, b: g[a, a, b, b]
+    K = lambda a, b: g[a, b, b, a]
+    pairs = [
+        (J(0, 0), F0ss), (J(0, 1), F0sp), (J(0, 2), F0sp), (J(0, 3), F0sp),
+        (K(0, 1), G1 / 3), (K(0, 2), G1 / 3), (K(0, 3), G1 / 3),
+        (J(1, 1), F0pp + 4 * F2pp / 25), (J(2, 2), F0pp + F2pp / 25), (J(3, 3), F0pp + F2pp / 25),
+        (J(2, 1), F0pp - 2 * F2pp / 25), (J(2, 3), F0pp + F2pp / 25),
+        (K(2, 1), 3 * F2pp / 25), (K(3, 1), 3 * F2pp / 25), (K(2, 3), 6 * F2pp / 25),
+    ]
+    for got, want in pairs:
+        assert abs(got - want) < 1e-11, (got, want)
+
+
+def _random_ci(o, seed):
+    rng = np.random.default_rng(seed)
+    c = rng.standard_normal(o.lcic) + 1j * rng.standard_normal(o.lcic)
+    return c / np.linalg.norm(c)
+
+
+@pytest.mark.parametrize("mval,ncore", [([0, 0, 0, 0], 0), ([0, 1, -1, 0], 0), ([0, 0, 1, -1, 0], 1)])
+def test_act2_mean_field_reproduces_two_electron_fock(mval, ncore):
+    """mfprod2 act2 (+ rden): sum_k <p|v2orb_k> D(k,q) equals the two-electron
+    part of the generalised Fock matrix sum_jkl int2e(p,j,k,l) den2(j,q,l,k)
+    (m-matched p, q), with int2e pinned by the Slater-integral test.  A swapped
+    RDM index or a transposed rden breaks the equality for a complex CI."""
+    na = 4
+    prob = Problem(nfe=6, ndvr=10, rmax=20.0, lmax1=3, lmax2=6, znuc=3.0, ndcore=ncore, nact=na,
+                   nel_alpha=2 + ncore, nel_beta=1 + ncore, mval=mval)
+    o = O.Oracle(prob)
+    nf, nb = prob.nfun, prob.nr * (prob.lmax1 + 1)
+    wf = W.make_wfn(prob, 11, o.lcic)
+    wf[nf * nb:] = _random_ci(o, 5)
+    o.set_wfn0(wf)
+    o.htot(1.0, np.zeros(9), wf)
+    orb = wf[: nf * nb].reshape(nf, nb)[ncore:]
+    v2 = o.table("v2orb")[: nf * nb].reshape(nf, nb)[ncore:]
+    D = o.table("den1").reshape(na, na, order="F")
+    G2 = o.table("den2").reshape((na,) * 4, order="F")
+    g = o.table("int2e").reshape((na,) * 4, order="F")
+    ma = np.array(mval[ncore:])
+    same = ma[:, None] == ma[None, :]
+    t = np.einsum("kx,kq->qx", v2, D)
+    M = np.array([[np.vdot(orb[p], t[q]) for q in range(na)] for p in range(na)]) * same
+    M2 = np.einsum("pjkl,jqlk->pq", g, G2) * same
+    assert np.abs(M - M2).max() < 1e-12 * max(1.0, np.abs(M2).max())
+
+
+@pytest.mark.parametrize("mval", [[0, 0], [0, 1], [1, -1]])
+def test_doubly_occupied_active_equals_dynamic_core(mval):
+    """mfprod2 actj/actx vs dcj/dcx, and ene_act + int1e vs ene_dcore: a
+    single active orbital holding two electrons (D = 2) gives the same
+    g + v2 on both orbitals and the same total energy as two dynamical cores."""
+    base = dict(nfe=6, ndvr=10, rmax=20.0, lmax1=3, lmax2=6, znuc=3.0, mval=mval, nel_alpha=2, nel_beta=2)
+    pa = Problem(ndcore=2, nact=0, **base)
+    pb = Problem(ndcore=1, nact=1, mtot=2 * mval[1], **base)
+    oa, ob = O.Oracle(pa), O.Oracle(pb)
+    wa = W.make_wfn(pa, 21, oa.lcic)
+    nb = pa.nr * (pa.lmax1 + 1)
+    wb = np.zeros(ob.wfn_size, complex)
+    wb[: 2 * nb] = wa[: 2 * nb]
+    wb[2 * nb:] = 1.0
+    for o, w in ((oa, wa), (ob, wb)):
+        o.set_wfn0(w)
+        o.htot(1.0, np.zeros(9), w)
+    assert abs(ob.table("den1")[0] - 2.0) < 1e-14
+    ga = oa.table("gorb")[: 2 * nb] + oa.table("v2orb")[: 2 * nb]
+    gb = ob.table("gorb")[: 2 * nb] + ob.table("v2orb")[: 2 * nb]
+    assert np.abs(ga - gb).max() < 1e-13 * np.abs(ga).max()
+    assert abs(oa.energies()[4] - ob.energies()[4]) < 1e-12 * abs(oa.energies()[4])
+
+
+@pytest.mark.parametrize("mval", [[0, 0], [0, 1], [0, -2]])
+def test_frozen_core_equals_dynamic_core_operator(mval):
+    """hprod_set_wfn0 (rhofc, v2jfc folded into tmat) + mfprod2 fcx1 vs
+    dcj/dcx: a frozen 1s core (type -1) acting on a dynamical orbital gives
+    the same (h0 + g + v2) psi as the same 1s held as a dynamical core."""
+    base = dict(nfe=6, ndvr=10, rmax=20.0, lmax1=3, lmax2=6, znuc=3.0, mval=mval, nel_alpha=2, nel_beta=2)
+    pa = Problem(ndcore=2, **base)
+    pb = Problem(nfcore1=1, ndcore=1, lval=[0, 0], **base)
+    oa, ob = O.Oracle(pa), O.Oracle(pb)
+    wa = W.make_wfn(pa, 21, oa.lcic)
+    nr, nb = pa.nr, pa.nr * (pa.lmax1 + 1)
+    o0 = np.zeros(nb, complex)
+    o0[:nr] = W.hydrogenic_1s(pa, 3.0)
+    o1 = wa[nb:2 * nb].copy()
+    if mval[1] == 0:
+        o1 -= np.vdot(o0, o1) * o0
+    wa[:nb], wa[nb:2 * nb] = o0, o1 / np.linalg.norm(o1)
+    for o in (oa, ob):
+        o.set_wfn0(wa)
+        o.htot(1.0, np.zeros(9), wa)
+    d = slice(nb, 2 * nb)
+    ga = oa.table("gorb")[d] + oa.table("v2orb")[d] + oa.table("h0orb")[d]
+    gb = ob.table("gorb")[d] + ob.table("v2orb")[d] + ob.table("h0orb")[d]
+    assert np.abs(ga - gb).max() < 1e-13 * np.abs(ga).max()
