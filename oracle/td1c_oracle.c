@@ -22,8 +22,21 @@
  *  - LAPACK dsyev for the Gauss-Lobatto nodes: replaced by Newton iteration
  *    on P'_n (same roots);
  *  - LAPACK zheev for the n x n Hermitian 1-RDM: cyclic complex Jacobi.
- * Parity status of each function is listed in the oracle README section of
- * the design notes and in tests/test_oracle_pins.py.
+ * Parity status.  The reference cannot be built in this image and ships no
+ * tests or golden vectors, so each function is pinned by closed forms,
+ * library routines and invariants (tests/test_oracle_pins.py names the
+ * function each test pins): GLL/Gauss-Legendre rules, Legendre tables,
+ * hydrogenic spectra, dipole matrix elements, band Cholesky, the 1s Hartree
+ * potential, hydrogen Slater integrals (L = 0..2, m != 0), the act2
+ * generalised-Fock identity, core/active and frozen/dynamical core
+ * equivalences, energy and orthonormality conservation of the equations of
+ * motion, CI sigma/RDM identities, RK order, the mask closed form.
+ * Parity unpinned:
+ *  - the sign convention of the m < 0 Legendre columns (copied from |m|
+ *    without (-1)^m, as sph_gen.F90 does; only products of two columns enter
+ *    hprod, so no pinned quantity sees it);
+ *  - the order and phase of the CAS determinants (the cic layout);
+ *  - ene_fcore (not computed, 0) and the KLI/LDA v2xfc (input only).
  */
 #include <complex.h>
 #include <math.h>
