@@ -151,6 +151,16 @@ int td1c_rk_prop(td1c_ctx* ctx, int rk_order, double dtime,
                  const double* lfield_stages, double* d_wfn, int apply_mask,
                  void* stream);
 
+/* nstep Runge-Kutta steps (+ mask) from time t0 with the stage fields
+ * evaluated on the device: the time loop of tdse_split2
+ * (src/main/tdse.cpp:225-266) without its printing, each step doing what
+ * td1c_rk_prop does with clfield::get_value's values.  The first call for a
+ * given (rk_order, dtime, field, d_wfn, apply_mask) captures one step as a
+ * HIP graph; every step replays it.  Enqueued on `stream`, asynchronous. */
+int td1c_propagate(td1c_ctx* ctx, const td1c_field* field, int rk_order,
+                   double t0, double dtime, int nstep, double* d_wfn,
+                   int apply_mask, void* stream);
+
 /* Applies the radial mask to the dynamical orbitals of a device wave
  * function (wfn_mask, src/wfn/wfn_mask.F90:2-35). */
 int td1c_mask(td1c_ctx* ctx, double* d_wfn, void* stream);

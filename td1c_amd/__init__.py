@@ -58,6 +58,9 @@ def load_library():
     L.td1c_get_energies.restype = ctypes.c_int
     L.td1c_rk_prop.argtypes = [vp, ctypes.c_int, ctypes.c_double, vp, vp, ctypes.c_int, vp]
     L.td1c_rk_prop.restype = ctypes.c_int
+    L.td1c_propagate.argtypes = [vp, ctypes.POINTER(TD1CField), ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                 ctypes.c_int, vp, ctypes.c_int, vp]
+    L.td1c_propagate.restype = ctypes.c_int
     L.td1c_mask.argtypes = [vp, vp, vp]
     L.td1c_mask.restype = ctypes.c_int
     L.td1c_field_value.argtypes = [ctypes.POINTER(TD1CField), ctypes.c_double, vp]
@@ -232,4 +235,14 @@ class CRK:
         _check(load_library().td1c_rk_prop(hprod.h, self.rk_order, float(dtime), _ptr(lf),
                                            ctypes.c_void_p(wfn.data_ptr()), int(apply_mask),
                                            hprod._stream(stream)), "td1c_rk_prop")
+        return wfn
+
+    def propagate(self, hprod: HProd, wfn, time: float, dtime: float, nstep: int, field: Field,
+                  apply_mask: bool = True, stream=None):
+        """nstep steps from `time` with the stage fields evaluated on the
+        device; one captured step replayed per step (tdse_split2's loop)."""
+        cf = field.to_c()
+        _check(load_library().td1c_propagate(hprod.h, ctypes.byref(cf), self.rk_order, float(time), float(dtime),
+                                             int(nstep), ctypes.c_void_p(wfn.data_ptr()), int(apply_mask),
+                                             hprod._stream(stream)), "td1c_propagate")
         return wfn

@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "../../include/td1c_hprod.h"
+#include "field.hpp"
 #include "kernels.hpp"
 #include "setup.hpp"
 
@@ -91,7 +92,17 @@ struct td1c_ctx {
   double ene_fcore = 0.0;
   // RK work: w0, t, k[stages] over the full wave function
   DevBuf<double2> rk_w0, rk_t, rk_k;
+  DevBuf<double> lf_dev;  // field values of the stages, [kMaxStages][9]
+  DevBuf<PropClock> clock;
   bool wfn0_set = false;
+  // device-driven propagation: one captured RK step, replayed per step
+  hipStream_t cap_stream = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  bool capturing = false;
+  int g_order = 0, g_mask = 0;
+  double g_dt = 0.0;
+  const double* g_wfn = nullptr;
+  Pulse g_pulse{};
   // optional per-kernel timing with HIP events on the launch stream
   bool timing = false;
   std::vector<hipEvent_t> ev_pool;
@@ -103,7 +114,7 @@ struct td1c_ctx {
 
   ~td1c_ctx() {
     DevBuf<double>* d[] = {&eK0, &eK, &eP, &xr, &band_k, &band_k0, &band_p, &alph, &cost, &mask, &m_l2g, &m_g2l, &m_g2L, &m_L2g,
-                           &pG, &pCl, &pCr, &pWl, &pWr, &pSd, &pSl, &pSu, &invr, &rpl0, &rpl1, &fac2, &fac1, &ene};
+                           &pG, &pCl, &pCr, &pWl, &pWr, &pSd, &pSl, &pSu, &invr, &rpl0, &rpl1, &fac2, &fac1, &ene, &lf_dev};
     for (auto* b : d) b->release();
     DevBuf<double2>* z[] = {&tdiag, &tzor, &lsp, &gsp, &orbg0, &rhoL, &vL, &pwork, &vg, &p_cl, &p_cr, &p_mp, &p_Q,
                             &gram_partial, &lprod, &pprod, &den1, &den2, &rden, &rrden, &int1e, &int2e, &xmat,
@@ -117,6 +128,9 @@ struct td1c_ctx {
     DevBuf<const double2*>* g[] = {&g_left_l, &g_right_l, &g_left_p, &g_right_p};
     for (auto* b : g) b->release();
     for (auto e : ev_pool) (void)hipEventDestroy(e);
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (cap_stream) (void)hipStreamDestroy(cap_stream);
+    if (clock.p) (void)hipFree(clock.p);
   }
 };
 
@@ -151,7 +165,7 @@ struct Timed {
   const char* name;
   int e0 = -1;
   Timed(td1c_ctx* c_, hipStream_t s_, const char* n) : c(c_), s(s_), name(n) {
-    if (!c->timing) return;
+    if (!c->timing || c->capturing) return;
     if (!c->timing_only.empty() && c->timing_only != name) return;
     if (c->ev_next + 2 > (int)c->ev_pool.size() && harvest_timing(c) != hipSuccess) return;
     e0 = c->ev_next;
@@ -412,6 +426,8 @@ int build_device(td1c_ctx* c) {
   HIP_TRY(c->rk_w0.alloc(wsz));
   HIP_TRY(c->rk_t.alloc(wsz));
   HIP_TRY(c->rk_k.alloc(wsz * kMaxStages));
+  HIP_TRY(c->lf_dev.alloc(9 * kMaxStages));
+  HIP_TRY(c->clock.alloc(1));
   return 0;
 }
 
@@ -435,12 +451,26 @@ IslandArgs island_args(td1c_ctx* c, const double2* cic, double dtime) {
 }
 
 // Everything of hprod_htotx up to the X matrix and the CI derivative.
-int enqueue_htotx(td1c_ctx* c, double dtime, const double* lfield, const double2* wfn, const double2* cic,
-                  hipStream_t s) {
+// Runge-Kutta bookkeeping of the CI part of one stage, done by the island
+// kernel right after it forms dC (see IslandArgs).
+struct CiStage {
+  double2* kout = nullptr;
+  double2* acc = nullptr;
+  double dtw = 0.0;
+  double2* next = nullptr;
+  const double2* w0 = nullptr;
+  int nprev = 0;
+  const double2* kprev[kMaxStages] = {};
+  double cprev[kMaxStages] = {};
+  double cnew = 0.0;
+};
+
+// d_lf: the stage's lfield[9] in device memory.
+int enqueue_htotx(td1c_ctx* c, double dtime, const double* d_lf, const double2* wfn, const double2* cic,
+                  const CiStage& cs, hipStream_t s) {
   Setup& S = c->S;
   const int nr = S.nr, nf = S.nfun, nfc = S.nfcore, L1 = S.L1, L2 = S.L2, nl = S.nlat;
   const size_t nb = (size_t)c->nbas;
-  const double2 zfield = make_double2(lfield[2], 0.0), efield = make_double2(lfield[5], 0.0);
   // current dynamical orbitals (hprod_orbin): copied into the work array by
   // the l -> theta transform that reads them
   {
@@ -449,7 +479,7 @@ int enqueue_htotx(td1c_ctx* c, double dtime, const double* lfield, const double2
   }
   // frozen core in the velocity gauge (hprod_fcorbg): phase on the grid, back to l-space
   if (nfc > 0 && S.igauge == 1) {
-    { TIMED("fc_phase"); launch_fc_phase(c->orbg0.p, c->orbg, c->xr.p, c->cost.p, lfield[8], nr, S.nradfc, nl, nfc, s); }
+    { TIMED("fc_phase"); launch_fc_phase(c->orbg0.p, c->orbg, c->xr.p, c->cost.p, d_lf, nr, S.nradfc, nl, nfc, s); }
     { TIMED("theta2l_fc"); launch_transform(0, c->orbg, c->orb, c->m_g2l.p, c->b_g2l_fc.p, c->n_g2l_fc, nr, nl, L1, pad_cols(L1), nullptr, s); }
   }
   IslandArgs ia = island_args(c, cic, dtime);
@@ -459,13 +489,13 @@ int enqueue_htotx(td1c_ctx* c, double dtime, const double* lfield, const double2
   ob.nr = nr; ob.L1 = L1; ob.ndvr = S.n; ob.nfcore = nfc; ob.nradfc = S.nradfc;
   ob.mmax1 = S.mmax1; ob.igauge = S.igauge; ob.fc_full = S.iprojfc == 2;
   ob.mval = c->mval.p; ob.band_k = c->band_k.p; ob.band_k0 = c->band_k0.p; ob.band_p = c->band_p.p;
-  ob.xr = c->xr.p; ob.alph = c->alph.p; ob.tdiag = c->tdiag.p; ob.zfield = zfield; ob.efield = efield;
+  ob.xr = c->xr.p; ob.alph = c->alph.p; ob.tdiag = c->tdiag.p; ob.lf = d_lf;
   ob.f0 = 0; ob.f1 = nf;
   OneBodyElArgs oe{};
   oe.nr = nr; oe.nfe = S.nfe; oe.L1 = L1; oe.ndvr = S.n; oe.nfcore = nfc; oe.nradfc = S.nradfc; oe.nfun = nf;
   oe.mmax1 = S.mmax1; oe.igauge = S.igauge; oe.fc_full = S.iprojfc == 2; oe.mval = c->mval.p;
   oe.eK0 = c->eK0.p; oe.eK = c->eK.p; oe.eP = c->eP.p; oe.xr = c->xr.p; oe.alph = c->alph.p; oe.tzor = c->tzor.p;
-  oe.zfield = zfield; oe.efield = efield;
+  oe.lf = d_lf;
   {
     TIMED("onebody");
     if (!launch_onebody_el(c->orb, c->h01, oe, s)) launch_onebody(c->orb, c->h01, ob, s);
@@ -491,7 +521,7 @@ int enqueue_htotx(td1c_ctx* c, double dtime, const double* lfield, const double2
     { TIMED("mfprod"); launch_mfprod(c->orbg, c->vg.p, c->gorbg, c->v2orbg, ma, s); }
     { TIMED("theta2l_mf"); launch_transform(0, c->gorbg, c->gorb, c->m_g2l.p, c->b_g2l_mf.p, c->n_g2l_mf, nr, nl, L1, pad_cols(L1), nullptr, s); }
   } else {
-    HIP_TRY(hipMemsetAsync(c->gorb, 0, 2 * nb * c->nd * sizeof(double2), s));
+    launch_zero(c->gorb, 2 * nb * c->nd, s);
   }
   (void)ng;
   // inner products
@@ -505,6 +535,9 @@ int enqueue_htotx(td1c_ctx* c, double dtime, const double* lfield, const double2
     h.left = c->g_left_p.p; h.right = c->g_right_p.p;
     { TIMED("gram_pair"); launch_gram(h, c->gram_partial.p, c->gram_blk, c->pprod.p, s); }
   }
+  ia.kout = cs.kout; ia.acc = cs.acc; ia.dtw = cs.dtw; ia.next = cs.next; ia.w0 = cs.w0;
+  ia.nprev = cs.nprev; ia.cnew = cs.cnew;
+  for (int j = 0; j < kMaxStages; ++j) { ia.kprev[j] = cs.kprev[j]; ia.cprev[j] = cs.cprev[j]; }
   { TIMED("island_b"); launch_island_b(ia, s); }
   return 0;
 }
@@ -679,13 +712,14 @@ int td1c_htot(td1c_ctx* c, double dtime, const double* lfield, const double* d_w
   hipStream_t s = (hipStream_t)stream;
   const double2* wfn = reinterpret_cast<const double2*>(d_wfn);
   const double2* cic = reinterpret_cast<const double2*>(d_cic);
-  int rc = enqueue_htotx(c, dtime, lfield, wfn, cic, s);
+  HIP_TRY(hipMemcpyAsync(c->lf_dev.p, lfield, 9 * sizeof(double), hipMemcpyHostToDevice, s));
+  CiStage cs;
+  cs.kout = reinterpret_cast<double2*>(d_hcic);
+  int rc = enqueue_htotx(c, dtime, c->lf_dev.p, wfn, cic, cs, s);
   if (rc) return rc;
   DtorbArgs d = dtorb_args(c, dtime);
   d.kout = reinterpret_cast<double2*>(d_hwfn);
   { TIMED("dtorb"); launch_dtorb(d, s); }
-  if (d_hcic)
-    HIP_TRY(hipMemcpyAsync(d_hcic, c->dcic.p, c->S.lcic * sizeof(double2), hipMemcpyDeviceToDevice, s));
   HIP_TRY(hipGetLastError());
   return 0;
 }
@@ -726,25 +760,37 @@ static int rk_tableau(int order, int& ns, double* nodes, double* w, double (*a)[
   }
 }
 
-int td1c_rk_prop(td1c_ctx* c, int rk_order, double dtime, const double* lfield_stages, double* d_wfn, int apply_mask,
-                 void* stream) {
-  if (!c || !lfield_stages || !d_wfn) return fail(-1, "null argument");
-  if (c->S.nfcore > 0 && !c->wfn0_set) return fail(-4, "frozen core needs td1c_set_wfn0 first");
+// One Runge-Kutta step of the whole wave function on stream s, the stage
+// field values read from d_lf (device, [stage][9]).
+static int enqueue_rk_step(td1c_ctx* c, int rk_order, double dtime, const double* d_lf, double2* wfn, int apply_mask,
+                           hipStream_t s) {
   int ns;
   double nodes[kMaxStages], w[kMaxStages], a[kMaxStages][kMaxStages];
   if (rk_tableau(rk_order, ns, nodes, w, a)) return fail(-1, "rk_order must be 1..4");
-  hipStream_t s = (hipStream_t)stream;
   Setup& S = c->S;
   const size_t so = (size_t)c->nbas * S.nfun, sz = so + S.lcic;
-  double2* wfn = reinterpret_cast<double2*>(d_wfn);
   double2* w0 = c->rk_w0.p;
   double2* t = c->rk_t.p;
-  HIP_TRY(hipMemcpyAsync(w0, wfn, sz * sizeof(double2), hipMemcpyDeviceToDevice, s));
+  launch_copy(w0, wfn, sz, s);
   // stage input of stage 0 is w0 itself
   const double2* tin = w0;
   for (int is = 0; is < ns; ++is) {
     double2* k = c->rk_k.p + sz * is;
-    int rc = enqueue_htotx(c, 1.0, lfield_stages + 9 * is, tin, tin + so, s);
+    CiStage cs;
+    cs.kout = k + so;
+    cs.acc = wfn + so;
+    cs.dtw = dtime * w[is];
+    if (is + 1 < ns) {
+      cs.next = t + so;
+      cs.w0 = w0 + so;
+      cs.nprev = is;
+      for (int js = 0; js < is; ++js) {
+        cs.kprev[js] = c->rk_k.p + sz * js + so;
+        cs.cprev[js] = dtime * a[is + 1][js];
+      }
+      cs.cnew = dtime * a[is + 1][is];
+    }
+    int rc = enqueue_htotx(c, 1.0, d_lf + 9 * is, tin, tin + so, cs, s);
     if (rc) return rc;
     DtorbArgs d = dtorb_args(c, 1.0);
     d.kout = k;
@@ -762,103 +808,84 @@ int td1c_rk_prop(td1c_ctx* c, int rk_order, double dtime, const double* lfield_s
       d.cnew = dtime * a[is + 1][is];
     }
     { TIMED("dtorb_rk"); launch_dtorb(d, s); }
-    // CI part: k_cic = dcic; acc += dtw k; next = w0 + sum a k
-    HIP_TRY(hipMemcpyAsync(k + so, c->dcic.p, S.lcic * sizeof(double2), hipMemcpyDeviceToDevice, s));
-    AxpyArgs ax{};
-    ax.nterm = 1; ax.k[0] = k + so; ax.c[0] = dtime * w[is];
-    launch_axpy_list(wfn + so, nullptr, ax, S.lcic, s);
-    if (is + 1 < ns) {
-      AxpyArgs an{};
-      an.nterm = is + 1;
-      for (int js = 0; js <= is; ++js) {
-        an.k[js] = c->rk_k.p + sz * js + so;
-        an.c[js] = dtime * a[is + 1][js];
-      }
-      launch_axpy_list(t + so, w0 + so, an, S.lcic, s);
-      tin = t;
-    }
+    if (is + 1 < ns) tin = t;
   }
   HIP_TRY(hipGetLastError());
   return 0;
 }
 
-// ---------------------------------------------------------------------------
-// Laser field (clfield1, src/field/clfield1.cpp:14-500): one pulse starting at
-// t = 0, envelopes zero / sin2 / flat, gauges length1 / velocity1.
-// ---------------------------------------------------------------------------
-struct PulseConst {
-  double famp, freq, period, cep, tau, tleft, tright, numcyc;
-};
-
-static PulseConst pulse_const(const td1c_field* f) {
-  PulseConst p{};
-  p.cep = f->cep / 180.0 * kPi;
-  p.famp = std::sqrt(f->fint / 3.50944506E+16);
-  p.freq = 1239.84190 / (f->wlen * 27.2113845);
-  p.period = 2.0 * kPi / p.freq;
-  p.numcyc = f->numcyc;
-  p.tau = f->env_type == 0 ? 0.0 : p.numcyc * p.period * 0.5;
-  double delay = f->delay / 24.188843265;
-  // clfield::gen: tshift = -(delay - tau) for a single pulse
-  double tshift = -std::min(0.0, delay - p.tau);
-  double tcenter = tshift + delay;
-  p.tleft = tcenter - p.tau;
-  p.tright = tcenter + p.tau;
-  return p;
+int td1c_rk_prop(td1c_ctx* c, int rk_order, double dtime, const double* lfield_stages, double* d_wfn, int apply_mask,
+                 void* stream) {
+  if (!c || !lfield_stages || !d_wfn) return fail(-1, "null argument");
+  if (c->S.nfcore > 0 && !c->wfn0_set) return fail(-4, "frozen core needs td1c_set_wfn0 first");
+  int ns;
+  double nodes[kMaxStages], w[kMaxStages], a[kMaxStages][kMaxStages];
+  if (rk_tableau(rk_order, ns, nodes, w, a)) return fail(-1, "rk_order must be 1..4");
+  hipStream_t s = (hipStream_t)stream;
+  HIP_TRY(hipMemcpyAsync(c->lf_dev.p, lfield_stages, 9 * ns * sizeof(double), hipMemcpyHostToDevice, s));
+  return enqueue_rk_step(c, rk_order, dtime, c->lf_dev.p, reinterpret_cast<double2*>(d_wfn), apply_mask, s);
 }
 
-static double e_explicit(const td1c_field* f, const PulseConst& p, double t) {
-  double trel = t - p.tleft;
-  if (f->env_type == 2) return p.famp * std::sin(p.freq * trel + p.cep);
-  if (f->env_type == 1) {
-    if (t < p.tleft || t > p.tright) return 0.0;
-    double env = std::sin(kPi * trel / (p.numcyc * p.period));
-    return env * env * p.famp * std::sin(p.freq * trel + p.cep);
+int td1c_propagate(td1c_ctx* c, const td1c_field* field, int rk_order, double t0, double dtime, int nstep,
+                   double* d_wfn, int apply_mask, void* stream) {
+  if (!c || !field || !d_wfn) return fail(-1, "null argument");
+  if (c->S.nfcore > 0 && !c->wfn0_set) return fail(-4, "frozen core needs td1c_set_wfn0 first");
+  if (field->env_type < 0 || field->env_type > 2 || field->gauge < 0 || field->gauge > 1)
+    return fail(-1, "bad field");
+  if (nstep < 0) return fail(-1, "nstep must be >= 0");
+  int ns;
+  double nodes[kMaxStages], w[kMaxStages], a[kMaxStages][kMaxStages];
+  if (rk_tableau(rk_order, ns, nodes, w, a)) return fail(-1, "rk_order must be 1..4");
+  if (nstep == 0) return 0;
+  hipStream_t s = (hipStream_t)stream;
+  const Pulse p = make_pulse(field);
+  const bool same = c->gexec && c->g_order == rk_order && c->g_mask == apply_mask && c->g_wfn == d_wfn &&
+                    c->g_dt == dtime && std::memcmp(&c->g_pulse, &p, sizeof(Pulse)) == 0;
+  if (!same) {
+    if (c->gexec) {
+      HIP_TRY(hipGraphExecDestroy(c->gexec));
+      c->gexec = nullptr;
+    }
+    if (!c->cap_stream) HIP_TRY(hipStreamCreateWithFlags(&c->cap_stream, hipStreamNonBlocking));
+    StageNodes sn{};
+    sn.ns = ns;
+    for (int i = 0; i < ns; ++i) sn.c[i] = nodes[i];
+    HIP_TRY(hipStreamBeginCapture(c->cap_stream, hipStreamCaptureModeThreadLocal));
+    c->capturing = true;
+    launch_field_stages(p, c->clock.p, sn, c->lf_dev.p, c->cap_stream);
+    int rc = enqueue_rk_step(c, rk_order, dtime, c->lf_dev.p, reinterpret_cast<double2*>(d_wfn), apply_mask,
+                             c->cap_stream);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(c->cap_stream, &g);
+    c->capturing = false;
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return fail(-3, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      c->gexec = nullptr;
+      return fail(-3, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+    c->g_order = rk_order;
+    c->g_mask = apply_mask;
+    c->g_wfn = d_wfn;
+    c->g_dt = dtime;
+    c->g_pulse = p;
   }
-  return 0.0;
-}
-
-static double a_explicit(const td1c_field* f, const PulseConst& p, double t) {
-  double trel = t - p.tleft;
-  if (f->env_type == 2) return p.famp / p.freq * std::sin(p.freq * trel + p.cep);
-  if (f->env_type == 1) {
-    if (t < p.tleft || t > p.tright) return 0.0;
-    double env = std::sin(kPi * trel / (p.numcyc * p.period));
-    return env * env * p.famp / p.freq * std::sin(p.freq * trel + p.cep);
-  }
-  return 0.0;
-}
-
-// E = -dA/dt by the 6-point central difference of clfield1::get_evalue_numder
-static double e_from_a(const td1c_field* f, const PulseConst& p, double t) {
-  const double h = 2.0 * kPi / (1239.84190 / (800.0 * 27.2113845)) / 100000.0;
-  double s = 0.0;
-  s += a_explicit(f, p, t + 3 * h) / 60.0;
-  s -= a_explicit(f, p, t + 2 * h) / 20.0 * 3.0;
-  s += a_explicit(f, p, t + h) / 4.0 * 3.0;
-  s -= a_explicit(f, p, t - h) / 4.0 * 3.0;
-  s += a_explicit(f, p, t - 2 * h) / 20.0 * 3.0;
-  s -= a_explicit(f, p, t - 3 * h) / 60.0;
-  return -s / h;
+  launch_clock_set(c->clock.p, t0, dtime, s);
+  for (int i = 0; i < nstep; ++i) HIP_TRY(hipGraphLaunch(c->gexec, s));
+  HIP_TRY(hipGetLastError());
+  return 0;
 }
 
 int td1c_field_value(const td1c_field* f, double t, double lfield[9]) {
   if (!f || !lfield) return fail(-1, "null argument");
   for (int i = 0; i < 9; ++i) lfield[i] = 0.0;
   if (f->env_type < 0 || f->env_type > 2 || f->gauge < 0 || f->gauge > 1) return fail(-1, "bad field");
-  if (f->env_type == 0) return 0;
-  PulseConst p = pulse_const(f);
-  if (!(t > p.tleft && t < p.tright)) return 0;
-  if (f->gauge == 0) {
-    lfield[2] = e_explicit(f, p, t);
-    lfield[5] = e_explicit(f, p, t);
-    // A in the length gauge is a running integral in the reference
-    // (get_avalue_numint); it is not used by the hprod and is left 0 here.
-  } else {
-    lfield[2] = a_explicit(f, p, t);
-    lfield[5] = e_from_a(f, p, t);
-    lfield[8] = a_explicit(f, p, t);
-  }
+  pulse_lfield(make_pulse(f), t, lfield);
   return 0;
 }
 

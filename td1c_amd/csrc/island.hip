@@ -361,6 +361,19 @@ __global__ void __launch_bounds__(kT) k_island_b(const IslandArgs a) {
   } else {
     for (int d = tid; d < a.lcic; d += kT) a.dcic[d] = zero2();
   }
+  if (a.kout != nullptr) {
+    __syncthreads();
+    for (int d = tid; d < a.lcic; d += kT) {
+      const double2 k = a.dcic[d];
+      a.kout[d] = k;
+      if (a.acc != nullptr) a.acc[d] = zadd(a.acc[d], zscal(k, a.dtw));
+      if (a.next != nullptr) {
+        double2 t = a.w0[d];
+        for (int j = 0; j < a.nprev; ++j) t = zadd(t, zscal(a.kprev[j][d], a.cprev[j]));
+        a.next[d] = zadd(t, zscal(k, a.cnew));
+      }
+    }
+  }
 }
 
 void launch_island_a(const IslandArgs& a, hipStream_t s) { hipLaunchKernelGGL(k_island_a, dim3(1), dim3(kT), 0, s, a); }

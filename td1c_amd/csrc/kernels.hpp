@@ -6,6 +6,8 @@
 
 #include <cstddef>
 
+#include "field.hpp"
+
 namespace td1c {
 
 constexpr int kMaxFun = 16;       // orbitals handled by the theta-grid mean-field kernel
@@ -13,6 +15,18 @@ constexpr int kMaxAct = 8;        // active orbitals handled on the theta grid
 constexpr int kMaxActPairs = kMaxAct * kMaxAct;
 constexpr int kMaxStages = 6;
 constexpr int kMaxBridge = 160;   // nfe - 1 bridges handled by the Poisson bridge kernel
+
+// Time of a device-driven propagation: step n runs at t0 + n * dt.
+struct PropClock {
+  double t0, dt;
+  long long n;
+};
+
+// Runge-Kutta stage nodes c_s.
+struct StageNodes {
+  int ns;
+  double c[kMaxStages];
+};
 
 struct TransformBatch {
   long long in0;   // element offset of the input block (MODE 1: orbital j)
@@ -31,7 +45,7 @@ struct OneBodyArgs {
   const double* xr;       // [nr]
   const double* alph;     // [2mmax1+1][L1]
   const double2* tdiag;   // [L1][nr]
-  double2 zfield, efield;
+  const double* lf;       // device lfield[9]: [2] gauge field, [5] E (frozen-core length term)
 };
 
 struct OneBodyElArgs {
@@ -41,7 +55,7 @@ struct OneBodyElArgs {
   const double* xr;
   const double* alph;
   const double2* tzor;          // [L1][nr] tmat diagonal without the kinetic part (+ v2jfc)
-  double2 zfield, efield;
+  const double* lf;             // device lfield[9]
 };
 
 struct PoissonArgs {
@@ -86,12 +100,6 @@ struct DtorbArgs {
   double cnew;
 };
 
-struct AxpyArgs {
-  int nterm;
-  const double2* k[kMaxStages];
-  double c[kMaxStages];
-};
-
 struct IslandArgs {
   int nfun, nfcore, ncore, nact, lcic, nelact2, icomp;
   const int* mval;
@@ -111,6 +119,17 @@ struct IslandArgs {
   // stage B outputs
   double2 *int1e, *int2e, *xmat, *dcic;
   double* ene;             // ene_fcore (input), dcore, core, act, tot
+  // Runge-Kutta stage update of the CI part, done after dcic (kout == null: none):
+  // kout = dcic; acc += dtw * dcic; next = w0 + sum cprev[j] kprev[j] + cnew * dcic
+  double2* kout;
+  double2* acc;
+  double dtw;
+  double2* next;
+  const double2* w0;
+  int nprev;
+  const double2* kprev[kMaxStages];
+  double cprev[kMaxStages];
+  double cnew;
 };
 
 void launch_transform(int mode, const double2* in, double2* out, const double* mats, const TransformBatch* batch,
@@ -121,12 +140,15 @@ bool launch_onebody_el(const double2* orb, double2* h01, const OneBodyElArgs& a,
 void launch_poisson(const double2* rho, double2* work, double2* v, const PoissonArgs& a, hipStream_t s);
 void launch_mfprod(const double2* orbg, const double2* vg, double2* gorbg, double2* v2orbg, const MfArgs& a,
                    hipStream_t s);
-void launch_fc_phase(const double2* orbg0, double2* orbg, const double* xr, const double* cost, double afield, int nr,
+void launch_fc_phase(const double2* orbg0, double2* orbg, const double* xr, const double* cost, const double* lf, int nr,
                      int nradfc, int nlat, int nfc, hipStream_t s);
 void launch_gram(const GramArgs& a, double2* partial, int nblk, double2* out, hipStream_t s);
+void launch_copy(double2* y, const double2* x, size_t n, hipStream_t s);
+void launch_zero(double2* y, size_t n, hipStream_t s);
+void launch_field_stages(const Pulse& p, PropClock* st, const StageNodes& nodes, double* lf, hipStream_t s);
+void launch_clock_set(PropClock* st, double t0, double dt, hipStream_t s);
 void launch_dtorb(const DtorbArgs& a, hipStream_t s);
 void launch_mask(double2* wfn, const double* mask, int nr, int L1, int nfcore, int nfun, hipStream_t s);
-void launch_axpy_list(double2* y, const double2* x0, const AxpyArgs& a, int n, hipStream_t s);
 void launch_island_a(const IslandArgs& a, hipStream_t s);
 void launch_island_b(const IslandArgs& a, hipStream_t s);
 
