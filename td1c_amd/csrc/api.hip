@@ -67,7 +67,7 @@ struct td1c_ctx {
   DevBuf<double2> tdiag, tzor;
   DevBuf<double> eK0, eK, eP;
   DevBuf<double> m_l2g, m_g2l, m_g2L, m_L2g;
-  DevBuf<double> pG, pCl, pCr, pWl, pWr, pSd, pSl, pSu, invr, rpl0, rpl1, fac2, fac1;
+  DevBuf<double> pG, pWl, pWr, pSd, pSl, pSu, invr, rpl0, rpl1, fac2, fac1;
   DevBuf<int> mval, pair_m, pj, pi, ptype;
   // batches
   DevBuf<TransformBatch> b_l2g_dyn, b_g2l_fc, b_rho, b_L2g, b_g2l_mf;
@@ -78,8 +78,8 @@ struct td1c_ctx {
   DevBuf<double2> gsp;       // orbg | gorbg | v2orbg
   double2 *orbg = nullptr, *gorbg = nullptr, *v2orbg = nullptr;
   DevBuf<double2> orbg0;     // frozen-core theta-grid orbitals at A = 0
-  DevBuf<double2> rhoL, vL, pwork, vg;
-  DevBuf<double2> p_cl, p_cr, p_mp, p_Q;
+  DevBuf<double2> rhoL, vL, vg;
+  DevBuf<double2> p_xb, p_Q;  // Poisson bridge solutions [pair][L][nfe-1], multipole moments [pair][L]
   // Gram
   DevBuf<const double2*> g_left_l, g_right_l, g_left_p, g_right_p;
   int gl_nL = 0, gl_nR = 0, gp_nL = 0, gp_nR = 0, gram_blk = 0;
@@ -114,9 +114,9 @@ struct td1c_ctx {
 
   ~td1c_ctx() {
     DevBuf<double>* d[] = {&eK0, &eK, &eP, &xr, &band_k, &band_k0, &band_p, &alph, &cost, &mask, &m_l2g, &m_g2l, &m_g2L, &m_L2g,
-                           &pG, &pCl, &pCr, &pWl, &pWr, &pSd, &pSl, &pSu, &invr, &rpl0, &rpl1, &fac2, &fac1, &ene, &lf_dev};
+                           &pG, &pWl, &pWr, &pSd, &pSl, &pSu, &invr, &rpl0, &rpl1, &fac2, &fac1, &ene, &lf_dev};
     for (auto* b : d) b->release();
-    DevBuf<double2>* z[] = {&tdiag, &tzor, &lsp, &gsp, &orbg0, &rhoL, &vL, &pwork, &vg, &p_cl, &p_cr, &p_mp, &p_Q,
+    DevBuf<double2>* z[] = {&tdiag, &tzor, &lsp, &gsp, &orbg0, &rhoL, &vL, &vg, &p_xb, &p_Q,
                             &gram_partial, &lprod, &pprod, &den1, &den2, &rden, &rrden, &int1e, &int2e, &xmat,
                             &dcic, &ci_work, &rk_w0, &rk_t, &rk_k};
     for (auto* b : z) b->release();
@@ -232,8 +232,6 @@ int build_device(td1c_ctx* c) {
   HIP_TRY(c->m_g2L.upload(padded_cols(S.mat_g2L, S.nlat, S.L2)));
   HIP_TRY(c->m_L2g.upload(padded_cols(S.mat_L2g, S.L2, S.nlat)));
   HIP_TRY(c->pG.upload(S.pG));
-  HIP_TRY(c->pCl.upload(S.pCl));
-  HIP_TRY(c->pCr.upload(S.pCr));
   HIP_TRY(c->pWl.upload(S.pWl));
   HIP_TRY(c->pWr.upload(S.pWr));
   HIP_TRY(c->pSd.upload(S.pSd));
@@ -276,14 +274,9 @@ int build_device(td1c_ctx* c) {
   const size_t np = std::max(1, c->npair);
   HIP_TRY(c->rhoL.alloc(np * L2 * nr));
   HIP_TRY(c->vL.alloc(np * L2 * nr));
-  HIP_TRY(c->pwork.alloc(np * L2 * nr));
   HIP_TRY(c->vg.alloc(np * nl * nr));
-  HIP_TRY(c->p_cl.alloc(np * L2 * S.nfe));
-  HIP_TRY(c->p_cr.alloc(np * L2 * S.nfe));
-  HIP_TRY(c->p_mp.alloc(np * L2 * S.nfe));
+  HIP_TRY(c->p_xb.alloc(np * L2 * std::max(1, S.nfe - 1)));
   HIP_TRY(c->p_Q.alloc(np * L2));
-  HIP_TRY(hipMemset(c->p_cl.p, 0, c->p_cl.n * sizeof(double2)));
-  HIP_TRY(hipMemset(c->p_cr.p, 0, c->p_cr.n * sizeof(double2)));
 
   // transform batches
   {
@@ -447,6 +440,7 @@ IslandArgs island_args(td1c_ctx* c, const double2* cic, double dtime) {
   a.pprod = c->pprod.p; a.map2e = c->map2e.p;
   a.int1e = c->int1e.p; a.int2e = c->int2e.p; a.xmat = c->xmat.p; a.dcic = c->dcic.p;
   a.ene = c->ene.p;
+  island_lds_plan(a);
   return a;
 }
 
@@ -507,11 +501,11 @@ int enqueue_htotx(td1c_ctx* c, double dtime, const double* d_lf, const double2* 
     PoissonArgs pa{};
     pa.nr = nr; pa.n = S.n; pa.nfe = S.nfe; pa.ni = S.ni; pa.L2 = L2; pa.npair = c->npair;
     pa.pair_m = c->pair_m.p;
-    pa.G = c->pG.p; pa.Cl = c->pCl.p; pa.Cr = c->pCr.p; pa.Wl = c->pWl.p; pa.Wr = c->pWr.p;
+    pa.G = c->pG.p; pa.Wl = c->pWl.p; pa.Wr = c->pWr.p;
     pa.Sd = c->pSd.p; pa.Sl = c->pSl.p; pa.Su = c->pSu.p;
     pa.invr = c->invr.p; pa.rpl0 = c->rpl0.p; pa.rpl1 = c->rpl1.p; pa.fac2 = c->fac2.p; pa.fac1 = c->fac1.p;
-    pa.cl = c->p_cl.p; pa.cr = c->p_cr.p; pa.mpart = c->p_mp.p; pa.Q = c->p_Q.p;
-    { TIMED("poisson"); launch_poisson(c->rhoL.p, c->pwork.p, c->vL.p, pa, s); }
+    pa.Q = c->p_Q.p;
+    { TIMED("poisson"); launch_poisson(c->rhoL.p, c->p_xb.p, c->vL.p, pa, s); }
     { TIMED("pair_L2theta"); launch_transform(0, c->vL.p, c->vg.p, c->m_L2g.p, c->b_L2g.p, c->npair, nr, L2, nl, pad_cols(nl), nullptr, s); }
     MfArgs ma{};
     ma.nr = nr; ma.nlat = nl; ma.nfun = nf; ma.nfcore = nfc; ma.nfcore2 = S.nfcore2; ma.ncore = S.ncore; ma.nact = S.nact;

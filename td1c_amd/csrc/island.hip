@@ -26,20 +26,68 @@ __device__ __forceinline__ double2 zscal(double2 a, double s) { return make_doub
 __device__ __forceinline__ double zabs(double2 a) { return sqrt(a.x * a.x + a.y * a.y); }
 __device__ __forceinline__ double2 zero2() { return make_double2(0.0, 0.0); }
 
-// y[full] = E_pq x[full] (overwrites targets; E_pq is injective on strings)
-// for the determinants handled by this thread, stride kT.  y must be zeroed.
-__device__ void apply_E_par(const IslandArgs& a, int p, int q, const double2* x, double2* y, bool accumulate) {
+// Where the CI work lives: LDS when it fits, else the global work buffer.
+struct CiView {
+  double2 *C, *D, *S, *T;       // C[nc], D[n*n][nc] (D_pq = E_pq C), S[nc], T[nc]
+  const int *exA, *sgA, *exB, *sgB;
+};
+
+// Bytes of dynamic LDS a CI view needs (0: it does not fit, use global).
+__host__ __device__ inline size_t ci_lds_bytes(int n, int nstr_a, int nstr_b) {
+  const size_t nc = (size_t)nstr_a * nstr_b;
+  const size_t z = nc * ((size_t)n * n + 3) * sizeof(double2);
+  const size_t t = (size_t)2 * (nstr_a + nstr_b) * n * n * sizeof(int);
+  const size_t b = z + t;
+  return b <= kIslandLds ? b : 0;
+}
+
+// Sets up the view; with LDS, copies the string tables in (caller syncs).
+__device__ CiView ci_view(const IslandArgs& a, unsigned char* dyn) {
+  const int n = a.nact, na = a.nstr_a, nb = a.nstr_b;
+  const size_t nc = (size_t)na * nb, nt = (size_t)n * n;
+  CiView v;
+  if (a.ci_lds) {
+    v.D = reinterpret_cast<double2*>(dyn);
+    v.C = v.D + nt * nc;
+    v.S = v.C + nc;
+    v.T = v.S + nc;
+    int* ti = reinterpret_cast<int*>(v.T + nc);
+    int* eA = ti;
+    int* gA = eA + na * nt;
+    int* eB = gA + na * nt;
+    int* gB = eB + nb * nt;
+    for (size_t q = threadIdx.x; q < na * nt; q += kT) {
+      eA[q] = a.exA[q];
+      gA[q] = a.sgA[q];
+    }
+    for (size_t q = threadIdx.x; q < nb * nt; q += kT) {
+      eB[q] = a.exB[q];
+      gB[q] = a.sgB[q];
+    }
+    v.exA = eA; v.sgA = gA; v.exB = eB; v.sgB = gB;
+  } else {
+    v.D = a.ci_work;
+    v.C = v.D + nt * nc;
+    v.S = v.C + nc;
+    v.T = v.S + nc;
+    v.exA = a.exA; v.sgA = a.sgA; v.exB = a.exB; v.sgB = a.sgB;
+  }
+  return v;
+}
+
+// y[full] += E_pq x[full] for the determinants handled by this thread
+// (stride kT); distinct sources map to distinct targets, alpha then beta.
+__device__ void apply_E_par(const IslandArgs& a, const CiView& v, int p, int q, const double2* x, double2* y) {
   const int na = a.nstr_a, nb = a.nstr_b, n = a.nact;
   const int nc = na * nb;
-  // alpha part: target (ib, ta) from (ib, ia); distinct sources -> distinct targets
   for (int f = threadIdx.x; f < nc; f += kT) {
     const int ib = f / na, ia = f % na;
     const double2 xv = x[f];
     if (xv.x == 0.0 && xv.y == 0.0) continue;
-    const int t = a.exA[(ia * n + p) * n + q];
+    const int t = v.exA[(ia * n + p) * n + q];
     if (t >= 0) {
-      const double s = (double)a.sgA[(ia * n + p) * n + q];
-      y[ib * na + t] = zadd(y[ib * na + t], zscal(xv, s));
+      const double sg = (double)v.sgA[(ia * n + p) * n + q];
+      y[ib * na + t] = zadd(y[ib * na + t], zscal(xv, sg));
     }
   }
   __syncthreads();
@@ -47,14 +95,13 @@ __device__ void apply_E_par(const IslandArgs& a, int p, int q, const double2* x,
     const int ib = f / na, ia = f % na;
     const double2 xv = x[f];
     if (xv.x == 0.0 && xv.y == 0.0) continue;
-    const int t = a.exB[(ib * n + p) * n + q];
+    const int t = v.exB[(ib * n + p) * n + q];
     if (t >= 0) {
-      const double s = (double)a.sgB[(ib * n + p) * n + q];
-      y[t * na + ia] = zadd(y[t * na + ia], zscal(xv, s));
+      const double sg = (double)v.sgB[(ib * n + p) * n + q];
+      y[t * na + ia] = zadd(y[t * na + ia], zscal(xv, sg));
     }
   }
   __syncthreads();
-  (void)accumulate;
 }
 
 // Hermitian eigen-decomposition by cyclic complex Jacobi rotations
@@ -120,24 +167,26 @@ __device__ void invden_inv(int n, double thresh, const double2* a, double2* inva
 // den2(a,b,c,d) = <E_ba E_dc> - delta_ad den1(c,b)), rden, rrden
 // (hprod_invden, src/hprod/hprod_invden.F90:2-60).
 __global__ void __launch_bounds__(kT) k_island_a(const IslandArgs a) {
+  extern __shared__ __align__(16) unsigned char dyn[];
   __shared__ double2 s_d1[kN2], s_A[kN2], s_U[kN2], s_T[kN2], s_out[kN2];
   __shared__ double s_w[kMaxAct];
   const int n = a.nact;
   if (n == 0) return;
   const int nc = a.nstr_a * a.nstr_b;
-  double2* D = a.ci_work;
-  double2* C = D + (size_t)n * n * nc;
+  const CiView v = ci_view(a, dyn);
+  double2* D = v.D;
+  double2* C = v.C;
   for (int f = threadIdx.x; f < nc; f += kT) C[f] = a.det_of[f] >= 0 ? a.cic[a.det_of[f]] : zero2();
   for (size_t q = threadIdx.x; q < (size_t)n * n * nc; q += kT) D[q] = zero2();
   __syncthreads();
   for (int p = 0; p < n; ++p)
-    for (int q = 0; q < n; ++q) apply_E_par(a, p, q, C, D + (size_t)(p * n + q) * nc, true);
+    for (int q = 0; q < n; ++q) apply_E_par(a, v, p, q, C, D + (size_t)(p * n + q) * nc);
   // den1(x,y) = sum conj(C) (E_yx C)
   for (int e = threadIdx.x; e < n * n; e += kT) {
     const int x = e % n, y = e / n;
-    const double2* v = D + (size_t)(y * n + x) * nc;
+    const double2* dv = D + (size_t)(y * n + x) * nc;
     double2 s = zero2();
-    for (int d = 0; d < nc; ++d) s = zadd(s, zmul(zconj(C[d]), v[d]));
+    for (int d = 0; d < nc; ++d) s = zadd(s, zmul(zconj(C[d]), dv[d]));
     s_d1[e] = s;
     a.den1[e] = s;
   }
@@ -145,12 +194,15 @@ __global__ void __launch_bounds__(kT) k_island_a(const IslandArgs a) {
   for (int e = threadIdx.x; e < n * n * n * n; e += kT) {
     const int p = e % n, q = (e / n) % n, r = (e / (n * n)) % n, t = e / (n * n * n);
     const double2* u = D + (size_t)(p * n + q) * nc;
-    const double2* v = D + (size_t)(t * n + r) * nc;
+    const double2* w = D + (size_t)(t * n + r) * nc;
     double2 s = zero2();
-    for (int k = 0; k < nc; ++k) s = zadd(s, zmul(zconj(u[k]), v[k]));
+    for (int k = 0; k < nc; ++k) s = zadd(s, zmul(zconj(u[k]), w[k]));
     if (p == t) s = zsub(s, s_d1[r + n * q]);
     a.den2[e] = s;
   }
+  // D goes to the global work buffer for stage B
+  if (a.ci_lds)
+    for (size_t q = threadIdx.x; q < (size_t)n * n * nc; q += kT) a.ci_work[q] = D[q];
   if (threadIdx.x == 0) {
     invden_inv(n, a.throcc1, s_d1, s_out, s_A, s_U, s_w);
     for (int i = 0; i < n * n; ++i) a.rden[i] = s_out[i];
@@ -172,7 +224,9 @@ __global__ void __launch_bounds__(kT) k_island_a(const IslandArgs a) {
 // src/hprod/hprod_htot_mkxmat.F90) and the CI derivative (ormas_xact2 with one
 // ORMAS subspace, src/ormas/ormas_xact2.F90:2-102).
 __global__ void __launch_bounds__(kT) k_island_b(const IslandArgs a) {
+  extern __shared__ __align__(16) unsigned char dyn[];
   __shared__ double2 s_H[kF2], s_G[kF2], s_V[kF2], s_F[kF2], s_X[kF2], s_FM[kF2];
+  __shared__ double2 s_hp[kN2];
   __shared__ double2 s_i1[kN2], s_d1[kN2], s_rr[kN2];
   __shared__ double s_red[kT];
   __shared__ double s_eact;
@@ -210,7 +264,10 @@ __global__ void __launch_bounds__(kT) k_island_b(const IslandArgs a) {
     s_i1[e] = v;
     a.int1e[e] = v;
   }
-  // int2e(i,j,k,l): i >= j from the products, the rest by conjugate symmetry
+  // int2e(i,j,k,l): i >= j from the products, the rest by conjugate symmetry;
+  // kept in LDS (after the CI view) when it fits
+  double2* I2 = a.int2e;
+  if (a.i2_lds) I2 = reinterpret_cast<double2*>(dyn + a.ci_lds_off);
   for (int e = tid; e < na * na * na * na; e += kT) {
     const int ia = e % na, ja = (e / na) % na, ka = (e / (na * na)) % na, la = e / (na * na * na);
     int src = e;
@@ -222,14 +279,15 @@ __global__ void __launch_bounds__(kT) k_island_b(const IslandArgs a) {
     const int q = a.map2e[src];
     double2 v = q >= 0 ? a.pprod[q] : zero2();
     if (cj) v = zconj(v);
-    a.int2e[e] = v;
+    I2[e] = v;
+    if (a.i2_lds) a.int2e[e] = v;
   }
   __syncthreads();
   // energies: ene_act = Re sum int1e(i,j) den1(j,i) + 1/2 Re sum int2e(i,j,k,l) den2(j,i,l,k)
   double part = 0.0;
   for (int e = tid; e < na * na * na * na; e += kT) {
     const int ia = e % na, ja = (e / na) % na, ka = (e / (na * na)) % na, la = e / (na * na * na);
-    part += 0.5 * zmul(a.int2e[e], a.den2[ja + na * (ia + na * (la + na * ka))]).x;
+    part += 0.5 * zmul(I2[e], a.den2[ja + na * (ia + na * (la + na * ka))]).x;
   }
   for (int e = tid; e < na * na; e += kT) {
     const int ia = e % na, ja = e / na;
@@ -312,30 +370,32 @@ __global__ void __launch_bounds__(kT) k_island_b(const IslandArgs a) {
 #undef V
 #undef F
 #undef HGV
-  // CI derivative: dC = fac (H_CI - E_act) C
+  // CI derivative: dC = fac (H_CI - E_act) C, with D_pq = E_pq C from stage A
   if (na > 0) {
     const int nc = a.nstr_a * a.nstr_b;
-    double2* D = a.ci_work;
-    double2* C = D + (size_t)na * na * nc;
-    double2* S = C + nc;
-    double2* T = S + nc;
+    const CiView v = ci_view(a, dyn);
+    double2* D = v.D;
+    double2* C = v.C;
+    double2* S = v.S;
+    double2* T = v.T;
+    if (a.ci_lds)
+      for (size_t q = tid; q < (size_t)na * na * nc; q += kT) D[q] = a.ci_work[q];
     for (int f = tid; f < nc; f += kT) {
       C[f] = a.det_of[f] >= 0 ? a.cic[a.det_of[f]] : zero2();
       S[f] = zero2();
     }
-    for (size_t q = tid; q < (size_t)na * na * nc; q += kT) D[q] = zero2();
+    // h'_pq = int1e(p,q) - 1/2 sum_r int2e(p,r,r,q)
+    for (int e = tid; e < na * na; e += kT) {
+      const int p = e % na, q = e / na;
+      double2 hpq = s_i1[p + na * q];
+      for (int r = 0; r < na; ++r) hpq = zsub(hpq, zscal(I2[p + na * (r + na * (r + na * q))], 0.5));
+      s_hp[e] = hpq;
+    }
     __syncthreads();
-    for (int r = 0; r < na; ++r)
-      for (int s = 0; s < na; ++s) apply_E_par(a, r, s, C, D + (size_t)(r * na + s) * nc, true);
-    // one-body part with h'_pq = int1e(p,q) - 1/2 sum_r int2e(p,r,r,q)
     for (int f = tid; f < nc; f += kT) {
       double2 acc = zero2();
       for (int p = 0; p < na; ++p)
-        for (int q = 0; q < na; ++q) {
-          double2 hpq = s_i1[p + na * q];
-          for (int r = 0; r < na; ++r) hpq = zsub(hpq, zscal(a.int2e[p + na * (r + na * (r + na * q))], 0.5));
-          acc = zadd(acc, zmul(hpq, D[(size_t)(p * na + q) * nc + f]));
-        }
+        for (int q = 0; q < na; ++q) acc = zadd(acc, zmul(s_hp[p + na * q], D[(size_t)(p * na + q) * nc + f]));
       S[f] = acc;
     }
     __syncthreads();
@@ -345,18 +405,18 @@ __global__ void __launch_bounds__(kT) k_island_b(const IslandArgs a) {
           double2 t = zero2();
           for (int r = 0; r < na; ++r)
             for (int s = 0; s < na; ++s)
-              t = zadd(t, zmul(zscal(a.int2e[p + na * (q + na * (r + na * s))], 0.5), D[(size_t)(r * na + s) * nc + f]));
+              t = zadd(t, zmul(zscal(I2[p + na * (q + na * (r + na * s))], 0.5), D[(size_t)(r * na + s) * nc + f]));
           T[f] = t;
         }
         __syncthreads();
-        apply_E_par(a, p, q, T, S, true);
+        apply_E_par(a, v, p, q, T, S);
       }
     const double2 fac = a.icomp == 1 ? make_double2(0.0, -a.dtime) : make_double2(-a.dtime, 0.0);
     const double eact = s_eact;
     for (int f = tid; f < nc; f += kT) {
       int d = a.det_of[f];
       if (d < 0) continue;
-      a.dcic[d] = zmul(zsub(S[f], zscal(a.cic[d], eact)), fac);
+      a.dcic[d] = zmul(zsub(S[f], zscal(C[f], eact)), fac);
     }
   } else {
     for (int d = tid; d < a.lcic; d += kT) a.dcic[d] = zero2();
@@ -376,7 +436,27 @@ __global__ void __launch_bounds__(kT) k_island_b(const IslandArgs a) {
   }
 }
 
-void launch_island_a(const IslandArgs& a, hipStream_t s) { hipLaunchKernelGGL(k_island_a, dim3(1), dim3(kT), 0, s, a); }
-void launch_island_b(const IslandArgs& a, hipStream_t s) { hipLaunchKernelGGL(k_island_b, dim3(1), dim3(kT), 0, s, a); }
+void island_lds_plan(IslandArgs& a) {
+  a.ci_lds = 0;
+  a.i2_lds = 0;
+  a.ci_lds_off = 0;
+  a.lds_bytes = 0;
+  if (a.nact == 0) return;
+  const size_t ci = ci_lds_bytes(a.nact, a.nstr_a, a.nstr_b);
+  if (ci > 0) {
+    a.ci_lds = 1;
+    a.ci_lds_off = (ci + 15) / 16 * 16;
+  }
+  const size_t i2 = (size_t)a.nact * a.nact * a.nact * a.nact * sizeof(double2);
+  if (a.ci_lds_off + i2 <= kIslandLds) a.i2_lds = 1;
+  a.lds_bytes = a.ci_lds_off + (a.i2_lds ? i2 : 0);
+}
+
+void launch_island_a(const IslandArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_island_a, dim3(1), dim3(kT), a.ci_lds ? a.ci_lds_off : 0, s, a);
+}
+void launch_island_b(const IslandArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_island_b, dim3(1), dim3(kT), a.lds_bytes, s, a);
+}
 
 }  // namespace td1c

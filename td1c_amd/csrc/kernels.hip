@@ -202,8 +202,10 @@ __global__ void __launch_bounds__(256) k_onebody_el(const double2* __restrict__ 
   constexpr int TILE = (GE + 1) * N + 1;     // points of elements e0-1 .. e0+GE-1
   __shared__ double2 sPsi[3][TILE];
   __shared__ double2 sT[256], sP1[256], sP0[256];
-  const int f = blockIdx.y;
+  const int f = blockIdx.z;
   const int e0 = blockIdx.x * GE;
+  const int l0 = blockIdx.y * kOneBodyLChunk;
+  const int l1 = min(a.L1, l0 + kOneBodyLChunk);
   const int tid = threadIdx.x;
   // lane -> (element, local row)
   int e, m;
@@ -245,9 +247,9 @@ __global__ void __launch_bounds__(256) k_onebody_el(const double2* __restrict__ 
       sPsi[buf][q] = v;
     }
   };
-  load_tile(0, 0);
-  load_tile(-1, 2);
-  for (int l = 0; l < a.L1; ++l) {
+  load_tile(l0, l0 % 3);
+  load_tile(l0 - 1, (l0 + 2) % 3);
+  for (int l = l0; l < l1; ++l) {
     const int b0 = l % 3, b1 = (l + 1) % 3, bm = (l + 2) % 3;
     __syncthreads();
     load_tile(l + 1, b1);
@@ -331,178 +333,192 @@ __global__ void __launch_bounds__(256) k_onebody_el(const double2* __restrict__ 
 // ---------------------------------------------------------------------------
 // Radial Poisson solve per (pair, L) (hprod_mkv2mf_poissonp,
 // src/hprod/hprod_mkv2mf.F90:118-182) by element Schur complement of the
-// same FEDVR operator A_L = 2K + L(L+1)/r^2.
-// P1: block per (element, L); the block's threads form groups of GS lanes,
-// one group per pair, lane i < ni owning interior row i; G_e (the inverse of
-// the element's interior block) sits in LDS and serves every pair.
-//   y = G_e b_I, contributions Cl.y / Cr.y to the bridge equations and the
-//   element's part of the multipole moment sum rho r^L.
+// same FEDVR operator A_L = 2K + L(L+1)/r^2, b = rho * d2invr.
+// Element e has interior block A_e (inverse G_e), coupling rows C_l / C_r to
+// its left / right bridge, and W_l = G_e C_l^T, W_r = G_e C_r^T.  The bridge
+// unknowns solve the tridiagonal Schur system
+//   S x_B = b_B - sum_e (W_r(e-side) . b_I(e)) - (W_l . b_I(e+1)),
+// and the interiors follow as x_I = G_e b_I - W_l x_left - W_r x_right.
+// v = (x + Q r^(L+1) sqrt(w)) * d2fac2, Q = d2fac1 * sum_r rho r^L sqrt(w)/..
+// (multipole boundary term, d2rpl0/d2rpl1).
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_poisson_elem(const double2* __restrict__ rho, double2* __restrict__ x,
-                                                      const PoissonArgs a) {
+// B1: one wavefront per (pair, L), the four wavefronts of a block sharing L:
+// bridge right-hand sides straight from rho (one lane per element for the
+// W.b dots), the multipole sum, then the two bidiagonal sweeps of the
+// factored Schur system on lane 0 out of LDS, then the bridge rows of v.
+__global__ void __launch_bounds__(256) k_poisson_bridge(const double2* __restrict__ rho, double2* __restrict__ v,
+                                                        double2* __restrict__ xb, const PoissonArgs a) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int job = blockIdx.x * 4 + wave;
+  const int L = job / a.npair, p = job % a.npair;
+  const int nb = a.nfe - 1;
+  __shared__ double2 sS[4][kMaxBridge + 1];
+  __shared__ double2 sCr[4][kMaxBridge + 1];
+  __shared__ double2 sY[4][kMaxBridge];
+  __shared__ double sL[4][kMaxBridge], sU[4][kMaxBridge], sD[4][kMaxBridge];
+  if (L >= a.L2) return;  // whole wavefront leaves; no block barrier follows
+  const int pm = a.pair_m[p];
+  const size_t row = ((size_t)p * a.L2 + L) * a.nr;
+  double2* vp = v + row;
+  double2* xbp = xb + ((size_t)p * a.L2 + L) * nb;
+  if (L < (pm < 0 ? -pm : pm)) {
+    for (int k = lane; k < nb; k += 64) {
+      vp[(k + 1) * a.n - 1] = make_double2(0.0, 0.0);
+      xbp[k] = make_double2(0.0, 0.0);
+    }
+    if (lane == 0) a.Q[(size_t)p * a.L2 + L] = make_double2(0.0, 0.0);
+    return;
+  }
+  const double2* rp = rho + row;
+  const double* invr = a.invr + (size_t)L * a.nr;
+  const double* Wl = a.Wl + (size_t)L * a.nr;
+  const double* Wr = a.Wr + (size_t)L * a.nr;
+  const double* rpl0 = a.rpl0 + (size_t)L * a.nr;
+  for (int k = lane; k < nb; k += 64) {
+    sL[wave][k] = a.Sl[(size_t)L * nb + k];
+    sU[wave][k] = a.Su[(size_t)L * nb + k];
+    sD[wave][k] = 1.0 / a.Sd[(size_t)L * nb + k];
+  }
+  // per element: cl = W_l . b_I, cr = W_r . b_I
+  for (int e = lane; e < a.nfe; e += 64) {
+    const int base = e * a.n;
+    double2 cl = make_double2(0.0, 0.0), cr = make_double2(0.0, 0.0);
+    for (int i = 0; i < a.ni; ++i) {
+      const double2 b = cscale(rp[base + i], invr[base + i]);
+      cl.x = fma(Wl[base + i], b.x, cl.x);
+      cl.y = fma(Wl[base + i], b.y, cl.y);
+      cr.x = fma(Wr[base + i], b.x, cr.x);
+      cr.y = fma(Wr[base + i], b.y, cr.y);
+    }
+    // bridge k = e - 1 (left of e) takes -cl; bridge k = e (right of e) takes -cr
+    if (e >= 1) sS[wave][e - 1] = cl;
+    sCr[wave][e] = cr;
+  }
+  // multipole moment over every radial row
+  double2 m = make_double2(0.0, 0.0);
+  for (int r = lane; r < a.nr; r += 64) {
+    const double2 x = rp[r];
+    m.x = fma(x.x, rpl0[r], m.x);
+    m.y = fma(x.y, rpl0[r], m.y);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    m.x += __shfl_xor(m.x, o, 64);
+    m.y += __shfl_xor(m.y, o, 64);
+  }
+  const double2 Q = cscale(m, a.fac1[L]);
+  // right-hand sides of the Schur system (wave-local LDS: the wavefront's
+  // own lanes wrote these entries, so a wavefront barrier orders them)
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  for (int k = lane; k < nb; k += 64) {
+    const int gi = (k + 1) * a.n - 1;
+    double2 s = cscale(rp[gi], invr[gi]);
+    s = csub(s, sS[wave][k]);
+    s = csub(s, sCr[wave][k]);
+    sS[wave][k] = s;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  if (lane == 0 && nb > 0) {
+    // forward: y_k = s_k - l_k y_(k-1); backward: x_k = (y_k - u_k x_(k+1)) / d_k
+    const double2* sv = sS[wave];
+    const double* lv = sL[wave];
+    const double* uv = sU[wave];
+    const double* dv = sD[wave];
+    double2* yv = sY[wave];
+    double2 y = sv[0];
+    yv[0] = y;
+    for (int k = 1; k < nb; ++k) {
+      const double2 sk = sv[k];
+      const double lk = lv[k];
+      y = make_double2(fma(-lk, y.x, sk.x), fma(-lk, y.y, sk.y));
+      yv[k] = y;
+    }
+    double2 x = cscale(y, dv[nb - 1]);
+    sCr[wave][nb - 1] = x;
+    for (int k = nb - 2; k >= 0; --k) {
+      const double2 yk = yv[k];
+      const double uk = uv[k], dk = dv[k];
+      x = make_double2(fma(-uk, x.x, yk.x) * dk, fma(-uk, x.y, yk.y) * dk);
+      sCr[wave][k] = x;
+    }
+  }
+  if (lane == 0) a.Q[(size_t)p * a.L2 + L] = Q;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const double* rpl1 = a.rpl1 + (size_t)L * a.nr;
+  const double* fac2 = a.fac2 + (size_t)L * a.nr;
+  for (int k = lane; k < nb; k += 64) {
+    const int gi = (k + 1) * a.n - 1;
+    const double2 x = sCr[wave][k];
+    xbp[k] = x;
+    vp[gi] = make_double2((x.x + Q.x * rpl1[gi]) * fac2[gi], (x.y + Q.y * rpl1[gi]) * fac2[gi]);
+  }
+}
+
+// B2: block per (element, L): lanes in groups of ni, one group per pair
+// (the block is sized to hold every pair when it can), lane i owning
+// interior row i; G_e in LDS serves every pair.
+//   x_I = G_e b_I - W_l x_left - W_r x_right,  v = (x + Q rpl1) * fac2.
+__global__ void __launch_bounds__(1024) k_poisson_elem(const double2* __restrict__ rho, double2* __restrict__ v,
+                                                      const double2* __restrict__ xb, const PoissonArgs a) {
   const int e = blockIdx.x;
   const int L = blockIdx.y;
   const int ni = a.ni;
-  const int GS = ni + 1;            // lanes per pair group: interior rows + the left bridge
-  const int ngroup = 256 / GS;
+  const int nt = blockDim.x;
+  const int ngroup = nt / ni;
+  const int nb = a.nfe - 1;
   __shared__ double sG[32 * 32];
-  __shared__ double sCl[32], sCr[32], sInv[32], sRp0[33];
-  __shared__ double2 sB[256], sY[256], sM[256];
+  __shared__ double sInv[32], sWl[32], sWr[32], sR1[32], sF2[32];
+  __shared__ double2 sB[1024];
   const double* G = a.G + (((size_t)L * a.nfe) + e) * ni * ni;
   const int base = e * a.n;  // interior global index of local point 1
-  for (int q = threadIdx.x; q < ni * ni; q += 256) sG[q] = G[q];
-  for (int q = threadIdx.x; q < ni; q += 256) {
-    sCl[q] = a.Cl[(((size_t)L * a.nfe) + e) * ni + q];
-    sCr[q] = a.Cr[(((size_t)L * a.nfe) + e) * ni + q];
-    sInv[q] = a.invr[(size_t)L * a.nr + base + q];
-    sRp0[q] = a.rpl0[(size_t)L * a.nr + base + q];
+  for (int q = threadIdx.x; q < ni * ni; q += nt) sG[q] = G[q];
+  for (int q = threadIdx.x; q < ni; q += nt) {
+    const size_t k = (size_t)L * a.nr + base + q;
+    sInv[q] = a.invr[k];
+    sWl[q] = a.Wl[k];
+    sWr[q] = a.Wr[k];
+    sR1[q] = a.rpl1[k];
+    sF2[q] = a.fac2[k];
   }
-  if (threadIdx.x == 0) sRp0[ni] = e >= 1 ? a.rpl0[(size_t)L * a.nr + base - 1] : 0.0;
   __syncthreads();
-  const int g = threadIdx.x / GS, lane = threadIdx.x % GS;
+  const int g = threadIdx.x / ni, lane = threadIdx.x % ni;
   const bool ingroup = g < ngroup;
   for (int p0 = 0; p0 < a.npair; p0 += ngroup) {
     const int p = p0 + g;
-    bool valid = ingroup && p < a.npair;
+    const bool active = ingroup && p < a.npair;
+    bool valid = active;
     if (valid) {
       const int pm = a.pair_m[p];
       valid = L >= (pm < 0 ? -pm : pm);
     }
-    const double2* rp = rho + ((size_t)p * a.L2 + L) * a.nr;
-    if (valid) {
-      if (lane < ni) {
-        const double2 v = rp[base + lane];
-        sB[threadIdx.x] = cscale(v, sInv[lane]);
-        sM[threadIdx.x] = cscale(v, sRp0[lane]);
-      } else {
-        sM[threadIdx.x] = e >= 1 ? cscale(rp[base - 1], sRp0[ni]) : make_double2(0.0, 0.0);
-      }
-    }
+    const size_t row = ((size_t)p * a.L2 + L) * a.nr;
+    if (valid) sB[threadIdx.x] = cscale(rho[row + base + lane], sInv[lane]);
     __syncthreads();
-    if (valid && lane < ni) {
-      const double* gr = sG + lane * ni;
-      const double2* b = sB + g * GS;
-      double2 y = make_double2(0.0, 0.0);
-      for (int k = 0; k < ni; ++k) {
-        y.x = fma(gr[k], b[k].x, y.x);
-        y.y = fma(gr[k], b[k].y, y.y);
+    if (active) {
+      double2 out = make_double2(0.0, 0.0);
+      if (valid) {
+        const double* gr = sG + lane * ni;
+        const double2* b = sB + g * ni;
+        double2 y = make_double2(0.0, 0.0);
+        for (int k = 0; k < ni; ++k) {
+          y.x = fma(gr[k], b[k].x, y.x);
+          y.y = fma(gr[k], b[k].y, y.y);
+        }
+        const double2* xbp = xb + ((size_t)p * a.L2 + L) * nb;
+        if (e >= 1) y = csub(y, cscale(xbp[e - 1], sWl[lane]));
+        if (e <= a.nfe - 2) y = csub(y, cscale(xbp[e], sWr[lane]));
+        const double2 Q = a.Q[(size_t)p * a.L2 + L];
+        out = make_double2((y.x + Q.x * sR1[lane]) * sF2[lane], (y.y + Q.y * sR1[lane]) * sF2[lane]);
       }
-      x[((size_t)p * a.L2 + L) * a.nr + base + lane] = y;
-      sY[threadIdx.x] = y;
-    }
-    __syncthreads();
-    if (valid && lane == 0) {
-      const double2* y = sY + g * GS;
-      const double2* m = sM + g * GS;
-      double2 cl = make_double2(0.0, 0.0), cr = make_double2(0.0, 0.0), mp = make_double2(0.0, 0.0);
-      for (int i = 0; i < ni; ++i) {
-        cl.x = fma(sCl[i], y[i].x, cl.x);
-        cl.y = fma(sCl[i], y[i].y, cl.y);
-        cr.x = fma(sCr[i], y[i].x, cr.x);
-        cr.y = fma(sCr[i], y[i].y, cr.y);
-      }
-      for (int i = 0; i <= ni; ++i) mp = cadd(mp, m[i]);
-      const size_t q = (((size_t)p * a.L2 + L) * a.nfe) + e;
-      a.cl[q] = cl;
-      a.cr[q] = cr;
-      a.mpart[q] = mp;
+      v[row + base + lane] = out;
     }
     __syncthreads();
   }
-}
-
-// P2: block per (L, chunk of 16 pairs): bridge right-hand sides staged in
-// LDS by all threads, then one lane per pair runs the tridiagonal (Thomas)
-// solve of the Schur system and sums the multipole moment.
-__global__ void __launch_bounds__(256) k_poisson_bridge(const double2* __restrict__ rho, double2* __restrict__ x,
-                                                        const PoissonArgs a) {
-  constexpr int PC = 16;
-  const int L = blockIdx.x;
-  const int p0 = blockIdx.y * PC;
-  const int nb = a.nfe - 1;
-  __shared__ double2 sS[PC * kMaxBridge];
-  __shared__ double sSd[kMaxBridge], sSl[kMaxBridge], sSu[kMaxBridge];
-  __shared__ double2 sQ[PC * 16];
-  for (int k = threadIdx.x; k < nb; k += 256) {
-    sSd[k] = a.Sd[(size_t)L * nb + k];
-    sSl[k] = a.Sl[(size_t)L * nb + k];
-    sSu[k] = a.Su[(size_t)L * nb + k];
-  }
-  const double* invr = a.invr + (size_t)L * a.nr;
-  for (int t = threadIdx.x; t < PC * nb; t += 256) {
-    const int pl = t / nb, k = t % nb, p = p0 + pl;
-    if (p >= a.npair) continue;
-    const size_t q0 = ((size_t)p * a.L2 + L) * a.nfe;
-    const int gi = (k + 1) * a.n - 1;
-    double2 s = cscale(rho[((size_t)p * a.L2 + L) * a.nr + gi], invr[gi]);
-    s = csub(s, a.cr[q0 + k]);
-    s = csub(s, a.cl[q0 + k + 1]);
-    sS[pl * kMaxBridge + k] = s;
-  }
-  // multipole moment: 16 lanes per pair sum the element parts
-  {
-    const int pl = threadIdx.x / 16, sub = threadIdx.x % 16, p = p0 + pl;
-    double2 m = make_double2(0.0, 0.0);
-    if (p < a.npair) {
-      const size_t q0 = ((size_t)p * a.L2 + L) * a.nfe;
-      for (int e = sub; e < a.nfe; e += 16) m = cadd(m, a.mpart[q0 + e]);
-    }
-    sQ[threadIdx.x] = m;
-  }
-  __syncthreads();
-  const int pl = threadIdx.x, p = p0 + pl;
-  if (pl < PC && p < a.npair) {
-    const int pm = a.pair_m[p];
-    double2* xp = x + ((size_t)p * a.L2 + L) * a.nr;
-    if (L < (pm < 0 ? -pm : pm)) {
-      a.Q[(size_t)p * a.L2 + L] = make_double2(0.0, 0.0);
-    } else {
-      double2* s = sS + pl * kMaxBridge;
-      for (int k = 1; k < nb; ++k) s[k] = csub(s[k], cscale(s[k - 1], sSl[k]));
-      double2 nxt = make_double2(0.0, 0.0);
-      for (int k = nb - 1; k >= 0; --k) {
-        double2 v = s[k];
-        if (k < nb - 1) v = csub(v, cscale(nxt, sSu[k]));
-        v = cscale(v, 1.0 / sSd[k]);
-        xp[(k + 1) * a.n - 1] = v;
-        nxt = v;
-      }
-      double2 m = make_double2(0.0, 0.0);
-      for (int k = 0; k < 16; ++k) m = cadd(m, sQ[pl * 16 + k]);
-      a.Q[(size_t)p * a.L2 + L] = cscale(m, a.fac1[L]);
-    }
-  }
-}
-
-// P3: per (pair, L, r): interior correction and potential assembly
-// v = (x + Q r^(L+1) sqrt(w)) * 4pi/(2L+1)/(r sqrt(w)); reads the work
-// buffer x (interior y values, bridge solutions) and writes v.
-__global__ void __launch_bounds__(256) k_poisson_final(const double2* __restrict__ x, double2* __restrict__ v,
-                                                       const PoissonArgs a) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  const int L = blockIdx.y, p = blockIdx.z;
-  if (r >= a.nr) return;
-  const double2* xp = x + ((size_t)p * a.L2 + L) * a.nr;
-  double2* vp = v + ((size_t)p * a.L2 + L) * a.nr;
-  const int pm = a.pair_m[p];
-  if (L < (pm < 0 ? -pm : pm)) {
-    vp[r] = make_double2(0.0, 0.0);
-    return;
-  }
-  const int e = (r + 1) / a.n;         // element of r (a bridge belongs to the element on its right)
-  const int mloc = r + 1 - e * a.n;    // local index 0..n-1
-  double2 y = xp[r];
-  const size_t k = (size_t)L * a.nr + r;
-  if (mloc != 0) {  // interior point
-    if (e >= 1) y = csub(y, cscale(xp[e * a.n - 1], a.Wl[k]));
-    if (e <= a.nfe - 2) y = csub(y, cscale(xp[(e + 1) * a.n - 1], a.Wr[k]));
-  }
-  const double2 Q = a.Q[(size_t)p * a.L2 + L];
-  const double rp1 = a.rpl1[k], f2 = a.fac2[k];
-  double2 out;
-  out.x = (y.x + Q.x * rp1) * f2;
-  out.y = (y.y + Q.y * rp1) * f2;
-  vp[r] = out;
 }
 
 // ---------------------------------------------------------------------------
@@ -600,15 +616,21 @@ __global__ void __launch_bounds__(256) k_mfprod_t(const double2* __restrict__ or
                                                   const MfArgs a) {
   constexpr int NCORE = NF - NA;
   constexpr int NA2 = NA > 0 ? NA * NA : 1;
+  constexpr int ND = NF - NFC;
+  constexpr int NDP = ND * (ND + 1) / 2;  // dynamical pairs, stored first
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   const int th = blockIdx.y;
   if (r >= a.nr) return;
   const size_t ng = (size_t)a.nlat * a.nr;
   const size_t g = (size_t)th * a.nr + r;
-  double2 psi[NF], ag[NF], av[NF];
+  // every load of the point is issued before any arithmetic
+  double2 psi[NF], ag[NF], av[NF], vdp[NDP];
+#pragma unroll
+  for (int f = 0; f < NF; ++f) psi[f] = orbg[(size_t)f * ng + g];
+#pragma unroll
+  for (int q = 0; q < NDP; ++q) vdp[q] = vg[(size_t)q * ng + g];
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
-    psi[f] = orbg[(size_t)f * ng + g];
     ag[f] = make_double2(0.0, 0.0);
     av[f] = make_double2(0.0, 0.0);
   }
@@ -628,7 +650,7 @@ __global__ void __launch_bounds__(256) k_mfprod_t(const double2* __restrict__ or
   for (int i = NFC; i < NF; ++i) {
 #pragma unroll
     for (int j = i; j < NF; ++j) {
-      const double2 v = vg[(size_t)(p++) * ng + g];
+      const double2 v = vdp[p++];
       const bool jdc = j < NCORE, idc = i < NCORE;
       if (j == i && jdc) d2v_dc = cadd(d2v_dc, cscale(v, 2.0));
       if (jdc) ag[i] = csub(ag[i], cmul(v, psi[j]));
@@ -676,7 +698,7 @@ __global__ void __launch_bounds__(256) k_mfprod_t(const double2* __restrict__ or
           for (int la = 0; la < NA; ++la) {
             if (a.mval[NCORE + ia] + a.mval[NCORE + ka] != a.mval[NCORE + ja] + a.mval[NCORE + la]) continue;
             const double2 dens = a.den2[ja + NA * (ia + NA * (la + NA * ka))];
-            if (sqrt(dens.x * dens.x + dens.y * dens.y) <= 1.0e-15) continue;
+            if (dens.x * dens.x + dens.y * dens.y <= 1.0e-30) continue;
             cfma(d, dens, vact[ka * NA + la]);
           }
         cfma(t[ia], psi[NCORE + ja], d);
@@ -912,7 +934,7 @@ void launch_transform(int mode, const double2* in, double2* out, const double* m
 bool launch_onebody_el(const double2* orb, double2* h01, const OneBodyElArgs& a, hipStream_t s) {
   if (a.ndvr != 20) return false;
   constexpr int GE = 256 / 21;
-  dim3 grid((a.nfe + GE - 1) / GE, a.nfun);
+  dim3 grid((a.nfe + GE - 1) / GE, (a.L1 + kOneBodyLChunk - 1) / kOneBodyLChunk, a.nfun);
   hipLaunchKernelGGL((k_onebody_el<20>), grid, dim3(256), 0, s, orb, h01, a);
   return true;
 }
@@ -923,10 +945,11 @@ void launch_onebody(const double2* orb, double2* h01, const OneBodyArgs& a, hipS
   hipLaunchKernelGGL(k_onebody, grid, block, 0, s, orb, h01, a);
 }
 
-void launch_poisson(const double2* rho, double2* work, double2* v, const PoissonArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_poisson_elem, dim3(a.nfe, a.L2), dim3(256), 0, s, rho, work, a);
-  hipLaunchKernelGGL(k_poisson_bridge, dim3(a.L2, (a.npair + 15) / 16), dim3(256), 0, s, rho, work, a);
-  hipLaunchKernelGGL(k_poisson_final, dim3((a.nr + 255) / 256, a.L2, a.npair), dim3(256), 0, s, work, v, a);
+void launch_poisson(const double2* rho, double2* xb, double2* v, const PoissonArgs& a, hipStream_t s) {
+  const int jobs = a.npair * a.L2;
+  hipLaunchKernelGGL(k_poisson_bridge, dim3((jobs + 3) / 4), dim3(256), 0, s, rho, v, xb, a);
+  const int nt = std::min(1024, (a.npair * a.ni + 63) / 64 * 64);
+  hipLaunchKernelGGL(k_poisson_elem, dim3(a.nfe, a.L2), dim3(std::max(nt, 64)), 0, s, rho, v, xb, a);
 }
 
 void launch_mfprod(const double2* orbg, const double2* vg, double2* gorbg, double2* v2orbg, const MfArgs& a,

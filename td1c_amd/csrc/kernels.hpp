@@ -15,6 +15,7 @@ constexpr int kMaxAct = 8;        // active orbitals handled on the theta grid
 constexpr int kMaxActPairs = kMaxAct * kMaxAct;
 constexpr int kMaxStages = 6;
 constexpr int kMaxBridge = 160;   // nfe - 1 bridges handled by the Poisson bridge kernel
+constexpr int kOneBodyLChunk = 2;  // l values per block of the element one-body kernel
 
 // Time of a device-driven propagation: step n runs at t0 + n * dt.
 struct PropClock {
@@ -61,9 +62,9 @@ struct OneBodyElArgs {
 struct PoissonArgs {
   int nr, n, nfe, ni, L2, npair;
   const int* pair_m;  // m_j - m_i of each pair
-  const double *G, *Cl, *Cr, *Wl, *Wr, *Sd, *Sl, *Su;
+  const double *G, *Wl, *Wr, *Sd, *Sl, *Su;
   const double *invr, *rpl0, *rpl1, *fac2, *fac1;
-  double2 *cl, *cr, *mpart, *Q;  // [npair][L2][nfe], Q: [npair][L2]
+  double2* Q;  // [npair][L2] multipole moments
 };
 
 struct MfArgs {
@@ -108,7 +109,11 @@ struct IslandArgs {
   int nstr_a, nstr_b;
   const int *det_of, *exA, *sgA, *exB, *sgB;
   const double2* cic;
-  double2* ci_work;       // (nact^2 + 3) * nstr_a * nstr_b
+  double2* ci_work;       // (nact^2 + 3) * nstr_a * nstr_b; holds D_pq = E_pq C between the stages
+  // dynamic LDS plan (island_lds_plan): CI work and string tables in LDS,
+  // int2e in LDS at byte offset ci_lds_off
+  int ci_lds, i2_lds;
+  size_t ci_lds_off, lds_bytes;
   // stage A outputs
   double2 *den1, *den2, *rden, *rrden;
   // stage B inputs: Gram results and index maps ([nfun*nfun], -1 = absent)
@@ -137,7 +142,7 @@ void launch_transform(int mode, const double2* in, double2* out, const double* m
 inline int pad_cols(int C) { return (C + 31) / 32 * 32; }
 void launch_onebody(const double2* orb, double2* h01, const OneBodyArgs& a, hipStream_t s);
 bool launch_onebody_el(const double2* orb, double2* h01, const OneBodyElArgs& a, hipStream_t s);
-void launch_poisson(const double2* rho, double2* work, double2* v, const PoissonArgs& a, hipStream_t s);
+void launch_poisson(const double2* rho, double2* xb, double2* v, const PoissonArgs& a, hipStream_t s);
 void launch_mfprod(const double2* orbg, const double2* vg, double2* gorbg, double2* v2orbg, const MfArgs& a,
                    hipStream_t s);
 void launch_fc_phase(const double2* orbg0, double2* orbg, const double* xr, const double* cost, const double* lf, int nr,
@@ -149,6 +154,8 @@ void launch_field_stages(const Pulse& p, PropClock* st, const StageNodes& nodes,
 void launch_clock_set(PropClock* st, double t0, double dt, hipStream_t s);
 void launch_dtorb(const DtorbArgs& a, hipStream_t s);
 void launch_mask(double2* wfn, const double* mask, int nr, int L1, int nfcore, int nfun, hipStream_t s);
+constexpr size_t kIslandLds = 32768;  // dynamic LDS budget of the island kernels
+void island_lds_plan(IslandArgs& a);
 void launch_island_a(const IslandArgs& a, hipStream_t s);
 void launch_island_b(const IslandArgs& a, hipStream_t s);
 

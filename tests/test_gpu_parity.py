@@ -91,8 +91,9 @@ def test_rk_step_matches_oracle(gpu_lib, order):
     hp.set_wfn0(w)
     field = td1c_amd.Field(env_type="sin2", gauge="velocity1", fint=1e14, wlen=800.0, numcyc=2.0)
     rk = td1c_amd.CRK(order)
-    # the full-hprod RK step is stable only for dt * 9e3 < 2.8 on 2-a.u. elements
-    dt, t = 1.0e-4, 30.0
+    # the one-body spectral radius of ne_small is 4.1e4 (l = 7), so the
+    # explicit RK4 step of the full hprod is stable for dt < 2.8 / 4.1e4
+    dt, t = 2.0e-5, 30.0
     d = hp.to_device(w)
     ref = w.copy()
     for step in range(3):
@@ -161,7 +162,7 @@ def test_graph_propagation_matches_oracle(gpu_lib, name):
     hp.set_wfn0(w)
     field = td1c_amd.Field(env_type="sin2", gauge="velocity1", fint=1e14, wlen=800.0, numcyc=2.0)
     rk = td1c_amd.CRK(4)
-    dt, t = 1.0e-4, 30.0
+    dt, t = 2.0e-5, 30.0  # inside the RK4 stability bound of both problems
     d = hp.to_device(w)
     scratch = hp.to_device(w)
     ref = w.copy()
