@@ -408,7 +408,7 @@ int build_device(td1c_ctx* c) {
   HIP_TRY(c->ene.alloc(5));
   HIP_TRY(hipMemset(c->ene.p, 0, 5 * sizeof(double)));
   const size_t nfull = (size_t)S.nstr_a * S.nstr_b;
-  HIP_TRY(c->ci_work.alloc(std::max<size_t>(1, ((size_t)S.nact * S.nact + 3) * nfull)));
+  HIP_TRY(c->ci_work.alloc(std::max<size_t>(1, ((size_t)2 * S.nact * S.nact + 2) * nfull)));
   HIP_TRY(c->det_of.upload(flat_or_one(S.det_of)));
   HIP_TRY(c->exA.upload(flat_or_one(S.exA)));
   HIP_TRY(c->sgA.upload(flat_or_one(S.sgA)));

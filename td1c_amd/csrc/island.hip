@@ -28,14 +28,14 @@ __device__ __forceinline__ double2 zero2() { return make_double2(0.0, 0.0); }
 
 // Where the CI work lives: LDS when it fits, else the global work buffer.
 struct CiView {
-  double2 *C, *D, *S, *T;       // C[nc], D[n*n][nc] (D_pq = E_pq C), S[nc], T[nc]
+  double2 *C, *D, *S, *T;       // D[n*n][nc] (D_pq = E_pq C), C[nc], S[nc], T[n*n][nc]
   const int *exA, *sgA, *exB, *sgB;
 };
 
 // Bytes of dynamic LDS a CI view needs (0: it does not fit, use global).
 __host__ __device__ inline size_t ci_lds_bytes(int n, int nstr_a, int nstr_b) {
   const size_t nc = (size_t)nstr_a * nstr_b;
-  const size_t z = nc * ((size_t)n * n + 3) * sizeof(double2);
+  const size_t z = nc * ((size_t)2 * n * n + 2) * sizeof(double2);
   const size_t t = (size_t)2 * (nstr_a + nstr_b) * n * n * sizeof(int);
   const size_t b = z + t;
   return b <= kIslandLds ? b : 0;
@@ -51,7 +51,7 @@ __device__ CiView ci_view(const IslandArgs& a, unsigned char* dyn) {
     v.C = v.D + nt * nc;
     v.S = v.C + nc;
     v.T = v.S + nc;
-    int* ti = reinterpret_cast<int*>(v.T + nc);
+    int* ti = reinterpret_cast<int*>(v.T + nt * nc);
     int* eA = ti;
     int* gA = eA + na * nt;
     int* eB = gA + na * nt;
@@ -75,33 +75,21 @@ __device__ CiView ci_view(const IslandArgs& a, unsigned char* dyn) {
   return v;
 }
 
-// y[full] += E_pq x[full] for the determinants handled by this thread
-// (stride kT); distinct sources map to distinct targets, alpha then beta.
-__device__ void apply_E_par(const IslandArgs& a, const CiView& v, int p, int q, const double2* x, double2* y) {
-  const int na = a.nstr_a, nb = a.nstr_b, n = a.nact;
-  const int nc = na * nb;
-  for (int f = threadIdx.x; f < nc; f += kT) {
-    const int ib = f / na, ia = f % na;
-    const double2 xv = x[f];
-    if (xv.x == 0.0 && xv.y == 0.0) continue;
-    const int t = v.exA[(ia * n + p) * n + q];
-    if (t >= 0) {
-      const double sg = (double)v.sgA[(ia * n + p) * n + q];
-      y[ib * na + t] = zadd(y[ib * na + t], zscal(xv, sg));
-    }
-  }
-  __syncthreads();
-  for (int f = threadIdx.x; f < nc; f += kT) {
-    const int ib = f / na, ia = f % na;
-    const double2 xv = x[f];
-    if (xv.x == 0.0 && xv.y == 0.0) continue;
-    const int t = v.exB[(ib * n + p) * n + q];
-    if (t >= 0) {
-      const double sg = (double)v.sgB[(ib * n + p) * n + q];
-      y[t * na + ia] = zadd(y[t * na + ia], zscal(xv, sg));
-    }
-  }
-  __syncthreads();
+// (E_pq x)[f] for determinant f = (ib, it) by gathering from the source
+// determinants: E_pq |s> = sg |t> exactly when E_qp |t> = sg |s>, so the
+// source string is the E_qp image of the target string, alpha then beta.
+__device__ __forceinline__ double2 gather_E(const IslandArgs& a, const CiView& v, int p, int q, const double2* x,
+                                            int f) {
+  const int ns = a.nstr_a, n = a.nact;
+  const int ib = f / ns, it = f % ns;
+  double2 t = zero2();
+  const int ka = (it * n + q) * n + p;
+  const int sa = v.exA[ka];
+  if (sa >= 0) t = zadd(t, zscal(x[ib * ns + sa], (double)v.sgA[ka]));
+  const int kb = (ib * n + q) * n + p;
+  const int sb = v.exB[kb];
+  if (sb >= 0) t = zadd(t, zscal(x[sb * ns + it], (double)v.sgB[kb]));
+  return t;
 }
 
 // Hermitian eigen-decomposition by cyclic complex Jacobi rotations
@@ -147,18 +135,17 @@ __device__ void herm_eig(int n, double2* A, double* w, double2* U) {
   for (int i = 0; i < n; ++i) w[i] = A[i + n * i].x;
 }
 
-// regularised inverse U diag(d/(d^2+thresh)) U^+ (one lane; LDS scratch)
-__device__ void invden_inv(int n, double thresh, const double2* a, double2* inva, double2* A, double2* U, double* w) {
-  for (int i = 0; i < n * n; ++i) A[i] = a[i];
-  herm_eig(n, A, w, U);
-  for (int i = 0; i < n * n; ++i) inva[i] = zero2();
+// entry e = i + n j of the regularised inverse U diag(d/(d^2+thresh)) U^+
+// (hprod_invden_inv, src/hprod/hprod_invden.F90:62-107)
+__device__ double2 inv_entry(int n, double thresh, const double2* U, const double* w, int e) {
+  const int i = e % n, j = e / n;
+  double2 t = zero2();
   for (int k = 0; k < n; ++k) {
-    double d = w[k];
-    double invd = thresh > 0.0 ? d / (d * d + thresh) : 1.0 / d;
-    for (int j = 0; j < n; ++j)
-      for (int i = 0; i < n; ++i)
-        inva[i + n * j] = zadd(inva[i + n * j], zscal(zmul(U[i + n * k], zconj(U[j + n * k])), invd));
+    const double d = w[k];
+    const double invd = thresh > 0.0 ? d / (d * d + thresh) : 1.0 / d;
+    t = zadd(t, zscal(zmul(U[i + n * k], zconj(U[j + n * k])), invd));
   }
+  return t;
 }
 
 }  // namespace
@@ -168,7 +155,7 @@ __device__ void invden_inv(int n, double thresh, const double2* a, double2* inva
 // (hprod_invden, src/hprod/hprod_invden.F90:2-60).
 __global__ void __launch_bounds__(kT) k_island_a(const IslandArgs a) {
   extern __shared__ __align__(16) unsigned char dyn[];
-  __shared__ double2 s_d1[kN2], s_A[kN2], s_U[kN2], s_T[kN2], s_out[kN2];
+  __shared__ double2 s_d1[kN2], s_A[kN2], s_U[kN2];
   __shared__ double s_w[kMaxAct];
   const int n = a.nact;
   if (n == 0) return;
@@ -177,10 +164,12 @@ __global__ void __launch_bounds__(kT) k_island_a(const IslandArgs a) {
   double2* D = v.D;
   double2* C = v.C;
   for (int f = threadIdx.x; f < nc; f += kT) C[f] = a.det_of[f] >= 0 ? a.cic[a.det_of[f]] : zero2();
-  for (size_t q = threadIdx.x; q < (size_t)n * n * nc; q += kT) D[q] = zero2();
   __syncthreads();
-  for (int p = 0; p < n; ++p)
-    for (int q = 0; q < n; ++q) apply_E_par(a, v, p, q, C, D + (size_t)(p * n + q) * nc);
+  for (size_t e = threadIdx.x; e < (size_t)n * n * nc; e += kT) {
+    const int pq = (int)(e / nc), f = (int)(e % nc);
+    D[e] = gather_E(a, v, pq / n, pq % n, C, f);
+  }
+  __syncthreads();
   // den1(x,y) = sum conj(C) (E_yx C)
   for (int e = threadIdx.x; e < n * n; e += kT) {
     const int x = e % n, y = e / n;
@@ -203,18 +192,26 @@ __global__ void __launch_bounds__(kT) k_island_a(const IslandArgs a) {
   // D goes to the global work buffer for stage B
   if (a.ci_lds)
     for (size_t q = threadIdx.x; q < (size_t)n * n * nc; q += kT) a.ci_work[q] = D[q];
+  // rden = D^-1 and rrden = (2 - D)^-1, regularised: eigen-decomposition on
+  // one lane, the n x n products on n*n lanes
   if (threadIdx.x == 0) {
-    invden_inv(n, a.throcc1, s_d1, s_out, s_A, s_U, s_w);
-    for (int i = 0; i < n * n; ++i) a.rden[i] = s_out[i];
-    if (a.ncore > 0) {
-      if (n * 2 == a.nelact2) {
-        for (int i = 0; i < n * n; ++i) a.rrden[i] = zero2();
-      } else {
-        for (int i = 0; i < n * n; ++i) s_T[i] = zscal(s_d1[i], -1.0);
-        for (int i = 0; i < n; ++i) s_T[i + n * i].x += 2.0;
-        invden_inv(n, a.throcc2, s_T, s_out, s_A, s_U, s_w);
-        for (int i = 0; i < n * n; ++i) a.rrden[i] = s_out[i];
+    for (int i = 0; i < n * n; ++i) s_A[i] = s_d1[i];
+    herm_eig(n, s_A, s_w, s_U);
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < n * n; e += kT) a.rden[e] = inv_entry(n, a.throcc1, s_U, s_w, e);
+  if (a.ncore > 0) {
+    if (n * 2 == a.nelact2) {
+      for (int e = threadIdx.x; e < n * n; e += kT) a.rrden[e] = zero2();
+    } else {
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        for (int i = 0; i < n * n; ++i) s_A[i] = zscal(s_d1[i], -1.0);
+        for (int i = 0; i < n; ++i) s_A[i + n * i].x += 2.0;
+        herm_eig(n, s_A, s_w, s_U);
       }
+      __syncthreads();
+      for (int e = threadIdx.x; e < n * n; e += kT) a.rrden[e] = inv_entry(n, a.throcc2, s_U, s_w, e);
     }
   }
 }
@@ -228,46 +225,30 @@ __global__ void __launch_bounds__(kT) k_island_b(const IslandArgs a) {
   __shared__ double2 s_H[kF2], s_G[kF2], s_V[kF2], s_F[kF2], s_X[kF2], s_FM[kF2];
   __shared__ double2 s_hp[kN2];
   __shared__ double2 s_i1[kN2], s_d1[kN2], s_rr[kN2];
-  __shared__ double s_red[kT];
+  __shared__ double s_red[kT / 64];
   __shared__ double s_eact;
+  __shared__ int s_mval[kMaxFun];
   const int nf = a.nfun, nfc = a.nfcore, ncore = a.ncore, na = a.nact;
   const int tid = threadIdx.x;
-  // gather the Gram results
+  // every global input first (the index maps, then what they point at), so
+  // the gather costs two memory round trips
+  double2* I2 = a.int2e;
+  if (a.i2_lds) I2 = reinterpret_cast<double2*>(dyn + a.ci_lds_off);
+  for (int e = tid; e < nf; e += kT) s_mval[e] = a.mval[e];
   for (int e = tid; e < nf * nf; e += kT) {
-    int q;
-    q = a.mapH[e]; s_H[e] = q >= 0 ? a.lprod[q] : zero2();
-    q = a.mapG[e]; s_G[e] = q >= 0 ? a.lprod[q] : zero2();
-    q = a.mapV[e]; s_V[e] = q >= 0 ? a.lprod[q] : zero2();
-    q = a.mapF[e]; s_F[e] = q >= 0 ? a.lprod[q] : zero2();
+    const int qh = a.mapH[e], qg = a.mapG[e], qv = a.mapV[e], qf = a.mapF[e];
+    s_H[e] = qh >= 0 ? a.lprod[qh] : zero2();
+    s_G[e] = qg >= 0 ? a.lprod[qg] : zero2();
+    s_V[e] = qv >= 0 ? a.lprod[qv] : zero2();
+    s_F[e] = qf >= 0 ? a.lprod[qf] : zero2();
     s_X[e] = zero2();
   }
   for (int e = tid; e < na * na; e += kT) {
     s_d1[e] = a.den1[e];
     s_rr[e] = a.rrden[e];
   }
-  __syncthreads();
-#define H(j, i) s_H[(j) * nf + (i)]
-#define G(j, i) s_G[(j) * nf + (i)]
-#define V(j, i) s_V[(j) * nf + (i)]
-#define F(j, i) s_F[(j) * nf + (i)]
-#define HGV(j, i) zadd(V(j, i), zadd(G(j, i), H(j, i)))
-  // int1e(j,i) = <j|g + h1 + h0|i> for j <= i, Hermitian fill
-  for (int e = tid; e < na * na; e += kT) {
-    const int ja = e % na, ia = e / na;
-    const int i = ncore + ia, j = ncore + ja;
-    double2 v = zero2();
-    if (ja <= ia) {
-      if (a.mval[i] == a.mval[j]) v = zadd(G(j, i), H(j, i));
-    } else if (a.mval[i] == a.mval[j]) {
-      v = zconj(zadd(G(i, j), H(i, j)));
-    }
-    s_i1[e] = v;
-    a.int1e[e] = v;
-  }
   // int2e(i,j,k,l): i >= j from the products, the rest by conjugate symmetry;
   // kept in LDS (after the CI view) when it fits
-  double2* I2 = a.int2e;
-  if (a.i2_lds) I2 = reinterpret_cast<double2*>(dyn + a.ci_lds_off);
   for (int e = tid; e < na * na * na * na; e += kT) {
     const int ia = e % na, ja = (e / na) % na, ka = (e / (na * na)) % na, la = e / (na * na * na);
     int src = e;
@@ -283,6 +264,25 @@ __global__ void __launch_bounds__(kT) k_island_b(const IslandArgs a) {
     if (a.i2_lds) a.int2e[e] = v;
   }
   __syncthreads();
+#define H(j, i) s_H[(j) * nf + (i)]
+#define G(j, i) s_G[(j) * nf + (i)]
+#define V(j, i) s_V[(j) * nf + (i)]
+#define F(j, i) s_F[(j) * nf + (i)]
+#define HGV(j, i) zadd(V(j, i), zadd(G(j, i), H(j, i)))
+  // int1e(j,i) = <j|g + h1 + h0|i> for j <= i, Hermitian fill
+  for (int e = tid; e < na * na; e += kT) {
+    const int ja = e % na, ia = e / na;
+    const int i = ncore + ia, j = ncore + ja;
+    double2 v = zero2();
+    if (ja <= ia) {
+      if (s_mval[i] == s_mval[j]) v = zadd(G(j, i), H(j, i));
+    } else if (s_mval[i] == s_mval[j]) {
+      v = zconj(zadd(G(i, j), H(i, j)));
+    }
+    s_i1[e] = v;
+    a.int1e[e] = v;
+  }
+  __syncthreads();
   // energies: ene_act = Re sum int1e(i,j) den1(j,i) + 1/2 Re sum int2e(i,j,k,l) den2(j,i,l,k)
   double part = 0.0;
   for (int e = tid; e < na * na * na * na; e += kT) {
@@ -293,16 +293,13 @@ __global__ void __launch_bounds__(kT) k_island_b(const IslandArgs a) {
     const int ia = e % na, ja = e / na;
     part += zmul(s_i1[ia + na * ja], s_d1[ja + na * ia]).x;
   }
-  s_red[tid] = part;
+  for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o, 64);
+  if ((tid & 63) == 0) s_red[tid >> 6] = part;
   __syncthreads();
-  for (int w = kT / 2; w > 0; w >>= 1) {
-    if (tid < w) s_red[tid] += s_red[tid + w];
-    __syncthreads();
-  }
   if (tid == 0) {
     double ene_dcore = 0.0;
     for (int i = nfc; i < ncore; ++i) ene_dcore += zadd(G(i, i), zscal(H(i, i), 2.0)).x;
-    const double ene_act = na > 0 ? s_red[0] : 0.0;
+    const double ene_act = na > 0 ? (s_red[0] + s_red[1]) + (s_red[2] + s_red[3]) : 0.0;
     s_eact = ene_act;
     a.ene[1] = ene_dcore;
     a.ene[2] = a.ene[0] + ene_dcore;
@@ -314,7 +311,7 @@ __global__ void __launch_bounds__(kT) k_island_b(const IslandArgs a) {
   // fc-dyn, core-core and active-active blocks (one entry per thread)
   for (int e = tid; e < nf * nf; e += kT) {
     const int j = e % nf, i = e / nf;
-    if (a.mval[j] != a.mval[i]) continue;
+    if (s_mval[j] != s_mval[i]) continue;
     if (j < nfc && i >= nfc) XM(j, i) = zmul(zadd(F(j, i), zadd(G(j, i), V(j, i))), tfac);
     else if (j >= nfc && j < ncore && i >= nfc && i < ncore) XM(j, i) = zmul(HGV(j, i), tfac);
     else if (j >= ncore && i >= ncore) XM(j, i) = zmul(HGV(j, i), tfac);
@@ -325,7 +322,7 @@ __global__ void __launch_bounds__(kT) k_island_b(const IslandArgs a) {
     if (na * 2 == a.nelact2) {
       for (int i = nfc; i < ncore; ++i)
         for (int j = ncore; j < nf; ++j)
-          if (a.mval[j] == a.mval[i]) {
+          if (s_mval[j] == s_mval[i]) {
             XM(j, i) = zmul(HGV(j, i), tfac);
             XM(i, j) = zmul(HGV(i, j), tfac);
           }
@@ -334,7 +331,7 @@ __global__ void __launch_bounds__(kT) k_island_b(const IslandArgs a) {
       for (int q = 0; q < nf * nf; ++q) FM[q] = zero2();
       for (int i = nfc; i < ncore; ++i)
         for (int j = ncore; j < nf; ++j)
-          if (a.mval[j] == a.mval[i]) {
+          if (s_mval[j] == s_mval[i]) {
             FM[i * nf + j] = HGV(j, i);
             FM[j * nf + i] = HGV(i, j);
           }
@@ -380,10 +377,7 @@ __global__ void __launch_bounds__(kT) k_island_b(const IslandArgs a) {
     double2* T = v.T;
     if (a.ci_lds)
       for (size_t q = tid; q < (size_t)na * na * nc; q += kT) D[q] = a.ci_work[q];
-    for (int f = tid; f < nc; f += kT) {
-      C[f] = a.det_of[f] >= 0 ? a.cic[a.det_of[f]] : zero2();
-      S[f] = zero2();
-    }
+    for (int f = tid; f < nc; f += kT) C[f] = a.det_of[f] >= 0 ? a.cic[a.det_of[f]] : zero2();
     // h'_pq = int1e(p,q) - 1/2 sum_r int2e(p,r,r,q)
     for (int e = tid; e < na * na; e += kT) {
       const int p = e % na, q = e / na;
@@ -392,25 +386,26 @@ __global__ void __launch_bounds__(kT) k_island_b(const IslandArgs a) {
       s_hp[e] = hpq;
     }
     __syncthreads();
+    // T_pq = 1/2 sum_rs int2e(p,q,r,s) D_rs for every (p, q) at once
+    for (size_t e = tid; e < (size_t)na * na * nc; e += kT) {
+      const int pq = (int)(e / nc), f = (int)(e % nc);
+      const int p = pq / na, q = pq % na;
+      double2 t = zero2();
+      for (int r = 0; r < na; ++r)
+        for (int s2 = 0; s2 < na; ++s2)
+          t = zadd(t, zmul(zscal(I2[p + na * (q + na * (r + na * s2))], 0.5), D[(size_t)(r * na + s2) * nc + f]));
+      T[e] = t;
+    }
+    __syncthreads();
+    // S = sum_pq h'_pq E_pq C + sum_pq E_pq T_pq, one determinant per thread
     for (int f = tid; f < nc; f += kT) {
       double2 acc = zero2();
       for (int p = 0; p < na; ++p)
         for (int q = 0; q < na; ++q) acc = zadd(acc, zmul(s_hp[p + na * q], D[(size_t)(p * na + q) * nc + f]));
+      for (int p = 0; p < na; ++p)
+        for (int q = 0; q < na; ++q) acc = zadd(acc, gather_E(a, v, p, q, T + (size_t)(p * na + q) * nc, f));
       S[f] = acc;
     }
-    __syncthreads();
-    for (int p = 0; p < na; ++p)
-      for (int q = 0; q < na; ++q) {
-        for (int f = tid; f < nc; f += kT) {
-          double2 t = zero2();
-          for (int r = 0; r < na; ++r)
-            for (int s = 0; s < na; ++s)
-              t = zadd(t, zmul(zscal(I2[p + na * (q + na * (r + na * s))], 0.5), D[(size_t)(r * na + s) * nc + f]));
-          T[f] = t;
-        }
-        __syncthreads();
-        apply_E_par(a, v, p, q, T, S);
-      }
     const double2 fac = a.icomp == 1 ? make_double2(0.0, -a.dtime) : make_double2(-a.dtime, 0.0);
     const double eact = s_eact;
     for (int f = tid; f < nc; f += kT) {

@@ -342,31 +342,34 @@ __global__ void __launch_bounds__(256) k_onebody_el(const double2* __restrict__ 
 // v = (x + Q r^(L+1) sqrt(w)) * d2fac2, Q = d2fac1 * sum_r rho r^L sqrt(w)/..
 // (multipole boundary term, d2rpl0/d2rpl1).
 // ---------------------------------------------------------------------------
-// B1: one wavefront per (pair, L), the four wavefronts of a block sharing L:
-// bridge right-hand sides straight from rho (one lane per element for the
-// W.b dots), the multipole sum, then the two bidiagonal sweeps of the
-// factored Schur system on lane 0 out of LDS, then the bridge rows of v.
+// B1: one block per (pair, L), jobs ordered L-major so consecutive blocks
+// share the L tables: element dots W.b (one thread per element, the NI
+// interior loads of a thread issued together), the multipole sum over all
+// rows, the Schur right-hand sides, the two bidiagonal sweeps of the
+// factored Schur system on thread 0 out of LDS, then the bridge rows of v.
+// NI = interior points per element (0: runtime a.ni).
+template <int NI>
 __global__ void __launch_bounds__(256) k_poisson_bridge(const double2* __restrict__ rho, double2* __restrict__ v,
                                                         double2* __restrict__ xb, const PoissonArgs a) {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int job = blockIdx.x * 4 + wave;
-  const int L = job / a.npair, p = job % a.npair;
+  const int tid = threadIdx.x;
+  const int L = blockIdx.x / a.npair, p = blockIdx.x % a.npair;
   const int nb = a.nfe - 1;
-  __shared__ double2 sS[4][kMaxBridge + 1];
-  __shared__ double2 sCr[4][kMaxBridge + 1];
-  __shared__ double2 sY[4][kMaxBridge];
-  __shared__ double sL[4][kMaxBridge], sU[4][kMaxBridge], sD[4][kMaxBridge];
-  if (L >= a.L2) return;  // whole wavefront leaves; no block barrier follows
+  const int ni = NI > 0 ? NI : a.ni;
+  __shared__ double2 sS[kMaxBridge + 1];
+  __shared__ double2 sCr[kMaxBridge + 1];
+  __shared__ double2 sY[kMaxBridge];
+  __shared__ double sL[kMaxBridge], sU[kMaxBridge], sD[kMaxBridge];
+  __shared__ double2 sRed[4];
   const int pm = a.pair_m[p];
   const size_t row = ((size_t)p * a.L2 + L) * a.nr;
   double2* vp = v + row;
   double2* xbp = xb + ((size_t)p * a.L2 + L) * nb;
-  if (L < (pm < 0 ? -pm : pm)) {
-    for (int k = lane; k < nb; k += 64) {
+  if (L < (pm < 0 ? -pm : pm)) {  // uniform over the block
+    for (int k = tid; k < nb; k += 256) {
       vp[(k + 1) * a.n - 1] = make_double2(0.0, 0.0);
       xbp[k] = make_double2(0.0, 0.0);
     }
-    if (lane == 0) a.Q[(size_t)p * a.L2 + L] = make_double2(0.0, 0.0);
+    if (tid == 0) a.Q[(size_t)p * a.L2 + L] = make_double2(0.0, 0.0);
     return;
   }
   const double2* rp = rho + row;
@@ -374,29 +377,49 @@ __global__ void __launch_bounds__(256) k_poisson_bridge(const double2* __restric
   const double* Wl = a.Wl + (size_t)L * a.nr;
   const double* Wr = a.Wr + (size_t)L * a.nr;
   const double* rpl0 = a.rpl0 + (size_t)L * a.nr;
-  for (int k = lane; k < nb; k += 64) {
-    sL[wave][k] = a.Sl[(size_t)L * nb + k];
-    sU[wave][k] = a.Su[(size_t)L * nb + k];
-    sD[wave][k] = 1.0 / a.Sd[(size_t)L * nb + k];
+  for (int k = tid; k < nb; k += 256) {
+    sL[k] = a.Sl[(size_t)L * nb + k];
+    sU[k] = a.Su[(size_t)L * nb + k];
+    sD[k] = 1.0 / a.Sd[(size_t)L * nb + k];
   }
-  // per element: cl = W_l . b_I, cr = W_r . b_I
-  for (int e = lane; e < a.nfe; e += 64) {
+  // per element: cl = W_l . b_I, cr = W_r . b_I with b = rho * invr
+  for (int e = tid; e < a.nfe; e += 256) {
     const int base = e * a.n;
     double2 cl = make_double2(0.0, 0.0), cr = make_double2(0.0, 0.0);
-    for (int i = 0; i < a.ni; ++i) {
-      const double2 b = cscale(rp[base + i], invr[base + i]);
-      cl.x = fma(Wl[base + i], b.x, cl.x);
-      cl.y = fma(Wl[base + i], b.y, cl.y);
-      cr.x = fma(Wr[base + i], b.x, cr.x);
-      cr.y = fma(Wr[base + i], b.y, cr.y);
+    if (NI > 0) {
+      double2 x[NI > 0 ? NI : 1];
+      double iv[NI > 0 ? NI : 1], wl[NI > 0 ? NI : 1], wr[NI > 0 ? NI : 1];
+#pragma unroll
+      for (int i = 0; i < NI; ++i) {
+        x[i] = rp[base + i];
+        iv[i] = invr[base + i];
+        wl[i] = Wl[base + i];
+        wr[i] = Wr[base + i];
+      }
+#pragma unroll
+      for (int i = 0; i < NI; ++i) {
+        const double2 b = cscale(x[i], iv[i]);
+        cl.x = fma(wl[i], b.x, cl.x);
+        cl.y = fma(wl[i], b.y, cl.y);
+        cr.x = fma(wr[i], b.x, cr.x);
+        cr.y = fma(wr[i], b.y, cr.y);
+      }
+    } else {
+      for (int i = 0; i < ni; ++i) {
+        const double2 b = cscale(rp[base + i], invr[base + i]);
+        cl.x = fma(Wl[base + i], b.x, cl.x);
+        cl.y = fma(Wl[base + i], b.y, cl.y);
+        cr.x = fma(Wr[base + i], b.x, cr.x);
+        cr.y = fma(Wr[base + i], b.y, cr.y);
+      }
     }
     // bridge k = e - 1 (left of e) takes -cl; bridge k = e (right of e) takes -cr
-    if (e >= 1) sS[wave][e - 1] = cl;
-    sCr[wave][e] = cr;
+    if (e >= 1) sS[e - 1] = cl;
+    sCr[e] = cr;
   }
   // multipole moment over every radial row
   double2 m = make_double2(0.0, 0.0);
-  for (int r = lane; r < a.nr; r += 64) {
+  for (int r = tid; r < a.nr; r += 256) {
     const double2 x = rp[r];
     m.x = fma(x.x, rpl0[r], m.x);
     m.y = fma(x.y, rpl0[r], m.y);
@@ -405,55 +428,43 @@ __global__ void __launch_bounds__(256) k_poisson_bridge(const double2* __restric
     m.x += __shfl_xor(m.x, o, 64);
     m.y += __shfl_xor(m.y, o, 64);
   }
-  const double2 Q = cscale(m, a.fac1[L]);
-  // right-hand sides of the Schur system (wave-local LDS: the wavefront's
-  // own lanes wrote these entries, so a wavefront barrier orders them)
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  for (int k = lane; k < nb; k += 64) {
+  if ((tid & 63) == 0) sRed[tid >> 6] = m;
+  __syncthreads();
+  const double2 Q = cscale(cadd(cadd(sRed[0], sRed[1]), cadd(sRed[2], sRed[3])), a.fac1[L]);
+  for (int k = tid; k < nb; k += 256) {
     const int gi = (k + 1) * a.n - 1;
-    double2 s = cscale(rp[gi], invr[gi]);
-    s = csub(s, sS[wave][k]);
-    s = csub(s, sCr[wave][k]);
-    sS[wave][k] = s;
+    double2 sk = cscale(rp[gi], invr[gi]);
+    sk = csub(sk, sS[k]);
+    sk = csub(sk, sCr[k]);
+    sY[k] = sk;
   }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  if (lane == 0 && nb > 0) {
+  __syncthreads();
+  if (tid == 0 && nb > 0) {
     // forward: y_k = s_k - l_k y_(k-1); backward: x_k = (y_k - u_k x_(k+1)) / d_k
-    const double2* sv = sS[wave];
-    const double* lv = sL[wave];
-    const double* uv = sU[wave];
-    const double* dv = sD[wave];
-    double2* yv = sY[wave];
-    double2 y = sv[0];
-    yv[0] = y;
+    double2 y = sY[0];
+    sS[0] = y;
     for (int k = 1; k < nb; ++k) {
-      const double2 sk = sv[k];
-      const double lk = lv[k];
+      const double2 sk = sY[k];
+      const double lk = sL[k];
       y = make_double2(fma(-lk, y.x, sk.x), fma(-lk, y.y, sk.y));
-      yv[k] = y;
+      sS[k] = y;
     }
-    double2 x = cscale(y, dv[nb - 1]);
-    sCr[wave][nb - 1] = x;
+    double2 x = cscale(y, sD[nb - 1]);
+    sCr[nb - 1] = x;
     for (int k = nb - 2; k >= 0; --k) {
-      const double2 yk = yv[k];
-      const double uk = uv[k], dk = dv[k];
+      const double2 yk = sS[k];
+      const double uk = sU[k], dk = sD[k];
       x = make_double2(fma(-uk, x.x, yk.x) * dk, fma(-uk, x.y, yk.y) * dk);
-      sCr[wave][k] = x;
+      sCr[k] = x;
     }
+    a.Q[(size_t)p * a.L2 + L] = Q;
   }
-  if (lane == 0) a.Q[(size_t)p * a.L2 + L] = Q;
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  __syncthreads();
   const double* rpl1 = a.rpl1 + (size_t)L * a.nr;
   const double* fac2 = a.fac2 + (size_t)L * a.nr;
-  for (int k = lane; k < nb; k += 64) {
+  for (int k = tid; k < nb; k += 256) {
     const int gi = (k + 1) * a.n - 1;
-    const double2 x = sCr[wave][k];
+    const double2 x = sCr[k];
     xbp[k] = x;
     vp[gi] = make_double2((x.x + Q.x * rpl1[gi]) * fac2[gi], (x.y + Q.y * rpl1[gi]) * fac2[gi]);
   }
@@ -947,7 +958,10 @@ void launch_onebody(const double2* orb, double2* h01, const OneBodyArgs& a, hipS
 
 void launch_poisson(const double2* rho, double2* xb, double2* v, const PoissonArgs& a, hipStream_t s) {
   const int jobs = a.npair * a.L2;
-  hipLaunchKernelGGL(k_poisson_bridge, dim3((jobs + 3) / 4), dim3(256), 0, s, rho, v, xb, a);
+  if (a.ni == 19)
+    hipLaunchKernelGGL((k_poisson_bridge<19>), dim3(jobs), dim3(256), 0, s, rho, v, xb, a);
+  else
+    hipLaunchKernelGGL((k_poisson_bridge<0>), dim3(jobs), dim3(256), 0, s, rho, v, xb, a);
   const int nt = std::min(1024, (a.npair * a.ni + 63) / 64 * 64);
   hipLaunchKernelGGL(k_poisson_elem, dim3(a.nfe, a.L2), dim3(std::max(nt, 64)), 0, s, rho, v, xb, a);
 }

@@ -109,7 +109,7 @@ struct IslandArgs {
   int nstr_a, nstr_b;
   const int *det_of, *exA, *sgA, *exB, *sgB;
   const double2* cic;
-  double2* ci_work;       // (nact^2 + 3) * nstr_a * nstr_b; holds D_pq = E_pq C between the stages
+  double2* ci_work;       // (2 nact^2 + 2) * nstr_a * nstr_b; holds D_pq = E_pq C between the stages
   // dynamic LDS plan (island_lds_plan): CI work and string tables in LDS,
   // int2e in LDS at byte offset ci_lds_off
   int ci_lds, i2_lds;
