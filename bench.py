@@ -26,6 +26,7 @@ Prints one JSON line (rank 0).  --gpus above 1 is not provided.
 from __future__ import annotations
 
 import argparse
+import hashlib
 import json
 import os
 import sys
@@ -39,6 +40,64 @@ if ROOT not in sys.path:
 
 HBM_PEAK_GBS = 8000.0
 NSTEP_PROP = 2000
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "ne_l31_fp64_traffic.json")
+# device kernels behind each timing name (HBM counters are per device kernel)
+TIMING_KERNELS = {
+    "poisson": ["k_poisson_bridge", "k_poisson_elem"],
+    "onebody": ["k_onebody_el"],
+    "mfprod": ["k_mfprod_t"],
+    "island_a": ["k_island_a"],
+    "island_b": ["k_island_b"],
+    "dtorb_rk": ["k_dtorb"],
+    "fc_phase": ["k_fc_phase"],
+}
+
+
+def source_hash() -> str:
+    """Hash of the HIP sources, to tell whether recorded counters still
+    describe the kernels being run."""
+    h = hashlib.sha256()
+    d = os.path.join(ROOT, "td1c_amd", "csrc")
+    for name in sorted(os.listdir(d)):
+        if name.endswith((".hip", ".hpp", ".cpp")):
+            with open(os.path.join(d, name), "rb") as f:
+                h.update(name.encode())
+                h.update(f.read())
+    return h.hexdigest()[:16]
+
+
+def recorded_traffic(timing_name: str):
+    """HBM bytes per launch of a timed kernel from the recorded rocprofv3
+    FETCH_SIZE / WRITE_SIZE counters (scaled by the copy-kernel
+    calibration), or None when no record matches the current sources."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if rec.get("source_hash") != source_hash() or timing_name not in TIMING_KERNELS:
+        return None
+    total = 0.0
+    for k in TIMING_KERNELS[timing_name]:
+        if k not in rec.get("kernels", {}):
+            return None
+        e = rec["kernels"][k]
+        total += e["fetch_kib"] * 1024.0 * rec["copy_factor"]["FETCH_SIZE"]
+        total += e["write_kib"] * 1024.0 * rec["copy_factor"]["WRITE_SIZE"]
+    return total
+
+
+def copy_probe(nbytes: int, reps: int = 5):
+    """A plain device copy of nbytes, the calibration of the HBM counters."""
+    import torch
+
+    n = nbytes // 16
+    src = torch.ones(n, dtype=torch.complex128, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n * 16, "reps": reps}), flush=True)
 
 
 def kernel_bytes(name: str, s: dict) -> float:
@@ -105,11 +164,16 @@ def main():
     ap.add_argument("--dt", type=float, default=1.0e-6)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only run a device copy of this many bytes (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not provided; this build runs one propagation set on one GPU",
               file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe > 0:
+        copy_probe(args.copy_probe)
+        return
 
     import torch
 
@@ -191,7 +255,7 @@ def main():
         "propagations_per_s": 1.0 / (NSTEP_PROP * ms_step * 1e-3),
         "hbm_frac_step": step_bytes / (ms_step * 1e-3) / (HBM_PEAK_GBS * 1e9),
         "roofline": {"bound": "hbm", "kernel": dname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": recorded_traffic(dname),
                      "algorithmic_bytes_per_launch": abytes, "avg_launch_us": avg_s * 1e6},
         "ms_per_step_stream_launch": ms_step_stream,
         "kernel_ms_per_step": {k: v[0] / nprof for k, v in sorted(breakdown.items(), key=lambda kv: -kv[1][0])},

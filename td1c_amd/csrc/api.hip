@@ -86,7 +86,7 @@ struct td1c_ctx {
   DevBuf<double2> gram_partial, lprod, pprod;
   DevBuf<int> mapH, mapG, mapV, mapF, map2e;
   // island
-  DevBuf<double2> den1, den2, rden, rrden, int1e, int2e, xmat, dcic, ci_work;
+  DevBuf<double2> den1, den2, rden, rrden, int1e, int2e, xmat, dcic, ci_work, mfc;
   DevBuf<double> ene;
   DevBuf<int> det_of, exA, sgA, exB, sgB;
   double ene_fcore = 0.0;
@@ -118,7 +118,7 @@ struct td1c_ctx {
     for (auto* b : d) b->release();
     DevBuf<double2>* z[] = {&tdiag, &tzor, &lsp, &gsp, &orbg0, &rhoL, &vL, &vg, &p_xb, &p_Q,
                             &gram_partial, &lprod, &pprod, &den1, &den2, &rden, &rrden, &int1e, &int2e, &xmat,
-                            &dcic, &ci_work, &rk_w0, &rk_t, &rk_k};
+                            &dcic, &ci_work, &mfc, &rk_w0, &rk_t, &rk_k};
     for (auto* b : z) b->release();
     DevBuf<int>* i[] = {&mval, &pair_m, &pj, &pi, &ptype, &mapH, &mapG, &mapV, &mapF, &map2e,
                         &det_of, &exA, &sgA, &exB, &sgB};
@@ -408,6 +408,7 @@ int build_device(td1c_ctx* c) {
   HIP_TRY(c->ene.alloc(5));
   HIP_TRY(hipMemset(c->ene.p, 0, 5 * sizeof(double)));
   const size_t nfull = (size_t)S.nstr_a * S.nstr_b;
+  HIP_TRY(c->mfc.alloc((size_t)na * na * na * na + 2 * na * na));
   HIP_TRY(c->ci_work.alloc(std::max<size_t>(1, ((size_t)2 * S.nact * S.nact + 2) * nfull)));
   HIP_TRY(c->det_of.upload(flat_or_one(S.det_of)));
   HIP_TRY(c->exA.upload(flat_or_one(S.exA)));
@@ -440,6 +441,7 @@ IslandArgs island_args(td1c_ctx* c, const double2* cic, double dtime) {
   a.pprod = c->pprod.p; a.map2e = c->map2e.p;
   a.int1e = c->int1e.p; a.int2e = c->int2e.p; a.xmat = c->xmat.p; a.dcic = c->dcic.p;
   a.ene = c->ene.p;
+  a.mfc = c->mfc.p;
   island_lds_plan(a);
   return a;
 }
@@ -511,7 +513,7 @@ int enqueue_htotx(td1c_ctx* c, double dtime, const double* d_lf, const double2* 
     ma.nr = nr; ma.nlat = nl; ma.nfun = nf; ma.nfcore = nfc; ma.nfcore2 = S.nfcore2; ma.ncore = S.ncore; ma.nact = S.nact;
     ma.nradfc = S.nradfc; ma.npair = c->npair; ma.nelact2 = S.nelact[2];
     ma.pj = c->pj.p; ma.pi = c->pi.p; ma.ptype = c->ptype.p; ma.mval = c->mval.p;
-    ma.den1 = c->den1.p; ma.den2 = c->den2.p; ma.rden = c->rden.p;
+    ma.den1 = c->den1.p; ma.den2 = c->den2.p; ma.rden = c->rden.p; ma.mfc = c->mfc.p;
     { TIMED("mfprod"); launch_mfprod(c->orbg, c->vg.p, c->gorbg, c->v2orbg, ma, s); }
     { TIMED("theta2l_mf"); launch_transform(0, c->gorbg, c->gorb, c->m_g2l.p, c->b_g2l_mf.p, c->n_g2l_mf, nr, nl, L1, pad_cols(L1), nullptr, s); }
   } else {

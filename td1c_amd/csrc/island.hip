@@ -178,6 +178,9 @@ __global__ void __launch_bounds__(kT) k_island_a(const IslandArgs a) {
     for (int d = 0; d < nc; ++d) s = zadd(s, zmul(zconj(C[d]), dv[d]));
     s_d1[e] = s;
     a.den1[e] = s;
+    // actx weight of the mean-field kernel: -den1/2 where m matches
+    const bool mm = a.mval[a.ncore + x] == a.mval[a.ncore + y];
+    a.mfc[(size_t)n * n * n * n + e] = mm ? zscal(s, -0.5) : zero2();
   }
   __syncthreads();
   for (int e = threadIdx.x; e < n * n * n * n; e += kT) {
@@ -188,6 +191,11 @@ __global__ void __launch_bounds__(kT) k_island_a(const IslandArgs a) {
     for (int k = 0; k < nc; ++k) s = zadd(s, zmul(zconj(u[k]), w[k]));
     if (p == t) s = zsub(s, s_d1[r + n * q]);
     a.den2[e] = s;
+    // act2 coefficient of the mean-field kernel: den2 where the m values
+    // balance and |den2| > 1e-15 (the terms hprod_mfprod2 keeps), else 0
+    const int* mv = a.mval + a.ncore;
+    const bool keep = mv[q] + mv[t] == mv[p] + mv[r] && s.x * s.x + s.y * s.y > 1.0e-30;
+    a.mfc[e] = keep ? s : zero2();
   }
   // D goes to the global work buffer for stage B
   if (a.ci_lds)
@@ -199,7 +207,12 @@ __global__ void __launch_bounds__(kT) k_island_a(const IslandArgs a) {
     herm_eig(n, s_A, s_w, s_U);
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < n * n; e += kT) a.rden[e] = inv_entry(n, a.throcc1, s_U, s_w, e);
+  for (int e = threadIdx.x; e < n * n; e += kT) {
+    const double2 r = inv_entry(n, a.throcc1, s_U, s_w, e);
+    a.rden[e] = r;
+    const bool mm = a.mval[a.ncore + e % n] == a.mval[a.ncore + e / n];
+    a.mfc[(size_t)n * n * n * n + n * n + e] = mm ? r : zero2();
+  }
   if (a.ncore > 0) {
     if (n * 2 == a.nelact2) {
       for (int e = threadIdx.x; e < n * n; e += kT) a.rrden[e] = zero2();

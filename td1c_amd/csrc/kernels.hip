@@ -39,29 +39,38 @@ __device__ __forceinline__ void cfma(double2& acc, double2 a, double2 b) {
 // MODE 0: input read directly (optionally copied to `copy`); MODE 1: input
 // formed on the fly as the pair density conj(psi_j) psi_i on the theta grid
 // (hprod_mkrho2p fused in).
-// Block: 4 wavefronts x 64 radial lanes; wave w owns CPW output columns.
-// The matrix row segment is wave-uniform and is read with scalar loads.
+// Block: 64 radial lanes; its 4 wavefronts form 4/KS column groups of CPW
+// columns, and KS wavefronts per group split the contraction index (the
+// first adds the parts through LDS).  The matrix row segment is
+// wave-uniform and is read with scalar loads.
 // ---------------------------------------------------------------------------
-template <int MODE, int CPW>
+template <int MODE, int CPW, int KS>
 __global__ void __launch_bounds__(256) k_transform(const double2* __restrict__ in, double2* __restrict__ out,
                                                    const double* __restrict__ mats,
                                                    const TransformBatch* __restrict__ batch, int nr, int K, int C,
                                                    int Cp, double2* __restrict__ copy) {
+  constexpr int NG = 4 / KS;  // column groups per block
   const TransformBatch bt = batch[blockIdx.z];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int c0 = __builtin_amdgcn_readfirstlane((blockIdx.y * 4 + wave) * CPW);
+  const int grp = wave % NG, ks = wave / NG;
+  const int c0 = (blockIdx.y * NG + grp) * CPW;
   const int r = blockIdx.x * 64 + lane;
-  if (c0 >= C) return;
-  const double* __restrict__ M = mats + (size_t)bt.mat * K * Cp + c0;
+  // KS wavefronts split the contraction index; the first of them adds the parts
+  const int kc = (K + KS - 1) / KS;
+  const int k0 = __builtin_amdgcn_readfirstlane(ks * kc);
+  const int k1 = min(K, k0 + kc);
+  const bool col_ok = c0 < C;
+  const double* __restrict__ M = mats + (size_t)bt.mat * K * Cp + (col_ok ? c0 : 0);
   double2 acc[CPW];
 #pragma unroll
   for (int c = 0; c < CPW; ++c) acc[c] = make_double2(0.0, 0.0);
-  if (r < bt.rows) {
+  const bool row_ok = r < bt.rows;
+  if (row_ok && col_ok) {
     const double2* __restrict__ x0 = in + (size_t)bt.in0 + r;
     const double2* __restrict__ x1 = in + (size_t)bt.in1 + r;
 #pragma unroll 8
-    for (int k = 0; k < K; ++k) {
+    for (int k = k0; k < k1; ++k) {
       double2 x;
       if (MODE == 0) {
         x = x0[(size_t)k * nr];
@@ -77,6 +86,22 @@ __global__ void __launch_bounds__(256) k_transform(const double2* __restrict__ i
         acc[c].y = fma(x.y, m, acc[c].y);
       }
     }
+  }
+  if (KS > 1) {
+    __shared__ double2 red[KS > 1 ? KS - 1 : 1][NG][CPW][64];
+    if (ks > 0) {
+#pragma unroll
+      for (int c = 0; c < CPW; ++c) red[ks - 1][grp][c][lane] = acc[c];
+    }
+    __syncthreads();
+    if (ks == 0) {
+#pragma unroll
+      for (int c = 0; c < CPW; ++c)
+#pragma unroll
+        for (int j = 0; j < KS - 1; ++j) acc[c] = cadd(acc[c], red[j][grp][c][lane]);
+    }
+  }
+  if (ks == 0 && row_ok && col_ok) {
 #pragma unroll
     for (int c = 0; c < CPW; ++c)
       if (c0 + c < C) out[(size_t)bt.out + (size_t)(c0 + c) * nr + r] = acc[c];
@@ -651,8 +676,7 @@ __global__ void __launch_bounds__(256) k_mfprod_t(const double2* __restrict__ or
   for (int ja = 0; ja < NA; ++ja) {
     wd[ja] = make_double2(0.0, 0.0);
 #pragma unroll
-    for (int ka = 0; ka < NA; ++ka)
-      if (a.mval[NCORE + ka] == a.mval[NCORE + ja]) cfma(wd[ja], psi[NCORE + ka], cscale(a.den1[ka + NA * ja], -0.5));
+    for (int ka = 0; ka < NA; ++ka) cfma(wd[ja], psi[NCORE + ka], a.mfc[NA * NA * NA * NA + ka + NA * ja]);
   }
   double2 d2v_dc = make_double2(0.0, 0.0);
   double2 vact[NA2];
@@ -706,20 +730,14 @@ __global__ void __launch_bounds__(256) k_mfprod_t(const double2* __restrict__ or
 #pragma unroll
         for (int ka = 0; ka < NA; ++ka)
 #pragma unroll
-          for (int la = 0; la < NA; ++la) {
-            if (a.mval[NCORE + ia] + a.mval[NCORE + ka] != a.mval[NCORE + ja] + a.mval[NCORE + la]) continue;
-            const double2 dens = a.den2[ja + NA * (ia + NA * (la + NA * ka))];
-            if (dens.x * dens.x + dens.y * dens.y <= 1.0e-30) continue;
-            cfma(d, dens, vact[ka * NA + la]);
-          }
+          for (int la = 0; la < NA; ++la) cfma(d, a.mfc[ja + NA * (ia + NA * (la + NA * ka))], vact[ka * NA + la]);
         cfma(t[ia], psi[NCORE + ja], d);
         if (ia > ja) cfma(t[ja], psi[NCORE + ia], cconj(d));
       }
 #pragma unroll
     for (int ia = 0; ia < NA; ++ia)
 #pragma unroll
-      for (int ka = 0; ka < NA; ++ka)
-        if (a.mval[NCORE + ia] == a.mval[NCORE + ka]) cfma(av[NCORE + ka], a.rden[ia + NA * ka], t[ia]);
+      for (int ka = 0; ka < NA; ++ka) cfma(av[NCORE + ka], a.mfc[NA * NA * NA * NA + NA * NA + ia + NA * ka], t[ia]);
   }
 #pragma unroll
   for (int f = NFC; f < NF; ++f) {
@@ -776,22 +794,34 @@ __global__ void __launch_bounds__(256) k_gram(const GramArgs a, double2* __restr
   for (int u = 0; u < TA; ++u) Lp[u] = a0 + u < a.nL ? a.left[a0 + u] : nullptr;
 #pragma unroll
   for (int v = 0; v < TB; ++v) Rp[v] = b0 + v < a.nR ? a.right[b0 + v] : nullptr;
-  for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < a.len; q += (size_t)nqblk * 256) {
-    double2 x[TA], y[TB];
+  // two index values per trip, all their loads issued before the products
+  const size_t stride = (size_t)nqblk * 256;
+  size_t q = (size_t)blockIdx.x * 256 + threadIdx.x;
+  for (; q < a.len; q += 2 * stride) {
+    const size_t q1 = q + stride;
+    const bool has1 = q1 < a.len;
+    double2 x[2][TA], y[2][TB];
 #pragma unroll
-    for (int u = 0; u < TA; ++u) {
-      x[u] = Lp[u] ? Lp[u][q] : make_double2(0.0, 0.0);
-      x[u].y *= sa;
+    for (int h = 0; h < 2; ++h) {
+      const size_t qq = h == 0 ? q : (has1 ? q1 : q);
+#pragma unroll
+      for (int u = 0; u < TA; ++u) x[h][u] = Lp[u] ? Lp[u][qq] : make_double2(0.0, 0.0);
+#pragma unroll
+      for (int v = 0; v < TB; ++v) y[h][v] = Rp[v] ? Rp[v][qq] : make_double2(0.0, 0.0);
     }
 #pragma unroll
-    for (int v = 0; v < TB; ++v) y[v] = Rp[v] ? Rp[v][q] : make_double2(0.0, 0.0);
+    for (int h = 0; h < 2; ++h) {
+      if (h == 1 && !has1) break;
 #pragma unroll
-    for (int u = 0; u < TA; ++u)
+      for (int u = 0; u < TA; ++u) {
+        const double2 xu = make_double2(x[h][u].x, x[h][u].y * sa);
 #pragma unroll
-      for (int v = 0; v < TB; ++v) {
-        cfma(s0[u][v], x[u], y[v]);
-        if (CR) cfma(s1[u][v], x[u], cconj(y[v]));
+        for (int v = 0; v < TB; ++v) {
+          cfma(s0[u][v], xu, y[h][v]);
+          if (CR) cfma(s1[u][v], xu, cconj(y[h][v]));
+        }
       }
+    }
   }
   // block reduction.  The tile holds NV = TA*TB*(CR ? 2 : 1) complex sums =
   // 2*NV doubles per lane; with 2*NV = 64 a transposing butterfly leaves lane
@@ -926,20 +956,29 @@ __global__ void k_mask(double2* __restrict__ wfn, const double* __restrict__ mas
 // ---------------------------------------------------------------------------
 // explicit instantiations / launch wrappers
 // ---------------------------------------------------------------------------
+template <int MODE>
+static void transform_variant(int ks, dim3 grid, hipStream_t s, const double2* in, double2* out, const double* mats,
+                              const TransformBatch* batch, int nr, int K, int C, int Cp, double2* copy) {
+  if (ks == 1)
+    hipLaunchKernelGGL((k_transform<MODE, 8, 1>), grid, dim3(256), 0, s, in, out, mats, batch, nr, K, C, Cp, copy);
+  else if (ks == 2)
+    hipLaunchKernelGGL((k_transform<MODE, 8, 2>), grid, dim3(256), 0, s, in, out, mats, batch, nr, K, C, Cp, copy);
+  else
+    hipLaunchKernelGGL((k_transform<MODE, 8, 4>), grid, dim3(256), 0, s, in, out, mats, batch, nr, K, C, Cp, copy);
+}
+
 void launch_transform(int mode, const double2* in, double2* out, const double* mats, const TransformBatch* batch,
                       int nbatch, int nr, int K, int C, int Cp, double2* copy, hipStream_t s) {
   const int rt = (nr + 63) / 64;
-  // 8 columns per wave when that gives enough blocks, else 4
-  const int blocks8 = rt * ((C + 31) / 32) * nbatch;
-  const int cpw = blocks8 >= 512 ? 8 : 4;
-  dim3 grid(rt, (C + 4 * cpw - 1) / (4 * cpw), nbatch), block(256);
-  if (mode == 0) {
-    if (cpw == 8) hipLaunchKernelGGL((k_transform<0, 8>), grid, block, 0, s, in, out, mats, batch, nr, K, C, Cp, copy);
-    else hipLaunchKernelGGL((k_transform<0, 4>), grid, block, 0, s, in, out, mats, batch, nr, K, C, Cp, copy);
-  } else {
-    if (cpw == 8) hipLaunchKernelGGL((k_transform<1, 8>), grid, block, 0, s, in, out, mats, batch, nr, K, C, Cp, nullptr);
-    else hipLaunchKernelGGL((k_transform<1, 4>), grid, block, 0, s, in, out, mats, batch, nr, K, C, Cp, nullptr);
-  }
+  // 8 columns per wavefront; split the contraction over 2 or 4 wavefronts
+  // only while the grid is too small to fill the chip (a split re-reads
+  // the input once per extra column block)
+  int ks = 1;
+  while (ks < 4 && rt * ((C + 8 * (4 / ks) - 1) / (8 * (4 / ks))) * nbatch < 800) ks *= 2;
+  const int cols = 8 * (4 / ks);
+  dim3 grid(rt, (C + cols - 1) / cols, nbatch);
+  if (mode == 0) transform_variant<0>(ks, grid, s, in, out, mats, batch, nr, K, C, Cp, copy);
+  else transform_variant<1>(ks, grid, s, in, out, mats, batch, nr, K, C, Cp, nullptr);
 }
 
 bool launch_onebody_el(const double2* orb, double2* h01, const OneBodyElArgs& a, hipStream_t s) {
@@ -997,8 +1036,9 @@ void launch_fc_phase(const double2* orbg0, double2* orbg, const double* xr, cons
 }
 
 void launch_gram(const GramArgs& a, double2* partial, int maxblk, double2* out, hipStream_t s) {
-  // about 8 index values per thread, at most maxblk blocks along the index
-  const int nblk = (int)std::max<size_t>(1, std::min<size_t>(maxblk, (a.len + 2047) / 2048));
+  // about 4 index values per thread (two trips of two), at most maxblk
+  // blocks along the index
+  const int nblk = (int)std::max<size_t>(1, std::min<size_t>(maxblk, (a.len + 1023) / 1024));
   if (a.with_conj_right) {
     constexpr int TA = 4, TB = 4;
     dim3 grid(nblk, ((a.nL + TA - 1) / TA) * ((a.nR + TB - 1) / TB));

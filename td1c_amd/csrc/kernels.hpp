@@ -71,6 +71,9 @@ struct MfArgs {
   int nr, nlat, nfun, nfcore, nfcore2, ncore, nact, nradfc, npair, nelact2;
   const int *pj, *pi, *ptype, *mval;
   const double2 *den1, *den2, *rden;
+  // m-filtered coefficients written by the CI island: act2 [n^4] (den2
+  // layout), actx weights -den1/2 [n^2], rden [n^2]
+  const double2* mfc;
 };
 
 constexpr int kMaxGram = 1024;   // nL * nR entries of one Gram launch (4 per thread)
@@ -123,6 +126,7 @@ struct IslandArgs {
   const int* map2e;        // [nact^4] (i,j,k,l) with i >= j, -1 otherwise
   // stage B outputs
   double2 *int1e, *int2e, *xmat, *dcic;
+  double2* mfc;            // mean-field coefficients (see MfArgs)
   double* ene;             // ene_fcore (input), dcore, core, act, tot
   // Runge-Kutta stage update of the CI part, done after dcic (kout == null: none):
   // kout = dcic; acc += dtw * dcic; next = w0 + sum cprev[j] kprev[j] + cnew * dcic
