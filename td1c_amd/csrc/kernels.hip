@@ -31,6 +31,15 @@ __device__ __forceinline__ void cfma(double2& acc, double2 a, double2 b) {
   acc.y = fma(a.y, b.x, acc.y);
 }
 
+// Workgroups are dealt to the 8 XCDs round-robin by linear id, and each XCD
+// has its own L2.  This maps the hardware linear id to a logical tile id so
+// that every XCD gets one contiguous range of logical tiles: tiles that
+// re-read the same input then share an L2.
+__device__ __forceinline__ int xcd_remap(int lin, int n) {
+  const int x = lin & 7, j = lin >> 3, q = n >> 3, r = n & 7;
+  return x * q + min(x, r) + j;
+}
+
 // ---------------------------------------------------------------------------
 // Legendre transforms along the middle axis (bas_sph2ang1/ang2sph1/
 // ang2sph2/sph2ang2, src/bas/*.F90): out[b][c][r] = sum_k in_b[k][r] M_b[k][c]
@@ -50,12 +59,17 @@ __global__ void __launch_bounds__(256) k_transform(const double2* __restrict__ i
                                                    const TransformBatch* __restrict__ batch, int nr, int K, int C,
                                                    int Cp, double2* __restrict__ copy) {
   constexpr int NG = 4 / KS;  // column groups per block
-  const TransformBatch bt = batch[blockIdx.z];
+  // logical tile order: column block fastest, so the column blocks that
+  // re-read one input tile run on one XCD
+  const int ny = gridDim.y, nx = gridDim.x;
+  const int t = xcd_remap(blockIdx.x + nx * (blockIdx.y + ny * blockIdx.z), nx * ny * gridDim.z);
+  const int by = t % ny, bx = (t / ny) % nx, bz = t / (ny * nx);
+  const TransformBatch bt = batch[bz];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int grp = wave % NG, ks = wave / NG;
-  const int c0 = (blockIdx.y * NG + grp) * CPW;
-  const int r = blockIdx.x * 64 + lane;
+  const int c0 = (by * NG + grp) * CPW;
+  const int r = bx * 64 + lane;
   // KS wavefronts split the contraction index; the first of them adds the parts
   const int kc = (K + KS - 1) / KS;
   const int k0 = __builtin_amdgcn_readfirstlane(ks * kc);
@@ -377,7 +391,8 @@ template <int NI>
 __global__ void __launch_bounds__(256) k_poisson_bridge(const double2* __restrict__ rho, double2* __restrict__ v,
                                                         double2* __restrict__ xb, const PoissonArgs a) {
   const int tid = threadIdx.x;
-  const int L = blockIdx.x / a.npair, p = blockIdx.x % a.npair;
+  const int job = xcd_remap(blockIdx.x, gridDim.x);  // the blocks of one L share an XCD (and its L2)
+  const int L = job / a.npair, p = job % a.npair;
   const int nb = a.nfe - 1;
   const int ni = NI > 0 ? NI : a.ni;
   __shared__ double2 sS[kMaxBridge + 1];
@@ -777,7 +792,12 @@ __global__ void k_fc_phase(const double2* __restrict__ orbg0, double2* __restric
 template <int TA, int TB, bool CR>
 __global__ void __launch_bounds__(256) k_gram(const GramArgs a, double2* __restrict__ partial, int nqblk) {
   const int ntb = (a.nR + TB - 1) / TB;
-  const int ta = blockIdx.y / ntb, tb = blockIdx.y % ntb;
+  // logical order: (a, b) tile fastest, so the tiles that read one index
+  // range run on one XCD
+  const int nt = gridDim.y;
+  const int lt = xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * nt);
+  const int tile = lt % nt, qb = lt / nt;
+  const int ta = tile / ntb, tb = tile % ntb;
   const int a0 = ta * TA, b0 = tb * TB;
   const double sa = a.conj_left ? -1.0 : 1.0;
   double2 s0[TA][TB], s1[CR ? TA : 1][CR ? TB : 1];
@@ -796,7 +816,7 @@ __global__ void __launch_bounds__(256) k_gram(const GramArgs a, double2* __restr
   for (int v = 0; v < TB; ++v) Rp[v] = b0 + v < a.nR ? a.right[b0 + v] : nullptr;
   // two index values per trip, all their loads issued before the products
   const size_t stride = (size_t)nqblk * 256;
-  size_t q = (size_t)blockIdx.x * 256 + threadIdx.x;
+  size_t q = (size_t)qb * 256 + threadIdx.x;
   for (; q < a.len; q += 2 * stride) {
     const size_t q1 = q + stride;
     const bool has1 = q1 < a.len;
@@ -867,7 +887,7 @@ __global__ void __launch_bounds__(256) k_gram(const GramArgs a, double2* __restr
       t.x = (red[0][2 * e] + red[1][2 * e]) + (red[2][2 * e] + red[3][2 * e]);
       t.y = (red[0][2 * e + 1] + red[1][2 * e + 1]) + (red[2][2 * e + 1] + red[3][2 * e + 1]);
       const int entry = (a0 + u) * a.nR + (b0 + w);
-      partial[((size_t)entry * 2 + c) * nqblk + blockIdx.x] = t;
+      partial[((size_t)entry * 2 + c) * nqblk + qb] = t;
     }
   }
 }
