@@ -516,8 +516,10 @@ __global__ void __launch_bounds__(256) k_poisson_bridge(const double2* __restric
 //   x_I = G_e b_I - W_l x_left - W_r x_right,  v = (x + Q rpl1) * fac2.
 __global__ void __launch_bounds__(1024) k_poisson_elem(const double2* __restrict__ rho, double2* __restrict__ v,
                                                       const double2* __restrict__ xb, const PoissonArgs a) {
-  const int e = blockIdx.x;
-  const int L = blockIdx.y;
+  // consecutive elements of one L on one XCD: neighbours share cache lines
+  const int lt = xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+  const int e = lt % gridDim.x;
+  const int L = lt / gridDim.x;
   const int ni = a.ni;
   const int nt = blockDim.x;
   const int ngroup = nt / ni;
