@@ -142,6 +142,14 @@ int td1c_htot(td1c_ctx* ctx, double dtime, const double* lfield,
  * ene_tot as bound by hprod_bind_); synchronises the stream. */
 int td1c_get_energies(td1c_ctx* ctx, double ene[5]);
 
+/* Length-form dipole <z> of a device wave function, written to d_out (one
+ * double in device memory) on `stream`: hprod_dipole's dip(1)
+ * (src/hprod/hprod_dipole.F90:2-146), i.e. hprod_trace1 of z with the 1-RDM,
+ * cores counted twice, frozen cores with the velocity-gauge phase of lfield.
+ * The velocity and acceleration forms are not provided. */
+int td1c_dipole(td1c_ctx* ctx, const double* lfield, const double* d_wfn,
+                const double* d_cic, double* d_out, void* stream);
+
 /* One explicit Runge-Kutta step of the whole wave function (orbitals then
  * CI) held on device: clcrk::prop with rk_order 1..4 ("standard").
  * lfield_stages holds the 9 field values at t + c_s*dt for each stage

@@ -52,6 +52,8 @@ def lib():
         L.orc_table.argtypes = [vp, ctypes.c_char_p]
         L.orc_table.restype = vp
         L.orc_energies.argtypes = [vp, vp]
+        L.orc_dipole.argtypes = [vp, vp, vp, vp]
+        L.orc_dipole.restype = ctypes.c_double
         L.orc_band_chol.argtypes = [ctypes.c_int, ctypes.c_int, vp]
         L.orc_band_chol.restype = ctypes.c_int
         L.orc_band_chol_solve.argtypes = [ctypes.c_int, ctypes.c_int, vp, vp]
@@ -128,6 +130,12 @@ class Oracle:
         w = np.array(wfn, dtype=np.complex128, copy=True)
         lib().orc_mask(self.h, _p(w))
         return w
+
+    def dipole(self, lfield, wfn: np.ndarray) -> float:
+        """Length-form dipole <z> of a wave function (hprod_dipole dip(1))."""
+        w = np.ascontiguousarray(wfn, dtype=np.complex128)
+        lf = np.ascontiguousarray(lfield, dtype=np.float64)
+        return float(lib().orc_dipole(self.h, _p(lf), _p(w[: self.norb]), _p(w[self.norb:])))
 
     def energies(self) -> np.ndarray:
         e = np.zeros(5)

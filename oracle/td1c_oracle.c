@@ -1668,6 +1668,42 @@ const void* orc_table(const orc* o, const char* name) {
   return NULL;
 }
 
+/* Length-form dipole <z> (hprod_dipole dip(1), src/hprod/hprod_dipole.F90:
+ * 2-146): orbitals as hprod_orbin builds them for lfield (frozen cores with
+ * the velocity-gauge phase), den1 from the CI, z on every orbital
+ * (hprod_zprod_all with zfac = 1) and hprod_trace1 with the frozen cores
+ * included (src/hprod/hprod_trace1.F90:2-88). */
+double orc_dipole(orc* o, const double* lfield, const zc* wfn, const zc* cic) {
+  int nf = o->nfun, nb = o->nbas, nfc = o->nfcore, ncore = o->ncore, na = o->nact, nr = o->nr;
+  orc_htotx(o, 1.0, lfield, wfn, cic); /* orbin + den1 */
+  zc* zo = xcalloc((size_t)nb * nf, sizeof(zc));
+  if (nfc > 0) zprod(o, 1.0, o->orb, zo, 0, nfc, o->nradfc);
+  zprod(o, 1.0, o->orb, zo, nfc, nf, nr);
+  zc t = 0;
+  for (int ifun = 0; ifun < ncore; ++ifun) {
+    int dim = ifun < nfc ? o->nradfc : nr;
+    for (int l = abs(o->mval[ifun]); l <= o->lmax1; ++l)
+      for (int i = 0; i < dim; ++i) t += conj(o->orb[OIDX(i, l, ifun)]) * zo[OIDX(i, l, ifun)];
+  }
+  t *= 2.0;
+  zc* tw = xcalloc((size_t)nb, sizeof(zc));
+  for (int ia = 0; ia < na; ++ia) {
+    int ifun = ncore + ia;
+    memset(tw, 0, sizeof(zc) * nb);
+    for (int ja = 0; ja < na; ++ja) {
+      int jfun = ncore + ja;
+      if (o->mval[ifun] != o->mval[jfun]) continue;
+      for (int l = abs(o->mval[ifun]); l <= o->lmax1; ++l)
+        for (int i = 0; i < nr; ++i) tw[(size_t)l * nr + i] += o->orb[OIDX(i, l, jfun)] * o->den1[ja + na * ia];
+    }
+    for (int l = 0; l <= o->lmax1; ++l)
+      for (int i = 0; i < nr; ++i) t += conj(tw[(size_t)l * nr + i]) * zo[OIDX(i, l, ifun)];
+  }
+  free(tw);
+  free(zo);
+  return creal(t);
+}
+
 void orc_energies(const orc* o, double* e) {
   e[0] = o->ene_fcore;
   e[1] = o->ene_dcore;

@@ -61,6 +61,8 @@ def load_library():
     L.td1c_propagate.argtypes = [vp, ctypes.POINTER(TD1CField), ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                  ctypes.c_int, vp, ctypes.c_int, vp]
     L.td1c_propagate.restype = ctypes.c_int
+    L.td1c_dipole.argtypes = [vp, vp, vp, vp, vp, vp]
+    L.td1c_dipole.restype = ctypes.c_int
     L.td1c_mask.argtypes = [vp, vp, vp]
     L.td1c_mask.restype = ctypes.c_int
     L.td1c_field_value.argtypes = [ctypes.POINTER(TD1CField), ctypes.c_double, vp]
@@ -183,6 +185,23 @@ class HProd:
         e = np.zeros(5)
         _check(load_library().td1c_get_energies(self.h, _ptr(e)), "td1c_get_energies")
         return e
+
+    def dipole(self, lfield: Sequence[float], wfn, out=None, stream=None):
+        """Length-form dipole <z> (hprod_dipole dip(1)) of a device wave
+        function; returns a float (synchronising) unless `out`, a float64
+        CUDA tensor of one element, is given."""
+        torch = self.torch
+        lf = np.ascontiguousarray(lfield, dtype=np.float64)
+        if lf.size != 9:
+            raise ValueError("lfield must have 9 entries")
+        ret = out is None
+        if out is None:
+            out = torch.zeros(1, dtype=torch.float64, device=wfn.device)
+        base = wfn.data_ptr()
+        _check(load_library().td1c_dipole(self.h, _ptr(lf), ctypes.c_void_p(base),
+                                          ctypes.c_void_p(base + 16 * self.norb), ctypes.c_void_p(out.data_ptr()),
+                                          self._stream(stream)), "td1c_dipole")
+        return float(out.item()) if ret else out
 
     def mask(self, wfn, stream=None):
         _check(load_library().td1c_mask(self.h, ctypes.c_void_p(wfn.data_ptr()), self._stream(stream)), "td1c_mask")

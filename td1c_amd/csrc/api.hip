@@ -86,7 +86,7 @@ struct td1c_ctx {
   DevBuf<double2> gram_partial, lprod, pprod;
   DevBuf<int> mapH, mapG, mapV, mapF, map2e;
   // island
-  DevBuf<double2> den1, den2, rden, rrden, int1e, int2e, xmat, dcic, ci_work, mfc;
+  DevBuf<double2> den1, den2, rden, rrden, int1e, int2e, xmat, dcic, ci_work, mfc, dip_part;
   DevBuf<double> ene;
   DevBuf<int> det_of, exA, sgA, exB, sgB;
   double ene_fcore = 0.0;
@@ -118,7 +118,7 @@ struct td1c_ctx {
     for (auto* b : d) b->release();
     DevBuf<double2>* z[] = {&tdiag, &tzor, &lsp, &gsp, &orbg0, &rhoL, &vL, &vg, &p_xb, &p_Q,
                             &gram_partial, &lprod, &pprod, &den1, &den2, &rden, &rrden, &int1e, &int2e, &xmat,
-                            &dcic, &ci_work, &mfc, &rk_w0, &rk_t, &rk_k};
+                            &dcic, &ci_work, &mfc, &dip_part, &rk_w0, &rk_t, &rk_k};
     for (auto* b : z) b->release();
     DevBuf<int>* i[] = {&mval, &pair_m, &pj, &pi, &ptype, &mapH, &mapG, &mapV, &mapF, &map2e,
                         &det_of, &exA, &sgA, &exB, &sgB};
@@ -409,6 +409,7 @@ int build_device(td1c_ctx* c) {
   HIP_TRY(hipMemset(c->ene.p, 0, 5 * sizeof(double)));
   const size_t nfull = (size_t)S.nstr_a * S.nstr_b;
   HIP_TRY(c->mfc.alloc((size_t)na * na * na * na + 2 * na * na));
+  HIP_TRY(c->dip_part.alloc((size_t)((S.nr + 255) / 256) * S.L1));
   HIP_TRY(c->ci_work.alloc(std::max<size_t>(1, ((size_t)2 * S.nact * S.nact + 2) * nfull)));
   HIP_TRY(c->det_of.upload(flat_or_one(S.det_of)));
   HIP_TRY(c->exA.upload(flat_or_one(S.exA)));
@@ -461,14 +462,13 @@ struct CiStage {
   double cnew = 0.0;
 };
 
-// d_lf: the stage's lfield[9] in device memory.
-int enqueue_htotx(td1c_ctx* c, double dtime, const double* d_lf, const double2* wfn, const double2* cic,
-                  const CiStage& cs, hipStream_t s) {
+// hprod_orbin + hprod_fcorb: the current orbitals in the work array (l and
+// theta grid), the frozen cores with the velocity-gauge phase of d_lf.
+void enqueue_orbin(td1c_ctx* c, const double* d_lf, const double2* wfn, hipStream_t s) {
   Setup& S = c->S;
-  const int nr = S.nr, nf = S.nfun, nfc = S.nfcore, L1 = S.L1, L2 = S.L2, nl = S.nlat;
-  const size_t nb = (size_t)c->nbas;
-  // current dynamical orbitals (hprod_orbin): copied into the work array by
-  // the l -> theta transform that reads them
+  const int nr = S.nr, nfc = S.nfcore, L1 = S.L1, nl = S.nlat;
+  // dynamical orbitals: copied into the work array by the l -> theta
+  // transform that reads them
   {
     TIMED("l2theta");
     launch_transform(0, wfn, c->orbg, c->m_l2g.p, c->b_l2g_dyn.p, c->n_l2g_dyn, nr, L1, nl, pad_cols(nl), c->orb, s);
@@ -478,6 +478,15 @@ int enqueue_htotx(td1c_ctx* c, double dtime, const double* d_lf, const double2* 
     { TIMED("fc_phase"); launch_fc_phase(c->orbg0.p, c->orbg, c->xr.p, c->cost.p, d_lf, nr, S.nradfc, nl, nfc, s); }
     { TIMED("theta2l_fc"); launch_transform(0, c->orbg, c->orb, c->m_g2l.p, c->b_g2l_fc.p, c->n_g2l_fc, nr, nl, L1, pad_cols(L1), nullptr, s); }
   }
+}
+
+// d_lf: the stage's lfield[9] in device memory.
+int enqueue_htotx(td1c_ctx* c, double dtime, const double* d_lf, const double2* wfn, const double2* cic,
+                  const CiStage& cs, hipStream_t s) {
+  Setup& S = c->S;
+  const int nr = S.nr, nf = S.nfun, nfc = S.nfcore, L1 = S.L1, L2 = S.L2, nl = S.nlat;
+  const size_t nb = (size_t)c->nbas;
+  enqueue_orbin(c, d_lf, wfn, s);
   IslandArgs ia = island_args(c, cic, dtime);
   if (S.nact > 0) { TIMED("island_a"); launch_island_a(ia, s); }
   // one-body part
@@ -727,6 +736,26 @@ int td1c_get_energies(td1c_ctx* c, double ene[5]) {
   ene[0] = c->ene_fcore;
   ene[2] += c->ene_fcore;
   ene[4] += c->ene_fcore;
+  return 0;
+}
+
+int td1c_dipole(td1c_ctx* c, const double* lfield, const double* d_wfn, const double* d_cic, double* d_out,
+                void* stream) {
+  if (!c || !lfield || !d_wfn || !d_out) return fail(-1, "null argument");
+  if (c->S.nfcore > 0 && !c->wfn0_set) return fail(-4, "frozen core needs td1c_set_wfn0 first");
+  if (c->S.nact > 0 && !d_cic) return fail(-1, "null CI vector");
+  hipStream_t s = (hipStream_t)stream;
+  Setup& S = c->S;
+  HIP_TRY(hipMemcpyAsync(c->lf_dev.p, lfield, 9 * sizeof(double), hipMemcpyHostToDevice, s));
+  enqueue_orbin(c, c->lf_dev.p, reinterpret_cast<const double2*>(d_wfn), s);
+  IslandArgs ia = island_args(c, reinterpret_cast<const double2*>(d_cic), 1.0);
+  if (S.nact > 0) launch_island_a(ia, s);
+  DipoleArgs a{};
+  a.nr = S.nr; a.L1 = S.L1; a.nfun = S.nfun; a.nfcore = S.nfcore; a.ncore = S.ncore; a.nact = S.nact;
+  a.nradfc = S.nradfc; a.mmax1 = S.mmax1; a.mval = c->mval.p; a.xr = c->xr.p; a.alph = c->alph.p;
+  a.orb = c->orb; a.den1 = c->den1.p;
+  launch_dipole(a, c->dip_part.p, d_out, s);
+  HIP_TRY(hipGetLastError());
   return 0;
 }
 

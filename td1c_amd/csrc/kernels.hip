@@ -1076,6 +1076,70 @@ void launch_gram(const GramArgs& a, double2* partial, int maxblk, double2* out, 
 }
 
 // ---------------------------------------------------------------------------
+// Length-form dipole (hprod_dipole dip(1), src/hprod/hprod_dipole.F90:2-146):
+// z = r cos(theta) couples l and l+1 with r alpha_lm (hprod_zprod), traced
+// with the 1-RDM (hprod_trace1, src/hprod/hprod_trace1.F90:2-88): cores twice,
+// frozen cores over their nradfc rows.  Thread per (r, l); block partial
+// sums, then one block adds them in a fixed order.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_dipole(const DipoleArgs a, double2* __restrict__ partial) {
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  const int l = blockIdx.y;
+  double2 acc = make_double2(0.0, 0.0);
+  if (r < a.nr) {
+    const double xr = a.xr[r];
+    for (int i = 0; i < a.nfun; ++i) {
+      const int m = a.mval[i], am = m < 0 ? -m : m;
+      if (l < am) continue;
+      if (i < a.nfcore && r >= a.nradfc) continue;
+      const double* al = a.alph + (size_t)(m + a.mmax1) * a.L1;
+      const double2* oi = a.orb + (size_t)i * a.L1 * a.nr + r;
+      double2 z = make_double2(0.0, 0.0);
+      if (l + 1 < a.L1) cfma(z, oi[(size_t)(l + 1) * a.nr], make_double2(xr * al[l], 0.0));
+      if (l - 1 >= am) cfma(z, oi[(size_t)(l - 1) * a.nr], make_double2(xr * al[l - 1], 0.0));
+      if (i < a.ncore) {
+        cfma(acc, cconj(oi[(size_t)l * a.nr]), cscale(z, 2.0));
+      } else {
+        const int ia = i - a.ncore;
+        double2 t = make_double2(0.0, 0.0);
+        for (int jb = 0; jb < a.nact; ++jb) {
+          const int j = a.ncore + jb;
+          if (a.mval[j] != m) continue;
+          cfma(t, a.orb[((size_t)j * a.L1 + l) * a.nr + r], a.den1[jb + a.nact * ia]);
+        }
+        cfma(acc, cconj(t), z);
+      }
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    acc.x += __shfl_xor(acc.x, o, 64);
+    acc.y += __shfl_xor(acc.y, o, 64);
+  }
+  __shared__ double2 red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    partial[blockIdx.y * gridDim.x + blockIdx.x] = cadd(cadd(red[0], red[1]), cadd(red[2], red[3]));
+}
+
+__global__ void __launch_bounds__(256) k_dipole_final(const double2* __restrict__ partial, int n,
+                                                      double* __restrict__ out) {
+  double s = 0.0;
+  for (int q = threadIdx.x; q < n; q += 256) s += partial[q].x;
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  __shared__ double red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) out[0] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+void launch_dipole(const DipoleArgs& a, double2* partial, double* out, hipStream_t s) {
+  dim3 grid((a.nr + 255) / 256, a.L1);
+  hipLaunchKernelGGL(k_dipole, grid, dim3(256), 0, s, a, partial);
+  hipLaunchKernelGGL(k_dipole_final, dim3(1), dim3(256), 0, s, partial, (int)(grid.x * grid.y), out);
+}
+
+// ---------------------------------------------------------------------------
 // Small helpers that keep memset/memcpy nodes out of captured graphs.
 // ---------------------------------------------------------------------------
 __global__ void k_copy(double2* __restrict__ y, const double2* __restrict__ x, size_t n) {

@@ -152,6 +152,17 @@ void launch_mfprod(const double2* orbg, const double2* vg, double2* gorbg, doubl
 void launch_fc_phase(const double2* orbg0, double2* orbg, const double* xr, const double* cost, const double* lf, int nr,
                      int nradfc, int nlat, int nfc, hipStream_t s);
 void launch_gram(const GramArgs& a, double2* partial, int nblk, double2* out, hipStream_t s);
+// Length-form dipole <z> = 2 sum_core <i|z|i> + sum_ab conj(den1(b,a)) <b|z|a>
+struct DipoleArgs {
+  int nr, L1, nfun, nfcore, ncore, nact, nradfc, mmax1;
+  const int* mval;
+  const double* xr;
+  const double* alph;     // [2mmax1+1][L1]
+  const double2* orb;     // orbitals as hprod_orbin builds them
+  const double2* den1;    // [nact][nact]
+};
+constexpr int kDipoleMaxBlocks = 4096;
+void launch_dipole(const DipoleArgs& a, double2* partial, double* out, hipStream_t s);
 void launch_copy(double2* y, const double2* x, size_t n, hipStream_t s);
 void launch_zero(double2* y, size_t n, hipStream_t s);
 void launch_field_stages(const Pulse& p, PropClock* st, const StageNodes& nodes, double* lf, hipStream_t s);

@@ -177,3 +177,21 @@ def test_graph_propagation_matches_oracle(gpu_lib, name):
         torch.cuda.synchronize()
         got = d.cpu().numpy()
         assert _rel(got, ref) < RTOL, (call, _rel(got, ref))
+
+
+@pytest.mark.parametrize("name,lf", [("he_small", C.LF_LEN), ("dc3_small", C.LF_LEN), ("ne_small", C.LF_VEL),
+                                     ("cas_fc_small", C.LF_VEL), ("cas_small", C.LF_LEN)])
+def test_dipole_matches_oracle(gpu_lib, name, lf):
+    """td1c_dipole (hprod_dipole dip(1)) against the oracle."""
+    import td1c_amd
+    from oracle.oracle import Oracle
+
+    prob = getattr(C, name)()
+    o = Oracle(prob)
+    w = C.wfn_for(prob, o.lcic)
+    o.set_wfn0(w)
+    ref = o.dipole(lf, w)
+    hp = td1c_amd.HProd(prob)
+    hp.set_wfn0(w)
+    got = hp.dipole(lf, hp.to_device(w))
+    assert abs(got - ref) <= RTOL * max(1.0, abs(ref)), (got, ref)

@@ -516,3 +516,43 @@ def test_equations_of_motion_conserve_energy_and_orthonormality(name):
     dE = (energy(w + eps * d) - energy(w - eps * d)) / (2 * eps)
     dR = (energy(w + eps * r) - energy(w - eps * r)) / (2 * eps)
     assert abs(dE) < 1e-7 * abs(dR), (dE, dR)
+
+
+def test_dipole_closed_form_and_core_equivalence():
+    """orc_dipole (hprod_dipole dip(1) + hprod_trace1): a doubly occupied
+    (1s + 2p0)/sqrt(2) of hydrogen has <z> = 2 * 128 sqrt(2)/243; a doubly
+    occupied active orbital gives the dipole of the same dynamical core, and a
+    frozen 1s core (length gauge) that of the same 1s held dynamically."""
+    prob = _prob(nfe=30, rmax=60.0, znuc=1.0, ndcore=1, nel_alpha=1, nel_beta=1)
+    o = O.Oracle(prob)
+    r, w = W.gll_grid(prob)
+    nr = prob.nr
+    wf = np.zeros(o.wfn_size, complex)
+    wf[:nr] = 2 * r * np.exp(-r) * np.sqrt(w) / math.sqrt(2)
+    wf[nr:2 * nr] = r * r * np.exp(-r / 2) / (2 * math.sqrt(6)) * np.sqrt(w) / math.sqrt(2)
+    wf[-1] = 1.0
+    o.set_wfn0(wf)
+    assert abs(o.dipole(np.zeros(9), wf) - 2 * 128 * math.sqrt(2) / 243) < 1e-10
+
+    base = dict(nfe=6, ndvr=10, rmax=20.0, lmax1=3, lmax2=6, znuc=3.0, mval=[0, 0], nel_alpha=2, nel_beta=2)
+    pa = Problem(ndcore=2, nact=0, **base)
+    pb = Problem(ndcore=1, nact=1, **base)
+    pc = Problem(nfcore1=1, ndcore=1, lval=[0, 0], **base)
+    oa, ob, oc = O.Oracle(pa), O.Oracle(pb), O.Oracle(pc)
+    wa = W.make_wfn(pa, 21, oa.lcic)
+    nb = pa.nr * (pa.lmax1 + 1)
+    o0 = np.zeros(nb, complex)
+    o0[:pa.nr] = W.hydrogenic_1s(pa, 3.0)
+    o1 = wa[nb:2 * nb] - np.vdot(o0, wa[nb:2 * nb]) * o0
+    wa[:nb], wa[nb:2 * nb] = o0, o1 / np.linalg.norm(o1)
+    wb = np.zeros(ob.wfn_size, complex)
+    wb[: 2 * nb] = wa[: 2 * nb]
+    wb[2 * nb:] = 1.0
+    for o_, w_ in ((oa, wa), (ob, wb), (oc, wa)):
+        o_.set_wfn0(w_)
+    da, db, dc = (o_.dipole(C_LEN, w_) for o_, w_ in ((oa, wa), (ob, wb), (oc, wa)))
+    assert abs(da) > 1e-3
+    assert abs(da - db) < 1e-13 * abs(da) and abs(da - dc) < 1e-13 * abs(da)
+
+
+C_LEN = np.array([0.0, 0.0, 0.05, 0.0, 0.0, 0.05, 0.0, 0.0, 0.0])
