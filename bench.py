@@ -100,25 +100,27 @@ def copy_probe(nbytes: int, reps: int = 5):
     print(json.dumps({"copy_probe_bytes": n * 16, "reps": reps}), flush=True)
 
 
-def kernel_bytes(name: str, s: dict) -> float:
-    """Algorithmic HBM bytes of one launch of a named kernel (c = 16 B per
-    complex128 element): every input read once, every output written once;
-    operator tables excluded."""
+def kernel_bytes(name: str, s: dict, cm: int = 16) -> float:
+    """Algorithmic HBM bytes of one launch of a named kernel: every input
+    read once, every output written once, operator tables excluded.  State
+    and l-space arrays are complex128 (16 B); the theta-grid and pair arrays
+    of the mean field take cm bytes per complex value (16, or 8 when they
+    are complex64)."""
     nr, L1, L2, nl = s["nr"], s["L1"], s["L2"], s["nlat"]
     nf, nd, npair, nfc = s["nfun"], s["ndyn"], s["npair"], s["nfcore"]
     nb, ng, c = nr * L1, nr * nl, 16
     table = {
-        "l2theta": nd * (nb + ng) * c,
-        "fc_phase": 2 * nfc * ng * c,
-        "theta2l_fc": nfc * (ng + nb) * c,
+        "l2theta": nd * (nb * c + ng * cm),
+        "fc_phase": nfc * ng * (c + cm),
+        "theta2l_fc": nfc * (ng * cm + nb * c),
         "onebody": 2 * nf * nb * c,
-        "pair_theta2L": (nf * ng + npair * L2 * nr) * c,
-        "poisson": 2 * npair * L2 * nr * c,
-        "pair_L2theta": npair * (L2 * nr + ng) * c,
-        "mfprod": (nf * ng + npair * ng + 2 * nd * ng) * c,
-        "theta2l_mf": 2 * nd * (ng + nb) * c,
+        "pair_theta2L": (nf * ng + npair * L2 * nr) * cm,
+        "poisson": 2 * npair * L2 * nr * cm,
+        "pair_L2theta": npair * (L2 * nr + ng) * cm,
+        "mfprod": (nf * ng + npair * ng + 2 * nd * ng) * cm,
+        "theta2l_mf": 2 * nd * (ng * cm + nb * c),
         "gram_l": (nf + nfc + 4 * nd) * nb * c,
-        "gram_pair": 2 * s["nactpair"] * L2 * nr * c,
+        "gram_pair": 2 * s["nactpair"] * L2 * nr * cm,
         # k: read h01, g, v2, orb of nf and all orbitals for X; acc r+w; w0 r; next w; k w; earlier k r
         "dtorb_rk": (nd * 3 + nf + nf * 2 + nf + nf + nf + 1.5 * nf) * nb * c,
         "dtorb": (nd * 3 + nf + nf) * nb * c,
@@ -164,6 +166,8 @@ def main():
     ap.add_argument("--dt", type=float, default=1.0e-6)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--only", choices=["fp64", "mixed"], default=None,
+                    help="measure one path only (for profiler runs; no dipole check)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a device copy of this many bytes (counter calibration)")
     args = ap.parse_args()
@@ -180,61 +184,94 @@ def main():
     import td1c_amd
     from td1c_amd import workloads as W
 
-    prob = W.ne_l31()
     field = W.ne_field()
-    hp = td1c_amd.HProd(prob)
-    wfn = W.make_wfn(prob, 2007, hp.lcic)
-    hp.set_wfn0(wfn)
     rk = td1c_amd.CRK(4)
-    d = hp.to_device(wfn)
-    nprof = max(10, min(args.steps, 50))
-    t = 0.0
-    # warm-up: the first call captures one RK step as a HIP graph
-    rk.propagate(hp, d, t, args.dt, args.warmup, field)
-    t += args.warmup * args.dt
-    torch.cuda.synchronize()
-    # untimed profiling pass (stream launches): events around every kernel
-    hp.set_timing(True)
-    for i in range(nprof):
-        rk.prop(hp, d, t, args.dt, field=field)
-        t += args.dt
-    breakdown = hp.get_timing()
-    hp.set_timing(False)
-    dname = max(breakdown.items(), key=lambda kv: kv[1][0])[0]
-    # timed region: device-driven steps, graph replays
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    rk.propagate(hp, d, t, args.dt, args.steps, field)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    t += args.steps * args.dt
-    ms_step = (t1 - t0) * 1e3 / args.steps
-    # timed region with HIP events around the dominant kernel only (stream
-    # launches on the same stream), for its average launch time
-    lfs = [rk.stage_fields(field, t + i * args.dt, args.dt) for i in range(args.steps)]
-    hp.set_timing(True, only=dname)
-    torch.cuda.synchronize()
-    t2 = time.perf_counter()
-    for i in range(args.steps):
-        rk.prop(hp, d, t + i * args.dt, args.dt, lfield_stages=lfs[i])
-    torch.cuda.synchronize()
-    t3 = time.perf_counter()
-    times = hp.get_timing()
-    hp.set_timing(False)
-    ms_step_stream = (t3 - t2) * 1e3 / args.steps
-    norm = float(torch.linalg.vector_norm(d[: hp.norb]).item() ** 2)
-    if not np.isfinite(norm):
-        raise RuntimeError("propagation produced non-finite values")
+
+    def measure(precision: int) -> dict:
+        prob = W.ne_l31(precision=precision)
+        hp = td1c_amd.HProd(prob)
+        wfn = W.make_wfn(prob, 2007, hp.lcic)
+        hp.set_wfn0(wfn)
+        d = hp.to_device(wfn)
+        nprof = max(10, min(args.steps, 50))
+        t = 0.0
+        # warm-up: the first call captures one RK step as a HIP graph
+        rk.propagate(hp, d, t, args.dt, args.warmup, field)
+        t += args.warmup * args.dt
+        torch.cuda.synchronize()
+        # untimed profiling pass (stream launches): events around every kernel
+        hp.set_timing(True)
+        for _ in range(nprof):
+            rk.prop(hp, d, t, args.dt, field=field)
+            t += args.dt
+        breakdown = hp.get_timing()
+        hp.set_timing(False)
+        dname = max(breakdown.items(), key=lambda kv: kv[1][0])[0]
+        # timed region: device-driven steps, graph replays
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rk.propagate(hp, d, t, args.dt, args.steps, field)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        t += args.steps * args.dt
+        # timed region with HIP events around the dominant kernel only (stream
+        # launches on the same stream), for its average launch time
+        lfs = [rk.stage_fields(field, t + i * args.dt, args.dt) for i in range(args.steps)]
+        hp.set_timing(True, only=dname)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        for i in range(args.steps):
+            rk.prop(hp, d, t + i * args.dt, args.dt, lfield_stages=lfs[i])
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        times = hp.get_timing()
+        hp.set_timing(False)
+        norm = float(torch.linalg.vector_norm(d[: hp.norb]).item() ** 2)
+        if not np.isfinite(norm):
+            raise RuntimeError("propagation produced non-finite values")
+        return dict(prob=prob, hp=hp, wfn=wfn, lf0=lfs[0][0], ms_step=(t1 - t0) * 1e3 / args.steps,
+                    ms_stream=(t3 - t2) * 1e3 / args.steps, breakdown=breakdown, nprof=nprof, dname=dname,
+                    dtime=times[dname], norm=norm)
+
+    def dipole_series(precision: int, nchunk: int = 10, chunk: int = 20) -> np.ndarray:
+        """<z>(t) every `chunk` steps of a propagation from the same state."""
+        prob = W.ne_l31(precision=precision)
+        hp = td1c_amd.HProd(prob)
+        wfn = W.make_wfn(prob, 2007, hp.lcic)
+        hp.set_wfn0(wfn)
+        d = hp.to_device(wfn)
+        out, t = [], 0.0
+        for _ in range(nchunk):
+            rk.propagate(hp, d, t, args.dt, chunk, field)
+            t += chunk * args.dt
+            out.append(hp.dipole(td1c_amd.field_value(field, t), d))
+        return np.array(out)
+
+    if args.only is not None:
+        r = measure(0 if args.only == "fp64" else 1)
+        print(json.dumps({"only": args.only, "ms_per_step": r["ms_step"],
+                          "hprod_evals_per_s": 4.0 / (r["ms_step"] * 1e-3)}), flush=True)
+        return
+    fp64 = measure(0)
+    mixed = measure(1)
+    dseries = {0: dipole_series(0), 1: dipole_series(1)}
+    dip_err = float(np.abs(dseries[1] - dseries[0]).max())
+    use = mixed if dip_err <= 1e-6 else fp64
+    cm = 8 if use is mixed else 16
+    prob, hp = use["prob"], use["hp"]
 
     s = dict(nr=prob.nr, L1=prob.lmax1 + 1, L2=prob.lmax2 + 1, nlat=prob.lmax2 + 1, nfun=prob.nfun,
              ndyn=prob.nfun - prob.nfcore, npair=hp.npair, nfcore=prob.nfcore, lcic=hp.lcic,
              nactpair=prob.nact * (prob.nact + 1) // 2)
-    dms, dcount = times[dname]
+    dname = use["dname"]
+    dms, dcount = use["dtime"]
     avg_s = dms / dcount * 1e-3
-    abytes = kernel_bytes(dname, s)
+    abytes = kernel_bytes(dname, s, cm)
     achieved = abytes / avg_s / 1e9 if abytes > 0 else 0.0
+    ms_step = use["ms_step"]
     hprods_per_s = 4.0 / (ms_step * 1e-3)
-    step_bytes = 4 * hprod_bytes(s) + 15 * (hp.wfn_size * 16)
+    step_bytes = 4 * hprod_bytes(s, cm) + 15 * (hp.wfn_size * 16)
+    nprof = use["nprof"]
     result = {
         "metric": "hprod evals/s + propagations/s, mixed vs FP64 at dipole err <=1e-6; %HBM peak",
         "value": hprods_per_s,
@@ -246,24 +283,32 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c128",
+        "dtype": "c64 mean field / c128 state" if use is mixed else "c128",
         "data": "synthetic (seeded orbitals, one CI determinant)",
         "config": {"workload": "ne_l31: Ne-like RK4, 80x21 FEDVR (1599 pts) on [0,160], lmax1 31/lmax2 62, "
                                "Z=10, 1 frozen core + 4 active, 800 nm sin2 1e14 W/cm2 velocity gauge",
-                   "precision": "fp64", "rk_order": 4, "dt": args.dt, "steps_per_propagation": NSTEP_PROP,
-                   "parallelism": "single GPU"},
+                   "precision": "mixed" if use is mixed else "fp64", "rk_order": 4, "dt": args.dt,
+                   "steps_per_propagation": NSTEP_PROP, "parallelism": "single GPU"},
         "propagations_per_s": 1.0 / (NSTEP_PROP * ms_step * 1e-3),
         "hbm_frac_step": step_bytes / (ms_step * 1e-3) / (HBM_PEAK_GBS * 1e9),
+        "fp64": {"hprod_evals_per_s": 4.0 / (fp64["ms_step"] * 1e-3), "ms_per_step": fp64["ms_step"],
+                 "propagations_per_s": 1.0 / (NSTEP_PROP * fp64["ms_step"] * 1e-3)},
+        "mixed": {"hprod_evals_per_s": 4.0 / (mixed["ms_step"] * 1e-3), "ms_per_step": mixed["ms_step"],
+                  "propagations_per_s": 1.0 / (NSTEP_PROP * mixed["ms_step"] * 1e-3),
+                  "dipole_max_abs_err_vs_fp64": dip_err,
+                  "dipole_check": "<z> every 20 steps over 200 steps, mixed vs fp64, same initial state"},
         "roofline": {"bound": "hbm", "kernel": dname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": recorded_traffic(dname),
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": recorded_traffic(dname) if use is fp64 else None,
                      "algorithmic_bytes_per_launch": abytes, "avg_launch_us": avg_s * 1e6},
-        "ms_per_step_stream_launch": ms_step_stream,
-        "kernel_ms_per_step": {k: v[0] / nprof for k, v in sorted(breakdown.items(), key=lambda kv: -kv[1][0])},
-        "kernel_us_per_launch": {k: 1e3 * v[0] / v[1] for k, v in sorted(breakdown.items(), key=lambda kv: -kv[1][0])},
-        "norm_after": norm,
+        "ms_per_step_stream_launch": use["ms_stream"],
+        "kernel_ms_per_step": {k: v[0] / nprof for k, v in sorted(use["breakdown"].items(), key=lambda kv: -kv[1][0])},
+        "kernel_us_per_launch": {k: 1e3 * v[0] / v[1]
+                                 for k, v in sorted(use["breakdown"].items(), key=lambda kv: -kv[1][0])},
+        "norm_after": use["norm"],
     }
     if not args.no_cpu:
-        result["cpu_baseline"] = cpu_baseline(prob, wfn, lfs[0][0], args.cpu_seconds)
+        result["cpu_baseline"] = cpu_baseline(fp64["prob"], fp64["wfn"], fp64["lf0"], args.cpu_seconds)
     else:
         result["cpu_baseline"] = None
     print(json.dumps(result), flush=True)

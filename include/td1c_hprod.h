@@ -76,9 +76,12 @@ typedef struct td1c_problem {
    * means 0.8 * rmax, mask_cos_order <= 0 means 4 */
   double rmask;
   double mask_cos_order;
-  /* arithmetic of the bulk one-body / mean-field work: 0 = complex128
-   * throughout, 1 = complex64 bulk with FP64 state, RK accumulation,
-   * projector, 1-RDM inverse and CI (no counterpart in the reference) */
+  /* arithmetic of the mean field: 0 = complex128 throughout; 1 = the
+   * theta-grid orbitals, pair densities and pair potentials in complex64
+   * (their Legendre transforms and mean-field products in FP32, the radial
+   * Poisson solve in FP64 on complex64 input/output), with the state, the
+   * one-body part, the Gram products, X matrix, projector, 1-RDM inverse, CI
+   * and RK accumulation in FP64 (no counterpart in the reference) */
   int precision;
 } td1c_problem;
 

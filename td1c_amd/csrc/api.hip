@@ -77,11 +77,16 @@ struct td1c_ctx {
   double2 *orb = nullptr, *h01 = nullptr, *gorb = nullptr, *v2orb = nullptr;
   DevBuf<double2> gsp;       // orbg | gorbg | v2orbg
   double2 *orbg = nullptr, *gorbg = nullptr, *v2orbg = nullptr;
+  // mixed precision (precision = 1): the theta-grid and pair arrays in complex64
+  bool mixed = false;
+  DevBuf<float2> gsp_f, rhoL_f, vL_f, vg_f;
+  float2 *orbg_f = nullptr, *gorbg_f = nullptr, *v2orbg_f = nullptr;
+  DevBuf<float> m_g2L_f, m_L2g_f;
   DevBuf<double2> orbg0;     // frozen-core theta-grid orbitals at A = 0
   DevBuf<double2> rhoL, vL, vg;
   DevBuf<double2> p_xb, p_Q;  // Poisson bridge solutions [pair][L][nfe-1], multipole moments [pair][L]
   // Gram
-  DevBuf<const double2*> g_left_l, g_right_l, g_left_p, g_right_p;
+  DevBuf<const void*> g_left_l, g_right_l, g_left_p, g_right_p;
   int gl_nL = 0, gl_nR = 0, gp_nL = 0, gp_nR = 0, gram_blk = 0;
   DevBuf<double2> gram_partial, lprod, pprod;
   DevBuf<int> mapH, mapG, mapV, mapF, map2e;
@@ -125,9 +130,13 @@ struct td1c_ctx {
     for (auto* b : i) b->release();
     DevBuf<TransformBatch>* t[] = {&b_l2g_dyn, &b_g2l_fc, &b_rho, &b_L2g, &b_g2l_mf};
     for (auto* b : t) b->release();
-    DevBuf<const double2*>* g[] = {&g_left_l, &g_right_l, &g_left_p, &g_right_p};
+    DevBuf<const void*>* g[] = {&g_left_l, &g_right_l, &g_left_p, &g_right_p};
     for (auto* b : g) b->release();
     for (auto e : ev_pool) (void)hipEventDestroy(e);
+    DevBuf<float2>* ff[] = {&gsp_f, &rhoL_f, &vL_f, &vg_f};
+    for (auto* b : ff) b->release();
+    DevBuf<float>* fm[] = {&m_g2L_f, &m_L2g_f};
+    for (auto* b : fm) b->release();
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (cap_stream) (void)hipStreamDestroy(cap_stream);
     if (clock.p) (void)hipFree(clock.p);
@@ -264,17 +273,33 @@ int build_device(td1c_ctx* c) {
   c->gorb = c->h01 + nb * nf;
   c->v2orb = c->gorb + nb * c->nd;
   HIP_TRY(hipMemset(c->lsp.p, 0, c->lsp.n * sizeof(double2)));
-  HIP_TRY(c->gsp.alloc(ng * (nf + 2 * c->nd)));
-  c->orbg = c->gsp.p;
-  c->gorbg = c->orbg + ng * nf;
-  c->v2orbg = c->gorbg + ng * c->nd;
-  HIP_TRY(hipMemset(c->gsp.p, 0, c->gsp.n * sizeof(double2)));
+  c->mixed = S.precision == 1;
+  const size_t np0 = std::max(1, c->npair);
+  if (!c->mixed) {
+    HIP_TRY(c->gsp.alloc(ng * (nf + 2 * c->nd)));
+    c->orbg = c->gsp.p;
+    c->gorbg = c->orbg + ng * nf;
+    c->v2orbg = c->gorbg + ng * c->nd;
+    HIP_TRY(hipMemset(c->gsp.p, 0, c->gsp.n * sizeof(double2)));
+    HIP_TRY(c->rhoL.alloc(np0 * L2 * nr));
+    HIP_TRY(c->vL.alloc(np0 * L2 * nr));
+    HIP_TRY(c->vg.alloc(np0 * nl * nr));
+  } else {
+    HIP_TRY(c->gsp_f.alloc(ng * (nf + 2 * c->nd)));
+    c->orbg_f = c->gsp_f.p;
+    c->gorbg_f = c->orbg_f + ng * nf;
+    c->v2orbg_f = c->gorbg_f + ng * c->nd;
+    HIP_TRY(hipMemset(c->gsp_f.p, 0, c->gsp_f.n * sizeof(float2)));
+    HIP_TRY(c->rhoL_f.alloc(np0 * L2 * nr));
+    HIP_TRY(c->vL_f.alloc(np0 * L2 * nr));
+    HIP_TRY(c->vg_f.alloc(np0 * nl * nr));
+    std::vector<double> g2L = padded_cols(S.mat_g2L, S.nlat, S.L2), L2g = padded_cols(S.mat_L2g, S.L2, S.nlat);
+    HIP_TRY(c->m_g2L_f.upload(std::vector<float>(g2L.begin(), g2L.end())));
+    HIP_TRY(c->m_L2g_f.upload(std::vector<float>(L2g.begin(), L2g.end())));
+  }
   HIP_TRY(c->orbg0.alloc(std::max<size_t>(1, ng * nfc)));
   HIP_TRY(hipMemset(c->orbg0.p, 0, c->orbg0.n * sizeof(double2)));
-  const size_t np = std::max(1, c->npair);
-  HIP_TRY(c->rhoL.alloc(np * L2 * nr));
-  HIP_TRY(c->vL.alloc(np * L2 * nr));
-  HIP_TRY(c->vg.alloc(np * nl * nr));
+  const size_t np = np0;
   HIP_TRY(c->p_xb.alloc(np * L2 * std::max(1, S.nfe - 1)));
   HIP_TRY(c->p_Q.alloc(np * L2));
 
@@ -321,7 +346,7 @@ int build_device(td1c_ctx* c) {
   // Gram products over the l-space basis: left = orb_0..orb_{nf-1}, h01 of
   // frozen core; right = h01, gorb, v2orb, orb of the dynamical orbitals.
   {
-    std::vector<const double2*> L, R;
+    std::vector<const void*> L, R;
     for (int f = 0; f < nf; ++f) L.push_back(c->orb + (size_t)f * nb);
     for (int f = 0; f < nfc; ++f) L.push_back(c->h01 + (size_t)f * nb);
     for (int f = nfc; f < nf; ++f) R.push_back(c->h01 + (size_t)f * nb);
@@ -354,14 +379,22 @@ int build_device(td1c_ctx* c) {
   // (i >= j stored as pair (i, j)), right = v of active pairs.
   {
     const int na = S.nact, ncore = S.ncore;
-    std::vector<const double2*> L, R;
+    std::vector<const void*> L, R;
     std::vector<int> act_pairs;
     for (int q = 0; q < c->npair; ++q) {
       auto& p = S.pairs[q];
       if (p.type == 0 && p.i >= ncore) act_pairs.push_back(q);
     }
-    for (int q : act_pairs) L.push_back(c->rhoL.p + (size_t)q * L2 * nr);
-    for (int q : act_pairs) R.push_back(c->vL.p + (size_t)q * L2 * nr);
+    for (int q : act_pairs) {
+      const size_t off = (size_t)q * L2 * nr;
+      if (c->mixed) {
+        L.push_back(c->rhoL_f.p + off);
+        R.push_back(c->vL_f.p + off);
+      } else {
+        L.push_back(c->rhoL.p + off);
+        R.push_back(c->vL.p + off);
+      }
+    }
     c->gp_nL = (int)L.size();
     c->gp_nR = (int)R.size();
     if (c->gp_nL * c->gp_nR > kMaxGram) return fail(-2, "too many active pairs for the Gram kernel");
@@ -471,12 +504,25 @@ void enqueue_orbin(td1c_ctx* c, const double* d_lf, const double2* wfn, hipStrea
   // transform that reads them
   {
     TIMED("l2theta");
-    launch_transform(0, wfn, c->orbg, c->m_l2g.p, c->b_l2g_dyn.p, c->n_l2g_dyn, nr, L1, nl, pad_cols(nl), c->orb, s);
+    if (c->mixed)
+      launch_transform_t<double2, float2>(0, wfn, c->orbg_f, c->m_l2g.p, nullptr, c->b_l2g_dyn.p, c->n_l2g_dyn, nr, L1,
+                                          nl, pad_cols(nl), c->orb, s);
+    else
+      launch_transform(0, wfn, c->orbg, c->m_l2g.p, c->b_l2g_dyn.p, c->n_l2g_dyn, nr, L1, nl, pad_cols(nl), c->orb, s);
   }
   // frozen core in the velocity gauge (hprod_fcorbg): phase on the grid, back to l-space
   if (nfc > 0 && S.igauge == 1) {
-    { TIMED("fc_phase"); launch_fc_phase(c->orbg0.p, c->orbg, c->xr.p, c->cost.p, d_lf, nr, S.nradfc, nl, nfc, s); }
-    { TIMED("theta2l_fc"); launch_transform(0, c->orbg, c->orb, c->m_g2l.p, c->b_g2l_fc.p, c->n_g2l_fc, nr, nl, L1, pad_cols(L1), nullptr, s); }
+    if (c->mixed) {
+      { TIMED("fc_phase"); launch_fc_phase(c->orbg0.p, c->orbg_f, c->xr.p, c->cost.p, d_lf, nr, S.nradfc, nl, nfc, s); }
+      {
+        TIMED("theta2l_fc");
+        launch_transform_t<float2, double2>(0, c->orbg_f, c->orb, c->m_g2l.p, nullptr, c->b_g2l_fc.p, c->n_g2l_fc, nr,
+                                            nl, L1, pad_cols(L1), nullptr, s);
+      }
+    } else {
+      { TIMED("fc_phase"); launch_fc_phase(c->orbg0.p, c->orbg, c->xr.p, c->cost.p, d_lf, nr, S.nradfc, nl, nfc, s); }
+      { TIMED("theta2l_fc"); launch_transform(0, c->orbg, c->orb, c->m_g2l.p, c->b_g2l_fc.p, c->n_g2l_fc, nr, nl, L1, pad_cols(L1), nullptr, s); }
+    }
   }
 }
 
@@ -508,7 +554,14 @@ int enqueue_htotx(td1c_ctx* c, double dtime, const double* d_lf, const double2* 
   // mean field
   const size_t ng = (size_t)c->ngrid;
   if (S.nel[2] >= 2 && c->npair > 0) {
-    { TIMED("pair_theta2L"); launch_transform(1, c->orbg, c->rhoL.p, c->m_g2L.p, c->b_rho.p, c->npair, nr, nl, L2, pad_cols(L2), nullptr, s); }
+    if (c->mixed) {
+      TIMED("pair_theta2L");
+      launch_transform_t<float2, float2>(1, c->orbg_f, c->rhoL_f.p, c->m_g2L.p, c->m_g2L_f.p, c->b_rho.p, c->npair, nr,
+                                         nl, L2, pad_cols(L2), nullptr, s);
+    } else {
+      TIMED("pair_theta2L");
+      launch_transform(1, c->orbg, c->rhoL.p, c->m_g2L.p, c->b_rho.p, c->npair, nr, nl, L2, pad_cols(L2), nullptr, s);
+    }
     PoissonArgs pa{};
     pa.nr = nr; pa.n = S.n; pa.nfe = S.nfe; pa.ni = S.ni; pa.L2 = L2; pa.npair = c->npair;
     pa.pair_m = c->pair_m.p;
@@ -516,15 +569,33 @@ int enqueue_htotx(td1c_ctx* c, double dtime, const double* d_lf, const double2* 
     pa.Sd = c->pSd.p; pa.Sl = c->pSl.p; pa.Su = c->pSu.p;
     pa.invr = c->invr.p; pa.rpl0 = c->rpl0.p; pa.rpl1 = c->rpl1.p; pa.fac2 = c->fac2.p; pa.fac1 = c->fac1.p;
     pa.Q = c->p_Q.p;
-    { TIMED("poisson"); launch_poisson(c->rhoL.p, c->p_xb.p, c->vL.p, pa, s); }
-    { TIMED("pair_L2theta"); launch_transform(0, c->vL.p, c->vg.p, c->m_L2g.p, c->b_L2g.p, c->npair, nr, L2, nl, pad_cols(nl), nullptr, s); }
+    if (c->mixed) {
+      { TIMED("poisson"); launch_poisson(c->rhoL_f.p, c->p_xb.p, c->vL_f.p, pa, s); }
+      {
+        TIMED("pair_L2theta");
+        launch_transform_t<float2, float2>(0, c->vL_f.p, c->vg_f.p, c->m_L2g.p, c->m_L2g_f.p, c->b_L2g.p, c->npair, nr,
+                                           L2, nl, pad_cols(nl), nullptr, s);
+      }
+    } else {
+      { TIMED("poisson"); launch_poisson(c->rhoL.p, c->p_xb.p, c->vL.p, pa, s); }
+      { TIMED("pair_L2theta"); launch_transform(0, c->vL.p, c->vg.p, c->m_L2g.p, c->b_L2g.p, c->npair, nr, L2, nl, pad_cols(nl), nullptr, s); }
+    }
     MfArgs ma{};
     ma.nr = nr; ma.nlat = nl; ma.nfun = nf; ma.nfcore = nfc; ma.nfcore2 = S.nfcore2; ma.ncore = S.ncore; ma.nact = S.nact;
     ma.nradfc = S.nradfc; ma.npair = c->npair; ma.nelact2 = S.nelact[2];
     ma.pj = c->pj.p; ma.pi = c->pi.p; ma.ptype = c->ptype.p; ma.mval = c->mval.p;
     ma.den1 = c->den1.p; ma.den2 = c->den2.p; ma.rden = c->rden.p; ma.mfc = c->mfc.p;
-    { TIMED("mfprod"); launch_mfprod(c->orbg, c->vg.p, c->gorbg, c->v2orbg, ma, s); }
-    { TIMED("theta2l_mf"); launch_transform(0, c->gorbg, c->gorb, c->m_g2l.p, c->b_g2l_mf.p, c->n_g2l_mf, nr, nl, L1, pad_cols(L1), nullptr, s); }
+    if (c->mixed) {
+      { TIMED("mfprod"); launch_mfprod(c->orbg_f, c->vg_f.p, c->gorbg_f, c->v2orbg_f, ma, s); }
+      {
+        TIMED("theta2l_mf");
+        launch_transform_t<float2, double2>(0, c->gorbg_f, c->gorb, c->m_g2l.p, nullptr, c->b_g2l_mf.p, c->n_g2l_mf, nr,
+                                            nl, L1, pad_cols(L1), nullptr, s);
+      }
+    } else {
+      { TIMED("mfprod"); launch_mfprod(c->orbg, c->vg.p, c->gorbg, c->v2orbg, ma, s); }
+      { TIMED("theta2l_mf"); launch_transform(0, c->gorbg, c->gorb, c->m_g2l.p, c->b_g2l_mf.p, c->n_g2l_mf, nr, nl, L1, pad_cols(L1), nullptr, s); }
+    }
   } else {
     launch_zero(c->gorb, 2 * nb * c->nd, s);
   }
@@ -537,7 +608,7 @@ int enqueue_htotx(td1c_ctx* c, double dtime, const double* d_lf, const double2* 
   if (c->gp_nL > 0) {
     GramArgs h{};
     h.len = (size_t)L2 * nr; h.nL = c->gp_nL; h.nR = c->gp_nR; h.conj_left = 0; h.with_conj_right = 1;
-    h.left = c->g_left_p.p; h.right = c->g_right_p.p;
+    h.left = c->g_left_p.p; h.right = c->g_right_p.p; h.f32 = c->mixed ? 1 : 0;
     { TIMED("gram_pair"); launch_gram(h, c->gram_partial.p, c->gram_blk, c->pprod.p, s); }
   }
   ia.kout = cs.kout; ia.acc = cs.acc; ia.dtw = cs.dtw; ia.next = cs.next; ia.w0 = cs.w0;
@@ -649,7 +720,13 @@ int td1c_set_wfn0(td1c_ctx* c, const double* wfn, const double* cic, const doubl
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipDeviceSynchronize());
     tb.release();
-    HIP_TRY(hipMemcpy(c->orbg, c->orbg0.p, (size_t)c->ngrid * nfc * sizeof(double2), hipMemcpyDeviceToDevice));
+    if (c->mixed) {
+      launch_to_f32(c->orbg_f, c->orbg0.p, (size_t)c->ngrid * nfc, nullptr);
+      HIP_TRY(hipGetLastError());
+      HIP_TRY(hipDeviceSynchronize());
+    } else {
+      HIP_TRY(hipMemcpy(c->orbg, c->orbg0.p, (size_t)c->ngrid * nfc * sizeof(double2), hipMemcpyDeviceToDevice));
+    }
   }
   // frozen-core orbitals as hprod_orbin sees them at lfield = 0
   std::vector<std::complex<double>> orbfc(nb * nfc);

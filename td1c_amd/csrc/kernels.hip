@@ -1,6 +1,7 @@
-// gfx950 kernels of the td1c orbital Hamiltonian product (FP64 path).
+// gfx950 kernels of the td1c orbital Hamiltonian product.
 //
-// Layouts (all complex values are double2 = (re, im), radial index fastest):
+// Layouts (complex values are double2 = (re, im), or float2 for the theta-grid
+// and pair arrays of the mixed-precision path; radial index fastest):
 //   orbitals, l-space   : [orbital][l][r]      (td1c's wfn layout)
 //   orbitals, theta-grid: [orbital][theta][r]
 //   pair densities / potentials: [pair][L][r] and [pair][theta][r]
@@ -9,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "kernels.hpp"
 
@@ -30,6 +32,42 @@ __device__ __forceinline__ void cfma(double2& acc, double2 a, double2 b) {
   acc.y = fma(a.x, b.y, acc.y);
   acc.y = fma(a.y, b.x, acc.y);
 }
+
+// complex64 counterparts, for the mixed-precision mean field
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cmulc(float2 a, float2 b) {
+  return make_float2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cscale(float2 a, double s) {
+  const float f = (float)s;
+  return make_float2(a.x * f, a.y * f);
+}
+__device__ __forceinline__ void cfma(float2& acc, float2 a, float2 b) {
+  acc.x = fmaf(a.x, b.x, acc.x);
+  acc.x = fmaf(-a.y, b.y, acc.x);
+  acc.y = fmaf(a.x, b.y, acc.y);
+  acc.y = fmaf(a.y, b.x, acc.y);
+}
+
+// conversions between the storage type of an array and the arithmetic type
+template <class A2> __device__ __forceinline__ A2 cz();
+template <> __device__ __forceinline__ double2 cz<double2>() { return make_double2(0.0, 0.0); }
+template <> __device__ __forceinline__ float2 cz<float2>() { return make_float2(0.0f, 0.0f); }
+template <class A2> __device__ __forceinline__ A2 cto(double2 v);
+template <class A2> __device__ __forceinline__ A2 cto(float2 v);
+template <> __device__ __forceinline__ double2 cto<double2>(double2 v) { return v; }
+template <> __device__ __forceinline__ double2 cto<double2>(float2 v) { return make_double2(v.x, v.y); }
+template <> __device__ __forceinline__ float2 cto<float2>(double2 v) { return make_float2((float)v.x, (float)v.y); }
+template <> __device__ __forceinline__ float2 cto<float2>(float2 v) { return v; }
+// real scalar type of a complex type
+template <class A2> struct CReal;
+template <> struct CReal<double2> { using T = double; };
+template <> struct CReal<float2> { using T = float; };
 
 // Workgroups are dealt to the 8 XCDs round-robin by linear id, and each XCD
 // has its own L2.  This maps the hardware linear id to a logical tile id so
@@ -53,11 +91,15 @@ __device__ __forceinline__ int xcd_remap(int lin, int n) {
 // first adds the parts through LDS).  The matrix row segment is
 // wave-uniform and is read with scalar loads.
 // ---------------------------------------------------------------------------
-template <int MODE, int CPW, int KS>
-__global__ void __launch_bounds__(256) k_transform(const double2* __restrict__ in, double2* __restrict__ out,
-                                                   const double* __restrict__ mats,
+template <int MODE, int CPW, int KS, class TI, class TO>
+__global__ void __launch_bounds__(256) k_transform(const TI* __restrict__ in, TO* __restrict__ out,
+                                                   const double* __restrict__ mats, const float* __restrict__ matsf,
                                                    const TransformBatch* __restrict__ batch, int nr, int K, int C,
                                                    int Cp, double2* __restrict__ copy) {
+  // complex64 arithmetic only when both sides are complex64
+  using A2 = typename std::conditional<std::is_same<TI, float2>::value && std::is_same<TO, float2>::value, float2,
+                                       double2>::type;
+  using AR = typename CReal<A2>::T;
   constexpr int NG = 4 / KS;  // column groups per block
   // logical tile order: column block fastest, so the column blocks that
   // re-read one input tile run on one XCD
@@ -75,34 +117,38 @@ __global__ void __launch_bounds__(256) k_transform(const double2* __restrict__ i
   const int k0 = __builtin_amdgcn_readfirstlane(ks * kc);
   const int k1 = min(K, k0 + kc);
   const bool col_ok = c0 < C;
-  const double* __restrict__ M = mats + (size_t)bt.mat * K * Cp + (col_ok ? c0 : 0);
-  double2 acc[CPW];
+  const AR* __restrict__ M;
+  if constexpr (std::is_same<AR, float>::value) M = matsf + (size_t)bt.mat * K * Cp + (col_ok ? c0 : 0);
+  else M = mats + (size_t)bt.mat * K * Cp + (col_ok ? c0 : 0);
+  A2 acc[CPW];
 #pragma unroll
-  for (int c = 0; c < CPW; ++c) acc[c] = make_double2(0.0, 0.0);
+  for (int c = 0; c < CPW; ++c) acc[c] = cz<A2>();
   const bool row_ok = r < bt.rows;
   if (row_ok && col_ok) {
-    const double2* __restrict__ x0 = in + (size_t)bt.in0 + r;
-    const double2* __restrict__ x1 = in + (size_t)bt.in1 + r;
+    const TI* __restrict__ x0 = in + (size_t)bt.in0 + r;
+    const TI* __restrict__ x1 = in + (size_t)bt.in1 + r;
 #pragma unroll 8
     for (int k = k0; k < k1; ++k) {
-      double2 x;
+      A2 x;
       if (MODE == 0) {
-        x = x0[(size_t)k * nr];
-        if (copy != nullptr && c0 == 0) copy[(size_t)bt.in0 + (size_t)k * nr + r] = x;
+        const TI xv = x0[(size_t)k * nr];
+        x = cto<A2>(xv);
+        if constexpr (std::is_same<TI, double2>::value)
+          if (copy != nullptr && c0 == 0) copy[(size_t)bt.in0 + (size_t)k * nr + r] = xv;
       } else {
-        x = cmulc(x0[(size_t)k * nr], x1[(size_t)k * nr]);
+        x = cmulc(cto<A2>(x0[(size_t)k * nr]), cto<A2>(x1[(size_t)k * nr]));
       }
-      const double* __restrict__ mrow = M + (size_t)k * Cp;
+      const AR* __restrict__ mrow = M + (size_t)k * Cp;
 #pragma unroll
       for (int c = 0; c < CPW; ++c) {
-        const double m = mrow[c];
+        const AR m = mrow[c];
         acc[c].x = fma(x.x, m, acc[c].x);
         acc[c].y = fma(x.y, m, acc[c].y);
       }
     }
   }
   if (KS > 1) {
-    __shared__ double2 red[KS > 1 ? KS - 1 : 1][NG][CPW][64];
+    __shared__ A2 red[KS > 1 ? KS - 1 : 1][NG][CPW][64];
     if (ks > 0) {
 #pragma unroll
       for (int c = 0; c < CPW; ++c) red[ks - 1][grp][c][lane] = acc[c];
@@ -118,7 +164,7 @@ __global__ void __launch_bounds__(256) k_transform(const double2* __restrict__ i
   if (ks == 0 && row_ok && col_ok) {
 #pragma unroll
     for (int c = 0; c < CPW; ++c)
-      if (c0 + c < C) out[(size_t)bt.out + (size_t)(c0 + c) * nr + r] = acc[c];
+      if (c0 + c < C) out[(size_t)bt.out + (size_t)(c0 + c) * nr + r] = cto<TO>(acc[c]);
   }
 }
 
@@ -387,8 +433,8 @@ __global__ void __launch_bounds__(256) k_onebody_el(const double2* __restrict__ 
 // rows, the Schur right-hand sides, the two bidiagonal sweeps of the
 // factored Schur system on thread 0 out of LDS, then the bridge rows of v.
 // NI = interior points per element (0: runtime a.ni).
-template <int NI>
-__global__ void __launch_bounds__(256) k_poisson_bridge(const double2* __restrict__ rho, double2* __restrict__ v,
+template <int NI, class T>
+__global__ void __launch_bounds__(256) k_poisson_bridge(const T* __restrict__ rho, T* __restrict__ v,
                                                         double2* __restrict__ xb, const PoissonArgs a) {
   const int tid = threadIdx.x;
   const int job = xcd_remap(blockIdx.x, gridDim.x);  // the blocks of one L share an XCD (and its L2)
@@ -402,17 +448,17 @@ __global__ void __launch_bounds__(256) k_poisson_bridge(const double2* __restric
   __shared__ double2 sRed[4];
   const int pm = a.pair_m[p];
   const size_t row = ((size_t)p * a.L2 + L) * a.nr;
-  double2* vp = v + row;
+  T* vp = v + row;
   double2* xbp = xb + ((size_t)p * a.L2 + L) * nb;
   if (L < (pm < 0 ? -pm : pm)) {  // uniform over the block
     for (int k = tid; k < nb; k += 256) {
-      vp[(k + 1) * a.n - 1] = make_double2(0.0, 0.0);
+      vp[(k + 1) * a.n - 1] = cz<T>();
       xbp[k] = make_double2(0.0, 0.0);
     }
     if (tid == 0) a.Q[(size_t)p * a.L2 + L] = make_double2(0.0, 0.0);
     return;
   }
-  const double2* rp = rho + row;
+  const T* rp = rho + row;
   const double* invr = a.invr + (size_t)L * a.nr;
   const double* Wl = a.Wl + (size_t)L * a.nr;
   const double* Wr = a.Wr + (size_t)L * a.nr;
@@ -431,7 +477,7 @@ __global__ void __launch_bounds__(256) k_poisson_bridge(const double2* __restric
       double iv[NI > 0 ? NI : 1], wl[NI > 0 ? NI : 1], wr[NI > 0 ? NI : 1];
 #pragma unroll
       for (int i = 0; i < NI; ++i) {
-        x[i] = rp[base + i];
+        x[i] = cto<double2>(rp[base + i]);
         iv[i] = invr[base + i];
         wl[i] = Wl[base + i];
         wr[i] = Wr[base + i];
@@ -446,7 +492,7 @@ __global__ void __launch_bounds__(256) k_poisson_bridge(const double2* __restric
       }
     } else {
       for (int i = 0; i < ni; ++i) {
-        const double2 b = cscale(rp[base + i], invr[base + i]);
+        const double2 b = cscale(cto<double2>(rp[base + i]), invr[base + i]);
         cl.x = fma(Wl[base + i], b.x, cl.x);
         cl.y = fma(Wl[base + i], b.y, cl.y);
         cr.x = fma(Wr[base + i], b.x, cr.x);
@@ -460,7 +506,7 @@ __global__ void __launch_bounds__(256) k_poisson_bridge(const double2* __restric
   // multipole moment over every radial row
   double2 m = make_double2(0.0, 0.0);
   for (int r = tid; r < a.nr; r += 256) {
-    const double2 x = rp[r];
+    const double2 x = cto<double2>(rp[r]);
     m.x = fma(x.x, rpl0[r], m.x);
     m.y = fma(x.y, rpl0[r], m.y);
   }
@@ -473,7 +519,7 @@ __global__ void __launch_bounds__(256) k_poisson_bridge(const double2* __restric
   const double2 Q = cscale(cadd(cadd(sRed[0], sRed[1]), cadd(sRed[2], sRed[3])), a.fac1[L]);
   for (int k = tid; k < nb; k += 256) {
     const int gi = (k + 1) * a.n - 1;
-    double2 sk = cscale(rp[gi], invr[gi]);
+    double2 sk = cscale(cto<double2>(rp[gi]), invr[gi]);
     sk = csub(sk, sS[k]);
     sk = csub(sk, sCr[k]);
     sY[k] = sk;
@@ -506,7 +552,7 @@ __global__ void __launch_bounds__(256) k_poisson_bridge(const double2* __restric
     const int gi = (k + 1) * a.n - 1;
     const double2 x = sCr[k];
     xbp[k] = x;
-    vp[gi] = make_double2((x.x + Q.x * rpl1[gi]) * fac2[gi], (x.y + Q.y * rpl1[gi]) * fac2[gi]);
+    vp[gi] = cto<T>(make_double2((x.x + Q.x * rpl1[gi]) * fac2[gi], (x.y + Q.y * rpl1[gi]) * fac2[gi]));
   }
 }
 
@@ -514,7 +560,8 @@ __global__ void __launch_bounds__(256) k_poisson_bridge(const double2* __restric
 // (the block is sized to hold every pair when it can), lane i owning
 // interior row i; G_e in LDS serves every pair.
 //   x_I = G_e b_I - W_l x_left - W_r x_right,  v = (x + Q rpl1) * fac2.
-__global__ void __launch_bounds__(1024) k_poisson_elem(const double2* __restrict__ rho, double2* __restrict__ v,
+template <class T>
+__global__ void __launch_bounds__(1024) k_poisson_elem(const T* __restrict__ rho, T* __restrict__ v,
                                                       const double2* __restrict__ xb, const PoissonArgs a) {
   // consecutive elements of one L on one XCD: neighbours share cache lines
   const int lt = xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
@@ -550,7 +597,7 @@ __global__ void __launch_bounds__(1024) k_poisson_elem(const double2* __restrict
       valid = L >= (pm < 0 ? -pm : pm);
     }
     const size_t row = ((size_t)p * a.L2 + L) * a.nr;
-    if (valid) sB[threadIdx.x] = cscale(rho[row + base + lane], sInv[lane]);
+    if (valid) sB[threadIdx.x] = cscale(cto<double2>(rho[row + base + lane]), sInv[lane]);
     __syncthreads();
     if (active) {
       double2 out = make_double2(0.0, 0.0);
@@ -568,7 +615,7 @@ __global__ void __launch_bounds__(1024) k_poisson_elem(const double2* __restrict
         const double2 Q = a.Q[(size_t)p * a.L2 + L];
         out = make_double2((y.x + Q.x * sR1[lane]) * sF2[lane], (y.y + Q.y * sR1[lane]) * sF2[lane]);
       }
-      v[row + base + lane] = out;
+      v[row + base + lane] = cto<T>(out);
     }
     __syncthreads();
   }
@@ -579,27 +626,28 @@ __global__ void __launch_bounds__(1024) k_poisson_elem(const double2* __restrict
 // src/hprod/hprod_mfprod2.F90: fcx1, dcj, dcx, actj, actx, act2).
 // Thread per grid point (theta, r); each pair potential is read once.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_mfprod(const double2* __restrict__ orbg, const double2* __restrict__ vg,
-                                                double2* __restrict__ gorbg, double2* __restrict__ v2orbg,
+template <class A2>
+__global__ void __launch_bounds__(256) k_mfprod(const A2* __restrict__ orbg, const A2* __restrict__ vg,
+                                                A2* __restrict__ gorbg, A2* __restrict__ v2orbg,
                                                 const MfArgs a) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   const int th = blockIdx.y;
   if (r >= a.nr) return;
   const size_t ng = (size_t)a.nlat * a.nr;
   const size_t g = (size_t)th * a.nr + r;
-  double2 psi[kMaxFun];
-  double2 ag[kMaxFun], av[kMaxFun];
+  A2 psi[kMaxFun];
+  A2 ag[kMaxFun], av[kMaxFun];
   for (int f = 0; f < a.nfun; ++f) {
     psi[f] = orbg[(size_t)f * ng + g];
-    ag[f] = make_double2(0.0, 0.0);
-    av[f] = make_double2(0.0, 0.0);
+    ag[f] = cz<A2>();
+    av[f] = cz<A2>();
   }
-  double2 d2v_dc = make_double2(0.0, 0.0);
-  double2 vact[kMaxActPairs];
+  A2 d2v_dc = cz<A2>();
+  A2 vact[kMaxActPairs];
   const int nfc = a.nfcore, ncore = a.ncore, na = a.nact;
   // actx needs wd1_j = sum_k psi_k (-den1(k,j)/2) for m_j = m_k
   for (int p = 0; p < a.npair; ++p) {
-    const double2 v = vg[(size_t)p * ng + g];
+    const A2 v = vg[(size_t)p * ng + g];
     const int j = a.pj[p], i = a.pi[p];
     if (a.ptype[p] == 1) {  // v(dyn j, fc1 i): fcx1 uses v(i, j) = conj(v)
       if (r < a.nradfc) ag[j] = csub(ag[j], cmul(cconj(v), psi[i]));
@@ -616,42 +664,42 @@ __global__ void __launch_bounds__(256) k_mfprod(const double2* __restrict__ orbg
     } else if (j >= ncore && idc) {
       // actx: v2orbg(i) += wd1(ja) v(ja, i), wd1(ja) = -1/2 sum_k den1(k,ja) psi_k
       const int ja = j - ncore;
-      double2 wd = make_double2(0.0, 0.0);
+      A2 wd = cz<A2>();
       for (int ka = 0; ka < na; ++ka)
-        if (a.mval[ncore + ka] == a.mval[j]) cfma(wd, psi[ncore + ka], cscale(a.den1[ka + na * ja], -0.5));
+        if (a.mval[ncore + ka] == a.mval[j]) cfma(wd, psi[ncore + ka], cscale(cto<A2>(a.den1[ka + na * ja]), -0.5));
       cfma(av[i], wd, v);
     }
   }
   for (int f = nfc; f < a.nfun; ++f) cfma(ag[f], d2v_dc, psi[f]);  // dcj
   if (ncore > nfc && na > 0) {
     // actj: d2v = sum_jk v(j,k) den1(k,j) applied to dynamical core orbitals
-    double2 d = make_double2(0.0, 0.0);
+    A2 d = cz<A2>();
     for (int ja = 0; ja < na; ++ja)
-      for (int ka = 0; ka < na; ++ka) cfma(d, vact[ja * na + ka], a.den1[ka + na * ja]);
+      for (int ka = 0; ka < na; ++ka) cfma(d, vact[ja * na + ka], cto<A2>(a.den1[ka + na * ja]));
     for (int f = nfc; f < ncore; ++f) cfma(av[f], d, psi[f]);
   }
   if (na > 0 && a.nelact2 >= 2) {
     // act2: t_i = sum_j psi_j d_ij, d_ij = sum_kl den2(j,i,l,k) v(k,l);
     // v2orbg_k += sum_i rden(i,k) t_i
-    double2 t[kMaxAct];
-    for (int q = 0; q < na; ++q) t[q] = make_double2(0.0, 0.0);
+    A2 t[kMaxAct];
+    for (int q = 0; q < na; ++q) t[q] = cz<A2>();
     for (int ia = 0; ia < na; ++ia)
       for (int ja = 0; ja <= ia; ++ja) {
-        double2 d = make_double2(0.0, 0.0);
+        A2 d = cz<A2>();
         for (int ka = 0; ka < na; ++ka)
           for (int la = 0; la < na; ++la) {
             if (a.mval[ncore + ia] + a.mval[ncore + ka] != a.mval[ncore + ja] + a.mval[ncore + la]) continue;
             const double2 dens = a.den2[ja + na * (ia + na * (la + na * ka))];
             if (fabs(dens.x) + fabs(dens.y) == 0.0) continue;
             if (sqrt(dens.x * dens.x + dens.y * dens.y) <= 1.0e-15) continue;
-            cfma(d, dens, vact[ka * na + la]);
+            cfma(d, cto<A2>(dens), vact[ka * na + la]);
           }
         cfma(t[ia], psi[ncore + ja], d);
         if (ia > ja) cfma(t[ja], psi[ncore + ia], cconj(d));
       }
     for (int ia = 0; ia < na; ++ia)
       for (int ka = 0; ka < na; ++ka)
-        if (a.mval[ncore + ia] == a.mval[ncore + ka]) cfma(av[ncore + ka], a.rden[ia + na * ka], t[ia]);
+        if (a.mval[ncore + ia] == a.mval[ncore + ka]) cfma(av[ncore + ka], cto<A2>(a.rden[ia + na * ka]), t[ia]);
   }
   for (int f = nfc; f < a.nfun; ++f) {
     gorbg[(size_t)(f - nfc) * ng + g] = ag[f];
@@ -663,9 +711,9 @@ __global__ void __launch_bounds__(256) k_mfprod(const double2* __restrict__ orbg
 // (NFC frozen cores, NF orbitals, NA active): every orbital and pair index is
 // a compile-time constant, the pairs are visited in the order they are
 // stored (dyn-dyn i <= j, then dyn-fc1).
-template <int NFC, int NF, int NA>
-__global__ void __launch_bounds__(256) k_mfprod_t(const double2* __restrict__ orbg, const double2* __restrict__ vg,
-                                                  double2* __restrict__ gorbg, double2* __restrict__ v2orbg,
+template <int NFC, int NF, int NA, class A2>
+__global__ void __launch_bounds__(256) k_mfprod_t(const A2* __restrict__ orbg, const A2* __restrict__ vg,
+                                                  A2* __restrict__ gorbg, A2* __restrict__ v2orbg,
                                                   const MfArgs a) {
   constexpr int NCORE = NF - NA;
   constexpr int NA2 = NA > 0 ? NA * NA : 1;
@@ -677,32 +725,32 @@ __global__ void __launch_bounds__(256) k_mfprod_t(const double2* __restrict__ or
   const size_t ng = (size_t)a.nlat * a.nr;
   const size_t g = (size_t)th * a.nr + r;
   // every load of the point is issued before any arithmetic
-  double2 psi[NF], ag[NF], av[NF], vdp[NDP];
+  A2 psi[NF], ag[NF], av[NF], vdp[NDP];
 #pragma unroll
   for (int f = 0; f < NF; ++f) psi[f] = orbg[(size_t)f * ng + g];
 #pragma unroll
   for (int q = 0; q < NDP; ++q) vdp[q] = vg[(size_t)q * ng + g];
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
-    ag[f] = make_double2(0.0, 0.0);
-    av[f] = make_double2(0.0, 0.0);
+    ag[f] = cz<A2>();
+    av[f] = cz<A2>();
   }
   // actx weights wd1_j = -1/2 sum_k den1(k,j) psi_k (m_k = m_j)
-  double2 wd[NA > 0 ? NA : 1];
+  A2 wd[NA > 0 ? NA : 1];
 #pragma unroll
   for (int ja = 0; ja < NA; ++ja) {
-    wd[ja] = make_double2(0.0, 0.0);
+    wd[ja] = cz<A2>();
 #pragma unroll
-    for (int ka = 0; ka < NA; ++ka) cfma(wd[ja], psi[NCORE + ka], a.mfc[NA * NA * NA * NA + ka + NA * ja]);
+    for (int ka = 0; ka < NA; ++ka) cfma(wd[ja], psi[NCORE + ka], cto<A2>(a.mfc[NA * NA * NA * NA + ka + NA * ja]));
   }
-  double2 d2v_dc = make_double2(0.0, 0.0);
-  double2 vact[NA2];
+  A2 d2v_dc = cz<A2>();
+  A2 vact[NA2];
   int p = 0;
 #pragma unroll
   for (int i = NFC; i < NF; ++i) {
 #pragma unroll
     for (int j = i; j < NF; ++j) {
-      const double2 v = vdp[p++];
+      const A2 v = vdp[p++];
       const bool jdc = j < NCORE, idc = i < NCORE;
       if (j == i && jdc) d2v_dc = cadd(d2v_dc, cscale(v, 2.0));
       if (jdc) ag[i] = csub(ag[i], cmul(v, psi[j]));
@@ -720,41 +768,41 @@ __global__ void __launch_bounds__(256) k_mfprod_t(const double2* __restrict__ or
     if (i < a.nfcore2) continue;  // type -2 cores act through v2xfc only
 #pragma unroll
     for (int j = NFC; j < NF; ++j) {
-      const double2 v = vg[(size_t)(p++) * ng + g];
+      const A2 v = vg[(size_t)(p++) * ng + g];
       if (r < a.nradfc) ag[j] = csub(ag[j], cmul(cconj(v), psi[i]));
     }
   }
 #pragma unroll
   for (int f = NFC; f < NF; ++f) cfma(ag[f], d2v_dc, psi[f]);
   if (NCORE > NFC && NA > 0) {
-    double2 d = make_double2(0.0, 0.0);
+    A2 d = cz<A2>();
 #pragma unroll
     for (int ja = 0; ja < NA; ++ja)
 #pragma unroll
-      for (int ka = 0; ka < NA; ++ka) cfma(d, vact[ja * NA + ka], a.den1[ka + NA * ja]);
+      for (int ka = 0; ka < NA; ++ka) cfma(d, vact[ja * NA + ka], cto<A2>(a.den1[ka + NA * ja]));
 #pragma unroll
     for (int f = NFC; f < NCORE; ++f) cfma(av[f], d, psi[f]);
   }
   if (NA > 0 && a.nelact2 >= 2) {
-    double2 t[NA > 0 ? NA : 1];
+    A2 t[NA > 0 ? NA : 1];
 #pragma unroll
-    for (int q = 0; q < NA; ++q) t[q] = make_double2(0.0, 0.0);
+    for (int q = 0; q < NA; ++q) t[q] = cz<A2>();
 #pragma unroll
     for (int ia = 0; ia < NA; ++ia)
 #pragma unroll
       for (int ja = 0; ja <= ia; ++ja) {
-        double2 d = make_double2(0.0, 0.0);
+        A2 d = cz<A2>();
 #pragma unroll
         for (int ka = 0; ka < NA; ++ka)
 #pragma unroll
-          for (int la = 0; la < NA; ++la) cfma(d, a.mfc[ja + NA * (ia + NA * (la + NA * ka))], vact[ka * NA + la]);
+          for (int la = 0; la < NA; ++la) cfma(d, cto<A2>(a.mfc[ja + NA * (ia + NA * (la + NA * ka))]), vact[ka * NA + la]);
         cfma(t[ia], psi[NCORE + ja], d);
         if (ia > ja) cfma(t[ja], psi[NCORE + ia], cconj(d));
       }
 #pragma unroll
     for (int ia = 0; ia < NA; ++ia)
 #pragma unroll
-      for (int ka = 0; ka < NA; ++ka) cfma(av[NCORE + ka], a.mfc[NA * NA * NA * NA + NA * NA + ia + NA * ka], t[ia]);
+      for (int ka = 0; ka < NA; ++ka) cfma(av[NCORE + ka], cto<A2>(a.mfc[NA * NA * NA * NA + NA * NA + ia + NA * ka]), t[ia]);
   }
 #pragma unroll
   for (int f = NFC; f < NF; ++f) {
@@ -767,7 +815,8 @@ __global__ void __launch_bounds__(256) k_mfprod_t(const double2* __restrict__ or
 // Velocity-gauge frozen core on the theta grid (hprod_fcorbg,
 // src/hprod/hprod_orbin.F90:68-105): orbg_fc = orbg0_fc * exp(-i A r cos(theta)).
 // ---------------------------------------------------------------------------
-__global__ void k_fc_phase(const double2* __restrict__ orbg0, double2* __restrict__ orbg, const double* __restrict__ xr,
+template <class T>
+__global__ void k_fc_phase(const double2* __restrict__ orbg0, T* __restrict__ orbg, const double* __restrict__ xr,
                            const double* __restrict__ cost, const double* __restrict__ lf, int nr, int nradfc, int nlat, int nfc) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   const int th = blockIdx.y;
@@ -777,7 +826,7 @@ __global__ void k_fc_phase(const double2* __restrict__ orbg0, double2* __restric
   sincos(ph, &s, &c);
   for (int f = 0; f < nfc; ++f) {
     size_t q = ((size_t)f * nlat + th) * nr + r;
-    orbg[q] = r < nradfc ? cmul(orbg0[q], make_double2(c, s)) : make_double2(0.0, 0.0);
+    orbg[q] = cto<T>(r < nradfc ? cmul(orbg0[q], make_double2(c, s)) : make_double2(0.0, 0.0));
   }
 }
 
@@ -791,7 +840,7 @@ __global__ void k_fc_phase(const double2* __restrict__ orbg0, double2* __restric
 // loaded element feeds TA or TB products.  The block result goes to
 // partial[entry][which][block-in-q].
 // ---------------------------------------------------------------------------
-template <int TA, int TB, bool CR>
+template <int TA, int TB, bool CR, class T>
 __global__ void __launch_bounds__(256) k_gram(const GramArgs a, double2* __restrict__ partial, int nqblk) {
   const int ntb = (a.nR + TB - 1) / TB;
   // logical order: (a, b) tile fastest, so the tiles that read one index
@@ -810,12 +859,12 @@ __global__ void __launch_bounds__(256) k_gram(const GramArgs a, double2* __restr
       s0[u][v] = make_double2(0.0, 0.0);
       if (CR) s1[u][v] = make_double2(0.0, 0.0);
     }
-  const double2* Lp[TA];
-  const double2* Rp[TB];
+  const T* Lp[TA];
+  const T* Rp[TB];
 #pragma unroll
-  for (int u = 0; u < TA; ++u) Lp[u] = a0 + u < a.nL ? a.left[a0 + u] : nullptr;
+  for (int u = 0; u < TA; ++u) Lp[u] = a0 + u < a.nL ? static_cast<const T*>(a.left[a0 + u]) : nullptr;
 #pragma unroll
-  for (int v = 0; v < TB; ++v) Rp[v] = b0 + v < a.nR ? a.right[b0 + v] : nullptr;
+  for (int v = 0; v < TB; ++v) Rp[v] = b0 + v < a.nR ? static_cast<const T*>(a.right[b0 + v]) : nullptr;
   // two index values per trip, all their loads issued before the products
   const size_t stride = (size_t)nqblk * 256;
   size_t q = (size_t)qb * 256 + threadIdx.x;
@@ -827,9 +876,9 @@ __global__ void __launch_bounds__(256) k_gram(const GramArgs a, double2* __restr
     for (int h = 0; h < 2; ++h) {
       const size_t qq = h == 0 ? q : (has1 ? q1 : q);
 #pragma unroll
-      for (int u = 0; u < TA; ++u) x[h][u] = Lp[u] ? Lp[u][qq] : make_double2(0.0, 0.0);
+      for (int u = 0; u < TA; ++u) x[h][u] = Lp[u] ? cto<double2>(Lp[u][qq]) : make_double2(0.0, 0.0);
 #pragma unroll
-      for (int v = 0; v < TB; ++v) y[h][v] = Rp[v] ? Rp[v][qq] : make_double2(0.0, 0.0);
+      for (int v = 0; v < TB; ++v) y[h][v] = Rp[v] ? cto<double2>(Rp[v][qq]) : make_double2(0.0, 0.0);
     }
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
@@ -978,19 +1027,25 @@ __global__ void k_mask(double2* __restrict__ wfn, const double* __restrict__ mas
 // ---------------------------------------------------------------------------
 // explicit instantiations / launch wrappers
 // ---------------------------------------------------------------------------
-template <int MODE>
-static void transform_variant(int ks, dim3 grid, hipStream_t s, const double2* in, double2* out, const double* mats,
-                              const TransformBatch* batch, int nr, int K, int C, int Cp, double2* copy) {
+template <int MODE, class TI, class TO>
+static void transform_variant(int ks, dim3 grid, hipStream_t s, const TI* in, TO* out, const double* mats,
+                              const float* matsf, const TransformBatch* batch, int nr, int K, int C, int Cp,
+                              double2* copy) {
   if (ks == 1)
-    hipLaunchKernelGGL((k_transform<MODE, 8, 1>), grid, dim3(256), 0, s, in, out, mats, batch, nr, K, C, Cp, copy);
+    hipLaunchKernelGGL((k_transform<MODE, 8, 1, TI, TO>), grid, dim3(256), 0, s, in, out, mats, matsf, batch, nr, K,
+                       C, Cp, copy);
   else if (ks == 2)
-    hipLaunchKernelGGL((k_transform<MODE, 8, 2>), grid, dim3(256), 0, s, in, out, mats, batch, nr, K, C, Cp, copy);
+    hipLaunchKernelGGL((k_transform<MODE, 8, 2, TI, TO>), grid, dim3(256), 0, s, in, out, mats, matsf, batch, nr, K,
+                       C, Cp, copy);
   else
-    hipLaunchKernelGGL((k_transform<MODE, 8, 4>), grid, dim3(256), 0, s, in, out, mats, batch, nr, K, C, Cp, copy);
+    hipLaunchKernelGGL((k_transform<MODE, 8, 4, TI, TO>), grid, dim3(256), 0, s, in, out, mats, matsf, batch, nr, K,
+                       C, Cp, copy);
 }
 
-void launch_transform(int mode, const double2* in, double2* out, const double* mats, const TransformBatch* batch,
-                      int nbatch, int nr, int K, int C, int Cp, double2* copy, hipStream_t s) {
+template <class TI, class TO>
+void launch_transform_t(int mode, const TI* in, TO* out, const double* mats, const float* matsf,
+                        const TransformBatch* batch, int nbatch, int nr, int K, int C, int Cp, double2* copy,
+                        hipStream_t s) {
   const int rt = (nr + 63) / 64;
   // 8 columns per wavefront; split the contraction over 2 or 4 wavefronts
   // only while the grid is too small to fill the chip (a split re-reads
@@ -999,8 +1054,22 @@ void launch_transform(int mode, const double2* in, double2* out, const double* m
   while (ks < 4 && rt * ((C + 8 * (4 / ks) - 1) / (8 * (4 / ks))) * nbatch < 800) ks *= 2;
   const int cols = 8 * (4 / ks);
   dim3 grid(rt, (C + cols - 1) / cols, nbatch);
-  if (mode == 0) transform_variant<0>(ks, grid, s, in, out, mats, batch, nr, K, C, Cp, copy);
-  else transform_variant<1>(ks, grid, s, in, out, mats, batch, nr, K, C, Cp, nullptr);
+  if (mode == 0) transform_variant<0>(ks, grid, s, in, out, mats, matsf, batch, nr, K, C, Cp, copy);
+  else transform_variant<1>(ks, grid, s, in, out, mats, matsf, batch, nr, K, C, Cp, nullptr);
+}
+template void launch_transform_t<double2, float2>(int, const double2*, float2*, const double*, const float*,
+                                                  const TransformBatch*, int, int, int, int, int, double2*,
+                                                  hipStream_t);
+template void launch_transform_t<float2, float2>(int, const float2*, float2*, const double*, const float*,
+                                                 const TransformBatch*, int, int, int, int, int, double2*,
+                                                 hipStream_t);
+template void launch_transform_t<float2, double2>(int, const float2*, double2*, const double*, const float*,
+                                                  const TransformBatch*, int, int, int, int, int, double2*,
+                                                  hipStream_t);
+
+void launch_transform(int mode, const double2* in, double2* out, const double* mats, const TransformBatch* batch,
+                      int nbatch, int nr, int K, int C, int Cp, double2* copy, hipStream_t s) {
+  launch_transform_t<double2, double2>(mode, in, out, mats, nullptr, batch, nbatch, nr, K, C, Cp, copy, s);
 }
 
 bool launch_onebody_el(const double2* orb, double2* h01, const OneBodyElArgs& a, hipStream_t s) {
@@ -1017,23 +1086,26 @@ void launch_onebody(const double2* orb, double2* h01, const OneBodyArgs& a, hipS
   hipLaunchKernelGGL(k_onebody, grid, block, 0, s, orb, h01, a);
 }
 
-void launch_poisson(const double2* rho, double2* xb, double2* v, const PoissonArgs& a, hipStream_t s) {
+template <class T>
+void launch_poisson(const T* rho, double2* xb, T* v, const PoissonArgs& a, hipStream_t s) {
   const int jobs = a.npair * a.L2;
   if (a.ni == 19)
-    hipLaunchKernelGGL((k_poisson_bridge<19>), dim3(jobs), dim3(256), 0, s, rho, v, xb, a);
+    hipLaunchKernelGGL((k_poisson_bridge<19, T>), dim3(jobs), dim3(256), 0, s, rho, v, xb, a);
   else
-    hipLaunchKernelGGL((k_poisson_bridge<0>), dim3(jobs), dim3(256), 0, s, rho, v, xb, a);
+    hipLaunchKernelGGL((k_poisson_bridge<0, T>), dim3(jobs), dim3(256), 0, s, rho, v, xb, a);
   const int nt = std::min(1024, (a.npair * a.ni + 63) / 64 * 64);
-  hipLaunchKernelGGL(k_poisson_elem, dim3(a.nfe, a.L2), dim3(std::max(nt, 64)), 0, s, rho, v, xb, a);
+  hipLaunchKernelGGL((k_poisson_elem<T>), dim3(a.nfe, a.L2), dim3(std::max(nt, 64)), 0, s, rho, v, xb, a);
 }
+template void launch_poisson<double2>(const double2*, double2*, double2*, const PoissonArgs&, hipStream_t);
+template void launch_poisson<float2>(const float2*, double2*, float2*, const PoissonArgs&, hipStream_t);
 
-void launch_mfprod(const double2* orbg, const double2* vg, double2* gorbg, double2* v2orbg, const MfArgs& a,
-                   hipStream_t s) {
+template <class T>
+void launch_mfprod(const T* orbg, const T* vg, T* gorbg, T* v2orbg, const MfArgs& a, hipStream_t s) {
   dim3 grid((a.nr + 255) / 256, a.nlat), block(256);
   const int key = a.nfcore * 10000 + a.nfun * 100 + a.nact;
 #define MF_CASE(NFC, NF, NA)                                                                            \
   case NFC * 10000 + NF * 100 + NA:                                                                     \
-    hipLaunchKernelGGL((k_mfprod_t<NFC, NF, NA>), grid, block, 0, s, orbg, vg, gorbg, v2orbg, a);     \
+    hipLaunchKernelGGL((k_mfprod_t<NFC, NF, NA, T>), grid, block, 0, s, orbg, vg, gorbg, v2orbg, a);  \
     return;
   switch (key) {
     MF_CASE(0, 1, 0)
@@ -1048,14 +1120,21 @@ void launch_mfprod(const double2* orbg, const double2* vg, double2* gorbg, doubl
       break;
   }
 #undef MF_CASE
-  hipLaunchKernelGGL(k_mfprod, grid, block, 0, s, orbg, vg, gorbg, v2orbg, a);
+  hipLaunchKernelGGL((k_mfprod<T>), grid, block, 0, s, orbg, vg, gorbg, v2orbg, a);
 }
+template void launch_mfprod<double2>(const double2*, const double2*, double2*, double2*, const MfArgs&, hipStream_t);
+template void launch_mfprod<float2>(const float2*, const float2*, float2*, float2*, const MfArgs&, hipStream_t);
 
-void launch_fc_phase(const double2* orbg0, double2* orbg, const double* xr, const double* cost, const double* lf, int nr,
+template <class T>
+void launch_fc_phase(const double2* orbg0, T* orbg, const double* xr, const double* cost, const double* lf, int nr,
                      int nradfc, int nlat, int nfc, hipStream_t s) {
-  hipLaunchKernelGGL(k_fc_phase, dim3((nr + 255) / 256, nlat), dim3(256), 0, s, orbg0, orbg, xr, cost, lf, nr,
+  hipLaunchKernelGGL((k_fc_phase<T>), dim3((nr + 255) / 256, nlat), dim3(256), 0, s, orbg0, orbg, xr, cost, lf, nr,
                      nradfc, nlat, nfc);
 }
+template void launch_fc_phase<double2>(const double2*, double2*, const double*, const double*, const double*, int, int,
+                                       int, int, hipStream_t);
+template void launch_fc_phase<float2>(const double2*, float2*, const double*, const double*, const double*, int, int,
+                                      int, int, hipStream_t);
 
 void launch_gram(const GramArgs& a, double2* partial, int maxblk, double2* out, hipStream_t s) {
   // about 4 index values per thread (two trips of two), at most maxblk
@@ -1064,11 +1143,13 @@ void launch_gram(const GramArgs& a, double2* partial, int maxblk, double2* out, 
   if (a.with_conj_right) {
     constexpr int TA = 4, TB = 4;
     dim3 grid(nblk, ((a.nL + TA - 1) / TA) * ((a.nR + TB - 1) / TB));
-    hipLaunchKernelGGL((k_gram<TA, TB, true>), grid, dim3(256), 0, s, a, partial, nblk);
+    if (a.f32) hipLaunchKernelGGL((k_gram<TA, TB, true, float2>), grid, dim3(256), 0, s, a, partial, nblk);
+    else hipLaunchKernelGGL((k_gram<TA, TB, true, double2>), grid, dim3(256), 0, s, a, partial, nblk);
   } else {
     constexpr int TA = 4, TB = 8;
     dim3 grid(nblk, ((a.nL + TA - 1) / TA) * ((a.nR + TB - 1) / TB));
-    hipLaunchKernelGGL((k_gram<TA, TB, false>), grid, dim3(256), 0, s, a, partial, nblk);
+    if (a.f32) hipLaunchKernelGGL((k_gram<TA, TB, false, float2>), grid, dim3(256), 0, s, a, partial, nblk);
+    else hipLaunchKernelGGL((k_gram<TA, TB, false, double2>), grid, dim3(256), 0, s, a, partial, nblk);
   }
   int nent = a.nL * a.nR * 2;
   hipLaunchKernelGGL(k_gram_final, dim3((nent + 3) / 4), dim3(256), 0, s, partial, nblk, nent, out,
@@ -1168,10 +1249,19 @@ __global__ void k_clock_set(PropClock* __restrict__ st, double t0, double dt) {
   st->n = 0;
 }
 
+__global__ void k_to_f32(float2* __restrict__ y, const double2* __restrict__ x, size_t n) {
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+    y[q] = make_float2((float)x[q].x, (float)x[q].y);
+}
+
 static int grid_for(size_t n) { return (int)std::min<size_t>(2048, std::max<size_t>(1, (n + 255) / 256)); }
 
 void launch_copy(double2* y, const double2* x, size_t n, hipStream_t s) {
   if (n) hipLaunchKernelGGL(k_copy, dim3(grid_for(n)), dim3(256), 0, s, y, x, n);
+}
+
+void launch_to_f32(float2* y, const double2* x, size_t n, hipStream_t s) {
+  if (n) hipLaunchKernelGGL(k_to_f32, dim3(grid_for(n)), dim3(256), 0, s, y, x, n);
 }
 
 void launch_zero(double2* y, size_t n, hipStream_t s) {

@@ -83,8 +83,9 @@ struct GramArgs {
   int nL, nR;
   int conj_left;        // 1: S = sum conj(L) R (inner products); 0: sum L R (zdotu)
   int with_conj_right;  // also accumulate sum opA(L) conj(R)
-  const double2* const* left;   // device array of nL device pointers
-  const double2* const* right;  // device array of nR device pointers
+  const void* const* left;   // device array of nL device pointers (complex128, or complex64 when f32)
+  const void* const* right;  // device array of nR device pointers
+  int f32;                   // the vectors are complex64 (accumulation stays FP64)
 };
 
 struct DtorbArgs {
@@ -144,12 +145,20 @@ struct IslandArgs {
 void launch_transform(int mode, const double2* in, double2* out, const double* mats, const TransformBatch* batch,
                       int nbatch, int nr, int K, int C, int Cp, double2* copy, hipStream_t s);
 inline int pad_cols(int C) { return (C + 31) / 32 * 32; }
+// the same transform with complex64 on either side (matsf: the matrices in
+// FP32, used when both sides are complex64)
+template <class TI, class TO>
+void launch_transform_t(int mode, const TI* in, TO* out, const double* mats, const float* matsf,
+                        const TransformBatch* batch, int nbatch, int nr, int K, int C, int Cp, double2* copy,
+                        hipStream_t s);
 void launch_onebody(const double2* orb, double2* h01, const OneBodyArgs& a, hipStream_t s);
 bool launch_onebody_el(const double2* orb, double2* h01, const OneBodyElArgs& a, hipStream_t s);
-void launch_poisson(const double2* rho, double2* xb, double2* v, const PoissonArgs& a, hipStream_t s);
-void launch_mfprod(const double2* orbg, const double2* vg, double2* gorbg, double2* v2orbg, const MfArgs& a,
-                   hipStream_t s);
-void launch_fc_phase(const double2* orbg0, double2* orbg, const double* xr, const double* cost, const double* lf, int nr,
+template <class T>  // rho and v complex128 or complex64; the solve is FP64
+void launch_poisson(const T* rho, double2* xb, T* v, const PoissonArgs& a, hipStream_t s);
+template <class T>  // theta-grid arrays complex128 or complex64 (arithmetic in the same)
+void launch_mfprod(const T* orbg, const T* vg, T* gorbg, T* v2orbg, const MfArgs& a, hipStream_t s);
+template <class T>  // theta-grid output complex128 or complex64
+void launch_fc_phase(const double2* orbg0, T* orbg, const double* xr, const double* cost, const double* lf, int nr,
                      int nradfc, int nlat, int nfc, hipStream_t s);
 void launch_gram(const GramArgs& a, double2* partial, int nblk, double2* out, hipStream_t s);
 // Length-form dipole <z> = 2 sum_core <i|z|i> + sum_ab conj(den1(b,a)) <b|z|a>
@@ -164,6 +173,7 @@ struct DipoleArgs {
 constexpr int kDipoleMaxBlocks = 4096;
 void launch_dipole(const DipoleArgs& a, double2* partial, double* out, hipStream_t s);
 void launch_copy(double2* y, const double2* x, size_t n, hipStream_t s);
+void launch_to_f32(float2* y, const double2* x, size_t n, hipStream_t s);
 void launch_zero(double2* y, size_t n, hipStream_t s);
 void launch_field_stages(const Pulse& p, PropClock* st, const StageNodes& nodes, double* lf, hipStream_t s);
 void launch_clock_set(PropClock* st, double t0, double dt, hipStream_t s);

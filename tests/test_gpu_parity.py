@@ -195,3 +195,57 @@ def test_dipole_matches_oracle(gpu_lib, name, lf):
     hp.set_wfn0(w)
     got = hp.dipole(lf, hp.to_device(w))
     assert abs(got - ref) <= RTOL * max(1.0, abs(ref)), (got, ref)
+
+
+# Mixed precision (precision = 1): the mean field (theta-grid orbitals, pair
+# densities and potentials, their transforms and products) in complex64, the
+# rest FP64.  Its single-precision rounding (~6e-8 relative per value, summed
+# over up to nlat = 15 terms here) bounds the hprod error well below 1e-5.
+MIXED_RTOL = 1e-5
+
+
+@pytest.mark.parametrize("name,lf", [("he_small", C.LF_LEN), ("dc3_small", C.LF_LEN), ("ne_small", C.LF_VEL),
+                                     ("cas_fc_small", C.LF_VEL), ("cas_small", C.LF_LEN), ("dc4_small", C.LF_LEN)])
+def test_mixed_htot_close_to_oracle(gpu_lib, name, lf):
+    import torch
+    import td1c_amd
+    from oracle.oracle import Oracle
+
+    prob = getattr(C, name)(precision=1)
+    o = Oracle(prob)
+    w = C.wfn_for(prob, o.lcic)
+    o.set_wfn0(w)
+    ref = o.htot(1.0, lf, w)
+    hp = td1c_amd.HProd(prob)
+    hp.set_wfn0(w)
+    out = hp.htot(1.0, lf, hp.to_device(w))
+    torch.cuda.synchronize()
+    err = _rel(out.cpu().numpy(), ref)
+    assert 1e-14 < err < MIXED_RTOL, err  # complex64 really used, and close
+
+
+def test_mixed_propagation_dipole_error(gpu_lib):
+    """Dipole of a mixed-precision propagation (ne_small, 60 RK4 steps in a
+    sin^2 field) within 1e-6 of the FP64 propagation at every 10th step."""
+    import torch
+    import td1c_amd
+
+    field = td1c_amd.Field(env_type="sin2", gauge="velocity1", fint=1e14, wlen=800.0, numcyc=2.0)
+    rk = td1c_amd.CRK(4)
+    dt, t = 2.0e-5, 30.0
+    dips = {}
+    for prec in (0, 1):
+        prob = C.ne_small(precision=prec)
+        hp = td1c_amd.HProd(prob)
+        w = C.wfn_for(prob, hp.lcic)
+        hp.set_wfn0(w)
+        d = hp.to_device(w)
+        series = []
+        for k in range(6):
+            rk.propagate(hp, d, t + 10 * k * dt, dt, 10, field)
+            lf = td1c_amd.field_value(field, t + 10 * (k + 1) * dt)
+            series.append(hp.dipole(lf, d))
+        torch.cuda.synchronize()
+        dips[prec] = np.array(series)
+    err = np.abs(dips[1] - dips[0]).max()
+    assert err < 1e-6, (err, dips)
