@@ -165,12 +165,21 @@ int td1c_rk_prop(td1c_ctx* ctx, int rk_order, double dtime,
 /* nstep Runge-Kutta steps (+ mask) from time t0 with the stage fields
  * evaluated on the device: the time loop of tdse_split2
  * (src/main/tdse.cpp:225-266) without its printing, each step doing what
- * td1c_rk_prop does with clfield::get_value's values.  The first call for a
- * given (rk_order, dtime, field, d_wfn, apply_mask) captures one step as a
- * HIP graph; every step replays it.  Enqueued on `stream`, asynchronous. */
+ * td1c_rk_prop does with clfield::get_value's values (computed on the
+ * device from the pulse parameters, no host round trip).  With
+ * td1c_set_graph(ctx, 1) the first call for a given (rk_order, dtime,
+ * field, d_wfn, apply_mask) captures one step as a HIP graph and every step
+ * replays it.  Enqueued on `stream`, asynchronous. */
 int td1c_propagate(td1c_ctx* ctx, const td1c_field* field, int rk_order,
                    double t0, double dtime, int nstep, double* d_wfn,
                    int apply_mask, void* stream);
+
+/* How td1c_propagate launches its steps: 0 (default) enqueues every kernel
+ * of every step on the stream, so that the independent branches of an hprod
+ * (CI density stage, one-body part, pair Gram products) run concurrently on
+ * the context's side streams; 1 captures one step as a HIP graph and
+ * replays it (fewer host calls, the branches serialised). */
+int td1c_set_graph(td1c_ctx* ctx, int on);
 
 /* Applies the radial mask to the dynamical orbitals of a device wave
  * function (wfn_mask, src/wfn/wfn_mask.F90:2-35). */

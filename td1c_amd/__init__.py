@@ -61,6 +61,8 @@ def load_library():
     L.td1c_propagate.argtypes = [vp, ctypes.POINTER(TD1CField), ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                  ctypes.c_int, vp, ctypes.c_int, vp]
     L.td1c_propagate.restype = ctypes.c_int
+    L.td1c_set_graph.argtypes = [vp, ctypes.c_int]
+    L.td1c_set_graph.restype = ctypes.c_int
     L.td1c_dipole.argtypes = [vp, vp, vp, vp, vp, vp]
     L.td1c_dipole.restype = ctypes.c_int
     L.td1c_mask.argtypes = [vp, vp, vp]
@@ -221,6 +223,11 @@ class HProd:
         buf = ctypes.create_string_buffer(1 << 16)
         _check(load_library().td1c_get_timing(self.h, buf, len(buf)), "td1c_get_timing")
         return {k: tuple(v) for k, v in json.loads(buf.value.decode()).items()}
+
+    def set_graph(self, on: bool):
+        """td1c_propagate replays one captured step (True) or enqueues every
+        kernel (False, the default)."""
+        _check(load_library().td1c_set_graph(self.h, int(on)), "td1c_set_graph")
 
     def bind(self):
         """Makes this context the one the legacy hprod_htot_ entry point uses."""

@@ -88,7 +88,7 @@ struct td1c_ctx {
   // Gram
   DevBuf<const void*> g_left_l, g_right_l, g_left_p, g_right_p;
   int gl_nL = 0, gl_nR = 0, gp_nL = 0, gp_nR = 0, gram_blk = 0;
-  DevBuf<double2> gram_partial, lprod, pprod;
+  DevBuf<double2> gram_partial, gram_partial_p, lprod, pprod;
   DevBuf<int> mapH, mapG, mapV, mapF, map2e;
   // island
   DevBuf<double2> den1, den2, rden, rrden, int1e, int2e, xmat, dcic, ci_work, mfc, dip_part;
@@ -100,9 +100,14 @@ struct td1c_ctx {
   DevBuf<double> lf_dev;  // field values of the stages, [kMaxStages][9]
   DevBuf<PropClock> clock;
   bool wfn0_set = false;
+  // two side streams (default priority) for the independent branches of an
+  // hprod, joined back by events; the events carry no timing
+  hipStream_t side[2] = {nullptr, nullptr};
+  hipEvent_t fj[6] = {};
   // device-driven propagation: one captured RK step, replayed per step
   hipStream_t cap_stream = nullptr;
   hipGraphExec_t gexec = nullptr;
+  bool use_graph = false;  // td1c_set_graph
   bool capturing = false;
   int g_order = 0, g_mask = 0;
   double g_dt = 0.0;
@@ -122,7 +127,7 @@ struct td1c_ctx {
                            &pG, &pWl, &pWr, &pSd, &pSl, &pSu, &invr, &rpl0, &rpl1, &fac2, &fac1, &ene, &lf_dev};
     for (auto* b : d) b->release();
     DevBuf<double2>* z[] = {&tdiag, &tzor, &lsp, &gsp, &orbg0, &rhoL, &vL, &vg, &p_xb, &p_Q,
-                            &gram_partial, &lprod, &pprod, &den1, &den2, &rden, &rrden, &int1e, &int2e, &xmat,
+                            &gram_partial, &gram_partial_p, &lprod, &pprod, &den1, &den2, &rden, &rrden, &int1e, &int2e, &xmat,
                             &dcic, &ci_work, &mfc, &dip_part, &rk_w0, &rk_t, &rk_k};
     for (auto* b : z) b->release();
     DevBuf<int>* i[] = {&mval, &pair_m, &pj, &pi, &ptype, &mapH, &mapG, &mapV, &mapF, &map2e,
@@ -139,6 +144,10 @@ struct td1c_ctx {
     for (auto* b : fm) b->release();
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (cap_stream) (void)hipStreamDestroy(cap_stream);
+    for (auto& e : fj)
+      if (e) (void)hipEventDestroy(e);
+    for (auto& q : side)
+      if (q) (void)hipStreamDestroy(q);
     if (clock.p) (void)hipFree(clock.p);
   }
 };
@@ -188,6 +197,7 @@ struct Timed {
   }
 };
 #define TIMED(name) Timed timed_##__LINE__(c, s, name)
+#define TIMED_ON(stream, name) Timed timed_##__LINE__(c, stream, name)
 
 template <class T>
 std::vector<T> flat_or_one(const std::vector<T>& v) {
@@ -423,6 +433,7 @@ int build_device(td1c_ctx* c) {
   {
     size_t ent = std::max((size_t)c->gl_nL * c->gl_nR, (size_t)c->gp_nL * c->gp_nR);
     HIP_TRY(c->gram_partial.alloc(std::max<size_t>(1, ent * 2 * c->gram_blk)));
+    HIP_TRY(c->gram_partial_p.alloc(std::max<size_t>(1, ent * 2 * c->gram_blk)));
   }
   // island
   const int na = std::max(1, S.nact);
@@ -456,6 +467,8 @@ int build_device(td1c_ctx* c) {
   HIP_TRY(c->rk_k.alloc(wsz * kMaxStages));
   HIP_TRY(c->lf_dev.alloc(9 * kMaxStages));
   HIP_TRY(c->clock.alloc(1));
+  for (auto& q : c->side) HIP_TRY(hipStreamCreateWithFlags(&q, hipStreamNonBlocking));
+  for (auto& e : c->fj) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   return 0;
 }
 
@@ -526,15 +539,39 @@ void enqueue_orbin(td1c_ctx* c, const double* d_lf, const double2* wfn, hipStrea
   }
 }
 
+// int2e Gram products on side stream s1 once the pair potentials exist on s;
+// records fj[4] on s1 when they are done.
+hipError_t enqueue_pair_gram(td1c_ctx* c, hipStream_t s, hipStream_t s1) {
+  hipError_t e = hipEventRecord(c->fj[5], s);
+  if (e == hipSuccess) e = hipStreamWaitEvent(s1, c->fj[5], 0);
+  if (e != hipSuccess) return e;
+  if (c->gp_nL > 0) {
+    GramArgs h{};
+    h.len = (size_t)c->S.L2 * c->S.nr; h.nL = c->gp_nL; h.nR = c->gp_nR; h.conj_left = 0; h.with_conj_right = 1;
+    h.left = c->g_left_p.p; h.right = c->g_right_p.p; h.f32 = c->mixed ? 1 : 0;
+    TIMED_ON(s1, "gram_pair");
+    launch_gram(h, c->gram_partial_p.p, c->gram_blk, c->pprod.p, s1);
+  }
+  return hipEventRecord(c->fj[4], s1);
+}
+
 // d_lf: the stage's lfield[9] in device memory.
 int enqueue_htotx(td1c_ctx* c, double dtime, const double* d_lf, const double2* wfn, const double2* cic,
                   const CiStage& cs, hipStream_t s) {
   Setup& S = c->S;
   const int nr = S.nr, nf = S.nfun, nfc = S.nfcore, L1 = S.L1, L2 = S.L2, nl = S.nlat;
   const size_t nb = (size_t)c->nbas;
-  enqueue_orbin(c, d_lf, wfn, s);
+  // Branches: side 0 runs the CI RDM stage and later the pair Gram products,
+  // side 1 the one-body part; the main stream runs the mean-field chain.
+  hipStream_t s1 = c->side[0], s2 = c->side[1];
+  HIP_TRY(hipEventRecord(c->fj[0], s));
+  HIP_TRY(hipStreamWaitEvent(s1, c->fj[0], 0));
   IslandArgs ia = island_args(c, cic, dtime);
-  if (S.nact > 0) { TIMED("island_a"); launch_island_a(ia, s); }
+  if (S.nact > 0) { TIMED_ON(s1, "island_a"); launch_island_a(ia, s1); }
+  HIP_TRY(hipEventRecord(c->fj[1], s1));
+  enqueue_orbin(c, d_lf, wfn, s);
+  HIP_TRY(hipEventRecord(c->fj[2], s));
+  HIP_TRY(hipStreamWaitEvent(s2, c->fj[2], 0));
   // one-body part
   OneBodyArgs ob{};
   ob.nr = nr; ob.L1 = L1; ob.ndvr = S.n; ob.nfcore = nfc; ob.nradfc = S.nradfc;
@@ -548,9 +585,11 @@ int enqueue_htotx(td1c_ctx* c, double dtime, const double* d_lf, const double2* 
   oe.eK0 = c->eK0.p; oe.eK = c->eK.p; oe.eP = c->eP.p; oe.xr = c->xr.p; oe.alph = c->alph.p; oe.tzor = c->tzor.p;
   oe.lf = d_lf;
   {
-    TIMED("onebody");
-    if (!launch_onebody_el(c->orb, c->h01, oe, s)) launch_onebody(c->orb, c->h01, ob, s);
+    TIMED_ON(s2, "onebody");
+    if (!launch_onebody_el(c->orb, c->h01, oe, s2)) launch_onebody(c->orb, c->h01, ob, s2);
   }
+  HIP_TRY(hipEventRecord(c->fj[3], s2));
+  bool pair_gram_done = false;
   // mean field
   const size_t ng = (size_t)c->ngrid;
   if (S.nel[2] >= 2 && c->npair > 0) {
@@ -571,6 +610,8 @@ int enqueue_htotx(td1c_ctx* c, double dtime, const double* d_lf, const double2* 
     pa.Q = c->p_Q.p;
     if (c->mixed) {
       { TIMED("poisson"); launch_poisson(c->rhoL_f.p, c->p_xb.p, c->vL_f.p, pa, s); }
+      HIP_TRY(enqueue_pair_gram(c, s, s1));
+      pair_gram_done = true;
       {
         TIMED("pair_L2theta");
         launch_transform_t<float2, float2>(0, c->vL_f.p, c->vg_f.p, c->m_L2g.p, c->m_L2g_f.p, c->b_L2g.p, c->npair, nr,
@@ -578,6 +619,8 @@ int enqueue_htotx(td1c_ctx* c, double dtime, const double* d_lf, const double2* 
       }
     } else {
       { TIMED("poisson"); launch_poisson(c->rhoL.p, c->p_xb.p, c->vL.p, pa, s); }
+      HIP_TRY(enqueue_pair_gram(c, s, s1));
+      pair_gram_done = true;
       { TIMED("pair_L2theta"); launch_transform(0, c->vL.p, c->vg.p, c->m_L2g.p, c->b_L2g.p, c->npair, nr, L2, nl, pad_cols(nl), nullptr, s); }
     }
     MfArgs ma{};
@@ -585,6 +628,7 @@ int enqueue_htotx(td1c_ctx* c, double dtime, const double* d_lf, const double2* 
     ma.nradfc = S.nradfc; ma.npair = c->npair; ma.nelact2 = S.nelact[2];
     ma.pj = c->pj.p; ma.pi = c->pi.p; ma.ptype = c->ptype.p; ma.mval = c->mval.p;
     ma.den1 = c->den1.p; ma.den2 = c->den2.p; ma.rden = c->rden.p; ma.mfc = c->mfc.p;
+    HIP_TRY(hipStreamWaitEvent(s, c->fj[1], 0));  // RDM stage done
     if (c->mixed) {
       { TIMED("mfprod"); launch_mfprod(c->orbg_f, c->vg_f.p, c->gorbg_f, c->v2orbg_f, ma, s); }
       {
@@ -600,17 +644,16 @@ int enqueue_htotx(td1c_ctx* c, double dtime, const double* d_lf, const double2* 
     launch_zero(c->gorb, 2 * nb * c->nd, s);
   }
   (void)ng;
-  // inner products
+  if (!pair_gram_done) HIP_TRY(enqueue_pair_gram(c, s, s1));
+  // inner products of the l-space results (needs the one-body branch)
+  HIP_TRY(hipStreamWaitEvent(s, c->fj[3], 0));
   GramArgs g{};
   g.len = nb; g.nL = c->gl_nL; g.nR = c->gl_nR; g.conj_left = 1; g.with_conj_right = 0;
   g.left = c->g_left_l.p; g.right = c->g_right_l.p;
   { TIMED("gram_l"); launch_gram(g, c->gram_partial.p, c->gram_blk, c->lprod.p, s); }
-  if (c->gp_nL > 0) {
-    GramArgs h{};
-    h.len = (size_t)L2 * nr; h.nL = c->gp_nL; h.nR = c->gp_nR; h.conj_left = 0; h.with_conj_right = 1;
-    h.left = c->g_left_p.p; h.right = c->g_right_p.p; h.f32 = c->mixed ? 1 : 0;
-    { TIMED("gram_pair"); launch_gram(h, c->gram_partial.p, c->gram_blk, c->pprod.p, s); }
-  }
+  // join the side streams
+  HIP_TRY(hipStreamWaitEvent(s, c->fj[1], 0));
+  HIP_TRY(hipStreamWaitEvent(s, c->fj[4], 0));
   ia.kout = cs.kout; ia.acc = cs.acc; ia.dtw = cs.dtw; ia.next = cs.next; ia.w0 = cs.w0;
   ia.nprev = cs.nprev; ia.cnew = cs.cnew;
   for (int j = 0; j < kMaxStages; ++j) { ia.kprev[j] = cs.kprev[j]; ia.cprev[j] = cs.cprev[j]; }
@@ -941,6 +984,20 @@ int td1c_propagate(td1c_ctx* c, const td1c_field* field, int rk_order, double t0
   if (nstep == 0) return 0;
   hipStream_t s = (hipStream_t)stream;
   const Pulse p = make_pulse(field);
+  if (!c->use_graph) {
+    // stream launches: the hprod's side-stream branches then run concurrently
+    StageNodes sn{};
+    sn.ns = ns;
+    for (int i = 0; i < ns; ++i) sn.c[i] = nodes[i];
+    launch_clock_set(c->clock.p, t0, dtime, s);
+    for (int i = 0; i < nstep; ++i) {
+      launch_field_stages(p, c->clock.p, sn, c->lf_dev.p, s);
+      int rc = enqueue_rk_step(c, rk_order, dtime, c->lf_dev.p, reinterpret_cast<double2*>(d_wfn), apply_mask, s);
+      if (rc) return rc;
+    }
+    HIP_TRY(hipGetLastError());
+    return 0;
+  }
   const bool same = c->gexec && c->g_order == rk_order && c->g_mask == apply_mask && c->g_wfn == d_wfn &&
                     c->g_dt == dtime && std::memcmp(&c->g_pulse, &p, sizeof(Pulse)) == 0;
   if (!same) {
@@ -1032,6 +1089,12 @@ int td1c_get_timing(td1c_ctx* c, char* buf, int buflen) {
   c->timing_agg.clear();
   if ((int)out.size() + 1 > buflen) return fail(-1, "timing buffer too small");
   std::memcpy(buf, out.c_str(), out.size() + 1);
+  return 0;
+}
+
+int td1c_set_graph(td1c_ctx* c, int on) {
+  if (!c) return fail(-1, "null argument");
+  c->use_graph = on != 0;
   return 0;
 }
 

@@ -144,12 +144,13 @@ def test_legacy_entry_point(gpu_lib):
     assert _rel(hw, ref[:o.norb]) < RTOL
 
 
+@pytest.mark.parametrize("graph", [True, False])
 @pytest.mark.parametrize("name", ["ne_small", "cas_fc_small"])
-def test_graph_propagation_matches_oracle(gpu_lib, name):
-    """td1c_propagate (device stage fields, one captured step replayed per
-    step) against the oracle stepping with host field values; the graph is
-    replayed over three calls with eager hprods and synchronisations in
-    between, each call checked."""
+def test_graph_propagation_matches_oracle(gpu_lib, name, graph):
+    """td1c_propagate (device stage fields; stream launches, or one captured
+    step replayed per step) against the oracle stepping with host field
+    values; three calls with eager hprods and synchronisations in between
+    (graph replays across them), each call checked."""
     import torch
     import td1c_amd
     from oracle.oracle import Oracle
@@ -160,6 +161,7 @@ def test_graph_propagation_matches_oracle(gpu_lib, name):
     o.set_wfn0(w)
     hp = td1c_amd.HProd(prob)
     hp.set_wfn0(w)
+    hp.set_graph(graph)
     field = td1c_amd.Field(env_type="sin2", gauge="velocity1", fint=1e14, wlen=800.0, numcyc=2.0)
     rk = td1c_amd.CRK(4)
     dt, t = 2.0e-5, 30.0  # inside the RK4 stability bound of both problems
