@@ -42,7 +42,7 @@ if ROOT not in sys.path:
 
 HBM_PEAK_GBS = 8000.0
 NSTEP_PROP = 2000
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "ne_l31_fp64_traffic.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "ne_l31_traffic.json")
 # device kernels behind each timing name (HBM counters are per device kernel)
 TIMING_KERNELS = {
     "poisson": ["k_poisson_bridge", "k_poisson_elem"],
@@ -68,10 +68,11 @@ def source_hash() -> str:
     return h.hexdigest()[:16]
 
 
-def recorded_traffic(timing_name: str):
-    """HBM bytes per launch of a timed kernel from the recorded rocprofv3
-    FETCH_SIZE / WRITE_SIZE counters (scaled by the copy-kernel
-    calibration), or None when no record matches the current sources."""
+def recorded_traffic(timing_name: str, path: str):
+    """HBM bytes per launch of a timed kernel of the fp64 or mixed path from
+    the recorded rocprofv3 FETCH_SIZE / WRITE_SIZE counters (scaled by the
+    copy-kernel calibration), or None when no record matches the current
+    sources."""
     try:
         with open(TRAFFIC_FILE) as f:
             rec = json.load(f)
@@ -81,9 +82,9 @@ def recorded_traffic(timing_name: str):
         return None
     total = 0.0
     for k in TIMING_KERNELS[timing_name]:
-        if k not in rec.get("kernels", {}):
+        if k not in rec.get(path, {}):
             return None
-        e = rec["kernels"][k]
+        e = rec[path][k]
         total += e["fetch_kib"] * 1024.0 * rec["copy_factor"]["FETCH_SIZE"]
         total += e["write_kib"] * 1024.0 * rec["copy_factor"]["WRITE_SIZE"]
     return total
@@ -301,7 +302,7 @@ def main():
                   "dipole_check": "<z> every 20 steps over 200 steps, mixed vs fp64, same initial state"},
         "roofline": {"bound": "hbm", "kernel": dname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": recorded_traffic(dname) if use is fp64 else None,
+                     "traffic": recorded_traffic(dname, "mixed" if use is mixed else "fp64"),
                      "algorithmic_bytes_per_launch": abytes, "avg_launch_us": avg_s * 1e6},
         "ms_per_step_stream_launch": use["ms_stream"],
         "kernel_ms_per_step": {k: v[0] / nprof for k, v in sorted(use["breakdown"].items(), key=lambda kv: -kv[1][0])},

@@ -444,7 +444,6 @@ __global__ void __launch_bounds__(256) k_poisson_bridge(const T* __restrict__ rh
   __shared__ double2 sS[kMaxBridge + 1];
   __shared__ double2 sCr[kMaxBridge + 1];
   __shared__ double2 sY[kMaxBridge];
-  __shared__ double sL[kMaxBridge], sU[kMaxBridge], sD[kMaxBridge];
   __shared__ double2 sRed[4];
   const int pm = a.pair_m[p];
   const size_t row = ((size_t)p * a.L2 + L) * a.nr;
@@ -463,11 +462,6 @@ __global__ void __launch_bounds__(256) k_poisson_bridge(const T* __restrict__ rh
   const double* Wl = a.Wl + (size_t)L * a.nr;
   const double* Wr = a.Wr + (size_t)L * a.nr;
   const double* rpl0 = a.rpl0 + (size_t)L * a.nr;
-  for (int k = tid; k < nb; k += 256) {
-    sL[k] = a.Sl[(size_t)L * nb + k];
-    sU[k] = a.Su[(size_t)L * nb + k];
-    sD[k] = 1.0 / a.Sd[(size_t)L * nb + k];
-  }
   // per element: cl = W_l . b_I, cr = W_r . b_I with b = rho * invr
   for (int e = tid; e < a.nfe; e += 256) {
     const int base = e * a.n;
@@ -525,26 +519,33 @@ __global__ void __launch_bounds__(256) k_poisson_bridge(const T* __restrict__ rh
     sY[k] = sk;
   }
   __syncthreads();
-  if (tid == 0 && nb > 0) {
-    // forward: y_k = s_k - l_k y_(k-1); backward: x_k = (y_k - u_k x_(k+1)) / d_k
-    double2 y = sY[0];
-    sS[0] = y;
-    for (int k = 1; k < nb; ++k) {
-      const double2 sk = sY[k];
-      const double lk = sL[k];
-      y = make_double2(fma(-lk, y.x, sk.x), fma(-lk, y.y, sk.y));
-      sS[k] = y;
+  // parallel cyclic reduction of the Schur system with the per-L
+  // coefficients: pcr_steps stencil sweeps over the right-hand side
+  double2* cur = sY;
+  double2* nxt = sS;
+  const int K = a.pcr_steps;
+  const double* pa = a.pcrA + (size_t)L * K * nb;
+  const double* pg = a.pcrG + (size_t)L * K * nb;
+  for (int k = 0, h = 1; k < K; ++k, h <<= 1) {
+    for (int i = tid; i < nb; i += 256) {
+      double2 d = cur[i];
+      if (i - h >= 0) {
+        const double al = pa[(size_t)k * nb + i];
+        d = make_double2(fma(al, cur[i - h].x, d.x), fma(al, cur[i - h].y, d.y));
+      }
+      if (i + h < nb) {
+        const double ga = pg[(size_t)k * nb + i];
+        d = make_double2(fma(ga, cur[i + h].x, d.x), fma(ga, cur[i + h].y, d.y));
+      }
+      nxt[i] = d;
     }
-    double2 x = cscale(y, sD[nb - 1]);
-    sCr[nb - 1] = x;
-    for (int k = nb - 2; k >= 0; --k) {
-      const double2 yk = sS[k];
-      const double uk = sU[k], dk = sD[k];
-      x = make_double2(fma(-uk, x.x, yk.x) * dk, fma(-uk, x.y, yk.y) * dk);
-      sCr[k] = x;
-    }
-    a.Q[(size_t)p * a.L2 + L] = Q;
+    __syncthreads();
+    double2* t = cur;
+    cur = nxt;
+    nxt = t;
   }
+  for (int i = tid; i < nb; i += 256) sCr[i] = cscale(cur[i], a.pcrBinv[(size_t)L * nb + i]);
+  if (tid == 0) a.Q[(size_t)p * a.L2 + L] = Q;
   __syncthreads();
   const double* rpl1 = a.rpl1 + (size_t)L * a.nr;
   const double* fac2 = a.fac2 + (size_t)L * a.nr;

@@ -379,6 +379,11 @@ bool Setup::build(const td1c_problem& p) {
   pSd.assign((size_t)L2 * std::max(nb, 1), 0);
   pSl.assign((size_t)L2 * std::max(nb, 1), 0);
   pSu.assign((size_t)L2 * std::max(nb, 1), 0);
+  pcr_steps = 0;
+  while ((1 << pcr_steps) < nb) ++pcr_steps;
+  pcrA.assign((size_t)L2 * std::max(pcr_steps, 1) * std::max(nb, 1), 0);
+  pcrG.assign((size_t)L2 * std::max(pcr_steps, 1) * std::max(nb, 1), 0);
+  pcrBinv.assign((size_t)L2 * std::max(nb, 1), 0);
   auto A = [&](int L, int i, int j) -> double {  // interior indices
     if (std::abs(i - j) > n) return 0.0;
     double v = 2.0 * radk[bw * (i + 1) + (n + j - i)];
@@ -424,6 +429,34 @@ bool Setup::build(const td1c_problem& p) {
       if (bl >= 0) sdiag[e - 1] -= gll;
       if (br >= 0) sdiag[e] -= grr;
       if (bl >= 0 && br >= 0) soff[e - 1] -= glr;
+    }
+    // parallel cyclic reduction coefficients of S (symmetric: sub = super)
+    {
+      const int K = pcr_steps;
+      std::vector<double> a(nb), b(nb), c(nb), na(nb), nbv(nb), nc(nb);
+      for (int i = 0; i < nb; ++i) {
+        a[i] = i > 0 ? soff[i - 1] : 0.0;
+        b[i] = sdiag[i];
+        c[i] = i + 1 < nb ? soff[i] : 0.0;
+      }
+      for (int k = 0, h = 1; k < K; ++k, h *= 2) {
+        for (int i = 0; i < nb; ++i) {
+          const double al = i - h >= 0 ? -a[i] / b[i - h] : 0.0;
+          const double ga = i + h < nb ? -c[i] / b[i + h] : 0.0;
+          pcrA[((size_t)L * K + k) * nb + i] = al;
+          pcrG[((size_t)L * K + k) * nb + i] = ga;
+          na[i] = i - h >= 0 ? al * a[i - h] : 0.0;
+          nc[i] = i + h < nb ? ga * c[i + h] : 0.0;
+          nbv[i] = b[i] + (i - h >= 0 ? al * c[i - h] : 0.0) + (i + h < nb ? ga * a[i + h] : 0.0);
+        }
+        a.swap(na);
+        b.swap(nbv);
+        c.swap(nc);
+      }
+      for (int i = 0; i < nb; ++i) {
+        if (!(b[i] > 0)) { error = "Poisson Schur system: cyclic reduction lost positivity"; return false; }
+        pcrBinv[(size_t)L * nb + i] = 1.0 / b[i];
+      }
     }
     // Thomas factorisation of the symmetric tridiagonal S
     double* Sd = &pSd[(size_t)L * nb];

@@ -64,6 +64,12 @@ struct Setup {
   std::vector<double> pSd;   // [L2][nfe-1]  Thomas: modified diagonal (pivots)
   std::vector<double> pSl;   // [L2][nfe-1]  Thomas: multipliers l_k = S(k,k-1)/piv(k-1)
   std::vector<double> pSu;   // [L2][nfe-1]  S(k,k+1)
+  // parallel cyclic reduction of the same Schur system, per L: step k
+  // (stride 2^k) updates d_i += pa[i] d_(i-h) + pg[i] d_(i+h); after
+  // pcr_steps steps x_i = d_i * pbinv[i]
+  int pcr_steps = 0;
+  std::vector<double> pcrA, pcrG;  // [L2][pcr_steps][nfe-1]
+  std::vector<double> pcrBinv;     // [L2][nfe-1]
 
   // orbital pair list evaluated per hprod: (j, i) as in hprod_mkrho2p
   struct Pair { int j, i, type; };  // type 0 = dyn-dyn, 1 = dyn-fc1
