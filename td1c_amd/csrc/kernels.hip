@@ -445,6 +445,7 @@ __global__ void __launch_bounds__(256) k_poisson_bridge(const T* __restrict__ rh
   __shared__ double2 sCr[kMaxBridge + 1];
   __shared__ double2 sY[kMaxBridge];
   __shared__ double2 sRed[4];
+  __shared__ double sPA[kMaxPcrSteps * kMaxBridge], sPG[kMaxPcrSteps * kMaxBridge];
   const int pm = a.pair_m[p];
   const size_t row = ((size_t)p * a.L2 + L) * a.nr;
   T* vp = v + row;
@@ -462,6 +463,17 @@ __global__ void __launch_bounds__(256) k_poisson_bridge(const T* __restrict__ rh
   const double* Wl = a.Wl + (size_t)L * a.nr;
   const double* Wr = a.Wr + (size_t)L * a.nr;
   const double* rpl0 = a.rpl0 + (size_t)L * a.nr;
+  // the cyclic-reduction coefficients of this L, staged in LDS early so
+  // their loads overlap the element dots
+  {
+    const int K = a.pcr_steps;
+    const double* pa = a.pcrA + (size_t)L * K * nb;
+    const double* pg = a.pcrG + (size_t)L * K * nb;
+    for (int q = tid; q < K * nb; q += 256) {
+      sPA[q] = pa[q];
+      sPG[q] = pg[q];
+    }
+  }
   // per element: cl = W_l . b_I, cr = W_r . b_I with b = rho * invr
   for (int e = tid; e < a.nfe; e += 256) {
     const int base = e * a.n;
@@ -524,8 +536,8 @@ __global__ void __launch_bounds__(256) k_poisson_bridge(const T* __restrict__ rh
   double2* cur = sY;
   double2* nxt = sS;
   const int K = a.pcr_steps;
-  const double* pa = a.pcrA + (size_t)L * K * nb;
-  const double* pg = a.pcrG + (size_t)L * K * nb;
+  const double* pa = sPA;
+  const double* pg = sPG;
   for (int k = 0, h = 1; k < K; ++k, h <<= 1) {
     for (int i = tid; i < nb; i += 256) {
       double2 d = cur[i];

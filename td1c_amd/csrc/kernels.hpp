@@ -15,6 +15,7 @@ constexpr int kMaxAct = 8;        // active orbitals handled on the theta grid
 constexpr int kMaxActPairs = kMaxAct * kMaxAct;
 constexpr int kMaxStages = 6;
 constexpr int kMaxBridge = 160;   // nfe - 1 bridges handled by the Poisson bridge kernel
+constexpr int kMaxPcrSteps = 8;   // ceil(log2(kMaxBridge))
 constexpr int kOneBodyLChunk = 2;  // l values per block of the element one-body kernel
 
 // Time of a device-driven propagation: step n runs at t0 + n * dt.
