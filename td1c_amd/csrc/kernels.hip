@@ -463,6 +463,16 @@ __global__ void __launch_bounds__(256) k_poisson_bridge(const T* __restrict__ rh
   const double* Wl = a.Wl + (size_t)L * a.nr;
   const double* Wr = a.Wr + (size_t)L * a.nr;
   const double* rpl0 = a.rpl0 + (size_t)L * a.nr;
+  // per-bridge inputs of thread k = tid (nfe - 1 <= 256), loaded up front
+  double2 rb = make_double2(0.0, 0.0);
+  double binv = 0.0, r1 = 0.0, f2 = 0.0;
+  if (tid < nb) {
+    const int gi = (tid + 1) * a.n - 1;
+    rb = cscale(cto<double2>(rp[gi]), invr[gi]);
+    binv = a.pcrBinv[(size_t)L * nb + tid];
+    r1 = a.rpl1[(size_t)L * a.nr + gi];
+    f2 = a.fac2[(size_t)L * a.nr + gi];
+  }
   // the cyclic-reduction coefficients of this L, staged in LDS early so
   // their loads overlap the element dots
   {
@@ -523,12 +533,11 @@ __global__ void __launch_bounds__(256) k_poisson_bridge(const T* __restrict__ rh
   if ((tid & 63) == 0) sRed[tid >> 6] = m;
   __syncthreads();
   const double2 Q = cscale(cadd(cadd(sRed[0], sRed[1]), cadd(sRed[2], sRed[3])), a.fac1[L]);
-  for (int k = tid; k < nb; k += 256) {
-    const int gi = (k + 1) * a.n - 1;
-    double2 sk = cscale(cto<double2>(rp[gi]), invr[gi]);
-    sk = csub(sk, sS[k]);
-    sk = csub(sk, sCr[k]);
-    sY[k] = sk;
+  if (tid < nb) {
+    double2 sk = rb;
+    sk = csub(sk, sS[tid]);
+    sk = csub(sk, sCr[tid]);
+    sY[tid] = sk;
   }
   __syncthreads();
   // parallel cyclic reduction of the Schur system with the per-L
@@ -556,16 +565,12 @@ __global__ void __launch_bounds__(256) k_poisson_bridge(const T* __restrict__ rh
     cur = nxt;
     nxt = t;
   }
-  for (int i = tid; i < nb; i += 256) sCr[i] = cscale(cur[i], a.pcrBinv[(size_t)L * nb + i]);
   if (tid == 0) a.Q[(size_t)p * a.L2 + L] = Q;
-  __syncthreads();
-  const double* rpl1 = a.rpl1 + (size_t)L * a.nr;
-  const double* fac2 = a.fac2 + (size_t)L * a.nr;
-  for (int k = tid; k < nb; k += 256) {
-    const int gi = (k + 1) * a.n - 1;
-    const double2 x = sCr[k];
-    xbp[k] = x;
-    vp[gi] = cto<T>(make_double2((x.x + Q.x * rpl1[gi]) * fac2[gi], (x.y + Q.y * rpl1[gi]) * fac2[gi]));
+  if (tid < nb) {
+    const int gi = (tid + 1) * a.n - 1;
+    const double2 x = cscale(cur[tid], binv);
+    xbp[tid] = x;
+    vp[gi] = cto<T>(make_double2((x.x + Q.x * r1) * f2, (x.y + Q.y * r1) * f2));
   }
 }
 
