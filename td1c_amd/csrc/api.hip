@@ -250,7 +250,16 @@ int build_device(td1c_ctx* c) {
   HIP_TRY(c->m_g2l.upload(padded_cols(S.mat_g2l, S.nlat, S.L1)));
   HIP_TRY(c->m_g2L.upload(padded_cols(S.mat_g2L, S.nlat, S.L2)));
   HIP_TRY(c->m_L2g.upload(padded_cols(S.mat_L2g, S.L2, S.nlat)));
-  HIP_TRY(c->pG.upload(S.pG));
+  {
+    // element inverses G_e stored transposed on the device, [L][e][k][i]:
+    // lanes owning consecutive rows i read consecutive addresses
+    const int ni = S.ni;
+    std::vector<double> gt(S.pG.size());
+    for (size_t blk = 0; blk < S.pG.size() / ((size_t)ni * ni); ++blk)
+      for (int i = 0; i < ni; ++i)
+        for (int k = 0; k < ni; ++k) gt[blk * ni * ni + (size_t)k * ni + i] = S.pG[blk * ni * ni + (size_t)i * ni + k];
+    HIP_TRY(c->pG.upload(gt));
+  }
   HIP_TRY(c->pWl.upload(S.pWl));
   HIP_TRY(c->pWr.upload(S.pWr));
   HIP_TRY(c->pcrA.upload(S.pcrA));

@@ -620,12 +620,12 @@ __global__ void __launch_bounds__(1024) k_poisson_elem(const T* __restrict__ rho
     if (active) {
       double2 out = make_double2(0.0, 0.0);
       if (valid) {
-        const double* gr = sG + lane * ni;
         const double2* b = sB + g * ni;
         double2 y = make_double2(0.0, 0.0);
         for (int k = 0; k < ni; ++k) {
-          y.x = fma(gr[k], b[k].x, y.x);
-          y.y = fma(gr[k], b[k].y, y.y);
+          const double gik = sG[k * ni + lane];  // G is stored transposed
+          y.x = fma(gik, b[k].x, y.x);
+          y.y = fma(gik, b[k].y, y.y);
         }
         const double2* xbp = xb + ((size_t)p * a.L2 + L) * nb;
         if (e >= 1) y = csub(y, cscale(xbp[e - 1], sWl[lane]));
@@ -1104,9 +1104,178 @@ void launch_onebody(const double2* orb, double2* h01, const OneBodyArgs& a, hipS
   hipLaunchKernelGGL(k_onebody, grid, block, 0, s, orb, h01, a);
 }
 
+// Fused Poisson solve, one block per (pair, L), for grids whose radial row
+// fits in LDS: the row b = rho * invr is read once into LDS and serves the
+// bridge dots, the Schur right-hand sides and the interior solves; the
+// bridge values never leave LDS.  Dynamic LDS: b[nr] (complex), then the
+// cyclic-reduction coefficients sPA, sPG [pcr_steps * (nfe - 1)] each.
+// Jobs are L-major and XCD-remapped, so the pairs of one L read that L's
+// element inverses through one L2.
+template <int NI, int BS, class T>
+__global__ void __launch_bounds__(BS) k_poisson_fused(const T* __restrict__ rho, T* __restrict__ v,
+                                                       const PoissonArgs a) {
+  extern __shared__ double2 dyn_lds[];
+  const int tid = threadIdx.x;
+  const int job = xcd_remap(blockIdx.x, gridDim.x);
+  const int L = job / a.npair, p = job % a.npair;
+  const int nb = a.nfe - 1, nr = a.nr, n = a.n;
+  const int ni = NI > 0 ? NI : a.ni;
+  const int K = a.pcr_steps;
+  __shared__ double2 sS[kMaxBridge];
+  __shared__ double2 sY[kMaxBridge];
+  __shared__ double2 sRed[BS / 64];
+  double2* sB = dyn_lds;
+  double* sPA = reinterpret_cast<double*>(dyn_lds + nr);
+  double* sPG = sPA + K * nb;
+  const int pm = a.pair_m[p];
+  const size_t row = ((size_t)p * a.L2 + L) * nr;
+  T* vp = v + row;
+  if (L < (pm < 0 ? -pm : pm)) {  // uniform over the block
+    for (int r = tid; r < nr; r += BS) vp[r] = cz<T>();
+    return;
+  }
+  const T* rp = rho + row;
+  const size_t lo = (size_t)L * nr;
+  // the cyclic-reduction coefficients of this L
+  {
+    const double* pa = a.pcrA + (size_t)L * K * nb;
+    const double* pg = a.pcrG + (size_t)L * K * nb;
+    for (int q = tid; q < K * nb; q += BS) {
+      sPA[q] = pa[q];
+      sPG[q] = pg[q];
+    }
+  }
+  // the row b = rho * invr into LDS and the multipole moment sum rho rpl0,
+  // four rows per thread per trip with the loads issued together
+  double2 m = make_double2(0.0, 0.0);
+  for (int r0 = 0; r0 < nr; r0 += 4 * BS) {
+    double2 x[4];
+    double iv[4], w0[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int r = r0 + u * BS + tid;
+      x[u] = r < nr ? cto<double2>(rp[r]) : make_double2(0.0, 0.0);
+      iv[u] = r < nr ? a.invr[lo + r] : 0.0;
+      w0[u] = r < nr ? a.rpl0[lo + r] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int r = r0 + u * BS + tid;
+      if (r < nr) sB[r] = cscale(x[u], iv[u]);
+      m.x = fma(x[u].x, w0[u], m.x);
+      m.y = fma(x[u].y, w0[u], m.y);
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    m.x += __shfl_xor(m.x, o, 64);
+    m.y += __shfl_xor(m.y, o, 64);
+  }
+  if ((tid & 63) == 0) sRed[tid >> 6] = m;
+  __syncthreads();
+  // Schur right-hand side of bridge k (thread k; nfe - 1 <= BS):
+  // b_B - W_l(k+1) . b_I(k+1) - W_r(k) . b_I(k), the two elements around it
+  if (tid < nb) {
+    const int bl = tid * n, br = (tid + 1) * n;  // first interior rows of elements k and k+1
+    double2 cl = make_double2(0.0, 0.0), cr = make_double2(0.0, 0.0);
+    for (int i = 0; i < ni; ++i) {
+      const double2 b0 = sB[bl + i], b1 = sB[br + i];
+      const double wr = a.Wr[lo + bl + i], wl = a.Wl[lo + br + i];
+      cr.x = fma(wr, b0.x, cr.x);
+      cr.y = fma(wr, b0.y, cr.y);
+      cl.x = fma(wl, b1.x, cl.x);
+      cl.y = fma(wl, b1.y, cl.y);
+    }
+    double2 sk = sB[br - 1];
+    sk = csub(sk, cl);
+    sk = csub(sk, cr);
+    sY[tid] = sk;
+  }
+  __syncthreads();
+  double2 msum = make_double2(0.0, 0.0);
+#pragma unroll
+  for (int w = 0; w < BS / 64; ++w) msum = cadd(msum, sRed[w]);
+  const double2 Q = cscale(msum, a.fac1[L]);
+  // parallel cyclic reduction of the Schur system (per-L coefficients)
+  double2* cur = sY;
+  double2* nxt = sS;
+  for (int k = 0, h = 1; k < K; ++k, h <<= 1) {
+    if (tid < nb) {
+      const int i = tid;
+      double2 d = cur[i];
+      if (i - h >= 0) {
+        const double al = sPA[k * nb + i];
+        d = make_double2(fma(al, cur[i - h].x, d.x), fma(al, cur[i - h].y, d.y));
+      }
+      if (i + h < nb) {
+        const double ga = sPG[k * nb + i];
+        d = make_double2(fma(ga, cur[i + h].x, d.x), fma(ga, cur[i + h].y, d.y));
+      }
+      nxt[i] = d;
+    }
+    __syncthreads();
+    double2* t = cur;
+    cur = nxt;
+    nxt = t;
+  }
+  // bridge values x_B (kept in nxt) and their rows of v
+  if (tid < nb) {
+    const int gi = (tid + 1) * n - 1;
+    const double2 x = cscale(cur[tid], a.pcrBinv[(size_t)L * nb + tid]);
+    nxt[tid] = x;
+    vp[gi] = cto<T>(make_double2((x.x + Q.x * a.rpl1[lo + gi]) * a.fac2[lo + gi],
+                                 (x.y + Q.y * a.rpl1[lo + gi]) * a.fac2[lo + gi]));
+  }
+  __syncthreads();
+  const double2* xB = nxt;
+  // interiors: x_I = G_e b_I - W_l x_left - W_r x_right, thread per row
+  const double* Gl = a.G + (size_t)L * a.nfe * ni * ni;
+  for (int q = tid; q < a.nfe * ni; q += BS) {
+    const int e = q / ni, i = q - e * ni;
+    const int r = e * n + i;
+    const double* ge = Gl + (size_t)e * ni * ni + i;
+    const double2* be = sB + e * n;
+    double2 y = make_double2(0.0, 0.0);
+    if (NI > 0) {
+      double gk[NI > 0 ? NI : 1];
+#pragma unroll
+      for (int k = 0; k < NI; ++k) gk[k] = ge[k * NI];
+#pragma unroll
+      for (int k = 0; k < NI; ++k) {
+        y.x = fma(gk[k], be[k].x, y.x);
+        y.y = fma(gk[k], be[k].y, y.y);
+      }
+    } else {
+      for (int k = 0; k < ni; ++k) {
+        const double g = ge[k * ni];
+        y.x = fma(g, be[k].x, y.x);
+        y.y = fma(g, be[k].y, y.y);
+      }
+    }
+    if (e >= 1) y = csub(y, cscale(xB[e - 1], a.Wl[lo + r]));
+    if (e <= a.nfe - 2) y = csub(y, cscale(xB[e], a.Wr[lo + r]));
+    vp[r] = cto<T>(make_double2((y.x + Q.x * a.rpl1[lo + r]) * a.fac2[lo + r],
+                                (y.y + Q.y * a.rpl1[lo + r]) * a.fac2[lo + r]));
+  }
+}
+
+constexpr int kPoissonBlock = 512;  // threads of the fused Poisson kernel (>= nfe)
+
+// LDS bytes of the fused Poisson kernel; 0 when the grid does not fit
+size_t poisson_fused_lds(const PoissonArgs& a) {
+  const size_t bytes = (size_t)a.nr * sizeof(double2) + 2 * (size_t)a.pcr_steps * (a.nfe - 1) * sizeof(double);
+  return bytes <= 40 * 1024 ? bytes : 0;  // four blocks per CU beside the static arrays
+}
+
 template <class T>
 void launch_poisson(const T* rho, double2* xb, T* v, const PoissonArgs& a, hipStream_t s) {
   const int jobs = a.npair * a.L2;
+  if (const size_t lds = poisson_fused_lds(a)) {
+    if (a.ni == 19)
+      hipLaunchKernelGGL((k_poisson_fused<19, kPoissonBlock, T>), dim3(jobs), dim3(kPoissonBlock), lds, s, rho, v, a);
+    else
+      hipLaunchKernelGGL((k_poisson_fused<0, kPoissonBlock, T>), dim3(jobs), dim3(kPoissonBlock), lds, s, rho, v, a);
+    return;
+  }
   if (a.ni == 19)
     hipLaunchKernelGGL((k_poisson_bridge<19, T>), dim3(jobs), dim3(256), 0, s, rho, v, xb, a);
   else

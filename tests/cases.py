@@ -45,6 +45,13 @@ def he_n16(**kw):
                    nel_alpha=1, nel_beta=1, **kw)
 
 
+def he_long(**kw):
+    # 140 elements (2799 radial rows): a radial row beyond the fused Poisson
+    # kernel's LDS, so the bridge + element kernel pair runs
+    return Problem(nfe=140, ndvr=20, rmax=280.0, lmax1=2, lmax2=4, znuc=2.0, ndcore=1,
+                   nel_alpha=1, nel_beta=1, **kw)
+
+
 def dc4_small(**kw):
     # four doubly occupied orbitals: the generic theta-grid mean-field kernel
     return Problem(nfe=6, ndvr=20, rmax=18.0, lmax1=5, lmax2=10, znuc=8.0, ndcore=4,

@@ -43,12 +43,13 @@ def _run(prob, lfield, seed=11, dt=1.0):
                                      ("dc3_small", C.LF_LEN), ("ne_small", C.LF_VEL),
                                      ("ne_small_len", C.LF_LEN), ("cas_small", C.LF_LEN),
                                      ("cas_fc_small", C.LF_VEL), ("he_n16", C.LF_VEL),
-                                     ("dc4_small", C.LF_LEN), ("fc2_small", C.LF_VEL)])
+                                     ("dc4_small", C.LF_LEN), ("fc2_small", C.LF_VEL),
+                                     ("he_long", C.LF_LEN)])
 def test_htot_matches_oracle(gpu_lib, name, lf):
     mk = {"he_small": C.he_small, "he_small_vel": lambda: C.he_small(igauge=1), "dc3_small": C.dc3_small,
           "ne_small": C.ne_small, "ne_small_len": lambda: C.ne_small(igauge=0), "cas_small": C.cas_small,
           "cas_fc_small": C.cas_fc_small, "he_n16": lambda: C.he_n16(igauge=1), "dc4_small": C.dc4_small,
-          "fc2_small": C.fc2_small}[name]
+          "fc2_small": C.fc2_small, "he_long": C.he_long}[name]
     prob = mk()
     print(f"case {name}", flush=True)
     o, ref, got, eref, e = _run(prob, lf)
