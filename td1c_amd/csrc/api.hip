@@ -480,7 +480,9 @@ int build_device(td1c_ctx* c) {
   HIP_TRY(c->lf_dev.alloc(9 * kMaxStages));
   HIP_TRY(c->clock.alloc(1));
   for (auto& q : c->side) HIP_TRY(hipStreamCreateWithFlags(&q, hipStreamNonBlocking));
-  for (auto& e : c->fj) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  // fork/join events order kernels of one device only: device-scope release
+  // and acquire suffice, no system-scope fence (cache write-back) per record
+  for (auto& e : c->fj) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming | hipEventDisableSystemFence));
   return 0;
 }
 
