@@ -181,6 +181,15 @@ int td1c_propagate(td1c_ctx* ctx, const td1c_field* field, int rk_order,
  * replays it (fewer host calls, the branches serialised). */
 int td1c_set_graph(td1c_ctx* ctx, int on);
 
+/* Which form of the theta <-> l transforms the context's kernels use:
+ * 0 (default) chooses per launch by grid size, 1 always the vector form,
+ * 2 always the matrix-core (MFMA) form.  The results agree to rounding.
+ * No counterpart in the reference (its transforms are the matrix products
+ * of src/bas/bas_sph2ang1.F90, bas_sph2ang2.F90, bas_ang2sph1.F90 and
+ * bas_ang2sph2.F90); provided so that either form can be run and checked
+ * on any problem size. */
+int td1c_set_transform_path(td1c_ctx* ctx, int path);
+
 /* Applies the radial mask to the dynamical orbitals of a device wave
  * function (wfn_mask, src/wfn/wfn_mask.F90:2-35). */
 int td1c_mask(td1c_ctx* ctx, double* d_wfn, void* stream);

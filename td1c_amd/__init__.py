@@ -63,6 +63,8 @@ def load_library():
     L.td1c_propagate.restype = ctypes.c_int
     L.td1c_set_graph.argtypes = [vp, ctypes.c_int]
     L.td1c_set_graph.restype = ctypes.c_int
+    L.td1c_set_transform_path.argtypes = [vp, ctypes.c_int]
+    L.td1c_set_transform_path.restype = ctypes.c_int
     L.td1c_dipole.argtypes = [vp, vp, vp, vp, vp, vp]
     L.td1c_dipole.restype = ctypes.c_int
     L.td1c_mask.argtypes = [vp, vp, vp]
@@ -228,6 +230,11 @@ class HProd:
         """td1c_propagate replays one captured step (True) or enqueues every
         kernel (False, the default)."""
         _check(load_library().td1c_set_graph(self.h, int(on)), "td1c_set_graph")
+
+    def set_transform_path(self, path: int):
+        """Form of the theta <-> l transforms: 0 by grid size (default), 1 the
+        vector form, 2 the matrix-core (MFMA) form."""
+        _check(load_library().td1c_set_transform_path(self.h, int(path)), "td1c_set_transform_path")
 
     def bind(self):
         """Makes this context the one the legacy hprod_htot_ entry point uses."""

@@ -104,6 +104,7 @@ struct td1c_ctx {
   // hprod, joined back by events; the events carry no timing
   hipStream_t side[2] = {nullptr, nullptr};
   hipEvent_t fj[6] = {};
+  int tpath = 0;  // transform form: 0 by grid size, 1 vector, 2 matrix cores
   // device-driven propagation: one captured RK step, replayed per step
   hipStream_t cap_stream = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -533,9 +534,9 @@ void enqueue_orbin(td1c_ctx* c, const double* d_lf, const double2* wfn, hipStrea
     TIMED("l2theta");
     if (c->mixed)
       launch_transform_t<double2, float2>(0, wfn, c->orbg_f, c->m_l2g.p, nullptr, c->b_l2g_dyn.p, c->n_l2g_dyn, nr, L1,
-                                          nl, pad_cols(nl), c->orb, s);
+                                          nl, pad_cols(nl), c->orb, s, c->tpath);
     else
-      launch_transform(0, wfn, c->orbg, c->m_l2g.p, c->b_l2g_dyn.p, c->n_l2g_dyn, nr, L1, nl, pad_cols(nl), c->orb, s);
+      launch_transform(0, wfn, c->orbg, c->m_l2g.p, c->b_l2g_dyn.p, c->n_l2g_dyn, nr, L1, nl, pad_cols(nl), c->orb, s, c->tpath);
   }
   // frozen core in the velocity gauge (hprod_fcorbg): phase on the grid, back to l-space
   if (nfc > 0 && S.igauge == 1) {
@@ -544,11 +545,11 @@ void enqueue_orbin(td1c_ctx* c, const double* d_lf, const double2* wfn, hipStrea
       {
         TIMED("theta2l_fc");
         launch_transform_t<float2, double2>(0, c->orbg_f, c->orb, c->m_g2l.p, nullptr, c->b_g2l_fc.p, c->n_g2l_fc, nr,
-                                            nl, L1, pad_cols(L1), nullptr, s);
+                                            nl, L1, pad_cols(L1), nullptr, s, c->tpath);
       }
     } else {
       { TIMED("fc_phase"); launch_fc_phase(c->orbg0.p, c->orbg, c->xr.p, c->cost.p, d_lf, nr, S.nradfc, nl, nfc, s); }
-      { TIMED("theta2l_fc"); launch_transform(0, c->orbg, c->orb, c->m_g2l.p, c->b_g2l_fc.p, c->n_g2l_fc, nr, nl, L1, pad_cols(L1), nullptr, s); }
+      { TIMED("theta2l_fc"); launch_transform(0, c->orbg, c->orb, c->m_g2l.p, c->b_g2l_fc.p, c->n_g2l_fc, nr, nl, L1, pad_cols(L1), nullptr, s, c->tpath); }
     }
   }
 }
@@ -610,10 +611,10 @@ int enqueue_htotx(td1c_ctx* c, double dtime, const double* d_lf, const double2* 
     if (c->mixed) {
       TIMED("pair_theta2L");
       launch_transform_t<float2, float2>(1, c->orbg_f, c->rhoL_f.p, c->m_g2L.p, c->m_g2L_f.p, c->b_rho.p, c->npair, nr,
-                                         nl, L2, pad_cols(L2), nullptr, s);
+                                         nl, L2, pad_cols(L2), nullptr, s, c->tpath);
     } else {
       TIMED("pair_theta2L");
-      launch_transform(1, c->orbg, c->rhoL.p, c->m_g2L.p, c->b_rho.p, c->npair, nr, nl, L2, pad_cols(L2), nullptr, s);
+      launch_transform(1, c->orbg, c->rhoL.p, c->m_g2L.p, c->b_rho.p, c->npair, nr, nl, L2, pad_cols(L2), nullptr, s, c->tpath);
     }
     PoissonArgs pa{};
     pa.nr = nr; pa.n = S.n; pa.nfe = S.nfe; pa.ni = S.ni; pa.L2 = L2; pa.npair = c->npair;
@@ -630,13 +631,13 @@ int enqueue_htotx(td1c_ctx* c, double dtime, const double* d_lf, const double2* 
       {
         TIMED("pair_L2theta");
         launch_transform_t<float2, float2>(0, c->vL_f.p, c->vg_f.p, c->m_L2g.p, c->m_L2g_f.p, c->b_L2g.p, c->npair, nr,
-                                           L2, nl, pad_cols(nl), nullptr, s);
+                                           L2, nl, pad_cols(nl), nullptr, s, c->tpath);
       }
     } else {
       { TIMED("poisson"); launch_poisson(c->rhoL.p, c->p_xb.p, c->vL.p, pa, s); }
       HIP_TRY(enqueue_pair_gram(c, s, s1));
       pair_gram_done = true;
-      { TIMED("pair_L2theta"); launch_transform(0, c->vL.p, c->vg.p, c->m_L2g.p, c->b_L2g.p, c->npair, nr, L2, nl, pad_cols(nl), nullptr, s); }
+      { TIMED("pair_L2theta"); launch_transform(0, c->vL.p, c->vg.p, c->m_L2g.p, c->b_L2g.p, c->npair, nr, L2, nl, pad_cols(nl), nullptr, s, c->tpath); }
     }
     MfArgs ma{};
     ma.nr = nr; ma.nlat = nl; ma.nfun = nf; ma.nfcore = nfc; ma.nfcore2 = S.nfcore2; ma.ncore = S.ncore; ma.nact = S.nact;
@@ -649,11 +650,11 @@ int enqueue_htotx(td1c_ctx* c, double dtime, const double* d_lf, const double2* 
       {
         TIMED("theta2l_mf");
         launch_transform_t<float2, double2>(0, c->gorbg_f, c->gorb, c->m_g2l.p, nullptr, c->b_g2l_mf.p, c->n_g2l_mf, nr,
-                                            nl, L1, pad_cols(L1), nullptr, s);
+                                            nl, L1, pad_cols(L1), nullptr, s, c->tpath);
       }
     } else {
       { TIMED("mfprod"); launch_mfprod(c->orbg, c->vg.p, c->gorbg, c->v2orbg, ma, s); }
-      { TIMED("theta2l_mf"); launch_transform(0, c->gorbg, c->gorb, c->m_g2l.p, c->b_g2l_mf.p, c->n_g2l_mf, nr, nl, L1, pad_cols(L1), nullptr, s); }
+      { TIMED("theta2l_mf"); launch_transform(0, c->gorbg, c->gorb, c->m_g2l.p, c->b_g2l_mf.p, c->n_g2l_mf, nr, nl, L1, pad_cols(L1), nullptr, s, c->tpath); }
     }
   } else {
     launch_zero(c->gorb, 2 * nb * c->nd, s);
@@ -1110,6 +1111,13 @@ int td1c_get_timing(td1c_ctx* c, char* buf, int buflen) {
 int td1c_set_graph(td1c_ctx* c, int on) {
   if (!c) return fail(-1, "null argument");
   c->use_graph = on != 0;
+  return 0;
+}
+
+int td1c_set_transform_path(td1c_ctx* c, int path) {
+  if (!c) return fail(-1, "null argument");
+  if (path < 0 || path > 2) return fail(-1, "transform path must be 0, 1 or 2");
+  c->tpath = path;
   return 0;
 }
 

@@ -169,6 +169,103 @@ __global__ void __launch_bounds__(256) k_transform(const TI* __restrict__ in, TO
 }
 
 // ---------------------------------------------------------------------------
+// The same transform on the matrix cores.  Per batch entry it is the real
+// GEMM Out^T = M^T X, with X[k][r] the complex input: its real and imaginary
+// parts are two MFMA B operands sharing the A operand (the real matrix).
+// Block: one tile of 16 radial rows and up to 4 column tiles of 16, one
+// wavefront per column tile, each running the whole contraction in steps of
+// 4 (8 steps' loads issued together).  Lane layouts of the 16x16x4 MFMAs on
+// gfx950: A[i][k] at lane i + 16k, B[k][j] at lane j + 16k; register v of
+// the result in lane l holds column l % 16 and row 4 (l / 16) + v (f32) or
+// l / 16 + 4 v (f64).
+// ---------------------------------------------------------------------------
+template <class AR> struct Mfma16;
+template <> struct Mfma16<float> {
+  typedef float V4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ V4 run(float a, float b, V4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
+};
+template <> struct Mfma16<double> {
+  typedef double V4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ V4 run(double a, double b, V4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int v) { return (lane >> 4) + 4 * v; }
+};
+
+template <int MODE, class TI, class TO>
+__global__ void __launch_bounds__(256) k_transform_mfma(const TI* __restrict__ in, TO* __restrict__ out,
+                                                        const double* __restrict__ mats,
+                                                        const float* __restrict__ matsf,
+                                                        const TransformBatch* __restrict__ batch, int nr, int K, int C,
+                                                        int Cp, double2* __restrict__ copy) {
+  using A2 = typename std::conditional<std::is_same<TI, float2>::value && std::is_same<TO, float2>::value, float2,
+                                       double2>::type;
+  using AR = typename CReal<A2>::T;
+  using MF = Mfma16<AR>;
+  using V4 = typename MF::V4;
+  constexpr int U = 4;  // contraction steps per load batch
+  const int ny = gridDim.y, nx = gridDim.x;
+  const int t = xcd_remap(blockIdx.x + nx * (blockIdx.y + ny * blockIdx.z), nx * ny * gridDim.z);
+  const int by = t % ny, bx = (t / ny) % nx, bz = t / (ny * nx);
+  const TransformBatch bt = batch[bz];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int c0 = (by * (int)(blockDim.x >> 6) + wave) * 16;  // first column of this wavefront's tile
+  if (c0 >= C) return;  // wave-uniform; the kernel has no barrier
+  const int r = bx * 16 + (lane & 15);
+  const int kq = lane >> 4;
+  const bool row_ok = r < bt.rows;
+  const int ca = c0 + (lane & 15);  // matrix column this lane feeds to A
+  const bool ca_ok = ca < Cp;       // columns C..Cp-1 of the padded matrix are zero
+  const AR* __restrict__ M;
+  if constexpr (std::is_same<AR, float>::value) M = matsf + (size_t)bt.mat * K * Cp;
+  else M = mats + (size_t)bt.mat * K * Cp;
+  const TI* __restrict__ x0 = in + (size_t)bt.in0 + r;
+  const TI* __restrict__ x1 = in + (size_t)bt.in1 + r;
+  V4 dre = {0, 0, 0, 0}, dim = {0, 0, 0, 0};
+  const int nks = (K + 3) >> 2;
+  for (int s0 = 0; s0 < nks; s0 += U) {
+    AR a[U];
+    A2 x[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = 4 * (s0 + u) + kq;
+      const bool kok = k < K;
+      a[u] = (kok && ca_ok) ? M[(size_t)k * Cp + ca] : AR(0);
+      x[u] = cz<A2>();
+      if (kok && row_ok) {
+        if (MODE == 1) {
+          x[u] = cmulc(cto<A2>(x0[(size_t)k * nr]), cto<A2>(x1[(size_t)k * nr]));
+        } else {
+          const TI xv = x0[(size_t)k * nr];
+          x[u] = cto<A2>(xv);
+          if constexpr (std::is_same<TI, double2>::value)
+            if (copy != nullptr && c0 == 0) copy[(size_t)bt.in0 + (size_t)k * nr + r] = xv;
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      dre = MF::run(a[u], x[u].x, dre);
+      dim = MF::run(a[u], x[u].y, dim);
+    }
+  }
+  if (row_ok) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int c = c0 + MF::row(lane, v);
+      A2 o;
+      o.x = dre[v];
+      o.y = dim[v];
+      if (c < C) out[(size_t)bt.out + (size_t)c * nr + r] = cto<TO>(o);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // One-body part h01 = (T_l + h1) psi (hprod_tprodp, src/hprod/hprod_tprod.F90:152-235;
 // hprod_zprodp, hprod_zprod.F90:108-135; hprod_pzprodp, hprod_pzprod.F90:114-213).
 // Thread per (r, l, orbital).  The FEDVR band is walked with the transposed
@@ -1060,34 +1157,53 @@ static void transform_variant(int ks, dim3 grid, hipStream_t s, const TI* in, TO
                        C, Cp, copy);
 }
 
+constexpr long long kMfmaMinWaves = 1200;  // wavefronts of the MFMA form below which the vector form runs
+
 template <class TI, class TO>
 void launch_transform_t(int mode, const TI* in, TO* out, const double* mats, const float* matsf,
                         const TransformBatch* batch, int nbatch, int nr, int K, int C, int Cp, double2* copy,
-                        hipStream_t s) {
-  const int rt = (nr + 63) / 64;
-  // 8 columns per wavefront; split the contraction over 2 or 4 wavefronts
-  // only while the grid is too small to fill the chip (a split re-reads
-  // the input once per extra column block)
-  int ks = 1;
-  while (ks < 4 && rt * ((C + 8 * (4 / ks) - 1) / (8 * (4 / ks))) * nbatch < 800) ks *= 2;
-  const int cols = 8 * (4 / ks);
-  dim3 grid(rt, (C + cols - 1) / cols, nbatch);
-  if (mode == 0) transform_variant<0>(ks, grid, s, in, out, mats, matsf, batch, nr, K, C, Cp, copy);
-  else transform_variant<1>(ks, grid, s, in, out, mats, matsf, batch, nr, K, C, Cp, nullptr);
+                        hipStream_t s, int path) {
+  // matrix cores (16 radial rows per block, one wavefront per 16 columns)
+  // when that gives enough wavefronts to fill the chip and at least three
+  // column tiles share each input load; otherwise the vector form, whose
+  // contraction split keeps short grids busy
+  const int ct = (C + 15) / 16;
+  const int wpb = std::min(4, ct);
+  const bool mfma = ct >= 3 && (long long)((nr + 15) / 16) * nbatch * ct >= kMfmaMinWaves;
+  if (path == 1 || (path == 0 && !mfma)) {
+    const int rt = (nr + 63) / 64;
+    // 8 columns per wavefront; split the contraction over 2 or 4 wavefronts
+    // only while the grid is too small to fill the chip (a split re-reads
+    // the input once per extra column block)
+    int ks = 1;
+    while (ks < 4 && rt * ((C + 8 * (4 / ks) - 1) / (8 * (4 / ks))) * nbatch < 800) ks *= 2;
+    const int cols = 8 * (4 / ks);
+    dim3 grid(rt, (C + cols - 1) / cols, nbatch);
+    if (mode == 0) transform_variant<0>(ks, grid, s, in, out, mats, matsf, batch, nr, K, C, Cp, copy);
+    else transform_variant<1>(ks, grid, s, in, out, mats, matsf, batch, nr, K, C, Cp, nullptr);
+    return;
+  }
+  dim3 grid((nr + 15) / 16, (ct + wpb - 1) / wpb, nbatch);
+  if (mode == 0)
+    hipLaunchKernelGGL((k_transform_mfma<0, TI, TO>), grid, dim3(64 * wpb), 0, s, in, out, mats, matsf, batch, nr, K,
+                       C, Cp, copy);
+  else
+    hipLaunchKernelGGL((k_transform_mfma<1, TI, TO>), grid, dim3(64 * wpb), 0, s, in, out, mats, matsf, batch, nr, K,
+                       C, Cp, nullptr);
 }
 template void launch_transform_t<double2, float2>(int, const double2*, float2*, const double*, const float*,
                                                   const TransformBatch*, int, int, int, int, int, double2*,
-                                                  hipStream_t);
+                                                  hipStream_t, int);
 template void launch_transform_t<float2, float2>(int, const float2*, float2*, const double*, const float*,
                                                  const TransformBatch*, int, int, int, int, int, double2*,
-                                                 hipStream_t);
+                                                 hipStream_t, int);
 template void launch_transform_t<float2, double2>(int, const float2*, double2*, const double*, const float*,
                                                   const TransformBatch*, int, int, int, int, int, double2*,
-                                                  hipStream_t);
+                                                  hipStream_t, int);
 
 void launch_transform(int mode, const double2* in, double2* out, const double* mats, const TransformBatch* batch,
-                      int nbatch, int nr, int K, int C, int Cp, double2* copy, hipStream_t s) {
-  launch_transform_t<double2, double2>(mode, in, out, mats, nullptr, batch, nbatch, nr, K, C, Cp, copy, s);
+                      int nbatch, int nr, int K, int C, int Cp, double2* copy, hipStream_t s, int path) {
+  launch_transform_t<double2, double2>(mode, in, out, mats, nullptr, batch, nbatch, nr, K, C, Cp, copy, s, path);
 }
 
 bool launch_onebody_el(const double2* orb, double2* h01, const OneBodyElArgs& a, hipStream_t s) {

@@ -227,6 +227,32 @@ def test_mixed_htot_close_to_oracle(gpu_lib, name, lf):
     assert 1e-14 < err < MIXED_RTOL, err  # complex64 really used, and close
 
 
+# Each form of the theta <-> l transforms (vector, matrix cores) forced on
+# every transform of an hprod, both precisions.
+@pytest.mark.parametrize("path", [1, 2])
+@pytest.mark.parametrize("name,lf", [("he_small", C.LF_LEN), ("dc3_small", C.LF_LEN), ("ne_small", C.LF_VEL),
+                                     ("cas_fc_small", C.LF_VEL), ("dc4_small", C.LF_LEN)])
+def test_transform_paths_match_oracle(gpu_lib, name, lf, path):
+    import torch
+    import td1c_amd
+    from oracle.oracle import Oracle
+
+    print(f"case {name} path {path}", flush=True)
+    for precision, tol in ((0, RTOL), (1, MIXED_RTOL)):
+        prob = getattr(C, name)(precision=precision)
+        o = Oracle(prob)
+        w = C.wfn_for(prob, o.lcic)
+        o.set_wfn0(w)
+        ref = o.htot(1.0, lf, w)
+        hp = td1c_amd.HProd(prob)
+        hp.set_transform_path(path)
+        hp.set_wfn0(w)
+        out = hp.htot(1.0, lf, hp.to_device(w))
+        torch.cuda.synchronize()
+        err = _rel(out.cpu().numpy(), ref)
+        assert err < tol, (precision, err)
+
+
 def test_mixed_propagation_dipole_error(gpu_lib):
     """Dipole of a mixed-precision propagation (ne_small, 60 RK4 steps in a
     sin^2 field) within 1e-6 of the FP64 propagation at every 10th step."""
