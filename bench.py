@@ -45,13 +45,12 @@ NSTEP_PROP = 2000
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "ne_l31_traffic.json")
 # device kernels behind each timing name (HBM counters are per device kernel)
 TIMING_KERNELS = {
-    "poisson": ["k_poisson_bridge", "k_poisson_elem"],
+    "poisson": ["k_poisson_fused"],
     "onebody": ["k_onebody_el"],
     "mfprod": ["k_mfprod_t"],
     "island_a": ["k_island_a"],
     "island_b": ["k_island_b"],
     "dtorb_rk": ["k_dtorb"],
-    "fc_phase": ["k_fc_phase"],
 }
 
 
@@ -113,8 +112,8 @@ def kernel_bytes(name: str, s: dict, cm: int = 16) -> float:
     nf, nd, npair, nfc = s["nfun"], s["ndyn"], s["npair"], s["nfcore"]
     nb, ng, c = nr * L1, nr * nl, 16
     table = {
-        "l2theta": nd * (nb * c + ng * cm),
-        "fc_phase": nfc * ng * (c + cm),
+        # with the velocity-gauge frozen-core phase carried by the same launch
+        "l2theta": nd * (nb * c + ng * cm) + nfc * ng * (c + cm),
         "theta2l_fc": nfc * (ng * cm + nb * c),
         "onebody": 2 * nf * nb * c,
         "pair_theta2L": (nf * ng + npair * L2 * nr) * cm,

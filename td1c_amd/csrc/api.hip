@@ -528,29 +528,34 @@ struct CiStage {
 void enqueue_orbin(td1c_ctx* c, const double* d_lf, const double2* wfn, hipStream_t s) {
   Setup& S = c->S;
   const int nr = S.nr, nfc = S.nfcore, L1 = S.L1, nl = S.nlat;
+  // frozen core in the velocity gauge (hprod_fcorbg): the phase on the theta
+  // grid rides on the l -> theta launch of the dynamical orbitals
+  const bool fc_vel = nfc > 0 && S.igauge == 1;
+  FcPhaseJob fj{};
+  fj.orbg0 = c->orbg0.p;
+  fj.orbg = c->mixed ? static_cast<void*>(c->orbg_f) : static_cast<void*>(c->orbg);
+  fj.xr = c->xr.p; fj.cost = c->cost.p; fj.lf = d_lf;
+  fj.nr = nr; fj.nradfc = S.nradfc; fj.nlat = nl; fj.nfc = nfc;
   // dynamical orbitals: copied into the work array by the l -> theta
   // transform that reads them
   {
     TIMED("l2theta");
     if (c->mixed)
       launch_transform_t<double2, float2>(0, wfn, c->orbg_f, c->m_l2g.p, nullptr, c->b_l2g_dyn.p, c->n_l2g_dyn, nr, L1,
-                                          nl, pad_cols(nl), c->orb, s, c->tpath);
+                                          nl, pad_cols(nl), c->orb, s, c->tpath, fc_vel ? &fj : nullptr);
     else
-      launch_transform(0, wfn, c->orbg, c->m_l2g.p, c->b_l2g_dyn.p, c->n_l2g_dyn, nr, L1, nl, pad_cols(nl), c->orb, s, c->tpath);
+      launch_transform_t<double2, double2>(0, wfn, c->orbg, c->m_l2g.p, nullptr, c->b_l2g_dyn.p, c->n_l2g_dyn, nr, L1,
+                                           nl, pad_cols(nl), c->orb, s, c->tpath, fc_vel ? &fj : nullptr);
   }
-  // frozen core in the velocity gauge (hprod_fcorbg): phase on the grid, back to l-space
-  if (nfc > 0 && S.igauge == 1) {
-    if (c->mixed) {
-      { TIMED("fc_phase"); launch_fc_phase(c->orbg0.p, c->orbg_f, c->xr.p, c->cost.p, d_lf, nr, S.nradfc, nl, nfc, s); }
-      {
-        TIMED("theta2l_fc");
-        launch_transform_t<float2, double2>(0, c->orbg_f, c->orb, c->m_g2l.p, nullptr, c->b_g2l_fc.p, c->n_g2l_fc, nr,
-                                            nl, L1, pad_cols(L1), nullptr, s, c->tpath);
-      }
-    } else {
-      { TIMED("fc_phase"); launch_fc_phase(c->orbg0.p, c->orbg, c->xr.p, c->cost.p, d_lf, nr, S.nradfc, nl, nfc, s); }
-      { TIMED("theta2l_fc"); launch_transform(0, c->orbg, c->orb, c->m_g2l.p, c->b_g2l_fc.p, c->n_g2l_fc, nr, nl, L1, pad_cols(L1), nullptr, s, c->tpath); }
-    }
+  // the phased frozen core back to l-space
+  if (fc_vel) {
+    TIMED("theta2l_fc");
+    if (c->mixed)
+      launch_transform_t<float2, double2>(0, c->orbg_f, c->orb, c->m_g2l.p, nullptr, c->b_g2l_fc.p, c->n_g2l_fc, nr, nl,
+                                          L1, pad_cols(L1), nullptr, s, c->tpath);
+    else
+      launch_transform(0, c->orbg, c->orb, c->m_g2l.p, c->b_g2l_fc.p, c->n_g2l_fc, nr, nl, L1, pad_cols(L1), nullptr, s,
+                       c->tpath);
   }
 }
 

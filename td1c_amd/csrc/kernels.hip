@@ -169,6 +169,28 @@ __global__ void __launch_bounds__(256) k_transform(const TI* __restrict__ in, TO
 }
 
 // ---------------------------------------------------------------------------
+// Velocity-gauge frozen core on the theta grid (hprod_fcorbg,
+// src/hprod/hprod_orbin.F90:68-105): orbg_fc = orbg0_fc * exp(-i A r cos(theta)),
+// one thread per (r, theta) point.
+// ---------------------------------------------------------------------------
+template <class T>
+__device__ __forceinline__ void fc_phase_point(const FcPhaseJob& j, int r, int th) {
+  if (r >= j.nr || th >= j.nlat) return;
+  double s, c;
+  sincos(-j.lf[8] * j.xr[r] * j.cost[th], &s, &c);
+  T* orbg = static_cast<T*>(j.orbg);
+  for (int f = 0; f < j.nfc; ++f) {
+    const size_t q = ((size_t)f * j.nlat + th) * j.nr + r;
+    orbg[q] = cto<T>(r < j.nradfc ? cmul(j.orbg0[q], make_double2(c, s)) : make_double2(0.0, 0.0));
+  }
+}
+
+template <class T>
+__global__ void k_fc_phase(const FcPhaseJob j) {
+  fc_phase_point<T>(j, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y);
+}
+
+// ---------------------------------------------------------------------------
 // The same transform on the matrix cores.  Per batch entry it is the real
 // GEMM Out^T = M^T X, with X[k][r] the complex input: its real and imaginary
 // parts are two MFMA B operands sharing the A operand (the real matrix).
@@ -200,15 +222,21 @@ __global__ void __launch_bounds__(256) k_transform_mfma(const TI* __restrict__ i
                                                         const double* __restrict__ mats,
                                                         const float* __restrict__ matsf,
                                                         const TransformBatch* __restrict__ batch, int nr, int K, int C,
-                                                        int Cp, double2* __restrict__ copy) {
+                                                        int Cp, double2* __restrict__ copy, int nx, int ny, int ntile,
+                                                        const FcPhaseJob fc) {
   using A2 = typename std::conditional<std::is_same<TI, float2>::value && std::is_same<TO, float2>::value, float2,
                                        double2>::type;
   using AR = typename CReal<A2>::T;
   using MF = Mfma16<AR>;
   using V4 = typename MF::V4;
   constexpr int U = 4;  // contraction steps per load batch
-  const int ny = gridDim.y, nx = gridDim.x;
-  const int t = xcd_remap(blockIdx.x + nx * (blockIdx.y + ny * blockIdx.z), nx * ny * gridDim.z);
+  // blocks past the ntile transform tiles run the frozen-core phase job
+  if ((int)blockIdx.x >= ntile) {
+    const int q = (int)blockIdx.x - ntile, rb = (fc.nr + blockDim.x - 1) / blockDim.x;
+    fc_phase_point<TO>(fc, (q % rb) * blockDim.x + threadIdx.x, q / rb);
+    return;
+  }
+  const int t = xcd_remap(blockIdx.x, ntile);
   const int by = t % ny, bx = (t / ny) % nx, bz = t / (ny * nx);
   const TransformBatch bt = batch[bz];
   const int lane = threadIdx.x & 63;
@@ -927,25 +955,6 @@ __global__ void __launch_bounds__(256) k_mfprod_t(const A2* __restrict__ orbg, c
 }
 
 // ---------------------------------------------------------------------------
-// Velocity-gauge frozen core on the theta grid (hprod_fcorbg,
-// src/hprod/hprod_orbin.F90:68-105): orbg_fc = orbg0_fc * exp(-i A r cos(theta)).
-// ---------------------------------------------------------------------------
-template <class T>
-__global__ void k_fc_phase(const double2* __restrict__ orbg0, T* __restrict__ orbg, const double* __restrict__ xr,
-                           const double* __restrict__ cost, const double* __restrict__ lf, int nr, int nradfc, int nlat, int nfc) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  const int th = blockIdx.y;
-  if (r >= nr || th >= nlat) return;
-  double ph = -lf[8] * xr[r] * cost[th];
-  double s, c;
-  sincos(ph, &s, &c);
-  for (int f = 0; f < nfc; ++f) {
-    size_t q = ((size_t)f * nlat + th) * nr + r;
-    orbg[q] = cto<T>(r < nradfc ? cmul(orbg0[q], make_double2(c, s)) : make_double2(0.0, 0.0));
-  }
-}
-
-// ---------------------------------------------------------------------------
 // Gram products over a common index range, FP64, deterministic two-stage sum
 // (the inner products of hprod_mkint1_sph, hprod_mkint2_sph, hprod_ene_dcore
 // and hprod_htot_mkxmat_*):
@@ -1157,12 +1166,17 @@ static void transform_variant(int ks, dim3 grid, hipStream_t s, const TI* in, TO
                        C, Cp, copy);
 }
 
+template <class T>
+static void launch_fc_phase(const FcPhaseJob& j, hipStream_t s) {
+  hipLaunchKernelGGL((k_fc_phase<T>), dim3((j.nr + 255) / 256, j.nlat), dim3(256), 0, s, j);
+}
+
 constexpr long long kMfmaMinWaves = 1200;  // wavefronts of the MFMA form below which the vector form runs
 
 template <class TI, class TO>
 void launch_transform_t(int mode, const TI* in, TO* out, const double* mats, const float* matsf,
                         const TransformBatch* batch, int nbatch, int nr, int K, int C, int Cp, double2* copy,
-                        hipStream_t s, int path) {
+                        hipStream_t s, int path, const FcPhaseJob* fc) {
   // matrix cores (16 radial rows per block, one wavefront per 16 columns)
   // when that gives enough wavefronts to fill the chip and at least three
   // column tiles share each input load; otherwise the vector form, whose
@@ -1181,29 +1195,33 @@ void launch_transform_t(int mode, const TI* in, TO* out, const double* mats, con
     dim3 grid(rt, (C + cols - 1) / cols, nbatch);
     if (mode == 0) transform_variant<0>(ks, grid, s, in, out, mats, matsf, batch, nr, K, C, Cp, copy);
     else transform_variant<1>(ks, grid, s, in, out, mats, matsf, batch, nr, K, C, Cp, nullptr);
+    if (fc != nullptr) launch_fc_phase<TO>(*fc, s);
     return;
   }
-  dim3 grid((nr + 15) / 16, (ct + wpb - 1) / wpb, nbatch);
+  const int nx = (nr + 15) / 16, ny = (ct + wpb - 1) / wpb, ntile = nx * ny * nbatch;
+  const int nfcb = fc != nullptr ? ((fc->nr + 64 * wpb - 1) / (64 * wpb)) * fc->nlat : 0;
+  const FcPhaseJob fj = fc != nullptr ? *fc : FcPhaseJob{};
   if (mode == 0)
-    hipLaunchKernelGGL((k_transform_mfma<0, TI, TO>), grid, dim3(64 * wpb), 0, s, in, out, mats, matsf, batch, nr, K,
-                       C, Cp, copy);
+    hipLaunchKernelGGL((k_transform_mfma<0, TI, TO>), dim3(ntile + nfcb), dim3(64 * wpb), 0, s, in, out, mats, matsf,
+                       batch, nr, K, C, Cp, copy, nx, ny, ntile, fj);
   else
-    hipLaunchKernelGGL((k_transform_mfma<1, TI, TO>), grid, dim3(64 * wpb), 0, s, in, out, mats, matsf, batch, nr, K,
-                       C, Cp, nullptr);
+    hipLaunchKernelGGL((k_transform_mfma<1, TI, TO>), dim3(ntile + nfcb), dim3(64 * wpb), 0, s, in, out, mats, matsf,
+                       batch, nr, K, C, Cp, nullptr, nx, ny, ntile, fj);
 }
 template void launch_transform_t<double2, float2>(int, const double2*, float2*, const double*, const float*,
                                                   const TransformBatch*, int, int, int, int, int, double2*,
-                                                  hipStream_t, int);
+                                                  hipStream_t, int, const FcPhaseJob*);
 template void launch_transform_t<float2, float2>(int, const float2*, float2*, const double*, const float*,
                                                  const TransformBatch*, int, int, int, int, int, double2*,
-                                                 hipStream_t, int);
+                                                 hipStream_t, int, const FcPhaseJob*);
 template void launch_transform_t<float2, double2>(int, const float2*, double2*, const double*, const float*,
                                                   const TransformBatch*, int, int, int, int, int, double2*,
-                                                  hipStream_t, int);
+                                                  hipStream_t, int, const FcPhaseJob*);
 
 void launch_transform(int mode, const double2* in, double2* out, const double* mats, const TransformBatch* batch,
-                      int nbatch, int nr, int K, int C, int Cp, double2* copy, hipStream_t s, int path) {
-  launch_transform_t<double2, double2>(mode, in, out, mats, nullptr, batch, nbatch, nr, K, C, Cp, copy, s, path);
+                      int nbatch, int nr, int K, int C, int Cp, double2* copy, hipStream_t s, int path,
+                      const FcPhaseJob* fc) {
+  launch_transform_t<double2, double2>(mode, in, out, mats, nullptr, batch, nbatch, nr, K, C, Cp, copy, s, path, fc);
 }
 
 bool launch_onebody_el(const double2* orb, double2* h01, const OneBodyElArgs& a, hipStream_t s) {
@@ -1428,16 +1446,6 @@ void launch_mfprod(const T* orbg, const T* vg, T* gorbg, T* v2orbg, const MfArgs
 template void launch_mfprod<double2>(const double2*, const double2*, double2*, double2*, const MfArgs&, hipStream_t);
 template void launch_mfprod<float2>(const float2*, const float2*, float2*, float2*, const MfArgs&, hipStream_t);
 
-template <class T>
-void launch_fc_phase(const double2* orbg0, T* orbg, const double* xr, const double* cost, const double* lf, int nr,
-                     int nradfc, int nlat, int nfc, hipStream_t s) {
-  hipLaunchKernelGGL((k_fc_phase<T>), dim3((nr + 255) / 256, nlat), dim3(256), 0, s, orbg0, orbg, xr, cost, lf, nr,
-                     nradfc, nlat, nfc);
-}
-template void launch_fc_phase<double2>(const double2*, double2*, const double*, const double*, const double*, int, int,
-                                       int, int, hipStream_t);
-template void launch_fc_phase<float2>(const double2*, float2*, const double*, const double*, const double*, int, int,
-                                      int, int, hipStream_t);
 
 void launch_gram(const GramArgs& a, double2* partial, int maxblk, double2* out, hipStream_t s) {
   // about 4 index values per thread (two trips of two), at most maxblk

@@ -145,25 +145,34 @@ struct IslandArgs {
   double cnew;
 };
 
+// The frozen-core phase of the velocity gauge (hprod_fcorbg,
+// src/hprod/hprod_orbin.F90:68-105) as a job a transform launch can carry:
+// orbg[f][th][r] = orbg0[f][th][r] exp(-i A r cos th) for r < nradfc, 0
+// beyond; orbg has the transform's output type.
+struct FcPhaseJob {
+  const double2* orbg0;
+  void* orbg;
+  const double *xr, *cost, *lf;
+  int nr, nradfc, nlat, nfc;
+};
+
 // path: 0 by grid size, 1 the vector form, 2 the matrix-core (MFMA) form
 void launch_transform(int mode, const double2* in, double2* out, const double* mats, const TransformBatch* batch,
-                      int nbatch, int nr, int K, int C, int Cp, double2* copy, hipStream_t s, int path = 0);
+                      int nbatch, int nr, int K, int C, int Cp, double2* copy, hipStream_t s, int path = 0,
+                      const FcPhaseJob* fc = nullptr);
 inline int pad_cols(int C) { return (C + 31) / 32 * 32; }
 // the same transform with complex64 on either side (matsf: the matrices in
 // FP32, used when both sides are complex64)
 template <class TI, class TO>
 void launch_transform_t(int mode, const TI* in, TO* out, const double* mats, const float* matsf,
                         const TransformBatch* batch, int nbatch, int nr, int K, int C, int Cp, double2* copy,
-                        hipStream_t s, int path = 0);
+                        hipStream_t s, int path = 0, const FcPhaseJob* fc = nullptr);
 void launch_onebody(const double2* orb, double2* h01, const OneBodyArgs& a, hipStream_t s);
 bool launch_onebody_el(const double2* orb, double2* h01, const OneBodyElArgs& a, hipStream_t s);
 template <class T>  // rho and v complex128 or complex64; the solve is FP64
 void launch_poisson(const T* rho, double2* xb, T* v, const PoissonArgs& a, hipStream_t s);
 template <class T>  // theta-grid arrays complex128 or complex64 (arithmetic in the same)
 void launch_mfprod(const T* orbg, const T* vg, T* gorbg, T* v2orbg, const MfArgs& a, hipStream_t s);
-template <class T>  // theta-grid output complex128 or complex64
-void launch_fc_phase(const double2* orbg0, T* orbg, const double* xr, const double* cost, const double* lf, int nr,
-                     int nradfc, int nlat, int nfc, hipStream_t s);
 void launch_gram(const GramArgs& a, double2* partial, int nblk, double2* out, hipStream_t s);
 // Length-form dipole <z> = 2 sum_core <i|z|i> + sum_ab conj(den1(b,a)) <b|z|a>
 struct DipoleArgs {
