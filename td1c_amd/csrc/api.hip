@@ -525,7 +525,17 @@ struct CiStage {
 
 // hprod_orbin + hprod_fcorb: the current orbitals in the work array (l and
 // theta grid), the frozen cores with the velocity-gauge phase of d_lf.
+// The frozen-core theta -> l transform goes on stream sfc (after an event
+// the caller records on s when sfc differs).
+void enqueue_orbin_theta(td1c_ctx* c, const double* d_lf, const double2* wfn, hipStream_t s);
+void enqueue_orbin_fc_l(td1c_ctx* c, hipStream_t sfc);
+
 void enqueue_orbin(td1c_ctx* c, const double* d_lf, const double2* wfn, hipStream_t s) {
+  enqueue_orbin_theta(c, d_lf, wfn, s);
+  enqueue_orbin_fc_l(c, s);
+}
+
+void enqueue_orbin_theta(td1c_ctx* c, const double* d_lf, const double2* wfn, hipStream_t s) {
   Setup& S = c->S;
   const int nr = S.nr, nfc = S.nfcore, L1 = S.L1, nl = S.nlat;
   // frozen core in the velocity gauge (hprod_fcorbg): the phase on the theta
@@ -547,9 +557,14 @@ void enqueue_orbin(td1c_ctx* c, const double* d_lf, const double2* wfn, hipStrea
       launch_transform_t<double2, double2>(0, wfn, c->orbg, c->m_l2g.p, nullptr, c->b_l2g_dyn.p, c->n_l2g_dyn, nr, L1,
                                            nl, pad_cols(nl), c->orb, s, c->tpath, fc_vel ? &fj : nullptr);
   }
+}
+
+void enqueue_orbin_fc_l(td1c_ctx* c, hipStream_t s) {
+  Setup& S = c->S;
+  const int nr = S.nr, nfc = S.nfcore, L1 = S.L1, nl = S.nlat;
   // the phased frozen core back to l-space
-  if (fc_vel) {
-    TIMED("theta2l_fc");
+  if (nfc > 0 && S.igauge == 1) {
+    TIMED_ON(s, "theta2l_fc");
     if (c->mixed)
       launch_transform_t<float2, double2>(0, c->orbg_f, c->orb, c->m_g2l.p, nullptr, c->b_g2l_fc.p, c->n_g2l_fc, nr, nl,
                                           L1, pad_cols(L1), nullptr, s, c->tpath);
@@ -589,9 +604,12 @@ int enqueue_htotx(td1c_ctx* c, double dtime, const double* d_lf, const double2* 
   IslandArgs ia = island_args(c, cic, dtime);
   if (S.nact > 0) { TIMED_ON(s1, "island_a"); launch_island_a(ia, s1); }
   HIP_TRY(hipEventRecord(c->fj[1], s1));
-  enqueue_orbin(c, d_lf, wfn, s);
+  enqueue_orbin_theta(c, d_lf, wfn, s);
   HIP_TRY(hipEventRecord(c->fj[2], s));
   HIP_TRY(hipStreamWaitEvent(s2, c->fj[2], 0));
+  // side 1: the frozen cores in l-space, then the one-body part (the main
+  // stream needs neither before the l-space Gram products)
+  enqueue_orbin_fc_l(c, s2);
   // one-body part
   OneBodyArgs ob{};
   ob.nr = nr; ob.L1 = L1; ob.ndvr = S.n; ob.nfcore = nfc; ob.nradfc = S.nradfc;
