@@ -954,6 +954,83 @@ __global__ void __launch_bounds__(256) k_mfprod_t(const A2* __restrict__ orbg, c
   }
 }
 
+// The same products for four active orbitals and no dynamical core
+// (nact = 4, nfun = nfcore + 4, ncore = nfcore), four lanes per point: lane
+// q owns active orbital q and takes the act2 rows that reach it,
+//   t_q = sum_{x <= q} psi_x d(q, x) + sum_{x > q} psi_x conj(d(x, q)),
+//   d(i, j) = sum_{k,l} act2[j + 4 (i + 4 (l + 4 k))] vact[k][l],
+// in the order of k_mfprod_t, so the results agree bit for bit; the rden
+// contraction v2_q = sum_i rden[i + 4 q] t_i takes t_i from the quad by
+// shuffles.  The coefficients are staged in LDS.
+template <int NFC, class A2>
+__global__ void __launch_bounds__(256) k_mfprod_quad(const A2* __restrict__ orbg, const A2* __restrict__ vg,
+                                                     A2* __restrict__ gorbg, A2* __restrict__ v2orbg,
+                                                     const MfArgs a) {
+  constexpr int NA = 4, NF = NFC + NA, NCORE = NFC, NDP = NA * (NA + 1) / 2;
+  __shared__ A2 s_act2[NA * NA * NA * NA], s_rden[NA * NA];
+  s_act2[threadIdx.x] = cto<A2>(a.mfc[threadIdx.x]);  // 256 threads, 256 coefficients
+  if (threadIdx.x < NA * NA) s_rden[threadIdx.x] = cto<A2>(a.mfc[NA * NA * NA * NA + NA * NA + threadIdx.x]);
+  const int q = threadIdx.x & 3;
+  const int r = blockIdx.x * 64 + (threadIdx.x >> 2);
+  const int th = blockIdx.y;
+  const bool ok = r < a.nr;
+  const size_t ng = (size_t)a.nlat * a.nr;
+  const size_t g = (size_t)th * a.nr + (ok ? r : 0);
+  A2 psi[NF], vdp[NDP], vfc[NFC > 0 ? NFC : 1];
+#pragma unroll
+  for (int f = 0; f < NF; ++f) psi[f] = ok ? orbg[(size_t)f * ng + g] : cz<A2>();
+#pragma unroll
+  for (int p = 0; p < NDP; ++p) vdp[p] = ok ? vg[(size_t)p * ng + g] : cz<A2>();
+  // the frozen-core pair (i, dynamical q) of each type -1 core i
+#pragma unroll
+  for (int i = 0; i < NFC; ++i)
+    vfc[i] = (ok && i >= a.nfcore2) ? vg[(size_t)(NDP + (i - a.nfcore2) * NA + q) * ng + g] : cz<A2>();
+  __syncthreads();
+  if (!ok) return;
+  A2 vact[NA * NA];
+  {
+    int p = 0;
+#pragma unroll
+    for (int i = 0; i < NA; ++i)
+#pragma unroll
+      for (int j = i; j < NA; ++j) {
+        const A2 v = vdp[p++];
+        vact[j * NA + i] = v;
+        if (j != i) vact[i * NA + j] = cconj(v);
+      }
+  }
+  A2 ag = cz<A2>(), av = cz<A2>();
+#pragma unroll
+  for (int i = 0; i < NFC; ++i) {
+    if (i < a.nfcore2) continue;  // type -2 cores act through v2xfc only
+    if (r < a.nradfc) ag = csub(ag, cmul(cconj(vfc[i]), psi[i]));
+  }
+  if (a.nelact2 >= 2) {
+    A2 t = cz<A2>();
+#pragma unroll
+    for (int x = 0; x < NA; ++x) {
+      // d(q, x) for x <= q, d(x, q) for x > q
+      const int ia = x <= q ? q : x, ja = x <= q ? x : q;
+      A2 d = cz<A2>();
+#pragma unroll
+      for (int ka = 0; ka < NA; ++ka)
+#pragma unroll
+        for (int la = 0; la < NA; ++la) cfma(d, s_act2[ja + NA * (ia + NA * (la + NA * ka))], vact[ka * NA + la]);
+      cfma(t, psi[NCORE + x], x <= q ? d : cconj(d));
+    }
+    const int base = threadIdx.x & 60;  // first lane of the quad within the wavefront
+#pragma unroll
+    for (int ia = 0; ia < NA; ++ia) {
+      A2 ti;
+      ti.x = __shfl(t.x, base + ia, 64);
+      ti.y = __shfl(t.y, base + ia, 64);
+      cfma(av, s_rden[ia + NA * q], ti);
+    }
+  }
+  gorbg[(size_t)q * ng + g] = ag;
+  v2orbg[(size_t)q * ng + g] = av;
+}
+
 // ---------------------------------------------------------------------------
 // Gram products over a common index range, FP64, deterministic two-stage sum
 // (the inner products of hprod_mkint1_sph, hprod_mkint2_sph, hprod_ene_dcore
@@ -1428,6 +1505,12 @@ void launch_mfprod(const T* orbg, const T* vg, T* gorbg, T* v2orbg, const MfArgs
   case NFC * 10000 + NF * 100 + NA:                                                                     \
     hipLaunchKernelGGL((k_mfprod_t<NFC, NF, NA, T>), grid, block, 0, s, orbg, vg, gorbg, v2orbg, a);  \
     return;
+  if (a.nact == 4 && a.nfun == a.nfcore + 4 && a.ncore == a.nfcore && a.nfcore <= 1) {
+    dim3 gq((a.nr + 63) / 64, a.nlat);
+    if (a.nfcore == 1) hipLaunchKernelGGL((k_mfprod_quad<1, T>), gq, block, 0, s, orbg, vg, gorbg, v2orbg, a);
+    else hipLaunchKernelGGL((k_mfprod_quad<0, T>), gq, block, 0, s, orbg, vg, gorbg, v2orbg, a);
+    return;
+  }
   switch (key) {
     MF_CASE(0, 1, 0)
     MF_CASE(0, 2, 0)
