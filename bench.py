@@ -45,7 +45,7 @@ NSTEP_PROP = 2000
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "ne_l31_traffic.json")
 # device kernels behind each timing name (HBM counters are per device kernel)
 TIMING_KERNELS = {
-    "poisson": ["k_poisson_fused"],
+    "poisson": ["k_poisson_bridge_lds", "k_poisson_interior"],
     "onebody": ["k_onebody_el"],
     "mfprod": ["k_mfprod_t"],
     "island_a": ["k_island_a"],

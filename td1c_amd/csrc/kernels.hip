@@ -1315,16 +1315,17 @@ void launch_onebody(const double2* orb, double2* h01, const OneBodyArgs& a, hipS
   hipLaunchKernelGGL(k_onebody, grid, block, 0, s, orb, h01, a);
 }
 
-// Fused Poisson solve, one block per (pair, L), for grids whose radial row
-// fits in LDS: the row b = rho * invr is read once into LDS and serves the
-// bridge dots, the Schur right-hand sides and the interior solves; the
-// bridge values never leave LDS.  Dynamic LDS: b[nr] (complex), then the
-// cyclic-reduction coefficients sPA, sPG [pcr_steps * (nfe - 1)] each.
-// Jobs are L-major and XCD-remapped, so the pairs of one L read that L's
-// element inverses through one L2.
+// Bridge pass of the Poisson solve, one block per (pair, L), for grids
+// whose radial row fits in LDS: the row b = rho * invr is read once
+// (coalesced) into LDS and serves the bridge dots and the Schur right-hand
+// sides; the Schur system is solved by cyclic reduction in LDS and the
+// bridge values x_B, the multipole moment Q and the bridge rows of v are
+// written out for k_poisson_interior.  Dynamic LDS: b[nr] (complex), then
+// the cyclic-reduction coefficients sPA, sPG [pcr_steps * (nfe - 1)] each.
+// Jobs are L-major and XCD-remapped.
 template <int NI, int BS, class T>
-__global__ void __launch_bounds__(BS) k_poisson_fused(const T* __restrict__ rho, T* __restrict__ v,
-                                                       const PoissonArgs a) {
+__global__ void __launch_bounds__(BS) k_poisson_bridge_lds(const T* __restrict__ rho, T* __restrict__ v,
+                                                       double2* __restrict__ xb, const PoissonArgs a) {
   extern __shared__ double2 dyn_lds[];
   const int tid = threadIdx.x;
   const int job = xcd_remap(blockIdx.x, gridDim.x);
@@ -1342,7 +1343,8 @@ __global__ void __launch_bounds__(BS) k_poisson_fused(const T* __restrict__ rho,
   const size_t row = ((size_t)p * a.L2 + L) * nr;
   T* vp = v + row;
   if (L < (pm < 0 ? -pm : pm)) {  // uniform over the block
-    for (int r = tid; r < nr; r += BS) vp[r] = cz<T>();
+    // the bridge rows here, the interior rows in k_poisson_interior
+    for (int k = tid; k < nb; k += BS) vp[(k + 1) * n - 1] = cz<T>();
     return;
   }
   const T* rp = rho + row;
@@ -1428,50 +1430,118 @@ __global__ void __launch_bounds__(BS) k_poisson_fused(const T* __restrict__ rho,
     cur = nxt;
     nxt = t;
   }
-  // bridge values x_B (kept in nxt) and their rows of v
+  // bridge values x_B and their rows of v
   if (tid < nb) {
     const int gi = (tid + 1) * n - 1;
     const double2 x = cscale(cur[tid], a.pcrBinv[(size_t)L * nb + tid]);
-    nxt[tid] = x;
+    xb[((size_t)p * a.L2 + L) * nb + tid] = x;
     vp[gi] = cto<T>(make_double2((x.x + Q.x * a.rpl1[lo + gi]) * a.fac2[lo + gi],
                                  (x.y + Q.y * a.rpl1[lo + gi]) * a.fac2[lo + gi]));
   }
-  __syncthreads();
-  const double2* xB = nxt;
-  // interiors: x_I = G_e b_I - W_l x_left - W_r x_right, thread per row
-  const double* Gl = a.G + (size_t)L * a.nfe * ni * ni;
-  for (int q = tid; q < a.nfe * ni; q += BS) {
-    const int e = q / ni, i = q - e * ni;
-    const int r = e * n + i;
-    const double* ge = Gl + (size_t)e * ni * ni + i;
-    const double2* be = sB + e * n;
-    double2 y = make_double2(0.0, 0.0);
+  if (tid == 0) a.Q[(size_t)p * a.L2 + L] = Q;
+}
+
+// Interiors of the element Schur solve after k_poisson_bridge_lds: one
+// block per (L, chunk of EPB elements, group of PG pairs), thread per
+// interior row.  Each thread loads its row of G_e once and applies it to
+// every pair of the group, whose rows b = rho * invr are staged in LDS:
+//   x_I = G_e b_I - W_l x_left - W_r x_right,  v = (x + Q rpl1) * fac2.
+constexpr int kPoissonPairGroup = 7;
+template <int NI, class T>
+__global__ void __launch_bounds__(256) k_poisson_interior(const T* __restrict__ rho, T* __restrict__ v,
+                                                         const double2* __restrict__ xb, const PoissonArgs a,
+                                                         int epb, int nchunk) {
+  constexpr int PG = kPoissonPairGroup;
+  const int ni = NI > 0 ? NI : a.ni;
+  const int nb = a.nfe - 1, nr = a.nr, n = a.n;
+  const int ngrp = (a.npair + PG - 1) / PG;
+  // L-major, then element chunk, then pair group; XCD-remapped
+  const int job = xcd_remap(blockIdx.x, gridDim.x);
+  const int L = job / (nchunk * ngrp), ch = (job / ngrp) % nchunk, pg = job % ngrp;
+  const int p0 = pg * PG, e0 = ch * epb;
+  const int t = threadIdx.x;
+  const int el = t / ni, i = t - el * ni;
+  const int e = e0 + el;
+  const bool row_ok = el < epb && e < a.nfe;
+  const int r = e * n + i;
+  const size_t lo = (size_t)L * nr;
+  __shared__ double2 sb[PG][256];
+  __shared__ double2 sx[PG][34];  // bridges e0-1 .. e0+epb-1 (epb <= 32)
+  __shared__ double2 sq[PG];
+  __shared__ int svalid[PG];
+  if (t < PG) {
+    const int p = p0 + t;
+    int ok = 0;
+    if (p < a.npair) {
+      const int pm = a.pair_m[p];
+      ok = L >= (pm < 0 ? -pm : pm);
+      sq[t] = ok ? a.Q[(size_t)p * a.L2 + L] : make_double2(0.0, 0.0);
+    }
+    svalid[t] = ok;
+  }
+  // stage the rows (all loads before the barrier)
+  double iv = 0.0;
+  if (row_ok) iv = a.invr[lo + r];
+#pragma unroll
+  for (int j = 0; j < PG; ++j) {
+    const int p = p0 + j;
+    if (row_ok && p < a.npair) sb[j][t] = cscale(cto<double2>(rho[((size_t)p * a.L2 + L) * nr + r]), iv);
+  }
+  for (int q = t; q < PG * (epb + 1); q += 256) {
+    const int j = q / (epb + 1), k = q % (epb + 1);
+    const int p = p0 + j, bidx = e0 - 1 + k;
+    sx[j][k] = (p < a.npair && bidx >= 0 && bidx < nb) ? xb[((size_t)p * a.L2 + L) * nb + bidx]
+                                                       : make_double2(0.0, 0.0);
+  }
+  double gk[NI > 0 ? NI : 1];
+  double wl = 0.0, wr = 0.0, r1 = 0.0, f2 = 0.0;
+  const double* ge = a.G + (((size_t)L * a.nfe) + (row_ok ? e : 0)) * ni * ni + i;
+  if (row_ok) {
     if (NI > 0) {
-      double gk[NI > 0 ? NI : 1];
 #pragma unroll
       for (int k = 0; k < NI; ++k) gk[k] = ge[k * NI];
+    }
+    wl = a.Wl[lo + r];
+    wr = a.Wr[lo + r];
+    r1 = a.rpl1[lo + r];
+    f2 = a.fac2[lo + r];
+  }
+  __syncthreads();
+  if (!row_ok) return;
+  const int eb = el * ni;  // this element's first row in the staged rows
+#pragma unroll 1
+  for (int j = 0; j < PG; ++j) {
+    const int p = p0 + j;
+    if (p >= a.npair) break;
+    T* vp = v + ((size_t)p * a.L2 + L) * nr;
+    if (!svalid[j]) {
+      vp[r] = cz<T>();
+      continue;
+    }
+    double2 y = make_double2(0.0, 0.0);
+    if (NI > 0) {
 #pragma unroll
       for (int k = 0; k < NI; ++k) {
-        y.x = fma(gk[k], be[k].x, y.x);
-        y.y = fma(gk[k], be[k].y, y.y);
+        y.x = fma(gk[k], sb[j][eb + k].x, y.x);
+        y.y = fma(gk[k], sb[j][eb + k].y, y.y);
       }
     } else {
       for (int k = 0; k < ni; ++k) {
         const double g = ge[k * ni];
-        y.x = fma(g, be[k].x, y.x);
-        y.y = fma(g, be[k].y, y.y);
+        y.x = fma(g, sb[j][eb + k].x, y.x);
+        y.y = fma(g, sb[j][eb + k].y, y.y);
       }
     }
-    if (e >= 1) y = csub(y, cscale(xB[e - 1], a.Wl[lo + r]));
-    if (e <= a.nfe - 2) y = csub(y, cscale(xB[e], a.Wr[lo + r]));
-    vp[r] = cto<T>(make_double2((y.x + Q.x * a.rpl1[lo + r]) * a.fac2[lo + r],
-                                (y.y + Q.y * a.rpl1[lo + r]) * a.fac2[lo + r]));
+    if (e >= 1) y = csub(y, cscale(sx[j][el], wl));
+    if (e <= a.nfe - 2) y = csub(y, cscale(sx[j][el + 1], wr));
+    const double2 Q = sq[j];
+    vp[r] = cto<T>(make_double2((y.x + Q.x * r1) * f2, (y.y + Q.y * r1) * f2));
   }
 }
 
-constexpr int kPoissonBlock = 512;  // threads of the fused Poisson kernel (>= nfe)
+constexpr int kPoissonBlock = 512;  // threads of the LDS bridge pass (>= nfe)
 
-// LDS bytes of the fused Poisson kernel; 0 when the grid does not fit
+// LDS bytes of the LDS bridge pass; 0 when the row does not fit
 size_t poisson_fused_lds(const PoissonArgs& a) {
   const size_t bytes = (size_t)a.nr * sizeof(double2) + 2 * (size_t)a.pcr_steps * (a.nfe - 1) * sizeof(double);
   return bytes <= 40 * 1024 ? bytes : 0;  // four blocks per CU beside the static arrays
@@ -1481,10 +1551,22 @@ template <class T>
 void launch_poisson(const T* rho, double2* xb, T* v, const PoissonArgs& a, hipStream_t s) {
   const int jobs = a.npair * a.L2;
   if (const size_t lds = poisson_fused_lds(a)) {
+    // bridge pass (rows staged in LDS), then the interiors with each row of
+    // G_e shared by a group of pairs
     if (a.ni == 19)
-      hipLaunchKernelGGL((k_poisson_fused<19, kPoissonBlock, T>), dim3(jobs), dim3(kPoissonBlock), lds, s, rho, v, a);
+      hipLaunchKernelGGL((k_poisson_bridge_lds<19, kPoissonBlock, T>), dim3(jobs), dim3(kPoissonBlock), lds, s, rho, v,
+                         xb, a);
     else
-      hipLaunchKernelGGL((k_poisson_fused<0, kPoissonBlock, T>), dim3(jobs), dim3(kPoissonBlock), lds, s, rho, v, a);
+      hipLaunchKernelGGL((k_poisson_bridge_lds<0, kPoissonBlock, T>), dim3(jobs), dim3(kPoissonBlock), lds, s, rho, v,
+                         xb, a);
+    const int epb = std::min(32, 256 / a.ni);
+    const int nchunk = (a.nfe + epb - 1) / epb;
+    const int ngrp = (a.npair + kPoissonPairGroup - 1) / kPoissonPairGroup;
+    const dim3 grid(a.L2 * nchunk * ngrp);
+    if (a.ni == 19)
+      hipLaunchKernelGGL((k_poisson_interior<19, T>), grid, dim3(256), 0, s, rho, v, xb, a, epb, nchunk);
+    else
+      hipLaunchKernelGGL((k_poisson_interior<0, T>), grid, dim3(256), 0, s, rho, v, xb, a, epb, nchunk);
     return;
   }
   if (a.ni == 19)
