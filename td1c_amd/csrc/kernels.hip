@@ -1447,8 +1447,9 @@ __global__ void __launch_bounds__(BS) k_poisson_bridge_lds(const T* __restrict__
 // every pair of the group, whose rows b = rho * invr are staged in LDS:
 //   x_I = G_e b_I - W_l x_left - W_r x_right,  v = (x + Q rpl1) * fac2.
 constexpr int kPoissonPairGroup = 7;
+constexpr int kPoissonInteriorThreads = 256;
 template <int NI, class T>
-__global__ void __launch_bounds__(256) k_poisson_interior(const T* __restrict__ rho, T* __restrict__ v,
+__global__ void __launch_bounds__(kPoissonInteriorThreads) k_poisson_interior(const T* __restrict__ rho, T* __restrict__ v,
                                                          const double2* __restrict__ xb, const PoissonArgs a,
                                                          int epb, int nchunk) {
   constexpr int PG = kPoissonPairGroup;
@@ -1465,7 +1466,7 @@ __global__ void __launch_bounds__(256) k_poisson_interior(const T* __restrict__ 
   const bool row_ok = el < epb && e < a.nfe;
   const int r = e * n + i;
   const size_t lo = (size_t)L * nr;
-  __shared__ double2 sb[PG][256];
+  __shared__ double2 sb[PG][kPoissonInteriorThreads];
   __shared__ double2 sx[PG][34];  // bridges e0-1 .. e0+epb-1 (epb <= 32)
   __shared__ double2 sq[PG];
   __shared__ int svalid[PG];
@@ -1487,7 +1488,7 @@ __global__ void __launch_bounds__(256) k_poisson_interior(const T* __restrict__ 
     const int p = p0 + j;
     if (row_ok && p < a.npair) sb[j][t] = cscale(cto<double2>(rho[((size_t)p * a.L2 + L) * nr + r]), iv);
   }
-  for (int q = t; q < PG * (epb + 1); q += 256) {
+  for (int q = t; q < PG * (epb + 1); q += kPoissonInteriorThreads) {
     const int j = q / (epb + 1), k = q % (epb + 1);
     const int p = p0 + j, bidx = e0 - 1 + k;
     sx[j][k] = (p < a.npair && bidx >= 0 && bidx < nb) ? xb[((size_t)p * a.L2 + L) * nb + bidx]
@@ -1559,14 +1560,16 @@ void launch_poisson(const T* rho, double2* xb, T* v, const PoissonArgs& a, hipSt
     else
       hipLaunchKernelGGL((k_poisson_bridge_lds<0, kPoissonBlock, T>), dim3(jobs), dim3(kPoissonBlock), lds, s, rho, v,
                          xb, a);
-    const int epb = std::min(32, 256 / a.ni);
+    const int epb = std::min(32, kPoissonInteriorThreads / a.ni);
     const int nchunk = (a.nfe + epb - 1) / epb;
     const int ngrp = (a.npair + kPoissonPairGroup - 1) / kPoissonPairGroup;
     const dim3 grid(a.L2 * nchunk * ngrp);
     if (a.ni == 19)
-      hipLaunchKernelGGL((k_poisson_interior<19, T>), grid, dim3(256), 0, s, rho, v, xb, a, epb, nchunk);
+      hipLaunchKernelGGL((k_poisson_interior<19, T>), grid, dim3(kPoissonInteriorThreads), 0, s, rho, v, xb, a, epb,
+                         nchunk);
     else
-      hipLaunchKernelGGL((k_poisson_interior<0, T>), grid, dim3(256), 0, s, rho, v, xb, a, epb, nchunk);
+      hipLaunchKernelGGL((k_poisson_interior<0, T>), grid, dim3(kPoissonInteriorThreads), 0, s, rho, v, xb, a, epb,
+                         nchunk);
     return;
   }
   if (a.ni == 19)
