@@ -1096,13 +1096,13 @@ __global__ void __launch_bounds__(256) k_gram(const GramArgs a, double2* __restr
     }
   }
   // block reduction.  The tile holds NV = TA*TB*(CR ? 2 : 1) complex sums =
-  // ND = 2*NV doubles per lane (32 or 64); a transposing butterfly over
+  // ND = 2*NV doubles per lane (16, 32 or 64); a transposing butterfly over
   // ND/2 ... 1 leaves lane t with the total of double t % ND over its group
-  // of ND lanes, one more exchange joins the two halves when ND = 32, and
-  // the 4 waves are summed through LDS.
+  // of ND lanes, exchanges at distance ND ... 32 join the groups, and the 4
+  // waves are summed through LDS.
   constexpr int NV = TA * TB * (CR ? 2 : 1);
   constexpr int ND = 2 * NV;
-  static_assert(ND == 32 || ND == 64, "tile must hold 32 or 64 doubles per lane");
+  static_assert(ND == 16 || ND == 32 || ND == 64, "tile must hold 16, 32 or 64 doubles per lane");
   double v[ND];
 #pragma unroll
   for (int u = 0; u < TA; ++u)
@@ -1126,13 +1126,16 @@ __global__ void __launch_bounds__(256) k_gram(const GramArgs a, double2* __restr
   if constexpr (ND == 64) {
     TD1C_GRAM_STEP(32)
   }
-  TD1C_GRAM_STEP(16)
+  if constexpr (ND >= 32) {
+    TD1C_GRAM_STEP(16)
+  }
   TD1C_GRAM_STEP(8)
   TD1C_GRAM_STEP(4)
   TD1C_GRAM_STEP(2)
   TD1C_GRAM_STEP(1)
 #undef TD1C_GRAM_STEP
-  if constexpr (ND == 32) v[0] += __shfl_xor(v[0], 32, 64);
+  if constexpr (ND == 16) v[0] += __shfl_xor(v[0], 16, 64);
+  if constexpr (ND <= 32) v[0] += __shfl_xor(v[0], 32, 64);
   __shared__ double red[4][64];
   red[wave][lane] = v[0];
   __syncthreads();
@@ -1624,15 +1627,15 @@ void launch_gram(const GramArgs& a, double2* partial, int maxblk, double2* out, 
   // about 4 index values per thread (two trips of two), at most maxblk
   // blocks along the index
   const int nblk = (int)std::max<size_t>(1, std::min<size_t>(maxblk, (a.len + 1023) / 1024));
-  // register tiles of 16 complex sums per thread (32 doubles), which keep
-  // the kernel at four or more wavefronts per SIMD
+  // register tiles of 8 complex sums per thread (16 doubles), which keep
+  // the kernel at many wavefronts per SIMD
   if (a.with_conj_right) {
-    constexpr int TA = 2, TB = 4;
+    constexpr int TA = 2, TB = 2;
     dim3 grid(nblk, ((a.nL + TA - 1) / TA) * ((a.nR + TB - 1) / TB));
     if (a.f32) hipLaunchKernelGGL((k_gram<TA, TB, true, float2>), grid, dim3(256), 0, s, a, partial, nblk);
     else hipLaunchKernelGGL((k_gram<TA, TB, true, double2>), grid, dim3(256), 0, s, a, partial, nblk);
   } else {
-    constexpr int TA = 4, TB = 4;
+    constexpr int TA = 2, TB = 4;
     dim3 grid(nblk, ((a.nL + TA - 1) / TA) * ((a.nR + TB - 1) / TB));
     if (a.f32) hipLaunchKernelGGL((k_gram<TA, TB, false, float2>), grid, dim3(256), 0, s, a, partial, nblk);
     else hipLaunchKernelGGL((k_gram<TA, TB, false, double2>), grid, dim3(256), 0, s, a, partial, nblk);
