@@ -276,6 +276,24 @@ __global__ void __launch_bounds__(kT) k_island_b(const IslandArgs a) {
     I2[e] = v;
     if (a.i2_lds) a.int2e[e] = v;
   }
+  // inputs of the CI derivative and of the Runge-Kutta update that do not
+  // depend on the Gram products, loaded together with the gather above
+  const int nc = na > 0 ? a.nstr_a * a.nstr_b : 0;
+  const CiView v = ci_view(a, dyn);
+  if (na > 0) {
+    if (a.ci_lds)
+      for (size_t q = tid; q < (size_t)na * na * nc; q += kT) v.D[q] = a.ci_work[q];
+    for (int f = tid; f < nc; f += kT) v.C[f] = a.det_of[f] >= 0 ? a.cic[a.det_of[f]] : zero2();
+  }
+  const bool rk_pre = a.kout != nullptr && tid < a.lcic;  // this thread's first CI coefficient
+  double2 rk_w0 = zero2(), rk_acc = zero2(), rk_kp[kMaxStages];
+  if (rk_pre) {
+    if (a.acc != nullptr) rk_acc = a.acc[tid];
+    if (a.next != nullptr) {
+      rk_w0 = a.w0[tid];
+      for (int j = 0; j < a.nprev; ++j) rk_kp[j] = a.kprev[j][tid];
+    }
+  }
   __syncthreads();
 #define H(j, i) s_H[(j) * nf + (i)]
 #define G(j, i) s_G[(j) * nf + (i)]
@@ -382,15 +400,10 @@ __global__ void __launch_bounds__(kT) k_island_b(const IslandArgs a) {
 #undef HGV
   // CI derivative: dC = fac (H_CI - E_act) C, with D_pq = E_pq C from stage A
   if (na > 0) {
-    const int nc = a.nstr_a * a.nstr_b;
-    const CiView v = ci_view(a, dyn);
     double2* D = v.D;
     double2* C = v.C;
     double2* S = v.S;
     double2* T = v.T;
-    if (a.ci_lds)
-      for (size_t q = tid; q < (size_t)na * na * nc; q += kT) D[q] = a.ci_work[q];
-    for (int f = tid; f < nc; f += kT) C[f] = a.det_of[f] >= 0 ? a.cic[a.det_of[f]] : zero2();
     // h'_pq = int1e(p,q) - 1/2 sum_r int2e(p,r,r,q)
     for (int e = tid; e < na * na; e += kT) {
       const int p = e % na, q = e / na;
@@ -433,11 +446,12 @@ __global__ void __launch_bounds__(kT) k_island_b(const IslandArgs a) {
     __syncthreads();
     for (int d = tid; d < a.lcic; d += kT) {
       const double2 k = a.dcic[d];
+      const bool pre = d == tid && rk_pre;  // inputs already in registers
       a.kout[d] = k;
-      if (a.acc != nullptr) a.acc[d] = zadd(a.acc[d], zscal(k, a.dtw));
+      if (a.acc != nullptr) a.acc[d] = zadd(pre ? rk_acc : a.acc[d], zscal(k, a.dtw));
       if (a.next != nullptr) {
-        double2 t = a.w0[d];
-        for (int j = 0; j < a.nprev; ++j) t = zadd(t, zscal(a.kprev[j][d], a.cprev[j]));
+        double2 t = pre ? rk_w0 : a.w0[d];
+        for (int j = 0; j < a.nprev; ++j) t = zadd(t, zscal(pre ? rk_kp[j] : a.kprev[j][d], a.cprev[j]));
         a.next[d] = zadd(t, zscal(k, a.cnew));
       }
     }
