@@ -196,7 +196,7 @@ __global__ void k_fc_phase(const FcPhaseJob j) {
 // parts are two MFMA B operands sharing the A operand (the real matrix).
 // Block: one tile of 16 radial rows and up to 4 column tiles of 16, one
 // wavefront per column tile, each running the whole contraction in steps of
-// 4 (8 steps' loads issued together).  Lane layouts of the 16x16x4 MFMAs on
+// 4 (MODE 1: over the pair products staged in LDS).  Lane layouts of the 16x16x4 MFMAs on
 // gfx950: A[i][k] at lane i + 16k, B[k][j] at lane j + 16k; register v of
 // the result in lane l holds column l % 16 and row 4 (l / 16) + v (f32) or
 // l / 16 + 4 v (f64).
@@ -229,59 +229,101 @@ __global__ void __launch_bounds__(256) k_transform_mfma(const TI* __restrict__ i
   using AR = typename CReal<A2>::T;
   using MF = Mfma16<AR>;
   using V4 = typename MF::V4;
-  constexpr int U = 4;  // contraction steps per load batch
+  constexpr int KC = 64;  // contraction values staged in LDS per pass
   // blocks past the ntile transform tiles run the frozen-core phase job
   if ((int)blockIdx.x >= ntile) {
     const int q = (int)blockIdx.x - ntile, rb = (fc.nr + blockDim.x - 1) / blockDim.x;
     fc_phase_point<TO>(fc, (q % rb) * blockDim.x + threadIdx.x, q / rb);
     return;
   }
+  __shared__ A2 sX[MODE == 1 ? KC * 16 : 1];
   const int t = xcd_remap(blockIdx.x, ntile);
   const int by = t % ny, bx = (t / ny) % nx, bz = t / (ny * nx);
   const TransformBatch bt = batch[bz];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int c0 = (by * (int)(blockDim.x >> 6) + wave) * 16;  // first column of this wavefront's tile
-  if (c0 >= C) return;  // wave-uniform; the kernel has no barrier
-  const int r = bx * 16 + (lane & 15);
+  const bool active = c0 < C;  // wave-uniform
+  const int r0 = bx * 16;
   const int kq = lane >> 4;
-  const bool row_ok = r < bt.rows;
   const int ca = c0 + (lane & 15);  // matrix column this lane feeds to A
   const bool ca_ok = ca < Cp;       // columns C..Cp-1 of the padded matrix are zero
   const AR* __restrict__ M;
   if constexpr (std::is_same<AR, float>::value) M = matsf + (size_t)bt.mat * K * Cp;
   else M = mats + (size_t)bt.mat * K * Cp;
-  const TI* __restrict__ x0 = in + (size_t)bt.in0 + r;
-  const TI* __restrict__ x1 = in + (size_t)bt.in1 + r;
   V4 dre = {0, 0, 0, 0}, dim = {0, 0, 0, 0};
-  const int nks = (K + 3) >> 2;
-  for (int s0 = 0; s0 < nks; s0 += U) {
-    AR a[U];
-    A2 x[U];
+  if constexpr (MODE == 1) {
+    // the pair products, formed once per block into LDS for all its
+    // column-tile wavefronts
+    for (int k0 = 0; k0 < K; k0 += KC) {
+      // stage: element q of the tile is (k = k0 + q / 16, row r0 + q % 16),
+      // four elements per thread per round with the loads issued together
+      for (int q0 = 0; q0 < KC * 16; q0 += 4 * (int)blockDim.x) {
+        TI x0[4], x1[4];
+        bool ok[4];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int k = 4 * (s0 + u) + kq;
-      const bool kok = k < K;
-      a[u] = (kok && ca_ok) ? M[(size_t)k * Cp + ca] : AR(0);
-      x[u] = cz<A2>();
-      if (kok && row_ok) {
-        if (MODE == 1) {
-          x[u] = cmulc(cto<A2>(x0[(size_t)k * nr]), cto<A2>(x1[(size_t)k * nr]));
-        } else {
+        for (int u = 0; u < 4; ++u) {
+          const int q = q0 + u * (int)blockDim.x + (int)threadIdx.x;
+          const int k = k0 + (q >> 4), r = r0 + (q & 15);
+          ok[u] = q < KC * 16 && k < K && r < bt.rows;
+          const size_t g = (size_t)k * nr + r;
+          x0[u] = ok[u] ? in[(size_t)bt.in0 + g] : TI{};
+          x1[u] = ok[u] ? in[(size_t)bt.in1 + g] : TI{};
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int q = q0 + u * (int)blockDim.x + (int)threadIdx.x;
+          if (q < KC * 16) sX[q] = ok[u] ? cmulc(cto<A2>(x0[u]), cto<A2>(x1[u])) : cz<A2>();
+        }
+      }
+      __syncthreads();
+      if (active) {
+        const int nks = (min(KC, K - k0) + 3) >> 2;
+#pragma unroll 4
+        for (int st = 0; st < nks; ++st) {
+          const int kl = 4 * st + kq;
+          const int k = k0 + kl;
+          const AR a = (k < K && ca_ok) ? M[(size_t)k * Cp + ca] : AR(0);
+          const A2 x = sX[kl * 16 + (lane & 15)];
+          dre = MF::run(a, x.x, dre);
+          dim = MF::run(a, x.y, dim);
+        }
+      }
+      __syncthreads();
+    }
+  } else if (active) {
+    // plain input: each wavefront reads its B operand directly (the other
+    // column tiles of the block hit the same lines in L1)
+    constexpr int U = 4;  // contraction steps per load batch
+    const int r = r0 + (lane & 15);
+    const bool row_ok = r < bt.rows;
+    const TI* __restrict__ x0 = in + (size_t)bt.in0 + r;
+    const int nks = (K + 3) >> 2;
+    for (int s0 = 0; s0 < nks; s0 += U) {
+      AR a[U];
+      A2 x[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int k = 4 * (s0 + u) + kq;
+        const bool kok = k < K;
+        a[u] = (kok && ca_ok) ? M[(size_t)k * Cp + ca] : AR(0);
+        x[u] = cz<A2>();
+        if (kok && row_ok) {
           const TI xv = x0[(size_t)k * nr];
           x[u] = cto<A2>(xv);
           if constexpr (std::is_same<TI, double2>::value)
             if (copy != nullptr && c0 == 0) copy[(size_t)bt.in0 + (size_t)k * nr + r] = xv;
         }
       }
-    }
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      dre = MF::run(a[u], x[u].x, dre);
-      dim = MF::run(a[u], x[u].y, dim);
+      for (int u = 0; u < U; ++u) {
+        dre = MF::run(a[u], x[u].x, dre);
+        dim = MF::run(a[u], x[u].y, dim);
+      }
     }
   }
-  if (row_ok) {
+  const int r = r0 + (lane & 15);
+  if (active && r < bt.rows) {
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int c = c0 + MF::row(lane, v);
