@@ -690,8 +690,8 @@ int enqueue_htotx(td1c_ctx* c, double dtime, const double* d_lf, const double2* 
   g.len = nb; g.nL = c->gl_nL; g.nR = c->gl_nR; g.conj_left = 1; g.with_conj_right = 0;
   g.left = c->g_left_l.p; g.right = c->g_right_l.p;
   { TIMED("gram_l"); launch_gram(g, c->gram_partial.p, c->gram_blk, c->lprod.p, s); }
-  // join the side streams
-  HIP_TRY(hipStreamWaitEvent(s, c->fj[1], 0));
+  // join the side streams (fj[4] follows fj[1] on side stream 0, so one
+  // wait covers the RDM stage and the pair Gram products)
   HIP_TRY(hipStreamWaitEvent(s, c->fj[4], 0));
   ia.kout = cs.kout; ia.acc = cs.acc; ia.dtw = cs.dtw; ia.next = cs.next; ia.w0 = cs.w0;
   ia.nprev = cs.nprev; ia.cnew = cs.cnew;
