@@ -103,7 +103,10 @@ struct td1c_ctx {
   // two side streams (default priority) for the independent branches of an
   // hprod, joined back by events; the events carry no timing
   hipStream_t side[2] = {nullptr, nullptr};
-  hipEvent_t fj[6] = {};
+  // fork/join events: [0] pair potentials ready, [1] RDM stage done,
+  // [2] orbitals on the theta grid, [3] one-body part done, [4] pair Gram
+  // products done
+  hipEvent_t fj[5] = {};
   int tpath = 0;  // transform form: 0 by grid size, 1 vector, 2 matrix cores
   // device-driven propagation: one captured RK step, replayed per step
   hipStream_t cap_stream = nullptr;
@@ -577,8 +580,8 @@ void enqueue_orbin_fc_l(td1c_ctx* c, hipStream_t s) {
 // int2e Gram products on side stream s1 once the pair potentials exist on s;
 // records fj[4] on s1 when they are done.
 hipError_t enqueue_pair_gram(td1c_ctx* c, hipStream_t s, hipStream_t s1) {
-  hipError_t e = hipEventRecord(c->fj[5], s);
-  if (e == hipSuccess) e = hipStreamWaitEvent(s1, c->fj[5], 0);
+  hipError_t e = hipEventRecord(c->fj[0], s);
+  if (e == hipSuccess) e = hipStreamWaitEvent(s1, c->fj[0], 0);
   if (e != hipSuccess) return e;
   if (c->gp_nL > 0) {
     GramArgs h{};
@@ -599,13 +602,14 @@ int enqueue_htotx(td1c_ctx* c, double dtime, const double* d_lf, const double2* 
   // Branches: side 0 runs the CI RDM stage and later the pair Gram products,
   // side 1 the one-body part; the main stream runs the mean-field chain.
   hipStream_t s1 = c->side[0], s2 = c->side[1];
-  HIP_TRY(hipEventRecord(c->fj[0], s));
-  HIP_TRY(hipStreamWaitEvent(s1, c->fj[0], 0));
+  // one fork after the orbitals' l -> theta launch serves both side streams
+  // (the RDM stage needs nothing from it, and starts early enough)
+  enqueue_orbin_theta(c, d_lf, wfn, s);
+  HIP_TRY(hipEventRecord(c->fj[2], s));
+  HIP_TRY(hipStreamWaitEvent(s1, c->fj[2], 0));
   IslandArgs ia = island_args(c, cic, dtime);
   if (S.nact > 0) { TIMED_ON(s1, "island_a"); launch_island_a(ia, s1); }
   HIP_TRY(hipEventRecord(c->fj[1], s1));
-  enqueue_orbin_theta(c, d_lf, wfn, s);
-  HIP_TRY(hipEventRecord(c->fj[2], s));
   HIP_TRY(hipStreamWaitEvent(s2, c->fj[2], 0));
   // side 1: the frozen cores in l-space, then the one-body part (the main
   // stream needs neither before the l-space Gram products)
