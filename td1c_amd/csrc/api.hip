@@ -520,6 +520,7 @@ struct CiStage {
   double dtw = 0.0;
   double2* next = nullptr;
   const double2* w0 = nullptr;
+  double2* w0out = nullptr;
   int nprev = 0;
   const double2* kprev[kMaxStages] = {};
   double cprev[kMaxStages] = {};
@@ -697,7 +698,7 @@ int enqueue_htotx(td1c_ctx* c, double dtime, const double* d_lf, const double2* 
   // join the side streams (fj[4] follows fj[1] on side stream 0, so one
   // wait covers the RDM stage and the pair Gram products)
   HIP_TRY(hipStreamWaitEvent(s, c->fj[4], 0));
-  ia.kout = cs.kout; ia.acc = cs.acc; ia.dtw = cs.dtw; ia.next = cs.next; ia.w0 = cs.w0;
+  ia.kout = cs.kout; ia.acc = cs.acc; ia.dtw = cs.dtw; ia.next = cs.next; ia.w0 = cs.w0; ia.w0out = cs.w0out;
   ia.nprev = cs.nprev; ia.cnew = cs.cnew;
   for (int j = 0; j < kMaxStages; ++j) { ia.kprev[j] = cs.kprev[j]; ia.cprev[j] = cs.cprev[j]; }
   { TIMED("island_b"); launch_island_b(ia, s); }
@@ -959,9 +960,10 @@ static int enqueue_rk_step(td1c_ctx* c, int rk_order, double dtime, const double
   const size_t so = (size_t)c->nbas * S.nfun, sz = so + S.lcic;
   double2* w0 = c->rk_w0.p;
   double2* t = c->rk_t.p;
-  launch_copy(w0, wfn, sz, s);
-  // stage input of stage 0 is w0 itself
-  const double2* tin = w0;
+  // stage 0 reads the state in place; its dtorb and island kernels read the
+  // step-start state before updating it and leave a copy in w0 for the
+  // later stages
+  const double2* tin = wfn;
   for (int is = 0; is < ns; ++is) {
     double2* k = c->rk_k.p + sz * is;
     CiStage cs;
@@ -970,7 +972,8 @@ static int enqueue_rk_step(td1c_ctx* c, int rk_order, double dtime, const double
     cs.dtw = dtime * w[is];
     if (is + 1 < ns) {
       cs.next = t + so;
-      cs.w0 = w0 + so;
+      cs.w0 = is == 0 ? wfn + so : w0 + so;
+      if (is == 0) cs.w0out = w0 + so;
       cs.nprev = is;
       for (int js = 0; js < is; ++js) {
         cs.kprev[js] = c->rk_k.p + sz * js + so;
@@ -987,7 +990,8 @@ static int enqueue_rk_step(td1c_ctx* c, int rk_order, double dtime, const double
     d.mask = (apply_mask && is == ns - 1) ? c->mask.p : nullptr;
     if (is + 1 < ns) {
       d.next = t;
-      d.w0 = w0;
+      d.w0 = is == 0 ? wfn : w0;
+      if (is == 0) d.w0out = w0;
       d.nprev = is;
       for (int js = 0; js < is; ++js) {
         d.kprev[js] = c->rk_k.p + sz * js;

@@ -447,10 +447,16 @@ __global__ void __launch_bounds__(kT) k_island_b(const IslandArgs a) {
     for (int d = tid; d < a.lcic; d += kT) {
       const double2 k = a.dcic[d];
       const bool pre = d == tid && rk_pre;  // inputs already in registers
+      // w0 may alias acc: both read before acc is written
+      const double2 acc0 = a.acc != nullptr ? (pre ? rk_acc : a.acc[d]) : zero2();
+      const double2 w0v = a.next != nullptr ? (pre ? rk_w0 : a.w0[d]) : zero2();
       a.kout[d] = k;
-      if (a.acc != nullptr) a.acc[d] = zadd(pre ? rk_acc : a.acc[d], zscal(k, a.dtw));
+      if (a.acc != nullptr) {
+        if (a.w0out != nullptr) a.w0out[d] = acc0;
+        a.acc[d] = zadd(acc0, zscal(k, a.dtw));
+      }
       if (a.next != nullptr) {
-        double2 t = pre ? rk_w0 : a.w0[d];
+        double2 t = w0v;
         for (int j = 0; j < a.nprev; ++j) t = zadd(t, zscal(pre ? rk_kp[j] : a.kprev[j][d], a.cprev[j]));
         a.next[d] = zadd(t, zscal(k, a.cnew));
       }

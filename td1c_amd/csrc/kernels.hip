@@ -1242,8 +1242,11 @@ __global__ void __launch_bounds__(256) k_dtorb(const DtorbArgs a) {
     }
   }
   if (a.kout) a.kout[idx] = k;
+  // the step-start state: read before acc (which it may alias) is written
+  const double2 w0v = a.next ? a.w0[idx] : make_double2(0.0, 0.0);
   if (a.acc) {
     double2 w = a.acc[idx];
+    if (a.w0out) a.w0out[idx] = w;
     w.x = fma(a.dtw, k.x, w.x);
     w.y = fma(a.dtw, k.y, w.y);
     if (a.mask && f >= a.nfcore) {
@@ -1253,7 +1256,7 @@ __global__ void __launch_bounds__(256) k_dtorb(const DtorbArgs a) {
     a.acc[idx] = w;
   }
   if (a.next) {
-    double2 t = a.w0[idx];
+    double2 t = w0v;
     for (int s = 0; s < a.nprev; ++s) {
       const double2 ks = a.kprev[s][idx];
       t.x = fma(a.cprev[s], ks.x, t.x);
@@ -1754,10 +1757,6 @@ void launch_dipole(const DipoleArgs& a, double2* partial, double* out, hipStream
 // ---------------------------------------------------------------------------
 // Small helpers that keep memset/memcpy nodes out of captured graphs.
 // ---------------------------------------------------------------------------
-__global__ void k_copy(double2* __restrict__ y, const double2* __restrict__ x, size_t n) {
-  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) y[q] = x[q];
-}
-
 __global__ void k_zero(double2* __restrict__ y, size_t n) {
   for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
     y[q] = make_double2(0.0, 0.0);
@@ -1765,12 +1764,15 @@ __global__ void k_zero(double2* __restrict__ y, size_t n) {
 
 // Stage field values of the propagation step st->n, then st->n += 1
 // (one thread; tdse_split2 evaluates clfield::get_value per stage).
+// one thread per stage; the clock advances after every thread has read it
 __global__ void k_field_stages(const Pulse p, PropClock* __restrict__ st, const StageNodes nodes, double* __restrict__ lf) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
   const long long n = st->n;
   const double t = st->t0 + (double)n * st->dt;
-  for (int s = 0; s < nodes.ns; ++s) pulse_lfield(p, t + nodes.c[s] * st->dt, lf + 9 * s);
-  st->n = n + 1;
+  const double dt = st->dt;
+  __syncthreads();
+  const int s = threadIdx.x;
+  if (s < nodes.ns) pulse_lfield(p, t + nodes.c[s] * dt, lf + 9 * s);
+  if (s == 0) st->n = n + 1;
 }
 
 __global__ void k_clock_set(PropClock* __restrict__ st, double t0, double dt) {
@@ -1786,10 +1788,6 @@ __global__ void k_to_f32(float2* __restrict__ y, const double2* __restrict__ x, 
 }
 
 static int grid_for(size_t n) { return (int)std::min<size_t>(2048, std::max<size_t>(1, (n + 255) / 256)); }
-
-void launch_copy(double2* y, const double2* x, size_t n, hipStream_t s) {
-  if (n) hipLaunchKernelGGL(k_copy, dim3(grid_for(n)), dim3(256), 0, s, y, x, n);
-}
 
 void launch_to_f32(float2* y, const double2* x, size_t n, hipStream_t s) {
   if (n) hipLaunchKernelGGL(k_to_f32, dim3(grid_for(n)), dim3(256), 0, s, y, x, n);

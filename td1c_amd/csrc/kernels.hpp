@@ -101,7 +101,8 @@ struct DtorbArgs {
   double dtw;
   const double* mask;     // applied to acc after the update (may be null)
   double2* next;          // next stage input (may be null)
-  const double2* w0;
+  const double2* w0;      // step-start state (may alias acc: read before acc is written)
+  double2* w0out;         // copy of acc as read, i.e. the step-start state (may be null)
   int nprev;
   const double2* kprev[kMaxStages];
   double cprev[kMaxStages];
@@ -133,12 +134,14 @@ struct IslandArgs {
   double2* mfc;            // mean-field coefficients (see MfArgs)
   double* ene;             // ene_fcore (input), dcore, core, act, tot
   // Runge-Kutta stage update of the CI part, done after dcic (kout == null: none):
-  // kout = dcic; acc += dtw * dcic; next = w0 + sum cprev[j] kprev[j] + cnew * dcic
+  // kout = dcic; acc += dtw * dcic; next = w0 + sum cprev[j] kprev[j] + cnew * dcic;
+  // w0 may alias acc (read before acc is written); w0out (may be null) gets acc as read
   double2* kout;
   double2* acc;
   double dtw;
   double2* next;
   const double2* w0;
+  double2* w0out;
   int nprev;
   const double2* kprev[kMaxStages];
   double cprev[kMaxStages];
@@ -185,7 +188,6 @@ struct DipoleArgs {
 };
 constexpr int kDipoleMaxBlocks = 4096;
 void launch_dipole(const DipoleArgs& a, double2* partial, double* out, hipStream_t s);
-void launch_copy(double2* y, const double2* x, size_t n, hipStream_t s);
 void launch_to_f32(float2* y, const double2* x, size_t n, hipStream_t s);
 void launch_zero(double2* y, size_t n, hipStream_t s);
 void launch_field_stages(const Pulse& p, PropClock* st, const StageNodes& nodes, double* lf, hipStream_t s);
