@@ -105,8 +105,12 @@ struct td1c_ctx {
   hipStream_t side[2] = {nullptr, nullptr};
   // fork/join events: [0] pair potentials ready, [1] RDM stage done,
   // [2] orbitals on the theta grid, [3] one-body part done, [4] pair Gram
-  // products done
-  hipEvent_t fj[5] = {};
+  // products done, [5] l-space Gram products done, [6] island stage B done
+  // (when it runs on side stream 0)
+  hipEvent_t fj[7] = {};
+  // X built by the orbital-derivative kernel (no core-active block), so the
+  // island stage B runs on side stream 0, off the main stream
+  bool x_local = false;
   int tpath = 0;  // transform form: 0 by grid size, 1 vector, 2 matrix cores
   // device-driven propagation: one captured RK step, replayed per step
   hipStream_t cap_stream = nullptr;
@@ -446,6 +450,7 @@ int build_device(td1c_ctx* c) {
     HIP_TRY(c->pprod.alloc(std::max<size_t>(1, (size_t)2 * c->gp_nL * c->gp_nR)));
   }
   c->gram_blk = 256;
+  c->x_local = S.ncore == S.nfcore || S.nact == 0;
   {
     size_t ent = std::max((size_t)c->gl_nL * c->gl_nR, (size_t)c->gp_nL * c->gp_nR);
     HIP_TRY(c->gram_partial.alloc(std::max<size_t>(1, ent * 2 * c->gram_blk)));
@@ -633,6 +638,7 @@ int enqueue_htotx(td1c_ctx* c, double dtime, const double* d_lf, const double2* 
   }
   HIP_TRY(hipEventRecord(c->fj[3], s2));
   bool pair_gram_done = false;
+  bool waited_rdm = false;  // the main stream has joined the RDM stage
   // mean field
   const size_t ng = (size_t)c->ngrid;
   if (S.nel[2] >= 2 && c->npair > 0) {
@@ -673,6 +679,7 @@ int enqueue_htotx(td1c_ctx* c, double dtime, const double* d_lf, const double2* 
     ma.pj = c->pj.p; ma.pi = c->pi.p; ma.ptype = c->ptype.p; ma.mval = c->mval.p;
     ma.den1 = c->den1.p; ma.den2 = c->den2.p; ma.rden = c->rden.p; ma.mfc = c->mfc.p;
     HIP_TRY(hipStreamWaitEvent(s, c->fj[1], 0));  // RDM stage done
+    waited_rdm = true;
     if (c->mixed) {
       { TIMED("mfprod"); launch_mfprod(c->orbg_f, c->vg_f.p, c->gorbg_f, c->v2orbg_f, ma, s); }
       {
@@ -691,17 +698,32 @@ int enqueue_htotx(td1c_ctx* c, double dtime, const double* d_lf, const double2* 
   if (!pair_gram_done) HIP_TRY(enqueue_pair_gram(c, s, s1));
   // inner products of the l-space results (needs the one-body branch)
   HIP_TRY(hipStreamWaitEvent(s, c->fj[3], 0));
+  // With stage B on side stream 0, the previous hprod's stage B reads lprod
+  // until it finishes; it precedes this hprod's RDM stage on that stream, so
+  // joining the RDM stage orders the l-space products below after it.
+  if (c->x_local && !waited_rdm) HIP_TRY(hipStreamWaitEvent(s, c->fj[1], 0));
   GramArgs g{};
   g.len = nb; g.nL = c->gl_nL; g.nR = c->gl_nR; g.conj_left = 1; g.with_conj_right = 0;
   g.left = c->g_left_l.p; g.right = c->g_right_l.p;
   { TIMED("gram_l"); launch_gram(g, c->gram_partial.p, c->gram_blk, c->lprod.p, s); }
-  // join the side streams (fj[4] follows fj[1] on side stream 0, so one
-  // wait covers the RDM stage and the pair Gram products)
-  HIP_TRY(hipStreamWaitEvent(s, c->fj[4], 0));
   ia.kout = cs.kout; ia.acc = cs.acc; ia.dtw = cs.dtw; ia.next = cs.next; ia.w0 = cs.w0; ia.w0out = cs.w0out;
   ia.nprev = cs.nprev; ia.cnew = cs.cnew;
   for (int j = 0; j < kMaxStages; ++j) { ia.kprev[j] = cs.kprev[j]; ia.cprev[j] = cs.cprev[j]; }
-  { TIMED("island_b"); launch_island_b(ia, s); }
+  if (c->x_local) {
+    // the orbital derivative builds X itself: stage B (energies, CI
+    // derivative and CI stage update) follows the pair Gram products on side
+    // stream 0 once the l-space products exist; callers join it through
+    // join_island
+    HIP_TRY(hipEventRecord(c->fj[5], s));
+    HIP_TRY(hipStreamWaitEvent(s1, c->fj[5], 0));
+    { TIMED_ON(s1, "island_b"); launch_island_b(ia, s1); }
+    HIP_TRY(hipEventRecord(c->fj[6], s1));
+  } else {
+    // join the side streams (fj[4] follows fj[1] on side stream 0, so one
+    // wait covers the RDM stage and the pair Gram products)
+    HIP_TRY(hipStreamWaitEvent(s, c->fj[4], 0));
+    { TIMED("island_b"); launch_island_b(ia, s); }
+  }
   return 0;
 }
 
@@ -711,7 +733,18 @@ DtorbArgs dtorb_args(td1c_ctx* c, double dtime) {
   d.nbas = c->nbas; d.nr = S.nr; d.nfun = S.nfun; d.nfcore = S.nfcore;
   d.mval = c->mval.p; d.h01 = c->h01; d.gorb = c->gorb; d.v2orb = c->v2orb; d.orb = c->orb; d.xmat = c->xmat.p;
   d.tfac = S.icomp == 0 ? make_double2(-dtime, 0.0) : make_double2(0.0, -dtime);
+  d.xlocal = c->x_local ? 1 : 0;
+  d.ncore = S.ncore;
+  d.xfac = S.icomp == 0 ? make_double2(dtime, 0.0) : make_double2(0.0, dtime);
+  d.lprod = c->lprod.p;
+  d.mapH = c->mapH.p; d.mapG = c->mapG.p; d.mapV = c->mapV.p; d.mapF = c->mapF.p;
   return d;
+}
+
+// makes the caller's stream wait for the island stage B when it ran on
+// side stream 0
+hipError_t join_island(td1c_ctx* c, hipStream_t s) {
+  return c->x_local ? hipStreamWaitEvent(s, c->fj[6], 0) : hipSuccess;
 }
 
 // Host banded solve of the L = 0 Poisson operator for the frozen-core
@@ -889,6 +922,7 @@ int td1c_htot(td1c_ctx* c, double dtime, const double* lfield, const double* d_w
   DtorbArgs d = dtorb_args(c, dtime);
   d.kout = reinterpret_cast<double2*>(d_hwfn);
   { TIMED("dtorb"); launch_dtorb(d, s); }
+  HIP_TRY(join_island(c, s));
   HIP_TRY(hipGetLastError());
   return 0;
 }
@@ -1002,6 +1036,7 @@ static int enqueue_rk_step(td1c_ctx* c, int rk_order, double dtime, const double
     { TIMED("dtorb_rk"); launch_dtorb(d, s); }
     if (is + 1 < ns) tin = t;
   }
+  HIP_TRY(join_island(c, s));
   HIP_TRY(hipGetLastError());
   return 0;
 }

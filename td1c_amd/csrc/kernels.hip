@@ -1224,10 +1224,29 @@ __global__ void __launch_bounds__(256) k_gram_final(const double2* __restrict__ 
 //   acc += dtw * k      (when acc != nullptr)
 //   next = w0 + sum_s c_s k_s  (when next != nullptr; k_s of earlier stages in kbuf)
 // ---------------------------------------------------------------------------
+// X(j, i) from the l-space Gram products (hprod_htot_mkxmat_fc/cc/aa1):
+// fc-dyn (F + G + V), core-core and active-active (H + G + V), times xfac
+__device__ double2 x_entry(const DtorbArgs& a, int j, int i) {
+  const double2 z = make_double2(0.0, 0.0);
+  if (i < a.nfcore || a.mval[j] != a.mval[i]) return z;
+  const int e = j * a.nfun + i;
+  const int qh = a.mapH[e], qg = a.mapG[e], qv = a.mapV[e], qf = a.mapF[e];
+  const double2 H = qh >= 0 ? a.lprod[qh] : z, G = qg >= 0 ? a.lprod[qg] : z;
+  const double2 V = qv >= 0 ? a.lprod[qv] : z, F = qf >= 0 ? a.lprod[qf] : z;
+  if (j < a.nfcore) return cmul(cadd(F, cadd(G, V)), a.xfac);
+  if ((j < a.ncore && i < a.ncore) || (j >= a.ncore && i >= a.ncore)) return cmul(cadd(V, cadd(G, H)), a.xfac);
+  return z;
+}
+
 __global__ void __launch_bounds__(256) k_dtorb(const DtorbArgs a) {
   const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t nb = (size_t)a.nbas;
   const int f = blockIdx.y;
+  __shared__ double2 sXc[kMaxFun];  // column f of X when built here
+  if (a.xlocal) {  // block-uniform
+    if ((int)threadIdx.x < a.nfun) sXc[threadIdx.x] = x_entry(a, threadIdx.x, f);
+    __syncthreads();
+  }
   if (q >= nb) return;
   const size_t idx = (size_t)f * nb + q;
   double2 k = make_double2(0.0, 0.0);
@@ -1238,7 +1257,7 @@ __global__ void __launch_bounds__(256) k_dtorb(const DtorbArgs a) {
     k = cmul(h, a.tfac);
     for (int j = 0; j < a.nfun; ++j) {
       if (a.mval[j] != a.mval[f]) continue;
-      cfma(k, a.orb[(size_t)j * nb + q], a.xmat[(size_t)f * a.nfun + j]);
+      cfma(k, a.orb[(size_t)j * nb + q], a.xlocal ? sXc[j] : a.xmat[(size_t)f * a.nfun + j]);
     }
   }
   if (a.kout) a.kout[idx] = k;

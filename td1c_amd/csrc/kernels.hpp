@@ -96,6 +96,13 @@ struct DtorbArgs {
   const int* mval;
   const double2 *h01, *gorb, *v2orb, *orb, *xmat;
   double2 tfac;
+  // xlocal: each block builds its column of X from the l-space Gram products
+  // (fc-dyn, core-core and active-active blocks; no core-active block, i.e.
+  // ncore == nfcore or no active orbitals) instead of reading xmat
+  int xlocal, ncore;
+  double2 xfac;  // X = <j|H|i> * xfac
+  const double2* lprod;
+  const int *mapH, *mapG, *mapV, *mapF;
   double2* kout;          // stage derivative (may be null)
   double2* acc;           // psi += dtw * k (may be null)
   double dtw;
