@@ -9,7 +9,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -53,6 +55,28 @@ struct DevBuf {
     p = nullptr;
   }
 };
+
+// The side streams of a device, created once per process and shared by all
+// contexts on it.  HIP maps streams onto a few hardware queues in creation
+// order; one fixed pair keeps every context's branches on the same queues
+// (per-context streams made the mapping, and the run time, depend on how
+// many contexts had been created before).
+hipError_t side_streams(int device, hipStream_t out[2]) {
+  static std::mutex mu;
+  static std::map<int, std::pair<hipStream_t, hipStream_t>> pool;
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = pool.find(device);
+  if (it == pool.end()) {
+    hipStream_t a = nullptr, b = nullptr;
+    hipError_t e = hipStreamCreateWithFlags(&a, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b, hipStreamNonBlocking);
+    if (e != hipSuccess) return e;
+    it = pool.emplace(device, std::make_pair(a, b)).first;
+  }
+  out[0] = it->second.first;
+  out[1] = it->second.second;
+  return hipSuccess;
+}
 
 }  // namespace td1c
 
@@ -101,7 +125,9 @@ struct td1c_ctx {
   DevBuf<PropClock> clock;
   bool wfn0_set = false;
   // two side streams (default priority) for the independent branches of an
-  // hprod, joined back by events; the events carry no timing
+  // hprod, joined back by events; the events carry no timing.  The streams
+  // belong to the process (side_streams) and are shared by every context on
+  // the device.
   hipStream_t side[2] = {nullptr, nullptr};
   // fork/join events: [0] pair potentials ready, [1] RDM stage done,
   // [2] orbitals on the theta grid, [3] one-body part done, [4] pair Gram
@@ -154,8 +180,6 @@ struct td1c_ctx {
     if (cap_stream) (void)hipStreamDestroy(cap_stream);
     for (auto& e : fj)
       if (e) (void)hipEventDestroy(e);
-    for (auto& q : side)
-      if (q) (void)hipStreamDestroy(q);
     if (clock.p) (void)hipFree(clock.p);
   }
 };
@@ -488,7 +512,7 @@ int build_device(td1c_ctx* c) {
   HIP_TRY(c->rk_k.alloc(wsz * kMaxStages));
   HIP_TRY(c->lf_dev.alloc(9 * kMaxStages));
   HIP_TRY(c->clock.alloc(1));
-  for (auto& q : c->side) HIP_TRY(hipStreamCreateWithFlags(&q, hipStreamNonBlocking));
+  HIP_TRY(side_streams(c->device, c->side));
   // fork/join events order kernels of one device only: device-scope release
   // and acquire suffice, no system-scope fence (cache write-back) per record
   for (auto& e : c->fj) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming | hipEventDisableSystemFence));
