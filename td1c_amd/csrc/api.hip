@@ -519,6 +519,16 @@ int build_device(td1c_ctx* c) {
   return 0;
 }
 
+// With X built in the orbital derivative, the l-space Gram products are left
+// as partial sums (no final-sum launch on the main stream): the number of
+// partials per entry, else 0.
+int l_gram_partials(td1c_ctx* c) {
+  if (!c->x_local) return 0;
+  GramArgs g{};
+  g.len = (size_t)c->nbas;  // the l-space products run over the orbital basis
+  return gram_blocks(g, c->gram_blk);
+}
+
 IslandArgs island_args(td1c_ctx* c, const double2* cic, double dtime) {
   Setup& S = c->S;
   IslandArgs a{};
@@ -531,6 +541,8 @@ IslandArgs island_args(td1c_ctx* c, const double2* cic, double dtime) {
   a.cic = cic; a.ci_work = c->ci_work.p;
   a.den1 = c->den1.p; a.den2 = c->den2.p; a.rden = c->rden.p; a.rrden = c->rrden.p;
   a.lprod = c->lprod.p;
+  a.lpart = c->gram_partial.p;
+  a.lnblk = l_gram_partials(c);
   a.mapH = c->mapH.p; a.mapG = c->mapG.p; a.mapV = c->mapV.p; a.mapF = c->mapF.p;
   a.pprod = c->pprod.p; a.map2e = c->map2e.p;
   a.int1e = c->int1e.p; a.int2e = c->int2e.p; a.xmat = c->xmat.p; a.dcic = c->dcic.p;
@@ -729,7 +741,9 @@ int enqueue_htotx(td1c_ctx* c, double dtime, const double* d_lf, const double2* 
   GramArgs g{};
   g.len = nb; g.nL = c->gl_nL; g.nR = c->gl_nR; g.conj_left = 1; g.with_conj_right = 0;
   g.left = c->g_left_l.p; g.right = c->g_right_l.p;
-  { TIMED("gram_l"); launch_gram(g, c->gram_partial.p, c->gram_blk, c->lprod.p, s); }
+  // (partial sums only when X is built in the orbital derivative: it and
+  // stage B sum what they need)
+  { TIMED("gram_l"); launch_gram(g, c->gram_partial.p, c->gram_blk, c->x_local ? nullptr : c->lprod.p, s); }
   ia.kout = cs.kout; ia.acc = cs.acc; ia.dtw = cs.dtw; ia.next = cs.next; ia.w0 = cs.w0; ia.w0out = cs.w0out;
   ia.nprev = cs.nprev; ia.cnew = cs.cnew;
   for (int j = 0; j < kMaxStages; ++j) { ia.kprev[j] = cs.kprev[j]; ia.cprev[j] = cs.cprev[j]; }
@@ -762,6 +776,8 @@ DtorbArgs dtorb_args(td1c_ctx* c, double dtime) {
   d.xfac = S.icomp == 0 ? make_double2(dtime, 0.0) : make_double2(0.0, dtime);
   d.lprod = c->lprod.p;
   d.mapH = c->mapH.p; d.mapG = c->mapG.p; d.mapV = c->mapV.p; d.mapF = c->mapF.p;
+  d.lpart = c->gram_partial.p;
+  d.lnblk = l_gram_partials(c);
   return d;
 }
 

@@ -241,6 +241,7 @@ __global__ void __launch_bounds__(kT) k_island_b(const IslandArgs a) {
   __shared__ double s_red[kT / 64];
   __shared__ double s_eact;
   __shared__ int s_mval[kMaxFun];
+  __shared__ int s_q[4 * kF2];  // the four l-space product maps (partial-sum mode)
   const int nf = a.nfun, nfc = a.nfcore, ncore = a.ncore, na = a.nact;
   const int tid = threadIdx.x;
   // every global input first (the index maps, then what they point at), so
@@ -248,13 +249,42 @@ __global__ void __launch_bounds__(kT) k_island_b(const IslandArgs a) {
   double2* I2 = a.int2e;
   if (a.i2_lds) I2 = reinterpret_cast<double2*>(dyn + a.ci_lds_off);
   for (int e = tid; e < nf; e += kT) s_mval[e] = a.mval[e];
-  for (int e = tid; e < nf * nf; e += kT) {
-    const int qh = a.mapH[e], qg = a.mapG[e], qv = a.mapV[e], qf = a.mapF[e];
-    s_H[e] = qh >= 0 ? a.lprod[qh] : zero2();
-    s_G[e] = qg >= 0 ? a.lprod[qg] : zero2();
-    s_V[e] = qv >= 0 ? a.lprod[qv] : zero2();
-    s_F[e] = qf >= 0 ? a.lprod[qf] : zero2();
-    s_X[e] = zero2();
+  const int nf2 = nf * nf;
+  if (a.lnblk > 0) {
+    // the l-space products from their partial sums: the maps first, then
+    // four lanes per referenced entry (quads stay together in the loop)
+    for (int e = tid; e < nf2; e += kT) {
+      s_q[e] = a.mapH[e];
+      s_q[nf2 + e] = a.mapG[e];
+      s_q[2 * nf2 + e] = a.mapV[e];
+      s_q[3 * nf2 + e] = a.mapF[e];
+      s_X[e] = zero2();
+    }
+    __syncthreads();
+    for (int slot = tid >> 2; slot < 4 * nf2; slot += kT / 4) {
+      const int q = s_q[slot], sub = tid & 3;
+      double2 acc = zero2();
+      if (q >= 0)
+        for (int b = sub; b < a.lnblk; b += 4) acc = zadd(acc, a.lpart[(size_t)q * a.lnblk + b]);
+      acc.x += __shfl_xor(acc.x, 1, 64);
+      acc.y += __shfl_xor(acc.y, 1, 64);
+      acc.x += __shfl_xor(acc.x, 2, 64);
+      acc.y += __shfl_xor(acc.y, 2, 64);
+      if (sub == 0) {
+        const int w = slot / nf2, e = slot - w * nf2;
+        double2* dst = w == 0 ? s_H : w == 1 ? s_G : w == 2 ? s_V : s_F;
+        dst[e] = acc;
+      }
+    }
+  } else {
+    for (int e = tid; e < nf2; e += kT) {
+      const int qh = a.mapH[e], qg = a.mapG[e], qv = a.mapV[e], qf = a.mapF[e];
+      s_H[e] = qh >= 0 ? a.lprod[qh] : zero2();
+      s_G[e] = qg >= 0 ? a.lprod[qg] : zero2();
+      s_V[e] = qv >= 0 ? a.lprod[qv] : zero2();
+      s_F[e] = qf >= 0 ? a.lprod[qf] : zero2();
+      s_X[e] = zero2();
+    }
   }
   for (int e = tid; e < na * na; e += kT) {
     s_d1[e] = a.den1[e];

@@ -1225,14 +1225,26 @@ __global__ void __launch_bounds__(256) k_gram_final(const double2* __restrict__ 
 //   next = w0 + sum_s c_s k_s  (when next != nullptr; k_s of earlier stages in kbuf)
 // ---------------------------------------------------------------------------
 // X(j, i) from the l-space Gram products (hprod_htot_mkxmat_fc/cc/aa1):
-// fc-dyn (F + G + V), core-core and active-active (H + G + V), times xfac
-__device__ double2 x_entry(const DtorbArgs& a, int j, int i) {
+// fc-dyn (F + G + V), core-core and active-active (H + G + V), times xfac.
+// sP: the products of column i summed from the partials ([H, G, V, F][j]),
+// when lnblk > 0.
+__device__ double2 x_entry(const DtorbArgs& a, int j, int i, const double2* sP) {
   const double2 z = make_double2(0.0, 0.0);
   if (i < a.nfcore || a.mval[j] != a.mval[i]) return z;
-  const int e = j * a.nfun + i;
-  const int qh = a.mapH[e], qg = a.mapG[e], qv = a.mapV[e], qf = a.mapF[e];
-  const double2 H = qh >= 0 ? a.lprod[qh] : z, G = qg >= 0 ? a.lprod[qg] : z;
-  const double2 V = qv >= 0 ? a.lprod[qv] : z, F = qf >= 0 ? a.lprod[qf] : z;
+  double2 H, G, V, F;
+  if (a.lnblk > 0) {
+    H = sP[j];
+    G = sP[a.nfun + j];
+    V = sP[2 * a.nfun + j];
+    F = sP[3 * a.nfun + j];
+  } else {
+    const int e = j * a.nfun + i;
+    const int qh = a.mapH[e], qg = a.mapG[e], qv = a.mapV[e], qf = a.mapF[e];
+    H = qh >= 0 ? a.lprod[qh] : z;
+    G = qg >= 0 ? a.lprod[qg] : z;
+    V = qv >= 0 ? a.lprod[qv] : z;
+    F = qf >= 0 ? a.lprod[qf] : z;
+  }
   if (j < a.nfcore) return cmul(cadd(F, cadd(G, V)), a.xfac);
   if ((j < a.ncore && i < a.ncore) || (j >= a.ncore && i >= a.ncore)) return cmul(cadd(V, cadd(G, H)), a.xfac);
   return z;
@@ -1242,9 +1254,28 @@ __global__ void __launch_bounds__(256) k_dtorb(const DtorbArgs a) {
   const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t nb = (size_t)a.nbas;
   const int f = blockIdx.y;
-  __shared__ double2 sXc[kMaxFun];  // column f of X when built here
+  __shared__ double2 sXc[kMaxFun];     // column f of X when built here
+  __shared__ double2 sP[4 * kMaxFun];  // its Gram products, summed from the partials
   if (a.xlocal) {  // block-uniform
-    if ((int)threadIdx.x < a.nfun) sXc[threadIdx.x] = x_entry(a, threadIdx.x, f);
+    if (a.lnblk > 0) {
+      // eight lanes per product: strided partial sums, then three exchanges
+      const int sub = threadIdx.x & 7;
+      for (int slot = threadIdx.x >> 3; slot < 4 * a.nfun; slot += 32) {
+        const int w = slot / a.nfun, j = slot - w * a.nfun;
+        const int* map = w == 0 ? a.mapH : w == 1 ? a.mapG : w == 2 ? a.mapV : a.mapF;
+        const int qq = map[j * a.nfun + f];
+        double2 acc = make_double2(0.0, 0.0);
+        if (qq >= 0)
+          for (int b = sub; b < a.lnblk; b += 8) acc = cadd(acc, a.lpart[(size_t)qq * a.lnblk + b]);
+        for (int o = 1; o < 8; o <<= 1) {
+          acc.x += __shfl_xor(acc.x, o, 64);
+          acc.y += __shfl_xor(acc.y, o, 64);
+        }
+        if (sub == 0) sP[slot] = acc;
+      }
+      __syncthreads();
+    }
+    if ((int)threadIdx.x < a.nfun) sXc[threadIdx.x] = x_entry(a, threadIdx.x, f, sP);
     __syncthreads();
   }
   if (q >= nb) return;
@@ -1687,10 +1718,14 @@ template void launch_mfprod<double2>(const double2*, const double2*, double2*, d
 template void launch_mfprod<float2>(const float2*, const float2*, float2*, float2*, const MfArgs&, hipStream_t);
 
 
-void launch_gram(const GramArgs& a, double2* partial, int maxblk, double2* out, hipStream_t s) {
+int gram_blocks(const GramArgs& a, int maxblk) {
   // about 4 index values per thread (two trips of two), at most maxblk
   // blocks along the index
-  const int nblk = (int)std::max<size_t>(1, std::min<size_t>(maxblk, (a.len + 1023) / 1024));
+  return (int)std::max<size_t>(1, std::min<size_t>(maxblk, (a.len + 1023) / 1024));
+}
+
+void launch_gram(const GramArgs& a, double2* partial, int maxblk, double2* out, hipStream_t s) {
+  const int nblk = gram_blocks(a, maxblk);
   // register tiles of 8 complex sums per thread (16 doubles), which keep
   // the kernel at many wavefronts per SIMD
   if (a.with_conj_right) {
@@ -1704,6 +1739,7 @@ void launch_gram(const GramArgs& a, double2* partial, int maxblk, double2* out, 
     if (a.f32) hipLaunchKernelGGL((k_gram<TA, TB, false, float2>), grid, dim3(256), 0, s, a, partial, nblk);
     else hipLaunchKernelGGL((k_gram<TA, TB, false, double2>), grid, dim3(256), 0, s, a, partial, nblk);
   }
+  if (out == nullptr) return;
   int nent = a.nL * a.nR * 2;
   hipLaunchKernelGGL(k_gram_final, dim3((nent + 3) / 4), dim3(256), 0, s, partial, nblk, nent, out,
                      a.with_conj_right);

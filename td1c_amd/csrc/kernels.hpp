@@ -103,6 +103,10 @@ struct DtorbArgs {
   double2 xfac;  // X = <j|H|i> * xfac
   const double2* lprod;
   const int *mapH, *mapG, *mapV, *mapF;
+  // lnblk > 0: the l-space products as launch_gram's partial sums
+  // (entry q at lpart[q * lnblk ...]), summed here instead of read from lprod
+  const double2* lpart;
+  int lnblk;
   double2* kout;          // stage derivative (may be null)
   double2* acc;           // psi += dtw * k (may be null)
   double dtw;
@@ -134,6 +138,10 @@ struct IslandArgs {
   // stage B inputs: Gram results and index maps ([nfun*nfun], -1 = absent)
   const double2* lprod;
   const int *mapH, *mapG, *mapV, *mapF;
+  // lnblk > 0: the l-space products as launch_gram's partial sums
+  // (entry q at lpart[q * lnblk ...]), summed here instead of read from lprod
+  const double2* lpart;
+  int lnblk;
   const double2* pprod;
   const int* map2e;        // [nact^4] (i,j,k,l) with i >= j, -1 otherwise
   // stage B outputs
@@ -183,7 +191,10 @@ template <class T>  // rho and v complex128 or complex64; the solve is FP64
 void launch_poisson(const T* rho, double2* xb, T* v, const PoissonArgs& a, hipStream_t s);
 template <class T>  // theta-grid arrays complex128 or complex64 (arithmetic in the same)
 void launch_mfprod(const T* orbg, const T* vg, T* gorbg, T* v2orbg, const MfArgs& a, hipStream_t s);
-void launch_gram(const GramArgs& a, double2* partial, int nblk, double2* out, hipStream_t s);
+// out == nullptr: the partial sums only (partial[q * nblk + b], q = entry
+// * 2 + c, nblk = gram_blocks(a, maxblk)), summed by the consumer
+void launch_gram(const GramArgs& a, double2* partial, int maxblk, double2* out, hipStream_t s);
+int gram_blocks(const GramArgs& a, int maxblk);
 // Length-form dipole <z> = 2 sum_core <i|z|i> + sum_ab conj(den1(b,a)) <b|z|a>
 struct DipoleArgs {
   int nr, L1, nfun, nfcore, ncore, nact, nradfc, mmax1;
