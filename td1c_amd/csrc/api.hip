@@ -840,7 +840,12 @@ int td1c_ctx_create(const td1c_problem* prob, td1c_ctx** out) {
 }
 
 int td1c_ctx_destroy(td1c_ctx* ctx) {
+  if (!ctx) return 0;
   if (g_bound == ctx) g_bound = nullptr;
+  // the side streams outlive the context: let its work on them finish
+  // before the buffers go
+  for (auto& q : ctx->side)
+    if (q) (void)hipStreamSynchronize(q);
   delete ctx;
   return 0;
 }
