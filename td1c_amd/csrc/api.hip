@@ -1030,8 +1030,11 @@ static int rk_tableau(int order, int& ns, double* nodes, double* w, double (*a)[
 
 // One Runge-Kutta step of the whole wave function on stream s, the stage
 // field values read from d_lf (device, [stage][9]).
+// join: make s wait for the island stage B at the end (always, unless the
+// caller enqueues further steps and joins after the last one: the CI data
+// of consecutive steps stays on side stream 0, the orbitals on s)
 static int enqueue_rk_step(td1c_ctx* c, int rk_order, double dtime, const double* d_lf, double2* wfn, int apply_mask,
-                           hipStream_t s) {
+                           hipStream_t s, bool join = true) {
   int ns;
   double nodes[kMaxStages], w[kMaxStages], a[kMaxStages][kMaxStages];
   if (rk_tableau(rk_order, ns, nodes, w, a)) return fail(-1, "rk_order must be 1..4");
@@ -1081,7 +1084,7 @@ static int enqueue_rk_step(td1c_ctx* c, int rk_order, double dtime, const double
     { TIMED("dtorb_rk"); launch_dtorb(d, s); }
     if (is + 1 < ns) tin = t;
   }
-  HIP_TRY(join_island(c, s));
+  if (join) HIP_TRY(join_island(c, s));
   HIP_TRY(hipGetLastError());
   return 0;
 }
@@ -1119,7 +1122,8 @@ int td1c_propagate(td1c_ctx* c, const td1c_field* field, int rk_order, double t0
     launch_clock_set(c->clock.p, t0, dtime, s);
     for (int i = 0; i < nstep; ++i) {
       launch_field_stages(p, c->clock.p, sn, c->lf_dev.p, s);
-      int rc = enqueue_rk_step(c, rk_order, dtime, c->lf_dev.p, reinterpret_cast<double2*>(d_wfn), apply_mask, s);
+      int rc = enqueue_rk_step(c, rk_order, dtime, c->lf_dev.p, reinterpret_cast<double2*>(d_wfn), apply_mask, s,
+                               i == nstep - 1);
       if (rc) return rc;
     }
     HIP_TRY(hipGetLastError());
