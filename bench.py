@@ -47,7 +47,7 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "ne_l31_traffic.json")
 TIMING_KERNELS = {
     "poisson": ["k_poisson_bridge_lds", "k_poisson_interior"],
     "onebody": ["k_onebody_el"],
-    "mfprod": ["k_mfprod_t"],
+    "mfprod": ["k_mfprod_quad"],
     "island_a": ["k_island_a"],
     "island_b": ["k_island_b"],
     "dtorb_rk": ["k_dtorb"],
