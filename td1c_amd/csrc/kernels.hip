@@ -1488,12 +1488,15 @@ __global__ void __launch_bounds__(BS) k_poisson_bridge_lds(const T* __restrict__
   }
   if ((tid & 63) == 0) sRed[tid >> 6] = m;
   __syncthreads();
-  // Schur right-hand side of bridge k (thread k; nfe - 1 <= BS):
-  // b_B - W_l(k+1) . b_I(k+1) - W_r(k) . b_I(k), the two elements around it
-  if (tid < nb) {
-    const int bl = tid * n, br = (tid + 1) * n;  // first interior rows of elements k and k+1
+  // Schur right-hand side of bridge k (four lanes per bridge):
+  // b_B - W_l(k+1) . b_I(k+1) - W_r(k) . b_I(k), the two elements around it;
+  // each lane takes every fourth row, the quad adds the parts (a quad stays
+  // together in the loop)
+  for (int t4 = tid; t4 < 4 * nb; t4 += BS) {
+    const int kb = t4 >> 2, sub = t4 & 3;
+    const int bl = kb * n, br = (kb + 1) * n;  // first interior rows of elements k and k+1
     double2 cl = make_double2(0.0, 0.0), cr = make_double2(0.0, 0.0);
-    for (int i = 0; i < ni; ++i) {
+    for (int i = sub; i < ni; i += 4) {
       const double2 b0 = sB[bl + i], b1 = sB[br + i];
       const double wr = a.Wr[lo + bl + i], wl = a.Wl[lo + br + i];
       cr.x = fma(wr, b0.x, cr.x);
@@ -1501,10 +1504,18 @@ __global__ void __launch_bounds__(BS) k_poisson_bridge_lds(const T* __restrict__
       cl.x = fma(wl, b1.x, cl.x);
       cl.y = fma(wl, b1.y, cl.y);
     }
-    double2 sk = sB[br - 1];
-    sk = csub(sk, cl);
-    sk = csub(sk, cr);
-    sY[tid] = sk;
+    for (int o = 1; o < 4; o <<= 1) {
+      cl.x += __shfl_xor(cl.x, o, 64);
+      cl.y += __shfl_xor(cl.y, o, 64);
+      cr.x += __shfl_xor(cr.x, o, 64);
+      cr.y += __shfl_xor(cr.y, o, 64);
+    }
+    if (sub == 0) {
+      double2 sk = sB[br - 1];
+      sk = csub(sk, cl);
+      sk = csub(sk, cr);
+      sY[kb] = sk;
+    }
   }
   __syncthreads();
   double2 msum = make_double2(0.0, 0.0);
