@@ -1010,10 +1010,10 @@ __global__ void __launch_bounds__(256) k_mfprod_quad(const A2* __restrict__ orbg
                                                      const MfArgs a) {
   constexpr int NA = 4, NF = NFC + NA, NCORE = NFC, NDP = NA * (NA + 1) / 2;
   __shared__ A2 s_act2[NA * NA * NA * NA], s_rden[NA * NA];
-  s_act2[threadIdx.x] = cto<A2>(a.mfc[threadIdx.x]);  // 256 threads, 256 coefficients
+  for (int e = threadIdx.x; e < NA * NA * NA * NA; e += blockDim.x) s_act2[e] = cto<A2>(a.mfc[e]);
   if (threadIdx.x < NA * NA) s_rden[threadIdx.x] = cto<A2>(a.mfc[NA * NA * NA * NA + NA * NA + threadIdx.x]);
   const int q = threadIdx.x & 3;
-  const int r = blockIdx.x * 64 + (threadIdx.x >> 2);
+  const int r = blockIdx.x * (blockDim.x >> 2) + (threadIdx.x >> 2);
   const int th = blockIdx.y;
   const bool ok = r < a.nr;
   const size_t ng = (size_t)a.nlat * a.nr;
@@ -1705,9 +1705,11 @@ void launch_mfprod(const T* orbg, const T* vg, T* gorbg, T* v2orbg, const MfArgs
     hipLaunchKernelGGL((k_mfprod_t<NFC, NF, NA, T>), grid, block, 0, s, orbg, vg, gorbg, v2orbg, a);  \
     return;
   if (a.nact == 4 && a.nfun == a.nfcore + 4 && a.ncore == a.nfcore && a.nfcore <= 1) {
-    dim3 gq((a.nr + 63) / 64, a.nlat);
-    if (a.nfcore == 1) hipLaunchKernelGGL((k_mfprod_quad<1, T>), gq, block, 0, s, orbg, vg, gorbg, v2orbg, a);
-    else hipLaunchKernelGGL((k_mfprod_quad<0, T>), gq, block, 0, s, orbg, vg, gorbg, v2orbg, a);
+    // 32 points per block in complex64, 64 in FP64 (measured)
+    const int bt = std::is_same<T, float2>::value ? 128 : 256;
+    dim3 gq((a.nr + bt / 4 - 1) / (bt / 4), a.nlat);
+    if (a.nfcore == 1) hipLaunchKernelGGL((k_mfprod_quad<1, T>), gq, dim3(bt), 0, s, orbg, vg, gorbg, v2orbg, a);
+    else hipLaunchKernelGGL((k_mfprod_quad<0, T>), gq, dim3(bt), 0, s, orbg, vg, gorbg, v2orbg, a);
     return;
   }
   switch (key) {
