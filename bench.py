@@ -163,8 +163,10 @@ def cpu_baseline(prob, wfn, lfield, seconds: float = 12.0) -> dict:
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
+    # ~0.5 s of timed steps per path after ~0.1 s of warm-up, so that the
+    # clocks have settled and one slow launch does not move the rate
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--dt", type=float, default=1.0e-6)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)

@@ -278,3 +278,32 @@ def test_mixed_propagation_dipole_error(gpu_lib):
         dips[prec] = np.array(series)
     err = np.abs(dips[1] - dips[0]).max()
     assert err < 1e-6, (err, dips)
+
+
+def test_two_contexts_interleaved(gpu_lib):
+    """Two contexts (one with the island stage B on the side stream, one
+    without) used alternately on one stream: the side streams are shared by
+    every context on the device, and each result still matches the oracle."""
+    import torch
+    import td1c_amd
+    from oracle.oracle import Oracle
+
+    setups = []
+    for mk, lf in ((C.ne_small, C.LF_VEL), (C.cas_fc_small, C.LF_VEL)):
+        prob = mk()
+        o = Oracle(prob)
+        w = C.wfn_for(prob, o.lcic)
+        o.set_wfn0(w)
+        hp = td1c_amd.HProd(prob)
+        hp.set_wfn0(w)
+        setups.append((o.htot(1.0, lf, w), hp, hp.to_device(w), lf, o.norb))
+    outs = [[], []]
+    for _ in range(3):
+        for k, (_, hp, d, lf, _) in enumerate(setups):
+            outs[k].append(hp.htot(1.0, lf, d))
+    torch.cuda.synchronize()
+    for k, (ref, _, _, _, n) in enumerate(setups):
+        for out in outs[k]:
+            got = out.cpu().numpy()
+            assert _rel(got[:n], ref[:n]) < RTOL, (k, _rel(got[:n], ref[:n]))
+            assert _rel(got[n:], ref[n:]) < 1e-10 or np.linalg.norm(ref[n:]) < 1e-12
