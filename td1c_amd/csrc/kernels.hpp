@@ -204,7 +204,7 @@ struct DipoleArgs {
   const double2* orb;     // orbitals as hprod_orbin builds them
   const double2* den1;    // [nact][nact]
 };
-constexpr int kDipoleMaxBlocks = 4096;
+// partial: one entry per block, ceil(nr / 256) * L1
 void launch_dipole(const DipoleArgs& a, double2* partial, double* out, hipStream_t s);
 void launch_to_f32(float2* y, const double2* x, size_t n, hipStream_t s);
 void launch_zero(double2* y, size_t n, hipStream_t s);

@@ -125,7 +125,7 @@ def make_wfn(prob: Problem, seed: int, lcic: int = 1) -> np.ndarray:
     return np.concatenate([orbs.ravel(), cic])
 
 
-def he_c1_wfn(prob: Problem, seed: int = 1007) -> np.ndarray:
+def he_l15_wfn(prob: Problem, seed: int = 1007) -> np.ndarray:
     """He orbital r exp(-r) times seeded per-l complex Gaussian coefficients
     with |c_l| ~ 2^-l, normalised."""
     rng = np.random.Generator(np.random.PCG64(seed))

@@ -60,15 +60,15 @@ def test_htot_matches_oracle(gpu_lib, name, lf):
     np.testing.assert_allclose(e[1:], eref[1:], rtol=RTOL, atol=1e-11)
 
 
-def test_he_full_size_c1(gpu_lib):
-    """BASELINE case C1: He, 40 x 21 on [0,40], lmax1 15, static E = 0.05."""
+def test_he_l15_full_size(gpu_lib):
+    """Full-size He workload: 40 x 21 on [0,40], lmax1 15, static E = 0.05."""
     from td1c_amd import workloads as W
     import torch
     import td1c_amd
     from oracle.oracle import Oracle
 
     prob = W.he_l15()
-    w = W.he_c1_wfn(prob)
+    w = W.he_l15_wfn(prob)
     o = Oracle(prob)
     ref = o.htot(1.0, C.LF_LEN, w)
     hp = td1c_amd.HProd(prob)
