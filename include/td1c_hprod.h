@@ -148,10 +148,23 @@ int td1c_get_energies(td1c_ctx* ctx, double ene[5]);
 /* Length-form dipole <z> of a device wave function, written to d_out (one
  * double in device memory) on `stream`: hprod_dipole's dip(1)
  * (src/hprod/hprod_dipole.F90:2-146), i.e. hprod_trace1 of z with the 1-RDM,
- * cores counted twice, frozen cores with the velocity-gauge phase of lfield.
- * The velocity and acceleration forms are not provided. */
+ * cores counted twice, frozen cores with the velocity-gauge phase of lfield. */
 int td1c_dipole(td1c_ctx* ctx, const double* lfield, const double* d_wfn,
                 const double* d_cic, double* d_out, void* stream);
+
+/* Length, velocity and acceleration forms of the dipole, written to d_out
+ * (three doubles in device memory) on `stream`: hprod_dipole's Ehrenfest
+ * terms (src/hprod/hprod_dipole.F90:51-72) d_out = {dip(1), vel(1), acc(1)},
+ * the 1-RDM traces (hprod_trace1) of the property vectors of
+ * hprod_zprod_all, hprod_pzprod_all and hprod_azprod_all at zfac = 1 (bare
+ * nucleus: a_z = -Z cos(theta) / r^2), plus A_z N in the velocity gauge for
+ * vel and -E_z N for acc (N = all electrons).  Without frozen cores these
+ * are hprod_dipole's totals dip(3), vel(3), acc(3).  With frozen cores the
+ * reference adds correction terms vel(2), acc(2) built on the generalised
+ * Fock vector of hprod_gfock (src/hprod/hprod_dipole.F90:82-125); they are
+ * not included. */
+int td1c_dipole3(td1c_ctx* ctx, const double* lfield, const double* d_wfn,
+                 const double* d_cic, double* d_out, void* stream);
 
 /* One explicit Runge-Kutta step of the whole wave function (orbitals then
  * CI) held on device: clcrk::prop with rk_order 1..4 ("standard").

@@ -54,6 +54,8 @@ def lib():
         L.orc_energies.argtypes = [vp, vp]
         L.orc_dipole.argtypes = [vp, vp, vp, vp]
         L.orc_dipole.restype = ctypes.c_double
+        L.orc_dipole3.argtypes = [vp, vp, vp, vp, vp]
+        L.orc_dipole3.restype = None
         L.orc_band_chol.argtypes = [ctypes.c_int, ctypes.c_int, vp]
         L.orc_band_chol.restype = ctypes.c_int
         L.orc_band_chol_solve.argtypes = [ctypes.c_int, ctypes.c_int, vp, vp]
@@ -136,6 +138,15 @@ class Oracle:
         w = np.ascontiguousarray(wfn, dtype=np.complex128)
         lf = np.ascontiguousarray(lfield, dtype=np.float64)
         return float(lib().orc_dipole(self.h, _p(lf), _p(w[: self.norb]), _p(w[self.norb:])))
+
+    def dipole3(self, lfield, wfn: np.ndarray) -> np.ndarray:
+        """hprod_dipole's Ehrenfest terms [dip(1), vel(1), acc(1)]: length,
+        velocity and acceleration forms (frozen-core corrections excluded)."""
+        w = np.ascontiguousarray(wfn, dtype=np.complex128)
+        lf = np.ascontiguousarray(lfield, dtype=np.float64)
+        out = np.zeros(3)
+        lib().orc_dipole3(self.h, _p(lf), _p(w[: self.norb]), _p(w[self.norb:]), _p(out))
+        return out
 
     def energies(self) -> np.ndarray:
         e = np.zeros(5)

@@ -1673,17 +1673,16 @@ const void* orc_table(const orc* o, const char* name) {
  * the velocity-gauge phase), den1 from the CI, z on every orbital
  * (hprod_zprod_all with zfac = 1) and hprod_trace1 with the frozen cores
  * included (src/hprod/hprod_trace1.F90:2-88). */
-double orc_dipole(orc* o, const double* lfield, const zc* wfn, const zc* cic) {
-  int nf = o->nfun, nb = o->nbas, nfc = o->nfcore, ncore = o->ncore, na = o->nact, nr = o->nr;
-  orc_htotx(o, 1.0, lfield, wfn, cic); /* orbin + den1 */
-  zc* zo = xcalloc((size_t)nb * nf, sizeof(zc));
-  if (nfc > 0) zprod(o, 1.0, o->orb, zo, 0, nfc, o->nradfc);
-  zprod(o, 1.0, o->orb, zo, nfc, nf, nr);
+/* hprod_trace1 (src/hprod/hprod_trace1.F90:2-88), dofc = true, ecs off:
+ * 2 sum_core <i|p_i> (frozen cores over r < nradfc) + sum_ab conj(D(b,a)) <b|p_a>
+ * on the orbitals of the last orbin (o->orb) */
+static zc trace1(const orc* o, const zc* pw) {
+  int nb = o->nbas, nfc = o->nfcore, ncore = o->ncore, na = o->nact, nr = o->nr;
   zc t = 0;
   for (int ifun = 0; ifun < ncore; ++ifun) {
     int dim = ifun < nfc ? o->nradfc : nr;
     for (int l = abs(o->mval[ifun]); l <= o->lmax1; ++l)
-      for (int i = 0; i < dim; ++i) t += conj(o->orb[OIDX(i, l, ifun)]) * zo[OIDX(i, l, ifun)];
+      for (int i = 0; i < dim; ++i) t += conj(o->orb[OIDX(i, l, ifun)]) * pw[OIDX(i, l, ifun)];
   }
   t *= 2.0;
   zc* tw = xcalloc((size_t)nb, sizeof(zc));
@@ -1697,11 +1696,70 @@ double orc_dipole(orc* o, const double* lfield, const zc* wfn, const zc* cic) {
         for (int i = 0; i < nr; ++i) tw[(size_t)l * nr + i] += o->orb[OIDX(i, l, jfun)] * o->den1[ja + na * ia];
     }
     for (int l = 0; l <= o->lmax1; ++l)
-      for (int i = 0; i < nr; ++i) t += conj(tw[(size_t)l * nr + i]) * zo[OIDX(i, l, ifun)];
+      for (int i = 0; i < nr; ++i) t += conj(tw[(size_t)l * nr + i]) * pw[OIDX(i, l, ifun)];
   }
   free(tw);
+  return t;
+}
+
+/* hprod_azprodp (src/hprod/hprod_azprod.F90:106-136) with bas_azfac of a
+ * bare nucleus (src/bas/bas_gen_azfac.F90, the plain branch):
+ * azfac(r,l,m) = -alph(l,m) Z / r^2 */
+static void azprod(orc* o, zc zf, const zc* wfn, zc* hwfn, int f0, int f1, int dim) {
+  for (int ifun = f0; ifun < f1; ++ifun) {
+    int m = o->mval[ifun];
+    for (int l = abs(m); l <= o->lmax1 - 1; ++l) {
+      double az = -ALPH(l, m) * o->znuc;
+      for (int i = 0; i < dim; ++i) {
+        double r = o->xrad[i + 1];
+        zc tmp = zf * (az / (r * r));
+        hwfn[OIDX(i, l, ifun)] += wfn[OIDX(i, l + 1, ifun)] * tmp;
+        hwfn[OIDX(i, l + 1, ifun)] += wfn[OIDX(i, l, ifun)] * tmp;
+      }
+    }
+  }
+}
+
+double orc_dipole(orc* o, const double* lfield, const zc* wfn, const zc* cic) {
+  int nf = o->nfun, nb = o->nbas, nfc = o->nfcore, nr = o->nr;
+  orc_htotx(o, 1.0, lfield, wfn, cic); /* orbin + den1 */
+  zc* zo = xcalloc((size_t)nb * nf, sizeof(zc));
+  if (nfc > 0) zprod(o, 1.0, o->orb, zo, 0, nfc, o->nradfc);
+  zprod(o, 1.0, o->orb, zo, nfc, nf, nr);
+  zc t = trace1(o, zo);
   free(zo);
   return creal(t);
+}
+
+/* hprod_dipole's Ehrenfest terms (src/hprod/hprod_dipole.F90:51-72):
+ * out[0] = dip(1) = Re tr(D z), out[1] = vel(1) = Re tr(D p_z) (+ A_z N in
+ * the velocity gauge), out[2] = acc(1) = Re tr(D a_z) - E_z N, with the
+ * property vectors of hprod_zprod_all / hprod_pzprod_all / hprod_azprod_all
+ * at zfac = 1 (frozen cores over r < nradfc) and N = neltot(3).  The
+ * frozen-core correction terms dip(2), vel(2), acc(2), which need the
+ * generalised Fock vector of hprod_gfock, are not part of this function. */
+void orc_dipole3(orc* o, const double* lfield, const zc* wfn, const zc* cic, double* out) {
+  int nf = o->nfun, nb = o->nbas, nfc = o->nfcore, nr = o->nr;
+  orc_htotx(o, 1.0, lfield, wfn, cic); /* orbin + den1 */
+  zc* zo = xcalloc((size_t)nb * nf, sizeof(zc));
+  zc* po = xcalloc((size_t)nb * nf, sizeof(zc));
+  zc* ao = xcalloc((size_t)nb * nf, sizeof(zc));
+  if (nfc > 0) {
+    zprod(o, 1.0, o->orb, zo, 0, nfc, o->nradfc);
+    pzprod(o, 1.0, o->orb, po, 0, nfc, o->nradfc);
+    azprod(o, 1.0, o->orb, ao, 0, nfc, o->nradfc);
+  }
+  zprod(o, 1.0, o->orb, zo, nfc, nf, nr);
+  pzprod(o, 1.0, o->orb, po, nfc, nf, nr);
+  azprod(o, 1.0, o->orb, ao, nfc, nf, nr);
+  double ne = (double)o->nel[2];
+  out[0] = creal(trace1(o, zo));
+  out[1] = creal(trace1(o, po));
+  if (o->igauge == 1) out[1] += lfield[8] * ne;
+  out[2] = creal(trace1(o, ao)) - lfield[5] * ne;
+  free(ao);
+  free(po);
+  free(zo);
 }
 
 void orc_energies(const orc* o, double* e) {

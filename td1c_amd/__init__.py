@@ -67,6 +67,8 @@ def load_library():
     L.td1c_set_transform_path.restype = ctypes.c_int
     L.td1c_dipole.argtypes = [vp, vp, vp, vp, vp, vp]
     L.td1c_dipole.restype = ctypes.c_int
+    L.td1c_dipole3.argtypes = [vp, vp, vp, vp, vp, vp]
+    L.td1c_dipole3.restype = ctypes.c_int
     L.td1c_mask.argtypes = [vp, vp, vp]
     L.td1c_mask.restype = ctypes.c_int
     L.td1c_field_value.argtypes = [ctypes.POINTER(TD1CField), ctypes.c_double, vp]
@@ -206,6 +208,28 @@ class HProd:
                                           ctypes.c_void_p(base + 16 * self.norb), ctypes.c_void_p(out.data_ptr()),
                                           self._stream(stream)), "td1c_dipole")
         return float(out.item()) if ret else out
+
+    def dipole3(self, lfield: Sequence[float], wfn, out=None, stream=None):
+        """hprod_dipole's Ehrenfest terms [dip(1), vel(1), acc(1)] (length,
+        velocity and acceleration forms; frozen-core corrections excluded) of
+        a device wave function; returns a numpy array (synchronising) unless
+        `out`, a float64 CUDA tensor of three elements, is given."""
+        torch = self.torch
+        lf = np.ascontiguousarray(lfield, dtype=np.float64)
+        if lf.size != 9:
+            raise ValueError("lfield must have 9 entries")
+        if wfn.dtype != torch.complex128 or not wfn.is_cuda or wfn.numel() != self.wfn_size:
+            raise ValueError("wfn must be a complex128 CUDA tensor of wfn_size elements")
+        ret = out is None
+        if out is None:
+            out = torch.zeros(3, dtype=torch.float64, device=wfn.device)
+        elif out.dtype != torch.float64 or not out.is_cuda or out.numel() < 3:
+            raise ValueError("out must be a float64 CUDA tensor of three elements")
+        base = wfn.data_ptr()
+        _check(load_library().td1c_dipole3(self.h, _ptr(lf), ctypes.c_void_p(base),
+                                           ctypes.c_void_p(base + 16 * self.norb), ctypes.c_void_p(out.data_ptr()),
+                                           self._stream(stream)), "td1c_dipole3")
+        return out.cpu().numpy() if ret else out
 
     def mask(self, wfn, stream=None):
         _check(load_library().td1c_mask(self.h, ctypes.c_void_p(wfn.data_ptr()), self._stream(stream)), "td1c_mask")

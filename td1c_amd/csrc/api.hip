@@ -498,7 +498,7 @@ int build_device(td1c_ctx* c) {
   HIP_TRY(hipMemset(c->ene.p, 0, 5 * sizeof(double)));
   const size_t nfull = (size_t)S.nstr_a * S.nstr_b;
   HIP_TRY(c->mfc.alloc((size_t)na * na * na * na + 2 * na * na));
-  HIP_TRY(c->dip_part.alloc((size_t)((S.nr + 255) / 256) * S.L1));
+  HIP_TRY(c->dip_part.alloc((size_t)3 * ((S.nr + 255) / 256) * S.L1));  // three forms x dipole_blocks
   HIP_TRY(c->ci_work.alloc(std::max<size_t>(1, ((size_t)2 * S.nact * S.nact + 2) * nfull)));
   HIP_TRY(c->det_of.upload(flat_or_one(S.det_of)));
   HIP_TRY(c->exA.upload(flat_or_one(S.exA)));
@@ -982,8 +982,9 @@ int td1c_get_energies(td1c_ctx* c, double ene[5]) {
   return 0;
 }
 
-int td1c_dipole(td1c_ctx* c, const double* lfield, const double* d_wfn, const double* d_cic, double* d_out,
-                void* stream) {
+// orbitals (hprod_orbin) and 1-RDM of a wave function, then the dipole traces
+static int enqueue_dipole(td1c_ctx* c, int nforms, const double* lfield, const double* d_wfn, const double* d_cic,
+                          double* d_out, void* stream) {
   if (!c || !lfield || !d_wfn || !d_out) return fail(-1, "null argument");
   if (c->S.nfcore > 0 && !c->wfn0_set) return fail(-4, "frozen core needs td1c_set_wfn0 first");
   if (c->S.nact > 0 && !d_cic) return fail(-1, "null CI vector");
@@ -997,9 +998,21 @@ int td1c_dipole(td1c_ctx* c, const double* lfield, const double* d_wfn, const do
   a.nr = S.nr; a.L1 = S.L1; a.nfun = S.nfun; a.nfcore = S.nfcore; a.ncore = S.ncore; a.nact = S.nact;
   a.nradfc = S.nradfc; a.mmax1 = S.mmax1; a.mval = c->mval.p; a.xr = c->xr.p; a.alph = c->alph.p;
   a.orb = c->orb; a.den1 = c->den1.p;
-  launch_dipole(a, c->dip_part.p, d_out, s);
+  a.ndvr = S.n; a.igauge = S.igauge; a.znuc = S.znuc; a.nel = (double)S.nel[2];
+  a.band_p = c->band_p.p; a.lf = c->lf_dev.p;
+  launch_dipole(a, nforms, c->dip_part.p, d_out, s);
   HIP_TRY(hipGetLastError());
   return 0;
+}
+
+int td1c_dipole(td1c_ctx* c, const double* lfield, const double* d_wfn, const double* d_cic, double* d_out,
+                void* stream) {
+  return enqueue_dipole(c, 1, lfield, d_wfn, d_cic, d_out, stream);
+}
+
+int td1c_dipole3(td1c_ctx* c, const double* lfield, const double* d_wfn, const double* d_cic, double* d_out,
+                 void* stream) {
+  return enqueue_dipole(c, 3, lfield, d_wfn, d_cic, d_out, stream);
 }
 
 int td1c_mask(td1c_ctx* c, double* d_wfn, void* stream) {

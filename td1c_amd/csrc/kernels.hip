@@ -1759,67 +1759,138 @@ void launch_gram(const GramArgs& a, double2* partial, int maxblk, double2* out, 
 }
 
 // ---------------------------------------------------------------------------
-// Length-form dipole (hprod_dipole dip(1), src/hprod/hprod_dipole.F90:2-146):
-// z = r cos(theta) couples l and l+1 with r alpha_lm (hprod_zprod), traced
-// with the 1-RDM (hprod_trace1, src/hprod/hprod_trace1.F90:2-88): cores twice,
-// frozen cores over their nradfc rows.  Thread per (r, l); block partial
-// sums, then one block adds them in a fixed order.
+// Dipole expectation values, hprod_dipole's Ehrenfest terms
+// (src/hprod/hprod_dipole.F90:51-72): the property vectors z psi
+// (hprod_zprodp), p_z psi (hprod_pzprodp) and a_z psi (hprod_azprodp) at
+// zfac = 1, each coupling l and l+1, traced with the 1-RDM (hprod_trace1,
+// src/hprod/hprod_trace1.F90:2-88): cores twice, frozen cores over their
+// nradfc rows.  NF = 1: z only; NF = 3: z, p_z, a_z.  Thread per (r, l);
+// block partial sums, then one block adds them in a fixed order.
 // ---------------------------------------------------------------------------
+// d psi(r) of one l row by the FEDVR first-derivative band (pmat), rows
+// |j - r| <= n inside [0, nr)
+__device__ __forceinline__ double2 radial_derivative(const double2* __restrict__ row, const double* __restrict__ band_p,
+                                                     int nr, int n, int r) {
+  const int jlo = r - n > 0 ? r - n : 0, jhi = r + n < nr - 1 ? r + n : nr - 1;
+  double2 d = make_double2(0.0, 0.0);
+  for (int j = jlo; j <= jhi; ++j) {
+    const double pm = band_p[(size_t)(j - r + n) * nr + r];
+    d.x = fma(row[j].x, pm, d.x);
+    d.y = fma(row[j].y, pm, d.y);
+  }
+  return d;
+}
+
+// x * (-i s)
+__device__ __forceinline__ double2 mul_minus_i(double2 x, double s) { return make_double2(s * x.y, -s * x.x); }
+
+template <int NF>
 __global__ void __launch_bounds__(256) k_dipole(const DipoleArgs a, double2* __restrict__ partial) {
   const int r = blockIdx.x * 256 + threadIdx.x;
   const int l = blockIdx.y;
-  double2 acc = make_double2(0.0, 0.0);
+  double sum[NF];
+#pragma unroll
+  for (int f = 0; f < NF; ++f) sum[f] = 0.0;
   if (r < a.nr) {
     const double xr = a.xr[r];
+    const size_t nr = a.nr;
     for (int i = 0; i < a.nfun; ++i) {
       const int m = a.mval[i], am = m < 0 ? -m : m;
       if (l < am) continue;
       if (i < a.nfcore && r >= a.nradfc) continue;
       const double* al = a.alph + (size_t)(m + a.mmax1) * a.L1;
-      const double2* oi = a.orb + (size_t)i * a.L1 * a.nr + r;
-      double2 z = make_double2(0.0, 0.0);
-      if (l + 1 < a.L1) cfma(z, oi[(size_t)(l + 1) * a.nr], make_double2(xr * al[l], 0.0));
-      if (l - 1 >= am) cfma(z, oi[(size_t)(l - 1) * a.nr], make_double2(xr * al[l - 1], 0.0));
+      const double2* oi = a.orb + (size_t)i * a.L1 * nr;
+      double2 op[NF];
+#pragma unroll
+      for (int f = 0; f < NF; ++f) op[f] = make_double2(0.0, 0.0);
+      if (l + 1 < a.L1) {
+        const double2 up = oi[(size_t)(l + 1) * nr + r];
+        cfma(op[0], up, make_double2(xr * al[l], 0.0));
+        if constexpr (NF == 3) {
+          // p_z: (-i alph) d psi(l+1) + (-i alph (l+1) / r) psi(l+1)
+          const double2 dup = radial_derivative(oi + (size_t)(l + 1) * nr, a.band_p, a.nr, a.ndvr, r);
+          op[1] = cadd(op[1], cadd(mul_minus_i(dup, al[l]), mul_minus_i(up, al[l] * (l + 1.0) / xr)));
+          // a_z: -alph Z / r^2
+          cfma(op[2], up, make_double2(-al[l] * a.znuc / (xr * xr), 0.0));
+        }
+      }
+      if (l - 1 >= am) {
+        const double2 dn = oi[(size_t)(l - 1) * nr + r];
+        cfma(op[0], dn, make_double2(xr * al[l - 1], 0.0));
+        if constexpr (NF == 3) {
+          // p_z: (-i alph) d psi(l-1) - (-i alph l / r) psi(l-1)
+          const double2 ddn = radial_derivative(oi + (size_t)(l - 1) * nr, a.band_p, a.nr, a.ndvr, r);
+          op[1] = cadd(op[1], cadd(mul_minus_i(ddn, al[l - 1]), mul_minus_i(dn, -(al[l - 1] * (double)l) / xr)));
+          cfma(op[2], dn, make_double2(-al[l - 1] * a.znuc / (xr * xr), 0.0));
+        }
+      }
+      // trace weight: 2 psi_i for a core, sum_j psi_j D(j, i) for an active
+      double2 w;
       if (i < a.ncore) {
-        cfma(acc, cconj(oi[(size_t)l * a.nr]), cscale(z, 2.0));
+        w = cscale(oi[(size_t)l * nr + r], 2.0);
       } else {
         const int ia = i - a.ncore;
-        double2 t = make_double2(0.0, 0.0);
+        w = make_double2(0.0, 0.0);
         for (int jb = 0; jb < a.nact; ++jb) {
           const int j = a.ncore + jb;
           if (a.mval[j] != m) continue;
-          cfma(t, a.orb[((size_t)j * a.L1 + l) * a.nr + r], a.den1[jb + a.nact * ia]);
+          cfma(w, a.orb[((size_t)j * a.L1 + l) * nr + r], a.den1[jb + a.nact * ia]);
         }
-        cfma(acc, cconj(t), z);
       }
+#pragma unroll
+      for (int f = 0; f < NF; ++f) sum[f] += w.x * op[f].x + w.y * op[f].y;  // Re conj(w) op
     }
   }
-  for (int o = 32; o > 0; o >>= 1) {
-    acc.x += __shfl_xor(acc.x, o, 64);
-    acc.y += __shfl_xor(acc.y, o, 64);
+  __shared__ double red[NF][4];
+#pragma unroll
+  for (int f = 0; f < NF; ++f) {
+    double v = sum[f];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) red[f][threadIdx.x >> 6] = v;
   }
-  __shared__ double2 red[4];
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
   __syncthreads();
-  if (threadIdx.x == 0)
-    partial[blockIdx.y * gridDim.x + blockIdx.x] = cadd(cadd(red[0], red[1]), cadd(red[2], red[3]));
+  if (threadIdx.x == 0) {
+    const int nblk = gridDim.x * gridDim.y, b = blockIdx.y * gridDim.x + blockIdx.x;
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+      partial[(size_t)f * nblk + b] = make_double2((red[f][0] + red[f][1]) + (red[f][2] + red[f][3]), 0.0);
+  }
 }
 
-__global__ void __launch_bounds__(256) k_dipole_final(const double2* __restrict__ partial, int n,
+// out[f] = sum of the n block partials of form f; NF = 3 adds the field
+// terms of hprod_dipole: vel += A_z N (velocity gauge), acc -= E_z N
+template <int NF>
+__global__ void __launch_bounds__(256) k_dipole_final(const double2* __restrict__ partial, int n, const DipoleArgs a,
                                                       double* __restrict__ out) {
-  double s = 0.0;
-  for (int q = threadIdx.x; q < n; q += 256) s += partial[q].x;
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
   __shared__ double red[4];
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
-  __syncthreads();
-  if (threadIdx.x == 0) out[0] = (red[0] + red[1]) + (red[2] + red[3]);
+  for (int f = 0; f < NF; ++f) {
+    double s = 0.0;
+    for (int q = threadIdx.x; q < n; q += 256) s += partial[(size_t)f * n + q].x;
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double v = (red[0] + red[1]) + (red[2] + red[3]);
+      if (NF == 3 && f == 1 && a.igauge == 1) v += a.lf[8] * a.nel;
+      if (NF == 3 && f == 2) v -= a.lf[5] * a.nel;
+      out[f] = v;
+    }
+    __syncthreads();
+  }
 }
 
-void launch_dipole(const DipoleArgs& a, double2* partial, double* out, hipStream_t s) {
+int dipole_blocks(const DipoleArgs& a) { return (a.nr + 255) / 256 * a.L1; }
+
+void launch_dipole(const DipoleArgs& a, int nforms, double2* partial, double* out, hipStream_t s) {
   dim3 grid((a.nr + 255) / 256, a.L1);
-  hipLaunchKernelGGL(k_dipole, grid, dim3(256), 0, s, a, partial);
-  hipLaunchKernelGGL(k_dipole_final, dim3(1), dim3(256), 0, s, partial, (int)(grid.x * grid.y), out);
+  const int nblk = (int)(grid.x * grid.y);
+  if (nforms == 3) {
+    hipLaunchKernelGGL(k_dipole<3>, grid, dim3(256), 0, s, a, partial);
+    hipLaunchKernelGGL(k_dipole_final<3>, dim3(1), dim3(256), 0, s, partial, nblk, a, out);
+  } else {
+    hipLaunchKernelGGL(k_dipole<1>, grid, dim3(256), 0, s, a, partial);
+    hipLaunchKernelGGL(k_dipole_final<1>, dim3(1), dim3(256), 0, s, partial, nblk, a, out);
+  }
 }
 
 // ---------------------------------------------------------------------------

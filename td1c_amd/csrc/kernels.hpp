@@ -195,7 +195,8 @@ void launch_mfprod(const T* orbg, const T* vg, T* gorbg, T* v2orbg, const MfArgs
 // * 2 + c, nblk = gram_blocks(a, maxblk)), summed by the consumer
 void launch_gram(const GramArgs& a, double2* partial, int maxblk, double2* out, hipStream_t s);
 int gram_blocks(const GramArgs& a, int maxblk);
-// Length-form dipole <z> = 2 sum_core <i|z|i> + sum_ab conj(den1(b,a)) <b|z|a>
+// Dipole expectation values: <z> = 2 sum_core <i|z|i> + sum_ab conj(den1(b,a)) <b|z|a>,
+// and with nforms = 3 the same traces of p_z and a_z plus the field terms
 struct DipoleArgs {
   int nr, L1, nfun, nfcore, ncore, nact, nradfc, mmax1;
   const int* mval;
@@ -203,9 +204,15 @@ struct DipoleArgs {
   const double* alph;     // [2mmax1+1][L1]
   const double2* orb;     // orbitals as hprod_orbin builds them
   const double2* den1;    // [nact][nact]
+  // nforms = 3 only
+  int ndvr, igauge;
+  double znuc, nel;       // nuclear charge, neltot(3)
+  const double* band_p;   // [2n+1][nr] d/dr band
+  const double* lf;       // device lfield[9]
 };
-// partial: one entry per block, ceil(nr / 256) * L1
-void launch_dipole(const DipoleArgs& a, double2* partial, double* out, hipStream_t s);
+// partial: nforms * dipole_blocks(a) entries
+int dipole_blocks(const DipoleArgs& a);
+void launch_dipole(const DipoleArgs& a, int nforms, double2* partial, double* out, hipStream_t s);
 void launch_to_f32(float2* y, const double2* x, size_t n, hipStream_t s);
 void launch_zero(double2* y, size_t n, hipStream_t s);
 void launch_field_stages(const Pulse& p, PropClock* st, const StageNodes& nodes, double* lf, hipStream_t s);

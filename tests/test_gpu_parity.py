@@ -200,6 +200,31 @@ def test_dipole_matches_oracle(gpu_lib, name, lf):
     assert abs(got - ref) <= RTOL * max(1.0, abs(ref)), (got, ref)
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("name,lf", [("he_small", C.LF_LEN), ("dc3_small", C.LF_LEN), ("ne_small", C.LF_VEL),
+                                     ("cas_fc_small", C.LF_VEL), ("cas_small", C.LF_LEN), ("dc4_small", C.LF_LEN)])
+def test_dipole_forms_match_oracle(gpu_lib, name, lf, precision):
+    """td1c_dipole3 (hprod_dipole dip(1), vel(1), acc(1)) against the oracle;
+    its length form equals td1c_dipole's.  The complex64 context differs only
+    in the velocity-gauge frozen core, whose phase passes the FP32 theta grid."""
+    import td1c_amd
+    from oracle.oracle import Oracle
+
+    prob = getattr(C, name)(precision=precision)
+    o = Oracle(prob)
+    w = C.wfn_for(prob, o.lcic)
+    o.set_wfn0(w)
+    ref = o.dipole3(lf, w)
+    hp = td1c_amd.HProd(prob)
+    hp.set_wfn0(w)
+    d = hp.to_device(w)
+    print(f"dipole3 {name} precision {precision}", flush=True)
+    got = hp.dipole3(lf, d)
+    tol = RTOL if precision == 0 else 1e-5
+    assert np.all(np.abs(got - ref) <= tol * np.maximum(1.0, np.abs(ref))), (got, ref)
+    assert abs(hp.dipole(lf, d) - got[0]) <= 1e-14 * max(1.0, abs(got[0]))
+
+
 # Mixed precision (precision = 1): the mean field (theta-grid orbitals, pair
 # densities and potentials, their transforms and products) in complex64, the
 # rest FP64.  Its single-precision rounding (~6e-8 relative per value, summed

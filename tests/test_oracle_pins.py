@@ -556,3 +556,105 @@ def test_dipole_closed_form_and_core_equivalence():
 
 
 C_LEN = np.array([0.0, 0.0, 0.05, 0.0, 0.0, 0.05, 0.0, 0.0, 0.0])
+
+
+def _h_1s_2p(phi, igauge=0, nfe=30, lmax1=3):
+    """Doubly occupied (1s + e^{i phi} 2p0)/sqrt(2) of hydrogen (Z = 1)."""
+    prob = _prob(nfe=nfe, rmax=60.0, lmax1=lmax1, lmax2=2 * lmax1, znuc=1.0, igauge=igauge)
+    o = O.Oracle(prob)
+    r, w = W.gll_grid(prob)
+    nr = prob.nr
+    wf = np.zeros(o.wfn_size, complex)
+    wf[:nr] = 2 * r * np.exp(-r) * np.sqrt(w) / math.sqrt(2)
+    wf[nr:2 * nr] = np.exp(1j * phi) * r * r * np.exp(-r / 2) / (2 * math.sqrt(6)) * np.sqrt(w) / math.sqrt(2)
+    wf[-1] = 1.0
+    o.set_wfn0(wf)
+    return o, wf
+
+
+def test_dipole_forms_closed_form():
+    """orc_dipole3 (hprod_dipole dip(1), vel(1), acc(1)): for the doubly
+    occupied (1s + e^{i phi} 2p0)/sqrt(2) of hydrogen, with
+    z12 = <1s|z|2p0> = 128 sqrt(2)/243:
+      <z>   = 2 cos(phi) z12,
+      <p_z> = 2 sin(phi) (E_2p - E_1s) z12      (<1s|p_z|2p0> = -i dE z12),
+      <a_z> = -2 cos(phi) Z <1s|cos(th)/r^2|2p0> = -2 cos(phi) 4/(27 sqrt 2);
+    the velocity gauge adds A N to vel and every gauge subtracts E N from acc."""
+    z12 = 128 * math.sqrt(2) / 243
+    a12 = 4 / (27 * math.sqrt(2))
+    for phi in (0.0, math.pi / 2, 1.0):
+        o, wf = _h_1s_2p(phi)
+        got = o.dipole3(np.zeros(9), wf)
+        ref = [2 * math.cos(phi) * z12, 2 * math.sin(phi) * 0.375 * z12, -2 * math.cos(phi) * a12]
+        assert np.abs(got - ref).max() < 1e-10, (phi, got, ref)
+    E, A = 0.03, 0.3
+    o, wf = _h_1s_2p(1.0)
+    got = o.dipole3([0, 0, E, 0, 0, E, 0, 0, 0], wf)
+    ref = [2 * math.cos(1.0) * z12, 2 * math.sin(1.0) * 0.375 * z12, -2 * math.cos(1.0) * a12 - 2 * E]
+    assert np.abs(got - ref).max() < 1e-10
+    o, wf = _h_1s_2p(1.0, igauge=1)
+    got = o.dipole3([0, 0, A, 0, 0, E, 0, 0, A], wf)
+    ref = [2 * math.cos(1.0) * z12, 2 * math.sin(1.0) * 0.375 * z12 + 2 * A, -2 * math.cos(1.0) * a12 - 2 * E]
+    assert np.abs(got - ref).max() < 1e-10
+
+
+@pytest.mark.parametrize("igauge", [0, 1])
+def test_dipole_forms_ehrenfest(igauge):
+    """orc_dipole3 against the equations of motion (orc_htot): along
+    d psi/dt (a central difference of the state), d<z>/dt = vel and
+    d vel/dt = acc (Ehrenfest; the two-electron forces cancel in TDHF; the
+    velocity-gauge case holds A fixed, so its E = -dA/dt is 0).  Smooth hydrogenic orbitals keep the FEDVR commutator
+    error near 1e-10; a sign or a missing term in p_z or a_z is O(1)."""
+    o, wf = _h_1s_2p(math.pi / 4, igauge=igauge, nfe=40, lmax1=5)
+    lf = np.zeros(9)
+    if igauge == 0:
+        lf[2] = lf[5] = 0.03
+    else:
+        lf[2] = lf[8] = 0.2
+    h = o.htot(1.0, lf, wf)
+    d0 = o.dipole3(lf, wf)
+    eps = 1e-5
+    fd = (o.dipole3(lf, wf + eps * h) - o.dipole3(lf, wf - eps * h)) / (2 * eps)
+    assert abs(fd[0] - d0[1]) < 1e-8 * abs(d0[1]), (fd, d0)
+    assert abs(fd[1] - d0[2]) < 1e-8 * abs(d0[2]), (fd, d0)
+
+
+def test_dipole_forms_velocity_ehrenfest_correlated():
+    """orc_dipole3's active-orbital trace (hprod_trace1 with the 1-RDM):
+    d<z>/dt = vel along the CAS equations of motion (no frozen core); the
+    synthetic orbitals leave a FEDVR commutator error of ~1e-4."""
+    import tests.cases as C
+
+    for name in ("cas_small", "dc4_small"):
+        prob = getattr(C, name)()
+        o = O.Oracle(prob)
+        wv = C.wfn_for(prob, o.lcic)
+        o.set_wfn0(wv)
+        lf = np.zeros(9)
+        lf[2] = lf[5] = 0.03
+        h = o.htot(1.0, lf, wv)
+        d0 = o.dipole3(lf, wv)
+        eps = 1e-5
+        fd = (o.dipole3(lf, wv + eps * h) - o.dipole3(lf, wv - eps * h)) / (2 * eps)
+        assert abs(fd[0] - d0[1]) < 1e-3 * abs(d0[1]), (name, fd, d0)
+
+
+def test_dipole_forms_frozen_core_equals_dynamic_core():
+    """orc_dipole3's frozen-core branch (property vectors over r < nradfc):
+    in the length gauge a frozen 1s core gives the same three traces as the
+    same 1s held as a dynamical core."""
+    base = dict(nfe=6, ndvr=10, rmax=20.0, lmax1=3, lmax2=6, znuc=3.0, mval=[0, 0], nel_alpha=2, nel_beta=2)
+    pa = Problem(ndcore=2, nact=0, **base)
+    pc = Problem(nfcore1=1, ndcore=1, lval=[0, 0], **base)
+    oa, oc = O.Oracle(pa), O.Oracle(pc)
+    wa = W.make_wfn(pa, 21, oa.lcic)
+    nb = pa.nr * (pa.lmax1 + 1)
+    o0 = np.zeros(nb, complex)
+    o0[:pa.nr] = W.hydrogenic_1s(pa, 3.0)
+    o1 = wa[nb:2 * nb] - np.vdot(o0, wa[nb:2 * nb]) * o0
+    wa[:nb], wa[nb:2 * nb] = o0, o1 / np.linalg.norm(o1)
+    oa.set_wfn0(wa)
+    oc.set_wfn0(wa)
+    da, dc = oa.dipole3(C_LEN, wa), oc.dipole3(C_LEN, wa)
+    assert np.abs(da).min() > 1e-3
+    assert np.abs(da - dc).max() < 1e-13 * np.abs(da).max()
