@@ -166,6 +166,18 @@ int td1c_dipole(td1c_ctx* ctx, const double* lfield, const double* d_wfn,
 int td1c_dipole3(td1c_ctx* ctx, const double* lfield, const double* d_wfn,
                  const double* d_cic, double* d_out, void* stream);
 
+/* Ionisation probabilities of a device wave function: ipx[k] (host memory,
+ * max_ipx + 1 doubles) = probability that exactly k electrons are outside
+ * rad_ipx, k = 0..max_ipx.  Replaces hprod_ipx (src/hprod/hprod_ipx.F90:2-19)
+ * with ormas_ipx (src/ormas/ormas_ipx.F90:2-35 -> ormas_ipx_core /
+ * ormas_ipx_ras): the orbitals of hprod_orbin at lfield, their overlap over
+ * the radial points up to and including the first r > rad_ipx
+ * (hprod_mkovlp), and the CI vector.  0 <= max_ipx <= min(8, electrons), as
+ * the reference requires (it stops otherwise; here the call returns -1).
+ * Synchronises `stream`. */
+int td1c_ipx(td1c_ctx* ctx, int max_ipx, double rad_ipx, const double* lfield,
+             const double* d_wfn, const double* d_cic, double* ipx, void* stream);
+
 /* One explicit Runge-Kutta step of the whole wave function (orbitals then
  * CI) held on device: clcrk::prop with rk_order 1..4 ("standard").
  * lfield_stages holds the 9 field values at t + c_s*dt for each stage

@@ -56,6 +56,8 @@ def lib():
         L.orc_dipole.restype = ctypes.c_double
         L.orc_dipole3.argtypes = [vp, vp, vp, vp, vp]
         L.orc_dipole3.restype = None
+        L.orc_ipx.argtypes = [vp, ctypes.c_int, ctypes.c_double, vp, vp, vp, vp]
+        L.orc_ipx.restype = ctypes.c_int
         L.orc_band_chol.argtypes = [ctypes.c_int, ctypes.c_int, vp]
         L.orc_band_chol.restype = ctypes.c_int
         L.orc_band_chol_solve.argtypes = [ctypes.c_int, ctypes.c_int, vp, vp]
@@ -146,6 +148,20 @@ class Oracle:
         lf = np.ascontiguousarray(lfield, dtype=np.float64)
         out = np.zeros(3)
         lib().orc_dipole3(self.h, _p(lf), _p(w[: self.norb]), _p(w[self.norb:]), _p(out))
+        return out
+
+    def ipx(self, max_ipx: int, rad_ipx: float, lfield, wfn: np.ndarray) -> np.ndarray:
+        """hprod_ipx: probabilities that exactly k = 0..max_ipx electrons are
+        outside rad_ipx."""
+        w = np.ascontiguousarray(wfn, dtype=np.complex128)
+        lf = np.ascontiguousarray(lfield, dtype=np.float64)
+        out = np.zeros(max_ipx + 1)
+        cic = np.ascontiguousarray(w[self.norb:])
+        if cic.size == 0:
+            cic = np.zeros(max(1, self.lcic), np.complex128)
+        rc = lib().orc_ipx(self.h, max_ipx, rad_ipx, _p(lf), _p(w[: self.norb]), _p(cic), _p(out))
+        if rc:
+            raise ValueError("bad max_ipx")
         return out
 
     def energies(self) -> np.ndarray:

@@ -1762,6 +1762,147 @@ void orc_dipole3(orc* o, const double* lfield, const zc* wfn, const zc* cic, dou
   free(zo);
 }
 
+/* hprod_mkovlp (src/hprod/hprod_mkovlp.F90:2-91) on o->orb:
+ * ovlp[j + nf*i] = sum_{l >= |m|} sum_{i_r < cut} conj(orb_j) orb_i for m_i == m_j,
+ * cut = rows up to and including the first one with r > rmax (all if none) */
+static void mkovlp(const orc* o, double rmax, zc* ovlp) {
+  int nf = o->nfun, nr = o->nr, cut = nr;
+  for (int i = 0; i < nr; ++i)
+    if (o->xrad[i + 1] > rmax) {
+      cut = i + 1;
+      break;
+    }
+  for (int i = 0; i < nf; ++i)
+    for (int j = 0; j < nf; ++j) {
+      zc t = 0;
+      if (o->mval[i] == o->mval[j])
+        for (int l = abs(o->mval[i]); l <= o->lmax1; ++l)
+          for (int r = 0; r < cut; ++r) t += conj(o->orb[OIDX(r, l, j)]) * o->orb[OIDX(r, l, i)];
+      ovlp[j + (size_t)nf * i] = t;
+    }
+}
+
+/* determinant of an n x n complex matrix (row-major, destroyed): Gaussian
+ * elimination with partial pivoting (util_det's role) */
+static zc zdet(int n, zc* a) {
+  zc d = 1;
+  for (int k = 0; k < n; ++k) {
+    int p = k;
+    for (int i = k + 1; i < n; ++i)
+      if (cabs(a[i * n + k]) > cabs(a[p * n + k])) p = i;
+    if (a[p * n + k] == 0) return 0;
+    if (p != k) {
+      for (int j = 0; j < n; ++j) {
+        zc t = a[k * n + j];
+        a[k * n + j] = a[p * n + j];
+        a[p * n + j] = t;
+      }
+      d = -d;
+    }
+    d *= a[k * n + k];
+    for (int i = k + 1; i < n; ++i) {
+      zc f = a[i * n + k] / a[k * n + k];
+      for (int j = k; j < n; ++j) a[i * n + j] -= f * a[k * n + j];
+    }
+  }
+  return d;
+}
+
+/* One spin of ormas_dets0 / ormas_dett / ormas_dett_1
+ * (src/ormas/ormas_dets0.F90:2-55; ormas_dett.F90:2-290; ormas_dett_1.F90:2-38):
+ * electrons are the ncore cores then the active orbitals of the string in
+ * ascending order; S[e][f] = ovlp(bra orbital e, ket orbital f).  d[k] is
+ * the sum over k-subsets K of electrons of det(S with the rows of K replaced
+ * by the orbital-identity rows delta(bra_e, ket_f)): the minors sums of
+ * ormas_dett1/2/3 (k = 0: ormas_dets0, k = n: 1 if the strings are equal). */
+static void spin_minors(const orc* o, const zc* ovlp, uint32_t si, uint32_t sj, int n, int kmax, zc* d) {
+  int nf = o->nfun, nc = o->ncore;
+  int bra[32], ket[32];
+  int e = 0, f = 0;
+  for (int c = 0; c < nc; ++c) bra[e++] = ket[f++] = c;
+  for (int p = 0; p < o->nact; ++p) {
+    if (si >> p & 1) bra[e++] = nc + p;
+    if (sj >> p & 1) ket[f++] = nc + p;
+  }
+  for (int k = 0; k <= kmax; ++k) d[k] = 0;
+  zc* m = xcalloc((size_t)n * n + 1, sizeof(zc));
+  for (uint32_t K = 0; K < (1u << n); ++K) {
+    int k = popcount32(K);
+    if (k > kmax) continue;
+    for (int a = 0; a < n; ++a)
+      for (int b = 0; b < n; ++b)
+        m[a * n + b] = (K >> a & 1) ? (bra[a] == ket[b] ? 1.0 : 0.0) : ovlp[bra[a] + (size_t)nf * ket[b]];
+    d[k] += n ? zdet(n, m) : 1.0;
+  }
+  free(m);
+}
+
+static double binom(int n, int k) {
+  if (k < 0 || k > n) return 0;
+  double b = 1;
+  for (int i = 1; i <= k; ++i) b = b * (n - k + i) / i;
+  return b;
+}
+
+/* hprod_ipx (src/hprod/hprod_ipx.F90:2-19) -> ormas_ipx (src/ormas/ormas_ipx.F90)
+ * -> ormas_ipx_core / ormas_ipx_ras (src/ormas/ormas_ipx_core.F90:2-130;
+ * src/ormas/ormas_ipx_ras.F90:2-180): with the windowed overlap of the
+ * orbitals of hprod_orbin, T(k) = sum_{ka+kb=k} Re sum_IJ conj(C_I) C_J
+ * d^a_ka(I,J) d^b_kb(I,J) (T(k): k electrons free, the others inside), then
+ * for k = max_ipx down to 1: ipx(k) = T(k) + sum_{j=1..k} (-1)^j C(N-k+j, j)
+ * ipx(k-j), so that ipx(k) is the probability that exactly k electrons are
+ * outside rad_ipx.  Returns -1 on a bad max_ipx. */
+int orc_ipx(orc* o, int max_ipx, double rad_ipx, const double* lfield, const zc* wfn, const zc* cic, double* ipx) {
+  int nf = o->nfun, nsa = o->nstr_a, nsb = o->nstr_b, N = o->nel[2];
+  if (max_ipx < 0 || max_ipx > 8 || max_ipx > N) return -1;
+  if (o->nact == 0) nsa = nsb = 1; /* cores only: one determinant, C = 1 (ormas_ipx_core) */
+  orc_htotx(o, 1.0, lfield, wfn, cic); /* orbin */
+  zc* ovlp = xcalloc((size_t)nf * nf, sizeof(zc));
+  mkovlp(o, rad_ipx, ovlp);
+  int na = o->nel[0], nb = o->nel[1];
+  int ka_max = na < max_ipx ? na : max_ipx, kb_max = nb < max_ipx ? nb : max_ipx;
+  zc* da = xcalloc((size_t)nsa * nsa * (ka_max + 1), sizeof(zc));
+  zc* db = xcalloc((size_t)nsb * nsb * (kb_max + 1), sizeof(zc));
+  for (int i = 0; i < nsa; ++i)
+    for (int j = 0; j < nsa; ++j)
+      spin_minors(o, ovlp, o->nact ? o->str_a[i] : 0, o->nact ? o->str_a[j] : 0, na, ka_max,
+                  da + ((size_t)i * nsa + j) * (ka_max + 1));
+  for (int i = 0; i < nsb; ++i)
+    for (int j = 0; j < nsb; ++j)
+      spin_minors(o, ovlp, o->nact ? o->str_b[i] : 0, o->nact ? o->str_b[j] : 0, nb, kb_max,
+                  db + ((size_t)i * nsb + j) * (kb_max + 1));
+  double T[9] = {0};
+  for (int ib = 0; ib < nsb; ++ib)
+    for (int ia = 0; ia < nsa; ++ia) {
+      int dI = o->nact == 0 ? 0 : o->det_of[(size_t)ib * nsa + ia];
+      if (dI < 0) continue;
+      zc cI = o->nact == 0 ? 1.0 : cic[dI];
+      for (int jb = 0; jb < nsb; ++jb)
+        for (int ja = 0; ja < nsa; ++ja) {
+          int dJ = o->nact == 0 ? 0 : o->det_of[(size_t)jb * nsa + ja];
+          if (dJ < 0) continue;
+          zc cJ = o->nact == 0 ? 1.0 : cic[dJ];
+          const zc* pa = da + ((size_t)ia * nsa + ja) * (ka_max + 1);
+          const zc* pb = db + ((size_t)ib * nsb + jb) * (kb_max + 1);
+          for (int k = 0; k <= max_ipx; ++k) {
+            zc s = 0;
+            for (int ka = 0; ka <= ka_max; ++ka) {
+              int kb = k - ka;
+              if (kb >= 0 && kb <= kb_max) s += pa[ka] * pb[kb];
+            }
+            T[k] += creal(conj(cI) * cJ * s);
+          }
+        }
+    }
+  for (int k = 0; k <= max_ipx; ++k) ipx[k] = T[k];
+  for (int k = max_ipx; k >= 1; --k)
+    for (int j = 1; j <= k; ++j) ipx[k] += ipx[k - j] * binom(N - k + j, j) * ((j & 1) ? -1.0 : 1.0);
+  free(db);
+  free(da);
+  free(ovlp);
+  return 0;
+}
+
 void orc_energies(const orc* o, double* e) {
   e[0] = o->ene_fcore;
   e[1] = o->ene_dcore;

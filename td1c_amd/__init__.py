@@ -69,6 +69,8 @@ def load_library():
     L.td1c_dipole.restype = ctypes.c_int
     L.td1c_dipole3.argtypes = [vp, vp, vp, vp, vp, vp]
     L.td1c_dipole3.restype = ctypes.c_int
+    L.td1c_ipx.argtypes = [vp, ctypes.c_int, ctypes.c_double, vp, vp, vp, vp, vp]
+    L.td1c_ipx.restype = ctypes.c_int
     L.td1c_mask.argtypes = [vp, vp, vp]
     L.td1c_mask.restype = ctypes.c_int
     L.td1c_field_value.argtypes = [ctypes.POINTER(TD1CField), ctypes.c_double, vp]
@@ -230,6 +232,22 @@ class HProd:
                                            ctypes.c_void_p(base + 16 * self.norb), ctypes.c_void_p(out.data_ptr()),
                                            self._stream(stream)), "td1c_dipole3")
         return out.cpu().numpy() if ret else out
+
+    def ipx(self, max_ipx: int, rad_ipx: float, lfield: Sequence[float], wfn, stream=None) -> np.ndarray:
+        """hprod_ipx: probabilities that exactly k = 0..max_ipx electrons are
+        outside rad_ipx, for a device wave function (synchronising)."""
+        torch = self.torch
+        lf = np.ascontiguousarray(lfield, dtype=np.float64)
+        if lf.size != 9:
+            raise ValueError("lfield must have 9 entries")
+        if wfn.dtype != torch.complex128 or not wfn.is_cuda or wfn.numel() != self.wfn_size:
+            raise ValueError("wfn must be a complex128 CUDA tensor of wfn_size elements")
+        out = np.zeros(max(int(max_ipx), 0) + 1)
+        base = wfn.data_ptr()
+        _check(load_library().td1c_ipx(self.h, int(max_ipx), float(rad_ipx), _ptr(lf), ctypes.c_void_p(base),
+                                       ctypes.c_void_p(base + 16 * self.norb), _ptr(out), self._stream(stream)),
+               "td1c_ipx")
+        return out
 
     def mask(self, wfn, stream=None):
         _check(load_library().td1c_mask(self.h, ctypes.c_void_p(wfn.data_ptr()), self._stream(stream)), "td1c_mask")

@@ -115,7 +115,7 @@ struct td1c_ctx {
   DevBuf<double2> gram_partial, gram_partial_p, lprod, pprod;
   DevBuf<int> mapH, mapG, mapV, mapF, map2e;
   // island
-  DevBuf<double2> den1, den2, rden, rrden, int1e, int2e, xmat, dcic, ci_work, mfc, dip_part;
+  DevBuf<double2> den1, den2, rden, rrden, int1e, int2e, xmat, dcic, ci_work, mfc, dip_part, ovlp;
   DevBuf<double> ene;
   DevBuf<int> det_of, exA, sgA, exB, sgB;
   double ene_fcore = 0.0;
@@ -162,7 +162,7 @@ struct td1c_ctx {
     for (auto* b : d) b->release();
     DevBuf<double2>* z[] = {&tdiag, &tzor, &lsp, &gsp, &orbg0, &rhoL, &vL, &vg, &p_xb, &p_Q,
                             &gram_partial, &gram_partial_p, &lprod, &pprod, &den1, &den2, &rden, &rrden, &int1e, &int2e, &xmat,
-                            &dcic, &ci_work, &mfc, &dip_part, &rk_w0, &rk_t, &rk_k};
+                            &dcic, &ci_work, &mfc, &dip_part, &ovlp, &rk_w0, &rk_t, &rk_k};
     for (auto* b : z) b->release();
     DevBuf<int>* i[] = {&mval, &pair_m, &pj, &pi, &ptype, &mapH, &mapG, &mapV, &mapF, &map2e,
                         &det_of, &exA, &sgA, &exB, &sgB};
@@ -499,6 +499,7 @@ int build_device(td1c_ctx* c) {
   const size_t nfull = (size_t)S.nstr_a * S.nstr_b;
   HIP_TRY(c->mfc.alloc((size_t)na * na * na * na + 2 * na * na));
   HIP_TRY(c->dip_part.alloc((size_t)3 * ((S.nr + 255) / 256) * S.L1));  // three forms x dipole_blocks
+  HIP_TRY(c->ovlp.alloc((size_t)S.nfun * S.nfun));
   HIP_TRY(c->ci_work.alloc(std::max<size_t>(1, ((size_t)2 * S.nact * S.nact + 2) * nfull)));
   HIP_TRY(c->det_of.upload(flat_or_one(S.det_of)));
   HIP_TRY(c->exA.upload(flat_or_one(S.exA)));
@@ -1013,6 +1014,134 @@ int td1c_dipole(td1c_ctx* c, const double* lfield, const double* d_wfn, const do
 int td1c_dipole3(td1c_ctx* c, const double* lfield, const double* d_wfn, const double* d_cic, double* d_out,
                  void* stream) {
   return enqueue_dipole(c, 3, lfield, d_wfn, d_cic, d_out, stream);
+}
+
+// Ionisation probabilities (hprod_ipx, src/hprod/hprod_ipx.F90:2-19, with
+// ormas_ipx_core / ormas_ipx_ras, src/ormas/ormas_ipx_core.F90:2-130,
+// src/ormas/ormas_ipx_ras.F90:2-180): ipx[k] = probability that exactly k
+// electrons are outside rad_ipx, k = 0..max_ipx.
+//
+// The windowed overlap S_in (hprod_mkovlp: rows up to and including the
+// first r > rad_ipx) of the orbitals of hprod_orbin is formed on the device.
+// The counting statistics follow from the generating function
+//   F(x) = <Psi| prod_electrons (P_in + x P_out) |Psi>
+//        = sum_IJ conj(C_I) C_J det A^a_IJ(x) det A^b_IJ(x),
+//   A_IJ(x)[e][f] = (1 - x) S_in(bra_e, ket_f) + x delta(bra_e, ket_f),
+// a polynomial of degree N whose coefficient k is ipx[k]: F is evaluated at
+// the N+1 roots of unity and transformed back.  The reference reaches the
+// same coefficients through sums of minors and a binomial inversion.
+namespace {
+
+std::complex<double> host_det(int n, std::vector<std::complex<double>>& a) {
+  std::complex<double> d = 1.0;
+  for (int k = 0; k < n; ++k) {
+    int p = k;
+    for (int i = k + 1; i < n; ++i)
+      if (std::abs(a[(size_t)i * n + k]) > std::abs(a[(size_t)p * n + k])) p = i;
+    if (a[(size_t)p * n + k] == 0.0) return 0.0;
+    if (p != k) {
+      for (int j = 0; j < n; ++j) std::swap(a[(size_t)k * n + j], a[(size_t)p * n + j]);
+      d = -d;
+    }
+    d *= a[(size_t)k * n + k];
+    for (int i = k + 1; i < n; ++i) {
+      const std::complex<double> f = a[(size_t)i * n + k] / a[(size_t)k * n + k];
+      for (int j = k; j < n; ++j) a[(size_t)i * n + j] -= f * a[(size_t)k * n + j];
+    }
+  }
+  return d;
+}
+
+// det A_IJ(x) for every string pair of one spin: out[I * ns + J]
+void spin_dets(const Setup& S, const std::vector<std::complex<double>>& ov, const std::vector<uint32_t>& str, int nel,
+               std::complex<double> x, std::vector<std::complex<double>>& out) {
+  const int nf = S.nfun, nc = S.ncore, ns = (int)str.size();
+  out.assign((size_t)ns * ns, 0.0);
+  std::vector<int> bra(nel), ket(nel);
+  std::vector<std::complex<double>> a((size_t)nel * nel);
+  for (int I = 0; I < ns; ++I)
+    for (int J = 0; J < ns; ++J) {
+      int e = 0, f = 0;
+      for (int k = 0; k < nc; ++k) bra[e++] = ket[f++] = k;
+      for (int p = 0; p < S.nact; ++p) {
+        if (str[I] >> p & 1) bra[e++] = nc + p;
+        if (str[J] >> p & 1) ket[f++] = nc + p;
+      }
+      for (int r = 0; r < nel; ++r)
+        for (int q = 0; q < nel; ++q)
+          a[(size_t)r * nel + q] = (1.0 - x) * ov[bra[r] + (size_t)nf * ket[q]] + (bra[r] == ket[q] ? x : 0.0);
+      out[(size_t)I * ns + J] = nel ? host_det(nel, a) : 1.0;
+    }
+}
+
+}  // namespace
+
+int td1c_ipx(td1c_ctx* c, int max_ipx, double rad_ipx, const double* lfield, const double* d_wfn, const double* d_cic,
+             double* ipx, void* stream) {
+  if (!c || !lfield || !d_wfn || !ipx) return fail(-1, "null argument");
+  Setup& S = c->S;
+  const int N = S.nel[2];
+  if (max_ipx < 0 || max_ipx > 8 || max_ipx > N) return fail(-1, "max_ipx must be in [0, min(8, electrons)]");
+  if (S.nfcore > 0 && !c->wfn0_set) return fail(-4, "frozen core needs td1c_set_wfn0 first");
+  if (S.nact > 0 && !d_cic) return fail(-1, "null CI vector");
+  if (S.nel[0] - S.ncore > 31 || S.nel[1] - S.ncore > 31) return fail(-1, "too many active electrons");
+  hipStream_t s = (hipStream_t)stream;
+  HIP_TRY(hipMemcpyAsync(c->lf_dev.p, lfield, 9 * sizeof(double), hipMemcpyHostToDevice, s));
+  enqueue_orbin(c, c->lf_dev.p, reinterpret_cast<const double2*>(d_wfn), s);
+  int cut = S.nr;
+  for (int i = 0; i < S.nr; ++i)
+    if (S.xr[i] > rad_ipx) {
+      cut = i + 1;
+      break;
+    }
+  const int nf = S.nfun;
+  launch_window_ovlp(c->orb, c->mval.p, nf, S.nr, S.L1, cut, c->ovlp.p, s);
+  HIP_TRY(hipGetLastError());
+  std::vector<std::complex<double>> ov((size_t)nf * nf), cic(S.nact > 0 ? S.lcic : 1, 1.0);
+  HIP_TRY(hipMemcpyAsync(ov.data(), c->ovlp.p, ov.size() * sizeof(double2), hipMemcpyDeviceToHost, s));
+  if (S.nact > 0)
+    HIP_TRY(hipMemcpyAsync(cic.data(), d_cic, cic.size() * sizeof(double2), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  // strings and the full alpha x beta coefficient table C[ib][ia]
+  const std::vector<uint32_t> one{0u};
+  const std::vector<uint32_t>& sa = S.nact > 0 ? S.str_a : one;
+  const std::vector<uint32_t>& sb = S.nact > 0 ? S.str_b : one;
+  const int nsa = (int)sa.size(), nsb = (int)sb.size();
+  std::vector<std::complex<double>> C((size_t)nsa * nsb, 0.0);
+  for (int ib = 0; ib < nsb; ++ib)
+    for (int ia = 0; ia < nsa; ++ia) {
+      const int d = S.nact > 0 ? S.det_of[(size_t)ib * nsa + ia] : 0;
+      if (d >= 0) C[(size_t)ib * nsa + ia] = cic[d];
+    }
+  const double two_pi = 6.283185307179586476925286766559;
+  std::vector<std::complex<double>> F(N + 1), da, db, X((size_t)nsb * nsa);
+  for (int q = 0; q <= N; ++q) {
+    const std::complex<double> x = std::polar(1.0, two_pi * q / (N + 1));
+    spin_dets(S, ov, sa, S.nel[0], x, da);
+    spin_dets(S, ov, sb, S.nel[1], x, db);
+    // X[ib][ja] = sum_ia conj(C[ib][ia]) da[ia][ja]
+    for (int ib = 0; ib < nsb; ++ib)
+      for (int ja = 0; ja < nsa; ++ja) {
+        std::complex<double> t = 0.0;
+        for (int ia = 0; ia < nsa; ++ia) t += std::conj(C[(size_t)ib * nsa + ia]) * da[(size_t)ia * nsa + ja];
+        X[(size_t)ib * nsa + ja] = t;
+      }
+    // F = sum_{ib,ja} X[ib][ja] sum_jb db[ib][jb] C[jb][ja]
+    std::complex<double> f = 0.0;
+    for (int ib = 0; ib < nsb; ++ib)
+      for (int ja = 0; ja < nsa; ++ja) {
+        std::complex<double> t = 0.0;
+        for (int jb = 0; jb < nsb; ++jb) t += db[(size_t)ib * nsb + jb] * C[(size_t)jb * nsa + ja];
+        f += X[(size_t)ib * nsa + ja] * t;
+      }
+    F[q] = f;
+  }
+  for (int k = 0; k <= max_ipx; ++k) {
+    std::complex<double> t = 0.0;
+    for (int q = 0; q <= N; ++q) t += F[q] * std::polar(1.0, -two_pi * (double)q * k / (N + 1));
+    ipx[k] = t.real() / (N + 1);
+  }
+  return 0;
 }
 
 int td1c_mask(td1c_ctx* c, double* d_wfn, void* stream) {

@@ -1894,6 +1894,43 @@ void launch_dipole(const DipoleArgs& a, int nforms, double2* partial, double* ou
 }
 
 // ---------------------------------------------------------------------------
+// Windowed orbital overlap (hprod_mkovlp, src/hprod/hprod_mkovlp.F90:2-91):
+// ovlp[j + nf i] = sum_{l >= |m|} sum_{r < cut} conj(orb_j(r, l)) orb_i(r, l)
+// for m_i == m_j, else 0.  One block per (j, i); threads stride over (l, r);
+// fixed-order block sum.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_window_ovlp(const double2* __restrict__ orb, const int* __restrict__ mval,
+                                                     int nf, int nr, int L1, int cut, double2* __restrict__ ovlp) {
+  const int i = blockIdx.x / nf, j = blockIdx.x % nf;
+  const int m = mval[i], am = m < 0 ? -m : m;
+  const bool same = mval[j] == m;
+  double2 acc = make_double2(0.0, 0.0);
+  if (same) {
+    const double2* oi = orb + (size_t)i * L1 * nr;
+    const double2* oj = orb + (size_t)j * L1 * nr;
+    const int n = (L1 - am) * cut;
+    for (int q = threadIdx.x; q < n; q += 256) {
+      const int l = am + q / cut, r = q % cut;
+      const size_t x = (size_t)l * nr + r;
+      acc = cadd(acc, cmulc(oj[x], oi[x]));
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    acc.x += __shfl_xor(acc.x, o, 64);
+    acc.y += __shfl_xor(acc.y, o, 64);
+  }
+  __shared__ double2 red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) ovlp[j + (size_t)nf * i] = cadd(cadd(red[0], red[1]), cadd(red[2], red[3]));
+}
+
+void launch_window_ovlp(const double2* orb, const int* mval, int nf, int nr, int L1, int cut, double2* ovlp,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(k_window_ovlp, dim3(nf * nf), dim3(256), 0, s, orb, mval, nf, nr, L1, cut, ovlp);
+}
+
+// ---------------------------------------------------------------------------
 // Small helpers that keep memset/memcpy nodes out of captured graphs.
 // ---------------------------------------------------------------------------
 __global__ void k_zero(double2* __restrict__ y, size_t n) {

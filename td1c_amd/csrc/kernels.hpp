@@ -213,6 +213,9 @@ struct DipoleArgs {
 // partial: nforms * dipole_blocks(a) entries
 int dipole_blocks(const DipoleArgs& a);
 void launch_dipole(const DipoleArgs& a, int nforms, double2* partial, double* out, hipStream_t s);
+// ovlp[j + nf*i] = <orb_j|orb_i> over the radial rows r < cut (m-matched), nf*nf entries
+void launch_window_ovlp(const double2* orb, const int* mval, int nf, int nr, int L1, int cut, double2* ovlp,
+                        hipStream_t s);
 void launch_to_f32(float2* y, const double2* x, size_t n, hipStream_t s);
 void launch_zero(double2* y, size_t n, hipStream_t s);
 void launch_field_stages(const Pulse& p, PropClock* st, const StageNodes& nodes, double* lf, hipStream_t s);

@@ -225,6 +225,36 @@ def test_dipole_forms_match_oracle(gpu_lib, name, lf, precision):
     assert abs(hp.dipole(lf, d) - got[0]) <= 1e-14 * max(1.0, abs(got[0]))
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("name,lf,R", [("he_small", C.LF_LEN, 1.0), ("dc3_small", C.LF_LEN, 0.8),
+                                       ("dc4_small", C.LF_LEN, 0.6), ("cas_small", C.LF_LEN, 1.0),
+                                       ("cas_fc_small", C.LF_VEL, 0.8), ("ne_small", C.LF_VEL, 0.6)])
+def test_ipx_matches_oracle(gpu_lib, name, lf, R, precision):
+    """td1c_ipx (hprod_ipx: probabilities of k electrons outside R) against the
+    oracle's minors sums.  The library takes another route (generating
+    function at roots of unity), so agreement is to rounding; the oracle's
+    binomial inversion loses up to ~1e-12 at ten electrons."""
+    import td1c_amd
+    from oracle.oracle import Oracle
+
+    prob = getattr(C, name)(precision=precision)
+    o = Oracle(prob)
+    w = C.wfn_for(prob, o.lcic)
+    o.set_wfn0(w)
+    N = prob.nel_alpha + prob.nel_beta
+    mx = min(8, N)
+    ref = o.ipx(mx, R, lf, w)
+    hp = td1c_amd.HProd(prob)
+    hp.set_wfn0(w)
+    d = hp.to_device(w)
+    print(f"ipx {name} precision {precision}", flush=True)
+    got = hp.ipx(mx, R, lf, d)
+    tol = 1e-11 if precision == 0 else 1e-6
+    assert np.abs(got - ref).max() < tol, (got, ref)
+    with pytest.raises(RuntimeError):
+        hp.ipx(9, R, lf, d)
+
+
 # Mixed precision (precision = 1): the mean field (theta-grid orbitals, pair
 # densities and potentials, their transforms and products) in complex64, the
 # rest FP64.  Its single-precision rounding (~6e-8 relative per value, summed

@@ -658,3 +658,85 @@ def test_dipole_forms_frozen_core_equals_dynamic_core():
     da, dc = oa.dipole3(C_LEN, wa), oc.dipole3(C_LEN, wa)
     assert np.abs(da).min() > 1e-3
     assert np.abs(da - dc).max() < 1e-13 * np.abs(da).max()
+
+
+def _window_ovlp(prob, orb_l, R):
+    """numpy hprod_mkovlp: <phi_j|phi_i> over the rows up to and including the
+    first r > R, m-matched."""
+    nf, nr, L1 = prob.nfun, prob.nr, prob.lmax1 + 1
+    r, _ = W.gll_grid(prob)
+    cut = int(np.argmax(r > R)) + 1 if np.any(r > R) else nr
+    orb = orb_l.reshape(nf, L1, nr)[:, :, :cut]
+    S = np.einsum("jlr,ilr->ji", orb.conj(), orb)
+    mv = np.array(prob.mval if prob.mval is not None else [0] * nf)
+    return S * (mv[:, None] == mv[None, :])
+
+
+@pytest.mark.parametrize("name,R", [("he_small", 1.0), ("dc3_small", 0.8), ("dc4_small", 0.6)])
+def test_ipx_slater_determinant_closed_form(name, R):
+    """orc_ipx (hprod_ipx -> ormas_ipx_core): for a closed-shell determinant
+    the number of electrons outside R has the generating function
+    prod_spin prod_i (s_i + (1 - s_i) x), s_i the eigenvalues of the windowed
+    overlap matrix (numpy eigvalsh); the minors sums and the binomial
+    inversion must reproduce its coefficients."""
+    import tests.cases as C
+
+    prob = getattr(C, name)()
+    o = O.Oracle(prob)
+    w = C.wfn_for(prob, o.lcic)
+    o.set_wfn0(w)
+    N = prob.nel_alpha + prob.nel_beta
+    mx = min(8, N)
+    got = o.ipx(mx, R, np.zeros(9), w)
+    s = np.linalg.eigvalsh(_window_ovlp(prob, w[: o.norb], R))
+    poly = np.array([1.0])
+    for si in s:
+        poly = np.convolve(poly, [si, 1 - si])
+    poly = np.convolve(poly, poly)
+    assert got[1] > 1e-2, got
+    assert np.abs(got - poly[: mx + 1]).max() < 1e-12, (got, poly)
+
+
+@pytest.mark.parametrize("name,R", [("cas_small", 1.0), ("cas_fc_small", 0.8)])
+def test_ipx_correlated_moments(name, R):
+    """orc_ipx on CAS states (ormas_ipx_ras): the probabilities sum to 1 and
+    their mean is N - tr(gamma S_in), the one-body expectation of the
+    electrons inside R with gamma = 2 on cores and the 1-RDM on the active
+    orbitals (den1 from the oracle's RDM stage)."""
+    import tests.cases as C
+
+    prob = getattr(C, name)()
+    o = O.Oracle(prob)
+    w = C.wfn_for(prob, o.lcic)
+    o.set_wfn0(w)
+    N = prob.nel_alpha + prob.nel_beta
+    lf = C.LF_VEL if prob.igauge == 1 else np.zeros(9)
+    got = o.ipx(min(8, N), R, lf, w)
+    orb = o.table("orb")  # orbitals as hprod_orbin built them (frozen core phased)
+    S = _window_ovlp(prob, orb, R)
+    na, nc = prob.nact, prob.nfun - prob.nact
+    D = o.table("den1")[: na * na].reshape(na, na, order="F")
+    inside = 2 * np.trace(S[:nc, :nc]).real + np.einsum("ab,ba->", D, S[nc:, nc:]).real
+    assert got.size == N + 1
+    assert abs(got.sum() - 1) < 1e-12
+    assert got[1] > 1e-2, got
+    assert abs(np.dot(np.arange(N + 1), got) - (N - inside)) < 1e-11, (np.dot(np.arange(N + 1), got), N - inside)
+
+
+def test_ipx_filled_active_space_equals_cores():
+    """orc_ipx: a doubly occupied active orbital (one determinant, CI = 1)
+    gives the distribution of the same orbital held as a dynamical core."""
+    base = dict(nfe=6, ndvr=10, rmax=20.0, lmax1=3, lmax2=6, znuc=3.0, mval=[0, 0], nel_alpha=2, nel_beta=2)
+    pa = Problem(ndcore=2, nact=0, **base)
+    pb = Problem(ndcore=1, nact=1, **base)
+    oa, ob = O.Oracle(pa), O.Oracle(pb)
+    wa = W.make_wfn(pa, 21, oa.lcic)
+    nb = pa.nr * (pa.lmax1 + 1)
+    wb = np.zeros(ob.wfn_size, complex)
+    wb[: 2 * nb] = wa[: 2 * nb]
+    wb[2 * nb:] = 1.0
+    oa.set_wfn0(wa)
+    ob.set_wfn0(wb)
+    pa_, pb_ = oa.ipx(4, 1.0, np.zeros(9), wa), ob.ipx(4, 1.0, np.zeros(9), wb)
+    assert pa_[1] > 1e-2
+    assert np.abs(pa_ - pb_).max() < 1e-13
