@@ -168,7 +168,8 @@ int td1c_dipole3(td1c_ctx* ctx, const double* lfield, const double* d_wfn,
 
 /* Ionisation probabilities of a device wave function: ipx[k] (host memory,
  * max_ipx + 1 doubles) = probability that exactly k electrons are outside
- * rad_ipx, k = 0..max_ipx.  Replaces hprod_ipx (src/hprod/hprod_ipx.F90:2-19)
+ * rad_ipx, k = 0..max_ipx.  Replaces hprod_ipx_ (include/wrapper.hpp:196;
+ * src/hprod/hprod_ipx.F90:2-19, called by clhprod::ionpx and clhprod::normx)
  * with ormas_ipx (src/ormas/ormas_ipx.F90:2-35 -> ormas_ipx_core /
  * ormas_ipx_ras): the orbitals of hprod_orbin at lfield, their overlap over
  * the radial points up to and including the first r > rad_ipx
