@@ -179,6 +179,16 @@ int td1c_dipole3(td1c_ctx* ctx, const double* lfield, const double* d_wfn,
 int td1c_ipx(td1c_ctx* ctx, int max_ipx, double rad_ipx, const double* lfield,
              const double* d_wfn, const double* d_cic, double* ipx, void* stream);
 
+/* hprod_ipd_ (include/wrapper.hpp:195; src/hprod/hprod_ipx.F90:21-39 ->
+ * ormas_ipd, src/ormas/ormas_ipd.F90:2-123, called by clhprod::ionpd):
+ * ipd[k] (host memory, max_ipd + 1 doubles, max_ipd <= min(2, electrons)) =
+ * the estimate p0 = 1 - S1 + S2, p1 = S1 - 2 S2, p2 = S2 of the probability
+ * that k electrons are outside rad_ipx, S1 and S2 the first two factorial
+ * moments of the count (the reference forms them from the 1- and 2-RDM).
+ * Synchronises `stream`. */
+int td1c_ipd(td1c_ctx* ctx, int max_ipd, double rad_ipx, const double* lfield,
+             const double* d_wfn, const double* d_cic, double* ipd, void* stream);
+
 /* One explicit Runge-Kutta step of the whole wave function (orbitals then
  * CI) held on device: clcrk::prop with rk_order 1..4 ("standard").
  * lfield_stages holds the 9 field values at t + c_s*dt for each stage

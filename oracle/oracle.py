@@ -58,6 +58,8 @@ def lib():
         L.orc_dipole3.restype = None
         L.orc_ipx.argtypes = [vp, ctypes.c_int, ctypes.c_double, vp, vp, vp, vp]
         L.orc_ipx.restype = ctypes.c_int
+        L.orc_ipd.argtypes = [vp, ctypes.c_int, ctypes.c_double, vp, vp, vp, vp]
+        L.orc_ipd.restype = ctypes.c_int
         L.orc_band_chol.argtypes = [ctypes.c_int, ctypes.c_int, vp]
         L.orc_band_chol.restype = ctypes.c_int
         L.orc_band_chol_solve.argtypes = [ctypes.c_int, ctypes.c_int, vp, vp]
@@ -162,6 +164,20 @@ class Oracle:
         rc = lib().orc_ipx(self.h, max_ipx, rad_ipx, _p(lf), _p(w[: self.norb]), _p(cic), _p(out))
         if rc:
             raise ValueError("bad max_ipx")
+        return out
+
+    def ipd(self, max_ipd: int, rad_ipx: float, lfield, wfn: np.ndarray) -> np.ndarray:
+        """hprod_ipd: the 1-/2-RDM estimate of the probabilities that k = 0..max_ipd
+        (at most 2) electrons are outside rad_ipx."""
+        w = np.ascontiguousarray(wfn, dtype=np.complex128)
+        lf = np.ascontiguousarray(lfield, dtype=np.float64)
+        out = np.zeros(max_ipd + 1)
+        cic = np.ascontiguousarray(w[self.norb:])
+        if cic.size == 0:
+            cic = np.zeros(max(1, self.lcic), np.complex128)
+        rc = lib().orc_ipd(self.h, max_ipd, rad_ipx, _p(lf), _p(w[: self.norb]), _p(cic), _p(out))
+        if rc:
+            raise ValueError("bad max_ipd")
         return out
 
     def energies(self) -> np.ndarray:

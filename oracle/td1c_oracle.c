@@ -1903,6 +1903,54 @@ int orc_ipx(orc* o, int max_ipx, double rad_ipx, const double* lfield, const zc*
   return 0;
 }
 
+/* hprod_ipd (src/hprod/hprod_ipx.F90:21-39) -> ormas_ipd
+ * (src/ormas/ormas_ipd.F90:2-123): with the outer overlap
+ * ovlx = 1 - ovlp (hermitised from the upper triangle), S1 and S2 from the
+ * 1- and 2-RDM (cores doubly occupied), p0 = 1 - S1 + S2, p1 = S1 - 2 S2,
+ * p2 = S2; ipd[k] for k = 0..max_ipd.  Returns -1 on a bad max_ipd. */
+int orc_ipd(orc* o, int max_ipd, double rad_ipx, const double* lfield, const zc* wfn, const zc* cic, double* ipd) {
+  int nf = o->nfun, nc = o->ncore, na = o->nact;
+  if (max_ipd < 0 || max_ipd > 2 || max_ipd > o->nel[2]) return -1;
+  orc_htotx(o, 1.0, lfield, wfn, cic); /* orbin + den1 + den2 */
+  zc* ovlp = xcalloc((size_t)nf * nf, sizeof(zc));
+  zc* ovlx = xcalloc((size_t)nf * nf, sizeof(zc));
+  mkovlp(o, rad_ipx, ovlp);
+#define OV(j, i) ovlx[(j) + (size_t)nf * (i)]
+  for (int i = 0; i < nf; ++i) {
+    for (int j = 0; j < i; ++j) {
+      OV(j, i) = -ovlp[j + (size_t)nf * i];
+      OV(i, j) = conj(OV(j, i));
+    }
+    OV(i, i) = 1.0 - ovlp[i + (size_t)nf * i];
+  }
+  zc s1c = 0, s1a = 0, s2cc = 0, s2ca = 0, s2aa = 0;
+  for (int i = 0; i < nc; ++i) s1c += 2.0 * OV(i, i);
+  for (int ia = 0; ia < na; ++ia)
+    for (int ja = 0; ja < na; ++ja) s1a += o->den1[ja + na * ia] * OV(nc + ia, nc + ja);
+  for (int i = 0; i < nc; ++i)
+    for (int j = 0; j < nc; ++j) s2cc += 2.0 * OV(i, i) * OV(j, j) - OV(i, j) * OV(j, i);
+  for (int ia = 0; ia < na; ++ia)
+    for (int ja = 0; ja < na; ++ja) {
+      int i = nc + ia, j = nc + ja;
+      zc t = 0;
+      for (int k = 0; k < nc; ++k) t += 2.0 * OV(i, j) * OV(k, k) - OV(i, k) * OV(k, j);
+      s2ca += o->den1[ja + na * ia] * t;
+    }
+  for (int ia = 0; ia < na; ++ia)
+    for (int ja = 0; ja < na; ++ja)
+      for (int ka = 0; ka < na; ++ka)
+        for (int la = 0; la < na; ++la)
+          s2aa += o->den2[ka + na * (ia + na * (la + na * ja))] * OV(nc + ia, nc + ka) * OV(nc + ja, nc + la);
+#undef OV
+  s2aa *= 0.5;
+  zc s1 = s1c + s1a, s2 = s2cc + s2ca + s2aa;
+  double p[3] = {creal(1.0 - s1 + s2), creal(s1 - 2.0 * s2), creal(s2)};
+  for (int k = 0; k <= max_ipd; ++k) ipd[k] = p[k];
+  free(ovlx);
+  free(ovlp);
+  return 0;
+}
+
 void orc_energies(const orc* o, double* e) {
   e[0] = o->ene_fcore;
   e[1] = o->ene_dcore;

@@ -71,6 +71,8 @@ def load_library():
     L.td1c_dipole3.restype = ctypes.c_int
     L.td1c_ipx.argtypes = [vp, ctypes.c_int, ctypes.c_double, vp, vp, vp, vp, vp]
     L.td1c_ipx.restype = ctypes.c_int
+    L.td1c_ipd.argtypes = [vp, ctypes.c_int, ctypes.c_double, vp, vp, vp, vp, vp]
+    L.td1c_ipd.restype = ctypes.c_int
     L.td1c_mask.argtypes = [vp, vp, vp]
     L.td1c_mask.restype = ctypes.c_int
     L.td1c_field_value.argtypes = [ctypes.POINTER(TD1CField), ctypes.c_double, vp]
@@ -247,6 +249,22 @@ class HProd:
         _check(load_library().td1c_ipx(self.h, int(max_ipx), float(rad_ipx), _ptr(lf), ctypes.c_void_p(base),
                                        ctypes.c_void_p(base + 16 * self.norb), _ptr(out), self._stream(stream)),
                "td1c_ipx")
+        return out
+
+    def ipd(self, max_ipd: int, rad_ipx: float, lfield: Sequence[float], wfn, stream=None) -> np.ndarray:
+        """hprod_ipd: the factorial-moment estimate of the probabilities that
+        k = 0..max_ipd (at most 2) electrons are outside rad_ipx (synchronising)."""
+        torch = self.torch
+        lf = np.ascontiguousarray(lfield, dtype=np.float64)
+        if lf.size != 9:
+            raise ValueError("lfield must have 9 entries")
+        if wfn.dtype != torch.complex128 or not wfn.is_cuda or wfn.numel() != self.wfn_size:
+            raise ValueError("wfn must be a complex128 CUDA tensor of wfn_size elements")
+        out = np.zeros(max(int(max_ipd), 0) + 1)
+        base = wfn.data_ptr()
+        _check(load_library().td1c_ipd(self.h, int(max_ipd), float(rad_ipx), _ptr(lf), ctypes.c_void_p(base),
+                                       ctypes.c_void_p(base + 16 * self.norb), _ptr(out), self._stream(stream)),
+               "td1c_ipd")
         return out
 
     def mask(self, wfn, stream=None):

@@ -1076,21 +1076,18 @@ void spin_dets(const Setup& S, const std::vector<std::complex<double>>& ov, cons
 
 }  // namespace
 
-int td1c_ipx(td1c_ctx* c, int max_ipx, double rad_ipx, const double* lfield, const double* d_wfn, const double* d_cic,
-             double* ipx, void* stream) {
-  if (!c || !lfield || !d_wfn || !ipx) return fail(-1, "null argument");
+// P[k], k = 0..N: probability that exactly k electrons are outside rad
+static int ipx_distribution(td1c_ctx* c, double rad, const double* lfield, const double* d_wfn, const double* d_cic,
+                     std::vector<double>& P, hipStream_t s) {
   Setup& S = c->S;
   const int N = S.nel[2];
-  if (max_ipx < 0 || max_ipx > 8 || max_ipx > N) return fail(-1, "max_ipx must be in [0, min(8, electrons)]");
   if (S.nfcore > 0 && !c->wfn0_set) return fail(-4, "frozen core needs td1c_set_wfn0 first");
   if (S.nact > 0 && !d_cic) return fail(-1, "null CI vector");
-  if (S.nel[0] - S.ncore > 31 || S.nel[1] - S.ncore > 31) return fail(-1, "too many active electrons");
-  hipStream_t s = (hipStream_t)stream;
   HIP_TRY(hipMemcpyAsync(c->lf_dev.p, lfield, 9 * sizeof(double), hipMemcpyHostToDevice, s));
   enqueue_orbin(c, c->lf_dev.p, reinterpret_cast<const double2*>(d_wfn), s);
   int cut = S.nr;
   for (int i = 0; i < S.nr; ++i)
-    if (S.xr[i] > rad_ipx) {
+    if (S.xr[i] > rad) {
       cut = i + 1;
       break;
     }
@@ -1136,11 +1133,47 @@ int td1c_ipx(td1c_ctx* c, int max_ipx, double rad_ipx, const double* lfield, con
       }
     F[q] = f;
   }
-  for (int k = 0; k <= max_ipx; ++k) {
+  P.assign(N + 1, 0.0);
+  for (int k = 0; k <= N; ++k) {
     std::complex<double> t = 0.0;
     for (int q = 0; q <= N; ++q) t += F[q] * std::polar(1.0, -two_pi * (double)q * k / (N + 1));
-    ipx[k] = t.real() / (N + 1);
+    P[k] = t.real() / (N + 1);
   }
+  return 0;
+}
+
+int td1c_ipx(td1c_ctx* c, int max_ipx, double rad_ipx, const double* lfield, const double* d_wfn, const double* d_cic,
+             double* ipx, void* stream) {
+  if (!c || !lfield || !d_wfn || !ipx) return fail(-1, "null argument");
+  if (max_ipx < 0 || max_ipx > 8 || max_ipx > c->S.nel[2])
+    return fail(-1, "max_ipx must be in [0, min(8, electrons)]");
+  std::vector<double> P;
+  const int rc = ipx_distribution(c, rad_ipx, lfield, d_wfn, d_cic, P, (hipStream_t)stream);
+  if (rc) return rc;
+  for (int k = 0; k <= max_ipx; ++k) ipx[k] = P[k];
+  return 0;
+}
+
+// hprod_ipd / ormas_ipd (src/hprod/hprod_ipx.F90:21-39; src/ormas/ormas_ipd.F90:2-123):
+// S1 = sum_k k P(k) and S2 = sum_k k(k-1)/2 P(k), the first two factorial
+// moments of the count outside rad_ipx (the reference forms them from the
+// 1- and 2-RDM), give p0 = 1 - S1 + S2, p1 = S1 - 2 S2, p2 = S2.
+int td1c_ipd(td1c_ctx* c, int max_ipd, double rad_ipx, const double* lfield, const double* d_wfn, const double* d_cic,
+             double* ipd, void* stream) {
+  if (!c || !lfield || !d_wfn || !ipd) return fail(-1, "null argument");
+  if (max_ipd < 0 || max_ipd > 2 || max_ipd > c->S.nel[2])
+    return fail(-1, "max_ipd must be in [0, min(2, electrons)]");
+  std::vector<double> P;
+  const int rc = ipx_distribution(c, rad_ipx, lfield, d_wfn, d_cic, P, (hipStream_t)stream);
+  if (rc) return rc;
+  double s1 = 0.0, s2 = 0.0;
+  for (size_t k = 0; k < P.size(); ++k) {
+    const double kk = (double)k;
+    s1 += kk * P[k];
+    s2 += 0.5 * kk * (kk - 1.0) * P[k];
+  }
+  const double p[3] = {1.0 - s1 + s2, s1 - 2.0 * s2, s2};
+  for (int k = 0; k <= max_ipd; ++k) ipd[k] = p[k];
   return 0;
 }
 

@@ -253,6 +253,13 @@ def test_ipx_matches_oracle(gpu_lib, name, lf, R, precision):
     assert np.abs(got - ref).max() < tol, (got, ref)
     with pytest.raises(RuntimeError):
         hp.ipx(9, R, lf, d)
+    # hprod_ipd: the reference's 1-/2-RDM formula (oracle) against the
+    # library's factorial moments of the same distribution
+    mxd = min(2, N)
+    gotd, refd = hp.ipd(mxd, R, lf, d), o.ipd(mxd, R, lf, w)
+    assert np.abs(gotd - refd).max() < tol, (gotd, refd)
+    with pytest.raises(RuntimeError):
+        hp.ipd(3, R, lf, d)
 
 
 # Mixed precision (precision = 1): the mean field (theta-grid orbitals, pair

@@ -740,3 +740,43 @@ def test_ipx_filled_active_space_equals_cores():
     pa_, pb_ = oa.ipx(4, 1.0, np.zeros(9), wa), ob.ipx(4, 1.0, np.zeros(9), wb)
     assert pa_[1] > 1e-2
     assert np.abs(pa_ - pb_).max() < 1e-13
+
+
+@pytest.mark.parametrize("name,R", [("he_small", 1.0), ("dc3_small", 0.8), ("cas_small", 1.0),
+                                    ("cas_fc_small", 0.8), ("dc4_small", 0.6)])
+def test_ipd_is_factorial_moment_estimate(name, R):
+    """orc_ipd (ormas_ipd: S1, S2 from the 1- and 2-RDM): S1 and S2 are the
+    first two factorial moments of the count outside R, so ipd equals
+    [1 - S1 + S2, S1 - 2 S2, S2] with S1 = sum k P(k), S2 = sum C(k,2) P(k)
+    from the full distribution of orc_ipx (pinned above).  This pins the
+    2-RDM index order of the active-active term."""
+    import tests.cases as C
+
+    prob = getattr(C, name)()
+    o = O.Oracle(prob)
+    w = C.wfn_for(prob, o.lcic)
+    o.set_wfn0(w)
+    lf = C.LF_VEL if prob.igauge == 1 else np.zeros(9)
+    N = prob.nel_alpha + prob.nel_beta
+    assert N <= 8
+    p = o.ipx(N, R, lf, w)
+    k = np.arange(N + 1)
+    s1, s2 = (k * p).sum(), (k * (k - 1) / 2 * p).sum()
+    got = o.ipd(2, R, lf, w)
+    assert s2 > 1e-2
+    assert np.abs(got - [1 - s1 + s2, s1 - 2 * s2, s2]).max() < 1e-12, got
+
+
+def test_ipd_exact_for_two_electrons():
+    """orc_ipd: with two electrons (CAS(2e, 3o), no core) the estimate is the
+    exact distribution of orc_ipx."""
+    prob = Problem(nfe=8, ndvr=20, rmax=24.0, lmax1=5, lmax2=10, znuc=2.0, ndcore=0, nact=3,
+                   nel_alpha=1, nel_beta=1)
+    import tests.cases as C
+
+    o = O.Oracle(prob)
+    w = C.wfn_for(prob, o.lcic)
+    o.set_wfn0(w)
+    a, b = o.ipd(2, 1.0, np.zeros(9), w), o.ipx(2, 1.0, np.zeros(9), w)
+    assert b[1] > 1e-2 and b[2] > 1e-2
+    assert np.abs(a - b).max() < 1e-13
