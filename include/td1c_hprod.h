@@ -189,6 +189,23 @@ int td1c_ipx(td1c_ctx* ctx, int max_ipx, double rad_ipx, const double* lfield,
 int td1c_ipd(td1c_ctx* ctx, int max_ipd, double rad_ipx, const double* lfield,
              const double* d_wfn, const double* d_cic, double* ipd, void* stream);
 
+/* clwfn::orth with orth_type "schmidt" (src/wfn/clwfn.cpp:313-321;
+ * wfn_orth_cic and wfn_orth_orb, src/wfn/wfn_orth.F90:2-176) on a device wave
+ * function: the CI vector normalised, each dynamical orbital in order made
+ * orthogonal to the earlier orbitals of the same m (modified Gram-Schmidt)
+ * and normalised.  Enqueued on `stream`. */
+int td1c_orth(td1c_ctx* ctx, double* d_wfn, void* stream);
+
+/* Imaginary-time relaxation, init_split2 with split_type 0 and the clcrk
+ * propagator (src/main/init.cpp:178-240): each cycle evaluates the energy
+ * (hprod_energy, as td1c_htot's ene_tot) at lfield, stops when it changed by
+ * less than tolene, else takes one rk_order step of dstep (no mask) and
+ * calls td1c_orth.  The context must have icomp = 0.  *ene gets the last
+ * energy, *ncyc the last cycle index.  Synchronises `stream`. */
+int td1c_init_split2(td1c_ctx* ctx, int rk_order, double dstep, double tolene,
+                     int maxcyc, const double* lfield, double* d_wfn,
+                     double* ene, int* ncyc, void* stream);
+
 /* One explicit Runge-Kutta step of the whole wave function (orbitals then
  * CI) held on device: clcrk::prop with rk_order 1..4 ("standard").
  * lfield_stages holds the 9 field values at t + c_s*dt for each stage

@@ -60,6 +60,8 @@ def lib():
         L.orc_ipx.restype = ctypes.c_int
         L.orc_ipd.argtypes = [vp, ctypes.c_int, ctypes.c_double, vp, vp, vp, vp]
         L.orc_ipd.restype = ctypes.c_int
+        L.orc_orth.argtypes = [vp, vp, vp]
+        L.orc_orth.restype = None
         L.orc_band_chol.argtypes = [ctypes.c_int, ctypes.c_int, vp]
         L.orc_band_chol.restype = ctypes.c_int
         L.orc_band_chol_solve.argtypes = [ctypes.c_int, ctypes.c_int, vp, vp]
@@ -179,6 +181,14 @@ class Oracle:
         if rc:
             raise ValueError("bad max_ipd")
         return out
+
+    def orth(self, wfn: np.ndarray) -> np.ndarray:
+        """clwfn::orth (schmidt): CI normalised, dynamical orbitals by modified
+        Gram-Schmidt; returns a new wave function."""
+        w = np.array(wfn, dtype=np.complex128, copy=True)
+        cic = w[self.norb:]
+        lib().orc_orth(self.h, _p(w[: self.norb]), _p(cic) if cic.size else None)
+        return w
 
     def energies(self) -> np.ndarray:
         e = np.zeros(5)

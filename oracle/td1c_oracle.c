@@ -1951,6 +1951,38 @@ int orc_ipd(orc* o, int max_ipd, double rad_ipx, const double* lfield, const zc*
   return 0;
 }
 
+/* clwfn::orth with orth_type "schmidt" (src/wfn/clwfn.cpp:313-321):
+ * wfn_orth_cic (src/wfn/wfn_orth.F90:2-29) normalises the CI vector;
+ * wfn_orth_orb (:31-176, fedvr_normalized, ecs off) orthonormalises the
+ * dynamical orbitals in order by modified Gram-Schmidt against every earlier
+ * orbital of the same m (frozen cores included), over l >= |m| and all rows. */
+void orc_orth(orc* o, zc* orb, zc* cic) {
+  if (o->lcic > 0 && cic) {
+    double nrm = 0;
+    for (int d = 0; d < o->lcic; ++d) nrm += creal(conj(cic[d]) * cic[d]);
+    nrm = 1.0 / sqrt(nrm);
+    for (int d = 0; d < o->lcic; ++d) cic[d] *= nrm;
+  }
+  int nr = o->nr;
+  for (int i = o->nfcore; i < o->nfun; ++i) {
+    int mi = o->mval[i];
+    for (int j = 0; j < i; ++j) {
+      if (o->mval[j] != mi) continue;
+      zc t = 0;
+      for (int l = abs(mi); l <= o->lmax1; ++l)
+        for (int r = 0; r < nr; ++r) t += conj(orb[OIDX(r, l, j)]) * orb[OIDX(r, l, i)];
+      for (int l = abs(mi); l <= o->lmax1; ++l)
+        for (int r = 0; r < nr; ++r) orb[OIDX(r, l, i)] -= orb[OIDX(r, l, j)] * t;
+    }
+    double nrm = 0;
+    for (int l = abs(mi); l <= o->lmax1; ++l)
+      for (int r = 0; r < nr; ++r) nrm += creal(conj(orb[OIDX(r, l, i)]) * orb[OIDX(r, l, i)]);
+    nrm = 1.0 / sqrt(nrm);
+    for (int l = abs(mi); l <= o->lmax1; ++l)
+      for (int r = 0; r < nr; ++r) orb[OIDX(r, l, i)] *= nrm;
+  }
+}
+
 void orc_energies(const orc* o, double* e) {
   e[0] = o->ene_fcore;
   e[1] = o->ene_dcore;

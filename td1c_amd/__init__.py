@@ -73,6 +73,11 @@ def load_library():
     L.td1c_ipx.restype = ctypes.c_int
     L.td1c_ipd.argtypes = [vp, ctypes.c_int, ctypes.c_double, vp, vp, vp, vp, vp]
     L.td1c_ipd.restype = ctypes.c_int
+    L.td1c_orth.argtypes = [vp, vp, vp]
+    L.td1c_orth.restype = ctypes.c_int
+    L.td1c_init_split2.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, vp, vp,
+                                   ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int), vp]
+    L.td1c_init_split2.restype = ctypes.c_int
     L.td1c_mask.argtypes = [vp, vp, vp]
     L.td1c_mask.restype = ctypes.c_int
     L.td1c_field_value.argtypes = [ctypes.POINTER(TD1CField), ctypes.c_double, vp]
@@ -266,6 +271,32 @@ class HProd:
                                        ctypes.c_void_p(base + 16 * self.norb), _ptr(out), self._stream(stream)),
                "td1c_ipd")
         return out
+
+    def orth(self, wfn, stream=None):
+        """clwfn::orth (schmidt) in place on a device wave function."""
+        torch = self.torch
+        if wfn.dtype != torch.complex128 or not wfn.is_cuda or wfn.numel() != self.wfn_size:
+            raise ValueError("wfn must be a complex128 CUDA tensor of wfn_size elements")
+        _check(load_library().td1c_orth(self.h, ctypes.c_void_p(wfn.data_ptr()), self._stream(stream)), "td1c_orth")
+        return wfn
+
+    def init_split2(self, wfn, dstep: float = 0.01, tolene: float = 1e-13, maxcyc: int = 1000000,
+                    lfield: Optional[Sequence[float]] = None, rk_order: int = 4, stream=None):
+        """init_split2 (imaginary-time relaxation with the RK propagator) in
+        place on a device wave function; returns (energy, last cycle).  The
+        problem must have icomp = 0."""
+        torch = self.torch
+        if wfn.dtype != torch.complex128 or not wfn.is_cuda or wfn.numel() != self.wfn_size:
+            raise ValueError("wfn must be a complex128 CUDA tensor of wfn_size elements")
+        lf = np.zeros(9) if lfield is None else np.ascontiguousarray(lfield, dtype=np.float64)
+        if lf.size != 9:
+            raise ValueError("lfield must have 9 entries")
+        e = ctypes.c_double()
+        n = ctypes.c_int()
+        _check(load_library().td1c_init_split2(self.h, int(rk_order), float(dstep), float(tolene), int(maxcyc),
+                                               _ptr(lf), ctypes.c_void_p(wfn.data_ptr()), ctypes.byref(e),
+                                               ctypes.byref(n), self._stream(stream)), "td1c_init_split2")
+        return e.value, n.value
 
     def mask(self, wfn, stream=None):
         _check(load_library().td1c_mask(self.h, ctypes.c_void_p(wfn.data_ptr()), self._stream(stream)), "td1c_mask")

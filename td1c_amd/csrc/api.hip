@@ -1185,6 +1185,17 @@ int td1c_mask(td1c_ctx* c, double* d_wfn, void* stream) {
   return 0;
 }
 
+int td1c_orth(td1c_ctx* c, double* d_wfn, void* stream) {
+  if (!c || !d_wfn) return fail(-1, "null argument");
+  Setup& S = c->S;
+  double2* w = reinterpret_cast<double2*>(d_wfn);
+  // wfn_orth_cic normalises the CI vector whatever the active space (one
+  // coefficient without active orbitals)
+  launch_orth(w, w + (size_t)c->nbas * S.nfun, c->mval.p, S.nr, S.L1, S.nfcore, S.nfun, S.lcic, (hipStream_t)stream);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
 // clcrk tableaux for rk_order 1..4 (src/prop/clcrk.cpp:183-235)
 static int rk_tableau(int order, int& ns, double* nodes, double* w, double (*a)[kMaxStages]) {
   for (int i = 0; i < kMaxStages; ++i) {
@@ -1274,6 +1285,42 @@ int td1c_rk_prop(td1c_ctx* c, int rk_order, double dtime, const double* lfield_s
   hipStream_t s = (hipStream_t)stream;
   HIP_TRY(hipMemcpyAsync(c->lf_dev.p, lfield_stages, 9 * ns * sizeof(double), hipMemcpyHostToDevice, s));
   return enqueue_rk_step(c, rk_order, dtime, c->lf_dev.p, reinterpret_cast<double2*>(d_wfn), apply_mask, s);
+}
+
+// init_split2 with split_type 0 (src/main/init.cpp:178-240): imaginary-time
+// relaxation.  Each cycle evaluates the energy at lfield (hprod_energy), stops
+// when it changed by less than tolene, else takes one rk_order step of dstep
+// (clcrk::prop, no mask) and orthonormalises (clwfn::orth, schmidt).  *ene
+// receives the last energy, *ncyc the last cycle index.
+int td1c_init_split2(td1c_ctx* c, int rk_order, double dstep, double tolene, int maxcyc, const double* lfield,
+                     double* d_wfn, double* ene, int* ncyc, void* stream) {
+  if (!c || !lfield || !d_wfn || !ene || !ncyc) return fail(-1, "null argument");
+  if (c->S.icomp != 0) return fail(-1, "imaginary-time relaxation needs a context with icomp = 0");
+  if (maxcyc < 0 || !(dstep > 0.0)) return fail(-1, "maxcyc must be >= 0 and dstep > 0");
+  int ns;
+  double nodes[kMaxStages], w[kMaxStages], a[kMaxStages][kMaxStages];
+  if (rk_tableau(rk_order, ns, nodes, w, a)) return fail(-1, "rk_order must be 1..4");
+  std::vector<double> lfs((size_t)9 * ns);
+  for (int st = 0; st < ns; ++st) std::copy(lfield, lfield + 9, lfs.begin() + 9 * st);
+  const size_t no = (size_t)c->nbas * c->S.nfun;
+  double* scratch = reinterpret_cast<double*>(c->rk_t.p);
+  double e1 = -1.0e10, e[5] = {0, 0, 0, 0, 0};
+  int last = 0, rc = 0;
+  for (int icyc = 0; icyc < maxcyc; ++icyc) {
+    rc = td1c_htot(c, dstep, lfield, d_wfn, d_wfn + 2 * no, scratch, scratch + 2 * no, stream);
+    if (rc == 0) rc = td1c_get_energies(c, e);
+    if (rc) return rc;
+    last = icyc;
+    if (std::fabs(e[4] - e1) < tolene) break;
+    e1 = e[4];
+    rc = td1c_rk_prop(c, rk_order, dstep, lfs.data(), d_wfn, 0, stream);
+    if (rc == 0) rc = td1c_orth(c, d_wfn, stream);
+    if (rc) return rc;
+  }
+  HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+  *ene = e[4];
+  *ncyc = last;
+  return 0;
 }
 
 int td1c_propagate(td1c_ctx* c, const td1c_field* field, int rk_order, double t0, double dtime, int nstep,

@@ -1931,6 +1931,68 @@ void launch_window_ovlp(const double2* orb, const int* mval, int nf, int nr, int
 }
 
 // ---------------------------------------------------------------------------
+// clwfn::orth, orth_type "schmidt" (src/wfn/clwfn.cpp:313-321;
+// src/wfn/wfn_orth.F90:2-176): the CI vector normalised, then each dynamical
+// orbital i in order made orthogonal to every earlier orbital of the same m
+// by modified Gram-Schmidt and normalised, over l >= |m| (a contiguous
+// segment of the [l][r] layout).  One workgroup: every thread owns the same
+// elements in every pass, so the only cross-thread traffic is the block sums.
+// ---------------------------------------------------------------------------
+constexpr int kOrthThreads = 1024;
+
+__device__ double2 orth_block_sum(double2 v, double2* red, double2* bcast) {
+  for (int o = 32; o > 0; o >>= 1) {
+    v.x += __shfl_xor(v.x, o, 64);
+    v.y += __shfl_xor(v.y, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double2 t = red[0];
+    for (int w = 1; w < kOrthThreads / 64; ++w) t = cadd(t, red[w]);
+    *bcast = t;
+  }
+  __syncthreads();
+  return *bcast;
+}
+
+__global__ void __launch_bounds__(kOrthThreads) k_orth(double2* __restrict__ orb, double2* __restrict__ cic,
+                                                       const int* __restrict__ mval, int nr, int L1, int nfcore,
+                                                       int nfun, int lcic) {
+  __shared__ double2 red[kOrthThreads / 64];
+  __shared__ double2 bcast;
+  if (lcic > 0) {
+    double2 a = make_double2(0.0, 0.0);
+    for (int q = threadIdx.x; q < lcic; q += kOrthThreads) a.x += cic[q].x * cic[q].x + cic[q].y * cic[q].y;
+    const double s = 1.0 / sqrt(orth_block_sum(a, red, &bcast).x);
+    for (int q = threadIdx.x; q < lcic; q += kOrthThreads) cic[q] = cscale(cic[q], s);
+  }
+  for (int i = nfcore; i < nfun; ++i) {
+    const int m = mval[i], am = m < 0 ? -m : m;
+    const int n = (L1 - am) * nr;
+    double2* oi = orb + ((size_t)i * L1 + am) * nr;
+    for (int j = 0; j < i; ++j) {
+      if (mval[j] != m) continue;
+      const double2* oj = orb + ((size_t)j * L1 + am) * nr;
+      double2 a = make_double2(0.0, 0.0);
+      for (int q = threadIdx.x; q < n; q += kOrthThreads) a = cadd(a, cmulc(oj[q], oi[q]));
+      const double2 t = orth_block_sum(a, red, &bcast);
+      for (int q = threadIdx.x; q < n; q += kOrthThreads) oi[q] = make_double2(oi[q].x - (oj[q].x * t.x - oj[q].y * t.y),
+                                                                              oi[q].y - (oj[q].x * t.y + oj[q].y * t.x));
+    }
+    double2 a = make_double2(0.0, 0.0);
+    for (int q = threadIdx.x; q < n; q += kOrthThreads) a.x += oi[q].x * oi[q].x + oi[q].y * oi[q].y;
+    const double s = 1.0 / sqrt(orth_block_sum(a, red, &bcast).x);
+    for (int q = threadIdx.x; q < n; q += kOrthThreads) oi[q] = cscale(oi[q], s);
+  }
+}
+
+void launch_orth(double2* orb, double2* cic, const int* mval, int nr, int L1, int nfcore, int nfun, int lcic,
+                 hipStream_t s) {
+  hipLaunchKernelGGL(k_orth, dim3(1), dim3(kOrthThreads), 0, s, orb, cic, mval, nr, L1, nfcore, nfun, lcic);
+}
+
+// ---------------------------------------------------------------------------
 // Small helpers that keep memset/memcpy nodes out of captured graphs.
 // ---------------------------------------------------------------------------
 __global__ void k_zero(double2* __restrict__ y, size_t n) {

@@ -216,6 +216,9 @@ void launch_dipole(const DipoleArgs& a, int nforms, double2* partial, double* ou
 // ovlp[j + nf*i] = <orb_j|orb_i> over the radial rows r < cut (m-matched), nf*nf entries
 void launch_window_ovlp(const double2* orb, const int* mval, int nf, int nr, int L1, int cut, double2* ovlp,
                         hipStream_t s);
+// clwfn::orth (schmidt): CI normalised (lcic > 0), dynamical orbitals by modified Gram-Schmidt
+void launch_orth(double2* orb, double2* cic, const int* mval, int nr, int L1, int nfcore, int nfun, int lcic,
+                 hipStream_t s);
 void launch_to_f32(float2* y, const double2* x, size_t n, hipStream_t s);
 void launch_zero(double2* y, size_t n, hipStream_t s);
 void launch_field_stages(const Pulse& p, PropClock* st, const StageNodes& nodes, double* lf, hipStream_t s);

@@ -77,3 +77,10 @@ def wfn_for(prob: Problem, lcic: int, seed: int = 11) -> np.ndarray:
 
 LF_LEN = np.array([0.0, 0.0, 0.05, 0.0, 0.0, 0.05, 0.0, 0.0, 0.0])
 LF_VEL = np.array([0.0, 0.0, 0.3, 0.0, 0.0, 0.04, 0.0, 0.0, 0.3])
+
+
+def he_itr(**kw):
+    # He for imaginary-time relaxation (icomp = 0): a grid coarse enough for an
+    # explicit RK4 step of 4e-3
+    kw.setdefault("icomp", 0)
+    return Problem(nfe=12, ndvr=10, rmax=30.0, lmax1=1, lmax2=2, znuc=2.0, ndcore=1, nel_alpha=1, nel_beta=1, **kw)

@@ -780,3 +780,72 @@ def test_ipd_exact_for_two_electrons():
     a, b = o.ipd(2, 1.0, np.zeros(9), w), o.ipx(2, 1.0, np.zeros(9), w)
     assert b[1] > 1e-2 and b[2] > 1e-2
     assert np.abs(a - b).max() < 1e-13
+
+
+def test_orth_modified_gram_schmidt():
+    """orc_orth (wfn_orth_cic + wfn_orth_orb): afterwards the orbitals are
+    orthonormal within each m, the first dynamical orbital keeps its direction,
+    every orbital stays in the span of itself and the earlier ones (the
+    triangular structure of Gram-Schmidt), frozen cores are untouched and the
+    CI vector has norm 1."""
+    import tests.cases as C
+
+    for name in ("dc3_small", "cas_fc_small"):
+        prob = getattr(C, name)()
+        o = O.Oracle(prob)
+        w = C.wfn_for(prob, o.lcic)
+        nf, nb, nfc = prob.nfun, prob.nr * (prob.lmax1 + 1), prob.nfcore
+        rng = np.random.default_rng(3)
+        noise = 0.05 * (rng.standard_normal(w.size) + 1j * rng.standard_normal(w.size))
+        noise[: nfc * nb] = 0.0  # frozen cores stay orthonormal, as the reference assumes
+        w = w + noise
+        mv = np.array(prob.mval if prob.mval is not None else [0] * nf)
+        L1 = prob.lmax1 + 1
+        for i in range(nf):  # components below |m| are outside the orbital's space
+            w[i * nb: i * nb + abs(mv[i]) * prob.nr] = 0.0
+        x = o.orth(w)
+        a, b = w[: nf * nb].reshape(nf, nb), x[: nf * nb].reshape(nf, nb)
+        S = (b.conj() @ b.T) * (mv[:, None] == mv[None, :])
+        assert np.abs(S - np.eye(nf)).max() < 1e-12
+        for i in range(nfc, nf):
+            for j in range(i):
+                if mv[j] == mv[i]:
+                    assert abs(np.vdot(b[j], b[i])) < 1e-12
+            assert abs(np.linalg.norm(b[i]) - 1) < 1e-13
+            # b_i lies in the span of a_j, j <= i, of the same m
+            basis = np.array([a[j] for j in range(i + 1) if mv[j] == mv[i]]).T
+            coef, *_ = np.linalg.lstsq(basis, b[i], rcond=None)
+            assert np.linalg.norm(basis @ coef - b[i]) < 1e-10
+        assert np.array_equal(a[:nfc], b[:nfc])
+        i0 = next((i for i in range(nfc, nf) if not any(mv[j] == mv[i] for j in range(i))), None)
+        if i0 is not None:
+            assert np.abs(b[i0] - a[i0] / np.linalg.norm(a[i0])).max() < 1e-14
+        if o.lcic > 0 and prob.nact > 0:
+            assert abs(np.linalg.norm(x[nf * nb:]) - 1) < 1e-14
+
+
+def _he_itr():
+    return Problem(nfe=12, ndvr=10, rmax=30.0, lmax1=1, lmax2=2, znuc=2.0, ndcore=1, nel_alpha=1, nel_beta=1, icomp=0)
+
+
+def test_imaginary_time_he_hartree_fock_energy():
+    """The oracle's imaginary-time hprod (icomp = 0: tfac = -dt), RK4 and
+    orc_orth, iterated as init_split2 does, relax He to its Hartree-Fock
+    ground state: E = -2.8616800 (Hartree-Fock limit), here -2.8616818 on a
+    12 x 10 grid of [0, 30]; a wrong sign or factor in any part of the
+    imaginary-time equations misses by far more than 1e-5."""
+    prob = _he_itr()
+    o = O.Oracle(prob)
+    w = W.make_wfn(prob, 3, o.lcic)
+    o.set_wfn0(w)
+    lf = np.zeros((4, 9))
+    e1 = 1e10
+    for _ in range(5000):
+        o.htot(4e-3, lf[0], w)
+        e = o.energies()[4]
+        if abs(e - e1) < 1e-12:
+            break
+        e1 = e
+        w = o.orth(o.rk_prop(4, 4e-3, lf, w, apply_mask=False))
+    assert abs(e - e1) < 1e-12
+    assert abs(e - (-2.8616800)) < 1e-5, e
