@@ -206,6 +206,23 @@ int td1c_init_split2(td1c_ctx* ctx, int rk_order, double dstep, double tolene,
                      int maxcyc, const double* lfield, double* d_wfn,
                      double* ene, int* ncyc, void* stream);
 
+/* Wave-function files in the reference's text format, host memory only (no
+ * GPU call).  td1c_wfn_write replaces clwfn::write
+ * (src/wfn/clwfn.cpp:268-302): the .orb file holds nrad, lmax1, mmax1, nfun
+ * ("%10d" lines) then every orbital coefficient as "%25.15le%25.15le" in
+ * [orbital][l][r] order (nrad-1 radial points); the .cic file holds lcic then
+ * the CI coefficients.  wfn: the state as complex128 (orbitals then CI). */
+int td1c_wfn_write(const char* orb_path, const char* cic_path, int nrad, int lmax1,
+                   int mmax1, int nfun, int lcic, const double* wfn);
+
+/* clwfn::read (src/wfn/clwfn.cpp:156-205): wfn (sized for nrad, lmax1, nfun,
+ * lcic) is cleared and filled from the files; orbitals beyond the current
+ * sizes are dropped, missing ones stay zero.  The CI vector is read when the
+ * file's lcic matches; otherwise it is set to the first determinant
+ * (ormas_cic0; left zero when nact == 0) and the call returns 1. */
+int td1c_wfn_read(const char* orb_path, const char* cic_path, int nrad, int lmax1,
+                  int nfun, int lcic, int nact, double* wfn);
+
 /* One explicit Runge-Kutta step of the whole wave function (orbitals then
  * CI) held on device: clcrk::prop with rk_order 1..4 ("standard").
  * lfield_stages holds the 9 field values at t + c_s*dt for each stage

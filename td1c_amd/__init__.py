@@ -22,7 +22,8 @@ import numpy as np
 
 from .abi import HEADER_FUNCTIONS, Field, Problem, TD1CField, TD1CProblem, TD1CSizes
 
-__all__ = ["Problem", "Field", "HProd", "CRK", "load_library", "library_path", "field_value", "field_period"]
+__all__ = ["Problem", "Field", "HProd", "CRK", "load_library", "library_path", "field_value", "field_period",
+           "wfn_write", "wfn_read"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "libtd1c_hprod.so")
@@ -78,6 +79,12 @@ def load_library():
     L.td1c_init_split2.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, vp, vp,
                                    ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int), vp]
     L.td1c_init_split2.restype = ctypes.c_int
+    cp = ctypes.c_char_p
+    ci = ctypes.c_int
+    L.td1c_wfn_write.argtypes = [cp, cp, ci, ci, ci, ci, ci, vp]
+    L.td1c_wfn_write.restype = ci
+    L.td1c_wfn_read.argtypes = [cp, cp, ci, ci, ci, ci, ci, vp]
+    L.td1c_wfn_read.restype = ci
     L.td1c_mask.argtypes = [vp, vp, vp]
     L.td1c_mask.restype = ctypes.c_int
     L.td1c_field_value.argtypes = [ctypes.POINTER(TD1CField), ctypes.c_double, vp]
@@ -123,6 +130,29 @@ def field_value(field: Field, t: float) -> np.ndarray:
 def field_period(field: Field) -> float:
     c = field.to_c()
     return float(load_library().td1c_field_period(ctypes.byref(c)))
+
+
+def wfn_write(orb_path: str, cic_path: str, prob: Problem, lcic: int, wfn: np.ndarray):
+    """clwfn::write: the state (host complex128, orbitals then CI) to the
+    reference's .orb / .cic text files."""
+    w = np.ascontiguousarray(wfn, dtype=np.complex128)
+    nrad = prob.nr + 1
+    if w.size != prob.nfun * prob.nbas + lcic:
+        raise ValueError("wave function size does not match the problem")
+    mmax1 = prob.mmax1 if prob.mmax1 >= 0 else max([abs(m) for m in (prob.mval or [0])])
+    _check(load_library().td1c_wfn_write(orb_path.encode(), cic_path.encode(), nrad, prob.lmax1, mmax1,
+                                         prob.nfun, lcic, _ptr(w)), "td1c_wfn_write")
+
+
+def wfn_read(orb_path: str, cic_path: str, prob: Problem, lcic: int):
+    """clwfn::read: returns (state, ci_read) for the problem's sizes."""
+    nrad = prob.nr + 1
+    w = np.zeros(prob.nfun * prob.nbas + lcic, np.complex128)
+    rc = load_library().td1c_wfn_read(orb_path.encode(), cic_path.encode(), nrad, prob.lmax1, prob.nfun, lcic,
+                                      prob.nact, _ptr(w))
+    if rc not in (0, 1):
+        _check(rc, "td1c_wfn_read")
+    return w, rc == 0
 
 
 class HProd:

@@ -1196,6 +1196,83 @@ int td1c_orth(td1c_ctx* c, double* d_wfn, void* stream) {
   return 0;
 }
 
+// clwfn::write (src/wfn/clwfn.cpp:268-302): the .orb file holds nrad, lmax1,
+// mmax1, nfun ("%10d" lines) then every orbital coefficient as "%25.15le%25.15le"
+// in [orbital][l][r] order; the .cic file holds lcic then the CI coefficients.
+int td1c_wfn_write(const char* orb_path, const char* cic_path, int nrad, int lmax1, int mmax1, int nfun, int lcic,
+                   const double* wfn) {
+  if (!orb_path || !cic_path || !wfn) return fail(-1, "null argument");
+  if (nrad < 2 || lmax1 < 0 || mmax1 < 0 || nfun < 0 || lcic < 0) return fail(-1, "bad sizes");
+  FILE* fo = std::fopen(orb_path, "w");
+  if (!fo) return fail(-5, std::string("cannot open ") + orb_path);
+  std::fprintf(fo, "%10d\n%10d\n%10d\n%10d\n", nrad, lmax1, mmax1, nfun);
+  const size_t no = (size_t)nfun * (lmax1 + 1) * (nrad - 1);
+  for (size_t q = 0; q < no; ++q) std::fprintf(fo, "%25.15le%25.15le\n", wfn[2 * q], wfn[2 * q + 1]);
+  const bool ok_o = std::fclose(fo) == 0;
+  FILE* fc = std::fopen(cic_path, "w");
+  if (!fc) return fail(-5, std::string("cannot open ") + cic_path);
+  std::fprintf(fc, "%10d\n", lcic);
+  for (int d = 0; d < lcic; ++d) std::fprintf(fc, "%25.15le%25.15le\n", wfn[2 * (no + d)], wfn[2 * (no + d) + 1]);
+  const bool ok_c = std::fclose(fc) == 0;
+  if (!ok_o || !ok_c) return fail(-5, "write error");
+  return 0;
+}
+
+// clwfn::read (src/wfn/clwfn.cpp:156-205): the wave function is cleared, the
+// file's orbitals are read and those inside (nfun, lmax1, nrad) kept (a
+// smaller file leaves zeros, a larger one is truncated); the CI vector is read
+// when the file's lcic equals lcic, else it is set to the first determinant
+// (ormas_cic0; left zero without active orbitals, nact == 0) and 1 is returned.
+int td1c_wfn_read(const char* orb_path, const char* cic_path, int nrad, int lmax1, int nfun, int lcic, int nact,
+                  double* wfn) {
+  if (!orb_path || !cic_path || !wfn) return fail(-1, "null argument");
+  if (nrad < 2 || lmax1 < 0 || nfun < 0 || lcic < 0) return fail(-1, "bad sizes");
+  const size_t no = (size_t)nfun * (lmax1 + 1) * (nrad - 1);
+  std::fill(wfn, wfn + 2 * (no + lcic), 0.0);
+  FILE* fo = std::fopen(orb_path, "r");
+  if (!fo) return fail(-5, std::string("cannot open ") + orb_path);
+  int nrad0, lmax0, mmax0, nfun0;
+  if (std::fscanf(fo, "%d %d %d %d", &nrad0, &lmax0, &mmax0, &nfun0) != 4) {
+    std::fclose(fo);
+    return fail(-5, std::string("bad header in ") + orb_path);
+  }
+  for (int f = 0; f < nfun0; ++f)
+    for (int l = 0; l <= lmax0; ++l)
+      for (int r = 1; r < nrad0; ++r) {
+        double re, im;
+        if (std::fscanf(fo, "%le %le", &re, &im) != 2) {
+          std::fclose(fo);
+          return fail(-5, std::string("short file ") + orb_path);
+        }
+        if (f < nfun && l <= lmax1 && r < nrad) {
+          const size_t q = ((size_t)f * (lmax1 + 1) + l) * (nrad - 1) + (r - 1);
+          wfn[2 * q] = re;
+          wfn[2 * q + 1] = im;
+        }
+      }
+  std::fclose(fo);
+  FILE* fc = std::fopen(cic_path, "r");
+  if (!fc) return fail(-5, std::string("cannot open ") + cic_path);
+  int lcic0;
+  if (std::fscanf(fc, "%d", &lcic0) != 1) {
+    std::fclose(fc);
+    return fail(-5, std::string("bad header in ") + cic_path);
+  }
+  int rc = 0;
+  if (lcic0 != lcic) {
+    if (nact > 0 && lcic > 0) wfn[2 * no] = 1.0;
+    rc = 1;
+  } else {
+    for (int d = 0; d < lcic; ++d)
+      if (std::fscanf(fc, "%le %le", &wfn[2 * (no + d)], &wfn[2 * (no + d) + 1]) != 2) {
+        std::fclose(fc);
+        return fail(-5, std::string("short file ") + cic_path);
+      }
+  }
+  std::fclose(fc);
+  return rc;
+}
+
 // clcrk tableaux for rk_order 1..4 (src/prop/clcrk.cpp:183-235)
 static int rk_tableau(int order, int& ns, double* nodes, double* w, double (*a)[kMaxStages]) {
   for (int i = 0; i < kMaxStages; ++i) {
