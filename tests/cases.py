@@ -84,3 +84,18 @@ def he_itr(**kw):
     # explicit RK4 step of 4e-3
     kw.setdefault("icomp", 0)
     return Problem(nfe=12, ndvr=10, rmax=30.0, lmax1=1, lmax2=2, znuc=2.0, ndcore=1, nel_alpha=1, nel_beta=1, **kw)
+
+
+def cas6_small(**kw):
+    # 1 dynamical core + CAS(4e, 6o) with m = 0, +-1 active orbitals (M = 0 determinants only)
+    kw.setdefault("mval", [0, 0, 0, 1, -1, 0, 0])
+    return Problem(nfe=6, ndvr=20, rmax=18.0, lmax1=4, lmax2=8, znuc=6.0, ndcore=1, nact=6,
+                   nel_alpha=3, nel_beta=3, **kw)
+
+
+def cas7_small(**kw):
+    # CAS(6e, 7o), no core: 1225 determinants, the CI work in global memory
+    # (beyond the island's LDS budget); 7 active orbitals is the largest space
+    # whose 28 active pairs fit the int2e Gram launch (28^2 <= 1024 entries)
+    return Problem(nfe=6, ndvr=20, rmax=18.0, lmax1=4, lmax2=8, znuc=6.0, ndcore=0, nact=7,
+                   nel_alpha=3, nel_beta=3, **kw)

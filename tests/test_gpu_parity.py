@@ -44,12 +44,14 @@ def _run(prob, lfield, seed=11, dt=1.0):
                                      ("ne_small_len", C.LF_LEN), ("cas_small", C.LF_LEN),
                                      ("cas_fc_small", C.LF_VEL), ("he_n16", C.LF_VEL),
                                      ("dc4_small", C.LF_LEN), ("fc2_small", C.LF_VEL),
-                                     ("he_long", C.LF_LEN)])
+                                     ("he_long", C.LF_LEN), ("cas6_small", C.LF_LEN),
+                                     ("cas7_small", C.LF_LEN)])
 def test_htot_matches_oracle(gpu_lib, name, lf):
     mk = {"he_small": C.he_small, "he_small_vel": lambda: C.he_small(igauge=1), "dc3_small": C.dc3_small,
           "ne_small": C.ne_small, "ne_small_len": lambda: C.ne_small(igauge=0), "cas_small": C.cas_small,
           "cas_fc_small": C.cas_fc_small, "he_n16": lambda: C.he_n16(igauge=1), "dc4_small": C.dc4_small,
-          "fc2_small": C.fc2_small, "he_long": C.he_long}[name]
+          "fc2_small": C.fc2_small, "he_long": C.he_long, "cas6_small": C.cas6_small,
+          "cas7_small": C.cas7_small}[name]
     prob = mk()
     print(f"case {name}", flush=True)
     o, ref, got, eref, e = _run(prob, lf)
@@ -342,7 +344,8 @@ MIXED_RTOL = 1e-5
 
 
 @pytest.mark.parametrize("name,lf", [("he_small", C.LF_LEN), ("dc3_small", C.LF_LEN), ("ne_small", C.LF_VEL),
-                                     ("cas_fc_small", C.LF_VEL), ("cas_small", C.LF_LEN), ("dc4_small", C.LF_LEN)])
+                                     ("cas_fc_small", C.LF_VEL), ("cas_small", C.LF_LEN), ("dc4_small", C.LF_LEN),
+                                     ("cas6_small", C.LF_LEN), ("cas7_small", C.LF_LEN)])
 def test_mixed_htot_close_to_oracle(gpu_lib, name, lf):
     import torch
     import td1c_amd
