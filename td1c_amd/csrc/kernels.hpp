@@ -79,7 +79,7 @@ struct MfArgs {
   const double2* mfc;
 };
 
-constexpr int kMaxGram = 1024;   // nL * nR entries of one Gram launch (4 per thread)
+constexpr int kMaxGram = 2048;   // nL * nR entries of one Gram launch (bounds its partial-sum buffer)
 
 struct GramArgs {
   size_t len;

@@ -95,7 +95,13 @@ def cas6_small(**kw):
 
 def cas7_small(**kw):
     # CAS(6e, 7o), no core: 1225 determinants, the CI work in global memory
-    # (beyond the island's LDS budget); 7 active orbitals is the largest space
-    # whose 28 active pairs fit the int2e Gram launch (28^2 <= 1024 entries)
+    # (beyond the island's LDS budget)
     return Problem(nfe=6, ndvr=20, rmax=18.0, lmax1=4, lmax2=8, znuc=6.0, ndcore=0, nact=7,
+                   nel_alpha=3, nel_beta=3, **kw)
+
+
+def cas8_small(**kw):
+    # CAS(6e, 8o), no core: 3136 determinants and 36 active pairs (1296 int2e
+    # Gram entries), the largest active space of the theta-grid mean-field kernel
+    return Problem(nfe=6, ndvr=20, rmax=18.0, lmax1=4, lmax2=8, znuc=6.0, ndcore=0, nact=8,
                    nel_alpha=3, nel_beta=3, **kw)

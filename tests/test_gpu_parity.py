@@ -45,13 +45,13 @@ def _run(prob, lfield, seed=11, dt=1.0):
                                      ("cas_fc_small", C.LF_VEL), ("he_n16", C.LF_VEL),
                                      ("dc4_small", C.LF_LEN), ("fc2_small", C.LF_VEL),
                                      ("he_long", C.LF_LEN), ("cas6_small", C.LF_LEN),
-                                     ("cas7_small", C.LF_LEN)])
+                                     ("cas7_small", C.LF_LEN), ("cas8_small", C.LF_LEN)])
 def test_htot_matches_oracle(gpu_lib, name, lf):
     mk = {"he_small": C.he_small, "he_small_vel": lambda: C.he_small(igauge=1), "dc3_small": C.dc3_small,
           "ne_small": C.ne_small, "ne_small_len": lambda: C.ne_small(igauge=0), "cas_small": C.cas_small,
           "cas_fc_small": C.cas_fc_small, "he_n16": lambda: C.he_n16(igauge=1), "dc4_small": C.dc4_small,
           "fc2_small": C.fc2_small, "he_long": C.he_long, "cas6_small": C.cas6_small,
-          "cas7_small": C.cas7_small}[name]
+          "cas7_small": C.cas7_small, "cas8_small": C.cas8_small}[name]
     prob = mk()
     print(f"case {name}", flush=True)
     o, ref, got, eref, e = _run(prob, lf)
