@@ -36,7 +36,7 @@
  *    without (-1)^m, as sph_gen.F90 does; only products of two columns enter
  *    hprod, so no pinned quantity sees it);
  *  - the order and phase of the CAS determinants (the cic layout);
- *  - ene_fcore (not computed, 0) and the KLI/LDA v2xfc (input only).
+ *  - the KLI/LDA v2xfc (input only).
  */
 #include <complex.h>
 #include <math.h>
@@ -1568,6 +1568,51 @@ void orc_set_wfn0(orc* o, const zc* wfn, const zc* cic, const zc* v2xfc_in) {
     memcpy(o->v2xfc, v2xfc_in, sizeof(zc) * nr);
     if (o->xfc_implicit)
       for (int i = 0; i < nr; ++i) o->v2jfc[i] += o->v2xfc[i];
+  }
+  /* hprod_v2fc_enefc (:790-865), before hprod_addfc_tmat: 2 <fc|h0|fc> with
+   * the bare one-body operator, plus <fc|g|fc> with g = v2jfc (fcj),
+   * v2xfc on every frozen core when type -2 cores exist (fcx2) and minus
+   * the exchange with the type -1 cores through the FC-FC pair potentials
+   * (fcx1; hprod_mkrho2p / mkv2mf_poisson / sph2ang2 / herm 'fcfc', :167-173),
+   * on the theta grid, projected back by bas_ang2sph1_fc */
+  {
+    int ng = o->ngrid;
+    if (o->igauge == 0) sph2ang1(o, o->orb, o->orbg, 0, nfc, o->nradfc);
+    if (o->nfcore1 > 0) {
+      mkrho2(o, 2);
+      poisson(o, 2);
+      sph2ang2(o, 2);
+      v2herm(o, 2);
+    }
+    zc* h0 = xcalloc((size_t)nb * nfc, sizeof(zc));
+    tprod(o, o->orb, h0, 0, nfc, o->nradfc);
+    zc t = 0;
+    for (int ifun = 0; ifun < nfc; ++ifun)
+      for (int l = 0; l <= o->lmax1; ++l)
+        for (int i = 0; i < o->nradfc; ++i) t += conj(o->orb[OIDX(i, l, ifun)]) * h0[OIDX(i, l, ifun)];
+    double enefc = 2.0 * creal(t);
+    zc* g = xcalloc((size_t)ng * nfc, sizeof(zc));
+    for (int ifun = 0; ifun < nfc; ++ifun)
+      for (int ilat = 0; ilat < o->nlat; ++ilat)
+        for (int i = 0; i < o->nradfc; ++i) {
+          zc v = o->v2jfc[i] + (o->nfcore2 > 0 ? o->v2xfc[i] : 0);
+          g[GIDX(i, ilat, ifun)] += v * o->orbg[GIDX(i, ilat, ifun)];
+        }
+    if (o->nfcore1 > 0)
+      for (int ifun = 0; ifun < nfc; ++ifun)
+        for (int jfun = o->nfcore2; jfun < nfc; ++jfun)
+          for (int ilat = 0; ilat < o->nlat; ++ilat)
+            for (int i = 0; i < o->nradfc; ++i)
+              g[GIDX(i, ilat, ifun)] -= o->v2ang[GIDX2(i, ilat, jfun, ifun)] * o->orbg[GIDX(i, ilat, jfun)];
+    memset(h0, 0, sizeof(zc) * (size_t)nb * nfc);
+    ang2sph1(o, g, h0, 0, nfc, o->nradfc);
+    t = 0;
+    for (int ifun = 0; ifun < nfc; ++ifun)
+      for (int l = 0; l <= o->lmax1; ++l)
+        for (int i = 0; i < o->nradfc; ++i) t += conj(o->orb[OIDX(i, l, ifun)]) * h0[OIDX(i, l, ifun)];
+    o->ene_fcore = enefc + creal(t);
+    free(g);
+    free(h0);
   }
   /* hprod_addfc_tmat (src/hprod/hprod_addfc_tmat.F90) */
   if (o->jfc_implicit) {

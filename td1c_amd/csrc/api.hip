@@ -788,13 +788,15 @@ hipError_t join_island(td1c_ctx* c, hipStream_t s) {
   return c->x_local ? hipStreamWaitEvent(s, c->fj[6], 0) : hipSuccess;
 }
 
-// Host banded solve of the L = 0 Poisson operator for the frozen-core
-// Hartree potential (set-up only, once per run): dense Cholesky of the band.
-std::vector<double> host_poisson_l0(const Setup& S, const std::vector<double>& rhs) {
+// Host band Cholesky factor of the Poisson operator A_L = 2 K + diag(L(L+1)/r^2)
+// (bas_gen_d2ll) for the frozen-core set-up (once per run): lower band
+// ab[(n+1)*j + (i-j)].
+std::vector<double> host_poisson_factor(const Setup& S, int L) {
   const int nr = S.nr, n = S.n, bw = 2 * n + 1;
-  std::vector<double> ab((size_t)(n + 1) * nr, 0.0);  // lower band, ab[(n+1)*j + (i-j)]
+  std::vector<double> ab((size_t)(n + 1) * nr, 0.0);
   for (int j = 0; j < nr; ++j)
     for (int i = j; i <= std::min(nr - 1, j + n); ++i) ab[(size_t)(n + 1) * j + (i - j)] = 2.0 * S.radk[(size_t)bw * (j + 1) + n + (i - j)];
+  for (int j = 0; j < nr; ++j) ab[(size_t)(n + 1) * j] += (double)(L * (L + 1)) / (S.xr[j] * S.xr[j]);
   for (int j = 0; j < nr; ++j) {
     double d = std::sqrt(ab[(size_t)(n + 1) * j]);
     ab[(size_t)(n + 1) * j] = d;
@@ -804,6 +806,11 @@ std::vector<double> host_poisson_l0(const Setup& S, const std::vector<double>& r
       for (int r2 = c2; r2 <= kn; ++r2)
         ab[(size_t)(n + 1) * (j + c2) + (r2 - c2)] -= ab[(size_t)(n + 1) * j + r2] * ab[(size_t)(n + 1) * j + c2];
   }
+  return ab;
+}
+
+std::vector<double> host_poisson_solve(const Setup& S, const std::vector<double>& ab, const std::vector<double>& rhs) {
+  const int nr = S.nr, n = S.n;
   std::vector<double> x = rhs;
   for (int j = 0; j < nr; ++j) {
     x[j] /= ab[(size_t)(n + 1) * j];
@@ -815,6 +822,117 @@ std::vector<double> host_poisson_l0(const Setup& S, const std::vector<double>& r
     x[j] = t / ab[(size_t)(n + 1) * j];
   }
   return x;
+}
+
+std::vector<double> host_poisson_l0(const Setup& S, const std::vector<double>& rhs) {
+  return host_poisson_solve(S, host_poisson_factor(S, 0), rhs);
+}
+
+// Frozen-core energy (hprod_v2fc_enefc, src/hprod/hprod_set_wfn0.F90:790-865),
+// once at set-up: 2 <fc|h0|fc> with the bare one-body operator (before
+// hprod_addfc_tmat) plus <fc|g|fc>, g = v2jfc (fcj, v2xfc included when
+// implicit), v2xfc again on every frozen core when type -2 cores exist (fcx2),
+// and minus the exchange with the type -1 cores through the FC-FC pair
+// potentials (fcx1; hprod_mkrho2p / mkv2mf_poisson / sph2ang2 / herm 'fcfc'),
+// formed on the theta grid and projected back as bas_ang2sph1_fc does.
+// orb: frozen cores in l-space [f][l][r]; og: on the theta grid [f][t][r].
+double host_fcore_energy(const Setup& S, const std::vector<std::complex<double>>& orb,
+                         const std::vector<std::complex<double>>& og, const std::vector<double2>& v2jfc,
+                         const double* v2xfc) {
+  using cd = std::complex<double>;
+  const int nr = S.nr, n = S.n, bw = 2 * n + 1, L1 = S.L1, L2 = S.L2, nl = S.nlat, nfc = S.nfcore;
+  const int nrf = S.nradfc;
+  const size_t nb = (size_t)L1 * nr, ng = (size_t)nl * nr;
+  // 2 <fc|h0|fc>
+  cd t = 0.0;
+  for (int f = 0; f < nfc; ++f)
+    for (int l = 0; l < L1; ++l) {
+      const std::vector<double>& band = l == 0 ? S.radk : S.radk0;
+      const cd* p = orb.data() + (size_t)f * nb + (size_t)l * nr;
+      for (int i = 0; i < nrf; ++i) {
+        cd h = S.tdiag[2 * ((size_t)l * nr + i)] * p[i];
+        for (int j = std::max(0, i - n); j <= std::min(nr - 1, i + n); ++j)
+          if (j != i) h += band[(size_t)bw * (i + 1) + n + (j - i)] * p[j];
+        t += std::conj(p[i]) * h;
+      }
+    }
+  double e = 2.0 * t.real();
+  // g on the theta grid
+  std::vector<cd> g(ng * nfc, 0.0);
+  for (int f = 0; f < nfc; ++f)
+    for (int a = 0; a < nl; ++a)
+      for (int i = 0; i < nrf; ++i) {
+        cd v(v2jfc[i].x, v2jfc[i].y);
+        if (S.nfcore2 > 0 && v2xfc) v += cd(v2xfc[2 * i], v2xfc[2 * i + 1]);
+        g[(size_t)f * ng + (size_t)a * nr + i] += v * og[(size_t)f * ng + (size_t)a * nr + i];
+      }
+  if (S.nfcore1 > 0) {
+    // pair potentials v(a, b) of the pairs a >= b on the theta grid
+    std::vector<cd> vg((size_t)nfc * nfc * ng, 0.0), rl((size_t)L2 * nr), vl((size_t)L2 * nr);
+    std::vector<double> re(nr), im(nr);
+    std::vector<std::vector<double>> fac(L2);  // band factors, made on first use
+    for (int b = 0; b < nfc; ++b)
+      for (int a = b; a < nfc; ++a) {
+        const int mji = -S.mval[a] + S.mval[b], am = std::abs(mji);
+        const cd* oa = og.data() + (size_t)a * ng;
+        const cd* ob = og.data() + (size_t)b * ng;
+        std::fill(rl.begin(), rl.end(), 0.0);
+        for (int L = am; L < L2; ++L)
+          for (int q = 0; q < nl; ++q) {
+            const double w = S.mat_g2L[((size_t)(mji + S.mmax2) * nl + q) * L2 + L];
+            for (int i = 0; i < nrf; ++i) rl[(size_t)L * nr + i] += std::conj(oa[(size_t)q * nr + i]) * ob[(size_t)q * nr + i] * w;
+          }
+        std::fill(vl.begin(), vl.end(), 0.0);
+        for (int L = am; L < L2; ++L) {
+          const double* rpl0 = &S.d2rpl0[(size_t)L * nr];
+          const double* invr = &S.d2invr[(size_t)L * nr];
+          double tr = 0.0, ti = 0.0;
+          for (int i = 0; i < nr; ++i) {
+            const cd r = rl[(size_t)L * nr + i];
+            tr += r.real() * rpl0[i];
+            ti += r.imag() * rpl0[i];
+            re[i] = r.real() * invr[i];
+            im[i] = r.imag() * invr[i];
+          }
+          tr *= S.d2fac1[L];
+          ti *= S.d2fac1[L];
+          if (fac[L].empty()) fac[L] = host_poisson_factor(S, L);
+          const std::vector<double> xr = host_poisson_solve(S, fac[L], re), xi = host_poisson_solve(S, fac[L], im);
+          for (int i = 0; i < nr; ++i) {
+            const double f2 = S.d2fac2[(size_t)L * nr + i], r1 = S.d2rpl1[(size_t)L * nr + i];
+            vl[(size_t)L * nr + i] = cd((xr[i] + tr * r1) * f2, (xi[i] + ti * r1) * f2);
+          }
+        }
+        cd* v = vg.data() + ((size_t)a * nfc + b) * ng;
+        for (int q = 0; q < nl; ++q)
+          for (int L = am; L < L2; ++L) {
+            const double w = S.mat_L2g[((size_t)(mji + S.mmax2) * L2 + L) * nl + q];
+            for (int i = 0; i < nr; ++i) v[(size_t)q * nr + i] += vl[(size_t)L * nr + i] * w;
+          }
+      }
+    // fcx1: g_i -= v(j, i) phi_j for the type -1 cores j; v(j, i) = conj(v(i, j)) for j < i
+    for (int i = 0; i < nfc; ++i)
+      for (int j = S.nfcore2; j < nfc; ++j)
+        for (int q = 0; q < nl; ++q)
+          for (int r = 0; r < nrf; ++r) {
+            const size_t x = (size_t)q * nr + r;
+            const cd v = j >= i ? vg[((size_t)j * nfc + i) * ng + x] : std::conj(vg[((size_t)i * nfc + j) * ng + x]);
+            g[(size_t)i * ng + x] -= v * og[(size_t)j * ng + x];
+          }
+  }
+  // back to l-space (rows < nradfc) and <fc|g|fc>
+  t = 0.0;
+  for (int f = 0; f < nfc; ++f) {
+    const int m = S.mval[f];
+    for (int l = 0; l < L1; ++l)
+      for (int i = 0; i < nrf; ++i) {
+        cd s = 0.0;
+        for (int q = 0; q < nl; ++q)
+          s += S.mat_g2l[((size_t)(m + S.mmax1) * nl + q) * L1 + l] * g[(size_t)f * ng + (size_t)q * nr + i];
+        t += std::conj(orb[(size_t)f * nb + (size_t)l * nr + i]) * s;
+      }
+  }
+  return e + t.real();
 }
 
 constexpr double kPi = 3.14159265358979323846264338327950288;
@@ -940,6 +1058,11 @@ int td1c_set_wfn0(td1c_ctx* c, const double* wfn, const double* cic, const doubl
       v2jfc[i].x += v2xfc[2 * i];
       v2jfc[i].y += v2xfc[2 * i + 1];
     }
+  {
+    std::vector<std::complex<double>> og((size_t)c->ngrid * nfc);
+    HIP_TRY(hipMemcpy(og.data(), c->orbg0.p, og.size() * sizeof(double2), hipMemcpyDeviceToHost));
+    c->ene_fcore = host_fcore_energy(S, orbfc, og, v2jfc, v2xfc);
+  }
   // tmat += v2jfc (hprod_addfc_tmat)
   std::vector<double2> td((size_t)L1 * nr), tz((size_t)L1 * nr);
   for (int l = 0; l < L1; ++l)

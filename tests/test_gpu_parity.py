@@ -59,7 +59,7 @@ def test_htot_matches_oracle(gpu_lib, name, lf):
     assert _rel(got[:n], ref[:n]) < RTOL, (_rel(got[:n], ref[:n]))
     if o.prob.nact > 0:
         assert _rel(got[n:], ref[n:]) < 1e-10 or np.linalg.norm(ref[n:]) < 1e-12
-    np.testing.assert_allclose(e[1:], eref[1:], rtol=RTOL, atol=1e-11)
+    np.testing.assert_allclose(e, eref, rtol=RTOL, atol=1e-11)
 
 
 def test_he_l15_full_size(gpu_lib):
@@ -334,6 +334,46 @@ def test_init_split2_he_hartree_fock(gpu_lib):
     e, last = hp.init_split2(d, dstep=4e-3, tolene=1e-12, maxcyc=6000)
     assert last < 5999
     assert abs(e - (-2.8616800)) < 1e-5, e
+
+
+@pytest.mark.parametrize("igauge", [0, 1])
+def test_frozen_core_energy_matches_oracle(gpu_lib, igauge):
+    """ene_fcore of td1c_set_wfn0 (hprod_v2fc_enefc): two frozen s cores of
+    type -1 (exchange through the FC-FC pair potentials) beside one dynamical
+    p orbital, against the oracle, and against the total energy of the two s
+    orbitals held as dynamical cores."""
+    import td1c_amd
+    from oracle.oracle import Oracle
+    from td1c_amd import workloads as W
+    from td1c_amd.abi import Problem
+
+    base = dict(nfe=10, ndvr=12, rmax=30.0, lmax1=2, lmax2=4, znuc=4.0)
+    pc = Problem(nfcore1=2, ndcore=1, mval=[0, 0, 0], lval=[0, 0, 1], nel_alpha=3, nel_beta=3, igauge=igauge,
+                 **base)
+    pa = Problem(ndcore=2, nact=0, mval=[0, 0], nel_alpha=2, nel_beta=2, **base)
+    oc, oa = Oracle(pc), Oracle(pa)
+    r, wq = W.gll_grid(pc)
+    nb = pc.nr * (pc.lmax1 + 1)
+    s1 = W.hydrogenic_1s(pc, 4.0)
+    s2 = (1 - 2 * r) * np.exp(-2 * r) * r * np.sqrt(wq)
+    s2 = s2 - np.vdot(s1, s2) * s1
+    p1 = r * r * np.exp(-r) * np.sqrt(wq)
+    w = np.zeros(oc.wfn_size, complex)
+    w[:pc.nr], w[nb:nb + pc.nr] = s1, s2 / np.linalg.norm(s2)
+    w[2 * nb + pc.nr: 2 * nb + 2 * pc.nr] = p1 / np.linalg.norm(p1)
+    w[-1] = 1.0
+    wa = np.concatenate([w[: 2 * nb], [1.0]])
+    oc.set_wfn0(w)
+    oa.set_wfn0(wa)
+    oc.htot(1.0, np.zeros(9), w)
+    oa.htot(1.0, np.zeros(9), wa)
+    hp = td1c_amd.HProd(pc)
+    hp.set_wfn0(w)
+    print(f"frozen-core energy igauge {igauge}", flush=True)
+    hp.htot(1.0, np.zeros(9), hp.to_device(w))
+    e = hp.energies()
+    np.testing.assert_allclose(e, oc.energies(), rtol=1e-12, atol=1e-11)
+    assert abs(e[0] - oa.energies()[4]) < 1e-11 * abs(e[0])
 
 
 # Mixed precision (precision = 1): the mean field (theta-grid orbitals, pair

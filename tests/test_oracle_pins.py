@@ -849,3 +849,40 @@ def test_imaginary_time_he_hartree_fock_energy():
         w = o.orth(o.rk_prop(4, 4e-3, lf, w, apply_mask=False))
     assert abs(e - e1) < 1e-12
     assert abs(e - (-2.8616800)) < 1e-5, e
+
+
+@pytest.mark.parametrize("igauge", [0, 1])
+def test_frozen_core_energy(igauge):
+    """orc_set_wfn0's ene_fcore (hprod_v2fc_enefc): a frozen hydrogenic 1s^2
+    with Z = 2 has 2 h + J = 2 (-2) + 5/4 = -2.75; two frozen s cores give the
+    total energy of the same two orbitals held as dynamical cores (one-body
+    part, frozen-core Hartree, and the type -1 exchange through the FC-FC pair
+    potentials, off-diagonal pairs included)."""
+    prob = Problem(nfe=20, ndvr=20, rmax=40.0, lmax1=2, lmax2=4, znuc=2.0, nfcore1=1, ndcore=0, nact=0,
+                   nel_alpha=1, nel_beta=1, lval=[0], igauge=igauge)
+    o = O.Oracle(prob)
+    w = np.zeros(o.wfn_size, complex)
+    w[:prob.nr] = W.hydrogenic_1s(prob, 2.0)
+    w[-1] = 1.0
+    o.set_wfn0(w)
+    o.htot(1.0, np.zeros(9), w)
+    e = o.energies()
+    assert abs(e[0] + 2.75) < 1e-9 and abs(e[4] + 2.75) < 1e-9, e
+
+    base = dict(nfe=10, ndvr=12, rmax=30.0, lmax1=2, lmax2=4, znuc=4.0, mval=[0, 0], nel_alpha=2, nel_beta=2)
+    pa = Problem(ndcore=2, nact=0, **base)
+    pc = Problem(nfcore1=2, ndcore=0, lval=[0, 0], igauge=igauge, **base)
+    oa, oc = O.Oracle(pa), O.Oracle(pc)
+    r, wq = W.gll_grid(pa)
+    nb = pa.nr * (pa.lmax1 + 1)
+    s1 = W.hydrogenic_1s(pa, 4.0)
+    s2 = (1 - 2 * r) * np.exp(-2 * r) * r * np.sqrt(wq)
+    s2 = s2 - np.vdot(s1, s2) * s1
+    w = np.zeros(oa.wfn_size, complex)
+    w[:pa.nr], w[nb:nb + pa.nr], w[-1] = s1, s2 / np.linalg.norm(s2), 1.0
+    oa.set_wfn0(w)
+    oc.set_wfn0(w)
+    oa.htot(1.0, np.zeros(9), w)
+    oc.htot(1.0, np.zeros(9), w)
+    ea, ec = oa.energies(), oc.energies()
+    assert abs(ea[4] - ec[0]) < 1e-11 * abs(ea[4]) and abs(ec[4] - ec[0]) < 1e-14 * abs(ec[0]), (ea, ec)
