@@ -141,6 +141,26 @@ int td1c_htot(td1c_ctx* ctx, double dtime, const double* lfield,
               const double* d_wfn, const double* d_cic,
               double* d_hwfn, double* d_hcic, void* stream);
 
+/* hprod_htotx_ (include/wrapper.hpp:161; src/hprod/hprod_htot.F90:44-208,
+ * called by clhprod::htotx, src/hprod/clhprod.cpp:84-88): the hprod without
+ * the orbital derivative (dtorb): RDMs, integrals, X matrix, energies and the
+ * CI derivative stay in the context for td1c_get_* below.  On `stream`. */
+int td1c_htotx(td1c_ctx* ctx, double dtime, const double* lfield,
+               const double* d_wfn, const double* d_cic, void* stream);
+
+/* The module arrays of the last td1c_htot / td1c_htotx, column-major
+ * complex128 in host memory; each synchronises the device.
+ * td1c_get_den1 / td1c_get_int1e: nact x nact (hprod_getden1_,
+ * hprod_getint1_, include/wrapper.hpp:150-151; src/hprod/hprod_getvar.F90:2-24);
+ * td1c_get_int2e / td1c_get_den2: nact^4 (hprod_getint2_, :152; :26-36, and
+ * den2 of src/module/mod_hprod.F90); td1c_get_xmat: nfun x nfun, X(j,i)
+ * multiplying orbital j into the derivative of orbital i, times dt. */
+int td1c_get_den1(td1c_ctx* ctx, double* out);
+int td1c_get_den2(td1c_ctx* ctx, double* out);
+int td1c_get_int1e(td1c_ctx* ctx, double* out);
+int td1c_get_int2e(td1c_ctx* ctx, double* out);
+int td1c_get_xmat(td1c_ctx* ctx, double* out);
+
 /* Energies of the last td1c_htot (ene_fcore, ene_dcore, ene_core, ene_act,
  * ene_tot as bound by hprod_bind_); synchronises the stream. */
 int td1c_get_energies(td1c_ctx* ctx, double ene[5]);

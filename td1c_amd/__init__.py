@@ -85,6 +85,11 @@ def load_library():
     L.td1c_wfn_write.restype = ci
     L.td1c_wfn_read.argtypes = [cp, cp, ci, ci, ci, ci, ci, vp]
     L.td1c_wfn_read.restype = ci
+    L.td1c_htotx.argtypes = [vp, ctypes.c_double, vp, vp, vp, vp]
+    L.td1c_htotx.restype = ci
+    for name in ("td1c_get_den1", "td1c_get_den2", "td1c_get_int1e", "td1c_get_int2e", "td1c_get_xmat"):
+        getattr(L, name).argtypes = [vp, vp]
+        getattr(L, name).restype = ci
     L.td1c_mask.argtypes = [vp, vp, vp]
     L.td1c_mask.restype = ctypes.c_int
     L.td1c_field_value.argtypes = [ctypes.POINTER(TD1CField), ctypes.c_double, vp]
@@ -230,6 +235,29 @@ class HProd:
         e = np.zeros(5)
         _check(load_library().td1c_get_energies(self.h, _ptr(e)), "td1c_get_energies")
         return e
+
+    def htotx(self, dtime: float, lfield: Sequence[float], wfn, stream=None):
+        """clhprod::htotx: the hprod without the orbital derivative; the
+        RDMs, integrals and X matrix are then read with get()."""
+        torch = self.torch
+        if wfn.dtype != torch.complex128 or not wfn.is_cuda or wfn.numel() != self.wfn_size:
+            raise ValueError("wfn must be a complex128 CUDA tensor of wfn_size elements")
+        lf = np.ascontiguousarray(lfield, dtype=np.float64)
+        if lf.size != 9:
+            raise ValueError("lfield must have 9 entries")
+        base = wfn.data_ptr()
+        _check(load_library().td1c_htotx(self.h, float(dtime), _ptr(lf), ctypes.c_void_p(base),
+                                         ctypes.c_void_p(base + 16 * self.norb), self._stream(stream)), "td1c_htotx")
+
+    def get(self, name: str) -> np.ndarray:
+        """den1, den2, int1e, int2e (active indices) or xmat (all orbitals)
+        of the last htot / htotx, as column-major numpy arrays."""
+        na, nf = self.prob.nact, self.nfun
+        shape = {"den1": (na, na), "int1e": (na, na), "den2": (na,) * 4, "int2e": (na,) * 4,
+                 "xmat": (nf, nf)}[name]
+        out = np.zeros(int(np.prod(shape)), np.complex128)
+        _check(getattr(load_library(), "td1c_get_" + name)(self.h, _ptr(out)), "td1c_get_" + name)
+        return out.reshape(shape, order="F")
 
     def dipole(self, lfield: Sequence[float], wfn, out=None, stream=None):
         """Length-form dipole <z> (hprod_dipole dip(1)) of a device wave

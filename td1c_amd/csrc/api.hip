@@ -1096,6 +1096,60 @@ int td1c_htot(td1c_ctx* c, double dtime, const double* lfield, const double* d_w
   return 0;
 }
 
+// hprod_htotx (src/hprod/hprod_htot.F90:44-208; clhprod::htotx,
+// src/hprod/clhprod.cpp:84-88): everything of the hprod but the orbital
+// derivative (dtorb): RDMs, integrals, X matrix, energies and the CI
+// derivative, left in the context for the getters below.
+int td1c_htotx(td1c_ctx* c, double dtime, const double* lfield, const double* d_wfn, const double* d_cic,
+               void* stream) {
+  if (!c || !lfield || !d_wfn) return fail(-1, "null argument");
+  if (c->S.nfcore > 0 && !c->wfn0_set) return fail(-4, "frozen core needs td1c_set_wfn0 first");
+  if (c->S.nact > 0 && !d_cic) return fail(-1, "null CI vector");
+  hipStream_t s = (hipStream_t)stream;
+  HIP_TRY(hipMemcpyAsync(c->lf_dev.p, lfield, 9 * sizeof(double), hipMemcpyHostToDevice, s));
+  CiStage cs;
+  int rc = enqueue_htotx(c, dtime, c->lf_dev.p, reinterpret_cast<const double2*>(d_wfn),
+                         reinterpret_cast<const double2*>(d_cic), cs, s);
+  if (rc) return rc;
+  HIP_TRY(join_island(c, s));
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+// The module arrays of the last td1c_htot / td1c_htotx, as hprod_getden1,
+// hprod_getint1, hprod_getint2 (src/hprod/hprod_getvar.F90:2-36) return them:
+// column-major (Fortran) complex128 into host memory; synchronises the device.
+static int copy_result(const DevBuf<double2>& b, size_t n, double* out) {
+  HIP_TRY(hipDeviceSynchronize());
+  if (n > 0) HIP_TRY(hipMemcpy(out, b.p, n * sizeof(double2), hipMemcpyDeviceToHost));
+  return 0;
+}
+int td1c_get_den1(td1c_ctx* c, double* out) {
+  if (!c || !out) return fail(-1, "null argument");
+  const size_t n = (size_t)c->S.nact;
+  return copy_result(c->den1, n * n, out);
+}
+int td1c_get_den2(td1c_ctx* c, double* out) {
+  if (!c || !out) return fail(-1, "null argument");
+  const size_t n = (size_t)c->S.nact;
+  return copy_result(c->den2, n * n * n * n, out);
+}
+int td1c_get_int1e(td1c_ctx* c, double* out) {
+  if (!c || !out) return fail(-1, "null argument");
+  const size_t n = (size_t)c->S.nact;
+  return copy_result(c->int1e, n * n, out);
+}
+int td1c_get_int2e(td1c_ctx* c, double* out) {
+  if (!c || !out) return fail(-1, "null argument");
+  const size_t n = (size_t)c->S.nact;
+  return copy_result(c->int2e, n * n * n * n, out);
+}
+int td1c_get_xmat(td1c_ctx* c, double* out) {
+  if (!c || !out) return fail(-1, "null argument");
+  const size_t n = (size_t)c->S.nfun;
+  return copy_result(c->xmat, n * n, out);
+}
+
 int td1c_get_energies(td1c_ctx* c, double ene[5]) {
   if (!c || !ene) return fail(-1, "null argument");
   HIP_TRY(hipDeviceSynchronize());

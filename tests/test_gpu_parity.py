@@ -376,6 +376,36 @@ def test_frozen_core_energy_matches_oracle(gpu_lib, igauge):
     assert abs(e[0] - oa.energies()[4]) < 1e-11 * abs(e[0])
 
 
+@pytest.mark.parametrize("name,lf", [("dc3_small", C.LF_LEN), ("ne_small", C.LF_VEL), ("cas_small", C.LF_LEN),
+                                     ("cas_fc_small", C.LF_VEL), ("cas6_small", C.LF_LEN)])
+def test_htotx_and_getters_match_oracle(gpu_lib, name, lf):
+    """td1c_htotx (hprod_htotx) and the module-array getters (den1, den2,
+    int1e, int2e, xmat) against the oracle's arrays after its hprod; the
+    energies of htotx equal those of the full hprod."""
+    import td1c_amd
+    from oracle.oracle import Oracle
+
+    prob = getattr(C, name)()
+    o = Oracle(prob)
+    w = C.wfn_for(prob, o.lcic)
+    o.set_wfn0(w)
+    o.htot(0.7, lf, w)
+    hp = td1c_amd.HProd(prob)
+    hp.set_wfn0(w)
+    d = hp.to_device(w)
+    print(f"htotx {name}", flush=True)
+    hp.htotx(0.7, lf, d)
+    np.testing.assert_allclose(hp.energies(), o.energies(), rtol=RTOL, atol=1e-11)
+    na, nf = prob.nact, prob.nfun
+    shapes = {"den1": (na, na), "int1e": (na, na), "den2": (na,) * 4, "int2e": (na,) * 4, "xmat": (nf, nf)}
+    for key, shape in shapes.items():
+        if key != "xmat" and na == 0:
+            continue
+        ref = o.table(key).reshape(shape, order="F")
+        got = hp.get(key)
+        assert np.abs(got - ref).max() <= 1e-10 * max(1.0, np.abs(ref).max()), (key, np.abs(got - ref).max())
+
+
 # Mixed precision (precision = 1): the mean field (theta-grid orbitals, pair
 # densities and potentials, their transforms and products) in complex64, the
 # rest FP64.  Its single-precision rounding (~6e-8 relative per value, summed
