@@ -406,6 +406,40 @@ def test_htotx_and_getters_match_oracle(gpu_lib, name, lf):
         assert np.abs(got - ref).max() <= 1e-10 * max(1.0, np.abs(ref).max()), (key, np.abs(got - ref).max())
 
 
+@pytest.mark.parametrize("name", ["ar_l63", "mf16_l63"])
+def test_full_size_against_golden(gpu_lib, name):
+    """One FP64 hprod at the full sizes of the Ar workload (9 orbitals, lmax1 63)
+    and the 16-orbital workload (136 pairs) against fixtures the oracle wrote
+    (tests/golden/make_full_size.py): energies, ||h|| and eight projections
+    onto seeded unit vectors, each of which an error d = ||h' - h|| moves by at
+    most d."""
+    import json
+    import os
+    import torch
+    import td1c_amd
+    from td1c_amd import workloads as W
+
+    with open(os.path.join(os.path.dirname(__file__), "golden", "full_size.json")) as f:
+        ref = json.load(f)[name]
+    prob = getattr(W, name)()
+    hp = td1c_amd.HProd(prob)
+    w = W.make_wfn(prob, ref["seed"], hp.lcic)
+    hp.set_wfn0(w)
+    print(f"full size {name}", flush=True)
+    out = hp.htot(ref["dtime"], np.array(ref["lfield"]), hp.to_device(w))
+    torch.cuda.synchronize()
+    h = out.cpu().numpy()
+    nrm = ref["norm"]
+    assert abs(np.linalg.norm(h) - nrm) <= RTOL * nrm
+    for k in range(len(ref["proj_re"])):
+        rng = np.random.Generator(np.random.PCG64(1000 + k))
+        u = rng.standard_normal(h.size) + 1j * rng.standard_normal(h.size)
+        u /= np.linalg.norm(u)
+        p = np.vdot(u, h)
+        assert abs(p - complex(ref["proj_re"][k], ref["proj_im"][k])) <= RTOL * nrm, (k, p)
+    np.testing.assert_allclose(hp.energies(), ref["energies"], rtol=RTOL, atol=1e-11)
+
+
 # Mixed precision (precision = 1): the mean field (theta-grid orbitals, pair
 # densities and potentials, their transforms and products) in complex64, the
 # rest FP64.  Its single-precision rounding (~6e-8 relative per value, summed

@@ -886,3 +886,27 @@ def test_frozen_core_energy(igauge):
     oc.htot(1.0, np.zeros(9), w)
     ea, ec = oa.energies(), oc.energies()
     assert abs(ea[4] - ec[0]) < 1e-11 * abs(ea[4]) and abs(ec[4] - ec[0]) < 1e-14 * abs(ec[0]), (ea, ec)
+
+
+def test_full_size_fixtures_match_oracle():
+    """tests/golden/full_size.json (written by tests/golden/make_full_size.py)
+    still holds what the oracle computes for those inputs (~40 s)."""
+    import json
+    import os
+
+    import tests.golden.make_full_size as G
+
+    with open(os.path.join(os.path.dirname(__file__), "golden", "full_size.json")) as f:
+        ref = json.load(f)
+    for name, seed in G.CASES.items():
+        prob = getattr(W, name)()
+        o = O.Oracle(prob)
+        w = W.make_wfn(prob, seed, o.lcic)
+        o.set_wfn0(w)
+        h = o.htot(1.0, np.array(G.LF), w)
+        r = ref[name]
+        assert r["seed"] == seed
+        assert abs(np.linalg.norm(h) - r["norm"]) <= 1e-13 * r["norm"]
+        for k, u in enumerate(G.projection_vectors(h.size)):
+            assert abs(np.vdot(u, h) - complex(r["proj_re"][k], r["proj_im"][k])) <= 1e-13 * r["norm"]
+        np.testing.assert_allclose(o.energies(), r["energies"], rtol=1e-13, atol=1e-13)
