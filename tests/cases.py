@@ -105,3 +105,11 @@ def cas8_small(**kw):
     # Gram entries), the largest active space of the theta-grid mean-field kernel
     return Problem(nfe=6, ndvr=20, rmax=18.0, lmax1=4, lmax2=8, znuc=6.0, ndcore=0, nact=8,
                    nel_alpha=3, nel_beta=3, **kw)
+
+
+def he_scan_max(**kw):
+    # the largest He problem of the scan workload: 120 x 21 on [0, 240]
+    # (2399 points), lmax1 63 / lmax2 126
+    kw.setdefault("igauge", 1)
+    return Problem(nfe=120, ndvr=20, rmax=240.0, lmax1=63, lmax2=126, znuc=2.0, ndcore=1,
+                   nel_alpha=1, nel_beta=1, **kw)

@@ -45,13 +45,14 @@ def _run(prob, lfield, seed=11, dt=1.0):
                                      ("cas_fc_small", C.LF_VEL), ("he_n16", C.LF_VEL),
                                      ("dc4_small", C.LF_LEN), ("fc2_small", C.LF_VEL),
                                      ("he_long", C.LF_LEN), ("cas6_small", C.LF_LEN),
-                                     ("cas7_small", C.LF_LEN), ("cas8_small", C.LF_LEN)])
+                                     ("cas7_small", C.LF_LEN), ("cas8_small", C.LF_LEN),
+                                     ("he_scan_max", C.LF_VEL)])
 def test_htot_matches_oracle(gpu_lib, name, lf):
     mk = {"he_small": C.he_small, "he_small_vel": lambda: C.he_small(igauge=1), "dc3_small": C.dc3_small,
           "ne_small": C.ne_small, "ne_small_len": lambda: C.ne_small(igauge=0), "cas_small": C.cas_small,
           "cas_fc_small": C.cas_fc_small, "he_n16": lambda: C.he_n16(igauge=1), "dc4_small": C.dc4_small,
           "fc2_small": C.fc2_small, "he_long": C.he_long, "cas6_small": C.cas6_small,
-          "cas7_small": C.cas7_small, "cas8_small": C.cas8_small}[name]
+          "cas7_small": C.cas7_small, "cas8_small": C.cas8_small, "he_scan_max": C.he_scan_max}[name]
     prob = mk()
     print(f"case {name}", flush=True)
     o, ref, got, eref, e = _run(prob, lf)
@@ -449,7 +450,7 @@ MIXED_RTOL = 1e-5
 
 @pytest.mark.parametrize("name,lf", [("he_small", C.LF_LEN), ("dc3_small", C.LF_LEN), ("ne_small", C.LF_VEL),
                                      ("cas_fc_small", C.LF_VEL), ("cas_small", C.LF_LEN), ("dc4_small", C.LF_LEN),
-                                     ("cas6_small", C.LF_LEN), ("cas7_small", C.LF_LEN)])
+                                     ("cas6_small", C.LF_LEN), ("cas7_small", C.LF_LEN), ("he_scan_max", C.LF_VEL)])
 def test_mixed_htot_close_to_oracle(gpu_lib, name, lf):
     import torch
     import td1c_amd
