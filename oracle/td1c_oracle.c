@@ -30,13 +30,19 @@
  * potential, hydrogen Slater integrals (L = 0..2, m != 0), the act2
  * generalised-Fock identity, core/active and frozen/dynamical core
  * equivalences, energy and orthonormality conservation of the equations of
- * motion, CI sigma/RDM identities, RK order, the mask closed form.
+ * motion, CI sigma/RDM identities, RK order, the mask closed form, the
+ * dipole forms (closed forms and Ehrenfest relations), the ionisation
+ * statistics (determinant generating function, moments), Gram-Schmidt and
+ * imaginary-time relaxation to the He Hartree-Fock energy, and the
+ * frozen-core energy (He 1s^2, all-frozen closed shell).
  * Parity unpinned:
  *  - the sign convention of the m < 0 Legendre columns (copied from |m|
  *    without (-1)^m, as sph_gen.F90 does; only products of two columns enter
  *    hprod, so no pinned quantity sees it);
  *  - the order and phase of the CAS determinants (the cic layout);
- *  - the KLI/LDA v2xfc (input only).
+ *  - the KLI/LDA v2xfc (input only), and with it the type -2 part of the
+ *    frozen-core energy (v2xfc enters through v2jfc and again as fcx2, as
+ *    hprod_v2fc_enefc has it).
  */
 #include <complex.h>
 #include <math.h>
