@@ -83,6 +83,12 @@ typedef struct td1c_problem {
    * one-body part, the Gram products, X matrix, projector, 1-RDM inverse, CI
    * and RK accumulation in FP64 (no counterpart in the reference) */
   int precision;
+  /* clbas lconst / lconst_core (clwfn::ladapt, src/wfn/clwfn.cpp:359-366):
+   * 0 off (default); 1 every orbital keeps only its l = lval component
+   * (wfn_ladapt, src/wfn/wfn_ladapt.F90:2-39; its CI part returns early);
+   * 2 the cores only (wfn_ladapt_core, :100-125).  Applied to the orbital
+   * derivative of every hprod, as clhprod::htot does to hWfn. */
+  int lconst;
 } td1c_problem;
 
 /* Sizes of a created context. */
@@ -136,7 +142,8 @@ int td1c_set_wfn0(td1c_ctx* ctx, const double* wfn, const double* cic,
  * (a hipStream_t, NULL for the default stream):
  *   hwfn = dt * d(psi)/dt (orbitals), hcic = dt * dC/dt (CI).
  * hprod_htot (src/hprod/hprod_htot.F90:2-42) plus the madapt/ladapt
- * post-ops of clhprod::htot (both no-ops for lconst = false). */
+ * post-ops of clhprod::htot (madapt is a no-op; ladapt follows
+ * td1c_problem.lconst). */
 int td1c_htot(td1c_ctx* ctx, double dtime, const double* lfield,
               const double* d_wfn, const double* d_cic,
               double* d_hwfn, double* d_hcic, void* stream);

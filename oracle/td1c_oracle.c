@@ -65,7 +65,7 @@ typedef struct orc {
   int nfcore2, nfcore1, nfcore, ndcore, ncore, nact, nfun;
   int *mval, *lval;
   int nel[3], nelact[3], mtot;
-  int icomp, igauge, isplit, ioorot, iprojfc, jfc_implicit, xfc_implicit;
+  int icomp, igauge, isplit, ioorot, iprojfc, jfc_implicit, xfc_implicit, lconst;
   double throcc1, throcc2, rmask, mask_order;
   int nbas, nbas2, ngrid;
   /* FEDVR (clfedvr) */
@@ -828,6 +828,7 @@ orc* orc_create(const td1c_problem* p) {
   o->nelact[2] = o->nelact[0] + o->nelact[1];
   o->mtot = p->mtot;
   o->icomp = p->icomp;
+  o->lconst = p->lconst;
   o->igauge = p->igauge;
   o->isplit = p->isplit;
   o->ioorot = p->ioorot;
@@ -1516,6 +1517,17 @@ void orc_htot(orc* o, double dtime, const double* lfield, const zc* wfn, const z
   memcpy(hcic, o->dcic, sizeof(zc) * o->lcic);
   memset(hwfn, 0, sizeof(zc) * (size_t)o->nbas * o->nfun);
   dtorb(o, dtime, hwfn);
+  /* clhprod::htot's post-op hWfn.ladapt (src/hprod/clhprod.cpp:66-67;
+   * clwfn::ladapt, src/wfn/clwfn.cpp:359-366): wfn_ladapto zeroes every
+   * orbital's l != lval components (src/wfn/wfn_ladapt.F90:14-39),
+   * wfn_ladapt_core those of the cores (:100-125); the CI part returns early */
+  if (o->lconst == 1 || o->lconst == 2) {
+    int f1 = o->lconst == 1 ? o->nfun : o->ncore;
+    for (int ifun = 0; ifun < f1; ++ifun)
+      for (int l = 0; l <= o->lmax1; ++l)
+        if (l != o->lval[ifun])
+          for (int i = 0; i < o->nr; ++i) hwfn[OIDX(i, l, ifun)] = 0;
+  }
 }
 
 /* ------------------------------------------------------------------ */

@@ -31,7 +31,7 @@ class TD1CProblem(ctypes.Structure):
         ("iprojfc", c_int), ("jfc_implicit", c_int), ("xfc_implicit", c_int),
         ("throcc1", c_double), ("throcc2", c_double),
         ("rmask", c_double), ("mask_cos_order", c_double),
-        ("precision", c_int),
+        ("precision", c_int), ("lconst", c_int),
     ]
 
 
@@ -91,6 +91,7 @@ class Problem:
     rmask: float = 0.0
     mask_cos_order: float = 4.0
     precision: int = 0
+    lconst: int = 0          # 0 off, 1 lconst (all orbitals), 2 lconst_core (cores)
 
     @property
     def nfun(self) -> int:
@@ -150,6 +151,7 @@ class Problem:
         p.throcc1, p.throcc2 = self.throcc1, self.throcc2
         p.rmask, p.mask_cos_order = self.rmask, self.mask_cos_order
         p.precision = self.precision
+        p.lconst = self.lconst
         return p, (edges, m_arr, l_arr)
 
 

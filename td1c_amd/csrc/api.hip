@@ -92,7 +92,7 @@ struct td1c_ctx {
   DevBuf<double> eK0, eK, eP;
   DevBuf<double> m_l2g, m_g2l, m_g2L, m_L2g;
   DevBuf<double> pG, pWl, pWr, pcrA, pcrG, pcrBinv, pSd, pSl, pSu, invr, rpl0, rpl1, fac2, fac1;
-  DevBuf<int> mval, pair_m, pj, pi, ptype;
+  DevBuf<int> mval, lval, pair_m, pj, pi, ptype;
   // batches
   DevBuf<TransformBatch> b_l2g_dyn, b_g2l_fc, b_rho, b_L2g, b_g2l_mf;
   int n_l2g_dyn = 0, n_g2l_fc = 0, n_g2l_mf = 0;
@@ -164,7 +164,7 @@ struct td1c_ctx {
                             &gram_partial, &gram_partial_p, &lprod, &pprod, &den1, &den2, &rden, &rrden, &int1e, &int2e, &xmat,
                             &dcic, &ci_work, &mfc, &dip_part, &ovlp, &rk_w0, &rk_t, &rk_k};
     for (auto* b : z) b->release();
-    DevBuf<int>* i[] = {&mval, &pair_m, &pj, &pi, &ptype, &mapH, &mapG, &mapV, &mapF, &map2e,
+    DevBuf<int>* i[] = {&mval, &lval, &pair_m, &pj, &pi, &ptype, &mapH, &mapG, &mapV, &mapF, &map2e,
                         &det_of, &exA, &sgA, &exB, &sgB};
     for (auto* b : i) b->release();
     DevBuf<TransformBatch>* t[] = {&b_l2g_dyn, &b_g2l_fc, &b_rho, &b_L2g, &b_g2l_mf};
@@ -306,6 +306,7 @@ int build_device(td1c_ctx* c) {
   HIP_TRY(c->fac2.upload(S.d2fac2));
   HIP_TRY(c->fac1.upload(S.d2fac1));
   HIP_TRY(c->mval.upload(S.mval));
+  HIP_TRY(c->lval.upload(S.lval));
   {
     std::vector<int> pm, pj, pi, pt;
     for (auto& p : S.pairs) {
@@ -774,6 +775,8 @@ DtorbArgs dtorb_args(td1c_ctx* c, double dtime) {
   d.tfac = S.icomp == 0 ? make_double2(-dtime, 0.0) : make_double2(0.0, -dtime);
   d.xlocal = c->x_local ? 1 : 0;
   d.ncore = S.ncore;
+  d.ladapt = S.lconst;
+  d.lval = c->lval.p;
   d.xfac = S.icomp == 0 ? make_double2(dtime, 0.0) : make_double2(0.0, dtime);
   d.lprod = c->lprod.p;
   d.mapH = c->mapH.p; d.mapG = c->mapG.p; d.mapV = c->mapV.p; d.mapF = c->mapF.p;

@@ -1290,6 +1290,8 @@ __global__ void __launch_bounds__(256) k_dtorb(const DtorbArgs a) {
       if (a.mval[j] != a.mval[f]) continue;
       cfma(k, a.orb[(size_t)j * nb + q], a.xlocal ? sXc[j] : a.xmat[(size_t)f * a.nfun + j]);
     }
+    if (a.ladapt && (a.ladapt == 1 || f < a.ncore) && (int)(q / (size_t)a.nr) != a.lval[f])
+      k = make_double2(0.0, 0.0);
   }
   if (a.kout) a.kout[idx] = k;
   // the step-start state: read before acc (which it may alias) is written

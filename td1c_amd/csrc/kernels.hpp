@@ -100,6 +100,10 @@ struct DtorbArgs {
   // (fc-dyn, core-core and active-active blocks; no core-active block, i.e.
   // ncore == nfcore or no active orbitals) instead of reading xmat
   int xlocal, ncore;
+  // ladapt: 1 zero the derivative's l != lval[f] components of every orbital,
+  // 2 of the cores only (clwfn::ladapt after the hprod)
+  int ladapt;
+  const int* lval;
   double2 xfac;  // X = <j|H|i> * xfac
   const double2* lprod;
   const int *mapH, *mapG, *mapV, *mapF;

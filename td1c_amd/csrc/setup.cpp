@@ -184,6 +184,8 @@ bool Setup::build(const td1c_problem& p) {
   throcc1 = p.throcc1; throcc2 = p.throcc2;
   precision = p.precision;
   if (precision != 0 && precision != 1) { error = "precision must be 0 (complex128) or 1 (complex64 mean field)"; return false; }
+  lconst = p.lconst;
+  if (lconst < 0 || lconst > 2) { error = "lconst must be 0, 1 (all orbitals) or 2 (cores)"; return false; }
   x0.resize(nfe); x1.resize(nfe);
   for (int e = 0; e < nfe; ++e) { x0[e] = p.edges[e]; x1[e] = p.edges[e + 1]; }
   rmask = p.rmask > 0 ? p.rmask : 0.8 * x1[nfe - 1];

@@ -24,6 +24,7 @@ struct Setup {
   int jfc_implicit = 1, xfc_implicit = 1;
   double throcc1 = 1e-15, throcc2 = 1e-15, rmask = 0, mask_order = 4;
   int precision = 0;
+  int lconst = 0;  // 0 off, 1 every orbital (wfn_ladapt), 2 the cores (wfn_ladapt_core)
 
   // FEDVR grid (clfedvr): global points 0..nrad
   std::vector<double> x0, x1;

@@ -441,6 +441,32 @@ def test_full_size_against_golden(gpu_lib, name):
     np.testing.assert_allclose(hp.energies(), ref["energies"], rtol=RTOL, atol=1e-11)
 
 
+@pytest.mark.parametrize("lconst", [1, 2])
+def test_lconst_matches_oracle(gpu_lib, lconst):
+    """lconst / lconst_core (clwfn::ladapt after every hprod): td1c_htot and
+    one RK4 step against the oracle on the CAS problem with lval = 0, 1, 2."""
+    import torch
+    import td1c_amd
+    from oracle.oracle import Oracle
+
+    prob = C.cas_small(lval=[0, 1, 2], lconst=lconst)
+    o = Oracle(prob)
+    w = C.wfn_for(prob, o.lcic)
+    ref = o.htot(1.0, C.LF_LEN, w)
+    hp = td1c_amd.HProd(prob)
+    d = hp.to_device(w)
+    print(f"lconst {lconst}", flush=True)
+    out = hp.htot(1.0, C.LF_LEN, d)
+    torch.cuda.synchronize()
+    assert _rel(out.cpu().numpy(), ref) < RTOL
+    dt = 2e-5
+    lf = np.tile(C.LF_LEN, (4, 1))
+    refs = o.rk_prop(4, dt, lf, w, apply_mask=True)
+    td1c_amd.CRK(rk_order=4).prop(hp, d, 0.0, dt, lfield_stages=lf)
+    torch.cuda.synchronize()
+    assert _rel(d.cpu().numpy(), refs) < RTOL
+
+
 # Mixed precision (precision = 1): the mean field (theta-grid orbitals, pair
 # densities and potentials, their transforms and products) in complex64, the
 # rest FP64.  Its single-precision rounding (~6e-8 relative per value, summed

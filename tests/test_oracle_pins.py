@@ -910,3 +910,29 @@ def test_full_size_fixtures_match_oracle():
         for k, u in enumerate(G.projection_vectors(h.size)):
             assert abs(np.vdot(u, h) - complex(r["proj_re"][k], r["proj_im"][k])) <= 1e-13 * r["norm"]
         np.testing.assert_allclose(o.energies(), r["energies"], rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("lconst", [1, 2])
+def test_ladapt_post_op(lconst):
+    """orc_htot with lconst (clhprod::htot's hWfn.ladapt): the orbital
+    derivative equals that of lconst = 0 with the l != lval components zeroed,
+    for every orbital (lconst 1, wfn_ladapto) or the cores only (lconst 2,
+    wfn_ladapt_core); the CI derivative is unchanged."""
+    import tests.cases as C
+
+    kw = dict(lval=[0, 1, 2])
+    p0 = C.cas_small(**kw)
+    p1 = C.cas_small(lconst=lconst, **kw)
+    o0, o1 = O.Oracle(p0), O.Oracle(p1)
+    w = C.wfn_for(p0, o0.lcic)
+    h0 = o0.htot(1.0, C.LF_LEN, w)
+    h1 = o1.htot(1.0, C.LF_LEN, w)
+    nf, L1, nr = p0.nfun, p0.lmax1 + 1, p0.nr
+    exp = h0.copy()
+    orb = exp[: nf * L1 * nr].reshape(nf, L1, nr)
+    for f in range(nf if lconst == 1 else p0.ncore):
+        for l in range(L1):
+            if l != kw["lval"][f]:
+                orb[f, l] = 0
+    assert np.array_equal(h1, exp)
+    assert np.abs(h0 - exp).max() > 1e-3
