@@ -141,7 +141,7 @@ def hprod_bytes(s: dict, c: int = 16) -> float:
     return 2 * No * 16 + 2 * No * c + 4 * Ng * c + 4 * Np * c + 4 * s["lcic"] * 16
 
 
-def cpu_baseline(prob, wfn, lfield, seconds: float = 12.0) -> dict:
+def cpu_baseline(prob, wfn, lfield, seconds: float = 12.0, name: str = "ne_l31") -> dict:
     """The FP64 oracle (single-threaded C restatement of hprod_htot) timed on
     this host on a bounded sample of the same workload."""
     from oracle.oracle import Oracle
@@ -157,7 +157,7 @@ def cpu_baseline(prob, wfn, lfield, seconds: float = 12.0) -> dict:
         if el > seconds or n >= 50:
             break
     return {"value": n / el, "unit": "hprod evals/s", "cores": 1, "kind": "port",
-            "sample": f"{n} hprod evaluations of ne_l31 (FP64 oracle, same inputs) in {el:.1f} s"}
+            "sample": f"{n} hprod evaluations of {name} (FP64 oracle, same inputs) in {el:.1f} s"}
 
 
 def main():
@@ -172,6 +172,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--only", choices=["fp64", "mixed"], default=None,
                     help="measure one path only (for profiler runs; no dipole check)")
+    ap.add_argument("--workload", default="ne_l31",
+                    help="ne_l31 (default, the headline), he_l15, ar_l63 or mf16_l63; the l63 grids need "
+                         "--dt 2e-7 for a stable explicit step")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a device copy of this many bytes (counter calibration)")
     args = ap.parse_args()
@@ -188,13 +191,21 @@ def main():
     import td1c_amd
     from td1c_amd import workloads as W
 
+    if args.workload not in W.WORKLOADS:
+        print(f"bench.py: unknown workload {args.workload}", file=sys.stderr)
+        sys.exit(2)
+    make_prob, windex = W.WORKLOADS[args.workload]
+    seed = windex * 1000 + 7
     field = W.ne_field()
+    if args.workload != "ne_l31" and make_prob().igauge == 0:
+        field = td1c_amd.Field(env_type="sin2", gauge="length1", fint=field.fint, wlen=field.wlen,
+                               numcyc=field.numcyc)
     rk = td1c_amd.CRK(4)
 
     def measure(precision: int) -> dict:
-        prob = W.ne_l31(precision=precision)
+        prob = make_prob(precision=precision)
         hp = td1c_amd.HProd(prob)
-        wfn = W.make_wfn(prob, 2007, hp.lcic)
+        wfn = W.make_wfn(prob, seed, hp.lcic)
         hp.set_wfn0(wfn)
         d = hp.to_device(wfn)
         nprof = max(10, min(args.steps, 50))
@@ -239,9 +250,9 @@ def main():
 
     def dipole_series(precision: int, nchunk: int = 10, chunk: int = 20) -> np.ndarray:
         """<z>(t) every `chunk` steps of a propagation from the same state."""
-        prob = W.ne_l31(precision=precision)
+        prob = make_prob(precision=precision)
         hp = td1c_amd.HProd(prob)
-        wfn = W.make_wfn(prob, 2007, hp.lcic)
+        wfn = W.make_wfn(prob, seed, hp.lcic)
         hp.set_wfn0(wfn)
         d = hp.to_device(wfn)
         out, t = [], 0.0
@@ -289,8 +300,13 @@ def main():
         "vs_baseline": None,
         "dtype": "c64 mean field / c128 state" if use is mixed else "c128",
         "data": "synthetic (seeded orbitals, one CI determinant)",
-        "config": {"workload": "ne_l31: Ne-like RK4, 80x21 FEDVR (1599 pts) on [0,160], lmax1 31/lmax2 62, "
-                               "Z=10, 1 frozen core + 4 active, 800 nm sin2 1e14 W/cm2 velocity gauge",
+        "config": {"workload": ("ne_l31: Ne-like RK4, 80x21 FEDVR (1599 pts) on [0,160], lmax1 31/lmax2 62, "
+                                "Z=10, 1 frozen core + 4 active, 800 nm sin2 1e14 W/cm2 velocity gauge")
+                   if args.workload == "ne_l31" else
+                   (f"{args.workload}: RK4, {prob.nfe}x{prob.ndvr + 1} FEDVR ({prob.nr} pts) on [0,{prob.rmax:g}], "
+                    f"lmax1 {prob.lmax1}/lmax2 {prob.lmax2}, Z={prob.znuc:g}, {prob.nfun} orbitals "
+                    f"({prob.nfcore} frozen, {prob.nact} active), 800 nm sin2 1e14 W/cm2 "
+                    f"{'velocity' if prob.igauge == 1 else 'length'} gauge"),
                    "precision": "mixed" if use is mixed else "fp64", "rk_order": 4, "dt": args.dt,
                    "steps_per_propagation": NSTEP_PROP, "parallelism": "single GPU"},
         "propagations_per_s": 1.0 / (NSTEP_PROP * ms_step * 1e-3),
@@ -303,7 +319,8 @@ def main():
                   "dipole_check": "<z> every 20 steps over 200 steps, mixed vs fp64, same initial state"},
         "roofline": {"bound": "hbm", "kernel": dname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": recorded_traffic(dname, "mixed" if use is mixed else "fp64"),
+                     "traffic": (recorded_traffic(dname, "mixed" if use is mixed else "fp64")
+                                 if args.workload == "ne_l31" else None),
                      "algorithmic_bytes_per_launch": abytes, "avg_launch_us": avg_s * 1e6},
         "ms_per_step_stream_launch": use["ms_stream"],
         "kernel_ms_per_step": {k: v[0] / nprof for k, v in sorted(use["breakdown"].items(), key=lambda kv: -kv[1][0])},
@@ -312,7 +329,8 @@ def main():
         "norm_after": use["norm"],
     }
     if not args.no_cpu:
-        result["cpu_baseline"] = cpu_baseline(fp64["prob"], fp64["wfn"], fp64["lf0"], args.cpu_seconds)
+        result["cpu_baseline"] = cpu_baseline(fp64["prob"], fp64["wfn"], fp64["lf0"], args.cpu_seconds,
+                                              args.workload)
     else:
         result["cpu_baseline"] = None
     print(json.dumps(result), flush=True)
