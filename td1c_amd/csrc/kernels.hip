@@ -1661,7 +1661,9 @@ constexpr int kPoissonBlock = 512;  // threads of the LDS bridge pass (>= nfe)
 // LDS bytes of the LDS bridge pass; 0 when the row does not fit
 size_t poisson_fused_lds(const PoissonArgs& a) {
   const size_t bytes = (size_t)a.nr * sizeof(double2) + 2 * (size_t)a.pcr_steps * (a.nfe - 1) * sizeof(double);
-  return bytes <= 40 * 1024 ? bytes : 0;  // four blocks per CU beside the static arrays
+  // with the 5 KB of static arrays a block stays within 64 KB of LDS (two or
+  // more blocks per CU); the 80-element grids need 34 KB (four per CU)
+  return bytes <= 56 * 1024 ? bytes : 0;
 }
 
 template <class T>
