@@ -91,7 +91,7 @@ struct td1c_ctx {
   DevBuf<double2> tdiag, tzor;
   DevBuf<double> eK0, eK, eP;
   DevBuf<double> m_l2g, m_g2l, m_g2L, m_L2g;
-  DevBuf<double> pG, pWl, pWr, pcrA, pcrG, pcrBinv, pSd, pSl, pSu, invr, rpl0, rpl1, fac2, fac1;
+  DevBuf<double> pG, pWl, pWr, pWlI, pWrI, pcrA, pcrG, pcrBinv, pSd, pSl, pSu, invr, rpl0, rpl1, fac2, fac1;
   DevBuf<int> mval, lval, pair_m, pj, pi, ptype;
   // batches
   DevBuf<TransformBatch> b_l2g_dyn, b_g2l_fc, b_rho, b_L2g, b_g2l_mf;
@@ -158,7 +158,7 @@ struct td1c_ctx {
 
   ~td1c_ctx() {
     DevBuf<double>* d[] = {&eK0, &eK, &eP, &xr, &band_k, &band_k0, &band_p, &alph, &cost, &mask, &m_l2g, &m_g2l, &m_g2L, &m_L2g,
-                           &pG, &pWl, &pWr, &pcrA, &pcrG, &pcrBinv, &pSd, &pSl, &pSu, &invr, &rpl0, &rpl1, &fac2, &fac1, &ene, &lf_dev};
+                           &pG, &pWl, &pWr, &pWlI, &pWrI, &pcrA, &pcrG, &pcrBinv, &pSd, &pSl, &pSu, &invr, &rpl0, &rpl1, &fac2, &fac1, &ene, &lf_dev};
     for (auto* b : d) b->release();
     DevBuf<double2>* z[] = {&tdiag, &tzor, &lsp, &gsp, &orbg0, &rhoL, &vL, &vg, &p_xb, &p_Q,
                             &gram_partial, &gram_partial_p, &lprod, &pprod, &den1, &den2, &rden, &rrden, &int1e, &int2e, &xmat,
@@ -294,6 +294,16 @@ int build_device(td1c_ctx* c) {
   }
   HIP_TRY(c->pWl.upload(S.pWl));
   HIP_TRY(c->pWr.upload(S.pWr));
+  {
+    // W with 1/r folded in: the bridge pass takes W . (rho invr) from raw rho
+    std::vector<double> wl(S.pWl.size()), wr(S.pWr.size());
+    for (size_t k = 0; k < wl.size(); ++k) {
+      wl[k] = S.pWl[k] * S.d2invr[k];
+      wr[k] = S.pWr[k] * S.d2invr[k];
+    }
+    HIP_TRY(c->pWlI.upload(wl));
+    HIP_TRY(c->pWrI.upload(wr));
+  }
   HIP_TRY(c->pcrA.upload(S.pcrA));
   HIP_TRY(c->pcrG.upload(S.pcrG));
   HIP_TRY(c->pcrBinv.upload(S.pcrBinv));
@@ -691,7 +701,7 @@ int enqueue_htotx(td1c_ctx* c, double dtime, const double* d_lf, const double2* 
     PoissonArgs pa{};
     pa.nr = nr; pa.n = S.n; pa.nfe = S.nfe; pa.ni = S.ni; pa.L2 = L2; pa.npair = c->npair;
     pa.pair_m = c->pair_m.p;
-    pa.G = c->pG.p; pa.Wl = c->pWl.p; pa.Wr = c->pWr.p;
+    pa.G = c->pG.p; pa.Wl = c->pWl.p; pa.Wr = c->pWr.p; pa.WlI = c->pWlI.p; pa.WrI = c->pWrI.p;
     pa.Sd = c->pSd.p; pa.Sl = c->pSl.p; pa.Su = c->pSu.p;
     pa.pcr_steps = S.pcr_steps; pa.pcrA = c->pcrA.p; pa.pcrG = c->pcrG.p; pa.pcrBinv = c->pcrBinv.p;
     pa.invr = c->invr.p; pa.rpl0 = c->rpl0.p; pa.rpl1 = c->rpl1.p; pa.fac2 = c->fac2.p; pa.fac1 = c->fac1.p;

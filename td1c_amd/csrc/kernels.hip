@@ -1441,8 +1441,8 @@ __global__ void __launch_bounds__(BS) k_poisson_bridge_lds(const T* __restrict__
   __shared__ double2 sS[kMaxBridge];
   __shared__ double2 sY[kMaxBridge];
   __shared__ double2 sRed[BS / 64];
-  double2* sB = dyn_lds;
-  double* sPA = reinterpret_cast<double*>(dyn_lds + nr);
+  T* sR = reinterpret_cast<T*>(dyn_lds);  // the row of rho as given (b = rho invr is formed on use)
+  double* sPA = reinterpret_cast<double*>(reinterpret_cast<unsigned char*>(dyn_lds) + (size_t)nr * sizeof(T));
   double* sPG = sPA + K * nb;
   const int pm = a.pair_m[p];
   const size_t row = ((size_t)p * a.L2 + L) * nr;
@@ -1463,25 +1463,25 @@ __global__ void __launch_bounds__(BS) k_poisson_bridge_lds(const T* __restrict__
       sPG[q] = pg[q];
     }
   }
-  // the row b = rho * invr into LDS and the multipole moment sum rho rpl0,
-  // four rows per thread per trip with the loads issued together
+  // the row of rho into LDS and the multipole moment sum rho rpl0, four rows
+  // per thread per trip with the loads issued together
   double2 m = make_double2(0.0, 0.0);
   for (int r0 = 0; r0 < nr; r0 += 4 * BS) {
-    double2 x[4];
-    double iv[4], w0[4];
+    T x[4];
+    double w0[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int r = r0 + u * BS + tid;
-      x[u] = r < nr ? cto<double2>(rp[r]) : make_double2(0.0, 0.0);
-      iv[u] = r < nr ? a.invr[lo + r] : 0.0;
+      x[u] = r < nr ? rp[r] : cz<T>();
       w0[u] = r < nr ? a.rpl0[lo + r] : 0.0;
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int r = r0 + u * BS + tid;
-      if (r < nr) sB[r] = cscale(x[u], iv[u]);
-      m.x = fma(x[u].x, w0[u], m.x);
-      m.y = fma(x[u].y, w0[u], m.y);
+      if (r < nr) sR[r] = x[u];
+      const double2 xd = cto<double2>(x[u]);
+      m.x = fma(xd.x, w0[u], m.x);
+      m.y = fma(xd.y, w0[u], m.y);
     }
   }
   for (int o = 32; o > 0; o >>= 1) {
@@ -1499,8 +1499,8 @@ __global__ void __launch_bounds__(BS) k_poisson_bridge_lds(const T* __restrict__
     const int bl = kb * n, br = (kb + 1) * n;  // first interior rows of elements k and k+1
     double2 cl = make_double2(0.0, 0.0), cr = make_double2(0.0, 0.0);
     for (int i = sub; i < ni; i += 4) {
-      const double2 b0 = sB[bl + i], b1 = sB[br + i];
-      const double wr = a.Wr[lo + bl + i], wl = a.Wl[lo + br + i];
+      const double2 b0 = cto<double2>(sR[bl + i]), b1 = cto<double2>(sR[br + i]);
+      const double wr = a.WrI[lo + bl + i], wl = a.WlI[lo + br + i];
       cr.x = fma(wr, b0.x, cr.x);
       cr.y = fma(wr, b0.y, cr.y);
       cl.x = fma(wl, b1.x, cl.x);
@@ -1513,7 +1513,7 @@ __global__ void __launch_bounds__(BS) k_poisson_bridge_lds(const T* __restrict__
       cr.y += __shfl_xor(cr.y, o, 64);
     }
     if (sub == 0) {
-      double2 sk = sB[br - 1];
+      double2 sk = cscale(cto<double2>(sR[br - 1]), a.invr[lo + br - 1]);
       sk = csub(sk, cl);
       sk = csub(sk, cr);
       sY[kb] = sk;
@@ -1659,8 +1659,9 @@ __global__ void __launch_bounds__(kPoissonInteriorThreads) k_poisson_interior(co
 constexpr int kPoissonBlock = 512;  // threads of the LDS bridge pass (>= nfe)
 
 // LDS bytes of the LDS bridge pass; 0 when the row does not fit
+template <class T>
 size_t poisson_fused_lds(const PoissonArgs& a) {
-  const size_t bytes = (size_t)a.nr * sizeof(double2) + 2 * (size_t)a.pcr_steps * (a.nfe - 1) * sizeof(double);
+  const size_t bytes = (size_t)a.nr * sizeof(T) + 2 * (size_t)a.pcr_steps * (a.nfe - 1) * sizeof(double);
   // with the 5 KB of static arrays a block stays within 64 KB of LDS (two or
   // more blocks per CU); the 80-element grids need 34 KB (four per CU)
   return bytes <= 56 * 1024 ? bytes : 0;
@@ -1669,7 +1670,7 @@ size_t poisson_fused_lds(const PoissonArgs& a) {
 template <class T>
 void launch_poisson(const T* rho, double2* xb, T* v, const PoissonArgs& a, hipStream_t s) {
   const int jobs = a.npair * a.L2;
-  if (const size_t lds = poisson_fused_lds(a)) {
+  if (const size_t lds = poisson_fused_lds<T>(a)) {
     // bridge pass (rows staged in LDS), then the interiors with each row of
     // G_e shared by a group of pairs
     if (a.ni == 19)

@@ -64,6 +64,7 @@ struct PoissonArgs {
   int nr, n, nfe, ni, L2, npair;
   const int* pair_m;  // m_j - m_i of each pair
   const double *G, *Wl, *Wr, *Sd, *Sl, *Su;  // G: element inverses, transposed [L2][nfe][k][i]
+  const double *WlI, *WrI;                  // Wl, Wr times invr ([L2][nr])
   int pcr_steps;                            // cyclic reduction of the Schur system
   const double *pcrA, *pcrG, *pcrBinv;      // [L2][pcr_steps][nfe-1] x2, [L2][nfe-1]
   const double *invr, *rpl0, *rpl1, *fac2, *fac1;
