@@ -61,7 +61,9 @@ typedef struct td1c_problem {
   const int* mval;          /* [nfun] magnetic quantum number per orbital */
   const int* lval;          /* [nfun] l of each orbital (frozen-core density) */
   int nel_alpha, nel_beta;  /* neltot(1:2) */
-  int mtot;                 /* total M of the CI space */
+  int mtot;                 /* total M of the determinants, cores included
+                               (clbas mtot; ormas_madapt_str adds the
+                               cores' M to each spin string) */
   int nradfc;               /* radial extent of frozen-core work, <=0: nrad-1 */
   /* control flags (src/control/clcontrol.cpp:67-191) */
   int icomp;                /* 1: real time (tfac = -i dt), 0: imaginary time */
@@ -245,10 +247,31 @@ int td1c_wfn_write(const char* orb_path, const char* cic_path, int nrad, int lma
 /* clwfn::read (src/wfn/clwfn.cpp:156-205): wfn (sized for nrad, lmax1, nfun,
  * lcic) is cleared and filled from the files; orbitals beyond the current
  * sizes are dropped, missing ones stay zero.  The CI vector is read when the
- * file's lcic matches; otherwise it is set to the first determinant
- * (ormas_cic0; left zero when nact == 0) and the call returns 1. */
+ * file's lcic matches; otherwise determinant cic0 is set to 1, as ormas_cic0
+ * does (src/ormas/ormas_cic0.F90:24; td1c_ci_layout gives cic0), and the call
+ * returns 1.  cic0 < 0 (no active orbitals, where ormas_cic0 returns at
+ * once) leaves the CI vector zero. */
 int td1c_wfn_read(const char* orb_path, const char* cic_path, int nrad, int lmax1,
-                  int nfun, int lcic, int nact, double* wfn);
+                  int nfun, int lcic, int cic0, double* wfn);
+
+/* Layout of the CI vector of a problem, computed on the host (no device, no
+ * context): the reference's ORMAS addressing for one subspace.  Strings of
+ * one spin are numbered by arc weights (ormas_arcwgt,
+ * src/ormas/ormas_arcwgt.F90:52-104; decoded by ormas_occvec,
+ * src/ormas/ormas_occvec.F90:2-29) and reordered by their total M, cores
+ * included (ormas_madapt_str, src/ormas/ormas_madapt.F90:170-246); the CI
+ * vector holds the determinants with M_alpha + M_beta = mtot in the order
+ * ntot_alph_beta(beta) + alpha (:381-435), beta major.  Outputs (each may be
+ * NULL): *nstr_alpha, *nstr_beta, *lcic, *cic0 (the determinant ormas_cic0
+ * sets, src/ormas/ormas_cic0.F90:24; -1 when nact == 0; >= lcic when the
+ * first alpha and beta strings together miss M = mtot, where ormas_cic0
+ * would write past the CI vector and td1c_wfn_read reports an error instead); str_alpha /
+ * str_beta (nstr_* entries): occupation bit patterns of the ordered strings,
+ * bit k = active orbital k; det_alpha / det_beta (lcic entries): the string
+ * indices of each determinant.  Call once with NULL arrays for the sizes. */
+int td1c_ci_layout(const td1c_problem* prob, int* nstr_alpha, int* nstr_beta, int* lcic,
+                   int* cic0, unsigned int* str_alpha, unsigned int* str_beta,
+                   int* det_alpha, int* det_beta);
 
 /* One explicit Runge-Kutta step of the whole wave function (orbitals then
  * CI) held on device: clcrk::prop with rk_order 1..4 ("standard").

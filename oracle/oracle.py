@@ -68,6 +68,10 @@ def lib():
         L.orc_herm_eig.argtypes = [ctypes.c_int, vp, vp, vp]
         L.orc_ci_sigma.argtypes = [vp, vp, vp, vp, vp]
         L.orc_ci_mkden.argtypes = [vp, vp]
+        L.orc_ci_layout.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.orc_ci_layout.restype = ctypes.c_int
+        L.orc_nstr.argtypes = [vp, ctypes.c_int]
+        L.orc_nstr.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -253,6 +257,19 @@ class Oracle:
         i2 = np.ascontiguousarray(i2)
         lib().orc_ci_sigma(self.h, _p(cic), _p(sig), _p(i1), _p(i2))
         return sig
+
+    def ci_layout(self):
+        """(alpha strings, beta strings, det_alpha, det_beta, cic0): the
+        occupation bit patterns of the ordered strings, the string pair of each
+        determinant and the determinant ormas_cic0 sets."""
+        na, nb = lib().orc_nstr(self.h, 0), lib().orc_nstr(self.h, 1)
+        sa = np.zeros(max(na, 1), np.uint32)
+        sb = np.zeros(max(nb, 1), np.uint32)
+        da = np.zeros(self.lcic, np.int32)
+        db = np.zeros(self.lcic, np.int32)
+        c0 = ctypes.c_int(0)
+        lib().orc_ci_layout(self.h, _p(sa), _p(sb), _p(da), _p(db), ctypes.byref(c0))
+        return sa[:na], sb[:nb], da, db, c0.value
 
     def ci_mkden(self, cic):
         cic = np.ascontiguousarray(cic, np.complex128)

@@ -97,6 +97,7 @@ typedef struct orc {
   uint32_t *str_a, *str_b;
   int *det_a, *det_b;  /* [lcic] string index per determinant */
   int *det_of;         /* [nstr_b*nstr_a] -> determinant index or -1 */
+  int cic0;            /* determinant ormas_cic0 sets to 1 */
   zc *ci_work;         /* nact^2 * lcic */
   /* RK work */
   zc *rk_w0, *rk_t, *rk_k;
@@ -545,37 +546,50 @@ static int bas_gen(orc* o) {
 /* ------------------------------------------------------------------ */
 static int popcount32(uint32_t v) { return __builtin_popcount(v); }
 
+/* Spin strings of one spin in the reference's ORMAS order, CAS (nsub = 1).
+ * ormas_arcwgt_spin (src/ormas/ormas_arcwgt.F90:52-104): vertex weights
+ * vtxwgt(m,k) of the (orbital, electron) graph and arc weights
+ * arc(m,k) = vtxwgt(m,k) - vtxwgt(m-1,k-1) for k <= m <= k + nact - nel.
+ * ormas_str_spin (src/ormas/ormas_str.F90:83-156) numbers the strings of the
+ * one distribution istr = 1..C(nact,nel) and decodes each with ormas_occvec
+ * (src/ormas/ormas_occvec.F90:2-29): from the highest orbital down, an
+ * orbital is occupied when the remaining rank is >= its arc weight. */
 static int make_strings(int nact, int nel, uint32_t** out) {
-  /* ascending lexicographic order of occupied-index tuples */
-  int cnt = 0, cap = 16;
-  uint32_t* s = xcalloc(cap, sizeof(uint32_t));
-  if (nel == 0) {
-    s[0] = 0;
-    *out = s;
-    return 1;
-  }
-  int idx[32];
-  for (int k = 0; k < nel; ++k) idx[k] = k;
-  for (;;) {
+  int vtx[33][33];
+  memset(vtx, 0, sizeof vtx);
+  vtx[0][0] = 1;
+  for (int k = 0; k <= nel; ++k) vtx[k][k] = 1;
+  for (int m = 0; m <= nact - nel; ++m) vtx[m][0] = 1;
+  for (int k = 1; k <= nel; ++k)
+    for (int m = k + 1; m <= k + nact - nel; ++m) vtx[m][k] = vtx[m - 1][k] + vtx[m - 1][k - 1];
+  int arc[33][33]; /* arc[m][k], 1-based m and k */
+  memset(arc, 0, sizeof arc);
+  for (int k = 1; k <= nel; ++k)
+    for (int m = k; m <= k + nact - nel; ++m) arc[m][k] = vtx[m][k] - vtx[m - 1][k - 1];
+  const int nstr = vtx[nact][nel];
+  uint32_t* s = xcalloc(nstr > 0 ? nstr : 1, sizeof(uint32_t));
+  for (int istr = 1; istr <= nstr; ++istr) {
     uint32_t v = 0;
-    for (int k = 0; k < nel; ++k) v |= 1u << idx[k];
-    if (cnt == cap) {
-      cap *= 2;
-      s = realloc(s, cap * sizeof(uint32_t));
+    int iel = nel, rest = istr - 1;
+    for (int iorb = nact; iorb >= 1; --iorb) {
+      if (iel <= 0) break;
+      if (rest >= arc[iorb][iel]) {
+        v |= 1u << (iorb - 1);
+        rest -= arc[iorb][iel];
+        --iel;
+      }
     }
-    s[cnt++] = v;
-    int k = nel - 1;
-    while (k >= 0 && idx[k] == nact - nel + k) --k;
-    if (k < 0) break;
-    idx[k]++;
-    for (int j = k + 1; j < nel; ++j) idx[j] = idx[j - 1] + 1;
+    s[istr - 1] = v;
   }
   *out = s;
-  return cnt;
+  return nstr;
 }
 
+/* total M of a spin string including the cores, mval_alph / mval_beta of
+ * ormas_madapt_str (src/ormas/ormas_madapt.F90:170-195) */
 static int str_m(const orc* o, uint32_t s) {
   int m = 0;
+  for (int k = 0; k < o->ncore; ++k) m += o->mval[k];
   for (int k = 0; k < o->nact; ++k)
     if (s >> k & 1u) m += o->mval[o->ncore + k];
   return m;
@@ -592,25 +606,68 @@ static int excite(uint32_t s, int p, int h, uint32_t* t) {
   return (popcount32(between) & 1) ? -1 : 1;
 }
 
+/* ormas_madapt_str (src/ormas/ormas_madapt.F90:102-501) for one
+ * distribution: the strings are reordered by their total M, keeping the
+ * ormas_str order within one M (map2to1, :221-246); *first2 receives
+ * map1to2(1), the new position of the first string. */
+static void madapt_order(const orc* o, uint32_t* s, int n, int* first2) {
+  int mmin = 0, mmax = 0;
+  for (int k = 0; k < n; ++k) {
+    int m = str_m(o, s[k]);
+    if (m < mmin) mmin = m;
+    if (m > mmax) mmax = m;
+  }
+  uint32_t* t = xcalloc(n, sizeof(uint32_t));
+  int istr2 = 0;
+  for (int m = mmin; m <= mmax; ++m)
+    for (int k = 0; k < n; ++k)
+      if (str_m(o, s[k]) == m) {
+        if (k == 0) *first2 = istr2;
+        t[istr2++] = s[k];
+      }
+  memcpy(s, t, n * sizeof(uint32_t));
+  free(t);
+}
+
 static void ci_setup(orc* o) {
   if (o->nact == 0) {
     o->lcic = 1;
+    o->cic0 = 0;
     return;
   }
   int na = o->nelact[0], nb = o->nelact[1];
   o->nstr_a = make_strings(o->nact, na, &o->str_a);
   o->nstr_b = make_strings(o->nact, nb, &o->str_b);
+  int first_a = 0, first_b = 0;
+  madapt_order(o, o->str_a, o->nstr_a, &first_a);
+  madapt_order(o, o->str_b, o->nstr_b, &first_b);
   size_t nfull = (size_t)o->nstr_a * o->nstr_b;
-  /* full alpha x beta product space (index ib*nstr_a + ia); the CI vector
-   * holds the determinants with M = mtot, beta-major, alpha fastest */
+  /* allowed alpha strings of each beta string (M_a + M_b = mtot, a
+   * contiguous range llstr_alph_beta .. + nstr_alph_beta - 1 of the M-ordered
+   * alpha strings) and the determinant offsets
+   * ntot_alph_beta(b) = sum of the earlier ranges - llstr_alph_beta(b) + 1
+   * (:381-435), 0-based here; determinant (a, b) is ntot_alph_beta(b) + a.
+   * det_of spans the full alpha x beta product space (index ib*nstr_a + ia). */
+  int* ntot = xcalloc(o->nstr_b, sizeof(int));
   o->det_of = xcalloc(nfull, sizeof(int));
+  for (size_t k = 0; k < nfull; ++k) o->det_of[k] = -1;
   int cnt = 0;
-  for (int ib = 0; ib < o->nstr_b; ++ib)
-    for (int ia = 0; ia < o->nstr_a; ++ia) {
-      int ok = str_m(o, o->str_a[ia]) + str_m(o, o->str_b[ib]) == o->mtot;
-      o->det_of[(size_t)ib * o->nstr_a + ia] = ok ? cnt++ : -1;
-    }
+  for (int ib = 0; ib < o->nstr_b; ++ib) {
+    int ll = -1, nal = 0;
+    for (int ia = 0; ia < o->nstr_a; ++ia)
+      if (str_m(o, o->str_a[ia]) + str_m(o, o->str_b[ib]) == o->mtot) {
+        if (ll < 0) ll = ia;
+        ++nal;
+      }
+    ntot[ib] = cnt - ll; /* llstr_alph_beta = 0 (ll = -1) when none is allowed */
+    for (int ia = ll; ia >= 0 && ia < ll + nal; ++ia) o->det_of[(size_t)ib * o->nstr_a + ia] = ntot[ib] + ia;
+    cnt += nal;
+  }
   o->lcic = cnt;
+  /* ormas_cic0 (src/ormas/ormas_cic0.F90:24): the determinant of the first
+   * alpha and beta strings, ntot_alph_beta(map1to2_beta(1)) + map1to2_alph(1) */
+  o->cic0 = ntot[first_b] + first_a;
+  free(ntot);
   o->det_a = xcalloc(cnt, sizeof(int));
   o->det_b = xcalloc(cnt, sizeof(int));
   for (int ib = 0; ib < o->nstr_b; ++ib)
@@ -2066,3 +2123,21 @@ void orc_ci_sigma(orc* o, const zc* cic, zc* sig, const zc* int1e, const zc* int
   ci_sigma(o, cic, sig);
 }
 void orc_ci_mkden(orc* o, const zc* cic) { ci_mkden(o, cic); }
+
+/* CI layout: the occupation bit patterns of the ordered alpha and beta
+ * strings (bit k = active orbital k), the string pair of every determinant
+ * and the ormas_cic0 determinant; returns lcic (1 and no strings without
+ * active orbitals). */
+int orc_ci_layout(const orc* o, uint32_t* str_a, uint32_t* str_b, int* det_a, int* det_b, int* cic0) {
+  if (o->nact == 0) {
+    *cic0 = 0;
+    return 1;
+  }
+  if (str_a) memcpy(str_a, o->str_a, o->nstr_a * sizeof(uint32_t));
+  if (str_b) memcpy(str_b, o->str_b, o->nstr_b * sizeof(uint32_t));
+  if (det_a) memcpy(det_a, o->det_a, o->lcic * sizeof(int));
+  if (det_b) memcpy(det_b, o->det_b, o->lcic * sizeof(int));
+  *cic0 = o->cic0;
+  return o->lcic;
+}
+int orc_nstr(const orc* o, int spin) { return o->nact == 0 ? 0 : (spin == 0 ? o->nstr_a : o->nstr_b); }

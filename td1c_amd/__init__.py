@@ -23,7 +23,7 @@ import numpy as np
 from .abi import HEADER_FUNCTIONS, Field, Problem, TD1CField, TD1CProblem, TD1CSizes
 
 __all__ = ["Problem", "Field", "HProd", "CRK", "load_library", "library_path", "field_value", "field_period",
-           "wfn_write", "wfn_read"]
+           "wfn_write", "wfn_read", "ci_layout"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "libtd1c_hprod.so")
@@ -85,6 +85,8 @@ def load_library():
     L.td1c_wfn_write.restype = ci
     L.td1c_wfn_read.argtypes = [cp, cp, ci, ci, ci, ci, ci, vp]
     L.td1c_wfn_read.restype = ci
+    L.td1c_ci_layout.argtypes = [ctypes.POINTER(TD1CProblem), vp, vp, vp, vp, vp, vp, vp, vp]
+    L.td1c_ci_layout.restype = ci
     L.td1c_htotx.argtypes = [vp, ctypes.c_double, vp, vp, vp, vp]
     L.td1c_htotx.restype = ci
     for name in ("td1c_get_den1", "td1c_get_den2", "td1c_get_int1e", "td1c_get_int2e", "td1c_get_xmat"):
@@ -149,12 +151,35 @@ def wfn_write(orb_path: str, cic_path: str, prob: Problem, lcic: int, wfn: np.nd
                                          prob.nfun, lcic, _ptr(w)), "td1c_wfn_write")
 
 
+def ci_layout(prob: Problem):
+    """td1c_ci_layout: the CI vector's layout in the reference's ORMAS order.
+    Returns a dict with the ordered alpha / beta string occupation bit
+    patterns (bit k = active orbital k), the string pair of each determinant
+    (det_alpha, det_beta), lcic and cic0 (the determinant ormas_cic0 sets,
+    -1 without active orbitals).  Host computation, no device."""
+    L = load_library()
+    c, keep = prob.to_c()
+    na, nb, lc, c0 = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    _check(L.td1c_ci_layout(ctypes.byref(c), ctypes.byref(na), ctypes.byref(nb), ctypes.byref(lc),
+                            ctypes.byref(c0), None, None, None, None), "td1c_ci_layout")
+    sa = np.zeros(max(na.value, 1), np.uint32)
+    sb = np.zeros(max(nb.value, 1), np.uint32)
+    da = np.zeros(lc.value, np.int32)
+    db = np.zeros(lc.value, np.int32)
+    _check(L.td1c_ci_layout(ctypes.byref(c), None, None, None, None, _ptr(sa), _ptr(sb), _ptr(da), _ptr(db)),
+           "td1c_ci_layout")
+    del keep
+    return {"str_alpha": sa[:na.value], "str_beta": sb[:nb.value], "det_alpha": da, "det_beta": db,
+            "lcic": lc.value, "cic0": c0.value}
+
+
 def wfn_read(orb_path: str, cic_path: str, prob: Problem, lcic: int):
     """clwfn::read: returns (state, ci_read) for the problem's sizes."""
     nrad = prob.nr + 1
     w = np.zeros(prob.nfun * prob.nbas + lcic, np.complex128)
+    cic0 = ci_layout(prob)["cic0"] if prob.nact > 0 else -1
     rc = load_library().td1c_wfn_read(orb_path.encode(), cic_path.encode(), nrad, prob.lmax1, prob.nfun, lcic,
-                                      prob.nact, _ptr(w))
+                                      cic0, _ptr(w))
     if rc not in (0, 1):
         _check(rc, "td1c_wfn_read")
     return w, rc == 0

@@ -982,6 +982,44 @@ int td1c_ctx_destroy(td1c_ctx* ctx) {
   return 0;
 }
 
+int td1c_ci_layout(const td1c_problem* p, int* nstr_alpha, int* nstr_beta, int* lcic, int* cic0,
+                   unsigned int* str_alpha, unsigned int* str_beta, int* det_alpha, int* det_beta) {
+  if (!p) return fail(-1, "null argument");
+  Setup S;
+  S.nfcore = p->nfcore2 + p->nfcore1;
+  S.ncore = S.nfcore + p->ndcore;
+  S.nact = p->nact;
+  S.nfun = S.ncore + S.nact;
+  if (S.nact < 0 || S.nact > 16 || S.ncore < 0) return fail(-1, "bad orbital counts (nact <= 16)");
+  S.mval.assign(S.nfun, 0);
+  for (int i = 0; i < S.nfun; ++i) S.mval[i] = p->mval ? p->mval[i] : 0;
+  S.nelact[0] = p->nel_alpha - S.ncore;
+  S.nelact[1] = p->nel_beta - S.ncore;
+  if (S.nelact[0] < 0 || S.nelact[1] < 0 || S.nelact[0] > S.nact || S.nelact[1] > S.nact)
+    return fail(-1, "electron count does not fit the orbital classes");
+  S.mtot = p->mtot;
+  if (!S.build_ci()) return fail(-1, S.error);
+  if (nstr_alpha) *nstr_alpha = S.nact > 0 ? S.nstr_a : 0;
+  if (nstr_beta) *nstr_beta = S.nact > 0 ? S.nstr_b : 0;
+  if (lcic) *lcic = S.lcic;
+  if (cic0) *cic0 = S.nact > 0 ? S.cic0 : -1;
+  if (S.nact > 0) {
+    if (str_alpha) std::copy(S.str_a.begin(), S.str_a.end(), str_alpha);
+    if (str_beta) std::copy(S.str_b.begin(), S.str_b.end(), str_beta);
+    for (int ib = 0; ib < S.nstr_b; ++ib)
+      for (int ia = 0; ia < S.nstr_a; ++ia) {
+        const int d = S.det_of[(size_t)ib * S.nstr_a + ia];
+        if (d < 0) continue;
+        if (det_alpha) det_alpha[d] = ia;
+        if (det_beta) det_beta[d] = ib;
+      }
+  } else {
+    if (det_alpha) det_alpha[0] = 0;
+    if (det_beta) det_beta[0] = 0;
+  }
+  return 0;
+}
+
 int td1c_get_sizes(const td1c_ctx* c, td1c_sizes* out) {
   if (!c || !out) return fail(-1, "null argument");
   out->nrad = c->S.nrad;
@@ -1411,12 +1449,14 @@ int td1c_wfn_write(const char* orb_path, const char* cic_path, int nrad, int lma
 // clwfn::read (src/wfn/clwfn.cpp:156-205): the wave function is cleared, the
 // file's orbitals are read and those inside (nfun, lmax1, nrad) kept (a
 // smaller file leaves zeros, a larger one is truncated); the CI vector is read
-// when the file's lcic equals lcic, else it is set to the first determinant
-// (ormas_cic0; left zero without active orbitals, nact == 0) and 1 is returned.
-int td1c_wfn_read(const char* orb_path, const char* cic_path, int nrad, int lmax1, int nfun, int lcic, int nact,
+// when the file's lcic equals lcic, else determinant cic0 is set to 1
+// (ormas_cic0, src/ormas/ormas_cic0.F90:24; left zero when cic0 < 0, i.e.
+// without active orbitals) and 1 is returned.
+int td1c_wfn_read(const char* orb_path, const char* cic_path, int nrad, int lmax1, int nfun, int lcic, int cic0,
                   double* wfn) {
   if (!orb_path || !cic_path || !wfn) return fail(-1, "null argument");
   if (nrad < 2 || lmax1 < 0 || nfun < 0 || lcic < 0) return fail(-1, "bad sizes");
+  if (cic0 >= lcic) return fail(-1, "cic0 outside the CI vector");
   const size_t no = (size_t)nfun * (lmax1 + 1) * (nrad - 1);
   std::fill(wfn, wfn + 2 * (no + lcic), 0.0);
   FILE* fo = std::fopen(orb_path, "r");
@@ -1450,7 +1490,7 @@ int td1c_wfn_read(const char* orb_path, const char* cic_path, int nrad, int lmax
   }
   int rc = 0;
   if (lcic0 != lcic) {
-    if (nact > 0 && lcic > 0) wfn[2 * no] = 1.0;
+    if (cic0 >= 0) wfn[2 * (no + cic0)] = 1.0;
     rc = 1;
   } else {
     for (int d = 0; d < lcic; ++d)
@@ -1721,6 +1761,11 @@ int td1c_set_graph(td1c_ctx* c, int on) {
 int td1c_set_transform_path(td1c_ctx* c, int path) {
   if (!c) return fail(-1, "null argument");
   if (path < 0 || path > 2) return fail(-1, "transform path must be 0, 1 or 2");
+  if (path != c->tpath && c->gexec) {
+    // a captured step holds the kernels of the old path: capture again
+    HIP_TRY(hipGraphExecDestroy(c->gexec));
+    c->gexec = nullptr;
+  }
   c->tpath = path;
   return 0;
 }

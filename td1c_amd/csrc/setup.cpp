@@ -491,62 +491,95 @@ bool Setup::build(const td1c_problem& p) {
     if (i != j) pair_conj[(size_t)i * nfun + j] = 1;  // v(i,j) = conj(v(j,i))
   }
 
-  // ---- CAS strings (ormas_init with one subspace) ----
-  if (nact > 0) {
-    auto strings = [&](int ne, std::vector<uint32_t>& s) {
-      s.clear();
-      for (uint32_t v = 0; v < (1u << nact); ++v)
-        if (popc(v) == ne) s.push_back(v);
-      // ascending lexicographic order of the occupied-index tuples
-      std::sort(s.begin(), s.end(), [&](uint32_t a, uint32_t b) {
-        for (int k = 0; k < nact; ++k) {
-          bool ia = a >> k & 1u, ib = b >> k & 1u;
-          if (ia != ib) return ia;  // the tuple with the smaller next index first
-        }
-        return false;
-      });
-    };
-    strings(nelact[0], str_a);
-    strings(nelact[1], str_b);
-    nstr_a = (int)str_a.size();
-    nstr_b = (int)str_b.size();
-    auto mstr = [&](uint32_t s) { int m = 0; for (int k = 0; k < nact; ++k) if (s >> k & 1u) m += mval[ncore + k]; return m; };
-    det_of.assign((size_t)nstr_a * nstr_b, -1);
-    lcic = 0;
-    for (int ib = 0; ib < nstr_b; ++ib)
-      for (int ia = 0; ia < nstr_a; ++ia)
-        if (mstr(str_a[ia]) + mstr(str_b[ib]) == mtot) det_of[(size_t)ib * nstr_a + ia] = lcic++;
-    if (lcic == 0) { error = "no determinant has M = mtot"; return false; }
-    auto table = [&](const std::vector<uint32_t>& s, std::vector<int>& ex, std::vector<int>& sg) {
-      int ns = (int)s.size();
-      ex.assign((size_t)ns * nact * nact, -1);
-      sg.assign((size_t)ns * nact * nact, 0);
-      for (int a = 0; a < ns; ++a)
-        for (int pp = 0; pp < nact; ++pp)
-          for (int q = 0; q < nact; ++q) {
-            uint32_t v = s[a];
-            if (!(v >> q & 1u)) continue;
-            uint32_t t;
-            int sign = 1;
-            if (pp == q) {
-              t = v;
-            } else {
-              if (v >> pp & 1u) continue;
-              int lo = std::min(pp, q), hi = std::max(pp, q);
-              uint32_t between = v & (((1u << hi) - 1u) & ~((1u << (lo + 1)) - 1u));
-              sign = (popc(between) & 1) ? -1 : 1;
-              t = (v & ~(1u << q)) | (1u << pp);
-            }
-            int tgt = (int)(std::find(s.begin(), s.end(), t) - s.begin());
-            ex[((size_t)a * nact + pp) * nact + q] = tgt;
-            sg[((size_t)a * nact + pp) * nact + q] = sign;
-          }
-    };
-    table(str_a, exA, sgA);
-    table(str_b, exB, sgB);
-  } else {
+  return build_ci();
+}
+
+// CAS determinant space in the reference's layout (ormas_init with one
+// subspace, src/ormas/ormas_init.F90:32-53).  Strings of one spin are first
+// numbered by arc weights (ormas_arcwgt, src/ormas/ormas_arcwgt.F90:100-104;
+// ormas_occvec, src/ormas/ormas_occvec.F90:17-25): the string with occupied
+// orbitals o_1 < ... < o_k has rank sum_k C(o_k - 1, k), which is the
+// ascending order of the occupation bit patterns (bit o = orbital o).
+// ormas_madapt_str (src/ormas/ormas_madapt.F90:170-246) then reorders them by
+// their total M (cores included), keeping that order within one M, and the CI
+// vector holds the determinants with M_alpha + M_beta = mtot, beta string
+// major and alpha fastest: index ntot_alph_beta(beta) + alpha (:381-435).
+bool Setup::build_ci() {
+  if (nact == 0) {
+    nstr_a = nstr_b = 1;
     lcic = 1;
+    cic0 = 0;
+    return true;
   }
+  int mcore = 0;
+  for (int k = 0; k < ncore; ++k) mcore += mval[k];
+  auto mstr = [&](uint32_t v) {
+    int m = mcore;
+    for (int k = 0; k < nact; ++k)
+      if (v >> k & 1u) m += mval[ncore + k];
+    return m;
+  };
+  // returns the new position of the first (rank-0) string
+  auto strings = [&](int ne, std::vector<uint32_t>& s) {
+    s.clear();
+    for (uint32_t v = 0; v < (1u << nact); ++v)
+      if (popc(v) == ne) s.push_back(v);
+    const uint32_t first = s.front();
+    std::stable_sort(s.begin(), s.end(), [&](uint32_t a, uint32_t b) { return mstr(a) < mstr(b); });
+    return (int)(std::find(s.begin(), s.end(), first) - s.begin());
+  };
+  const int first_a = strings(nelact[0], str_a);
+  const int first_b = strings(nelact[1], str_b);
+  nstr_a = (int)str_a.size();
+  nstr_b = (int)str_b.size();
+  det_of.assign((size_t)nstr_a * nstr_b, -1);
+  std::vector<int> ntot(nstr_b);
+  lcic = 0;
+  for (int ib = 0; ib < nstr_b; ++ib) {
+    // the alpha strings allowed with beta string ib are one contiguous range
+    int ll = -1, cnt = 0;
+    for (int ia = 0; ia < nstr_a; ++ia)
+      if (mstr(str_a[ia]) + mstr(str_b[ib]) == mtot) {
+        if (ll < 0) ll = ia;
+        ++cnt;
+      }
+    ntot[ib] = lcic - ll;  // ll = -1 (llstr_alph_beta = 0) when none is allowed
+    for (int ia = ll; ia >= 0 && ia < ll + cnt; ++ia) det_of[(size_t)ib * nstr_a + ia] = ntot[ib] + ia;
+    lcic += cnt;
+  }
+  if (lcic == 0) { error = "no determinant has M = mtot"; return false; }
+  // ormas_cic0 (src/ormas/ormas_cic0.F90:24): the determinant of the first
+  // alpha and beta strings, ntot_alph_beta(map1to2_beta(1)) + map1to2_alph(1)
+  cic0 = ntot[first_b] + first_a;
+  auto table = [&](const std::vector<uint32_t>& s, std::vector<int>& ex, std::vector<int>& sg) {
+    const int ns = (int)s.size();
+    std::vector<int> pos(1u << nact, -1);
+    for (int a = 0; a < ns; ++a) pos[s[a]] = a;
+    ex.assign((size_t)ns * nact * nact, -1);
+    sg.assign((size_t)ns * nact * nact, 0);
+    for (int a = 0; a < ns; ++a)
+      for (int pp = 0; pp < nact; ++pp)
+        for (int q = 0; q < nact; ++q) {
+          const uint32_t v = s[a];
+          if (!(v >> q & 1u)) continue;
+          uint32_t t;
+          int sign = 1;
+          if (pp == q) {
+            t = v;
+          } else {
+            if (v >> pp & 1u) continue;
+            // ormas_sgn1x (src/ormas/ormas_sgn1x.F90): (-1)^(occupied orbitals strictly between)
+            const int lo = std::min(pp, q), hi = std::max(pp, q);
+            const uint32_t between = v & (((1u << hi) - 1u) & ~((1u << (lo + 1)) - 1u));
+            sign = (popc(between) & 1) ? -1 : 1;
+            t = (v & ~(1u << q)) | (1u << pp);
+          }
+          ex[((size_t)a * nact + pp) * nact + q] = pos[t];
+          sg[((size_t)a * nact + pp) * nact + q] = sign;
+        }
+  };
+  table(str_a, exA, sgA);
+  table(str_b, exB, sgB);
   return true;
 }
 

@@ -82,6 +82,7 @@ struct Setup {
   int nstr_a = 1, nstr_b = 1, lcic = 1;
   std::vector<uint32_t> str_a, str_b;
   std::vector<int> det_of;   // [nstr_b*nstr_a] -> CI index or -1
+  int cic0 = 0;              // the determinant ormas_cic0 sets to 1
   // single-excitation tables on the full string spaces:
   // exA[(ia*nact + p)*nact + q] = target string index, sgA = sign (0 if none)
   std::vector<int> exA, exB;
@@ -89,6 +90,7 @@ struct Setup {
 
   std::string error;
   bool build(const td1c_problem& p);
+  bool build_ci();  // the CAS string tables (called by build)
 };
 
 }  // namespace td1c

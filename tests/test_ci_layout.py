@@ -1,0 +1,212 @@
+"""CI vector layout: the reference's ORMAS determinant addressing for one
+subspace (CAS).  CPU tests: the library's td1c_ci_layout is host code and the
+oracle is plain C.
+
+The expected layout is enumerated here straight from the reference's
+algorithm, step by step:
+  * ormas_arcwgt_spin (src/ormas/ormas_arcwgt.F90:52-104): vertex weights and
+    arc weights arc(m, k) = vtxwgt(m, k) - vtxwgt(m-1, k-1);
+  * ormas_str_spin / ormas_occvec (src/ormas/ormas_str.F90:83-156,
+    src/ormas/ormas_occvec.F90:2-29): string istr = 1..nstr decoded from its
+    rank by the arc weights;
+  * ormas_madapt_str (src/ormas/ormas_madapt.F90:170-246): strings listed per
+    total M (cores included) in increasing M, the original order within one M;
+  * the allowed alpha range of each beta string, ntot_alph_beta and the
+    determinant list mapr_detx (:381-435, :448-461);
+  * ormas_cic0 (src/ormas/ormas_cic0.F90:24).
+It is compared entry for entry with the library and the oracle.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import td1c_amd
+from td1c_amd.abi import Problem
+from oracle.oracle import Oracle
+
+needs_lib = pytest.mark.skipif(not os.path.exists(td1c_amd.library_path()), reason="library not built")
+
+
+def ref_strings(nact, nel):
+    """ormas_arcwgt_spin + ormas_occvec: occupation vectors (1-based orbital
+    lists) of strings istr = 1..nstr."""
+    vtx = np.zeros((nact + 1, nel + 1), dtype=np.int64)
+    vtx[0, 0] = 1
+    for k in range(nel + 1):
+        vtx[k, k] = 1
+    for m in range(nact - nel + 1):
+        vtx[m, 0] = 1
+    for k in range(1, nel + 1):
+        for m in range(k + 1, k + nact - nel + 1):
+            vtx[m, k] = vtx[m - 1, k] + vtx[m - 1, k - 1]
+    arc = np.zeros((nact + 1, nel + 1), dtype=np.int64)  # arc[m, k], 1-based
+    for k in range(1, nel + 1):
+        for m in range(k, k + nact - nel + 1):
+            arc[m, k] = vtx[m, k] - vtx[m - 1, k - 1]
+    out = []
+    for istr in range(1, int(vtx[nact, nel]) + 1):
+        occ = []
+        iel, rest = nel, istr - 1
+        for iorb in range(nact, 0, -1):
+            if iel <= 0:
+                break
+            if rest >= arc[iorb, iel]:
+                occ.append(iorb)
+                rest -= arc[iorb, iel]
+                iel -= 1
+        out.append(sorted(occ))
+    return out
+
+
+def ref_layout(nact, nel_a, nel_b, mval_core, mval_act, mtot):
+    """ormas_madapt_str for one distribution.  Returns (alpha strings, beta
+    strings as bit patterns in M order, det list of (alpha, beta) 0-based
+    string indices, cic0 0-based)."""
+    mvalc = sum(mval_core)
+
+    def order(nel):
+        strs = ref_strings(nact, nel)
+        mv = [mvalc + sum(mval_act[o - 1] for o in s) for s in strs]
+        mmin, mmax = min(0, min(mv)), max(0, max(mv))
+        map2to1 = []
+        for m in range(mmin, mmax + 1):
+            map2to1 += [i for i in range(len(strs)) if mv[i] == m]
+        map1to2 = {i: j for j, i in enumerate(map2to1)}
+        bits = [sum(1 << (o - 1) for o in strs[i]) for i in map2to1]
+        return bits, [mv[i] for i in map2to1], map1to2
+
+    sa, ma, a1to2 = order(nel_a)
+    sb, mb, b1to2 = order(nel_b)
+    llstr, nstr = [], []
+    for ib in range(len(sb)):
+        allowed = [ja for ja in range(len(sa)) if ma[ja] + mb[ib] == mtot]
+        llstr.append(allowed[0] + 1 if allowed else 0)  # 1-based, 0 = none
+        nstr.append(len(allowed))
+    ntot, tmp = [], 0
+    for ib in range(len(sb)):
+        ntot.append(tmp - llstr[ib] + 1)
+        tmp += nstr[ib]
+    dets = []
+    for ib in range(len(sb)):
+        for ja in range(llstr[ib], llstr[ib] + nstr[ib]):  # 1-based alpha index
+            dets.append((ja - 1, ib))
+            assert ntot[ib] + ja == len(dets)  # cic(ntot_alph_beta(b) + a) is this determinant
+    cic0 = ntot[b1to2[0]] + (a1to2[0] + 1) - 1
+    return sa, sb, dets, cic0
+
+
+def test_arc_weights_give_colex_order():
+    """The arc-weight numbering of 2 electrons in 4 orbitals is colex:
+    (1,2), (1,3), (2,3), (1,4), (2,4), (3,4); 3 in 5 starts (1,2,3), (1,2,4),
+    (1,3,4), (2,3,4), (1,2,5)."""
+    assert ref_strings(4, 2) == [[1, 2], [1, 3], [2, 3], [1, 4], [2, 4], [3, 4]]
+    assert ref_strings(5, 3)[:5] == [[1, 2, 3], [1, 2, 4], [1, 3, 4], [2, 3, 4], [1, 2, 5]]
+    assert ref_strings(3, 0) == [[]]
+    assert len(ref_strings(8, 3)) == 56
+
+
+def test_madapt_order_hand_case():
+    """CAS(2e, 3o) with active m = (0, 1, -1), one electron per spin: strings
+    {1}, {2}, {3} have M = 0, 1, -1, so the M order is {3}, {1}, {2}; the
+    M = 0 determinants are beta {3} x alpha {2}, beta {1} x alpha {1},
+    beta {2} x alpha {3}, and ormas_cic0 sets the second one."""
+    sa, sb, dets, cic0 = ref_layout(3, 1, 1, [], [0, 1, -1], 0)
+    assert sa == [0b100, 0b001, 0b010] and sb == sa
+    assert dets == [(2, 0), (1, 1), (0, 2)]
+    assert cic0 == 1
+
+
+def _cases():
+    # (name, Problem): nact, electrons, m values and mtot as the layout sees them
+    def P(**kw):
+        base = dict(nfe=3, ndvr=4, rmax=6.0, lmax1=2, lmax2=4, znuc=4.0)
+        base.update(kw)
+        return Problem(**base)
+
+    return [
+        ("cas2e4o", P(nact=4, nel_alpha=1, nel_beta=1)),
+        ("cas4e4o", P(nact=4, nel_alpha=2, nel_beta=2)),
+        ("core_cas4e6o_m", P(ndcore=1, nact=6, nel_alpha=3, nel_beta=3,
+                                   mval=[0, 0, 0, 1, -1, 0, 0], lmax1=3, lmax2=6)),
+        ("cas4e6o_m1", P(nact=6, nel_alpha=2, nel_beta=2, mval=[1, 0, -1, 1, 0, -1], mtot=1,
+                               lmax1=3, lmax2=6)),
+        ("cas4e6o_mm1", P(nact=6, nel_alpha=2, nel_beta=2, mval=[1, 0, -1, 1, 0, -1], mtot=-1,
+                                lmax1=3, lmax2=6)),
+        ("cas6e7o", P(nact=7, nel_alpha=3, nel_beta=3)),
+        ("cas6e8o", P(nact=8, nel_alpha=3, nel_beta=3)),
+        ("cas6e8o_m", P(nact=8, nel_alpha=3, nel_beta=3, mval=[0, 1, -1, 0, 2, -2, 1, -1],
+                              lmax1=3, lmax2=6)),
+        ("cas5e6o_open", P(nact=6, nel_alpha=3, nel_beta=2, mval=[0, 1, -1, 0, 1, -1],
+                                 lmax1=2, lmax2=4)),
+        ("fc_core_cas2e3o", P(nfcore1=1, ndcore=1, nact=3, nel_alpha=3, nel_beta=3,
+                                    mval=[0, 0, 0, 1, -1])),
+    ]
+
+
+def _expected(prob):
+    nc = prob.ncore
+    mval = list(prob.mval) if prob.mval is not None else [0] * prob.nfun
+    return ref_layout(prob.nact, prob.nel_alpha - nc, prob.nel_beta - nc, mval[:nc], mval[nc:], prob.mtot)
+
+
+@needs_lib
+@pytest.mark.parametrize("name,prob", _cases(), ids=[c[0] for c in _cases()])
+def test_library_layout_is_the_reference_order(name, prob):
+    sa, sb, dets, cic0 = _expected(prob)
+    lay = td1c_amd.ci_layout(prob)
+    assert lay["str_alpha"].tolist() == sa
+    assert lay["str_beta"].tolist() == sb
+    assert lay["lcic"] == len(dets)
+    assert list(zip(lay["det_alpha"].tolist(), lay["det_beta"].tolist())) == dets
+    assert lay["cic0"] == cic0
+
+
+@pytest.mark.parametrize("name,prob", _cases(), ids=[c[0] for c in _cases()])
+def test_oracle_layout_is_the_reference_order(name, prob):
+    sa, sb, dets, cic0 = _expected(prob)
+    o = Oracle(prob)
+    osa, osb, oda, odb, oc0 = o.ci_layout()
+    assert osa.tolist() == sa and osb.tolist() == sb
+    assert list(zip(oda.tolist(), odb.tolist())) == dets
+    assert oc0 == cic0
+
+
+@needs_lib
+def test_layout_without_active_orbitals():
+    prob = Problem(nfe=3, ndvr=4, rmax=6.0, lmax1=2, lmax2=4, znuc=2.0, ndcore=1)
+    lay = td1c_amd.ci_layout(prob)
+    assert lay["lcic"] == 1 and lay["cic0"] == -1 and lay["str_alpha"].size == 0
+
+
+@needs_lib
+def test_wfn_read_resets_to_the_cic0_determinant(tmp_path):
+    """A .cic file of another length resets the CI vector to ormas_cic0's
+    determinant, which is not index 0 once the strings are M-ordered."""
+    prob = Problem(nfe=3, ndvr=4, rmax=6.0, lmax1=2, lmax2=4, znuc=2.0, nact=3, nel_alpha=1, nel_beta=1,
+                   mval=[0, 1, -1])
+    lay = td1c_amd.ci_layout(prob)
+    assert lay["cic0"] == 1 and lay["lcic"] == 3
+    n = prob.nfun * prob.nbas
+    w = np.zeros(n + 5, np.complex128)
+    po, pc = str(tmp_path / "x.orb"), str(tmp_path / "x.cic")
+    td1c_amd.wfn_write(po, pc, prob, 5, w)
+    r, ci_read = td1c_amd.wfn_read(po, pc, prob, 3)
+    assert not ci_read
+    assert r[n:].tolist() == [0, 1, 0]
+
+
+@needs_lib
+def test_wfn_read_rejects_an_impossible_reset(tmp_path):
+    """When the first alpha and beta strings together miss M = mtot, the
+    reference's ormas_cic0 index lies past the CI vector; reading reports an
+    error instead of writing there."""
+    prob = Problem(nfe=3, ndvr=4, rmax=6.0, lmax1=2, lmax2=4, znuc=2.0, nact=3, nel_alpha=1, nel_beta=1,
+                   mval=[1, -1, 0])
+    lay = td1c_amd.ci_layout(prob)
+    assert lay["cic0"] >= lay["lcic"]
+    n = prob.nfun * prob.nbas
+    po, pc = str(tmp_path / "x.orb"), str(tmp_path / "x.cic")
+    td1c_amd.wfn_write(po, pc, prob, 5, np.zeros(n + 5, np.complex128))
+    with pytest.raises(RuntimeError):
+        td1c_amd.wfn_read(po, pc, prob, lay["lcic"])

@@ -124,17 +124,21 @@ def test_mask_matches_oracle(gpu_lib):
     np.testing.assert_allclose(d.cpu().numpy(), o.mask(w), rtol=0, atol=1e-15)
 
 
-def test_legacy_entry_point(gpu_lib):
-    """hprod_htot_ with the reference's argument list, host pointers."""
+@pytest.mark.parametrize("name", ["he_small", "cas_small", "cas6_small"])
+def test_legacy_entry_point(gpu_lib, name):
+    """hprod_htot_ with the reference's argument list, host pointers: the
+    orbital and the CI derivative (hcic) against the oracle."""
     import ctypes
     import td1c_amd
     from oracle.oracle import Oracle
 
-    prob = C.he_small()
+    prob = getattr(C, name)()
     o = Oracle(prob)
     w = C.wfn_for(prob, o.lcic)
+    o.set_wfn0(w)
     ref = o.htot(0.5, C.LF_LEN, w)
     hp = td1c_amd.HProd(prob)
+    hp.set_wfn0(w)
     hp.bind()
     orb = np.ascontiguousarray(w[:o.norb])
     cic = np.ascontiguousarray(w[o.norb:])
@@ -146,6 +150,8 @@ def test_legacy_entry_point(gpu_lib):
                         cic.ctypes.data_as(ctypes.c_void_p), hw.ctypes.data_as(ctypes.c_void_p),
                         hc.ctypes.data_as(ctypes.c_void_p))
     assert _rel(hw, ref[:o.norb]) < RTOL
+    if o.lcic > 1:
+        assert _rel(hc, ref[o.norb:]) < RTOL
 
 
 @pytest.mark.parametrize("graph", [True, False])

@@ -435,7 +435,8 @@ def test_doubly_occupied_active_equals_dynamic_core(mval):
     g + v2 on both orbitals and the same total energy as two dynamical cores."""
     base = dict(nfe=6, ndvr=10, rmax=20.0, lmax1=3, lmax2=6, znuc=3.0, mval=mval, nel_alpha=2, nel_beta=2)
     pa = Problem(ndcore=2, nact=0, **base)
-    pb = Problem(ndcore=1, nact=1, mtot=2 * mval[1], **base)
+    # mtot is the total M with the cores (ormas_madapt_str adds their M to each spin string)
+    pb = Problem(ndcore=1, nact=1, mtot=2 * (mval[0] + mval[1]), **base)
     oa, ob = O.Oracle(pa), O.Oracle(pb)
     wa = W.make_wfn(pa, 21, oa.lcic)
     nb = pa.nr * (pa.lmax1 + 1)
