@@ -34,12 +34,14 @@
  * dipole forms (closed forms and Ehrenfest relations), the ionisation
  * statistics (determinant generating function, moments), Gram-Schmidt and
  * imaginary-time relaxation to the He Hartree-Fock energy, and the
- * frozen-core energy (He 1s^2, all-frozen closed shell).
+ * frozen-core energy (He 1s^2, all-frozen closed shell), and the CI layout
+ * (determinant order by an enumeration of the ORMAS arc-weight / M-sort
+ * algorithm, determinant phase by RDMs built with plain second
+ * quantisation, tests/test_ci_layout.py).
  * Parity unpinned:
  *  - the sign convention of the m < 0 Legendre columns (copied from |m|
  *    without (-1)^m, as sph_gen.F90 does; only products of two columns enter
  *    hprod, so no pinned quantity sees it);
- *  - the order and phase of the CAS determinants (the cic layout);
  *  - the KLI/LDA v2xfc (input only), and with it the type -2 part of the
  *    frozen-core energy (v2xfc enters through v2jfc and again as fcx2, as
  *    hprod_v2fc_enefc has it).

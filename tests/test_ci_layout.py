@@ -210,3 +210,84 @@ def test_wfn_read_rejects_an_impossible_reset(tmp_path):
     td1c_amd.wfn_write(po, pc, prob, 5, np.zeros(n + 5, np.complex128))
     with pytest.raises(RuntimeError):
         td1c_amd.wfn_read(po, pc, prob, lay["lcic"])
+
+
+def _apply(op, mode, det):
+    """Fermionic creation (op = +1) or annihilation (op = -1) on mode `mode`
+    of an occupation bit pattern over spin-orbital modes; returns
+    (sign, new pattern) or None.  The sign is (-1) to the number of occupied
+    modes before `mode`: determinants are products of creation operators in
+    ascending mode order."""
+    if det is None:
+        return None
+    sgn, bits = det
+    occ = (bits >> mode) & 1
+    if (op > 0 and occ) or (op < 0 and not occ):
+        return None
+    if bin(bits & ((1 << mode) - 1)).count("1") % 2:
+        sgn = -sgn
+    return sgn, bits ^ (1 << mode)
+
+
+def _rdms_second_quantised(nact, sa, sb, dets, cic):
+    """Spin-summed den1(a, b) = <a+_b a_a> and den2(a, b, c, d) =
+    <a+_b a+_d a_c a_a> of a CI vector, by applying the operators to each
+    determinant in the spin-orbital basis (alpha modes 0..n-1, beta modes
+    n..2n-1).  Uses nothing of the ORMAS string tables but the determinant
+    list, so it checks the layout's phase convention independently."""
+    n = nact
+    index = {}
+    occs = []
+    for k, (ia, ib) in enumerate(dets):
+        bits = int(sa[ia]) | (int(sb[ib]) << n)
+        index[bits] = k
+        occs.append(bits)
+    d1 = np.zeros((n, n), np.complex128)
+    d2 = np.zeros((n, n, n, n), np.complex128)
+    for k, bits in enumerate(occs):
+        ck = cic[k]
+        if ck == 0:
+            continue
+        for s in range(2):
+            for a in range(n):
+                r1 = _apply(-1, a + s * n, (1, bits))
+                if r1 is None:
+                    continue
+                for b in range(n):
+                    r2 = _apply(+1, b + s * n, r1)
+                    if r2 is not None and r2[1] in index:
+                        d1[a, b] += np.conj(cic[index[r2[1]]]) * r2[0] * ck
+                for t in range(2):
+                    for c in range(n):
+                        r2 = _apply(-1, c + t * n, r1)
+                        if r2 is None:
+                            continue
+                        for d in range(n):
+                            r3 = _apply(+1, d + t * n, r2)
+                            if r3 is None:
+                                continue
+                            for b in range(n):
+                                r4 = _apply(+1, b + s * n, r3)
+                                if r4 is not None and r4[1] in index:
+                                    d2[a, b, c, d] += np.conj(cic[index[r4[1]]]) * r4[0] * ck
+    return d1, d2
+
+
+@pytest.mark.parametrize("name", ["cas4e4o", "cas4e6o_m1", "cas5e6o_open", "fc_core_cas2e3o"])
+def test_oracle_rdms_follow_the_reference_phase(name):
+    """Determinant phase: with the strings in the reference's order, the
+    oracle's den1 / den2 of a random complex CI vector equal the RDMs built
+    by plain second quantisation (determinant = ascending creation operators,
+    E_pq phase (-1)^(occupied orbitals between p and q), as ormas_sgn1x has
+    it, src/ormas/ormas_sgn1x.F90:2-20, used by ormas_mkden1_ras_aa/_bb,
+    src/ormas/ormas_mkden1_ras.F90:43-183)."""
+    prob = dict(_cases())[name]
+    sa, sb, dets, _ = _expected(prob)
+    o = Oracle(prob)
+    rng = np.random.default_rng(11)
+    cic = rng.standard_normal(len(dets)) + 1j * rng.standard_normal(len(dets))
+    cic /= np.linalg.norm(cic)
+    d1, d2 = o.ci_mkden(cic)
+    e1, e2 = _rdms_second_quantised(prob.nact, sa, sb, dets, cic)
+    np.testing.assert_allclose(d1, e1, rtol=0, atol=1e-13)
+    np.testing.assert_allclose(d2, e2, rtol=0, atol=1e-13)
