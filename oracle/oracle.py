@@ -72,6 +72,16 @@ def lib():
         L.orc_ci_layout.restype = ctypes.c_int
         L.orc_nstr.argtypes = [vp, ctypes.c_int]
         L.orc_nstr.restype = ctypes.c_int
+        L.orc_band_lu.argtypes = [ctypes.c_int, ctypes.c_int, vp, vp]
+        L.orc_band_lu.restype = ctypes.c_int
+        L.orc_band_lu_solve.argtypes = [ctypes.c_int, ctypes.c_int, vp, vp, vp]
+        L.orc_h1itr_gen.argtypes = [vp, ctypes.c_double]
+        L.orc_h1itr_gen.restype = ctypes.c_int
+        L.orc_h1itr_prop.argtypes = [vp, vp, ctypes.c_double, ctypes.c_int, ctypes.c_double, vp]
+        L.orc_h1itr_prop.restype = ctypes.c_int
+        L.orc_split2_step.argtypes = [vp, ctypes.c_int, ctypes.c_double, vp, vp, vp, ctypes.c_int,
+                                      ctypes.c_double, vp, ctypes.c_int, vp]
+        L.orc_split2_step.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -137,6 +147,35 @@ class Oracle:
         if rc:
             raise ValueError("bad rk_order")
         return w
+
+    def h1itr_gen(self, dt_half: float):
+        """clh1itr::gen: LU factors of 1 + i T dt_half/2 per l (dt_half is the
+        reference's dt2 = dt/2)."""
+        rc = lib().orc_h1itr_gen(self.h, dt_half)
+        if rc:
+            raise RuntimeError(f"h1itr_gen: singular factor at l = {-rc - 1}")
+
+    def h1itr_prop(self, lfield, dt_half: float, wfn: np.ndarray, maxcyc: int = 20, thresh: float = 1e-10):
+        """h1itr_prop on a copy of wfn; returns (wfn, cycles)."""
+        w = np.array(wfn, dtype=np.complex128, copy=True)
+        lf = np.ascontiguousarray(lfield, dtype=np.float64)
+        n = lib().orc_h1itr_prop(self.h, _p(lf), dt_half, maxcyc, thresh, _p(w))
+        return w, n
+
+    def split2_step(self, order: int, dtime: float, lf_quarter, lfield_stages, lf_three_quarter,
+                    wfn: np.ndarray, maxcyc: int = 20, thresh: float = 1e-10, apply_mask: bool = True):
+        """One tdse_split2 step (h1itr half step, RK step of the split hprod,
+        h1itr half step, mask) on a copy of wfn; returns (wfn, (cycles1, cycles2))."""
+        w = np.array(wfn, dtype=np.complex128, copy=True)
+        q1 = np.ascontiguousarray(lf_quarter, dtype=np.float64)
+        q3 = np.ascontiguousarray(lf_three_quarter, dtype=np.float64)
+        st = np.ascontiguousarray(lfield_stages, dtype=np.float64)
+        nc = np.zeros(2, np.int32)
+        rc = lib().orc_split2_step(self.h, order, dtime, _p(q1), _p(st), _p(q3), maxcyc, thresh, _p(w),
+                                   int(apply_mask), _p(nc))
+        if rc:
+            raise ValueError("bad rk_order")
+        return w, (int(nc[0]), int(nc[1]))
 
     def mask(self, wfn: np.ndarray) -> np.ndarray:
         w = np.array(wfn, dtype=np.complex128, copy=True)
@@ -289,6 +328,19 @@ def band_chol(ab: np.ndarray, kd: int) -> int:
 def band_chol_solve(ab: np.ndarray, kd: int, x: np.ndarray) -> None:
     n = ab.size // (kd + 1)
     lib().orc_band_chol_solve(n, kd, _p(ab), _p(x))
+
+
+def band_lu(ab: np.ndarray, kd: int):
+    """In-place zgbtrf (kl = ku = kd) of ab[(3kd+1)*j + row]; returns (info, ipiv)."""
+    n = ab.size // (3 * kd + 1)
+    ipiv = np.zeros(n, np.int32)
+    info = lib().orc_band_lu(n, kd, _p(ab), _p(ipiv))
+    return info, ipiv
+
+
+def band_lu_solve(ab: np.ndarray, kd: int, ipiv: np.ndarray, b: np.ndarray) -> None:
+    n = ab.size // (3 * kd + 1)
+    lib().orc_band_lu_solve(n, kd, _p(ab), _p(ipiv), _p(b))
 
 
 def herm_eig(a: np.ndarray):

@@ -82,6 +82,8 @@ typedef struct orc {
   /* operators (clbas::gen) */
   double *alph;                 /* (lmax1+1)*(2mmax1+1) */
   zc *tmat;                     /* (2n+1)*nr*L1 */
+  zc *cninv;                    /* clh1itr LU factors: (3n+1)*nr*L1, NULL until orc_h1itr_gen */
+  int *cnpiv;
   zc *pmat;                     /* (2n+1)*nr */
   double *d2ll;                 /* (n+1)*nr*L2 */
   zc *zfac, *pzfac1, *pzfac2;   /* zfac,pzfac2: nr*L1*(2mmax1+1); pzfac1: L1*(2mmax1+1) */
@@ -956,7 +958,7 @@ void orc_destroy(orc* o) {
                   o->rho2, o->v2sph, o->v2ang, o->orb0, o->rhofc, o->v2jfc, o->v2xfc, o->den1,
                   o->rden, o->rrden, o->int1e, o->den2, o->int2e, o->xmat, o->fmat, o->bmat,
                   o->dcic, o->str_a, o->str_b, o->det_a, o->det_b, o->det_of, o->ci_work,
-                  o->rk_w0, o->rk_t, o->rk_k, o->mval, o->lval};
+                  o->rk_w0, o->rk_t, o->rk_k, o->mval, o->lval, o->cninv, o->cnpiv};
   for (size_t k = 0; k < sizeof(ptrs) / sizeof(ptrs[0]); ++k) free(ptrs[k]);
   free(o);
 }
@@ -1550,7 +1552,9 @@ void orc_htotx(orc* o, double dtime, const double* lfield, const zc* wfn, const 
 #undef XM
 }
 
-/* hprod_htot_dtorb (src/hprod/hprod_htot_dtorb.F90:2-111), isplit = 0 */
+/* hprod_htot_dtorb (src/hprod/hprod_htot_dtorb.F90:2-111): the Q part is
+ * (v2 + g + h1 + h0) psi for isplit = 0 (:60-67), (v2 + g + h1) psi for
+ * isplit = 2 (:68-75) and (v2 + g) psi for isplit = 1 (:76-83) */
 static void dtorb(orc* o, double dtime, zc* hwfn) {
   int nf = o->nfun, nb = o->nbas;
   zc tfac = o->icomp == 0 ? (zc)(-dtime) : -I * dtime;
@@ -1558,7 +1562,12 @@ static void dtorb(orc* o, double dtime, zc* hwfn) {
     zc *h = hwfn + (size_t)ifun * nb;
     const zc *v = o->v2orb + (size_t)ifun * nb, *g = o->gorb + (size_t)ifun * nb,
              *h1 = o->h1orb + (size_t)ifun * nb, *h0 = o->h0orb + (size_t)ifun * nb;
-    for (int k = 0; k < nb; ++k) h[k] = (v[k] + g[k] + h1[k] + h0[k]) * tfac;
+    if (o->isplit == 0)
+      for (int k = 0; k < nb; ++k) h[k] = (v[k] + g[k] + h1[k] + h0[k]) * tfac;
+    else if (o->isplit == 2)
+      for (int k = 0; k < nb; ++k) h[k] = (v[k] + g[k] + h1[k]) * tfac;
+    else
+      for (int k = 0; k < nb; ++k) h[k] = (v[k] + g[k]) * tfac;
   }
   for (int ifun = o->nfcore; ifun < nf; ++ifun)
     for (int jfun = 0; jfun < nf; ++jfun)
@@ -1758,6 +1767,190 @@ int orc_rk_prop(orc* o, int order, double dtime, const double* lfield_stages, zc
     orc_htot(o, 1.0, lfield_stages + 9 * is, t, t + so, k, k + so);
     for (size_t q = 0; q < sz; ++q) wfn[q] += dtw * k[q];
   }
+  if (apply_mask) orc_mask(o, wfn);
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* clh1itr: the one-body half steps of split2 (src/prop/clh1itr.cpp)    */
+/* ------------------------------------------------------------------ */
+/* zgbtrf for kl = ku = kd (LAPACK band storage, ldab = 3 kd + 1, column j
+ * at ab + ldab*j, A(i,j) at row 2 kd + i - j), restating the unblocked
+ * netlib zgbtf2: partial pivoting by izamax (|re| + |im|), fill-in rows
+ * zeroed, the row interchange then the rank-1 update of the trailing band. */
+static int band_lu(int n, int kd, zc* ab, int* ipiv) {
+  int kl = kd, ku = kd, kv = ku + kl, ld = 3 * kd + 1;
+#define AB(i, j) ab[(size_t)(j) * ld + (i)]
+  for (int j = ku + 1; j < (kv < n ? kv : n); ++j)
+    for (int i = kv - j; i < kl; ++i) AB(i, j) = 0;
+  int ju = 0;
+  for (int j = 0; j < n; ++j) {
+    if (j + kv < n)
+      for (int i = 0; i < kl; ++i) AB(i, j + kv) = 0;
+    int km = kl < n - 1 - j ? kl : n - 1 - j;
+    int jp = 0;
+    double best = -1.0;
+    for (int i = 0; i <= km; ++i) {
+      double a = fabs(creal(AB(kv + i, j))) + fabs(cimag(AB(kv + i, j)));
+      if (a > best) {
+        best = a;
+        jp = i;
+      }
+    }
+    ipiv[j] = jp + j;
+    if (AB(kv + jp, j) == 0) return j + 1;
+    int jmax = j + ku + jp < n - 1 ? j + ku + jp : n - 1;
+    if (jmax > ju) ju = jmax;
+    if (jp != 0)
+      for (int c = 0; c <= ju - j; ++c) {
+        zc t = AB(kv + jp - c, j + c);
+        AB(kv + jp - c, j + c) = AB(kv - c, j + c);
+        AB(kv - c, j + c) = t;
+      }
+    if (km > 0) {
+      zc rp = 1.0 / AB(kv, j);
+      for (int i = 1; i <= km; ++i) AB(kv + i, j) *= rp;
+      for (int c = 1; c <= ju - j; ++c) {
+        zc y = AB(kv - c, j + c);
+        if (y != 0)
+          for (int i = 1; i <= km; ++i) AB(kv - c + i, j + c) -= AB(kv + i, j) * y;
+      }
+    }
+  }
+  return 0;
+}
+
+/* zgbtrs('N') for the factor of band_lu: the row interchanges with the unit
+ * lower triangle (column sweeps), then the banded upper triangle (ztbsv). */
+static void band_lu_solve(int n, int kd, const zc* ab, const int* ipiv, zc* b) {
+  int kl = kd, ku = kd, kv = kl + ku, ld = 3 * kd + 1;
+  for (int j = 0; j < n - 1; ++j) {
+    int lm = kl < n - 1 - j ? kl : n - 1 - j;
+    int l = ipiv[j];
+    if (l != j) {
+      zc t = b[l];
+      b[l] = b[j];
+      b[j] = t;
+    }
+    for (int i = 1; i <= lm; ++i) b[j + i] -= b[j] * AB(kv + i, j);
+  }
+  for (int j = n - 1; j >= 0; --j) {
+    if (b[j] == 0) continue;
+    b[j] /= AB(kv, j);
+    zc t = b[j];
+    int i0 = j - kv > 0 ? j - kv : 0;
+    for (int i = j - 1; i >= i0; --i) b[i] -= t * AB(kv + i - j, j);
+  }
+#undef AB
+}
+
+/* h1itr_gen_cninv (src/prop/h1itr_gen_cninv.F90:2-58): the LU factors of
+ * 1 + fac T_l, fac = i dtime/2 (icomp = 1) or dtime/2 (icomp = 0), T_l the
+ * bound tmat (with v2jfc once hprod_set_wfn0 has run).  clh1itr::gen calls
+ * it with dtime = dt/2 (src/prop/clh1itr.cpp:29-44). */
+int orc_h1itr_gen(orc* o, double dtime) {
+  int n = o->ndvr, nr = o->nr, ld = 3 * n + 1;
+  size_t bw = 2 * (size_t)n + 1;
+  free(o->cninv);
+  free(o->cnpiv);
+  o->cninv = xcalloc((size_t)ld * nr * o->L1, sizeof(zc));
+  o->cnpiv = xcalloc((size_t)nr * o->L1, sizeof(int));
+  zc fac = o->icomp == 1 ? I * dtime * 0.5 : (zc)(dtime * 0.5);
+  for (int l = 0; l <= o->lmax1; ++l) {
+    zc* ab = o->cninv + (size_t)ld * nr * l;
+    for (int i = 0; i < nr; ++i) {
+      ab[(size_t)ld * i + 2 * n] = 1.0;
+      int jll = i - n > 0 ? i - n : 0, jul = i + n < nr - 1 ? i + n : nr - 1;
+      for (int j = jll; j <= jul; ++j)
+        ab[(size_t)ld * i + 2 * n + j - i] += fac * o->tmat[bw * ((size_t)l * nr + i) + n + j - i];
+    }
+    int info = band_lu(nr, n, ab, o->cnpiv + (size_t)nr * l);
+    if (info) return -(l + 1);
+  }
+  return 0;
+}
+
+/* h1itr_cninv (src/prop/h1itr_cninv.F90:2-32): x <- (1 + fac T_l)^-1 x for
+ * every orbital and l >= |m| */
+static void h1itr_cninv(orc* o, zc* x) {
+  int nr = o->nr, ld = 3 * o->ndvr + 1;
+  for (int ifun = 0; ifun < o->nfun; ++ifun)
+    for (int l = abs(o->mval[ifun]); l <= o->lmax1; ++l)
+      band_lu_solve(nr, o->ndvr, o->cninv + (size_t)ld * nr * l, o->cnpiv + (size_t)nr * l,
+                    x + OIDX(0, l, ifun));
+}
+
+/* h1itr_prop (src/prop/h1itr_prop.F90:2-134), iprojfc = 2 (no frozen-core
+ * projection, the default of clcontrol, src/control/clcontrol.cpp:110):
+ * x0 = (1 + i T dt/2)^-1 (1 - i (T [+ V]) dt/2) psi, then for isplit = 1
+ * the Neumann series x_k = (1 + i T dt/2)^-1 (-i V dt/2) x_(k-1) summed while
+ * the RMS orbital norm of x_(k-1) is >= thresh, at most maxcyc terms; the
+ * dynamical orbitals take the sum.  Here dtime is the half step (the
+ * reference's dt2 = dt/2) and V = lfield[2] z (length) or lfield[2] p_z
+ * (velocity).  Returns the number of cycles run. */
+int orc_h1itr_prop(orc* o, const double* lfield, double dtime, int maxcyc, double thresh, zc* wfn) {
+  int nf = o->nfun, nfc = o->nfcore, nb = o->nbas;
+  size_t size1 = (size_t)nb * nf;
+  zc zfield = lfield[2];
+  zc zfac1 = o->icomp == 1 ? -I * dtime * 0.5 : (zc)(-dtime * 0.5);
+  zc zfac2 = zfac1 * zfield;
+  zc* orb = xcalloc(size1, sizeof(zc));
+  zc* horb = xcalloc(size1, sizeof(zc));
+  zc* torb = xcalloc(size1, sizeof(zc));
+  memcpy(orb, wfn, size1 * sizeof(zc));
+  memcpy(torb, wfn, size1 * sizeof(zc));
+  tprod(o, orb, horb, nfc, nf, o->nr);
+  if (o->isplit == 1) {
+    if (o->igauge == 0)
+      zprod(o, zfield, orb, horb, nfc, nf, o->nr);
+    else
+      pzprod(o, zfield, orb, horb, nfc, nf, o->nr);
+  }
+  for (size_t k = (size_t)nb * nfc; k < size1; ++k) orb[k] += zfac1 * horb[k];
+  h1itr_cninv(o, orb);
+  memcpy(torb, orb, size1 * sizeof(zc));
+  int ncyc = 0;
+  if (o->isplit == 1) {
+    for (int icyc = 1; icyc <= maxcyc; ++icyc) {
+      ncyc = icyc;
+      double rms = 0.0;
+      for (int ifun = 0; ifun < nf; ++ifun) {
+        double t = 0.0;
+        const zc* x = orb + (size_t)ifun * nb;
+        for (int k = 0; k < nb; ++k) t += creal(x[k]) * creal(x[k]) + cimag(x[k]) * cimag(x[k]);
+        rms += t;
+      }
+      rms = sqrt(rms / nf);
+      if (rms < thresh) break;
+      memset(horb, 0, size1 * sizeof(zc));
+      if (o->igauge == 0)
+        zprod(o, zfac2, orb, horb, nfc, nf, o->nr);
+      else
+        pzprod(o, zfac2, orb, horb, nfc, nf, o->nr);
+      memcpy(orb, horb, size1 * sizeof(zc));
+      h1itr_cninv(o, orb);
+      for (size_t k = 0; k < size1; ++k) torb[k] += orb[k];
+    }
+  }
+  memcpy(wfn + (size_t)nb * nfc, torb + (size_t)nb * nfc, sizeof(zc) * (size1 - (size_t)nb * nfc));
+  free(orb);
+  free(horb);
+  free(torb);
+  return ncyc;
+}
+
+/* One step of tdse_split2 with clh1itr + clcrk (src/main/tdse.cpp:225-266):
+ * H1P.prop(STEP1) with the field at t + dt/4, the RK step over dt with the
+ * split hprod (stage fields lfield_stages), H1P.prop(STEP2) with the field
+ * at t + 3 dt/4 (clh1itr::prop, src/prop/clh1itr.cpp:61-91), then the mask.
+ * h1itr_gen must have been called with dt/2. */
+int orc_split2_step(orc* o, int order, double dtime, const double* lf_quarter, const double* lfield_stages,
+                    const double* lf_three_quarter, int maxcyc, double thresh, zc* wfn, int apply_mask,
+                    int* ncyc) {
+  ncyc[0] = orc_h1itr_prop(o, lf_quarter, 0.5 * dtime, maxcyc, thresh, wfn);
+  int rc = orc_rk_prop(o, order, dtime, lfield_stages, wfn, 0);
+  if (rc) return rc;
+  ncyc[1] = orc_h1itr_prop(o, lf_three_quarter, 0.5 * dtime, maxcyc, thresh, wfn);
   if (apply_mask) orc_mask(o, wfn);
   return 0;
 }
@@ -2115,6 +2308,8 @@ void orc_energies(const orc* o, double* e) {
 
 /* band Cholesky helpers exposed for pin tests */
 int orc_band_chol(int n, int kd, double* ab) { return band_chol(n, kd, ab); }
+int orc_band_lu(int n, int kd, zc* ab, int* ipiv) { return band_lu(n, kd, ab, ipiv); }
+void orc_band_lu_solve(int n, int kd, const zc* ab, const int* ipiv, zc* b) { band_lu_solve(n, kd, ab, ipiv, b); }
 void orc_band_chol_solve(int n, int kd, const double* ab, double* x) { band_chol_solve(n, kd, ab, x); }
 void orc_herm_eig(int n, const zc* a, double* w, zc* u) { herm_eig(n, a, w, u); }
 void orc_ci_sigma(orc* o, const zc* cic, zc* sig, const zc* int1e, const zc* int2e) {
