@@ -123,6 +123,9 @@ struct td1c_ctx {
   DevBuf<double2> rk_w0, rk_t, rk_k;
   DevBuf<double> lf_dev;  // field values of the stages, [kMaxStages][9]
   DevBuf<PropClock> clock;
+  // device copies of hprod_htot_'s host arguments (wfn | hwfn | cic | hcic),
+  // allocated by its first call and kept with the context
+  DevBuf<double2> legacy_stage;
   bool wfn0_set = false;
   // two side streams (default priority) for the independent branches of an
   // hprod, joined back by events; the events carry no timing.  The streams
@@ -160,7 +163,7 @@ struct td1c_ctx {
     DevBuf<double>* d[] = {&eK0, &eK, &eP, &xr, &band_k, &band_k0, &band_p, &alph, &cost, &mask, &m_l2g, &m_g2l, &m_g2L, &m_L2g,
                            &pG, &pWl, &pWr, &pWlI, &pWrI, &pcrA, &pcrG, &pcrBinv, &pSd, &pSl, &pSu, &invr, &rpl0, &rpl1, &fac2, &fac1, &ene, &lf_dev};
     for (auto* b : d) b->release();
-    DevBuf<double2>* z[] = {&tdiag, &tzor, &lsp, &gsp, &orbg0, &rhoL, &vL, &vg, &p_xb, &p_Q,
+    DevBuf<double2>* z[] = {&tdiag, &tzor, &lsp, &gsp, &orbg0, &rhoL, &vL, &vg, &p_xb, &p_Q, &legacy_stage,
                             &gram_partial, &gram_partial_p, &lprod, &pprod, &den1, &den2, &rden, &rrden, &int1e, &int2e, &xmat,
                             &dcic, &ci_work, &mfc, &dip_part, &ovlp, &rk_w0, &rk_t, &rk_k};
     for (auto* b : z) b->release();
@@ -1782,31 +1785,29 @@ void hprod_htot_(const double* dtime, const double* lfield, const double* wfn, c
     std::fprintf(stderr, "hprod_htot_: no bound td1c context\n");
     std::abort();
   }
-  const size_t so = (size_t)c->nbas * c->S.nfun * sizeof(double2), sc = (size_t)c->S.lcic * sizeof(double2);
-  double *dw = nullptr, *dc = nullptr, *dh = nullptr, *dhc = nullptr;
+  const size_t no = (size_t)c->nbas * c->S.nfun, nc = (size_t)c->S.lcic;
   auto chk = [](hipError_t e, const char* what) {
     if (e != hipSuccess) {
       std::fprintf(stderr, "hprod_htot_: %s: %s\n", what, hipGetErrorString(e));
       std::abort();
     }
   };
-  chk(hipMalloc(&dw, so), "hipMalloc");
-  chk(hipMalloc(&dh, so), "hipMalloc");
-  chk(hipMalloc(&dc, sc), "hipMalloc");
-  chk(hipMalloc(&dhc, sc), "hipMalloc");
-  chk(hipMemcpy(dw, wfn, so, hipMemcpyHostToDevice), "hipMemcpy");
-  chk(hipMemcpy(dc, cic, sc, hipMemcpyHostToDevice), "hipMemcpy");
-  if (td1c_htot(c, *dtime, lfield, dw, dc, dh, dhc, nullptr)) {
+  // the staging copies live as long as the context: one allocation, on the
+  // first call, not one per hprod
+  if (!c->legacy_stage.p) chk(c->legacy_stage.alloc(2 * (no + nc)), "hipMalloc");
+  double2* dw = c->legacy_stage.p;
+  double2* dh = dw + no;
+  double2* dc = dh + no;
+  double2* dhc = dc + nc;
+  chk(hipMemcpy(dw, wfn, no * sizeof(double2), hipMemcpyHostToDevice), "hipMemcpy");
+  chk(hipMemcpy(dc, cic, nc * sizeof(double2), hipMemcpyHostToDevice), "hipMemcpy");
+  if (td1c_htot(c, *dtime, lfield, reinterpret_cast<const double*>(dw), reinterpret_cast<const double*>(dc),
+                reinterpret_cast<double*>(dh), reinterpret_cast<double*>(dhc), nullptr)) {
     std::fprintf(stderr, "hprod_htot_: %s\n", g_error.c_str());
     std::abort();
   }
-  chk(hipDeviceSynchronize(), "hipDeviceSynchronize");
-  chk(hipMemcpy(hwfn, dh, so, hipMemcpyDeviceToHost), "hipMemcpy");
-  chk(hipMemcpy(hcic, dhc, sc, hipMemcpyDeviceToHost), "hipMemcpy");
-  (void)hipFree(dw);
-  (void)hipFree(dh);
-  (void)hipFree(dc);
-  (void)hipFree(dhc);
+  chk(hipMemcpy(hwfn, dh, no * sizeof(double2), hipMemcpyDeviceToHost), "hipMemcpy");
+  chk(hipMemcpy(hcic, dhc, nc * sizeof(double2), hipMemcpyDeviceToHost), "hipMemcpy");
 }
 
 }  // extern "C"
