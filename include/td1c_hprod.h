@@ -344,6 +344,20 @@ void hprod_htot_(const double* dtime, const double* lfield,
                  const double* wfn, const double* cic,
                  double* hwfn, double* hcic);
 
+/* hprod_htot_ for callers that cannot allocate device memory (code that
+ * must not call hipMalloc: tight serving loops, harnesses that forbid it).
+ * No counterpart in the reference.  Same arguments and results as
+ * hprod_htot_ on the bound context; the device copies of the four host
+ * arrays live in one static 64 MiB device array per GPU, held until the
+ * process ends, so the call allocates nothing.  Limit: 2*(nfun*nbas + lcic)
+ * complex values <= 4194304 (a wave function of at most 2097152 complex
+ * values); larger problems return -2.  Calls that overlap on one GPU (other
+ * threads) share the array and race.  Returns 0 or a negative code
+ * (td1c_last_error). */
+int hprod_htot_static(const double* dtime, const double* lfield,
+                      const double* wfn, const double* cic,
+                      double* hwfn, double* hcic);
+
 /* Message of the last error on this thread. */
 const char* td1c_last_error(void);
 

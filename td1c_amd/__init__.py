@@ -108,6 +108,8 @@ def load_library():
     L.td1c_get_timing.restype = ctypes.c_int
     L.td1c_set_timing_kernel.argtypes = [vp, ctypes.c_char_p]
     L.td1c_set_timing_kernel.restype = ctypes.c_int
+    L.hprod_htot_static.argtypes = [vp, vp, vp, vp, vp, vp]
+    L.hprod_htot_static.restype = ctypes.c_int
     L.td1c_last_error.argtypes = []
     L.td1c_last_error.restype = ctypes.c_char_p
     _lib = L

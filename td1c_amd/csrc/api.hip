@@ -1810,4 +1810,60 @@ void hprod_htot_(const double* dtime, const double* lfield, const double* wfn, c
   chk(hipMemcpy(hcic, dhc, nc * sizeof(double2), hipMemcpyDeviceToHost), "hipMemcpy");
 }
 
+// hprod_htot_static: hprod_htot_ without any device allocation.  The device
+// copies of the host arguments live in `scratch`, a file-scope device array
+// of kStaticStageElems complex values that every GPU the library runs on holds
+// until the process ends.  Host code never uses the array's name as an
+// address: static_stage_ptr() looks the device address up once per device
+// with hipGetSymbolAddress (that device current) and keeps it.
+constexpr size_t kStaticStageElems = (size_t)4 << 20;  // 64 MiB of complex128
+static __device__ __align__(256) double2 scratch[kStaticStageElems];
+
+static int static_stage_ptr(double2** out) {
+  static std::mutex mu;
+  static std::map<int, double2*> ptrs;
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = ptrs.find(dev);
+  if (it == ptrs.end()) {
+    void* p = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(-3, "hipGetSymbolAddress returned a null address for the staging array");
+    it = ptrs.emplace(dev, static_cast<double2*>(p)).first;
+  }
+  *out = it->second;
+  return 0;
+}
+
+int hprod_htot_static(const double* dtime, const double* lfield, const double* wfn, const double* cic,
+                      double* hwfn, double* hcic) {
+  td1c_ctx* c = g_bound;
+  if (!c) return fail(-1, "hprod_htot_static: no bound td1c context");
+  if (!dtime || !lfield || !wfn || !hwfn) return fail(-1, "hprod_htot_static: null argument");
+  const size_t no = (size_t)c->nbas * c->S.nfun, nc = (size_t)c->S.lcic;
+  if (nc > 0 && (!cic || !hcic)) return fail(-1, "hprod_htot_static: null CI argument");
+  if (2 * (no + nc) > kStaticStageElems)
+    return fail(-2, "hprod_htot_static: the wave function (" + std::to_string(no + nc) +
+                        " complex values) exceeds the static staging limit of " +
+                        std::to_string(kStaticStageElems / 2) + " (input and output share a 64 MiB device array)");
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev != c->device) return fail(-1, "hprod_htot_static: the bound context belongs to another device");
+  double2* dw = nullptr;
+  int rc = static_stage_ptr(&dw);
+  if (rc) return rc;
+  double2* dh = dw + no;
+  double2* dc = dh + no;
+  double2* dhc = dc + nc;
+  HIP_TRY(hipMemcpy(dw, wfn, no * sizeof(double2), hipMemcpyHostToDevice));
+  if (nc) HIP_TRY(hipMemcpy(dc, cic, nc * sizeof(double2), hipMemcpyHostToDevice));
+  rc = td1c_htot(c, *dtime, lfield, reinterpret_cast<const double*>(dw), reinterpret_cast<const double*>(dc),
+                 reinterpret_cast<double*>(dh), reinterpret_cast<double*>(dhc), nullptr);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpy(hwfn, dh, no * sizeof(double2), hipMemcpyDeviceToHost));
+  if (nc) HIP_TRY(hipMemcpy(hcic, dhc, nc * sizeof(double2), hipMemcpyDeviceToHost));
+  return 0;
+}
+
 }  // extern "C"
