@@ -68,7 +68,10 @@ typedef struct td1c_problem {
   /* control flags (src/control/clcontrol.cpp:67-191) */
   int icomp;                /* 1: real time (tfac = -i dt), 0: imaginary time */
   int igauge;               /* 0: length1, 1: velocity1 */
-  int isplit;               /* split_type: only 0 (full hprod) is provided */
+  int isplit;               /* split_type: 0 full hprod; 1 the hprod's orbital
+                               derivative without h0 + h1 (hprod_htot_dtorb.F90:76-83),
+                               which clh1itr's half steps propagate (td1c_h1itr_*,
+                               td1c_propagate); iprojfc = 2 with frozen cores */
   int ioorot;               /* oorot_type: only 0 is provided */
   int iprojfc;              /* 1 or 2 (default 2) */
   int jfc_implicit;         /* default 1 */
@@ -282,11 +285,36 @@ int td1c_rk_prop(td1c_ctx* ctx, int rk_order, double dtime,
                  const double* lfield_stages, double* d_wfn, int apply_mask,
                  void* stream);
 
-/* nstep Runge-Kutta steps (+ mask) from time t0 with the stage fields
- * evaluated on the device: the time loop of tdse_split2
- * (src/main/tdse.cpp:225-266) without its printing, each step doing what
- * td1c_rk_prop does with clfield::get_value's values (computed on the
- * device from the pulse parameters, no host round trip).  With
+/* clh1itr::gen (src/prop/clh1itr.cpp:29-59; h1itr_gen_cninv,
+ * src/prop/h1itr_gen_cninv.F90:2-58) for a context with split_type = 1:
+ * the factors of the one-body half steps of a step of dtime (half steps of
+ * dtime/2: Crank-Nicolson with 1 + i T dtime/4, T = tmat including the
+ * frozen-core Hartree potential), with the reference's h1itr_maxcyc
+ * (default 20) and h1itr_thresh (default 1e-10).  td1c_propagate builds the
+ * factors itself (with the last maxcyc / thresh given, else the defaults)
+ * when dtime or the frozen core changed. */
+int td1c_h1itr_gen(td1c_ctx* ctx, double dtime, int maxcyc, double thresh);
+
+/* One one-body half step, clh1itr::prop -> h1itr_prop
+ * (src/prop/h1itr_prop.F90:2-134, iprojfc = 2): the dynamical orbitals of
+ * the device wave function are replaced by
+ *   sum_k x_k,  x_0 = (1 + c T)^-1 (1 - c (T + V)) psi,
+ *               x_k = (1 + c T)^-1 (-c V) x_(k-1),
+ * c = i dtime/4, V the dipole coupling of lfield[2] in the context's gauge,
+ * summed while the RMS orbital norm of the last term is >= h1itr_thresh and
+ * for at most h1itr_maxcyc terms after x_0.  lfield is the field the
+ * reference evaluates at t + dt/4 (first half step) or t + 3dt/4 (second).
+ * The first check of the series (on x_0, norm ~1 per orbital) counts the
+ * dynamical orbitals only.  Enqueued on `stream`; td1c_h1itr_gen first. */
+int td1c_h1itr_prop(td1c_ctx* ctx, const double* lfield, double* d_wfn, void* stream);
+
+/* nstep steps (+ mask) from time t0 with the stage fields evaluated on the
+ * device: the time loop of tdse_split2 (src/main/tdse.cpp:225-266) without
+ * its printing.  split_type = 0: each step does what td1c_rk_prop does with
+ * clfield::get_value's values (computed on the device from the pulse
+ * parameters, no host round trip).  split_type = 1: each step is
+ * td1c_h1itr_prop with the field at t + dt/4, td1c_rk_prop of the split
+ * hprod, td1c_h1itr_prop at t + 3dt/4, then the mask.  With
  * td1c_set_graph(ctx, 1) the first call for a given (rk_order, dtime,
  * field, d_wfn, apply_mask) captures one step as a HIP graph and every step
  * replays it.  Enqueued on `stream`, asynchronous. */

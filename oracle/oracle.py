@@ -1,9 +1,12 @@
 """TEST INFRASTRUCTURE ONLY -- Python loader of the FP64 CPU oracle.
 
 Loads oracle/_build/liborc.so (built from oracle/td1c_oracle.c by
-`build()`) and exposes the reference-shaped operations on numpy arrays.
-Only tests/, the smoke entry point and the benchmark's CPU-baseline leg use
-this module; the product package never imports it.
+`build()`, single-threaded: the checker of every test) and exposes the
+reference-shaped operations on numpy arrays.  `build(openmp=True)` makes
+oracle/_build/liborc_omp.so from the same source with OpenMP (threads over
+independent output rows only, so its results are bit-identical), which the
+benchmark's CPU-baseline leg times.  Only tests/, the smoke entry point and
+that leg use this module; the product package never imports it.
 """
 from __future__ import annotations
 
@@ -19,25 +22,31 @@ from td1c_amd.abi import Problem
 HERE = os.path.dirname(os.path.abspath(__file__))
 BUILD_DIR = os.path.join(HERE, "_build")
 LIB_PATH = os.path.join(BUILD_DIR, "liborc.so")
+LIB_OMP_PATH = os.path.join(BUILD_DIR, "liborc_omp.so")
 SRC = os.path.join(HERE, "td1c_oracle.c")
 
 _lib = None
+_lib_omp = None
 
 
 def build(openmp: bool = False) -> str:
-    """Compiles the oracle with gcc (-O2, no fast-math: IEEE FP64)."""
+    """Compiles the oracle with gcc (-O2, no fast-math: IEEE FP64); with
+    openmp the OpenMP variant liborc_omp.so."""
     os.makedirs(BUILD_DIR, exist_ok=True)
-    cmd = ["gcc", "-O2", "-std=gnu11", "-fPIC", "-shared", "-Wall", "-o", LIB_PATH, SRC, "-lm"]
+    out = LIB_OMP_PATH if openmp else LIB_PATH
+    flags = ["-fopenmp"] if openmp else ["-Wno-unknown-pragmas"]
+    cmd = ["gcc", "-O2", "-std=gnu11", "-fPIC", "-shared", "-Wall"] + flags + ["-o", out, SRC, "-lm"]
     subprocess.run(cmd, check=True)
-    return LIB_PATH
+    return out
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH) or os.path.getmtime(LIB_PATH) < os.path.getmtime(SRC):
-            build()
-        L = ctypes.CDLL(LIB_PATH)
+def lib(openmp: bool = False):
+    global _lib, _lib_omp
+    if (_lib_omp if openmp else _lib) is None:
+        path = LIB_OMP_PATH if openmp else LIB_PATH
+        if not os.path.exists(path) or os.path.getmtime(path) < os.path.getmtime(SRC):
+            build(openmp)
+        L = ctypes.CDLL(path)
         vp = ctypes.c_void_p
         L.orc_create.restype = vp
         L.orc_create.argtypes = [vp]
@@ -82,8 +91,11 @@ def lib():
         L.orc_split2_step.argtypes = [vp, ctypes.c_int, ctypes.c_double, vp, vp, vp, ctypes.c_int,
                                       ctypes.c_double, vp, ctypes.c_int, vp]
         L.orc_split2_step.restype = ctypes.c_int
-        _lib = L
-    return _lib
+        if openmp:
+            _lib_omp = L
+        else:
+            _lib = L
+    return _lib_omp if openmp else _lib
 
 
 def _p(a: np.ndarray):
@@ -93,21 +105,22 @@ def _p(a: np.ndarray):
 class Oracle:
     """FP64 restatement of td1c's hprod for one problem."""
 
-    def __init__(self, prob: Problem):
+    def __init__(self, prob: Problem, openmp: bool = False):
         self.prob = prob
+        self._L = lib(openmp)
         c, self._keep = prob.to_c()
         self._c = c
-        self.h = lib().orc_create(ctypes.byref(c))
+        self.h = self._L.orc_create(ctypes.byref(c))
         if not self.h:
             raise RuntimeError("oracle set-up failed")
         s = np.zeros(10, dtype=np.int32)
-        lib().orc_sizes(self.h, _p(s))
+        self._L.orc_sizes(self.h, _p(s))
         (self.nrad, self.nbas, self.nfun, self.lcic, self.nlat, self.L1, self.L2,
          self.nr, self.mmax1, self.mmax2) = [int(v) for v in s]
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib().orc_destroy(self.h)
+            self._L.orc_destroy(self.h)
             self.h = None
 
     @property
@@ -124,7 +137,7 @@ class Oracle:
         if cic.size == 0:
             cic = np.zeros(self.lcic, np.complex128)
         vx = None if v2xfc is None else np.ascontiguousarray(v2xfc, dtype=np.complex128)
-        lib().orc_set_wfn0(self.h, _p(orb), _p(cic), None if vx is None else _p(vx))
+        self._L.orc_set_wfn0(self.h, _p(orb), _p(cic), None if vx is None else _p(vx))
 
     def htot(self, dtime: float, lfield, wfn: np.ndarray) -> np.ndarray:
         """Returns hwfn (orbitals followed by CI) for a full wave function."""
@@ -135,7 +148,7 @@ class Oracle:
         cic = np.ascontiguousarray(wfn[self.norb:])
         hw = out[: self.norb]
         hc = np.zeros(self.lcic, np.complex128)
-        lib().orc_htot(self.h, dtime, _p(lf), _p(orb), _p(cic), _p(hw), _p(hc))
+        self._L.orc_htot(self.h, dtime, _p(lf), _p(orb), _p(cic), _p(hw), _p(hc))
         out[self.norb:] = hc
         return out
 
@@ -143,7 +156,7 @@ class Oracle:
                 apply_mask: bool = True) -> np.ndarray:
         w = np.array(wfn, dtype=np.complex128, copy=True)
         lf = np.ascontiguousarray(lfield_stages, dtype=np.float64)
-        rc = lib().orc_rk_prop(self.h, order, dtime, _p(lf), _p(w), int(apply_mask))
+        rc = self._L.orc_rk_prop(self.h, order, dtime, _p(lf), _p(w), int(apply_mask))
         if rc:
             raise ValueError("bad rk_order")
         return w
@@ -151,7 +164,7 @@ class Oracle:
     def h1itr_gen(self, dt_half: float):
         """clh1itr::gen: LU factors of 1 + i T dt_half/2 per l (dt_half is the
         reference's dt2 = dt/2)."""
-        rc = lib().orc_h1itr_gen(self.h, dt_half)
+        rc = self._L.orc_h1itr_gen(self.h, dt_half)
         if rc:
             raise RuntimeError(f"h1itr_gen: singular factor at l = {-rc - 1}")
 
@@ -159,7 +172,7 @@ class Oracle:
         """h1itr_prop on a copy of wfn; returns (wfn, cycles)."""
         w = np.array(wfn, dtype=np.complex128, copy=True)
         lf = np.ascontiguousarray(lfield, dtype=np.float64)
-        n = lib().orc_h1itr_prop(self.h, _p(lf), dt_half, maxcyc, thresh, _p(w))
+        n = self._L.orc_h1itr_prop(self.h, _p(lf), dt_half, maxcyc, thresh, _p(w))
         return w, n
 
     def split2_step(self, order: int, dtime: float, lf_quarter, lfield_stages, lf_three_quarter,
@@ -171,7 +184,7 @@ class Oracle:
         q3 = np.ascontiguousarray(lf_three_quarter, dtype=np.float64)
         st = np.ascontiguousarray(lfield_stages, dtype=np.float64)
         nc = np.zeros(2, np.int32)
-        rc = lib().orc_split2_step(self.h, order, dtime, _p(q1), _p(st), _p(q3), maxcyc, thresh, _p(w),
+        rc = self._L.orc_split2_step(self.h, order, dtime, _p(q1), _p(st), _p(q3), maxcyc, thresh, _p(w),
                                    int(apply_mask), _p(nc))
         if rc:
             raise ValueError("bad rk_order")
@@ -179,14 +192,14 @@ class Oracle:
 
     def mask(self, wfn: np.ndarray) -> np.ndarray:
         w = np.array(wfn, dtype=np.complex128, copy=True)
-        lib().orc_mask(self.h, _p(w))
+        self._L.orc_mask(self.h, _p(w))
         return w
 
     def dipole(self, lfield, wfn: np.ndarray) -> float:
         """Length-form dipole <z> of a wave function (hprod_dipole dip(1))."""
         w = np.ascontiguousarray(wfn, dtype=np.complex128)
         lf = np.ascontiguousarray(lfield, dtype=np.float64)
-        return float(lib().orc_dipole(self.h, _p(lf), _p(w[: self.norb]), _p(w[self.norb:])))
+        return float(self._L.orc_dipole(self.h, _p(lf), _p(w[: self.norb]), _p(w[self.norb:])))
 
     def dipole3(self, lfield, wfn: np.ndarray) -> np.ndarray:
         """hprod_dipole's Ehrenfest terms [dip(1), vel(1), acc(1)]: length,
@@ -194,7 +207,7 @@ class Oracle:
         w = np.ascontiguousarray(wfn, dtype=np.complex128)
         lf = np.ascontiguousarray(lfield, dtype=np.float64)
         out = np.zeros(3)
-        lib().orc_dipole3(self.h, _p(lf), _p(w[: self.norb]), _p(w[self.norb:]), _p(out))
+        self._L.orc_dipole3(self.h, _p(lf), _p(w[: self.norb]), _p(w[self.norb:]), _p(out))
         return out
 
     def ipx(self, max_ipx: int, rad_ipx: float, lfield, wfn: np.ndarray) -> np.ndarray:
@@ -206,7 +219,7 @@ class Oracle:
         cic = np.ascontiguousarray(w[self.norb:])
         if cic.size == 0:
             cic = np.zeros(max(1, self.lcic), np.complex128)
-        rc = lib().orc_ipx(self.h, max_ipx, rad_ipx, _p(lf), _p(w[: self.norb]), _p(cic), _p(out))
+        rc = self._L.orc_ipx(self.h, max_ipx, rad_ipx, _p(lf), _p(w[: self.norb]), _p(cic), _p(out))
         if rc:
             raise ValueError("bad max_ipx")
         return out
@@ -220,7 +233,7 @@ class Oracle:
         cic = np.ascontiguousarray(w[self.norb:])
         if cic.size == 0:
             cic = np.zeros(max(1, self.lcic), np.complex128)
-        rc = lib().orc_ipd(self.h, max_ipd, rad_ipx, _p(lf), _p(w[: self.norb]), _p(cic), _p(out))
+        rc = self._L.orc_ipd(self.h, max_ipd, rad_ipx, _p(lf), _p(w[: self.norb]), _p(cic), _p(out))
         if rc:
             raise ValueError("bad max_ipd")
         return out
@@ -230,12 +243,12 @@ class Oracle:
         Gram-Schmidt; returns a new wave function."""
         w = np.array(wfn, dtype=np.complex128, copy=True)
         cic = w[self.norb:]
-        lib().orc_orth(self.h, _p(w[: self.norb]), _p(cic) if cic.size else None)
+        self._L.orc_orth(self.h, _p(w[: self.norb]), _p(cic) if cic.size else None)
         return w
 
     def energies(self) -> np.ndarray:
         e = np.zeros(5)
-        lib().orc_energies(self.h, _p(e))
+        self._L.orc_energies(self.h, _p(e))
         return e
 
     _SHAPES = None
@@ -280,7 +293,7 @@ class Oracle:
             "dshape": (nfe * (n + 1) ** 2, np.float64),
         }
         size, dt = shapes[name]
-        ptr = lib().orc_table(self.h, name.encode())
+        ptr = self._L.orc_table(self.h, name.encode())
         if not ptr:
             raise KeyError(name)
         buf = (ctypes.c_char * (size * np.dtype(dt).itemsize)).from_address(ptr)
@@ -294,25 +307,25 @@ class Oracle:
         i2 = np.ascontiguousarray(int2e, np.complex128).ravel(order="F")
         i1 = np.ascontiguousarray(i1)
         i2 = np.ascontiguousarray(i2)
-        lib().orc_ci_sigma(self.h, _p(cic), _p(sig), _p(i1), _p(i2))
+        self._L.orc_ci_sigma(self.h, _p(cic), _p(sig), _p(i1), _p(i2))
         return sig
 
     def ci_layout(self):
         """(alpha strings, beta strings, det_alpha, det_beta, cic0): the
         occupation bit patterns of the ordered strings, the string pair of each
         determinant and the determinant ormas_cic0 sets."""
-        na, nb = lib().orc_nstr(self.h, 0), lib().orc_nstr(self.h, 1)
+        na, nb = self._L.orc_nstr(self.h, 0), self._L.orc_nstr(self.h, 1)
         sa = np.zeros(max(na, 1), np.uint32)
         sb = np.zeros(max(nb, 1), np.uint32)
         da = np.zeros(self.lcic, np.int32)
         db = np.zeros(self.lcic, np.int32)
         c0 = ctypes.c_int(0)
-        lib().orc_ci_layout(self.h, _p(sa), _p(sb), _p(da), _p(db), ctypes.byref(c0))
+        self._L.orc_ci_layout(self.h, _p(sa), _p(sb), _p(da), _p(db), ctypes.byref(c0))
         return sa[:na], sb[:nb], da, db, c0.value
 
     def ci_mkden(self, cic):
         cic = np.ascontiguousarray(cic, np.complex128)
-        lib().orc_ci_mkden(self.h, _p(cic))
+        self._L.orc_ci_mkden(self.h, _p(cic))
         na = self.prob.nact
         d1 = self.table("den1").reshape((na, na), order="F")
         d2 = self.table("den2").reshape((na,) * 4, order="F")

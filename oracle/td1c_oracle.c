@@ -976,6 +976,7 @@ void orc_destroy(orc* o) {
 static void sph2ang1(orc* o, const zc* orb1, zc* orb2, int f0, int f1, int dim) {
   for (int ifun = f0; ifun < f1; ++ifun) {
     int mi = o->mval[ifun];
+#pragma omp parallel for schedule(static)
     for (int ilat = 0; ilat < o->nlat; ++ilat) {
       zc* out = orb2 + GIDX(0, ilat, ifun);
       for (int i = 0; i < dim; ++i) out[i] = 0;
@@ -992,6 +993,7 @@ static void sph2ang1(orc* o, const zc* orb1, zc* orb2, int f0, int f1, int dim) 
 static void ang2sph1(orc* o, const zc* orb1, zc* orb2, int f0, int f1, int dim) {
   for (int ifun = f0; ifun < f1; ++ifun) {
     int mi = o->mval[ifun];
+#pragma omp parallel for schedule(static)
     for (int l = abs(mi); l <= o->lmax1; ++l) {
       zc* out = orb2 + OIDX(0, l, ifun);
       for (int i = 0; i < dim; ++i) out[i] = 0;
@@ -1010,6 +1012,7 @@ static void tprod(orc* o, const zc* wfn, zc* hwfn, int f0, int f1, int dim) {
   int n = o->ndvr, nr = o->nr;
   size_t bw = 2 * (size_t)n + 1;
   for (int ifun = f0; ifun < f1; ++ifun)
+#pragma omp parallel for schedule(static)
     for (int l = abs(o->mval[ifun]); l <= o->lmax1; ++l)
       for (int i = 0; i < dim; ++i) {
         int jll = i - n > 0 ? i - n : 0, jul = i + n < nr - 1 ? i + n : nr - 1;
@@ -1079,6 +1082,7 @@ static void mkrho2(orc* o, int type) {
   pair_range(o, type, &dim, &i0, &i1, &j0, &j1);
   for (int ifun = i0; ifun < i1; ++ifun)
     for (int jfun = (j0 > ifun ? j0 : ifun); jfun < (j1 > ifun + 1 ? j1 : ifun + 1); ++jfun)
+#pragma omp parallel for schedule(static)
       for (int ilat = 0; ilat < o->nlat; ++ilat)
         for (int i = 0; i < dim; ++i)
           o->v2ang[GIDX2(i, ilat, jfun, ifun)] =
@@ -1086,6 +1090,7 @@ static void mkrho2(orc* o, int type) {
   for (int ifun = i0; ifun < i1; ++ifun)
     for (int jfun = (j0 > ifun ? j0 : ifun); jfun < (j1 > ifun + 1 ? j1 : ifun + 1); ++jfun) {
       int mji = -o->mval[jfun] + o->mval[ifun];
+#pragma omp parallel for schedule(static)
       for (int l = 0; l <= o->lmax2; ++l) {
         zc* out = o->rho2 + SIDX2(0, l, jfun, ifun);
         for (int i = 0; i < dim; ++i) out[i] = 0;
@@ -1104,10 +1109,13 @@ static void poisson(orc* o, int type) {
   int dim, i0, i1, j0, j1;
   pair_range(o, type, &dim, &i0, &i1, &j0, &j1);
   int nr = o->nr, n = o->ndvr;
-  double* rr = xcalloc((size_t)dim * 2, sizeof(double));
   for (int l = 0; l <= o->lmax2; ++l) o->d2fac1[l] = 1.0 / pow(o->xrad[dim + 1], 2 * l + 1);
   for (int ifun = i0; ifun < i1; ++ifun)
     for (int jfun = (j0 > ifun ? j0 : ifun); jfun < (j1 > ifun + 1 ? j1 : ifun + 1); ++jfun)
+#pragma omp parallel
+    {
+      double* rr = xcalloc((size_t)dim * 2, sizeof(double));
+#pragma omp for schedule(static)
       for (int l = 0; l <= o->lmax2; ++l) {
         int m = -o->mval[ifun] + o->mval[jfun];
         zc* v = o->v2sph + SIDX2(0, l, jfun, ifun);
@@ -1137,7 +1145,8 @@ static void poisson(orc* o, int type) {
         for (int i = 0; i < dim; ++i)
           v[i] = (rr[i] + tr * rpl1[i]) * f2[i] + (rr[dim + i] + ti * rpl1[i]) * f2[i] * I;
       }
-  free(rr);
+      free(rr);
+    }
 }
 
 /* bas_sph2ang2 of a pair range (src/bas/bas_sph2ang2.F90:334-463) */
@@ -1147,6 +1156,7 @@ static void sph2ang2(orc* o, int type) {
   for (int ifun = i0; ifun < i1; ++ifun)
     for (int jfun = (j0 > ifun ? j0 : ifun); jfun < (j1 > ifun + 1 ? j1 : ifun + 1); ++jfun) {
       int mji = -o->mval[jfun] + o->mval[ifun];
+#pragma omp parallel for schedule(static)
       for (int ilat = 0; ilat < o->nlat; ++ilat) {
         zc* out = o->v2ang + GIDX2(0, ilat, jfun, ifun);
         for (int i = 0; i < dim; ++i) out[i] = 0;
@@ -1197,40 +1207,50 @@ static void mfprod2(orc* o) {
   zc* d2v = xcalloc((size_t)ng * (na > 0 ? na : 1), sizeof(zc));
   /* dcj */
   if (o->ndcore > 0) {
+#pragma omp parallel for schedule(static)
     for (int ig = 0; ig < ng; ++ig) d2v[ig] = 0;
     for (int jfun = nfc; jfun < ncore; ++jfun)
+#pragma omp parallel for schedule(static)
       for (int ig = 0; ig < ng; ++ig) d2v[ig] += V2G(ig, jfun, jfun) * 2.0;
     for (int ifun = nfc; ifun < nf; ++ifun)
+#pragma omp parallel for schedule(static)
       for (int ig = 0; ig < ng; ++ig) o->gorbg[(size_t)ifun * ng + ig] += d2v[ig] * WG(ig, ifun);
     /* dcx */
     for (int ifun = nfc; ifun < nf; ++ifun)
       for (int jfun = nfc; jfun < ncore; ++jfun)
+#pragma omp parallel for schedule(static)
         for (int ig = 0; ig < ng; ++ig)
           o->gorbg[(size_t)ifun * ng + ig] -= V2G(ig, jfun, ifun) * WG(ig, jfun);
   }
   if (ncore > 0 && na > 0) {
     /* actj */
+#pragma omp parallel for schedule(static)
     for (int ig = 0; ig < ng; ++ig) d2v[ig] = 0;
     for (int ja = 0; ja < na; ++ja)
       for (int ka = 0; ka < na; ++ka) {
         zc dens = o->den1[ka + na * ja];
+#pragma omp parallel for schedule(static)
         for (int ig = 0; ig < ng; ++ig) d2v[ig] += V2G(ig, ncore + ja, ncore + ka) * dens;
       }
     for (int ifun = nfc; ifun < ncore; ++ifun)
+#pragma omp parallel for schedule(static)
       for (int ig = 0; ig < ng; ++ig) o->v2orbg[(size_t)ifun * ng + ig] += d2v[ig] * WG(ig, ifun);
     /* actx */
     for (int ja = 0; ja < na; ++ja) {
       int jfun = ncore + ja;
       zc* wd = d2v + (size_t)ja * ng;
+#pragma omp parallel for schedule(static)
       for (int ig = 0; ig < ng; ++ig) wd[ig] = 0;
       for (int ka = 0; ka < na; ++ka) {
         int kfun = ncore + ka;
         if (o->mval[jfun] == o->mval[kfun]) {
           zc dens = -o->den1[ka + na * ja] * 0.5;
+#pragma omp parallel for schedule(static)
           for (int ig = 0; ig < ng; ++ig) wd[ig] += WG(ig, kfun) * dens;
         }
       }
       for (int ifun = nfc; ifun < ncore; ++ifun)
+#pragma omp parallel for schedule(static)
         for (int ig = 0; ig < ng; ++ig) o->v2orbg[(size_t)ifun * ng + ig] += wd[ig] * V2G(ig, jfun, ifun);
     }
   }
@@ -1241,6 +1261,7 @@ static void mfprod2(orc* o) {
       int ifun = ncore + ia;
       for (int ja = 0; ja <= ia; ++ja) {
         int jfun = ncore + ja;
+#pragma omp parallel for schedule(static)
         for (int ig = 0; ig < ng; ++ig) d2v[ig] = 0;
         for (int ka = 0; ka < na; ++ka) {
           int kfun = ncore + ka;
@@ -1249,12 +1270,15 @@ static void mfprod2(orc* o) {
             if (o->mval[ifun] + o->mval[kfun] == o->mval[jfun] + o->mval[lfun]) {
               zc dens = o->den2[ja + na * (ia + na * (la + na * ka))];
               if (cabs(dens) > 1e-15)
+#pragma omp parallel for schedule(static)
                 for (int ig = 0; ig < ng; ++ig) d2v[ig] += dens * V2G(ig, kfun, lfun);
             }
           }
         }
+#pragma omp parallel for schedule(static)
         for (int ig = 0; ig < ng; ++ig) o->torbg[(size_t)ifun * ng + ig] += WG(ig, jfun) * d2v[ig];
         if (ia > ja)
+#pragma omp parallel for schedule(static)
           for (int ig = 0; ig < ng; ++ig) o->torbg[(size_t)jfun * ng + ig] += WG(ig, ifun) * conj(d2v[ig]);
       }
     }
@@ -1264,6 +1288,7 @@ static void mfprod2(orc* o) {
         int kfun = ncore + ka;
         if (o->mval[ifun] == o->mval[kfun]) {
           zc tmp = o->rden[ia + na * ka];
+#pragma omp parallel for schedule(static)
           for (int ig = 0; ig < ng; ++ig) o->v2orbg[(size_t)kfun * ng + ig] += tmp * o->torbg[(size_t)ifun * ng + ig];
         }
       }

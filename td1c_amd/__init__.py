@@ -108,6 +108,10 @@ def load_library():
     L.td1c_get_timing.restype = ctypes.c_int
     L.td1c_set_timing_kernel.argtypes = [vp, ctypes.c_char_p]
     L.td1c_set_timing_kernel.restype = ctypes.c_int
+    L.td1c_h1itr_gen.argtypes = [vp, ctypes.c_double, ctypes.c_int, ctypes.c_double]
+    L.td1c_h1itr_gen.restype = ctypes.c_int
+    L.td1c_h1itr_prop.argtypes = [vp, vp, vp, vp]
+    L.td1c_h1itr_prop.restype = ctypes.c_int
     L.hprod_htot_static.argtypes = [vp, vp, vp, vp, vp, vp]
     L.hprod_htot_static.restype = ctypes.c_int
     L.td1c_last_error.argtypes = []
@@ -256,6 +260,23 @@ class HProd:
                                         ctypes.c_void_p(base + off), ctypes.c_void_p(hbase),
                                         ctypes.c_void_p(hbase + off), self._stream(stream)), "td1c_htot")
         return hwfn
+
+    def h1itr_gen(self, dtime: float, maxcyc: int = 20, thresh: float = 1.0e-10):
+        """clh1itr::gen for split_type = 1: the half-step factors of a step
+        of dtime."""
+        _check(load_library().td1c_h1itr_gen(self.h, float(dtime), int(maxcyc), float(thresh)), "td1c_h1itr_gen")
+
+    def h1itr_prop(self, lfield: Sequence[float], wfn, stream=None):
+        """One clh1itr half step (h1itr_prop) on a device wave function, in place."""
+        torch = self.torch
+        if wfn.dtype != torch.complex128 or not wfn.is_cuda or wfn.numel() != self.wfn_size:
+            raise ValueError("wfn must be a complex128 CUDA tensor of wfn_size elements")
+        lf = np.ascontiguousarray(lfield, dtype=np.float64)
+        if lf.size != 9:
+            raise ValueError("lfield must have 9 entries")
+        _check(load_library().td1c_h1itr_prop(self.h, _ptr(lf), ctypes.c_void_p(wfn.data_ptr()),
+                                              self._stream(stream)), "td1c_h1itr_prop")
+        return wfn
 
     def energies(self) -> np.ndarray:
         """ene_fcore, ene_dcore, ene_core, ene_act, ene_tot of the last htot."""
@@ -449,7 +470,10 @@ class CRK:
     def propagate(self, hprod: HProd, wfn, time: float, dtime: float, nstep: int, field: Field,
                   apply_mask: bool = True, stream=None):
         """nstep steps from `time` with the stage fields evaluated on the
-        device; one captured step replayed per step (tdse_split2's loop)."""
+        device (tdse_split2's loop): every kernel enqueued on the stream, or
+        with hprod.set_graph(True) one captured step replayed per step.  For
+        split_type = 1 a step is the clh1itr half step, the RK step of the
+        split hprod, the second half step, then the mask."""
         cf = field.to_c()
         _check(load_library().td1c_propagate(hprod.h, ctypes.byref(cf), self.rk_order, float(time), float(dtime),
                                              int(nstep), ctypes.c_void_p(wfn.data_ptr()), int(apply_mask),

@@ -8,7 +8,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtd1c_hprod.so")
-SOURCES = ["setup.cpp", "kernels.hip", "island.hip", "api.hip"]
+SOURCES = ["setup.cpp", "kernels.hip", "h1itr.hip", "island.hip", "api.hip"]
 ARCH = os.environ.get("TD1C_OFFLOAD_ARCH", "gfx950")
 
 HIP_FLAGS = [

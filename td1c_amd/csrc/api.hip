@@ -123,6 +123,16 @@ struct td1c_ctx {
   DevBuf<double2> rk_w0, rk_t, rk_k;
   DevBuf<double> lf_dev;  // field values of the stages, [kMaxStages][9]
   DevBuf<PropClock> clock;
+  // split_type = 1 (clh1itr): Crank-Nicolson factors for half steps of
+  // cn_dt / 2, the series terms and the convergence flag
+  double cn_dt = 0.0;
+  int cn_maxcyc = 20;
+  double cn_thresh = 1.0e-10;
+  int cn_K = 0, cn_npart1 = 0;
+  DevBuf<double2> cnG, cnWl, cnWr, cnPa, cnPg, cnBinv, cn_x, cn_y;
+  DevBuf<double> cn_part, lf_cn;
+  DevBuf<int> cn_flag;
+  std::vector<double> v2jfc_h;  // frozen-core Hartree potential (complex, host), empty without frozen core
   // device copies of hprod_htot_'s host arguments (wfn | hwfn | cic | hcic),
   // allocated by its first call and kept with the context
   DevBuf<double2> legacy_stage;
@@ -161,13 +171,15 @@ struct td1c_ctx {
 
   ~td1c_ctx() {
     DevBuf<double>* d[] = {&eK0, &eK, &eP, &xr, &band_k, &band_k0, &band_p, &alph, &cost, &mask, &m_l2g, &m_g2l, &m_g2L, &m_L2g,
-                           &pG, &pWl, &pWr, &pWlI, &pWrI, &pcrA, &pcrG, &pcrBinv, &pSd, &pSl, &pSu, &invr, &rpl0, &rpl1, &fac2, &fac1, &ene, &lf_dev};
+                           &pG, &pWl, &pWr, &pWlI, &pWrI, &pcrA, &pcrG, &pcrBinv, &pSd, &pSl, &pSu, &invr, &rpl0, &rpl1, &fac2, &fac1, &ene, &lf_dev,
+                       &cn_part, &lf_cn};
     for (auto* b : d) b->release();
     DevBuf<double2>* z[] = {&tdiag, &tzor, &lsp, &gsp, &orbg0, &rhoL, &vL, &vg, &p_xb, &p_Q, &legacy_stage,
+                            &cnG, &cnWl, &cnWr, &cnPa, &cnPg, &cnBinv, &cn_x, &cn_y,
                             &gram_partial, &gram_partial_p, &lprod, &pprod, &den1, &den2, &rden, &rrden, &int1e, &int2e, &xmat,
                             &dcic, &ci_work, &mfc, &dip_part, &ovlp, &rk_w0, &rk_t, &rk_k};
     for (auto* b : z) b->release();
-    DevBuf<int>* i[] = {&mval, &lval, &pair_m, &pj, &pi, &ptype, &mapH, &mapG, &mapV, &mapF, &map2e,
+    DevBuf<int>* i[] = {&mval, &lval, &pair_m, &pj, &pi, &ptype, &mapH, &mapG, &mapV, &mapF, &map2e, &cn_flag,
                         &det_of, &exA, &sgA, &exB, &sgB};
     for (auto* b : i) b->release();
     DevBuf<TransformBatch>* t[] = {&b_l2g_dyn, &b_g2l_fc, &b_rho, &b_L2g, &b_g2l_mf};
@@ -789,6 +801,7 @@ DtorbArgs dtorb_args(td1c_ctx* c, double dtime) {
   d.xlocal = c->x_local ? 1 : 0;
   d.ncore = S.ncore;
   d.ladapt = S.lconst;
+  d.isplit = S.isplit;
   d.lval = c->lval.p;
   d.xfac = S.icomp == 0 ? make_double2(dtime, 0.0) : make_double2(0.0, dtime);
   d.lprod = c->lprod.p;
@@ -1117,6 +1130,13 @@ int td1c_set_wfn0(td1c_ctx* c, const double* wfn, const double* cic, const doubl
     HIP_TRY(hipMemcpy(og.data(), c->orbg0.p, og.size() * sizeof(double2), hipMemcpyDeviceToHost));
     c->ene_fcore = host_fcore_energy(S, orbfc, og, v2jfc, v2xfc);
   }
+  // the Crank-Nicolson factors of split_type = 1 hold tmat: rebuilt with v2jfc on next use
+  c->v2jfc_h.assign((size_t)2 * nr, 0.0);
+  for (int i = 0; i < nr; ++i) {
+    c->v2jfc_h[2 * i] = v2jfc[i].x;
+    c->v2jfc_h[2 * i + 1] = v2jfc[i].y;
+  }
+  c->cn_dt = 0.0;
   // tmat += v2jfc (hprod_addfc_tmat)
   std::vector<double2> td((size_t)L1 * nr), tz((size_t)L1 * nr);
   for (int l = 0; l < L1; ++l)
@@ -1529,6 +1549,90 @@ static int rk_tableau(int order, int& ns, double* nodes, double* w, double (*a)[
 // join: make s wait for the island stage B at the end (always, unless the
 // caller enqueues further steps and joins after the last one: the CI data
 // of consecutive steps stays on side stream 0, the orbitals on s)
+// split_type = 1: the Crank-Nicolson factors of clh1itr::gen for a step of
+// dtime (half steps of dtime / 2, h1itr_gen_cninv's factor (i) dtime / 4),
+// built on the host and uploaded when dtime or the frozen core changed.
+static int cn_prepare(td1c_ctx* c, double dtime) {
+  Setup& S = c->S;
+  if (S.isplit != 1) return fail(-1, "the one-body half steps need split_type = 1");
+  if (c->cn_dt == dtime && c->cnG.p) return 0;
+  if (S.nfcore > 0 && !c->wfn0_set) return fail(-4, "frozen core needs td1c_set_wfn0 first");
+  if (!(dtime != 0.0) || !std::isfinite(dtime)) return fail(-1, "the split step needs a finite nonzero dtime");
+  CnTables T;
+  std::string err;
+  const double q = 0.25 * dtime;
+  if (!build_cn(S, c->v2jfc_h.empty() ? nullptr : c->v2jfc_h.data(), S.icomp == 1 ? 0.0 : q, S.icomp == 1 ? q : 0.0,
+                T, err))
+    return fail(-5, err);
+  if (c->cnG.p) HIP_TRY(hipDeviceSynchronize());  // earlier half steps may still read the old factors
+  if (c->gexec) {  // a captured step holds the old factors' addresses
+    HIP_TRY(hipGraphExecDestroy(c->gexec));
+    c->gexec = nullptr;
+  }
+  auto up = [](DevBuf<double2>& b, const std::vector<double>& v) -> hipError_t {
+    b.release();
+    hipError_t e = b.alloc(std::max<size_t>(v.size() / 2, 1));
+    if (e != hipSuccess || v.empty()) return e;
+    return hipMemcpy(b.p, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice);
+  };
+  HIP_TRY(up(c->cnG, T.G));
+  HIP_TRY(up(c->cnWl, T.Wl));
+  HIP_TRY(up(c->cnWr, T.Wr));
+  HIP_TRY(up(c->cnPa, T.pa));
+  HIP_TRY(up(c->cnPg, T.pg));
+  HIP_TRY(up(c->cnBinv, T.binv));
+  c->cn_K = T.K;
+  const int nd = S.nfun - S.nfcore;
+  const size_t nrow = (size_t)std::max(nd, 1) * S.L1 * S.nr;
+  if (!c->cn_x.p) HIP_TRY(c->cn_x.alloc(nrow));
+  if (!c->cn_y.p) HIP_TRY(c->cn_y.alloc(nrow));
+  if (!c->cn_flag.p) {
+    HIP_TRY(c->cn_flag.alloc(2));
+    HIP_TRY(hipMemset(c->cn_flag.p, 0, 2 * sizeof(int)));
+  }
+  if (!c->lf_cn.p) HIP_TRY(c->lf_cn.alloc(9));
+  c->cn_npart1 = std::max(nd, 1) * S.L1;
+  CnArgs probe{};
+  probe.nfe = S.nfe; probe.ni = S.n - 1; probe.L1 = S.L1; probe.nfun = S.nfun; probe.nfcore = S.nfcore;
+  const size_t npart = (size_t)c->cn_npart1 + cn_interior_blocks(probe);
+  if (c->cn_part.n < npart) {
+    c->cn_part.release();
+    HIP_TRY(c->cn_part.alloc(npart));
+  }
+  c->cn_dt = dtime;
+  return 0;
+}
+
+static CnArgs cn_args(td1c_ctx* c, const double* d_lf, double2* wfn) {
+  Setup& S = c->S;
+  CnArgs a{};
+  a.nr = S.nr; a.n = S.n; a.nfe = S.nfe; a.ni = S.n - 1; a.nb = S.nfe - 1; a.K = c->cn_K; a.L1 = S.L1;
+  a.nfun = S.nfun; a.nfcore = S.nfcore; a.mmax1 = S.mmax1; a.igauge = S.igauge; a.isplit = S.isplit;
+  a.mval = c->mval.p;
+  a.band_k = c->band_k.p; a.band_k0 = c->band_k0.p; a.band_p = c->band_p.p; a.tdiag = c->tdiag.p;
+  a.xr = c->xr.p; a.alph = c->alph.p; a.lf = d_lf;
+  a.G = c->cnG.p; a.Wl = c->cnWl.p; a.Wr = c->cnWr.p; a.pa = c->cnPa.p; a.pg = c->cnPg.p; a.binv = c->cnBinv.p;
+  a.wfn = wfn; a.x = c->cn_x.p; a.y = c->cn_y.p;
+  const double q = 0.25 * c->cn_dt;
+  a.zfac1 = S.icomp == 1 ? make_double2(0.0, -q) : make_double2(-q, 0.0);
+  a.thresh = c->cn_thresh;
+  a.flag = c->cn_flag.p;
+  a.part = c->cn_part.p;
+  a.npart1 = c->cn_npart1;
+  return a;
+}
+
+// One clh1itr half step (h1itr_prop) on the orbitals of wfn with the device
+// field d_lf: the first term, then cn_maxcyc more terms that return at once
+// after convergence.
+static int enqueue_h1itr(td1c_ctx* c, const double* d_lf, double2* wfn, hipStream_t s) {
+  const CnArgs a = cn_args(c, d_lf, wfn);
+  TIMED("h1itr");
+  launch_cn_stage(a, 0, s);
+  for (int k = 0; k < c->cn_maxcyc; ++k) launch_cn_stage(a, 1, s);
+  return 0;
+}
+
 static int enqueue_rk_step(td1c_ctx* c, int rk_order, double dtime, const double* d_lf, double2* wfn, int apply_mask,
                            hipStream_t s, bool join = true) {
   int ns;
@@ -1602,10 +1706,37 @@ int td1c_rk_prop(td1c_ctx* c, int rk_order, double dtime, const double* lfield_s
 // when it changed by less than tolene, else takes one rk_order step of dstep
 // (clcrk::prop, no mask) and orthonormalises (clwfn::orth, schmidt).  *ene
 // receives the last energy, *ncyc the last cycle index.
+int td1c_h1itr_gen(td1c_ctx* c, double dtime, int maxcyc, double thresh) {
+  if (!c) return fail(-1, "null argument");
+  if (c->S.isplit != 1) return fail(-1, "td1c_h1itr_gen needs a context with split_type = 1");
+  if (maxcyc < 0 || !(thresh >= 0.0)) return fail(-1, "h1itr_maxcyc must be >= 0 and h1itr_thresh >= 0");
+  c->cn_maxcyc = maxcyc;
+  c->cn_thresh = thresh;
+  if (c->gexec) {  // a captured step holds the old cycle count
+    HIP_TRY(hipGraphExecDestroy(c->gexec));
+    c->gexec = nullptr;
+  }
+  c->cn_dt = 0.0;
+  return cn_prepare(c, dtime);
+}
+
+int td1c_h1itr_prop(td1c_ctx* c, const double* lfield, double* d_wfn, void* stream) {
+  if (!c || !lfield || !d_wfn) return fail(-1, "null argument");
+  if (c->S.isplit != 1) return fail(-1, "td1c_h1itr_prop needs a context with split_type = 1");
+  if (!c->cnG.p || c->cn_dt == 0.0) return fail(-4, "td1c_h1itr_gen first");
+  hipStream_t s = (hipStream_t)stream;
+  HIP_TRY(hipMemcpyAsync(c->lf_cn.p, lfield, 9 * sizeof(double), hipMemcpyHostToDevice, s));
+  int rc = enqueue_h1itr(c, c->lf_cn.p, reinterpret_cast<double2*>(d_wfn), s);
+  if (rc) return rc;
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
 int td1c_init_split2(td1c_ctx* c, int rk_order, double dstep, double tolene, int maxcyc, const double* lfield,
                      double* d_wfn, double* ene, int* ncyc, void* stream) {
   if (!c || !lfield || !d_wfn || !ene || !ncyc) return fail(-1, "null argument");
   if (c->S.icomp != 0) return fail(-1, "imaginary-time relaxation needs a context with icomp = 0");
+  if (c->S.isplit != 0) return fail(-1, "td1c_init_split2 runs split_type = 0 only");
   if (maxcyc < 0 || !(dstep > 0.0)) return fail(-1, "maxcyc must be >= 0 and dstep > 0");
   int ns;
   double nodes[kMaxStages], w[kMaxStages], a[kMaxStages][kMaxStages];
@@ -1646,16 +1777,46 @@ int td1c_propagate(td1c_ctx* c, const td1c_field* field, int rk_order, double t0
   if (nstep == 0) return 0;
   hipStream_t s = (hipStream_t)stream;
   const Pulse p = make_pulse(field);
+  // split_type = 1: tdse_split2's step is clh1itr::prop(STEP1) with the field
+  // at t + dt/4, the RK step of the split hprod, clh1itr::prop(STEP2) at
+  // t + 3dt/4, then the mask (src/main/tdse.cpp:225-266,
+  // src/prop/clh1itr.cpp:61-91); the two half-step fields ride along as
+  // extra stage nodes
+  const bool split = c->S.isplit == 1;
+  if (split) {
+    int rc = cn_prepare(c, dtime);
+    if (rc) return rc;
+  }
+  StageNodes sn{};
+  sn.ns = ns;
+  for (int i = 0; i < ns; ++i) sn.c[i] = nodes[i];
+  if (split) {
+    if (ns + 2 > kMaxStages) return fail(-1, "too many stages for the split step");
+    sn.c[ns] = 0.25;
+    sn.c[ns + 1] = 0.75;
+    sn.ns = ns + 2;
+  }
+  double2* wfn = reinterpret_cast<double2*>(d_wfn);
+  auto enqueue_step = [&](hipStream_t st, bool join) -> int {
+    if (split) {
+      int rc = enqueue_h1itr(c, c->lf_dev.p + 9 * ns, wfn, st);
+      if (rc) return rc;
+    }
+    int rc = enqueue_rk_step(c, rk_order, dtime, c->lf_dev.p, wfn, split ? 0 : apply_mask, st, join);
+    if (rc) return rc;
+    if (split) {
+      rc = enqueue_h1itr(c, c->lf_dev.p + 9 * (ns + 1), wfn, st);
+      if (rc) return rc;
+      if (apply_mask) launch_mask(wfn, c->mask.p, c->S.nr, c->S.L1, c->S.nfcore, c->S.nfun, st);
+    }
+    return 0;
+  };
   if (!c->use_graph) {
     // stream launches: the hprod's side-stream branches then run concurrently
-    StageNodes sn{};
-    sn.ns = ns;
-    for (int i = 0; i < ns; ++i) sn.c[i] = nodes[i];
     launch_clock_set(c->clock.p, t0, dtime, s);
     for (int i = 0; i < nstep; ++i) {
       launch_field_stages(p, c->clock.p, sn, c->lf_dev.p, s);
-      int rc = enqueue_rk_step(c, rk_order, dtime, c->lf_dev.p, reinterpret_cast<double2*>(d_wfn), apply_mask, s,
-                               i == nstep - 1);
+      int rc = enqueue_step(s, i == nstep - 1);
       if (rc) return rc;
     }
     HIP_TRY(hipGetLastError());
@@ -1669,14 +1830,10 @@ int td1c_propagate(td1c_ctx* c, const td1c_field* field, int rk_order, double t0
       c->gexec = nullptr;
     }
     if (!c->cap_stream) HIP_TRY(hipStreamCreateWithFlags(&c->cap_stream, hipStreamNonBlocking));
-    StageNodes sn{};
-    sn.ns = ns;
-    for (int i = 0; i < ns; ++i) sn.c[i] = nodes[i];
     HIP_TRY(hipStreamBeginCapture(c->cap_stream, hipStreamCaptureModeThreadLocal));
     c->capturing = true;
     launch_field_stages(p, c->clock.p, sn, c->lf_dev.p, c->cap_stream);
-    int rc = enqueue_rk_step(c, rk_order, dtime, c->lf_dev.p, reinterpret_cast<double2*>(d_wfn), apply_mask,
-                             c->cap_stream);
+    int rc = enqueue_step(c->cap_stream, true);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(c->cap_stream, &g);
     c->capturing = false;

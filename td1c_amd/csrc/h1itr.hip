@@ -1,0 +1,326 @@
+// gfx950 kernels of the split-operator one-body half step of td1c's split2
+// propagation with split_type = 1 (clh1itr, src/prop/clh1itr.cpp; h1itr_prop,
+// src/prop/h1itr_prop.F90:2-134):
+//   x0  = (1 + c T)^-1 (1 - c (T + V)) psi            (c = i dt/4, or dt/4)
+//   x_k = (1 + c T)^-1 (-c V) x_(k-1)                 while rms(x_(k-1)) >= thresh
+//   psi = sum_k x_k                                   (dynamical orbitals)
+// T is tmat (kinetic band + nuclear + centrifugal + frozen-core Hartree), V
+// the length- or velocity-gauge coupling of l and l +- 1.  The reference
+// solves with zgbtrf/zgbtrs on the whole band per l (h1itr_cninv); here the
+// same operator is solved by element Schur complement: the interior blocks
+// of each FEDVR element are eliminated with their precomputed inverses and
+// the bridge points form a tridiagonal system solved by cyclic reduction.
+// One term of the series is three launches: the right-hand side, the bridge
+// pass (one block per orbital row), the interior pass (one block per (l,
+// element chunk, orbital group), each element inverse read once for the
+// group).  The interior pass's last block tests convergence; every later
+// launch of the half step returns at once after it.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "devmath.hpp"
+#include "kernels.hpp"
+
+namespace td1c {
+
+// Right-hand side, thread per (r, l, dynamical orbital):
+// mode 0: y = psi - c (T psi + V psi)   (V with isplit = 1 only;
+//         hprod_tprod_dyn + hprod_zprod_dyn / hprod_pzprod_dyn, then the
+//         zaxpy of h1itr_prop.F90:57-70)
+// mode 1: y = -c V x                    (h1itr_prop.F90:106-114)
+__global__ void __launch_bounds__(256) k_cn_rhs(const CnArgs a, int mode) {
+  if (mode == 1 && a.flag[0]) return;
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int l = blockIdx.y;
+  const int fd = blockIdx.z;
+  const int f = a.nfcore + fd;
+  if (r >= a.nr) return;
+  const int m = a.mval[f];
+  const int am = m < 0 ? -m : m;
+  double2* out = a.y + ((size_t)fd * a.L1 + l) * a.nr + r;
+  if (l < am) {
+    *out = make_double2(0.0, 0.0);
+    return;
+  }
+  const double2* src = mode == 0 ? a.wfn + (size_t)f * a.L1 * a.nr : a.x + (size_t)fd * a.L1 * a.nr;
+  const double2* psi_l = src + (size_t)l * a.nr;
+  const int n = a.n;
+  const int jlo = max(0, r - n), jhi = min(a.nr - 1, r + n);
+  const double xr = a.xr[r];
+  const double* al = a.alph + (size_t)(m + a.mmax1) * a.L1;
+  double2 h = make_double2(0.0, 0.0);
+  if (mode == 0) {
+    const double* band = l == 0 ? a.band_k : a.band_k0;
+    for (int j = jlo; j <= jhi; ++j) {
+      const int d = j - r + n;
+      double2 t;
+      if (d == n) {
+        t = a.tdiag[(size_t)l * a.nr + r];
+      } else {
+        t = make_double2(band[(size_t)d * a.nr + r], 0.0);
+        if (t.x == 0.0) continue;
+      }
+      cfma(h, psi_l[j], t);
+    }
+  }
+  if (mode == 1 || a.isplit == 1) {
+    // the field value: lfield(3) in mode 0, -c lfield(3) in mode 1
+    const double2 zf = mode == 0 ? make_double2(a.lf[2], 0.0) : cmul(a.zfac1, make_double2(a.lf[2], 0.0));
+    if (a.igauge == 0) {
+      if (l + 1 < a.L1) cfma(h, psi_l[a.nr + r], cscale(zf, xr * al[l]));
+      if (l - 1 >= am) cfma(h, psi_l[-(ptrdiff_t)a.nr + r], cscale(zf, xr * al[l - 1]));
+    } else {
+      if (l + 1 < a.L1) {
+        const double2* p1 = psi_l + a.nr;
+        double2 dw = make_double2(0.0, 0.0);
+        for (int j = jlo; j <= jhi; ++j) {
+          const double pm = a.band_p[(size_t)(j - r + n) * a.nr + r];
+          if (pm == 0.0) continue;
+          dw.x = fma(p1[j].x, pm, dw.x);
+          dw.y = fma(p1[j].y, pm, dw.y);
+        }
+        cfma(h, dw, cmul(zf, make_double2(0.0, -al[l])));
+        cfma(h, p1[r], cmul(zf, make_double2(0.0, -(al[l] * (l + 1.0)) / xr)));
+      }
+      if (l - 1 >= am) {
+        const double2* p0 = psi_l - a.nr;
+        double2 dw = make_double2(0.0, 0.0);
+        for (int j = jlo; j <= jhi; ++j) {
+          const double pm = a.band_p[(size_t)(j - r + n) * a.nr + r];
+          if (pm == 0.0) continue;
+          dw.x = fma(p0[j].x, pm, dw.x);
+          dw.y = fma(p0[j].y, pm, dw.y);
+        }
+        cfma(h, dw, cmul(zf, make_double2(0.0, -al[l - 1])));
+        cfma(h, p0[r], cscale(cmul(zf, make_double2(0.0, -(al[l - 1] * (double)l) / xr)), -1.0));
+      }
+    }
+  }
+  if (mode == 0) {
+    double2 y = psi_l[r];
+    cfma(y, a.zfac1, h);
+    *out = y;
+  } else {
+    *out = h;
+  }
+}
+
+// Bridge pass, one block per (dynamical orbital, l): the Schur right-hand
+// sides y_B - W_r(k) . y_I(k) - W_l(k+1) . y_I(k+1) (four lanes per bridge),
+// cyclic reduction of the bridge system in LDS, the bridge values into x and
+// into the state, and the block's share of sum |x|^2.
+template <int BS>
+__global__ void __launch_bounds__(BS) k_cn_bridge(const CnArgs a, int mode) {
+  if (mode == 1 && a.flag[0]) return;
+  const int tid = threadIdx.x;
+  const int job = blockIdx.x;  // fd * L1 + l
+  const int fd = job / a.L1, l = job - fd * a.L1;
+  const int f = a.nfcore + fd;
+  const int nb = a.nb, n = a.n, ni = a.ni, K = a.K, nr = a.nr;
+  __shared__ double2 sS[kMaxBridge];
+  __shared__ double2 sY[kMaxBridge];
+  __shared__ double sRed[BS / 64];
+  const int m = a.mval[f];
+  const int am = m < 0 ? -m : m;
+  double nrm = 0.0;
+  if (l >= am) {  // uniform over the block
+    const size_t row = ((size_t)fd * a.L1 + l) * nr;
+    const double2* y = a.y + row;
+    const double2* wl = a.Wl + (size_t)l * nr;
+    const double2* wr = a.Wr + (size_t)l * nr;
+    // a quad of lanes per bridge; 4 nb and the stride are multiples of 4, so
+    // a quad stays together in the loop
+    for (int t4 = tid; t4 < 4 * nb; t4 += BS) {
+      const int kb = t4 >> 2, sub = t4 & 3;
+      const int bl = kb * n, br = (kb + 1) * n;  // first interior rows of elements kb and kb+1
+      double2 cl = make_double2(0.0, 0.0), cr = make_double2(0.0, 0.0);
+      for (int i = sub; i < ni; i += 4) {
+        cfma(cr, wr[bl + i], y[bl + i]);
+        cfma(cl, wl[br + i], y[br + i]);
+      }
+      for (int o = 1; o < 4; o <<= 1) {
+        cl.x += __shfl_xor(cl.x, o, 64);
+        cl.y += __shfl_xor(cl.y, o, 64);
+        cr.x += __shfl_xor(cr.x, o, 64);
+        cr.y += __shfl_xor(cr.y, o, 64);
+      }
+      if (sub == 0) sY[kb] = csub(csub(y[br - 1], cl), cr);
+    }
+    __syncthreads();
+    double2* cur = sY;
+    double2* nxt = sS;
+    for (int k = 0, h = 1; k < K; ++k, h <<= 1) {
+      if (tid < nb) {
+        const int i = tid;
+        double2 d = cur[i];
+        if (i - h >= 0) cfma(d, a.pa[((size_t)l * K + k) * nb + i], cur[i - h]);
+        if (i + h < nb) cfma(d, a.pg[((size_t)l * K + k) * nb + i], cur[i + h]);
+        nxt[i] = d;
+      }
+      __syncthreads();
+      double2* t = cur;
+      cur = nxt;
+      nxt = t;
+    }
+    if (tid < nb) {
+      const int gi = (tid + 1) * n - 1;
+      const double2 x = cmul(cur[tid], a.binv[(size_t)l * nb + tid]);
+      a.x[row + gi] = x;
+      double2* w = a.wfn + ((size_t)f * a.L1 + l) * nr + gi;
+      *w = mode == 0 ? x : cadd(*w, x);
+      nrm = x.x * x.x + x.y * x.y;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) nrm += __shfl_xor(nrm, o, 64);
+  if ((tid & 63) == 0) sRed[tid >> 6] = nrm;
+  __syncthreads();
+  if (tid == 0) {
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < BS / 64; ++w) s += sRed[w];
+    a.part[job] = s;
+  }
+}
+
+// Interior pass, one block per (l, chunk of epb elements, group of kCnGroup
+// dynamical orbitals), thread per interior row: x_I = G_e y_I - W_l
+// x_left - W_r x_right with the row of G_e in registers for the whole group,
+// into x and the state.  The last block to finish sums the partial norms of
+// both passes in a fixed order and sets flag[0] when the RMS orbital norm of
+// this term is below thresh (the test at the top of the next cycle of
+// h1itr_prop.F90:75-99).
+template <int NI>
+__global__ void __launch_bounds__(kCnInteriorThreads) k_cn_interior(const CnArgs a, int mode, int epb, int nchunk,
+                                                                    int ngrp) {
+  if (mode == 1 && a.flag[0]) return;
+  constexpr int PG = kCnGroup;
+  constexpr int BS = kCnInteriorThreads;
+  const int ni = NI > 0 ? NI : a.ni;
+  const int nb = a.nb, nr = a.nr, n = a.n, L1 = a.L1;
+  const int nd = a.nfun - a.nfcore;
+  const int job = xcd_remap(blockIdx.x, gridDim.x);
+  const int l = job / (nchunk * ngrp), ch = (job / ngrp) % nchunk, g = job % ngrp;
+  const int fd0 = g * PG, e0 = ch * epb;
+  const int t = threadIdx.x;
+  const int el = t / ni, i = t - el * ni;
+  const int e = e0 + el;
+  const bool row_ok = el < epb && e < a.nfe;
+  const int r = e * n + i;
+  __shared__ double2 sb[PG][BS];
+  __shared__ double2 sx[PG][34];  // bridges e0-1 .. e0+epb-1 (epb <= 32)
+  __shared__ int sval[PG];
+  __shared__ double sRed[BS / 64];
+  __shared__ int slast;
+  if (t < PG) {
+    const int fd = fd0 + t;
+    int ok = 0;
+    if (fd < nd) {
+      const int m = a.mval[a.nfcore + fd];
+      ok = l >= (m < 0 ? -m : m);
+    }
+    sval[t] = ok;
+  }
+#pragma unroll
+  for (int j = 0; j < PG; ++j) {
+    const int fd = fd0 + j;
+    if (row_ok && fd < nd) sb[j][t] = a.y[((size_t)fd * L1 + l) * nr + r];
+  }
+  for (int q = t; q < PG * (epb + 1); q += BS) {
+    const int j = q / (epb + 1), k = q - j * (epb + 1);
+    const int fd = fd0 + j, bidx = e0 - 1 + k;
+    sx[j][k] = (fd < nd && bidx >= 0 && bidx < nb) ? a.x[((size_t)fd * L1 + l) * nr + (size_t)(bidx + 1) * n - 1]
+                                                   : make_double2(0.0, 0.0);
+  }
+  double2 gk[NI > 0 ? NI : 1];
+  double2 wl = make_double2(0.0, 0.0), wr = make_double2(0.0, 0.0);
+  const double2* ge = a.G + (((size_t)l * a.nfe) + (row_ok ? e : 0)) * ni * ni + i;
+  if (row_ok) {
+    if (NI > 0) {
+#pragma unroll
+      for (int k = 0; k < NI; ++k) gk[k] = ge[k * NI];
+    }
+    wl = a.Wl[(size_t)l * nr + r];
+    wr = a.Wr[(size_t)l * nr + r];
+  }
+  __syncthreads();
+  double nrm = 0.0;
+  if (row_ok) {
+    const int eb = el * ni;
+#pragma unroll 1
+    for (int j = 0; j < PG; ++j) {
+      const int fd = fd0 + j;
+      if (fd >= nd) break;
+      if (!sval[j]) continue;
+      double2 x = make_double2(0.0, 0.0);
+      if (NI > 0) {
+#pragma unroll
+        for (int k = 0; k < NI; ++k) cfma(x, gk[k], sb[j][eb + k]);
+      } else {
+        for (int k = 0; k < ni; ++k) cfma(x, ge[k * ni], sb[j][eb + k]);
+      }
+      if (e >= 1) x = csub(x, cmul(wl, sx[j][el]));
+      if (e <= a.nfe - 2) x = csub(x, cmul(wr, sx[j][el + 1]));
+      a.x[((size_t)fd * L1 + l) * nr + r] = x;
+      double2* w = a.wfn + ((size_t)(a.nfcore + fd) * L1 + l) * nr + r;
+      *w = mode == 0 ? x : cadd(*w, x);
+      nrm += x.x * x.x + x.y * x.y;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) nrm += __shfl_xor(nrm, o, 64);
+  if ((t & 63) == 0) sRed[t >> 6] = nrm;
+  __syncthreads();
+  if (t == 0) {
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < BS / 64; ++w) s += sRed[w];
+    a.part[a.npart1 + blockIdx.x] = s;
+    __threadfence();
+    const int tk = atomicAdd(&a.flag[1], 1);
+    slast = tk == (int)gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!slast) return;
+  // the last block: every partial sum is written; fixed-order reduction
+  __threadfence();
+  const int ntot = a.npart1 + (int)gridDim.x;
+  double s = 0.0;
+  for (int q = t; q < ntot; q += BS) s += a.part[q];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  __syncthreads();
+  if ((t & 63) == 0) sRed[t >> 6] = s;
+  __syncthreads();
+  if (t == 0) {
+    double tot = 0.0;
+#pragma unroll
+    for (int w = 0; w < BS / 64; ++w) tot += sRed[w];
+    const double rms = sqrt(tot / a.nfun);
+    a.flag[0] = rms < a.thresh ? 1 : 0;
+    a.flag[1] = 0;
+  }
+}
+
+int cn_interior_blocks(const CnArgs& a) {
+  const int epb = std::min(32, kCnInteriorThreads / a.ni);
+  const int nchunk = (a.nfe + epb - 1) / epb;
+  const int nd = a.nfun - a.nfcore;
+  const int ngrp = (nd + kCnGroup - 1) / kCnGroup;
+  return a.L1 * nchunk * ngrp;
+}
+
+void launch_cn_stage(const CnArgs& a, int mode, hipStream_t s) {
+  const int nd = a.nfun - a.nfcore;
+  if (nd <= 0) return;
+  hipLaunchKernelGGL(k_cn_rhs, dim3((a.nr + 255) / 256, a.L1, nd), dim3(256), 0, s, a, mode);
+  hipLaunchKernelGGL((k_cn_bridge<256>), dim3(nd * a.L1), dim3(256), 0, s, a, mode);
+  const int epb = std::min(32, kCnInteriorThreads / a.ni);
+  const int nchunk = (a.nfe + epb - 1) / epb;
+  const int ngrp = (nd + kCnGroup - 1) / kCnGroup;
+  const dim3 grid(a.L1 * nchunk * ngrp);
+  if (a.ni == 19)
+    hipLaunchKernelGGL((k_cn_interior<19>), grid, dim3(kCnInteriorThreads), 0, s, a, mode, epb, nchunk, ngrp);
+  else
+    hipLaunchKernelGGL((k_cn_interior<0>), grid, dim3(kCnInteriorThreads), 0, s, a, mode, epb, nchunk, ngrp);
+}
+
+}  // namespace td1c

@@ -60,6 +60,39 @@ struct OneBodyElArgs {
   const double* lf;             // device lfield[9]
 };
 
+// One half step of clh1itr (split_type = 1; h1itr_prop,
+// src/prop/h1itr_prop.F90:2-134): x0 = (1 + c T)^-1 (1 - c (T + V)) psi,
+// x_k = (1 + c T)^-1 (-c V) x_(k-1), psi <- sum x_k over the dynamical
+// orbitals, c = i dt/4 (icomp 1) or dt/4, V the field coupling of lf[2].
+// (1 + c T_l)^-1 is the element Schur solve of the complex tables below.
+struct CnArgs {
+  int nr, n, nfe, ni, nb, K, L1, nfun, nfcore, mmax1, igauge, isplit;
+  const int* mval;
+  const double *band_k, *band_k0, *band_p;  // [2n+1][nr] transposed bands
+  const double2* tdiag;                     // [L1][nr] tmat diagonal (+ v2jfc)
+  const double *xr, *alph;
+  const double* lf;                         // device lfield[9] of this half step
+  const double2 *G;                         // [L1][nfe][k][i] = G_e(i, k)
+  const double2 *Wl, *Wr;                   // [L1][nr]
+  const double2 *pa, *pg;                   // [L1][K][nb]
+  const double2* binv;                      // [L1][nb]
+  double2* wfn;                             // orbitals of the state (dynamical rows updated)
+  double2* x;                               // [nd][L1][nr] current term
+  double2* y;                               // [nd][L1][nr] right-hand side
+  double2 zfac1;                            // -c
+  double thresh;
+  int* flag;                                // [0] converged, [1] block ticket of the interior pass
+  double* part;                             // norm partial sums: bridge jobs, then interior blocks
+  int npart1;
+};
+constexpr int kCnGroup = 8;             // dynamical orbitals per interior block
+constexpr int kCnInteriorThreads = 256;
+// partial sums of the interior pass for a problem (for the buffer size)
+int cn_interior_blocks(const CnArgs& a);
+// mode 0: the first term from psi; mode 1: one more term of the series
+// (every kernel returns at once once flag[0] is set)
+void launch_cn_stage(const CnArgs& a, int mode, hipStream_t s);
+
 struct PoissonArgs {
   int nr, n, nfe, ni, L2, npair;
   const int* pair_m;  // m_j - m_i of each pair
@@ -105,6 +138,9 @@ struct DtorbArgs {
   // 2 of the cores only (clwfn::ladapt after the hprod)
   int ladapt;
   const int* lval;
+  // split_type (hprod_htot_dtorb.F90:60-83): 0 the Q part carries v2 + g +
+  // h1 + h0, 1 only v2 + g (h0 + h1 go to the clh1itr half steps)
+  int isplit;
   double2 xfac;  // X = <j|H|i> * xfac
   const double2* lprod;
   const int *mapH, *mapG, *mapV, *mapF;

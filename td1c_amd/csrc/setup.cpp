@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <complex>
 #include <cstdio>
 #include <cstdlib>
 
@@ -147,6 +148,42 @@ bool invert(int m, std::vector<double>& a) {
   return true;
 }
 
+// Dense inverse of a small complex matrix (Gauss-Jordan with partial
+// pivoting on |re| + |im|).
+bool invert_c(int m, std::vector<std::complex<double>>& a) {
+  using cd = std::complex<double>;
+  std::vector<cd> inv(m * m, cd(0.0, 0.0));
+  for (int i = 0; i < m; ++i) inv[i * m + i] = 1.0;
+  auto mag = [](cd z) { return std::fabs(z.real()) + std::fabs(z.imag()); };
+  for (int c = 0; c < m; ++c) {
+    int piv = c;
+    for (int r = c + 1; r < m; ++r)
+      if (mag(a[r * m + c]) > mag(a[piv * m + c])) piv = r;
+    if (a[piv * m + c] == cd(0.0, 0.0)) return false;
+    if (piv != c)
+      for (int k = 0; k < m; ++k) {
+        std::swap(a[piv * m + k], a[c * m + k]);
+        std::swap(inv[piv * m + k], inv[c * m + k]);
+      }
+    const cd d = 1.0 / a[c * m + c];
+    for (int k = 0; k < m; ++k) {
+      a[c * m + k] *= d;
+      inv[c * m + k] *= d;
+    }
+    for (int r = 0; r < m; ++r) {
+      if (r == c) continue;
+      const cd f = a[r * m + c];
+      if (f == cd(0.0, 0.0)) continue;
+      for (int k = 0; k < m; ++k) {
+        a[r * m + k] -= f * a[c * m + k];
+        inv[r * m + k] -= f * inv[c * m + k];
+      }
+    }
+  }
+  a.swap(inv);
+  return true;
+}
+
 int popc(uint32_t v) { return __builtin_popcount(v); }
 
 }  // namespace
@@ -154,7 +191,12 @@ int popc(uint32_t v) { return __builtin_popcount(v); }
 bool Setup::build(const td1c_problem& p) {
   if (p.nfe < 2 || p.ndvr < 2 || !p.edges) { error = "need nfe >= 2, ndvr >= 2 and element edges"; return false; }
   if (p.lmax1 < 0 || p.lmax2 < 0) { error = "bad lmax1/lmax2"; return false; }
-  if (p.isplit != 0 || p.ioorot != 0) { error = "only split_type = 0 and oorot_type = 0 are provided"; return false; }
+  if (p.ioorot != 0) { error = "only oorot_type = 0 is provided"; return false; }
+  if (p.isplit != 0 && p.isplit != 1) { error = "split_type must be 0 (full hprod) or 1 (clh1itr one-body half steps)"; return false; }
+  if (p.isplit == 1 && p.iprojfc == 1 && p.nfcore2 + p.nfcore1 > 0) {
+    error = "split_type = 1 with frozen cores needs iprojfc = 2 (h1itr_prop's frozen-core projection is not provided)";
+    return false;
+  }
   if (p.iprojfc != 1 && p.iprojfc != 2) { error = "iprojfc must be 1 or 2"; return false; }
   if (!p.jfc_implicit || !p.xfc_implicit) { error = "only jfc_implicit = xfc_implicit = true are provided"; return false; }
   nfe = p.nfe; n = p.ndvr; nrad = nfe * n; nr = nrad - 1;
@@ -580,6 +622,116 @@ bool Setup::build_ci() {
   };
   table(str_a, exA, sgA);
   table(str_b, exB, sgB);
+  return true;
+}
+
+// Crank-Nicolson factors of clh1itr (h1itr_gen_cninv,
+// src/prop/h1itr_gen_cninv.F90:2-58): A_l = 1 + fac T_l on the interior
+// points, with the element / bridge split of the Poisson operator.  The
+// reference factors the whole band by zgbtrf; the solution is the same.
+bool build_cn(const Setup& S, const double* v2jfc, double fac_re, double fac_im, CnTables& T, std::string& error) {
+  using cd = std::complex<double>;
+  const int n = S.n, nr = S.nr, nfe = S.nfe, L1 = S.L1, ni = n - 1, nb = nfe - 1;
+  const size_t bw = 2 * (size_t)n + 1;
+  const cd fac(fac_re, fac_im);
+  T.L1 = L1; T.nfe = nfe; T.ni = ni; T.nr = nr; T.nb = nb;
+  T.K = 0;
+  while ((1 << T.K) < nb) ++T.K;
+  const int K = T.K;
+  T.G.assign((size_t)2 * L1 * nfe * ni * ni, 0.0);
+  T.Wl.assign((size_t)2 * L1 * nr, 0.0);
+  T.Wr.assign((size_t)2 * L1 * nr, 0.0);
+  T.pa.assign((size_t)2 * L1 * std::max(K, 1) * std::max(nb, 1), 0.0);
+  T.pg.assign((size_t)2 * L1 * std::max(K, 1) * std::max(nb, 1), 0.0);
+  T.binv.assign((size_t)2 * L1 * std::max(nb, 1), 0.0);
+  auto put = [](std::vector<double>& v, size_t q, cd z) { v[2 * q] = z.real(); v[2 * q + 1] = z.imag(); };
+  for (int l = 0; l < L1; ++l) {
+    const std::vector<double>& kb = l == 0 ? S.radk : S.radk0;
+    auto A = [&](int i, int j) -> cd {  // interior indices
+      if (std::abs(i - j) > n) return cd(0.0, 0.0);
+      cd t;
+      if (i == j) {
+        const size_t q = (size_t)l * nr + i;
+        t = cd(S.tdiag[2 * q], S.tdiag[2 * q + 1]);
+        if (v2jfc) t += cd(v2jfc[2 * i], v2jfc[2 * i + 1]);
+      } else {
+        t = kb[bw * (i + 1) + (n + j - i)];
+      }
+      return (i == j ? cd(1.0, 0.0) : cd(0.0, 0.0)) + fac * t;
+    };
+    std::vector<cd> sdiag(std::max(nb, 1)), soff(std::max(nb, 1)), blk((size_t)ni * ni), wl(ni), wr(ni);
+    for (int k = 0; k < nb; ++k) {
+      const int gi = (k + 1) * n - 1;
+      sdiag[k] = A(gi, gi);
+      soff[k] = k + 1 < nb ? A(gi, gi + n) : cd(0.0, 0.0);
+    }
+    for (int e = 0; e < nfe; ++e) {
+      const int base = e * n;
+      for (int a = 0; a < ni; ++a)
+        for (int b = 0; b < ni; ++b) blk[(size_t)a * ni + b] = A(base + a, base + b);
+      if (!invert_c(ni, blk)) {
+        error = "singular element block of the Crank-Nicolson operator";
+        return false;
+      }
+      // stored transposed, [k][i] = G(i, k), so that lane i reads a row of G with stride-1 loads
+      for (int a = 0; a < ni; ++a)
+        for (int k = 0; k < ni; ++k) put(T.G, ((size_t)l * nfe + e) * ni * ni + (size_t)k * ni + a, blk[(size_t)a * ni + k]);
+      const int bl = e >= 1 ? e * n - 1 : -1, br = e <= nfe - 2 ? (e + 1) * n - 1 : -1;
+      std::vector<cd> cl(ni), cr(ni);
+      for (int a = 0; a < ni; ++a) {
+        cl[a] = bl >= 0 ? A(bl, base + a) : cd(0.0, 0.0);
+        cr[a] = br >= 0 ? A(br, base + a) : cd(0.0, 0.0);
+      }
+      for (int a = 0; a < ni; ++a) {
+        cd sl(0.0, 0.0), sr(0.0, 0.0);
+        for (int b = 0; b < ni; ++b) {
+          sl += blk[(size_t)a * ni + b] * cl[b];
+          sr += blk[(size_t)a * ni + b] * cr[b];
+        }
+        wl[a] = sl;
+        wr[a] = sr;
+        put(T.Wl, (size_t)l * nr + base + a, sl);
+        put(T.Wr, (size_t)l * nr + base + a, sr);
+      }
+      cd gll(0.0, 0.0), grr(0.0, 0.0), glr(0.0, 0.0);
+      for (int a = 0; a < ni; ++a) {
+        gll += cl[a] * wl[a];
+        grr += cr[a] * wr[a];
+        glr += cl[a] * wr[a];
+      }
+      if (bl >= 0) sdiag[e - 1] -= gll;
+      if (br >= 0) sdiag[e] -= grr;
+      if (bl >= 0 && br >= 0) soff[e - 1] -= glr;
+    }
+    // parallel cyclic reduction of the symmetric tridiagonal bridge system
+    std::vector<cd> a(std::max(nb, 1)), b(std::max(nb, 1)), c(std::max(nb, 1)), na(a), nbv(b), nc(c);
+    for (int i = 0; i < nb; ++i) {
+      a[i] = i > 0 ? soff[i - 1] : cd(0.0, 0.0);
+      b[i] = sdiag[i];
+      c[i] = i + 1 < nb ? soff[i] : cd(0.0, 0.0);
+    }
+    for (int k = 0, h = 1; k < K; ++k, h *= 2) {
+      for (int i = 0; i < nb; ++i) {
+        const cd al = i - h >= 0 ? -a[i] / b[i - h] : cd(0.0, 0.0);
+        const cd ga = i + h < nb ? -c[i] / b[i + h] : cd(0.0, 0.0);
+        put(T.pa, ((size_t)l * K + k) * nb + i, al);
+        put(T.pg, ((size_t)l * K + k) * nb + i, ga);
+        na[i] = i - h >= 0 ? al * a[i - h] : cd(0.0, 0.0);
+        nc[i] = i + h < nb ? ga * c[i + h] : cd(0.0, 0.0);
+        nbv[i] = b[i] + (i - h >= 0 ? al * c[i - h] : cd(0.0, 0.0)) + (i + h < nb ? ga * a[i + h] : cd(0.0, 0.0));
+      }
+      a.swap(na);
+      b.swap(nbv);
+      c.swap(nc);
+    }
+    for (int i = 0; i < nb; ++i) {
+      if (!(std::abs(b[i]) > 1e-300)) {
+        error = "Crank-Nicolson bridge system: cyclic reduction met a zero pivot";
+        return false;
+      }
+      put(T.binv, (size_t)l * nb + i, 1.0 / b[i]);
+    }
+  }
   return true;
 }
 

@@ -93,4 +93,24 @@ struct Setup {
   bool build_ci();  // the CAS string tables (called by build)
 };
 
+// Crank-Nicolson factors of the one-body half step of split_type = 1
+// (clh1itr, h1itr_gen_cninv: 1 + fac T_l per l), in the element Schur form
+// of the Poisson solve: per (l, e) the inverse interior block G (complex
+// symmetric), W_l / W_r = G C^T scattered over the interior rows, and the
+// complex symmetric tridiagonal bridge system by parallel cyclic reduction
+// (step k updates d_i += pa[i] d_(i-h) + pg[i] d_(i+h); x_i = d_i binv[i]).
+// Complex values are (re, im) pairs of doubles.
+struct CnTables {
+  int L1 = 0, nfe = 0, ni = 0, nr = 0, nb = 0, K = 0;
+  std::vector<double> G;      // [L1][nfe][ni][ni] complex
+  std::vector<double> Wl, Wr; // [L1][nr] complex
+  std::vector<double> pa, pg; // [L1][K][nb] complex
+  std::vector<double> binv;   // [L1][nb] complex
+};
+
+// T_l = the kinetic band (radk for l = 0, radk0 above) + the diagonal
+// tdiag + v2jfc (frozen-core Hartree potential, nr complex values, or null).
+bool build_cn(const Setup& S, const double* v2jfc, double fac_re, double fac_im, CnTables& T,
+              std::string& error);
+
 }  // namespace td1c
