@@ -361,6 +361,19 @@ int td1c_get_timing(td1c_ctx* ctx, char* buf, int buflen);
  * that a timed run carries events around one kernel only. */
 int td1c_set_timing_kernel(td1c_ctx* ctx, const char* name);
 
+/* Launch shapes the autotuner chose for this context (no counterpart in
+ * the reference): at td1c_ctx_create the element one-body kernel's
+ * candidates (3, 6 or 12 elements per block x 1..16 l values per block)
+ * are timed on the context's buffers and the fastest kept, cached per
+ * problem shape; all candidates give the same bits.  TD1C_AUTOTUNE=0 in the
+ * environment keeps the defaults.  Writes JSON with the choice and the
+ * measured microseconds per launch. */
+int td1c_get_shapes(const td1c_ctx* ctx, char* buf, int buflen);
+/* Forces the element one-body kernel's launch shape (3, 6 or 12 elements
+ * per block; 1..64 l values per block), so that tests run every candidate
+ * (no counterpart in the reference). */
+int td1c_set_onebody_shape(td1c_ctx* ctx, int elements_per_block, int l_per_block);
+
 /* Binds a context as the process-wide one used by hprod_htot_ (the
  * reference allows a single clhprod per process, clhprod.cpp:32-34). */
 int td1c_bind(td1c_ctx* ctx);

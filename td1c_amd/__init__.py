@@ -114,6 +114,10 @@ def load_library():
     L.td1c_h1itr_prop.restype = ctypes.c_int
     L.hprod_htot_static.argtypes = [vp, vp, vp, vp, vp, vp]
     L.hprod_htot_static.restype = ctypes.c_int
+    L.td1c_set_onebody_shape.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.td1c_set_onebody_shape.restype = ctypes.c_int
+    L.td1c_get_shapes.argtypes = [vp, ctypes.c_char_p, ctypes.c_int]
+    L.td1c_get_shapes.restype = ctypes.c_int
     L.td1c_last_error.argtypes = []
     L.td1c_last_error.restype = ctypes.c_char_p
     _lib = L
@@ -422,6 +426,18 @@ class HProd:
         buf = ctypes.create_string_buffer(1 << 16)
         _check(load_library().td1c_get_timing(self.h, buf, len(buf)), "td1c_get_timing")
         return {k: tuple(v) for k, v in json.loads(buf.value.decode()).items()}
+
+    def shapes(self) -> dict:
+        """The launch shapes the autotuner chose for this context."""
+        import json
+
+        buf = ctypes.create_string_buffer(8192)
+        _check(load_library().td1c_get_shapes(self.h, buf, 8192), "td1c_get_shapes")
+        return json.loads(buf.value.decode())
+
+    def set_onebody_shape(self, elements_per_block: int, l_per_block: int):
+        _check(load_library().td1c_set_onebody_shape(self.h, int(elements_per_block), int(l_per_block)),
+               "td1c_set_onebody_shape")
 
     def set_graph(self, on: bool):
         """td1c_propagate replays one captured step (True) or enqueues every

@@ -151,6 +151,10 @@ struct td1c_ctx {
   // island stage B runs on side stream 0, off the main stream
   bool x_local = false;
   int tpath = 0;  // transform form: 0 by grid size, 1 vector, 2 matrix cores
+  // launch shape of the element one-body kernel (autotuned per problem):
+  // elements per block (0: 3, 1: 6, 2: 12) and l values per block
+  int ob_shape = 2, ob_lc = 2;
+  std::string tune_log;  // what the autotuner measured (JSON)
   // device-driven propagation: one captured RK step, replayed per step
   hipStream_t cap_stream = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -260,6 +264,16 @@ std::vector<double> padded_cols(const std::vector<double>& m, int K, int C) {
     for (int k = 0; k < K; ++k)
       for (int j = 0; j < C; ++j) out[(i * K + k) * Cp + j] = m[(i * K + k) * C + j];
   return out;
+}
+
+OneBodyElArgs onebody_el_args(td1c_ctx* c, const double* d_lf) {
+  const Setup& S = c->S;
+  OneBodyElArgs oe{};
+  oe.nr = S.nr; oe.nfe = S.nfe; oe.L1 = S.L1; oe.ndvr = S.n; oe.nfcore = S.nfcore; oe.nradfc = S.nradfc;
+  oe.nfun = S.nfun; oe.mmax1 = S.mmax1; oe.igauge = S.igauge; oe.fc_full = S.iprojfc == 2; oe.mval = c->mval.p;
+  oe.eK0 = c->eK0.p; oe.eK = c->eK.p; oe.eP = c->eP.p; oe.xr = c->xr.p; oe.alph = c->alph.p; oe.tzor = c->tzor.p;
+  oe.lf = d_lf;
+  return oe;
 }
 
 int build_device(td1c_ctx* c) {
@@ -690,14 +704,10 @@ int enqueue_htotx(td1c_ctx* c, double dtime, const double* d_lf, const double2* 
   ob.mval = c->mval.p; ob.band_k = c->band_k.p; ob.band_k0 = c->band_k0.p; ob.band_p = c->band_p.p;
   ob.xr = c->xr.p; ob.alph = c->alph.p; ob.tdiag = c->tdiag.p; ob.lf = d_lf;
   ob.f0 = 0; ob.f1 = nf;
-  OneBodyElArgs oe{};
-  oe.nr = nr; oe.nfe = S.nfe; oe.L1 = L1; oe.ndvr = S.n; oe.nfcore = nfc; oe.nradfc = S.nradfc; oe.nfun = nf;
-  oe.mmax1 = S.mmax1; oe.igauge = S.igauge; oe.fc_full = S.iprojfc == 2; oe.mval = c->mval.p;
-  oe.eK0 = c->eK0.p; oe.eK = c->eK.p; oe.eP = c->eP.p; oe.xr = c->xr.p; oe.alph = c->alph.p; oe.tzor = c->tzor.p;
-  oe.lf = d_lf;
+  const OneBodyElArgs oe = onebody_el_args(c, d_lf);
   {
     TIMED_ON(s2, "onebody");
-    if (!launch_onebody_el(c->orb, c->h01, oe, s2)) launch_onebody(c->orb, c->h01, ob, s2);
+    if (!launch_onebody_el(c->orb, c->h01, oe, s2, c->ob_shape, c->ob_lc)) launch_onebody(c->orb, c->h01, ob, s2);
   }
   HIP_TRY(hipEventRecord(c->fj[3], s2));
   bool pair_gram_done = false;
@@ -722,7 +732,8 @@ int enqueue_htotx(td1c_ctx* c, double dtime, const double* d_lf, const double2* 
     pa.invr = c->invr.p; pa.rpl0 = c->rpl0.p; pa.rpl1 = c->rpl1.p; pa.fac2 = c->fac2.p; pa.fac1 = c->fac1.p;
     pa.Q = c->p_Q.p;
     if (c->mixed) {
-      { TIMED("poisson"); launch_poisson(c->rhoL_f.p, c->p_xb.p, c->vL_f.p, pa, s); }
+      { TIMED("poisson_bridge"); launch_poisson(c->rhoL_f.p, c->p_xb.p, c->vL_f.p, pa, s, 1); }
+      { TIMED("poisson_interior"); launch_poisson(c->rhoL_f.p, c->p_xb.p, c->vL_f.p, pa, s, 2); }
       HIP_TRY(enqueue_pair_gram(c, s, s1));
       pair_gram_done = true;
       {
@@ -731,7 +742,8 @@ int enqueue_htotx(td1c_ctx* c, double dtime, const double* d_lf, const double2* 
                                            L2, nl, pad_cols(nl), nullptr, s, c->tpath);
       }
     } else {
-      { TIMED("poisson"); launch_poisson(c->rhoL.p, c->p_xb.p, c->vL.p, pa, s); }
+      { TIMED("poisson_bridge"); launch_poisson(c->rhoL.p, c->p_xb.p, c->vL.p, pa, s, 1); }
+      { TIMED("poisson_interior"); launch_poisson(c->rhoL.p, c->p_xb.p, c->vL.p, pa, s, 2); }
       HIP_TRY(enqueue_pair_gram(c, s, s1));
       pair_gram_done = true;
       { TIMED("pair_L2theta"); launch_transform(0, c->vL.p, c->vg.p, c->m_L2g.p, c->b_L2g.p, c->npair, nr, L2, nl, pad_cols(nl), nullptr, s, c->tpath); }
@@ -972,6 +984,87 @@ extern "C" {
 
 const char* td1c_last_error(void) { return g_error.c_str(); }
 
+// ---------------------------------------------------------------------------
+// Autotuner of launch shapes.  Candidates of a kernel are timed on the
+// context's own buffers (HIP events, a private stream, one warm-up and five
+// timed launches each) and the fastest is kept; every candidate computes the
+// same bits, so the choice changes speed only.  Results are cached per
+// process by problem shape, so contexts of the same shape tune once.
+// TD1C_AUTOTUNE=0 keeps the built-in defaults.
+// ---------------------------------------------------------------------------
+namespace {
+
+std::string onebody_shape_key(const Setup& S) {
+  return std::to_string(S.nr) + "/" + std::to_string(S.nfe) + "/" + std::to_string(S.L1) + "/" +
+         std::to_string(S.nfun) + "/" + std::to_string(S.nfcore) + "/" + std::to_string(S.igauge);
+}
+
+int autotune_onebody(td1c_ctx* c) {
+  static std::mutex mu;
+  static std::map<std::string, std::pair<int, int>> cache;
+  const Setup& S = c->S;
+  if (S.n != 20) return 0;  // the band kernel runs for other ndvr
+  const char* env = std::getenv("TD1C_AUTOTUNE");
+  if (env && std::string(env) == "0") return 0;
+  const std::string key = onebody_shape_key(S);
+  {
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) {
+      c->ob_shape = it->second.first;
+      c->ob_lc = it->second.second;
+      return 0;
+    }
+  }
+  hipStream_t st = nullptr;
+  HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  const OneBodyElArgs oe = onebody_el_args(c, c->lf_dev.p);
+  HIP_TRY(hipMemsetAsync(c->orb, 0, (size_t)c->nbas * S.nfun * sizeof(double2), st));
+  HIP_TRY(hipMemsetAsync(c->lf_dev.p, 0, 9 * sizeof(double), st));
+  float best = 1e30f;
+  int bs = c->ob_shape, bl = c->ob_lc;
+  std::string log = "{\"onebody\": [";
+  const int lcs[] = {1, 2, 4, 8, 16};
+  bool first = true;
+  for (int shape = 0; shape < 3; ++shape)
+    for (int lc : lcs) {
+      if (lc > S.L1) continue;
+      launch_onebody_el(c->orb, c->h01, oe, st, shape, lc);
+      HIP_TRY(hipEventRecord(e0, st));
+      for (int r = 0; r < 5; ++r) launch_onebody_el(c->orb, c->h01, oe, st, shape, lc);
+      HIP_TRY(hipEventRecord(e1, st));
+      HIP_TRY(hipEventSynchronize(e1));
+      float ms = 0.0f;
+      HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+      ms /= 5.0f;
+      char buf[96];
+      std::snprintf(buf, sizeof buf, "%s[%d, %d, %.2f]", first ? "" : ", ", shape, lc, ms * 1e3f);
+      log += buf;
+      first = false;
+      if (ms < best) {
+        best = ms;
+        bs = shape;
+        bl = lc;
+      }
+    }
+  log += "]}";
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipStreamDestroy(st);
+  HIP_TRY(hipGetLastError());
+  c->ob_shape = bs;
+  c->ob_lc = bl;
+  c->tune_log = log;
+  std::lock_guard<std::mutex> lock(mu);
+  cache[key] = {bs, bl};
+  return 0;
+}
+
+}  // namespace
+
 int td1c_ctx_create(const td1c_problem* prob, td1c_ctx** out) {
   if (!prob || !out) return fail(-1, "null argument");
   std::unique_ptr<td1c_ctx> c(new td1c_ctx());
@@ -983,7 +1076,33 @@ int td1c_ctx_create(const td1c_problem* prob, td1c_ctx** out) {
   if (c->S.nfe - 1 > kMaxBridge) return fail(-2, "more than 161 finite elements are not provided by the Poisson kernel");
   int rc = build_device(c.get());
   if (rc) return rc;
+  rc = autotune_onebody(c.get());
+  if (rc) return rc;
   *out = c.release();
+  return 0;
+}
+
+int td1c_set_onebody_shape(td1c_ctx* c, int elements_per_block, int l_per_block) {
+  if (!c) return fail(-1, "null argument");
+  const int shape = elements_per_block == 3 ? 0 : elements_per_block == 6 ? 1 : elements_per_block == 12 ? 2 : -1;
+  if (shape < 0 || l_per_block < 1 || l_per_block > 64)
+    return fail(-1, "elements_per_block must be 3, 6 or 12 and l_per_block 1..64");
+  if (c->gexec) {  // a captured step holds the old launch shape
+    HIP_TRY(hipGraphExecDestroy(c->gexec));
+    c->gexec = nullptr;
+  }
+  c->ob_shape = shape;
+  c->ob_lc = l_per_block;
+  return 0;
+}
+
+int td1c_get_shapes(const td1c_ctx* c, char* buf, int buflen) {
+  if (!c || !buf || buflen <= 0) return fail(-1, "null argument");
+  std::string js = "{\"onebody_elements_per_block\": " + std::to_string(c->ob_shape == 0 ? 3 : c->ob_shape == 1 ? 6 : 12) +
+                   ", \"onebody_l_per_block\": " + std::to_string(c->ob_lc) +
+                   ", \"measured_us\": " + (c->tune_log.empty() ? std::string("null") : c->tune_log) + "}";
+  if ((int)js.size() + 1 > buflen) return fail(-1, "buffer too small");
+  std::memcpy(buf, js.c_str(), js.size() + 1);
   return 0;
 }
 
@@ -1627,9 +1746,19 @@ static CnArgs cn_args(td1c_ctx* c, const double* d_lf, double2* wfn) {
 // after convergence.
 static int enqueue_h1itr(td1c_ctx* c, const double* d_lf, double2* wfn, hipStream_t s) {
   const CnArgs a = cn_args(c, d_lf, wfn);
+  // the right-hand sides by the element one-body kernel (ndvr = 20): over
+  // the dynamical orbitals, with the field of this half step
+  OneBodyElArgs ob = onebody_el_args(c, d_lf);
+  ob.nfun = c->S.nfun - c->S.nfcore;
+  ob.nfcore = 0;
+  ob.fshift = c->S.nfcore;
+  ob.isplit = c->S.isplit;
+  ob.cv = a.zfac1;
+  ob.stop = a.flag;
+  const OneBodyElArgs* obp = c->S.n == 20 ? &ob : nullptr;
   TIMED("h1itr");
-  launch_cn_stage(a, 0, s);
-  for (int k = 0; k < c->cn_maxcyc; ++k) launch_cn_stage(a, 1, s);
+  launch_cn_stage(a, obp, c->ob_shape, c->ob_lc, 0, s);
+  for (int k = 0; k < c->cn_maxcyc; ++k) launch_cn_stage(a, obp, c->ob_shape, c->ob_lc, 1, s);
   return 0;
 }
 

@@ -10,10 +10,11 @@
 // same operator is solved by element Schur complement: the interior blocks
 // of each FEDVR element are eliminated with their precomputed inverses and
 // the bridge points form a tridiagonal system solved by cyclic reduction.
-// One term of the series is three launches: the right-hand side, the bridge
-// pass (one block per orbital row), the interior pass (one block per (l,
-// element chunk, orbital group), each element inverse read once for the
-// group).  The interior pass's last block tests convergence; every later
+// One term of the series is three launches: the right-hand side (the
+// element one-body kernel in its Crank-Nicolson modes, or a band kernel for
+// ndvr != 20), the bridge pass (one block per orbital row), the interior
+// pass (one block per (l, element chunk), each element inverse read once for
+// every orbital).  The interior pass's last block tests convergence; every later
 // launch of the half step returns at once after it.
 #include <hip/hip_runtime.h>
 
@@ -183,16 +184,15 @@ __global__ void __launch_bounds__(BS) k_cn_bridge(const CnArgs a, int mode) {
   }
 }
 
-// Interior pass, one block per (l, chunk of epb elements, group of kCnGroup
-// dynamical orbitals), thread per interior row: x_I = G_e y_I - W_l
-// x_left - W_r x_right with the row of G_e in registers for the whole group,
-// into x and the state.  The last block to finish sums the partial norms of
-// both passes in a fixed order and sets flag[0] when the RMS orbital norm of
-// this term is below thresh (the test at the top of the next cycle of
-// h1itr_prop.F90:75-99).
+// Interior pass, one block per (l, chunk of epb elements), thread per
+// interior row, every dynamical orbital in groups of kCnGroup: x_I = G_e y_I
+// - W_l x_left - W_r x_right with the row of G_e loaded once into registers
+// for all orbitals, into x and the state.  The last block to finish sums
+// the partial norms of both passes in a fixed order and sets flag[0] when
+// the RMS orbital norm of this term is below thresh (the test at the top of
+// the next cycle of h1itr_prop.F90:75-99).
 template <int NI>
-__global__ void __launch_bounds__(kCnInteriorThreads) k_cn_interior(const CnArgs a, int mode, int epb, int nchunk,
-                                                                    int ngrp) {
+__global__ void __launch_bounds__(kCnInteriorThreads) k_cn_interior(const CnArgs a, int mode, int epb, int nchunk) {
   if (mode == 1 && a.flag[0]) return;
   constexpr int PG = kCnGroup;
   constexpr int BS = kCnInteriorThreads;
@@ -200,8 +200,8 @@ __global__ void __launch_bounds__(kCnInteriorThreads) k_cn_interior(const CnArgs
   const int nb = a.nb, nr = a.nr, n = a.n, L1 = a.L1;
   const int nd = a.nfun - a.nfcore;
   const int job = xcd_remap(blockIdx.x, gridDim.x);
-  const int l = job / (nchunk * ngrp), ch = (job / ngrp) % nchunk, g = job % ngrp;
-  const int fd0 = g * PG, e0 = ch * epb;
+  const int l = job / nchunk, ch = job - l * nchunk;
+  const int e0 = ch * epb;
   const int t = threadIdx.x;
   const int el = t / ni, i = t - el * ni;
   const int e = e0 + el;
@@ -212,26 +212,6 @@ __global__ void __launch_bounds__(kCnInteriorThreads) k_cn_interior(const CnArgs
   __shared__ int sval[PG];
   __shared__ double sRed[BS / 64];
   __shared__ int slast;
-  if (t < PG) {
-    const int fd = fd0 + t;
-    int ok = 0;
-    if (fd < nd) {
-      const int m = a.mval[a.nfcore + fd];
-      ok = l >= (m < 0 ? -m : m);
-    }
-    sval[t] = ok;
-  }
-#pragma unroll
-  for (int j = 0; j < PG; ++j) {
-    const int fd = fd0 + j;
-    if (row_ok && fd < nd) sb[j][t] = a.y[((size_t)fd * L1 + l) * nr + r];
-  }
-  for (int q = t; q < PG * (epb + 1); q += BS) {
-    const int j = q / (epb + 1), k = q - j * (epb + 1);
-    const int fd = fd0 + j, bidx = e0 - 1 + k;
-    sx[j][k] = (fd < nd && bidx >= 0 && bidx < nb) ? a.x[((size_t)fd * L1 + l) * nr + (size_t)(bidx + 1) * n - 1]
-                                                   : make_double2(0.0, 0.0);
-  }
   double2 gk[NI > 0 ? NI : 1];
   double2 wl = make_double2(0.0, 0.0), wr = make_double2(0.0, 0.0);
   const double2* ge = a.G + (((size_t)l * a.nfe) + (row_ok ? e : 0)) * ni * ni + i;
@@ -243,28 +223,51 @@ __global__ void __launch_bounds__(kCnInteriorThreads) k_cn_interior(const CnArgs
     wl = a.Wl[(size_t)l * nr + r];
     wr = a.Wr[(size_t)l * nr + r];
   }
-  __syncthreads();
   double nrm = 0.0;
-  if (row_ok) {
-    const int eb = el * ni;
-#pragma unroll 1
+  for (int fd0 = 0; fd0 < nd; fd0 += PG) {
+    __syncthreads();  // the previous group's staged rows are no longer read
+    if (t < PG) {
+      const int fd = fd0 + t;
+      int ok = 0;
+      if (fd < nd) {
+        const int m = a.mval[a.nfcore + fd];
+        ok = l >= (m < 0 ? -m : m);
+      }
+      sval[t] = ok;
+    }
+#pragma unroll
     for (int j = 0; j < PG; ++j) {
       const int fd = fd0 + j;
-      if (fd >= nd) break;
-      if (!sval[j]) continue;
-      double2 x = make_double2(0.0, 0.0);
-      if (NI > 0) {
+      if (row_ok && fd < nd) sb[j][t] = a.y[((size_t)fd * L1 + l) * nr + r];
+    }
+    for (int q = t; q < PG * (epb + 1); q += BS) {
+      const int j = q / (epb + 1), k = q - j * (epb + 1);
+      const int fd = fd0 + j, bidx = e0 - 1 + k;
+      sx[j][k] = (fd < nd && bidx >= 0 && bidx < nb) ? a.x[((size_t)fd * L1 + l) * nr + (size_t)(bidx + 1) * n - 1]
+                                                     : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    if (row_ok) {
+      const int eb = el * ni;
+#pragma unroll 1
+      for (int j = 0; j < PG; ++j) {
+        const int fd = fd0 + j;
+        if (fd >= nd) break;
+        if (!sval[j]) continue;
+        double2 x = make_double2(0.0, 0.0);
+        if (NI > 0) {
 #pragma unroll
-        for (int k = 0; k < NI; ++k) cfma(x, gk[k], sb[j][eb + k]);
-      } else {
-        for (int k = 0; k < ni; ++k) cfma(x, ge[k * ni], sb[j][eb + k]);
+          for (int k = 0; k < NI; ++k) cfma(x, gk[k], sb[j][eb + k]);
+        } else {
+          for (int k = 0; k < ni; ++k) cfma(x, ge[k * ni], sb[j][eb + k]);
+        }
+        if (e >= 1) x = csub(x, cmul(wl, sx[j][el]));
+        if (e <= a.nfe - 2) x = csub(x, cmul(wr, sx[j][el + 1]));
+        a.x[((size_t)fd * L1 + l) * nr + r] = x;
+        double2* w = a.wfn + ((size_t)(a.nfcore + fd) * L1 + l) * nr + r;
+        *w = mode == 0 ? x : cadd(*w, x);
+        nrm += x.x * x.x + x.y * x.y;
       }
-      if (e >= 1) x = csub(x, cmul(wl, sx[j][el]));
-      if (e <= a.nfe - 2) x = csub(x, cmul(wr, sx[j][el + 1]));
-      a.x[((size_t)fd * L1 + l) * nr + r] = x;
-      double2* w = a.wfn + ((size_t)(a.nfcore + fd) * L1 + l) * nr + r;
-      *w = mode == 0 ? x : cadd(*w, x);
-      nrm += x.x * x.x + x.y * x.y;
     }
   }
   for (int o = 32; o > 0; o >>= 1) nrm += __shfl_xor(nrm, o, 64);
@@ -303,24 +306,27 @@ __global__ void __launch_bounds__(kCnInteriorThreads) k_cn_interior(const CnArgs
 int cn_interior_blocks(const CnArgs& a) {
   const int epb = std::min(32, kCnInteriorThreads / a.ni);
   const int nchunk = (a.nfe + epb - 1) / epb;
-  const int nd = a.nfun - a.nfcore;
-  const int ngrp = (nd + kCnGroup - 1) / kCnGroup;
-  return a.L1 * nchunk * ngrp;
+  return a.L1 * nchunk;
 }
 
-void launch_cn_stage(const CnArgs& a, int mode, hipStream_t s) {
+void launch_cn_stage(const CnArgs& a, const OneBodyElArgs* ob, int shape, int lc, int mode, hipStream_t s) {
   const int nd = a.nfun - a.nfcore;
   if (nd <= 0) return;
-  hipLaunchKernelGGL(k_cn_rhs, dim3((a.nr + 255) / 256, a.L1, nd), dim3(256), 0, s, a, mode);
+  if (ob) {  // element form of the right-hand side (ndvr = 20)
+    OneBodyElArgs e = *ob;
+    e.cn = mode + 1;
+    launch_onebody_el(mode == 0 ? a.wfn + (size_t)a.nfcore * a.L1 * a.nr : a.x, a.y, e, s, shape, lc);
+  } else {
+    hipLaunchKernelGGL(k_cn_rhs, dim3((a.nr + 255) / 256, a.L1, nd), dim3(256), 0, s, a, mode);
+  }
   hipLaunchKernelGGL((k_cn_bridge<256>), dim3(nd * a.L1), dim3(256), 0, s, a, mode);
   const int epb = std::min(32, kCnInteriorThreads / a.ni);
   const int nchunk = (a.nfe + epb - 1) / epb;
-  const int ngrp = (nd + kCnGroup - 1) / kCnGroup;
-  const dim3 grid(a.L1 * nchunk * ngrp);
+  const dim3 grid(a.L1 * nchunk);
   if (a.ni == 19)
-    hipLaunchKernelGGL((k_cn_interior<19>), grid, dim3(kCnInteriorThreads), 0, s, a, mode, epb, nchunk, ngrp);
+    hipLaunchKernelGGL((k_cn_interior<19>), grid, dim3(kCnInteriorThreads), 0, s, a, mode, epb, nchunk);
   else
-    hipLaunchKernelGGL((k_cn_interior<0>), grid, dim3(kCnInteriorThreads), 0, s, a, mode, epb, nchunk, ngrp);
+    hipLaunchKernelGGL((k_cn_interior<0>), grid, dim3(kCnInteriorThreads), 0, s, a, mode, epb, nchunk);
 }
 
 }  // namespace td1c

@@ -378,26 +378,31 @@ __global__ void __launch_bounds__(256) k_onebody(const double2* __restrict__ orb
 
 // ---------------------------------------------------------------------------
 // One-body part in element-local form (same operator as k_onebody): a block
-// owns GE consecutive elements of one orbital and walks l = 0..lmax1.  Lane
-// (element, m) keeps row m of the element's kinetic and d/dr matrices in
-// registers; psi(l-1), psi(l), psi(l+1) of the block's points slide through
-// LDS.  An interior point needs one row product; a bridge adds the last row
-// of the element on its left (lane 252 recomputes it for the block's first
-// element).  N = ndvr.
+// owns GE consecutive elements of one orbital and walks lc consecutive l
+// values.  Lane (element, m) keeps row m of the element's kinetic and d/dr
+// matrices in registers; psi(l-1), psi(l), psi(l+1) of the block's points
+// slide through LDS.  An interior point needs one row product; a bridge adds
+// the last row of the element on its left (the lane after the GE elements
+// recomputes it for the block's first element).  With GE = 3 the block is a
+// single wavefront (63 + 1 lanes) and the bridge partials move by lane
+// shuffles instead of LDS.  The (GE, lc) pair is a launch shape chosen per
+// problem by the autotuner; every shape gives the same bits.  N = ndvr.
 // ---------------------------------------------------------------------------
-template <int N>
-__global__ void __launch_bounds__(256) k_onebody_el(const double2* __restrict__ orb, double2* __restrict__ h01,
-                                                    const OneBodyElArgs a) {
+template <int N, int GE, int CN>
+__global__ void __launch_bounds__(((GE * (N + 1) + 1) + 63) / 64 * 64)
+    k_onebody_el(const double2* __restrict__ orb, double2* __restrict__ h01, const OneBodyElArgs a, int lc) {
+  if (CN == 2 && a.stop[0]) return;
   const double2 zf = make_double2(a.lf[2], 0.0), ef = make_double2(a.lf[5], 0.0);
   constexpr int NP = N + 1;
-  constexpr int GE = 256 / NP;               // elements per block
+  constexpr int BS = ((GE * NP + 1) + 63) / 64 * 64;
+  constexpr bool ONE_WAVE = BS == 64;
   constexpr int TILE = (GE + 1) * N + 1;     // points of elements e0-1 .. e0+GE-1
   __shared__ double2 sPsi[3][TILE];
-  __shared__ double2 sT[256], sP1[256], sP0[256];
+  __shared__ double2 sT[ONE_WAVE ? 1 : BS], sP1[ONE_WAVE ? 1 : BS], sP0[ONE_WAVE ? 1 : BS];
   const int f = blockIdx.z;
   const int e0 = blockIdx.x * GE;
-  const int l0 = blockIdx.y * kOneBodyLChunk;
-  const int l1 = min(a.L1, l0 + kOneBodyLChunk);
+  const int l0 = blockIdx.y * lc;
+  const int l1 = min(a.L1, l0 + lc);
   const int tid = threadIdx.x;
   // lane -> (element, local row)
   int e, m;
@@ -412,9 +417,9 @@ __global__ void __launch_bounds__(256) k_onebody_el(const double2* __restrict__ 
     m = 0;
   }
   const bool lane_ok = e >= 0 && e < a.nfe;
-  const int mf = a.mval[f];
+  const int mf = a.mval[CN ? a.fshift + f : f];
   const int am = mf < 0 ? -mf : mf;
-  const bool fc = f < a.nfcore;
+  const bool fc = CN ? false : f < a.nfcore;
   const int rows = fc ? a.nradfc : a.nr;
   const int ibase = (e0 - 1) * N - 1;        // global interior index of tile slot 0
   // tile slot of this lane's element point 0
@@ -431,8 +436,11 @@ __global__ void __launch_bounds__(256) k_onebody_el(const double2* __restrict__ 
   const double xr = owner ? a.xr[i] : 1.0;
   const double* al = a.alph + (size_t)(mf + a.mmax1) * a.L1;
   const double2* base_f = orb + (size_t)f * a.L1 * a.nr;
+  // the lane holding the left element's last row: the previous lane, or the
+  // extra lane for the first element
+  const int pl = tid >= NP ? tid - 1 : GE * NP;
   auto load_tile = [&](int l, int buf) {
-    for (int q = tid; q < TILE; q += 256) {
+    for (int q = tid; q < TILE; q += BS) {
       const int gi = ibase + q;
       double2 v = make_double2(0.0, 0.0);
       if (l >= 0 && l < a.L1 && gi >= 0 && gi < a.nr) v = base_f[(size_t)l * a.nr + gi];
@@ -443,12 +451,12 @@ __global__ void __launch_bounds__(256) k_onebody_el(const double2* __restrict__ 
   load_tile(l0 - 1, (l0 + 2) % 3);
   for (int l = l0; l < l1; ++l) {
     const int b0 = l % 3, b1 = (l + 1) % 3, bm = (l + 2) % 3;
-    __syncthreads();
+    __syncthreads();  // the previous trip's reads of buffer b1 are done
     load_tile(l + 1, b1);
     __syncthreads();
     // element-row partial products
     double2 pT = make_double2(0.0, 0.0), pP1 = make_double2(0.0, 0.0), pP0 = make_double2(0.0, 0.0);
-    if (lane_ok && l >= am) {
+    if (lane_ok && l >= am && CN != 2) {
       const double2* s0 = sPsi[b0] + eslot;
       if (l == 0) {
 #pragma unroll
@@ -464,7 +472,9 @@ __global__ void __launch_bounds__(256) k_onebody_el(const double2* __restrict__ 
           pT.y = fma(Kr[k], s0[k].y, pT.y);
         }
       }
-      if (a.igauge == 1) {
+    }
+    if (lane_ok && l >= am) {
+      if (a.igauge == 1 && (CN != 1 || a.isplit == 1)) {
         const double2* s1 = sPsi[b1] + eslot;
         const double2* sm = sPsi[bm] + eslot;
 #pragma unroll
@@ -476,24 +486,61 @@ __global__ void __launch_bounds__(256) k_onebody_el(const double2* __restrict__ 
         }
       }
     }
-    sT[tid] = pT;
-    sP1[tid] = pP1;
-    sP0[tid] = pP0;
-    __syncthreads();
+    // the left neighbour's partials (bridge rows)
+    double2 lT, lP1, lP0;
+    if (ONE_WAVE) {
+      lT = make_double2(__shfl(pT.x, pl, 64), __shfl(pT.y, pl, 64));
+      lP1 = make_double2(__shfl(pP1.x, pl, 64), __shfl(pP1.y, pl, 64));
+      lP0 = make_double2(__shfl(pP0.x, pl, 64), __shfl(pP0.y, pl, 64));
+    } else {
+      sT[tid] = pT;
+      sP1[tid] = pP1;
+      sP0[tid] = pP0;
+      __syncthreads();
+      lT = sT[pl];
+      lP1 = sP1[pl];
+      lP0 = sP0[pl];
+    }
     if (owner) {
       double2 out = make_double2(0.0, 0.0);
       if (l >= am && i < rows) {
         double2 T = pT, P1 = pP1, P0 = pP0;
         if (m == 0) {  // bridge: add the left element's last row
-          const int pl = tid >= NP ? tid - 1 : GE * NP;
-          T = cadd(T, sT[pl]);
-          P1 = cadd(P1, sP1[pl]);
-          P0 = cadd(P0, sP0[pl]);
+          T = cadd(T, lT);
+          P1 = cadd(P1, lP1);
+          P0 = cadd(P0, lP0);
         }
         const int slot = eslot + m;
         const double2 p0 = sPsi[b0][slot], pu = sPsi[b1][slot], pd = sPsi[bm][slot];
         double2 h = make_double2(0.0, 0.0);
         const bool full = !fc || a.fc_full;
+        if (CN) {
+          // the clh1itr right-hand sides: T psi (CN 1), then V psi (CN 1 with
+          // isplit = 1, and CN 2), as h1itr_prop.F90:57-70,106-114
+          if (CN == 1) {
+            h = T;
+            cfma(h, p0, a.tzor[(size_t)l * a.nr + i]);
+          }
+          if (CN == 2 || a.isplit == 1) {
+            if (a.igauge == 0) {
+              if (l + 1 < a.L1) cfma(h, pu, cscale(zf, xr * al[l]));
+              if (l - 1 >= am) cfma(h, pd, cscale(zf, xr * al[l - 1]));
+            } else {
+              if (l + 1 < a.L1) {
+                cfma(h, P1, cmul(zf, make_double2(0.0, -al[l])));
+                cfma(h, pu, cmul(zf, make_double2(0.0, -(al[l] * (l + 1.0)) / xr)));
+              }
+              if (l - 1 >= am) {
+                cfma(h, P0, cmul(zf, make_double2(0.0, -al[l - 1])));
+                cfma(h, pd, cscale(cmul(zf, make_double2(0.0, -(al[l - 1] * (double)l) / xr)), -1.0));
+              }
+            }
+          }
+          double2 y = CN == 1 ? p0 : make_double2(0.0, 0.0);
+          cfma(y, a.cv, h);
+          h01[((size_t)f * a.L1 + l) * a.nr + i] = y;
+          continue;
+        }
         if (full) {
           h = T;
           cfma(h, p0, a.tzor[(size_t)l * a.nr + i]);
@@ -1351,11 +1398,31 @@ void launch_transform(int mode, const double2* in, double2* out, const double* m
   launch_transform_t<double2, double2>(mode, in, out, mats, nullptr, batch, nbatch, nr, K, C, Cp, copy, s, path, fc);
 }
 
-bool launch_onebody_el(const double2* orb, double2* h01, const OneBodyElArgs& a, hipStream_t s) {
+// shape: 0 = 3 elements per block (one wavefront), 1 = 6 (128 threads),
+// 2 = 12 (256 threads); lc = l values per block
+template <int CN>
+static void launch_onebody_el_t(const double2* orb, double2* h01, const OneBodyElArgs& a, hipStream_t s, int shape,
+                                int lc) {
+  const int nl = (a.L1 + lc - 1) / lc;
+  if (shape == 0)
+    hipLaunchKernelGGL((k_onebody_el<20, 3, CN>), dim3((a.nfe + 2) / 3, nl, a.nfun), dim3(64), 0, s, orb, h01, a, lc);
+  else if (shape == 1)
+    hipLaunchKernelGGL((k_onebody_el<20, 6, CN>), dim3((a.nfe + 5) / 6, nl, a.nfun), dim3(128), 0, s, orb, h01, a,
+                       lc);
+  else
+    hipLaunchKernelGGL((k_onebody_el<20, 12, CN>), dim3((a.nfe + 11) / 12, nl, a.nfun), dim3(256), 0, s, orb, h01,
+                       a, lc);
+}
+
+bool launch_onebody_el(const double2* orb, double2* h01, const OneBodyElArgs& a, hipStream_t s, int shape, int lc) {
   if (a.ndvr != 20) return false;
-  constexpr int GE = 256 / 21;
-  dim3 grid((a.nfe + GE - 1) / GE, (a.L1 + kOneBodyLChunk - 1) / kOneBodyLChunk, a.nfun);
-  hipLaunchKernelGGL((k_onebody_el<20>), grid, dim3(256), 0, s, orb, h01, a);
+  lc = std::max(1, std::min(lc, a.L1));
+  if (a.cn == 1)
+    launch_onebody_el_t<1>(orb, h01, a, s, shape, lc);
+  else if (a.cn == 2)
+    launch_onebody_el_t<2>(orb, h01, a, s, shape, lc);
+  else
+    launch_onebody_el_t<0>(orb, h01, a, s, shape, lc);
   return true;
 }
 
@@ -1613,17 +1680,20 @@ size_t poisson_fused_lds(const PoissonArgs& a) {
 }
 
 template <class T>
-void launch_poisson(const T* rho, double2* xb, T* v, const PoissonArgs& a, hipStream_t s) {
+void launch_poisson(const T* rho, double2* xb, T* v, const PoissonArgs& a, hipStream_t s, int part) {
   const int jobs = a.npair * a.L2;
   if (const size_t lds = poisson_fused_lds<T>(a)) {
     // bridge pass (rows staged in LDS), then the interiors with each row of
     // G_e shared by a group of pairs
-    if (a.ni == 19)
-      hipLaunchKernelGGL((k_poisson_bridge_lds<19, kPoissonBlock, T>), dim3(jobs), dim3(kPoissonBlock), lds, s, rho, v,
-                         xb, a);
-    else
-      hipLaunchKernelGGL((k_poisson_bridge_lds<0, kPoissonBlock, T>), dim3(jobs), dim3(kPoissonBlock), lds, s, rho, v,
-                         xb, a);
+    if (part != 2) {
+      if (a.ni == 19)
+        hipLaunchKernelGGL((k_poisson_bridge_lds<19, kPoissonBlock, T>), dim3(jobs), dim3(kPoissonBlock), lds, s, rho,
+                           v, xb, a);
+      else
+        hipLaunchKernelGGL((k_poisson_bridge_lds<0, kPoissonBlock, T>), dim3(jobs), dim3(kPoissonBlock), lds, s, rho,
+                           v, xb, a);
+    }
+    if (part == 1) return;
     const int epb = std::min(32, kPoissonInteriorThreads / a.ni);
     const int nchunk = (a.nfe + epb - 1) / epb;
     const int ngrp = (a.npair + kPoissonPairGroup - 1) / kPoissonPairGroup;
@@ -1636,15 +1706,18 @@ void launch_poisson(const T* rho, double2* xb, T* v, const PoissonArgs& a, hipSt
                          nchunk);
     return;
   }
-  if (a.ni == 19)
-    hipLaunchKernelGGL((k_poisson_bridge<19, T>), dim3(jobs), dim3(256), 0, s, rho, v, xb, a);
-  else
-    hipLaunchKernelGGL((k_poisson_bridge<0, T>), dim3(jobs), dim3(256), 0, s, rho, v, xb, a);
+  if (part != 2) {
+    if (a.ni == 19)
+      hipLaunchKernelGGL((k_poisson_bridge<19, T>), dim3(jobs), dim3(256), 0, s, rho, v, xb, a);
+    else
+      hipLaunchKernelGGL((k_poisson_bridge<0, T>), dim3(jobs), dim3(256), 0, s, rho, v, xb, a);
+  }
+  if (part == 1) return;
   const int nt = std::min(1024, (a.npair * a.ni + 63) / 64 * 64);
   hipLaunchKernelGGL((k_poisson_elem<T>), dim3(a.nfe, a.L2), dim3(std::max(nt, 64)), 0, s, rho, v, xb, a);
 }
-template void launch_poisson<double2>(const double2*, double2*, double2*, const PoissonArgs&, hipStream_t);
-template void launch_poisson<float2>(const float2*, double2*, float2*, const PoissonArgs&, hipStream_t);
+template void launch_poisson<double2>(const double2*, double2*, double2*, const PoissonArgs&, hipStream_t, int);
+template void launch_poisson<float2>(const float2*, double2*, float2*, const PoissonArgs&, hipStream_t, int);
 
 template <class T>
 void launch_mfprod(const T* orbg, const T* vg, T* gorbg, T* v2orbg, const MfArgs& a, hipStream_t s) {

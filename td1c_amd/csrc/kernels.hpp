@@ -16,7 +16,6 @@ constexpr int kMaxActPairs = kMaxAct * kMaxAct;
 constexpr int kMaxStages = 6;
 constexpr int kMaxBridge = 160;   // nfe - 1 bridges handled by the Poisson bridge kernel
 constexpr int kMaxPcrSteps = 8;   // ceil(log2(kMaxBridge))
-constexpr int kOneBodyLChunk = 2;  // l values per block of the element one-body kernel
 
 // Time of a device-driven propagation: step n runs at t0 + n * dt.
 struct PropClock {
@@ -58,6 +57,13 @@ struct OneBodyElArgs {
   const double* alph;
   const double2* tzor;          // [L1][nr] tmat diagonal without the kinetic part (+ v2jfc)
   const double* lf;             // device lfield[9]
+  // right-hand sides of the clh1itr half step (cn != 0): the block's orbital
+  // blockIdx.z is dynamical orbital fshift + blockIdx.z of the problem;
+  // cn = 1: out = psi + cv (T psi + [isplit = 1] V psi), cn = 2: out = cv V psi
+  // (returns at once when stop[0] is set)
+  int cn, fshift, isplit;
+  double2 cv;
+  const int* stop;
 };
 
 // One half step of clh1itr (split_type = 1; h1itr_prop,
@@ -91,7 +97,9 @@ constexpr int kCnInteriorThreads = 256;
 int cn_interior_blocks(const CnArgs& a);
 // mode 0: the first term from psi; mode 1: one more term of the series
 // (every kernel returns at once once flag[0] is set)
-void launch_cn_stage(const CnArgs& a, int mode, hipStream_t s);
+// ob: the element one-body arguments for the right-hand side (ndvr = 20),
+// with its launch shape; null for the band kernel
+void launch_cn_stage(const CnArgs& a, const OneBodyElArgs* ob, int shape, int lc, int mode, hipStream_t s);
 
 struct PoissonArgs {
   int nr, n, nfe, ni, L2, npair;
@@ -227,9 +235,12 @@ void launch_transform_t(int mode, const TI* in, TO* out, const double* mats, con
                         const TransformBatch* batch, int nbatch, int nr, int K, int C, int Cp, double2* copy,
                         hipStream_t s, int path = 0, const FcPhaseJob* fc = nullptr);
 void launch_onebody(const double2* orb, double2* h01, const OneBodyArgs& a, hipStream_t s);
-bool launch_onebody_el(const double2* orb, double2* h01, const OneBodyElArgs& a, hipStream_t s);
+// shape 0/1/2: 3/6/12 elements per block; lc: l values per block (any
+// shape gives the same bits; the autotuner picks one per problem)
+bool launch_onebody_el(const double2* orb, double2* h01, const OneBodyElArgs& a, hipStream_t s, int shape = 2,
+                       int lc = 2);
 template <class T>  // rho and v complex128 or complex64; the solve is FP64
-void launch_poisson(const T* rho, double2* xb, T* v, const PoissonArgs& a, hipStream_t s);
+void launch_poisson(const T* rho, double2* xb, T* v, const PoissonArgs& a, hipStream_t s, int part = 0);  // part 1 bridge pass, 2 interior pass, 0 both
 template <class T>  // theta-grid arrays complex128 or complex64 (arithmetic in the same)
 void launch_mfprod(const T* orbg, const T* vg, T* gorbg, T* v2orbg, const MfArgs& a, hipStream_t s);
 // out == nullptr: the partial sums only (partial[q * nblk + b], q = entry

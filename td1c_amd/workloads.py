@@ -12,7 +12,12 @@ Workloads (sizes as quoted for the hprod / RK4 benchmark):
              lmax1 63 / lmax2 126 (136 pair densities)
 Inputs are seeded (numpy PCG64, seed = index*1000 + 7): smooth radial
 shapes r^p exp(-zeta r) times complex l-coefficients with |c_l| ~ 2^-l,
-orthonormalised; frozen cores are pure hydrogenic 1s functions.
+orthonormalised; frozen cores are pure hydrogenic 1s functions.  For
+propagations at the production step (dt = 0.01 with split_type = 1),
+bound_wfn gives each l component a nodeless hydrogenic shape, r^(l+1) at
+the origin as a bound orbital (make_wfn's r^p for every l puts large
+one-body matrix elements <phi_j|h0|phi_i> into the split hprod and makes
+it stiff).
 """
 from __future__ import annotations
 
@@ -115,6 +120,52 @@ WORKLOADS = {"he_l15": (he_l15, 1), "ne_l31": (ne_l31, 2), "ar_l63": (ar_l63, 3)
 def ne_field() -> Field:
     """800 nm, 4-cycle sin^2 pulse at 1e14 W/cm^2, velocity gauge."""
     return Field(env_type="sin2", gauge="velocity1", fint=1.0e14, wlen=800.0, numcyc=4.0)
+
+
+def bound_orbitals(prob: Problem, seed: int) -> np.ndarray:
+    """Seeded orthonormal orbitals (nfun, L1, nr) whose l components are
+    c_l u_l(r), u_l = r^(l+1) exp(-zeta r) normalised on the grid (|c_l| ~
+    2^-l, zeta = Z / (1 + 0.75 k) for the k-th dynamical orbital): each
+    component lives where the orbital does, near r = (l+1) / zeta, and goes
+    as r^(l+1) at the origin, so the one-body operator stays of order
+    zeta^2 on them; frozen cores as in synth_orbitals."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    r, w = gll_grid(prob)
+    nr, L1, nf = prob.nr, prob.lmax1 + 1, prob.nfun
+    orbs = synth_orbitals(prob, seed)
+    mval = list(prob.mval) if prob.mval is not None else [0] * nf
+    ls = np.arange(L1)[:, None]
+    for f in range(prob.nfcore, nf):
+        k = f - prob.nfcore
+        zeta = max(0.5, prob.znuc / (1.0 + 0.75 * k)) if prob.znuc > 0 else 1.0
+        c = (rng.standard_normal(L1) + 1j * rng.standard_normal(L1)) * (2.0 ** -np.arange(L1))
+        c[:abs(mval[f])] = 0.0
+        # r^(l+1) exp(-zeta r) as exp((l+1) log (r / rp) - zeta (r - rp)), rp = (l+1) / zeta
+        # (1 at the peak: no overflow at l = 63), then normalised per l
+        rp = (ls + 1) / zeta
+        rad = np.exp((ls + 1) * np.log(r[None, :] / rp) - zeta * (r[None, :] - rp)) * np.sqrt(w)[None, :]
+        rad /= np.linalg.norm(rad, axis=1, keepdims=True)
+        orbs[f] = c[:, None] * rad
+    for f in range(prob.nfcore, nf):
+        v = orbs[f].ravel()
+        for g in range(f):
+            if mval[g] == mval[f]:
+                u = orbs[g].ravel()
+                v = v - u * np.vdot(u, v)
+        orbs[f] = (v / np.linalg.norm(v)).reshape(L1, nr)
+    return orbs
+
+
+def bound_wfn(prob: Problem, seed: int, lcic: int = 1) -> np.ndarray:
+    """bound_orbitals followed by a seeded normalised complex CI vector (1
+    for a single determinant)."""
+    orbs = bound_orbitals(prob, seed)
+    rng = np.random.Generator(np.random.PCG64(seed + 1))
+    cic = rng.standard_normal(lcic) + 1j * rng.standard_normal(lcic)
+    if lcic == 1:
+        cic[:] = 1.0
+    cic /= np.linalg.norm(cic)
+    return np.concatenate([orbs.ravel(), cic])
 
 
 def make_wfn(prob: Problem, seed: int, lcic: int = 1) -> np.ndarray:

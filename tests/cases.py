@@ -116,34 +116,10 @@ def he_scan_max(**kw):
 
 
 def smooth_wfn_for(prob: Problem, lcic: int, seed: int = 11) -> np.ndarray:
-    """Like wfn_for, but every l component of the dynamical orbitals goes
-    as r^(l+1) at the origin (c_l r^(l+1) exp(-zeta r), |c_l| ~ 2^-l), as a
-    bound orbital does.  The one-body operator is then bounded on the span
-    of the orbitals, so the split hprod (whose projector terms carry
-    <phi_j|h0|phi_i>) is not stiff and an explicit RK4 step of dt = 0.01 is
-    stable; wfn_for's r^p exp(-zeta r) for every l is not."""
-    rng = np.random.Generator(np.random.PCG64(seed))
-    r, w = W.gll_grid(prob)
-    nr, L1, nf = prob.nr, prob.lmax1 + 1, prob.nfun
-    orbs = W.synth_orbitals(prob, seed)
-    mval = list(prob.mval) if prob.mval is not None else [0] * nf
-    for f in range(prob.nfcore, nf):
-        k = f - prob.nfcore
-        zeta = max(0.5, prob.znuc / (1.0 + 0.75 * k)) if prob.znuc > 0 else 1.0
-        c = (rng.standard_normal(L1) + 1j * rng.standard_normal(L1)) * (2.0 ** -np.arange(L1))
-        c[:abs(mval[f])] = 0.0
-        ls = np.arange(L1)[:, None]
-        orbs[f] = c[:, None] * (r[None, :] ** (ls + 1)) * np.exp(-zeta * r)[None, :] * np.sqrt(w)[None, :]
-    for f in range(prob.nfcore, nf):
-        v = orbs[f].ravel()
-        for g in range(f):
-            if mval[g] == mval[f]:
-                u = orbs[g].ravel()
-                v = v - u * np.vdot(u, v)
-        orbs[f] = (v / np.linalg.norm(v)).reshape(L1, nr)
-    crng = np.random.Generator(np.random.PCG64(seed + 1))
-    cic = crng.standard_normal(lcic) + 1j * crng.standard_normal(lcic)
-    if lcic == 1:
-        cic[:] = 1.0
-    cic /= np.linalg.norm(cic)
-    return np.concatenate([orbs.ravel(), cic])
+    """workloads.bound_wfn: every l component of the dynamical orbitals goes
+    as r^(l+1) at the origin, as a bound orbital does.  The one-body
+    operator is then bounded on the span of the orbitals, so the split hprod
+    (whose projector terms carry <phi_j|h0|phi_i>) is not stiff and an
+    explicit RK4 step of dt = 0.01 is stable; wfn_for's r^p exp(-zeta r) for
+    every l is not."""
+    return W.bound_wfn(prob, seed, lcic)

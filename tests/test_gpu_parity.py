@@ -581,3 +581,35 @@ def test_two_contexts_interleaved(gpu_lib):
             got = out.cpu().numpy()
             assert _rel(got[:n], ref[:n]) < RTOL, (k, _rel(got[:n], ref[:n]))
             assert _rel(got[n:], ref[n:]) < 1e-10 or np.linalg.norm(ref[n:]) < 1e-12
+
+
+@pytest.mark.parametrize("name,lf", [("ne_small", C.LF_VEL), ("dc3_small", C.LF_LEN), ("cas_fc_small", C.LF_VEL)])
+def test_onebody_launch_shapes_agree(gpu_lib, name, lf):
+    """Every launch shape the autotuner may pick for the element one-body
+    kernel (3, 6, 12 elements per block; 1..16 l values per block) gives the
+    same hprod bit for bit, and the oracle's to the parity tolerance."""
+    import torch
+    import td1c_amd
+    from oracle.oracle import Oracle
+
+    prob = getattr(C, name)()
+    o = Oracle(prob)
+    w = C.wfn_for(prob, o.lcic)
+    o.set_wfn0(w)
+    ref = o.htot(1.0, lf, w)
+    hp = td1c_amd.HProd(prob)
+    hp.set_wfn0(w)
+    sh = hp.shapes()
+    assert sh["onebody_elements_per_block"] in (3, 6, 12)
+    d = hp.to_device(w)
+    base = None
+    for ge in (3, 6, 12):
+        for lc in (1, 2, 3, 4, 8, 16):
+            hp.set_onebody_shape(ge, lc)
+            got = hp.htot(1.0, lf, d).cpu().numpy()
+            torch.cuda.synchronize()
+            if base is None:
+                base = got
+                assert _rel(got, ref) < RTOL
+            else:
+                assert np.array_equal(got, base), (ge, lc)

@@ -18,7 +18,7 @@ def _rel(a, b):
 
 
 CASES = [("he_small", C.LF_LEN), ("dc3_small", C.LF_LEN), ("ne_small", C.LF_VEL), ("cas_fc_small", C.LF_VEL),
-         ("cas_small", C.LF_LEN)]
+         ("cas_small", C.LF_LEN), ("he_n16", C.LF_VEL)]  # ndvr = 16: the band right-hand-side kernel
 
 
 @pytest.mark.parametrize("name,lf", CASES)

@@ -937,3 +937,20 @@ def test_ladapt_post_op(lconst):
                 orb[f, l] = 0
     assert np.array_equal(h1, exp)
     assert np.abs(h0 - exp).max() > 1e-3
+
+
+@pytest.mark.parametrize("name,lf", [("dc3_small", [0, 0, 0.05, 0, 0, 0.05, 0, 0, 0]),
+                                     ("ne_small", [0, 0, 0.3, 0, 0, 0.04, 0, 0, 0.3]),
+                                     ("cas_small", [0, 0, 0.05, 0, 0, 0.05, 0, 0, 0])])
+def test_openmp_oracle_is_bit_identical(name, lf):
+    """The OpenMP build (the benchmark's CPU baseline) threads over
+    independent output rows only: its hprod equals the serial oracle's bit
+    for bit."""
+    from tests import cases as C
+
+    prob = getattr(C, name)()
+    a, b = O.Oracle(prob), O.Oracle(prob, openmp=True)
+    w = C.wfn_for(prob, a.lcic)
+    a.set_wfn0(w)
+    b.set_wfn0(w)
+    np.testing.assert_array_equal(a.htot(1.0, np.array(lf, float), w), b.htot(1.0, np.array(lf, float), w))
