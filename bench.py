@@ -282,14 +282,20 @@ def main():
     mixed = measure(1)
     progress(f"mixed path {mixed['ms_step']:.3f} ms/step")
 
-    # hprod residual on one state: the split hprod of both paths at t0
+    # hprod residual on one state: the split hprod of both paths on the
+    # initial state with the field at the pulse peak
     lf0 = td1c_amd.field_value(field, 0.0)
+    lfp = td1c_amd.field_value(field, 0.5 * field.numcyc * td1c_amd.field_period(field))
     hps = {0: fp64["hp"], 1: mixed["hp"]}
     ks = {}
     for p in (0, 1):
         dw = hps[p].to_device(fp64["wfn"])
-        ks[p] = hps[p].htot(1.0, lf0, dw).cpu().numpy()
+        hw = hps[p].htot(1.0, lfp, dw)
+        torch.cuda.synchronize()
+        ks[p] = hw.cpu().numpy().copy()
     residual = float(np.linalg.norm(ks[1] - ks[0]) / np.linalg.norm(ks[0]))
+    progress(f"hprod residual {residual:.3e} (norms {np.linalg.norm(ks[0]):.6e}, {np.linalg.norm(ks[1]):.6e}; "
+             f"precisions {fp64['prob'].precision}, {mixed['prob'].precision})")
 
     # dipole(t) and final ionisation probabilities, mixed against FP64
     gate = None

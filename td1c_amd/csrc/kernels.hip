@@ -275,6 +275,94 @@ __global__ void __launch_bounds__(256) k_transform_mfma(const TI* __restrict__ i
   }
 }
 
+// The matrix-core transform with register blocking over columns: a
+// wavefront computes 16 radial rows x 16 CT columns, so that each input
+// value it loads (the MFMA B operand) feeds 2 CT MFMAs.  Block: four
+// wavefronts on consecutive row tiles sharing the same columns (their A
+// operands hit the same L1 lines).  Each output element is accumulated in
+// the same order as in k_transform_mfma, so both kernels give the same bits.
+template <int CT, class TI, class TO>
+__global__ void __launch_bounds__(256) k_transform_mfma_w(const TI* __restrict__ in, TO* __restrict__ out,
+                                                          const double* __restrict__ mats,
+                                                          const float* __restrict__ matsf,
+                                                          const TransformBatch* __restrict__ batch, int nr, int K,
+                                                          int C, int Cp, double2* __restrict__ copy, int nx, int ny,
+                                                          int ntile, const FcPhaseJob fc) {
+  using A2 = typename std::conditional<std::is_same<TI, float2>::value && std::is_same<TO, float2>::value, float2,
+                                       double2>::type;
+  using AR = typename CReal<A2>::T;
+  using MF = Mfma16<AR>;
+  using V4 = typename MF::V4;
+  if ((int)blockIdx.x >= ntile) {
+    const int q = (int)blockIdx.x - ntile, rb = (fc.nr + blockDim.x - 1) / blockDim.x;
+    fc_phase_point<TO>(fc, (q % rb) * blockDim.x + threadIdx.x, q / rb);
+    return;
+  }
+  const int t = xcd_remap(blockIdx.x, ntile);
+  const int by = t % ny, bx = (t / ny) % nx, bz = t / (ny * nx);
+  const TransformBatch bt = batch[bz];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int c0 = by * 16 * CT;                   // first column of this wavefront's tiles
+  const int nct = min(CT, (C - c0 + 15) >> 4);   // its column tiles (wave-uniform)
+  const int r0 = (bx * 4 + wave) * 16;
+  const int kq = lane >> 4;
+  const int r = r0 + (lane & 15);
+  const bool row_ok = r < bt.rows;
+  const AR* __restrict__ M;
+  if constexpr (std::is_same<AR, float>::value) M = matsf + (size_t)bt.mat * K * Cp;
+  else M = mats + (size_t)bt.mat * K * Cp;
+  const AR* __restrict__ Mc = M + c0 + (lane & 15);
+  V4 dre[CT], dim[CT];
+#pragma unroll
+  for (int q = 0; q < CT; ++q) {
+    dre[q] = V4{0, 0, 0, 0};
+    dim[q] = V4{0, 0, 0, 0};
+  }
+  constexpr int U = 4;  // contraction steps per load batch
+  const int nks = (K + 3) >> 2;
+  for (int s0 = 0; s0 < nks; s0 += U) {
+    AR a[U][CT];
+    A2 x[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = 4 * (s0 + u) + kq;
+      const bool kok = k < K;
+#pragma unroll
+      for (int q = 0; q < CT; ++q) a[u][q] = (kok && q < nct) ? Mc[(size_t)k * Cp + 16 * q] : AR(0);
+      x[u] = cz<A2>();
+      if (kok && row_ok) {
+        const size_t g = (size_t)k * nr + r;
+        const TI xv = in[(size_t)bt.in0 + g];
+        x[u] = cto<A2>(xv);
+        if constexpr (std::is_same<TI, double2>::value)
+          if (copy != nullptr && by == 0) copy[(size_t)bt.in0 + g] = xv;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int q = 0; q < CT; ++q) {
+        dre[q] = MF::run(a[u][q], x[u].x, dre[q]);
+        dim[q] = MF::run(a[u][q], x[u].y, dim[q]);
+      }
+  }
+  if (row_ok) {
+#pragma unroll
+    for (int q = 0; q < CT; ++q) {
+      if (q >= nct) break;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int c = c0 + 16 * q + MF::row(lane, v);
+        A2 o;
+        o.x = dre[q][v];
+        o.y = dim[q][v];
+        if (c < C) out[(size_t)bt.out + (size_t)c * nr + r] = cto<TO>(o);
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // One-body part h01 = (T_l + h1) psi (hprod_tprodp, src/hprod/hprod_tprod.F90:152-235;
 // hprod_zprodp, hprod_zprod.F90:108-135; hprod_pzprodp, hprod_pzprod.F90:114-213).
@@ -1372,9 +1460,20 @@ void launch_transform_t(int mode, const TI* in, TO* out, const double* mats, con
     if (fc != nullptr) launch_fc_phase<TO>(*fc, s);
     return;
   }
+  const FcPhaseJob fj = fc != nullptr ? *fc : FcPhaseJob{};
+  if (ct >= 2 && mode == 0) {
+    // register-blocked form: 64 rows x 64 columns per block, 4 column tiles
+    // per wavefront (for the pair products, mode 1, the form below, which
+    // forms them once per block in LDS, measured faster)
+    constexpr int CT = 4;
+    const int nx = (nr + 63) / 64, ny = (ct + CT - 1) / CT, ntile = nx * ny * nbatch;
+    const int nfcb = fc != nullptr ? ((fc->nr + 255) / 256) * fc->nlat : 0;
+    hipLaunchKernelGGL((k_transform_mfma_w<CT, TI, TO>), dim3(ntile + nfcb), dim3(256), 0, s, in, out, mats, matsf,
+                       batch, nr, K, C, Cp, copy, nx, ny, ntile, fj);
+    return;
+  }
   const int nx = (nr + 15) / 16, ny = (ct + wpb - 1) / wpb, ntile = nx * ny * nbatch;
   const int nfcb = fc != nullptr ? ((fc->nr + 64 * wpb - 1) / (64 * wpb)) * fc->nlat : 0;
-  const FcPhaseJob fj = fc != nullptr ? *fc : FcPhaseJob{};
   if (mode == 0)
     hipLaunchKernelGGL((k_transform_mfma<0, TI, TO>), dim3(ntile + nfcb), dim3(64 * wpb), 0, s, in, out, mats, matsf,
                        batch, nr, K, C, Cp, copy, nx, ny, ntile, fj);
