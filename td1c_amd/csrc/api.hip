@@ -154,6 +154,11 @@ struct td1c_ctx {
   // launch shape of the element one-body kernel (autotuned per problem):
   // elements per block (0: 3, 1: 6, 2: 12) and l values per block
   int ob_shape = 2, ob_lc = 2;
+  // launch shape of the register-blocked matrix-core transforms (0..5;
+  // TD1C_TVAR in the environment overrides it): 1 = 2 column tiles per
+  // wavefront, 4 contraction steps per load batch, measured best on the Ar
+  // workload among (4,4) (2,4) (4,8) (8,4) (8,2) (2,8)
+  int tvar = 1;
   std::string tune_log;  // what the autotuner measured (JSON)
   // device-driven propagation: one captured RK step, replayed per step
   hipStream_t cap_stream = nullptr;
@@ -638,10 +643,10 @@ void enqueue_orbin_theta(td1c_ctx* c, const double* d_lf, const double2* wfn, hi
     TIMED("l2theta");
     if (c->mixed)
       launch_transform_t<double2, float2>(0, wfn, c->orbg_f, c->m_l2g.p, nullptr, c->b_l2g_dyn.p, c->n_l2g_dyn, nr, L1,
-                                          nl, pad_cols(nl), c->orb, s, c->tpath, fc_vel ? &fj : nullptr);
+                                          nl, pad_cols(nl), c->orb, s, c->tpath, fc_vel ? &fj : nullptr, c->tvar);
     else
       launch_transform_t<double2, double2>(0, wfn, c->orbg, c->m_l2g.p, nullptr, c->b_l2g_dyn.p, c->n_l2g_dyn, nr, L1,
-                                           nl, pad_cols(nl), c->orb, s, c->tpath, fc_vel ? &fj : nullptr);
+                                           nl, pad_cols(nl), c->orb, s, c->tpath, fc_vel ? &fj : nullptr, c->tvar);
   }
 }
 
@@ -653,10 +658,10 @@ void enqueue_orbin_fc_l(td1c_ctx* c, hipStream_t s) {
     TIMED_ON(s, "theta2l_fc");
     if (c->mixed)
       launch_transform_t<float2, double2>(0, c->orbg_f, c->orb, c->m_g2l.p, nullptr, c->b_g2l_fc.p, c->n_g2l_fc, nr, nl,
-                                          L1, pad_cols(L1), nullptr, s, c->tpath);
+                                          L1, pad_cols(L1), nullptr, s, c->tpath, nullptr, c->tvar);
     else
       launch_transform(0, c->orbg, c->orb, c->m_g2l.p, c->b_g2l_fc.p, c->n_g2l_fc, nr, nl, L1, pad_cols(L1), nullptr, s,
-                       c->tpath);
+                       c->tpath, nullptr, c->tvar);
   }
 }
 
@@ -718,10 +723,10 @@ int enqueue_htotx(td1c_ctx* c, double dtime, const double* d_lf, const double2* 
     if (c->mixed) {
       TIMED("pair_theta2L");
       launch_transform_t<float2, float2>(1, c->orbg_f, c->rhoL_f.p, c->m_g2L.p, c->m_g2L_f.p, c->b_rho.p, c->npair, nr,
-                                         nl, L2, pad_cols(L2), nullptr, s, c->tpath);
+                                         nl, L2, pad_cols(L2), nullptr, s, c->tpath, nullptr, c->tvar);
     } else {
       TIMED("pair_theta2L");
-      launch_transform(1, c->orbg, c->rhoL.p, c->m_g2L.p, c->b_rho.p, c->npair, nr, nl, L2, pad_cols(L2), nullptr, s, c->tpath);
+      launch_transform(1, c->orbg, c->rhoL.p, c->m_g2L.p, c->b_rho.p, c->npair, nr, nl, L2, pad_cols(L2), nullptr, s, c->tpath, nullptr, c->tvar);
     }
     PoissonArgs pa{};
     pa.nr = nr; pa.n = S.n; pa.nfe = S.nfe; pa.ni = S.ni; pa.L2 = L2; pa.npair = c->npair;
@@ -739,14 +744,14 @@ int enqueue_htotx(td1c_ctx* c, double dtime, const double* d_lf, const double2* 
       {
         TIMED("pair_L2theta");
         launch_transform_t<float2, float2>(0, c->vL_f.p, c->vg_f.p, c->m_L2g.p, c->m_L2g_f.p, c->b_L2g.p, c->npair, nr,
-                                           L2, nl, pad_cols(nl), nullptr, s, c->tpath);
+                                           L2, nl, pad_cols(nl), nullptr, s, c->tpath, nullptr, c->tvar);
       }
     } else {
       { TIMED("poisson_bridge"); launch_poisson(c->rhoL.p, c->p_xb.p, c->vL.p, pa, s, 1); }
       { TIMED("poisson_interior"); launch_poisson(c->rhoL.p, c->p_xb.p, c->vL.p, pa, s, 2); }
       HIP_TRY(enqueue_pair_gram(c, s, s1));
       pair_gram_done = true;
-      { TIMED("pair_L2theta"); launch_transform(0, c->vL.p, c->vg.p, c->m_L2g.p, c->b_L2g.p, c->npair, nr, L2, nl, pad_cols(nl), nullptr, s, c->tpath); }
+      { TIMED("pair_L2theta"); launch_transform(0, c->vL.p, c->vg.p, c->m_L2g.p, c->b_L2g.p, c->npair, nr, L2, nl, pad_cols(nl), nullptr, s, c->tpath, nullptr, c->tvar); }
     }
     MfArgs ma{};
     ma.nr = nr; ma.nlat = nl; ma.nfun = nf; ma.nfcore = nfc; ma.nfcore2 = S.nfcore2; ma.ncore = S.ncore; ma.nact = S.nact;
@@ -760,11 +765,11 @@ int enqueue_htotx(td1c_ctx* c, double dtime, const double* d_lf, const double2* 
       {
         TIMED("theta2l_mf");
         launch_transform_t<float2, double2>(0, c->gorbg_f, c->gorb, c->m_g2l.p, nullptr, c->b_g2l_mf.p, c->n_g2l_mf, nr,
-                                            nl, L1, pad_cols(L1), nullptr, s, c->tpath);
+                                            nl, L1, pad_cols(L1), nullptr, s, c->tpath, nullptr, c->tvar);
       }
     } else {
       { TIMED("mfprod"); launch_mfprod(c->orbg, c->vg.p, c->gorbg, c->v2orbg, ma, s); }
-      { TIMED("theta2l_mf"); launch_transform(0, c->gorbg, c->gorb, c->m_g2l.p, c->b_g2l_mf.p, c->n_g2l_mf, nr, nl, L1, pad_cols(L1), nullptr, s, c->tpath); }
+      { TIMED("theta2l_mf"); launch_transform(0, c->gorbg, c->gorb, c->m_g2l.p, c->b_g2l_mf.p, c->n_g2l_mf, nr, nl, L1, pad_cols(L1), nullptr, s, c->tpath, nullptr, c->tvar); }
     }
   } else {
     launch_zero(c->gorb, 2 * nb * c->nd, s);
@@ -1078,6 +1083,7 @@ int td1c_ctx_create(const td1c_problem* prob, td1c_ctx** out) {
   if (rc) return rc;
   rc = autotune_onebody(c.get());
   if (rc) return rc;
+  if (const char* tv = std::getenv("TD1C_TVAR")) c->tvar = std::max(0, std::min(5, std::atoi(tv)));
   *out = c.release();
   return 0;
 }

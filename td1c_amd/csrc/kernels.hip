@@ -281,7 +281,7 @@ __global__ void __launch_bounds__(256) k_transform_mfma(const TI* __restrict__ i
 // wavefronts on consecutive row tiles sharing the same columns (their A
 // operands hit the same L1 lines).  Each output element is accumulated in
 // the same order as in k_transform_mfma, so both kernels give the same bits.
-template <int CT, class TI, class TO>
+template <int CT, int U, class TI, class TO>
 __global__ void __launch_bounds__(256) k_transform_mfma_w(const TI* __restrict__ in, TO* __restrict__ out,
                                                           const double* __restrict__ mats,
                                                           const float* __restrict__ matsf,
@@ -319,8 +319,7 @@ __global__ void __launch_bounds__(256) k_transform_mfma_w(const TI* __restrict__
     dre[q] = V4{0, 0, 0, 0};
     dim[q] = V4{0, 0, 0, 0};
   }
-  constexpr int U = 4;  // contraction steps per load batch
-  const int nks = (K + 3) >> 2;
+  const int nks = (K + 3) >> 2;  // contraction steps, U per load batch
   for (int s0 = 0; s0 < nks; s0 += U) {
     AR a[U][CT];
     A2 x[U];
@@ -1438,7 +1437,7 @@ constexpr long long kMfmaMinWaves = 1200;  // wavefronts of the MFMA form below 
 template <class TI, class TO>
 void launch_transform_t(int mode, const TI* in, TO* out, const double* mats, const float* matsf,
                         const TransformBatch* batch, int nbatch, int nr, int K, int C, int Cp, double2* copy,
-                        hipStream_t s, int path, const FcPhaseJob* fc) {
+                        hipStream_t s, int path, const FcPhaseJob* fc, int tvar) {
   // matrix cores (16 radial rows per block, one wavefront per 16 columns)
   // when that gives enough wavefronts to fill the chip and at least three
   // column tiles share each input load; otherwise the vector form, whose
@@ -1462,14 +1461,24 @@ void launch_transform_t(int mode, const TI* in, TO* out, const double* mats, con
   }
   const FcPhaseJob fj = fc != nullptr ? *fc : FcPhaseJob{};
   if (ct >= 2 && mode == 0) {
-    // register-blocked form: 64 rows x 64 columns per block, 4 column tiles
-    // per wavefront (for the pair products, mode 1, the form below, which
-    // forms them once per block in LDS, measured faster)
-    constexpr int CT = 4;
-    const int nx = (nr + 63) / 64, ny = (ct + CT - 1) / CT, ntile = nx * ny * nbatch;
+    // register-blocked form: 64 rows x 16 CT columns per block, CT column
+    // tiles per wavefront, U contraction steps per load batch (for the pair
+    // products, mode 1, the form below, which forms them once per block in
+    // LDS, measured faster)
     const int nfcb = fc != nullptr ? ((fc->nr + 255) / 256) * fc->nlat : 0;
-    hipLaunchKernelGGL((k_transform_mfma_w<CT, TI, TO>), dim3(ntile + nfcb), dim3(256), 0, s, in, out, mats, matsf,
-                       batch, nr, K, C, Cp, copy, nx, ny, ntile, fj);
+    auto go = [&](auto kern, int CT) {
+      const int nx = (nr + 63) / 64, ny = (ct + CT - 1) / CT, ntile = nx * ny * nbatch;
+      hipLaunchKernelGGL(kern, dim3(ntile + nfcb), dim3(256), 0, s, in, out, mats, matsf, batch, nr, K, C, Cp, copy,
+                         nx, ny, ntile, fj);
+    };
+    switch (tvar) {
+      case 1: go(k_transform_mfma_w<2, 4, TI, TO>, 2); break;
+      case 2: go(k_transform_mfma_w<4, 8, TI, TO>, 4); break;
+      case 3: go(k_transform_mfma_w<8, 4, TI, TO>, 8); break;
+      case 4: go(k_transform_mfma_w<8, 2, TI, TO>, 8); break;
+      case 5: go(k_transform_mfma_w<2, 8, TI, TO>, 2); break;
+      default: go(k_transform_mfma_w<4, 4, TI, TO>, 4); break;
+    }
     return;
   }
   const int nx = (nr + 15) / 16, ny = (ct + wpb - 1) / wpb, ntile = nx * ny * nbatch;
@@ -1483,18 +1492,19 @@ void launch_transform_t(int mode, const TI* in, TO* out, const double* mats, con
 }
 template void launch_transform_t<double2, float2>(int, const double2*, float2*, const double*, const float*,
                                                   const TransformBatch*, int, int, int, int, int, double2*,
-                                                  hipStream_t, int, const FcPhaseJob*);
+                                                  hipStream_t, int, const FcPhaseJob*, int);
 template void launch_transform_t<float2, float2>(int, const float2*, float2*, const double*, const float*,
                                                  const TransformBatch*, int, int, int, int, int, double2*,
-                                                 hipStream_t, int, const FcPhaseJob*);
+                                                 hipStream_t, int, const FcPhaseJob*, int);
 template void launch_transform_t<float2, double2>(int, const float2*, double2*, const double*, const float*,
                                                   const TransformBatch*, int, int, int, int, int, double2*,
-                                                  hipStream_t, int, const FcPhaseJob*);
+                                                  hipStream_t, int, const FcPhaseJob*, int);
 
 void launch_transform(int mode, const double2* in, double2* out, const double* mats, const TransformBatch* batch,
                       int nbatch, int nr, int K, int C, int Cp, double2* copy, hipStream_t s, int path,
-                      const FcPhaseJob* fc) {
-  launch_transform_t<double2, double2>(mode, in, out, mats, nullptr, batch, nbatch, nr, K, C, Cp, copy, s, path, fc);
+                      const FcPhaseJob* fc, int tvar) {
+  launch_transform_t<double2, double2>(mode, in, out, mats, nullptr, batch, nbatch, nr, K, C, Cp, copy, s, path, fc,
+                                       tvar);
 }
 
 // shape: 0 = 3 elements per block (one wavefront), 1 = 6 (128 threads),

@@ -224,16 +224,17 @@ struct FcPhaseJob {
 };
 
 // path: 0 by grid size, 1 the vector form, 2 the matrix-core (MFMA) form
+// tvar: launch shape of the register-blocked matrix-core form (0..5, same bits)
 void launch_transform(int mode, const double2* in, double2* out, const double* mats, const TransformBatch* batch,
                       int nbatch, int nr, int K, int C, int Cp, double2* copy, hipStream_t s, int path = 0,
-                      const FcPhaseJob* fc = nullptr);
+                      const FcPhaseJob* fc = nullptr, int tvar = 0);
 inline int pad_cols(int C) { return (C + 31) / 32 * 32; }
 // the same transform with complex64 on either side (matsf: the matrices in
 // FP32, used when both sides are complex64)
 template <class TI, class TO>
 void launch_transform_t(int mode, const TI* in, TO* out, const double* mats, const float* matsf,
                         const TransformBatch* batch, int nbatch, int nr, int K, int C, int Cp, double2* copy,
-                        hipStream_t s, int path = 0, const FcPhaseJob* fc = nullptr);
+                        hipStream_t s, int path = 0, const FcPhaseJob* fc = nullptr, int tvar = 0);
 void launch_onebody(const double2* orb, double2* h01, const OneBodyArgs& a, hipStream_t s);
 // shape 0/1/2: 3/6/12 elements per block; lc: l values per block (any
 // shape gives the same bits; the autotuner picks one per problem)
