@@ -104,6 +104,15 @@ def ne_l31(**kw) -> Problem:
                    nel_alpha=5, nel_beta=5, lval=[0, 0, 0, 0, 0], **kw)
 
 
+def ne_l31_cas(**kw) -> Problem:
+    """The Ne grid with a correlated active space: frozen 1s + CAS(8e, 6o)
+    (225 determinants), so that the RDMs and the CI derivative are not those
+    of one determinant (the 4-orbital Ne case fills its active space)."""
+    kw.setdefault("igauge", 1)
+    return Problem(nfe=80, ndvr=20, rmax=160.0, lmax1=31, lmax2=62, znuc=10.0, nfcore1=1, nact=6,
+                   nel_alpha=5, nel_beta=5, lval=[0] * 7, **kw)
+
+
 def ar_l63(**kw) -> Problem:
     return Problem(nfe=120, ndvr=20, rmax=240.0, lmax1=63, lmax2=126, znuc=18.0, ndcore=9,
                    nel_alpha=9, nel_beta=9, **kw)

@@ -613,3 +613,45 @@ def test_onebody_launch_shapes_agree(gpu_lib, name, lf):
                 assert _rel(got, ref) < RTOL
             else:
                 assert np.array_equal(got, base), (ge, lc)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("name", ["ne_l31_cas", "ar_l63_split"])
+def test_full_size_split_step_against_golden(gpu_lib, name, precision):
+    """Full-size production step against the oracle's fixtures
+    (tests/golden/make_full_size.py): the correlated Ne workload (frozen 1s +
+    CAS(8e, 6o), 225 determinants) and the Ar workload with split_type = 1,
+    one hprod and one tdse_split2 step at dt = 0.01 in the velocity gauge
+    (td1c_propagate, device fields).  FP64: hprod to 1e-11 and the step to
+    1e-10 (relative, through the projections); mixed precision (complex64 mean
+    field): 1e-6 for both, the hprod residual of the mixed path."""
+    import json
+    import os
+    import torch
+    import td1c_amd
+    import tests.golden.make_full_size as G
+    from td1c_amd import workloads as W
+
+    with open(os.path.join(os.path.dirname(__file__), "golden", "full_size.json")) as f:
+        ref = json.load(f)[name]
+    prob0, seed, fint, t0 = G.split_problem(name)
+    prob = getattr(W, G.SPLIT_CASES[name][0])(isplit=1, igauge=1, precision=precision)
+    hp = td1c_amd.HProd(prob)
+    w = W.bound_wfn(prob, seed, hp.lcic)
+    hp.set_wfn0(w)
+    print(f"full size split {name} precision {precision}", flush=True)
+    q1, st, q3 = G.field_values(fint, t0, G.SPLIT_DT)
+    h = hp.htot(1.0, st[0], hp.to_device(w)).cpu().numpy()
+    ene = hp.energies()  # of this hprod (the step's hprods overwrite them)
+    d = hp.to_device(w)
+    td1c_amd.CRK(4).propagate(hp, d, t0, G.SPLIT_DT, 1, G.split_field(fint), apply_mask=True)
+    torch.cuda.synchronize()
+    w1 = d.cpu().numpy()
+    tol = {"hprod": 1e-11, "step": 1e-10} if precision == 0 else {"hprod": 1e-6, "step": 1e-6}
+    for part, x in (("hprod", h), ("step", w1)):
+        r = ref[part]
+        assert abs(np.linalg.norm(x) - r["norm"]) <= tol[part] * r["norm"], part
+        for k, u in enumerate(G.projection_vectors(x.size)):
+            p = np.vdot(u, x)
+            assert abs(p - complex(r["proj_re"][k], r["proj_im"][k])) <= tol[part] * r["norm"], (part, k, p)
+    np.testing.assert_allclose(ene, ref["energies"], rtol=1e-11 if precision == 0 else 1e-7, atol=1e-10)

@@ -891,7 +891,7 @@ def test_frozen_core_energy(igauge):
 
 def test_full_size_fixtures_match_oracle():
     """tests/golden/full_size.json (written by tests/golden/make_full_size.py)
-    still holds what the oracle computes for those inputs (~40 s)."""
+    still holds what the oracle computes for those inputs (~1 min)."""
     import json
     import os
 
@@ -911,6 +911,15 @@ def test_full_size_fixtures_match_oracle():
         for k, u in enumerate(G.projection_vectors(h.size)):
             assert abs(np.vdot(u, h) - complex(r["proj_re"][k], r["proj_im"][k])) <= 1e-13 * r["norm"]
         np.testing.assert_allclose(o.energies(), r["energies"], rtol=1e-13, atol=1e-13)
+    for name in G.SPLIT_CASES:
+        got = G.split_reference(name)  # OpenMP build: the serial oracle's bits
+        r = ref[name]
+        assert got["h1itr_cycles"] == r["h1itr_cycles"]
+        for part in ("hprod", "step"):
+            assert abs(got[part]["norm"] - r[part]["norm"]) <= 1e-13 * r[part]["norm"]
+            for k in range(len(r[part]["proj_re"])):
+                assert abs(complex(got[part]["proj_re"][k], got[part]["proj_im"][k])
+                           - complex(r[part]["proj_re"][k], r[part]["proj_im"][k])) <= 1e-13 * r[part]["norm"]
 
 
 @pytest.mark.parametrize("lconst", [1, 2])
