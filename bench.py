@@ -348,6 +348,32 @@ def main():
                 "ipx_max_abs_err": float(np.abs(np.array(ipx[1]) - np.array(ipx[0])).max()),
                 "ipx_radius": 20.0}
 
+    # the host-pointer entry point hprod_htot_ (wave function copied in and
+    # the derivative copied out on every call, pageable host memory)
+    host_entry = None
+    if not args.no_extra:
+        import ctypes
+
+        hp = fp64["hp"]
+        hp.bind()
+        lib = td1c_amd.load_library()
+        w0 = fp64["wfn"]
+        orb = np.ascontiguousarray(w0[:hp.norb])
+        cic = np.ascontiguousarray(w0[hp.norb:])
+        hw, hc = np.zeros_like(orb), np.zeros_like(cic)
+        dtc = ctypes.c_double(1.0)
+        lfh = np.ascontiguousarray(lfp)
+        ptrs = [a.ctypes.data_as(ctypes.c_void_p) for a in (lfh, orb, cic, hw, hc)]
+        for _ in range(2):
+            lib.hprod_htot_(ctypes.byref(dtc), *ptrs)
+        nh, t0 = 10, time.perf_counter()
+        for _ in range(nh):
+            lib.hprod_htot_(ctypes.byref(dtc), *ptrs)
+        host_entry = {"hprod_evals_per_s": nh / (time.perf_counter() - t0),
+                      "bytes_over_pcie_per_call": 2 * (orb.nbytes + cic.nbytes),
+                      "entry": "hprod_htot_ (host arrays in pageable memory, synchronous)"}
+        progress(f"host entry {host_entry['hprod_evals_per_s']:.1f} hprod evals/s")
+
     # the 16-orbital mean-field workload: hprod rates and residual
     extra = None
     if not args.no_extra:
@@ -445,6 +471,7 @@ def main():
         "mixed_kernel_ms_per_step": mkms,
         "norm_after": fp64["norm"],
         "mean_field_stress": extra,
+        "host_entry": host_entry,
     }
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(fp64["prob"], fp64["wfn"], lf0, args.cpu_seconds, args.workload)
