@@ -322,6 +322,43 @@ int td1c_propagate(td1c_ctx* ctx, const td1c_field* field, int rk_order,
                    double t0, double dtime, int nstep, double* d_wfn,
                    int apply_mask, void* stream);
 
+/* Batches of independent propagations (an intensity / wavelength / CEP scan
+ * over grids and lmax): the reference runs one tdse_split2 time loop
+ * (src/main/tdse.cpp:225-266) per problem, each in its own process; a batch
+ * runs the same steps for many problems at once, each member's kernels
+ * issued as one launch per kernel site for the whole batch.  Members are
+ * contexts the caller created (and keeps; destroy the batch first) on one
+ * device with ndvr = 20, dynamical cores only (no frozen cores, no active
+ * orbitals), and the same nfun, precision, split_type, gauge and icomp;
+ * their grids, lmax, nuclear charges and pulses may differ.  The batch
+ * uses the matrix-core form of every transform (td1c_set_transform_path 2
+ * gives a member's own td1c_propagate the same bits). */
+typedef struct td1c_batch td1c_batch;
+int td1c_batch_create(td1c_ctx* const* members, int n, td1c_batch** out);
+int td1c_batch_destroy(td1c_batch* batch);
+
+/* nstep steps of every member from t0: member m's wave function d_wfns[m]
+ * (device) under fields[m], exactly as td1c_propagate(members[m],
+ * &fields[m], rk_order, t0, dtime, nstep, d_wfns[m], apply_mask, stream)
+ * would step it.  The first call for a given set of arguments records one
+ * step of every member and uploads the launch table; with graphs on (the
+ * default) one step of the whole batch is captured once and replayed per
+ * step.  Enqueued on `stream`, asynchronous. */
+int td1c_batch_propagate(td1c_batch* batch, const td1c_field* fields, int rk_order, double t0, double dtime,
+                         int nstep, double* const* d_wfns, int apply_mask, void* stream);
+
+/* 1 (default): one captured HIP graph per batch step; 0: stream launches. */
+int td1c_batch_set_graph(td1c_batch* batch, int on);
+
+/* Launch shapes of the batch: the element one-body kernel's elements per
+ * block (3, 6, 12) and l values per block, and the register-blocked
+ * transform's variant (0: 4 column tiles, 1: 2 column tiles per wavefront);
+ * every choice gives the same bits. */
+int td1c_batch_set_shapes(td1c_batch* batch, int elements_per_block, int l_per_block, int transform_variant);
+
+/* JSON: members, launch sites and blocks per step, table bytes, shapes. */
+int td1c_batch_get_info(const td1c_batch* batch, char* buf, int buflen);
+
 /* How td1c_propagate launches its steps: 0 (default) enqueues every kernel
  * of every step on the stream, so that the independent branches of an hprod
  * (CI density stage, one-body part, pair Gram products) run concurrently on

@@ -118,6 +118,19 @@ def load_library():
     L.td1c_set_onebody_shape.restype = ctypes.c_int
     L.td1c_get_shapes.argtypes = [vp, ctypes.c_char_p, ctypes.c_int]
     L.td1c_get_shapes.restype = ctypes.c_int
+    L.td1c_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.POINTER(vp)]
+    L.td1c_batch_create.restype = ctypes.c_int
+    L.td1c_batch_destroy.argtypes = [vp]
+    L.td1c_batch_destroy.restype = ctypes.c_int
+    L.td1c_batch_propagate.argtypes = [vp, ctypes.POINTER(TD1CField), ctypes.c_int, ctypes.c_double,
+                                       ctypes.c_double, ctypes.c_int, ctypes.POINTER(vp), ctypes.c_int, vp]
+    L.td1c_batch_propagate.restype = ctypes.c_int
+    L.td1c_batch_set_graph.argtypes = [vp, ctypes.c_int]
+    L.td1c_batch_set_graph.restype = ctypes.c_int
+    L.td1c_batch_set_shapes.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.td1c_batch_set_shapes.restype = ctypes.c_int
+    L.td1c_batch_get_info.argtypes = [vp, ctypes.c_char_p, ctypes.c_int]
+    L.td1c_batch_get_info.restype = ctypes.c_int
     L.td1c_last_error.argtypes = []
     L.td1c_last_error.restype = ctypes.c_char_p
     _lib = L
@@ -495,3 +508,63 @@ class CRK:
                                              int(nstep), ctypes.c_void_p(wfn.data_ptr()), int(apply_mask),
                                              hprod._stream(stream)), "td1c_propagate")
         return wfn
+
+
+class Batch:
+    """Independent propagations run together (td1c_batch_*): the members are
+    HProd contexts (dynamical cores only, ndvr = 20, one nfun, precision,
+    split_type and gauge); their grids, lmax and pulses may differ.  Each
+    step issues one launch per kernel site for all members.  Keep the
+    members alive while the batch exists."""
+
+    def __init__(self, members: Sequence["HProd"]):
+        import torch
+
+        self.torch = torch
+        self.members = list(members)
+        arr = (ctypes.c_void_p * len(self.members))(*[m.h.value for m in self.members])
+        self.h = ctypes.c_void_p()
+        _check(load_library().td1c_batch_create(arr, len(self.members), ctypes.byref(self.h)), "td1c_batch_create")
+
+    def close(self):
+        if self.h:
+            load_library().td1c_batch_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_graph(self, on: bool):
+        _check(load_library().td1c_batch_set_graph(self.h, int(on)), "td1c_batch_set_graph")
+
+    def set_shapes(self, elements_per_block: int, l_per_block: int, transform_variant: int):
+        _check(load_library().td1c_batch_set_shapes(self.h, int(elements_per_block), int(l_per_block),
+                                                    int(transform_variant)), "td1c_batch_set_shapes")
+
+    def info(self) -> dict:
+        import json
+
+        buf = ctypes.create_string_buffer(8192)
+        _check(load_library().td1c_batch_get_info(self.h, buf, 8192), "td1c_batch_get_info")
+        return json.loads(buf.value.decode())
+
+    def propagate(self, wfns, time: float, dtime: float, nstep: int, fields: Sequence[Field], rk_order: int = 4,
+                  apply_mask: bool = True, stream=None):
+        """nstep steps of every member from `time`: member m's device wave
+        function wfns[m] under fields[m], as CRK(rk_order).propagate would
+        step each one alone."""
+        n = len(self.members)
+        if len(wfns) != n or len(fields) != n:
+            raise ValueError("need one wave function and one field per member")
+        cf = (TD1CField * n)(*[f.to_c() for f in fields])
+        ptrs = (ctypes.c_void_p * n)(*[w.data_ptr() for w in wfns])
+        if stream is None:
+            st = ctypes.c_void_p(self.torch.cuda.current_stream().cuda_stream)
+        else:
+            st = ctypes.c_void_p(stream.cuda_stream if hasattr(stream, "cuda_stream") else stream)
+        _check(load_library().td1c_batch_propagate(self.h, cf, int(rk_order), float(time), float(dtime), int(nstep),
+                                                   ptrs, int(apply_mask), st), "td1c_batch_propagate")
+        return wfns

@@ -208,6 +208,36 @@ struct td1c_ctx {
   }
 };
 
+// A batch of independent problems propagated together (td1c_batch_*): the
+// members' own contexts hold every table and work array; the batch holds one
+// recorded step (a list of launch sites, each with one argument record per
+// member) in a device table, and issues one launch per site for all members.
+struct td1c_batch {
+  std::vector<td1c_ctx*> mem;
+  int device = 0;
+  bool use_graph = true;
+  int ob_shape = 2, ob_lc = 2;  // element one-body launch shape of the batch
+  int tvar = 1;                 // register-blocked transform shape of the batch
+  struct Site {
+    LaunchRec proto;  // kind, variant and block of every member's launch
+    int total = 0;    // blocks over all members
+    size_t lds = 0;
+    size_t off_recs = 0, off_start = 0, off_dims = 0;
+  };
+  std::vector<Site> sites;
+  DevBuf<unsigned char> table;
+  size_t table_bytes = 0;
+  std::vector<unsigned char> key;  // what the recorded step depends on
+  hipStream_t cap_stream = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  std::string tune_log;
+  ~td1c_batch() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (cap_stream) (void)hipStreamDestroy(cap_stream);
+    table.release();
+  }
+};
+
 namespace {
 
 td1c_ctx* g_bound = nullptr;
@@ -1899,59 +1929,74 @@ int td1c_init_split2(td1c_ctx* c, int rk_order, double dstep, double tolene, int
   return 0;
 }
 
-int td1c_propagate(td1c_ctx* c, const td1c_field* field, int rk_order, double t0, double dtime, int nstep,
-                   double* d_wfn, int apply_mask, void* stream) {
-  if (!c || !field || !d_wfn) return fail(-1, "null argument");
-  if (c->S.nfcore > 0 && !c->wfn0_set) return fail(-4, "frozen core needs td1c_set_wfn0 first");
-  if (field->env_type < 0 || field->env_type > 2 || field->gauge < 0 || field->gauge > 1)
-    return fail(-1, "bad field");
-  if (nstep < 0) return fail(-1, "nstep must be >= 0");
-  int ns;
+// The stage nodes of one propagation step: the RK nodes, and with
+// split_type = 1 the two half-step field times t + dt/4 and t + 3dt/4 as two
+// more nodes (ns = the RK stages).
+static int step_nodes(td1c_ctx* c, int rk_order, StageNodes& sn, int& ns) {
   double nodes[kMaxStages], w[kMaxStages], a[kMaxStages][kMaxStages];
   if (rk_tableau(rk_order, ns, nodes, w, a)) return fail(-1, "rk_order must be 1..4");
-  if (nstep == 0) return 0;
-  hipStream_t s = (hipStream_t)stream;
-  const Pulse p = make_pulse(field);
-  // split_type = 1: tdse_split2's step is clh1itr::prop(STEP1) with the field
-  // at t + dt/4, the RK step of the split hprod, clh1itr::prop(STEP2) at
-  // t + 3dt/4, then the mask (src/main/tdse.cpp:225-266,
-  // src/prop/clh1itr.cpp:61-91); the two half-step fields ride along as
-  // extra stage nodes
-  const bool split = c->S.isplit == 1;
-  if (split) {
-    int rc = cn_prepare(c, dtime);
-    if (rc) return rc;
-  }
-  StageNodes sn{};
+  sn = StageNodes{};
   sn.ns = ns;
   for (int i = 0; i < ns; ++i) sn.c[i] = nodes[i];
-  if (split) {
+  if (c->S.isplit == 1) {
     if (ns + 2 > kMaxStages) return fail(-1, "too many stages for the split step");
     sn.c[ns] = 0.25;
     sn.c[ns + 1] = 0.75;
     sn.ns = ns + 2;
   }
-  double2* wfn = reinterpret_cast<double2*>(d_wfn);
-  auto enqueue_step = [&](hipStream_t st, bool join) -> int {
-    if (split) {
-      int rc = enqueue_h1itr(c, c->lf_dev.p + 9 * ns, wfn, st);
-      if (rc) return rc;
-    }
-    int rc = enqueue_rk_step(c, rk_order, dtime, c->lf_dev.p, wfn, split ? 0 : apply_mask, st, join);
+  return 0;
+}
+
+// One step of td1c_propagate after its stage-field launch.  split_type = 1:
+// tdse_split2's step is clh1itr::prop(STEP1) with the field at t + dt/4, the
+// RK step of the split hprod, clh1itr::prop(STEP2) at t + 3dt/4, then the
+// mask (src/main/tdse.cpp:225-266, src/prop/clh1itr.cpp:61-91).
+static int enqueue_prop_step(td1c_ctx* c, int rk_order, int ns, double dtime, double2* wfn, int apply_mask,
+                             hipStream_t st, bool join) {
+  const bool split = c->S.isplit == 1;
+  if (split) {
+    int rc = enqueue_h1itr(c, c->lf_dev.p + 9 * ns, wfn, st);
     if (rc) return rc;
-    if (split) {
-      rc = enqueue_h1itr(c, c->lf_dev.p + 9 * (ns + 1), wfn, st);
-      if (rc) return rc;
-      if (apply_mask) launch_mask(wfn, c->mask.p, c->S.nr, c->S.L1, c->S.nfcore, c->S.nfun, st);
-    }
-    return 0;
-  };
+  }
+  int rc = enqueue_rk_step(c, rk_order, dtime, c->lf_dev.p, wfn, split ? 0 : apply_mask, st, join);
+  if (rc) return rc;
+  if (split) {
+    rc = enqueue_h1itr(c, c->lf_dev.p + 9 * (ns + 1), wfn, st);
+    if (rc) return rc;
+    if (apply_mask) launch_mask(wfn, c->mask.p, c->S.nr, c->S.L1, c->S.nfcore, c->S.nfun, st);
+  }
+  return 0;
+}
+
+static int check_field(const td1c_field* field) {
+  if (field->env_type < 0 || field->env_type > 2 || field->gauge < 0 || field->gauge > 1)
+    return fail(-1, "bad field");
+  return 0;
+}
+
+int td1c_propagate(td1c_ctx* c, const td1c_field* field, int rk_order, double t0, double dtime, int nstep,
+                   double* d_wfn, int apply_mask, void* stream) {
+  if (!c || !field || !d_wfn) return fail(-1, "null argument");
+  if (c->S.nfcore > 0 && !c->wfn0_set) return fail(-4, "frozen core needs td1c_set_wfn0 first");
+  if (check_field(field)) return -1;
+  if (nstep < 0) return fail(-1, "nstep must be >= 0");
+  int ns;
+  StageNodes sn;
+  if (int rc = step_nodes(c, rk_order, sn, ns)) return rc;
+  if (nstep == 0) return 0;
+  hipStream_t s = (hipStream_t)stream;
+  const Pulse p = make_pulse(field);
+  if (c->S.isplit == 1) {
+    int rc = cn_prepare(c, dtime);
+    if (rc) return rc;
+  }
+  double2* wfn = reinterpret_cast<double2*>(d_wfn);
   if (!c->use_graph) {
     // stream launches: the hprod's side-stream branches then run concurrently
     launch_clock_set(c->clock.p, t0, dtime, s);
     for (int i = 0; i < nstep; ++i) {
       launch_field_stages(p, c->clock.p, sn, c->lf_dev.p, s);
-      int rc = enqueue_step(s, i == nstep - 1);
+      int rc = enqueue_prop_step(c, rk_order, ns, dtime, wfn, apply_mask, s, i == nstep - 1);
       if (rc) return rc;
     }
     HIP_TRY(hipGetLastError());
@@ -1968,7 +2013,7 @@ int td1c_propagate(td1c_ctx* c, const td1c_field* field, int rk_order, double t0
     HIP_TRY(hipStreamBeginCapture(c->cap_stream, hipStreamCaptureModeThreadLocal));
     c->capturing = true;
     launch_field_stages(p, c->clock.p, sn, c->lf_dev.p, c->cap_stream);
-    int rc = enqueue_step(c->cap_stream, true);
+    int rc = enqueue_prop_step(c, rk_order, ns, dtime, wfn, apply_mask, c->cap_stream, true);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(c->cap_stream, &g);
     c->capturing = false;
@@ -1992,6 +2037,288 @@ int td1c_propagate(td1c_ctx* c, const td1c_field* field, int rk_order, double t0
   launch_clock_set(c->clock.p, t0, dtime, s);
   for (int i = 0; i < nstep; ++i) HIP_TRY(hipGraphLaunch(c->gexec, s));
   HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Batches (scans of pulses, grids and lmax): one launch per recorded site
+// for all members.  No counterpart in the reference, whose scans run one
+// propagation per process.
+// ---------------------------------------------------------------------------
+int td1c_batch_create(td1c_ctx* const* members, int n, td1c_batch** out) {
+  if (!members || !out || n <= 0) return fail(-1, "null argument or empty batch");
+  *out = nullptr;
+  td1c_ctx* c0 = members[0];
+  if (!c0) return fail(-1, "null member");
+  for (int i = 0; i < n; ++i) {
+    td1c_ctx* c = members[i];
+    if (!c) return fail(-1, "null member");
+    for (int j = 0; j < i; ++j)
+      if (members[j] == c) return fail(-1, "a context appears twice in the batch");
+    const Setup& S = c->S;
+    if (c->device != c0->device) return fail(-1, "batch members must share a device");
+    if (S.n != 20) return fail(-1, "batched propagation needs ndvr = 20");
+    if (S.nfcore != 0 || S.nact != 0)
+      return fail(-1, "batched propagation holds dynamical cores only (no frozen cores, no active orbitals)");
+    if (S.nfun != c0->S.nfun || c->mixed != c0->mixed || S.isplit != c0->S.isplit ||
+        S.igauge != c0->S.igauge || S.icomp != c0->S.icomp)
+      return fail(-1, "batch members must share nfun, precision, split_type, gauge and icomp");
+  }
+  td1c_batch* b = new td1c_batch();
+  b->mem.assign(members, members + n);
+  b->device = c0->device;
+  b->tvar = c0->tvar;
+  *out = b;
+  return 0;
+}
+
+int td1c_batch_destroy(td1c_batch* b) {
+  if (!b) return 0;
+  HIP_TRY(hipDeviceSynchronize());  // replays of the batch's graph may still read its table
+  delete b;
+  return 0;
+}
+
+int td1c_batch_set_graph(td1c_batch* b, int on) {
+  if (!b) return fail(-1, "null argument");
+  b->use_graph = on != 0;
+  return 0;
+}
+
+int td1c_batch_set_shapes(td1c_batch* b, int elements_per_block, int l_per_block, int transform_variant) {
+  if (!b) return fail(-1, "null argument");
+  const int shape = elements_per_block == 3 ? 0 : (elements_per_block == 6 ? 1 : (elements_per_block == 12 ? 2 : -1));
+  if (shape < 0) return fail(-1, "elements_per_block must be 3, 6 or 12");
+  if (l_per_block < 1) return fail(-1, "l_per_block must be >= 1");
+  if (transform_variant != 0 && transform_variant != 1) return fail(-1, "transform_variant must be 0 or 1");
+  b->ob_shape = shape;
+  b->ob_lc = l_per_block;
+  b->tvar = transform_variant;
+  b->key.clear();  // record again
+  return 0;
+}
+
+namespace {
+
+extern "C++" {
+template <class T>
+void key_put(std::vector<unsigned char>& k, const T& v) {
+  const unsigned char* p = reinterpret_cast<const unsigned char*>(&v);
+  k.insert(k.end(), p, p + sizeof(T));
+}
+}
+
+// Records one step of every member and lays the sites out in a device table:
+// per site, the members' argument records, then start[nmem + 1], then
+// dims[nmem][3], each 256-byte aligned; identical sites share one block.
+int batch_record(td1c_batch* b, const std::vector<Pulse>& pulses, int rk_order, double dtime, double* const* d_wfns,
+                 int apply_mask) {
+  const int nmem = (int)b->mem.size();
+  std::vector<std::vector<LaunchRec>> recs(nmem);
+  for (int m = 0; m < nmem; ++m) {
+    td1c_ctx* c = b->mem[m];
+    int ns;
+    StageNodes sn;
+    if (int rc = step_nodes(c, rk_order, sn, ns)) return rc;
+    Recorder R;
+    const int tpath = c->tpath, shape = c->ob_shape, lc = c->ob_lc, tvar = c->tvar;
+    const bool capturing = c->capturing;
+    c->tpath = 2;  // the matrix-core transforms (their batched form)
+    c->ob_shape = b->ob_shape;
+    c->ob_lc = b->ob_lc;
+    c->tvar = b->tvar;
+    c->capturing = true;  // no per-kernel timing events
+    active_recorder() = &R;
+    launch_field_stages(pulses[m], c->clock.p, sn, c->lf_dev.p, c->side[1]);
+    int rc = enqueue_prop_step(c, rk_order, ns, dtime, reinterpret_cast<double2*>(d_wfns[m]), apply_mask, c->side[1],
+                               true);
+    active_recorder() = nullptr;
+    c->tpath = tpath;
+    c->ob_shape = shape;
+    c->ob_lc = lc;
+    c->tvar = tvar;
+    c->capturing = capturing;
+    if (rc) return rc;
+    if (!R.error.empty()) return fail(-1, "not batched: " + R.error);
+    recs[m] = std::move(R.recs);
+  }
+  const size_t nsite = recs[0].size();
+  for (int m = 1; m < nmem; ++m) {
+    if (recs[m].size() != nsite) return fail(-1, "batch members record different steps");
+    for (size_t i = 0; i < nsite; ++i) {
+      const LaunchRec &a = recs[0][i], &r = recs[m][i];
+      if (a.kind != r.kind || a.block != r.block || std::memcmp(a.var, r.var, sizeof a.var) != 0 ||
+          a.args.size() != r.args.size())
+        return fail(-1, "batch members record different launches");
+    }
+  }
+  auto align = [](size_t x) { return (x + 255) / 256 * 256; };
+  std::vector<unsigned char> host;
+  std::map<std::string, size_t> seen;
+  std::vector<td1c_batch::Site> sites(nsite);
+  for (size_t i = 0; i < nsite; ++i) {
+    td1c_batch::Site& st = sites[i];
+    st.proto = recs[0][i];
+    st.proto.args.clear();
+    const size_t rsz = recs[0][i].args.size();
+    if (rsz != rec_size(st.proto.kind)) return fail(-1, "batch record size mismatch");
+    std::vector<int> start(nmem + 1, 0), dims(3 * nmem, 1);
+    size_t lds = 0;
+    long long tot = 0;
+    for (int m = 0; m < nmem; ++m) {
+      const LaunchRec& r = recs[m][i];
+      start[m] = (int)tot;
+      for (int d = 0; d < 3; ++d) dims[3 * m + d] = r.grid[d];
+      tot += (long long)r.grid[0] * r.grid[1] * r.grid[2];
+      lds = std::max(lds, r.lds);
+    }
+    if (tot <= 0 || tot > 0x7fffffffLL) return fail(-1, "batched launch too large");
+    start[nmem] = (int)tot;
+    st.total = (int)tot;
+    st.lds = lds;
+    BatchMap probe{nullptr, nullptr, nmem};
+    if (!launch_batch_kernels(st.proto, nullptr, probe, 0, 0, nullptr, true) &&
+        !launch_batch_h1itr(st.proto, nullptr, probe, 0, 0, nullptr, true) &&
+        !launch_batch_island(st.proto, nullptr, probe, 0, 0, nullptr, true))
+      return fail(-1, "no batched kernel for launch kind " + std::to_string(st.proto.kind));
+    std::string blob;
+    for (int m = 0; m < nmem; ++m) blob.append(recs[m][i].args.begin(), recs[m][i].args.end());
+    const size_t nrec = blob.size();
+    blob.append(reinterpret_cast<const char*>(start.data()), start.size() * sizeof(int));
+    blob.append(reinterpret_cast<const char*>(dims.data()), dims.size() * sizeof(int));
+    auto it = seen.find(blob);
+    size_t off;
+    if (it != seen.end()) {
+      off = it->second;
+    } else {
+      off = align(host.size());
+      host.resize(off);
+      host.insert(host.end(), blob.begin(), blob.begin() + nrec);
+      host.resize(align(host.size()));
+      host.insert(host.end(), blob.begin() + nrec, blob.begin() + nrec + start.size() * sizeof(int));
+      host.resize(align(host.size()));
+      host.insert(host.end(), blob.begin() + nrec + start.size() * sizeof(int), blob.end());
+      seen.emplace(std::move(blob), off);
+    }
+    st.off_recs = off;
+    st.off_start = align(off + nrec);
+    st.off_dims = align(st.off_start + start.size() * sizeof(int));
+  }
+  // the old table may still be read by queued launches
+  HIP_TRY(hipDeviceSynchronize());
+  if (b->gexec) {
+    HIP_TRY(hipGraphExecDestroy(b->gexec));
+    b->gexec = nullptr;
+  }
+  b->table.release();
+  HIP_TRY(b->table.alloc(std::max<size_t>(host.size(), 256)));
+  HIP_TRY(hipMemcpy(b->table.p, host.data(), host.size(), hipMemcpyHostToDevice));
+  b->table_bytes = host.size();
+  b->sites = std::move(sites);
+  return 0;
+}
+
+hipError_t batch_launch_step(td1c_batch* b, hipStream_t s) {
+  const int nmem = (int)b->mem.size();
+  for (const td1c_batch::Site& st : b->sites) {
+    const unsigned char* base = b->table.p;
+    const BatchMap map{reinterpret_cast<const int*>(base + st.off_start), reinterpret_cast<const int*>(base + st.off_dims),
+                       nmem};
+    const void* recs = base + st.off_recs;
+    if (!launch_batch_kernels(st.proto, recs, map, st.total, st.lds, s, false) &&
+        !launch_batch_h1itr(st.proto, recs, map, st.total, st.lds, s, false) &&
+        !launch_batch_island(st.proto, recs, map, st.total, st.lds, s, false))
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace
+
+int td1c_batch_propagate(td1c_batch* b, const td1c_field* fields, int rk_order, double t0, double dtime, int nstep,
+                         double* const* d_wfns, int apply_mask, void* stream) {
+  if (!b || !fields || !d_wfns) return fail(-1, "null argument");
+  const int nmem = (int)b->mem.size();
+  for (int m = 0; m < nmem; ++m) {
+    if (!d_wfns[m]) return fail(-1, "null wave function");
+    if (check_field(&fields[m])) return -1;
+  }
+  if (nstep < 0) return fail(-1, "nstep must be >= 0");
+  {
+    int ns;
+    StageNodes sn;
+    if (int rc = step_nodes(b->mem[0], rk_order, sn, ns)) return rc;
+  }
+  if (nstep == 0) return 0;
+  hipStream_t s = (hipStream_t)stream;
+  std::vector<Pulse> pulses(nmem);
+  for (int m = 0; m < nmem; ++m) pulses[m] = make_pulse(&fields[m]);
+  if (b->mem[0]->S.isplit == 1)
+    for (td1c_ctx* c : b->mem)
+      if (int rc = cn_prepare(c, dtime)) return rc;
+  // the recorded step depends on the call's arguments and on the members'
+  // tables that a new dt or frozen core reallocates
+  std::vector<unsigned char> key;
+  key_put(key, rk_order);
+  key_put(key, apply_mask);
+  key_put(key, dtime);
+  key_put(key, b->ob_shape);
+  key_put(key, b->ob_lc);
+  key_put(key, b->tvar);
+  for (int m = 0; m < nmem; ++m) {
+    td1c_ctx* c = b->mem[m];
+    key_put(key, d_wfns[m]);
+    key_put(key, pulses[m]);
+    key_put(key, c->cnG.p);
+    key_put(key, c->cn_part.p);
+    key_put(key, c->cn_dt);
+  }
+  if (key != b->key) {
+    b->key.clear();
+    if (int rc = batch_record(b, pulses, rk_order, dtime, d_wfns, apply_mask)) return rc;
+    b->key = std::move(key);
+  }
+  for (td1c_ctx* c : b->mem) launch_clock_set(c->clock.p, t0, dtime, s);
+  if (!b->use_graph) {
+    for (int i = 0; i < nstep; ++i)
+      if (hipError_t e = batch_launch_step(b, s)) return fail(-3, std::string("batched launch: ") + hipGetErrorString(e));
+    return 0;
+  }
+  if (!b->gexec) {
+    if (!b->cap_stream) HIP_TRY(hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking));
+    HIP_TRY(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal));
+    hipError_t le = batch_launch_step(b, b->cap_stream);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(b->cap_stream, &g);
+    if (le != hipSuccess || e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return fail(-3, std::string("batch capture: ") + hipGetErrorString(le != hipSuccess ? le : e));
+    }
+    e = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      b->gexec = nullptr;
+      return fail(-3, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+  }
+  for (int i = 0; i < nstep; ++i) HIP_TRY(hipGraphLaunch(b->gexec, s));
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+int td1c_batch_get_info(const td1c_batch* b, char* buf, int buflen) {
+  if (!b || !buf || buflen <= 0) return fail(-1, "null argument");
+  long long blocks = 0;
+  for (const auto& st : b->sites) blocks += st.total;
+  const char* ge[3] = {"3", "6", "12"};
+  std::string js = "{\"members\": " + std::to_string(b->mem.size()) + ", \"sites_per_step\": " +
+                   std::to_string(b->sites.size()) + ", \"blocks_per_step\": " + std::to_string(blocks) +
+                   ", \"table_bytes\": " + std::to_string(b->table_bytes) + ", \"onebody\": [" + ge[b->ob_shape] +
+                   ", " + std::to_string(b->ob_lc) + "], \"transform_variant\": " + std::to_string(b->tvar) +
+                   ", \"graph\": " + (b->use_graph ? "true" : "false") +
+                   (b->tune_log.empty() ? std::string() : ", \"tune\": " + b->tune_log) + "}";
+  if ((int)js.size() + 1 > buflen) return fail(-2, "buffer too small");
+  std::memcpy(buf, js.c_str(), js.size() + 1);
   return 0;
 }
 

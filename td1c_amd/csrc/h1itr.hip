@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstring>
 
 #include "devmath.hpp"
 #include "kernels.hpp"
@@ -112,10 +113,10 @@ __global__ void __launch_bounds__(256) k_cn_rhs(const CnArgs a, int mode) {
 // cyclic reduction of the bridge system in LDS, the bridge values into x and
 // into the state, and the block's share of sum |x|^2.
 template <int BS>
-__global__ void __launch_bounds__(BS) k_cn_bridge(const CnArgs a, int mode) {
+__device__ __forceinline__ void cn_bridge_body(const CnArgs& a, int mode, const Blk bk) {
   if (mode == 1 && a.flag[0]) return;
   const int tid = threadIdx.x;
-  const int job = blockIdx.x;  // fd * L1 + l
+  const int job = bk.x;  // fd * L1 + l
   const int fd = job / a.L1, l = job - fd * a.L1;
   const int f = a.nfcore + fd;
   const int nb = a.nb, n = a.n, ni = a.ni, K = a.K, nr = a.nr;
@@ -184,6 +185,11 @@ __global__ void __launch_bounds__(BS) k_cn_bridge(const CnArgs a, int mode) {
   }
 }
 
+template <int BS>
+__global__ void __launch_bounds__(BS) k_cn_bridge(const CnArgs a, int mode) {
+  cn_bridge_body<BS>(a, mode, hw_blk());
+}
+
 // Interior pass, one block per (l, chunk of epb elements), thread per
 // interior row, every dynamical orbital in groups of kCnGroup: x_I = G_e y_I
 // - W_l x_left - W_r x_right with the row of G_e loaded once into registers
@@ -192,14 +198,14 @@ __global__ void __launch_bounds__(BS) k_cn_bridge(const CnArgs a, int mode) {
 // the RMS orbital norm of this term is below thresh (the test at the top of
 // the next cycle of h1itr_prop.F90:75-99).
 template <int NI>
-__global__ void __launch_bounds__(kCnInteriorThreads) k_cn_interior(const CnArgs a, int mode, int epb, int nchunk) {
+__device__ __forceinline__ void cn_interior_body(const CnArgs& a, int mode, int epb, int nchunk, const Blk bk) {
   if (mode == 1 && a.flag[0]) return;
   constexpr int PG = kCnGroup;
   constexpr int BS = kCnInteriorThreads;
   const int ni = NI > 0 ? NI : a.ni;
   const int nb = a.nb, nr = a.nr, n = a.n, L1 = a.L1;
   const int nd = a.nfun - a.nfcore;
-  const int job = xcd_remap(blockIdx.x, gridDim.x);
+  const int job = xcd_remap(bk.x, bk.nx);
   const int l = job / nchunk, ch = job - l * nchunk;
   const int e0 = ch * epb;
   const int t = threadIdx.x;
@@ -277,16 +283,16 @@ __global__ void __launch_bounds__(kCnInteriorThreads) k_cn_interior(const CnArgs
     double s = 0.0;
 #pragma unroll
     for (int w = 0; w < BS / 64; ++w) s += sRed[w];
-    a.part[a.npart1 + blockIdx.x] = s;
+    a.part[a.npart1 + bk.x] = s;
     __threadfence();
     const int tk = atomicAdd(&a.flag[1], 1);
-    slast = tk == (int)gridDim.x - 1;
+    slast = tk == (int)bk.nx - 1;
   }
   __syncthreads();
   if (!slast) return;
   // the last block: every partial sum is written; fixed-order reduction
   __threadfence();
-  const int ntot = a.npart1 + (int)gridDim.x;
+  const int ntot = a.npart1 + (int)bk.nx;
   double s = 0.0;
   for (int q = t; q < ntot; q += BS) s += a.part[q];
   for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
@@ -303,6 +309,11 @@ __global__ void __launch_bounds__(kCnInteriorThreads) k_cn_interior(const CnArgs
   }
 }
 
+template <int NI>
+__global__ void __launch_bounds__(kCnInteriorThreads) k_cn_interior(const CnArgs a, int mode, int epb, int nchunk) {
+  cn_interior_body<NI>(a, mode, epb, nchunk, hw_blk());
+}
+
 int cn_interior_blocks(const CnArgs& a) {
   const int epb = std::min(32, kCnInteriorThreads / a.ni);
   const int nchunk = (a.nfe + epb - 1) / epb;
@@ -317,16 +328,55 @@ void launch_cn_stage(const CnArgs& a, const OneBodyElArgs* ob, int shape, int lc
     e.cn = mode + 1;
     launch_onebody_el(mode == 0 ? a.wfn + (size_t)a.nfcore * a.L1 * a.nr : a.x, a.y, e, s, shape, lc);
   } else {
+    if (active_recorder()) return record_unsupported("the band right-hand side of the half step (ndvr != 20)");
     hipLaunchKernelGGL(k_cn_rhs, dim3((a.nr + 255) / 256, a.L1, nd), dim3(256), 0, s, a, mode);
   }
-  hipLaunchKernelGGL((k_cn_bridge<256>), dim3(nd * a.L1), dim3(256), 0, s, a, mode);
   const int epb = std::min(32, kCnInteriorThreads / a.ni);
   const int nchunk = (a.nfe + epb - 1) / epb;
   const dim3 grid(a.L1 * nchunk);
+  if (active_recorder()) {
+    if (a.ni != 19) return record_unsupported("the half step with ndvr != 20");
+    CnRec r;
+    std::memset(&r, 0, sizeof r);
+    r.a = a; r.mode = mode; r.epb = epb; r.nchunk = nchunk;
+    const int var[4] = {0, 0, 0, 0};
+    record_launch(kRecCnBridge, var, dim3(nd * a.L1), 256, 0, r);
+    record_launch(kRecCnInterior, var, grid, kCnInteriorThreads, 0, r);
+    return;
+  }
+  hipLaunchKernelGGL((k_cn_bridge<256>), dim3(nd * a.L1), dim3(256), 0, s, a, mode);
   if (a.ni == 19)
     hipLaunchKernelGGL((k_cn_interior<19>), grid, dim3(kCnInteriorThreads), 0, s, a, mode, epb, nchunk);
   else
     hipLaunchKernelGGL((k_cn_interior<0>), grid, dim3(kCnInteriorThreads), 0, s, a, mode, epb, nchunk);
+}
+
+__global__ void __launch_bounds__(256) k_cn_bridge_batch(const CnRec* __restrict__ recs, const BatchMap map) {
+  int m;
+  const Blk bk = batch_blk(map, m);
+  const CnRec& r = recs[m];
+  cn_bridge_body<256>(r.a, r.mode, bk);
+}
+
+__global__ void __launch_bounds__(kCnInteriorThreads) k_cn_interior_batch(const CnRec* __restrict__ recs,
+                                                                          const BatchMap map) {
+  int m;
+  const Blk bk = batch_blk(map, m);
+  const CnRec& r = recs[m];
+  cn_interior_body<19>(r.a, r.mode, r.epb, r.nchunk, bk);
+}
+
+bool launch_batch_h1itr(const LaunchRec& p, const void* recs, const BatchMap& map, int total, size_t lds,
+                        hipStream_t s, bool dry) {
+  const CnRec* r = static_cast<const CnRec*>(recs);
+  if (p.kind == kRecCnBridge && p.block == 256) {
+    if (!dry) hipLaunchKernelGGL(k_cn_bridge_batch, dim3(total), dim3(256), lds, s, r, map);
+  } else if (p.kind == kRecCnInterior && p.block == kCnInteriorThreads) {
+    if (!dry) hipLaunchKernelGGL(k_cn_interior_batch, dim3(total), dim3(kCnInteriorThreads), lds, s, r, map);
+  } else {
+    return false;
+  }
+  return true;
 }
 
 }  // namespace td1c

@@ -233,7 +233,7 @@ __global__ void __launch_bounds__(kT) k_island_a(const IslandArgs a) {
 // (hprod_ene_dcore, hprod_ene_act), X matrix (hprod_htot_mkxmat_fc/cc/ca/aa1,
 // src/hprod/hprod_htot_mkxmat.F90) and the CI derivative (ormas_xact2 with one
 // ORMAS subspace, src/ormas/ormas_xact2.F90:2-102).
-__global__ void __launch_bounds__(kT) k_island_b(const IslandArgs a) {
+__device__ __forceinline__ void island_b_body(const IslandArgs& a, const Blk bk) {
   extern __shared__ __align__(16) unsigned char dyn[];
   __shared__ double2 s_H[kF2], s_G[kF2], s_V[kF2], s_F[kF2], s_X[kF2], s_FM[kF2];
   __shared__ double2 s_hp[kN2];
@@ -494,6 +494,10 @@ __global__ void __launch_bounds__(kT) k_island_b(const IslandArgs a) {
   }
 }
 
+__global__ void __launch_bounds__(kT) k_island_b(const IslandArgs a) {
+  island_b_body(a, hw_blk());
+}
+
 void island_lds_plan(IslandArgs& a) {
   a.ci_lds = 0;
   a.i2_lds = 0;
@@ -511,10 +515,29 @@ void island_lds_plan(IslandArgs& a) {
 }
 
 void launch_island_a(const IslandArgs& a, hipStream_t s) {
+  if (active_recorder()) return record_unsupported("the CI density stage (active orbitals)");
   hipLaunchKernelGGL(k_island_a, dim3(1), dim3(kT), a.ci_lds ? a.ci_lds_off : 0, s, a);
 }
 void launch_island_b(const IslandArgs& a, hipStream_t s) {
+  if (active_recorder()) {
+    const int var[4] = {0, 0, 0, 0};
+    return record_launch(kRecIslandB, var, dim3(1), kT, a.lds_bytes, a);
+  }
   hipLaunchKernelGGL(k_island_b, dim3(1), dim3(kT), a.lds_bytes, s, a);
+}
+
+__global__ void __launch_bounds__(kT) k_island_b_batch(const IslandArgs* __restrict__ recs, const BatchMap map) {
+  int m;
+  const Blk bk = batch_blk(map, m);
+  island_b_body(recs[m], bk);
+}
+
+bool launch_batch_island(const LaunchRec& p, const void* recs, const BatchMap& map, int total, size_t lds,
+                         hipStream_t s, bool dry) {
+  if (p.kind != kRecIslandB || p.block != kT) return false;
+  if (!dry)
+    hipLaunchKernelGGL(k_island_b_batch, dim3(total), dim3(kT), lds, s, static_cast<const IslandArgs*>(recs), map);
+  return true;
 }
 
 }  // namespace td1c

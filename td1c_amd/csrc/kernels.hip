@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstring>
 #include <type_traits>
 
 #include "devmath.hpp"
@@ -158,12 +159,7 @@ template <> struct Mfma16<double> {
 };
 
 template <int MODE, class TI, class TO>
-__global__ void __launch_bounds__(256) k_transform_mfma(const TI* __restrict__ in, TO* __restrict__ out,
-                                                        const double* __restrict__ mats,
-                                                        const float* __restrict__ matsf,
-                                                        const TransformBatch* __restrict__ batch, int nr, int K, int C,
-                                                        int Cp, double2* __restrict__ copy, int nx, int ny, int ntile,
-                                                        const FcPhaseJob fc) {
+__device__ __forceinline__ void transform_mfma_body(const TI* __restrict__ in, TO* __restrict__ out, const double* __restrict__ mats, const float* __restrict__ matsf, const TransformBatch* __restrict__ batch, int nr, int K, int C, int Cp, double2* __restrict__ copy, int nx, int ny, int ntile, const FcPhaseJob& fc, const Blk bk) {
   using A2 = typename std::conditional<std::is_same<TI, float2>::value && std::is_same<TO, float2>::value, float2,
                                        double2>::type;
   using AR = typename CReal<A2>::T;
@@ -171,13 +167,13 @@ __global__ void __launch_bounds__(256) k_transform_mfma(const TI* __restrict__ i
   using V4 = typename MF::V4;
   constexpr int KC = 64;  // contraction values staged in LDS per pass
   // blocks past the ntile transform tiles run the frozen-core phase job
-  if ((int)blockIdx.x >= ntile) {
-    const int q = (int)blockIdx.x - ntile, rb = (fc.nr + blockDim.x - 1) / blockDim.x;
+  if ((int)bk.x >= ntile) {
+    const int q = (int)bk.x - ntile, rb = (fc.nr + blockDim.x - 1) / blockDim.x;
     fc_phase_point<TO>(fc, (q % rb) * blockDim.x + threadIdx.x, q / rb);
     return;
   }
   __shared__ A2 sX[MODE == 1 ? KC * 16 : 1];
-  const int t = xcd_remap(blockIdx.x, ntile);
+  const int t = xcd_remap(bk.x, ntile);
   const int by = t % ny, bx = (t / ny) % nx, bz = t / (ny * nx);
   const TransformBatch bt = batch[bz];
   const int lane = threadIdx.x & 63;
@@ -275,6 +271,16 @@ __global__ void __launch_bounds__(256) k_transform_mfma(const TI* __restrict__ i
   }
 }
 
+template <int MODE, class TI, class TO>
+__global__ void __launch_bounds__(256) k_transform_mfma(const TI* __restrict__ in, TO* __restrict__ out,
+                                                        const double* __restrict__ mats,
+                                                        const float* __restrict__ matsf,
+                                                        const TransformBatch* __restrict__ batch, int nr, int K, int C,
+                                                        int Cp, double2* __restrict__ copy, int nx, int ny, int ntile,
+                                                        const FcPhaseJob fc) {
+  transform_mfma_body<MODE, TI, TO>(in, out, mats, matsf, batch, nr, K, C, Cp, copy, nx, ny, ntile, fc, hw_blk());
+}
+
 // The matrix-core transform with register blocking over columns: a
 // wavefront computes 16 radial rows x 16 CT columns, so that each input
 // value it loads (the MFMA B operand) feeds 2 CT MFMAs.  Block: four
@@ -282,23 +288,18 @@ __global__ void __launch_bounds__(256) k_transform_mfma(const TI* __restrict__ i
 // operands hit the same L1 lines).  Each output element is accumulated in
 // the same order as in k_transform_mfma, so both kernels give the same bits.
 template <int CT, int U, class TI, class TO>
-__global__ void __launch_bounds__(256) k_transform_mfma_w(const TI* __restrict__ in, TO* __restrict__ out,
-                                                          const double* __restrict__ mats,
-                                                          const float* __restrict__ matsf,
-                                                          const TransformBatch* __restrict__ batch, int nr, int K,
-                                                          int C, int Cp, double2* __restrict__ copy, int nx, int ny,
-                                                          int ntile, const FcPhaseJob fc) {
+__device__ __forceinline__ void transform_mfma_w_body(const TI* __restrict__ in, TO* __restrict__ out, const double* __restrict__ mats, const float* __restrict__ matsf, const TransformBatch* __restrict__ batch, int nr, int K, int C, int Cp, double2* __restrict__ copy, int nx, int ny, int ntile, const FcPhaseJob& fc, const Blk bk) {
   using A2 = typename std::conditional<std::is_same<TI, float2>::value && std::is_same<TO, float2>::value, float2,
                                        double2>::type;
   using AR = typename CReal<A2>::T;
   using MF = Mfma16<AR>;
   using V4 = typename MF::V4;
-  if ((int)blockIdx.x >= ntile) {
-    const int q = (int)blockIdx.x - ntile, rb = (fc.nr + blockDim.x - 1) / blockDim.x;
+  if ((int)bk.x >= ntile) {
+    const int q = (int)bk.x - ntile, rb = (fc.nr + blockDim.x - 1) / blockDim.x;
     fc_phase_point<TO>(fc, (q % rb) * blockDim.x + threadIdx.x, q / rb);
     return;
   }
-  const int t = xcd_remap(blockIdx.x, ntile);
+  const int t = xcd_remap(bk.x, ntile);
   const int by = t % ny, bx = (t / ny) % nx, bz = t / (ny * nx);
   const TransformBatch bt = batch[bz];
   const int lane = threadIdx.x & 63;
@@ -360,6 +361,16 @@ __global__ void __launch_bounds__(256) k_transform_mfma_w(const TI* __restrict__
       }
     }
   }
+}
+
+template <int CT, int U, class TI, class TO>
+__global__ void __launch_bounds__(256) k_transform_mfma_w(const TI* __restrict__ in, TO* __restrict__ out,
+                                                          const double* __restrict__ mats,
+                                                          const float* __restrict__ matsf,
+                                                          const TransformBatch* __restrict__ batch, int nr, int K,
+                                                          int C, int Cp, double2* __restrict__ copy, int nx, int ny,
+                                                          int ntile, const FcPhaseJob fc) {
+  transform_mfma_w_body<CT, U, TI, TO>(in, out, mats, matsf, batch, nr, K, C, Cp, copy, nx, ny, ntile, fc, hw_blk());
 }
 
 // ---------------------------------------------------------------------------
@@ -476,8 +487,7 @@ __global__ void __launch_bounds__(256) k_onebody(const double2* __restrict__ orb
 // problem by the autotuner; every shape gives the same bits.  N = ndvr.
 // ---------------------------------------------------------------------------
 template <int N, int GE, int CN>
-__global__ void __launch_bounds__(((GE * (N + 1) + 1) + 63) / 64 * 64)
-    k_onebody_el(const double2* __restrict__ orb, double2* __restrict__ h01, const OneBodyElArgs a, int lc) {
+__device__ __forceinline__ void onebody_el_body(const double2* __restrict__ orb, double2* __restrict__ h01, const OneBodyElArgs& a, int lc, const Blk bk) {
   if (CN == 2 && a.stop[0]) return;
   const double2 zf = make_double2(a.lf[2], 0.0), ef = make_double2(a.lf[5], 0.0);
   constexpr int NP = N + 1;
@@ -486,9 +496,9 @@ __global__ void __launch_bounds__(((GE * (N + 1) + 1) + 63) / 64 * 64)
   constexpr int TILE = (GE + 1) * N + 1;     // points of elements e0-1 .. e0+GE-1
   __shared__ double2 sPsi[3][TILE];
   __shared__ double2 sT[ONE_WAVE ? 1 : BS], sP1[ONE_WAVE ? 1 : BS], sP0[ONE_WAVE ? 1 : BS];
-  const int f = blockIdx.z;
-  const int e0 = blockIdx.x * GE;
-  const int l0 = blockIdx.y * lc;
+  const int f = bk.z;
+  const int e0 = bk.x * GE;
+  const int l0 = bk.y * lc;
   const int l1 = min(a.L1, l0 + lc);
   const int tid = threadIdx.x;
   // lane -> (element, local row)
@@ -654,6 +664,12 @@ __global__ void __launch_bounds__(((GE * (N + 1) + 1) + 63) / 64 * 64)
       h01[((size_t)f * a.L1 + l) * a.nr + i] = out;
     }
   }
+}
+
+template <int N, int GE, int CN>
+__global__ void __launch_bounds__(((GE * (N + 1) + 1) + 63) / 64 * 64)
+    k_onebody_el(const double2* __restrict__ orb, double2* __restrict__ h01, const OneBodyElArgs a, int lc) {
+  onebody_el_body<N, GE, CN>(orb, h01, a, lc, hw_blk());
 }
 
 // ---------------------------------------------------------------------------
@@ -886,11 +902,9 @@ __global__ void __launch_bounds__(1024) k_poisson_elem(const T* __restrict__ rho
 // Thread per grid point (theta, r); each pair potential is read once.
 // ---------------------------------------------------------------------------
 template <class A2>
-__global__ void __launch_bounds__(256) k_mfprod(const A2* __restrict__ orbg, const A2* __restrict__ vg,
-                                                A2* __restrict__ gorbg, A2* __restrict__ v2orbg,
-                                                const MfArgs a) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  const int th = blockIdx.y;
+__device__ __forceinline__ void mfprod_body(const A2* __restrict__ orbg, const A2* __restrict__ vg, A2* __restrict__ gorbg, A2* __restrict__ v2orbg, const MfArgs& a, const Blk bk) {
+  const int r = bk.x * blockDim.x + threadIdx.x;
+  const int th = bk.y;
   if (r >= a.nr) return;
   const size_t ng = (size_t)a.nlat * a.nr;
   const size_t g = (size_t)th * a.nr + r;
@@ -966,20 +980,25 @@ __global__ void __launch_bounds__(256) k_mfprod(const A2* __restrict__ orbg, con
   }
 }
 
+template <class A2>
+__global__ void __launch_bounds__(256) k_mfprod(const A2* __restrict__ orbg, const A2* __restrict__ vg,
+                                                A2* __restrict__ gorbg, A2* __restrict__ v2orbg,
+                                                const MfArgs a) {
+  mfprod_body<A2>(orbg, vg, gorbg, v2orbg, a, hw_blk());
+}
+
 // Register-resident variant of k_mfprod for a fixed orbital partition
 // (NFC frozen cores, NF orbitals, NA active): every orbital and pair index is
 // a compile-time constant, the pairs are visited in the order they are
 // stored (dyn-dyn i <= j, then dyn-fc1).
 template <int NFC, int NF, int NA, class A2>
-__global__ void __launch_bounds__(256) k_mfprod_t(const A2* __restrict__ orbg, const A2* __restrict__ vg,
-                                                  A2* __restrict__ gorbg, A2* __restrict__ v2orbg,
-                                                  const MfArgs a) {
+__device__ __forceinline__ void mfprod_t_body(const A2* __restrict__ orbg, const A2* __restrict__ vg, A2* __restrict__ gorbg, A2* __restrict__ v2orbg, const MfArgs& a, const Blk bk) {
   constexpr int NCORE = NF - NA;
   constexpr int NA2 = NA > 0 ? NA * NA : 1;
   constexpr int ND = NF - NFC;
   constexpr int NDP = ND * (ND + 1) / 2;  // dynamical pairs, stored first
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  const int th = blockIdx.y;
+  const int r = bk.x * blockDim.x + threadIdx.x;
+  const int th = bk.y;
   if (r >= a.nr) return;
   const size_t ng = (size_t)a.nlat * a.nr;
   const size_t g = (size_t)th * a.nr + r;
@@ -1070,6 +1089,13 @@ __global__ void __launch_bounds__(256) k_mfprod_t(const A2* __restrict__ orbg, c
   }
 }
 
+template <int NFC, int NF, int NA, class A2>
+__global__ void __launch_bounds__(256) k_mfprod_t(const A2* __restrict__ orbg, const A2* __restrict__ vg,
+                                                  A2* __restrict__ gorbg, A2* __restrict__ v2orbg,
+                                                  const MfArgs a) {
+  mfprod_t_body<NFC, NF, NA, A2>(orbg, vg, gorbg, v2orbg, a, hw_blk());
+}
+
 // The same products for four active orbitals and no dynamical core
 // (nact = 4, nfun = nfcore + 4, ncore = nfcore), four lanes per point: lane
 // q owns active orbital q and takes the act2 rows that reach it,
@@ -1158,12 +1184,12 @@ __global__ void __launch_bounds__(256) k_mfprod_quad(const A2* __restrict__ orbg
 // partial[entry][which][block-in-q].
 // ---------------------------------------------------------------------------
 template <int TA, int TB, bool CR, class T>
-__global__ void __launch_bounds__(256) k_gram(const GramArgs a, double2* __restrict__ partial, int nqblk) {
+__device__ __forceinline__ void gram_body(const GramArgs& a, double2* __restrict__ partial, int nqblk, const Blk bk) {
   const int ntb = (a.nR + TB - 1) / TB;
   // logical order: (a, b) tile fastest, so the tiles that read one index
   // range run on one XCD
-  const int nt = gridDim.y;
-  const int lt = xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * nt);
+  const int nt = bk.ny;
+  const int lt = xcd_remap(bk.x + bk.nx * bk.y, bk.nx * nt);
   const int tile = lt % nt, qb = lt / nt;
   const int ta = tile / ntb, tb = tile % ntb;
   const int a0 = ta * TA, b0 = tb * TB;
@@ -1268,6 +1294,11 @@ __global__ void __launch_bounds__(256) k_gram(const GramArgs a, double2* __restr
   }
 }
 
+template <int TA, int TB, bool CR, class T>
+__global__ void __launch_bounds__(256) k_gram(const GramArgs a, double2* __restrict__ partial, int nqblk) {
+  gram_body<TA, TB, CR, T>(a, partial, nqblk, hw_blk());
+}
+
 // one wavefront per Gram entry: strided partial sums, then a butterfly
 __global__ void __launch_bounds__(256) k_gram_final(const double2* __restrict__ partial, int nblk, int nent,
                                                     double2* __restrict__ out, int with_conj) {
@@ -1324,10 +1355,10 @@ __device__ double2 x_entry(const DtorbArgs& a, int j, int i, const double2* sP) 
   return z;
 }
 
-__global__ void __launch_bounds__(256) k_dtorb(const DtorbArgs a) {
-  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void dtorb_body(const DtorbArgs& a, const Blk bk) {
+  const size_t q = (size_t)bk.x * blockDim.x + threadIdx.x;
   const size_t nb = (size_t)a.nbas;
-  const int f = blockIdx.y;
+  const int f = bk.y;
   __shared__ double2 sXc[kMaxFun];     // column f of X when built here
   __shared__ double2 sP[4 * kMaxFun];  // its Gram products, summed from the partials
   if (a.xlocal) {  // block-uniform
@@ -1399,14 +1430,22 @@ __global__ void __launch_bounds__(256) k_dtorb(const DtorbArgs a) {
   }
 }
 
+__global__ void __launch_bounds__(256) k_dtorb(const DtorbArgs a) {
+  dtorb_body(a, hw_blk());
+}
+
 // radial mask of the dynamical orbitals (wfn_mask, src/wfn/wfn_mask.F90:2-35)
-__global__ void k_mask(double2* __restrict__ wfn, const double* __restrict__ mask, int nr, int L1, int nfcore,
-                       int nfun) {
-  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void mask_body(double2* __restrict__ wfn, const double* __restrict__ mask, int nr, int L1, int nfcore, int nfun, const Blk bk) {
+  const size_t q = (size_t)bk.x * blockDim.x + threadIdx.x;
   const size_t nb = (size_t)nr * L1;
-  const int f = blockIdx.y;
+  const int f = bk.y;
   if (q >= nb || f < nfcore || f >= nfun) return;
   wfn[(size_t)f * nb + q] = cscale(wfn[(size_t)f * nb + q], mask[q % nr]);
+}
+
+__global__ void k_mask(double2* __restrict__ wfn, const double* __restrict__ mask, int nr, int L1, int nfcore,
+                       int nfun) {
+  mask_body(wfn, mask, nr, L1, nfcore, nfun, hw_blk());
 }
 
 // ---------------------------------------------------------------------------
@@ -1427,6 +1466,19 @@ static void transform_variant(int ks, dim3 grid, hipStream_t s, const TI* in, TO
                        C, Cp, copy);
 }
 
+template <class TI, class TO>
+constexpr int type_code() {
+  return std::is_same<TI, double2>::value ? (std::is_same<TO, double2>::value ? 0 : 1)
+                                          : (std::is_same<TO, float2>::value ? 2 : 3);
+}
+
+static void tvar_shape(int tvar, int& CT, int& U) {
+  static const int t[6][2] = {{4, 4}, {2, 4}, {4, 8}, {8, 4}, {8, 2}, {2, 8}};
+  const int v = tvar >= 0 && tvar < 6 ? tvar : 0;
+  CT = t[v][0];
+  U = t[v][1];
+}
+
 template <class T>
 static void launch_fc_phase(const FcPhaseJob& j, hipStream_t s) {
   hipLaunchKernelGGL((k_fc_phase<T>), dim3((j.nr + 255) / 256, j.nlat), dim3(256), 0, s, j);
@@ -1438,6 +1490,28 @@ template <class TI, class TO>
 void launch_transform_t(int mode, const TI* in, TO* out, const double* mats, const float* matsf,
                         const TransformBatch* batch, int nbatch, int nr, int K, int C, int Cp, double2* copy,
                         hipStream_t s, int path, const FcPhaseJob* fc, int tvar) {
+  if (active_recorder()) {
+    // batched form: the matrix-core kernels for every member (the same bits
+    // as the member's own matrix-core launch), 256 threads
+    if (fc != nullptr) return record_unsupported("the frozen-core phase job");
+    TransformRec r;
+    std::memset(&r, 0, sizeof r);
+    r.in = in; r.out = out; r.mats = mats; r.matsf = matsf; r.batch = batch;
+    r.nr = nr; r.K = K; r.C = C; r.Cp = Cp; r.copy = mode == 0 ? copy : nullptr;
+    const int ct = (C + 15) / 16;
+    if (mode == 0) {
+      int CT, U;
+      tvar_shape(tvar, CT, U);
+      r.nx = (nr + 63) / 64; r.ny = (ct + CT - 1) / CT; r.ntile = r.nx * r.ny * nbatch;
+      const int var[4] = {CT, U, type_code<TI, TO>(), 0};
+      record_launch(kRecTransformW, var, dim3(r.ntile), 256, 0, r);
+    } else {
+      r.nx = (nr + 15) / 16; r.ny = (ct + 3) / 4; r.ntile = r.nx * r.ny * nbatch;
+      const int var[4] = {1, type_code<TI, TO>(), 0, 0};
+      record_launch(kRecTransformM, var, dim3(r.ntile), 256, 0, r);
+    }
+    return;
+  }
   // matrix cores (16 radial rows per block, one wavefront per 16 columns)
   // when that gives enough wavefronts to fill the chip and at least three
   // column tiles share each input load; otherwise the vector form, whose
@@ -1526,6 +1600,16 @@ static void launch_onebody_el_t(const double2* orb, double2* h01, const OneBodyE
 bool launch_onebody_el(const double2* orb, double2* h01, const OneBodyElArgs& a, hipStream_t s, int shape, int lc) {
   if (a.ndvr != 20) return false;
   lc = std::max(1, std::min(lc, a.L1));
+  if (active_recorder()) {
+    const int GE = shape == 0 ? 3 : (shape == 1 ? 6 : 12);
+    OneBodyRec r;
+    std::memset(&r, 0, sizeof r);
+    r.orb = orb; r.h01 = h01; r.a = a; r.lc = lc;
+    const int var[4] = {GE, a.cn, 0, 0};
+    record_launch(kRecOneBody, var, dim3((a.nfe + GE - 1) / GE, (a.L1 + lc - 1) / lc, a.nfun),
+                  ((GE * 21 + 1) + 63) / 64 * 64, 0, r);
+    return true;
+  }
   if (a.cn == 1)
     launch_onebody_el_t<1>(orb, h01, a, s, shape, lc);
   else if (a.cn == 2)
@@ -1536,6 +1620,7 @@ bool launch_onebody_el(const double2* orb, double2* h01, const OneBodyElArgs& a,
 }
 
 void launch_onebody(const double2* orb, double2* h01, const OneBodyArgs& a, hipStream_t s) {
+  if (active_recorder()) return record_unsupported("the band one-body kernel (ndvr != 20)");
   dim3 block(256);
   dim3 grid((a.nr + 255) / 256, a.L1, a.f1 - a.f0);
   hipLaunchKernelGGL(k_onebody, grid, block, 0, s, orb, h01, a);
@@ -1550,11 +1635,10 @@ void launch_onebody(const double2* orb, double2* h01, const OneBodyArgs& a, hipS
 // the cyclic-reduction coefficients sPA, sPG [pcr_steps * (nfe - 1)] each.
 // Jobs are L-major and XCD-remapped.
 template <int NI, int BS, class T>
-__global__ void __launch_bounds__(BS) k_poisson_bridge_lds(const T* __restrict__ rho, T* __restrict__ v,
-                                                       double2* __restrict__ xb, const PoissonArgs a) {
+__device__ __forceinline__ void poisson_bridge_lds_body(const T* __restrict__ rho, T* __restrict__ v, double2* __restrict__ xb, const PoissonArgs& a, const Blk bk) {
   extern __shared__ double2 dyn_lds[];
   const int tid = threadIdx.x;
-  const int job = xcd_remap(blockIdx.x, gridDim.x);
+  const int job = xcd_remap(bk.x, bk.nx);
   const int L = job / a.npair, p = job % a.npair;
   const int nb = a.nfe - 1, nr = a.nr, n = a.n;
   const int ni = NI > 0 ? NI : a.ni;
@@ -1678,6 +1762,12 @@ __global__ void __launch_bounds__(BS) k_poisson_bridge_lds(const T* __restrict__
   if (tid == 0) a.Q[(size_t)p * a.L2 + L] = Q;
 }
 
+template <int NI, int BS, class T>
+__global__ void __launch_bounds__(BS) k_poisson_bridge_lds(const T* __restrict__ rho, T* __restrict__ v,
+                                                       double2* __restrict__ xb, const PoissonArgs a) {
+  poisson_bridge_lds_body<NI, BS, T>(rho, v, xb, a, hw_blk());
+}
+
 // Interiors of the element Schur solve after k_poisson_bridge_lds: one
 // block per (L, chunk of EPB elements, group of PG pairs), thread per
 // interior row.  Each thread loads its row of G_e once and applies it to
@@ -1686,15 +1776,13 @@ __global__ void __launch_bounds__(BS) k_poisson_bridge_lds(const T* __restrict__
 constexpr int kPoissonPairGroup = 7;
 constexpr int kPoissonInteriorThreads = 256;
 template <int NI, class T>
-__global__ void __launch_bounds__(kPoissonInteriorThreads) k_poisson_interior(const T* __restrict__ rho, T* __restrict__ v,
-                                                         const double2* __restrict__ xb, const PoissonArgs a,
-                                                         int epb, int nchunk) {
+__device__ __forceinline__ void poisson_interior_body(const T* __restrict__ rho, T* __restrict__ v, const double2* __restrict__ xb, const PoissonArgs& a, int epb, int nchunk, const Blk bk) {
   constexpr int PG = kPoissonPairGroup;
   const int ni = NI > 0 ? NI : a.ni;
   const int nb = a.nfe - 1, nr = a.nr, n = a.n;
   const int ngrp = (a.npair + PG - 1) / PG;
   // L-major, then element chunk, then pair group; XCD-remapped
-  const int job = xcd_remap(blockIdx.x, gridDim.x);
+  const int job = xcd_remap(bk.x, bk.nx);
   const int L = job / (nchunk * ngrp), ch = (job / ngrp) % nchunk, pg = job % ngrp;
   const int p0 = pg * PG, e0 = ch * epb;
   const int t = threadIdx.x;
@@ -1777,6 +1865,13 @@ __global__ void __launch_bounds__(kPoissonInteriorThreads) k_poisson_interior(co
   }
 }
 
+template <int NI, class T>
+__global__ void __launch_bounds__(kPoissonInteriorThreads) k_poisson_interior(const T* __restrict__ rho, T* __restrict__ v,
+                                                         const double2* __restrict__ xb, const PoissonArgs a,
+                                                         int epb, int nchunk) {
+  poisson_interior_body<NI, T>(rho, v, xb, a, epb, nchunk, hw_blk());
+}
+
 constexpr int kPoissonBlock = 512;  // threads of the LDS bridge pass (>= nfe)
 
 // LDS bytes of the LDS bridge pass; 0 when the row does not fit
@@ -1791,6 +1886,22 @@ size_t poisson_fused_lds(const PoissonArgs& a) {
 template <class T>
 void launch_poisson(const T* rho, double2* xb, T* v, const PoissonArgs& a, hipStream_t s, int part) {
   const int jobs = a.npair * a.L2;
+  if (active_recorder()) {
+    const size_t lds = poisson_fused_lds<T>(a);
+    if (!lds || a.ni != 19) return record_unsupported("the Poisson solve of a grid without the LDS bridge pass");
+    PoissonRec r;
+    std::memset(&r, 0, sizeof r);
+    r.rho = rho; r.v = v; r.xb = xb; r.a = a;
+    r.epb = std::min(32, kPoissonInteriorThreads / a.ni);
+    r.nchunk = (a.nfe + r.epb - 1) / r.epb;
+    const int var[4] = {type_code<T, T>(), 0, 0, 0};
+    if (part != 2) record_launch(kRecPoissonBridge, var, dim3(jobs), kPoissonBlock, lds, r);
+    if (part != 1) {
+      const int ngrp = (a.npair + kPoissonPairGroup - 1) / kPoissonPairGroup;
+      record_launch(kRecPoissonInterior, var, dim3(a.L2 * r.nchunk * ngrp), kPoissonInteriorThreads, 0, r);
+    }
+    return;
+  }
   if (const size_t lds = poisson_fused_lds<T>(a)) {
     // bridge pass (rows staged in LDS), then the interiors with each row of
     // G_e shared by a group of pairs
@@ -1832,6 +1943,21 @@ template <class T>
 void launch_mfprod(const T* orbg, const T* vg, T* gorbg, T* v2orbg, const MfArgs& a, hipStream_t s) {
   dim3 grid((a.nr + 255) / 256, a.nlat), block(256);
   const int key = a.nfcore * 10000 + a.nfun * 100 + a.nact;
+  if (active_recorder()) {
+    MfRec r;
+    std::memset(&r, 0, sizeof r);
+    r.orbg = orbg; r.vg = vg; r.gorbg = gorbg; r.v2orbg = v2orbg; r.a = a;
+    if (a.nfcore == 0 && a.nact == 0 && (a.nfun <= 3 || a.nfun == 9 || a.nfun == 16)) {
+      const int var[4] = {a.nfun, type_code<T, T>(), 0, 0};
+      record_launch(kRecMfprodT, var, grid, 256, 0, r);
+    } else if (a.nfcore == 0 && a.nact == 0) {
+      const int var[4] = {type_code<T, T>(), 0, 0, 0};
+      record_launch(kRecMfprod, var, grid, 256, 0, r);
+    } else {
+      record_unsupported("the mean-field products with frozen cores or active orbitals");
+    }
+    return;
+  }
 #define MF_CASE(NFC, NF, NA)                                                                            \
   case NFC * 10000 + NF * 100 + NA:                                                                     \
     hipLaunchKernelGGL((k_mfprod_t<NFC, NF, NA, T>), grid, block, 0, s, orbg, vg, gorbg, v2orbg, a);  \
@@ -1871,6 +1997,16 @@ int gram_blocks(const GramArgs& a, int maxblk) {
 
 void launch_gram(const GramArgs& a, double2* partial, int maxblk, double2* out, hipStream_t s) {
   const int nblk = gram_blocks(a, maxblk);
+  if (active_recorder()) {
+    if (out != nullptr) return record_unsupported("the summed Gram products (X built outside dtorb)");
+    GramRec r;
+    std::memset(&r, 0, sizeof r);
+    r.a = a; r.partial = partial; r.nqblk = nblk;
+    const int TA = 2, TB = a.with_conj_right ? 2 : 4;
+    const int var[4] = {TA, TB, a.with_conj_right ? 1 : 0, a.f32};
+    record_launch(kRecGram, var, dim3(nblk, ((a.nL + TA - 1) / TA) * ((a.nR + TB - 1) / TB)), 256, 0, r);
+    return;
+  }
   // register tiles of 8 complex sums per thread (16 doubles), which keep
   // the kernel at many wavefronts per SIMD
   if (a.with_conj_right) {
@@ -2135,7 +2271,7 @@ __global__ void k_zero(double2* __restrict__ y, size_t n) {
 // Stage field values of the propagation step st->n, then st->n += 1
 // (one thread; tdse_split2 evaluates clfield::get_value per stage).
 // one thread per stage; the clock advances after every thread has read it
-__global__ void k_field_stages(const Pulse p, PropClock* __restrict__ st, const StageNodes nodes, double* __restrict__ lf) {
+__device__ __forceinline__ void field_stages_body(const Pulse& p, PropClock* __restrict__ st, const StageNodes& nodes, double* __restrict__ lf, const Blk bk) {
   const long long n = st->n;
   const double t = st->t0 + (double)n * st->dt;
   const double dt = st->dt;
@@ -2143,6 +2279,10 @@ __global__ void k_field_stages(const Pulse p, PropClock* __restrict__ st, const 
   const int s = threadIdx.x;
   if (s < nodes.ns) pulse_lfield(p, t + nodes.c[s] * dt, lf + 9 * s);
   if (s == 0) st->n = n + 1;
+}
+
+__global__ void k_field_stages(const Pulse p, PropClock* __restrict__ st, const StageNodes nodes, double* __restrict__ lf) {
+  field_stages_body(p, st, nodes, lf, hw_blk());
 }
 
 __global__ void k_clock_set(PropClock* __restrict__ st, double t0, double dt) {
@@ -2164,10 +2304,18 @@ void launch_to_f32(float2* y, const double2* x, size_t n, hipStream_t s) {
 }
 
 void launch_zero(double2* y, size_t n, hipStream_t s) {
+  if (active_recorder()) return record_unsupported("a fill kernel");
   if (n) hipLaunchKernelGGL(k_zero, dim3(grid_for(n)), dim3(256), 0, s, y, n);
 }
 
 void launch_field_stages(const Pulse& p, PropClock* st, const StageNodes& nodes, double* lf, hipStream_t s) {
+  if (active_recorder()) {
+    FieldRec r;
+    std::memset(&r, 0, sizeof r);
+    r.p = p; r.st = st; r.nodes = nodes; r.lf = lf;
+    const int var[4] = {0, 0, 0, 0};
+    return record_launch(kRecField, var, dim3(1), 64, 0, r);
+  }
   hipLaunchKernelGGL(k_field_stages, dim3(1), dim3(64), 0, s, p, st, nodes, lf);
 }
 
@@ -2176,13 +2324,241 @@ void launch_clock_set(PropClock* st, double t0, double dt, hipStream_t s) {
 }
 
 void launch_dtorb(const DtorbArgs& a, hipStream_t s) {
+  if (active_recorder()) {
+    const int var[4] = {0, 0, 0, 0};
+    return record_launch(kRecDtorb, var, dim3((a.nbas + 255) / 256, a.nfun), 256, 0, a);
+  }
   hipLaunchKernelGGL(k_dtorb, dim3((a.nbas + 255) / 256, a.nfun), dim3(256), 0, s, a);
 }
 
 void launch_mask(double2* wfn, const double* mask, int nr, int L1, int nfcore, int nfun, hipStream_t s) {
   size_t nb = (size_t)nr * L1;
+  if (active_recorder()) {
+    MaskRec r;
+    std::memset(&r, 0, sizeof r);
+    r.wfn = wfn; r.mask = mask; r.nr = nr; r.L1 = L1; r.nfcore = nfcore; r.nfun = nfun;
+    const int var[4] = {0, 0, 0, 0};
+    return record_launch(kRecMask, var, dim3((unsigned)((nb + 255) / 256), nfun), 256, 0, r);
+  }
   hipLaunchKernelGGL(k_mask, dim3((unsigned)((nb + 255) / 256), nfun), dim3(256), 0, s, wfn, mask, nr, L1, nfcore,
                      nfun);
+}
+
+// ---------------------------------------------------------------------------
+// Batched launches: one block range per member, each running the kernel
+// body on the member's recorded arguments (kernels.hpp).
+// ---------------------------------------------------------------------------
+Recorder*& active_recorder() {
+  static thread_local Recorder* r = nullptr;
+  return r;
+}
+
+size_t rec_size(int kind) {
+  switch (kind) {
+    case kRecTransformW:
+    case kRecTransformM: return sizeof(TransformRec);
+    case kRecOneBody: return sizeof(OneBodyRec);
+    case kRecPoissonBridge:
+    case kRecPoissonInterior: return sizeof(PoissonRec);
+    case kRecMfprodT:
+    case kRecMfprod: return sizeof(MfRec);
+    case kRecGram: return sizeof(GramRec);
+    case kRecDtorb: return sizeof(DtorbArgs);
+    case kRecMask: return sizeof(MaskRec);
+    case kRecField: return sizeof(FieldRec);
+    case kRecIslandB: return sizeof(IslandArgs);
+    case kRecCnBridge:
+    case kRecCnInterior: return sizeof(CnRec);
+    default: return 0;
+  }
+}
+
+template <int CT, int U, class TI, class TO>
+__global__ void __launch_bounds__(256) k_transform_mfma_w_batch(const TransformRec* __restrict__ recs,
+                                                                const BatchMap map) {
+  int m;
+  const Blk bk = batch_blk(map, m);
+  const TransformRec& r = recs[m];
+  transform_mfma_w_body<CT, U, TI, TO>(static_cast<const TI*>(r.in), static_cast<TO*>(r.out), r.mats, r.matsf,
+                                       r.batch, r.nr, r.K, r.C, r.Cp, r.copy, r.nx, r.ny, r.ntile, r.fc, bk);
+}
+
+template <int MODE, class TI, class TO>
+__global__ void __launch_bounds__(256) k_transform_mfma_batch(const TransformRec* __restrict__ recs,
+                                                              const BatchMap map) {
+  int m;
+  const Blk bk = batch_blk(map, m);
+  const TransformRec& r = recs[m];
+  transform_mfma_body<MODE, TI, TO>(static_cast<const TI*>(r.in), static_cast<TO*>(r.out), r.mats, r.matsf, r.batch,
+                                    r.nr, r.K, r.C, r.Cp, r.copy, r.nx, r.ny, r.ntile, r.fc, bk);
+}
+
+template <int GE, int CN>
+__global__ void __launch_bounds__(((GE * 21 + 1) + 63) / 64 * 64)
+    k_onebody_el_batch(const OneBodyRec* __restrict__ recs, const BatchMap map) {
+  int m;
+  const Blk bk = batch_blk(map, m);
+  const OneBodyRec& r = recs[m];
+  onebody_el_body<20, GE, CN>(r.orb, r.h01, r.a, r.lc, bk);
+}
+
+template <class T>
+__global__ void __launch_bounds__(kPoissonBlock) k_poisson_bridge_lds_batch(const PoissonRec* __restrict__ recs,
+                                                                            const BatchMap map) {
+  int m;
+  const Blk bk = batch_blk(map, m);
+  const PoissonRec& r = recs[m];
+  poisson_bridge_lds_body<19, kPoissonBlock, T>(static_cast<const T*>(r.rho), static_cast<T*>(r.v), r.xb, r.a, bk);
+}
+
+template <class T>
+__global__ void __launch_bounds__(kPoissonInteriorThreads) k_poisson_interior_batch(const PoissonRec* __restrict__ recs,
+                                                                                    const BatchMap map) {
+  int m;
+  const Blk bk = batch_blk(map, m);
+  const PoissonRec& r = recs[m];
+  poisson_interior_body<19, T>(static_cast<const T*>(r.rho), static_cast<T*>(r.v), r.xb, r.a, r.epb, r.nchunk, bk);
+}
+
+template <int NF, class T>
+__global__ void __launch_bounds__(256) k_mfprod_t_batch(const MfRec* __restrict__ recs, const BatchMap map) {
+  int m;
+  const Blk bk = batch_blk(map, m);
+  const MfRec& r = recs[m];
+  mfprod_t_body<0, NF, 0, T>(static_cast<const T*>(r.orbg), static_cast<const T*>(r.vg), static_cast<T*>(r.gorbg),
+                             static_cast<T*>(r.v2orbg), r.a, bk);
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_mfprod_batch(const MfRec* __restrict__ recs, const BatchMap map) {
+  int m;
+  const Blk bk = batch_blk(map, m);
+  const MfRec& r = recs[m];
+  mfprod_body<T>(static_cast<const T*>(r.orbg), static_cast<const T*>(r.vg), static_cast<T*>(r.gorbg),
+                 static_cast<T*>(r.v2orbg), r.a, bk);
+}
+
+template <int TA, int TB, bool CR, class T>
+__global__ void __launch_bounds__(256) k_gram_batch(const GramRec* __restrict__ recs, const BatchMap map) {
+  int m;
+  const Blk bk = batch_blk(map, m);
+  const GramRec& r = recs[m];
+  gram_body<TA, TB, CR, T>(r.a, r.partial, r.nqblk, bk);
+}
+
+__global__ void __launch_bounds__(256) k_dtorb_batch(const DtorbArgs* __restrict__ recs, const BatchMap map) {
+  int m;
+  const Blk bk = batch_blk(map, m);
+  dtorb_body(recs[m], bk);
+}
+
+__global__ void __launch_bounds__(256) k_mask_batch(const MaskRec* __restrict__ recs, const BatchMap map) {
+  int m;
+  const Blk bk = batch_blk(map, m);
+  const MaskRec& r = recs[m];
+  mask_body(r.wfn, r.mask, r.nr, r.L1, r.nfcore, r.nfun, bk);
+}
+
+__global__ void __launch_bounds__(64) k_field_stages_batch(const FieldRec* __restrict__ recs, const BatchMap map) {
+  int m;
+  const Blk bk = batch_blk(map, m);
+  const FieldRec& r = recs[m];
+  field_stages_body(r.p, r.st, r.nodes, r.lf, bk);
+}
+
+bool launch_batch_kernels(const LaunchRec& p, const void* recs, const BatchMap& map, int total, size_t lds,
+                          hipStream_t s, bool dry) {
+  const dim3 g(total), b(p.block);
+#define TD1C_BATCH(K, R)                                                              \
+  do {                                                                                \
+    if (!dry) hipLaunchKernelGGL((K), g, b, lds, s, static_cast<const R*>(recs), map); \
+  } while (0)
+  switch (p.kind) {
+    case kRecTransformW: {
+      if (p.block != 256) return false;
+      switch (p.var[0] * 100 + p.var[1] * 10 + p.var[2]) {
+        case 240: TD1C_BATCH((k_transform_mfma_w_batch<2, 4, double2, double2>), TransformRec); return true;
+        case 241: TD1C_BATCH((k_transform_mfma_w_batch<2, 4, double2, float2>), TransformRec); return true;
+        case 242: TD1C_BATCH((k_transform_mfma_w_batch<2, 4, float2, float2>), TransformRec); return true;
+        case 243: TD1C_BATCH((k_transform_mfma_w_batch<2, 4, float2, double2>), TransformRec); return true;
+        case 440: TD1C_BATCH((k_transform_mfma_w_batch<4, 4, double2, double2>), TransformRec); return true;
+        case 441: TD1C_BATCH((k_transform_mfma_w_batch<4, 4, double2, float2>), TransformRec); return true;
+        case 442: TD1C_BATCH((k_transform_mfma_w_batch<4, 4, float2, float2>), TransformRec); return true;
+        case 443: TD1C_BATCH((k_transform_mfma_w_batch<4, 4, float2, double2>), TransformRec); return true;
+        default: return false;
+      }
+    }
+    case kRecTransformM: {
+      if (p.block != 256 || p.var[0] != 1) return false;
+      if (p.var[1] == 0) TD1C_BATCH((k_transform_mfma_batch<1, double2, double2>), TransformRec);
+      else if (p.var[1] == 2) TD1C_BATCH((k_transform_mfma_batch<1, float2, float2>), TransformRec);
+      else return false;
+      return true;
+    }
+    case kRecOneBody: {
+      switch (p.var[0] * 10 + p.var[1]) {
+        case 30: TD1C_BATCH((k_onebody_el_batch<3, 0>), OneBodyRec); return true;
+        case 31: TD1C_BATCH((k_onebody_el_batch<3, 1>), OneBodyRec); return true;
+        case 32: TD1C_BATCH((k_onebody_el_batch<3, 2>), OneBodyRec); return true;
+        case 60: TD1C_BATCH((k_onebody_el_batch<6, 0>), OneBodyRec); return true;
+        case 61: TD1C_BATCH((k_onebody_el_batch<6, 1>), OneBodyRec); return true;
+        case 62: TD1C_BATCH((k_onebody_el_batch<6, 2>), OneBodyRec); return true;
+        case 120: TD1C_BATCH((k_onebody_el_batch<12, 0>), OneBodyRec); return true;
+        case 121: TD1C_BATCH((k_onebody_el_batch<12, 1>), OneBodyRec); return true;
+        case 122: TD1C_BATCH((k_onebody_el_batch<12, 2>), OneBodyRec); return true;
+        default: return false;
+      }
+    }
+    case kRecPoissonBridge:
+      if (p.var[0] == 0) TD1C_BATCH((k_poisson_bridge_lds_batch<double2>), PoissonRec);
+      else if (p.var[0] == 2) TD1C_BATCH((k_poisson_bridge_lds_batch<float2>), PoissonRec);
+      else return false;
+      return true;
+    case kRecPoissonInterior:
+      if (p.var[0] == 0) TD1C_BATCH((k_poisson_interior_batch<double2>), PoissonRec);
+      else if (p.var[0] == 2) TD1C_BATCH((k_poisson_interior_batch<float2>), PoissonRec);
+      else return false;
+      return true;
+    case kRecMfprodT: {
+      const bool f = p.var[1] == 2;
+      if (p.var[1] != 0 && !f) return false;
+      switch (p.var[0]) {
+        case 1: if (f) TD1C_BATCH((k_mfprod_t_batch<1, float2>), MfRec); else TD1C_BATCH((k_mfprod_t_batch<1, double2>), MfRec); return true;
+        case 2: if (f) TD1C_BATCH((k_mfprod_t_batch<2, float2>), MfRec); else TD1C_BATCH((k_mfprod_t_batch<2, double2>), MfRec); return true;
+        case 3: if (f) TD1C_BATCH((k_mfprod_t_batch<3, float2>), MfRec); else TD1C_BATCH((k_mfprod_t_batch<3, double2>), MfRec); return true;
+        case 9: if (f) TD1C_BATCH((k_mfprod_t_batch<9, float2>), MfRec); else TD1C_BATCH((k_mfprod_t_batch<9, double2>), MfRec); return true;
+        case 16: if (f) TD1C_BATCH((k_mfprod_t_batch<16, float2>), MfRec); else TD1C_BATCH((k_mfprod_t_batch<16, double2>), MfRec); return true;
+        default: return false;
+      }
+    }
+    case kRecMfprod:
+      if (p.var[0] == 0) TD1C_BATCH((k_mfprod_batch<double2>), MfRec);
+      else if (p.var[0] == 2) TD1C_BATCH((k_mfprod_batch<float2>), MfRec);
+      else return false;
+      return true;
+    case kRecGram:
+      switch (p.var[0] * 1000 + p.var[1] * 100 + p.var[2] * 10 + p.var[3]) {
+        case 2400: TD1C_BATCH((k_gram_batch<2, 4, false, double2>), GramRec); return true;
+        case 2401: TD1C_BATCH((k_gram_batch<2, 4, false, float2>), GramRec); return true;
+        case 2210: TD1C_BATCH((k_gram_batch<2, 2, true, double2>), GramRec); return true;
+        case 2211: TD1C_BATCH((k_gram_batch<2, 2, true, float2>), GramRec); return true;
+        default: return false;
+      }
+    case kRecDtorb:
+      if (p.block != 256) return false;
+      TD1C_BATCH(k_dtorb_batch, DtorbArgs);
+      return true;
+    case kRecMask:
+      if (p.block != 256) return false;
+      TD1C_BATCH(k_mask_batch, MaskRec);
+      return true;
+    case kRecField:
+      if (p.block != 64) return false;
+      TD1C_BATCH(k_field_stages_batch, FieldRec);
+      return true;
+    default: return false;
+  }
+#undef TD1C_BATCH
 }
 
 }  // namespace td1c

@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
+#include <string>
+#include <vector>
 
 #include "field.hpp"
 
@@ -282,5 +284,153 @@ constexpr size_t kIslandLds = 32768;  // dynamic LDS budget of the island kernel
 void island_lds_plan(IslandArgs& a);
 void launch_island_a(const IslandArgs& a, hipStream_t s);
 void launch_island_b(const IslandArgs& a, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// Batches of independent problems (scans over pulses, grids and lmax).  While
+// a recorder is active on the calling thread, the launch functions above
+// append what they would launch (kind, template variant, grid, block,
+// dynamic LDS and the kernel's arguments) to it instead of launching.  A
+// batch records one step of every member, then issues one launch per
+// recorded site over all members: block b of such a launch belongs to the
+// member m with start[m] <= b < start[m + 1], takes its coordinates within
+// that member's own grid from dims[3m .. 3m + 2], and runs the kernel body
+// on the member's arguments recs[m], as the member's own launch would.
+// ---------------------------------------------------------------------------
+struct Blk {
+  int x, y, z, nx, ny, nz;
+};
+__device__ __forceinline__ Blk hw_blk() {
+  return Blk{(int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, (int)gridDim.x, (int)gridDim.y, (int)gridDim.z};
+}
+struct BatchMap {
+  const int* start;  // [nmem + 1] first block of each member
+  const int* dims;   // [nmem][3] the member's grid
+  int nmem;
+};
+__device__ __forceinline__ Blk batch_blk(const BatchMap& map, int& mem) {
+  const int b = (int)blockIdx.x;
+  int lo = 0, hi = map.nmem;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (map.start[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  mem = lo;
+  const int q = b - map.start[lo];
+  const int nx = map.dims[3 * lo], ny = map.dims[3 * lo + 1], nz = map.dims[3 * lo + 2];
+  return Blk{q % nx, (q / nx) % ny, q / (nx * ny), nx, ny, nz};
+}
+
+enum RecKind {
+  kRecTransformW = 1,   // var: CT, U, type code
+  kRecTransformM,       // var: MODE, type code (256 threads)
+  kRecOneBody,          // var: GE, CN
+  kRecPoissonBridge,    // var: type code (LDS form, ndvr = 20)
+  kRecPoissonInterior,  // var: type code
+  kRecMfprodT,          // var: NF, type code (dynamical cores only)
+  kRecMfprod,           // var: type code
+  kRecGram,             // var: TA, TB, CR, f32
+  kRecDtorb,
+  kRecMask,
+  kRecField,
+  kRecIslandB,
+  kRecCnBridge,
+  kRecCnInterior,
+};
+// type codes of (input, output): 0 (c128, c128), 1 (c128, c64), 2 (c64, c64), 3 (c64, c128)
+struct TransformRec {
+  const void* in;
+  void* out;
+  const double* mats;
+  const float* matsf;
+  const TransformBatch* batch;
+  int nr, K, C, Cp;
+  double2* copy;
+  int nx, ny, ntile;
+  FcPhaseJob fc;
+};
+struct OneBodyRec {
+  const double2* orb;
+  double2* h01;
+  OneBodyElArgs a;
+  int lc;
+};
+struct PoissonRec {
+  const void* rho;
+  void* v;
+  double2* xb;
+  PoissonArgs a;
+  int epb, nchunk;
+};
+struct MfRec {
+  const void *orbg, *vg;
+  void *gorbg, *v2orbg;
+  MfArgs a;
+};
+struct GramRec {
+  GramArgs a;
+  double2* partial;
+  int nqblk;
+};
+struct MaskRec {
+  double2* wfn;
+  const double* mask;
+  int nr, L1, nfcore, nfun;
+};
+struct FieldRec {
+  Pulse p;
+  PropClock* st;
+  StageNodes nodes;
+  double* lf;
+};
+struct CnRec {
+  CnArgs a;
+  int mode, epb, nchunk;
+};
+
+struct LaunchRec {
+  int kind = 0;
+  int var[4] = {0, 0, 0, 0};
+  int grid[3] = {1, 1, 1};
+  int block = 0;
+  size_t lds = 0;
+  std::vector<unsigned char> args;
+};
+struct Recorder {
+  std::vector<LaunchRec> recs;
+  std::string error;  // the first launch that has no batched form
+};
+// the calling thread's active recorder (null: launch normally)
+Recorder*& active_recorder();
+template <class R>
+void record_launch(int kind, const int (&var)[4], dim3 grid, int block, size_t lds, const R& rec) {
+  LaunchRec L;
+  L.kind = kind;
+  for (int i = 0; i < 4; ++i) L.var[i] = var[i];
+  L.grid[0] = (int)grid.x;
+  L.grid[1] = (int)grid.y;
+  L.grid[2] = (int)grid.z;
+  L.block = block;
+  L.lds = lds;
+  const unsigned char* b = reinterpret_cast<const unsigned char*>(&rec);
+  L.args.assign(b, b + sizeof(R));
+  active_recorder()->recs.push_back(std::move(L));
+}
+inline void record_unsupported(const char* what) {
+  Recorder* r = active_recorder();
+  if (r->error.empty()) r->error = what;
+}
+// One batched launch of site `p` over nmem members: recs holds nmem
+// argument records of the site's kind, map the block ranges; `total` blocks.
+// Returns false when the kind or variant has no batched kernel in that
+// file; dry: only report whether it has one.
+bool launch_batch_kernels(const LaunchRec& p, const void* recs, const BatchMap& map, int total, size_t lds,
+                          hipStream_t s, bool dry);
+bool launch_batch_h1itr(const LaunchRec& p, const void* recs, const BatchMap& map, int total, size_t lds,
+                        hipStream_t s, bool dry);
+bool launch_batch_island(const LaunchRec& p, const void* recs, const BatchMap& map, int total, size_t lds,
+                         hipStream_t s, bool dry);
+// the argument record size of a kind
+size_t rec_size(int kind);
 
 }  // namespace td1c

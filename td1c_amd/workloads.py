@@ -126,6 +126,30 @@ def mf16_l63(**kw) -> Problem:
 WORKLOADS = {"he_l15": (he_l15, 1), "ne_l31": (ne_l31, 2), "ar_l63": (ar_l63, 3), "mf16_l63": (mf16_l63, 5)}
 
 
+def he_scan(n: int = 64, seed: int = 4007, **kw):
+    """The scan workload: n He TDHF problems (one doubly occupied orbital,
+    Z = 2) on grids of nfe elements x 21 points, nfe uniform in [40, 120],
+    rmax = 2 nfe, lmax1 uniform in {15, 31, 47, 63}, lmax2 = 2 lmax1;
+    velocity-gauge sin^2 pulses of 4 cycles, wavelengths uniform in
+    [400, 1600] nm, intensities log-spaced over 1e13 - 1e15 W/cm^2 (in
+    member order), carrier-envelope phases uniform in [0, 360) degrees.  Returns
+    (problems, fields); kw goes to every Problem (precision, isplit, ...)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    kw.setdefault("igauge", 1)
+    lmaxes = (15, 31, 47, 63)
+    ints = np.logspace(13.0, 15.0, n) if n > 1 else np.array([1.0e14])
+    probs, fields = [], []
+    for i in range(n):
+        nfe = int(rng.integers(40, 121))
+        l1 = lmaxes[int(rng.integers(0, 4))]
+        probs.append(Problem(nfe=nfe, ndvr=20, rmax=2.0 * nfe, lmax1=l1, lmax2=2 * l1, znuc=2.0, ndcore=1,
+                             nel_alpha=1, nel_beta=1, **kw))
+        fields.append(Field(env_type="sin2", gauge="velocity1", fint=float(ints[i]),
+                            wlen=float(rng.uniform(400.0, 1600.0)), cep=float(rng.uniform(0.0, 360.0)),
+                            numcyc=4.0))
+    return probs, fields
+
+
 def ne_field() -> Field:
     """800 nm, 4-cycle sin^2 pulse at 1e14 W/cm^2, velocity gauge."""
     return Field(env_type="sin2", gauge="velocity1", fint=1.0e14, wlen=800.0, numcyc=4.0)
