@@ -49,7 +49,7 @@ MFMA_PEAK = {"f64": 78.6, "f32": 157.3}
 MAIN_STREAM = {"l2theta", "pair_theta2L", "poisson_bridge", "poisson_interior", "pair_L2theta", "mfprod",
                "theta2l_mf", "gram_l", "dtorb_rk", "dtorb"}
 # steps of one propagation per workload (the configurations' run lengths)
-STEPS_PER_PROP = {"he_l15": 3000, "ne_l31": 2000, "ar_l63": 5000, "mf16_l63": 5000}
+STEPS_PER_PROP = {"he_l15": 3000, "ne_l31": 2000, "ar_l63": 5000, "mf16_l63": 5000, "he_scan": 3000}
 FINT = {"ne_l31": 1.0e14, "ar_l63": 1.6e14}
 # timing names that are a sequence of launches, not one kernel
 COMPOUND = {"h1itr"}
@@ -204,6 +204,108 @@ def cpu_baseline(prob, wfn, lfield, seconds: float, name: str) -> dict:
                       f"same state and split_type) in {el:.1f} s"}
 
 
+def scan_batch(n: int, dt: float, warmup: int, steps: int, gate_steps: int) -> dict:
+    """The scan workload (n He TDHF problems of uneven grids, lmax and pulses)
+    propagated as one batch (td1c_batch_propagate, one launch per kernel site
+    for all members, one captured graph per step) in both precisions, against
+    the plain FP64 path: each member alone through td1c_propagate."""
+    import torch
+
+    import td1c_amd
+    from td1c_amd import workloads as W
+
+    tune = os.environ.get("TD1C_AUTOTUNE")
+    os.environ["TD1C_AUTOTUNE"] = "0"  # the batch chooses its own launch shapes
+    rk = td1c_amd.CRK(4)
+    out = {}
+    members = {}
+    try:
+        for p in (1, 0):
+            probs, fields = W.he_scan(n, precision=p, isplit=1)
+            t0 = time.perf_counter()
+            hps = [td1c_amd.HProd(pr) for pr in probs]
+            wfns = [W.bound_wfn(pr, 4007 + i, hp.lcic) for i, (pr, hp) in enumerate(zip(probs, hps))]
+            for hp, w in zip(hps, wfns):
+                hp.set_wfn0(w)
+            create_s = time.perf_counter() - t0
+            b = td1c_amd.Batch(hps)
+            t0 = time.perf_counter()
+            b.autotune([hp.to_device(w) for hp, w in zip(hps, wfns)], dt, fields)
+            tune_s = time.perf_counter() - t0
+            d = [hp.to_device(w) for hp, w in zip(hps, wfns)]
+            t = 0.0
+            b.propagate(d, t, dt, warmup, fields)
+            t += warmup * dt
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            b.propagate(d, t, dt, steps, fields)
+            torch.cuda.synchronize()
+            ms = (time.perf_counter() - t0) * 1e3 / steps
+            t += steps * dt
+            if gate_steps > steps + warmup:
+                b.propagate(d, t, dt, gate_steps - steps - warmup, fields)
+                t = gate_steps * dt
+            dip = np.array([hp.dipole(td1c_amd.field_value(f, t), x) for hp, f, x in zip(hps, fields, d)])
+            info = b.info()
+            step_bytes = 0.0
+            for pr, hp in zip(probs, hps):
+                s = dict(nr=pr.nr, L1=pr.lmax1 + 1, L2=pr.lmax2 + 1, nlat=pr.lmax2 + 1, nfun=pr.nfun,
+                         ndyn=pr.nfun, npair=hp.npair, nfcore=0, lcic=hp.lcic)
+                step_bytes += 4 * hprod_bytes(s, 8 if p == 1 else 16) + 15 * hp.wfn_size * 16
+            out[p] = {"ms_per_step": ms, "propagations_per_s": n / (STEPS_PER_PROP["he_scan"] * ms * 1e-3),
+                      "hbm_frac_step": step_bytes / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9),
+                      "bytes_per_step": step_bytes, "create_s": create_s, "autotune_s": tune_s, "info": info,
+                      "dipole": dip}
+            progress(f"scan batch precision {p}: {ms:.3f} ms per batch step ({info['sites_per_step']} launches)")
+            b.close()
+            members[p] = (probs, fields, hps, wfns)
+        # the plain FP64 path: every member alone, stream launches, its own
+        # autotuned shapes
+        os.environ["TD1C_AUTOTUNE"] = tune if tune is not None else "1"
+        probs, fields, _, wfns = members[0]
+        alone = []
+        for i, (pr, f, w) in enumerate(zip(probs, fields, wfns)):
+            hp = td1c_amd.HProd(pr)
+            hp.set_wfn0(w)
+            x = hp.to_device(w)
+            rk.propagate(hp, x, 0.0, dt, 2, f)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            rk.propagate(hp, x, 2 * dt, dt, 4, f)
+            torch.cuda.synchronize()
+            alone.append((time.perf_counter() - t0) * 1e3 / 4)
+            hp.close()
+        for p in (0, 1):
+            for hp in members[p][2]:
+                hp.close()
+    finally:
+        if tune is None:
+            os.environ.pop("TD1C_AUTOTUNE", None)
+        else:
+            os.environ["TD1C_AUTOTUNE"] = tune
+    alone_ms = float(np.sum(alone))
+    d0, d1 = out[0]["dipole"], out[1]["dipole"]
+    err = np.abs(d1 - d0)
+    return {
+        "workload": (f"he_scan: {n} He TDHF problems, nfe 40-120 x 21 points (rmax = 2 nfe), lmax1 in "
+                     "{15, 31, 47, 63} (lmax2 = 2 lmax1), velocity-gauge sin2 pulses of 4 cycles, 400-1600 nm, "
+                     "1e13-1e15 W/cm2, random CEP; split2 steps at dt " + f"{dt:g}"),
+        "members": n,
+        "steps_per_propagation": STEPS_PER_PROP["he_scan"],
+        "mixed_batched": {k: v for k, v in out[1].items() if k != "dipole"},
+        "fp64_batched": {k: v for k, v in out[0].items() if k != "dipole"},
+        "fp64_alone": {"ms_per_step_all_members": alone_ms,
+                       "propagations_per_s": n / (STEPS_PER_PROP["he_scan"] * alone_ms * 1e-3),
+                       "ms_per_step_min_max": [float(np.min(alone)), float(np.max(alone))]},
+        "speedup_mixed_batched_vs_fp64_alone": alone_ms / out[1]["ms_per_step"],
+        "speedup_fp64_batched_vs_fp64_alone": alone_ms / out[0]["ms_per_step"],
+        "dipole_gate": {"steps": max(gate_steps, steps + warmup), "dt": dt,
+                        "max_abs_err": float(err.max()),
+                        "max_err_over_max_dipole": float(err.max() / max(np.abs(d0).max(), 1e-300)),
+                        "max_rel_err_per_member": float(np.max(err / np.maximum(np.abs(d0), 1e-12)))},
+    }
+
+
 def progress(msg: str):
     print(f"bench: {msg}", file=sys.stderr, flush=True)
 
@@ -220,7 +322,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the 16-orbital mean-field hprod rates")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--only", choices=["fp64", "mixed"], default=None,
+    ap.add_argument("--scan-members", type=int, default=64, help="problems of the batched scan (0: skip it)")
+    ap.add_argument("--scan-steps", type=int, default=100, help="timed batch steps of the scan")
+    ap.add_argument("--scan-gate-steps", type=int, default=1000,
+                    help="steps after which the scan's mixed and FP64 dipoles are compared")
+    ap.add_argument("--only", choices=["fp64", "mixed", "batch"], default=None,
                     help="measure one path only (for profiler runs; no comparisons)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a device copy of this many bytes (counter calibration)")
@@ -292,6 +398,11 @@ def main():
             raise RuntimeError("propagation produced non-finite values")
         return dict(prob=prob, hp=hp, wfn=wfn, ms_step=(t1 - t0) * 1e3 / args.steps, breakdown=breakdown,
                     nprof=nprof, dname=dname, dtime=times[dname], norm=norm)
+
+    if args.only == "batch":
+        r = scan_batch(args.scan_members, dt, args.warmup, args.scan_steps, args.scan_gate_steps)
+        print(json.dumps({"only": "batch", **r}), flush=True)
+        return
 
     if args.only is not None:
         r = measure(0 if args.only == "fp64" else 1)

@@ -356,7 +356,16 @@ int td1c_batch_set_graph(td1c_batch* batch, int on);
  * every choice gives the same bits. */
 int td1c_batch_set_shapes(td1c_batch* batch, int elements_per_block, int l_per_block, int transform_variant);
 
-/* JSON: members, launch sites and blocks per step, table bytes, shapes. */
+/* Chooses the batch's launch shapes by timing: every element one-body shape
+ * (3, 6, 12 elements x 2, 4, 8, 16 l values per block), then the other
+ * transform variant, each as one warm and two timed batch steps of dtime
+ * on the scratch wave functions d_scratch[m] (device copies the caller
+ * provides; they are overwritten).  Synchronises the stream. */
+int td1c_batch_autotune(td1c_batch* batch, const td1c_field* fields, int rk_order, double dtime,
+                        double* const* d_scratch, void* stream);
+
+/* JSON: members, launch sites and blocks per step, table bytes, shapes, the
+ * members reading shared element tables, the autotuner's timings. */
 int td1c_batch_get_info(const td1c_batch* batch, char* buf, int buflen);
 
 /* How td1c_propagate launches its steps: 0 (default) enqueues every kernel

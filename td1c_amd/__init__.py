@@ -129,6 +129,9 @@ def load_library():
     L.td1c_batch_set_graph.restype = ctypes.c_int
     L.td1c_batch_set_shapes.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     L.td1c_batch_set_shapes.restype = ctypes.c_int
+    L.td1c_batch_autotune.argtypes = [vp, ctypes.POINTER(TD1CField), ctypes.c_int, ctypes.c_double,
+                                      ctypes.POINTER(vp), vp]
+    L.td1c_batch_autotune.restype = ctypes.c_int
     L.td1c_batch_get_info.argtypes = [vp, ctypes.c_char_p, ctypes.c_int]
     L.td1c_batch_get_info.restype = ctypes.c_int
     L.td1c_last_error.argtypes = []
@@ -544,6 +547,20 @@ class Batch:
         _check(load_library().td1c_batch_set_shapes(self.h, int(elements_per_block), int(l_per_block),
                                                     int(transform_variant)), "td1c_batch_set_shapes")
 
+    def autotune(self, scratch, dtime: float, fields: Sequence[Field], rk_order: int = 4, stream=None):
+        """Times the batch's launch shapes on the scratch device wave
+        functions (overwritten) and keeps the fastest."""
+        n = len(self.members)
+        cf = (TD1CField * n)(*[f.to_c() for f in fields])
+        ptrs = (ctypes.c_void_p * n)(*[w.data_ptr() for w in scratch])
+        _check(load_library().td1c_batch_autotune(self.h, cf, int(rk_order), float(dtime), ptrs, self._stream(stream)),
+               "td1c_batch_autotune")
+
+    def _stream(self, stream):
+        if stream is None:
+            return ctypes.c_void_p(self.torch.cuda.current_stream().cuda_stream)
+        return ctypes.c_void_p(stream.cuda_stream if hasattr(stream, "cuda_stream") else stream)
+
     def info(self) -> dict:
         import json
 
@@ -561,10 +578,6 @@ class Batch:
             raise ValueError("need one wave function and one field per member")
         cf = (TD1CField * n)(*[f.to_c() for f in fields])
         ptrs = (ctypes.c_void_p * n)(*[w.data_ptr() for w in wfns])
-        if stream is None:
-            st = ctypes.c_void_p(self.torch.cuda.current_stream().cuda_stream)
-        else:
-            st = ctypes.c_void_p(stream.cuda_stream if hasattr(stream, "cuda_stream") else stream)
         _check(load_library().td1c_batch_propagate(self.h, cf, int(rk_order), float(time), float(dtime), int(nstep),
-                                                   ptrs, int(apply_mask), st), "td1c_batch_propagate")
+                                                   ptrs, int(apply_mask), self._stream(stream)), "td1c_batch_propagate")
         return wfns

@@ -133,6 +133,7 @@ struct td1c_ctx {
   DevBuf<double> cn_part, lf_cn;
   DevBuf<int> cn_flag;
   std::vector<double> v2jfc_h;  // frozen-core Hartree potential (complex, host), empty without frozen core
+  std::vector<double> cnG_h;    // host copy of cnG (complex, [L1][nfe][k][i])
   // device copies of hprod_htot_'s host arguments (wfn | hwfn | cic | hcic),
   // allocated by its first call and kept with the context
   DevBuf<double2> legacy_stage;
@@ -221,12 +222,19 @@ struct td1c_batch {
   struct Site {
     LaunchRec proto;  // kind, variant and block of every member's launch
     int total = 0;    // blocks over all members
+    int max_blocks = 0;  // blocks of the largest member
     size_t lds = 0;
     size_t off_recs = 0, off_start = 0, off_dims = 0;
   };
   std::vector<Site> sites;
   DevBuf<unsigned char> table;
   size_t table_bytes = 0;
+  // element tables shared by the members whose own tables equal them
+  // (equal element boundaries, charges and dt give equal element blocks):
+  // Crank-Nicolson and Poisson interior inverses, element one-body matrices
+  DevBuf<double2> u_cnG;
+  DevBuf<double> u_pG, u_eK0, u_eK, u_eP;
+  int shared[4] = {0, 0, 0, 0};  // members reading each shared table
   std::vector<unsigned char> key;  // what the recorded step depends on
   hipStream_t cap_stream = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -235,6 +243,11 @@ struct td1c_batch {
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (cap_stream) (void)hipStreamDestroy(cap_stream);
     table.release();
+    u_cnG.release();
+    u_pG.release();
+    u_eK0.release();
+    u_eK.release();
+    u_eP.release();
   }
 };
 
@@ -761,7 +774,7 @@ int enqueue_htotx(td1c_ctx* c, double dtime, const double* d_lf, const double2* 
     PoissonArgs pa{};
     pa.nr = nr; pa.n = S.n; pa.nfe = S.nfe; pa.ni = S.ni; pa.L2 = L2; pa.npair = c->npair;
     pa.pair_m = c->pair_m.p;
-    pa.G = c->pG.p; pa.Wl = c->pWl.p; pa.Wr = c->pWr.p; pa.WlI = c->pWlI.p; pa.WrI = c->pWrI.p;
+    pa.G = c->pG.p; pa.gstride = S.nfe; pa.Wl = c->pWl.p; pa.Wr = c->pWr.p; pa.WlI = c->pWlI.p; pa.WrI = c->pWrI.p;
     pa.Sd = c->pSd.p; pa.Sl = c->pSl.p; pa.Su = c->pSu.p;
     pa.pcr_steps = S.pcr_steps; pa.pcrA = c->pcrA.p; pa.pcrG = c->pcrG.p; pa.pcrBinv = c->pcrBinv.p;
     pa.invr = c->invr.p; pa.rpl0 = c->rpl0.p; pa.rpl1 = c->rpl1.p; pa.fac2 = c->fac2.p; pa.fac1 = c->fac1.p;
@@ -1731,6 +1744,7 @@ static int cn_prepare(td1c_ctx* c, double dtime) {
     return hipMemcpy(b.p, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice);
   };
   HIP_TRY(up(c->cnG, T.G));
+  c->cnG_h = std::move(T.G);  // host copy: batches share equal tables
   HIP_TRY(up(c->cnWl, T.Wl));
   HIP_TRY(up(c->cnWr, T.Wr));
   HIP_TRY(up(c->cnPa, T.pa));
@@ -1766,7 +1780,7 @@ static CnArgs cn_args(td1c_ctx* c, const double* d_lf, double2* wfn) {
   a.mval = c->mval.p;
   a.band_k = c->band_k.p; a.band_k0 = c->band_k0.p; a.band_p = c->band_p.p; a.tdiag = c->tdiag.p;
   a.xr = c->xr.p; a.alph = c->alph.p; a.lf = d_lf;
-  a.G = c->cnG.p; a.Wl = c->cnWl.p; a.Wr = c->cnWr.p; a.pa = c->cnPa.p; a.pg = c->cnPg.p; a.binv = c->cnBinv.p;
+  a.G = c->cnG.p; a.gstride = S.nfe; a.Wl = c->cnWl.p; a.Wr = c->cnWr.p; a.pa = c->cnPa.p; a.pg = c->cnPg.p; a.binv = c->cnBinv.p;
   a.wfn = wfn; a.x = c->cn_x.p; a.y = c->cn_y.p;
   const double q = 0.25 * c->cn_dt;
   a.zfac1 = S.icomp == 1 ? make_double2(0.0, -q) : make_double2(-q, 0.0);
@@ -2108,6 +2122,149 @@ void key_put(std::vector<unsigned char>& k, const T& v) {
 }
 }
 
+// Blocks [row][elem] of blk doubles gathered from tables of rows x elems
+// blocks: a table joins when each of its blocks equals the block already
+// there (bitwise), and fills the blocks nobody had.
+struct TableUnion {
+  int rows = 0, elems = 0;
+  size_t blk = 0;
+  std::vector<double> v;
+  std::vector<char> set;
+  void init(int r, int e, size_t b) {
+    rows = r; elems = e; blk = b;
+    v.assign((size_t)r * e * b, 0.0);
+    set.assign((size_t)r * e, 0);
+  }
+  bool add(const double* t, int r, int e) {
+    if (r > rows || e > elems) return false;
+    for (int i = 0; i < r; ++i)
+      for (int k = 0; k < e; ++k) {
+        const size_t u = (size_t)i * elems + k;
+        if (set[u] && std::memcmp(&v[u * blk], t + ((size_t)i * e + k) * blk, blk * sizeof(double)) != 0) return false;
+      }
+    for (int i = 0; i < r; ++i)
+      for (int k = 0; k < e; ++k) {
+        const size_t u = (size_t)i * elems + k;
+        if (!set[u]) {
+          std::memcpy(&v[u * blk], t + ((size_t)i * e + k) * blk, blk * sizeof(double));
+          set[u] = 1;
+        }
+      }
+    return true;
+  }
+};
+
+extern "C++" {
+template <class R, class F>
+void patch_recs(std::vector<LaunchRec>& recs, int kind0, int kind1, F f) {
+  for (LaunchRec& L : recs) {
+    if (L.kind != kind0 && L.kind != kind1) continue;
+    R r;
+    std::memcpy(&r, L.args.data(), sizeof(R));
+    f(r);
+    std::memcpy(L.args.data(), &r, sizeof(R));
+  }
+}
+}
+
+// Points the members' recorded element-table arguments at shared copies
+// where their own tables are equal: a scan over pulses on grids with the
+// same element boundaries then reads each element block once per launch.
+int batch_share_tables(td1c_batch* b, std::vector<std::vector<LaunchRec>>& recs) {
+  const int nmem = (int)b->mem.size();
+  int L1 = 0, L2 = 0, ne = 0;
+  for (td1c_ctx* c : b->mem) {
+    L1 = std::max(L1, c->S.L1);
+    L2 = std::max(L2, c->S.L2);
+    ne = std::max(ne, c->S.nfe);
+  }
+  const int ni = b->mem[0]->S.n - 1, np = b->mem[0]->S.n + 1;
+  for (int& s : b->shared) s = 0;
+  // Crank-Nicolson interior inverses (complex), [l][e][k][i]
+  if (b->mem[0]->S.isplit == 1) {
+    TableUnion u;
+    u.init(L1, ne, (size_t)2 * ni * ni);
+    std::vector<char> ok(nmem, 0);
+    for (int m = 0; m < nmem; ++m) {
+      td1c_ctx* c = b->mem[m];
+      if (c->cnG_h.size() == (size_t)c->S.L1 * c->S.nfe * 2 * ni * ni) ok[m] = u.add(c->cnG_h.data(), c->S.L1, c->S.nfe);
+    }
+    b->u_cnG.release();
+    HIP_TRY(b->u_cnG.alloc(u.v.size() / 2));
+    HIP_TRY(hipMemcpy(b->u_cnG.p, u.v.data(), u.v.size() * sizeof(double), hipMemcpyHostToDevice));
+    for (int m = 0; m < nmem; ++m)
+      if (ok[m]) {
+        ++b->shared[0];
+        patch_recs<CnRec>(recs[m], kRecCnBridge, kRecCnInterior, [&](CnRec& r) {
+          r.a.G = b->u_cnG.p;
+          r.a.gstride = ne;
+        });
+      }
+  }
+  // Poisson interior inverses, transposed as on the device, [L][e][k][i]
+  {
+    TableUnion u;
+    u.init(L2, ne, (size_t)ni * ni);
+    std::vector<char> ok(nmem, 0);
+    for (int m = 0; m < nmem; ++m) {
+      const Setup& S = b->mem[m]->S;
+      if (S.pG.size() != (size_t)S.L2 * S.nfe * ni * ni) continue;
+      std::vector<double> gt(S.pG.size());
+      for (size_t blk = 0; blk < S.pG.size() / ((size_t)ni * ni); ++blk)
+        for (int i = 0; i < ni; ++i)
+          for (int k = 0; k < ni; ++k) gt[blk * ni * ni + (size_t)k * ni + i] = S.pG[blk * ni * ni + (size_t)i * ni + k];
+      ok[m] = u.add(gt.data(), S.L2, S.nfe);
+    }
+    b->u_pG.release();
+    HIP_TRY(b->u_pG.upload(u.v));
+    for (int m = 0; m < nmem; ++m)
+      if (ok[m]) {
+        ++b->shared[1];
+        patch_recs<PoissonRec>(recs[m], kRecPoissonBridge, kRecPoissonInterior, [&](PoissonRec& r) {
+          r.a.G = b->u_pG.p;
+          r.a.gstride = ne;
+        });
+      }
+  }
+  // element one-body matrices, [e][k][m] each
+  {
+    TableUnion u0, u1, u2;
+    u0.init(1, ne, (size_t)np * np);
+    u1.init(1, ne, (size_t)np * np);
+    u2.init(1, ne, (size_t)np * np);
+    std::vector<char> ok(nmem, 0);
+    for (int m = 0; m < nmem; ++m) {
+      const Setup& S = b->mem[m]->S;
+      const size_t want = (size_t)S.nfe * np * np;
+      if (S.eK0.size() != want || S.eK.size() != want || S.eP.size() != want) continue;
+      // the three tables join together or not at all
+      TableUnion t0 = u0, t1 = u1, t2 = u2;
+      if (t0.add(S.eK0.data(), 1, S.nfe) && t1.add(S.eK.data(), 1, S.nfe) && t2.add(S.eP.data(), 1, S.nfe)) {
+        u0 = std::move(t0);
+        u1 = std::move(t1);
+        u2 = std::move(t2);
+        ok[m] = 1;
+      }
+    }
+    b->u_eK0.release();
+    b->u_eK.release();
+    b->u_eP.release();
+    HIP_TRY(b->u_eK0.upload(u0.v));
+    HIP_TRY(b->u_eK.upload(u1.v));
+    HIP_TRY(b->u_eP.upload(u2.v));
+    for (int m = 0; m < nmem; ++m)
+      if (ok[m]) {
+        ++b->shared[2];
+        patch_recs<OneBodyRec>(recs[m], kRecOneBody, kRecOneBody, [&](OneBodyRec& r) {
+          r.a.eK0 = b->u_eK0.p;
+          r.a.eK = b->u_eK.p;
+          r.a.eP = b->u_eP.p;
+        });
+      }
+  }
+  return 0;
+}
+
 // Records one step of every member and lays the sites out in a device table:
 // per site, the members' argument records, then start[nmem + 1], then
 // dims[nmem][3], each 256-byte aligned; identical sites share one block.
@@ -2152,6 +2309,9 @@ int batch_record(td1c_batch* b, const std::vector<Pulse>& pulses, int rk_order, 
         return fail(-1, "batch members record different launches");
     }
   }
+  // the old tables may still be read by queued launches
+  HIP_TRY(hipDeviceSynchronize());
+  if (int rc = batch_share_tables(b, recs)) return rc;
   auto align = [](size_t x) { return (x + 255) / 256 * 256; };
   std::vector<unsigned char> host;
   std::map<std::string, size_t> seen;
@@ -2169,14 +2329,16 @@ int batch_record(td1c_batch* b, const std::vector<Pulse>& pulses, int rk_order, 
       const LaunchRec& r = recs[m][i];
       start[m] = (int)tot;
       for (int d = 0; d < 3; ++d) dims[3 * m + d] = r.grid[d];
-      tot += (long long)r.grid[0] * r.grid[1] * r.grid[2];
+      const long long nb = (long long)r.grid[0] * r.grid[1] * r.grid[2];
+      tot += nb;
+      st.max_blocks = std::max(st.max_blocks, (int)std::min<long long>(nb, 0x7fffffffLL));
       lds = std::max(lds, r.lds);
     }
     if (tot <= 0 || tot > 0x7fffffffLL) return fail(-1, "batched launch too large");
     start[nmem] = (int)tot;
     st.total = (int)tot;
     st.lds = lds;
-    BatchMap probe{nullptr, nullptr, nmem};
+    BatchMap probe{nullptr, nullptr, nmem, 0};
     if (!launch_batch_kernels(st.proto, nullptr, probe, 0, 0, nullptr, true) &&
         !launch_batch_h1itr(st.proto, nullptr, probe, 0, 0, nullptr, true) &&
         !launch_batch_island(st.proto, nullptr, probe, 0, 0, nullptr, true))
@@ -2222,12 +2384,18 @@ hipError_t batch_launch_step(td1c_batch* b, hipStream_t s) {
   const int nmem = (int)b->mem.size();
   for (const td1c_batch::Site& st : b->sites) {
     const unsigned char* base = b->table.p;
-    const BatchMap map{reinterpret_cast<const int*>(base + st.off_start), reinterpret_cast<const int*>(base + st.off_dims),
-                       nmem};
+    BatchMap map{reinterpret_cast<const int*>(base + st.off_start), reinterpret_cast<const int*>(base + st.off_dims),
+                 nmem, 0};
     const void* recs = base + st.off_recs;
-    if (!launch_batch_kernels(st.proto, recs, map, st.total, st.lds, s, false) &&
-        !launch_batch_h1itr(st.proto, recs, map, st.total, st.lds, s, false) &&
-        !launch_batch_island(st.proto, recs, map, st.total, st.lds, s, false))
+    int grid = st.total;
+    if (batch_skips(st.proto)) {
+      // two blocks of the largest member per hardware block
+      map.per_member = std::max(1, (st.max_blocks + 1) / 2);
+      grid = (int)std::min<long long>((long long)nmem * map.per_member, 0x7fffffffLL);
+    }
+    if (!launch_batch_kernels(st.proto, recs, map, grid, st.lds, s, false) &&
+        !launch_batch_h1itr(st.proto, recs, map, grid, st.lds, s, false) &&
+        !launch_batch_island(st.proto, recs, map, grid, st.lds, s, false))
       return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -2306,6 +2474,61 @@ int td1c_batch_propagate(td1c_batch* b, const td1c_field* fields, int rk_order, 
   return 0;
 }
 
+int td1c_batch_autotune(td1c_batch* b, const td1c_field* fields, int rk_order, double dtime,
+                        double* const* d_scratch, void* stream) {
+  if (!b || !fields || !d_scratch) return fail(-1, "null argument");
+  hipStream_t s = (hipStream_t)stream;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  std::string log = "[";
+  int rc = 0;
+  float best = 1e30f;
+  int bs = b->ob_shape, bl = b->ob_lc, bt = b->tvar;
+  auto trial = [&](int shape, int lc, int tvar) -> int {
+    b->ob_shape = shape;
+    b->ob_lc = lc;
+    b->tvar = tvar;
+    b->key.clear();
+    // record and warm (one step), then two timed steps, on the scratch states
+    if (int r = td1c_batch_propagate(b, fields, rk_order, 0.0, dtime, 1, d_scratch, 1, stream)) return r;
+    HIP_TRY(hipEventRecord(e0, s));
+    if (int r = td1c_batch_propagate(b, fields, rk_order, dtime, dtime, 2, d_scratch, 1, stream)) return r;
+    HIP_TRY(hipEventRecord(e1, s));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    ms *= 0.5f;
+    static const int ge[3] = {3, 6, 12};
+    char buf[96];
+    std::snprintf(buf, sizeof buf, "%s[%d, %d, %d, %.4f]", log.size() > 1 ? ", " : "", ge[shape], lc, tvar, ms);
+    log += buf;
+    if (ms < best) {
+      best = ms;
+      bs = shape;
+      bl = lc;
+      bt = tvar;
+    }
+    return 0;
+  };
+  // the one-body shape first (transform variant fixed), then the transform variant
+  const int t0 = b->tvar;
+  for (int shape = 0; shape < 3 && !rc; ++shape)
+    for (int lc : {2, 4, 8, 16}) {
+      rc = trial(shape, lc, t0);
+      if (rc) break;
+    }
+  if (!rc) rc = trial(bs, bl, 1 - t0);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  b->ob_shape = bs;
+  b->ob_lc = bl;
+  b->tvar = bt;
+  b->key.clear();  // the next call records with its own wave functions
+  b->tune_log = "{\"ms_per_step\": " + log + "]}";
+  return rc;
+}
+
 int td1c_batch_get_info(const td1c_batch* b, char* buf, int buflen) {
   if (!b || !buf || buflen <= 0) return fail(-1, "null argument");
   long long blocks = 0;
@@ -2315,7 +2538,9 @@ int td1c_batch_get_info(const td1c_batch* b, char* buf, int buflen) {
                    std::to_string(b->sites.size()) + ", \"blocks_per_step\": " + std::to_string(blocks) +
                    ", \"table_bytes\": " + std::to_string(b->table_bytes) + ", \"onebody\": [" + ge[b->ob_shape] +
                    ", " + std::to_string(b->ob_lc) + "], \"transform_variant\": " + std::to_string(b->tvar) +
-                   ", \"graph\": " + (b->use_graph ? "true" : "false") +
+                   ", \"graph\": " + (b->use_graph ? "true" : "false") + ", \"shared_tables\": {\"cn\": " +
+                   std::to_string(b->shared[0]) + ", \"poisson\": " + std::to_string(b->shared[1]) +
+                   ", \"onebody\": " + std::to_string(b->shared[2]) + "}" +
                    (b->tune_log.empty() ? std::string() : ", \"tune\": " + b->tune_log) + "}";
   if ((int)js.size() + 1 > buflen) return fail(-2, "buffer too small");
   std::memcpy(buf, js.c_str(), js.size() + 1);

@@ -10,12 +10,12 @@
 // same operator is solved by element Schur complement: the interior blocks
 // of each FEDVR element are eliminated with their precomputed inverses and
 // the bridge points form a tridiagonal system solved by cyclic reduction.
-// One term of the series is three launches: the right-hand side (the
+// One term of the series is four launches: the right-hand side (the
 // element one-body kernel in its Crank-Nicolson modes, or a band kernel for
 // ndvr != 20), the bridge pass (one block per orbital row), the interior
 // pass (one block per (l, element chunk), each element inverse read once for
-// every orbital).  The interior pass's last block tests convergence; every later
-// launch of the half step returns at once after it.
+// every orbital) and the convergence test (one block); every later launch of
+// the half step returns at once after convergence.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -217,10 +217,9 @@ __device__ __forceinline__ void cn_interior_body(const CnArgs& a, int mode, int 
   __shared__ double2 sx[PG][34];  // bridges e0-1 .. e0+epb-1 (epb <= 32)
   __shared__ int sval[PG];
   __shared__ double sRed[BS / 64];
-  __shared__ int slast;
   double2 gk[NI > 0 ? NI : 1];
   double2 wl = make_double2(0.0, 0.0), wr = make_double2(0.0, 0.0);
-  const double2* ge = a.G + (((size_t)l * a.nfe) + (row_ok ? e : 0)) * ni * ni + i;
+  const double2* ge = a.G + (((size_t)l * a.gstride) + (row_ok ? e : 0)) * ni * ni + i;
   if (row_ok) {
     if (NI > 0) {
 #pragma unroll
@@ -284,19 +283,25 @@ __device__ __forceinline__ void cn_interior_body(const CnArgs& a, int mode, int 
 #pragma unroll
     for (int w = 0; w < BS / 64; ++w) s += sRed[w];
     a.part[a.npart1 + bk.x] = s;
-    __threadfence();
-    const int tk = atomicAdd(&a.flag[1], 1);
-    slast = tk == (int)bk.nx - 1;
   }
-  __syncthreads();
-  if (!slast) return;
-  // the last block: every partial sum is written; fixed-order reduction
-  __threadfence();
-  const int ntot = a.npart1 + (int)bk.nx;
+}
+
+// Convergence test of the term just formed (the test at the top of the next
+// cycle of h1itr_prop.F90:75-99): the partial norms of the bridge pass
+// (npart1 of them) and of the nblk interior blocks summed in a fixed order,
+// flag[0] set when the RMS orbital norm is below thresh.  A launch of its own
+// after the interior pass, so that the stream order makes the partial sums
+// visible (a last-block ticket would need a device-scope release fence in
+// every block of the interior pass).
+__device__ __forceinline__ void cn_check_body(const CnArgs& a, int mode, int nblk) {
+  if (mode == 1 && a.flag[0]) return;
+  constexpr int BS = kCnInteriorThreads;
+  __shared__ double sRed[BS / 64];
+  const int t = threadIdx.x;
+  const int ntot = a.npart1 + nblk;
   double s = 0.0;
   for (int q = t; q < ntot; q += BS) s += a.part[q];
   for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-  __syncthreads();
   if ((t & 63) == 0) sRed[t >> 6] = s;
   __syncthreads();
   if (t == 0) {
@@ -305,8 +310,11 @@ __device__ __forceinline__ void cn_interior_body(const CnArgs& a, int mode, int 
     for (int w = 0; w < BS / 64; ++w) tot += sRed[w];
     const double rms = sqrt(tot / a.nfun);
     a.flag[0] = rms < a.thresh ? 1 : 0;
-    a.flag[1] = 0;
   }
+}
+
+__global__ void __launch_bounds__(kCnInteriorThreads) k_cn_check(const CnArgs a, int mode, int nblk) {
+  cn_check_body(a, mode, nblk);
 }
 
 template <int NI>
@@ -339,9 +347,10 @@ void launch_cn_stage(const CnArgs& a, const OneBodyElArgs* ob, int shape, int lc
     CnRec r;
     std::memset(&r, 0, sizeof r);
     r.a = a; r.mode = mode; r.epb = epb; r.nchunk = nchunk;
-    const int var[4] = {0, 0, 0, 0};
+    const int var[4] = {mode, 0, 0, 0};
     record_launch(kRecCnBridge, var, dim3(nd * a.L1), 256, 0, r);
     record_launch(kRecCnInterior, var, grid, kCnInteriorThreads, 0, r);
+    record_launch(kRecCnCheck, var, dim3(1), kCnInteriorThreads, 0, r);
     return;
   }
   hipLaunchKernelGGL((k_cn_bridge<256>), dim3(nd * a.L1), dim3(256), 0, s, a, mode);
@@ -349,30 +358,47 @@ void launch_cn_stage(const CnArgs& a, const OneBodyElArgs* ob, int shape, int lc
     hipLaunchKernelGGL((k_cn_interior<19>), grid, dim3(kCnInteriorThreads), 0, s, a, mode, epb, nchunk);
   else
     hipLaunchKernelGGL((k_cn_interior<0>), grid, dim3(kCnInteriorThreads), 0, s, a, mode, epb, nchunk);
+  hipLaunchKernelGGL(k_cn_check, dim3(1), dim3(kCnInteriorThreads), 0, s, a, mode, (int)grid.x);
 }
 
 __global__ void __launch_bounds__(256) k_cn_bridge_batch(const CnRec* __restrict__ recs, const BatchMap map) {
-  int m;
-  const Blk bk = batch_blk(map, m);
-  const CnRec& r = recs[m];
-  cn_bridge_body<256>(r.a, r.mode, bk);
+  __shared__ __align__(16) unsigned char raw[sizeof(CnRec)];
+  int have = -1;
+  batch_walk(map, [&](int m) { return recs[m].mode == 1 && recs[m].a.flag[0] != 0; }, [&](int m, const Blk& bk) {
+    const CnRec& r = stage_rec(recs, m, raw, have);
+    cn_bridge_body<256>(r.a, r.mode, bk);
+  });
 }
 
 __global__ void __launch_bounds__(kCnInteriorThreads) k_cn_interior_batch(const CnRec* __restrict__ recs,
                                                                           const BatchMap map) {
-  int m;
-  const Blk bk = batch_blk(map, m);
-  const CnRec& r = recs[m];
-  cn_interior_body<19>(r.a, r.mode, r.epb, r.nchunk, bk);
+  __shared__ __align__(16) unsigned char raw[sizeof(CnRec)];
+  int have = -1;
+  batch_walk(map, [&](int m) { return recs[m].mode == 1 && recs[m].a.flag[0] != 0; }, [&](int m, const Blk& bk) {
+    const CnRec& r = stage_rec(recs, m, raw, have);
+    cn_interior_body<19>(r.a, r.mode, r.epb, r.nchunk, bk);
+  });
 }
 
-bool launch_batch_h1itr(const LaunchRec& p, const void* recs, const BatchMap& map, int total, size_t lds,
+__global__ void __launch_bounds__(kCnInteriorThreads) k_cn_check_batch(const CnRec* __restrict__ recs,
+                                                                       const BatchMap map) {
+  __shared__ __align__(16) unsigned char raw[sizeof(CnRec)];
+  int have = -1;
+  batch_walk(map, [&](int m) { return recs[m].mode == 1 && recs[m].a.flag[0] != 0; }, [&](int m, const Blk& bk) {
+    const CnRec& r = stage_rec(recs, m, raw, have);
+    cn_check_body(r.a, r.mode, r.a.L1 * r.nchunk);
+  });
+}
+
+bool launch_batch_h1itr(const LaunchRec& p, const void* recs, const BatchMap& map, int grid, size_t lds,
                         hipStream_t s, bool dry) {
   const CnRec* r = static_cast<const CnRec*>(recs);
   if (p.kind == kRecCnBridge && p.block == 256) {
-    if (!dry) hipLaunchKernelGGL(k_cn_bridge_batch, dim3(total), dim3(256), lds, s, r, map);
+    if (!dry) hipLaunchKernelGGL(k_cn_bridge_batch, dim3(grid), dim3(256), lds, s, r, map);
   } else if (p.kind == kRecCnInterior && p.block == kCnInteriorThreads) {
-    if (!dry) hipLaunchKernelGGL(k_cn_interior_batch, dim3(total), dim3(kCnInteriorThreads), lds, s, r, map);
+    if (!dry) hipLaunchKernelGGL(k_cn_interior_batch, dim3(grid), dim3(kCnInteriorThreads), lds, s, r, map);
+  } else if (p.kind == kRecCnCheck && p.block == kCnInteriorThreads) {
+    if (!dry) hipLaunchKernelGGL(k_cn_check_batch, dim3(grid), dim3(kCnInteriorThreads), lds, s, r, map);
   } else {
     return false;
   }

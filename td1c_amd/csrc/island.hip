@@ -527,16 +527,19 @@ void launch_island_b(const IslandArgs& a, hipStream_t s) {
 }
 
 __global__ void __launch_bounds__(kT) k_island_b_batch(const IslandArgs* __restrict__ recs, const BatchMap map) {
-  int m;
-  const Blk bk = batch_blk(map, m);
-  island_b_body(recs[m], bk);
+  __shared__ __align__(16) unsigned char raw[sizeof(IslandArgs)];
+  int have = -1;
+  batch_walk(map, NoSkip(), [&](int m, const Blk& bk) {
+    const IslandArgs& r = stage_rec(recs, m, raw, have);
+    island_b_body(r, bk);
+  });
 }
 
-bool launch_batch_island(const LaunchRec& p, const void* recs, const BatchMap& map, int total, size_t lds,
+bool launch_batch_island(const LaunchRec& p, const void* recs, const BatchMap& map, int grid, size_t lds,
                          hipStream_t s, bool dry) {
   if (p.kind != kRecIslandB || p.block != kT) return false;
   if (!dry)
-    hipLaunchKernelGGL(k_island_b_batch, dim3(total), dim3(kT), lds, s, static_cast<const IslandArgs*>(recs), map);
+    hipLaunchKernelGGL(k_island_b_batch, dim3(grid), dim3(kT), lds, s, static_cast<const IslandArgs*>(recs), map);
   return true;
 }
 

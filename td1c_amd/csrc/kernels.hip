@@ -849,7 +849,7 @@ __global__ void __launch_bounds__(1024) k_poisson_elem(const T* __restrict__ rho
   __shared__ double sG[32 * 32];
   __shared__ double sInv[32], sWl[32], sWr[32], sR1[32], sF2[32];
   __shared__ double2 sB[1024];
-  const double* G = a.G + (((size_t)L * a.nfe) + e) * ni * ni;
+  const double* G = a.G + (((size_t)L * a.gstride) + e) * ni * ni;
   const int base = e * a.n;  // interior global index of local point 1
   for (int q = threadIdx.x; q < ni * ni; q += nt) sG[q] = G[q];
   for (int q = threadIdx.x; q < ni; q += nt) {
@@ -1821,7 +1821,7 @@ __device__ __forceinline__ void poisson_interior_body(const T* __restrict__ rho,
   }
   double gk[NI > 0 ? NI : 1];
   double wl = 0.0, wr = 0.0, r1 = 0.0, f2 = 0.0;
-  const double* ge = a.G + (((size_t)L * a.nfe) + (row_ok ? e : 0)) * ni * ni + i;
+  const double* ge = a.G + (((size_t)L * a.gstride) + (row_ok ? e : 0)) * ni * ni + i;
   if (row_ok) {
     if (NI > 0) {
 #pragma unroll
@@ -2368,7 +2368,8 @@ size_t rec_size(int kind) {
     case kRecField: return sizeof(FieldRec);
     case kRecIslandB: return sizeof(IslandArgs);
     case kRecCnBridge:
-    case kRecCnInterior: return sizeof(CnRec);
+    case kRecCnInterior:
+    case kRecCnCheck: return sizeof(CnRec);
     default: return 0;
   }
 }
@@ -2376,99 +2377,122 @@ size_t rec_size(int kind) {
 template <int CT, int U, class TI, class TO>
 __global__ void __launch_bounds__(256) k_transform_mfma_w_batch(const TransformRec* __restrict__ recs,
                                                                 const BatchMap map) {
-  int m;
-  const Blk bk = batch_blk(map, m);
-  const TransformRec& r = recs[m];
-  transform_mfma_w_body<CT, U, TI, TO>(static_cast<const TI*>(r.in), static_cast<TO*>(r.out), r.mats, r.matsf,
-                                       r.batch, r.nr, r.K, r.C, r.Cp, r.copy, r.nx, r.ny, r.ntile, r.fc, bk);
+  __shared__ __align__(16) unsigned char raw[sizeof(TransformRec)];
+  int have = -1;
+  batch_walk(map, NoSkip(), [&](int m, const Blk& bk) {
+    const TransformRec& r = stage_rec(recs, m, raw, have);
+    transform_mfma_w_body<CT, U, TI, TO>(static_cast<const TI*>(r.in), static_cast<TO*>(r.out), r.mats, r.matsf,
+                                         r.batch, r.nr, r.K, r.C, r.Cp, r.copy, r.nx, r.ny, r.ntile, r.fc, bk);
+  });
 }
 
 template <int MODE, class TI, class TO>
 __global__ void __launch_bounds__(256) k_transform_mfma_batch(const TransformRec* __restrict__ recs,
                                                               const BatchMap map) {
-  int m;
-  const Blk bk = batch_blk(map, m);
-  const TransformRec& r = recs[m];
-  transform_mfma_body<MODE, TI, TO>(static_cast<const TI*>(r.in), static_cast<TO*>(r.out), r.mats, r.matsf, r.batch,
-                                    r.nr, r.K, r.C, r.Cp, r.copy, r.nx, r.ny, r.ntile, r.fc, bk);
+  __shared__ __align__(16) unsigned char raw[sizeof(TransformRec)];
+  int have = -1;
+  batch_walk(map, NoSkip(), [&](int m, const Blk& bk) {
+    const TransformRec& r = stage_rec(recs, m, raw, have);
+    transform_mfma_body<MODE, TI, TO>(static_cast<const TI*>(r.in), static_cast<TO*>(r.out), r.mats, r.matsf, r.batch,
+                                      r.nr, r.K, r.C, r.Cp, r.copy, r.nx, r.ny, r.ntile, r.fc, bk);
+  });
 }
 
 template <int GE, int CN>
 __global__ void __launch_bounds__(((GE * 21 + 1) + 63) / 64 * 64)
     k_onebody_el_batch(const OneBodyRec* __restrict__ recs, const BatchMap map) {
-  int m;
-  const Blk bk = batch_blk(map, m);
-  const OneBodyRec& r = recs[m];
-  onebody_el_body<20, GE, CN>(r.orb, r.h01, r.a, r.lc, bk);
+  __shared__ __align__(16) unsigned char raw[sizeof(OneBodyRec)];
+  int have = -1;
+  batch_walk(map, [&](int m) { return CN == 2 && recs[m].a.stop[0] != 0; }, [&](int m, const Blk& bk) {
+    const OneBodyRec& r = stage_rec(recs, m, raw, have);
+    onebody_el_body<20, GE, CN>(r.orb, r.h01, r.a, r.lc, bk);
+  });
 }
 
 template <class T>
 __global__ void __launch_bounds__(kPoissonBlock) k_poisson_bridge_lds_batch(const PoissonRec* __restrict__ recs,
                                                                             const BatchMap map) {
-  int m;
-  const Blk bk = batch_blk(map, m);
-  const PoissonRec& r = recs[m];
-  poisson_bridge_lds_body<19, kPoissonBlock, T>(static_cast<const T*>(r.rho), static_cast<T*>(r.v), r.xb, r.a, bk);
+  __shared__ __align__(16) unsigned char raw[sizeof(PoissonRec)];
+  int have = -1;
+  batch_walk(map, NoSkip(), [&](int m, const Blk& bk) {
+    const PoissonRec& r = stage_rec(recs, m, raw, have);
+    poisson_bridge_lds_body<19, kPoissonBlock, T>(static_cast<const T*>(r.rho), static_cast<T*>(r.v), r.xb, r.a, bk);
+  });
 }
 
 template <class T>
 __global__ void __launch_bounds__(kPoissonInteriorThreads) k_poisson_interior_batch(const PoissonRec* __restrict__ recs,
                                                                                     const BatchMap map) {
-  int m;
-  const Blk bk = batch_blk(map, m);
-  const PoissonRec& r = recs[m];
-  poisson_interior_body<19, T>(static_cast<const T*>(r.rho), static_cast<T*>(r.v), r.xb, r.a, r.epb, r.nchunk, bk);
+  __shared__ __align__(16) unsigned char raw[sizeof(PoissonRec)];
+  int have = -1;
+  batch_walk(map, NoSkip(), [&](int m, const Blk& bk) {
+    const PoissonRec& r = stage_rec(recs, m, raw, have);
+    poisson_interior_body<19, T>(static_cast<const T*>(r.rho), static_cast<T*>(r.v), r.xb, r.a, r.epb, r.nchunk, bk);
+  });
 }
 
 template <int NF, class T>
 __global__ void __launch_bounds__(256) k_mfprod_t_batch(const MfRec* __restrict__ recs, const BatchMap map) {
-  int m;
-  const Blk bk = batch_blk(map, m);
-  const MfRec& r = recs[m];
-  mfprod_t_body<0, NF, 0, T>(static_cast<const T*>(r.orbg), static_cast<const T*>(r.vg), static_cast<T*>(r.gorbg),
-                             static_cast<T*>(r.v2orbg), r.a, bk);
+  __shared__ __align__(16) unsigned char raw[sizeof(MfRec)];
+  int have = -1;
+  batch_walk(map, NoSkip(), [&](int m, const Blk& bk) {
+    const MfRec& r = stage_rec(recs, m, raw, have);
+    mfprod_t_body<0, NF, 0, T>(static_cast<const T*>(r.orbg), static_cast<const T*>(r.vg), static_cast<T*>(r.gorbg),
+                               static_cast<T*>(r.v2orbg), r.a, bk);
+  });
 }
 
 template <class T>
 __global__ void __launch_bounds__(256) k_mfprod_batch(const MfRec* __restrict__ recs, const BatchMap map) {
-  int m;
-  const Blk bk = batch_blk(map, m);
-  const MfRec& r = recs[m];
-  mfprod_body<T>(static_cast<const T*>(r.orbg), static_cast<const T*>(r.vg), static_cast<T*>(r.gorbg),
-                 static_cast<T*>(r.v2orbg), r.a, bk);
+  __shared__ __align__(16) unsigned char raw[sizeof(MfRec)];
+  int have = -1;
+  batch_walk(map, NoSkip(), [&](int m, const Blk& bk) {
+    const MfRec& r = stage_rec(recs, m, raw, have);
+    mfprod_body<T>(static_cast<const T*>(r.orbg), static_cast<const T*>(r.vg), static_cast<T*>(r.gorbg),
+                   static_cast<T*>(r.v2orbg), r.a, bk);
+  });
 }
 
 template <int TA, int TB, bool CR, class T>
 __global__ void __launch_bounds__(256) k_gram_batch(const GramRec* __restrict__ recs, const BatchMap map) {
-  int m;
-  const Blk bk = batch_blk(map, m);
-  const GramRec& r = recs[m];
-  gram_body<TA, TB, CR, T>(r.a, r.partial, r.nqblk, bk);
+  __shared__ __align__(16) unsigned char raw[sizeof(GramRec)];
+  int have = -1;
+  batch_walk(map, NoSkip(), [&](int m, const Blk& bk) {
+    const GramRec& r = stage_rec(recs, m, raw, have);
+    gram_body<TA, TB, CR, T>(r.a, r.partial, r.nqblk, bk);
+  });
 }
 
 __global__ void __launch_bounds__(256) k_dtorb_batch(const DtorbArgs* __restrict__ recs, const BatchMap map) {
-  int m;
-  const Blk bk = batch_blk(map, m);
-  dtorb_body(recs[m], bk);
+  __shared__ __align__(16) unsigned char raw[sizeof(DtorbArgs)];
+  int have = -1;
+  batch_walk(map, NoSkip(), [&](int m, const Blk& bk) {
+    const DtorbArgs& r = stage_rec(recs, m, raw, have);
+    dtorb_body(r, bk);
+  });
 }
 
 __global__ void __launch_bounds__(256) k_mask_batch(const MaskRec* __restrict__ recs, const BatchMap map) {
-  int m;
-  const Blk bk = batch_blk(map, m);
-  const MaskRec& r = recs[m];
-  mask_body(r.wfn, r.mask, r.nr, r.L1, r.nfcore, r.nfun, bk);
+  __shared__ __align__(16) unsigned char raw[sizeof(MaskRec)];
+  int have = -1;
+  batch_walk(map, NoSkip(), [&](int m, const Blk& bk) {
+    const MaskRec& r = stage_rec(recs, m, raw, have);
+    mask_body(r.wfn, r.mask, r.nr, r.L1, r.nfcore, r.nfun, bk);
+  });
 }
 
 __global__ void __launch_bounds__(64) k_field_stages_batch(const FieldRec* __restrict__ recs, const BatchMap map) {
-  int m;
-  const Blk bk = batch_blk(map, m);
-  const FieldRec& r = recs[m];
-  field_stages_body(r.p, r.st, r.nodes, r.lf, bk);
+  __shared__ __align__(16) unsigned char raw[sizeof(FieldRec)];
+  int have = -1;
+  batch_walk(map, NoSkip(), [&](int m, const Blk& bk) {
+    const FieldRec& r = stage_rec(recs, m, raw, have);
+    field_stages_body(r.p, r.st, r.nodes, r.lf, bk);
+  });
 }
 
-bool launch_batch_kernels(const LaunchRec& p, const void* recs, const BatchMap& map, int total, size_t lds,
+bool launch_batch_kernels(const LaunchRec& p, const void* recs, const BatchMap& map, int grid, size_t lds,
                           hipStream_t s, bool dry) {
-  const dim3 g(total), b(p.block);
+  const dim3 g(grid), b(p.block);
 #define TD1C_BATCH(K, R)                                                              \
   do {                                                                                \
     if (!dry) hipLaunchKernelGGL((K), g, b, lds, s, static_cast<const R*>(recs), map); \

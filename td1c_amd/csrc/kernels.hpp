@@ -80,7 +80,8 @@ struct CnArgs {
   const double2* tdiag;                     // [L1][nr] tmat diagonal (+ v2jfc)
   const double *xr, *alph;
   const double* lf;                         // device lfield[9] of this half step
-  const double2 *G;                         // [L1][nfe][k][i] = G_e(i, k)
+  const double2 *G;                         // [L1][gstride][k][i] = G_e(i, k)
+  int gstride;                              // elements per l in G (nfe, or more in a shared table)
   const double2 *Wl, *Wr;                   // [L1][nr]
   const double2 *pa, *pg;                   // [L1][K][nb]
   const double2* binv;                      // [L1][nb]
@@ -89,7 +90,7 @@ struct CnArgs {
   double2* y;                               // [nd][L1][nr] right-hand side
   double2 zfac1;                            // -c
   double thresh;
-  int* flag;                                // [0] converged, [1] block ticket of the interior pass
+  int* flag;                                // [0] converged
   double* part;                             // norm partial sums: bridge jobs, then interior blocks
   int npart1;
 };
@@ -106,7 +107,8 @@ void launch_cn_stage(const CnArgs& a, const OneBodyElArgs* ob, int shape, int lc
 struct PoissonArgs {
   int nr, n, nfe, ni, L2, npair;
   const int* pair_m;  // m_j - m_i of each pair
-  const double *G, *Wl, *Wr, *Sd, *Sl, *Su;  // G: element inverses, transposed [L2][nfe][k][i]
+  const double *G, *Wl, *Wr, *Sd, *Sl, *Su;  // G: element inverses, transposed [L2][gstride][k][i]
+  int gstride;                              // elements per L in G (nfe, or more in a shared table)
   const double *WlI, *WrI;                  // Wl, Wr times invr ([L2][nr])
   int pcr_steps;                            // cyclic reduction of the Schur system
   const double *pcrA, *pcrG, *pcrBinv;      // [L2][pcr_steps][nfe-1] x2, [L2][nfe-1]
@@ -306,19 +308,80 @@ struct BatchMap {
   const int* start;  // [nmem + 1] first block of each member
   const int* dims;   // [nmem][3] the member's grid
   int nmem;
+  // 0: hardware block b walks blocks b, b + gridDim.x, ... of the whole
+  // launch; k > 0: hardware block b serves member b / k alone and walks its
+  // blocks b % k, b % k + k, ... (gridDim.x = nmem * k)
+  int per_member;
 };
-__device__ __forceinline__ Blk batch_blk(const BatchMap& map, int& mem) {
-  const int b = (int)blockIdx.x;
-  int lo = 0, hi = map.nmem;
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (map.start[mid] <= b) lo = mid;
-    else hi = mid;
+// Walks the blocks of a batched launch: block b of member m (start[m] <= b <
+// start[m + 1]) is run as f(m, its coordinates in m's grid) by hardware
+// block b mod gridDim.x, in order, with a barrier between two runs.  A launch
+// with fewer hardware blocks than blocks walks them grid-stride, and the
+// blocks of a member for which skip(m) holds are passed over in O(1) (the
+// half-step series of a converged member).
+template <class Skip, class F>
+__device__ __forceinline__ void batch_walk(const BatchMap& map, Skip skip, F f) {
+  if (map.per_member > 0) {
+    const int k = map.per_member;
+    const int m = (int)blockIdx.x / k;
+    if (m >= map.nmem || skip(m)) return;
+    const int beg = map.start[m], n = map.start[m + 1] - beg;
+    const int nx = map.dims[3 * m], ny = map.dims[3 * m + 1], nz = map.dims[3 * m + 2];
+    bool first = true;
+    for (int q = (int)blockIdx.x - m * k; q < n; q += k) {
+      if (!first) __syncthreads();
+      first = false;
+      f(m, Blk{q % nx, (q / nx) % ny, q / (nx * ny), nx, ny, nz});
+    }
+    return;
   }
-  mem = lo;
-  const int q = b - map.start[lo];
-  const int nx = map.dims[3 * lo], ny = map.dims[3 * lo + 1], nz = map.dims[3 * lo + 2];
-  return Blk{q % nx, (q / nx) % ny, q / (nx * ny), nx, ny, nz};
+  const int total = map.start[map.nmem];
+  const int G = (int)gridDim.x;
+  int m = 0, m_beg = 0, m_end = -1;
+  bool first = true;
+  for (int vb = (int)blockIdx.x; vb < total; vb += G) {
+    if (vb >= m_end) {
+      int lo = 0, hi = map.nmem;
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (map.start[mid] <= vb) lo = mid;
+        else hi = mid;
+      }
+      m = lo;
+      m_beg = map.start[m];
+      m_end = map.start[m + 1];
+      if (skip(m)) {
+        // the first block of this hardware block past member m, less one stride
+        vb += ((m_end - vb + G - 1) / G - 1) * G;
+        continue;
+      }
+    }
+    if (!first) __syncthreads();
+    first = false;
+    const int q = vb - m_beg;
+    const int nx = map.dims[3 * m], ny = map.dims[3 * m + 1], nz = map.dims[3 * m + 2];
+    f(m, Blk{q % nx, (q / nx) % ny, q / (nx * ny), nx, ny, nz});
+  }
+}
+struct NoSkip {
+  __device__ bool operator()(int) const { return false; }
+};
+// Member m's argument record, copied into the block's LDS buffer `raw` when
+// the member changes (have: the member held).  The kernel body then reads
+// its arguments from LDS, which its stores to global memory cannot alias, so
+// they stay in registers or cheap LDS reads instead of being reloaded from
+// global memory after every store.
+template <class R>
+__device__ __forceinline__ const R& stage_rec(const R* __restrict__ recs, int m, unsigned char* raw, int& have) {
+  static_assert(sizeof(R) % 4 == 0, "argument records are whole words");
+  if (m != have) {
+    const int* src = reinterpret_cast<const int*>(recs + m);
+    int* dst = reinterpret_cast<int*>(raw);
+    for (int i = (int)threadIdx.x; i < (int)(sizeof(R) / 4); i += (int)blockDim.x) dst[i] = src[i];
+    __syncthreads();
+    have = m;
+  }
+  return *reinterpret_cast<const R*>(raw);
 }
 
 enum RecKind {
@@ -336,6 +399,7 @@ enum RecKind {
   kRecIslandB,
   kRecCnBridge,
   kRecCnInterior,
+  kRecCnCheck,
 };
 // type codes of (input, output): 0 (c128, c128), 1 (c128, c64), 2 (c64, c64), 3 (c64, c128)
 struct TransformRec {
@@ -421,15 +485,24 @@ inline void record_unsupported(const char* what) {
   if (r->error.empty()) r->error = what;
 }
 // One batched launch of site `p` over nmem members: recs holds nmem
-// argument records of the site's kind, map the block ranges; `total` blocks.
+// argument records of the site's kind, map the block ranges; `grid`
+// hardware blocks (the blocks of all members, or fewer for a site that
+// batch_skips() says skips converged members).
 // Returns false when the kind or variant has no batched kernel in that
 // file; dry: only report whether it has one.
-bool launch_batch_kernels(const LaunchRec& p, const void* recs, const BatchMap& map, int total, size_t lds,
+bool launch_batch_kernels(const LaunchRec& p, const void* recs, const BatchMap& map, int grid, size_t lds,
                           hipStream_t s, bool dry);
-bool launch_batch_h1itr(const LaunchRec& p, const void* recs, const BatchMap& map, int total, size_t lds,
+bool launch_batch_h1itr(const LaunchRec& p, const void* recs, const BatchMap& map, int grid, size_t lds,
                         hipStream_t s, bool dry);
-bool launch_batch_island(const LaunchRec& p, const void* recs, const BatchMap& map, int total, size_t lds,
+bool launch_batch_island(const LaunchRec& p, const void* recs, const BatchMap& map, int grid, size_t lds,
                          hipStream_t s, bool dry);
+// sites whose blocks return at once for a converged member (the later terms
+// of the half-step series): launched member-major (BatchMap::per_member), so
+// that a converged member costs one flag read per hardware block
+inline bool batch_skips(const LaunchRec& p) {
+  return ((p.kind == kRecCnBridge || p.kind == kRecCnInterior || p.kind == kRecCnCheck) && p.var[0] == 1) ||
+         (p.kind == kRecOneBody && p.var[1] == 2);
+}
 // the argument record size of a kind
 size_t rec_size(int kind);
 

@@ -50,9 +50,9 @@ def _members(probs):
     return hps, ws
 
 
-def _run_pair(probs, graph, steps=(2, 3), t0=20.0):
+def _run_pair(probs, graph, steps=(2, 3), t0=20.0, tune=False, info=None):
     """The batch and the members alone from the same states; returns the two
-    lists of final states."""
+    lists of final states (info: gets the batch's get_info)."""
     import torch
     import td1c_amd
 
@@ -60,6 +60,8 @@ def _run_pair(probs, graph, steps=(2, 3), t0=20.0):
     fields = _fields(len(probs))
     b = td1c_amd.Batch(hps)
     b.set_graph(graph)
+    if tune:
+        b.autotune([hp.to_device(w) for hp, w in zip(hps, ws)], DT, fields)
     db = [hp.to_device(w) for hp, w in zip(hps, ws)]
     t = t0
     for n in steps:
@@ -67,6 +69,8 @@ def _run_pair(probs, graph, steps=(2, 3), t0=20.0):
         t += n * DT
     torch.cuda.synchronize()
     got = [d.cpu().numpy() for d in db]
+    if info is not None:
+        info.update(b.info())
     b.close()
     rk = td1c_amd.CRK(4)
     ref = []
@@ -89,10 +93,30 @@ def test_batch_equals_members_alone(gpu_lib, precision, graph):
     two batch calls (2 + 3 steps) equal each member's own td1c_propagate."""
     probs = [_he(6, 3, precision=precision), _he(9, 7, precision=precision), _he(4, 5, precision=precision),
              _he(12, 2, precision=precision)]
-    hps, ws, fields, got, ref = _run_pair(probs, graph)
+    info = {}
+    hps, ws, fields, got, ref = _run_pair(probs, graph, info=info)
     for m, (g, r, w) in enumerate(zip(got, ref, ws)):
         assert _rel(g, r) < 1e-13, (m, _rel(g, r))
         assert _rel(g, w) > 1e-6  # the state moved
+    # the same element boundaries and charge: every member reads the shared
+    # interior inverses; the one-body element matrices of a member's last
+    # element carry its outer boundary, so grids of different length keep
+    # their own
+    assert info["shared_tables"] == {"cn": 4, "poisson": 4, "onebody": 1}, info
+    for hp in hps:
+        hp.close()
+
+
+def test_batch_autotuned_shapes_give_the_same_bits(gpu_lib):
+    """The batch autotuner (every element one-body shape, both transform
+    variants) on scratch copies, then two batch calls equal the members
+    alone."""
+    probs = [_he(7, 4), _he(10, 6), _he(5, 3)]
+    info = {}
+    hps, ws, fields, got, ref = _run_pair(probs, True, tune=True, info=info)
+    assert len(info["tune"]["ms_per_step"]) == 13, info
+    for m, (g, r) in enumerate(zip(got, ref)):
+        assert _rel(g, r) < 1e-13, (m, _rel(g, r))
     for hp in hps:
         hp.close()
 
@@ -102,9 +126,13 @@ def test_batch_of_three_orbital_members(gpu_lib):
     and nuclear charges: the pair densities, exchange and X matrix of each
     member stay within it."""
     probs = [_dc3(8, 6, 6.0), _dc3(5, 4, 4.0), _dc3(10, 3, 8.0)]
-    hps, ws, fields, got, ref = _run_pair(probs, True)
+    info = {}
+    hps, ws, fields, got, ref = _run_pair(probs, True, info=info)
     for m, (g, r) in enumerate(zip(got, ref)):
         assert _rel(g, r) < 1e-13, (m, _rel(g, r))
+    # the Crank-Nicolson inverses depend on the charge: only the first member
+    # reads the shared copy; the Poisson inverses do not
+    assert info["shared_tables"] == {"cn": 1, "poisson": 3, "onebody": 1}, info
     for hp in hps:
         hp.close()
 
