@@ -514,6 +514,10 @@ def main():
                  "fp64_hprod_evals_per_s": rates[0], "mixed_hprod_evals_per_s": rates[1],
                  "hprod_residual": float(np.linalg.norm(kk[1] - kk[0]) / np.linalg.norm(kk[0]))}
 
+    scan = None
+    if args.scan_members > 0:
+        scan = scan_batch(args.scan_members, dt, 10, args.scan_steps, args.scan_gate_steps)
+
     prob, hp = fp64["prob"], fp64["hp"]
     s = dict(nr=prob.nr, L1=prob.lmax1 + 1, L2=prob.lmax2 + 1, nlat=prob.lmax2 + 1, nfun=prob.nfun,
              ndyn=prob.nfun - prob.nfcore, npair=hp.npair, nfcore=prob.nfcore, lcic=hp.lcic,
@@ -583,6 +587,7 @@ def main():
         "norm_after": fp64["norm"],
         "mean_field_stress": extra,
         "host_entry": host_entry,
+        "scan_batch": scan,
     }
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(fp64["prob"], fp64["wfn"], lf0, args.cpu_seconds, args.workload)

@@ -92,7 +92,7 @@ struct td1c_ctx {
   DevBuf<double> eK0, eK, eP;
   DevBuf<double> m_l2g, m_g2l, m_g2L, m_L2g;
   DevBuf<double> pG, pWl, pWr, pWlI, pWrI, pcrA, pcrG, pcrBinv, pSd, pSl, pSu, invr, rpl0, rpl1, fac2, fac1;
-  DevBuf<int> mval, lval, pair_m, pj, pi, ptype;
+  DevBuf<int> mval, lval, pair_m, pj, pi, ptype, emap;
   // batches
   DevBuf<TransformBatch> b_l2g_dyn, b_g2l_fc, b_rho, b_L2g, b_g2l_mf;
   int n_l2g_dyn = 0, n_g2l_fc = 0, n_g2l_mf = 0;
@@ -189,7 +189,7 @@ struct td1c_ctx {
                             &gram_partial, &gram_partial_p, &lprod, &pprod, &den1, &den2, &rden, &rrden, &int1e, &int2e, &xmat,
                             &dcic, &ci_work, &mfc, &dip_part, &ovlp, &rk_w0, &rk_t, &rk_k};
     for (auto* b : z) b->release();
-    DevBuf<int>* i[] = {&mval, &lval, &pair_m, &pj, &pi, &ptype, &mapH, &mapG, &mapV, &mapF, &map2e, &cn_flag,
+    DevBuf<int>* i[] = {&mval, &lval, &pair_m, &pj, &pi, &ptype, &emap, &mapH, &mapG, &mapV, &mapF, &map2e, &cn_flag,
                         &det_of, &exA, &sgA, &exB, &sgB};
     for (auto* b : i) b->release();
     DevBuf<TransformBatch>* t[] = {&b_l2g_dyn, &b_g2l_fc, &b_rho, &b_L2g, &b_g2l_mf};
@@ -319,7 +319,7 @@ OneBodyElArgs onebody_el_args(td1c_ctx* c, const double* d_lf) {
   OneBodyElArgs oe{};
   oe.nr = S.nr; oe.nfe = S.nfe; oe.L1 = S.L1; oe.ndvr = S.n; oe.nfcore = S.nfcore; oe.nradfc = S.nradfc;
   oe.nfun = S.nfun; oe.mmax1 = S.mmax1; oe.igauge = S.igauge; oe.fc_full = S.iprojfc == 2; oe.mval = c->mval.p;
-  oe.eK0 = c->eK0.p; oe.eK = c->eK.p; oe.eP = c->eP.p; oe.xr = c->xr.p; oe.alph = c->alph.p; oe.tzor = c->tzor.p;
+  oe.eK0 = c->eK0.p; oe.eK = c->eK.p; oe.eP = c->eP.p; oe.emap = c->emap.p; oe.xr = c->xr.p; oe.alph = c->alph.p; oe.tzor = c->tzor.p;
   oe.lf = d_lf;
   return oe;
 }
@@ -355,6 +355,7 @@ int build_device(td1c_ctx* c) {
   HIP_TRY(c->eK0.upload(S.eK0));
   HIP_TRY(c->eK.upload(S.eK));
   HIP_TRY(c->eP.upload(S.eP));
+  HIP_TRY(c->emap.upload(S.emap));
   HIP_TRY(c->m_l2g.upload(padded_cols(S.mat_l2g, S.L1, S.nlat)));
   HIP_TRY(c->m_g2l.upload(padded_cols(S.mat_g2l, S.nlat, S.L1)));
   HIP_TRY(c->m_g2L.upload(padded_cols(S.mat_g2L, S.nlat, S.L2)));
@@ -2235,11 +2236,11 @@ int batch_share_tables(td1c_batch* b, std::vector<std::vector<LaunchRec>>& recs)
     std::vector<char> ok(nmem, 0);
     for (int m = 0; m < nmem; ++m) {
       const Setup& S = b->mem[m]->S;
-      const size_t want = (size_t)S.nfe * np * np;
-      if (S.eK0.size() != want || S.eK.size() != want || S.eP.size() != want) continue;
+      const int nu = (int)(S.eK.size() / ((size_t)np * np));
+      if (S.eK0.size() != S.eK.size() || S.eP.size() != S.eK.size() || nu > ne) continue;
       // the three tables join together or not at all
       TableUnion t0 = u0, t1 = u1, t2 = u2;
-      if (t0.add(S.eK0.data(), 1, S.nfe) && t1.add(S.eK.data(), 1, S.nfe) && t2.add(S.eP.data(), 1, S.nfe)) {
+      if (t0.add(S.eK0.data(), 1, nu) && t1.add(S.eK.data(), 1, nu) && t2.add(S.eP.data(), 1, nu)) {
         u0 = std::move(t0);
         u1 = std::move(t1);
         u2 = std::move(t2);

@@ -514,6 +514,7 @@ __device__ __forceinline__ void onebody_el_body(const double2* __restrict__ orb,
     m = 0;
   }
   const bool lane_ok = e >= 0 && e < a.nfe;
+  const int ee = lane_ok ? a.emap[e] : 0;  // the element's matrix block
   const int mf = a.mval[CN ? a.fshift + f : f];
   const int am = mf < 0 ? -mf : mf;
   const bool fc = CN ? false : f < a.nfcore;
@@ -524,8 +525,8 @@ __device__ __forceinline__ void onebody_el_body(const double2* __restrict__ orb,
   double Kr[NP], Pr[NP];
 #pragma unroll
   for (int k = 0; k < NP; ++k) {
-    Kr[k] = lane_ok ? a.eK[((size_t)e * NP + k) * NP + m] : 0.0;
-    Pr[k] = lane_ok ? a.eP[((size_t)e * NP + k) * NP + m] : 0.0;
+    Kr[k] = lane_ok ? a.eK[((size_t)ee * NP + k) * NP + m] : 0.0;
+    Pr[k] = lane_ok ? a.eP[((size_t)ee * NP + k) * NP + m] : 0.0;
   }
   const int i = e * N + m - 1;               // global interior index of (e, m)
   // owner: interior rows, and row 0 of elements >= 1 (the bridge)
@@ -558,7 +559,7 @@ __device__ __forceinline__ void onebody_el_body(const double2* __restrict__ orb,
       if (l == 0) {
 #pragma unroll
         for (int k = 0; k < NP; ++k) {
-          const double kk = a.eK0[((size_t)e * NP + k) * NP + m];
+          const double kk = a.eK0[((size_t)ee * NP + k) * NP + m];
           pT.x = fma(kk, s0[k].x, pT.x);
           pT.y = fma(kk, s0[k].y, pT.y);
         }

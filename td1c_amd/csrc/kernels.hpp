@@ -54,7 +54,8 @@ struct OneBodyArgs {
 struct OneBodyElArgs {
   int nr, nfe, L1, ndvr, nfcore, nradfc, nfun, mmax1, igauge, fc_full;
   const int* mval;
-  const double *eK0, *eK, *eP;  // [e][k][m] element matrices
+  const double *eK0, *eK, *eP;  // [u][k][m] element matrices of the distinct elements
+  const int* emap;              // [nfe] element -> its block u
   const double* xr;
   const double* alph;
   const double2* tzor;          // [L1][nr] tmat diagonal without the kinetic part (+ v2jfc)

@@ -8,6 +8,8 @@
 #include <complex>
 #include <cstdio>
 #include <cstdlib>
+#include <map>
+#include <string>
 
 namespace td1c {
 
@@ -330,6 +332,32 @@ bool Setup::build(const td1c_problem& p) {
         eK[at] = 0.5 * t0 / nrm;
         eP[at] = we[m] * ds[np * k + m] / nrm;
       }
+  }
+  {
+    // keep each distinct (eK0, eK, eP) element block once
+    const size_t bs = (size_t)np * np;
+    std::map<std::string, int> seen;
+    std::vector<double> u0, u1, u2;
+    emap.assign(nfe, 0);
+    for (int e = 0; e < nfe; ++e) {
+      std::string key(reinterpret_cast<const char*>(&eK0[e * bs]), bs * sizeof(double));
+      key.append(reinterpret_cast<const char*>(&eK[e * bs]), bs * sizeof(double));
+      key.append(reinterpret_cast<const char*>(&eP[e * bs]), bs * sizeof(double));
+      auto it = seen.find(key);
+      if (it != seen.end()) {
+        emap[e] = it->second;
+        continue;
+      }
+      const int u = (int)seen.size();
+      seen.emplace(std::move(key), u);
+      emap[e] = u;
+      u0.insert(u0.end(), eK0.begin() + e * bs, eK0.begin() + (e + 1) * bs);
+      u1.insert(u1.end(), eK.begin() + e * bs, eK.begin() + (e + 1) * bs);
+      u2.insert(u2.end(), eP.begin() + e * bs, eP.begin() + (e + 1) * bs);
+    }
+    eK0.swap(u0);
+    eK.swap(u1);
+    eP.swap(u2);
   }
 
   // ---- kernel-side one-body tables ----

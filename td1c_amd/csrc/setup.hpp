@@ -41,9 +41,12 @@ struct Setup {
   std::vector<double> bandT_p;   // [2n+1][nr]: radp row band
   std::vector<double> tdiag;     // [L1][nr] complex (re,im): diagonal of tmat
   std::vector<double> tzor;      // [L1][nr] complex (re,im): tmat diagonal without the kinetic part
-  // element-local matrices [e][k][m] (m fastest): kinetic for l = 0 (radk),
-  // kinetic for l >= 1 (radk0), d/dr (radp); rows/columns of r = 0 and r = R are zero
+  // element-local matrices [u][k][m] (m fastest) of the distinct elements
+  // (equal element widths give bitwise-equal blocks, stored once; element e
+  // uses block emap[e]): kinetic for l = 0 (radk), kinetic for l >= 1
+  // (radk0), d/dr (radp); rows/columns of r = 0 and r = R are zero
   std::vector<double> eK0, eK, eP;
+  std::vector<int> emap;
   std::vector<double> alph;      // [2mmax1+1][L1]
 
   // angular quadrature and transform matrices, [m-index][K][C]
