@@ -65,6 +65,8 @@ def lib(openmp: bool = False):
         L.orc_dipole.restype = ctypes.c_double
         L.orc_dipole3.argtypes = [vp, vp, vp, vp, vp]
         L.orc_dipole3.restype = None
+        L.orc_dipole_all.argtypes = [vp, vp, vp, vp, vp]
+        L.orc_dipole_all.restype = None
         L.orc_ipx.argtypes = [vp, ctypes.c_int, ctypes.c_double, vp, vp, vp, vp]
         L.orc_ipx.restype = ctypes.c_int
         L.orc_ipd.argtypes = [vp, ctypes.c_int, ctypes.c_double, vp, vp, vp, vp]
@@ -208,6 +210,15 @@ class Oracle:
         lf = np.ascontiguousarray(lfield, dtype=np.float64)
         out = np.zeros(3)
         self._L.orc_dipole3(self.h, _p(lf), _p(w[: self.norb]), _p(w[self.norb:]), _p(out))
+        return out
+
+    def dipole_all(self, lfield, wfn: np.ndarray) -> np.ndarray:
+        """hprod_dipole with the frozen-core corrections: [dip(1..3),
+        vel(1..3), acc(1..3)], (2) the corrections, (3) = (1) + (2)."""
+        w = np.ascontiguousarray(wfn, dtype=np.complex128)
+        lf = np.ascontiguousarray(lfield, dtype=np.float64)
+        out = np.zeros(9)
+        self._L.orc_dipole_all(self.h, _p(lf), _p(w[: self.norb]), _p(w[self.norb:]), _p(out))
         return out
 
     def ipx(self, max_ipx: int, rad_ipx: float, lfield, wfn: np.ndarray) -> np.ndarray:

@@ -2077,7 +2077,7 @@ double orc_dipole(orc* o, const double* lfield, const zc* wfn, const zc* cic) {
  * property vectors of hprod_zprod_all / hprod_pzprod_all / hprod_azprod_all
  * at zfac = 1 (frozen cores over r < nradfc) and N = neltot(3).  The
  * frozen-core correction terms dip(2), vel(2), acc(2), which need the
- * generalised Fock vector of hprod_gfock, are not part of this function. */
+ * generalised Fock vector of hprod_gfock, are in orc_dipole_all. */
 void orc_dipole3(orc* o, const double* lfield, const zc* wfn, const zc* cic, double* out) {
   int nf = o->nfun, nb = o->nbas, nfc = o->nfcore, nr = o->nr;
   orc_htotx(o, 1.0, lfield, wfn, cic); /* orbin + den1 */
@@ -2097,6 +2097,197 @@ void orc_dipole3(orc* o, const double* lfield, const zc* wfn, const zc* cic, dou
   out[1] = creal(trace1(o, po));
   if (o->igauge == 1) out[1] += lfield[8] * ne;
   out[2] = creal(trace1(o, ao)) - lfield[5] * ne;
+  free(ao);
+  free(po);
+  free(zo);
+}
+
+/* hprod_trace1 (src/hprod/hprod_trace1.F90:2-88) with dofc: 2 sum over the
+ * cores of <a_i|b_i> (frozen cores over r < nradfc, l >= |m|) plus, for each
+ * active a, <sum_j a_j den1(j, a) | b_a> (m_j = m_a) */
+static zc trace1ab(const orc* o, const zc* a, const zc* b) {
+  int nb = o->nbas, nfc = o->nfcore, ncore = o->ncore, na = o->nact, nr = o->nr;
+  zc t = 0;
+  for (int ifun = 0; ifun < ncore; ++ifun) {
+    int dim = ifun < nfc ? o->nradfc : nr;
+    for (int l = abs(o->mval[ifun]); l <= o->lmax1; ++l)
+      for (int i = 0; i < dim; ++i) t += conj(a[OIDX(i, l, ifun)]) * b[OIDX(i, l, ifun)];
+  }
+  t *= 2.0;
+  zc* tw = xcalloc((size_t)nb, sizeof(zc));
+  for (int ia = 0; ia < na; ++ia) {
+    int ifun = ncore + ia;
+    memset(tw, 0, sizeof(zc) * nb);
+    for (int ja = 0; ja < na; ++ja) {
+      int jfun = ncore + ja;
+      if (o->mval[ifun] != o->mval[jfun]) continue;
+      for (int l = abs(o->mval[ifun]); l <= o->lmax1; ++l)
+        for (int i = 0; i < nr; ++i) tw[(size_t)l * nr + i] += a[OIDX(i, l, jfun)] * o->den1[ja + na * ia];
+    }
+    for (int l = 0; l <= o->lmax1; ++l)
+      for (int i = 0; i < nr; ++i) t += conj(tw[(size_t)l * nr + i]) * b[OIDX(i, l, ifun)];
+  }
+  free(tw);
+  return t;
+}
+
+/* hprod_gfock (src/hprod/hprod_gfock.F90:2-92; jfc_implicit, xfc_implicit)
+ * and hprod_gfock_sum (:198-230): o->gorb <- (h0 + h1 + g + v2) phi of
+ * every orbital, the frozen cores over r < nradfc, with the mean field
+ * acting on the frozen cores as hprod_mfprod2_gfock's onFC branches
+ * (src/hprod/hprod_mfprod2.F90: fcx1 :158-200, dcj_onFC :282-304,
+ * dcx_onFC :343-365, actj_onFC :414-436, actx_onFC :489-513). */
+static void gfock(orc* o, const double* lfield, const zc* wfn, const zc* cic) {
+  int nf = o->nfun, nb = o->nbas, nfc = o->nfcore, nfc2 = o->nfcore2, ncore = o->ncore, na = o->nact;
+  int nr = o->nr, nlat = o->nlat, ng = o->ngrid, dimf = o->nradfc;
+  /* orbin, RDMs, the pair potentials and the mean field of the dynamical
+   * orbitals as in the hprod */
+  orc_htotx(o, 1.0, lfield, wfn, cic);
+  /* 1e operators of hprod_gfock: tprod_all, zprod_all / pzprod_all (the
+   * frozen cores over r < nradfc) */
+  zc zfield = lfield[2];
+  memset(o->h0orb, 0, sizeof(zc) * (size_t)nb * nf);
+  memset(o->h1orb, 0, sizeof(zc) * (size_t)nb * nf);
+  if (nfc > 0) tprod(o, o->orb, o->h0orb, 0, nfc, dimf);
+  tprod(o, o->orb, o->h0orb, nfc, nf, nr);
+  if (o->igauge == 0) {
+    if (nfc > 0) zprod(o, zfield, o->orb, o->h1orb, 0, nfc, dimf);
+    zprod(o, zfield, o->orb, o->h1orb, nfc, nf, nr);
+  } else {
+    if (nfc > 0) pzprod(o, zfield, o->orb, o->h1orb, 0, nfc, dimf);
+    pzprod(o, zfield, o->orb, o->h1orb, nfc, nf, nr);
+  }
+  /* the mean field acting on the frozen cores (theta grid, r < nradfc) */
+  if (nfc > 0 && o->nel[2] >= 2) {
+    mkrho2(o, 2);
+    poisson(o, 2);
+    sph2ang2(o, 2);
+    v2herm(o, 2);
+    zc* v2 = o->v2ang;
+#define V2G(ig, jfun, ifun) v2[((size_t)(ifun) * nf + (jfun)) * ng + (ig)]
+#define WG(ig, ifun) o->orbg[(size_t)(ifun) * ng + (ig)]
+    for (size_t k = 0; k < (size_t)ng * nfc; ++k) o->gorbg[k] = o->v2orbg[k] = 0;
+    zc* d2v = xcalloc((size_t)ng * (na > 0 ? na : 1), sizeof(zc));
+    for (int ilat = 0; ilat < nlat; ++ilat)
+      for (int i = 0; i < dimf; ++i) {
+        size_t ig = (size_t)ilat * nr + i;
+        /* fcx1 on FC */
+        if (o->nfcore1 > 0)
+          for (int ifun = 0; ifun < nfc; ++ifun)
+            for (int jfun = nfc2; jfun < nfc; ++jfun) o->gorbg[(size_t)ifun * ng + ig] -= V2G(ig, jfun, ifun) * WG(ig, jfun);
+        if (o->ndcore > 0) {
+          /* dcj on FC */
+          zc dv = 0;
+          for (int jfun = nfc; jfun < ncore; ++jfun) dv += V2G(ig, jfun, jfun) * 2.0;
+          for (int ifun = 0; ifun < nfc; ++ifun) o->gorbg[(size_t)ifun * ng + ig] += dv * WG(ig, ifun);
+          /* dcx on FC */
+          for (int ifun = 0; ifun < nfc; ++ifun)
+            for (int jfun = nfc; jfun < ncore; ++jfun) o->gorbg[(size_t)ifun * ng + ig] -= V2G(ig, jfun, ifun) * WG(ig, jfun);
+        }
+        if (ncore > 0 && na > 0) {
+          /* actj on FC */
+          zc dv = 0;
+          for (int ja = 0; ja < na; ++ja)
+            for (int ka = 0; ka < na; ++ka) dv += V2G(ig, ncore + ja, ncore + ka) * o->den1[ka + na * ja];
+          for (int ifun = 0; ifun < nfc; ++ifun) o->v2orbg[(size_t)ifun * ng + ig] += dv * WG(ig, ifun);
+          /* actx on FC */
+          for (int ja = 0; ja < na; ++ja) {
+            int jfun = ncore + ja;
+            zc wd = 0;
+            for (int ka = 0; ka < na; ++ka) {
+              int kfun = ncore + ka;
+              if (o->mval[jfun] == o->mval[kfun]) wd += WG(ig, kfun) * (-o->den1[ka + na * ja] * 0.5);
+            }
+            for (int ifun = 0; ifun < nfc; ++ifun) o->v2orbg[(size_t)ifun * ng + ig] += wd * V2G(ig, jfun, ifun);
+          }
+        }
+      }
+    free(d2v);
+#undef V2G
+#undef WG
+    ang2sph1(o, o->gorbg, o->gorb, 0, nfc, dimf);
+    ang2sph1(o, o->v2orbg, o->v2orb, 0, nfc, dimf);
+  }
+  /* hprod_gfock_sum */
+  for (int ifun = 0; ifun < nf; ++ifun) {
+    int dim = ifun < nfc ? dimf : nr;
+    for (int l = 0; l <= o->lmax1; ++l)
+      for (int i = 0; i < dim; ++i) {
+        size_t q = OIDX(i, l, ifun);
+        o->gorb[q] = o->v2orb[q] + o->gorb[q] + o->h1orb[q] + o->h0orb[q];
+      }
+  }
+}
+
+/* hprod_dipole (src/hprod/hprod_dipole.F90:2-146) with frozen cores: out =
+ * {dip(1..3), vel(1..3), acc(1..3)}.  dip/vel/acc(1) are the Ehrenfest
+ * traces (orc_dipole3); with frozen cores the generalised Fock vector f of
+ * gfock gives the corrections (:82-125):
+ *   vel(2) = -4 Im sum_{fc i} <z i|f i> - 2 Im tr(D z, P_fc f),
+ *   acc(2) = -4 Im sum_{fc i} <p_z i|f i> - 2 Im tr(D p_z, P_fc f),
+ * f += E_z z phi first in the velocity gauge (:60-63), P_fc the projector on
+ * the frozen cores over r < nradfc (hprod_projfc, src/hprod/hprod_projfc.F90:2-158
+ * with inner); dip(2) = 0 and (3) = (1) + (2). */
+void orc_dipole_all(orc* o, const double* lfield, const zc* wfn, const zc* cic, double* out) {
+  int nf = o->nfun, nb = o->nbas, nfc = o->nfcore, nr = o->nr, dimf = o->nradfc;
+  if (nfc > 0) gfock(o, lfield, wfn, cic);
+  else orc_htotx(o, 1.0, lfield, wfn, cic);
+  zc* zo = xcalloc((size_t)nb * nf, sizeof(zc));
+  zc* po = xcalloc((size_t)nb * nf, sizeof(zc));
+  zc* ao = xcalloc((size_t)nb * nf, sizeof(zc));
+  if (nfc > 0) {
+    zprod(o, 1.0, o->orb, zo, 0, nfc, dimf);
+    pzprod(o, 1.0, o->orb, po, 0, nfc, dimf);
+    azprod(o, 1.0, o->orb, ao, 0, nfc, dimf);
+  }
+  zprod(o, 1.0, o->orb, zo, nfc, nf, nr);
+  pzprod(o, 1.0, o->orb, po, nfc, nf, nr);
+  azprod(o, 1.0, o->orb, ao, nfc, nf, nr);
+  double ne = (double)o->nel[2];
+  double dip1 = creal(trace1ab(o, o->orb, zo));
+  double vel1 = creal(trace1ab(o, o->orb, po));
+  if (o->igauge == 1) vel1 += lfield[8] * ne;
+  double acc1 = creal(trace1ab(o, o->orb, ao)) - lfield[5] * ne;
+  double vc1 = 0, ac1 = 0, vc2 = 0, ac2 = 0;
+  if (nfc > 0) {
+    zc* f = o->gorb;
+    if (o->igauge == 1)
+      for (size_t k = 0; k < (size_t)nb * nf; ++k) f[k] += lfield[5] * zo[k];
+    /* first correction term */
+    zc vp = 0, ap = 0;
+    for (int ifun = 0; ifun < nfc; ++ifun)
+      for (int l = 0; l <= o->lmax1; ++l)
+        for (int i = 0; i < dimf; ++i) {
+          size_t q = OIDX(i, l, ifun);
+          vp += conj(zo[q]) * f[q];
+          ap += conj(po[q]) * f[q];
+        }
+    vc1 = -4.0 * cimag(vp);
+    ac1 = -4.0 * cimag(ap);
+    /* second correction term: f <- P_fc f */
+    zc* fm = xcalloc((size_t)nf * nf, sizeof(zc));
+    for (int ifun = 0; ifun < nfc; ++ifun)
+      for (int jfun = 0; jfun < nf; ++jfun) {
+        if (o->mval[jfun] != o->mval[ifun]) continue;
+        zc t = 0;
+        for (int l = 0; l <= o->lmax1; ++l)
+          for (int i = 0; i < dimf; ++i) t += conj(o->orb[OIDX(i, l, ifun)]) * f[OIDX(i, l, jfun)];
+        fm[ifun + nf * jfun] = t;
+      }
+    memset(f, 0, sizeof(zc) * (size_t)nb * nf);
+    for (int ifun = 0; ifun < nf; ++ifun)
+      for (int jfun = 0; jfun < nfc; ++jfun) {
+        if (o->mval[jfun] != o->mval[ifun]) continue;
+        for (int l = 0; l <= o->lmax1; ++l)
+          for (int i = 0; i < dimf; ++i) f[OIDX(i, l, ifun)] += o->orb[OIDX(i, l, jfun)] * fm[jfun + nf * ifun];
+      }
+    free(fm);
+    vc2 = -2.0 * cimag(trace1ab(o, zo, f));
+    ac2 = -2.0 * cimag(trace1ab(o, po, f));
+  }
+  out[0] = dip1; out[1] = 0.0; out[2] = dip1;
+  out[3] = vel1; out[4] = vc1 + vc2; out[5] = vel1 + vc1 + vc2;
+  out[6] = acc1; out[7] = ac1 + ac2; out[8] = acc1 + ac1 + ac2;
   free(ao);
   free(po);
   free(zo);

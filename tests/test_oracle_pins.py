@@ -661,6 +661,72 @@ def test_dipole_forms_frozen_core_equals_dynamic_core():
     assert np.abs(da - dc).max() < 1e-13 * np.abs(da).max()
 
 
+def _fc_li_like(phi, igauge):
+    """Frozen hydrogenic 1s of Z = 3 (type -1 core) and one doubly occupied
+    dynamical orbital (2p0 + e^{i phi} 3d0)/sqrt(2) with hydrogenic radial
+    shapes of Z = 1: smooth functions whose l stays far below lmax1."""
+    prob = Problem(nfe=30, ndvr=20, rmax=60.0, lmax1=5, lmax2=10, znuc=3.0, nfcore1=1, ndcore=1, nel_alpha=2,
+                   nel_beta=2, lval=[0, 1], igauge=igauge)
+    o = O.Oracle(prob)
+    r, w = W.gll_grid(prob)
+    nr = prob.nr
+    nb = nr * (prob.lmax1 + 1)
+    wf = np.zeros(o.wfn_size, complex)
+    wf[:nr] = 2 * 3.0 ** 1.5 * r * np.exp(-3.0 * r) * np.sqrt(w)
+    u2p = r * r * np.exp(-r / 2) * np.sqrt(w)
+    u3d = r ** 3 * np.exp(-r / 3) * np.sqrt(w)
+    wf[nb + nr:nb + 2 * nr] = u2p / np.linalg.norm(u2p) / math.sqrt(2)
+    wf[nb + 2 * nr:nb + 3 * nr] = np.exp(1j * phi) * u3d / np.linalg.norm(u3d) / math.sqrt(2)
+    wf[-1] = 1.0
+    o.set_wfn0(wf)
+    return o, wf
+
+
+@pytest.mark.parametrize("igauge", [0, 1])
+@pytest.mark.parametrize("phi", [0.3, 1.2])
+def test_dipole_frozen_core_corrections_ehrenfest(igauge, phi):
+    """orc_dipole_all (hprod_dipole with hprod_gfock, hprod_projfc): along the
+    equations of motion of a frozen-core state (a central difference along
+    d psi/dt at fixed field), d<z>/dt = vel(1) + vel(2) and
+    d vel(1)/dt = acc(1) + acc(2), the frozen-core corrections restoring
+    Ehrenfest's theorem; without them the gaps are 1e-5 to 1e-1.  Smooth
+    hydrogenic orbitals keep the FEDVR commutator error near 1e-10; a sign,
+    a missing mean-field term on the frozen core or a wrong projector is
+    visible at the size of the correction itself."""
+    o, wf = _fc_li_like(phi, igauge)
+    lf = np.zeros(9)
+    if igauge == 0:
+        lf[2] = lf[5] = 0.03
+    else:
+        lf[2] = lf[8] = 0.2
+    h = o.htot(1.0, lf, wf)
+    d0 = o.dipole_all(lf, wf)
+    eps = 1e-5
+    fd = (o.dipole_all(lf, wf + eps * h) - o.dipole_all(lf, wf - eps * h)) / (2 * eps)
+    assert abs(d0[4]) > 1e-5 and abs(d0[7]) > 1e-2, d0  # the corrections are not small
+    assert abs(fd[0] - d0[5]) < 2e-9 and abs(fd[0] - d0[5]) < 1e-3 * abs(d0[4]), (fd, d0)
+    assert abs(fd[3] - d0[8]) < 2e-9 and abs(fd[3] - d0[8]) < 1e-6 * abs(d0[7]), (fd, d0)
+    assert d0[1] == 0.0 and d0[2] == d0[0]
+    assert abs(d0[5] - d0[3] - d0[4]) < 1e-15 * abs(d0[5]) + 1e-300
+
+
+def test_dipole_all_without_frozen_core_is_dipole3():
+    """orc_dipole_all with no frozen core: the corrections vanish and the
+    totals are orc_dipole3's traces."""
+    import tests.cases as C
+
+    for name in ("he_small", "cas_small", "dc3_small"):
+        prob = getattr(C, name)(igauge=1)
+        o = O.Oracle(prob)
+        wv = C.wfn_for(prob, o.lcic)
+        o.set_wfn0(wv)
+        d = o.dipole_all(C.LF_VEL, wv)
+        d3 = o.dipole3(C.LF_VEL, wv)
+        assert d[1] == d[4] == d[7] == 0.0
+        assert np.abs(np.array([d[0], d[3], d[6]]) - d3).max() <= 1e-14 * np.abs(d3).max()
+        assert np.array_equal(d[[2, 5, 8]], d[[0, 3, 6]])
+
+
 def _window_ovlp(prob, orb_l, R):
     """numpy hprod_mkovlp: <phi_j|phi_i> over the rows up to and including the
     first r > R, m-matched."""
