@@ -194,9 +194,22 @@ int td1c_dipole(td1c_ctx* ctx, const double* lfield, const double* d_wfn,
  * are hprod_dipole's totals dip(3), vel(3), acc(3).  With frozen cores the
  * reference adds correction terms vel(2), acc(2) built on the generalised
  * Fock vector of hprod_gfock (src/hprod/hprod_dipole.F90:82-125); they are
- * not included. */
+ * in td1c_dipole_all. */
 int td1c_dipole3(td1c_ctx* ctx, const double* lfield, const double* d_wfn,
                  const double* d_cic, double* d_out, void* stream);
+
+/* hprod_dipole in full (src/hprod/hprod_dipole.F90:2-146): d_out (nine
+ * doubles in device memory) = {dip(1), dip(2), dip(3), vel(1), vel(2),
+ * vel(3), acc(1), acc(2), acc(3)}, the reference's Phys.dip, vel, acc.
+ * (1) are td1c_dipole3's traces; with frozen cores (2) are the corrections
+ * of :74-125 on the generalised Fock vector f of hprod_gfock (the one-body
+ * part, the mean field including its action on the frozen cores, plus E_z z
+ * in the velocity gauge): vel(2) = -4 Im sum_fc <z i|f i> - 2 Im tr(D z,
+ * P_fc f), acc(2) the same with p_z, P_fc the frozen-core projector of
+ * hprod_projfc; dip(2) = 0; (3) = (1) + (2).  Costs one hprod evaluation
+ * (print cadence).  Enqueued on `stream`. */
+int td1c_dipole_all(td1c_ctx* ctx, const double* lfield, const double* d_wfn, const double* d_cic, double* d_out,
+                    void* stream);
 
 /* Ionisation probabilities of a device wave function: ipx[k] (host memory,
  * max_ipx + 1 doubles) = probability that exactly k electrons are outside

@@ -70,6 +70,8 @@ def load_library():
     L.td1c_dipole.restype = ctypes.c_int
     L.td1c_dipole3.argtypes = [vp, vp, vp, vp, vp, vp]
     L.td1c_dipole3.restype = ctypes.c_int
+    L.td1c_dipole_all.argtypes = [vp, vp, vp, vp, vp, vp]
+    L.td1c_dipole_all.restype = ctypes.c_int
     L.td1c_ipx.argtypes = [vp, ctypes.c_int, ctypes.c_double, vp, vp, vp, vp, vp]
     L.td1c_ipx.restype = ctypes.c_int
     L.td1c_ipd.argtypes = [vp, ctypes.c_int, ctypes.c_double, vp, vp, vp, vp, vp]
@@ -365,6 +367,32 @@ class HProd:
                                            ctypes.c_void_p(base + 16 * self.norb), ctypes.c_void_p(out.data_ptr()),
                                            self._stream(stream)), "td1c_dipole3")
         return out.cpu().numpy() if ret else out
+
+    def dipole_all(self, lfield: Sequence[float], wfn, out=None, stream=None):
+        """hprod_dipole in full: [dip(1..3), vel(1..3), acc(1..3)], (2) the
+        frozen-core corrections, (3) = (1) + (2); returns a numpy array
+        (synchronising) unless `out`, a float64 CUDA tensor of nine elements,
+        is given."""
+        torch = self.torch
+        lf = np.ascontiguousarray(lfield, dtype=np.float64)
+        if lf.size != 9:
+            raise ValueError("lfield must have 9 entries")
+        if wfn.dtype != torch.complex128 or not wfn.is_cuda or wfn.numel() != self.wfn_size:
+            raise ValueError("wfn must be a complex128 CUDA tensor of wfn_size elements")
+        ret = out is None
+        if out is None:
+            out = torch.zeros(9, dtype=torch.float64, device=wfn.device)
+        elif out.dtype != torch.float64 or not out.is_cuda or out.numel() < 9:
+            raise ValueError("out must be a float64 CUDA tensor of nine elements")
+        base = wfn.data_ptr()
+        _check(load_library().td1c_dipole_all(self.h, _ptr(lf), ctypes.c_void_p(base),
+                                              ctypes.c_void_p(base + 16 * self.norb),
+                                              ctypes.c_void_p(out.data_ptr()), self._stream(stream)),
+               "td1c_dipole_all")
+        if ret:
+            torch.cuda.synchronize()
+            return out.cpu().numpy()
+        return out
 
     def ipx(self, max_ipx: int, rad_ipx: float, lfield: Sequence[float], wfn, stream=None) -> np.ndarray:
         """hprod_ipx: probabilities that exactly k = 0..max_ipx electrons are

@@ -181,5 +181,5 @@ HEADER_FUNCTIONS: List[str] = [
     "td1c_get_int2e", "td1c_get_xmat", "td1c_set_graph", "td1c_set_transform_path", "td1c_ci_layout",
     "hprod_htot_static", "td1c_h1itr_gen", "td1c_h1itr_prop", "td1c_get_shapes", "td1c_set_onebody_shape",
     "td1c_batch_create", "td1c_batch_destroy", "td1c_batch_propagate", "td1c_batch_set_graph",
-    "td1c_batch_set_shapes", "td1c_batch_get_info", "td1c_batch_autotune",
+    "td1c_batch_set_shapes", "td1c_batch_get_info", "td1c_batch_autotune", "td1c_dipole_all",
 ]

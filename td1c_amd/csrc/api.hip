@@ -134,6 +134,13 @@ struct td1c_ctx {
   DevBuf<int> cn_flag;
   std::vector<double> v2jfc_h;  // frozen-core Hartree potential (complex, host), empty without frozen core
   std::vector<double> cnG_h;    // host copy of cnG (complex, [L1][nfe][k][i])
+  // frozen-core dipole corrections (td1c_dipole_all): the frozen-core pair
+  // potentials v(j, i) on the theta grid, the frozen cores' generalised Fock
+  // rows (theta grid, then l-space: g, v2), their theta -> l batch, partials
+  DevBuf<double2> vfc_tab, gfc_g, gfc_l, dipfc_part;
+  DevBuf<float2> gfc_gf;
+  DevBuf<TransformBatch> b_g2l_gfc;
+  DevBuf<double> dip3_tmp;
   // device copies of hprod_htot_'s host arguments (wfn | hwfn | cic | hcic),
   // allocated by its first call and kept with the context
   DevBuf<double2> legacy_stage;
@@ -185,7 +192,8 @@ struct td1c_ctx {
                        &cn_part, &lf_cn};
     for (auto* b : d) b->release();
     DevBuf<double2>* z[] = {&tdiag, &tzor, &lsp, &gsp, &orbg0, &rhoL, &vL, &vg, &p_xb, &p_Q, &legacy_stage,
-                            &cnG, &cnWl, &cnWr, &cnPa, &cnPg, &cnBinv, &cn_x, &cn_y,
+                            &cnG, &cnWl, &cnWr, &cnPa, &cnPg, &cnBinv, &cn_x, &cn_y, &vfc_tab, &gfc_g, &gfc_l,
+                            &dipfc_part,
                             &gram_partial, &gram_partial_p, &lprod, &pprod, &den1, &den2, &rden, &rrden, &int1e, &int2e, &xmat,
                             &dcic, &ci_work, &mfc, &dip_part, &ovlp, &rk_w0, &rk_t, &rk_k};
     for (auto* b : z) b->release();
@@ -197,7 +205,9 @@ struct td1c_ctx {
     DevBuf<const void*>* g[] = {&g_left_l, &g_right_l, &g_left_p, &g_right_p};
     for (auto* b : g) b->release();
     for (auto e : ev_pool) (void)hipEventDestroy(e);
-    DevBuf<float2>* ff[] = {&gsp_f, &rhoL_f, &vL_f, &vg_f};
+    DevBuf<float2>* ff[] = {&gsp_f, &rhoL_f, &vL_f, &vg_f, &gfc_gf};
+    b_g2l_gfc.release();
+    dip3_tmp.release();
     for (auto* b : ff) b->release();
     DevBuf<float>* fm[] = {&m_g2L_f, &m_L2g_f};
     for (auto* b : fm) b->release();
@@ -588,6 +598,7 @@ int build_device(td1c_ctx* c) {
   const size_t nfull = (size_t)S.nstr_a * S.nstr_b;
   HIP_TRY(c->mfc.alloc((size_t)na * na * na * na + 2 * na * na));
   HIP_TRY(c->dip_part.alloc((size_t)3 * ((S.nr + 255) / 256) * S.L1));  // three forms x dipole_blocks
+  HIP_TRY(c->dip3_tmp.alloc(3));
   HIP_TRY(c->ovlp.alloc((size_t)S.nfun * S.nfun));
   HIP_TRY(c->ci_work.alloc(std::max<size_t>(1, ((size_t)2 * S.nact * S.nact + 2) * nfull)));
   HIP_TRY(c->det_of.upload(flat_or_one(S.det_of)));
@@ -928,7 +939,7 @@ std::vector<double> host_poisson_l0(const Setup& S, const std::vector<double>& r
 // orb: frozen cores in l-space [f][l][r]; og: on the theta grid [f][t][r].
 double host_fcore_energy(const Setup& S, const std::vector<std::complex<double>>& orb,
                          const std::vector<std::complex<double>>& og, const std::vector<double2>& v2jfc,
-                         const double* v2xfc) {
+                         const double* v2xfc, std::vector<std::complex<double>>* vfc_out) {
   using cd = std::complex<double>;
   const int nr = S.nr, n = S.n, bw = 2 * n + 1, L1 = S.L1, L2 = S.L2, nl = S.nlat, nfc = S.nfcore;
   const int nrf = S.nradfc;
@@ -1000,6 +1011,15 @@ double host_fcore_energy(const Setup& S, const std::vector<std::complex<double>>
             for (int i = 0; i < nr; ++i) v[(size_t)q * nr + i] += vl[(size_t)L * nr + i] * w;
           }
       }
+    if (vfc_out) {
+      // every v(j, i) as vfc[(i nfc + j) ng + x] (the potential of conj(phi_j) phi_i)
+      vfc_out->assign((size_t)nfc * nfc * ng, 0.0);
+      for (int i = 0; i < nfc; ++i)
+        for (int j = 0; j < nfc; ++j)
+          for (size_t x = 0; x < ng; ++x)
+            (*vfc_out)[((size_t)i * nfc + j) * ng + x] =
+                j >= i ? vg[((size_t)j * nfc + i) * ng + x] : std::conj(vg[((size_t)i * nfc + j) * ng + x]);
+    }
     // fcx1: g_i -= v(j, i) phi_j for the type -1 cores j; v(j, i) = conj(v(i, j)) for j < i
     for (int i = 0; i < nfc; ++i)
       for (int j = S.nfcore2; j < nfc; ++j)
@@ -1297,7 +1317,27 @@ int td1c_set_wfn0(td1c_ctx* c, const double* wfn, const double* cic, const doubl
   {
     std::vector<std::complex<double>> og((size_t)c->ngrid * nfc);
     HIP_TRY(hipMemcpy(og.data(), c->orbg0.p, og.size() * sizeof(double2), hipMemcpyDeviceToHost));
-    c->ene_fcore = host_fcore_energy(S, orbfc, og, v2jfc, v2xfc);
+    std::vector<std::complex<double>> vfc;
+    c->ene_fcore = host_fcore_energy(S, orbfc, og, v2jfc, v2xfc, &vfc);
+    // the frozen-core dipole corrections' tables and work arrays
+    const size_t ng = (size_t)c->ngrid;
+    if (vfc.empty()) vfc.assign((size_t)nfc * nfc * ng, 0.0);
+    c->vfc_tab.release();
+    HIP_TRY(c->vfc_tab.alloc(vfc.size()));
+    HIP_TRY(hipMemcpy(c->vfc_tab.p, vfc.data(), vfc.size() * sizeof(double2), hipMemcpyHostToDevice));
+    if (!c->gfc_l.p) {
+      HIP_TRY(c->gfc_l.alloc((size_t)2 * nfc * nb));
+      HIP_TRY(hipMemset(c->gfc_l.p, 0, (size_t)2 * nfc * nb * sizeof(double2)));
+      if (c->mixed) HIP_TRY(c->gfc_gf.alloc((size_t)2 * nfc * ng));
+      else HIP_TRY(c->gfc_g.alloc((size_t)2 * nfc * ng));
+      std::vector<TransformBatch> b;
+      for (int k = 0; k < 2 * nfc; ++k)
+        b.push_back({(long long)(k * ng), 0, (long long)(k * nb), S.mval[k % nfc] + S.mmax1, S.nradfc});
+      HIP_TRY(c->b_g2l_gfc.upload(b));
+      DipoleFcArgs probe{};
+      probe.nfun = S.nfun; probe.nfcore = nfc; probe.nradfc = S.nradfc; probe.L1 = S.L1;
+      HIP_TRY(c->dipfc_part.alloc((size_t)dipole_fc_quantities(probe) * dipole_fc_blocks(probe)));
+    }
   }
   // the Crank-Nicolson factors of split_type = 1 hold tmat: rebuilt with v2jfc on next use
   c->v2jfc_h.assign((size_t)2 * nr, 0.0);
@@ -1434,6 +1474,68 @@ int td1c_dipole(td1c_ctx* c, const double* lfield, const double* d_wfn, const do
 int td1c_dipole3(td1c_ctx* c, const double* lfield, const double* d_wfn, const double* d_cic, double* d_out,
                  void* stream) {
   return enqueue_dipole(c, 3, lfield, d_wfn, d_cic, d_out, stream);
+}
+
+// hprod_dipole (src/hprod/hprod_dipole.F90:2-146) with the frozen-core
+// corrections: out9 = {dip, vel, acc} x {(1), (2), (3)}.  With frozen cores
+// the generalised Fock vector (hprod_gfock) is one hprod's intermediate
+// arrays (the one-body part of every orbital, the mean field of the
+// dynamical orbitals) plus the frozen cores' own mean-field rows
+// (k_gfock_fc, then theta -> l); k_dipole_fc forms the correction sums.
+int td1c_dipole_all(td1c_ctx* c, const double* lfield, const double* d_wfn, const double* d_cic, double* d_out,
+                    void* stream) {
+  if (!c || !lfield || !d_wfn || !d_out) return fail(-1, "null argument");
+  Setup& S = c->S;
+  hipStream_t s = (hipStream_t)stream;
+  if (S.nfcore == 0) {
+    if (int rc = enqueue_dipole(c, 3, lfield, d_wfn, d_cic, c->dip3_tmp.p, stream)) return rc;
+    launch_dipole_expand(c->dip3_tmp.p, d_out, s);
+    HIP_TRY(hipGetLastError());
+    return 0;
+  }
+  if (!c->wfn0_set) return fail(-4, "frozen core needs td1c_set_wfn0 first");
+  if (S.nact > 0 && !d_cic) return fail(-1, "null CI vector");
+  const double2* wfn = reinterpret_cast<const double2*>(d_wfn);
+  const double2* cic = reinterpret_cast<const double2*>(d_cic);
+  HIP_TRY(hipMemcpyAsync(c->lf_dev.p, lfield, 9 * sizeof(double), hipMemcpyHostToDevice, s));
+  CiStage cs;
+  if (int rc = enqueue_htotx(c, 1.0, c->lf_dev.p, wfn, cic, cs, s)) return rc;
+  HIP_TRY(join_island(c, s));
+  const int nfc = S.nfcore, nr = S.nr, nl = S.nlat, L1 = S.L1;
+  GfockFcArgs ga{};
+  ga.nr = nr; ga.nlat = nl; ga.nradfc = S.nradfc; ga.nfun = S.nfun; ga.nfcore = nfc; ga.nfcore2 = S.nfcore2;
+  ga.ncore = S.ncore; ga.nact = S.nact; ga.npair = c->npair;
+  ga.pj = c->pj.p; ga.pi = c->pi.p; ga.ptype = c->ptype.p; ga.mval = c->mval.p; ga.den1 = c->den1.p;
+  ga.vfc = c->vfc_tab.p;
+  if (c->mixed) {
+    launch_gfock_fc<float2>(c->orbg_f, c->vg_f.p, c->gfc_gf.p, ga, s);
+    launch_transform_t<float2, double2>(0, c->gfc_gf.p, c->gfc_l.p, c->m_g2l.p, nullptr, c->b_g2l_gfc.p, 2 * nfc, nr,
+                                        nl, L1, pad_cols(L1), nullptr, s, c->tpath, nullptr, c->tvar);
+  } else {
+    launch_gfock_fc<double2>(c->orbg, c->vg.p, c->gfc_g.p, ga, s);
+    launch_transform(0, c->gfc_g.p, c->gfc_l.p, c->m_g2l.p, c->b_g2l_gfc.p, 2 * nfc, nr, nl, L1, pad_cols(L1),
+                     nullptr, s, c->tpath, nullptr, c->tvar);
+  }
+  DipoleArgs da{};
+  da.nr = nr; da.L1 = L1; da.nfun = S.nfun; da.nfcore = nfc; da.ncore = S.ncore; da.nact = S.nact;
+  da.nradfc = S.nradfc; da.mmax1 = S.mmax1; da.mval = c->mval.p; da.xr = c->xr.p; da.alph = c->alph.p;
+  da.orb = c->orb; da.den1 = c->den1.p;
+  da.ndvr = S.n; da.igauge = S.igauge; da.znuc = S.znuc; da.nel = (double)S.nel[2];
+  da.band_p = c->band_p.p; da.lf = c->lf_dev.p;
+  launch_dipole(da, 3, c->dip_part.p, c->dip3_tmp.p, s);
+  DipoleFcArgs fa{};
+  fa.nr = nr; fa.L1 = L1; fa.nfun = S.nfun; fa.nfcore = nfc; fa.ncore = S.ncore; fa.nact = S.nact;
+  fa.nradfc = S.nradfc; fa.mmax1 = S.mmax1; fa.ndvr = S.n; fa.igauge = S.igauge;
+  fa.mval = c->mval.p; fa.xr = c->xr.p; fa.alph = c->alph.p; fa.band_p = c->band_p.p; fa.lf = c->lf_dev.p;
+  const size_t nb = (size_t)c->nbas;
+  fa.orb = c->orb; fa.h01 = c->h01; fa.gdyn = c->gorb; fa.vdyn = c->v2orb;
+  fa.gfc = c->gfc_l.p; fa.vfc = c->gfc_l.p + (size_t)nfc * nb;
+  fa.den1 = c->den1.p;
+  const size_t need = (size_t)dipole_fc_quantities(fa) * dipole_fc_blocks(fa);
+  if (c->dipfc_part.n < need) return fail(-2, "dipole correction buffer too small");
+  launch_dipole_fc(fa, c->dipfc_part.p, c->dip3_tmp.p, d_out, s);
+  HIP_TRY(hipGetLastError());
+  return 0;
 }
 
 // Ionisation probabilities (hprod_ipx, src/hprod/hprod_ipx.F90:2-19, with

@@ -2163,6 +2163,239 @@ void launch_dipole(const DipoleArgs& a, int nforms, double2* partial, double* ou
 }
 
 // ---------------------------------------------------------------------------
+// hprod_dipole's frozen-core corrections (see kernels.hpp).  Print cadence:
+// thread per point, the small sums in fixed order.
+// ---------------------------------------------------------------------------
+// The frozen cores' mean-field rows on the theta grid, thread per (r <
+// nradfc, theta) (hprod_mfprod2_gfock's onFC branches,
+// src/hprod/hprod_mfprod2.F90: fcx1 :158-200, dcj :282-304, dcx :343-365,
+// actj :414-436, actx :489-513).
+template <class A2>
+__global__ void __launch_bounds__(256) k_gfock_fc(const A2* __restrict__ orbg, const A2* __restrict__ vg,
+                                                  A2* __restrict__ out, const GfockFcArgs a) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int th = blockIdx.y;
+  if (r >= a.nradfc) return;
+  const size_t ng = (size_t)a.nlat * a.nr;
+  const size_t g = (size_t)th * a.nr + r;
+  const int nfc = a.nfcore, ncore = a.ncore, na = a.nact;
+  A2 psi[kMaxFun], ag[kMaxFun], av[kMaxFun];
+  for (int f = 0; f < a.nfun; ++f) {
+    psi[f] = orbg[(size_t)f * ng + g];
+    ag[f] = cz<A2>();
+    av[f] = cz<A2>();
+  }
+  A2 d2v_dc = cz<A2>();
+  A2 vact[kMaxActPairs];
+  for (int p = 0; p < a.npair; ++p) {
+    const A2 v = vg[(size_t)p * ng + g];
+    const int j = a.pj[p], i = a.pi[p];
+    if (a.ptype[p] == 1) {  // v(dyn j, frozen core i)
+      if (j < ncore) {
+        ag[i] = csub(ag[i], cmul(v, psi[j]));  // dcx on FC
+      } else {
+        // actx on FC: v2_i += wd1(j) v(j, i), wd1(j) = -1/2 sum_k den1(k, j) psi_k
+        const int ja = j - ncore;
+        A2 wd = cz<A2>();
+        for (int ka = 0; ka < na; ++ka)
+          if (a.mval[ncore + ka] == a.mval[j]) cfma(wd, psi[ncore + ka], cscale(cto<A2>(a.den1[ka + na * ja]), -0.5));
+        cfma(av[i], wd, v);
+      }
+      continue;
+    }
+    if (j == i && j < ncore) d2v_dc = cadd(d2v_dc, cscale(v, 2.0));
+    if (j >= ncore && i >= ncore) {
+      vact[(j - ncore) * na + (i - ncore)] = v;
+      if (j != i) vact[(i - ncore) * na + (j - ncore)] = cconj(v);
+    }
+  }
+  for (int i = 0; i < nfc; ++i) cfma(ag[i], d2v_dc, psi[i]);  // dcj on FC
+  if (ncore > 0 && na > 0) {
+    A2 d = cz<A2>();
+    for (int ja = 0; ja < na; ++ja)
+      for (int ka = 0; ka < na; ++ka) cfma(d, vact[ja * na + ka], cto<A2>(a.den1[ka + na * ja]));
+    for (int i = 0; i < nfc; ++i) cfma(av[i], d, psi[i]);  // actj on FC
+  }
+  for (int i = 0; i < nfc; ++i)
+    for (int j = a.nfcore2; j < nfc; ++j)  // fcx1 on FC
+      ag[i] = csub(ag[i], cmul(cto<A2>(a.vfc[((size_t)i * nfc + j) * ng + g]), psi[j]));
+  for (int i = 0; i < nfc; ++i) {
+    out[(size_t)i * ng + g] = ag[i];
+    out[(size_t)(nfc + i) * ng + g] = av[i];
+  }
+}
+
+template <class T>
+void launch_gfock_fc(const T* orbg, const T* vg, T* out, const GfockFcArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL((k_gfock_fc<T>), dim3((a.nradfc + 255) / 256, a.nlat), dim3(256), 0, s, orbg, vg, out, a);
+}
+template void launch_gfock_fc<double2>(const double2*, const double2*, double2*, const GfockFcArgs&, hipStream_t);
+template void launch_gfock_fc<float2>(const float2*, const float2*, float2*, const GfockFcArgs&, hipStream_t);
+
+int dipole_fc_quantities(const DipoleFcArgs& a) { return 2 + 3 * a.nfcore * a.nfun; }
+int dipole_fc_blocks(const DipoleFcArgs& a) { return ((a.nradfc + 255) / 256) * a.L1; }
+
+// z phi and p_z phi of orbital x at (r, l), zfac = 1 (hprod_zprodp,
+// hprod_pzprodp as in k_dipole)
+__device__ void zp_point(const DipoleFcArgs& a, int x, int r, int l, double2& z, double2& p) {
+  const size_t nr = a.nr;
+  const int m = a.mval[x], am = m < 0 ? -m : m;
+  z = make_double2(0.0, 0.0);
+  p = make_double2(0.0, 0.0);
+  if (l < am) return;
+  const double xr = a.xr[r];
+  const double* al = a.alph + (size_t)(m + a.mmax1) * a.L1;
+  const double2* ox = a.orb + (size_t)x * a.L1 * nr;
+  if (l + 1 < a.L1) {
+    const double2 up = ox[(size_t)(l + 1) * nr + r];
+    cfma(z, up, make_double2(xr * al[l], 0.0));
+    const double2 dup = radial_derivative(ox + (size_t)(l + 1) * nr, a.band_p, a.nr, a.ndvr, r);
+    p = cadd(p, cadd(mul_minus_i(dup, al[l]), mul_minus_i(up, al[l] * (l + 1.0) / xr)));
+  }
+  if (l - 1 >= am) {
+    const double2 dn = ox[(size_t)(l - 1) * nr + r];
+    cfma(z, dn, make_double2(xr * al[l - 1], 0.0));
+    const double2 ddn = radial_derivative(ox + (size_t)(l - 1) * nr, a.band_p, a.nr, a.ndvr, r);
+    p = cadd(p, cadd(mul_minus_i(ddn, al[l - 1]), mul_minus_i(dn, -(al[l - 1] * (double)l) / xr)));
+  }
+}
+
+// the generalised Fock vector f_x at (r, l)
+__device__ double2 gfock_point(const DipoleFcArgs& a, int x, int r, int l) {
+  const size_t q = (size_t)l * a.nr + r, nb = (size_t)a.L1 * a.nr;
+  double2 f = a.h01[(size_t)x * nb + q];
+  if (x < a.nfcore) {
+    f = cadd(f, cadd(a.gfc[(size_t)x * nb + q], a.vfc[(size_t)x * nb + q]));
+  } else {
+    const int d = x - a.nfcore;
+    f = cadd(f, cadd(a.gdyn[(size_t)d * nb + q], a.vdyn[(size_t)d * nb + q]));
+    if (a.igauge == 1) {
+      double2 z, p;
+      zp_point(a, x, r, l, z, p);
+      cfma(f, z, make_double2(a.lf[5], 0.0));
+    }
+  }
+  return f;
+}
+
+__global__ void __launch_bounds__(256) k_dipole_fc(const DipoleFcArgs a, double2* __restrict__ partial) {
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  const int l = blockIdx.y;
+  const int q = blockIdx.z;
+  const int nf = a.nfun, nfc = a.nfcore;
+  double2 s = make_double2(0.0, 0.0);
+  if (r < a.nradfc) {
+    const size_t nb = (size_t)a.L1 * a.nr, at = (size_t)l * a.nr + r;
+    if (q < 2) {
+      for (int i = 0; i < nfc; ++i) {
+        double2 z, p;
+        zp_point(a, i, r, l, z, p);
+        cfma(s, cconj(q == 0 ? z : p), gfock_point(a, i, r, l));
+      }
+    } else if (q < 2 + nfc * nf) {
+      const int i = (q - 2) / nf, x = (q - 2) % nf;
+      if (a.mval[i] == a.mval[x]) s = cmul(cconj(a.orb[(size_t)i * nb + at]), gfock_point(a, x, r, l));
+    } else {
+      const int q2 = q - 2 - nfc * nf;
+      const int which = q2 / (nf * nfc), x = (q2 % (nf * nfc)) / nfc, i = q2 % nfc;
+      if (a.mval[i] == a.mval[x]) {
+        double2 z, p;
+        zp_point(a, x, r, l, z, p);
+        s = cmul(cconj(which == 0 ? z : p), a.orb[(size_t)i * nb + at]);
+      }
+    }
+  }
+  __shared__ double2 red[4];
+  for (int o = 32; o > 0; o >>= 1) {
+    s.x += __shfl_xor(s.x, o, 64);
+    s.y += __shfl_xor(s.y, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int nblk = gridDim.x * gridDim.y, b = blockIdx.y * gridDim.x + blockIdx.x;
+    partial[(size_t)q * nblk + b] = cadd(cadd(red[0], red[1]), cadd(red[2], red[3]));
+  }
+}
+
+// vel(2) = -4 Im S0 - 2 Im T_z, acc(2) = -4 Im S1 - 2 Im T_p with the trace
+// T = tr(D A, P_fc f) of hprod_trace1 over the projected f (hprod_projfc):
+// T = 2 sum_{core x} sum_{fc i} F(i, x) S_A(x, i)
+//   + sum_{act a} sum_{act j, m_j = m_a} conj(D(j, a)) sum_{fc i} F(i, a) S_A(j, i)
+__global__ void __launch_bounds__(256) k_dipole_fc_final(const double2* __restrict__ partial, int nblk, int nq,
+                                                         const DipoleFcArgs a, const double* __restrict__ d3,
+                                                         double* __restrict__ out) {
+  extern __shared__ double2 sq[];  // nq sums
+  __shared__ double2 red[4];
+  for (int q = 0; q < nq; ++q) {
+    double2 s = make_double2(0.0, 0.0);
+    for (int b = threadIdx.x; b < nblk; b += 256) s = cadd(s, partial[(size_t)q * nblk + b]);
+    for (int o = 32; o > 0; o >>= 1) {
+      s.x += __shfl_xor(s.x, o, 64);
+      s.y += __shfl_xor(s.y, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) sq[q] = cadd(cadd(red[0], red[1]), cadd(red[2], red[3]));
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  const int nf = a.nfun, nfc = a.nfcore, ncore = a.ncore, na = a.nact;
+  auto F = [&](int i, int x) { return sq[2 + i * nf + x]; };
+  auto S = [&](int w, int x, int i) { return sq[2 + nfc * nf + w * nf * nfc + x * nfc + i]; };
+  double corr[2];
+  for (int w = 0; w < 2; ++w) {
+    double2 t = make_double2(0.0, 0.0);
+    for (int x = 0; x < ncore; ++x)
+      for (int i = 0; i < nfc; ++i)
+        if (a.mval[i] == a.mval[x]) cfma(t, F(i, x), S(w, x, i));
+    t = cscale(t, 2.0);
+    for (int ia = 0; ia < na; ++ia) {
+      const int x = ncore + ia;
+      for (int ja = 0; ja < na; ++ja) {
+        const int j = ncore + ja;
+        if (a.mval[j] != a.mval[x]) continue;
+        double2 u = make_double2(0.0, 0.0);
+        for (int i = 0; i < nfc; ++i)
+          if (a.mval[i] == a.mval[x]) cfma(u, F(i, x), S(w, j, i));
+        cfma(t, cconj(a.den1[ja + na * ia]), u);
+      }
+    }
+    corr[w] = -4.0 * sq[w].y - 2.0 * t.y;
+  }
+  out[0] = d3[0];
+  out[1] = 0.0;
+  out[2] = d3[0];
+  out[3] = d3[1];
+  out[4] = corr[0];
+  out[5] = d3[1] + corr[0];
+  out[6] = d3[2];
+  out[7] = corr[1];
+  out[8] = d3[2] + corr[1];
+}
+
+__global__ void k_dipole_expand(const double* __restrict__ d3, double* __restrict__ out) {
+  if (threadIdx.x != 0) return;
+  for (int f = 0; f < 3; ++f) {
+    out[3 * f] = d3[f];
+    out[3 * f + 1] = 0.0;
+    out[3 * f + 2] = d3[f];
+  }
+}
+
+void launch_dipole_expand(const double* d3, double* out9, hipStream_t s) {
+  hipLaunchKernelGGL(k_dipole_expand, dim3(1), dim3(64), 0, s, d3, out9);
+}
+
+void launch_dipole_fc(const DipoleFcArgs& a, double2* partial, const double* d3, double* out9, hipStream_t s) {
+  const int nq = dipole_fc_quantities(a);
+  dim3 grid((a.nradfc + 255) / 256, a.L1, nq);
+  hipLaunchKernelGGL(k_dipole_fc, grid, dim3(256), 0, s, a, partial);
+  hipLaunchKernelGGL(k_dipole_fc_final, dim3(1), dim3(256), (size_t)nq * sizeof(double2), s, partial,
+                     (int)(grid.x * grid.y), nq, a, d3, out9);
+}
+
+// ---------------------------------------------------------------------------
 // Windowed orbital overlap (hprod_mkovlp, src/hprod/hprod_mkovlp.F90:2-91):
 // ovlp[j + nf i] = sum_{l >= |m|} sum_{r < cut} conj(orb_j(r, l)) orb_i(r, l)
 // for m_i == m_j, else 0.  One block per (j, i); threads stride over (l, r);

@@ -268,6 +268,41 @@ struct DipoleArgs {
   const double* band_p;   // [2n+1][nr] d/dr band
   const double* lf;       // device lfield[9]
 };
+// hprod_dipole's frozen-core corrections (src/hprod/hprod_dipole.F90:74-125).
+// The generalised Fock rows of the frozen cores (hprod_gfock with the onFC
+// branches of hprod_mfprod2_gfock): on the theta grid, rows r < nradfc,
+// out[0][i] = g_i (dcj, dcx, fcx1 on the frozen core i), out[1][i] = v2_i
+// (actj, actx on it).  The pair potentials are the hprod's (vg, with pj, pi,
+// ptype as MfArgs), the frozen-core pairs vfc[(i nfc + j) ng + x] = v(j, i)
+// (the potential of conj(phi_j) phi_i).
+struct GfockFcArgs {
+  int nr, nlat, nradfc, nfun, nfcore, nfcore2, ncore, nact, npair;
+  const int *pj, *pi, *ptype, *mval;
+  const double2* den1;
+  const double2* vfc;
+};
+template <class T>
+void launch_gfock_fc(const T* orbg, const T* vg, T* out, const GfockFcArgs& a, hipStream_t s);
+// The correction sums over the rows r < nradfc and all l, with
+// f_x = h01_x + g_x + v2_x (+ E_z z phi_x for the dynamical orbitals in the
+// velocity gauge; the frozen cores' h01 carries it already), Z = z phi,
+// P = p_z phi at zfac = 1: q = 0: sum_fc <Z_i|f_i>, 1: sum_fc <P_i|f_i>,
+// 2 + i nf + x: <phi_i|f_x> (i frozen, m_i = m_x), 2 + nfc nf + x nfc + i:
+// <Z_x|phi_i>, 2 + 2 nfc nf + x nfc + i: <P_x|phi_i>.  The final kernel
+// forms vel(2), acc(2) from them and writes {dip, vel, acc} x {(1), (2),
+// (3)} with the (1) terms read from d3.
+struct DipoleFcArgs {
+  int nr, L1, nfun, nfcore, ncore, nact, nradfc, mmax1, ndvr, igauge;
+  const int* mval;
+  const double *xr, *alph, *band_p, *lf;
+  const double2 *orb, *h01, *gdyn, *vdyn, *gfc, *vfc;  // gdyn, vdyn: [nfun - nfcore][L1][nr]; gfc, vfc: [nfcore][L1][nr]
+  const double2* den1;
+};
+int dipole_fc_quantities(const DipoleFcArgs& a);
+int dipole_fc_blocks(const DipoleFcArgs& a);  // per quantity
+void launch_dipole_fc(const DipoleFcArgs& a, double2* partial, const double* d3, double* out9, hipStream_t s);
+// out9 = {d3[f], 0, d3[f]} per form f (no frozen core: no corrections)
+void launch_dipole_expand(const double* d3, double* out9, hipStream_t s);
 // partial: nforms * dipole_blocks(a) entries
 int dipole_blocks(const DipoleArgs& a);
 void launch_dipole(const DipoleArgs& a, int nforms, double2* partial, double* out, hipStream_t s);

@@ -7,6 +7,8 @@ Hence relative L2 tolerance 1e-11 on the hprod output and energies."""
 import numpy as np
 import pytest
 
+from td1c_amd.abi import Problem
+from td1c_amd import workloads as W
 from tests import cases as C
 
 pytestmark = pytest.mark.gpu
@@ -232,6 +234,64 @@ def test_dipole_forms_match_oracle(gpu_lib, name, lf, precision):
     tol = RTOL if precision == 0 else 1e-5
     assert np.all(np.abs(got - ref) <= tol * np.maximum(1.0, np.abs(ref))), (got, ref)
     assert abs(hp.dipole(lf, d) - got[0]) <= 1e-14 * max(1.0, abs(got[0]))
+
+
+def _fc_li_like_gpu(igauge):
+    """A frozen hydrogenic 1s (Z = 3) and one dynamical orbital
+    (2p0 + e^{0.7 i} 3d0)/sqrt(2) of hydrogenic radial shapes."""
+    prob = Problem(nfe=30, ndvr=20, rmax=60.0, lmax1=5, lmax2=10, znuc=3.0, nfcore1=1, ndcore=1, nel_alpha=2,
+                   nel_beta=2, lval=[0, 1], igauge=igauge)
+    r, w = W.gll_grid(prob)
+    nr = prob.nr
+    nb = nr * (prob.lmax1 + 1)
+    wf = np.zeros(2 * nb + 1, complex)
+    wf[:nr] = 2 * 3.0 ** 1.5 * r * np.exp(-3.0 * r) * np.sqrt(w)
+    u2p = r * r * np.exp(-r / 2) * np.sqrt(w)
+    u3d = r ** 3 * np.exp(-r / 3) * np.sqrt(w)
+    wf[nb + nr:nb + 2 * nr] = u2p / np.linalg.norm(u2p) / np.sqrt(2)
+    wf[nb + 2 * nr:nb + 3 * nr] = np.exp(0.7j) * u3d / np.linalg.norm(u3d) / np.sqrt(2)
+    wf[-1] = 1.0
+    return prob, wf
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("name,lf", [("ne_small", C.LF_VEL), ("ne_small_len", C.LF_LEN), ("cas_fc_small", C.LF_VEL),
+                                     ("li_like", C.LF_VEL), ("li_like_len", C.LF_LEN), ("he_small", C.LF_LEN),
+                                     ("cas_small", C.LF_LEN)])
+def test_dipole_all_matches_oracle(gpu_lib, name, lf, precision):
+    """td1c_dipole_all (hprod_dipole with the frozen-core corrections of the
+    generalised Fock vector) against the oracle's orc_dipole_all, frozen-core
+    problems in both gauges and, without frozen cores, the plain traces."""
+    import td1c_amd
+    from oracle.oracle import Oracle
+
+    if name.startswith("li_like"):
+        prob, w = _fc_li_like_gpu(0 if name.endswith("_len") else 1)
+        prob.precision = precision
+    elif name == "ne_small_len":
+        prob = C.ne_small(igauge=0, precision=precision)
+        w = None
+    else:
+        prob = getattr(C, name)(precision=precision)
+        w = None
+    o = Oracle(prob)
+    if w is None:
+        w = C.wfn_for(prob, o.lcic)
+    o.set_wfn0(w)
+    ref = o.dipole_all(lf, w)
+    hp = td1c_amd.HProd(prob)
+    hp.set_wfn0(w)
+    d = hp.to_device(w)
+    print(f"dipole_all {name} precision {precision}", flush=True)
+    got = hp.dipole_all(lf, d)
+    tol = RTOL if precision == 0 else 1e-5
+    scale = max(1.0, np.abs(ref).max())
+    assert np.all(np.abs(got - ref) <= tol * scale), (got, ref)
+    if prob.nfcore > 0:
+        assert abs(ref[4]) + abs(ref[7]) > 1e-6  # the corrections are present
+    else:
+        assert got[4] == got[7] == 0.0
+    assert np.abs(got[[0, 3, 6]] - hp.dipole3(lf, d)).max() <= 1e-13 * scale
 
 
 @pytest.mark.parametrize("precision", [0, 1])
