@@ -270,6 +270,23 @@ int td1c_wfn_write(const char* orb_path, const char* cic_path, int nrad, int lma
 int td1c_wfn_read(const char* orb_path, const char* cic_path, int nrad, int lmax1,
                   int nfun, int lcic, int cic0, double* wfn);
 
+/* Grid projection of a state between two problems, host memory only (no GPU
+ * call): clbas::proj (src/bas/clbas.cpp:280-332) with the CI copy of
+ * guess_type = "read_proj" (src/main/guess.cpp:37-47).  Each interior point r
+ * of `to`'s grid gets sqrt(w(r)) * sum_m c_m L_m(r) over the stored points of
+ * the `from` element holding r (the first whose right edge exceeds r + 1e-10,
+ * clfedvr::get_ife, src/fedvr/clfedvr.cpp:628-638), L_m the element's
+ * Lagrange polynomials (cldvr::get_val, src/dvr/cldvr.cpp:223-235) and c_m the
+ * source coefficients as stored.  Orbitals and l up to the smaller of the two
+ * problems' counts are mapped (the reference loops to the larger and then
+ * reads across orbital blocks); points beyond `from`'s grid stay zero (the
+ * reference indexes element -1 there).  The first min(lcic) CI coefficients
+ * are copied, the rest are zero.  wfn_from / wfn_to: complex128 states
+ * (orbitals then CI) of the two problems.  The reference then calls
+ * clwfn::orth: td1c_orth on the device copy. */
+int td1c_wfn_proj(const td1c_problem* from, const double* wfn_from,
+                  const td1c_problem* to, double* wfn_to);
+
 /* Layout of the CI vector of a problem, computed on the host (no device, no
  * context): the reference's ORMAS addressing for one subspace.  Strings of
  * one spin are numbered by arc weights (ormas_arcwgt,

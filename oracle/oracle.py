@@ -73,6 +73,8 @@ def lib(openmp: bool = False):
         L.orc_ipd.restype = ctypes.c_int
         L.orc_orth.argtypes = [vp, vp, vp]
         L.orc_orth.restype = None
+        L.orc_wfn_proj.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.orc_wfn_proj.restype = None
         L.orc_band_chol.argtypes = [ctypes.c_int, ctypes.c_int, vp]
         L.orc_band_chol.restype = ctypes.c_int
         L.orc_band_chol_solve.argtypes = [ctypes.c_int, ctypes.c_int, vp, vp]
@@ -256,6 +258,18 @@ class Oracle:
         cic = w[self.norb:]
         self._L.orc_orth(self.h, _p(w[: self.norb]), _p(cic) if cic.size else None)
         return w
+
+    def proj_to(self, other: "Oracle", wfn: np.ndarray) -> np.ndarray:
+        """clbas::proj from this problem's grid to other's, with the CI copy of
+        guess read_proj; returns other's wave function (not orthonormalised)."""
+        w = np.ascontiguousarray(wfn, dtype=np.complex128)
+        if w.size != self.wfn_size:
+            raise ValueError("wave function size does not match the source problem")
+        out = np.zeros(other.wfn_size, np.complex128)
+        cic1 = np.ascontiguousarray(w[self.norb:])
+        self._L.orc_wfn_proj(self.h, _p(w[: self.norb]), _p(cic1), other.h, _p(out[: other.norb]),
+                             _p(out[other.norb:]))
+        return out
 
     def energies(self) -> np.ndarray:
         e = np.zeros(5)

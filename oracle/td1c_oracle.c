@@ -37,7 +37,9 @@
  * frozen-core energy (He 1s^2, all-frozen closed shell), and the CI layout
  * (determinant order by an enumeration of the ORMAS arc-weight / M-sort
  * algorithm, determinant phase by RDMs built with plain second
- * quantisation, tests/test_ci_layout.py).
+ * quantisation, tests/test_ci_layout.py), and the grid projection
+ * (Kronecker-delta nodes, polynomial reproduction across grids, zero past
+ * the source box, tests/test_wfn_proj.py).
  * Parity unpinned:
  *  - the sign convention of the m < 0 Legendre columns (copied from |m|
  *    without (-1)^m, as sph_gen.F90 does; only products of two columns enter
@@ -2511,6 +2513,60 @@ void orc_orth(orc* o, zc* orb, zc* cic) {
     nrm = 1.0 / sqrt(nrm);
     for (int l = abs(mi); l <= o->lmax1; ++l)
       for (int r = 0; r < nr; ++r) orb[OIDX(r, l, i)] *= nrm;
+  }
+}
+
+/* cldvr::get_val (src/dvr/cldvr.cpp:223-235): the Lagrange polynomial of node
+ * m of element e at r, over the element's ndvr+1 nodes. */
+static double dvr_val(const orc* o, int e, int m, double r) {
+  const double* xp = o->dvr_x + (size_t)e * (o->ndvr + 1);
+  double v = 1.0;
+  for (int k = 0; k <= o->ndvr; ++k)
+    if (k != m) v *= (r - xp[k]) / (xp[m] - xp[k]);
+  return v;
+}
+
+/* clfedvr::get_ife (src/fedvr/clfedvr.cpp:628-638). */
+static int dvr_ife(const orc* o, double r) {
+  for (int i = 0; i < o->nfe; ++i)
+    if (r + 1e-10 < o->x1[i]) return i;
+  return -1;
+}
+
+/* clbas::proj (src/bas/clbas.cpp:280-332) from grid a to grid b, then the CI
+ * copy of guess read_proj (src/main/guess.cpp:47).  Loops run over the orbitals
+ * and l both problems hold (the reference runs them to the larger counts);
+ * a target point beyond a's last edge (ife = -1, where the reference indexes
+ * element -1) gets nothing.  CI: the first min(lcic) coefficients. */
+void orc_wfn_proj(const orc* a, const zc* orb1, const zc* cic1, const orc* b, zc* orb2, zc* cic2) {
+  int nfun = a->nfun < b->nfun ? a->nfun : b->nfun;
+  int lmax = a->lmax1 < b->lmax1 ? a->lmax1 : b->lmax1;
+  memset(orb2, 0, sizeof(zc) * (size_t)b->nfun * b->nbas);
+  for (int irad2 = 1; irad2 < b->nrad; ++irad2) {
+    double rval = b->xrad[irad2], swgt = sqrt(b->wrad[irad2]);
+    int ife1 = dvr_ife(a, rval);
+    if (ife1 < 0) continue;
+    for (int m = (1 - ife1 > 0 ? 1 - ife1 : 0); m < a->ndvr; ++m) {
+      int irad1 = a->mapf[ife1] + m;
+      double mat = dvr_val(a, ife1, m, rval) * swgt;
+      for (int ifun = 0; ifun < nfun; ++ifun)
+        for (int l = 0; l <= lmax; ++l)
+          orb2[(size_t)ifun * b->nbas + (size_t)l * b->nr + irad2 - 1] +=
+              orb1[(size_t)ifun * a->nbas + (size_t)l * a->nr + irad1 - 1] * mat;
+    }
+    if (ife1 < a->nfe - 1) {
+      int irad1 = a->mapf[ife1 + 1];
+      double mat = dvr_val(a, ife1, a->ndvr, rval) * swgt;
+      for (int ifun = 0; ifun < nfun; ++ifun)
+        for (int l = 0; l <= lmax; ++l)
+          orb2[(size_t)ifun * b->nbas + (size_t)l * b->nr + irad2 - 1] +=
+              orb1[(size_t)ifun * a->nbas + (size_t)l * a->nr + irad1 - 1] * mat;
+    }
+  }
+  if (cic2) {
+    memset(cic2, 0, sizeof(zc) * b->lcic);
+    int nci = a->lcic < b->lcic ? a->lcic : b->lcic;
+    if (cic1) memcpy(cic2, cic1, sizeof(zc) * nci);
   }
 }
 

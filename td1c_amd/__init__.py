@@ -87,6 +87,8 @@ def load_library():
     L.td1c_wfn_write.restype = ci
     L.td1c_wfn_read.argtypes = [cp, cp, ci, ci, ci, ci, ci, vp]
     L.td1c_wfn_read.restype = ci
+    L.td1c_wfn_proj.argtypes = [ctypes.POINTER(TD1CProblem), vp, ctypes.POINTER(TD1CProblem), vp]
+    L.td1c_wfn_proj.restype = ci
     L.td1c_ci_layout.argtypes = [ctypes.POINTER(TD1CProblem), vp, vp, vp, vp, vp, vp, vp, vp]
     L.td1c_ci_layout.restype = ci
     L.td1c_htotx.argtypes = [vp, ctypes.c_double, vp, vp, vp, vp]
@@ -211,6 +213,24 @@ def wfn_read(orb_path: str, cic_path: str, prob: Problem, lcic: int):
     if rc not in (0, 1):
         _check(rc, "td1c_wfn_read")
     return w, rc == 0
+
+
+def wfn_proj(prob_from: Problem, wfn_from: np.ndarray, prob_to: Problem) -> np.ndarray:
+    """clbas::proj with the CI copy of guess_type "read_proj": the state of
+    prob_from (host complex128, orbitals then CI) on prob_to's grid and sizes.
+    Host computation, no device; orthonormalise the device copy with
+    HProd.orth as clwfn::orth follows in the reference."""
+    lc_from = ci_layout(prob_from)["lcic"]
+    lc_to = ci_layout(prob_to)["lcic"]
+    w1 = np.ascontiguousarray(wfn_from, dtype=np.complex128)
+    if w1.size != prob_from.nfun * prob_from.nbas + lc_from:
+        raise ValueError("wave function size does not match the source problem")
+    w2 = np.zeros(prob_to.nfun * prob_to.nbas + lc_to, np.complex128)
+    a, ka = prob_from.to_c()
+    b, kb = prob_to.to_c()
+    _check(load_library().td1c_wfn_proj(ctypes.byref(a), _ptr(w1), ctypes.byref(b), _ptr(w2)), "td1c_wfn_proj")
+    del ka, kb
+    return w2
 
 
 class HProd:

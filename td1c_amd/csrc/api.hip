@@ -1061,6 +1061,7 @@ const char* td1c_last_error(void) { return g_error.c_str(); }
 // process by problem shape, so contexts of the same shape tune once.
 // TD1C_AUTOTUNE=0 keeps the built-in defaults.
 // ---------------------------------------------------------------------------
+extern "C++" {
 namespace {
 
 std::string onebody_shape_key(const Setup& S) {
@@ -1133,6 +1134,7 @@ int autotune_onebody(td1c_ctx* c) {
 }
 
 }  // namespace
+}  // extern "C++"
 
 int td1c_ctx_create(const td1c_problem* prob, td1c_ctx** out) {
   if (!prob || !out) return fail(-1, "null argument");
@@ -1187,10 +1189,8 @@ int td1c_ctx_destroy(td1c_ctx* ctx) {
   return 0;
 }
 
-int td1c_ci_layout(const td1c_problem* p, int* nstr_alpha, int* nstr_beta, int* lcic, int* cic0,
-                   unsigned int* str_alpha, unsigned int* str_beta, int* det_alpha, int* det_beta) {
-  if (!p) return fail(-1, "null argument");
-  Setup S;
+// The orbital counts and CAS string tables of a problem, without its grid.
+static int ci_setup(const td1c_problem* p, Setup& S) {
   S.nfcore = p->nfcore2 + p->nfcore1;
   S.ncore = S.nfcore + p->ndcore;
   S.nact = p->nact;
@@ -1204,6 +1204,14 @@ int td1c_ci_layout(const td1c_problem* p, int* nstr_alpha, int* nstr_beta, int* 
     return fail(-1, "electron count does not fit the orbital classes");
   S.mtot = p->mtot;
   if (!S.build_ci()) return fail(-1, S.error);
+  return 0;
+}
+
+int td1c_ci_layout(const td1c_problem* p, int* nstr_alpha, int* nstr_beta, int* lcic, int* cic0,
+                   unsigned int* str_alpha, unsigned int* str_beta, int* det_alpha, int* det_beta) {
+  if (!p) return fail(-1, "null argument");
+  Setup S;
+  if (int rc = ci_setup(p, S)) return rc;
   if (nstr_alpha) *nstr_alpha = S.nact > 0 ? S.nstr_a : 0;
   if (nstr_beta) *nstr_beta = S.nact > 0 ? S.nstr_b : 0;
   if (lcic) *lcic = S.lcic;
@@ -1795,6 +1803,71 @@ int td1c_wfn_read(const char* orb_path, const char* cic_path, int nrad, int lmax
   }
   std::fclose(fc);
   return rc;
+}
+
+// clbas::proj (src/bas/clbas.cpp:280-332) as guess_type = "read_proj" uses it
+// (src/main/guess.cpp:37-47): every interior point r of the target grid gets
+// sqrt(w_target(r)) * sum_m c_m L_m(r), L_m the Lagrange polynomials of the
+// source element holding r (clfedvr::get_val, src/fedvr/clfedvr.cpp:651-675,
+// cldvr::get_val, src/dvr/cldvr.cpp:223-235) over its stored points, its right
+// bridge included; the source coefficients c_m enter as they are (without
+// 1/sqrt(w_source), as in the reference).  The source element is the first
+// with r + 1e-10 < its right edge (clfedvr::get_ife, :628-638); points beyond
+// the source grid stay zero.  The CI vector is copied (zcopy of the target's
+// length, guess.cpp:47); orthonormalisation (clwfn::orth) is td1c_orth's.
+int td1c_wfn_proj(const td1c_problem* from, const double* wfn_from, const td1c_problem* to, double* wfn_to) {
+  if (!from || !to || !wfn_from || !wfn_to) return fail(-1, "null argument");
+  for (const td1c_problem* p : {from, to}) {
+    if (p->nfe < 2 || p->ndvr < 2 || !p->edges) return fail(-1, "need nfe >= 2, ndvr >= 2 and element edges");
+    if (p->lmax1 < 0) return fail(-1, "bad lmax1");
+    for (int e = 0; e < p->nfe; ++e)
+      if (!(p->edges[e + 1] > p->edges[e])) return fail(-1, "element edges must increase");
+  }
+  Setup A, B;
+  if (int rc = ci_setup(from, A)) return rc;
+  if (int rc = ci_setup(to, B)) return rc;
+  const int n1 = from->ndvr, nfe1 = from->nfe, np1 = n1 + 1;
+  const int n2 = to->ndvr, nfe2 = to->nfe;
+  const int nr1 = n1 * nfe1 - 1, nr2 = n2 * nfe2 - 1;
+  const int L1a = from->lmax1 + 1, L1b = to->lmax1 + 1;
+  std::vector<double> ex1, ew1, xr1, wr1, ex2, ew2, xr2, wr2;
+  fedvr_nodes(nfe1, n1, from->edges, ex1, ew1, xr1, wr1);
+  fedvr_nodes(nfe2, n2, to->edges, ex2, ew2, xr2, wr2);
+  const size_t no1 = (size_t)A.nfun * L1a * nr1, no2 = (size_t)B.nfun * L1b * nr2;
+  std::fill(wfn_to, wfn_to + 2 * (no2 + B.lcic), 0.0);
+  const int nf = std::min(A.nfun, B.nfun), nl = std::min(L1a, L1b);
+  // one target point: the source points it reads and their weights
+  std::vector<int> src(np1);
+  std::vector<double> val(np1);
+  for (int i2 = 1; i2 <= nr2; ++i2) {
+    const double r = xr2[i2], sw = std::sqrt(wr2[i2]);
+    int e1 = -1;
+    for (int e = 0; e < nfe1; ++e)
+      if (r + 1e-10 < from->edges[e + 1]) { e1 = e; break; }
+    if (e1 < 0) continue;
+    const double* xe = &ex1[(size_t)e1 * np1];
+    auto lagrange = [&](int m) {
+      double v = 1.0;
+      for (int k = 0; k <= n1; ++k)
+        if (k != m) v *= (r - xe[k]) / (xe[m] - xe[k]);
+      return v;
+    };
+    int nt = 0;
+    for (int m = e1 == 0 ? 1 : 0; m < n1; ++m) { src[nt] = e1 * n1 + m; val[nt++] = lagrange(m) * sw; }
+    if (e1 < nfe1 - 1) { src[nt] = (e1 + 1) * n1; val[nt++] = lagrange(n1) * sw; }
+    for (int f = 0; f < nf; ++f)
+      for (int l = 0; l < nl; ++l) {
+        const double* c1 = wfn_from + 2 * ((size_t)f * L1a + l) * nr1;  // c1[2 (i1 - 1)]: point i1
+        double* c2 = wfn_to + 2 * (((size_t)f * L1b + l) * nr2 + i2 - 1);
+        for (int t = 0; t < nt; ++t) {
+          c2[0] += c1[2 * (src[t] - 1)] * val[t];
+          c2[1] += c1[2 * (src[t] - 1) + 1] * val[t];
+        }
+      }
+  }
+  const int nci = std::min(A.lcic, B.lcic);
+  std::copy(wfn_from + 2 * no1, wfn_from + 2 * (no1 + nci), wfn_to + 2 * no2);
+  return 0;
 }
 
 // clcrk tableaux for rk_order 1..4 (src/prop/clcrk.cpp:183-235)

@@ -190,6 +190,41 @@ int popc(uint32_t v) { return __builtin_popcount(v); }
 
 }  // namespace
 
+// Element nodes / weights (cldvr::gen, src/dvr/cldvr.cpp:57-221): Gauss-Lobatto
+// points of order n mapped onto [edges[e], edges[e+1]]; global points 0..n*nfe
+// with the weights of the two elements summed at each bridge
+// (clfedvr::gen_grid, src/fedvr/clfedvr.cpp:400-511).
+void fedvr_nodes(int nfe, int n, const double* edges, std::vector<double>& ex, std::vector<double>& ew,
+                 std::vector<double>& xrad, std::vector<double>& wrad) {
+  const int np = n + 1, nrad = n * nfe;
+  std::vector<long double> inner;
+  lobatto_interior(n, inner);
+  std::vector<double> ref_x(np), ref_w(np);
+  ref_x[0] = -1.0; ref_x[n] = 1.0;
+  for (int k = 1; k < n; ++k) ref_x[k] = (double)inner[k - 1];
+  const double scale = 2.0 / ((double)n * (n + 1.0));
+  ref_w[0] = ref_w[n] = scale;
+  for (int k = 1; k < n; ++k) {
+    double pv = (double)legendre_value(n, std::fabs(ref_x[k]));
+    ref_w[k] = scale / (pv * pv);
+  }
+  ex.assign((size_t)nfe * np, 0); ew.assign((size_t)nfe * np, 0);
+  for (int e = 0; e < nfe; ++e) {
+    double half = 0.5 * (edges[e + 1] - edges[e]), mid = 0.5 * (edges[e + 1] + edges[e]);
+    for (int k = 0; k < np; ++k) {
+      ex[(size_t)e * np + k] = ref_x[k] * half + mid;
+      ew[(size_t)e * np + k] = ref_w[k] * half;
+    }
+  }
+  xrad.assign(nrad + 1, 0); wrad.assign(nrad + 1, 0);
+  for (int ib = 0; ib <= nrad; ++ib) {
+    int e = std::min(ib / n, nfe - 1), m = ib - e * n;
+    xrad[ib] = ex[(size_t)e * np + m];
+    wrad[ib] = ew[(size_t)e * np + m];
+  }
+  for (int e = 1; e < nfe; ++e) wrad[e * n] += ew[(size_t)(e - 1) * np + n];
+}
+
 bool Setup::build(const td1c_problem& p) {
   if (p.nfe < 2 || p.ndvr < 2 || !p.edges) { error = "need nfe >= 2, ndvr >= 2 and element edges"; return false; }
   if (p.lmax1 < 0 || p.lmax2 < 0) { error = "bad lmax1/lmax2"; return false; }
@@ -237,25 +272,13 @@ bool Setup::build(const td1c_problem& p) {
   nradfc = p.nradfc > 0 ? p.nradfc : nr;
   if (nradfc != nr && nfcore1 > 0) { error = "frozen-core exchange pairs need nradfc = nrad-1"; return false; }
 
-  // ---- element DVR (cldvr::gen, src/dvr/cldvr.cpp:57-221) ----
+  // ---- element DVR and global points ----
   const int np = n + 1;
-  std::vector<long double> inner;
-  lobatto_interior(n, inner);
-  std::vector<double> ref_x(np), ref_w(np);
-  ref_x[0] = -1.0; ref_x[n] = 1.0;
-  for (int k = 1; k < n; ++k) ref_x[k] = (double)inner[k - 1];
-  const double scale = 2.0 / ((double)n * (n + 1.0));
-  ref_w[0] = ref_w[n] = scale;
-  for (int k = 1; k < n; ++k) {
-    double pv = (double)legendre_value(n, std::fabs(ref_x[k]));
-    ref_w[k] = scale / (pv * pv);
-  }
-  ex.assign((size_t)nfe * np, 0); ew.assign((size_t)nfe * np, 0); eds.assign((size_t)nfe * np * np, 0);
+  fedvr_nodes(nfe, n, p.edges, ex, ew, xrad, wrad);
+  eds.assign((size_t)nfe * np * np, 0);
   for (int e = 0; e < nfe; ++e) {
-    double half = 0.5 * (x1[e] - x0[e]), mid = 0.5 * (x1[e] + x0[e]);
-    double* xe = &ex[(size_t)e * np];
-    double* we = &ew[(size_t)e * np];
-    for (int k = 0; k < np; ++k) { xe[k] = ref_x[k] * half + mid; we[k] = ref_w[k] * half; }
+    const double* xe = &ex[(size_t)e * np];
+    const double* we = &ew[(size_t)e * np];
     double* ds = &eds[(size_t)e * np * np];  // ds[np*m + k] = f_m'(x_k)
     for (int m = 0; m < np; ++m)
       for (int k = 0; k < np; ++k) {
@@ -268,14 +291,7 @@ bool Setup::build(const td1c_problem& p) {
     ds[0] = -0.5 / we[0];
     ds[np * np - 1] = 0.5 / we[n];
   }
-  // global points (clfedvr::gen_grid, src/fedvr/clfedvr.cpp:400-511)
-  xrad.assign(nrad + 1, 0); wrad.assign(nrad + 1, 0); mask.assign(nrad + 1, 0);
-  for (int ib = 0; ib <= nrad; ++ib) {
-    int e = std::min(ib / n, nfe - 1), m = ib - e * n;
-    xrad[ib] = ex[(size_t)e * np + m];
-    wrad[ib] = ew[(size_t)e * np + m];
-  }
-  for (int e = 1; e < nfe; ++e) wrad[e * n] += ew[(size_t)(e - 1) * np + n];
+  mask.assign(nrad + 1, 0);
   // mask (clfedvr::gen_mask, :361-398)
   {
     double order = 1.0 / mask_order, rul = x1[nfe - 1];

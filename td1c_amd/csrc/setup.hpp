@@ -96,6 +96,12 @@ struct Setup {
   bool build_ci();  // the CAS string tables (called by build)
 };
 
+// FEDVR nodes and weights of nfe elements of order n on edges[0..nfe]:
+// ex / ew [nfe][n+1] per element, xrad / wrad [n*nfe+1] global points with the
+// bridge weights summed.
+void fedvr_nodes(int nfe, int n, const double* edges, std::vector<double>& ex, std::vector<double>& ew,
+                 std::vector<double>& xrad, std::vector<double>& wrad);
+
 // Crank-Nicolson factors of the one-body half step of split_type = 1
 // (clh1itr, h1itr_gen_cninv: 1 + fac T_l per l), in the element Schur form
 // of the Poisson solve: per (l, e) the inverse interior block G (complex
