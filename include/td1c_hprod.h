@@ -270,6 +270,35 @@ int td1c_wfn_write(const char* orb_path, const char* cic_path, int nrad, int lma
 int td1c_wfn_read(const char* orb_path, const char* cic_path, int nrad, int lmax1,
                   int nfun, int lcic, int cic0, double* wfn);
 
+/* cletdrb (src/prop/cletdrb.cpp), the reference's default prop_type:
+ * exponential Rosenbrock RK4 (prop4_orb, :321-434) with the [2/3] Pade forms
+ * of phi1, phi2, phi3 (gen_phi1 / gen_phid1 / gen_phi2 / gen_phi3,
+ * :636-1185) applied to the orbitals as rational functions of h1 = T + V(t)
+ * (clh1rat, src/prop/clh1rat.cpp; h1rat_gen / h1rat_prod,
+ * src/prop/h1rat_gen.F90, h1rat_prod.F90): numerators by applications of h1,
+ * each denominator factor cd0 + cd1 (T + V) by the element Schur solve of
+ * cd0 + cd1 T and the Neumann series in V (at most maxcyc terms, RMS
+ * threshold thresh; h1rat_maxcyc 20 and h1rat_thresh 1e-10 by default,
+ * src/control/clcontrol.cpp:123-124).  The settings are cletdrb::gen's
+ * defaults (:25-92): etd_rk_order 4, etd_dim_numer* 2, etd_dim_denom* 3,
+ * etd_time_of_h 0, etd_cisplit -1; rosenbrock is etd_rosenbrock (default
+ * 1).  Needs split_type = 1 and lconst = 0; with frozen cores call after
+ * td1c_set_wfn0 (the factors include the frozen-core Hartree potential).
+ * td1c_etdrb_gen builds the tables for step dtime (cletdrb::gen). */
+int td1c_etdrb_gen(td1c_ctx* ctx, double dtime, int rosenbrock, int maxcyc, double thresh);
+
+/* One cletdrb step of the device wave function (orbitals then CI) with the
+ * field arrays given on the host, lfields[6][9]: lfield(t), lfield(t + dt/2),
+ * lfield(t + dt), d lfield/dt at t (td1c_field_der), lfield(t + dt/2) -
+ * lfield(t), lfield(t + dt) - lfield(t); then the mask when apply_mask != 0,
+ * as tdse_prop12 does after every step (src/main/tdse.cpp:268-309). */
+int td1c_etdrb_prop(td1c_ctx* ctx, const double* lfields, double* d_wfn, int apply_mask, void* stream);
+
+/* nstep cletdrb steps of tdse_prop12 from t0 with the field evaluated on the
+ * device; dtime must be td1c_etdrb_gen's. */
+int td1c_etdrb_propagate(td1c_ctx* ctx, const td1c_field* field, double t0, double dtime, int nstep,
+                         double* d_wfn, int apply_mask, void* stream);
+
 /* Grid projection of a state between two problems, host memory only (no GPU
  * call): clbas::proj (src/bas/clbas.cpp:280-332) with the CI copy of
  * guess_type = "read_proj" (src/main/guess.cpp:37-47).  Each interior point r
@@ -421,6 +450,13 @@ int td1c_mask(td1c_ctx* ctx, double* d_wfn, void* stream);
 /* Field value at time t: lfield[0..2] chosen gauge, [3..5] E, [6..8] A
  * (src/field/clfield1.cpp:131-154). t is measured from the pulse start. */
 int td1c_field_value(const td1c_field* field, double t, double lfield[9]);
+
+/* Time derivative of the field at t in dfield[0..2] (clfield::get_der,
+ * src/field/clfield.cpp:142-154, clfield1::get_der,
+ * src/field/clfield1.cpp:193-211): the analytic dE/dt in the length gauge,
+ * dA/dt = -E (the 6-point difference of get_evalue_numder) in the velocity
+ * gauge; zero outside the pulse. */
+int td1c_field_der(const td1c_field* field, double t, double dfield[9]);
 
 /* Optical period (a.u.) of the field, used to set dtime = period/nstep. */
 double td1c_field_period(const td1c_field* field);

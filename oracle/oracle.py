@@ -75,6 +75,12 @@ def lib(openmp: bool = False):
         L.orc_orth.restype = None
         L.orc_wfn_proj.argtypes = [vp, vp, vp, vp, vp, vp]
         L.orc_wfn_proj.restype = None
+        L.orc_etdrb_gen.argtypes = [vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_double]
+        L.orc_etdrb_gen.restype = ctypes.c_int
+        L.orc_etdrb_step.argtypes = [vp, vp, vp, ctypes.c_int]
+        L.orc_etdrb_step.restype = ctypes.c_int
+        L.orc_etd_rat_scalar.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, vp]
+        L.orc_etd_rat_scalar.restype = None
         L.orc_band_chol.argtypes = [ctypes.c_int, ctypes.c_int, vp]
         L.orc_band_chol.restype = ctypes.c_int
         L.orc_band_chol_solve.argtypes = [ctypes.c_int, ctypes.c_int, vp, vp]
@@ -258,6 +264,33 @@ class Oracle:
         cic = w[self.norb:]
         self._L.orc_orth(self.h, _p(w[: self.norb]), _p(cic) if cic.size else None)
         return w
+
+    # the rational functions of cletdrb, in orc_etdrb_gen's order
+    ETD_RATS = ("phi1dt1", "phi1dt2", "phid1dt1", "phid1dt2", "phi2dt1", "phi2dt2", "phi3dt1")
+
+    def etdrb_gen(self, dtime: float, rosenbrock: bool = True, maxcyc: int = 20, thresh: float = 1e-10):
+        """cletdrb::gen for step dtime (the band factors use tmat as it is, so
+        call after set_wfn0 with frozen cores)."""
+        rc = self._L.orc_etdrb_gen(self.h, dtime, int(rosenbrock), maxcyc, thresh)
+        if rc:
+            raise RuntimeError(f"orc_etdrb_gen failed ({rc})")
+
+    def etdrb_step(self, lfields: np.ndarray, wfn: np.ndarray, apply_mask: bool = True) -> np.ndarray:
+        """cletdrb::prop4_orb then the mask; lfields: the six arrays of
+        td1c_amd.etd_fields.  Returns the new wave function."""
+        w = np.array(wfn, dtype=np.complex128, copy=True)
+        lf = np.ascontiguousarray(lfields, dtype=np.float64)
+        assert lf.size == 54
+        rc = self._L.orc_etdrb_step(self.h, _p(lf), _p(w), int(apply_mask))
+        if rc:
+            raise RuntimeError(f"orc_etdrb_step failed ({rc})")
+        return w
+
+    def etd_rat_scalar(self, name: str, lam: complex) -> complex:
+        """A cletdrb rational function at a scalar eigenvalue lam of h1."""
+        out = np.zeros(2)
+        self._L.orc_etd_rat_scalar(self.h, self.ETD_RATS.index(name), lam.real, lam.imag, _p(out))
+        return complex(out[0], out[1])
 
     def proj_to(self, other: "Oracle", wfn: np.ndarray) -> np.ndarray:
         """clbas::proj from this problem's grid to other's, with the CI copy of

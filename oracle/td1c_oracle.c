@@ -107,6 +107,14 @@ typedef struct orc {
   zc *ci_work;         /* nact^2 * lcic */
   /* RK work */
   zc *rk_w0, *rk_t, *rk_k;
+  /* cletdrb: the clh1rat rational functions (ETD_NRAT of them) */
+  int etd_ready, etd_rosen, etd_maxcyc;
+  double etd_dt1, etd_thresh;
+  double etd_dt[8];                       /* clh1rat dtime of each */
+  zc etd_cn[8][4], etd_cd0[8][3], etd_cd1[8][3];
+  zc etd_numer_limit[8], etd_denom_limit[8], etd_coeff_limit[8];
+  zc *etd_inv[8][3];                      /* (3n+1)*nr*L1 band LU of cd0 + cd1 T_l */
+  int *etd_piv[8][3];
 } orc;
 
 static void* xcalloc(size_t n, size_t sz) {
@@ -962,6 +970,11 @@ void orc_destroy(orc* o) {
                   o->dcic, o->str_a, o->str_b, o->det_a, o->det_b, o->det_of, o->ci_work,
                   o->rk_w0, o->rk_t, o->rk_k, o->mval, o->lval, o->cninv, o->cnpiv};
   for (size_t k = 0; k < sizeof(ptrs) / sizeof(ptrs[0]); ++k) free(ptrs[k]);
+  for (int r = 0; r < 8; ++r)
+    for (int i = 0; i < 3; ++i) {
+      free(o->etd_inv[r][i]);
+      free(o->etd_piv[r][i]);
+    }
   free(o);
 }
 
@@ -1979,6 +1992,310 @@ int orc_split2_step(orc* o, int order, double dtime, const double* lf_quarter, c
   if (rc) return rc;
   ncyc[1] = orc_h1itr_prop(o, lf_three_quarter, 0.5 * dtime, maxcyc, thresh, wfn);
   if (apply_mask) orc_mask(o, wfn);
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* cletdrb: exponential Rosenbrock RK4 with Pade phi functions          */
+/* (src/prop/cletdrb.cpp, clh1rat src/prop/clh1rat.cpp, h1rat_gen /      */
+/* h1rat_prod src/prop/h1rat_gen.F90, h1rat_prod.F90), with the defaults */
+/* of cletdrb::gen (:25-92): etd_rk_order 4, numerator / denominator     */
+/* degrees 2 / 3 for phi1, phi2, phi3, etd_rosenbrock as given,          */
+/* etd_time_of_h 0, etd_cisplit -1, split_type 1.                       */
+/* ------------------------------------------------------------------ */
+enum { ETD_PHI1_DT1, ETD_PHI1_DT2, ETD_PHID1_DT1, ETD_PHID1_DT2, ETD_PHI2_DT1, ETD_PHI2_DT2, ETD_PHI3_DT1, ETD_NRAT };
+
+/* Numerator coefficients, sign of the first denominator factor and the
+ * roots of the denominator of the [2/3] forms in gen_phi1 (:636-771),
+ * gen_phid1 (:773-886), gen_phi2 (:888-1035), gen_phi3 (:1037-1185). */
+static void etd_family(int fam, double* dfac, double* n, zc* roots) {
+  static const double r1[3][2] = {{4.644370709252173768, 0.0},
+                                  {3.677814645373918445, 3.508761919567442433},
+                                  {3.677814645373918445, -3.508761919567442433}};
+  static const double r2[3][2] = {{5.648485971016884655, 0.0},
+                                  {4.675757014491553676, 3.913489560603714335},
+                                  {4.675757014491553676, -3.913489560603714335}};
+  static const double r3[3][2] = {{6.651316808950019421, 0.0},
+                                  {5.674341595524996507, 4.279971984629763249},
+                                  {5.674341595524996507, -4.279971984629763249}};
+  const double(*r)[2] = fam <= 1 ? r1 : (fam == 2 ? r2 : r3);
+  *dfac = -1.0;
+  if (fam == 0) { n[0] = 120.0; n[1] = 0.0; n[2] = 2.0; }
+  else if (fam == 1) { n[0] = 60.0; n[1] = -10.0; n[2] = 1.0; }
+  else if (fam == 2) { n[0] = 420.0 / 4.0; n[1] = -40.0 / 4.0; n[2] = 5.0 / 4.0; }
+  else { n[0] = 1120.0 / 20.0; n[1] = -140.0 / 20.0; n[2] = 11.0 / 20.0; }
+  for (int i = 0; i < 3; ++i) roots[i] = r[i][0] + I * r[i][1];
+}
+
+/* One clh1rat: the scaling of the gen_phi* loops (numerator by dt1 tunit^k,
+ * denominator slopes by tunit dt) and clh1rat::gen (:15-47) with h1rat_gen's
+ * band LU of cd0 + cd1 T_l (T = tmat), skipped for |cd1| <= h1rat_thresh. */
+static int etd_rat_gen(orc* o, int ir, int fam, double dt) {
+  double dfac, nn[3];
+  zc roots[3];
+  etd_family(fam, &dfac, nn, roots);
+  zc tunit = o->icomp == 0 ? -1.0 : -I;
+  zc tfac = tunit * dt, ttmp = o->etd_dt1;
+  for (int k = 0; k <= 2; ++k) {
+    o->etd_cn[ir][k] = nn[k] * ttmp;
+    ttmp *= tunit;
+  }
+  o->etd_cd1[ir][0] = dfac * tfac;
+  o->etd_cd0[ir][0] = -dfac * roots[0];
+  for (int i = 1; i < 3; ++i) {
+    o->etd_cd1[ir][i] = tfac;
+    o->etd_cd0[ir][i] = -roots[i];
+  }
+  o->etd_dt[ir] = dt;
+  o->etd_numer_limit[ir] = o->etd_cn[ir][0];
+  zc den = 1.0;
+  for (int i = 0; i < 3; ++i) den *= o->etd_cd0[ir][i];
+  o->etd_denom_limit[ir] = den;
+  o->etd_coeff_limit[ir] = o->etd_numer_limit[ir] / den;
+  int n = o->ndvr, nr = o->nr, ld = 3 * n + 1;
+  size_t bw = 2 * (size_t)n + 1;
+  for (int i = 0; i < 3; ++i) {
+    free(o->etd_inv[ir][i]);
+    free(o->etd_piv[ir][i]);
+    o->etd_inv[ir][i] = NULL;
+    o->etd_piv[ir][i] = NULL;
+    if (!(cabs(o->etd_cd1[ir][i]) > o->etd_thresh)) continue;
+    o->etd_inv[ir][i] = xcalloc((size_t)ld * nr * o->L1, sizeof(zc));
+    o->etd_piv[ir][i] = xcalloc((size_t)nr * o->L1, sizeof(int));
+    for (int l = 0; l <= o->lmax1; ++l) {
+      zc* ab = o->etd_inv[ir][i] + (size_t)ld * nr * l;
+      for (int r = 0; r < nr; ++r) {
+        ab[(size_t)ld * r + 2 * n] = o->etd_cd0[ir][i];
+        int jll = r - n > 0 ? r - n : 0, jul = r + n < nr - 1 ? r + n : nr - 1;
+        for (int j = jll; j <= jul; ++j)
+          ab[(size_t)ld * r + 2 * n + j - r] += o->tmat[bw * ((size_t)l * nr + r) + n + j - r] * o->etd_cd1[ir][i];
+      }
+      if (band_lu(nr, n, ab, o->etd_piv[ir][i] + (size_t)nr * l)) return -1;
+    }
+  }
+  return 0;
+}
+
+/* cletdrb::gen: the rational functions prop4_orb uses, for step dtime; the
+ * band factors take tmat as it is now (with v2jfc after orc_set_wfn0). */
+int orc_etdrb_gen(orc* o, double dtime, int rosenbrock, int maxcyc, double thresh) {
+  if (o->isplit != 1) return -2;
+  o->etd_dt1 = dtime;
+  o->etd_rosen = rosenbrock;
+  o->etd_maxcyc = maxcyc;
+  o->etd_thresh = thresh;
+  const int fam[ETD_NRAT] = {0, 0, 1, 1, 2, 2, 3};
+  const double dts[ETD_NRAT] = {dtime, 0.5 * dtime, dtime, 0.5 * dtime, dtime, 0.5 * dtime, dtime};
+  for (int r = 0; r < ETD_NRAT; ++r)
+    if (etd_rat_gen(o, r, fam[r], dts[r])) return -1;
+  o->etd_ready = 1;
+  return 0;
+}
+
+/* The scalar value of rational function ir at an eigenvalue lam of h1:
+ * sum_k cn[k] (dt lam)^k / prod_i (cd0[i] + cd1[i] lam), the form
+ * h1rat_prod applies to orbitals (for tests of the tables). */
+void orc_etd_rat_scalar(const orc* o, int ir, double lam_re, double lam_im, double* out) {
+  zc lam = lam_re + I * lam_im, x = o->etd_dt[ir] * lam, num = 0.0, p = 1.0;
+  for (int k = 0; k <= 2; ++k) {
+    num += o->etd_cn[ir][k] * p;
+    p *= x;
+  }
+  zc den = 1.0;
+  for (int i = 0; i < 3; ++i) den *= o->etd_cd0[ir][i] + o->etd_cd1[ir][i] * lam;
+  zc v = num / den;
+  out[0] = creal(v);
+  out[1] = cimag(v);
+}
+
+/* hprod_h1tot with dofc = false (src/hprod/hprod_h1tot.F90:2-53):
+ * hw = (T + V(lf)) w on the dynamical orbitals, the rest of hw cleared (as
+ * clhprod::h1tot clears it, src/hprod/clhprod.cpp:110-118). */
+static void etd_h1tot(orc* o, const double* lf, const zc* w, zc* hw) {
+  memset(hw, 0, sizeof(zc) * ((size_t)o->nbas * o->nfun + o->lcic));
+  tprod(o, w, hw, o->nfcore, o->nfun, o->nr);
+  if (o->igauge == 0)
+    zprod(o, lf[2], w, hw, o->nfcore, o->nfun, o->nr);
+  else
+    pzprod(o, lf[2], w, hw, o->nfcore, o->nfun, o->nr);
+}
+
+/* hprod_v1ext with dofc = false (src/hprod/hprod_v1ext.F90:2-36):
+ * hw += V(zfac lf) w on the dynamical orbitals. */
+static void etd_v1ext(orc* o, zc zfac, const double* lf, const zc* w, zc* hw) {
+  zc zf = lf[2] * zfac;
+  if (o->igauge == 0)
+    zprod(o, zf, w, hw, o->nfcore, o->nfun, o->nr);
+  else
+    pzprod(o, zf, w, hw, o->nfcore, o->nfun, o->nr);
+}
+
+/* h1rat_h0inv_v2 (h1rat_prod.F90): x <- (cd0 + cd1 T_l)^-1 x for every
+ * dynamical orbital and l. */
+static void etd_h0inv(orc* o, int ir, int i, zc* x) {
+  int nr = o->nr, ld = 3 * o->ndvr + 1;
+  for (int ifun = o->nfcore; ifun < o->nfun; ++ifun)
+    for (int l = 0; l <= o->lmax1; ++l)
+      band_lu_solve(nr, o->ndvr, o->etd_inv[ir][i] + (size_t)ld * nr * l, o->etd_piv[ir][i] + (size_t)nr * l,
+                    x + OIDX(0, l, ifun));
+}
+
+/* h1rat_prod (h1rat_prod.F90:2-121) on the orbitals of w, field lf. */
+static void etd_h1rat_prod(orc* o, int ir, int do_numer, int do_denom, const double* lf, zc* w) {
+  size_t nb = o->nbas, size1 = nb * o->nfun, d0 = nb * o->nfcore;
+  zc* orb = xcalloc(size1, sizeof(zc));
+  zc* torb = xcalloc(size1, sizeof(zc));
+  zc* h1orb = xcalloc(size1 + o->lcic, sizeof(zc));
+  double tfac = o->etd_dt[ir];
+  memcpy(orb + d0, w + d0, sizeof(zc) * (size1 - d0));
+  if (do_numer) {
+    memcpy(torb + d0, w + d0, sizeof(zc) * (size1 - d0));
+    for (size_t k = d0; k < size1; ++k) orb[k] *= o->etd_cn[ir][0];
+    for (int idim = 1; idim <= 2; ++idim) {
+      etd_h1tot(o, lf, torb, h1orb);
+      for (size_t k = d0; k < size1; ++k) torb[k] = tfac * h1orb[k];
+      for (size_t k = d0; k < size1; ++k) orb[k] += o->etd_cn[ir][idim] * torb[k];
+    }
+  }
+  if (do_denom) {
+    for (int idim = 0; idim < 3; ++idim) {
+      if (cabs(o->etd_cd1[ir][idim]) < o->etd_thresh) continue;
+      etd_h0inv(o, ir, idim, orb);
+      if (o->isplit == 1 && o->icomp == 1) {
+        memcpy(torb + d0, orb + d0, sizeof(zc) * (size1 - d0));
+        for (int icyc = 1; icyc <= o->etd_maxcyc; ++icyc) {
+          double rms = 0.0;
+          for (int ifun = o->nfcore; ifun < o->nfun; ++ifun) {
+            double t = 0.0;
+            const zc* x = orb + (size_t)ifun * nb;
+            for (size_t k = 0; k < nb; ++k) t += creal(x[k]) * creal(x[k]) + cimag(x[k]) * cimag(x[k]);
+            rms += t;
+          }
+          rms = sqrt(rms / o->nfun);
+          if (rms < o->etd_thresh) break;
+          memset(h1orb, 0, sizeof(zc) * size1);
+          etd_v1ext(o, -o->etd_cd1[ir][idim], lf, orb, h1orb);
+          memcpy(orb + d0, h1orb + d0, sizeof(zc) * (size1 - d0));
+          etd_h0inv(o, ir, idim, orb);
+          for (size_t k = d0; k < size1; ++k) torb[k] += orb[k];
+        }
+        memcpy(orb + d0, torb + d0, sizeof(zc) * (size1 - d0));
+      }
+    }
+  }
+  memcpy(w + d0, orb + d0, sizeof(zc) * (size1 - d0));
+  free(orb);
+  free(torb);
+  free(h1orb);
+}
+
+/* clh1rat::prod_numer (clh1rat.cpp:196-216) with h1rat_prod_numer
+ * (h1rat_prod.F90:123-181): wout.ci += zfac numer_limit win.ci,
+ * wout.orb(dyn) += zfac N(dt h1) win. */
+static void etd_prod_numer(orc* o, int ir, zc zfac, const double* lf, const zc* win, zc* wout) {
+  size_t nb = o->nbas, size1 = nb * o->nfun, d0 = nb * o->nfcore;
+  for (int k = 0; k < o->lcic; ++k) wout[size1 + k] += zfac * o->etd_numer_limit[ir] * win[size1 + k];
+  zc* orb = xcalloc(size1, sizeof(zc));
+  zc* torb = xcalloc(size1, sizeof(zc));
+  zc* h1orb = xcalloc(size1 + o->lcic, sizeof(zc));
+  double tfac = o->etd_dt[ir];
+  for (size_t k = d0; k < size1; ++k) {
+    orb[k] = win[k];
+    torb[k] = win[k];
+    orb[k] *= o->etd_cn[ir][0];
+  }
+  for (int idim = 1; idim <= 2; ++idim) {
+    etd_h1tot(o, lf, torb, h1orb);
+    for (size_t k = d0; k < size1; ++k) torb[k] = tfac * h1orb[k];
+    for (size_t k = d0; k < size1; ++k) orb[k] += o->etd_cn[ir][idim] * torb[k];
+  }
+  for (size_t k = d0; k < size1; ++k) wout[k] += zfac * orb[k];
+  free(orb);
+  free(torb);
+  free(h1orb);
+}
+
+/* clh1rat::prod_denom (:360-384) / prod2 (:108-131): wout.ci += c win.ci
+ * (c = zfac / denom_limit, or zfac coeff_limit), win.orb <- D(h) win or
+ * R(h) win in place, wout.orb += zfac win.orb. */
+static void etd_prod(orc* o, int ir, int numer, zc zfac, const double* lf, zc* win, zc* wout) {
+  size_t size1 = (size_t)o->nbas * o->nfun;
+  zc c = numer ? zfac * o->etd_coeff_limit[ir] : zfac / o->etd_denom_limit[ir];
+  for (int k = 0; k < o->lcic; ++k) wout[size1 + k] += c * win[size1 + k];
+  etd_h1rat_prod(o, ir, numer, 1, lf, win);
+  for (size_t k = 0; k < size1; ++k) wout[k] += zfac * win[k];
+}
+
+/* cletdrb::prop4_orb (src/prop/cletdrb.cpp:321-434), then the mask of
+ * tdse_prop12 (src/main/tdse.cpp:268-309).  lf holds six field arrays of 9:
+ * lfield(t), lfield(t + dt/2), lfield(t + dt), d lfield/dt at t
+ * (clfield::get_der), lfield(t + dt/2) - lfield(t), lfield(t + dt) -
+ * lfield(t).  wfn: orbitals then CI. */
+int orc_etdrb_step(orc* o, const double* lf, zc* wfn, int apply_mask) {
+  if (!o->etd_ready) return -1;
+  size_t size1 = (size_t)o->nbas * o->nfun, sz = size1 + o->lcic;
+  const double *lf0 = lf, *lf1 = lf + 9, *lf3 = lf + 18, *dlf0 = lf + 27, *dv1 = lf + 36, *dv3 = lf + 45;
+  zc tunit = o->icomp == 0 ? -1.0 : -I;
+  double dt1 = o->etd_dt1;
+  zc* buf = xcalloc(sz * 10, sizeof(zc));
+  zc *w0 = buf, *w1 = buf + sz, *w2 = buf + 2 * sz, *w3 = buf + 3 * sz, *hw0 = buf + 4 * sz, *hw1 = buf + 5 * sz,
+     *hw2 = buf + 6 * sz, *hw3 = buf + 7 * sz, *tw = buf + 8 * sz, *dw = buf + 9 * sz;
+  zc c1 = 0.5, c3 = 1.0;
+  memcpy(w0, wfn, sizeof(zc) * sz);
+  /* stage 0 */
+  etd_h1tot(o, lf0, w0, tw);
+  orc_htot(o, 1.0, lf0, w0, w0 + size1, hw0, hw0 + size1);
+  for (size_t k = 0; k < sz; ++k) tw[k] = tunit * tw[k] + hw0[k];
+  etd_prod_numer(o, ETD_PHI1_DT2, c1, lf0, tw, w1);
+  etd_prod_numer(o, ETD_PHI1_DT1, c3, lf0, tw, w3);
+  if (o->etd_rosen) {
+    memset(dw, 0, sizeof(zc) * sz);
+    etd_v1ext(o, 1.0, dlf0, w0, dw);
+    for (size_t k = 0; k < sz; ++k) dw[k] *= tunit * dt1;
+    etd_prod_numer(o, ETD_PHID1_DT2, c1 * c1, lf0, dw, w1);
+    etd_prod_numer(o, ETD_PHID1_DT1, c3 * c3, lf0, dw, w3);
+  }
+  /* stage 1 */
+  memcpy(tw, w1, sizeof(zc) * sz);
+  memcpy(w1, w0, sizeof(zc) * sz);
+  etd_prod(o, ETD_PHI1_DT2, 0, 1.0, lf0, tw, w1);
+  orc_htot(o, 1.0, lf1, w1, w1 + size1, hw1, hw1 + size1);
+  for (size_t k = 0; k < sz; ++k) hw1[k] -= hw0[k];
+  if (o->etd_rosen) {
+    etd_v1ext(o, tunit, dv1, w1, hw1);
+    for (size_t k = 0; k < sz; ++k) hw1[k] += -0.5 * dw[k];
+  }
+  /* stage 2 */
+  memcpy(w2, w1, sizeof(zc) * sz);
+  memcpy(tw, hw1, sizeof(zc) * sz);
+  etd_prod(o, ETD_PHI2_DT2, 1, 1.0, lf0, tw, w2);
+  orc_htot(o, 1.0, lf1, w2, w2 + size1, hw2, hw2 + size1);
+  for (size_t k = 0; k < sz; ++k) hw2[k] -= hw0[k];
+  if (o->etd_rosen) {
+    etd_v1ext(o, tunit, dv1, w2, hw2);
+    for (size_t k = 0; k < sz; ++k) hw2[k] += -0.5 * dw[k];
+  }
+  /* stage 3 */
+  memcpy(tw, w3, sizeof(zc) * sz);
+  memcpy(w3, w0, sizeof(zc) * sz);
+  etd_prod(o, ETD_PHI1_DT1, 0, 1.0, lf0, tw, w3);
+  memcpy(tw, hw2, sizeof(zc) * sz);
+  etd_prod(o, ETD_PHI2_DT1, 1, 2.0, lf0, tw, w3);
+  orc_htot(o, 1.0, lf3, w3, w3 + size1, hw3, hw3 + size1);
+  for (size_t k = 0; k < sz; ++k) hw3[k] -= hw0[k];
+  if (o->etd_rosen) {
+    etd_v1ext(o, tunit, dv3, w3, hw3);
+    for (size_t k = 0; k < sz; ++k) hw3[k] += -1.0 * dw[k];
+  }
+  /* summation */
+  memcpy(wfn, w3, sizeof(zc) * sz);
+  for (size_t k = 0; k < sz; ++k) tw[k] = 2.0 * hw1[k] - hw3[k];
+  etd_prod(o, ETD_PHI2_DT1, 1, 1.0, lf0, tw, wfn);
+  for (size_t k = 0; k < sz; ++k) tw[k] = hw1[k] + hw2[k];
+  for (size_t k = 0; k < sz; ++k) tw[k] = tw[k] - hw3[k];
+  etd_prod(o, ETD_PHI3_DT1, 1, -4.0, lf0, tw, wfn);
+  if (apply_mask) orc_mask(o, wfn);
+  free(buf);
   return 0;
 }
 

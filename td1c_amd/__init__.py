@@ -62,6 +62,13 @@ def load_library():
     L.td1c_propagate.argtypes = [vp, ctypes.POINTER(TD1CField), ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                  ctypes.c_int, vp, ctypes.c_int, vp]
     L.td1c_propagate.restype = ctypes.c_int
+    L.td1c_etdrb_gen.argtypes = [vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_double]
+    L.td1c_etdrb_gen.restype = ctypes.c_int
+    L.td1c_etdrb_prop.argtypes = [vp, vp, vp, ctypes.c_int, vp]
+    L.td1c_etdrb_prop.restype = ctypes.c_int
+    L.td1c_etdrb_propagate.argtypes = [vp, ctypes.POINTER(TD1CField), ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                       vp, ctypes.c_int, vp]
+    L.td1c_etdrb_propagate.restype = ctypes.c_int
     L.td1c_set_graph.argtypes = [vp, ctypes.c_int]
     L.td1c_set_graph.restype = ctypes.c_int
     L.td1c_set_transform_path.argtypes = [vp, ctypes.c_int]
@@ -100,6 +107,8 @@ def load_library():
     L.td1c_mask.restype = ctypes.c_int
     L.td1c_field_value.argtypes = [ctypes.POINTER(TD1CField), ctypes.c_double, vp]
     L.td1c_field_value.restype = ctypes.c_int
+    L.td1c_field_der.argtypes = [ctypes.POINTER(TD1CField), ctypes.c_double, vp]
+    L.td1c_field_der.restype = ctypes.c_int
     L.td1c_field_period.argtypes = [ctypes.POINTER(TD1CField)]
     L.td1c_field_period.restype = ctypes.c_double
     L.td1c_bind.argtypes = [vp]
@@ -162,6 +171,22 @@ def field_value(field: Field, t: float) -> np.ndarray:
     c = field.to_c()
     _check(load_library().td1c_field_value(ctypes.byref(c), float(t), _ptr(out)), "td1c_field_value")
     return out
+
+
+def field_der(field: Field, t: float) -> np.ndarray:
+    """clfield::get_der: d lfield[0..2] / dt (the rest zero)."""
+    out = np.zeros(9)
+    c = field.to_c()
+    _check(load_library().td1c_field_der(ctypes.byref(c), float(t), _ptr(out)), "td1c_field_der")
+    return out
+
+
+def etd_fields(field: Field, t: float, dt: float) -> np.ndarray:
+    """The six field arrays of one cletdrb step from t (etd_time_of_h 0):
+    lfield(t), lfield(t + dt/2), lfield(t + dt), d lfield/dt (t),
+    lfield(t + dt/2) - lfield(t), lfield(t + dt) - lfield(t)."""
+    f0, f1, f3 = field_value(field, t), field_value(field, t + 0.5 * dt), field_value(field, t + dt)
+    return np.concatenate([f0, f1, f3, field_der(field, t), f1 - f0, f3 - f0])
 
 
 def field_period(field: Field) -> float:
@@ -558,6 +583,42 @@ class CRK:
         _check(load_library().td1c_propagate(hprod.h, ctypes.byref(cf), self.rk_order, float(time), float(dtime),
                                              int(nstep), ctypes.c_void_p(wfn.data_ptr()), int(apply_mask),
                                              hprod._stream(stream)), "td1c_propagate")
+        return wfn
+
+
+class ETDRB:
+    """Exponential Rosenbrock RK4 with Pade phi functions (cletdrb, the
+    reference's default prop_type; td1c_etdrb_*).  Needs split_type = 1."""
+
+    def __init__(self, hprod: "HProd", dtime: float, rosenbrock: bool = True, maxcyc: int = 20,
+                 thresh: float = 1e-10):
+        self.hprod = hprod
+        self.dtime = float(dtime)
+        _check(load_library().td1c_etdrb_gen(hprod.h, self.dtime, int(rosenbrock), int(maxcyc), float(thresh)),
+               "td1c_etdrb_gen")
+
+    def prop(self, wfn, time: float = 0.0, field: Optional[Field] = None, lfields: Optional[np.ndarray] = None,
+             apply_mask: bool = True, stream=None):
+        """One step in place with host field arrays (etd_fields of the field
+        at `time`, or lfields given), then the mask."""
+        if lfields is None:
+            if field is None:
+                raise ValueError("need a field or explicit lfields")
+            lfields = etd_fields(field, time, self.dtime)
+        lf = np.ascontiguousarray(lfields, dtype=np.float64)
+        if lf.size != 54:
+            raise ValueError("lfields must hold 6 x 9 values")
+        _check(load_library().td1c_etdrb_prop(self.hprod.h, _ptr(lf), ctypes.c_void_p(wfn.data_ptr()),
+                                              int(apply_mask), self.hprod._stream(stream)), "td1c_etdrb_prop")
+        return wfn
+
+    def propagate(self, wfn, time: float, nstep: int, field: Field, apply_mask: bool = True, stream=None):
+        """nstep steps of tdse_prop12 from `time`, the fields evaluated on the
+        device."""
+        cf = field.to_c()
+        _check(load_library().td1c_etdrb_propagate(self.hprod.h, ctypes.byref(cf), float(time), self.dtime,
+                                                   int(nstep), ctypes.c_void_p(wfn.data_ptr()), int(apply_mask),
+                                                   self.hprod._stream(stream)), "td1c_etdrb_propagate")
         return wfn
 
 

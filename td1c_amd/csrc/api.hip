@@ -180,6 +180,29 @@ struct td1c_ctx {
   // optional per-kernel timing with HIP events on the launch stream
   bool timing = false;
   std::vector<hipEvent_t> ev_pool;
+  // cletdrb (td1c_etdrb_gen): the seven rational functions of h1 of
+  // prop4_orb, the element Schur tables of their distinct denominator
+  // factors cd0 + cd1 T (as 1 + (cd1/cd0) T), the step's states and fields
+  struct EtdTab {
+    DevBuf<double2> G, Wl, Wr, pa, pg, binv;
+    int K = 0;
+    double fre = 0.0, fim = 0.0;
+  };
+  struct EtdRat {
+    double dt = 0.0;
+    double2 cn[3], cd0[3], cd1[3];
+    double2 numer_limit, denom_limit, coeff_limit;
+    int tab[3];
+  };
+  std::vector<std::unique_ptr<EtdTab>> etd_tabs;
+  EtdRat etd[7];
+  bool etd_ready = false;
+  int etd_rosen = 1, etd_maxcyc = 20;
+  double etd_dt = 0.0, etd_thresh = 1.0e-10;
+  DevBuf<double2> etd_buf;  // ten states, then orb (orbital block) and three dynamical blocks
+  DevBuf<double> etd_lf;    // [6][9] field arrays of the step
+  DevBuf<int> etd_zero;     // an always-clear stop flag
+
   struct Rec { const char* name; int e0, e1; };
   std::vector<Rec> recs;
   int ev_next = 0;
@@ -216,6 +239,13 @@ struct td1c_ctx {
     for (auto& e : fj)
       if (e) (void)hipEventDestroy(e);
     if (clock.p) (void)hipFree(clock.p);
+    for (auto& t : etd_tabs) {
+      DevBuf<double2>* e[] = {&t->G, &t->Wl, &t->Wr, &t->pa, &t->pg, &t->binv};
+      for (auto* b : e) b->release();
+    }
+    etd_buf.release();
+    etd_lf.release();
+    etd_zero.release();
   }
 };
 
@@ -2231,6 +2261,371 @@ int td1c_propagate(td1c_ctx* c, const td1c_field* field, int rk_order, double t0
 }
 
 // ---------------------------------------------------------------------------
+// cletdrb: exponential Rosenbrock RK4 with Pade forms of the phi functions
+// (cletdrb::prop4_orb, src/prop/cletdrb.cpp:321-434; clh1rat,
+// src/prop/clh1rat.cpp; h1rat_gen / h1rat_prod, src/prop/h1rat_gen.F90,
+// src/prop/h1rat_prod.F90), with the defaults of cletdrb::gen (:25-92):
+// etd_rk_order 4, the [2/3] forms of phi1, phi2 and phi3, etd_time_of_h 0,
+// etd_cisplit -1, split_type 1.  A step is four split hprods and seven
+// rational functions of h1: numerators by two applications of h1, each
+// denominator factor cd0 + cd1 (T + V) by the element Schur solve of
+// 1 + (cd1/cd0) T with the Neumann series in V of the half steps.
+// ---------------------------------------------------------------------------
+namespace {
+
+enum { kPhi1Dt1, kPhi1Dt2, kPhid1Dt1, kPhid1Dt2, kPhi2Dt1, kPhi2Dt2, kPhi3Dt1, kEtdRats };
+
+// The [2/3] forms: numerator coefficients, the sign of the first denominator
+// factor and the denominator roots (gen_phi1 :636-771, gen_phid1 :773-886,
+// gen_phi2 :888-1035, gen_phi3 :1037-1185).  fam 0 phi1, 1 phid1 (phi2
+// over phi1's denominator), 2 phi2, 3 phi3.
+void etd_form(int fam, double& dfac, double num[3], std::complex<double> root[3]) {
+  static const double kRoots[3][3][2] = {
+      {{4.644370709252173768, 0.0}, {3.677814645373918445, 3.508761919567442433},
+       {3.677814645373918445, -3.508761919567442433}},
+      {{5.648485971016884655, 0.0}, {4.675757014491553676, 3.913489560603714335},
+       {4.675757014491553676, -3.913489560603714335}},
+      {{6.651316808950019421, 0.0}, {5.674341595524996507, 4.279971984629763249},
+       {5.674341595524996507, -4.279971984629763249}}};
+  static const double kNum[4][3] = {{120.0, 0.0, 2.0},
+                                    {60.0, -10.0, 1.0},
+                                    {420.0 / 4.0, -40.0 / 4.0, 5.0 / 4.0},
+                                    {1120.0 / 20.0, -140.0 / 20.0, 11.0 / 20.0}};
+  const int rs = fam <= 1 ? 0 : fam - 1;
+  dfac = -1.0;
+  for (int k = 0; k < 3; ++k) {
+    num[k] = kNum[fam][k];
+    root[k] = std::complex<double>(kRoots[rs][k][0], kRoots[rs][k][1]);
+  }
+}
+
+double2 d2(std::complex<double> z) { return make_double2(z.real(), z.imag()); }
+
+// the Crank-Nicolson-form work buffers shared with the half steps
+int cn_buffers(td1c_ctx* c) {
+  Setup& S = c->S;
+  const int nd = S.nfun - S.nfcore;
+  const size_t nrow = (size_t)std::max(nd, 1) * S.L1 * S.nr;
+  if (!c->cn_x.p) HIP_TRY(c->cn_x.alloc(nrow));
+  if (!c->cn_y.p) HIP_TRY(c->cn_y.alloc(nrow));
+  if (!c->cn_flag.p) {
+    HIP_TRY(c->cn_flag.alloc(2));
+    HIP_TRY(hipMemset(c->cn_flag.p, 0, 2 * sizeof(int)));
+  }
+  c->cn_npart1 = std::max(nd, 1) * S.L1;
+  CnArgs probe{};
+  probe.nfe = S.nfe; probe.ni = S.n - 1; probe.L1 = S.L1; probe.nfun = S.nfun; probe.nfcore = S.nfcore;
+  const size_t npart = (size_t)c->cn_npart1 + cn_interior_blocks(probe);
+  if (c->cn_part.n < npart) {
+    c->cn_part.release();
+    HIP_TRY(c->cn_part.alloc(npart));
+  }
+  return 0;
+}
+
+int etd_generate(td1c_ctx* c, double dt, int rosen, int maxcyc, double thresh) {
+  Setup& S = c->S;
+  if (S.isplit != 1) return fail(-1, "etdrb needs split_type = 1 (the split hprod)");
+  if (S.lconst != 0) return fail(-1, "etdrb with lconst is not provided");
+  if (S.nfcore > 0 && !c->wfn0_set) return fail(-4, "frozen core needs td1c_set_wfn0 first");
+  if (!(dt != 0.0) || !std::isfinite(dt)) return fail(-1, "etdrb needs a finite nonzero dtime");
+  if (maxcyc < 0 || !(thresh >= 0.0)) return fail(-1, "bad maxcyc or thresh");
+  if (c->etd_buf.p) HIP_TRY(hipDeviceSynchronize());  // earlier steps may still read the old tables
+  const std::complex<double> tunit = S.icomp == 0 ? std::complex<double>(-1.0, 0.0) : std::complex<double>(0.0, -1.0);
+  const int fam[kEtdRats] = {0, 0, 1, 1, 2, 2, 3};
+  const double dts[kEtdRats] = {dt, 0.5 * dt, dt, 0.5 * dt, dt, 0.5 * dt, dt};
+  for (auto& t : c->etd_tabs) {
+    DevBuf<double2>* e[] = {&t->G, &t->Wl, &t->Wr, &t->pa, &t->pg, &t->binv};
+    for (auto* b : e) b->release();
+  }
+  c->etd_tabs.clear();
+  c->etd_ready = false;
+  auto up = [](DevBuf<double2>& b, const std::vector<double>& v) -> hipError_t {
+    hipError_t e = b.alloc(std::max<size_t>(v.size() / 2, 1));
+    if (e != hipSuccess || v.empty()) return e;
+    return hipMemcpy(b.p, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice);
+  };
+  for (int r = 0; r < kEtdRats; ++r) {
+    td1c_ctx::EtdRat& R = c->etd[r];
+    double dfac, num[3];
+    std::complex<double> root[3];
+    etd_form(fam[r], dfac, num, root);
+    // the scaling of the gen_phi* loops: numerator by dt1 tunit^k, the
+    // denominator slopes by tunit dt
+    std::complex<double> ttmp = dt, cd0[3], cd1[3];
+    for (int k = 0; k < 3; ++k) {
+      R.cn[k] = d2(num[k] * ttmp);
+      ttmp *= tunit;
+    }
+    const std::complex<double> tfac = tunit * dts[r];
+    cd1[0] = dfac * tfac;
+    cd0[0] = -dfac * root[0];
+    for (int i = 1; i < 3; ++i) {
+      cd1[i] = tfac;
+      cd0[i] = -root[i];
+    }
+    R.dt = dts[r];
+    std::complex<double> den = 1.0;
+    for (int i = 0; i < 3; ++i) {
+      R.cd0[i] = d2(cd0[i]);
+      R.cd1[i] = d2(cd1[i]);
+      den *= cd0[i];
+    }
+    const std::complex<double> nl = num[0] * std::complex<double>(dt);
+    R.numer_limit = d2(nl);
+    R.denom_limit = d2(den);
+    R.coeff_limit = d2(nl / den);
+    for (int i = 0; i < 3; ++i) {
+      R.tab[i] = -1;
+      if (!(std::abs(cd1[i]) > thresh)) continue;  // h1rat_gen skips the factor
+      const std::complex<double> f = cd1[i] / cd0[i];
+      for (size_t t = 0; t < c->etd_tabs.size(); ++t)
+        if (c->etd_tabs[t]->fre == f.real() && c->etd_tabs[t]->fim == f.imag()) R.tab[i] = (int)t;
+      if (R.tab[i] >= 0) continue;
+      CnTables T;
+      std::string err;
+      if (!build_cn(S, c->v2jfc_h.empty() ? nullptr : c->v2jfc_h.data(), f.real(), f.imag(), T, err))
+        return fail(-5, err);
+      auto tab = std::make_unique<td1c_ctx::EtdTab>();
+      HIP_TRY(up(tab->G, T.G));
+      HIP_TRY(up(tab->Wl, T.Wl));
+      HIP_TRY(up(tab->Wr, T.Wr));
+      HIP_TRY(up(tab->pa, T.pa));
+      HIP_TRY(up(tab->pg, T.pg));
+      HIP_TRY(up(tab->binv, T.binv));
+      tab->K = T.K;
+      tab->fre = f.real();
+      tab->fim = f.imag();
+      R.tab[i] = (int)c->etd_tabs.size();
+      c->etd_tabs.push_back(std::move(tab));
+    }
+  }
+  if (int rc = cn_buffers(c)) return rc;
+  const size_t so = (size_t)c->nbas * S.nfun, sz = so + S.lcic;
+  const size_t ndn = (size_t)c->nbas * std::max(S.nfun - S.nfcore, 1);
+  const size_t need = 10 * sz + so + 3 * ndn;
+  if (c->etd_buf.n < need) {
+    c->etd_buf.release();
+    HIP_TRY(c->etd_buf.alloc(need));
+  }
+  if (!c->etd_lf.p) HIP_TRY(c->etd_lf.alloc(54));
+  if (!c->etd_zero.p) {
+    HIP_TRY(c->etd_zero.alloc(1));
+    HIP_TRY(hipMemset(c->etd_zero.p, 0, sizeof(int)));
+  }
+  if (!c->clock.p) HIP_TRY(c->clock.alloc(1));
+  c->etd_dt = dt;
+  c->etd_rosen = rosen ? 1 : 0;
+  c->etd_maxcyc = maxcyc;
+  c->etd_thresh = thresh;
+  c->etd_ready = true;
+  return 0;
+}
+
+// One cletdrb step (prop4_orb) on wfn with the field arrays d_lf[6][9] on
+// the device, then the mask of tdse_prop12 (src/main/tdse.cpp:268-309).
+int enqueue_etdrb_step(td1c_ctx* c, const double* d_lf, double2* wfn, int apply_mask, hipStream_t s) {
+  Setup& S = c->S;
+  const size_t nb = c->nbas, so = nb * S.nfun, sz = so + S.lcic, dof = nb * S.nfcore;
+  const int nd = S.nfun - S.nfcore;
+  const size_t ndn = nb * nd;
+  double2* base = c->etd_buf.p;
+  double2 *W0 = base, *W1 = base + sz, *W2 = base + 2 * sz, *W3 = base + 3 * sz, *hW0 = base + 4 * sz,
+          *hW1 = base + 5 * sz, *hW2 = base + 6 * sz, *hW3 = base + 7 * sz, *tW = base + 8 * sz, *dW = base + 9 * sz;
+  double2* orb = base + 10 * sz;  // an orbital block; the rational functions work on its dynamical rows
+  double2* tb[2] = {orb + so, orb + so + ndn};
+  double2* vs = orb + so + 2 * ndn;
+  const double *lf0 = d_lf, *lf1 = d_lf + 9, *lf3 = d_lf + 18, *dlf0 = d_lf + 27, *dv1 = d_lf + 36, *dv3 = d_lf + 45;
+  const double2 one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
+  const double2 tunit = S.icomp == 0 ? make_double2(-1.0, 0.0) : make_double2(0.0, -1.0);
+  const bool ob_el = S.n == 20;
+  auto cpy = [&](double2* z, const double2* x, size_t n) { launch_zaxpby(n, one, x, zero, nullptr, z, s); };
+  auto clr = [&](double2* z, size_t n) { launch_zaxpby(n, zero, nullptr, zero, nullptr, z, s); };
+  // y += a x
+  auto axpy = [&](double2* y, double2 a, const double2* x, size_t n) { launch_zaxpby(n, one, y, a, x, y, s); };
+  auto ob_args = [&](const double* lf) {
+    OneBodyElArgs ob = onebody_el_args(c, lf);
+    ob.nfun = nd;
+    ob.nfcore = 0;
+    ob.fshift = S.nfcore;
+    ob.isplit = S.isplit;
+    return ob;
+  };
+  // out = cv (T + V(lf)) in, or cv V(lf) in (vonly), over the dynamical rows
+  auto h1 = [&](const double2* in, double2* out, double2 cv, const double* lf, bool vonly) {
+    CnArgs a = cn_args(c, lf, wfn);
+    a.x = const_cast<double2*>(in);
+    a.y = out;
+    a.zfac1 = cv;
+    a.flag = c->etd_zero.p;
+    const OneBodyElArgs ob = ob_args(lf);
+    launch_cn_stage(a, ob_el ? &ob : nullptr, c->ob_shape, c->ob_lc, vonly ? 4 : 3, s);
+  };
+  // the numerator sum_k cn[k] (dt h1)^k of R on in (dynamical rows) into out
+  auto numer = [&](const td1c_ctx::EtdRat& R, const double2* in, double2* out) {
+    launch_zaxpby(ndn, R.cn[0], in, zero, nullptr, out, s);
+    const double2* src = in;
+    for (int k = 1; k <= 2; ++k) {
+      double2* t = tb[k & 1];
+      h1(src, t, make_double2(R.dt, 0.0), lf0, false);
+      axpy(out, R.cn[k], t, ndn);
+      src = t;
+    }
+  };
+  // the denominator factors of R on the dynamical rows of orb, in place
+  auto denom = [&](const td1c_ctx::EtdRat& R) {
+    for (int i = 0; i < 3; ++i) {
+      if (R.tab[i] < 0) continue;
+      const td1c_ctx::EtdTab& T = *c->etd_tabs[R.tab[i]];
+      CnArgs a = cn_args(c, lf0, orb);
+      a.G = T.G.p; a.Wl = T.Wl.p; a.Wr = T.Wr.p; a.pa = T.pa.p; a.pg = T.pg.p; a.binv = T.binv.p; a.K = T.K;
+      a.gstride = S.nfe;
+      const std::complex<double> c0(R.cd0[i].x, R.cd0[i].y), c1(R.cd1[i].x, R.cd1[i].y);
+      a.zfac0 = d2(1.0 / c0);
+      a.zfac1 = d2(-(c1 / c0));
+      a.thresh = c->etd_thresh;
+      OneBodyElArgs ob = ob_args(lf0);
+      ob.cv = a.zfac1;
+      ob.stop = a.flag;
+      const OneBodyElArgs* obp = ob_el ? &ob : nullptr;
+      launch_cn_stage(a, obp, c->ob_shape, c->ob_lc, 2, s);
+      if (S.isplit == 1 && S.icomp == 1)
+        for (int k = 0; k < c->etd_maxcyc; ++k) launch_cn_stage(a, obp, c->ob_shape, c->ob_lc, 1, s);
+    }
+  };
+  auto cmulh = [](double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); };
+  // clh1rat::prod_numer: wout.ci += zfac numer_limit win.ci, wout.orb += zfac N win.orb
+  auto prod_numer = [&](const td1c_ctx::EtdRat& R, double2 zfac, const double2* win, double2* wout) {
+    axpy(wout + so, cmulh(zfac, R.numer_limit), win + so, S.lcic);
+    numer(R, win + dof, orb + dof);
+    axpy(wout + dof, zfac, orb + dof, ndn);
+  };
+  // clh1rat::prod_denom (with_numer false) / prod2: wout.ci += c win.ci,
+  // wout.orb += zfac R(h1) win.orb
+  auto prod = [&](const td1c_ctx::EtdRat& R, bool with_numer, double2 zfac, const double2* win, double2* wout) {
+    const std::complex<double> z(zfac.x, zfac.y);
+    const std::complex<double> cc = with_numer ? z * std::complex<double>(R.coeff_limit.x, R.coeff_limit.y)
+                                               : z / std::complex<double>(R.denom_limit.x, R.denom_limit.y);
+    axpy(wout + so, d2(cc), win + so, S.lcic);
+    if (with_numer)
+      numer(R, win + dof, orb + dof);
+    else
+      cpy(orb + dof, win + dof, ndn);
+    denom(R);
+    axpy(wout + dof, zfac, orb + dof, ndn);
+  };
+  auto htot = [&](const double* lf, const double2* in, double2* out) -> int {
+    CiStage cs;
+    cs.kout = out + so;
+    if (int rc = enqueue_htotx(c, 1.0, lf, in, in + so, cs, s)) return rc;
+    DtorbArgs d = dtorb_args(c, 1.0);
+    d.kout = out;
+    { TIMED("dtorb"); launch_dtorb(d, s); }
+    HIP_TRY(join_island(c, s));
+    return 0;
+  };
+  const auto& R = c->etd;
+  TIMED("etdrb");
+  // stage 0: tW = -i h1 u0 + W0; u1, u3 from the phi1 numerators (and the
+  // Rosenbrock term -i dt dV/dt u0)
+  cpy(W0, wfn, sz);
+  clr(W1, sz);
+  clr(W3, sz);
+  clr(tW, sz);
+  h1(W0 + dof, tW + dof, one, lf0, false);
+  if (int rc = htot(lf0, W0, hW0)) return rc;
+  launch_zaxpby(sz, tunit, tW, one, hW0, tW, s);
+  prod_numer(R[kPhi1Dt2], make_double2(0.5, 0.0), tW, W1);
+  prod_numer(R[kPhi1Dt1], one, tW, W3);
+  if (c->etd_rosen) {
+    clr(dW, sz);
+    h1(W0 + dof, dW + dof, one, dlf0, true);
+    launch_zaxpby(sz, cmulh(tunit, make_double2(c->etd_dt, 0.0)), dW, zero, nullptr, dW, s);
+    prod_numer(R[kPhid1Dt2], make_double2(0.25, 0.0), dW, W1);
+    prod_numer(R[kPhid1Dt1], one, dW, W3);
+  }
+  // stage 1
+  cpy(tW, W1, sz);
+  cpy(W1, W0, sz);
+  prod(R[kPhi1Dt2], false, one, tW, W1);
+  if (int rc = htot(lf1, W1, hW1)) return rc;
+  axpy(hW1, make_double2(-1.0, 0.0), hW0, sz);
+  if (c->etd_rosen) {
+    h1(W1 + dof, vs, tunit, dv1, true);
+    axpy(hW1 + dof, one, vs, ndn);
+    axpy(hW1, make_double2(-0.5, 0.0), dW, sz);
+  }
+  // stage 2
+  cpy(W2, W1, sz);
+  cpy(tW, hW1, sz);
+  prod(R[kPhi2Dt2], true, one, tW, W2);
+  if (int rc = htot(lf1, W2, hW2)) return rc;
+  axpy(hW2, make_double2(-1.0, 0.0), hW0, sz);
+  if (c->etd_rosen) {
+    h1(W2 + dof, vs, tunit, dv1, true);
+    axpy(hW2 + dof, one, vs, ndn);
+    axpy(hW2, make_double2(-0.5, 0.0), dW, sz);
+  }
+  // stage 3
+  cpy(tW, W3, sz);
+  cpy(W3, W0, sz);
+  prod(R[kPhi1Dt1], false, one, tW, W3);
+  cpy(tW, hW2, sz);
+  prod(R[kPhi2Dt1], true, make_double2(2.0, 0.0), tW, W3);
+  if (int rc = htot(lf3, W3, hW3)) return rc;
+  axpy(hW3, make_double2(-1.0, 0.0), hW0, sz);
+  if (c->etd_rosen) {
+    h1(W3 + dof, vs, tunit, dv3, true);
+    axpy(hW3 + dof, one, vs, ndn);
+    axpy(hW3, make_double2(-1.0, 0.0), dW, sz);
+  }
+  // summation
+  cpy(wfn, W3, sz);
+  launch_zaxpby(sz, make_double2(2.0, 0.0), hW1, make_double2(-1.0, 0.0), hW3, tW, s);
+  prod(R[kPhi2Dt1], true, one, tW, wfn);
+  launch_zaxpby(sz, one, hW1, one, hW2, tW, s);
+  launch_zaxpby(sz, one, tW, make_double2(-1.0, 0.0), hW3, tW, s);
+  prod(R[kPhi3Dt1], true, make_double2(-4.0, 0.0), tW, wfn);
+  if (apply_mask) launch_mask(wfn, c->mask.p, S.nr, S.L1, S.nfcore, S.nfun, s);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+}  // namespace
+
+int td1c_etdrb_gen(td1c_ctx* c, double dtime, int rosenbrock, int maxcyc, double thresh) {
+  if (!c) return fail(-1, "null argument");
+  return etd_generate(c, dtime, rosenbrock, maxcyc, thresh);
+}
+
+int td1c_etdrb_prop(td1c_ctx* c, const double* lfields, double* d_wfn, int apply_mask, void* stream) {
+  if (!c || !lfields || !d_wfn) return fail(-1, "null argument");
+  if (!c->etd_ready) return fail(-1, "call td1c_etdrb_gen first");
+  if (active_recorder()) return fail(-1, "etdrb steps are not batched");
+  hipStream_t s = (hipStream_t)stream;
+  HIP_TRY(hipMemcpyAsync(c->etd_lf.p, lfields, 54 * sizeof(double), hipMemcpyHostToDevice, s));
+  return enqueue_etdrb_step(c, c->etd_lf.p, reinterpret_cast<double2*>(d_wfn), apply_mask, s);
+}
+
+int td1c_etdrb_propagate(td1c_ctx* c, const td1c_field* field, double t0, double dtime, int nstep, double* d_wfn,
+                         int apply_mask, void* stream) {
+  if (!c || !field || !d_wfn) return fail(-1, "null argument");
+  if (!c->etd_ready) return fail(-1, "call td1c_etdrb_gen first");
+  if (dtime != c->etd_dt) return fail(-1, "dtime differs from the one td1c_etdrb_gen was called with");
+  if (check_field(field)) return -1;
+  if (nstep < 0) return fail(-1, "nstep must be >= 0");
+  hipStream_t s = (hipStream_t)stream;
+  const Pulse p = make_pulse(field);
+  launch_clock_set(c->clock.p, t0, dtime, s);
+  for (int i = 0; i < nstep; ++i) {
+    launch_etd_fields(p, c->clock.p, c->etd_lf.p, s);
+    if (int rc = enqueue_etdrb_step(c, c->etd_lf.p, reinterpret_cast<double2*>(d_wfn), apply_mask, s)) return rc;
+  }
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
 // Batches (scans of pulses, grids and lmax): one launch per recorded site
 // for all members.  No counterpart in the reference, whose scans run one
 // propagation per process.
@@ -2728,6 +3123,14 @@ int td1c_field_value(const td1c_field* f, double t, double lfield[9]) {
   for (int i = 0; i < 9; ++i) lfield[i] = 0.0;
   if (f->env_type < 0 || f->env_type > 2 || f->gauge < 0 || f->gauge > 1) return fail(-1, "bad field");
   pulse_lfield(make_pulse(f), t, lfield);
+  return 0;
+}
+
+int td1c_field_der(const td1c_field* f, double t, double dfield[9]) {
+  if (!f || !dfield) return fail(-1, "null argument");
+  for (int i = 0; i < 9; ++i) dfield[i] = 0.0;
+  if (f->env_type < 0 || f->env_type > 2 || f->gauge < 0 || f->gauge > 1) return fail(-1, "bad field");
+  pulse_dlfield(make_pulse(f), t, dfield);
   return 0;
 }
 

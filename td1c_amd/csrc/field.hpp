@@ -82,4 +82,29 @@ __host__ __device__ inline void pulse_lfield(const Pulse& p, double t, double* l
   }
 }
 
+// d lfield / dt in lfield[0..2] (clfield1::get_der, src/field/clfield1.cpp:
+// 193-211): length1 the analytic derivative of E (get_eder_explicit,
+// :324-378), velocity1 dA/dt = -E of the 6-point difference
+// (get_ader_explicit, :228-238); zero outside (tleft, tright).
+__host__ __device__ inline void pulse_dlfield(const Pulse& p, double t, double* df) {
+  constexpr double pi = 3.14159265358979323846264338327950288;
+  for (int i = 0; i < 9; ++i) df[i] = 0.0;
+  if (p.env_type == 0) return;
+  if (!(t > p.tleft && t < p.tright)) return;
+  if (p.gauge == 0) {
+    const double trel = t - p.tleft;
+    if (p.env_type == 2) {
+      df[2] = p.famp * p.freq * cos(p.freq * trel + p.cep);
+    } else {
+      const double fac = pi / (p.numcyc * p.period);
+      double env = sin(fac * trel);
+      env = env * env;
+      const double denv = 2.0 * fac * sin(fac * trel) * cos(fac * trel);
+      df[2] = p.famp * (p.freq * env * cos(p.freq * trel + p.cep) + denv * sin(p.freq * trel + p.cep));
+    }
+  } else {
+    df[2] = -pulse_e_from_a(p, t);
+  }
+}
+
 }  // namespace td1c

@@ -31,6 +31,7 @@ namespace td1c {
 //         hprod_tprod_dyn + hprod_zprod_dyn / hprod_pzprod_dyn, then the
 //         zaxpy of h1itr_prop.F90:57-70)
 // mode 1: y = -c V x                    (h1itr_prop.F90:106-114)
+// mode 3: y = zfac1 (T x + V x)         (hprod_h1tot, hprod_h1tot.F90:2-53)
 __global__ void __launch_bounds__(256) k_cn_rhs(const CnArgs a, int mode) {
   if (mode == 1 && a.flag[0]) return;
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -46,13 +47,14 @@ __global__ void __launch_bounds__(256) k_cn_rhs(const CnArgs a, int mode) {
     return;
   }
   const double2* src = mode == 0 ? a.wfn + (size_t)f * a.L1 * a.nr : a.x + (size_t)fd * a.L1 * a.nr;
+  const bool tpart = mode == 0 || mode == 3;
   const double2* psi_l = src + (size_t)l * a.nr;
   const int n = a.n;
   const int jlo = max(0, r - n), jhi = min(a.nr - 1, r + n);
   const double xr = a.xr[r];
   const double* al = a.alph + (size_t)(m + a.mmax1) * a.L1;
   double2 h = make_double2(0.0, 0.0);
-  if (mode == 0) {
+  if (tpart) {
     const double* band = l == 0 ? a.band_k : a.band_k0;
     for (int j = jlo; j <= jhi; ++j) {
       const int d = j - r + n;
@@ -66,9 +68,9 @@ __global__ void __launch_bounds__(256) k_cn_rhs(const CnArgs a, int mode) {
       cfma(h, psi_l[j], t);
     }
   }
-  if (mode == 1 || a.isplit == 1) {
-    // the field value: lfield(3) in mode 0, -c lfield(3) in mode 1
-    const double2 zf = mode == 0 ? make_double2(a.lf[2], 0.0) : cmul(a.zfac1, make_double2(a.lf[2], 0.0));
+  if (mode != 0 || a.isplit == 1) {
+    // the field value: lfield(3) in modes 0 and 3, -c lfield(3) in mode 1
+    const double2 zf = mode == 1 ? cmul(a.zfac1, make_double2(a.lf[2], 0.0)) : make_double2(a.lf[2], 0.0);
     if (a.igauge == 0) {
       if (l + 1 < a.L1) cfma(h, psi_l[a.nr + r], cscale(zf, xr * al[l]));
       if (l - 1 >= am) cfma(h, psi_l[-(ptrdiff_t)a.nr + r], cscale(zf, xr * al[l - 1]));
@@ -103,6 +105,8 @@ __global__ void __launch_bounds__(256) k_cn_rhs(const CnArgs a, int mode) {
     double2 y = psi_l[r];
     cfma(y, a.zfac1, h);
     *out = y;
+  } else if (mode == 3) {
+    *out = cmul(a.zfac1, h);
   } else {
     *out = h;
   }
@@ -331,7 +335,25 @@ int cn_interior_blocks(const CnArgs& a) {
 void launch_cn_stage(const CnArgs& a, const OneBodyElArgs* ob, int shape, int lc, int mode, hipStream_t s) {
   const int nd = a.nfun - a.nfcore;
   if (nd <= 0) return;
-  if (ob) {  // element form of the right-hand side (ndvr = 20)
+  if (mode == 3 || mode == 4) {  // y = zfac1 (T + V) x, or zfac1 V x (flag clear)
+    if (active_recorder()) return record_unsupported("an application of h1");
+    if (ob) {
+      OneBodyElArgs e = *ob;
+      e.cn = mode == 3 ? 3 : 2;
+      e.cv = a.zfac1;
+      e.stop = a.flag;
+      launch_onebody_el(a.x, a.y, e, s, shape, lc);
+    } else {
+      hipLaunchKernelGGL(k_cn_rhs, dim3((a.nr + 255) / 256, a.L1, nd), dim3(256), 0, s, a, mode == 3 ? 3 : 1);
+    }
+    return;
+  }
+  if (mode == 2) {  // a plain solve: y = zfac0 psi, then the first term
+    if (active_recorder()) return record_unsupported("a rational-function solve");
+    launch_zaxpby((size_t)nd * a.L1 * a.nr, a.zfac0, a.wfn + (size_t)a.nfcore * a.L1 * a.nr, make_double2(0.0, 0.0),
+                  nullptr, a.y, s);
+    mode = 0;
+  } else if (ob) {  // element form of the right-hand side (ndvr = 20)
     OneBodyElArgs e = *ob;
     e.cn = mode + 1;
     launch_onebody_el(mode == 0 ? a.wfn + (size_t)a.nfcore * a.L1 * a.nr : a.x, a.y, e, s, shape, lc);

@@ -614,12 +614,13 @@ __device__ __forceinline__ void onebody_el_body(const double2* __restrict__ orb,
         const bool full = !fc || a.fc_full;
         if (CN) {
           // the clh1itr right-hand sides: T psi (CN 1), then V psi (CN 1 with
-          // isplit = 1, and CN 2), as h1itr_prop.F90:57-70,106-114
-          if (CN == 1) {
+          // isplit = 1, and CN 2), as h1itr_prop.F90:57-70,106-114; CN 3
+          // (T + V) psi, hprod_h1tot (hprod_h1tot.F90:2-53)
+          if (CN == 1 || CN == 3) {
             h = T;
             cfma(h, p0, a.tzor[(size_t)l * a.nr + i]);
           }
-          if (CN == 2 || a.isplit == 1) {
+          if (CN >= 2 || a.isplit == 1) {
             if (a.igauge == 0) {
               if (l + 1 < a.L1) cfma(h, pu, cscale(zf, xr * al[l]));
               if (l - 1 >= am) cfma(h, pd, cscale(zf, xr * al[l - 1]));
@@ -1615,6 +1616,8 @@ bool launch_onebody_el(const double2* orb, double2* h01, const OneBodyElArgs& a,
     launch_onebody_el_t<1>(orb, h01, a, s, shape, lc);
   else if (a.cn == 2)
     launch_onebody_el_t<2>(orb, h01, a, s, shape, lc);
+  else if (a.cn == 3)
+    launch_onebody_el_t<3>(orb, h01, a, s, shape, lc);
   else
     launch_onebody_el_t<0>(orb, h01, a, s, shape, lc);
   return true;
@@ -2540,6 +2543,51 @@ void launch_to_f32(float2* y, const double2* x, size_t n, hipStream_t s) {
 void launch_zero(double2* y, size_t n, hipStream_t s) {
   if (active_recorder()) return record_unsupported("a fill kernel");
   if (n) hipLaunchKernelGGL(k_zero, dim3(grid_for(n)), dim3(256), 0, s, y, n);
+}
+
+// z = a x + b y (a null x or y drops its term), grid-stride; the vector
+// operations of cletdrb's stages (clhprod_blas.cpp: copy, clear, scal,
+// axpy, axpyz, axpbyz, xpyz, xmyz on a range of the state)
+__global__ void k_zaxpby(size_t n, double2 a, const double2* __restrict__ x, double2 b, const double2* __restrict__ y,
+                         double2* __restrict__ z) {
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+    double2 v = make_double2(0.0, 0.0);
+    if (x) v = cmul(a, x[q]);
+    if (y) cfma(v, b, y[q]);
+    z[q] = v;
+  }
+}
+
+void launch_zaxpby(size_t n, double2 a, const double2* x, double2 b, const double2* y, double2* z, hipStream_t s) {
+  if (active_recorder()) return record_unsupported("a vector operation");
+  if (n) hipLaunchKernelGGL(k_zaxpby, dim3(grid_for(n)), dim3(256), 0, s, n, a, x, b, y, z);
+}
+
+// The field arrays of one cletdrb step (cletdrb::prop4_orb with
+// etd_time_of_h = 0: Field.get_value at t, t + dt/2, t + dt, Field.get_der at
+// t, and the differences Field.get_value(t1, t, 1, -1)) for step st->n, one
+// thread each; the clock advances after every thread has read it.
+__global__ void k_etd_fields(const Pulse p, PropClock* __restrict__ st, double* __restrict__ lf) {
+  const long long n = st->n;
+  const double dt = st->dt;
+  const double t = st->t0 + (double)n * dt;
+  __syncthreads();
+  const int q = threadIdx.x;
+  if (q == 0) pulse_lfield(p, t, lf);
+  if (q == 1) pulse_lfield(p, t + 0.5 * dt, lf + 9);
+  if (q == 2) pulse_lfield(p, t + dt, lf + 18);
+  if (q == 3) pulse_dlfield(p, t, lf + 27);
+  if (q == 4 || q == 5) {
+    double a[9], b[9];
+    pulse_lfield(p, q == 4 ? t + 0.5 * dt : t + dt, a);
+    pulse_lfield(p, t, b);
+    for (int i = 0; i < 9; ++i) lf[9 * q + i] = 1.0 * a[i] + (-1.0) * b[i];
+  }
+  if (q == 0) st->n = n + 1;
+}
+
+void launch_etd_fields(const Pulse& p, PropClock* st, double* lf, hipStream_t s) {
+  hipLaunchKernelGGL(k_etd_fields, dim3(1), dim3(64), 0, s, p, st, lf);
 }
 
 void launch_field_stages(const Pulse& p, PropClock* st, const StageNodes& nodes, double* lf, hipStream_t s) {

@@ -63,7 +63,7 @@ struct OneBodyElArgs {
   // right-hand sides of the clh1itr half step (cn != 0): the block's orbital
   // blockIdx.z is dynamical orbital fshift + blockIdx.z of the problem;
   // cn = 1: out = psi + cv (T psi + [isplit = 1] V psi), cn = 2: out = cv V psi
-  // (returns at once when stop[0] is set)
+  // (returns at once when stop[0] is set), cn = 3: out = cv (T psi + V psi)
   int cn, fshift, isplit;
   double2 cv;
   const int* stop;
@@ -90,6 +90,7 @@ struct CnArgs {
   double2* x;                               // [nd][L1][nr] current term
   double2* y;                               // [nd][L1][nr] right-hand side
   double2 zfac1;                            // -c
+  double2 zfac0;                            // mode 2: the first right-hand side zfac0 psi
   double thresh;
   int* flag;                                // [0] converged
   double* part;                             // norm partial sums: bridge jobs, then interior blocks
@@ -100,7 +101,10 @@ constexpr int kCnInteriorThreads = 256;
 // partial sums of the interior pass for a problem (for the buffer size)
 int cn_interior_blocks(const CnArgs& a);
 // mode 0: the first term from psi; mode 1: one more term of the series
-// (every kernel returns at once once flag[0] is set)
+// (every kernel returns at once once flag[0] is set); mode 2: the first term
+// of a plain solve, right-hand side zfac0 psi; mode 3: only the right-hand
+// side y = zfac1 (T + V) x, an application of h1 (x, y of the arguments);
+// mode 4: only y = zfac1 V x (flag[0] must be clear)
 // ob: the element one-body arguments for the right-hand side (ndvr = 20),
 // with its launch shape; null for the band kernel
 void launch_cn_stage(const CnArgs& a, const OneBodyElArgs* ob, int shape, int lc, int mode, hipStream_t s);
@@ -314,6 +318,12 @@ void launch_orth(double2* orb, double2* cic, const int* mval, int nr, int L1, in
                  hipStream_t s);
 void launch_to_f32(float2* y, const double2* x, size_t n, hipStream_t s);
 void launch_zero(double2* y, size_t n, hipStream_t s);
+// z = a x + b y over n complex values; a null x or y drops its term
+void launch_zaxpby(size_t n, double2 a, const double2* x, double2 b, const double2* y, double2* z, hipStream_t s);
+// the six field arrays of cletdrb step st->n (t = t0 + n dt), then n += 1:
+// lfield(t), lfield(t + dt/2), lfield(t + dt), d lfield/dt (t),
+// lfield(t + dt/2) - lfield(t), lfield(t + dt) - lfield(t)
+void launch_etd_fields(const Pulse& p, PropClock* st, double* lf, hipStream_t s);
 void launch_field_stages(const Pulse& p, PropClock* st, const StageNodes& nodes, double* lf, hipStream_t s);
 void launch_clock_set(PropClock* st, double t0, double dt, hipStream_t s);
 void launch_dtorb(const DtorbArgs& a, hipStream_t s);

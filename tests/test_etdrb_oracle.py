@@ -1,0 +1,126 @@
+"""Pins of the oracle's exponential Rosenbrock propagator (cletdrb with
+clh1rat, src/prop/cletdrb.cpp, clh1rat.cpp, h1rat_gen.F90, h1rat_prod.F90)
+and of the field derivative it reads (clfield::get_der).
+
+- The rational functions: each of the seven Pade forms, at a scalar
+  eigenvalue lam of h1, equals dt1 phi_k(x) (x = -i dt lam in real time,
+  -dt lam in imaginary time) with the error of its degree: O(x^6) for the
+  [2/3] approximants of phi1, phi2, phi3, O(x^5) for the phid1 form (phi2
+  over phi1's denominator), and exactly dt1 phi_k(0) at lam = 0.
+- The step: cletdrb (split hprod + the rational functions of h1) solves the
+  same equations of motion as the full hprod, so its trajectory converges to
+  the oracle's RK4 trajectory of split_type 0 at a small step, at fourth
+  order (He and a CAS(2e,2o) state under a sin^2 pulse, both gauges).  A
+  dropped term, a wrong sign or a wrong coefficient leaves order <= 2.
+- The field derivative: central differences of td1c_field_value.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import td1c_amd
+from td1c_amd import workloads as W
+from td1c_amd.abi import Problem
+
+pytestmark = pytest.mark.skipif(not os.path.exists(td1c_amd.library_path()), reason="library not built")
+
+
+def _phi(k, x):
+    return sum(x ** j / math.factorial(j + k) for j in range(40))
+
+
+def _oracle(p):
+    from oracle.oracle import Oracle
+
+    return Oracle(p)
+
+
+def _prob(gauge, isplit, cas=False, icomp=1):
+    ig = 1 if gauge.startswith("velocity") else 0
+    occ = dict(ndcore=0, nact=2) if cas else dict(ndcore=1)
+    return Problem(nfe=3, ndvr=6, rmax=15.0, lmax1=3, lmax2=6, znuc=2.0, nel_alpha=1, nel_beta=1, igauge=ig,
+                   isplit=isplit, icomp=icomp, **occ)
+
+
+# the function each rational function approximates, its time step, order
+RATS = {"phi1dt1": (1, 1.0, 6), "phi1dt2": (1, 0.5, 6), "phid1dt1": (2, 1.0, 5), "phid1dt2": (2, 0.5, 5),
+        "phi2dt1": (2, 1.0, 6), "phi2dt2": (2, 0.5, 6), "phi3dt1": (3, 1.0, 6)}
+
+
+@pytest.mark.parametrize("icomp", [1, 0])
+def test_pade_forms_approximate_the_phi_functions(icomp):
+    dt = 0.05
+    o = _oracle(_prob("velocity1", 1, icomp=icomp))
+    o.etdrb_gen(dt)
+    tunit = -1j if icomp == 1 else -1.0
+    for name, (k, frac, order) in RATS.items():
+        r0 = o.etd_rat_scalar(name, 0j)
+        assert abs(r0 - dt * _phi(k, 0.0)) < 1e-14 * dt, name
+        errs = []
+        for lam in (8.0, 4.0, 2.0):
+            x = tunit * frac * dt * lam
+            errs.append(abs(o.etd_rat_scalar(name, complex(lam)) - dt * _phi(k, x)))
+        for a, b in zip(errs, errs[1:]):
+            assert math.log2(a / b) > order - 0.3, (name, errs)
+
+
+def _converge(gauge, cas, rosenbrock=True):
+    from td1c_amd import CRK
+
+    p1 = _prob(gauge, 1, cas)
+    o1 = _oracle(p1)
+    w0 = W.bound_wfn(p1, 7, o1.lcic)
+    f = td1c_amd.Field(env_type="sin2", gauge=gauge, fint=2e14, wlen=400.0, cep=30.0, numcyc=2.0)
+    t0, span = 10.0, 0.4
+    o0 = _oracle(_prob(gauge, 0, cas))
+    o0.set_wfn0(w0)
+    rk = CRK(4)
+    ref, t, h = w0.copy(), t0, 0.0025
+    for _ in range(int(round(span / h))):
+        ref = o0.rk_prop(4, h, rk.stage_fields(f, t, h), ref, apply_mask=False)
+        t += h
+    o1.set_wfn0(w0)
+    errs = []
+    for dt in (0.1, 0.05, 0.025):
+        o1.etdrb_gen(dt, rosenbrock=rosenbrock)
+        w, t = w0.copy(), t0
+        for _ in range(int(round(span / dt))):
+            w = o1.etdrb_step(td1c_amd.etd_fields(f, t, dt), w, apply_mask=False)
+            t += dt
+        errs.append(np.linalg.norm(w - ref))
+    return errs, np.linalg.norm(ref - w0)
+
+
+@pytest.mark.parametrize("cas", [False, True])
+@pytest.mark.parametrize("gauge", ["velocity1", "length1"])
+def test_etdrb_converges_at_fourth_order_to_the_full_hprod(gauge, cas):
+    errs, moved = _converge(gauge, cas)
+    assert moved > 0.3
+    orders = [math.log2(a / b) for a, b in zip(errs, errs[1:])]
+    assert min(orders) > 3.8, (errs, orders)
+    assert errs[-1] < 1e-6 * 10, errs
+
+
+def test_without_the_rosenbrock_terms_the_order_drops():
+    """etd_rosenbrock = false freezes V at the step start: first order under
+    a pulse, which shows the derivative terms are what carries the fourth
+    order above."""
+    errs, _ = _converge("velocity1", False, rosenbrock=False)
+    assert math.log2(errs[-2] / errs[-1]) < 1.5, errs
+
+
+@pytest.mark.parametrize("gauge", ["velocity1", "length1"])
+def test_field_derivative_is_the_time_derivative(gauge):
+    f = td1c_amd.Field(env_type="sin2", gauge=gauge, fint=1e14, wlen=800.0, cep=20.0, numcyc=3.0)
+    per = td1c_amd.field_period(f)
+    h = 1e-4
+    for t in np.linspace(0.1, 3 * per - 0.1, 7):
+        d = td1c_amd.field_der(f, t)
+        fd = (td1c_amd.field_value(f, t + h)[2] - td1c_amd.field_value(f, t - h)[2]) / (2 * h)
+        amp = np.max(np.abs([td1c_amd.field_value(f, s)[2] for s in np.linspace(0, 3 * per, 50)]))
+        assert abs(d[2] - fd) < 1e-6 * amp, (t, d[2], fd)
+        if gauge == "velocity1":
+            assert d[2] == -td1c_amd.field_value(f, t)[5]  # dA/dt = -E of the same difference
+    assert np.all(td1c_amd.field_der(f, -1.0) == 0) and np.all(td1c_amd.field_der(f, 3 * per + 1.0) == 0)
