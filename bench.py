@@ -326,7 +326,8 @@ def main():
     ap.add_argument("--scan-steps", type=int, default=100, help="timed batch steps of the scan")
     ap.add_argument("--scan-gate-steps", type=int, default=1000,
                     help="steps after which the scan's mixed and FP64 dipoles are compared")
-    ap.add_argument("--only", choices=["fp64", "mixed", "batch"], default=None,
+    ap.add_argument("--etd-steps", type=int, default=20, help="timed cletdrb steps on the workload (0: skip)")
+    ap.add_argument("--only", choices=["fp64", "mixed", "batch", "etdrb"], default=None,
                     help="measure one path only (for profiler runs; no comparisons)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a device copy of this many bytes (counter calibration)")
@@ -399,9 +400,47 @@ def main():
         return dict(prob=prob, hp=hp, wfn=wfn, ms_step=(t1 - t0) * 1e3 / args.steps, breakdown=breakdown,
                     nprof=nprof, dname=dname, dtime=times[dname], norm=norm)
 
+    def measure_etdrb(hp, wfn, steps: int, compare_steps: int = 0) -> dict:
+        """cletdrb (the reference's default prop_type) on the same context:
+        ms per step after two warm-up steps, the event breakdown of one step,
+        and the dipole after compare_steps steps against split2's."""
+        etd = td1c_amd.ETDRB(hp, dt)
+        d = hp.to_device(wfn)
+        etd.propagate(d, 0.0, 2, field)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        etd.propagate(d, 2 * dt, steps, field)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        hp.set_timing(True)
+        etd.propagate(d, (2 + steps) * dt, 1, field)
+        torch.cuda.synchronize()
+        bd = hp.get_timing()
+        hp.set_timing(False)
+        out = {"ms_per_step": ms, "hprod_evals_per_s": 4.0 / (ms * 1e-3),
+               "propagations_per_s": 1.0 / (nprop * ms * 1e-3),
+               "event_ms_one_step": {k: round(v[0], 4) for k, v in bd.items()}}
+        if compare_steps > 0:
+            a = hp.to_device(wfn)
+            b = hp.to_device(wfn)
+            etd.propagate(a, 0.0, compare_steps, field)
+            rk.propagate(hp, b, 0.0, dt, compare_steps, field)
+            torch.cuda.synchronize()
+            lf = td1c_amd.field_value(field, compare_steps * dt)
+            da, db = hp.dipole(lf, a), hp.dipole(lf, b)
+            out["dipole_vs_split2"] = {"steps": compare_steps, "etdrb": float(da), "split2": float(db),
+                                       "abs_diff": float(abs(da - db))}
+        return out
+
     if args.only == "batch":
         r = scan_batch(args.scan_members, dt, args.warmup, args.scan_steps, args.scan_gate_steps)
         print(json.dumps({"only": "batch", **r}), flush=True)
+        return
+
+    if args.only == "etdrb":
+        _, hp, wfn = context(0)
+        print(json.dumps({"only": "etdrb", "workload": args.workload, **measure_etdrb(hp, wfn, args.etd_steps)}),
+              flush=True)
         return
 
     if args.only is not None:
@@ -514,6 +553,13 @@ def main():
                  "fp64_hprod_evals_per_s": rates[0], "mixed_hprod_evals_per_s": rates[1],
                  "hprod_residual": float(np.linalg.norm(kk[1] - kk[0]) / np.linalg.norm(kk[0]))}
 
+    etd = None
+    if args.etd_steps > 0:
+        etd = {"fp64": measure_etdrb(fp64["hp"], fp64["wfn"], args.etd_steps, compare_steps=100),
+               "mixed": measure_etdrb(mixed["hp"], fp64["wfn"], args.etd_steps)}
+        etd["split2_ms_per_step_fp64"] = fp64["ms_step"]
+        progress(f"etdrb {etd['fp64']['ms_per_step']:.3f} / {etd['mixed']['ms_per_step']:.3f} ms/step")
+
     scan = None
     if args.scan_members > 0:
         scan = scan_batch(args.scan_members, dt, 10, args.scan_steps, args.scan_gate_steps)
@@ -588,6 +634,7 @@ def main():
         "mean_field_stress": extra,
         "host_entry": host_entry,
         "scan_batch": scan,
+        "etdrb": etd,
     }
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(fp64["prob"], fp64["wfn"], lf0, args.cpu_seconds, args.workload)
