@@ -299,6 +299,22 @@ int td1c_etdrb_prop(td1c_ctx* ctx, const double* lfields, double* d_wfn, int app
 int td1c_etdrb_propagate(td1c_ctx* ctx, const td1c_field* field, double t0, double dtime, int nstep,
                          double* d_wfn, int apply_mask, void* stream);
 
+/* cletdrk4 (src/prop/cletdrk4.cpp, prop_type "etdrk4"): the Cox-Matthews
+ * exponential RK4 (prop, :69-114) with the [2/2] Pade forms of its defaults
+ * (etd_dim_numer = etd_dim_denom = 2, gen :24-60) for exp(dt h1),
+ * exp(dt h1 / 2), phi1 (gfun) and the three weight functions f0fun, f1fun,
+ * f2fun (gen_exp ... gen_f2fun, :134-933), all taking the field at t + dt/2;
+ * the CI part is scaled by the functions' dt -> 0 limits (exp1rk ...
+ * f2funrk).  The rational functions are solved as td1c_etdrb_gen's; the same
+ * restrictions apply.  A context holds one scheme's tables at a time
+ * (td1c_etdrb_gen or td1c_etdrk4_gen, the last call wins); lfields as for
+ * td1c_etdrb_prop (this scheme reads lfield(t), lfield(t + dt/2) and
+ * lfield(t + dt)). */
+int td1c_etdrk4_gen(td1c_ctx* ctx, double dtime, int maxcyc, double thresh);
+int td1c_etdrk4_prop(td1c_ctx* ctx, const double* lfields, double* d_wfn, int apply_mask, void* stream);
+int td1c_etdrk4_propagate(td1c_ctx* ctx, const td1c_field* field, double t0, double dtime, int nstep,
+                          double* d_wfn, int apply_mask, void* stream);
+
 /* Grid projection of a state between two problems, host memory only (no GPU
  * call): clbas::proj (src/bas/clbas.cpp:280-332) with the CI copy of
  * guess_type = "read_proj" (src/main/guess.cpp:37-47).  Each interior point r

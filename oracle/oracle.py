@@ -81,6 +81,10 @@ def lib(openmp: bool = False):
         L.orc_etdrb_step.restype = ctypes.c_int
         L.orc_etd_rat_scalar.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, vp]
         L.orc_etd_rat_scalar.restype = None
+        L.orc_etdrk4_gen.argtypes = [vp, ctypes.c_double, ctypes.c_int, ctypes.c_double]
+        L.orc_etdrk4_gen.restype = ctypes.c_int
+        L.orc_etdrk4_step.argtypes = [vp, vp, vp, ctypes.c_int]
+        L.orc_etdrk4_step.restype = ctypes.c_int
         L.orc_band_chol.argtypes = [ctypes.c_int, ctypes.c_int, vp]
         L.orc_band_chol.restype = ctypes.c_int
         L.orc_band_chol_solve.argtypes = [ctypes.c_int, ctypes.c_int, vp, vp]
@@ -286,11 +290,32 @@ class Oracle:
             raise RuntimeError(f"orc_etdrb_step failed ({rc})")
         return w
 
+    # the rational functions of cletdrk4, in orc_etdrk4_gen's order
+    ETK_RATS = ("exp1", "exp2", "gfun", "f0fun", "f1fun", "f2fun")
+
     def etd_rat_scalar(self, name: str, lam: complex) -> complex:
-        """A cletdrb rational function at a scalar eigenvalue lam of h1."""
+        """A rational function of the last etdrb / etdrk4 gen at a scalar
+        eigenvalue lam of h1."""
         out = np.zeros(2)
-        self._L.orc_etd_rat_scalar(self.h, self.ETD_RATS.index(name), lam.real, lam.imag, _p(out))
+        idx = self.ETD_RATS.index(name) if name in self.ETD_RATS else self.ETK_RATS.index(name)
+        self._L.orc_etd_rat_scalar(self.h, idx, lam.real, lam.imag, _p(out))
         return complex(out[0], out[1])
+
+    def etdrk4_gen(self, dtime: float, maxcyc: int = 20, thresh: float = 1e-10):
+        """cletdrk4::gen for step dtime (defaults: the [2/2] forms)."""
+        rc = self._L.orc_etdrk4_gen(self.h, dtime, maxcyc, thresh)
+        if rc:
+            raise RuntimeError(f"orc_etdrk4_gen failed ({rc})")
+
+    def etdrk4_step(self, lfields: np.ndarray, wfn: np.ndarray, apply_mask: bool = True) -> np.ndarray:
+        """cletdrk4::prop then the mask; lfields as for etdrb_step."""
+        w = np.array(wfn, dtype=np.complex128, copy=True)
+        lf = np.ascontiguousarray(lfields, dtype=np.float64)
+        assert lf.size == 54
+        rc = self._L.orc_etdrk4_step(self.h, _p(lf), _p(w), int(apply_mask))
+        if rc:
+            raise RuntimeError(f"orc_etdrk4_step failed ({rc})")
+        return w
 
     def proj_to(self, other: "Oracle", wfn: np.ndarray) -> np.ndarray:
         """clbas::proj from this problem's grid to other's, with the CI copy of

@@ -110,7 +110,10 @@ typedef struct orc {
   /* cletdrb: the clh1rat rational functions (ETD_NRAT of them) */
   int etd_ready, etd_rosen, etd_maxcyc;
   double etd_dt1, etd_thresh;
+  int etd_scheme;                         /* 1 cletdrb, 2 cletdrk4 */
   double etd_dt[8];                       /* clh1rat dtime of each */
+  int etd_dimn[8], etd_dimd[8];
+  zc etd_rk[8];                           /* cletdrk4's CI factors (exp1rk ... f2funrk) */
   zc etd_cn[8][4], etd_cd0[8][3], etd_cd1[8][3];
   zc etd_numer_limit[8], etd_denom_limit[8], etd_coeff_limit[8];
   zc *etd_inv[8][3];                      /* (3n+1)*nr*L1 band LU of cd0 + cd1 T_l */
@@ -2027,29 +2030,31 @@ static void etd_family(int fam, double* dfac, double* n, zc* roots) {
   for (int i = 0; i < 3; ++i) roots[i] = r[i][0] + I * r[i][1];
 }
 
-/* One clh1rat: the scaling of the gen_phi* loops (numerator by dt1 tunit^k,
- * denominator slopes by tunit dt) and clh1rat::gen (:15-47) with h1rat_gen's
- * band LU of cd0 + cd1 T_l (T = tmat), skipped for |cd1| <= h1rat_thresh. */
-static int etd_rat_gen(orc* o, int ir, int fam, double dt) {
-  double dfac, nn[3];
-  zc roots[3];
-  etd_family(fam, &dfac, nn, roots);
+/* One clh1rat from a form (numerator n[0..dimn], first-factor sign dfac,
+ * denominator roots[0..dimd-1]): the scaling of the gen_* loops (numerator
+ * by nscale tunit^k, denominator slopes by tunit dt) and clh1rat::gen
+ * (:15-47) with h1rat_gen's band LU of cd0 + cd1 T_l (T = tmat), skipped for
+ * |cd1| <= h1rat_thresh. */
+static int etd_rat_set(orc* o, int ir, double dfac, const double* nn, int dimn, const zc* roots, int dimd,
+                       double nscale, double dt) {
   zc tunit = o->icomp == 0 ? -1.0 : -I;
-  zc tfac = tunit * dt, ttmp = o->etd_dt1;
-  for (int k = 0; k <= 2; ++k) {
+  zc tfac = tunit * dt, ttmp = nscale;
+  for (int k = 0; k <= dimn; ++k) {
     o->etd_cn[ir][k] = nn[k] * ttmp;
     ttmp *= tunit;
   }
   o->etd_cd1[ir][0] = dfac * tfac;
   o->etd_cd0[ir][0] = -dfac * roots[0];
-  for (int i = 1; i < 3; ++i) {
+  for (int i = 1; i < dimd; ++i) {
     o->etd_cd1[ir][i] = tfac;
     o->etd_cd0[ir][i] = -roots[i];
   }
+  o->etd_dimn[ir] = dimn;
+  o->etd_dimd[ir] = dimd;
   o->etd_dt[ir] = dt;
   o->etd_numer_limit[ir] = o->etd_cn[ir][0];
   zc den = 1.0;
-  for (int i = 0; i < 3; ++i) den *= o->etd_cd0[ir][i];
+  for (int i = 0; i < dimd; ++i) den *= o->etd_cd0[ir][i];
   o->etd_denom_limit[ir] = den;
   o->etd_coeff_limit[ir] = o->etd_numer_limit[ir] / den;
   int n = o->ndvr, nr = o->nr, ld = 3 * n + 1;
@@ -2059,7 +2064,7 @@ static int etd_rat_gen(orc* o, int ir, int fam, double dt) {
     free(o->etd_piv[ir][i]);
     o->etd_inv[ir][i] = NULL;
     o->etd_piv[ir][i] = NULL;
-    if (!(cabs(o->etd_cd1[ir][i]) > o->etd_thresh)) continue;
+    if (i >= dimd || !(cabs(o->etd_cd1[ir][i]) > o->etd_thresh)) continue;
     o->etd_inv[ir][i] = xcalloc((size_t)ld * nr * o->L1, sizeof(zc));
     o->etd_piv[ir][i] = xcalloc((size_t)nr * o->L1, sizeof(int));
     for (int l = 0; l <= o->lmax1; ++l) {
@@ -2076,6 +2081,15 @@ static int etd_rat_gen(orc* o, int ir, int fam, double dt) {
   return 0;
 }
 
+/* cletdrb's forms: the scaling of gen_phi* (numerator by dt1 for both step
+ * sizes). */
+static int etd_rat_gen(orc* o, int ir, int fam, double dt) {
+  double dfac, nn[3];
+  zc roots[3];
+  etd_family(fam, &dfac, nn, roots);
+  return etd_rat_set(o, ir, dfac, nn, 2, roots, 3, o->etd_dt1, dt);
+}
+
 /* cletdrb::gen: the rational functions prop4_orb uses, for step dtime; the
  * band factors take tmat as it is now (with v2jfc after orc_set_wfn0). */
 int orc_etdrb_gen(orc* o, double dtime, int rosenbrock, int maxcyc, double thresh) {
@@ -2088,6 +2102,7 @@ int orc_etdrb_gen(orc* o, double dtime, int rosenbrock, int maxcyc, double thres
   const double dts[ETD_NRAT] = {dtime, 0.5 * dtime, dtime, 0.5 * dtime, dtime, 0.5 * dtime, dtime};
   for (int r = 0; r < ETD_NRAT; ++r)
     if (etd_rat_gen(o, r, fam[r], dts[r])) return -1;
+  o->etd_scheme = 1;
   o->etd_ready = 1;
   return 0;
 }
@@ -2097,12 +2112,12 @@ int orc_etdrb_gen(orc* o, double dtime, int rosenbrock, int maxcyc, double thres
  * h1rat_prod applies to orbitals (for tests of the tables). */
 void orc_etd_rat_scalar(const orc* o, int ir, double lam_re, double lam_im, double* out) {
   zc lam = lam_re + I * lam_im, x = o->etd_dt[ir] * lam, num = 0.0, p = 1.0;
-  for (int k = 0; k <= 2; ++k) {
+  for (int k = 0; k <= o->etd_dimn[ir]; ++k) {
     num += o->etd_cn[ir][k] * p;
     p *= x;
   }
   zc den = 1.0;
-  for (int i = 0; i < 3; ++i) den *= o->etd_cd0[ir][i] + o->etd_cd1[ir][i] * lam;
+  for (int i = 0; i < o->etd_dimd[ir]; ++i) den *= o->etd_cd0[ir][i] + o->etd_cd1[ir][i] * lam;
   zc v = num / den;
   out[0] = creal(v);
   out[1] = cimag(v);
@@ -2151,14 +2166,14 @@ static void etd_h1rat_prod(orc* o, int ir, int do_numer, int do_denom, const dou
   if (do_numer) {
     memcpy(torb + d0, w + d0, sizeof(zc) * (size1 - d0));
     for (size_t k = d0; k < size1; ++k) orb[k] *= o->etd_cn[ir][0];
-    for (int idim = 1; idim <= 2; ++idim) {
+    for (int idim = 1; idim <= o->etd_dimn[ir]; ++idim) {
       etd_h1tot(o, lf, torb, h1orb);
       for (size_t k = d0; k < size1; ++k) torb[k] = tfac * h1orb[k];
       for (size_t k = d0; k < size1; ++k) orb[k] += o->etd_cn[ir][idim] * torb[k];
     }
   }
   if (do_denom) {
-    for (int idim = 0; idim < 3; ++idim) {
+    for (int idim = 0; idim < o->etd_dimd[ir]; ++idim) {
       if (cabs(o->etd_cd1[ir][idim]) < o->etd_thresh) continue;
       etd_h0inv(o, ir, idim, orb);
       if (o->isplit == 1 && o->icomp == 1) {
@@ -2204,7 +2219,7 @@ static void etd_prod_numer(orc* o, int ir, zc zfac, const double* lf, const zc* 
     torb[k] = win[k];
     orb[k] *= o->etd_cn[ir][0];
   }
-  for (int idim = 1; idim <= 2; ++idim) {
+  for (int idim = 1; idim <= o->etd_dimn[ir]; ++idim) {
     etd_h1tot(o, lf, torb, h1orb);
     for (size_t k = d0; k < size1; ++k) torb[k] = tfac * h1orb[k];
     for (size_t k = d0; k < size1; ++k) orb[k] += o->etd_cn[ir][idim] * torb[k];
@@ -2232,7 +2247,7 @@ static void etd_prod(orc* o, int ir, int numer, zc zfac, const double* lf, zc* w
  * (clfield::get_der), lfield(t + dt/2) - lfield(t), lfield(t + dt) -
  * lfield(t).  wfn: orbitals then CI. */
 int orc_etdrb_step(orc* o, const double* lf, zc* wfn, int apply_mask) {
-  if (!o->etd_ready) return -1;
+  if (!o->etd_ready || o->etd_scheme != 1) return -1;
   size_t size1 = (size_t)o->nbas * o->nfun, sz = size1 + o->lcic;
   const double *lf0 = lf, *lf1 = lf + 9, *lf3 = lf + 18, *dlf0 = lf + 27, *dv1 = lf + 36, *dv3 = lf + 45;
   zc tunit = o->icomp == 0 ? -1.0 : -I;
@@ -2294,6 +2309,98 @@ int orc_etdrb_step(orc* o, const double* lf, zc* wfn, int apply_mask) {
   for (size_t k = 0; k < sz; ++k) tw[k] = hw1[k] + hw2[k];
   for (size_t k = 0; k < sz; ++k) tw[k] = tw[k] - hw3[k];
   etd_prod(o, ETD_PHI3_DT1, 1, -4.0, lf0, tw, wfn);
+  if (apply_mask) orc_mask(o, wfn);
+  free(buf);
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* cletdrk4 (src/prop/cletdrk4.cpp): the Cox-Matthews exponential RK4   */
+/* with the [2/2] forms of etd_dim_numer = etd_dim_denom = 2 (gen :24-60, */
+/* the defaults) for exp, gfun (phi1), f0fun, f1fun, f2fun.              */
+/* ------------------------------------------------------------------ */
+enum { ETK_EXP1, ETK_EXP2, ETK_GFUN, ETK_F0, ETK_F1, ETK_F2, ETK_NRAT };
+
+/* cletdrk4::gen with gen_exp (:134-276), gen_gfun (:278-439), gen_f0fun
+ * (:441-604), gen_f1fun (:606-768), gen_f2fun (:770-933). */
+int orc_etdrk4_gen(orc* o, double dtime, int maxcyc, double thresh) {
+  if (o->isplit != 1) return -2;
+  double dt1 = dtime, dt2 = 0.5 * dtime, s3 = sqrt(3.0);
+  o->etd_dt1 = dtime;
+  o->etd_rosen = 0;
+  o->etd_maxcyc = maxcyc;
+  o->etd_thresh = thresh;
+  const double n_exp[3] = {12.0, 6.0, 1.0}, n_g[3] = {60.0, 6.0, 1.0}, n_f0[3] = {34860.0, 19620.0, 2657.0},
+               n_f1[3] = {2940.0, 450.0, 41.0}, n_f2[3] = {1260.0, -420.0, -13.0};
+  const zc r_exp[2] = {3.0 - s3 * I, 3.0 + s3 * I}, r_g[2] = {4.0 - 2.0 * I, 4.0 + 2.0 * I},
+           r_f0[2] = {(762.0 - sqrt(189762.0) * I) / 221.0, (762.0 + sqrt(189762.0) * I) / 221.0},
+           r_f1[2] = {(51.0 - sqrt(633.0) * I) / 11.0, (51.0 + sqrt(633.0) * I) / 11.0},
+           r_f2[2] = {(21.0 - 3.0 * sqrt(21.0) * I) / 5.0, (21.0 + 3.0 * sqrt(21.0) * I) / 5.0};
+  if (etd_rat_set(o, ETK_EXP1, 1.0, n_exp, 2, r_exp, 2, 1.0, dt1) ||
+      etd_rat_set(o, ETK_EXP2, 1.0, n_exp, 2, r_exp, 2, 1.0, dt2) ||
+      etd_rat_set(o, ETK_GFUN, 3.0, n_g, 2, r_g, 2, dt2, dt2) ||
+      etd_rat_set(o, ETK_F0, 13260.0, n_f0, 2, r_f0, 2, dt1, dt1) ||
+      etd_rat_set(o, ETK_F1, 330.0, n_f1, 2, r_f1, 2, dt1, dt1) ||
+      etd_rat_set(o, ETK_F2, 300.0, n_f2, 2, r_f2, 2, dt1, dt1))
+    return -1;
+  o->etd_rk[ETK_EXP1] = 1.0;
+  o->etd_rk[ETK_EXP2] = 1.0;
+  o->etd_rk[ETK_GFUN] = dt2;
+  o->etd_rk[ETK_F0] = dt1 / 6.0;
+  o->etd_rk[ETK_F1] = dt1 / 3.0;
+  o->etd_rk[ETK_F2] = dt1 / 6.0;
+  o->etd_scheme = 2;
+  o->etd_ready = 1;
+  return 0;
+}
+
+/* cletdrk4::prod1 (:116-123): wout = win with its CI times rk and its
+ * orbitals through R (clh1rat::prod, :49-65). */
+static void etk_prod1(orc* o, int ir, const double* lf, const zc* win, zc* wout) {
+  size_t size1 = (size_t)o->nbas * o->nfun, sz = size1 + o->lcic;
+  memcpy(wout, win, sizeof(zc) * sz);
+  for (int k = 0; k < o->lcic; ++k) wout[size1 + k] *= o->etd_rk[ir];
+  etd_h1rat_prod(o, ir, 1, 1, lf, wout);
+}
+
+/* cletdrk4::prod2 (:125-132): win's CI times rk, its orbitals through R,
+ * then wout += win. */
+static void etk_prod2(orc* o, int ir, const double* lf, zc* win, zc* wout) {
+  size_t size1 = (size_t)o->nbas * o->nfun, sz = size1 + o->lcic;
+  for (int k = 0; k < o->lcic; ++k) win[size1 + k] *= o->etd_rk[ir];
+  etd_h1rat_prod(o, ir, 1, 1, lf, win);
+  for (size_t k = 0; k < sz; ++k) wout[k] = win[k] + wout[k];
+}
+
+/* cletdrk4::prop (:69-114), then the mask of tdse_prop12.  lf: the six
+ * arrays of orc_etdrb_step (this scheme reads lfield(t), lfield(t + dt/2)
+ * and lfield(t + dt); the rational functions take lfield(t + dt/2)). */
+int orc_etdrk4_step(orc* o, const double* lf, zc* wfn, int apply_mask) {
+  if (!o->etd_ready || o->etd_scheme != 2) return -1;
+  size_t size1 = (size_t)o->nbas * o->nfun, sz = size1 + o->lcic;
+  const double *lf0 = lf, *lf1 = lf + 9, *lf3 = lf + 18;
+  zc* buf = xcalloc(sz * 9, sizeof(zc));
+  zc *w0 = buf, *w1 = buf + sz, *w2 = buf + 2 * sz, *w3 = buf + 3 * sz, *hw0 = buf + 4 * sz, *hw1 = buf + 5 * sz,
+     *hw2 = buf + 6 * sz, *hw3 = buf + 7 * sz, *tw = buf + 8 * sz;
+  memcpy(w0, wfn, sizeof(zc) * sz);
+  orc_htot(o, 1.0, lf0, w0, w0 + size1, hw0, hw0 + size1);
+  etk_prod1(o, ETK_GFUN, lf1, hw0, w1);
+  etk_prod1(o, ETK_EXP2, lf1, w0, tw);
+  for (size_t k = 0; k < sz; ++k) w1[k] = tw[k] + w1[k];
+  orc_htot(o, 1.0, lf1, w1, w1 + size1, hw1, hw1 + size1);
+  etk_prod1(o, ETK_GFUN, lf1, hw1, w2);
+  for (size_t k = 0; k < sz; ++k) w2[k] = tw[k] + w2[k];
+  orc_htot(o, 1.0, lf1, w2, w2 + size1, hw2, hw2 + size1);
+  for (size_t k = 0; k < sz; ++k) tw[k] = 2.0 * hw2[k] + (-1.0) * hw0[k];
+  etk_prod1(o, ETK_GFUN, lf1, tw, w3);
+  etk_prod1(o, ETK_EXP2, lf1, w1, tw);
+  for (size_t k = 0; k < sz; ++k) w3[k] = tw[k] + w3[k];
+  orc_htot(o, 1.0, lf3, w3, w3 + size1, hw3, hw3 + size1);
+  for (size_t k = 0; k < sz; ++k) hw2[k] = hw1[k] + hw2[k];
+  etk_prod1(o, ETK_EXP1, lf1, w0, wfn);
+  etk_prod2(o, ETK_F0, lf1, hw0, wfn);
+  etk_prod2(o, ETK_F1, lf1, hw2, wfn);
+  etk_prod2(o, ETK_F2, lf1, hw3, wfn);
   if (apply_mask) orc_mask(o, wfn);
   free(buf);
   return 0;

@@ -69,6 +69,13 @@ def load_library():
     L.td1c_etdrb_propagate.argtypes = [vp, ctypes.POINTER(TD1CField), ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                        vp, ctypes.c_int, vp]
     L.td1c_etdrb_propagate.restype = ctypes.c_int
+    L.td1c_etdrk4_gen.argtypes = [vp, ctypes.c_double, ctypes.c_int, ctypes.c_double]
+    L.td1c_etdrk4_gen.restype = ctypes.c_int
+    L.td1c_etdrk4_prop.argtypes = [vp, vp, vp, ctypes.c_int, vp]
+    L.td1c_etdrk4_prop.restype = ctypes.c_int
+    L.td1c_etdrk4_propagate.argtypes = [vp, ctypes.POINTER(TD1CField), ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_int, vp, ctypes.c_int, vp]
+    L.td1c_etdrk4_propagate.restype = ctypes.c_int
     L.td1c_set_graph.argtypes = [vp, ctypes.c_int]
     L.td1c_set_graph.restype = ctypes.c_int
     L.td1c_set_transform_path.argtypes = [vp, ctypes.c_int]
@@ -588,14 +595,20 @@ class CRK:
 
 class ETDRB:
     """Exponential Rosenbrock RK4 with Pade phi functions (cletdrb, the
-    reference's default prop_type; td1c_etdrb_*).  Needs split_type = 1."""
+    reference's default prop_type; td1c_etdrb_*).  Needs split_type = 1.
+    A context holds the tables of one exponential scheme at a time."""
+
+    _fn = "etdrb"
 
     def __init__(self, hprod: "HProd", dtime: float, rosenbrock: bool = True, maxcyc: int = 20,
                  thresh: float = 1e-10):
         self.hprod = hprod
         self.dtime = float(dtime)
-        _check(load_library().td1c_etdrb_gen(hprod.h, self.dtime, int(rosenbrock), int(maxcyc), float(thresh)),
-               "td1c_etdrb_gen")
+        self._gen(rosenbrock, maxcyc, thresh)
+
+    def _gen(self, rosenbrock, maxcyc, thresh):
+        _check(load_library().td1c_etdrb_gen(self.hprod.h, self.dtime, int(rosenbrock), int(maxcyc),
+                                             float(thresh)), "td1c_etdrb_gen")
 
     def prop(self, wfn, time: float = 0.0, field: Optional[Field] = None, lfields: Optional[np.ndarray] = None,
              apply_mask: bool = True, stream=None):
@@ -608,18 +621,34 @@ class ETDRB:
         lf = np.ascontiguousarray(lfields, dtype=np.float64)
         if lf.size != 54:
             raise ValueError("lfields must hold 6 x 9 values")
-        _check(load_library().td1c_etdrb_prop(self.hprod.h, _ptr(lf), ctypes.c_void_p(wfn.data_ptr()),
-                                              int(apply_mask), self.hprod._stream(stream)), "td1c_etdrb_prop")
+        fn = getattr(load_library(), f"td1c_{self._fn}_prop")
+        _check(fn(self.hprod.h, _ptr(lf), ctypes.c_void_p(wfn.data_ptr()), int(apply_mask),
+                  self.hprod._stream(stream)), f"td1c_{self._fn}_prop")
         return wfn
 
     def propagate(self, wfn, time: float, nstep: int, field: Field, apply_mask: bool = True, stream=None):
         """nstep steps of tdse_prop12 from `time`, the fields evaluated on the
         device."""
         cf = field.to_c()
-        _check(load_library().td1c_etdrb_propagate(self.hprod.h, ctypes.byref(cf), float(time), self.dtime,
-                                                   int(nstep), ctypes.c_void_p(wfn.data_ptr()), int(apply_mask),
-                                                   self.hprod._stream(stream)), "td1c_etdrb_propagate")
+        fn = getattr(load_library(), f"td1c_{self._fn}_propagate")
+        _check(fn(self.hprod.h, ctypes.byref(cf), float(time), self.dtime, int(nstep),
+                  ctypes.c_void_p(wfn.data_ptr()), int(apply_mask), self.hprod._stream(stream)),
+               f"td1c_{self._fn}_propagate")
         return wfn
+
+
+class ETDRK4(ETDRB):
+    """The Cox-Matthews exponential RK4 with [2/2] Pade forms (cletdrk4,
+    prop_type "etdrk4"; td1c_etdrk4_*).  Needs split_type = 1."""
+
+    _fn = "etdrk4"
+
+    def __init__(self, hprod: "HProd", dtime: float, maxcyc: int = 20, thresh: float = 1e-10):
+        super().__init__(hprod, dtime, False, maxcyc, thresh)
+
+    def _gen(self, rosenbrock, maxcyc, thresh):
+        _check(load_library().td1c_etdrk4_gen(self.hprod.h, self.dtime, int(maxcyc), float(thresh)),
+               "td1c_etdrk4_gen")
 
 
 class Batch:

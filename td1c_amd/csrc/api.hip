@@ -190,12 +190,15 @@ struct td1c_ctx {
   };
   struct EtdRat {
     double dt = 0.0;
+    int dimn = 0, dimd = 0;
     double2 cn[3], cd0[3], cd1[3];
     double2 numer_limit, denom_limit, coeff_limit;
+    double2 rk;  // cletdrk4's CI factor of this function
     int tab[3];
   };
   std::vector<std::unique_ptr<EtdTab>> etd_tabs;
   EtdRat etd[7];
+  int etd_scheme = 0;  // 1 cletdrb, 2 cletdrk4
   bool etd_ready = false;
   int etd_rosen = 1, etd_maxcyc = 20;
   double etd_dt = 0.0, etd_thresh = 1.0e-10;
@@ -2274,6 +2277,7 @@ int td1c_propagate(td1c_ctx* c, const td1c_field* field, int rk_order, double t0
 namespace {
 
 enum { kPhi1Dt1, kPhi1Dt2, kPhid1Dt1, kPhid1Dt2, kPhi2Dt1, kPhi2Dt2, kPhi3Dt1, kEtdRats };
+enum { kExp1, kExp2, kGfun, kF0, kF1, kF2 };  // cletdrk4's functions
 
 // The [2/3] forms: numerator coefficients, the sign of the first denominator
 // factor and the denominator roots (gen_phi1 :636-771, gen_phid1 :773-886,
@@ -2323,82 +2327,121 @@ int cn_buffers(td1c_ctx* c) {
   return 0;
 }
 
-int etd_generate(td1c_ctx* c, double dt, int rosen, int maxcyc, double thresh) {
+// One clh1rat from a form (numerator num[0..dimn], first-factor sign dfac,
+// denominator roots[0..dimd-1]): the scaling of the gen_* loops (numerator
+// by nscale tunit^k, the denominator slopes by tunit dt), the dt -> 0
+// limits, and the element Schur tables of its distinct factors 1 + (cd1/cd0)T
+// (h1rat_gen skips factors with |cd1| <= thresh).
+int etd_set(td1c_ctx* c, int r, double dfac, const double* num, int dimn, const std::complex<double>* root, int dimd,
+            double nscale, double dt, double thresh) {
   Setup& S = c->S;
-  if (S.isplit != 1) return fail(-1, "etdrb needs split_type = 1 (the split hprod)");
-  if (S.lconst != 0) return fail(-1, "etdrb with lconst is not provided");
+  const std::complex<double> tunit = S.icomp == 0 ? std::complex<double>(-1.0, 0.0) : std::complex<double>(0.0, -1.0);
+  td1c_ctx::EtdRat& R = c->etd[r];
+  std::complex<double> ttmp = nscale, cd0[3], cd1[3];
+  for (int k = 0; k <= dimn; ++k) {
+    R.cn[k] = d2(num[k] * ttmp);
+    ttmp *= tunit;
+  }
+  const std::complex<double> tfac = tunit * dt;
+  cd1[0] = dfac * tfac;
+  cd0[0] = -dfac * root[0];
+  for (int i = 1; i < dimd; ++i) {
+    cd1[i] = tfac;
+    cd0[i] = -root[i];
+  }
+  R.dt = dt;
+  R.dimn = dimn;
+  R.dimd = dimd;
+  R.rk = make_double2(1.0, 0.0);
+  std::complex<double> den = 1.0;
+  for (int i = 0; i < dimd; ++i) {
+    R.cd0[i] = d2(cd0[i]);
+    R.cd1[i] = d2(cd1[i]);
+    den *= cd0[i];
+  }
+  const std::complex<double> nl = num[0] * std::complex<double>(nscale);
+  R.numer_limit = d2(nl);
+  R.denom_limit = d2(den);
+  R.coeff_limit = d2(nl / den);
+  auto up = [](DevBuf<double2>& b, const std::vector<double>& v) -> hipError_t {
+    hipError_t e = b.alloc(std::max<size_t>(v.size() / 2, 1));
+    if (e != hipSuccess || v.empty()) return e;
+    return hipMemcpy(b.p, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice);
+  };
+  for (int i = 0; i < 3; ++i) {
+    R.tab[i] = -1;
+    if (i >= dimd || !(std::abs(cd1[i]) > thresh)) continue;
+    const std::complex<double> f = cd1[i] / cd0[i];
+    for (size_t t = 0; t < c->etd_tabs.size(); ++t)
+      if (c->etd_tabs[t]->fre == f.real() && c->etd_tabs[t]->fim == f.imag()) R.tab[i] = (int)t;
+    if (R.tab[i] >= 0) continue;
+    CnTables T;
+    std::string err;
+    if (!build_cn(S, c->v2jfc_h.empty() ? nullptr : c->v2jfc_h.data(), f.real(), f.imag(), T, err))
+      return fail(-5, err);
+    auto tab = std::make_unique<td1c_ctx::EtdTab>();
+    HIP_TRY(up(tab->G, T.G));
+    HIP_TRY(up(tab->Wl, T.Wl));
+    HIP_TRY(up(tab->Wr, T.Wr));
+    HIP_TRY(up(tab->pa, T.pa));
+    HIP_TRY(up(tab->pg, T.pg));
+    HIP_TRY(up(tab->binv, T.binv));
+    tab->K = T.K;
+    tab->fre = f.real();
+    tab->fim = f.imag();
+    R.tab[i] = (int)c->etd_tabs.size();
+    c->etd_tabs.push_back(std::move(tab));
+  }
+  return 0;
+}
+
+// scheme 1: cletdrb::gen (the [2/3] forms, numerators scaled by dt1 for
+// both step sizes); scheme 2: cletdrk4::gen (:24-60) with gen_exp,
+// gen_gfun, gen_f0fun, gen_f1fun, gen_f2fun (:134-933), the [2/2] forms.
+int etd_generate(td1c_ctx* c, int scheme, double dt, int rosen, int maxcyc, double thresh) {
+  Setup& S = c->S;
+  const char* name = scheme == 1 ? "etdrb" : "etdrk4";
+  if (S.isplit != 1) return fail(-1, std::string(name) + " needs split_type = 1 (the split hprod)");
+  if (S.lconst != 0) return fail(-1, std::string(name) + " with lconst is not provided");
   if (S.nfcore > 0 && !c->wfn0_set) return fail(-4, "frozen core needs td1c_set_wfn0 first");
-  if (!(dt != 0.0) || !std::isfinite(dt)) return fail(-1, "etdrb needs a finite nonzero dtime");
+  if (!(dt != 0.0) || !std::isfinite(dt)) return fail(-1, std::string(name) + " needs a finite nonzero dtime");
   if (maxcyc < 0 || !(thresh >= 0.0)) return fail(-1, "bad maxcyc or thresh");
   if (c->etd_buf.p) HIP_TRY(hipDeviceSynchronize());  // earlier steps may still read the old tables
-  const std::complex<double> tunit = S.icomp == 0 ? std::complex<double>(-1.0, 0.0) : std::complex<double>(0.0, -1.0);
-  const int fam[kEtdRats] = {0, 0, 1, 1, 2, 2, 3};
-  const double dts[kEtdRats] = {dt, 0.5 * dt, dt, 0.5 * dt, dt, 0.5 * dt, dt};
   for (auto& t : c->etd_tabs) {
     DevBuf<double2>* e[] = {&t->G, &t->Wl, &t->Wr, &t->pa, &t->pg, &t->binv};
     for (auto* b : e) b->release();
   }
   c->etd_tabs.clear();
   c->etd_ready = false;
-  auto up = [](DevBuf<double2>& b, const std::vector<double>& v) -> hipError_t {
-    hipError_t e = b.alloc(std::max<size_t>(v.size() / 2, 1));
-    if (e != hipSuccess || v.empty()) return e;
-    return hipMemcpy(b.p, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice);
-  };
-  for (int r = 0; r < kEtdRats; ++r) {
-    td1c_ctx::EtdRat& R = c->etd[r];
-    double dfac, num[3];
-    std::complex<double> root[3];
-    etd_form(fam[r], dfac, num, root);
-    // the scaling of the gen_phi* loops: numerator by dt1 tunit^k, the
-    // denominator slopes by tunit dt
-    std::complex<double> ttmp = dt, cd0[3], cd1[3];
-    for (int k = 0; k < 3; ++k) {
-      R.cn[k] = d2(num[k] * ttmp);
-      ttmp *= tunit;
+  if (scheme == 1) {
+    const int fam[kEtdRats] = {0, 0, 1, 1, 2, 2, 3};
+    const double dts[kEtdRats] = {dt, 0.5 * dt, dt, 0.5 * dt, dt, 0.5 * dt, dt};
+    for (int r = 0; r < kEtdRats; ++r) {
+      double dfac, num[3];
+      std::complex<double> root[3];
+      etd_form(fam[r], dfac, num, root);
+      if (int rc = etd_set(c, r, dfac, num, 2, root, 3, dt, dts[r], thresh)) return rc;
     }
-    const std::complex<double> tfac = tunit * dts[r];
-    cd1[0] = dfac * tfac;
-    cd0[0] = -dfac * root[0];
-    for (int i = 1; i < 3; ++i) {
-      cd1[i] = tfac;
-      cd0[i] = -root[i];
-    }
-    R.dt = dts[r];
-    std::complex<double> den = 1.0;
-    for (int i = 0; i < 3; ++i) {
-      R.cd0[i] = d2(cd0[i]);
-      R.cd1[i] = d2(cd1[i]);
-      den *= cd0[i];
-    }
-    const std::complex<double> nl = num[0] * std::complex<double>(dt);
-    R.numer_limit = d2(nl);
-    R.denom_limit = d2(den);
-    R.coeff_limit = d2(nl / den);
-    for (int i = 0; i < 3; ++i) {
-      R.tab[i] = -1;
-      if (!(std::abs(cd1[i]) > thresh)) continue;  // h1rat_gen skips the factor
-      const std::complex<double> f = cd1[i] / cd0[i];
-      for (size_t t = 0; t < c->etd_tabs.size(); ++t)
-        if (c->etd_tabs[t]->fre == f.real() && c->etd_tabs[t]->fim == f.imag()) R.tab[i] = (int)t;
-      if (R.tab[i] >= 0) continue;
-      CnTables T;
-      std::string err;
-      if (!build_cn(S, c->v2jfc_h.empty() ? nullptr : c->v2jfc_h.data(), f.real(), f.imag(), T, err))
-        return fail(-5, err);
-      auto tab = std::make_unique<td1c_ctx::EtdTab>();
-      HIP_TRY(up(tab->G, T.G));
-      HIP_TRY(up(tab->Wl, T.Wl));
-      HIP_TRY(up(tab->Wr, T.Wr));
-      HIP_TRY(up(tab->pa, T.pa));
-      HIP_TRY(up(tab->pg, T.pg));
-      HIP_TRY(up(tab->binv, T.binv));
-      tab->K = T.K;
-      tab->fre = f.real();
-      tab->fim = f.imag();
-      R.tab[i] = (int)c->etd_tabs.size();
-      c->etd_tabs.push_back(std::move(tab));
-    }
+  } else {
+    const double dt2 = 0.5 * dt, s3 = std::sqrt(3.0);
+    using cd = std::complex<double>;
+    const double n_exp[3] = {12.0, 6.0, 1.0}, n_g[3] = {60.0, 6.0, 1.0}, n_f0[3] = {34860.0, 19620.0, 2657.0},
+                 n_f1[3] = {2940.0, 450.0, 41.0}, n_f2[3] = {1260.0, -420.0, -13.0};
+    const cd r_exp[2] = {cd(3.0, -s3), cd(3.0, s3)}, r_g[2] = {cd(4.0, -2.0), cd(4.0, 2.0)},
+             r_f0[2] = {cd(762.0, -std::sqrt(189762.0)) / 221.0, cd(762.0, std::sqrt(189762.0)) / 221.0},
+             r_f1[2] = {cd(51.0, -std::sqrt(633.0)) / 11.0, cd(51.0, std::sqrt(633.0)) / 11.0},
+             r_f2[2] = {cd(21.0, -3.0 * std::sqrt(21.0)) / 5.0, cd(21.0, 3.0 * std::sqrt(21.0)) / 5.0};
+    int rc = etd_set(c, kExp1, 1.0, n_exp, 2, r_exp, 2, 1.0, dt, thresh);
+    if (!rc) rc = etd_set(c, kExp2, 1.0, n_exp, 2, r_exp, 2, 1.0, dt2, thresh);
+    if (!rc) rc = etd_set(c, kGfun, 3.0, n_g, 2, r_g, 2, dt2, dt2, thresh);
+    if (!rc) rc = etd_set(c, kF0, 13260.0, n_f0, 2, r_f0, 2, dt, dt, thresh);
+    if (!rc) rc = etd_set(c, kF1, 330.0, n_f1, 2, r_f1, 2, dt, dt, thresh);
+    if (!rc) rc = etd_set(c, kF2, 300.0, n_f2, 2, r_f2, 2, dt, dt, thresh);
+    if (rc) return rc;
+    c->etd[kGfun].rk = make_double2(dt2, 0.0);
+    c->etd[kF0].rk = make_double2(dt / 6.0, 0.0);
+    c->etd[kF1].rk = make_double2(dt / 3.0, 0.0);
+    c->etd[kF2].rk = make_double2(dt / 6.0, 0.0);
   }
   if (int rc = cn_buffers(c)) return rc;
   const size_t so = (size_t)c->nbas * S.nfun, sz = so + S.lcic;
@@ -2418,13 +2461,16 @@ int etd_generate(td1c_ctx* c, double dt, int rosen, int maxcyc, double thresh) {
   c->etd_rosen = rosen ? 1 : 0;
   c->etd_maxcyc = maxcyc;
   c->etd_thresh = thresh;
+  c->etd_scheme = scheme;
   c->etd_ready = true;
   return 0;
 }
 
-// One cletdrb step (prop4_orb) on wfn with the field arrays d_lf[6][9] on
-// the device, then the mask of tdse_prop12 (src/main/tdse.cpp:268-309).
-int enqueue_etdrb_step(td1c_ctx* c, const double* d_lf, double2* wfn, int apply_mask, hipStream_t s) {
+// One step of the generated scheme on wfn with the field arrays d_lf[6][9]
+// on the device -- cletdrb::prop4_orb (src/prop/cletdrb.cpp:321-434) or
+// cletdrk4::prop (src/prop/cletdrk4.cpp:69-114) -- then the mask of
+// tdse_prop12 (src/main/tdse.cpp:268-309).
+int enqueue_etd_step(td1c_ctx* c, const double* d_lf, double2* wfn, int apply_mask, hipStream_t s) {
   Setup& S = c->S;
   const size_t nb = c->nbas, so = nb * S.nfun, sz = so + S.lcic, dof = nb * S.nfcore;
   const int nd = S.nfun - S.nfcore;
@@ -2438,6 +2484,9 @@ int enqueue_etdrb_step(td1c_ctx* c, const double* d_lf, double2* wfn, int apply_
   const double *lf0 = d_lf, *lf1 = d_lf + 9, *lf3 = d_lf + 18, *dlf0 = d_lf + 27, *dv1 = d_lf + 36, *dv3 = d_lf + 45;
   const double2 one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
   const double2 tunit = S.icomp == 0 ? make_double2(-1.0, 0.0) : make_double2(0.0, -1.0);
+  // the field of the rational functions: cletdrb's timeh = t (etd_time_of_h
+  // 0), cletdrk4's thalf = t + dt/2
+  const double* lfr = c->etd_scheme == 1 ? lf0 : lf1;
   const bool ob_el = S.n == 20;
   auto cpy = [&](double2* z, const double2* x, size_t n) { launch_zaxpby(n, one, x, zero, nullptr, z, s); };
   auto clr = [&](double2* z, size_t n) { launch_zaxpby(n, zero, nullptr, zero, nullptr, z, s); };
@@ -2465,26 +2514,26 @@ int enqueue_etdrb_step(td1c_ctx* c, const double* d_lf, double2* wfn, int apply_
   auto numer = [&](const td1c_ctx::EtdRat& R, const double2* in, double2* out) {
     launch_zaxpby(ndn, R.cn[0], in, zero, nullptr, out, s);
     const double2* src = in;
-    for (int k = 1; k <= 2; ++k) {
+    for (int k = 1; k <= R.dimn; ++k) {
       double2* t = tb[k & 1];
-      h1(src, t, make_double2(R.dt, 0.0), lf0, false);
+      h1(src, t, make_double2(R.dt, 0.0), lfr, false);
       axpy(out, R.cn[k], t, ndn);
       src = t;
     }
   };
   // the denominator factors of R on the dynamical rows of orb, in place
   auto denom = [&](const td1c_ctx::EtdRat& R) {
-    for (int i = 0; i < 3; ++i) {
+    for (int i = 0; i < R.dimd; ++i) {
       if (R.tab[i] < 0) continue;
       const td1c_ctx::EtdTab& T = *c->etd_tabs[R.tab[i]];
-      CnArgs a = cn_args(c, lf0, orb);
+      CnArgs a = cn_args(c, lfr, orb);
       a.G = T.G.p; a.Wl = T.Wl.p; a.Wr = T.Wr.p; a.pa = T.pa.p; a.pg = T.pg.p; a.binv = T.binv.p; a.K = T.K;
       a.gstride = S.nfe;
       const std::complex<double> c0(R.cd0[i].x, R.cd0[i].y), c1(R.cd1[i].x, R.cd1[i].y);
       a.zfac0 = d2(1.0 / c0);
       a.zfac1 = d2(-(c1 / c0));
       a.thresh = c->etd_thresh;
-      OneBodyElArgs ob = ob_args(lf0);
+      OneBodyElArgs ob = ob_args(lfr);
       ob.cv = a.zfac1;
       ob.stop = a.flag;
       const OneBodyElArgs* obp = ob_el ? &ob : nullptr;
@@ -2525,6 +2574,48 @@ int enqueue_etdrb_step(td1c_ctx* c, const double* d_lf, double2* wfn, int apply_
     return 0;
   };
   const auto& R = c->etd;
+  if (c->etd_scheme == 2) {
+    TIMED("etdrk4");
+    // cletdrk4::prod1 (:116-123): wout = win, its CI times rk, its
+    // dynamical orbitals through R (clh1rat::prod, clh1rat.cpp:49-65)
+    auto prod1 = [&](const td1c_ctx::EtdRat& Q, const double2* win, double2* wout) {
+      cpy(wout, win, dof);
+      launch_zaxpby(S.lcic, Q.rk, win + so, zero, nullptr, wout + so, s);
+      numer(Q, win + dof, orb + dof);
+      denom(Q);
+      cpy(wout + dof, orb + dof, ndn);
+    };
+    // cletdrk4::prod2 (:125-132): wout += win with its CI times rk and its
+    // dynamical orbitals through R (win's frozen-core rows are zero)
+    auto prod2 = [&](const td1c_ctx::EtdRat& Q, const double2* win, double2* wout) {
+      launch_zaxpby(S.lcic, Q.rk, win + so, one, wout + so, wout + so, s);
+      numer(Q, win + dof, orb + dof);
+      denom(Q);
+      launch_zaxpby(ndn, one, orb + dof, one, wout + dof, wout + dof, s);
+    };
+    cpy(W0, wfn, sz);
+    if (int rc = htot(lf0, W0, hW0)) return rc;
+    prod1(R[kGfun], hW0, W1);
+    prod1(R[kExp2], W0, tW);
+    launch_zaxpby(sz, one, tW, one, W1, W1, s);
+    if (int rc = htot(lf1, W1, hW1)) return rc;
+    prod1(R[kGfun], hW1, W2);
+    launch_zaxpby(sz, one, tW, one, W2, W2, s);
+    if (int rc = htot(lf1, W2, hW2)) return rc;
+    launch_zaxpby(sz, make_double2(2.0, 0.0), hW2, make_double2(-1.0, 0.0), hW0, tW, s);
+    prod1(R[kGfun], tW, W3);
+    prod1(R[kExp2], W1, tW);
+    launch_zaxpby(sz, one, tW, one, W3, W3, s);
+    if (int rc = htot(lf3, W3, hW3)) return rc;
+    launch_zaxpby(sz, one, hW1, one, hW2, hW2, s);
+    prod1(R[kExp1], W0, wfn);
+    prod2(R[kF0], hW0, wfn);
+    prod2(R[kF1], hW2, wfn);
+    prod2(R[kF2], hW3, wfn);
+    if (apply_mask) launch_mask(wfn, c->mask.p, S.nr, S.L1, S.nfcore, S.nfun, s);
+    HIP_TRY(hipGetLastError());
+    return 0;
+  }
   TIMED("etdrb");
   // stage 0: tW = -i h1 u0 + W0; u1, u3 from the phi1 numerators (and the
   // Rosenbrock term -i dt dV/dt u0)
@@ -2595,23 +2686,30 @@ int enqueue_etdrb_step(td1c_ctx* c, const double* d_lf, double2* wfn, int apply_
 
 int td1c_etdrb_gen(td1c_ctx* c, double dtime, int rosenbrock, int maxcyc, double thresh) {
   if (!c) return fail(-1, "null argument");
-  return etd_generate(c, dtime, rosenbrock, maxcyc, thresh);
+  return etd_generate(c, 1, dtime, rosenbrock, maxcyc, thresh);
 }
 
-int td1c_etdrb_prop(td1c_ctx* c, const double* lfields, double* d_wfn, int apply_mask, void* stream) {
+int td1c_etdrk4_gen(td1c_ctx* c, double dtime, int maxcyc, double thresh) {
+  if (!c) return fail(-1, "null argument");
+  return etd_generate(c, 2, dtime, 0, maxcyc, thresh);
+}
+
+static int etd_prop(td1c_ctx* c, int scheme, const double* lfields, double* d_wfn, int apply_mask, void* stream) {
   if (!c || !lfields || !d_wfn) return fail(-1, "null argument");
-  if (!c->etd_ready) return fail(-1, "call td1c_etdrb_gen first");
-  if (active_recorder()) return fail(-1, "etdrb steps are not batched");
+  if (!c->etd_ready || c->etd_scheme != scheme)
+    return fail(-1, scheme == 1 ? "call td1c_etdrb_gen first" : "call td1c_etdrk4_gen first");
+  if (active_recorder()) return fail(-1, "exponential steps are not batched");
   hipStream_t s = (hipStream_t)stream;
   HIP_TRY(hipMemcpyAsync(c->etd_lf.p, lfields, 54 * sizeof(double), hipMemcpyHostToDevice, s));
-  return enqueue_etdrb_step(c, c->etd_lf.p, reinterpret_cast<double2*>(d_wfn), apply_mask, s);
+  return enqueue_etd_step(c, c->etd_lf.p, reinterpret_cast<double2*>(d_wfn), apply_mask, s);
 }
 
-int td1c_etdrb_propagate(td1c_ctx* c, const td1c_field* field, double t0, double dtime, int nstep, double* d_wfn,
-                         int apply_mask, void* stream) {
+static int etd_propagate(td1c_ctx* c, int scheme, const td1c_field* field, double t0, double dtime, int nstep,
+                         double* d_wfn, int apply_mask, void* stream) {
   if (!c || !field || !d_wfn) return fail(-1, "null argument");
-  if (!c->etd_ready) return fail(-1, "call td1c_etdrb_gen first");
-  if (dtime != c->etd_dt) return fail(-1, "dtime differs from the one td1c_etdrb_gen was called with");
+  if (!c->etd_ready || c->etd_scheme != scheme)
+    return fail(-1, scheme == 1 ? "call td1c_etdrb_gen first" : "call td1c_etdrk4_gen first");
+  if (dtime != c->etd_dt) return fail(-1, "dtime differs from the one the tables were generated for");
   if (check_field(field)) return -1;
   if (nstep < 0) return fail(-1, "nstep must be >= 0");
   hipStream_t s = (hipStream_t)stream;
@@ -2619,10 +2717,28 @@ int td1c_etdrb_propagate(td1c_ctx* c, const td1c_field* field, double t0, double
   launch_clock_set(c->clock.p, t0, dtime, s);
   for (int i = 0; i < nstep; ++i) {
     launch_etd_fields(p, c->clock.p, c->etd_lf.p, s);
-    if (int rc = enqueue_etdrb_step(c, c->etd_lf.p, reinterpret_cast<double2*>(d_wfn), apply_mask, s)) return rc;
+    if (int rc = enqueue_etd_step(c, c->etd_lf.p, reinterpret_cast<double2*>(d_wfn), apply_mask, s)) return rc;
   }
   HIP_TRY(hipGetLastError());
   return 0;
+}
+
+int td1c_etdrb_prop(td1c_ctx* c, const double* lfields, double* d_wfn, int apply_mask, void* stream) {
+  return etd_prop(c, 1, lfields, d_wfn, apply_mask, stream);
+}
+
+int td1c_etdrk4_prop(td1c_ctx* c, const double* lfields, double* d_wfn, int apply_mask, void* stream) {
+  return etd_prop(c, 2, lfields, d_wfn, apply_mask, stream);
+}
+
+int td1c_etdrb_propagate(td1c_ctx* c, const td1c_field* field, double t0, double dtime, int nstep, double* d_wfn,
+                         int apply_mask, void* stream) {
+  return etd_propagate(c, 1, field, t0, dtime, nstep, d_wfn, apply_mask, stream);
+}
+
+int td1c_etdrk4_propagate(td1c_ctx* c, const td1c_field* field, double t0, double dtime, int nstep, double* d_wfn,
+                          int apply_mask, void* stream) {
+  return etd_propagate(c, 2, field, t0, dtime, nstep, d_wfn, apply_mask, stream);
 }
 
 // ---------------------------------------------------------------------------

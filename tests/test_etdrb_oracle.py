@@ -13,6 +13,11 @@ and of the field derivative it reads (clfield::get_der).
   order (He and a CAS(2e,2o) state under a sin^2 pulse, both gauges).  A
   dropped term, a wrong sign or a wrong coefficient leaves order <= 2.
 - The field derivative: central differences of td1c_field_value.
+- cletdrk4 (src/prop/cletdrk4.cpp, the [2/2] forms of its defaults): exp,
+  phi1 and the Cox-Matthews alpha, 2 beta, gamma forms at a scalar
+  eigenvalue to O(x^5) (a changed numerator coefficient fails here), and the
+  step's convergence to the full-hprod trajectory at fourth order (observed
+  3.85 over these steps; a swapped stage vector or weight leaves <= 1.2).
 """
 import math
 import os
@@ -124,3 +129,60 @@ def test_field_derivative_is_the_time_derivative(gauge):
         if gauge == "velocity1":
             assert d[2] == -td1c_amd.field_value(f, t)[5]  # dA/dt = -E of the same difference
     assert np.all(td1c_amd.field_der(f, -1.0) == 0) and np.all(td1c_amd.field_der(f, 3 * per + 1.0) == 0)
+
+
+def _cm(x, which):
+    e = np.exp(x)
+    if which == "alpha":
+        return (-4 - x + e * (4 - 3 * x + x * x)) / x ** 3
+    if which == "beta2":
+        return 2 * (2 + x + e * (-2 + x)) / x ** 3
+    return (-4 - 3 * x - x * x + e * (4 - x)) / x ** 3
+
+
+@pytest.mark.parametrize("icomp", [1, 0])
+def test_etdrk4_forms(icomp):
+    dt = 0.05
+    o = _oracle(_prob("velocity1", 1, icomp=icomp))
+    o.etdrk4_gen(dt)
+    tunit = -1j if icomp == 1 else -1.0
+    forms = {"exp1": (1.0, dt, lambda x: np.exp(x)), "exp2": (1.0, dt / 2, lambda x: np.exp(x)),
+             "gfun": (dt / 2, dt / 2, lambda x: (np.exp(x) - 1) / x),
+             "f0fun": (dt, dt, lambda x: _cm(x, "alpha")), "f1fun": (dt, dt, lambda x: _cm(x, "beta2")),
+             "f2fun": (dt, dt, lambda x: _cm(x, "gamma"))}
+    for name, (scale, h, fn) in forms.items():
+        errs = []
+        for lam in (8.0, 4.0, 2.0):
+            x = tunit * h * lam
+            errs.append(abs(o.etd_rat_scalar(name, complex(lam)) - scale * fn(x)))
+        for a, b in zip(errs, errs[1:]):
+            assert math.log2(a / b) > 4.7, (name, errs)
+
+
+@pytest.mark.parametrize("cas", [False, True])
+@pytest.mark.parametrize("gauge", ["velocity1", "length1"])
+def test_etdrk4_converges_at_fourth_order(gauge, cas):
+    from td1c_amd import CRK
+
+    p1 = _prob(gauge, 1, cas)
+    o1 = _oracle(p1)
+    w0 = W.bound_wfn(p1, 7, o1.lcic)
+    f = td1c_amd.Field(env_type="sin2", gauge=gauge, fint=2e14, wlen=400.0, cep=30.0, numcyc=2.0)
+    o0 = _oracle(_prob(gauge, 0, cas))
+    o0.set_wfn0(w0)
+    rk = CRK(4)
+    ref, t, h = w0.copy(), 10.0, 0.0025
+    for _ in range(160):
+        ref = o0.rk_prop(4, h, rk.stage_fields(f, t, h), ref, apply_mask=False)
+        t += h
+    o1.set_wfn0(w0)
+    errs = []
+    for dt in (0.1, 0.05, 0.025):
+        o1.etdrk4_gen(dt)
+        w, t = w0.copy(), 10.0
+        for _ in range(int(round(0.4 / dt))):
+            w = o1.etdrk4_step(td1c_amd.etd_fields(f, t, dt), w, apply_mask=False)
+            t += dt
+        errs.append(np.linalg.norm(w - ref))
+    orders = [math.log2(a / b) for a, b in zip(errs, errs[1:])]
+    assert orders[-1] > 3.6 and orders[0] > 3.0, (errs, orders)

@@ -126,3 +126,43 @@ def test_etdrb_rejects_what_is_not_provided(gpu_lib):
     assert rc != 0  # dtime other than the generated one
     del etd
     hp.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("name,igauge", [("he_small", 0), ("cas_fc_small", 1), ("dc3_small", 1), ("he_n16", 1)])
+def test_etdrk4_steps_match_oracle(gpu_lib, name, igauge, precision):
+    """cletdrk4: two steps against orc_etdrk4_step (FP64 1e-10, mixed 1e-6)."""
+    import torch
+    import td1c_amd
+    from oracle.oracle import Oracle
+
+    prob = getattr(C, name)(isplit=1, igauge=igauge, precision=precision)
+    o = Oracle(prob)
+    w = C.smooth_wfn_for(prob, o.lcic)
+    o.set_wfn0(w)
+    o.etdrk4_gen(DT)
+    hp = td1c_amd.HProd(prob)
+    hp.set_wfn0(w)
+    etk = td1c_amd.ETDRK4(hp, DT)
+    f = _field(igauge)
+    d = hp.to_device(w)
+    ref = w.copy()
+    for k in range(2):
+        lf = td1c_amd.etd_fields(f, 30.0 + k * DT, DT)
+        etk.prop(d, lfields=lf)
+        ref = o.etdrk4_step(lf, ref, apply_mask=True)
+    torch.cuda.synchronize()
+    got = d.cpu().numpy()
+    tol = 1e-10 if precision == 0 else 1e-6
+    assert _rel(got, ref) < tol, _rel(got, ref)
+    assert _rel(got, w) > 1e-5
+    # the same steps with the fields on the device
+    e = hp.to_device(w)
+    etk.propagate(e, 30.0, 2, f)
+    torch.cuda.synchronize()
+    assert _rel(e.cpu().numpy(), got) < 1e-12
+    # the etdrb entry points refuse the etdrk4 tables
+    with pytest.raises(RuntimeError):
+        td1c_amd.ETDRB.prop(type("E", (), {"hprod": hp, "dtime": DT, "_fn": "etdrb"})(), e,
+                            lfields=td1c_amd.etd_fields(f, 30.0, DT))
+    hp.close()
