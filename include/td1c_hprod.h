@@ -315,6 +315,21 @@ int td1c_etdrk4_prop(td1c_ctx* ctx, const double* lfields, double* d_wfn, int ap
 int td1c_etdrk4_propagate(td1c_ctx* ctx, const td1c_field* field, double t0, double dtime, int nstep,
                           double* d_wfn, int apply_mask, void* stream);
 
+/* cllawrk (src/prop/cllawrk.cpp, prop_type "lawrk"): Lawson RK4 (prop4,
+ * :188-280) with exp(-i h1 dt/2) in the (1,1) Pade form of cldpade
+ * (lawrk_npade 11, h1itr_midpt true: expwfn11, :296-324): out = 2 (1 + i T
+ * dt/4)^-1 in - in on the orbitals, the inverse with the Neumann series in V
+ * of dpade_prod (src/prop/dpade_prod.F90; at most maxcyc terms, RMS
+ * threshold thresh: h1itr_maxcyc, h1itr_thresh), the field at t + dt/2.
+ * The factor is the one td1c_h1itr_gen builds for this dt.  split_type = 1
+ * and lconst = 0; lfields as for td1c_etdrb_prop (lfield(t), lfield(t +
+ * dt/2), lfield(t + dt) are read).  A context holds one exponential scheme at
+ * a time. */
+int td1c_lawrk_gen(td1c_ctx* ctx, double dtime, int maxcyc, double thresh);
+int td1c_lawrk_prop(td1c_ctx* ctx, const double* lfields, double* d_wfn, int apply_mask, void* stream);
+int td1c_lawrk_propagate(td1c_ctx* ctx, const td1c_field* field, double t0, double dtime, int nstep,
+                         double* d_wfn, int apply_mask, void* stream);
+
 /* Grid projection of a state between two problems, host memory only (no GPU
  * call): clbas::proj (src/bas/clbas.cpp:280-332) with the CI copy of
  * guess_type = "read_proj" (src/main/guess.cpp:37-47).  Each interior point r

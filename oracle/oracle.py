@@ -85,6 +85,10 @@ def lib(openmp: bool = False):
         L.orc_etdrk4_gen.restype = ctypes.c_int
         L.orc_etdrk4_step.argtypes = [vp, vp, vp, ctypes.c_int]
         L.orc_etdrk4_step.restype = ctypes.c_int
+        L.orc_lawrk_gen.argtypes = [vp, ctypes.c_double, ctypes.c_int, ctypes.c_double]
+        L.orc_lawrk_gen.restype = ctypes.c_int
+        L.orc_lawrk_step.argtypes = [vp, vp, vp, ctypes.c_int]
+        L.orc_lawrk_step.restype = ctypes.c_int
         L.orc_band_chol.argtypes = [ctypes.c_int, ctypes.c_int, vp]
         L.orc_band_chol.restype = ctypes.c_int
         L.orc_band_chol_solve.argtypes = [ctypes.c_int, ctypes.c_int, vp, vp]
@@ -306,6 +310,22 @@ class Oracle:
         rc = self._L.orc_etdrk4_gen(self.h, dtime, maxcyc, thresh)
         if rc:
             raise RuntimeError(f"orc_etdrk4_gen failed ({rc})")
+
+    def lawrk_gen(self, dtime: float, maxcyc: int = 20, thresh: float = 1e-10):
+        """cllawrk::gen for step dtime (lawrk_order 4, lawrk_npade 11)."""
+        rc = self._L.orc_lawrk_gen(self.h, dtime, maxcyc, thresh)
+        if rc:
+            raise RuntimeError(f"orc_lawrk_gen failed ({rc})")
+
+    def lawrk_step(self, lfields: np.ndarray, wfn: np.ndarray, apply_mask: bool = True) -> np.ndarray:
+        """cllawrk::prop4 then the mask; lfields as for etdrb_step."""
+        w = np.array(wfn, dtype=np.complex128, copy=True)
+        lf = np.ascontiguousarray(lfields, dtype=np.float64)
+        assert lf.size == 54
+        rc = self._L.orc_lawrk_step(self.h, _p(lf), _p(w), int(apply_mask))
+        if rc:
+            raise RuntimeError(f"orc_lawrk_step failed ({rc})")
+        return w
 
     def etdrk4_step(self, lfields: np.ndarray, wfn: np.ndarray, apply_mask: bool = True) -> np.ndarray:
         """cletdrk4::prop then the mask; lfields as for etdrb_step."""

@@ -114,6 +114,8 @@ typedef struct orc {
   double etd_dt[8];                       /* clh1rat dtime of each */
   int etd_dimn[8], etd_dimd[8];
   zc etd_rk[8];                           /* cletdrk4's CI factors (exp1rk ... f2funrk) */
+  int law_ready, law_maxcyc;              /* cllawrk (h1itr_gen(dt/2) factors) */
+  double law_dt, law_thresh;
   zc etd_cn[8][4], etd_cd0[8][3], etd_cd1[8][3];
   zc etd_numer_limit[8], etd_denom_limit[8], etd_coeff_limit[8];
   zc *etd_inv[8][3];                      /* (3n+1)*nr*L1 band LU of cd0 + cd1 T_l */
@@ -2401,6 +2403,102 @@ int orc_etdrk4_step(orc* o, const double* lf, zc* wfn, int apply_mask) {
   etk_prod2(o, ETK_F0, lf1, hw0, wfn);
   etk_prod2(o, ETK_F1, lf1, hw2, wfn);
   etk_prod2(o, ETK_F2, lf1, hw3, wfn);
+  if (apply_mask) orc_mask(o, wfn);
+  free(buf);
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* cllawrk (src/prop/cllawrk.cpp): Lawson RK4 (prop4, :188-280) with    */
+/* the (1,1) Pade of cldpade for exp(-i h1 dt/2) (lawrk_npade 11,       */
+/* h1itr_midpt true: expwfn11, :296-324), the defaults of gen (:20-75). */
+/* ------------------------------------------------------------------ */
+/* cllawrk::gen: dPade2p.gen(dt/2, alpha 2) = dpade_gen's band LU of
+ * 1 + (i dt/4) T_l, the factor orc_h1itr_gen builds for its argument dt/2. */
+int orc_lawrk_gen(orc* o, double dtime, int maxcyc, double thresh) {
+  if (o->isplit != 1) return -2;
+  int rc = orc_h1itr_gen(o, 0.5 * dtime);
+  if (rc) return rc;
+  o->law_dt = dtime;
+  o->law_maxcyc = maxcyc;
+  o->law_thresh = thresh;
+  o->law_ready = 1;
+  return 0;
+}
+
+/* dpade_prod (src/prop/dpade_prod.F90) with dtime dt/2, alpha 2, split_type
+ * 1, iprojfc 2: x0 = (1 + i T dt/4)^-1 w, x_k = (1 + i T dt/4)^-1 (-i V dt/4)
+ * x_(k-1) while the RMS orbital norm of x_(k-1) >= thresh (at most maxcyc
+ * more terms), w(dyn) = sum x_k. */
+static void law_dpade(orc* o, const double* lf, zc* w) {
+  int nf = o->nfun, nfc = o->nfcore, nb = o->nbas;
+  size_t size1 = (size_t)nb * nf;
+  zc zfac = (o->icomp == 1 ? -I : (zc)-1.0) * (0.5 * o->law_dt) / 2.0 * lf[2];
+  zc* orb = xcalloc(size1, sizeof(zc));
+  zc* horb = xcalloc(size1, sizeof(zc));
+  zc* torb = xcalloc(size1, sizeof(zc));
+  memcpy(orb, w, size1 * sizeof(zc));
+  h1itr_cninv(o, orb);
+  memcpy(torb, orb, size1 * sizeof(zc));
+  for (int icyc = 1; icyc <= o->law_maxcyc; ++icyc) {
+    double rms = 0.0;
+    for (int ifun = nfc; ifun < nf; ++ifun) {
+      const zc* x = orb + (size_t)ifun * nb;
+      for (int k = 0; k < nb; ++k) rms += creal(x[k]) * creal(x[k]) + cimag(x[k]) * cimag(x[k]);
+    }
+    rms = sqrt(rms / nf);
+    if (rms < o->law_thresh) break;
+    memset(horb, 0, size1 * sizeof(zc));
+    if (o->igauge == 0)
+      zprod(o, zfac, orb, horb, nfc, nf, o->nr);
+    else
+      pzprod(o, zfac, orb, horb, nfc, nf, o->nr);
+    memcpy(orb, horb, size1 * sizeof(zc));
+    h1itr_cninv(o, orb);
+    for (size_t k = 0; k < size1; ++k) torb[k] += orb[k];
+  }
+  memcpy(w + (size_t)nb * nfc, torb + (size_t)nb * nfc, sizeof(zc) * (size1 - (size_t)nb * nfc));
+  free(orb);
+  free(horb);
+  free(torb);
+}
+
+/* cllawrk::expwfn11: out = in, the Pade factor on out's orbitals with the
+ * field at t + dt/2, then out.orb = -in.orb + 2 out.orb (the Cayley form of
+ * exp(-i h1 dt/2)); the CI part is copied. */
+static void law_expwfn(orc* o, const double* lf_half, const zc* in, zc* out) {
+  size_t size1 = (size_t)o->nbas * o->nfun, sz = size1 + o->lcic;
+  memcpy(out, in, sizeof(zc) * sz);
+  law_dpade(o, lf_half, out);
+  for (size_t k = 0; k < size1; ++k) out[k] = -1.0 * in[k] + 2.0 * out[k];
+}
+
+/* cllawrk::prop4, then the mask of tdse_prop12; lf as for orc_etdrb_step
+ * (this scheme reads lfield(t), lfield(t + dt/2), lfield(t + dt)). */
+int orc_lawrk_step(orc* o, const double* lf, zc* wfn, int apply_mask) {
+  if (!o->law_ready) return -1;
+  size_t size1 = (size_t)o->nbas * o->nfun, sz = size1 + o->lcic;
+  const double *lf0 = lf, *lf1 = lf + 9, *lf3 = lf + 18;
+  double dt = o->law_dt;
+  zc cdt1 = dt, cdt2 = dt / 2.0, cdt3 = dt / 3.0, cdt6 = dt / 6.0;
+  zc* buf = xcalloc(sz * 4, sizeof(zc));
+  zc *w0 = buf, *w1 = buf + sz, *hw0 = buf + 2 * sz, *hw1 = buf + 3 * sz;
+  memcpy(w0, wfn, sizeof(zc) * sz);
+  orc_htot(o, 1.0, lf0, w0, w0 + size1, hw0, hw0 + size1);
+  law_expwfn(o, lf1, w0, w1);
+  law_expwfn(o, lf1, hw0, hw1);
+  for (size_t k = 0; k < sz; ++k) wfn[k] = cdt6 * hw1[k] + w1[k];
+  for (size_t k = 0; k < sz; ++k) w0[k] = cdt2 * hw1[k] + w1[k];
+  orc_htot(o, 1.0, lf1, w0, w0 + size1, hw0, hw0 + size1);
+  for (size_t k = 0; k < sz; ++k) wfn[k] += cdt3 * hw0[k];
+  for (size_t k = 0; k < sz; ++k) w0[k] = cdt2 * hw0[k] + w1[k];
+  orc_htot(o, 1.0, lf1, w0, w0 + size1, hw0, hw0 + size1);
+  for (size_t k = 0; k < sz; ++k) wfn[k] += cdt3 * hw0[k];
+  for (size_t k = 0; k < sz; ++k) w1[k] += cdt1 * hw0[k];
+  law_expwfn(o, lf1, w1, w0);
+  orc_htot(o, 1.0, lf3, w0, w0 + size1, hw0, hw0 + size1);
+  law_expwfn(o, lf1, wfn, w1);
+  for (size_t k = 0; k < sz; ++k) wfn[k] = cdt6 * hw0[k] + w1[k];
   if (apply_mask) orc_mask(o, wfn);
   free(buf);
   return 0;

@@ -76,6 +76,13 @@ def load_library():
     L.td1c_etdrk4_propagate.argtypes = [vp, ctypes.POINTER(TD1CField), ctypes.c_double, ctypes.c_double,
                                         ctypes.c_int, vp, ctypes.c_int, vp]
     L.td1c_etdrk4_propagate.restype = ctypes.c_int
+    L.td1c_lawrk_gen.argtypes = [vp, ctypes.c_double, ctypes.c_int, ctypes.c_double]
+    L.td1c_lawrk_gen.restype = ctypes.c_int
+    L.td1c_lawrk_prop.argtypes = [vp, vp, vp, ctypes.c_int, vp]
+    L.td1c_lawrk_prop.restype = ctypes.c_int
+    L.td1c_lawrk_propagate.argtypes = [vp, ctypes.POINTER(TD1CField), ctypes.c_double, ctypes.c_double,
+                                       ctypes.c_int, vp, ctypes.c_int, vp]
+    L.td1c_lawrk_propagate.restype = ctypes.c_int
     L.td1c_set_graph.argtypes = [vp, ctypes.c_int]
     L.td1c_set_graph.restype = ctypes.c_int
     L.td1c_set_transform_path.argtypes = [vp, ctypes.c_int]
@@ -649,6 +656,20 @@ class ETDRK4(ETDRB):
     def _gen(self, rosenbrock, maxcyc, thresh):
         _check(load_library().td1c_etdrk4_gen(self.hprod.h, self.dtime, int(maxcyc), float(thresh)),
                "td1c_etdrk4_gen")
+
+
+class LAWRK(ETDRB):
+    """Lawson RK4 with the (1,1) Pade of exp(-i h1 dt/2) (cllawrk,
+    prop_type "lawrk"; td1c_lawrk_*).  Needs split_type = 1."""
+
+    _fn = "lawrk"
+
+    def __init__(self, hprod: "HProd", dtime: float, maxcyc: int = 20, thresh: float = 1e-10):
+        super().__init__(hprod, dtime, False, maxcyc, thresh)
+
+    def _gen(self, rosenbrock, maxcyc, thresh):
+        _check(load_library().td1c_lawrk_gen(self.hprod.h, self.dtime, int(maxcyc), float(thresh)),
+               "td1c_lawrk_gen")
 
 
 class Batch:

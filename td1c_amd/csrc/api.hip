@@ -2400,7 +2400,7 @@ int etd_set(td1c_ctx* c, int r, double dfac, const double* num, int dimn, const 
 // gen_gfun, gen_f0fun, gen_f1fun, gen_f2fun (:134-933), the [2/2] forms.
 int etd_generate(td1c_ctx* c, int scheme, double dt, int rosen, int maxcyc, double thresh) {
   Setup& S = c->S;
-  const char* name = scheme == 1 ? "etdrb" : "etdrk4";
+  const char* name = scheme == 1 ? "etdrb" : (scheme == 2 ? "etdrk4" : "lawrk");
   if (S.isplit != 1) return fail(-1, std::string(name) + " needs split_type = 1 (the split hprod)");
   if (S.lconst != 0) return fail(-1, std::string(name) + " with lconst is not provided");
   if (S.nfcore > 0 && !c->wfn0_set) return fail(-4, "frozen core needs td1c_set_wfn0 first");
@@ -2422,6 +2422,10 @@ int etd_generate(td1c_ctx* c, int scheme, double dt, int rosen, int maxcyc, doub
       etd_form(fam[r], dfac, num, root);
       if (int rc = etd_set(c, r, dfac, num, 2, root, 3, dt, dts[r], thresh)) return rc;
     }
+  } else if (scheme == 3) {
+    // cllawrk::gen: dPade2p.gen(dt/2, alpha 2) is dpade_gen's 1 + (i dt/4) T,
+    // the half step's factor for this dt
+    if (int rc = cn_prepare(c, dt)) return rc;
   } else {
     const double dt2 = 0.5 * dt, s3 = std::sqrt(3.0);
     using cd = std::complex<double>;
@@ -2574,6 +2578,48 @@ int enqueue_etd_step(td1c_ctx* c, const double* d_lf, double2* wfn, int apply_ma
     return 0;
   };
   const auto& R = c->etd;
+  if (c->etd_scheme == 3) {
+    TIMED("lawrk");
+    // cllawrk::expwfn11 (:296-324): out = in, dpade_prod on out's dynamical
+    // orbitals with the field at t + dt/2 -- x0 = (1 + i T dt/4)^-1 out, then
+    // the Neumann series in V (dpade_prod.F90) -- and out.orb = -in.orb + 2 out.orb
+    auto expwfn = [&](const double2* in, double2* out) {
+      cpy(out, in, sz);
+      CnArgs a = cn_args(c, lf1, out);
+      a.zfac0 = one;
+      a.thresh = c->etd_thresh;
+      OneBodyElArgs ob = ob_args(lf1);
+      ob.cv = a.zfac1;
+      ob.stop = a.flag;
+      const OneBodyElArgs* obp = ob_el ? &ob : nullptr;
+      launch_cn_stage(a, obp, c->ob_shape, c->ob_lc, 2, s);
+      for (int k = 0; k < c->etd_maxcyc; ++k) launch_cn_stage(a, obp, c->ob_shape, c->ob_lc, 1, s);
+      launch_zaxpby(so, make_double2(-1.0, 0.0), in, make_double2(2.0, 0.0), out, out, s);
+    };
+    const double dt = c->etd_dt;
+    const double2 cdt1 = make_double2(dt, 0.0), cdt2 = make_double2(dt / 2.0, 0.0),
+                  cdt3 = make_double2(dt / 3.0, 0.0), cdt6 = make_double2(dt / 6.0, 0.0);
+    // cllawrk::prop4 (:188-280): W1 = e u0, hW1 = e N0 (e = exp(-i h1 dt/2))
+    cpy(W0, wfn, sz);
+    if (int rc = htot(lf0, W0, hW0)) return rc;
+    expwfn(W0, W1);
+    expwfn(hW0, hW1);
+    launch_zaxpby(sz, cdt6, hW1, one, W1, wfn, s);
+    launch_zaxpby(sz, cdt2, hW1, one, W1, W0, s);
+    if (int rc = htot(lf1, W0, hW0)) return rc;
+    axpy(wfn, cdt3, hW0, sz);
+    launch_zaxpby(sz, cdt2, hW0, one, W1, W0, s);
+    if (int rc = htot(lf1, W0, hW0)) return rc;
+    axpy(wfn, cdt3, hW0, sz);
+    axpy(W1, cdt1, hW0, sz);
+    expwfn(W1, W0);
+    if (int rc = htot(lf3, W0, hW0)) return rc;
+    expwfn(wfn, W1);
+    launch_zaxpby(sz, cdt6, hW0, one, W1, wfn, s);
+    if (apply_mask) launch_mask(wfn, c->mask.p, S.nr, S.L1, S.nfcore, S.nfun, s);
+    HIP_TRY(hipGetLastError());
+    return 0;
+  }
   if (c->etd_scheme == 2) {
     TIMED("etdrk4");
     // cletdrk4::prod1 (:116-123): wout = win, its CI times rk, its
@@ -2694,11 +2740,21 @@ int td1c_etdrk4_gen(td1c_ctx* c, double dtime, int maxcyc, double thresh) {
   return etd_generate(c, 2, dtime, 0, maxcyc, thresh);
 }
 
+int td1c_lawrk_gen(td1c_ctx* c, double dtime, int maxcyc, double thresh) {
+  if (!c) return fail(-1, "null argument");
+  return etd_generate(c, 3, dtime, 0, maxcyc, thresh);
+}
+
+static const char* etd_gen_name(int scheme) {
+  return scheme == 1 ? "call td1c_etdrb_gen first" : (scheme == 2 ? "call td1c_etdrk4_gen first" : "call td1c_lawrk_gen first");
+}
+
 static int etd_prop(td1c_ctx* c, int scheme, const double* lfields, double* d_wfn, int apply_mask, void* stream) {
   if (!c || !lfields || !d_wfn) return fail(-1, "null argument");
-  if (!c->etd_ready || c->etd_scheme != scheme)
-    return fail(-1, scheme == 1 ? "call td1c_etdrb_gen first" : "call td1c_etdrk4_gen first");
+  if (!c->etd_ready || c->etd_scheme != scheme) return fail(-1, etd_gen_name(scheme));
   if (active_recorder()) return fail(-1, "exponential steps are not batched");
+  if (scheme == 3)  // the half-step factors, in case a split2 call of another dt replaced them
+    if (int rc = cn_prepare(c, c->etd_dt)) return rc;
   hipStream_t s = (hipStream_t)stream;
   HIP_TRY(hipMemcpyAsync(c->etd_lf.p, lfields, 54 * sizeof(double), hipMemcpyHostToDevice, s));
   return enqueue_etd_step(c, c->etd_lf.p, reinterpret_cast<double2*>(d_wfn), apply_mask, s);
@@ -2707,9 +2763,10 @@ static int etd_prop(td1c_ctx* c, int scheme, const double* lfields, double* d_wf
 static int etd_propagate(td1c_ctx* c, int scheme, const td1c_field* field, double t0, double dtime, int nstep,
                          double* d_wfn, int apply_mask, void* stream) {
   if (!c || !field || !d_wfn) return fail(-1, "null argument");
-  if (!c->etd_ready || c->etd_scheme != scheme)
-    return fail(-1, scheme == 1 ? "call td1c_etdrb_gen first" : "call td1c_etdrk4_gen first");
+  if (!c->etd_ready || c->etd_scheme != scheme) return fail(-1, etd_gen_name(scheme));
   if (dtime != c->etd_dt) return fail(-1, "dtime differs from the one the tables were generated for");
+  if (scheme == 3)
+    if (int rc = cn_prepare(c, c->etd_dt)) return rc;
   if (check_field(field)) return -1;
   if (nstep < 0) return fail(-1, "nstep must be >= 0");
   hipStream_t s = (hipStream_t)stream;
@@ -2729,6 +2786,15 @@ int td1c_etdrb_prop(td1c_ctx* c, const double* lfields, double* d_wfn, int apply
 
 int td1c_etdrk4_prop(td1c_ctx* c, const double* lfields, double* d_wfn, int apply_mask, void* stream) {
   return etd_prop(c, 2, lfields, d_wfn, apply_mask, stream);
+}
+
+int td1c_lawrk_prop(td1c_ctx* c, const double* lfields, double* d_wfn, int apply_mask, void* stream) {
+  return etd_prop(c, 3, lfields, d_wfn, apply_mask, stream);
+}
+
+int td1c_lawrk_propagate(td1c_ctx* c, const td1c_field* field, double t0, double dtime, int nstep, double* d_wfn,
+                         int apply_mask, void* stream) {
+  return etd_propagate(c, 3, field, t0, dtime, nstep, d_wfn, apply_mask, stream);
 }
 
 int td1c_etdrb_propagate(td1c_ctx* c, const td1c_field* field, double t0, double dtime, int nstep, double* d_wfn,

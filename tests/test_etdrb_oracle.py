@@ -18,6 +18,10 @@ and of the field derivative it reads (clfield::get_der).
   eigenvalue to O(x^5) (a changed numerator coefficient fails here), and the
   step's convergence to the full-hprod trajectory at fourth order (observed
   3.85 over these steps; a swapped stage vector or weight leaves <= 1.2).
+- cllawrk (src/prop/cllawrk.cpp, lawrk_npade 11): second order, the order of
+  its (1,1) Pade exponential (observed 1.91; a wrong stage weight, a skipped
+  exponential, the Pade's dt/4 or its 2x - x recombination changed leave
+  <= 1.4).
 """
 import math
 import os
@@ -186,3 +190,32 @@ def test_etdrk4_converges_at_fourth_order(gauge, cas):
         errs.append(np.linalg.norm(w - ref))
     orders = [math.log2(a / b) for a, b in zip(errs, errs[1:])]
     assert orders[-1] > 3.6 and orders[0] > 3.0, (errs, orders)
+
+
+@pytest.mark.parametrize("cas", [False, True])
+@pytest.mark.parametrize("gauge", ["velocity1", "length1"])
+def test_lawrk_converges_at_second_order(gauge, cas):
+    from td1c_amd import CRK
+
+    p1 = _prob(gauge, 1, cas)
+    o1 = _oracle(p1)
+    w0 = W.bound_wfn(p1, 7, o1.lcic)
+    f = td1c_amd.Field(env_type="sin2", gauge=gauge, fint=2e14, wlen=400.0, cep=30.0, numcyc=2.0)
+    o0 = _oracle(_prob(gauge, 0, cas))
+    o0.set_wfn0(w0)
+    rk = CRK(4)
+    ref, t, h = w0.copy(), 10.0, 0.0025
+    for _ in range(160):
+        ref = o0.rk_prop(4, h, rk.stage_fields(f, t, h), ref, apply_mask=False)
+        t += h
+    o1.set_wfn0(w0)
+    errs = []
+    for dt in (0.1, 0.05, 0.025):
+        o1.lawrk_gen(dt)
+        w, t = w0.copy(), 10.0
+        for _ in range(int(round(0.4 / dt))):
+            w = o1.lawrk_step(td1c_amd.etd_fields(f, t, dt), w, apply_mask=False)
+            t += dt
+        errs.append(np.linalg.norm(w - ref))
+    orders = [math.log2(a / b) for a, b in zip(errs, errs[1:])]
+    assert orders[-1] > 1.8 and errs[-1] < 1.2e-3, (errs, orders)

@@ -166,3 +166,41 @@ def test_etdrk4_steps_match_oracle(gpu_lib, name, igauge, precision):
         td1c_amd.ETDRB.prop(type("E", (), {"hprod": hp, "dtime": DT, "_fn": "etdrb"})(), e,
                             lfields=td1c_amd.etd_fields(f, 30.0, DT))
     hp.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("name,igauge", [("he_small", 0), ("cas_fc_small", 1), ("ne_small", 1), ("he_n16", 1)])
+def test_lawrk_steps_match_oracle(gpu_lib, name, igauge, precision):
+    """cllawrk: two steps against orc_lawrk_step (FP64 1e-10, mixed 1e-6), the
+    device-field propagation equal to them, and the factors rebuilt after a
+    split2 call of another dt replaced them."""
+    import torch
+    import td1c_amd
+    from oracle.oracle import Oracle
+
+    prob = getattr(C, name)(isplit=1, igauge=igauge, precision=precision)
+    o = Oracle(prob)
+    w = C.smooth_wfn_for(prob, o.lcic)
+    o.set_wfn0(w)
+    o.lawrk_gen(DT)
+    hp = td1c_amd.HProd(prob)
+    hp.set_wfn0(w)
+    law = td1c_amd.LAWRK(hp, DT)
+    f = _field(igauge)
+    d = hp.to_device(w)
+    ref = w.copy()
+    for k in range(2):
+        lf = td1c_amd.etd_fields(f, 30.0 + k * DT, DT)
+        law.prop(d, lfields=lf)
+        ref = o.lawrk_step(lf, ref, apply_mask=True)
+    torch.cuda.synchronize()
+    got = d.cpu().numpy()
+    tol = 1e-10 if precision == 0 else 1e-6
+    assert _rel(got, ref) < tol, _rel(got, ref)
+    assert _rel(got, w) > 1e-5
+    td1c_amd.CRK(4).propagate(hp, hp.to_device(w), 0.0, 2 * DT, 1, f)  # split2 factors of 2 dt
+    e = hp.to_device(w)
+    law.propagate(e, 30.0, 2, f)
+    torch.cuda.synchronize()
+    assert _rel(e.cpu().numpy(), got) < 1e-12
+    hp.close()
