@@ -330,6 +330,22 @@ int td1c_lawrk_prop(td1c_ctx* ctx, const double* lfields, double* d_wfn, int app
 int td1c_lawrk_propagate(td1c_ctx* ctx, const td1c_field* field, double t0, double dtime, int nstep,
                          double* d_wfn, int apply_mask, void* stream);
 
+/* clexpab (src/prop/clexpab.cpp, prop_type "expab"): the exponential
+ * Adams-Bashforth method of prop_general (:370-496) with gen's defaults
+ * (:25-110): order K = 4, the [1/2] form of phi4 consistent with the (1,2)
+ * Pade of exp (gen_phi4, :1148-1354), no corrector, h1type -1.  One split
+ * hprod per step; the history of the last K states and derivatives (filled
+ * with the start state on the first step after td1c_expab_gen) lives in the
+ * context, so consecutive calls continue one propagation.
+ * td1c_expab_prop: one step at time t with lfields[4][9] = lfield(t), then
+ * lfield at the history times timeP[1..3] (t_(n-k) for the n-th step since
+ * gen, t_0 while n < k).  td1c_expab_propagate evaluates them itself. */
+int td1c_expab_gen(td1c_ctx* ctx, double dtime, int maxcyc, double thresh);
+int td1c_expab_prop(td1c_ctx* ctx, double t, const double* lfields, double* d_wfn, int apply_mask,
+                    void* stream);
+int td1c_expab_propagate(td1c_ctx* ctx, const td1c_field* field, double t0, double dtime, int nstep,
+                         double* d_wfn, int apply_mask, void* stream);
+
 /* Grid projection of a state between two problems, host memory only (no GPU
  * call): clbas::proj (src/bas/clbas.cpp:280-332) with the CI copy of
  * guess_type = "read_proj" (src/main/guess.cpp:37-47).  Each interior point r

@@ -89,6 +89,10 @@ def lib(openmp: bool = False):
         L.orc_lawrk_gen.restype = ctypes.c_int
         L.orc_lawrk_step.argtypes = [vp, vp, vp, ctypes.c_int]
         L.orc_lawrk_step.restype = ctypes.c_int
+        L.orc_expab_gen.argtypes = [vp, ctypes.c_double, ctypes.c_int, ctypes.c_double]
+        L.orc_expab_gen.restype = ctypes.c_int
+        L.orc_expab_step.argtypes = [vp, vp, vp, ctypes.c_int]
+        L.orc_expab_step.restype = ctypes.c_int
         L.orc_band_chol.argtypes = [ctypes.c_int, ctypes.c_int, vp]
         L.orc_band_chol.restype = ctypes.c_int
         L.orc_band_chol_solve.argtypes = [ctypes.c_int, ctypes.c_int, vp, vp]
@@ -301,7 +305,10 @@ class Oracle:
         """A rational function of the last etdrb / etdrk4 gen at a scalar
         eigenvalue lam of h1."""
         out = np.zeros(2)
-        idx = self.ETD_RATS.index(name) if name in self.ETD_RATS else self.ETK_RATS.index(name)
+        if name == "expab_phi":  # clexpab's form, slot 7
+            idx = 7
+        else:
+            idx = self.ETD_RATS.index(name) if name in self.ETD_RATS else self.ETK_RATS.index(name)
         self._L.orc_etd_rat_scalar(self.h, idx, lam.real, lam.imag, _p(out))
         return complex(out[0], out[1])
 
@@ -325,6 +332,23 @@ class Oracle:
         rc = self._L.orc_lawrk_step(self.h, _p(lf), _p(w), int(apply_mask))
         if rc:
             raise RuntimeError(f"orc_lawrk_step failed ({rc})")
+        return w
+
+    def expab_gen(self, dtime: float, maxcyc: int = 20, thresh: float = 1e-10):
+        """clexpab::gen for step dtime (order 4, ndpade 12); restarts the history."""
+        rc = self._L.orc_expab_gen(self.h, dtime, maxcyc, thresh)
+        if rc:
+            raise RuntimeError(f"orc_expab_gen failed ({rc})")
+
+    def expab_step(self, lfields: np.ndarray, wfn: np.ndarray, apply_mask: bool = True) -> np.ndarray:
+        """clexpab::prop_general then the mask; lfields: lfield(t), then
+        lfield at the history times timeP[1..3] (4 x 9)."""
+        w = np.array(wfn, dtype=np.complex128, copy=True)
+        lf = np.ascontiguousarray(lfields, dtype=np.float64)
+        assert lf.size == 36
+        rc = self._L.orc_expab_step(self.h, _p(lf), _p(w), int(apply_mask))
+        if rc:
+            raise RuntimeError(f"orc_expab_step failed ({rc})")
         return w
 
     def etdrk4_step(self, lfields: np.ndarray, wfn: np.ndarray, apply_mask: bool = True) -> np.ndarray:

@@ -116,6 +116,9 @@ typedef struct orc {
   zc etd_rk[8];                           /* cletdrk4's CI factors (exp1rk ... f2funrk) */
   int law_ready, law_maxcyc;              /* cllawrk (h1itr_gen(dt/2) factors) */
   double law_dt, law_thresh;
+  int xab_ready, xab_nstep;               /* clexpab: steps taken since gen */
+  double xab_ainv[16];                    /* inverse Taylor matrix, column-major */
+  zc *xab_buf;                            /* 3 K + 1 states: WfnP, hWfnP, WfnB, tWfn */
   zc etd_cn[8][4], etd_cd0[8][3], etd_cd1[8][3];
   zc etd_numer_limit[8], etd_denom_limit[8], etd_coeff_limit[8];
   zc *etd_inv[8][3];                      /* (3n+1)*nr*L1 band LU of cd0 + cd1 T_l */
@@ -980,6 +983,7 @@ void orc_destroy(orc* o) {
       free(o->etd_inv[r][i]);
       free(o->etd_piv[r][i]);
     }
+  free(o->xab_buf);
   free(o);
 }
 
@@ -2501,6 +2505,138 @@ int orc_lawrk_step(orc* o, const double* lf, zc* wfn, int apply_mask) {
   for (size_t k = 0; k < sz; ++k) wfn[k] = cdt6 * hw0[k] + w1[k];
   if (apply_mask) orc_mask(o, wfn);
   free(buf);
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* clexpab (src/prop/clexpab.cpp): the exponential Adams-Bashforth      */
+/* method of prop_general (:370-496) with gen's defaults (:25-110):     */
+/* expab_order 4, expab_ndpade 12 (the [1/2] form of gen_phi4,           */
+/* :1148-1354), expab_pc false, expab_h1type -1.                         */
+/* ------------------------------------------------------------------ */
+#define XAB_K 4
+#define XAB_RAT 7 /* its rational function's slot */
+
+/* A^-1 of the n x n Taylor matrix A(i, j) = (-i)^j / j!, A(i, 0) = 1,
+ * A(0, j >= 1) = 0 (column-major, as prop_general builds it), by
+ * Gauss-Jordan with partial pivoting (the reference inverts it through
+ * util_gmatinv_real, src/util/util_gmatinv.F90:46-87). */
+static void xab_taylor_inverse(int n, double* ainv) {
+  double a[XAB_K * XAB_K], b[XAB_K * XAB_K];
+  for (int i = 0; i < n; ++i) a[i] = 1.0;
+  for (int j = 1; j < n; ++j) a[n * j] = 0.0;
+  double fact = 1.0;
+  for (int j = 1; j < n; ++j) {
+    fact *= j;
+    for (int i = 1; i < n; ++i) a[n * j + i] = pow(-i, j) / fact;
+  }
+  for (int k = 0; k < n * n; ++k) b[k] = 0.0;
+  for (int i = 0; i < n; ++i) b[n * i + i] = 1.0;
+  for (int c = 0; c < n; ++c) {
+    int pr = c;
+    for (int r = c + 1; r < n; ++r)
+      if (fabs(a[n * c + r]) > fabs(a[n * c + pr])) pr = r;
+    for (int j = 0; j < n; ++j) {
+      double t = a[n * j + c]; a[n * j + c] = a[n * j + pr]; a[n * j + pr] = t;
+      t = b[n * j + c]; b[n * j + c] = b[n * j + pr]; b[n * j + pr] = t;
+    }
+    double piv = a[n * c + c];
+    for (int j = 0; j < n; ++j) {
+      a[n * j + c] /= piv;
+      b[n * j + c] /= piv;
+    }
+    for (int r = 0; r < n; ++r) {
+      if (r == c) continue;
+      double f = a[n * c + r];
+      for (int j = 0; j < n; ++j) {
+        a[n * j + r] -= f * a[n * j + c];
+        b[n * j + r] -= f * b[n * j + c];
+      }
+    }
+  }
+  for (int k = 0; k < n * n; ++k) ainv[k] = b[k];
+}
+
+/* clexpab::gen: the [1/2] form of phi4 (dfac 6, numerator 1 - x, roots
+ * 2 -+ sqrt(2) i), numerator scaled by dt, denominator slopes by tunit dt;
+ * the history restarts. */
+int orc_expab_gen(orc* o, double dtime, int maxcyc, double thresh) {
+  if (o->isplit != 1) return -2;
+  o->etd_maxcyc = maxcyc;
+  o->etd_thresh = thresh;
+  const double nn[2] = {1.0, -1.0};
+  const zc roots[2] = {2.0 - 1.4142135623730950488 * I, 2.0 + 1.4142135623730950488 * I};
+  if (etd_rat_set(o, XAB_RAT, 6.0, nn, 1, roots, 2, dtime, dtime)) return -1;
+  xab_taylor_inverse(XAB_K, o->xab_ainv);
+  size_t sz = (size_t)o->nbas * o->nfun + o->lcic;
+  free(o->xab_buf);
+  o->xab_buf = xcalloc(sz * (3 * XAB_K + 1), sizeof(zc));
+  o->xab_nstep = 0;
+  o->law_dt = dtime;
+  o->xab_ready = 1;
+  return 0;
+}
+
+/* clexpab::prop_general, then the mask of tdse_prop12.  lf: lfield(t) then
+ * lfield(timeP[is]) for is = 1 .. K-1, the times of the history after this
+ * step's shift (t itself while the history is being filled: the first step
+ * copies the start state into every slot). */
+int orc_expab_step(orc* o, const double* lf, zc* wfn, int apply_mask) {
+  if (!o->xab_ready) return -1;
+  const int K = XAB_K, K1 = XAB_K - 1;
+  size_t size1 = (size_t)o->nbas * o->nfun, sz = size1 + o->lcic;
+  zc tunit = o->icomp == 0 ? -1.0 : -I;
+  double dt = o->law_dt;
+  zc *P[XAB_K], *hP[XAB_K], *B[XAB_K], *tw = o->xab_buf + (size_t)3 * K * sz;
+  for (int is = 0; is < K; ++is) {
+    P[is] = o->xab_buf + (size_t)is * sz;
+    hP[is] = o->xab_buf + (size_t)(K + is) * sz;
+    B[is] = o->xab_buf + (size_t)(2 * K + is) * sz;
+  }
+  if (o->xab_nstep == 0) {
+    memcpy(P[0], wfn, sizeof(zc) * sz);
+    orc_htot(o, 1.0, lf, wfn, wfn + size1, hP[0], hP[0] + size1);
+    for (int is = 1; is < K; ++is) {
+      memcpy(P[is], P[0], sizeof(zc) * sz);
+      memcpy(hP[is], hP[0], sizeof(zc) * sz);
+    }
+  } else {
+    for (int is = K1; is > 0; --is) {
+      memcpy(P[is], P[is - 1], sizeof(zc) * sz);
+      memcpy(hP[is], hP[is - 1], sizeof(zc) * sz);
+    }
+    memcpy(P[0], wfn, sizeof(zc) * sz);
+    orc_htot(o, 1.0, lf, wfn, wfn + size1, hP[0], hP[0] + size1);
+  }
+  o->xab_nstep++;
+  for (int is = 0; is < K; ++is) memcpy(B[is], hP[is], sizeof(zc) * sz);
+  for (int is = 1; is < K; ++is) {
+    double lfp[9];
+    for (int ii = 0; ii < 9; ++ii) lfp[ii] = lf[9 * is + ii] - lf[ii];
+    etd_v1ext(o, tunit, lfp, P[is], B[is]);
+  }
+  etd_h1tot(o, lf, wfn, tw);
+  for (size_t k = 0; k < sz; ++k) tw[k] = tunit * tw[k] + hP[0][k];
+  memcpy(hP[K1], tw, sizeof(zc) * sz);
+  int jfact = 1;
+  for (int js = 1; js < K; ++js) {
+    jfact *= js;
+    zc rj = dt / jfact;
+    for (size_t k = 0; k < sz; ++k) wfn[k] += rj * tw[k];
+    memcpy(P[K1], tw, sizeof(zc) * sz);
+    etd_h1tot(o, lf, P[K1], tw);
+    for (size_t k = 0; k < size1; ++k) tw[k] *= tunit * dt;
+    for (int is = 0; is < K; ++is) {
+      zc af = o->xab_ainv[K * is + js];
+      for (size_t k = 0; k < sz; ++k) tw[k] += af * B[is][k];
+    }
+  }
+  jfact *= K;
+  zc rj = dt / jfact;
+  for (int k = 0; k < o->lcic; ++k) tw[size1 + k] *= rj;
+  etd_h1rat_prod(o, XAB_RAT, 1, 1, lf, tw);
+  for (size_t k = 0; k < sz; ++k) wfn[k] += tw[k];
+  if (apply_mask) orc_mask(o, wfn);
   return 0;
 }
 

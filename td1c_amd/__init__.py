@@ -83,6 +83,13 @@ def load_library():
     L.td1c_lawrk_propagate.argtypes = [vp, ctypes.POINTER(TD1CField), ctypes.c_double, ctypes.c_double,
                                        ctypes.c_int, vp, ctypes.c_int, vp]
     L.td1c_lawrk_propagate.restype = ctypes.c_int
+    L.td1c_expab_gen.argtypes = [vp, ctypes.c_double, ctypes.c_int, ctypes.c_double]
+    L.td1c_expab_gen.restype = ctypes.c_int
+    L.td1c_expab_prop.argtypes = [vp, ctypes.c_double, vp, vp, ctypes.c_int, vp]
+    L.td1c_expab_prop.restype = ctypes.c_int
+    L.td1c_expab_propagate.argtypes = [vp, ctypes.POINTER(TD1CField), ctypes.c_double, ctypes.c_double,
+                                       ctypes.c_int, vp, ctypes.c_int, vp]
+    L.td1c_expab_propagate.restype = ctypes.c_int
     L.td1c_set_graph.argtypes = [vp, ctypes.c_int]
     L.td1c_set_graph.restype = ctypes.c_int
     L.td1c_set_transform_path.argtypes = [vp, ctypes.c_int]
@@ -670,6 +677,48 @@ class LAWRK(ETDRB):
     def _gen(self, rosenbrock, maxcyc, thresh):
         _check(load_library().td1c_lawrk_gen(self.hprod.h, self.dtime, int(maxcyc), float(thresh)),
                "td1c_lawrk_gen")
+
+
+class EXPAB(ETDRB):
+    """Exponential Adams-Bashforth of order 4 (clexpab, prop_type "expab";
+    td1c_expab_*).  The history lives in the context: consecutive calls
+    continue one propagation; constructing restarts it."""
+
+    _fn = "expab"
+
+    def __init__(self, hprod: "HProd", dtime: float, maxcyc: int = 20, thresh: float = 1e-10):
+        super().__init__(hprod, dtime, False, maxcyc, thresh)
+        self.nstep = 0
+
+    def _gen(self, rosenbrock, maxcyc, thresh):
+        _check(load_library().td1c_expab_gen(self.hprod.h, self.dtime, int(maxcyc), float(thresh)),
+               "td1c_expab_gen")
+
+    @staticmethod
+    def history_fields(field: Field, t0: float, dtime: float, n: int) -> np.ndarray:
+        """lfield(t_n) and lfield at the history times t_max(n-k, 0), k = 1..3."""
+        return np.concatenate([field_value(field, t0 + max(n - k, 0) * dtime) for k in range(4)])
+
+    def prop(self, wfn, time: float = 0.0, field: Optional[Field] = None, lfields: Optional[np.ndarray] = None,
+             apply_mask: bool = True, stream=None, t0: Optional[float] = None):
+        """One step at `time`; lfields (4 x 9) as history_fields gives them
+        (from the field and the propagation start t0 when not given)."""
+        if lfields is None:
+            if field is None or t0 is None:
+                raise ValueError("need lfields, or a field and the start time t0")
+            lfields = self.history_fields(field, t0, self.dtime, self.nstep)
+        lf = np.ascontiguousarray(lfields, dtype=np.float64)
+        if lf.size != 36:
+            raise ValueError("lfields must hold 4 x 9 values")
+        _check(load_library().td1c_expab_prop(self.hprod.h, float(time), _ptr(lf), ctypes.c_void_p(wfn.data_ptr()),
+                                              int(apply_mask), self.hprod._stream(stream)), "td1c_expab_prop")
+        self.nstep += 1
+        return wfn
+
+    def propagate(self, wfn, time: float, nstep: int, field: Field, apply_mask: bool = True, stream=None):
+        super().propagate(wfn, time, nstep, field, apply_mask, stream)
+        self.nstep += nstep
+        return wfn
 
 
 class Batch:

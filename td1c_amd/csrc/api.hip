@@ -198,7 +198,12 @@ struct td1c_ctx {
   };
   std::vector<std::unique_ptr<EtdTab>> etd_tabs;
   EtdRat etd[7];
-  int etd_scheme = 0;  // 1 cletdrb, 2 cletdrk4
+  int etd_scheme = 0;  // 1 cletdrb, 2 cletdrk4, 3 cllawrk, 4 clexpab
+  // clexpab: the inverse Taylor matrix (column-major), steps since gen, the
+  // times of the history states
+  double xab_ainv[16];
+  int xab_nstep = 0;
+  double xab_times[4] = {0.0, 0.0, 0.0, 0.0};
   bool etd_ready = false;
   int etd_rosen = 1, etd_maxcyc = 20;
   double etd_dt = 0.0, etd_thresh = 1.0e-10;
@@ -2395,12 +2400,52 @@ int etd_set(td1c_ctx* c, int r, double dfac, const double* num, int dimn, const 
   return 0;
 }
 
+// The inverse of clexpab's 4 x 4 Taylor matrix A(i, j) = (-i)^j / j!
+// (A(i, 0) = 1, A(0, j >= 1) = 0; prop_general, src/prop/clexpab.cpp:
+// 407-425), column-major, by Gauss-Jordan elimination with row pivoting.
+void xab_inverse(double* out) {
+  constexpr int n = 4;
+  double a[n][n], b[n][n];  // [row][column]
+  double fact = 1.0;
+  for (int i = 0; i < n; ++i) {
+    a[i][0] = 1.0;
+    for (int j = 0; j < n; ++j) b[i][j] = i == j ? 1.0 : 0.0;
+  }
+  for (int j = 1; j < n; ++j) {
+    fact *= j;
+    a[0][j] = 0.0;
+    for (int i = 1; i < n; ++i) a[i][j] = std::pow(-(double)i, j) / fact;
+  }
+  for (int col = 0; col < n; ++col) {
+    int piv = col;
+    for (int r = col + 1; r < n; ++r)
+      if (std::fabs(a[r][col]) > std::fabs(a[piv][col])) piv = r;
+    std::swap(a[col], a[piv]);
+    std::swap(b[col], b[piv]);
+    const double d = a[col][col];
+    for (int j = 0; j < n; ++j) {
+      a[col][j] /= d;
+      b[col][j] /= d;
+    }
+    for (int r = 0; r < n; ++r) {
+      if (r == col) continue;
+      const double f = a[r][col];
+      for (int j = 0; j < n; ++j) {
+        a[r][j] -= f * a[col][j];
+        b[r][j] -= f * b[col][j];
+      }
+    }
+  }
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < n; ++i) out[n * j + i] = b[i][j];
+}
+
 // scheme 1: cletdrb::gen (the [2/3] forms, numerators scaled by dt1 for
 // both step sizes); scheme 2: cletdrk4::gen (:24-60) with gen_exp,
 // gen_gfun, gen_f0fun, gen_f1fun, gen_f2fun (:134-933), the [2/2] forms.
 int etd_generate(td1c_ctx* c, int scheme, double dt, int rosen, int maxcyc, double thresh) {
   Setup& S = c->S;
-  const char* name = scheme == 1 ? "etdrb" : (scheme == 2 ? "etdrk4" : "lawrk");
+  const char* name = scheme == 1 ? "etdrb" : (scheme == 2 ? "etdrk4" : (scheme == 3 ? "lawrk" : "expab"));
   if (S.isplit != 1) return fail(-1, std::string(name) + " needs split_type = 1 (the split hprod)");
   if (S.lconst != 0) return fail(-1, std::string(name) + " with lconst is not provided");
   if (S.nfcore > 0 && !c->wfn0_set) return fail(-4, "frozen core needs td1c_set_wfn0 first");
@@ -2422,6 +2467,15 @@ int etd_generate(td1c_ctx* c, int scheme, double dt, int rosen, int maxcyc, doub
       etd_form(fam[r], dfac, num, root);
       if (int rc = etd_set(c, r, dfac, num, 2, root, 3, dt, dts[r], thresh)) return rc;
     }
+  } else if (scheme == 4) {
+    // clexpab::gen (:25-110) with gen_phi4's [1/2] form (:1148-1354): numerator
+    // 1 - x scaled by dt, first factor 6, roots 2 -+ sqrt(2) i
+    const double num[2] = {1.0, -1.0};
+    const std::complex<double> root[2] = {std::complex<double>(2.0, -1.4142135623730950488),
+                                          std::complex<double>(2.0, 1.4142135623730950488)};
+    if (int rc = etd_set(c, 0, 6.0, num, 1, root, 2, dt, dt, thresh)) return rc;
+    xab_inverse(c->xab_ainv);
+    c->xab_nstep = 0;
   } else if (scheme == 3) {
     // cllawrk::gen: dPade2p.gen(dt/2, alpha 2) is dpade_gen's 1 + (i dt/4) T,
     // the half step's factor for this dt
@@ -2450,7 +2504,7 @@ int etd_generate(td1c_ctx* c, int scheme, double dt, int rosen, int maxcyc, doub
   if (int rc = cn_buffers(c)) return rc;
   const size_t so = (size_t)c->nbas * S.nfun, sz = so + S.lcic;
   const size_t ndn = (size_t)c->nbas * std::max(S.nfun - S.nfcore, 1);
-  const size_t need = 10 * sz + so + 3 * ndn;
+  const size_t need = 13 * sz + so + 3 * ndn;
   if (c->etd_buf.n < need) {
     c->etd_buf.release();
     HIP_TRY(c->etd_buf.alloc(need));
@@ -2482,15 +2536,17 @@ int enqueue_etd_step(td1c_ctx* c, const double* d_lf, double2* wfn, int apply_ma
   double2* base = c->etd_buf.p;
   double2 *W0 = base, *W1 = base + sz, *W2 = base + 2 * sz, *W3 = base + 3 * sz, *hW0 = base + 4 * sz,
           *hW1 = base + 5 * sz, *hW2 = base + 6 * sz, *hW3 = base + 7 * sz, *tW = base + 8 * sz, *dW = base + 9 * sz;
-  double2* orb = base + 10 * sz;  // an orbital block; the rational functions work on its dynamical rows
+  // an orbital block after the 13 state slots (10 of the RK-like schemes, 3K +
+  // 1 of clexpab); the rational functions work on its dynamical rows
+  double2* orb = base + 13 * sz;
   double2* tb[2] = {orb + so, orb + so + ndn};
   double2* vs = orb + so + 2 * ndn;
   const double *lf0 = d_lf, *lf1 = d_lf + 9, *lf3 = d_lf + 18, *dlf0 = d_lf + 27, *dv1 = d_lf + 36, *dv3 = d_lf + 45;
   const double2 one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
   const double2 tunit = S.icomp == 0 ? make_double2(-1.0, 0.0) : make_double2(0.0, -1.0);
   // the field of the rational functions: cletdrb's timeh = t (etd_time_of_h
-  // 0), cletdrk4's thalf = t + dt/2
-  const double* lfr = c->etd_scheme == 1 ? lf0 : lf1;
+  // 0), cletdrk4's thalf = t + dt/2, clexpab's t
+  const double* lfr = c->etd_scheme == 2 ? lf1 : lf0;
   const bool ob_el = S.n == 20;
   auto cpy = [&](double2* z, const double2* x, size_t n) { launch_zaxpby(n, one, x, zero, nullptr, z, s); };
   auto clr = [&](double2* z, size_t n) { launch_zaxpby(n, zero, nullptr, zero, nullptr, z, s); };
@@ -2578,6 +2634,63 @@ int enqueue_etd_step(td1c_ctx* c, const double* d_lf, double2* wfn, int apply_ma
     return 0;
   };
   const auto& R = c->etd;
+  if (c->etd_scheme == 4) {
+    TIMED("expab");
+    // clexpab::prop_general (:370-496) with K = 4; d_lf: lfield(t), then
+    // lfield(timeP[k]) - lfield(t) for k = 1..3
+    constexpr int K = 4, K1 = K - 1;
+    double2 *P[K], *hP[K], *B[K];
+    for (int k = 0; k < K; ++k) {
+      P[k] = base + (size_t)k * sz;
+      hP[k] = base + (size_t)(K + k) * sz;
+      B[k] = base + (size_t)(2 * K + k) * sz;
+    }
+    double2* tw = base + (size_t)3 * K * sz;
+    if (c->xab_nstep == 0) {
+      cpy(P[0], wfn, sz);
+      if (int rc = htot(lf0, wfn, hP[0])) return rc;
+      for (int k = 1; k < K; ++k) {
+        cpy(P[k], P[0], sz);
+        cpy(hP[k], hP[0], sz);
+      }
+    } else {
+      for (int k = K1; k > 0; --k) {
+        cpy(P[k], P[k - 1], sz);
+        cpy(hP[k], hP[k - 1], sz);
+      }
+      cpy(P[0], wfn, sz);
+      if (int rc = htot(lf0, wfn, hP[0])) return rc;
+    }
+    c->xab_nstep++;
+    for (int k = 0; k < K; ++k) cpy(B[k], hP[k], sz);
+    for (int k = 1; k < K; ++k) {  // + tunit (V(timeP[k]) - V(t)) u_k
+      h1(P[k] + dof, vs, tunit, d_lf + 9 * k, true);
+      axpy(B[k] + dof, one, vs, ndn);
+    }
+    clr(tw, sz);
+    h1(wfn + dof, tw + dof, one, lf0, false);
+    launch_zaxpby(sz, tunit, tw, one, hP[0], tw, s);
+    const double dt = c->etd_dt;
+    int jfact = 1;
+    for (int js = 1; js < K; ++js) {
+      jfact *= js;
+      axpy(wfn, make_double2(dt / jfact, 0.0), tw, sz);
+      cpy(P[K1], tw, sz);
+      clr(tw, sz);
+      h1(P[K1] + dof, tw + dof, cmulh(tunit, make_double2(dt, 0.0)), lf0, false);
+      for (int k = 0; k < K; ++k) axpy(tw, make_double2(c->xab_ainv[K * k + js], 0.0), B[k], sz);
+    }
+    jfact *= K;
+    launch_zaxpby(S.lcic, make_double2(dt / jfact, 0.0), tw + so, zero, nullptr, tw + so, s);
+    // expab_phi.prod on tw's dynamical orbitals, then wfn += tw
+    numer(R[0], tw + dof, orb + dof);
+    denom(R[0]);
+    axpy(wfn + so, one, tw + so, S.lcic);
+    axpy(wfn + dof, one, orb + dof, ndn);
+    if (apply_mask) launch_mask(wfn, c->mask.p, S.nr, S.L1, S.nfcore, S.nfun, s);
+    HIP_TRY(hipGetLastError());
+    return 0;
+  }
   if (c->etd_scheme == 3) {
     TIMED("lawrk");
     // cllawrk::expwfn11 (:296-324): out = in, dpade_prod on out's dynamical
@@ -2738,6 +2851,57 @@ int td1c_etdrb_gen(td1c_ctx* c, double dtime, int rosenbrock, int maxcyc, double
 int td1c_etdrk4_gen(td1c_ctx* c, double dtime, int maxcyc, double thresh) {
   if (!c) return fail(-1, "null argument");
   return etd_generate(c, 2, dtime, 0, maxcyc, thresh);
+}
+
+int td1c_expab_gen(td1c_ctx* c, double dtime, int maxcyc, double thresh) {
+  if (!c) return fail(-1, "null argument");
+  return etd_generate(c, 4, dtime, 0, maxcyc, thresh);
+}
+
+// clexpab's history times for the step at t: prop_general's shift
+// (src/prop/clexpab.cpp:378-397), every slot t on the first step
+static void xab_advance_times(td1c_ctx* c, double t) {
+  if (c->xab_nstep == 0) {
+    for (int k = 0; k < 4; ++k) c->xab_times[k] = t;
+  } else {
+    for (int k = 3; k > 0; --k) c->xab_times[k] = c->xab_times[k - 1];
+    c->xab_times[0] = t;
+  }
+}
+
+int td1c_expab_prop(td1c_ctx* c, double t, const double* lfields, double* d_wfn, int apply_mask, void* stream) {
+  if (!c || !lfields || !d_wfn) return fail(-1, "null argument");
+  if (!c->etd_ready || c->etd_scheme != 4) return fail(-1, "call td1c_expab_gen first");
+  if (active_recorder()) return fail(-1, "exponential steps are not batched");
+  double lf[36];
+  for (int i = 0; i < 9; ++i) lf[i] = lfields[i];
+  for (int k = 1; k < 4; ++k)
+    for (int i = 0; i < 9; ++i) lf[9 * k + i] = lfields[9 * k + i] - lfields[i];
+  hipStream_t s = (hipStream_t)stream;
+  HIP_TRY(hipMemcpyAsync(c->etd_lf.p, lf, 36 * sizeof(double), hipMemcpyHostToDevice, s));
+  // the copy must leave lf before it goes out of scope
+  HIP_TRY(hipStreamSynchronize(s));
+  xab_advance_times(c, t);
+  return enqueue_etd_step(c, c->etd_lf.p, reinterpret_cast<double2*>(d_wfn), apply_mask, s);
+}
+
+int td1c_expab_propagate(td1c_ctx* c, const td1c_field* field, double t0, double dtime, int nstep, double* d_wfn,
+                         int apply_mask, void* stream) {
+  if (!c || !field || !d_wfn) return fail(-1, "null argument");
+  if (!c->etd_ready || c->etd_scheme != 4) return fail(-1, "call td1c_expab_gen first");
+  if (dtime != c->etd_dt) return fail(-1, "dtime differs from the one the tables were generated for");
+  if (check_field(field)) return -1;
+  if (nstep < 0) return fail(-1, "nstep must be >= 0");
+  hipStream_t s = (hipStream_t)stream;
+  const Pulse p = make_pulse(field);
+  for (int i = 0; i < nstep; ++i) {
+    const double t = t0 + i * dtime;
+    xab_advance_times(c, t);
+    launch_xab_fields(p, t, c->xab_times, c->etd_lf.p, s);
+    if (int rc = enqueue_etd_step(c, c->etd_lf.p, reinterpret_cast<double2*>(d_wfn), apply_mask, s)) return rc;
+  }
+  HIP_TRY(hipGetLastError());
+  return 0;
 }
 
 int td1c_lawrk_gen(td1c_ctx* c, double dtime, int maxcyc, double thresh) {

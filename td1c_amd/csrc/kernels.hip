@@ -2586,6 +2586,26 @@ __global__ void k_etd_fields(const Pulse p, PropClock* __restrict__ st, double* 
   if (q == 0) st->n = n + 1;
 }
 
+// The field arrays of one clexpab step: lfield(t), then lfield(timeP[k]) -
+// lfield(t) for the history times timeP[1..3] (prop_general,
+// src/prop/clexpab.cpp:400-405), one thread each.
+__global__ void k_xab_fields(const Pulse p, double t, double t1, double t2, double t3, double* __restrict__ lf) {
+  const int q = threadIdx.x;
+  if (q >= 4) return;
+  double a[9], b[9];
+  pulse_lfield(p, t, b);
+  if (q == 0) {
+    for (int i = 0; i < 9; ++i) lf[i] = b[i];
+    return;
+  }
+  pulse_lfield(p, q == 1 ? t1 : (q == 2 ? t2 : t3), a);
+  for (int i = 0; i < 9; ++i) lf[9 * q + i] = a[i] - b[i];
+}
+
+void launch_xab_fields(const Pulse& p, double t, const double* tp, double* lf, hipStream_t s) {
+  hipLaunchKernelGGL(k_xab_fields, dim3(1), dim3(64), 0, s, p, t, tp[1], tp[2], tp[3], lf);
+}
+
 void launch_etd_fields(const Pulse& p, PropClock* st, double* lf, hipStream_t s) {
   hipLaunchKernelGGL(k_etd_fields, dim3(1), dim3(64), 0, s, p, st, lf);
 }

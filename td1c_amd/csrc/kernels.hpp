@@ -324,6 +324,8 @@ void launch_zaxpby(size_t n, double2 a, const double2* x, double2 b, const doubl
 // lfield(t), lfield(t + dt/2), lfield(t + dt), d lfield/dt (t),
 // lfield(t + dt/2) - lfield(t), lfield(t + dt) - lfield(t)
 void launch_etd_fields(const Pulse& p, PropClock* st, double* lf, hipStream_t s);
+// clexpab's field arrays: lfield(t), then lfield(tp[k]) - lfield(t), k = 1..3
+void launch_xab_fields(const Pulse& p, double t, const double* tp, double* lf, hipStream_t s);
 void launch_field_stages(const Pulse& p, PropClock* st, const StageNodes& nodes, double* lf, hipStream_t s);
 void launch_clock_set(PropClock* st, double t0, double dt, hipStream_t s);
 void launch_dtorb(const DtorbArgs& a, hipStream_t s);

@@ -22,6 +22,15 @@ and of the field derivative it reads (clfield::get_der).
   its (1,1) Pade exponential (observed 1.91; a wrong stage weight, a skipped
   exponential, the Pade's dt/4 or its 2x - x recombination changed leave
   <= 1.4).
+- clexpab (src/prop/clexpab.cpp, order 4, ndpade 12): its [1/2] form R
+  satisfies 1 + x + x^2/2 + x^3/6 + x^4 R(x)/dt = the (1,2) Pade of e^x, the
+  identity "consistent with Pade(1,2)" of gen_phi4 (to rounding); the
+  multistep (constant history at the start) converges to the full-hprod
+  trajectory at order 2.7-3.2 over dt = 0.1 ... 0.0125.  A transposed
+  finite-difference matrix or a wrong h1 scaling leaves order <= 1.6, a wrong
+  sign of the history's field correction raises the finest error by 40 %;
+  the CI factor of the remainder term (dt/24) does not show at these steps
+  and is pinned only by the reading of prop_general.
 """
 import math
 import os
@@ -219,3 +228,49 @@ def test_lawrk_converges_at_second_order(gauge, cas):
         errs.append(np.linalg.norm(w - ref))
     orders = [math.log2(a / b) for a, b in zip(errs, errs[1:])]
     assert orders[-1] > 1.8 and errs[-1] < 1.2e-3, (errs, orders)
+
+
+def _xab_fields(f, t0, dt, n):
+    """lfield(t_n) and lfield at the history times t_max(n-k, 0), k = 1..3."""
+    return np.concatenate([td1c_amd.field_value(f, t0 + max(n - k, 0) * dt) for k in range(4)])
+
+
+@pytest.mark.parametrize("icomp", [1, 0])
+def test_expab_form_is_consistent_with_pade12(icomp):
+    dt = 0.05
+    o = _oracle(_prob("velocity1", 1, icomp=icomp))
+    o.expab_gen(dt)
+    tunit = -1j if icomp == 1 else -1.0
+    for lam in (3.0, 11.0, 40.0, 0.5):
+        x = tunit * dt * lam
+        p12 = (1 + x / 3) / (1 - 2 * x / 3 + x * x / 6)
+        got = 1 + x + x * x / 2 + x ** 3 / 6 + x ** 4 * o.etd_rat_scalar("expab_phi", complex(lam)) / dt
+        assert abs(got - p12) < 1e-14, (lam, got, p12)
+
+
+@pytest.mark.parametrize("cas", [False, True])
+@pytest.mark.parametrize("gauge", ["velocity1", "length1"])
+def test_expab_converges(gauge, cas):
+    from td1c_amd import CRK
+
+    p1 = _prob(gauge, 1, cas)
+    o1 = _oracle(p1)
+    w0 = W.bound_wfn(p1, 7, o1.lcic)
+    f = td1c_amd.Field(env_type="sin2", gauge=gauge, fint=2e14, wlen=400.0, cep=30.0, numcyc=2.0)
+    o0 = _oracle(_prob(gauge, 0, cas))
+    o0.set_wfn0(w0)
+    rk = CRK(4)
+    ref, t, h = w0.copy(), 10.0, 0.0025
+    for _ in range(160):
+        ref = o0.rk_prop(4, h, rk.stage_fields(f, t, h), ref, apply_mask=False)
+        t += h
+    o1.set_wfn0(w0)
+    errs = []
+    for dt in (0.1, 0.05, 0.025, 0.0125):
+        o1.expab_gen(dt)
+        w = w0.copy()
+        for n in range(int(round(0.4 / dt))):
+            w = o1.expab_step(_xab_fields(f, 10.0, dt, n), w, apply_mask=False)
+        errs.append(np.linalg.norm(w - ref))
+    orders = [math.log2(a / b) for a, b in zip(errs, errs[1:])]
+    assert orders[-1] > 2.65 and errs[-1] < 1.0e-4, (errs, orders)

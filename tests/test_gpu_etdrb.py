@@ -204,3 +204,44 @@ def test_lawrk_steps_match_oracle(gpu_lib, name, igauge, precision):
     torch.cuda.synchronize()
     assert _rel(e.cpu().numpy(), got) < 1e-12
     hp.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("name,igauge", [("he_small", 0), ("cas_fc_small", 1), ("ne_small", 1), ("he_n16", 1)])
+def test_expab_steps_match_oracle(gpu_lib, name, igauge, precision):
+    """clexpab: five steps (the history filled at the start, then shifted)
+    against orc_expab_step (FP64 1e-10, mixed 1e-6), and td1c_expab_propagate
+    over two calls (2 + 3 steps, the history carried between them) equal to
+    them."""
+    import torch
+    import td1c_amd
+    from oracle.oracle import Oracle
+
+    prob = getattr(C, name)(isplit=1, igauge=igauge, precision=precision)
+    o = Oracle(prob)
+    w = C.smooth_wfn_for(prob, o.lcic)
+    o.set_wfn0(w)
+    o.expab_gen(DT)
+    hp = td1c_amd.HProd(prob)
+    hp.set_wfn0(w)
+    xab = td1c_amd.EXPAB(hp, DT)
+    f = _field(igauge)
+    t0 = 30.0
+    d = hp.to_device(w)
+    ref = w.copy()
+    for n in range(5):
+        lf = td1c_amd.EXPAB.history_fields(f, t0, DT, n)
+        xab.prop(d, time=t0 + n * DT, lfields=lf)
+        ref = o.expab_step(lf, ref, apply_mask=True)
+    torch.cuda.synchronize()
+    got = d.cpu().numpy()
+    tol = 1e-10 if precision == 0 else 1e-6
+    assert _rel(got, ref) < tol, _rel(got, ref)
+    assert _rel(got, w) > 1e-5
+    xab2 = td1c_amd.EXPAB(hp, DT)  # restarts the history
+    e = hp.to_device(w)
+    xab2.propagate(e, t0, 2, f)
+    xab2.propagate(e, t0 + 2 * DT, 3, f)
+    torch.cuda.synchronize()
+    assert _rel(e.cpu().numpy(), got) < 1e-12
+    hp.close()
