@@ -30,6 +30,7 @@ from __future__ import annotations
 import argparse
 import hashlib
 import json
+import math
 import os
 import platform
 import sys
@@ -306,6 +307,18 @@ def scan_batch(n: int, dt: float, warmup: int, steps: int, gate_steps: int) -> d
     }
 
 
+def strict(x):
+    """x with every non-finite float replaced by None, so that the line is
+    strict JSON."""
+    if isinstance(x, dict):
+        return {k: strict(v) for k, v in x.items()}
+    if isinstance(x, (list, tuple)):
+        return [strict(v) for v in x]
+    if isinstance(x, float) and not math.isfinite(x):
+        return None
+    return x
+
+
 def progress(msg: str):
     print(f"bench: {msg}", file=sys.stderr, flush=True)
 
@@ -400,11 +413,12 @@ def main():
         return dict(prob=prob, hp=hp, wfn=wfn, ms_step=(t1 - t0) * 1e3 / args.steps, breakdown=breakdown,
                     nprof=nprof, dname=dname, dtime=times[dname], norm=norm)
 
-    def measure_etdrb(hp, wfn, steps: int, compare_steps: int = 0) -> dict:
-        """cletdrb (the reference's default prop_type) on the same context:
-        ms per step after two warm-up steps, the event breakdown of one step,
-        and the dipole after compare_steps steps against split2's."""
-        etd = td1c_amd.ETDRB(hp, dt)
+    def measure_etdrb(hp, wfn, steps: int, compare_steps: int = 0, scheme=None) -> dict:
+        """An exponential propagator (cletdrb, the reference's default
+        prop_type, unless scheme gives another) on the same context: ms per
+        step after two warm-up steps, the event breakdown of one step, and
+        the dipole after compare_steps steps against split2's."""
+        etd = (scheme or td1c_amd.ETDRB)(hp, dt)
         d = hp.to_device(wfn)
         etd.propagate(d, 0.0, 2, field)
         torch.cuda.synchronize()
@@ -417,19 +431,23 @@ def main():
         torch.cuda.synchronize()
         bd = hp.get_timing()
         hp.set_timing(False)
-        out = {"ms_per_step": ms, "hprod_evals_per_s": 4.0 / (ms * 1e-3),
+        nh = 1.0 if scheme is td1c_amd.EXPAB else 4.0  # split hprods per step
+        out = {"ms_per_step": ms, "hprod_evals_per_s": nh / (ms * 1e-3),
                "propagations_per_s": 1.0 / (nprop * ms * 1e-3),
                "event_ms_one_step": {k: round(v[0], 4) for k, v in bd.items()}}
         if compare_steps > 0:
             a = hp.to_device(wfn)
             b = hp.to_device(wfn)
+            etd = (scheme or td1c_amd.ETDRB)(hp, dt)  # a fresh history for clexpab
             etd.propagate(a, 0.0, compare_steps, field)
             rk.propagate(hp, b, 0.0, dt, compare_steps, field)
             torch.cuda.synchronize()
             lf = td1c_amd.field_value(field, compare_steps * dt)
-            da, db = hp.dipole(lf, a), hp.dipole(lf, b)
-            out["dipole_vs_split2"] = {"steps": compare_steps, "etdrb": float(da), "split2": float(db),
-                                       "abs_diff": float(abs(da - db))}
+            da, db = float(hp.dipole(lf, a)), float(hp.dipole(lf, b))
+            fin = bool(np.isfinite(da) and torch.isfinite(a).all().item())
+            # non-finite values are reported as null (the line stays strict JSON)
+            out["dipole_vs_split2"] = {"steps": compare_steps, "finite": fin, "this": da if fin else None,
+                                       "split2": db, "abs_diff": abs(da - db) if fin else None}
         return out
 
     if args.only == "batch":
@@ -559,6 +577,12 @@ def main():
                "mixed": measure_etdrb(mixed["hp"], fp64["wfn"], args.etd_steps)}
         etd["split2_ms_per_step_fp64"] = fp64["ms_step"]
         progress(f"etdrb {etd['fp64']['ms_per_step']:.3f} / {etd['mixed']['ms_per_step']:.3f} ms/step")
+        etd["others_fp64"] = {}
+        for nm, cls in (("etdrk4", td1c_amd.ETDRK4), ("lawrk", td1c_amd.LAWRK), ("expab", td1c_amd.EXPAB)):
+            r = measure_etdrb(fp64["hp"], fp64["wfn"], args.etd_steps, compare_steps=100, scheme=cls)
+            r.pop("event_ms_one_step", None)
+            etd["others_fp64"][nm] = r
+            progress(f"{nm} {r['ms_per_step']:.3f} ms/step")
 
     scan = None
     if args.scan_members > 0:
@@ -640,7 +664,7 @@ def main():
         result["cpu_baseline"] = cpu_baseline(fp64["prob"], fp64["wfn"], lf0, args.cpu_seconds, args.workload)
     else:
         result["cpu_baseline"] = None
-    print(json.dumps(result), flush=True)
+    print(json.dumps(strict(result), allow_nan=False), flush=True)
 
 
 if __name__ == "__main__":
