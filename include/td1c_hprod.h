@@ -346,6 +346,26 @@ int td1c_expab_prop(td1c_ctx* ctx, double t, const double* lfields, double* d_wf
 int td1c_expab_propagate(td1c_ctx* ctx, const td1c_field* field, double t0, double dtime, int nstep,
                          double* d_wfn, int apply_mask, void* stream);
 
+/* hprod_chkconv (src/hprod/hprod_chkconv.F90:2-63), the convergence test of
+ * init_prop12: out[0] the energy of an hprod at lfield, out[1] |hcic| /
+ * lcic, and from the generalised Fock vectors (h0 + h1 + g + v2) phi_i of
+ * the dynamical orbitals (hprod_gfock), projected on the complement of all
+ * orbitals (hprod_projall, inner false) and overlapped over the whole box
+ * (hprod_mkovlp): out[2] = sqrt(sum_i <g_i|g_i>) / nfun, out[3] =
+ * sqrt(2 sum_dcore <g_i|g_i> + sum_act <g_a|g_b> den1(b, a)).  Synchronises
+ * the stream.  Needs nradfc = nrad-1. */
+int td1c_chkconv(td1c_ctx* ctx, const double* lfield, const double* d_wfn, double out[4], void* stream);
+
+/* init_prop12 (src/main/init.cpp:242-310) with cletdrb, the default
+ * prop_type: until |E - E_prev| < tolene and out[3] of td1c_chkconv <
+ * tolwfn (init_tolene 1e-12, init_tolwfn 1e-6 by default), one cletdrb step
+ * of dstep at the fixed field lfield (icomp = 0) then clwfn::orth.  The
+ * context must have icomp = 0 and split_type = 1; the series settings are
+ * td1c_h1itr_gen's (20 terms, 1e-10 by default).  *ene gets the last energy,
+ * *ncyc the last cycle index. */
+int td1c_init_prop12(td1c_ctx* ctx, double dstep, double tolene, double tolwfn, int maxcyc,
+                     const double* lfield, double* d_wfn, double* ene, int* ncyc, void* stream);
+
 /* Grid projection of a state between two problems, host memory only (no GPU
  * call): clbas::proj (src/bas/clbas.cpp:280-332) with the CI copy of
  * guess_type = "read_proj" (src/main/guess.cpp:37-47).  Each interior point r

@@ -93,6 +93,8 @@ def lib(openmp: bool = False):
         L.orc_expab_gen.restype = ctypes.c_int
         L.orc_expab_step.argtypes = [vp, vp, vp, ctypes.c_int]
         L.orc_expab_step.restype = ctypes.c_int
+        L.orc_chkconv.argtypes = [vp, vp, vp, vp, vp]
+        L.orc_chkconv.restype = None
         L.orc_band_chol.argtypes = [ctypes.c_int, ctypes.c_int, vp]
         L.orc_band_chol.restype = ctypes.c_int
         L.orc_band_chol_solve.argtypes = [ctypes.c_int, ctypes.c_int, vp, vp]
@@ -333,6 +335,15 @@ class Oracle:
         if rc:
             raise RuntimeError(f"orc_lawrk_step failed ({rc})")
         return w
+
+    def chkconv(self, lfield, wfn: np.ndarray) -> np.ndarray:
+        """hprod_chkconv: (energy, |hcic| / lcic, grad(2), grad(3))."""
+        w = np.ascontiguousarray(wfn, dtype=np.complex128)
+        lf = np.ascontiguousarray(lfield, dtype=np.float64)
+        cic = np.ascontiguousarray(w[self.norb:])
+        out = np.zeros(4)
+        self._L.orc_chkconv(self.h, _p(lf), _p(w[: self.norb]), _p(cic), _p(out))
+        return out
 
     def expab_gen(self, dtime: float, maxcyc: int = 20, thresh: float = 1e-10):
         """clexpab::gen for step dtime (order 4, ndpade 12); restarts the history."""

@@ -2641,6 +2641,66 @@ int orc_expab_step(orc* o, const double* lf, zc* wfn, int apply_mask) {
 }
 
 /* ------------------------------------------------------------------ */
+/* hprod_chkconv (src/hprod/hprod_chkconv.F90:2-63) and init_prop12's   */
+/* convergence test (src/main/init.cpp:242-310)                         */
+/* ------------------------------------------------------------------ */
+/* out[0] the energy of an hprod (ene_tot); out[1] |hcic| / lcic; through
+ * hprod_gfock, hprod_projall with inner false (src/hprod/hprod_projall.F90:
+ * 2-160: fmat(i, j) = <phi_i|g_j> over r < nradfc, g_i -= sum_j phi_j
+ * fmat(j, i) for the dynamical i) and hprod_mkovlp over the whole box
+ * (ovlp(a, b) = <g_a|g_b>): out[2] = sqrt(sum_dyn ovlp(i, i)) / nfun,
+ * out[3] = sqrt(2 sum_dcore ovlp(i, i) + sum_act ovlp(a, b) den1(b, a)). */
+static void gfock(orc* o, const double* lfield, const zc* wfn, const zc* cic);
+
+void orc_chkconv(orc* o, const double* lfield, const zc* wfn, const zc* cic, double* out) {
+  int nf = o->nfun, nfc = o->nfcore, ncore = o->ncore, na = o->nact, nr = o->nr, L1 = o->L1;
+  size_t nb = o->nbas, size1 = nb * nf;
+  zc* hw = xcalloc(size1, sizeof(zc));
+  zc* hc = xcalloc(o->lcic > 0 ? o->lcic : 1, sizeof(zc));
+  orc_htot(o, 1.0, lfield, wfn, cic, hw, hc);
+  out[0] = o->ene_tot;
+  double hn = 0.0;
+  for (int k = 0; k < o->lcic; ++k) hn += creal(conj(hc[k]) * hc[k]);
+  out[1] = sqrt(hn) / o->lcic;
+  gfock(o, lfield, wfn, cic);
+  zc* fmat = xcalloc((size_t)nf * nf, sizeof(zc));
+  for (int i = 0; i < nf; ++i)
+    for (int j = 0; j < nf; ++j)
+      if (o->mval[j] == o->mval[i]) {
+        zc t = 0;
+        for (int l = 0; l <= o->lmax1; ++l)
+          for (int r = 0; r < o->nradfc; ++r) t += conj(o->orb[OIDX(r, l, i)]) * o->gorb[OIDX(r, l, j)];
+        fmat[i + (size_t)nf * j] = t;
+      }
+  for (int i = nfc; i < nf; ++i)
+    for (int j = 0; j < nf; ++j)
+      if (o->mval[j] == o->mval[i])
+        for (int l = 0; l <= o->lmax1; ++l)
+          for (int r = 0; r < o->nradfc; ++r) o->gorb[OIDX(r, l, i)] -= o->orb[OIDX(r, l, j)] * fmat[j + (size_t)nf * i];
+  zc* ov = xcalloc((size_t)nf * nf, sizeof(zc));
+  for (int i = 0; i < nf; ++i)
+    for (int j = 0; j < nf; ++j)
+      if (o->mval[j] == o->mval[i]) {
+        int am = abs(o->mval[i]);
+        zc t = 0;
+        for (int l = am; l < L1; ++l)
+          for (int r = 0; r < nr; ++r) t += conj(o->gorb[OIDX(r, l, j)]) * o->gorb[OIDX(r, l, i)];
+        ov[j + (size_t)nf * i] = t;
+      }
+  zc c2 = 0, c3 = 0;
+  for (int i = nfc; i < nf; ++i) c2 += ov[i + (size_t)nf * i];
+  for (int i = nfc; i < ncore; ++i) c3 += 2.0 * ov[i + (size_t)nf * i];
+  for (int ia = 0; ia < na; ++ia)
+    for (int ja = 0; ja < na; ++ja) c3 += ov[(ncore + ia) + (size_t)nf * (ncore + ja)] * o->den1[ja + na * ia];
+  out[2] = sqrt(creal(c2)) / nf;
+  out[3] = sqrt(creal(c3));
+  free(hw);
+  free(hc);
+  free(fmat);
+  free(ov);
+}
+
+/* ------------------------------------------------------------------ */
 /* accessors for tests                                                 */
 /* ------------------------------------------------------------------ */
 void orc_sizes(const orc* o, int* out) {

@@ -90,6 +90,11 @@ def load_library():
     L.td1c_expab_propagate.argtypes = [vp, ctypes.POINTER(TD1CField), ctypes.c_double, ctypes.c_double,
                                        ctypes.c_int, vp, ctypes.c_int, vp]
     L.td1c_expab_propagate.restype = ctypes.c_int
+    L.td1c_chkconv.argtypes = [vp, vp, vp, vp, vp]
+    L.td1c_chkconv.restype = ctypes.c_int
+    L.td1c_init_prop12.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int, vp, vp,
+                                   ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int), vp]
+    L.td1c_init_prop12.restype = ctypes.c_int
     L.td1c_set_graph.argtypes = [vp, ctypes.c_int]
     L.td1c_set_graph.restype = ctypes.c_int
     L.td1c_set_transform_path.argtypes = [vp, ctypes.c_int]
@@ -499,6 +504,26 @@ class HProd:
             raise ValueError("wfn must be a complex128 CUDA tensor of wfn_size elements")
         _check(load_library().td1c_orth(self.h, ctypes.c_void_p(wfn.data_ptr()), self._stream(stream)), "td1c_orth")
         return wfn
+
+    def chkconv(self, lfield, wfn, stream=None) -> np.ndarray:
+        """hprod_chkconv: (energy, |hcic| / lcic, grad(2), grad(3))."""
+        lf = np.ascontiguousarray(lfield, dtype=np.float64)
+        out = np.zeros(4)
+        _check(load_library().td1c_chkconv(self.h, _ptr(lf), ctypes.c_void_p(wfn.data_ptr()), _ptr(out),
+                                           self._stream(stream)), "td1c_chkconv")
+        return out
+
+    def init_prop12(self, wfn, dstep: float = 0.01, tolene: float = 1e-12, tolwfn: float = 1e-6,
+                    maxcyc: int = 1000000, lfield: Optional[Sequence[float]] = None, stream=None):
+        """init_prop12 with cletdrb (imaginary-time relaxation, the reference's
+        default) in place on a device wave function; returns (energy, last
+        cycle).  The problem must have icomp = 0 and split_type = 1."""
+        lf = np.ascontiguousarray(lfield if lfield is not None else np.zeros(9), dtype=np.float64)
+        e, n = ctypes.c_double(), ctypes.c_int()
+        _check(load_library().td1c_init_prop12(self.h, float(dstep), float(tolene), float(tolwfn), int(maxcyc),
+                                               _ptr(lf), ctypes.c_void_p(wfn.data_ptr()), ctypes.byref(e),
+                                               ctypes.byref(n), self._stream(stream)), "td1c_init_prop12")
+        return e.value, n.value
 
     def init_split2(self, wfn, dstep: float = 0.01, tolene: float = 1e-13, maxcyc: int = 1000000,
                     lfield: Optional[Sequence[float]] = None, rk_order: int = 4, stream=None):

@@ -177,7 +177,7 @@ HEADER_FUNCTIONS: List[str] = [
     "td1c_ctx_create", "td1c_ctx_destroy", "td1c_get_sizes", "td1c_set_wfn0",
     "td1c_htot", "td1c_get_energies", "td1c_rk_prop", "td1c_mask",
     "td1c_field_value", "td1c_field_der", "td1c_field_period", "td1c_bind", "hprod_htot_",
-    "td1c_last_error", "td1c_set_timing", "td1c_get_timing", "td1c_set_timing_kernel", "td1c_propagate", "td1c_dipole", "td1c_dipole3", "td1c_ipx", "td1c_ipd", "td1c_orth", "td1c_init_split2", "td1c_wfn_write", "td1c_wfn_read", "td1c_wfn_proj", "td1c_etdrb_gen", "td1c_etdrb_prop", "td1c_etdrb_propagate", "td1c_etdrk4_gen", "td1c_etdrk4_prop", "td1c_etdrk4_propagate", "td1c_lawrk_gen", "td1c_lawrk_prop", "td1c_lawrk_propagate", "td1c_expab_gen", "td1c_expab_prop", "td1c_expab_propagate", "td1c_htotx", "td1c_get_den1", "td1c_get_den2", "td1c_get_int1e",
+    "td1c_last_error", "td1c_set_timing", "td1c_get_timing", "td1c_set_timing_kernel", "td1c_propagate", "td1c_dipole", "td1c_dipole3", "td1c_ipx", "td1c_ipd", "td1c_orth", "td1c_init_split2", "td1c_wfn_write", "td1c_wfn_read", "td1c_wfn_proj", "td1c_etdrb_gen", "td1c_etdrb_prop", "td1c_etdrb_propagate", "td1c_etdrk4_gen", "td1c_etdrk4_prop", "td1c_etdrk4_propagate", "td1c_lawrk_gen", "td1c_lawrk_prop", "td1c_lawrk_propagate", "td1c_expab_gen", "td1c_expab_prop", "td1c_expab_propagate", "td1c_chkconv", "td1c_init_prop12", "td1c_htotx", "td1c_get_den1", "td1c_get_den2", "td1c_get_int1e",
     "td1c_get_int2e", "td1c_get_xmat", "td1c_set_graph", "td1c_set_transform_path", "td1c_ci_layout",
     "hprod_htot_static", "td1c_h1itr_gen", "td1c_h1itr_prop", "td1c_get_shapes", "td1c_set_onebody_shape",
     "td1c_batch_create", "td1c_batch_destroy", "td1c_batch_propagate", "td1c_batch_set_graph",

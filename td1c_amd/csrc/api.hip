@@ -207,7 +207,11 @@ struct td1c_ctx {
   bool etd_ready = false;
   int etd_rosen = 1, etd_maxcyc = 20;
   double etd_dt = 0.0, etd_thresh = 1.0e-10;
-  DevBuf<double2> etd_buf;  // ten states, then orb (orbital block) and three dynamical blocks
+  DevBuf<double2> etd_buf;  // 13 states, then orb (orbital block) and three dynamical blocks
+  // hprod_chkconv: the orbitals then the generalised Fock vectors of the
+  // dynamical ones, the CI derivative; their overlaps; the combined m values
+  DevBuf<double2> chk_buf, chk_ov;
+  DevBuf<int> chk_mval;
   DevBuf<double> etd_lf;    // [6][9] field arrays of the step
   DevBuf<int> etd_zero;     // an always-clear stop flag
 
@@ -252,6 +256,9 @@ struct td1c_ctx {
       for (auto* b : e) b->release();
     }
     etd_buf.release();
+    chk_buf.release();
+    chk_ov.release();
+    chk_mval.release();
     etd_lf.release();
     etd_zero.release();
   }
@@ -2157,6 +2164,81 @@ int td1c_init_split2(td1c_ctx* c, int rk_order, double dstep, double tolene, int
   return 0;
 }
 
+// hprod_chkconv (src/hprod/hprod_chkconv.F90:2-63): out[0] the energy of an
+// hprod, out[1] |hcic| / lcic, and from the generalised Fock vectors g_i of
+// the dynamical orbitals (hprod_gfock: (h0 + h1 + g + v2) phi_i, here the
+// hprod's own h01, g and v2 of that orbital) projected on the complement of
+// all orbitals (hprod_projall, inner false: g_i -= sum_j phi_j <phi_j|g_i>)
+// and their overlaps over the whole box (hprod_mkovlp): out[2] =
+// sqrt(sum_i <g_i|g_i>) / nfun, out[3] = sqrt(2 sum_dcore <g_i|g_i> +
+// sum_act <g_a|g_b> den1(b, a)).  Synchronises the stream.
+int td1c_chkconv(td1c_ctx* c, const double* lfield, const double* d_wfn, double out[4], void* stream) {
+  if (!c || !lfield || !d_wfn || !out) return fail(-1, "null argument");
+  Setup& S = c->S;
+  if (S.nfcore > 0 && !c->wfn0_set) return fail(-4, "frozen core needs td1c_set_wfn0 first");
+  if (S.nradfc != S.nr) return fail(-1, "td1c_chkconv needs nradfc = nrad-1");
+  hipStream_t s = (hipStream_t)stream;
+  const int nf = S.nfun, nd = nf - S.nfcore, NF = nf + nd;
+  const size_t nb = c->nbas, so = nb * nf, ndn = nb * nd;
+  if (!c->chk_buf.p) {
+    HIP_TRY(c->chk_buf.alloc((size_t)NF * nb + std::max(S.lcic, 1)));
+    HIP_TRY(c->chk_ov.alloc((size_t)NF * NF));
+    std::vector<int> mv(NF);
+    for (int i = 0; i < nf; ++i) mv[i] = S.mval[i];
+    for (int i = 0; i < nd; ++i) mv[nf + i] = S.mval[S.nfcore + i];
+    HIP_TRY(c->chk_mval.upload(mv));
+  }
+  const double2* wfn = reinterpret_cast<const double2*>(d_wfn);
+  double2* cb = c->chk_buf.p;
+  double2* hcic = cb + (size_t)NF * nb;
+  HIP_TRY(hipMemcpyAsync(c->lf_dev.p, lfield, 9 * sizeof(double), hipMemcpyHostToDevice, s));
+  CiStage cs;
+  cs.kout = hcic;
+  if (int rc = enqueue_htotx(c, 1.0, c->lf_dev.p, wfn, wfn + so, cs, s)) return rc;
+  HIP_TRY(join_island(c, s));
+  const double2 one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
+  double2* g = cb + so;
+  launch_zaxpby(so, one, c->orb, zero, nullptr, cb, s);
+  launch_zaxpby(ndn, one, c->v2orb, one, c->gorb, g, s);
+  launch_zaxpby(ndn, one, g, one, c->h01 + (size_t)S.nfcore * nb, g, s);
+  launch_window_ovlp(cb, c->chk_mval.p, NF, S.nr, S.L1, S.nr, c->chk_ov.p, s);
+  std::vector<double2> ov((size_t)NF * NF);
+  HIP_TRY(hipMemcpyAsync(ov.data(), c->chk_ov.p, ov.size() * sizeof(double2), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  for (int i = 0; i < nd; ++i)  // g_i -= sum_j phi_j fmat(j, i), fmat(j, i) = <phi_j|g_i>
+    for (int j = 0; j < nf; ++j) {
+      if (S.mval[j] != S.mval[S.nfcore + i]) continue;
+      const double2 f = ov[j + (size_t)NF * (nf + i)];
+      launch_zaxpby(nb, make_double2(-f.x, -f.y), cb + (size_t)j * nb, one, g + (size_t)i * nb, g + (size_t)i * nb, s);
+    }
+  launch_window_ovlp(g, c->chk_mval.p + nf, nd, S.nr, S.L1, S.nr, c->chk_ov.p, s);
+  std::vector<double2> go((size_t)nd * nd), hc(std::max(S.lcic, 1));
+  std::vector<double2> den1((size_t)std::max(S.nact * S.nact, 1));
+  HIP_TRY(hipMemcpyAsync(go.data(), c->chk_ov.p, go.size() * sizeof(double2), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(hc.data(), hcic, hc.size() * sizeof(double2), hipMemcpyDeviceToHost, s));
+  if (S.nact > 0)
+    HIP_TRY(hipMemcpyAsync(den1.data(), c->den1.p, den1.size() * sizeof(double2), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  double e[5];
+  if (int rc = td1c_get_energies(c, e)) return rc;
+  out[0] = e[4];
+  double hn = 0.0;
+  for (int k = 0; k < S.lcic; ++k) hn += hc[k].x * hc[k].x + hc[k].y * hc[k].y;
+  out[1] = std::sqrt(hn) / S.lcic;
+  double c2 = 0.0, c3 = 0.0;
+  for (int i = 0; i < nd; ++i) c2 += go[i + (size_t)nd * i].x;
+  for (int i = 0; i < S.ndcore; ++i) c3 += 2.0 * go[i + (size_t)nd * i].x;
+  const int a0 = S.ndcore;  // the first active orbital among the dynamical ones
+  for (int ia = 0; ia < S.nact; ++ia)
+    for (int ja = 0; ja < S.nact; ++ja) {
+      const double2 o = go[(a0 + ia) + (size_t)nd * (a0 + ja)], d = den1[ja + (size_t)S.nact * ia];
+      c3 += o.x * d.x - o.y * d.y;
+    }
+  out[2] = std::sqrt(c2) / nf;
+  out[3] = std::sqrt(c3);
+  return 0;
+}
+
 // The stage nodes of one propagation step: the RK nodes, and with
 // split_type = 1 the two half-step field times t + dt/4 and t + 3dt/4 as two
 // more nodes (ns = the RK stages).
@@ -2969,6 +3051,39 @@ int td1c_etdrb_propagate(td1c_ctx* c, const td1c_field* field, double t0, double
 int td1c_etdrk4_propagate(td1c_ctx* c, const td1c_field* field, double t0, double dtime, int nstep, double* d_wfn,
                           int apply_mask, void* stream) {
   return etd_propagate(c, 2, field, t0, dtime, nstep, d_wfn, apply_mask, stream);
+}
+
+// init_prop12 (src/main/init.cpp:242-310) with the cletdrb propagator (its
+// default prop_type): each cycle tests convergence by hprod_chkconv (energy
+// change below tolene and out[3] below tolwfn), else takes one cletdrb step of
+// dstep (icomp = 0, the field fixed at lfield: init_prop12 never advances the
+// field's time) and orthonormalises (clwfn::orth).  ladapt / madapt and
+// ProjHigh are identities for lconst = 0 and projhigh off.  *ene gets the
+// last energy, *ncyc the last cycle index.
+int td1c_init_prop12(td1c_ctx* c, double dstep, double tolene, double tolwfn, int maxcyc, const double* lfield,
+                     double* d_wfn, double* ene, int* ncyc, void* stream) {
+  if (!c || !lfield || !d_wfn || !ene || !ncyc) return fail(-1, "null argument");
+  if (c->S.icomp != 0) return fail(-1, "imaginary-time relaxation needs a context with icomp = 0");
+  if (maxcyc < 0 || !(dstep > 0.0)) return fail(-1, "maxcyc must be >= 0 and dstep > 0");
+  if (int rc = etd_generate(c, 1, dstep, 1, c->cn_maxcyc, c->cn_thresh)) return rc;
+  double lf[54];
+  for (int k = 0; k < 3; ++k)
+    for (int i = 0; i < 9; ++i) lf[9 * k + i] = lfield[i];
+  for (int i = 27; i < 54; ++i) lf[i] = 0.0;  // no time derivative, no differences
+  double e1 = -1.0e10, chk[4] = {0, 0, 0, 0};
+  int last = 0;
+  for (int icyc = 0; icyc < maxcyc; ++icyc) {
+    if (int rc = td1c_chkconv(c, lfield, d_wfn, chk, stream)) return rc;
+    last = icyc;
+    if (std::fabs(chk[0] - e1) < tolene && chk[3] < tolwfn) break;
+    e1 = chk[0];
+    if (int rc = td1c_etdrb_prop(c, lf, d_wfn, 0, stream)) return rc;
+    if (int rc = td1c_orth(c, d_wfn, stream)) return rc;
+  }
+  HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+  *ene = chk[0];
+  *ncyc = last;
+  return 0;
 }
 
 // ---------------------------------------------------------------------------

@@ -274,3 +274,54 @@ def test_expab_converges(gauge, cas):
         errs.append(np.linalg.norm(w - ref))
     orders = [math.log2(a / b) for a, b in zip(errs, errs[1:])]
     assert orders[-1] > 2.65 and errs[-1] < 1.0e-4, (errs, orders)
+
+
+def _relax(o, w, dt=0.01, maxcyc=4000):
+    """init_prop12's loop on the oracle: chkconv, cletdrb step (icomp 0),
+    orth."""
+    lf = np.zeros(9)
+    o.etdrb_gen(dt)
+    e1 = -1e10
+    for icyc in range(maxcyc):
+        c = o.chkconv(lf, w)
+        if abs(c[0] - e1) < 1e-12 and c[3] < 1e-6:
+            return icyc, c, w
+        e1 = c[0]
+        w = o.orth(o.etdrb_step(np.zeros(54), w, apply_mask=False))
+    raise AssertionError("no convergence")
+
+
+def test_chkconv_vanishes_at_the_hartree_fock_state():
+    """hprod_chkconv's orbital gradient (the projected generalised Fock
+    vectors) is large at a seeded state and falls below 1e-6 exactly where the
+    imaginary-time relaxation reaches the He Hartree-Fock energy of this grid
+    (-2.8616818, the value the init_split2 pin holds)."""
+    from tests import cases as C
+
+    prob = C.he_itr(isplit=1)
+    o = _oracle(prob)
+    w = C.smooth_wfn_for(prob, o.lcic)
+    o.set_wfn0(w)
+    c0 = o.chkconv(np.zeros(9), w)
+    assert c0[3] > 0.5
+    _, c, _ = _relax(o, w)
+    assert abs(c[0] - (-2.8616818)) < 2e-6, c
+    assert c[3] < 1e-6 and c[2] < 1e-6
+
+
+def test_chkconv_vanishes_at_the_casscf_state():
+    """CAS(2e,2o) He: the relaxation ends below the Hartree-Fock energy (the
+    variational bound), with the orbital gradient and the CI residual below
+    1e-6."""
+    prob = Problem(nfe=12, ndvr=10, rmax=30.0, lmax1=1, lmax2=2, znuc=2.0, ndcore=0, nact=2, nel_alpha=1,
+                   nel_beta=1, icomp=0, isplit=1)
+    from tests import cases as C
+
+    o = _oracle(prob)
+    w = C.smooth_wfn_for(prob, o.lcic)
+    o.set_wfn0(w)
+    c0 = o.chkconv(np.zeros(9), w)
+    assert c0[3] > 0.5 and c0[1] > 0.05
+    _, c, _ = _relax(o, w)
+    assert c[0] < -2.8616818 - 1e-3, c
+    assert c[3] < 1e-6 and c[1] < 1e-6

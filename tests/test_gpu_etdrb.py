@@ -245,3 +245,70 @@ def test_expab_steps_match_oracle(gpu_lib, name, igauge, precision):
     torch.cuda.synchronize()
     assert _rel(e.cpu().numpy(), got) < 1e-12
     hp.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("name,igauge", [("he_small", 0), ("cas_small", 0), ("cas_fc_small", 1), ("dc3_small", 1),
+                                         ("ne_small", 1)])
+def test_chkconv_matches_oracle(gpu_lib, name, igauge, precision):
+    """td1c_chkconv against orc_chkconv at a seeded state and a field:
+    energy, CI residual and both orbital-gradient norms (FP64 1e-10,
+    mixed 1e-6, relative to max(|value|, 1): the CI residual of a single
+    CAS root is near zero)."""
+    import torch
+    import td1c_amd
+    from oracle.oracle import Oracle
+
+    prob = getattr(C, name)(isplit=1, igauge=igauge, precision=precision)
+    o = Oracle(prob)
+    w = C.smooth_wfn_for(prob, o.lcic)
+    o.set_wfn0(w)
+    lf = C.LF_VEL if igauge else C.LF_LEN
+    ref = o.chkconv(lf, w)
+    hp = td1c_amd.HProd(prob)
+    hp.set_wfn0(w)
+    got = hp.chkconv(lf, hp.to_device(w))
+    torch.cuda.synchronize()
+    tol = 1e-10 if precision == 0 else 1e-6
+    for k in range(4):
+        assert abs(got[k] - ref[k]) <= tol * max(abs(ref[k]), 1.0), (k, got, ref)
+    assert ref[3] > 1e-3
+    hp.close()
+
+
+@pytest.mark.parametrize("cas", [False, True])
+def test_init_prop12_relaxes_as_the_oracle(gpu_lib, cas):
+    """td1c_init_prop12 (chkconv, cletdrb with icomp = 0, orth) against the
+    same loop on the oracle: the same final energy (1e-10) and cycle count
+    (within 2), the gradient below 1e-6."""
+    import torch
+    import td1c_amd
+    from oracle.oracle import Oracle
+    from td1c_amd.abi import Problem
+
+    if cas:
+        prob = Problem(nfe=12, ndvr=10, rmax=30.0, lmax1=1, lmax2=2, znuc=2.0, ndcore=0, nact=2, nel_alpha=1,
+                       nel_beta=1, icomp=0, isplit=1)
+    else:
+        prob = C.he_itr(isplit=1)
+    o = Oracle(prob)
+    w = C.smooth_wfn_for(prob, o.lcic)
+    o.set_wfn0(w)
+    o.etdrb_gen(0.01)
+    e1, ref, ncyc_ref = -1e10, w.copy(), None
+    for icyc in range(4000):
+        c = o.chkconv(np.zeros(9), ref)
+        if abs(c[0] - e1) < 1e-12 and c[3] < 1e-6:
+            ncyc_ref = icyc
+            break
+        e1 = c[0]
+        ref = o.orth(o.etdrb_step(np.zeros(54), ref, apply_mask=False))
+    assert ncyc_ref is not None
+    hp = td1c_amd.HProd(prob)
+    hp.set_wfn0(w)
+    d = hp.to_device(w)
+    e, n = hp.init_prop12(d, dstep=0.01, maxcyc=4000)
+    torch.cuda.synchronize()
+    assert abs(e - c[0]) < 1e-10 and abs(n - ncyc_ref) <= 2, (e, c[0], n, ncyc_ref)
+    assert hp.chkconv(np.zeros(9), d)[3] < 1e-6
+    hp.close()
