@@ -280,6 +280,11 @@ struct td1c_batch {
     int max_blocks = 0;  // blocks of the largest member
     size_t lds = 0;
     size_t off_recs = 0, off_start = 0, off_dims = 0;
+    // the Poisson interior pass over groups of (member, pair) items
+    // reading the shared tables (launch_poisson_interior_grouped)
+    int grouped = 0, njobs = 0;
+    size_t off_items = 0, off_jobs = 0;
+    PoissonShared psh{};
   };
   std::vector<Site> sites;
   DevBuf<unsigned char> table;
@@ -289,7 +294,12 @@ struct td1c_batch {
   // Crank-Nicolson and Poisson interior inverses, element one-body matrices
   DevBuf<double2> u_cnG;
   DevBuf<double> u_pG, u_eK0, u_eK, u_eP;
-  int shared[4] = {0, 0, 0, 0};  // members reading each shared table
+  // the Poisson row tables (Wl, Wr, invr, rpl1, fac2) at [L][point] with
+  // rstride points per L, when every member's equal the shared copy
+  DevBuf<double> u_prow[5];
+  int rstride = 0;
+  int shared[4] = {0, 0, 0, 0};  // members reading each shared table (3: the Poisson row tables)
+  int pgroups = 0;                // item groups of the grouped Poisson interior pass
   std::vector<unsigned char> key;  // what the recorded step depends on
   hipStream_t cap_stream = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -303,6 +313,7 @@ struct td1c_batch {
     u_eK0.release();
     u_eK.release();
     u_eP.release();
+    for (auto& t : u_prow) t.release();
   }
 };
 
@@ -1605,6 +1616,7 @@ int td1c_dipole_all(td1c_ctx* c, const double* lfield, const double* d_wfn, cons
 // a polynomial of degree N whose coefficient k is ipx[k]: F is evaluated at
 // the N+1 roots of unity and transformed back.  The reference reaches the
 // same coefficients through sums of minors and a binomial inversion.
+extern "C++" {
 namespace {
 
 std::complex<double> host_det(int n, std::vector<std::complex<double>>& a) {
@@ -1650,6 +1662,7 @@ void spin_dets(const Setup& S, const std::vector<std::complex<double>>& ov, cons
 }
 
 }  // namespace
+}  // extern "C++"
 
 // P[k], k = 0..N: probability that exactly k electrons are outside rad
 static int ipx_distribution(td1c_ctx* c, double rad, const double* lfield, const double* d_wfn, const double* d_cic,
@@ -3258,6 +3271,50 @@ int batch_share_tables(td1c_batch* b, std::vector<std::vector<LaunchRec>>& recs)
         });
       }
   }
+  // Poisson row tables at [L][point]: when every member shares the
+  // inverses, the same element geometry gives the same numbers at the same
+  // (L, point); the right coupling Wr of a member's last element (zero
+  // there, the box edge, and never read) takes the value of the members
+  // for which that element is an inner one, so members join largest first
+  for (auto& t : b->u_prow) t.release();
+  b->rstride = 0;
+  b->shared[3] = 0;
+  if (b->shared[1] == nmem) {
+    const int n = b->mem[0]->S.n, rs = ne * n - 1;
+    std::vector<int> order(nmem);
+    for (int m = 0; m < nmem; ++m) order[m] = m;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int x, int y) { return b->mem[x]->S.nfe > b->mem[y]->S.nfe; });
+    TableUnion u[5];
+    for (auto& t : u) t.init(L2, rs, 1);
+    bool all = true;
+    for (int m : order) {
+      const Setup& S = b->mem[m]->S;
+      const std::vector<double>* tab[5] = {&S.pWl, &S.pWr, &S.d2invr, &S.d2rpl1, &S.d2fac2};
+      const size_t want = (size_t)S.L2 * S.nr;
+      if (S.n != n) all = false;
+      for (int k = 0; k < 5 && all; ++k) {
+        if (tab[k]->size() != want) {
+          all = false;
+          break;
+        }
+        std::vector<double> t(*tab[k]);
+        if (k == 1)
+          for (int L = 0; L < S.L2; ++L)
+            for (int r = (S.nfe - 1) * n; r < S.nr; ++r) {
+              const size_t q = (size_t)L * rs + r;
+              if (u[1].set[q]) t[(size_t)L * S.nr + r] = u[1].v[q];
+            }
+        all = u[k].add(t.data(), S.L2, S.nr);
+      }
+      if (!all) break;
+    }
+    if (all) {
+      for (int k = 0; k < 5; ++k) HIP_TRY(b->u_prow[k].upload(u[k].v));
+      b->rstride = rs;
+      b->shared[3] = nmem;
+    }
+  }
   // element one-body matrices, [e][k][m] each
   {
     TableUnion u0, u1, u2;
@@ -3398,6 +3455,69 @@ int batch_record(td1c_batch* b, const std::vector<Pulse>& pulses, int rk_order, 
     st.off_start = align(off + nrec);
     st.off_dims = align(st.off_start + start.size() * sizeof(int));
   }
+  // Poisson interior sites over shared tables: (member, pair) items in
+  // groups of kPoissonSlots, ordered by extent (L2, element chunks) so that
+  // a group's items end near the same (L, chunk); jobs (group, L, chunk)
+  // for every (L, chunk) that some item of the group covers
+  b->pgroups = 0;
+  if (b->shared[3] == nmem) {
+    for (size_t i = 0; i < nsite; ++i) {
+      td1c_batch::Site& st = sites[i];
+      if (st.proto.kind != kRecPoissonInterior || (st.proto.var[0] != 0 && st.proto.var[0] != 2)) continue;
+      std::vector<PoissonRec> pr(nmem);
+      bool ok = true;
+      for (int m = 0; m < nmem && ok; ++m) {
+        std::memcpy(&pr[m], recs[m][i].args.data(), sizeof(PoissonRec));
+        const PoissonRec& r = pr[m];
+        ok = r.a.ni == 19 && r.a.n == 20 && r.a.G == b->u_pG.p && r.epb == pr[0].epb && r.epb <= 32 &&
+             r.epb * 19 <= kPoissonInteriorThreads && r.a.nfe * r.a.n - 1 == r.a.nr && r.a.nr <= b->rstride;
+      }
+      if (!ok) continue;
+      struct Item { int m, p, L2, nch; };
+      std::vector<Item> items;
+      for (int m = 0; m < nmem; ++m)
+        for (int p = 0; p < pr[m].a.npair; ++p) items.push_back({m, p, pr[m].a.L2, pr[m].nchunk});
+      std::stable_sort(items.begin(), items.end(), [](const Item& x, const Item& y) {
+        return x.L2 != y.L2 ? x.L2 > y.L2 : x.nch > y.nch;
+      });
+      const int ng = ((int)items.size() + kPoissonSlots - 1) / kPoissonSlots;
+      std::vector<int2> slots((size_t)ng * kPoissonSlots, make_int2(-1, 0));
+      std::vector<int> gL(ng, 0), gC(ng, 0);
+      int nL = 0, nC = 0;
+      for (size_t k = 0; k < items.size(); ++k) {
+        const int g = (int)k / kPoissonSlots;
+        slots[k] = make_int2(items[k].m, items[k].p);
+        gL[g] = std::max(gL[g], items[k].L2);
+        gC[g] = std::max(gC[g], items[k].nch);
+        nL = std::max(nL, gL[g]);
+        nC = std::max(nC, gC[g]);
+      }
+      std::vector<int> jobs;
+      for (int L = 0; L < nL; ++L)
+        for (int c = 0; c < nC; ++c)
+          for (int g = 0; g < ng; ++g)
+            if (L < gL[g] && c < gC[g]) {
+              jobs.push_back(g);
+              jobs.push_back(L);
+              jobs.push_back(c);
+            }
+      if (jobs.size() / 3 > 0x7fffffffULL) continue;
+      st.off_items = align(host.size());
+      host.resize(st.off_items);
+      host.insert(host.end(), reinterpret_cast<const unsigned char*>(slots.data()),
+                  reinterpret_cast<const unsigned char*>(slots.data() + slots.size()));
+      st.off_jobs = align(host.size());
+      host.resize(st.off_jobs);
+      host.insert(host.end(), reinterpret_cast<const unsigned char*>(jobs.data()),
+                  reinterpret_cast<const unsigned char*>(jobs.data() + jobs.size()));
+      host.resize(align(host.size()));
+      st.grouped = 1;
+      st.njobs = (int)(jobs.size() / 3);
+      st.psh = PoissonShared{b->u_pG.p, b->u_prow[0].p, b->u_prow[1].p, b->u_prow[2].p, b->u_prow[3].p,
+                             b->u_prow[4].p, pr[0].a.gstride, b->rstride, pr[0].epb};
+      b->pgroups = ng;
+    }
+  }
   // the old table may still be read by queued launches
   HIP_TRY(hipDeviceSynchronize());
   if (b->gexec) {
@@ -3419,6 +3539,12 @@ hipError_t batch_launch_step(td1c_batch* b, hipStream_t s) {
     BatchMap map{reinterpret_cast<const int*>(base + st.off_start), reinterpret_cast<const int*>(base + st.off_dims),
                  nmem, 0};
     const void* recs = base + st.off_recs;
+    if (st.grouped) {
+      if (!launch_poisson_interior_grouped(st.proto.var[0], recs, reinterpret_cast<const int2*>(base + st.off_items),
+                                           reinterpret_cast<const int*>(base + st.off_jobs), st.njobs, st.psh, s))
+        return hipErrorInvalidValue;
+      continue;
+    }
     int grid = st.total;
     if (batch_skips(st.proto)) {
       // two blocks of the largest member per hardware block
@@ -3564,7 +3690,7 @@ int td1c_batch_autotune(td1c_batch* b, const td1c_field* fields, int rk_order, d
 int td1c_batch_get_info(const td1c_batch* b, char* buf, int buflen) {
   if (!b || !buf || buflen <= 0) return fail(-1, "null argument");
   long long blocks = 0;
-  for (const auto& st : b->sites) blocks += st.total;
+  for (const auto& st : b->sites) blocks += st.grouped ? st.njobs : st.total;
   const char* ge[3] = {"3", "6", "12"};
   std::string js = "{\"members\": " + std::to_string(b->mem.size()) + ", \"sites_per_step\": " +
                    std::to_string(b->sites.size()) + ", \"blocks_per_step\": " + std::to_string(blocks) +
@@ -3572,7 +3698,8 @@ int td1c_batch_get_info(const td1c_batch* b, char* buf, int buflen) {
                    ", " + std::to_string(b->ob_lc) + "], \"transform_variant\": " + std::to_string(b->tvar) +
                    ", \"graph\": " + (b->use_graph ? "true" : "false") + ", \"shared_tables\": {\"cn\": " +
                    std::to_string(b->shared[0]) + ", \"poisson\": " + std::to_string(b->shared[1]) +
-                   ", \"onebody\": " + std::to_string(b->shared[2]) + "}" +
+                   ", \"onebody\": " + std::to_string(b->shared[2]) + ", \"poisson_rows\": " +
+                   std::to_string(b->shared[3]) + "}, \"poisson_groups\": " + std::to_string(b->pgroups) +
                    (b->tune_log.empty() ? std::string() : ", \"tune\": " + b->tune_log) + "}";
   if ((int)js.size() + 1 > buflen) return fail(-2, "buffer too small");
   std::memcpy(buf, js.c_str(), js.size() + 1);

@@ -1778,7 +1778,6 @@ __global__ void __launch_bounds__(BS) k_poisson_bridge_lds(const T* __restrict__
 // every pair of the group, whose rows b = rho * invr are staged in LDS:
 //   x_I = G_e b_I - W_l x_left - W_r x_right,  v = (x + Q rpl1) * fac2.
 constexpr int kPoissonPairGroup = 7;
-constexpr int kPoissonInteriorThreads = 256;
 template <int NI, class T>
 __device__ __forceinline__ void poisson_interior_body(const T* __restrict__ rho, T* __restrict__ v, const double2* __restrict__ xb, const PoissonArgs& a, int epb, int nchunk, const Blk bk) {
   constexpr int PG = kPoissonPairGroup;
@@ -2731,6 +2730,122 @@ __global__ void __launch_bounds__(kPoissonInteriorThreads) k_poisson_interior_ba
     const PoissonRec& r = stage_rec(recs, m, raw, have);
     poisson_interior_body<19, T>(static_cast<const T*>(r.rho), static_cast<T*>(r.v), r.xb, r.a, r.epb, r.nchunk, bk);
   });
+}
+
+// One block per job (group g, L, element chunk): thread per interior row as
+// in poisson_interior_body, with G_e's row and the row tables loaded once
+// for the kPoissonSlots items of the group (the same element geometry makes
+// them the same numbers as each member's own tables, so every item gets the
+// bits of its member's own pass).  Jobs are ordered L-major, then chunk,
+// then group, so that the groups reading one G_e share an XCD's L2.
+template <class T>
+__global__ void __launch_bounds__(kPoissonInteriorThreads)
+    k_poisson_interior_grouped(const PoissonRec* __restrict__ recs, const int2* __restrict__ items,
+                               const int* __restrict__ jobs, const PoissonShared sh) {
+  constexpr int S = kPoissonSlots, NI = 19, n = 20;
+  const int job = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+  const int g = jobs[3 * job], L = jobs[3 * job + 1], ch = jobs[3 * job + 2];
+  const int epb = sh.epb, e0 = ch * epb;
+  const int t = threadIdx.x;
+  const int el = t / NI, i = t - el * NI;
+  const int e = e0 + el;
+  const int r = e * n + i;
+  __shared__ double2 sb[S][kPoissonInteriorThreads];
+  __shared__ double2 sx[S][34];  // bridges e0-1 .. e0+epb-1 (epb <= 32)
+  __shared__ double2 sq[S];
+  __shared__ int sstate[S], snfe[S];  // state 0: not in range, 1: zero (L < |m|), 2: solve
+  __shared__ const void* srho[S];
+  __shared__ void* sv[S];
+  __shared__ const double2* sxb[S];
+  __shared__ size_t soff[S];  // ((p L2 + L) nr): the item's row start in rho and v
+  __shared__ size_t sxoff[S];
+  if (t < S) {
+    const int2 it = items[g * S + t];
+    int st = 0, nfe = 0;
+    double2 q = make_double2(0.0, 0.0);
+    if (it.x >= 0) {
+      const PoissonRec& rc = recs[it.x];
+      const int p = it.y, L2 = rc.a.L2;
+      nfe = rc.a.nfe;
+      if (L < L2 && ch < rc.nchunk) {
+        const int pm = rc.a.pair_m[p];
+        st = L >= (pm < 0 ? -pm : pm) ? 2 : 1;
+        if (st == 2) q = rc.a.Q[(size_t)p * L2 + L];
+      }
+      srho[t] = rc.rho;
+      sv[t] = rc.v;
+      sxb[t] = rc.xb;
+      soff[t] = ((size_t)p * L2 + L) * rc.a.nr;
+      sxoff[t] = ((size_t)p * L2 + L) * (nfe - 1);
+    }
+    sstate[t] = st;
+    snfe[t] = nfe;
+    sq[t] = q;
+  }
+  __syncthreads();
+  const bool in_blk = el < epb && e < sh.gstride;
+  const size_t lo = (size_t)L * sh.rstride;
+  double iv = 0.0;
+  if (in_blk) iv = sh.invr[lo + r];
+#pragma unroll
+  for (int j = 0; j < S; ++j)
+    if (sstate[j] == 2 && in_blk && e < snfe[j])
+      sb[j][t] = cscale(cto<double2>(static_cast<const T*>(srho[j])[soff[j] + r]), iv);
+  for (int q = t; q < S * (epb + 1); q += kPoissonInteriorThreads) {
+    const int j = q / (epb + 1), k = q % (epb + 1);
+    const int bidx = e0 - 1 + k;
+    sx[j][k] = (sstate[j] == 2 && bidx >= 0 && bidx < snfe[j] - 1) ? sxb[j][sxoff[j] + bidx]
+                                                                   : make_double2(0.0, 0.0);
+  }
+  double gk[NI];
+  double wl = 0.0, wr = 0.0, r1 = 0.0, f2 = 0.0;
+  if (in_blk) {
+    const double* ge = sh.G + (((size_t)L * sh.gstride) + e) * NI * NI + i;
+#pragma unroll
+    for (int k = 0; k < NI; ++k) gk[k] = ge[k * NI];
+    wl = sh.Wl[lo + r];
+    wr = sh.Wr[lo + r];
+    r1 = sh.rpl1[lo + r];
+    f2 = sh.fac2[lo + r];
+  }
+  __syncthreads();
+  if (!in_blk) return;
+  const int eb = el * NI;
+#pragma unroll 1
+  for (int j = 0; j < S; ++j) {
+    const int st = sstate[j], nfe = snfe[j];
+    if (st == 0 || e >= nfe) continue;
+    T* vp = static_cast<T*>(sv[j]) + soff[j];
+    if (st == 1) {
+      vp[r] = cz<T>();
+      continue;
+    }
+    double2 y = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int k = 0; k < NI; ++k) {
+      y.x = fma(gk[k], sb[j][eb + k].x, y.x);
+      y.y = fma(gk[k], sb[j][eb + k].y, y.y);
+    }
+    if (e >= 1) y = csub(y, cscale(sx[j][el], wl));
+    if (e <= nfe - 2) y = csub(y, cscale(sx[j][el + 1], wr));
+    const double2 Q = sq[j];
+    vp[r] = cto<T>(make_double2((y.x + Q.x * r1) * f2, (y.y + Q.y * r1) * f2));
+  }
+}
+
+bool launch_poisson_interior_grouped(int type, const void* recs, const int2* items, const int* jobs, int njobs,
+                                     const PoissonShared& sh, hipStream_t s) {
+  if (njobs <= 0) return true;
+  const auto* rr = static_cast<const PoissonRec*>(recs);
+  if (type == 0)
+    hipLaunchKernelGGL(k_poisson_interior_grouped<double2>, dim3(njobs), dim3(kPoissonInteriorThreads), 0, s, rr,
+                       items, jobs, sh);
+  else if (type == 2)
+    hipLaunchKernelGGL(k_poisson_interior_grouped<float2>, dim3(njobs), dim3(kPoissonInteriorThreads), 0, s, rr,
+                       items, jobs, sh);
+  else
+    return false;
+  return true;
 }
 
 template <int NF, class T>

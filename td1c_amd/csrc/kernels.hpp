@@ -544,6 +544,24 @@ bool launch_batch_h1itr(const LaunchRec& p, const void* recs, const BatchMap& ma
                         hipStream_t s, bool dry);
 bool launch_batch_island(const LaunchRec& p, const void* recs, const BatchMap& map, int grid, size_t lds,
                          hipStream_t s, bool dry);
+
+// The Poisson interior pass of a batch whose members share one element
+// geometry (equal element widths, orders and charges-free operator): the
+// element inverses G_e and the row tables at (L, point) are then the same
+// for every member and are read from one copy of the largest extent.
+// Items are (member, pair) pairs in groups of kPoissonSlots (member -1: an
+// empty slot); a job is (group, L, element chunk) and runs as one block.
+constexpr int kPoissonSlots = 8;
+constexpr int kPoissonInteriorThreads = 256;  // threads of the interior passes
+struct PoissonShared {
+  const double *G, *Wl, *Wr, *invr, *rpl1, *fac2;
+  int gstride;  // elements per L in G
+  int rstride;  // points per L in the row tables
+  int epb;      // elements per block (every member's)
+};
+// type: the site's type code (0: complex128, 2: complex64); false for others
+bool launch_poisson_interior_grouped(int type, const void* recs, const int2* items, const int* jobs, int njobs,
+                                     const PoissonShared& sh, hipStream_t s);
 // sites whose blocks return at once for a converged member (the later terms
 // of the half-step series): launched member-major (BatchMap::per_member), so
 // that a converged member costs one flag read per hardware block

@@ -102,6 +102,9 @@ def test_batch_equals_members_alone(gpu_lib, precision, graph):
     # interior inverses (the one-body element matrices differ in their last
     # bits with the element's position, so they are shared only where equal)
     assert info["shared_tables"]["cn"] == 4 and info["shared_tables"]["poisson"] == 4, info
+    # and the Poisson row tables: the interior pass runs over groups of
+    # (member, pair) items reading one copy of the tables
+    assert info["shared_tables"]["poisson_rows"] == 4 and info["poisson_groups"] == 1, info
     for hp in hps:
         hp.close()
 
@@ -132,6 +135,8 @@ def test_batch_of_three_orbital_members(gpu_lib):
     # the Crank-Nicolson inverses depend on the charge: only the first member
     # reads the shared copy; the Poisson inverses do not
     assert info["shared_tables"]["cn"] == 1 and info["shared_tables"]["poisson"] == 3, info
+    # several pairs per member (m_j - m_i up to 2: the L < |m| rows are zero)
+    assert info["shared_tables"]["poisson_rows"] == 3 and info["poisson_groups"] >= 3, info
     for hp in hps:
         hp.close()
 
