@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <array>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -280,11 +281,13 @@ struct td1c_batch {
     int max_blocks = 0;  // blocks of the largest member
     size_t lds = 0;
     size_t off_recs = 0, off_start = 0, off_dims = 0;
-    // the Poisson interior pass over groups of (member, pair) items
-    // reading the shared tables (launch_poisson_interior_grouped)
+    // interior passes over groups of items reading the shared tables:
+    // 1 the Poisson solve (launch_poisson_interior_grouped), 2 the clh1itr
+    // half step (launch_cn_interior_grouped)
     int grouped = 0, njobs = 0;
     size_t off_items = 0, off_jobs = 0;
     PoissonShared psh{};
+    CnShared csh{};
   };
   std::vector<Site> sites;
   DevBuf<unsigned char> table;
@@ -299,7 +302,8 @@ struct td1c_batch {
   DevBuf<double> u_prow[5];
   int rstride = 0;
   int shared[4] = {0, 0, 0, 0};  // members reading each shared table (3: the Poisson row tables)
-  int pgroups = 0;                // item groups of the grouped Poisson interior pass
+  int pgroups = 0, cgroups = 0;   // item groups of the grouped Poisson and half-step interior passes
+  DevBuf<int> cn_word;            // the half-step series' word (SeriesGate)
   std::vector<unsigned char> key;  // what the recorded step depends on
   hipStream_t cap_stream = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -314,6 +318,7 @@ struct td1c_batch {
     u_eK.release();
     u_eP.release();
     for (auto& t : u_prow) t.release();
+    cn_word.release();
   }
 };
 
@@ -3401,6 +3406,30 @@ int batch_record(td1c_batch* b, const std::vector<Pulse>& pulses, int rk_order, 
   // the old tables may still be read by queued launches
   HIP_TRY(hipDeviceSynchronize());
   if (int rc = batch_share_tables(b, recs)) return rc;
+  // the terms of each half-step series: the bridge pass of the first term
+  // (mode 0) opens a series, each later bridge pass starts the next term; the
+  // right-hand side of a later term (cn = 2) precedes its bridge pass
+  if (!b->cn_word.p) {
+    HIP_TRY(b->cn_word.alloc(1));
+    HIP_TRY(hipMemset(b->cn_word.p, 0, sizeof(int)));
+  }
+  for (int m = 0; m < nmem; ++m) {
+    int term = 0;
+    for (LaunchRec& L : recs[m]) {
+      if (L.kind == kRecCnBridge || L.kind == kRecCnInterior || L.kind == kRecCnCheck) {
+        CnRec r;
+        std::memcpy(&r, L.args.data(), sizeof r);
+        if (L.kind == kRecCnBridge) term = r.mode == 0 ? 1 : term + 1;
+        r.gate = SeriesGate{b->cn_word.p, term};
+        std::memcpy(L.args.data(), &r, sizeof r);
+      } else if (L.kind == kRecOneBody && L.var[1] == 2) {
+        OneBodyRec r;
+        std::memcpy(&r, L.args.data(), sizeof r);
+        r.gate = SeriesGate{b->cn_word.p, term + 1};
+        std::memcpy(L.args.data(), &r, sizeof r);
+      }
+    }
+  }
   auto align = [](size_t x) { return (x + 255) / 256 * 256; };
   std::vector<unsigned char> host;
   std::map<std::string, size_t> seen;
@@ -3455,15 +3484,57 @@ int batch_record(td1c_batch* b, const std::vector<Pulse>& pulses, int rk_order, 
     st.off_start = align(off + nrec);
     st.off_dims = align(st.off_start + start.size() * sizeof(int));
   }
-  // Poisson interior sites over shared tables: (member, pair) items in
-  // groups of kPoissonSlots, ordered by extent (L2, element chunks) so that
-  // a group's items end near the same (L, chunk); jobs (group, L, chunk)
-  // for every (L, chunk) that some item of the group covers
-  b->pgroups = 0;
-  if (b->shared[3] == nmem) {
-    for (size_t i = 0; i < nsite; ++i) {
-      td1c_batch::Site& st = sites[i];
-      if (st.proto.kind != kRecPoissonInterior || (st.proto.var[0] != 0 && st.proto.var[0] != 2)) continue;
+  // Interior passes over shared tables (the Poisson solve, the clh1itr
+  // half step): items (member, pair or orbital) in groups, ordered by extent
+  // (L, element chunks) so that a group's items end near the same
+  // (L, chunk); jobs (group, L, chunk) for every (L, chunk) that some item of
+  // the group covers.  Sites with the same items share one layout.
+  struct Item { int m, p, L, nch; };
+  std::map<std::vector<int>, std::array<size_t, 4>> layouts;  // items -> off_items, off_jobs, njobs, ngroups
+  auto layout = [&](std::vector<Item> items, int slots) -> std::array<size_t, 4> {
+    std::stable_sort(items.begin(), items.end(),
+                     [](const Item& x, const Item& y) { return x.L != y.L ? x.L > y.L : x.nch > y.nch; });
+    std::vector<int> key{slots};
+    for (const Item& it : items) key.insert(key.end(), {it.m, it.p, it.L, it.nch});
+    auto f = layouts.find(key);
+    if (f != layouts.end()) return f->second;
+    const int ng = ((int)items.size() + slots - 1) / slots;
+    std::vector<int2> sl((size_t)ng * slots, make_int2(-1, 0));
+    std::vector<int> gL(ng, 0), gC(ng, 0);
+    int nL = 0, nC = 0;
+    for (size_t k = 0; k < items.size(); ++k) {
+      const int g = (int)k / slots;
+      sl[k] = make_int2(items[k].m, items[k].p);
+      gL[g] = std::max(gL[g], items[k].L);
+      gC[g] = std::max(gC[g], items[k].nch);
+      nL = std::max(nL, gL[g]);
+      nC = std::max(nC, gC[g]);
+    }
+    std::vector<int> jobs;
+    for (int L = 0; L < nL; ++L)
+      for (int c = 0; c < nC; ++c)
+        for (int g = 0; g < ng; ++g)
+          if (L < gL[g] && c < gC[g]) jobs.insert(jobs.end(), {g, L, c});
+    std::array<size_t, 4> out;
+    out[0] = align(host.size());
+    host.resize(out[0]);
+    host.insert(host.end(), reinterpret_cast<const unsigned char*>(sl.data()),
+                reinterpret_cast<const unsigned char*>(sl.data() + sl.size()));
+    out[1] = align(host.size());
+    host.resize(out[1]);
+    host.insert(host.end(), reinterpret_cast<const unsigned char*>(jobs.data()),
+                reinterpret_cast<const unsigned char*>(jobs.data() + jobs.size()));
+    host.resize(align(host.size()));
+    out[2] = jobs.size() / 3;
+    out[3] = (size_t)ng;
+    layouts.emplace(std::move(key), out);
+    return out;
+  };
+  b->pgroups = b->cgroups = 0;
+  for (size_t i = 0; i < nsite; ++i) {
+    td1c_batch::Site& st = sites[i];
+    const int kind = st.proto.kind;
+    if (kind == kRecPoissonInterior && b->shared[3] == nmem && (st.proto.var[0] == 0 || st.proto.var[0] == 2)) {
       std::vector<PoissonRec> pr(nmem);
       bool ok = true;
       for (int m = 0; m < nmem && ok; ++m) {
@@ -3473,49 +3544,39 @@ int batch_record(td1c_batch* b, const std::vector<Pulse>& pulses, int rk_order, 
              r.epb * 19 <= kPoissonInteriorThreads && r.a.nfe * r.a.n - 1 == r.a.nr && r.a.nr <= b->rstride;
       }
       if (!ok) continue;
-      struct Item { int m, p, L2, nch; };
       std::vector<Item> items;
       for (int m = 0; m < nmem; ++m)
         for (int p = 0; p < pr[m].a.npair; ++p) items.push_back({m, p, pr[m].a.L2, pr[m].nchunk});
-      std::stable_sort(items.begin(), items.end(), [](const Item& x, const Item& y) {
-        return x.L2 != y.L2 ? x.L2 > y.L2 : x.nch > y.nch;
-      });
-      const int ng = ((int)items.size() + kPoissonSlots - 1) / kPoissonSlots;
-      std::vector<int2> slots((size_t)ng * kPoissonSlots, make_int2(-1, 0));
-      std::vector<int> gL(ng, 0), gC(ng, 0);
-      int nL = 0, nC = 0;
-      for (size_t k = 0; k < items.size(); ++k) {
-        const int g = (int)k / kPoissonSlots;
-        slots[k] = make_int2(items[k].m, items[k].p);
-        gL[g] = std::max(gL[g], items[k].L2);
-        gC[g] = std::max(gC[g], items[k].nch);
-        nL = std::max(nL, gL[g]);
-        nC = std::max(nC, gC[g]);
-      }
-      std::vector<int> jobs;
-      for (int L = 0; L < nL; ++L)
-        for (int c = 0; c < nC; ++c)
-          for (int g = 0; g < ng; ++g)
-            if (L < gL[g] && c < gC[g]) {
-              jobs.push_back(g);
-              jobs.push_back(L);
-              jobs.push_back(c);
-            }
-      if (jobs.size() / 3 > 0x7fffffffULL) continue;
-      st.off_items = align(host.size());
-      host.resize(st.off_items);
-      host.insert(host.end(), reinterpret_cast<const unsigned char*>(slots.data()),
-                  reinterpret_cast<const unsigned char*>(slots.data() + slots.size()));
-      st.off_jobs = align(host.size());
-      host.resize(st.off_jobs);
-      host.insert(host.end(), reinterpret_cast<const unsigned char*>(jobs.data()),
-                  reinterpret_cast<const unsigned char*>(jobs.data() + jobs.size()));
-      host.resize(align(host.size()));
+      const auto lay = layout(std::move(items), kPoissonSlots);
+      if (lay[2] > 0x7fffffffULL) continue;
       st.grouped = 1;
-      st.njobs = (int)(jobs.size() / 3);
+      st.off_items = lay[0];
+      st.off_jobs = lay[1];
+      st.njobs = (int)lay[2];
       st.psh = PoissonShared{b->u_pG.p, b->u_prow[0].p, b->u_prow[1].p, b->u_prow[2].p, b->u_prow[3].p,
                              b->u_prow[4].p, pr[0].a.gstride, b->rstride, pr[0].epb};
-      b->pgroups = ng;
+      b->pgroups = (int)lay[3];
+    } else if (kind == kRecCnInterior && b->shared[0] == nmem && st.proto.block == kCnInteriorThreads) {
+      std::vector<CnRec> cr(nmem);
+      bool ok = true;
+      for (int m = 0; m < nmem && ok; ++m) {
+        std::memcpy(&cr[m], recs[m][i].args.data(), sizeof(CnRec));
+        const CnRec& r = cr[m];
+        ok = r.a.ni == 19 && r.a.n == 20 && r.a.G == b->u_cnG.p && r.a.nfun - r.a.nfcore == 1 &&
+             r.epb == cr[0].epb && r.epb <= 32 && r.epb * 19 <= kCnInteriorThreads && r.a.nb == r.a.nfe - 1 &&
+             r.a.nfe <= r.a.gstride && r.a.gstride == cr[0].a.gstride;
+      }
+      if (!ok) continue;
+      std::vector<Item> items;
+      for (int m = 0; m < nmem; ++m) items.push_back({m, 0, cr[m].a.L1, cr[m].nchunk});
+      const auto lay = layout(std::move(items), kCnGroup);
+      if (lay[2] > 0x7fffffffULL) continue;
+      st.grouped = 2;
+      st.off_items = lay[0];
+      st.off_jobs = lay[1];
+      st.njobs = (int)lay[2];
+      st.csh = CnShared{b->u_cnG.p, cr[0].a.gstride, cr[0].epb, cr[0].gate};
+      b->cgroups = (int)lay[3];
     }
   }
   // the old table may still be read by queued launches
@@ -3539,10 +3600,15 @@ hipError_t batch_launch_step(td1c_batch* b, hipStream_t s) {
     BatchMap map{reinterpret_cast<const int*>(base + st.off_start), reinterpret_cast<const int*>(base + st.off_dims),
                  nmem, 0};
     const void* recs = base + st.off_recs;
-    if (st.grouped) {
+    if (st.grouped == 1) {
       if (!launch_poisson_interior_grouped(st.proto.var[0], recs, reinterpret_cast<const int2*>(base + st.off_items),
                                            reinterpret_cast<const int*>(base + st.off_jobs), st.njobs, st.psh, s))
         return hipErrorInvalidValue;
+      continue;
+    }
+    if (st.grouped == 2) {
+      launch_cn_interior_grouped(recs, reinterpret_cast<const int2*>(base + st.off_items),
+                                 reinterpret_cast<const int*>(base + st.off_jobs), st.njobs, st.csh, s);
       continue;
     }
     int grid = st.total;
@@ -3700,6 +3766,7 @@ int td1c_batch_get_info(const td1c_batch* b, char* buf, int buflen) {
                    std::to_string(b->shared[0]) + ", \"poisson\": " + std::to_string(b->shared[1]) +
                    ", \"onebody\": " + std::to_string(b->shared[2]) + ", \"poisson_rows\": " +
                    std::to_string(b->shared[3]) + "}, \"poisson_groups\": " + std::to_string(b->pgroups) +
+                   ", \"cn_groups\": " + std::to_string(b->cgroups) +
                    (b->tune_log.empty() ? std::string() : ", \"tune\": " + b->tune_log) + "}";
   if ((int)js.size() + 1 > buflen) return fail(-2, "buffer too small");
   std::memcpy(buf, js.c_str(), js.size() + 1);

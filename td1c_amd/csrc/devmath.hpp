@@ -68,5 +68,12 @@ __device__ __forceinline__ int xcd_remap(int lin, int n) {
   const int x = lin & 7, j = lin >> 3, q = n >> 3, r = n & 7;
   return x * q + min(x, r) + j;
 }
+// The inverse of xcd_remap: the linear id whose logical tile is `job`.
+__device__ __forceinline__ int xcd_unmap(int job, int n) {
+  const int q = n >> 3, r = n & 7, head = r * (q + 1);
+  if (job < head) return (job % (q + 1)) * 8 + job / (q + 1);
+  const int d = job - head;
+  return (d % q) * 8 + r + d / q;
+}
 
 }  // namespace td1c

@@ -386,7 +386,10 @@ void launch_cn_stage(const CnArgs& a, const OneBodyElArgs* ob, int shape, int lc
 __global__ void __launch_bounds__(256) k_cn_bridge_batch(const CnRec* __restrict__ recs, const BatchMap map) {
   __shared__ __align__(16) unsigned char raw[sizeof(CnRec)];
   int have = -1;
-  batch_walk(map, [&](int m) { return recs[m].mode == 1 && recs[m].a.flag[0] != 0; }, [&](int m, const Blk& bk) {
+  // the first term opens the series' word (read by no launch of this term)
+  if (blockIdx.x == 0 && threadIdx.x == 0 && recs[0].mode == 0 && recs[0].gate.word) *recs[0].gate.word = 0;
+  batch_walk(map, [&](int m) { return recs[m].mode == 1 && (gate_closed(recs[m].gate) || recs[m].a.flag[0] != 0); },
+             [&](int m, const Blk& bk) {
     const CnRec& r = stage_rec(recs, m, raw, have);
     cn_bridge_body<256>(r.a, r.mode, bk);
   });
@@ -396,7 +399,8 @@ __global__ void __launch_bounds__(kCnInteriorThreads) k_cn_interior_batch(const 
                                                                           const BatchMap map) {
   __shared__ __align__(16) unsigned char raw[sizeof(CnRec)];
   int have = -1;
-  batch_walk(map, [&](int m) { return recs[m].mode == 1 && recs[m].a.flag[0] != 0; }, [&](int m, const Blk& bk) {
+  batch_walk(map, [&](int m) { return recs[m].mode == 1 && (gate_closed(recs[m].gate) || recs[m].a.flag[0] != 0); },
+             [&](int m, const Blk& bk) {
     const CnRec& r = stage_rec(recs, m, raw, have);
     cn_interior_body<19>(r.a, r.mode, r.epb, r.nchunk, bk);
   });
@@ -409,7 +413,119 @@ __global__ void __launch_bounds__(kCnInteriorThreads) k_cn_check_batch(const CnR
   batch_walk(map, [&](int m) { return recs[m].mode == 1 && recs[m].a.flag[0] != 0; }, [&](int m, const Blk& bk) {
     const CnRec& r = stage_rec(recs, m, raw, have);
     cn_check_body(r.a, r.mode, r.a.L1 * r.nchunk);
+    // (no gate on this launch: its blocks write the word that the gate reads)
+    if (threadIdx.x == 0 && r.gate.word && r.a.flag[0] == 0) *r.gate.word = r.gate.term;
   });
+}
+
+// One block per job (group g, l, element chunk) over the group's kCnGroup
+// (member, orbital) items: cn_interior_body with the row of G_e loaded once
+// for all items (the shared table holds every member's own numbers), each
+// item's block norm reduced in the order of the member's own pass and
+// written to the member's partial-sum slot of that (l, chunk) block.
+// Items of a member whose series has converged (mode 1, flag set) or
+// outside its extent are passed over.
+__global__ void __launch_bounds__(kCnInteriorThreads)
+    k_cn_interior_grouped(const CnRec* __restrict__ recs, const int2* __restrict__ items,
+                          const int* __restrict__ jobs, const CnShared sh) {
+  constexpr int S = kCnGroup, NI = 19, n = 20, BS = kCnInteriorThreads;
+  if (gate_closed(sh.gate)) return;
+  const int job = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+  const int g = jobs[3 * job], l = jobs[3 * job + 1], ch = jobs[3 * job + 2];
+  const int epb = sh.epb, e0 = ch * epb;
+  const int t = threadIdx.x;
+  const int el = t / NI, i = t - el * NI;
+  const int e = e0 + el;
+  const int r = e * n + i;
+  __shared__ double2 sb[S][BS];
+  __shared__ double2 sx[S][34];  // bridges e0-1 .. e0+epb-1 (epb <= 32)
+  __shared__ double sRed[S][BS / 64];
+  __shared__ int sstate[S], snfe[S], smode[S];  // state 0: pass over, 1: norm 0 only (l < |m|), 2: solve
+  __shared__ const double2 *sy[S], *sWl[S], *sWr[S], *sxb[S];
+  __shared__ double2 *sxo[S], *swf[S];
+  __shared__ double* spart[S];
+  if (t < S) {
+    const int2 it = items[g * S + t];
+    int st = 0;
+    if (it.x >= 0) {
+      const CnRec& rc = recs[it.x];
+      const CnArgs& a = rc.a;
+      if (!(rc.mode == 1 && a.flag[0]) && l < a.L1 && ch < rc.nchunk) {
+        const int fd = it.y, m = a.mval[a.nfcore + fd];
+        st = l >= (m < 0 ? -m : m) ? 2 : 1;
+        const size_t off = ((size_t)fd * a.L1 + l) * a.nr;
+        snfe[t] = a.nfe;
+        smode[t] = rc.mode;
+        sy[t] = a.y + off;
+        sxo[t] = a.x + off;
+        sxb[t] = a.x + off + (n - 1);  // bridge b at sxb[b * n]
+        swf[t] = a.wfn + ((size_t)(a.nfcore + fd) * a.L1 + l) * a.nr;
+        sWl[t] = a.Wl + (size_t)l * a.nr;
+        sWr[t] = a.Wr + (size_t)l * a.nr;
+        spart[t] = a.part + a.npart1 + xcd_unmap(l * rc.nchunk + ch, a.L1 * rc.nchunk);
+      }
+    }
+    sstate[t] = st;
+  }
+  __syncthreads();
+  const bool in_blk = el < epb && e < sh.gstride;
+#pragma unroll
+  for (int j = 0; j < S; ++j)
+    if (sstate[j] == 2 && in_blk && e < snfe[j]) sb[j][t] = sy[j][r];
+  for (int q = t; q < S * (epb + 1); q += BS) {
+    const int j = q / (epb + 1), k = q - j * (epb + 1);
+    const int bidx = e0 - 1 + k;
+    sx[j][k] = (sstate[j] == 2 && bidx >= 0 && bidx < snfe[j] - 1) ? sxb[j][(size_t)bidx * n]
+                                                                   : make_double2(0.0, 0.0);
+  }
+  double2 gk[NI];
+  if (in_blk) {
+    const double2* ge = sh.G + (((size_t)l * sh.gstride) + e) * NI * NI + i;
+#pragma unroll
+    for (int k = 0; k < NI; ++k) gk[k] = ge[k * NI];
+  }
+  __syncthreads();
+  double nrm[S];
+#pragma unroll
+  for (int j = 0; j < S; ++j) nrm[j] = 0.0;
+  if (in_blk) {
+    const int eb = el * NI;
+#pragma unroll
+    for (int j = 0; j < S; ++j) {
+      const int nfe = snfe[j];
+      if (sstate[j] != 2 || e >= nfe) continue;
+      double2 x = make_double2(0.0, 0.0);
+#pragma unroll
+      for (int k = 0; k < NI; ++k) cfma(x, gk[k], sb[j][eb + k]);
+      if (e >= 1) x = csub(x, cmul(sWl[j][r], sx[j][el]));
+      if (e <= nfe - 2) x = csub(x, cmul(sWr[j][r], sx[j][el + 1]));
+      sxo[j][r] = x;
+      double2* w = swf[j] + r;
+      *w = smode[j] == 0 ? x : cadd(*w, x);
+      nrm[j] += x.x * x.x + x.y * x.y;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < S; ++j) {
+    if (sstate[j] == 0) continue;
+    double v = nrm[j];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((t & 63) == 0) sRed[j][t >> 6] = v;
+  }
+  __syncthreads();
+  if (t < S && sstate[t] != 0) {
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < BS / 64; ++w) s += sRed[t][w];
+    *spart[t] = s;
+  }
+}
+
+void launch_cn_interior_grouped(const void* recs, const int2* items, const int* jobs, int njobs, const CnShared& sh,
+                                hipStream_t s) {
+  if (njobs <= 0) return;
+  hipLaunchKernelGGL(k_cn_interior_grouped, dim3(njobs), dim3(kCnInteriorThreads), 0, s,
+                     static_cast<const CnRec*>(recs), items, jobs, sh);
 }
 
 bool launch_batch_h1itr(const LaunchRec& p, const void* recs, const BatchMap& map, int grid, size_t lds,

@@ -2704,7 +2704,8 @@ __global__ void __launch_bounds__(((GE * 21 + 1) + 63) / 64 * 64)
     k_onebody_el_batch(const OneBodyRec* __restrict__ recs, const BatchMap map) {
   __shared__ __align__(16) unsigned char raw[sizeof(OneBodyRec)];
   int have = -1;
-  batch_walk(map, [&](int m) { return CN == 2 && recs[m].a.stop[0] != 0; }, [&](int m, const Blk& bk) {
+  batch_walk(map, [&](int m) { return CN == 2 && (gate_closed(recs[m].gate) || recs[m].a.stop[0] != 0); },
+             [&](int m, const Blk& bk) {
     const OneBodyRec& r = stage_rec(recs, m, raw, have);
     onebody_el_body<20, GE, CN>(r.orb, r.h01, r.a, r.lc, bk);
   });

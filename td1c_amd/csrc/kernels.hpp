@@ -461,11 +461,25 @@ struct TransformRec {
   int nx, ny, ntile;
   FcPhaseJob fc;
 };
+// In a batch, the terms of the half-step series share one word: the check
+// of term k stores k when some member has not converged, and a launch of
+// term k > 1 (right-hand side, bridge, interior, check) runs only when the
+// word holds k - 1, so that a term after the last member has converged
+// costs one load per block.  The first term's bridge pass clears it.
+struct SeriesGate {
+  int* word;  // null outside a batch
+  int term;   // the term this launch forms (1: the first)
+};
+// true when the launch is a later term of a series that has ended
+__device__ __forceinline__ bool gate_closed(const SeriesGate& g) {
+  return g.word && g.term > 1 && *g.word != g.term - 1;
+}
 struct OneBodyRec {
   const double2* orb;
   double2* h01;
   OneBodyElArgs a;
   int lc;
+  SeriesGate gate;  // cn = 2 launches
 };
 struct PoissonRec {
   const void* rho;
@@ -498,6 +512,7 @@ struct FieldRec {
 struct CnRec {
   CnArgs a;
   int mode, epb, nchunk;
+  SeriesGate gate;
 };
 
 struct LaunchRec {
@@ -562,6 +577,18 @@ struct PoissonShared {
 // type: the site's type code (0: complex128, 2: complex64); false for others
 bool launch_poisson_interior_grouped(int type, const void* recs, const int2* items, const int* jobs, int njobs,
                                      const PoissonShared& sh, hipStream_t s);
+// The interior pass of the clh1itr half step in the same form: items are
+// (member, dynamical orbital) pairs in groups of kCnGroup over the shared
+// element inverses; each member must have one dynamical orbital (its block
+// norm is then the item's, written where the member's own pass writes it).
+struct CnShared {
+  const double2* G;
+  int gstride;  // elements per l in G
+  int epb;      // elements per block (every member's)
+  SeriesGate gate;
+};
+void launch_cn_interior_grouped(const void* recs, const int2* items, const int* jobs, int njobs, const CnShared& sh,
+                                hipStream_t s);
 // sites whose blocks return at once for a converged member (the later terms
 // of the half-step series): launched member-major (BatchMap::per_member), so
 // that a converged member costs one flag read per hardware block

@@ -105,6 +105,8 @@ def test_batch_equals_members_alone(gpu_lib, precision, graph):
     # and the Poisson row tables: the interior pass runs over groups of
     # (member, pair) items reading one copy of the tables
     assert info["shared_tables"]["poisson_rows"] == 4 and info["poisson_groups"] == 1, info
+    # one dynamical orbital each: the half-step interior pass is grouped too
+    assert info["cn_groups"] == 1, info
     for hp in hps:
         hp.close()
 
@@ -137,6 +139,7 @@ def test_batch_of_three_orbital_members(gpu_lib):
     assert info["shared_tables"]["cn"] == 1 and info["shared_tables"]["poisson"] == 3, info
     # several pairs per member (m_j - m_i up to 2: the L < |m| rows are zero)
     assert info["shared_tables"]["poisson_rows"] == 3 and info["poisson_groups"] >= 3, info
+    assert info["cn_groups"] == 0, info  # three dynamical orbitals: each member's own half-step blocks
     for hp in hps:
         hp.close()
 
