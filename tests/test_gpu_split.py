@@ -73,6 +73,35 @@ def test_h1itr_half_step_matches_oracle(gpu_lib, name, lf):
         assert np.array_equal(got[hp.norb:], w[hp.norb:])
 
 
+@pytest.mark.parametrize("name,lf,scale,ncyc", [("ne_small", C.LF_VEL, 10.0, 12), ("ne_small", C.LF_VEL, 30.0, 20),
+                                                ("dc3_small", C.LF_LEN, 100.0, 7), ("he_small", C.LF_VEL, 100.0, 20)])
+def test_long_series_match_oracle(gpu_lib, name, lf, scale, ncyc):
+    """Half steps whose series runs 7 to 20 terms (fields scaled up; 20 is
+    h1itr_maxcyc, a series that stops unconverged): the device convergence
+    test stops where h1itr_prop's does, and the state agrees (1e-11)."""
+    import torch
+    import td1c_amd
+    from oracle.oracle import Oracle
+
+    prob = getattr(C, name)(isplit=1)
+    lfs = [x * scale for x in lf]
+    o = Oracle(prob)
+    w = C.wfn_for(prob, o.lcic)
+    o.set_wfn0(w)
+    o.h1itr_gen(DT / 2)
+    ref, n = o.h1itr_prop(lfs, DT / 2, w)
+    assert n == ncyc
+    hp = td1c_amd.HProd(prob)
+    hp.set_wfn0(w)
+    hp.h1itr_gen(DT)
+    d = hp.to_device(w)
+    hp.h1itr_prop(lfs, d)
+    torch.cuda.synchronize()
+    got = d.cpu().numpy()
+    hp.close()
+    assert _rel(got, ref) < 1e-11, _rel(got, ref)
+
+
 @pytest.mark.parametrize("graph", [False, True])
 @pytest.mark.parametrize("name", ["ne_small", "cas_fc_small", "dc3_small"])
 def test_split2_propagation_matches_oracle(gpu_lib, name, graph):
