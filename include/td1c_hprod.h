@@ -491,7 +491,10 @@ int td1c_batch_autotune(td1c_batch* batch, const td1c_field* fields, int rk_orde
                         double* const* d_scratch, void* stream);
 
 /* JSON: members, launch sites and blocks per step, table bytes, shapes, the
- * members reading shared element tables, the autotuner's timings. */
+ * members reading shared element tables (shared_tables: cn, poisson,
+ * onebody, poisson_rows), the item groups of the grouped Poisson and
+ * half-step interior passes (poisson_groups, cn_groups; 0 when not
+ * grouped), the autotuner's timings. */
 int td1c_batch_get_info(const td1c_batch* batch, char* buf, int buflen);
 
 /* How td1c_propagate launches its steps: 0 (default) enqueues every kernel
