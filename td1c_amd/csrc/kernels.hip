@@ -545,12 +545,28 @@ __device__ __forceinline__ void onebody_el_body(const double2* __restrict__ orb,
       sPsi[buf][q] = v;
     }
   };
+  // plane l + 1 is read from global memory one trip ahead, into registers,
+  // so that its latency overlaps the previous trip's products
+  constexpr int RP = (TILE + BS - 1) / BS;
+  double2 nx[RP];
+  auto fetch = [&](int l) {
+#pragma unroll
+    for (int j = 0; j < RP; ++j) {
+      const int q = tid + j * BS, gi = ibase + q;
+      nx[j] = (q < TILE && l >= 0 && l < a.L1 && gi >= 0 && gi < a.nr) ? base_f[(size_t)l * a.nr + gi]
+                                                                       : make_double2(0.0, 0.0);
+    }
+  };
   load_tile(l0, l0 % 3);
   load_tile(l0 - 1, (l0 + 2) % 3);
+  fetch(l0 + 1);
   for (int l = l0; l < l1; ++l) {
     const int b0 = l % 3, b1 = (l + 1) % 3, bm = (l + 2) % 3;
     __syncthreads();  // the previous trip's reads of buffer b1 are done
-    load_tile(l + 1, b1);
+#pragma unroll
+    for (int j = 0; j < RP; ++j)
+      if (tid + j * BS < TILE) sPsi[b1][tid + j * BS] = nx[j];
+    if (l + 1 < l1) fetch(l + 2);
     __syncthreads();
     // element-row partial products
     double2 pT = make_double2(0.0, 0.0), pP1 = make_double2(0.0, 0.0), pP0 = make_double2(0.0, 0.0);
