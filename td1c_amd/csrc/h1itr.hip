@@ -197,9 +197,8 @@ __global__ void __launch_bounds__(BS) k_cn_bridge(const CnArgs a, int mode) {
 // Interior pass, one block per (l, chunk of epb elements), thread per
 // interior row, every dynamical orbital in groups of kCnGroup: x_I = G_e y_I
 // - W_l x_left - W_r x_right with the row of G_e loaded once into registers
-// for all orbitals, into x and the state.  The last block to finish sums
-// the partial norms of both passes in a fixed order and sets flag[0] when
-// the RMS orbital norm of this term is below thresh (the test at the top of
+// for all orbitals, into x and the state, and the block's share of sum
+// |x|^2 into the partial sums that k_cn_check adds (the test at the top of
 // the next cycle of h1itr_prop.F90:75-99).
 template <int NI>
 __device__ __forceinline__ void cn_interior_body(const CnArgs& a, int mode, int epb, int nchunk, const Blk bk) {
