@@ -111,6 +111,23 @@ def test_batch_equals_members_alone(gpu_lib, precision, graph):
         hp.close()
 
 
+def test_batch_of_members_of_other_element_widths(gpu_lib):
+    """Members whose elements differ in width (rmax = 2 nfe and 3 nfe):
+    their element tables differ, so only some members share them and the
+    interior passes run per member (no item groups); the batch still equals
+    the members alone."""
+    probs = [_he(6, 3), _he(8, 4), Problem(nfe=7, ndvr=20, rmax=21.0, lmax1=5, lmax2=10, znuc=2.0, ndcore=1,
+                                         nel_alpha=1, nel_beta=1, igauge=1, isplit=1)]
+    info = {}
+    hps, ws, fields, got, ref = _run_pair(probs, True, info=info)
+    for m, (g, r) in enumerate(zip(got, ref)):
+        assert _rel(g, r) < 1e-13, (m, _rel(g, r))
+    assert info["shared_tables"]["poisson"] < 3 and info["poisson_groups"] == 0, info
+    assert info["shared_tables"]["cn"] < 3 and info["cn_groups"] == 0, info
+    for hp in hps:
+        hp.close()
+
+
 def test_batch_autotuned_shapes_give_the_same_bits(gpu_lib):
     """The batch autotuner (every element one-body shape, both transform
     variants) on scratch copies, then two batch calls equal the members
