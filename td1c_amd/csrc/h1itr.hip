@@ -446,6 +446,7 @@ __global__ void __launch_bounds__(kCnInteriorThreads)
   if (t < S) {
     const int2 it = items[g * S + t];
     int st = 0;
+    snfe[t] = 0;
     if (it.x >= 0) {
       const CnRec& rc = recs[it.x];
       const CnArgs& a = rc.a;
