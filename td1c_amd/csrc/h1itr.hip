@@ -120,6 +120,8 @@ template <int BS>
 __device__ __forceinline__ void cn_bridge_body(const CnArgs& a, int mode, const Blk bk) {
   if (mode == 1 && a.flag[0]) return;
   const int tid = threadIdx.x;
+  // a new series: no launch of the first term reads the flag
+  if (mode == 0 && bk.x == 0 && tid == 0) a.flag[0] = 0;
   const int job = bk.x;  // fd * L1 + l
   const int fd = job / a.L1, l = job - fd * a.L1;
   const int f = a.nfcore + fd;
@@ -347,6 +349,11 @@ void launch_cn_stage(const CnArgs& a, const OneBodyElArgs* ob, int shape, int lc
     }
     return;
   }
+  // with the element right-hand side (stream launches) each term's
+  // convergence test runs at the start of the next term's right-hand side
+  // (the test at the top of h1itr_prop's next cycle) instead of as a launch
+  // of its own; a batch records the test as a launch (its series word)
+  const bool fused = ob != nullptr && !active_recorder();
   if (mode == 2) {  // a plain solve: y = zfac0 psi, then the first term
     if (active_recorder()) return record_unsupported("a rational-function solve");
     launch_zaxpby((size_t)nd * a.L1 * a.nr, a.zfac0, a.wfn + (size_t)a.nfcore * a.L1 * a.nr, make_double2(0.0, 0.0),
@@ -355,6 +362,14 @@ void launch_cn_stage(const CnArgs& a, const OneBodyElArgs* ob, int shape, int lc
   } else if (ob) {  // element form of the right-hand side (ndvr = 20)
     OneBodyElArgs e = *ob;
     e.cn = mode + 1;
+    if (mode == 1 && fused) {  // the previous term's convergence test first
+      e.chk_part = a.part;
+      e.chk_n = a.npart1 + a.L1 * ((a.nfe + std::min(32, kCnInteriorThreads / a.ni) - 1) /
+                                   std::min(32, kCnInteriorThreads / a.ni));
+      e.chk_nfun = a.nfun;
+      e.chk_thresh = a.thresh;
+      e.chk_flag = a.flag;
+    }
     launch_onebody_el(mode == 0 ? a.wfn + (size_t)a.nfcore * a.L1 * a.nr : a.x, a.y, e, s, shape, lc);
   } else {
     if (active_recorder()) return record_unsupported("the band right-hand side of the half step (ndvr != 20)");
@@ -379,7 +394,7 @@ void launch_cn_stage(const CnArgs& a, const OneBodyElArgs* ob, int shape, int lc
     hipLaunchKernelGGL((k_cn_interior<19>), grid, dim3(kCnInteriorThreads), 0, s, a, mode, epb, nchunk);
   else
     hipLaunchKernelGGL((k_cn_interior<0>), grid, dim3(kCnInteriorThreads), 0, s, a, mode, epb, nchunk);
-  hipLaunchKernelGGL(k_cn_check, dim3(1), dim3(kCnInteriorThreads), 0, s, a, mode, (int)grid.x);
+  if (!fused) hipLaunchKernelGGL(k_cn_check, dim3(1), dim3(kCnInteriorThreads), 0, s, a, mode, (int)grid.x);
 }
 
 __global__ void __launch_bounds__(256) k_cn_bridge_batch(const CnRec* __restrict__ recs, const BatchMap map) {

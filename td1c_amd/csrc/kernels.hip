@@ -486,9 +486,42 @@ __global__ void __launch_bounds__(256) k_onebody(const double2* __restrict__ orb
 // shuffles instead of LDS.  The (GE, lc) pair is a launch shape chosen per
 // problem by the autotuner; every shape gives the same bits.  N = ndvr.
 // ---------------------------------------------------------------------------
+// The convergence test of a clh1itr term (k_cn_check's arithmetic) in a
+// block of BS threads: the partial sums added by 256 lanes in k_cn_check's
+// order (lane v takes v, v + 256, ...; a butterfly in each wave of 64
+// lanes; the four wave sums in order), so that every block size decides
+// with the same bits.  Every thread of the block gets the result.
+template <int BS>
+__device__ __forceinline__ bool cn_series_done(const double* __restrict__ part, int ntot, int nfun, double thresh) {
+  constexpr int V = 256;
+  static_assert(V % BS == 0 && BS % 64 == 0, "block of whole waves dividing 256");
+  __shared__ double sw[V / 64];
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int j = 0; j < V / BS; ++j) {
+    const int v = t + j * BS;
+    double s = 0.0;
+    for (int q = v; q < ntot; q += V) s += part[q];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if ((v & 63) == 0) sw[v >> 6] = s;
+  }
+  __syncthreads();
+  double tot = 0.0;
+#pragma unroll
+  for (int w = 0; w < V / 64; ++w) tot += sw[w];
+  return sqrt(tot / nfun) < thresh;
+}
+
 template <int N, int GE, int CN>
 __device__ __forceinline__ void onebody_el_body(const double2* __restrict__ orb, double2* __restrict__ h01, const OneBodyElArgs& a, int lc, const Blk bk) {
   if (CN == 2 && a.stop[0]) return;
+  if (CN == 2 && a.chk_part) {
+    constexpr int BSC = ((GE * (N + 1) + 1) + 63) / 64 * 64;
+    if (cn_series_done<BSC>(a.chk_part, a.chk_n, a.chk_nfun, a.chk_thresh)) {
+      if (bk.x == 0 && bk.y == 0 && bk.z == 0 && threadIdx.x == 0) *a.chk_flag = 1;
+      return;
+    }
+  }
   const double2 zf = make_double2(a.lf[2], 0.0), ef = make_double2(a.lf[5], 0.0);
   constexpr int NP = N + 1;
   constexpr int BS = ((GE * NP + 1) + 63) / 64 * 64;

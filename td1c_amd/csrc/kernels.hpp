@@ -67,6 +67,14 @@ struct OneBodyElArgs {
   int cn, fshift, isplit;
   double2 cv;
   const int* stop;
+  // cn = 2 with chk_part set: first the convergence test of the previous
+  // term (the sum of its chk_n block partial norms in k_cn_check's order,
+  // rms over chk_nfun orbitals below chk_thresh): when it passes, chk_flag
+  // is set (by one block) and the launch returns
+  const double* chk_part;
+  int chk_n, chk_nfun;
+  double chk_thresh;
+  int* chk_flag;
 };
 
 // One half step of clh1itr (split_type = 1; h1itr_prop,
