@@ -102,6 +102,42 @@ def test_long_series_match_oracle(gpu_lib, name, lf, scale, ncyc):
     assert _rel(got, ref) < 1e-11, _rel(got, ref)
 
 
+@pytest.mark.parametrize("name,lf,scale", [("ne_small", C.LF_VEL, 10.0), ("dc3_small", C.LF_LEN, 1.0)])
+def test_half_step_same_bits_for_every_onebody_shape(gpu_lib, name, lf, scale):
+    """The convergence test inside the right-hand side launch sums the
+    partial norms in the same order for 64-, 128- and 256-thread blocks (3,
+    6, 12 elements per block): every element one-body shape gives the same
+    half step bit for bit, and the oracle's (1e-11)."""
+    import torch
+    import td1c_amd
+    from oracle.oracle import Oracle
+
+    prob = getattr(C, name)(isplit=1)
+    lfs = [x * scale for x in lf]
+    o = Oracle(prob)
+    w = C.wfn_for(prob, o.lcic)
+    o.set_wfn0(w)
+    o.h1itr_gen(DT / 2)
+    ref, _ = o.h1itr_prop(lfs, DT / 2, w)
+    hp = td1c_amd.HProd(prob)
+    hp.set_wfn0(w)
+    hp.h1itr_gen(DT)
+    base = None
+    for ge in (3, 6, 12):
+        for lc in (1, 4, 16):
+            hp.set_onebody_shape(ge, lc)
+            d = hp.to_device(w)
+            hp.h1itr_prop(lfs, d)
+            torch.cuda.synchronize()
+            got = d.cpu().numpy()
+            if base is None:
+                base = got
+                assert _rel(got, ref) < 1e-11
+            else:
+                assert np.array_equal(got, base), (ge, lc)
+    hp.close()
+
+
 @pytest.mark.parametrize("graph", [False, True])
 @pytest.mark.parametrize("name", ["ne_small", "cas_fc_small", "dc3_small"])
 def test_split2_propagation_matches_oracle(gpu_lib, name, graph):
