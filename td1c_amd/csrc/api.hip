@@ -128,6 +128,7 @@ struct td1c_ctx {
   // cn_dt / 2, the series terms and the convergence flag
   double cn_dt = 0.0;
   int cn_maxcyc = 20;
+  int* cn_flag_host = nullptr;  // pinned: the series' flag read back by stream launches
   double cn_thresh = 1.0e-10;
   int cn_K = 0, cn_npart1 = 0;
   DevBuf<double2> cnG, cnWl, cnWr, cnPa, cnPg, cnBinv, cn_x, cn_y;
@@ -248,6 +249,7 @@ struct td1c_ctx {
     DevBuf<float>* fm[] = {&m_g2L_f, &m_L2g_f};
     for (auto* b : fm) b->release();
     if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (cn_flag_host) (void)hipHostFree(cn_flag_host);
     if (cap_stream) (void)hipStreamDestroy(cap_stream);
     for (auto& e : fj)
       if (e) (void)hipEventDestroy(e);
@@ -2047,7 +2049,28 @@ static int enqueue_h1itr(td1c_ctx* c, const double* d_lf, double2* wfn, hipStrea
   const OneBodyElArgs* obp = c->S.n == 20 ? &ob : nullptr;
   TIMED("h1itr");
   launch_cn_stage(a, obp, c->ob_shape, c->ob_lc, 0, s);
-  for (int k = 0; k < c->cn_maxcyc; ++k) launch_cn_stage(a, obp, c->ob_shape, c->ob_lc, 1, s);
+  if (c->capturing || active_recorder()) {
+    // a captured or recorded step holds every term; the converged ones
+    // return at once
+    for (int k = 0; k < c->cn_maxcyc; ++k) launch_cn_stage(a, obp, c->ob_shape, c->ob_lc, 1, s);
+    return 0;
+  }
+  // stream launches: after the second term, and every two terms after
+  // that, the host reads the flag back (the call waits for the device to
+  // get there) and stops enqueuing once the series has converged: the later
+  // terms would all return at once, and each would still cost its launches
+  if (!c->cn_flag_host) HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&c->cn_flag_host), sizeof(int)));
+  int k = 0;
+  while (k < c->cn_maxcyc) {
+    const int chunk = k == 0 ? 1 : 2;
+    for (int j = 0; j < chunk && k < c->cn_maxcyc; ++j, ++k)
+      launch_cn_stage(a, obp, c->ob_shape, c->ob_lc, 1, s);
+    if (k >= c->cn_maxcyc) break;
+    launch_cn_check(a, s);
+    HIP_TRY(hipMemcpyAsync(c->cn_flag_host, a.flag, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (*c->cn_flag_host) break;
+  }
   return 0;
 }
 

@@ -397,6 +397,12 @@ void launch_cn_stage(const CnArgs& a, const OneBodyElArgs* ob, int shape, int lc
   if (!fused) hipLaunchKernelGGL(k_cn_check, dim3(1), dim3(kCnInteriorThreads), 0, s, a, mode, (int)grid.x);
 }
 
+void launch_cn_check(const CnArgs& a, hipStream_t s) {
+  const int epb = std::min(32, kCnInteriorThreads / a.ni);
+  const int nblk = a.L1 * ((a.nfe + epb - 1) / epb);
+  hipLaunchKernelGGL(k_cn_check, dim3(1), dim3(kCnInteriorThreads), 0, s, a, 1, nblk);
+}
+
 __global__ void __launch_bounds__(256) k_cn_bridge_batch(const CnRec* __restrict__ recs, const BatchMap map) {
   __shared__ __align__(16) unsigned char raw[sizeof(CnRec)];
   int have = -1;

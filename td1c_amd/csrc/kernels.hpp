@@ -116,6 +116,8 @@ int cn_interior_blocks(const CnArgs& a);
 // ob: the element one-body arguments for the right-hand side (ndvr = 20),
 // with its launch shape; null for the band kernel
 void launch_cn_stage(const CnArgs& a, const OneBodyElArgs* ob, int shape, int lc, int mode, hipStream_t s);
+// the convergence test of the term just enqueued (mode 1: kept set once set)
+void launch_cn_check(const CnArgs& a, hipStream_t s);
 
 struct PoissonArgs {
   int nr, n, nfe, ni, L2, npair;
