@@ -2032,6 +2032,32 @@ static CnArgs cn_args(td1c_ctx* c, const double* d_lf, double2* wfn) {
   return a;
 }
 
+// maxcyc more terms (mode 1) of a series whose first term is enqueued.
+// A captured or recorded step holds every term (the converged ones return
+// at once).  Under stream launches, after the second term and every two
+// terms after that, the host reads the flag back (the call waits for the
+// device to get there) and stops enqueuing once the series has converged:
+// the later terms would all return at once, and each would still cost its
+// launches.
+static int enqueue_series(td1c_ctx* c, const CnArgs& a, const OneBodyElArgs* obp, int maxcyc, hipStream_t s) {
+  if (c->capturing || active_recorder()) {
+    for (int k = 0; k < maxcyc; ++k) launch_cn_stage(a, obp, c->ob_shape, c->ob_lc, 1, s);
+    return 0;
+  }
+  if (!c->cn_flag_host) HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&c->cn_flag_host), sizeof(int)));
+  int k = 0;
+  while (k < maxcyc) {
+    const int chunk = k == 0 ? 1 : 2;
+    for (int j = 0; j < chunk && k < maxcyc; ++j, ++k) launch_cn_stage(a, obp, c->ob_shape, c->ob_lc, 1, s);
+    if (k >= maxcyc) break;
+    launch_cn_check(a, s);
+    HIP_TRY(hipMemcpyAsync(c->cn_flag_host, a.flag, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (*c->cn_flag_host) break;
+  }
+  return 0;
+}
+
 // One clh1itr half step (h1itr_prop) on the orbitals of wfn with the device
 // field d_lf: the first term, then cn_maxcyc more terms that return at once
 // after convergence.
@@ -2049,29 +2075,7 @@ static int enqueue_h1itr(td1c_ctx* c, const double* d_lf, double2* wfn, hipStrea
   const OneBodyElArgs* obp = c->S.n == 20 ? &ob : nullptr;
   TIMED("h1itr");
   launch_cn_stage(a, obp, c->ob_shape, c->ob_lc, 0, s);
-  if (c->capturing || active_recorder()) {
-    // a captured or recorded step holds every term; the converged ones
-    // return at once
-    for (int k = 0; k < c->cn_maxcyc; ++k) launch_cn_stage(a, obp, c->ob_shape, c->ob_lc, 1, s);
-    return 0;
-  }
-  // stream launches: after the second term, and every two terms after
-  // that, the host reads the flag back (the call waits for the device to
-  // get there) and stops enqueuing once the series has converged: the later
-  // terms would all return at once, and each would still cost its launches
-  if (!c->cn_flag_host) HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&c->cn_flag_host), sizeof(int)));
-  int k = 0;
-  while (k < c->cn_maxcyc) {
-    const int chunk = k == 0 ? 1 : 2;
-    for (int j = 0; j < chunk && k < c->cn_maxcyc; ++j, ++k)
-      launch_cn_stage(a, obp, c->ob_shape, c->ob_lc, 1, s);
-    if (k >= c->cn_maxcyc) break;
-    launch_cn_check(a, s);
-    HIP_TRY(hipMemcpyAsync(c->cn_flag_host, a.flag, sizeof(int), hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    if (*c->cn_flag_host) break;
-  }
-  return 0;
+  return enqueue_series(c, a, obp, c->cn_maxcyc, s);
 }
 
 static int enqueue_rk_step(td1c_ctx* c, int rk_order, double dtime, const double* d_lf, double2* wfn, int apply_mask,
@@ -2705,6 +2709,7 @@ int enqueue_etd_step(td1c_ctx* c, const double* d_lf, double2* wfn, int apply_ma
     }
   };
   // the denominator factors of R on the dynamical rows of orb, in place
+  int series_err = 0;  // a failed read-back of a series flag (enqueue_series)
   auto denom = [&](const td1c_ctx::EtdRat& R) {
     for (int i = 0; i < R.dimd; ++i) {
       if (R.tab[i] < 0) continue;
@@ -2722,7 +2727,7 @@ int enqueue_etd_step(td1c_ctx* c, const double* d_lf, double2* wfn, int apply_ma
       const OneBodyElArgs* obp = ob_el ? &ob : nullptr;
       launch_cn_stage(a, obp, c->ob_shape, c->ob_lc, 2, s);
       if (S.isplit == 1 && S.icomp == 1)
-        for (int k = 0; k < c->etd_maxcyc; ++k) launch_cn_stage(a, obp, c->ob_shape, c->ob_lc, 1, s);
+        if (int rc = enqueue_series(c, a, obp, c->etd_maxcyc, s)) series_err = rc;
     }
   };
   auto cmulh = [](double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); };
@@ -2812,7 +2817,7 @@ int enqueue_etd_step(td1c_ctx* c, const double* d_lf, double2* wfn, int apply_ma
     axpy(wfn + dof, one, orb + dof, ndn);
     if (apply_mask) launch_mask(wfn, c->mask.p, S.nr, S.L1, S.nfcore, S.nfun, s);
     HIP_TRY(hipGetLastError());
-    return 0;
+    return series_err;
   }
   if (c->etd_scheme == 3) {
     TIMED("lawrk");
@@ -2829,7 +2834,7 @@ int enqueue_etd_step(td1c_ctx* c, const double* d_lf, double2* wfn, int apply_ma
       ob.stop = a.flag;
       const OneBodyElArgs* obp = ob_el ? &ob : nullptr;
       launch_cn_stage(a, obp, c->ob_shape, c->ob_lc, 2, s);
-      for (int k = 0; k < c->etd_maxcyc; ++k) launch_cn_stage(a, obp, c->ob_shape, c->ob_lc, 1, s);
+      if (int rc = enqueue_series(c, a, obp, c->etd_maxcyc, s)) series_err = rc;
       launch_zaxpby(so, make_double2(-1.0, 0.0), in, make_double2(2.0, 0.0), out, out, s);
     };
     const double dt = c->etd_dt;
@@ -2854,7 +2859,7 @@ int enqueue_etd_step(td1c_ctx* c, const double* d_lf, double2* wfn, int apply_ma
     launch_zaxpby(sz, cdt6, hW0, one, W1, wfn, s);
     if (apply_mask) launch_mask(wfn, c->mask.p, S.nr, S.L1, S.nfcore, S.nfun, s);
     HIP_TRY(hipGetLastError());
-    return 0;
+    return series_err;
   }
   if (c->etd_scheme == 2) {
     TIMED("etdrk4");
@@ -2896,7 +2901,7 @@ int enqueue_etd_step(td1c_ctx* c, const double* d_lf, double2* wfn, int apply_ma
     prod2(R[kF2], hW3, wfn);
     if (apply_mask) launch_mask(wfn, c->mask.p, S.nr, S.L1, S.nfcore, S.nfun, s);
     HIP_TRY(hipGetLastError());
-    return 0;
+    return series_err;
   }
   TIMED("etdrb");
   // stage 0: tW = -i h1 u0 + W0; u1, u3 from the phi1 numerators (and the
@@ -2961,7 +2966,7 @@ int enqueue_etd_step(td1c_ctx* c, const double* d_lf, double2* wfn, int apply_ma
   prod(R[kPhi3Dt1], true, make_double2(-4.0, 0.0), tW, wfn);
   if (apply_mask) launch_mask(wfn, c->mask.p, S.nr, S.L1, S.nfcore, S.nfun, s);
   HIP_TRY(hipGetLastError());
-  return 0;
+  return series_err;
 }
 
 }  // namespace
