@@ -431,7 +431,13 @@ int td1c_h1itr_gen(td1c_ctx* ctx, double dtime, int maxcyc, double thresh);
  * for at most h1itr_maxcyc terms after x_0.  lfield is the field the
  * reference evaluates at t + dt/4 (first half step) or t + 3dt/4 (second).
  * The first check of the series (on x_0, norm ~1 per orbital) counts the
- * dynamical orbitals only.  Enqueued on `stream`; td1c_h1itr_gen first. */
+ * dynamical orbitals only.  Enqueued on `stream`; td1c_h1itr_gen first.
+ * Under stream launches the call reads the series' flag back after the
+ * second term and every two terms after (it waits for `stream` to get
+ * there) and enqueues no more terms once the series has converged; under
+ * graph capture every term is enqueued.  The same holds for the half steps
+ * of td1c_propagate and the denominator series of the exponential
+ * propagators. */
 int td1c_h1itr_prop(td1c_ctx* ctx, const double* lfield, double* d_wfn, void* stream);
 
 /* nstep steps (+ mask) from time t0 with the stage fields evaluated on the
